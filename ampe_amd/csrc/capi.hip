@@ -1,0 +1,299 @@
+// extern "C" entry points of libampe_hip.so (declared in include/ampe_hip.h).
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <string>
+
+#include "../../include/ampe_hip.h"
+#include "sim.h"
+
+namespace ampe {
+const char* last_error();
+}
+
+using ampe::Level;
+
+struct ampe_level {
+   Level* L;
+};
+struct ampe_sim {
+   ampe::Sim* S;
+};
+
+#define AMPE_REQUIRE(cond, msg)        \
+   do {                                \
+      if (!(cond)) {                   \
+         ampe::set_error(msg);         \
+         return -1;                    \
+      }                                \
+   } while (0)
+
+static hipStream_t hs(ampe_stream_t s) { return reinterpret_cast<hipStream_t>(s); }
+
+static int check_params(const ampe_model_params* p, const Level& L)
+{
+   AMPE_REQUIRE(p, "null model params");
+   AMPE_REQUIRE(p->ndim == L.ndim, "model ndim does not match the level");
+   AMPE_REQUIRE(!p->evolve_quat || p->qlen == 4, "qlen must be 4");
+   for (int d = 0; d < p->ndim; d++) AMPE_REQUIRE(p->dx[d] > 0.0, "dx must be positive");
+   return 0;
+}
+
+extern "C" {
+
+const char* ampe_last_error(void) { return ampe::last_error(); }
+int ampe_version(void) { return 1; }
+
+int ampe_level_create(int ndim, const ampe_box* boxes, int npatch, int ghost,
+                      const ampe_box* periodic_domain, ampe_level** out)
+{
+   AMPE_REQUIRE(out, "null output handle");
+   Level* L = nullptr;
+   if (ampe::level_create(ndim, boxes, npatch, ghost, periodic_domain, &L)) return -1;
+   *out = new ampe_level{L};
+   return 0;
+}
+
+int ampe_level_destroy(ampe_level* level)
+{
+   if (!level) return 0;
+   ampe::level_destroy(level->L);
+   delete level;
+   return 0;
+}
+
+int ampe_level_sizes(const ampe_level* level, long long* ncell_ghost, long long* ncell,
+                     long long nside[3])
+{
+   AMPE_REQUIRE(level, "null level");
+   if (ncell_ghost) *ncell_ghost = level->L->ncg;
+   if (ncell) *ncell = level->L->ncell;
+   if (nside)
+      for (int d = 0; d < 3; d++) nside[d] = level->L->nside[d];
+   return 0;
+}
+
+int ampe_level_patch_offsets(const ampe_level* level, long long* cell_off, long long* side_off)
+{
+   AMPE_REQUIRE(level, "null level");
+   const Level& L = *level->L;
+   for (int p = 0; p < L.npatch; p++) {
+      if (cell_off) cell_off[p] = L.hp[p].cell_off;
+      if (side_off)
+         for (int d = 0; d < 3; d++)
+            side_off[3 * p + d] = L.side_dir_off[d] + L.hp[p].side_off[d];
+   }
+   return 0;
+}
+
+int ampe_ghost_fill(const ampe_level* level, double* field, int ncomp, ampe_stream_t stream)
+{
+   AMPE_REQUIRE(level && field, "null argument");
+   return ampe::ghost_fill(*level->L, field, ncomp, hs(stream));
+}
+
+int ampe_quat_grad(const ampe_level* level, const ampe_model_params* p, const double* q,
+                   double* gmod, double* face_norm2, ampe_stream_t stream)
+{
+   AMPE_REQUIRE(level && q, "null argument");
+   if (check_params(p, *level->L)) return -1;
+   return ampe::launch_quat_grad(*level->L, *p, q, gmod, face_norm2, hs(stream));
+}
+
+int ampe_phase_rhs(const ampe_level* level, const ampe_model_params* p, const double* phi,
+                   const double* q, const double* cl, const double* cs, double* rhs,
+                   ampe_stream_t stream)
+{
+   AMPE_REQUIRE(level && phi && rhs, "null argument");
+   if (check_params(p, *level->L)) return -1;
+   AMPE_REQUIRE(!p->evolve_quat || q, "orientation field required");
+   AMPE_REQUIRE(p->flux_type != AMPE_FLUX_ANISOTROPIC || q, "anisotropy needs q");
+   AMPE_REQUIRE(!p->with_conc || (cl && cs), "phase concentrations required");
+   return ampe::launch_phase_rhs(*level->L, *p, phi, q, cl, cs, rhs, hs(stream));
+}
+
+int ampe_quat_rhs(const ampe_level* level, const ampe_model_params* p, const double* phi,
+                  const double* q, const double* face_norm2, double* rhs, ampe_stream_t stream)
+{
+   AMPE_REQUIRE(level && phi && q && face_norm2 && rhs, "null argument");
+   if (check_params(p, *level->L)) return -1;
+   return ampe::launch_quat_rhs(*level->L, *p, phi, q, face_norm2, rhs, hs(stream));
+}
+
+int ampe_quat_project(const ampe_level* level, const ampe_model_params* p, const double* q,
+                      double* corr, double* err, ampe_stream_t stream)
+{
+   AMPE_REQUIRE(level && q && corr && err, "null argument");
+   if (check_params(p, *level->L)) return -1;
+   return ampe::launch_quat_project(*level->L, *p, q, corr, err, hs(stream));
+}
+
+int ampe_conc_coeffs(const ampe_level* level, const ampe_model_params* p, const double* phi,
+                     const double* cl, const double* cs, double* Dc, double* Dphi,
+                     ampe_stream_t stream)
+{
+   AMPE_REQUIRE(level && phi && cl && cs && Dc && Dphi, "null argument");
+   if (check_params(p, *level->L)) return -1;
+   return ampe::launch_conc_coeffs(*level->L, *p, phi, cl, cs, Dc, Dphi, hs(stream));
+}
+
+int ampe_conc_rhs(const ampe_level* level, const ampe_model_params* p, const double* c,
+                  const double* phi, const double* Dc, const double* Dphi, double* rhs,
+                  ampe_stream_t stream)
+{
+   AMPE_REQUIRE(level && c && phi && Dc && Dphi && rhs, "null argument");
+   if (check_params(p, *level->L)) return -1;
+   return ampe::launch_conc_rhs(*level->L, *p, c, phi, Dc, Dphi, rhs, hs(stream));
+}
+
+int ampe_kks_solve(const ampe_level* level, const ampe_model_params* p, const double* phi,
+                   const double* c, const double* cl_ref, const double* cs_ref, double* cl,
+                   double* cs, int* max_iters, ampe_stream_t stream)
+{
+   AMPE_REQUIRE(level && phi && c && cl_ref && cs_ref && cl && cs, "null argument");
+   if (check_params(p, *level->L)) return -1;
+   int it = 0;
+   if (ampe::launch_kks(*level->L, *p, phi, c, cl_ref, cs_ref, cl, cs, &it, nullptr,
+                        hs(stream)))
+      return -1;
+   if (max_iters) *max_iters = it;
+   return it < 0 ? 1 : 0;
+}
+
+int ampe_tag_cells(const ampe_level* level, const ampe_model_params* p, const double* phi,
+                   double tol, int* tags, ampe_stream_t stream)
+{
+   AMPE_REQUIRE(level && phi && tags, "null argument");
+   if (check_params(p, *level->L)) return -1;
+   return ampe::launch_tag(*level->L, *p, phi, tol, tags, hs(stream));
+}
+
+int ampe_fac_phase_coeffs(const ampe_level* level, const ampe_model_params* p, double gamma,
+                          const double* phi, double* C, ampe_stream_t stream)
+{
+   AMPE_REQUIRE(level && phi && C, "null argument");
+   if (check_params(p, *level->L)) return -1;
+   return ampe::launch_phase_C(*level->L, *p, gamma, phi, C, hs(stream));
+}
+
+static ampe::EllOp<double> to_op(const ampe_elliptic_op* op)
+{
+   ampe::EllOp<double> o;
+   o.D = op->D;
+   o.Dc = op->D_const;
+   o.C = op->C;
+   o.Cc = op->C_const;
+   o.M = op->M;
+   o.Mc = op->M_const;
+   return o;
+}
+
+int ampe_fac_smooth(const ampe_level* level, const ampe_elliptic_op* op, const double* rhs,
+                    double* u, int num_sweeps, double tol, double* maxres,
+                    ampe_stream_t stream)
+{
+   AMPE_REQUIRE(level && op && rhs && u, "null argument");
+   const Level& L = *level->L;
+   hipStream_t s = hs(stream);
+   ampe::EllOp<double> o = to_op(op);
+   bool want = tol >= 0.0 || maxres;
+   double m = 0.0;
+   if (ampe::ghost_fill(L, u, 1, s)) return -1;
+   for (int sw = 0; sw < num_sweeps; sw++) {
+      double mr = 0.0, mb = 0.0;
+      if (ampe::ghost_fill(L, u, 1, s)) return -1;
+      if (ampe::launch_rbgs<double>(L, o, op->dx, rhs, u, 0, want, s)) return -1;
+      if (want && ampe::reduce_partials(L, L.ncell_tiles, 1, 1, &mr, s)) return -1;
+      if (ampe::ghost_fill(L, u, 1, s)) return -1;
+      if (ampe::launch_rbgs<double>(L, o, op->dx, rhs, u, 1, want, s)) return -1;
+      if (want && ampe::reduce_partials(L, L.ncell_tiles, 1, 1, &mb, s)) return -1;
+      if (ampe::ghost_fill(L, u, 1, s)) return -1;
+      m = mr > mb ? mr : mb;
+      if (tol >= 0.0 && m <= tol) break;
+   }
+   if (maxres) *maxres = m;
+   return 0;
+}
+
+int ampe_fac_residual(const ampe_level* level, const ampe_elliptic_op* op, const double* rhs,
+                      double* u, double* res, double* rms, ampe_stream_t stream)
+{
+   AMPE_REQUIRE(level && op && rhs && u && res, "null argument");
+   const Level& L = *level->L;
+   hipStream_t s = hs(stream);
+   ampe::EllOp<double> o = to_op(op);
+   if (ampe::ghost_fill(L, u, 1, s)) return -1;
+   if (ampe::launch_residual<double>(L, o, op->dx, rhs, u, res, rms != nullptr, s)) return -1;
+   if (rms) {
+      double ss = 0.0;
+      if (ampe::reduce_partials(L, L.ncell_tiles, 1, 0, &ss, s)) return -1;
+      *rms = std::sqrt(ss / (double)L.ncell);
+   }
+   return 0;
+}
+
+int ampe_vec_dot(const ampe_level* level, int ncomp, const double* x, const double* y,
+                 double* out, ampe_stream_t stream)
+{
+   AMPE_REQUIRE(level && x && y && out, "null argument");
+   return ampe::vec_dot(*level->L, ncomp, x, y, out, hs(stream));
+}
+
+int ampe_vec_wrms(const ampe_level* level, int ncomp, const double* x, const double* w,
+                  double* out, ampe_stream_t stream)
+{
+   AMPE_REQUIRE(level && x && w && out, "null argument");
+   double ss = 0.0;
+   if (ampe::vec_sumsq_w(*level->L, ncomp, x, w, &ss, hs(stream))) return -1;
+   *out = std::sqrt(ss / ((double)ncomp * level->L->ncell));
+   return 0;
+}
+
+int ampe_sim_create(const ampe_model_params* p, const ampe_solver_params* s,
+                    const ampe_box* boxes, int npatch, const ampe_box* periodic_domain,
+                    ampe_sim** out)
+{
+   AMPE_REQUIRE(p && s && boxes && out, "null argument");
+   AMPE_REQUIRE(p->ndim == 2 || p->ndim == 3, "ndim must be 2 or 3");
+   AMPE_REQUIRE(s->dt > 0.0 && s->max_newton > 0, "bad solver parameters");
+   ampe::Sim* S = nullptr;
+   if (ampe::sim_create(*p, *s, boxes, npatch, periodic_domain, &S)) return -1;
+   *out = new ampe_sim{S};
+   return 0;
+}
+
+int ampe_sim_destroy(ampe_sim* sim)
+{
+   if (!sim) return 0;
+   ampe::sim_destroy(sim->S);
+   delete sim;
+   return 0;
+}
+
+int ampe_sim_set_state(ampe_sim* sim, const double* phi, const double* c, const double* q)
+{
+   AMPE_REQUIRE(sim && phi, "null argument");
+   return ampe::sim_set_state(sim->S, phi, c, q);
+}
+
+int ampe_sim_get_state(ampe_sim* sim, double* phi, double* c, double* q, double* cl,
+                       double* cs)
+{
+   AMPE_REQUIRE(sim, "null argument");
+   return ampe::sim_get_state(sim->S, phi, c, q, cl, cs);
+}
+
+int ampe_sim_advance(ampe_sim* sim, int nsteps, ampe_stats* stats)
+{
+   AMPE_REQUIRE(sim && nsteps >= 0, "bad argument");
+   return ampe::sim_advance(sim->S, nsteps, stats);
+}
+
+int ampe_sim_last_time_ms(const ampe_sim* sim, double* ms)
+{
+   AMPE_REQUIRE(sim && ms, "null argument");
+   *ms = sim->S->last_ms;
+   return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,63 @@
+// FAC preconditioner launch interface (see fac.hip).
+#pragma once
+
+#include "kernels.h"
+
+namespace ampe {
+
+// A u = M div(D grad u) + C u; a NULL field pointer selects the constant.
+template <typename T>
+struct EllOp {
+   const T* D = nullptr;
+   T Dc = 0;
+   const T* C = nullptr;
+   T Cc = 0;
+   const T* M = nullptr;
+   T Mc = 0;
+};
+
+int launch_phase_C(const Level& L, const ampe_model_params& p, double gamma,
+                   const double* phi, double* C, hipStream_t s);
+
+// One colour sweep; with want_max the per-tile max |res| lands in L.partial.
+template <typename T>
+int launch_rbgs(const Level& L, const EllOp<T>& op, const double* dx, const T* rhs, T* u,
+                int color, bool want_max, hipStream_t s);
+
+// res = rhs - A u; with want_sum the per-tile sum res^2 lands in L.partial.
+template <typename T>
+int launch_residual(const Level& L, const EllOp<T>& op, const double* dx, const T* rhs,
+                    const T* u, T* res, bool want_sum, hipStream_t s);
+
+template <typename T>
+int launch_quat_rbgs(const Level& L, const T* fc, const T* sm, double gamma, const double* dx,
+                     int ncomp, const T* rhs, T* z, int color, hipStream_t s);
+
+template <typename T>
+int launch_quat_residual(const Level& L, const T* fc, const T* sm, double gamma,
+                         const double* dx, int ncomp, const T* rhs, const T* z, T* res,
+                         bool want_sum, hipStream_t s);
+
+// ------------------------------------------------------------ vectors
+// Flat element-wise ops over n entries (whole ghosted vectors).
+int vec_axpby(long long n, double a, const double* x, double b, const double* y, double* z,
+              hipStream_t s);
+int vec_fill(long long n, double v, double* z, hipStream_t s);
+int vec_mul(long long n, const double* x, const double* y, double* z, hipStream_t s);
+int vec_div(long long n, const double* x, const double* y, double* z, hipStream_t s);
+int vec_ewt(long long n, const double* y, double rtol, double atol, double* w, hipStream_t s);
+int vec_bdf2_hist(long long n, double c1, const double* y, double c2, const double* yp,
+                  double* out, hipStream_t s);
+int vec_to_f32(long long n, const double* x, float* y, hipStream_t s);
+int vec_axpby_f(long long n, float* u, const float* e, hipStream_t s);  // u += e
+int vec_to_f64(long long n, const float* x, double* y, hipStream_t s);
+int vec_div_f(long long n, const double* x, const double* sm, double* y, hipStream_t s);
+int vec_sqrt(long long n, const double* x, double* y, hipStream_t s);
+// Reductions over interior cells of ncomp component blocks (host result,
+// synchronises s).
+int vec_dot(const Level& L, int ncomp, const double* x, const double* y, double* out,
+            hipStream_t s);
+int vec_sumsq_w(const Level& L, int ncomp, const double* x, const double* w, double* out,
+                hipStream_t s);
+
+}  // namespace ampe

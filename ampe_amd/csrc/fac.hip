@@ -1,0 +1,318 @@
+// FAC preconditioner kernels: phase operator coefficient C, red-black
+// Gauss-Seidel colour sweeps and residuals of A u = M div(D grad u) + C u, and
+// the orientation block's symmetrised operator I + gamma sqrt(M) L sqrt(M).
+// Templated on the storage/compute type: double for the FP64 path, float for
+// the mixed-precision preconditioner.
+#include "fac.h"
+
+namespace ampe {
+
+// PhaseFACOps::setCOnPatchPrivate (source/PhaseFACOps.cc:101-183)
+__global__ void __launch_bounds__(256) k_phase_C(LevelArgs a, ampe_model_params p, double gamma,
+                                                 const double* __restrict__ phi,
+                                                 double* __restrict__ C)
+{
+   CellPos c;
+   if (!cell_pos(a, 0, c)) return;
+   double g2 = d2well(phi[c.c0], p.phi_well_type);
+   double gm = gamma * p.phi_mobility;
+   C[c.c0] = 1.0 + gm * p.phi_well_scale * g2;
+}
+
+// Colour of the update: efo_rbgswithfluxmax* walk row (j, k) from
+// ifirst + ioffset in steps of 2, with ioffset = 1 when the parity of
+// (jfirst + j) [2-D] or (ifirst + j + k) [3-D] differs from the colour
+// (source/fortran/2d/ellipticfacops.m4:111-130, 3d/ellipticfacops.m4).
+template <int NDIM>
+__device__ __forceinline__ bool is_color(const CellPos& c, int color)
+{
+   int gj = c.pd.lo[1] + c.j;
+   int x = (NDIM == 3) ? c.pd.lo[0] + gj + (c.pd.lo[2] + c.k) : c.pd.lo[1] + gj;
+   int io = ((x - (x / 2) * 2) != color) ? 1 : 0;
+   return ((c.i - io) & 1) == 0 && c.i >= io;
+}
+
+template <typename T>
+struct OpArgs {
+   const T* D;
+   T Dc;
+   const T* C;
+   T Cc;
+   const T* M;
+   T Mc;
+};
+
+// res = rhs - M div(D grad u) - C u at one cell; *diag = M sum D/h^2 - C
+// (the reciprocal is dudr of efo_rbgswithfluxmax*).
+template <typename T, int NDIM>
+__device__ __forceinline__ T op_residual(const LevelArgs& a, const CellPos& c, const OpArgs<T>& op,
+                                         const T* __restrict__ rhs, const T* __restrict__ u,
+                                         const T* dinv, T* diag)
+{
+   const long long c0 = c.c0;
+   const int ijk[3] = {c.i, c.j, c.k};
+   T div = 0, dsum = 0;
+#pragma unroll
+   for (int d = 0; d < NDIM; d++) {
+      T dlo, dhi;
+      if (op.D) {
+         int lo[3] = {ijk[0], ijk[1], ijk[2]};
+         long long flo = face_idx(a, c.pd, d, lo[0], lo[1], lo[2]);
+         lo[d] += 1;
+         long long fhi = face_idx(a, c.pd, d, lo[0], lo[1], lo[2]);
+         dlo = op.D[flo];
+         dhi = op.D[fhi];
+      } else {
+         dlo = dhi = op.Dc;
+      }
+      long long s = c.st[d];
+      T flo = dinv[d] * dlo * (u[c0] - u[c0 - s]);
+      T fhi = dinv[d] * dhi * (u[c0 + s] - u[c0]);
+      T t = dinv[d] * (fhi - flo);
+      T ds = dinv[d] * dinv[d] * (dhi + dlo);
+      div = (d == 0) ? t : div + t;
+      dsum = (d == 0) ? ds : dsum + ds;
+   }
+   T m = op.M ? op.M[c0] : op.Mc;
+   T cc = op.C ? op.C[c0] : op.Cc;
+   *diag = m * dsum - cc;
+   return rhs[c0] - m * div - cc * u[c0];
+}
+
+template <typename T>
+__device__ __forceinline__ void block_store_max(T v, double* partial)
+{
+   __shared__ double sh[TX * TY];
+   int t = threadIdx.x + TX * threadIdx.y;
+   sh[t] = (double)v;
+   __syncthreads();
+   for (int w = TX * TY / 2; w > 0; w >>= 1) {
+      if (t < w) sh[t] = sh[t + w] > sh[t] ? sh[t + w] : sh[t];
+      __syncthreads();
+   }
+   if (t == 0) partial[blockIdx.x] = sh[0];
+}
+
+template <typename T>
+__device__ __forceinline__ void block_store_sum(double v, double* partial)
+{
+   __shared__ double sh[TX * TY];
+   int t = threadIdx.x + TX * threadIdx.y;
+   sh[t] = v;
+   __syncthreads();
+   for (int w = TX * TY / 2; w > 0; w >>= 1) {
+      if (t < w) sh[t] = sh[t] + sh[t + w];
+      __syncthreads();
+   }
+   if (t == 0) partial[blockIdx.x] = sh[0];
+}
+
+// One colour of efo_rbgswithfluxmax{var,con}dc{var,con}sf
+// (source/fortran/2d/ellipticfacops.m4:60-345): u += -res * dudr.
+template <typename T, int NDIM>
+__global__ void __launch_bounds__(256) k_rbgs(LevelArgs a, OpArgs<T> op, T dx0, T dx1, T dx2,
+                                              const T* __restrict__ rhs, T* __restrict__ u,
+                                              int color, double* __restrict__ partial)
+{
+   CellPos c;
+   bool ok = cell_pos(a, 0, c);
+   T absres = 0;
+   if (ok && is_color<NDIM>(c, color)) {
+      T dinv[3] = {T(1) / dx0, T(1) / dx1, T(1) / dx2};
+      T diag;
+      T res = op_residual<T, NDIM>(a, c, op, rhs, u, dinv, &diag);
+      T dudr = T(1) / diag;
+      T du = -res * dudr;
+      u[c.c0] = u[c.c0] + du * T(1);
+      absres = res < 0 ? -res : res;
+   }
+   if (partial) block_store_max<T>(absres, partial);
+}
+
+// efo_compres*sca (source/fortran/2d/ellipticfacops.m4:346-446) + sum res^2
+template <typename T, int NDIM>
+__global__ void __launch_bounds__(256) k_residual(LevelArgs a, OpArgs<T> op, T dx0, T dx1, T dx2,
+                                                  const T* __restrict__ rhs, const T* __restrict__ u,
+                                                  T* __restrict__ res, double* __restrict__ partial)
+{
+   CellPos c;
+   double sq = 0.0;
+   if (cell_pos(a, 0, c)) {
+      T dinv[3] = {T(1) / dx0, T(1) / dx1, T(1) / dx2};
+      T diag;
+      T r = op_residual<T, NDIM>(a, c, op, rhs, u, dinv, &diag);
+      res[c.c0] = r;
+      sq = (double)r * (double)r;
+   }
+   if (partial) block_store_sum<T>(sq, partial);
+}
+
+// ------------------------------------------------------------ orientation
+// Row of A = I + gamma sqrt(M) L sqrt(M), L from face coefficients:
+// off-diagonal fc/h^2, diagonal -sum fc/h^2 (set_j_ij2d/3d, set_stencil2d/3d,
+// source/fortran/2d/quatlevelsolver.m4:9-120).  Returns A z at one cell for
+// component block zc and the centre coefficient.
+template <typename T, int NDIM>
+__device__ __forceinline__ T quat_row(const LevelArgs& a, const CellPos& c, const T* __restrict__ fc,
+                                      const T* __restrict__ sm, T gamma, const T* hinv2,
+                                      const T* __restrict__ z, T* center)
+{
+   const long long c0 = c.c0;
+   const int ijk[3] = {c.i, c.j, c.k};
+   T olo[3], ohi[3];
+   T diag = 0;
+#pragma unroll
+   for (int d = 0; d < NDIM; d++) {
+      int lo[3] = {ijk[0], ijk[1], ijk[2]};
+      long long flo = face_idx(a, c.pd, d, lo[0], lo[1], lo[2]);
+      lo[d] += 1;
+      long long fhi = face_idx(a, c.pd, d, lo[0], lo[1], lo[2]);
+      olo[d] = fc[flo] * hinv2[d];
+      ohi[d] = fc[fhi] * hinv2[d];
+      diag = (d == 0) ? -olo[0] - ohi[0] : diag - olo[d] - ohi[d];
+   }
+   T fac = gamma * sm[c0];
+   *center = fac * diag * sm[c0] + T(1);
+   T r = *center * z[c0];
+#pragma unroll
+   for (int d = 0; d < NDIM; d++) {
+      long long s = c.st[d];
+      r = r + fac * olo[d] * sm[c0 - s] * z[c0 - s] + fac * ohi[d] * sm[c0 + s] * z[c0 + s];
+   }
+   return r;
+}
+
+template <typename T, int NDIM>
+__global__ void __launch_bounds__(256) k_quat_rbgs(LevelArgs a, const T* __restrict__ fc,
+                                                   const T* __restrict__ sm, T gamma, T h0, T h1,
+                                                   T h2, int ncomp, const T* __restrict__ rhs,
+                                                   T* __restrict__ z, int color)
+{
+   CellPos c;
+   if (!cell_pos(a, 0, c) || !is_color<NDIM>(c, color)) return;
+   T hinv2[3] = {T(1) / (h0 * h0), T(1) / (h1 * h1), T(1) / (h2 * h2)};
+   for (int m = 0; m < ncomp; m++) {
+      const T* rm = rhs + m * a.ncg;
+      T* zm = z + m * a.ncg;
+      T center;
+      T az = quat_row<T, NDIM>(a, c, fc, sm, gamma, hinv2, zm, &center);
+      T res = rm[c.c0] - az;
+      zm[c.c0] = zm[c.c0] + res / center;
+   }
+}
+
+template <typename T, int NDIM>
+__global__ void __launch_bounds__(256) k_quat_residual(LevelArgs a, const T* __restrict__ fc,
+                                                       const T* __restrict__ sm, T gamma, T h0,
+                                                       T h1, T h2, int ncomp,
+                                                       const T* __restrict__ rhs,
+                                                       const T* __restrict__ z, T* __restrict__ res,
+                                                       double* __restrict__ partial)
+{
+   CellPos c;
+   double sq = 0.0;
+   if (cell_pos(a, 0, c)) {
+      T hinv2[3] = {T(1) / (h0 * h0), T(1) / (h1 * h1), T(1) / (h2 * h2)};
+      for (int m = 0; m < ncomp; m++) {
+         T center;
+         T az = quat_row<T, NDIM>(a, c, fc, sm, gamma, hinv2, z + m * a.ncg, &center);
+         T r = rhs[m * a.ncg + c.c0] - az;
+         res[m * a.ncg + c.c0] = r;
+         sq += (double)r * (double)r;
+      }
+   }
+   if (partial) block_store_sum<T>(sq, partial);
+}
+
+// ------------------------------------------------------------ launchers
+static dim3 tblock() { return dim3(TX, TY); }
+
+int launch_phase_C(const Level& L, const ampe_model_params& p, double gamma,
+                   const double* phi, double* C, hipStream_t s)
+{
+   hipLaunchKernelGGL(k_phase_C, dim3(L.ncell_tiles), tblock(), 0, s, level_args(L), p, gamma,
+                      phi, C);
+   AMPE_HIP_CHECK(hipGetLastError());
+   return 0;
+}
+
+template <typename T>
+int launch_rbgs(const Level& L, const EllOp<T>& op, const double* dx, const T* rhs, T* u,
+                int color, bool want_max, hipStream_t s)
+{
+   OpArgs<T> oa{op.D, op.Dc, op.C, op.Cc, op.M, op.Mc};
+   double* part = want_max ? L.partial : nullptr;
+   if (L.ndim == 3)
+      hipLaunchKernelGGL((k_rbgs<T, 3>), dim3(L.ncell_tiles), tblock(), 0, s, level_args(L), oa,
+                         (T)dx[0], (T)dx[1], (T)dx[2], rhs, u, color, part);
+   else
+      hipLaunchKernelGGL((k_rbgs<T, 2>), dim3(L.ncell_tiles), tblock(), 0, s, level_args(L), oa,
+                         (T)dx[0], (T)dx[1], (T)dx[2], rhs, u, color, part);
+   AMPE_HIP_CHECK(hipGetLastError());
+   return 0;
+}
+
+template <typename T>
+int launch_residual(const Level& L, const EllOp<T>& op, const double* dx, const T* rhs,
+                    const T* u, T* res, bool want_sum, hipStream_t s)
+{
+   OpArgs<T> oa{op.D, op.Dc, op.C, op.Cc, op.M, op.Mc};
+   double* part = want_sum ? L.partial : nullptr;
+   if (L.ndim == 3)
+      hipLaunchKernelGGL((k_residual<T, 3>), dim3(L.ncell_tiles), tblock(), 0, s, level_args(L),
+                         oa, (T)dx[0], (T)dx[1], (T)dx[2], rhs, u, res, part);
+   else
+      hipLaunchKernelGGL((k_residual<T, 2>), dim3(L.ncell_tiles), tblock(), 0, s, level_args(L),
+                         oa, (T)dx[0], (T)dx[1], (T)dx[2], rhs, u, res, part);
+   AMPE_HIP_CHECK(hipGetLastError());
+   return 0;
+}
+
+template <typename T>
+int launch_quat_rbgs(const Level& L, const T* fc, const T* sm, double gamma, const double* dx,
+                     int ncomp, const T* rhs, T* z, int color, hipStream_t s)
+{
+   if (L.ndim == 3)
+      hipLaunchKernelGGL((k_quat_rbgs<T, 3>), dim3(L.ncell_tiles), tblock(), 0, s,
+                         level_args(L), fc, sm, (T)gamma, (T)dx[0], (T)dx[1], (T)dx[2], ncomp,
+                         rhs, z, color);
+   else
+      hipLaunchKernelGGL((k_quat_rbgs<T, 2>), dim3(L.ncell_tiles), tblock(), 0, s,
+                         level_args(L), fc, sm, (T)gamma, (T)dx[0], (T)dx[1], (T)dx[2], ncomp,
+                         rhs, z, color);
+   AMPE_HIP_CHECK(hipGetLastError());
+   return 0;
+}
+
+template <typename T>
+int launch_quat_residual(const Level& L, const T* fc, const T* sm, double gamma,
+                         const double* dx, int ncomp, const T* rhs, const T* z, T* res,
+                         bool want_sum, hipStream_t s)
+{
+   double* part = want_sum ? L.partial : nullptr;
+   if (L.ndim == 3)
+      hipLaunchKernelGGL((k_quat_residual<T, 3>), dim3(L.ncell_tiles), tblock(), 0, s,
+                         level_args(L), fc, sm, (T)gamma, (T)dx[0], (T)dx[1], (T)dx[2], ncomp,
+                         rhs, z, res, part);
+   else
+      hipLaunchKernelGGL((k_quat_residual<T, 2>), dim3(L.ncell_tiles), tblock(), 0, s,
+                         level_args(L), fc, sm, (T)gamma, (T)dx[0], (T)dx[1], (T)dx[2], ncomp,
+                         rhs, z, res, part);
+   AMPE_HIP_CHECK(hipGetLastError());
+   return 0;
+}
+
+#define AMPE_INST(T)                                                                     \
+   template int launch_rbgs<T>(const Level&, const EllOp<T>&, const double*, const T*, T*, \
+                               int, bool, hipStream_t);                                  \
+   template int launch_residual<T>(const Level&, const EllOp<T>&, const double*, const T*, \
+                                   const T*, T*, bool, hipStream_t);                     \
+   template int launch_quat_rbgs<T>(const Level&, const T*, const T*, double, const double*, \
+                                    int, const T*, T*, int, hipStream_t);                \
+   template int launch_quat_residual<T>(const Level&, const T*, const T*, double,         \
+                                        const double*, int, const T*, const T*, T*, bool, \
+                                        hipStream_t);
+AMPE_INST(double)
+AMPE_INST(float)
+
+}  // namespace ampe

@@ -1,0 +1,80 @@
+// Patch-level layout on the device: uneven boxes stored back to back, each a
+// ghosted cell block (i fastest), side fields per direction without ghosts,
+// and the work lists every kernel launches over (one launch per level).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/ampe_hip.h"
+
+namespace ampe {
+
+void set_error(const std::string& msg);
+
+#define AMPE_HIP_CHECK(call)                                              \
+   do {                                                                   \
+      hipError_t e_ = (call);                                             \
+      if (e_ != hipSuccess) {                                             \
+         ::ampe::set_error(std::string(#call) + ": " + hipGetErrorString(e_)); \
+         return -1;                                                       \
+      }                                                                   \
+   } while (0)
+
+// Tile of cells processed by one workgroup: TX x TY cells of one (patch, k).
+constexpr int TX = 64;
+constexpr int TY = 4;
+
+struct PatchDev {
+   int nx, ny, nz;         // interior extents (nz = 1 in 2-D)
+   int sy;                 // ghost-box row length nx + 2g
+   long long sz;           // ghost-box plane size
+   long long cell_off;     // ghost-box offset inside one component block
+   long long side_off[3];  // patch faces offset inside direction block d
+   long long int_off;      // compact interior offset (host I/O, tags)
+   int lo[3];              // box lower corner (global index space)
+};
+
+struct Tile {
+   int patch, i0, j0, k;
+};
+
+struct Level {
+   int ndim = 2, g = 1, gz = 0, npatch = 0;
+   long long ncg = 0;         // ghosted cells per component
+   long long ncell = 0;       // interior cells
+   long long nside[3] = {0, 0, 0};
+   long long side_dir_off[3] = {0, 0, 0};
+   long long nside_tot = 0;
+   std::vector<PatchDev> hp;
+   PatchDev* dp = nullptr;
+   Tile* cell_tiles = nullptr;  // interior cells
+   int ncell_tiles = 0;
+   Tile* face_tiles = nullptr;  // cells grown by one in each direction
+   int nface_tiles = 0;
+   long long* ghost_dst = nullptr;  // ghost gather (periodic / neighbours)
+   long long* ghost_src = nullptr;
+   long long nghost = 0;
+   double* partial = nullptr;  // per-workgroup reduction partials (8 per tile)
+   double* dred = nullptr;     // reduction results (16 doubles)
+   double* hpinned = nullptr;  // pinned host scalars
+   int* iscratch = nullptr;    // device integer flags
+};
+
+int level_create(int ndim, const ampe_box* boxes, int npatch, int ghost,
+                 const ampe_box* periodic, Level** out);
+void level_destroy(Level* L);
+
+// Ghost gather for ncomp component blocks (double or float fields).
+int ghost_fill(const Level& L, double* f, int ncomp, hipStream_t s);
+int ghost_fill(const Level& L, float* f, int ncomp, hipStream_t s);
+
+// Fixed-order device reduction of the per-tile partials written by a cell
+// kernel: sum (op = 0) or max (op = 1) of nparts values (ncol columns
+// interleaved), result to host.
+int reduce_partials(const Level& L, int nparts, int ncol, int op,
+                    double* host_out, hipStream_t s);
+
+}  // namespace ampe

@@ -1,0 +1,271 @@
+// Level layout, ghost gather and deterministic reductions.
+#include "level.h"
+
+#include <algorithm>
+#include <cstring>
+#include <mutex>
+
+namespace ampe {
+
+static std::mutex g_err_mu;
+static std::string g_err;
+
+void set_error(const std::string& msg)
+{
+   std::lock_guard<std::mutex> lk(g_err_mu);
+   g_err = msg;
+}
+
+const char* last_error()
+{
+   std::lock_guard<std::mutex> lk(g_err_mu);
+   return g_err.c_str();
+}
+
+static int wrapi(int i, int lo, int n)
+{
+   int r = (i - lo) % n;
+   if (r < 0) r += n;
+   return lo + r;
+}
+
+int level_create(int ndim, const ampe_box* boxes, int npatch, int ghost,
+                 const ampe_box* periodic, Level** out)
+{
+   if (ndim != 2 && ndim != 3) {
+      set_error("level_create: ndim must be 2 or 3");
+      return -1;
+   }
+   if (npatch <= 0 || !boxes || ghost < 1) {
+      set_error("level_create: need npatch > 0 boxes and ghost >= 1");
+      return -1;
+   }
+   Level* L = new Level();
+   L->ndim = ndim;
+   L->g = ghost;
+   L->gz = (ndim == 3) ? ghost : 0;
+   L->npatch = npatch;
+   L->hp.resize(npatch);
+   long long coff = 0, ioff = 0, soff[3] = {0, 0, 0};
+   for (int p = 0; p < npatch; p++) {
+      PatchDev& d = L->hp[p];
+      int n[3];
+      for (int a = 0; a < 3; a++) {
+         n[a] = boxes[p].hi[a] - boxes[p].lo[a] + 1;
+         d.lo[a] = boxes[p].lo[a];
+      }
+      if (ndim == 2) {
+         n[2] = 1;
+         d.lo[2] = 0;
+      }
+      if (n[0] <= 0 || n[1] <= 0 || n[2] <= 0) {
+         set_error("level_create: empty box");
+         delete L;
+         return -1;
+      }
+      d.nx = n[0];
+      d.ny = n[1];
+      d.nz = n[2];
+      d.sy = n[0] + 2 * ghost;
+      d.sz = (long long)d.sy * (n[1] + 2 * ghost);
+      d.cell_off = coff;
+      coff += d.sz * (n[2] + 2 * L->gz);
+      d.int_off = ioff;
+      ioff += (long long)n[0] * n[1] * n[2];
+      for (int a = 0; a < 3; a++) {
+         d.side_off[a] = soff[a];
+         if (a < ndim)
+            soff[a] += (long long)(n[0] + (a == 0)) * (n[1] + (a == 1)) *
+                       (n[2] + (a == 2));
+      }
+   }
+   L->ncg = coff;
+   L->ncell = ioff;
+   for (int a = 0; a < 3; a++) L->nside[a] = soff[a];
+   L->side_dir_off[0] = 0;
+   L->side_dir_off[1] = soff[0];
+   L->side_dir_off[2] = soff[0] + soff[1];
+   L->nside_tot = soff[0] + soff[1] + soff[2];
+
+   // work lists
+   std::vector<Tile> ct, ft;
+   for (int p = 0; p < npatch; p++) {
+      const PatchDev& d = L->hp[p];
+      for (int k = 0; k < d.nz; k++)
+         for (int j0 = 0; j0 < d.ny; j0 += TY)
+            for (int i0 = 0; i0 < d.nx; i0 += TX) ct.push_back({p, i0, j0, k});
+      int fz = (ndim == 3) ? d.nz + 1 : 1;
+      for (int k = 0; k < fz; k++)
+         for (int j0 = 0; j0 < d.ny + 1; j0 += TY)
+            for (int i0 = 0; i0 < d.nx + 1; i0 += TX) ft.push_back({p, i0, j0, k});
+   }
+   L->ncell_tiles = (int)ct.size();
+   L->nface_tiles = (int)ft.size();
+
+   // ghost gather: each ghost cell copies the interior cell it images
+   std::vector<long long> gd, gs;
+   for (int p = 0; p < npatch; p++) {
+      const PatchDev& d = L->hp[p];
+      int g = ghost, gz = L->gz;
+      for (int k = -gz; k < d.nz + gz; k++)
+         for (int j = -g; j < d.ny + g; j++)
+            for (int i = -g; i < d.nx + g; i++) {
+               if (i >= 0 && i < d.nx && j >= 0 && j < d.ny && k >= 0 && k < d.nz)
+                  continue;
+               int gi[3] = {d.lo[0] + i, d.lo[1] + j, d.lo[2] + k};
+               if (periodic) {
+                  for (int a = 0; a < ndim; a++)
+                     gi[a] = wrapi(gi[a], periodic->lo[a],
+                                   periodic->hi[a] - periodic->lo[a] + 1);
+               }
+               int src_p = -1, si = 0, sj = 0, sk = 0;
+               for (int q = 0; q < npatch && src_p < 0; q++) {
+                  const PatchDev& e = L->hp[q];
+                  int li = gi[0] - e.lo[0], lj = gi[1] - e.lo[1],
+                      lk = (ndim == 3) ? gi[2] - e.lo[2] : 0;
+                  if (li >= 0 && li < e.nx && lj >= 0 && lj < e.ny && lk >= 0 &&
+                      lk < e.nz) {
+                     src_p = q;
+                     si = li;
+                     sj = lj;
+                     sk = lk;
+                  }
+               }
+               if (src_p < 0) {
+                  // physical (non-periodic) boundary: zero-slope extension
+                  src_p = p;
+                  si = std::min(std::max(i, 0), d.nx - 1);
+                  sj = std::min(std::max(j, 0), d.ny - 1);
+                  sk = std::min(std::max(k, 0), d.nz - 1);
+               }
+               const PatchDev& e = L->hp[src_p];
+               gd.push_back(d.cell_off + (i + g) + (long long)(j + g) * d.sy +
+                            (long long)(k + gz) * d.sz);
+               gs.push_back(e.cell_off + (si + g) + (long long)(sj + g) * e.sy +
+                            (long long)(sk + gz) * e.sz);
+            }
+   }
+   L->nghost = (long long)gd.size();
+
+   int maxparts = std::max(L->ncell_tiles, L->nface_tiles);
+   hipError_t e = hipSuccess;
+   e = hipMalloc(&L->dp, sizeof(PatchDev) * npatch);
+   if (e == hipSuccess) e = hipMalloc(&L->cell_tiles, sizeof(Tile) * ct.size());
+   if (e == hipSuccess) e = hipMalloc(&L->face_tiles, sizeof(Tile) * ft.size());
+   if (e == hipSuccess && L->nghost > 0) {
+      e = hipMalloc(&L->ghost_dst, sizeof(long long) * gd.size());
+      if (e == hipSuccess) e = hipMalloc(&L->ghost_src, sizeof(long long) * gs.size());
+   }
+   if (e == hipSuccess) e = hipMalloc(&L->partial, sizeof(double) * (size_t)maxparts * 8);
+   if (e == hipSuccess) e = hipMalloc(&L->dred, sizeof(double) * 16);
+   if (e == hipSuccess) e = hipHostMalloc(&L->hpinned, sizeof(double) * 64);
+   if (e == hipSuccess) e = hipMalloc(&L->iscratch, sizeof(int) * 64);
+   if (e == hipSuccess)
+      e = hipMemcpy(L->dp, L->hp.data(), sizeof(PatchDev) * npatch, hipMemcpyHostToDevice);
+   if (e == hipSuccess)
+      e = hipMemcpy(L->cell_tiles, ct.data(), sizeof(Tile) * ct.size(), hipMemcpyHostToDevice);
+   if (e == hipSuccess)
+      e = hipMemcpy(L->face_tiles, ft.data(), sizeof(Tile) * ft.size(), hipMemcpyHostToDevice);
+   if (e == hipSuccess && L->nghost > 0) {
+      e = hipMemcpy(L->ghost_dst, gd.data(), sizeof(long long) * gd.size(), hipMemcpyHostToDevice);
+      if (e == hipSuccess)
+         e = hipMemcpy(L->ghost_src, gs.data(), sizeof(long long) * gs.size(), hipMemcpyHostToDevice);
+   }
+   if (e != hipSuccess) {
+      set_error(std::string("level_create: ") + hipGetErrorString(e));
+      level_destroy(L);
+      return -1;
+   }
+   *out = L;
+   return 0;
+}
+
+void level_destroy(Level* L)
+{
+   if (!L) return;
+   (void)hipFree(L->dp);
+   (void)hipFree(L->cell_tiles);
+   (void)hipFree(L->face_tiles);
+   (void)hipFree(L->ghost_dst);
+   (void)hipFree(L->ghost_src);
+   (void)hipFree(L->partial);
+   (void)hipFree(L->dred);
+   (void)hipHostFree(L->hpinned);
+   (void)hipFree(L->iscratch);
+   delete L;
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_ghost_gather(T* __restrict__ f, long long ncg,
+                                                      const long long* __restrict__ dst,
+                                                      const long long* __restrict__ src,
+                                                      long long n)
+{
+   long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+   if (t >= n) return;
+   long long off = (long long)blockIdx.y * ncg;
+   f[off + dst[t]] = f[off + src[t]];
+}
+
+template <typename T>
+static int ghost_fill_t(const Level& L, T* f, int ncomp, hipStream_t s)
+{
+   if (L.nghost == 0 || ncomp <= 0) return 0;
+   dim3 grid((unsigned)((L.nghost + 255) / 256), (unsigned)ncomp);
+   hipLaunchKernelGGL(k_ghost_gather<T>, grid, dim3(256), 0, s, f, L.ncg,
+                      L.ghost_dst, L.ghost_src, L.nghost);
+   AMPE_HIP_CHECK(hipGetLastError());
+   return 0;
+}
+
+int ghost_fill(const Level& L, double* f, int ncomp, hipStream_t s)
+{
+   return ghost_fill_t(L, f, ncomp, s);
+}
+int ghost_fill(const Level& L, float* f, int ncomp, hipStream_t s)
+{
+   return ghost_fill_t(L, f, ncomp, s);
+}
+
+// Single-workgroup fixed-order reduction of nparts x ncol partials
+// (column-interleaved).  Each lane walks a fixed stride, then a fixed tree.
+__global__ void __launch_bounds__(1024) k_reduce_partials(const double* __restrict__ part,
+                                                          int nparts, int ncol, int op,
+                                                          double* __restrict__ out)
+{
+   __shared__ double sh[1024];
+   for (int c = 0; c < ncol; c++) {
+      double acc = 0.0;  // sums start at 0; maxima are of non-negative values
+      for (int i = threadIdx.x; i < nparts; i += 1024) {
+         double v = part[(long long)i * ncol + c];
+         acc = (op == 0) ? acc + v : (v > acc ? v : acc);
+      }
+      sh[threadIdx.x] = acc;
+      __syncthreads();
+      for (int w = 512; w > 0; w >>= 1) {
+         if (threadIdx.x < w) {
+            double a = sh[threadIdx.x], b = sh[threadIdx.x + w];
+            sh[threadIdx.x] = (op == 0) ? a + b : (b > a ? b : a);
+         }
+         __syncthreads();
+      }
+      if (threadIdx.x == 0) out[c] = sh[0];
+      __syncthreads();
+   }
+}
+
+int reduce_partials(const Level& L, int nparts, int ncol, int op,
+                    double* host_out, hipStream_t s)
+{
+   double* dout = L.dred;
+   hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(1024), 0, s, L.partial, nparts,
+                      ncol, op, dout);
+   AMPE_HIP_CHECK(hipGetLastError());
+   AMPE_HIP_CHECK(hipMemcpyAsync(L.hpinned, dout, sizeof(double) * ncol,
+                                 hipMemcpyDeviceToHost, s));
+   AMPE_HIP_CHECK(hipStreamSynchronize(s));
+   for (int c = 0; c < ncol; c++) host_out[c] = L.hpinned[c];
+   return 0;
+}
+
+}  // namespace ampe

@@ -1,0 +1,233 @@
+// Point-wise physics shared by the HIP kernels: interpolation / well
+// functions, face averages, the |grad q| floor, free energies and the per-cell
+// KKS solve.  Operation order follows the reference Fortran so FP64 results
+// agree with the CPU restatement to the last bits (transcendentals aside).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "../../include/ampe_hip.h"
+
+#define AMPE_HD __host__ __device__ __forceinline__
+
+namespace ampe {
+
+constexpr double kRGas = 8.314472;  // J/(K mol)
+
+AMPE_HD double dmax(double a, double b) { return a > b ? a : b; }
+AMPE_HD double dmin(double a, double b) { return a < b ? a : b; }
+
+// interp_func (source/fortran/functions.f:17-92)
+AMPE_HD double interp(double phi, char type)
+{
+   double t;
+   switch (type) {
+      case 'q': t = dmax(0.0, phi); return t * t;
+      case 'p':
+         t = dmax(0.0, dmin(1.0, phi));
+         return t * t * t * (10.0 - 15.0 * t + 6.0 * t * t);
+      case 'h':
+         t = dmax(0.0, dmin(1.0, phi));
+         return t * t * (3.0 - 2.0 * t);
+      case 'w': t = dmax(0.0, phi); return t * t * (2.0 - t);
+      case 'l':
+         t = dmax(0.0, phi);
+         t = dmin(1.0, t);
+         return t;
+      case 'm':
+         t = dmax(0.0, dmin(1.0, phi));
+         return t * t / (2.0 * t * (t - 1.0) + 1.0);
+      case '3': t = dmax(0.0, phi); return t * t * t;
+      case 's': return log(cosh(10.0 * phi)) / log(cosh(10.0));
+      case 'c': return 1.0;
+      default: return __builtin_nan("");
+   }
+}
+
+// deriv_interp_func (source/fortran/functions.f:95-173)
+AMPE_HD double dinterp(double phi, char type)
+{
+   double t, s;
+   switch (type) {
+      case 'q': t = dmax(0.0, phi); return 2.0 * t;
+      case 'p':
+         t = dmax(0.0, dmin(1.0, phi));
+         return 30.0 * t * t * (1.0 - t) * (1.0 - t);
+      case 'h':
+         t = dmax(0.0, dmin(1.0, phi));
+         return 6.0 * t * (1.0 - t);
+      case 'w': t = dmax(0.0, phi); return t * (4.0 - 3.0 * t);
+      case 'l': return (phi > 0.0 || phi < 1.0) ? 1.0 : 0.0;
+      case 'm':
+         t = dmax(0.0, dmin(1.0, phi));
+         s = 2.0 * t * (t - 1.0) + 1.0;
+         return 2.0 * t * (1.0 - t) / (s * s);
+      case '3': t = dmax(0.0, phi); return 3.0 * t * t;
+      case 's':
+         t = dmax(0.0, phi);
+         return 10.0 * tanh(10.0 * t) / log(cosh(10.0));
+      case 'c': return 0.0;
+      default: return __builtin_nan("");
+   }
+}
+
+// deriv_well_func / second_deriv_well_func (source/fortran/functions.f:246-330)
+AMPE_HD double dwell(double phi, char type)
+{
+   if (type == 'd') return 32.0 * phi * (1.0 - phi) * (1.0 - 2.0 * phi);
+   return 2.0 * (phi - 1.0);
+}
+AMPE_HD double d2well(double phi, char type)
+{
+   if (type == 'd') return 32.0 * (1.0 + 6.0 * phi * (phi - 1.0));
+   return 2.0;
+}
+
+// average_func (source/fortran/functions.f:333-365)
+AMPE_HD double face_avg(double a, double b, char type)
+{
+   if (type == 'h') {
+      if (a < 1.0e-16 || b < 1.0e-16) return 0.0;
+      return 2.0 / (1.0 / a + 1.0 / b);
+   }
+   return 0.5 * (a + b);
+}
+
+// eval_grad_normi (source/fortran/quat.f:1497-1537)
+AMPE_HD double grad_normi(double gn2, char floor_type, double floor2,
+                          double max_gni)
+{
+   if (floor_type == 'm') return (gn2 > floor2) ? pow(gn2, -0.5) : max_gni;
+   if (floor_type == 't') {
+      double gng2 = gn2 * max_gni * max_gni;
+      if (gng2 > 0.01) {
+         double gn = sqrt(gn2);
+         return tanh(max_gni * gn) / gn;
+      }
+      return max_gni * (1.0 - gng2 * (5.0 - 2.0 * gng2) / 15.0);
+   }
+   return pow(gn2 + floor2, -0.5);
+}
+
+// quatmobility (source/fortran/2d/mobility.m4:13-90)
+AMPE_HD double quat_mobility(const ampe_model_params& p, double ph)
+{
+   double qf;
+   if (p.quat_mobility_type == 'e') {
+      double c = p.quat_mobility_scale;
+      ph = dmax(0.0, dmin(1.0, ph));
+      qf = (1.0 - exp(c * ph)) / (1.0 - exp(c));
+      qf = 1.0 - qf;
+   } else if (p.quat_mobility_type == 'i') {
+      ph = dmax(1.0e-6, dmin(1.0, ph));
+      qf = dmax(0.0, (1.0 - ph) / (ph * ph));
+      qf = dmin(qf, p.quat_mobility_scale);
+   } else {
+      ph = dmax(0.0, dmin(1.0, ph));
+      qf = ph * ph * ph * (10.0 - 15.0 * ph + 6.0 * ph * ph);
+      qf = 1.0 - qf;
+   }
+   return p.min_quat_mobility + (p.quat_mobility - p.min_quat_mobility) * qf;
+}
+
+// ---------------------------------------------------------------- energies
+// Quadratic wells A (c - ceq(T))^2 and binary CALPHAD (end members, ideal
+// mixing, Redlich-Kister excess), J/mol.  phase 0 = liquid, 1 = solid A.
+AMPE_HD double quad_ceq(const ampe_model_params& p, int ph)
+{
+   return p.quad_ceq[ph] + p.quad_m[ph] * (p.temperature - p.quad_Tref);
+}
+
+AMPE_HD double fenergy(const ampe_model_params& p, double c, int ph)
+{
+   if (p.fe_model == AMPE_FE_QUADRATIC) {
+      double d = c - quad_ceq(p, ph);
+      return p.quad_A[ph] * d * d;
+   }
+   const ampe_calphad_phase& cp = p.calphad[ph];
+   double RT = kRGas * p.temperature;
+   double x = 2.0 * c - 1.0;
+   double P = cp.L[0] + x * (cp.L[1] + x * (cp.L[2] + x * cp.L[3]));
+   return c * cp.gA + (1.0 - c) * cp.gB +
+          RT * (c * log(c) + (1.0 - c) * log(1.0 - c)) + c * (1.0 - c) * P;
+}
+
+AMPE_HD double dfenergy(const ampe_model_params& p, double c, int ph)
+{
+   if (p.fe_model == AMPE_FE_QUADRATIC)
+      return 2.0 * p.quad_A[ph] * (c - quad_ceq(p, ph));
+   const ampe_calphad_phase& cp = p.calphad[ph];
+   double RT = kRGas * p.temperature;
+   double x = 2.0 * c - 1.0;
+   double P = cp.L[0] + x * (cp.L[1] + x * (cp.L[2] + x * cp.L[3]));
+   double dP = cp.L[1] + x * (2.0 * cp.L[2] + x * 3.0 * cp.L[3]);
+   return cp.gA - cp.gB + RT * (log(c) - log(1.0 - c)) + (1.0 - 2.0 * c) * P +
+          2.0 * c * (1.0 - c) * dP;
+}
+
+AMPE_HD double d2fenergy(const ampe_model_params& p, double c, int ph)
+{
+   if (p.fe_model == AMPE_FE_QUADRATIC) return 2.0 * p.quad_A[ph];
+   const ampe_calphad_phase& cp = p.calphad[ph];
+   double RT = kRGas * p.temperature;
+   double x = 2.0 * c - 1.0;
+   double P = cp.L[0] + x * (cp.L[1] + x * (cp.L[2] + x * cp.L[3]));
+   double dP = cp.L[1] + x * (2.0 * cp.L[2] + x * 3.0 * cp.L[3]);
+   double d2P = 2.0 * cp.L[2] + x * 6.0 * cp.L[3];
+   return RT / (c * (1.0 - c)) - 2.0 * P + 4.0 * (1.0 - 2.0 * c) * dP +
+          4.0 * c * (1.0 - c) * d2P;
+}
+
+// Per-cell KKS: mu_L(cL) = mu_S(cS), (1-h) cL + h cS = c.  Quadratic wells:
+// closed form.  CALPHAD: Newton on [(mu_L - mu_S)/RT, mass balance] from the
+// guess in (cl, cs), step halving to stay in (0,1).  Returns the iteration
+// count or -1.
+AMPE_HD int kks_cell(const ampe_model_params& p, double c, double phi,
+                     double& cl, double& cs)
+{
+   double h = interp(phi, p.conc_interp);
+   if (p.fe_model == AMPE_FE_QUADRATIC) {
+      double eL = quad_ceq(p, 0), eS = quad_ceq(p, 1);
+      double r = p.quad_A[0] / p.quad_A[1];
+      double xl = (c - h * (eS - r * eL)) / (1.0 - h + h * r);
+      cl = xl;
+      cs = eS + r * (xl - eL);
+      return 0;
+   }
+   double RTinv = 1.0 / (kRGas * p.temperature);
+   double xl = cl, xs = cs;
+   const double lo = 1.0e-12, hi = 1.0 - 1.0e-12;
+   if (!(xl > lo)) xl = lo;
+   if (!(xl < hi)) xl = hi;
+   if (!(xs > lo)) xs = lo;
+   if (!(xs < hi)) xs = hi;
+   for (int it = 0; it <= p.newton_max_its; it++) {
+      double f1 = (dfenergy(p, xl, 0) - dfenergy(p, xs, 1)) * RTinv;
+      double f2 = (1.0 - h) * xl + h * xs - c;
+      if (fabs(f1) < p.newton_tol && fabs(f2) < p.newton_tol) {
+         cl = xl;
+         cs = xs;
+         return it;
+      }
+      if (it == p.newton_max_its) break;
+      double a = d2fenergy(p, xl, 0) * RTinv;
+      double b = -d2fenergy(p, xs, 1) * RTinv;
+      double cc = 1.0 - h, d = h;
+      double det = a * d - b * cc;
+      double dl = -(d * f1 - b * f2) / det;
+      double ds = -(a * f2 - cc * f1) / det;
+      double alpha = 1.0;
+      for (int k = 0; k < 60; k++) {
+         double nl = xl + alpha * dl, ns = xs + alpha * ds;
+         if (nl > 0.0 && nl < 1.0 && ns > 0.0 && ns < 1.0) break;
+         alpha *= 0.5;
+      }
+      xl = xl + alpha * dl;
+      xs = xs + alpha * ds;
+   }
+   cl = xl;
+   cs = xs;
+   return -1;
+}
+
+}  // namespace ampe

@@ -1,0 +1,600 @@
+// Right-hand-side kernels of the phase, orientation and composition
+// equations, the per-cell KKS solve, the q projection and AMR tagging.
+//
+// Every kernel processes a whole level (all patches) in one launch over the
+// level's tile list.  Face quantities (fluxes, face coefficients) are
+// recomputed per cell from the stencil instead of being materialised as side
+// arrays, so each kernel reads its cell fields once and writes its outputs
+// once; the arithmetic of each face is the reference's, in its order.
+#include "kernels.h"
+
+#include <cstring>
+
+namespace ampe {
+
+// Single precision literal 1.e-12 of anisotropic_gradient_flux promoted to
+// double (source/fortran/2d/quatrhs.m4:193, 3d/quatrhs.m4:184).
+#define AMPE_ANISO_THRESHOLD ((double)1.e-12f)
+
+// ------------------------------------------------------------ phase flux
+// Face between cells cm = c0 - st[d] and c0, direction d.
+// Simple: gradient_flux (source/fortran/2d/quatrhs.m4:12-53).
+// Isotropic (2-D): compute_flux_isotropic (2d/quatrhs.m4:106-151).
+// Anisotropic: anisotropic_gradient_flux (2d/quatrhs.m4:154-256,
+// 3d/quatrhs.m4:149-356).
+__device__ __forceinline__ void quatmult4(const double* a, const double* b, double* r)
+{
+   r[0] = a[0] * b[0] - a[1] * b[1] - a[2] * b[2] - a[3] * b[3];
+   r[1] = a[0] * b[1] + a[1] * b[0] + a[2] * b[3] - a[3] * b[2];
+   r[2] = a[0] * b[2] + a[2] * b[0] + a[3] * b[1] - a[1] * b[3];
+   r[3] = a[0] * b[3] + a[3] * b[0] + a[1] * b[2] - a[2] * b[1];
+}
+
+template <int NDIM>
+__device__ double phase_face_flux(const ampe_model_params& p, const double* __restrict__ ph,
+                                  const double* __restrict__ q, long long ncg,
+                                  const long long* st, int d, long long cm, long long c0)
+{
+   const double* dx = p.dx;
+   if (p.flux_type == AMPE_FLUX_ANISOTROPIC) {
+      if (NDIM == 2) {
+         double eps = p.epsilon_phi, nu = p.aniso_nu;
+         int kn = p.aniso_knumber;
+         double pi = 4.0 * atan(1.0);
+         double dinv0 = 1.0 / dx[0], dinv1 = 1.0 / dx[1];
+         int t = 1 - d;  // transverse direction
+         double gn = (ph[c0] - ph[cm]) * (d == 0 ? dinv0 : dinv1);
+         double gt = 0.25 * (ph[cm + st[t]] - ph[cm - st[t]] + ph[c0 + st[t]] - ph[c0 - st[t]]) *
+                     (d == 0 ? dinv1 : dinv0);
+         double dphidx = (d == 0) ? gn : gt;
+         double dphidy = (d == 0) ? gt : gn;
+         double theta = (fabs(dphidx) > AMPE_ANISO_THRESHOLD) ? atan(dphidy / dphidx) : 0.5 * pi;
+         double qa = 0.5 * (q[cm] + q[c0]);
+         if (qa > 1.0) qa = 1.0;
+         if (qa < -1.0) qa = -1.0;
+         double phia = (p.qlen == 4) ? 2.0 * acos(qa) : acos(qa);
+         double epst = eps * (1.0 + nu * cos(kn * (theta - phia)));
+         double deps = -kn * eps * nu * sin(kn * (theta - phia));
+         if (d == 0) return epst * epst * dphidx - epst * deps * dphidy;
+         return epst * epst * dphidy + epst * deps * dphidx;
+      } else {
+         double eps = p.epsilon_phi, eps4 = p.aniso_nu;
+         double dinv[3] = {1.0 / dx[0], 1.0 / dx[1], 1.0 / dx[2]};
+         double factor = 4.0 * eps4 / (1.0 - 3.0 * eps4);
+         double gr[3];
+#pragma unroll
+         for (int e = 0; e < 3; e++) {
+            if (e == d)
+               gr[e] = (ph[c0] - ph[cm]) * dinv[e];
+            else
+               gr[e] = 0.25 * (ph[cm + st[e]] - ph[cm - st[e]] + ph[c0 + st[e]] - ph[c0 - st[e]]) *
+                       dinv[e];
+         }
+         double gphi2 = gr[0] * gr[0] + gr[1] * gr[1] + gr[2] * gr[2];
+         double dg[4], n4;
+         if (fabs(gphi2) > AMPE_ANISO_THRESHOLD) {
+            double nni = 1.0 / sqrt(gphi2);
+            double n[4] = {0.0, gr[0] * nni, gr[1] * nni, gr[2] * nni};
+            double qa[4];
+#pragma unroll
+            for (int m = 0; m < 4; m++) qa[m] = 0.5 * (q[m * ncg + cm] + q[m * ncg + c0]);
+            double qp[4] = {qa[0], -qa[1], -qa[2], -qa[3]}, tq[4], np[4];
+            quatmult4(n, qp, tq);
+            quatmult4(qa, tq, np);
+            double a1 = np[1] * np[1], a2 = np[2] * np[2], a3 = np[3] * np[3];
+            n4 = a1 * a1 + a2 * a2 + a3 * a3;
+            double dd[4] = {0.0, np[1] * (np[1] * np[1] - n4), np[2] * (np[2] * np[2] - n4),
+                            np[3] * (np[3] * np[3] - n4)};
+            quatmult4(dd, qa, tq);
+            quatmult4(qp, tq, dg);
+         } else {
+            dg[0] = dg[1] = dg[2] = dg[3] = 0.0;
+            n4 = 0.0;
+         }
+         double gamma = eps * (1.0 - 3.0 * eps4) * (1.0 + factor * n4);
+         return gamma * gamma * gr[d] + 16.0 * eps * gamma * eps4 * sqrt(gphi2) * dg[d + 1];
+      }
+   }
+   double e2 = p.epsilon_phi * p.epsilon_phi;
+   if (p.flux_type == AMPE_FLUX_ISOTROPIC && NDIM == 2) {
+      double dinv = (1.0 / 12.0) * e2 / dx[d];
+      long long t = st[1 - d];
+      return dinv * ((ph[c0 - t] - ph[cm - t]) + (ph[c0] - ph[cm]) * 10.0 + (ph[c0 + t] - ph[cm + t]));
+   }
+   double dinv = e2 / dx[d];
+   return (ph[c0] - ph[cm]) * dinv;
+}
+
+// |grad q| from cell-centred gradients: quatdiffs -> quatgrad_cell ->
+// quatgrad_modulus (2d/quatgrad.m4:12-60, 882-915).
+template <int NDIM>
+__device__ __forceinline__ double quat_cell_modulus(const ampe_model_params& p,
+                                                    const double* __restrict__ q, long long ncg,
+                                                    const long long* st, long long c0)
+{
+   double p5[3] = {0.5 / p.dx[0], 0.5 / p.dx[1], 0.5 / p.dx[2]};
+   double s = 0.0;
+   for (int m = 0; m < p.qlen; m++) {
+      const double* a = q + m * ncg;
+      double gx = ((a[c0 + st[0]] - a[c0]) + (a[c0] - a[c0 - st[0]])) * p5[0];
+      double gy = ((a[c0 + st[1]] - a[c0]) + (a[c0] - a[c0 - st[1]])) * p5[1];
+      if (NDIM == 3) {
+         double gz = ((a[c0 + st[2]] - a[c0]) + (a[c0] - a[c0 - st[2]])) * p5[2];
+         s = s + gx * gx + gy * gy + gz * gz;
+      } else {
+         s = s + gx * gx + gy * gy;
+      }
+   }
+   return sqrt(s);
+}
+
+// ------------------------------------------------------------ quat grads
+template <int NDIM>
+__global__ void __launch_bounds__(256) k_quat_modulus(LevelArgs a, ampe_model_params p,
+                                                      const double* __restrict__ q,
+                                                      double* __restrict__ gmod)
+{
+   CellPos c;
+   if (!cell_pos(a, 0, c)) return;
+   gmod[c.c0] = quat_cell_modulus<NDIM>(p, q, a.ncg, c.st, c.c0);
+}
+
+// |grad q|^2 on faces from the full side gradient (quatgrad_side
+// 2d/quatgrad.m4:157-229, summed as compute_face_coef2d quatfacops.m4:83-92).
+template <int NDIM>
+__device__ __forceinline__ double quat_face_norm2(const ampe_model_params& p,
+                                                  const double* __restrict__ q, long long ncg,
+                                                  const long long* st, int d, long long cm,
+                                                  long long c0)
+{
+   double s = 0.0;
+   for (int n = 0; n < NDIM; n++) {
+      double dinv = 1.0 / p.dx[n];
+      double p25 = 0.25 * dinv;
+      for (int m = 0; m < p.qlen; m++) {
+         const double* aa = q + m * ncg;
+         double gv;
+         if (n == d) {
+            gv = dinv * (aa[c0] - aa[cm]);
+         } else {
+            long long t = st[n];
+            gv = p25 * ((aa[cm + t] - aa[cm]) + (aa[cm] - aa[cm - t]) + (aa[c0 + t] - aa[c0]) +
+                        (aa[c0] - aa[c0 - t]));
+         }
+         s = s + gv * gv;
+      }
+   }
+   return s;
+}
+
+template <int NDIM>
+__global__ void __launch_bounds__(256) k_quat_face_norm2(LevelArgs a, ampe_model_params p,
+                                                         const double* __restrict__ q,
+                                                         double* __restrict__ n2)
+{
+   CellPos c;
+   if (!cell_pos(a, 1, c)) return;
+   if (NDIM == 2 && c.k > 0) return;
+   for (int d = 0; d < NDIM; d++) {
+      bool ok = (c.i < c.pd.nx + (d == 0)) && (c.j < c.pd.ny + (d == 1)) &&
+                (NDIM == 2 || c.k < c.pd.nz + (d == 2));
+      if (!ok) continue;
+      n2[face_idx(a, c.pd, d, c.i, c.j, c.k)] =
+          quat_face_norm2<NDIM>(p, q, a.ncg, c.st, d, c.c0 - c.st[d], c.c0);
+   }
+}
+
+// ------------------------------------------------------------ phase RHS
+// PhaseRHSStrategyWithQ::evaluateRHS per patch (source/PhaseRHSStrategyWithQ.cc
+// :151-312): flux divergence, double well, orientation terms (computerhspbg,
+// 2d/quatrhs.m4:264-405), KKS driving force (QuadraticFreeEnergyStrategy.cc
+// :393-511 / CALPHADFreeEnergyStrategyBinary.cc:521-651), times M_phi.
+template <int NDIM>
+__global__ void __launch_bounds__(256) k_phase_rhs(LevelArgs a, ampe_model_params p,
+                                                   const double* __restrict__ phi,
+                                                   const double* __restrict__ q,
+                                                   const double* __restrict__ cl,
+                                                   const double* __restrict__ cs,
+                                                   double* __restrict__ rhs)
+{
+   CellPos c;
+   if (!cell_pos(a, 0, c)) return;
+   const long long c0 = c.c0;
+   double r = 0.0;
+#pragma unroll
+   for (int d = 0; d < NDIM; d++) {
+      double flo = phase_face_flux<NDIM>(p, phi, q, a.ncg, c.st, d, c0 - c.st[d], c0);
+      double fhi = phase_face_flux<NDIM>(p, phi, q, a.ncg, c.st, d, c0, c0 + c.st[d]);
+      double dt = (fhi - flo) * (1.0 / p.dx[d]);
+      r = (d == 0) ? dt : r + dt;
+   }
+   double ph = phi[c0];
+   r = r - p.phi_well_scale * dwell(ph, p.phi_well_type);
+   if (p.evolve_quat) {
+      double mf = 2.0 * p.H_parameter, T = p.temperature;
+      double eq2 = 0.5 * p.epsilon_q * p.epsilon_q;
+      double p1 = dinterp(ph, p.orient_interp1);
+      double p2 = dinterp(ph, p.orient_interp2);
+      double gm = quat_cell_modulus<NDIM>(p, q, a.ncg, c.st, c0);
+      r = r - mf * T * p1 * gm - p2 * eq2 * gm * gm;
+   }
+   if (p.with_conc) {
+      double convL = 1.0e-6 / p.vm_L, convA = 1.0e-6 / p.vm_A;
+      double c_l = cl[c0], c_a = cs[c0];
+      double f_l = fenergy(p, c_l, 0) * convL;
+      double f_a = fenergy(p, c_a, 1) * convA;
+      double mu = (p.fe_model == AMPE_FE_QUADRATIC) ? dfenergy(p, c_l, 0) * convL
+                                                    : dfenergy(p, c_a, 1) * convA;
+      double hp = dinterp(ph, p.energy_interp);
+      r += hp * ((f_l - f_a) - mu * (c_l - c_a));
+   }
+   rhs[c0] = p.phi_mobility * r;
+}
+
+// ------------------------------------------------------------ quat RHS
+// compute_face_coef2d/3d (2d/quatfacops.m4:14-122) from the face |grad q|^2
+__device__ __forceinline__ double quat_face_coef(const ampe_model_params& p, double phm,
+                                                 double ph0, double n2f)
+{
+   double floor2 = p.gradient_floor * p.gradient_floor;
+   double maxgni = 1.0 / p.gradient_floor;
+   double eps2 = p.epsilon_q * p.epsilon_q;
+   double mf = 2.0 * p.H_parameter;
+   double phia = face_avg(phm, ph0, p.avg_type);
+   double tempa = 0.5 * (p.temperature + p.temperature);
+   double diff = mf * tempa * interp(phia, p.orient_interp1);
+   double hphi2 = interp(phia, p.orient_interp2);
+   double gni = grad_normi(n2f, p.floor_type, floor2, maxgni);
+   return -gni * diff - eps2 * hphi2;
+}
+
+// QuatFACOps::evaluateRHS with use_gradq_for_flux and projection
+// (source/QuatFACOps.cc:1861-1890, 2089-2108): flux = fc dq/dh
+// (compute_flux2d_from_gradq), lambda (compute_lambda_flux2d/3d),
+// rhs = -M_q (div + 2 q lambda) (add_quat_proj_op2d/3d).
+template <int NDIM>
+__global__ void __launch_bounds__(256) k_quat_rhs(LevelArgs a, ampe_model_params p,
+                                                  const double* __restrict__ phi,
+                                                  const double* __restrict__ q,
+                                                  const double* __restrict__ n2,
+                                                  double* __restrict__ rhs)
+{
+   CellPos c;
+   if (!cell_pos(a, 0, c)) return;
+   const long long c0 = c.c0, ncg = a.ncg;
+   const int ql = p.qlen;
+   double fclo[3], fchi[3];
+   const int ijk[3] = {c.i, c.j, c.k};
+#pragma unroll
+   for (int d = 0; d < NDIM; d++) {
+      int lo[3] = {ijk[0], ijk[1], ijk[2]};
+      long long flo = face_idx(a, c.pd, d, lo[0], lo[1], lo[2]);
+      lo[d] += 1;
+      long long fhi = face_idx(a, c.pd, d, lo[0], lo[1], lo[2]);
+      fclo[d] = quat_face_coef(p, phi[c0 - c.st[d]], phi[c0], n2[flo]);
+      fchi[d] = quat_face_coef(p, phi[c0], phi[c0 + c.st[d]], n2[fhi]);
+   }
+   double dinv[3] = {1.0 / p.dx[0], 1.0 / p.dx[1], 1.0 / p.dx[2]};
+   double fac[3] = {0.5 / p.dx[0], 0.5 / p.dx[1], 0.5 / p.dx[2]};
+   double lam = 0.0, sumq2 = 0.0;
+   double dv[4];
+   for (int m = 0; m < ql; m++) {
+      const double* aa = q + m * ncg;
+      double qm = aa[c0];
+      double dif[3];
+#pragma unroll
+      for (int d = 0; d < NDIM; d++) {
+         long long s = c.st[d];
+         double left = fclo[d] * (dinv[d] * (aa[c0] - aa[c0 - s]));
+         double right = fchi[d] * (dinv[d] * (aa[c0 + s] - aa[c0]));
+         dif[d] = right - left;
+      }
+      if (NDIM == 3) {
+         lam = lam - qm * (dif[0] * fac[0] + dif[1] * fac[1] + dif[2] * fac[2]);
+         dv[m] = dif[0] * dinv[0] + dif[1] * dinv[1] + dif[2] * dinv[2];
+      } else {
+         lam = lam - qm * (dif[0] * fac[0] + dif[1] * fac[1]);
+         dv[m] = dif[0] * dinv[0] + dif[1] * dinv[1];
+      }
+      sumq2 = sumq2 + qm * qm;
+   }
+   lam = lam / sumq2;
+   double mq = quat_mobility(p, phi[c0]);
+   for (int m = 0; m < ql; m++) {
+      double qm = q[m * ncg + c0];
+      rhs[m * ncg + c0] = 0.0 - mq * (dv[m] + 2.0 * qm * lam);
+   }
+}
+
+// project2d/3d (2d/quatfacops.m4:940-997)
+__global__ void __launch_bounds__(256) k_quat_project(LevelArgs a, int ql,
+                                                      const double* __restrict__ q,
+                                                      double* __restrict__ corr,
+                                                      double* __restrict__ err)
+{
+   CellPos c;
+   if (!cell_pos(a, 0, c)) return;
+   const long long c0 = c.c0, ncg = a.ncg;
+   double qv[4], cv[4];
+   double fac = 0.0;
+   for (int m = 0; m < ql; m++) {
+      qv[m] = q[m * ncg + c0];
+      fac = fac + qv[m] * qv[m];
+   }
+   fac = 1.0 / sqrt(fac);
+   for (int m = 0; m < ql; m++) cv[m] = qv[m] * fac;
+   fac = 0.0;
+   for (int m = 0; m < ql; m++) fac = fac + cv[m] * err[m * ncg + c0];
+   for (int m = 0; m < ql; m++) err[m * ncg + c0] = err[m * ncg + c0] - cv[m] * fac;
+   for (int m = 0; m < ql; m++) corr[m * ncg + c0] = cv[m] - qv[m];
+}
+
+// ------------------------------------------------------------ composition
+// concentration_pfmdiffusion (2d/concentrationdiffusion.m4:14-96) and
+// setDiffCoeffForPhaseOnPatch (source/KKSCompositionRHSStrategy.cc:210-373)
+template <int NDIM>
+__global__ void __launch_bounds__(256) k_conc_coeffs(LevelArgs a, ampe_model_params p,
+                                                     const double* __restrict__ phi,
+                                                     const double* __restrict__ cl,
+                                                     const double* __restrict__ cs,
+                                                     double* __restrict__ Dc,
+                                                     double* __restrict__ Dphi)
+{
+   CellPos c;
+   if (!cell_pos(a, 1, c)) return;
+   if (NDIM == 2 && c.k > 0) return;
+   double qlr = p.Q0_liquid / kRGas, qsr = p.Q0_solid / kRGas;
+   for (int d = 0; d < NDIM; d++) {
+      bool ok = (c.i < c.pd.nx + (d == 0)) && (c.j < c.pd.ny + (d == 1)) &&
+                (NDIM == 2 || c.k < c.pd.nz + (d == 2));
+      if (!ok) continue;
+      long long c0 = c.c0, cm = c0 - c.st[d];
+      long long fi = face_idx(a, c.pd, d, c.i, c.j, c.k);
+      double vphi = face_avg(phi[cm], phi[c0], p.avg_type);
+      double hphi = interp(vphi, p.energy_interp);
+      double invT = 2.0 / (p.temperature + p.temperature);
+      double dl = p.D_liquid * exp(-qlr * invT);
+      double ds = p.D_solid * exp(-qsr * invT);
+      double dd = (1.0 - hphi) * dl + hphi * ds;
+      Dc[fi] = dd;
+      double ph = face_avg(phi[c0], phi[cm], p.avg_type);
+      double c_l = 0.5 * (cl[c0] + cl[cm]);
+      double c_a = 0.5 * (cs[c0] + cs[cm]);
+      double hp = dinterp(ph, p.energy_interp);
+      Dphi[fi] = dd * hp * (c_l - c_a);
+   }
+}
+
+// concentrationflux (2d/concentrationrhs.m4:15-82) + computerhsconcentration
+// (2d/concentrationrhs.m4:326-367)
+template <int NDIM>
+__global__ void __launch_bounds__(256) k_conc_rhs(LevelArgs a, ampe_model_params p,
+                                                  const double* __restrict__ cc,
+                                                  const double* __restrict__ phi,
+                                                  const double* __restrict__ Dc,
+                                                  const double* __restrict__ Dphi,
+                                                  double* __restrict__ rhs)
+{
+   CellPos c;
+   if (!cell_pos(a, 0, c)) return;
+   const long long c0 = c.c0;
+   const int ijk[3] = {c.i, c.j, c.k};
+   double s = 0.0;
+#pragma unroll
+   for (int d = 0; d < NDIM; d++) {
+      double dinv = 1.0 / p.dx[d];
+      int lo[3] = {ijk[0], ijk[1], ijk[2]};
+      long long flo = face_idx(a, c.pd, d, lo[0], lo[1], lo[2]);
+      lo[d] += 1;
+      long long fhi = face_idx(a, c.pd, d, lo[0], lo[1], lo[2]);
+      long long cm = c0 - c.st[d], cp = c0 + c.st[d];
+      double jl = dinv * (Dc[flo] * (cc[c0] - cc[cm]) + Dphi[flo] * (phi[c0] - phi[cm]));
+      double jh = dinv * (Dc[fhi] * (cc[cp] - cc[c0]) + Dphi[fhi] * (phi[cp] - phi[c0]));
+      double t = dinv * (jh - jl);
+      s = (d == 0) ? t : s + t;
+   }
+   rhs[c0] = p.conc_mobility * s;
+}
+
+// ------------------------------------------------------------ KKS
+// Per cell over the ghost box (PhaseConcentrationsStrategy::
+// computePhaseConcentrations, source/PhaseConcentrationsStrategy.cc:26-127).
+__global__ void __launch_bounds__(256) k_kks(long long n, ampe_model_params p,
+                                             const double* __restrict__ phi,
+                                             const double* __restrict__ c,
+                                             const double* __restrict__ clr,
+                                             const double* __restrict__ csr,
+                                             double* __restrict__ cl, double* __restrict__ cs,
+                                             int* __restrict__ flags)
+{
+   long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+   int it = 0;
+   if (t < n) {
+      double xl = clr[t], xs = csr[t];
+      it = kks_cell(p, c[t], phi[t], xl, xs);
+      cl[t] = xl;
+      cs[t] = xs;
+   }
+   // wave-level aggregation before one atomic per wave
+   int itmax = it < 0 ? 0 : it;
+   int fail = it < 0 ? 1 : 0;
+   long long itsum = itmax;
+   for (int o = 32; o > 0; o >>= 1) {
+      int v = __shfl_xor(itmax, o);
+      itmax = v > itmax ? v : itmax;
+      fail |= __shfl_xor(fail, o);
+      itsum += __shfl_xor(itsum, o);
+   }
+   if ((threadIdx.x & 63) == 0) {
+      atomicMax(&flags[0], itmax);
+      atomicOr(&flags[1], fail);
+      atomicAdd((unsigned long long*)(flags + 2), (unsigned long long)itsum);
+   }
+}
+
+// ------------------------------------------------------------ tagging
+// tagfromgrads (2d/quattag.m4:9-55) on the cell gradient of phi
+template <int NDIM>
+__global__ void __launch_bounds__(256) k_tag(LevelArgs a, ampe_model_params p,
+                                             const double* __restrict__ phi, double tol2,
+                                             int* __restrict__ tags)
+{
+   CellPos c;
+   if (!cell_pos(a, 0, c)) return;
+   const long long c0 = c.c0;
+   double g = 0.0;
+#pragma unroll
+   for (int d = 0; d < NDIM; d++) {
+      double p5 = 0.5 / p.dx[d];
+      long long s = c.st[d];
+      double gd = ((phi[c0 + s] - phi[c0]) + (phi[c0] - phi[c0 - s])) * p5;
+      double gh = gd * p.dx[d];
+      g = g + gh * gh;
+   }
+   tags[int_idx(c)] = (g >= tol2) ? 1 : 0;
+}
+
+// ------------------------------------------------------------ launchers
+static dim3 tile_block() { return dim3(TX, TY); }
+
+#define AMPE_DISPATCH(L, KERNEL, GRID, ...)                                    \
+   do {                                                                        \
+      if ((L).ndim == 3)                                                       \
+         hipLaunchKernelGGL(KERNEL<3>, dim3(GRID), tile_block(), 0, s, __VA_ARGS__); \
+      else                                                                     \
+         hipLaunchKernelGGL(KERNEL<2>, dim3(GRID), tile_block(), 0, s, __VA_ARGS__); \
+      AMPE_HIP_CHECK(hipGetLastError());                                       \
+   } while (0)
+
+int launch_quat_grad(const Level& L, const ampe_model_params& p, const double* q,
+                     double* gmod, double* face_norm2, hipStream_t s)
+{
+   if (gmod) AMPE_DISPATCH(L, k_quat_modulus, L.ncell_tiles, level_args(L), p, q, gmod);
+   if (face_norm2)
+      AMPE_DISPATCH(L, k_quat_face_norm2, L.nface_tiles, level_args(L, true), p, q, face_norm2);
+   return 0;
+}
+
+int launch_phase_rhs(const Level& L, const ampe_model_params& p, const double* phi,
+                     const double* q, const double* cl, const double* cs, double* rhs,
+                     hipStream_t s)
+{
+   AMPE_DISPATCH(L, k_phase_rhs, L.ncell_tiles, level_args(L), p, phi, q, cl, cs, rhs);
+   return 0;
+}
+
+int launch_quat_rhs(const Level& L, const ampe_model_params& p, const double* phi,
+                    const double* q, const double* n2, double* rhs, hipStream_t s)
+{
+   AMPE_DISPATCH(L, k_quat_rhs, L.ncell_tiles, level_args(L), p, phi, q, n2, rhs);
+   return 0;
+}
+
+int launch_quat_project(const Level& L, const ampe_model_params& p, const double* q,
+                        double* corr, double* err, hipStream_t s)
+{
+   hipLaunchKernelGGL(k_quat_project, dim3(L.ncell_tiles), tile_block(), 0, s, level_args(L),
+                      p.qlen, q, corr, err);
+   AMPE_HIP_CHECK(hipGetLastError());
+   return 0;
+}
+
+int launch_conc_coeffs(const Level& L, const ampe_model_params& p, const double* phi,
+                       const double* cl, const double* cs, double* Dc, double* Dphi,
+                       hipStream_t s)
+{
+   AMPE_DISPATCH(L, k_conc_coeffs, L.nface_tiles, level_args(L, true), p, phi, cl, cs, Dc, Dphi);
+   return 0;
+}
+
+int launch_conc_rhs(const Level& L, const ampe_model_params& p, const double* c,
+                    const double* phi, const double* Dc, const double* Dphi, double* rhs,
+                    hipStream_t s)
+{
+   AMPE_DISPATCH(L, k_conc_rhs, L.ncell_tiles, level_args(L), p, c, phi, Dc, Dphi, rhs);
+   return 0;
+}
+
+int launch_kks(const Level& L, const ampe_model_params& p, const double* phi,
+               const double* c, const double* clr, const double* csr, double* cl,
+               double* cs, int* max_iters, long long* total_iters, hipStream_t s)
+{
+   AMPE_HIP_CHECK(hipMemsetAsync(L.iscratch, 0, sizeof(int) * 4, s));
+   unsigned nb = (unsigned)((L.ncg + 255) / 256);
+   hipLaunchKernelGGL(k_kks, dim3(nb), dim3(256), 0, s, L.ncg, p, phi, c, clr, csr, cl, cs,
+                      L.iscratch);
+   AMPE_HIP_CHECK(hipGetLastError());
+   if (max_iters || total_iters) {
+      AMPE_HIP_CHECK(hipMemcpyAsync(L.hpinned, L.iscratch, sizeof(int) * 4,
+                                    hipMemcpyDeviceToHost, s));
+      AMPE_HIP_CHECK(hipStreamSynchronize(s));
+      const int* f = reinterpret_cast<const int*>(L.hpinned);
+      if (max_iters) *max_iters = f[1] ? -1 : f[0];
+      if (total_iters) {
+         unsigned long long tot;
+         memcpy(&tot, f + 2, sizeof(tot));
+         *total_iters = (long long)tot;
+      }
+   }
+   return 0;
+}
+
+int launch_tag(const Level& L, const ampe_model_params& p, const double* phi, double tol,
+               int* tags, hipStream_t s)
+{
+   AMPE_DISPATCH(L, k_tag, L.ncell_tiles, level_args(L), p, phi, tol * tol, tags);
+   return 0;
+}
+
+}  // namespace ampe
+
+namespace ampe {
+
+// Face coefficients of the orientation operator on every face (used by the
+// preconditioner; QuatFACOps::setOperatorCoefficients,
+// source/QuatFACOps.cc:735-829).
+template <int NDIM>
+__global__ void __launch_bounds__(256) k_quat_face_coef(LevelArgs a, ampe_model_params p,
+                                                        const double* __restrict__ phi,
+                                                        const double* __restrict__ n2,
+                                                        double* __restrict__ fc)
+{
+   CellPos c;
+   if (!cell_pos(a, 1, c)) return;
+   if (NDIM == 2 && c.k > 0) return;
+   for (int d = 0; d < NDIM; d++) {
+      bool ok = (c.i < c.pd.nx + (d == 0)) && (c.j < c.pd.ny + (d == 1)) &&
+                (NDIM == 2 || c.k < c.pd.nz + (d == 2));
+      if (!ok) continue;
+      long long fi = face_idx(a, c.pd, d, c.i, c.j, c.k);
+      fc[fi] = quat_face_coef(p, phi[c.c0 - c.st[d]], phi[c.c0], n2[fi]);
+   }
+}
+
+// sqrt of the orientation mobility over all ghosted cells
+__global__ void __launch_bounds__(256) k_sqrt_quat_mobility(long long n, ampe_model_params p,
+                                                            const double* __restrict__ phi,
+                                                            double* __restrict__ sm)
+{
+   long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+   if (t >= n) return;
+   sm[t] = sqrt(quat_mobility(p, phi[t]));
+}
+
+int launch_quat_face_coef(const Level& L, const ampe_model_params& p, const double* phi,
+                          const double* n2, double* fc, hipStream_t s)
+{
+   AMPE_DISPATCH(L, k_quat_face_coef, L.nface_tiles, level_args(L, true), p, phi, n2, fc);
+   return 0;
+}
+
+int launch_sqrt_quat_mobility(const Level& L, const ampe_model_params& p, const double* phi,
+                              double* sm, hipStream_t s)
+{
+   hipLaunchKernelGGL(k_sqrt_quat_mobility, dim3((unsigned)((L.ncg + 255) / 256)), dim3(256), 0,
+                      s, L.ncg, p, phi, sm);
+   AMPE_HIP_CHECK(hipGetLastError());
+   return 0;
+}
+
+}  // namespace ampe

@@ -1,0 +1,52 @@
+// Device-resident BDF Newton-Krylov integrator (see sim.hip).
+#pragma once
+
+#include <type_traits>
+#include <vector>
+
+#include "fac.h"
+
+namespace ampe {
+
+struct Sim {
+   ampe_model_params p;
+   ampe_solver_params s;
+   Level* L = nullptr;
+   int nc = 0, iq = -1, ic = -1;
+   long long ng = 0, nv = 0, nf = 0;
+   double t = 0, h = 0, hprev = 0, gamma_setup = 1, crate = 1;
+   long long nst = 0, nstlp = 0;
+   bool have_prev = false;
+   // state and Newton-Krylov work vectors (nc ghosted component blocks)
+   double *y = nullptr, *yprev = nullptr, *ypred = nullptr, *ycur = nullptr,
+          *ahist = nullptr, *fcur = nullptr, *b = nullptr, *del = nullptr, *ewt = nullptr,
+          *work = nullptr, *work2 = nullptr, *jv = nullptr, *xk = nullptr, *corr = nullptr,
+          *acor = nullptr;
+   std::vector<double*> V;
+   std::vector<double> H;
+   // phase concentrations and their per-step Newton guesses (ghosted)
+   double *cl = nullptr, *cs = nullptr, *clr = nullptr, *csr = nullptr;
+   // lagged face data and preconditioner coefficients
+   double *n2 = nullptr, *Dc = nullptr, *Dphi = nullptr, *Dconc = nullptr, *fc = nullptr;
+   double *Cphi = nullptr, *sqrtm = nullptr;
+   double *fac_u = nullptr, *fac_e = nullptr, *fac_r = nullptr;
+   // FP32 preconditioner copies (mixed precision)
+   float *f_u = nullptr, *f_e = nullptr, *f_r = nullptr, *f_f = nullptr;
+   float *f_Cphi = nullptr, *f_sqrtm = nullptr, *f_Dconc = nullptr, *f_fc = nullptr;
+   hipStream_t stream = nullptr;
+   hipEvent_t ev0 = nullptr, ev1 = nullptr;
+   float last_ms = 0.f;
+   ampe_stats st{};
+};
+
+int sim_create(const ampe_model_params& p, const ampe_solver_params& s, const ampe_box* boxes,
+               int npatch, const ampe_box* periodic, Sim** out);
+void sim_destroy(Sim* S);
+int sim_set_state(Sim* S, const double* phi, const double* c, const double* q);
+int sim_get_state(Sim* S, double* phi, double* c, double* q, double* cl, double* cs);
+int sim_advance(Sim* S, int nsteps, ampe_stats* stats);
+
+int vec_div_comp(Sim* S, const double* x, const double* sm, double* y, int ncomp);
+int vec_mul_comp(Sim* S, double* x, const double* sm, int ncomp);
+
+}  // namespace ampe

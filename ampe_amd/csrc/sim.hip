@@ -1,0 +1,755 @@
+// BDF Newton-Krylov integrator on one patch level, device resident.
+//
+// The host loop is the QuatIntegrator / CVODE callback structure
+// (source/QuatIntegrator.cc:2096-2233, 3134-3771; CVODE options
+// source/samrai/CVODESolver.cc:150-247): RHS evaluations with lagged
+// orientation face gradients and composition coefficients in Jv, SPGMR with
+// modified Gram-Schmidt and left block preconditioning, difference-quotient
+// Jv, Newton with CVODE's convergence-rate test, projection of q onto |q| = 1
+// after each step.  Step size is the imposed dt (BDF1 first, variable-step
+// BDF2 after); a Newton failure with a fresh preconditioner cuts h by 4 and h
+// regrows by 2 per accepted step.  All vectors live in HBM; only Hessenberg
+// scalars cross to the host.
+#include "sim.h"
+
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+namespace ampe {
+
+static int dalloc(double** p, long long n)
+{
+   if (n <= 0) n = 1;
+   hipError_t e = hipMalloc(p, sizeof(double) * (size_t)n);
+   if (e != hipSuccess) {
+      set_error(std::string("hipMalloc: ") + hipGetErrorString(e));
+      return -1;
+   }
+   e = hipMemset(*p, 0, sizeof(double) * (size_t)n);
+   return e == hipSuccess ? 0 : -1;
+}
+
+static int falloc(float** p, long long n)
+{
+   if (n <= 0) n = 1;
+   hipError_t e = hipMalloc(p, sizeof(float) * (size_t)n);
+   if (e != hipSuccess) {
+      set_error(std::string("hipMalloc: ") + hipGetErrorString(e));
+      return -1;
+   }
+   e = hipMemset(*p, 0, sizeof(float) * (size_t)n);
+   return e == hipSuccess ? 0 : -1;
+}
+
+int sim_create(const ampe_model_params& p, const ampe_solver_params& s, const ampe_box* boxes,
+               int npatch, const ampe_box* periodic, Sim** out)
+{
+   if (s.maxl < 1 || s.maxl > 30) {
+      set_error("sim_create: maxl must be in [1, 30]");
+      return -1;
+   }
+   if (p.evolve_quat && p.qlen != 4) {
+      set_error("sim_create: qlen must be 4");
+      return -1;
+   }
+   Sim* S = new Sim();
+   S->p = p;
+   S->s = s;
+   if (level_create(p.ndim, boxes, npatch, 1, periodic, &S->L)) {
+      delete S;
+      return -1;
+   }
+   const Level& L = *S->L;
+   S->nc = 1 + (p.with_conc ? 1 : 0) + (p.evolve_quat ? p.qlen : 0);
+   S->ic = p.with_conc ? 1 : -1;
+   S->iq = p.evolve_quat ? 1 + (p.with_conc ? 1 : 0) : -1;
+   S->ng = L.ncg;
+   S->nv = (long long)S->nc * S->ng;
+   S->nf = L.nside_tot;
+   int ql = p.evolve_quat ? p.qlen : 1;
+   int rc = 0;
+   double** vecs[] = {&S->y, &S->yprev, &S->ypred, &S->ycur, &S->ahist, &S->fcur, &S->b,
+                      &S->del, &S->ewt, &S->work, &S->work2, &S->jv, &S->xk, &S->corr,
+                      &S->acor};
+   for (auto v : vecs) rc |= dalloc(v, S->nv);
+   S->V.assign(s.maxl + 1, nullptr);
+   for (auto& v : S->V) rc |= dalloc(&v, S->nv);
+   double** cells[] = {&S->cl, &S->cs, &S->clr, &S->csr, &S->Cphi, &S->sqrtm};
+   for (auto v : cells) rc |= dalloc(v, S->ng);
+   rc |= dalloc(&S->fac_u, S->ng * ql);
+   rc |= dalloc(&S->fac_e, S->ng * ql);
+   rc |= dalloc(&S->fac_r, S->ng * ql);
+   double** sides[] = {&S->n2, &S->Dc, &S->Dphi, &S->Dconc, &S->fc};
+   for (auto v : sides) rc |= dalloc(v, S->nf);
+   if (s.precision == AMPE_FAC_FP32) {
+      rc |= falloc(&S->f_u, S->ng * ql);
+      rc |= falloc(&S->f_e, S->ng * ql);
+      rc |= falloc(&S->f_r, S->ng * ql);
+      rc |= falloc(&S->f_f, S->ng * ql);
+      rc |= falloc(&S->f_Cphi, S->ng);
+      rc |= falloc(&S->f_sqrtm, S->ng);
+      rc |= falloc(&S->f_Dconc, S->nf);
+      rc |= falloc(&S->f_fc, S->nf);
+   }
+   if (rc || hipStreamCreate(&S->stream) != hipSuccess ||
+       hipEventCreate(&S->ev0) != hipSuccess || hipEventCreate(&S->ev1) != hipSuccess) {
+      if (!rc) set_error("sim_create: stream/event creation failed");
+      sim_destroy(S);
+      return -1;
+   }
+   S->h = s.dt;
+   *out = S;
+   return 0;
+}
+
+void sim_destroy(Sim* S)
+{
+   if (!S) return;
+   double* all[] = {S->y, S->yprev, S->ypred, S->ycur, S->ahist, S->fcur, S->b, S->del,
+                    S->ewt, S->work, S->work2, S->jv, S->xk, S->corr, S->acor, S->cl, S->cs,
+                    S->clr, S->csr, S->Cphi, S->sqrtm, S->fac_u, S->fac_e, S->fac_r, S->n2,
+                    S->Dc, S->Dphi, S->Dconc, S->fc};
+   for (double* v : all) (void)hipFree(v);
+   for (double* v : S->V) (void)hipFree(v);
+   float* fall[] = {S->f_u, S->f_e, S->f_r, S->f_f, S->f_Cphi, S->f_sqrtm, S->f_Dconc, S->f_fc};
+   for (float* v : fall) (void)hipFree(v);
+   if (S->stream) (void)hipStreamDestroy(S->stream);
+   if (S->ev0) (void)hipEventDestroy(S->ev0);
+   if (S->ev1) (void)hipEventDestroy(S->ev1);
+   level_destroy(S->L);
+   delete S;
+}
+
+// ------------------------------------------------------------ host I/O
+static int scatter_interior(Sim* S, const double* host, double* dev)
+{
+   const Level& L = *S->L;
+   std::vector<double> buf((size_t)L.ncg, 0.0);
+   AMPE_HIP_CHECK(hipMemcpy(buf.data(), dev, sizeof(double) * L.ncg, hipMemcpyDeviceToHost));
+   for (const PatchDev& d : L.hp) {
+      long long n = d.int_off;
+      for (int k = 0; k < d.nz; k++)
+         for (int j = 0; j < d.ny; j++)
+            for (int i = 0; i < d.nx; i++)
+               buf[(size_t)(d.cell_off + (i + L.g) + (long long)(j + L.g) * d.sy +
+                            (long long)(k + L.gz) * d.sz)] = host[n++];
+   }
+   AMPE_HIP_CHECK(hipMemcpy(dev, buf.data(), sizeof(double) * L.ncg, hipMemcpyHostToDevice));
+   return 0;
+}
+
+static int gather_interior(Sim* S, const double* dev, double* host)
+{
+   const Level& L = *S->L;
+   std::vector<double> buf((size_t)L.ncg);
+   AMPE_HIP_CHECK(hipMemcpy(buf.data(), dev, sizeof(double) * L.ncg, hipMemcpyDeviceToHost));
+   for (const PatchDev& d : L.hp) {
+      long long n = d.int_off;
+      for (int k = 0; k < d.nz; k++)
+         for (int j = 0; j < d.ny; j++)
+            for (int i = 0; i < d.nx; i++)
+               host[n++] = buf[(size_t)(d.cell_off + (i + L.g) + (long long)(j + L.g) * d.sy +
+                                        (long long)(k + L.gz) * d.sz)];
+   }
+   return 0;
+}
+
+static int kks(Sim* S, const double* y)
+{
+   const Level& L = *S->L;
+   int maxit = 0;
+   long long tot = 0;
+   const double* phi = y;
+   const double* c = y + (long long)S->ic * S->ng;
+   if (launch_kks(L, S->p, phi, c, S->clr, S->csr, S->cl, S->cs, &maxit, &tot, S->stream))
+      return -1;
+   S->st.kks_calls++;
+   S->st.kks_cells += L.ncg;
+   if (maxit < 0) {
+      set_error("KKS Newton did not converge in a cell");
+      return -1;
+   }
+   S->st.kks_iters += tot;
+   return 0;
+}
+
+int sim_set_state(Sim* S, const double* phi, const double* c, const double* q)
+{
+   const Level& L = *S->L;
+   hipStream_t st = S->stream;
+   AMPE_HIP_CHECK(hipMemset(S->y, 0, sizeof(double) * S->nv));
+   if (scatter_interior(S, phi, S->y)) return -1;
+   if (S->ic >= 0) {
+      if (!c) {
+         set_error("set_state: model has a composition field");
+         return -1;
+      }
+      if (scatter_interior(S, c, S->y + (long long)S->ic * S->ng)) return -1;
+   }
+   if (S->iq >= 0) {
+      if (!q) {
+         set_error("set_state: model has an orientation field");
+         return -1;
+      }
+      for (int m = 0; m < S->p.qlen; m++)
+         if (scatter_interior(S, q + (long long)m * L.ncell,
+                              S->y + (long long)(S->iq + m) * S->ng))
+            return -1;
+   }
+   if (ghost_fill(L, S->y, S->nc, st)) return -1;
+   std::memset(&S->st, 0, sizeof(S->st));
+   if (S->ic >= 0) {
+      const double* cc = S->y + (long long)S->ic * S->ng;
+      AMPE_HIP_CHECK(hipMemcpyAsync(S->clr, cc, sizeof(double) * S->ng,
+                                    hipMemcpyDeviceToDevice, st));
+      AMPE_HIP_CHECK(hipMemcpyAsync(S->csr, cc, sizeof(double) * S->ng,
+                                    hipMemcpyDeviceToDevice, st));
+      if (kks(S, S->y)) return -1;
+      std::memset(&S->st, 0, sizeof(S->st));
+   }
+   S->t = 0.0;
+   S->nst = 0;
+   S->nstlp = 0;
+   S->have_prev = false;
+   S->h = S->s.dt;
+   if (vec_ewt(S->nv, S->y, S->s.rtol, S->s.atol, S->ewt, st)) return -1;
+   AMPE_HIP_CHECK(hipStreamSynchronize(st));
+   return 0;
+}
+
+int sim_get_state(Sim* S, double* phi, double* c, double* q, double* cl, double* cs)
+{
+   const Level& L = *S->L;
+   AMPE_HIP_CHECK(hipStreamSynchronize(S->stream));
+   if (phi && gather_interior(S, S->y, phi)) return -1;
+   if (c && S->ic >= 0 && gather_interior(S, S->y + (long long)S->ic * S->ng, c)) return -1;
+   if (q && S->iq >= 0)
+      for (int m = 0; m < S->p.qlen; m++)
+         if (gather_interior(S, S->y + (long long)(S->iq + m) * S->ng, q + (long long)m * L.ncell))
+            return -1;
+   if (cl && S->ic >= 0 && gather_interior(S, S->cl, cl)) return -1;
+   if (cs && S->ic >= 0 && gather_interior(S, S->cs, cs)) return -1;
+   return 0;
+}
+
+// ------------------------------------------------------------ callbacks
+// evaluateRHSFunction (source/QuatIntegrator.cc:3134-3298)
+static int rhs_eval(Sim* S, double* y, double* ydot, bool fd)
+{
+   const Level& L = *S->L;
+   const ampe_model_params& p = S->p;
+   hipStream_t st = S->stream;
+   bool recompute = !fd || !S->s.lag_quat_sidegrad;
+   S->st.nfe++;
+   if (ghost_fill(L, y, S->nc, st)) return -1;
+   const double* phi = y;
+   const double* q = S->iq >= 0 ? y + (long long)S->iq * S->ng : nullptr;
+   const double* c = S->ic >= 0 ? y + (long long)S->ic * S->ng : nullptr;
+   if (q && recompute && launch_quat_grad(L, p, q, nullptr, S->n2, st)) return -1;
+   if (c && kks(S, y)) return -1;
+   if (launch_phase_rhs(L, p, phi, q, S->cl, S->cs, ydot, st)) return -1;
+   if (q && launch_quat_rhs(L, p, phi, q, S->n2, ydot + (long long)S->iq * S->ng, st)) return -1;
+   if (c) {
+      if (recompute && launch_conc_coeffs(L, p, phi, S->cl, S->cs, S->Dc, S->Dphi, st)) return -1;
+      if (launch_conc_rhs(L, p, c, phi, S->Dc, S->Dphi, ydot + (long long)S->ic * S->ng, st))
+         return -1;
+   }
+   return 0;
+}
+
+// CVSpgmrPrecondSet (source/QuatIntegrator.cc:3300-3376)
+static int psetup(Sim* S, double* y, double gamma)
+{
+   const Level& L = *S->L;
+   const ampe_model_params& p = S->p;
+   hipStream_t st = S->stream;
+   S->st.npe++;
+   if (ghost_fill(L, y, S->nc, st)) return -1;
+   const double* phi = y;
+   const double* q = S->iq >= 0 ? y + (long long)S->iq * S->ng : nullptr;
+   const double* c = S->ic >= 0 ? y + (long long)S->ic * S->ng : nullptr;
+   if (q && launch_quat_grad(L, p, q, nullptr, S->n2, st)) return -1;
+   if (c && kks(S, y)) return -1;
+   // phase block: C = 1 + gamma M w g'' (source/PhaseFACOps.cc:33-51)
+   if (launch_phase_C(L, p, gamma, phi, S->Cphi, st)) return -1;
+   // composition block: D = -gamma D_c (source/ConcFACOps.cc:28-61)
+   if (c && vec_axpby(S->nf, -gamma, S->Dc, 0.0, S->Dc, S->Dconc, st)) return -1;
+   // orientation block (source/QuatFACOps.cc:735-829)
+   if (q) {
+      if (launch_quat_face_coef(L, p, phi, S->n2, S->fc, st)) return -1;
+      if (launch_sqrt_quat_mobility(L, p, phi, S->sqrtm, st)) return -1;
+   }
+   if (S->s.precision == AMPE_FAC_FP32) {
+      if (vec_to_f32(S->ng, S->Cphi, S->f_Cphi, st)) return -1;
+      if (c && vec_to_f32(S->nf, S->Dconc, S->f_Dconc, st)) return -1;
+      if (q) {
+         if (vec_to_f32(S->nf, S->fc, S->f_fc, st)) return -1;
+         if (vec_to_f32(S->ng, S->sqrtm, S->f_sqrtm, st)) return -1;
+      }
+   }
+   S->gamma_setup = gamma;
+   return 0;
+}
+
+template <typename T>
+static int smooth(Sim* S, const EllOp<T>& op, const T* r, T* e)
+{
+   const Level& L = *S->L;
+   hipStream_t st = S->stream;
+   const double* dx = S->p.dx;
+   bool want = S->s.fac_tol >= 0.0;
+   if (ghost_fill(L, e, 1, st)) return -1;
+   for (int sw = 0; sw < S->s.fac_sweeps; sw++) {
+      double mr = 0.0, mb = 0.0;
+      if (ghost_fill(L, e, 1, st)) return -1;
+      if (launch_rbgs<T>(L, op, dx, r, e, 0, want, st)) return -1;
+      if (want && reduce_partials(L, L.ncell_tiles, 1, 1, &mr, st)) return -1;
+      if (ghost_fill(L, e, 1, st)) return -1;
+      if (launch_rbgs<T>(L, op, dx, r, e, 1, want, st)) return -1;
+      if (want && reduce_partials(L, L.ncell_tiles, 1, 1, &mb, st)) return -1;
+      if (ghost_fill(L, e, 1, st)) return -1;
+      if (want && (mr > mb ? mr : mb) <= S->s.fac_tol) break;
+   }
+   return 0;
+}
+
+// Single-level FAC solve with the redblack coarse solver
+// (FACPreconditioner::solveSystem, source/samrai/FACPreconditioner.cc:267-429).
+template <typename T>
+static int fac_solve(Sim* S, const EllOp<T>& op, const T* f, T* u, T* e, T* r, double tol)
+{
+   const Level& L = *S->L;
+   hipStream_t st = S->stream;
+   const double* dx = S->p.dx;
+   AMPE_HIP_CHECK(hipMemsetAsync(u, 0, sizeof(T) * S->ng, st));
+   if (ghost_fill(L, u, 1, st)) return -1;
+   double ss = 0.0;
+   if (launch_residual<T>(L, op, dx, f, u, r, true, st)) return -1;
+   if (reduce_partials(L, L.ncell_tiles, 1, 0, &ss, st)) return -1;
+   double norm = std::sqrt(ss / (double)L.ncell);
+   int iters = 0;
+   do {
+      AMPE_HIP_CHECK(hipMemsetAsync(e, 0, sizeof(T) * S->ng, st));
+      if (smooth<T>(S, op, r, e)) return -1;
+      if (std::is_same<T, double>::value) {
+         if (vec_axpby(S->ng, 1.0, (const double*)u, 1.0, (const double*)e, (double*)u, st))
+            return -1;
+      } else {
+         if (vec_axpby_f((long long)S->ng, (float*)u, (const float*)e, st)) return -1;
+      }
+      if (ghost_fill(L, u, 1, st)) return -1;
+      if (launch_residual<T>(L, op, dx, f, u, r, true, st)) return -1;
+      if (reduce_partials(L, L.ncell_tiles, 1, 0, &ss, st)) return -1;
+      norm = std::sqrt(ss / (double)L.ncell);
+      iters++;
+   } while (norm > tol && iters < S->s.fac_max_cycles);
+   return norm < tol ? 0 : 1;
+}
+
+// Orientation block: divide by sqrt(M), red-black Gauss-Seidel cycles on
+// I + gamma sqrt(M) L sqrt(M), multiply by sqrt(M) (QuatSysSolver::
+// solveSystem, source/QuatSysSolver.cc:296-349; the hypre level solve of
+// QuatLevelSolver is replaced).
+template <typename T>
+static int quat_solve(Sim* S, const T* fc, const T* sm, const T* f, T* z, T* e, T* r,
+                      double tol)
+{
+   const Level& L = *S->L;
+   hipStream_t st = S->stream;
+   const double* dx = S->p.dx;
+   int ql = S->p.qlen;
+   double gamma = S->gamma_setup;
+   long long n = S->ng * ql;
+   AMPE_HIP_CHECK(hipMemsetAsync(z, 0, sizeof(T) * n, st));
+   if (ghost_fill(L, z, ql, st)) return -1;
+   double ss = 0.0;
+   if (launch_quat_residual<T>(L, fc, sm, gamma, dx, ql, f, z, r, true, st)) return -1;
+   if (reduce_partials(L, L.ncell_tiles, 1, 0, &ss, st)) return -1;
+   double norm = std::sqrt(ss / ((double)ql * L.ncell));
+   int iters = 0;
+   do {
+      AMPE_HIP_CHECK(hipMemsetAsync(e, 0, sizeof(T) * n, st));
+      if (ghost_fill(L, e, ql, st)) return -1;
+      for (int sw = 0; sw < S->s.quat_sweeps; sw++) {
+         if (ghost_fill(L, e, ql, st)) return -1;
+         if (launch_quat_rbgs<T>(L, fc, sm, gamma, dx, ql, r, e, 0, st)) return -1;
+         if (ghost_fill(L, e, ql, st)) return -1;
+         if (launch_quat_rbgs<T>(L, fc, sm, gamma, dx, ql, r, e, 1, st)) return -1;
+         if (ghost_fill(L, e, ql, st)) return -1;
+      }
+      if (std::is_same<T, double>::value) {
+         if (vec_axpby(n, 1.0, (const double*)z, 1.0, (const double*)e, (double*)z, st))
+            return -1;
+      } else {
+         if (vec_axpby_f(n, (float*)z, (const float*)e, st)) return -1;
+      }
+      if (ghost_fill(L, z, ql, st)) return -1;
+      if (launch_quat_residual<T>(L, fc, sm, gamma, dx, ql, f, z, r, true, st)) return -1;
+      if (reduce_partials(L, L.ncell_tiles, 1, 0, &ss, st)) return -1;
+      norm = std::sqrt(ss / ((double)ql * L.ncell));
+      iters++;
+   } while (norm > tol && iters < S->s.quat_max_cycles);
+   return norm < tol ? 0 : 1;
+}
+
+// CVSpgmrPrecondSolve (source/QuatIntegrator.cc:3666-3771)
+static int psolve(Sim* S, const double* r, double* z, double delta)
+{
+   const ampe_model_params& p = S->p;
+   hipStream_t st = S->stream;
+   long long ng = S->ng;
+   int ret = 0, rc;
+   S->st.nps++;
+   AMPE_HIP_CHECK(hipMemsetAsync(z, 0, sizeof(double) * S->nv, st));
+   double dphi = -S->gamma_setup * p.epsilon_phi * p.epsilon_phi;
+   int ql = S->iq >= 0 ? p.qlen : 1;
+   if (S->s.precision == AMPE_FAC_FP32) {
+      EllOp<float> ph;
+      ph.Dc = (float)dphi;
+      ph.C = S->f_Cphi;
+      ph.Cc = 1.0f;
+      ph.Mc = (float)p.phi_mobility;
+      if (vec_to_f32(ng, r, S->f_f, st)) return -1;
+      rc = fac_solve<float>(S, ph, S->f_f, S->f_u, S->f_e, S->f_r, delta);
+      if (rc < 0) return -1;
+      ret |= rc;
+      if (vec_to_f64(ng, S->f_u, z, st)) return -1;
+      if (S->iq >= 0) {
+         const double* rq = r + (long long)S->iq * ng;
+         // divide by sqrt(M) in FP64, then round
+         if (vec_div_comp(S, rq, S->sqrtm, S->fac_u, ql)) return -1;
+         if (vec_to_f32(ng * ql, S->fac_u, S->f_f, st)) return -1;
+         rc = quat_solve<float>(S, S->f_fc, S->f_sqrtm, S->f_f, S->f_u, S->f_e, S->f_r, delta);
+         if (rc < 0) return -1;
+         ret |= rc;
+         if (vec_to_f64(ng * ql, S->f_u, z + (long long)S->iq * ng, st)) return -1;
+         if (vec_mul_comp(S, z + (long long)S->iq * ng, S->sqrtm, ql)) return -1;
+      }
+      if (S->ic >= 0) {
+         EllOp<float> co;
+         co.D = S->f_Dconc;
+         co.Cc = 1.0f;
+         co.Mc = (float)p.conc_mobility;
+         if (vec_to_f32(ng, r + (long long)S->ic * ng, S->f_f, st)) return -1;
+         rc = fac_solve<float>(S, co, S->f_f, S->f_u, S->f_e, S->f_r, delta);
+         if (rc < 0) return -1;
+         ret |= rc;
+         if (vec_to_f64(ng, S->f_u, z + (long long)S->ic * ng, st)) return -1;
+      }
+      return ret;
+   }
+   EllOp<double> ph;
+   ph.Dc = dphi;
+   ph.C = S->Cphi;
+   ph.Cc = 1.0;
+   ph.Mc = p.phi_mobility;
+   rc = fac_solve<double>(S, ph, r, z, S->fac_e, S->fac_r, delta);
+   if (rc < 0) return -1;
+   ret |= rc;
+   if (S->iq >= 0) {
+      const double* rq = r + (long long)S->iq * ng;
+      double* zq = z + (long long)S->iq * ng;
+      if (vec_div_comp(S, rq, S->sqrtm, S->fac_u, ql)) return -1;
+      rc = quat_solve<double>(S, S->fc, S->sqrtm, S->fac_u, zq, S->fac_e, S->fac_r, delta);
+      if (rc < 0) return -1;
+      ret |= rc;
+      if (vec_mul_comp(S, zq, S->sqrtm, ql)) return -1;
+   }
+   if (S->ic >= 0) {
+      EllOp<double> co;
+      co.D = S->Dconc;
+      co.Cc = 1.0;
+      co.Mc = p.conc_mobility;
+      rc = fac_solve<double>(S, co, r + (long long)S->ic * ng, z + (long long)S->ic * ng,
+                             S->fac_e, S->fac_r, delta);
+      if (rc < 0) return -1;
+      ret |= rc;
+   }
+   return ret;
+}
+
+int vec_div_comp(Sim* S, const double* x, const double* sm, double* y, int ncomp)
+{
+   for (int m = 0; m < ncomp; m++)
+      if (vec_div(S->ng, x + (long long)m * S->ng, sm, y + (long long)m * S->ng, S->stream))
+         return -1;
+   return 0;
+}
+
+int vec_mul_comp(Sim* S, double* x, const double* sm, int ncomp)
+{
+   for (int m = 0; m < ncomp; m++)
+      if (vec_mul(S->ng, x + (long long)m * S->ng, sm, x + (long long)m * S->ng, S->stream))
+         return -1;
+   return 0;
+}
+
+// ------------------------------------------------------------ Krylov
+static double cellvol(const Sim* S)
+{
+   const double* dx = S->p.dx;
+   return dx[0] * dx[1] * (S->p.ndim == 3 ? dx[2] : 1.0);
+}
+
+static int dot(Sim* S, const double* x, const double* y, double* out)
+{
+   double v;
+   if (vec_dot(*S->L, S->nc, x, y, &v, S->stream)) return -1;
+   *out = v * cellvol(S);
+   return 0;
+}
+
+static int wrms(Sim* S, const double* x, double* out)
+{
+   double v;
+   if (vec_sumsq_w(*S->L, S->nc, x, S->ewt, &v, S->stream)) return -1;
+   *out = std::sqrt(v / ((double)S->nc * S->L->ncell));
+   return 0;
+}
+
+// Jv by difference quotient (sigma = 1/||v||_wrms), out = v - gamma Jv
+static int atimes(Sim* S, double gamma, double* v, double* out)
+{
+   hipStream_t st = S->stream;
+   double nv;
+   if (wrms(S, v, &nv)) return -1;
+   double sig = 1.0 / nv;
+   if (vec_axpby(S->nv, sig, v, 1.0, S->ycur, S->work2, st)) return -1;
+   if (rhs_eval(S, S->work2, S->jv, true)) return -1;
+   double siginv = 1.0 / sig;
+   if (vec_axpby(S->nv, siginv, S->jv, -siginv, S->fcur, S->jv, st)) return -1;
+   if (vec_axpby(S->nv, 1.0, v, -gamma, S->jv, out, st)) return -1;
+   return 0;
+}
+
+// SPGMR, left preconditioned, MGS, x0 = 0, scaling by ewt.
+// Returns 0 converged, 1 residual reduced, 2 failure, -1 fatal.
+static int spgmr(Sim* S, double gamma, const double* bvec, double* x, double delta)
+{
+   hipStream_t st = S->stream;
+   int maxl = S->s.maxl;
+   long long nv = S->nv;
+   std::vector<double>& Hm = S->H;
+   Hm.assign((size_t)(maxl + 1) * (maxl + 1), 0.0);
+   int ld = maxl + 1;
+#define HH(i, j) Hm[(size_t)(i) * ld + (j)]
+   double givens[64], g[32], yv[32];
+   AMPE_HIP_CHECK(hipMemsetAsync(x, 0, sizeof(double) * nv, st));
+   int pr = psolve(S, bvec, S->work, delta);
+   if (pr < 0) return -1;
+   if (vec_mul(nv, S->ewt, S->work, S->V[0], st)) return -1;
+   double rho;
+   if (dot(S, S->V[0], S->V[0], &rho)) return -1;
+   rho = std::sqrt(rho);
+   double rho0 = rho;
+   if (rho <= delta) return 0;
+   double rinv = 1.0 / rho;
+   if (vec_axpby(nv, rinv, S->V[0], 0.0, S->V[0], S->V[0], st)) return -1;
+   std::memset(g, 0, sizeof(g));
+   g[0] = rho;
+   int lused = 0;
+   bool converged = false;
+   for (int l = 0; l < maxl; l++) {
+      S->st.nli++;
+      if (vec_div(nv, S->V[l], S->ewt, S->xk, st)) return -1;
+      if (atimes(S, gamma, S->xk, S->V[l + 1])) return -1;
+      if (psolve(S, S->V[l + 1], S->work, delta) < 0) return -1;
+      if (vec_mul(nv, S->ewt, S->work, S->V[l + 1], st)) return -1;
+      for (int i = 0; i <= l; i++) {
+         double hij;
+         if (dot(S, S->V[i], S->V[l + 1], &hij)) return -1;
+         HH(i, l) = hij;
+         if (vec_axpby(nv, 1.0, S->V[l + 1], -hij, S->V[i], S->V[l + 1], st)) return -1;
+      }
+      double hn;
+      if (dot(S, S->V[l + 1], S->V[l + 1], &hn)) return -1;
+      hn = std::sqrt(hn);
+      HH(l + 1, l) = hn;
+      for (int i = 0; i < l; i++) {
+         double c = givens[2 * i], s = givens[2 * i + 1];
+         double t1 = HH(i, l), t2 = HH(i + 1, l);
+         HH(i, l) = c * t1 - s * t2;
+         HH(i + 1, l) = s * t1 + c * t2;
+      }
+      double a = HH(l, l), bb = HH(l + 1, l), c, s;
+      if (bb == 0.0) {
+         c = 1.0;
+         s = 0.0;
+      } else if (std::fabs(bb) > std::fabs(a)) {
+         double t = a / bb;
+         s = -1.0 / std::sqrt(1.0 + t * t);
+         c = -s * t;
+      } else {
+         double t = bb / a;
+         c = 1.0 / std::sqrt(1.0 + t * t);
+         s = -c * t;
+      }
+      givens[2 * l] = c;
+      givens[2 * l + 1] = s;
+      HH(l, l) = c * a - s * bb;
+      HH(l + 1, l) = 0.0;
+      g[l + 1] = s * g[l];
+      g[l] = c * g[l];
+      rho = std::fabs(g[l + 1]);
+      lused = l + 1;
+      if (rho <= delta) {
+         converged = true;
+         break;
+      }
+      if (hn == 0.0) break;
+      double hinv = 1.0 / hn;
+      if (vec_axpby(nv, hinv, S->V[l + 1], 0.0, S->V[l + 1], S->V[l + 1], st)) return -1;
+   }
+   for (int i = lused - 1; i >= 0; i--) {
+      double sacc = g[i];
+      for (int j = i + 1; j < lused; j++) sacc -= HH(i, j) * yv[j];
+      yv[i] = sacc / HH(i, i);
+   }
+   for (int i = 0; i < lused; i++)
+      if (vec_axpby(nv, 1.0, x, yv[i], S->V[i], x, st)) return -1;
+   if (vec_div(nv, x, S->ewt, x, st)) return -1;
+#undef HH
+   if (converged) return 0;
+   return rho < rho0 ? 1 : 2;
+}
+
+// ------------------------------------------------------------ BDF step
+static int newton(Sim* S, double gamma, double delta, bool force_setup, bool* did_setup)
+{
+   hipStream_t st = S->stream;
+   long long nv = S->nv;
+   double tq4 = S->s.nlscoef;
+   bool call_setup = force_setup || S->nst == 0 ||
+                     std::fabs(gamma / S->gamma_setup - 1.0) > 0.3 ||
+                     S->nst >= S->nstlp + S->s.msbp;
+   *did_setup = false;
+   AMPE_HIP_CHECK(hipMemcpyAsync(S->ycur, S->ypred, sizeof(double) * nv,
+                                 hipMemcpyDeviceToDevice, st));
+   double delp = 0.0;
+   for (int m = 0; m < S->s.max_newton; m++) {
+      S->st.nni++;
+      if (rhs_eval(S, S->ycur, S->fcur, false)) return -1;
+      // b = gamma f - (y - a)
+      if (vec_axpby(nv, 1.0, S->ycur, -1.0, S->ahist, S->b, st)) return -1;
+      if (vec_axpby(nv, gamma, S->fcur, -1.0, S->b, S->b, st)) return -1;
+      if (m == 0 && call_setup) {
+         if (psetup(S, S->ycur, gamma)) return -1;
+         S->nstlp = S->nst;
+         S->crate = 1.0;
+         *did_setup = true;
+      }
+      int ls = spgmr(S, gamma, S->b, S->del, delta);
+      if (ls < 0) return -1;
+      if (ls == 2 || (ls == 1 && m > 0)) {
+         S->st.nlcf++;
+         return 1;
+      }
+      double del;
+      if (wrms(S, S->del, &del)) return -1;
+      if (vec_axpby(nv, 1.0, S->ycur, 1.0, S->del, S->ycur, st)) return -1;
+      if (m > 0) {
+         double r = del / delp;
+         S->crate = (0.3 * S->crate > r) ? 0.3 * S->crate : r;
+      }
+      double dcon = del * (S->crate < 1.0 ? S->crate : 1.0) / tq4;
+      if (dcon <= 1.0) return 0;
+      if (m >= 1 && del > 2.0 * delp) return 1;
+      delp = del;
+   }
+   return 1;
+}
+
+static int bdf_step(Sim* S)
+{
+   const Level& L = *S->L;
+   hipStream_t st = S->stream;
+   long long nv = S->nv, ng = S->ng;
+   if (S->ic >= 0) {
+      // resetRefPhaseConcentrations (source/QuatModel.cc:5218-5235)
+      AMPE_HIP_CHECK(hipMemcpyAsync(S->clr, S->cl, sizeof(double) * ng,
+                                    hipMemcpyDeviceToDevice, st));
+      AMPE_HIP_CHECK(hipMemcpyAsync(S->csr, S->cs, sizeof(double) * ng,
+                                    hipMemcpyDeviceToDevice, st));
+   }
+   double nrmfac = std::sqrt((double)S->nc * L.ncell * cellvol(S));
+   double delta = S->s.eplifac * S->s.nlscoef * nrmfac;
+   bool force_setup = false;
+   for (;;) {
+      double h = S->h, gamma;
+      if (!S->have_prev) {
+         AMPE_HIP_CHECK(hipMemcpyAsync(S->ypred, S->y, sizeof(double) * nv,
+                                       hipMemcpyDeviceToDevice, st));
+      } else {
+         double w = h / S->hprev;
+         // ypred = y + w (y - yprev) = (1 + w) y - w yprev, in the restated order
+         if (vec_axpby(nv, 1.0, S->y, -1.0, S->yprev, S->work, st)) return -1;
+         if (vec_axpby(nv, 1.0, S->y, w, S->work, S->ypred, st)) return -1;
+      }
+      if (!S->have_prev || S->s.max_order < 2) {
+         gamma = h;
+         AMPE_HIP_CHECK(hipMemcpyAsync(S->ahist, S->y, sizeof(double) * nv,
+                                       hipMemcpyDeviceToDevice, st));
+      } else {
+         double w = h / S->hprev;
+         double den = 1.0 + 2.0 * w;
+         gamma = h * (1.0 + w) / den;
+         double c1 = (1.0 + w) * (1.0 + w) / den, c2 = w * w / den;
+         if (vec_bdf2_hist(nv, c1, S->y, c2, S->yprev, S->ahist, st)) return -1;
+      }
+      bool did_setup = false;
+      int r = newton(S, gamma, delta, force_setup, &did_setup);
+      if (r < 0) return -1;
+      if (r == 0) {
+         if (vec_axpby(nv, 1.0, S->ycur, -1.0, S->ypred, S->acor, st)) return -1;
+         if (S->iq >= 0 && S->p.qlen > 1) {
+            AMPE_HIP_CHECK(hipMemsetAsync(S->corr, 0, sizeof(double) * nv, st));
+            if (launch_quat_project(L, S->p, S->ycur + (long long)S->iq * ng,
+                                    S->corr + (long long)S->iq * ng,
+                                    S->acor + (long long)S->iq * ng, st))
+               return -1;
+            if (vec_axpby(nv, 1.0, S->ycur, 1.0, S->corr, S->ycur, st)) return -1;
+         }
+         if (wrms(S, S->acor, &S->st.last_acnrm)) return -1;
+         std::swap(S->yprev, S->y);
+         std::swap(S->y, S->ycur);
+         S->have_prev = true;
+         S->hprev = h;
+         S->t += h;
+         S->nst++;
+         S->st.nst++;
+         S->h = (2.0 * h < S->s.dt) ? 2.0 * h : S->s.dt;
+         if (vec_ewt(nv, S->y, S->s.rtol, S->s.atol, S->ewt, st)) return -1;
+         return 0;
+      }
+      S->st.ncfn++;
+      if (!did_setup) {
+         force_setup = true;
+         continue;
+      }
+      S->h = 0.25 * h;
+      force_setup = true;
+      if (S->h < S->s.dt * 1.0e-8) return 1;
+   }
+}
+
+int sim_advance(Sim* S, int nsteps, ampe_stats* stats)
+{
+   int ret = 0;
+   AMPE_HIP_CHECK(hipEventRecord(S->ev0, S->stream));
+   for (int n = 0; n < nsteps; n++) {
+      ret = bdf_step(S);
+      if (ret) break;
+   }
+   AMPE_HIP_CHECK(hipEventRecord(S->ev1, S->stream));
+   AMPE_HIP_CHECK(hipEventSynchronize(S->ev1));
+   float ms = 0.f;
+   AMPE_HIP_CHECK(hipEventElapsedTime(&ms, S->ev0, S->ev1));
+   S->last_ms = ms;
+   S->st.t = S->t;
+   S->st.h = S->h;
+   if (stats) *stats = S->st;
+   return ret;
+}
+
+}  // namespace ampe

@@ -1,0 +1,183 @@
+"""ctypes binding of libampe_hip.so (C ABI in include/ampe_hip.h).
+
+There is no fallback: if the HIP library is missing or fails to load, every
+entry point raises.  Device buffers are passed as raw pointers (torch
+tensors' data_ptr() in the tests and the benchmark).
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+from ampe_amd.params import Box, ModelParams, SolverParams, Stats
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libampe_hip.so")
+_lib = None
+
+_vp = C.c_void_p
+_i = C.c_int
+_d = C.c_double
+_ll = C.c_longlong
+
+
+class EllipticOp(C.Structure):
+    _fields_ = [("D", _vp), ("D_const", _d), ("C", _vp), ("C_const", _d),
+                ("M", _vp), ("M_const", _d), ("dx", _d * 3)]
+
+
+_SIGS = {
+    "ampe_last_error": (C.c_char_p, []),
+    "ampe_version": (_i, []),
+    "ampe_level_create": (_i, [_i, C.POINTER(Box), _i, _i, C.POINTER(Box), C.POINTER(_vp)]),
+    "ampe_level_destroy": (_i, [_vp]),
+    "ampe_level_sizes": (_i, [_vp, C.POINTER(_ll), C.POINTER(_ll), C.POINTER(_ll)]),
+    "ampe_level_patch_offsets": (_i, [_vp, C.POINTER(_ll), C.POINTER(_ll)]),
+    "ampe_ghost_fill": (_i, [_vp, _vp, _i, _vp]),
+    "ampe_quat_grad": (_i, [_vp, C.POINTER(ModelParams), _vp, _vp, _vp, _vp]),
+    "ampe_phase_rhs": (_i, [_vp, C.POINTER(ModelParams), _vp, _vp, _vp, _vp, _vp, _vp]),
+    "ampe_quat_rhs": (_i, [_vp, C.POINTER(ModelParams), _vp, _vp, _vp, _vp, _vp]),
+    "ampe_quat_project": (_i, [_vp, C.POINTER(ModelParams), _vp, _vp, _vp, _vp]),
+    "ampe_conc_coeffs": (_i, [_vp, C.POINTER(ModelParams), _vp, _vp, _vp, _vp, _vp, _vp]),
+    "ampe_conc_rhs": (_i, [_vp, C.POINTER(ModelParams), _vp, _vp, _vp, _vp, _vp, _vp]),
+    "ampe_kks_solve": (_i, [_vp, C.POINTER(ModelParams), _vp, _vp, _vp, _vp, _vp, _vp,
+                            C.POINTER(_i), _vp]),
+    "ampe_tag_cells": (_i, [_vp, C.POINTER(ModelParams), _vp, _d, _vp, _vp]),
+    "ampe_fac_phase_coeffs": (_i, [_vp, C.POINTER(ModelParams), _d, _vp, _vp, _vp]),
+    "ampe_fac_smooth": (_i, [_vp, C.POINTER(EllipticOp), _vp, _vp, _i, _d, C.POINTER(_d), _vp]),
+    "ampe_fac_residual": (_i, [_vp, C.POINTER(EllipticOp), _vp, _vp, _vp, C.POINTER(_d), _vp]),
+    "ampe_vec_dot": (_i, [_vp, _i, _vp, _vp, C.POINTER(_d), _vp]),
+    "ampe_vec_wrms": (_i, [_vp, _i, _vp, _vp, C.POINTER(_d), _vp]),
+    "ampe_sim_create": (_i, [C.POINTER(ModelParams), C.POINTER(SolverParams), C.POINTER(Box),
+                             _i, C.POINTER(Box), C.POINTER(_vp)]),
+    "ampe_sim_destroy": (_i, [_vp]),
+    "ampe_sim_set_state": (_i, [_vp, _vp, _vp, _vp]),
+    "ampe_sim_get_state": (_i, [_vp, _vp, _vp, _vp, _vp, _vp]),
+    "ampe_sim_advance": (_i, [_vp, _i, C.POINTER(Stats)]),
+    "ampe_sim_last_time_ms": (_i, [_vp, C.POINTER(_d)]),
+}
+
+EXPORTED = tuple(_SIGS)
+
+
+class AmpeError(RuntimeError):
+    pass
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise AmpeError(f"HIP library not built: {LIB_PATH} (run __graft_entry__.build())")
+        L = C.CDLL(LIB_PATH)
+        for name, (res, args) in _SIGS.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def check(rc, what=""):
+    if rc < 0:
+        raise AmpeError(f"{what}: {lib().ampe_last_error().decode()}")
+    return rc
+
+
+def boxes_of(shape, ndim, origin=(0, 0, 0)):
+    """Single box covering an array of shape ([nz,] ny, nx)."""
+    b = Box()
+    n = list(shape[::-1]) + [1] * (3 - len(shape))
+    for a in range(3):
+        b.lo[a] = origin[a]
+        b.hi[a] = origin[a] + n[a] - 1
+    return b
+
+
+class Level:
+    """A patch level on the device (ampe_level)."""
+
+    def __init__(self, ndim, boxes, ghost=1, periodic=None):
+        self.ndim = ndim
+        if isinstance(boxes, Box):
+            boxes = [boxes]
+        self.boxes = list(boxes)
+        arr = (Box * len(self.boxes))(*self.boxes)
+        h = _vp()
+        per = C.byref(periodic) if periodic is not None else None
+        check(lib().ampe_level_create(ndim, arr, len(self.boxes), ghost, per, C.byref(h)),
+              "ampe_level_create")
+        self.h = h
+        ncg, nc = _ll(), _ll()
+        ns = (_ll * 3)()
+        check(lib().ampe_level_sizes(h, C.byref(ncg), C.byref(nc), ns), "ampe_level_sizes")
+        self.ncg, self.ncell = ncg.value, nc.value
+        self.nside = [ns[0], ns[1], ns[2]]
+        self.nside_tot = sum(self.nside)
+        np_ = len(self.boxes)
+        co = (_ll * np_)()
+        so = (_ll * (3 * np_))()
+        check(lib().ampe_level_patch_offsets(h, co, so), "ampe_level_patch_offsets")
+        self.cell_off = list(co)
+        self.side_off = [list(so[3 * p:3 * p + 3]) for p in range(np_)]
+
+    def __del__(self):
+        if getattr(self, "h", None) is not None and _lib is not None:
+            _lib.ampe_level_destroy(self.h)
+            self.h = None
+
+
+class Sim:
+    """Device-resident integrator (ampe_sim)."""
+
+    def __init__(self, p, s, boxes, periodic=None):
+        if isinstance(boxes, Box):
+            boxes = [boxes]
+        arr = (Box * len(boxes))(*boxes)
+        h = _vp()
+        per = C.byref(periodic) if periodic is not None else None
+        check(lib().ampe_sim_create(C.byref(p), C.byref(s), arr, len(boxes), per, C.byref(h)),
+              "ampe_sim_create")
+        self.h = h
+        self.p, self.s = p, s
+        self.ncell = sum((b.hi[0] - b.lo[0] + 1) * (b.hi[1] - b.lo[1] + 1) *
+                         ((b.hi[2] - b.lo[2] + 1) if p.ndim == 3 else 1) for b in boxes)
+
+    def __del__(self):
+        if getattr(self, "h", None) is not None and _lib is not None:
+            _lib.ampe_sim_destroy(self.h)
+            self.h = None
+
+    @staticmethod
+    def _ptr(a):
+        if a is None:
+            return None
+        assert a.dtype == np.float64 and a.flags.c_contiguous
+        return a.ctypes.data_as(_vp)
+
+    def set_state(self, phi, c, q):
+        phi = np.ascontiguousarray(phi, dtype=np.float64)
+        c = None if c is None else np.ascontiguousarray(c, dtype=np.float64)
+        q = None if q is None else np.ascontiguousarray(q, dtype=np.float64)
+        check(lib().ampe_sim_set_state(self.h, self._ptr(phi), self._ptr(c), self._ptr(q)),
+              "ampe_sim_set_state")
+
+    def get_state(self, shape):
+        phi = np.zeros(shape)
+        c = np.zeros(shape)
+        q = np.zeros((self.p.qlen,) + tuple(shape))
+        cl = np.zeros(shape)
+        cs = np.zeros(shape)
+        check(lib().ampe_sim_get_state(self.h, self._ptr(phi), self._ptr(c), self._ptr(q),
+                                       self._ptr(cl), self._ptr(cs)), "ampe_sim_get_state")
+        return phi, c, q, cl, cs
+
+    def advance(self, nsteps):
+        st = Stats()
+        rc = check(lib().ampe_sim_advance(self.h, nsteps, C.byref(st)), "ampe_sim_advance")
+        return rc, st.as_dict()
+
+    def last_time_ms(self):
+        ms = _d()
+        check(lib().ampe_sim_last_time_ms(self.h, C.byref(ms)), "ampe_sim_last_time_ms")
+        return ms.value

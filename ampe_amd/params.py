@@ -1,0 +1,335 @@
+"""ctypes mirrors of the C ABI structs in include/ampe_hip.h and the seeded
+workloads (initial conditions + model/solver parameters) the tests and the
+benchmark run.
+
+Model parameters follow the reference's input-deck keys (QuatModelParameters,
+source/QuatModelParameters.cc) with the values documented per workload below.
+"""
+import ctypes as C
+
+import numpy as np
+
+
+class Box(C.Structure):
+    _fields_ = [("lo", C.c_int * 3), ("hi", C.c_int * 3)]
+
+
+class CalphadPhase(C.Structure):
+    _fields_ = [("gA", C.c_double), ("gB", C.c_double), ("L", C.c_double * 4)]
+
+
+class ModelParams(C.Structure):
+    _fields_ = [
+        ("ndim", C.c_int),
+        ("dx", C.c_double * 3),
+        ("temperature", C.c_double),
+        ("epsilon_phi", C.c_double),
+        ("phi_well_scale", C.c_double),
+        ("phi_mobility", C.c_double),
+        ("phi_well_type", C.c_char),
+        ("energy_interp", C.c_char),
+        ("conc_interp", C.c_char),
+        ("avg_type", C.c_char),
+        ("flux_type", C.c_int),
+        ("aniso_nu", C.c_double),
+        ("aniso_knumber", C.c_int),
+        ("evolve_quat", C.c_int),
+        ("qlen", C.c_int),
+        ("H_parameter", C.c_double),
+        ("epsilon_q", C.c_double),
+        ("quat_mobility", C.c_double),
+        ("min_quat_mobility", C.c_double),
+        ("quat_mobility_scale", C.c_double),
+        ("quat_mobility_type", C.c_char),
+        ("orient_interp1", C.c_char),
+        ("orient_interp2", C.c_char),
+        ("floor_type", C.c_char),
+        ("gradient_floor", C.c_double),
+        ("with_conc", C.c_int),
+        ("D_liquid", C.c_double),
+        ("Q0_liquid", C.c_double),
+        ("D_solid", C.c_double),
+        ("Q0_solid", C.c_double),
+        ("conc_mobility", C.c_double),
+        ("fe_model", C.c_int),
+        ("vm_L", C.c_double),
+        ("vm_A", C.c_double),
+        ("quad_A", C.c_double * 2),
+        ("quad_ceq", C.c_double * 2),
+        ("quad_m", C.c_double * 2),
+        ("quad_Tref", C.c_double),
+        ("calphad", CalphadPhase * 2),
+        ("newton_max_its", C.c_int),
+        ("newton_tol", C.c_double),
+    ]
+
+
+class SolverParams(C.Structure):
+    _fields_ = [
+        ("dt", C.c_double),
+        ("max_order", C.c_int),
+        ("rtol", C.c_double),
+        ("atol", C.c_double),
+        ("maxl", C.c_int),
+        ("max_newton", C.c_int),
+        ("nlscoef", C.c_double),
+        ("eplifac", C.c_double),
+        ("msbp", C.c_int),
+        ("lag_quat_sidegrad", C.c_int),
+        ("fac_max_cycles", C.c_int),
+        ("fac_sweeps", C.c_int),
+        ("fac_tol", C.c_double),
+        ("quat_max_cycles", C.c_int),
+        ("quat_sweeps", C.c_int),
+        ("precision", C.c_int),
+        ("kks_guess", C.c_int),
+    ]
+
+
+class Stats(C.Structure):
+    _fields_ = [(n, C.c_longlong) for n in
+                ("nst", "nni", "nli", "nfe", "npe", "nps", "ncfn", "nlcf",
+                 "kks_calls", "kks_cells", "kks_iters", "kks_fallbacks")] + [
+        ("last_acnrm", C.c_double), ("t", C.c_double), ("h", C.c_double)]
+
+    def as_dict(self):
+        return {n: getattr(self, n) for n, _ in self._fields_}
+
+
+FLUX_SIMPLE, FLUX_ISOTROPIC, FLUX_ANISOTROPIC = 0, 1, 2
+FE_QUADRATIC, FE_CALPHAD_BINARY = 0, 1
+FP64, FAC_FP32 = 0, 1
+GUESS_PREVIOUS, GUESS_SURROGATE = 0, 1
+
+R_GAS = 8.314472
+
+
+# ---------------------------------------------------------------------------
+# CALPHAD AuNi end members (thermodynamic_data/calphadAuNi.dat:7-45): piecewise
+# G = a + b T + c T ln T + d2 T^2 + d3 T^3 + d7 T^7 + dm1/T + dm9 T^-9.
+# ---------------------------------------------------------------------------
+_AUNI = {
+    ("A", "L"): dict(Tc=[298.15, 933.51, 1337.58, 1735.8, 3000.0],
+                     a=[5613.147, -81023.261, 326614.987, 413.343],
+                     b=[97.446385, 1012.21732, -2025.7579, 155.893158],
+                     c=[-22.75455, -155.6947, 263.2523, -30.9616],
+                     d2=[-0.00385924, .08756015, -0.11821685, 0.0],
+                     d3=[3.79625e-7, -1.1518713e-05, 8.923845e-06, 0.0],
+                     dm1=[-25097.0, 10637210.0, -67999850.0, 0.0]),
+    ("A", "S"): dict(Tc=[298.15, 933.51, 1337.58, 1735.8, 3000.0],
+                     a=[-6938.853, -93575.261, 314062.987, -12138.657],
+                     b=[106.830495, 1021.60143, -2016.37379, 165.277268],
+                     c=[-22.75455, -155.6947, 263.2523, -30.9616],
+                     d2=[-0.00385924, .08756015, -0.11821685, 0.0],
+                     d3=[3.79625e-7, -1.1518713e-05, 8.923845e-06, 0.0],
+                     dm1=[-25097.0, 10637210.0, -67999850.0, 0.0]),
+    ("B", "L"): dict(Tc=[298.15, 1728.0, 3000.0],
+                     a=[11235.527, -9549.775], b=[108.457, 268.598],
+                     c=[-22.096, -43.1], d2=[-.0048407, 0.0],
+                     d7=[-3.82318e-21, 0.0]),
+    ("B", "S"): dict(Tc=[298.15, 1728.0, 3000.0],
+                     a=[-5179.159, -27840.655], b=[117.854, 279.135],
+                     c=[-22.096, -43.1], d2=[-.0048407, 0.0],
+                     dm9=[0.0, 1.12754e31]),
+}
+# Redlich-Kister L_i = L_i[0] + L_i[1] T (calphadAuNi.dat:49-63)
+_AUNI_LMIX = {"L": [(9500.0, -5.429), (1614.0, 0.0)],
+              "S": [(28696.0, -11.274), (-10945.0, 6.154), (2519.0, 0.0)]}
+
+
+def calphad_g(species, phase, T):
+    d = _AUNI[(species, phase)]
+    Tc = d["Tc"]
+    k = 0
+    while k < len(Tc) - 2 and T >= Tc[k + 1]:
+        k += 1
+    g = d["a"][k] + d["b"][k] * T + d["c"][k] * T * np.log(T)
+    for key, powr in (("d2", 2), ("d3", 3), ("d7", 7), ("dm1", -1), ("dm9", -9)):
+        if key in d:
+            g += d[key][k] * T ** powr
+    return float(g)
+
+
+def auni_phases(T, rk=None):
+    """[liquid, solid] CalphadPhase at temperature T.  rk optionally overrides
+    the Redlich-Kister (L0, L1) per phase: {'L': (L0, L1), 'S': (L0, L1)}."""
+    out = (CalphadPhase * 2)()
+    for i, ph in enumerate("LS"):
+        out[i].gA = calphad_g("A", ph, T)
+        out[i].gB = calphad_g("B", ph, T)
+        Ls = [a + b * T for a, b in _AUNI_LMIX[ph]]
+        if rk is not None:
+            Ls = list(rk[ph])
+        Ls = (Ls + [0.0] * 4)[:4]
+        for j in range(4):
+            out[i].L[j] = Ls[j]
+    return out
+
+
+# ---------------------------------------------------------------------------
+# Workloads
+# ---------------------------------------------------------------------------
+
+def solver_defaults(**kw):
+    s = SolverParams(dt=1.0e-3, max_order=2, rtol=1.0e-6, atol=1.0e-4, maxl=5,
+                     max_newton=3, nlscoef=0.1, eplifac=0.05, msbp=1,
+                     lag_quat_sidegrad=1, fac_max_cycles=1, fac_sweeps=2,
+                     fac_tol=-1.0, quat_max_cycles=1, quat_sweeps=2,
+                     precision=FP64, kks_guess=GUESS_PREVIOUS)
+    for k, v in kw.items():
+        setattr(s, k, v)
+    return s
+
+
+def two_grain_params(ndim=2, **kw):
+    """Quadratic-KKS two-grain model: values of tests/TwoGrainsQuadratic/
+    2d.input:23-58 (H, eps_q, eps_phi, mobilities, well scale, hbsm
+    interpolation, quadratic wells, D/Q), lengths rescaled by 1/0.05 so that
+    dx = 1 (eps -> eps/L, H -> H/L, D -> D/L^2 with L = 0.05 um)."""
+    L = 0.05
+    p = ModelParams()
+    p.ndim = ndim
+    for d in range(3):
+        p.dx[d] = 1.0
+    p.temperature = 873.0
+    p.epsilon_phi = 0.165 / L
+    p.phi_well_scale = 0.4125
+    p.phi_mobility = 200.0
+    p.phi_well_type = b"d"
+    p.energy_interp = b"h"
+    p.conc_interp = b"h"
+    p.avg_type = b"a"
+    p.flux_type = FLUX_SIMPLE
+    p.aniso_nu = 0.0
+    p.aniso_knumber = 4
+    p.evolve_quat = 1
+    p.qlen = 4
+    p.H_parameter = 0.001 / L
+    p.epsilon_q = 0.1 / L
+    p.quat_mobility = 200.0
+    p.min_quat_mobility = 1.0e-6
+    p.quat_mobility_scale = 1.0
+    p.quat_mobility_type = b"p"
+    p.orient_interp1 = b"q"
+    p.orient_interp2 = b"c"
+    p.floor_type = b"m"
+    p.gradient_floor = 1.0e-2
+    p.with_conc = 1
+    p.D_liquid = 5.6e4 / L ** 2
+    p.Q0_liquid = 55329.0
+    p.D_solid = 1.3e8 / L ** 2
+    p.Q0_solid = 156377.0
+    p.conc_mobility = 1.0
+    p.fe_model = FE_QUADRATIC
+    p.vm_L = 1.5e-5
+    p.vm_A = 1.5e-5
+    p.quad_A[0], p.quad_A[1] = 1.0e4, 1.0e4
+    p.quad_ceq[0], p.quad_ceq[1] = 0.05, 0.10
+    p.quad_m[0], p.quad_m[1] = 0.0, 0.0
+    p.quad_Tref = 873.0
+    p.calphad = auni_phases(873.0)
+    p.newton_max_its = 50
+    p.newton_tol = 1.0e-12
+    for k, v in kw.items():
+        setattr(p, k, v)
+    return p
+
+
+def polycrystal_params(ndim=3, seed=1002, **kw):
+    """Binary CALPHAD polycrystal at T = 1500 K in the reference's physical
+    units (um, pJ, s): interface energy sigma = 0.372678 J/m^2 and width
+    delta = 0.035 um (tests/KKScomposition/2d.input:32-35) converted as
+    QuatModelParameters does (eps_phi = sqrt(6 sigma delta), w = 3 sigma /
+    (16 delta), source/QuatModelParameters.cc:842-845), phase mobility 6.4 and
+    AuNi diffusivities D_S = 0.125, D_L = 1224.23 um^2/s (same deck), molar
+    volume 7.68e-6 m^3/mol, pbg interpolation, AuNi end members
+    (thermodynamic_data/calphadAuNi.dat) with seeded Redlich-Kister
+    L0 ~ U[-2e4, 2e4], L1 ~ U[-5e3, 5e3] J/mol per phase.  Orientation
+    coefficients of tests/TwoGrainsQuadratic/2d.input.  dx = 0.02 um.
+    No penalty term (the reference's penalty solver is not compiled)."""
+    rng = np.random.Generator(np.random.Philox(seed))
+    rk = {ph: (float(rng.uniform(-2e4, 2e4)), float(rng.uniform(-5e3, 5e3)))
+          for ph in "LS"}
+    sigma, delta = 0.372678, 0.035
+    p = two_grain_params(ndim=ndim)
+    for d in range(3):
+        p.dx[d] = 0.02
+    p.temperature = 1500.0
+    p.epsilon_phi = float(np.sqrt(6.0 * sigma * delta))
+    p.phi_well_scale = (3.0 * sigma / delta) / 16.0
+    p.phi_mobility = 6.4
+    p.energy_interp = b"p"
+    p.conc_interp = b"p"
+    p.H_parameter = 0.001
+    p.epsilon_q = 0.1
+    p.quat_mobility = 200.0
+    p.fe_model = FE_CALPHAD_BINARY
+    p.vm_L = 7.68e-6
+    p.vm_A = 7.68e-6
+    p.calphad = auni_phases(1500.0, rk)
+    p.D_liquid = 1224.23
+    p.Q0_liquid = 0.0
+    p.D_solid = 0.125
+    p.Q0_solid = 0.0
+    for k, v in kw.items():
+        setattr(p, k, v)
+    return p
+
+
+def random_unit_quaternions(rng, n):
+    """Uniform on S3 (Shoemake's method)."""
+    u1, u2, u3 = rng.random(n), rng.random(n), rng.random(n)
+    s1, s2 = np.sqrt(1 - u1), np.sqrt(u1)
+    return np.stack([s1 * np.sin(2 * np.pi * u2), s1 * np.cos(2 * np.pi * u2),
+                     s2 * np.sin(2 * np.pi * u3), s2 * np.cos(2 * np.pi * u3)])
+
+
+def two_grain_state(n=128, seed=1001, radius=32.0, width=4.0, ndim=2):
+    """Disk of radius 32 cells (phi = 1 inside, tanh interface of width 4) in
+    a phi = 0 matrix; c = 0.45 in the solid, 0.40 in the liquid, plus uniform
+    noise +-0.005; one S3-uniform quaternion per grain.  Arrays are (comp,)
+    (nz,) ny, nx with i fastest."""
+    rng = np.random.Generator(np.random.Philox(seed))
+    shape = (n, n) if ndim == 2 else (n, n, n)
+    idx = np.indices(shape).astype(np.float64) + 0.5
+    r = np.sqrt(sum((x - n / 2.0) ** 2 for x in idx))
+    phi = 0.5 * (1.0 - np.tanh((r - radius) / (width / 2.0)))
+    c = np.where(phi >= 0.5, 0.45, 0.40) + rng.uniform(-0.005, 0.005, shape)
+    qs = random_unit_quaternions(rng, 2)
+    inside = (phi >= 0.5)
+    q = np.empty((4,) + shape)
+    for m in range(4):
+        q[m] = np.where(inside, qs[m, 0], qs[m, 1])
+    return phi, c, q
+
+
+def polycrystal_state(n=256, seed=1002, ngrains=64, solid_fraction=0.7,
+                      width=3.0, ndim=3):
+    """Periodic Voronoi polycrystal: ngrains seeds U[0,n)^d, each with an
+    S3-uniform quaternion.  The smooth field d(x) = distance to the nearest
+    seed selects the liquid pockets: cells above the solid_fraction-quantile
+    of d are liquid (phi = 0), with tanh interfaces of the given width.
+    c ~ U[0.3, 0.5] per cell."""
+    rng = np.random.Generator(np.random.Philox(seed))
+    shape = (n,) * ndim
+    seeds = rng.uniform(0, n, (ngrains, ndim))
+    quats = random_unit_quaternions(rng, ngrains)
+    idx = [np.arange(n, dtype=np.float64) + 0.5 for _ in range(ndim)]
+    grids = np.meshgrid(*idx[::-1], indexing="ij")[::-1]  # x fastest
+    best = np.full(shape, np.inf)
+    label = np.zeros(shape, dtype=np.int32)
+    for g in range(ngrains):
+        d2 = np.zeros(shape)
+        for d in range(ndim):
+            t = np.abs(grids[d] - seeds[g, d])
+            t = np.minimum(t, n - t)
+            d2 += t * t
+        closer = d2 < best
+        best = np.where(closer, d2, best)
+        label = np.where(closer, g, label)
+    dist = np.sqrt(best)
+    thr = np.quantile(dist, solid_fraction)
+    phi = 0.5 * (1.0 - np.tanh((dist - thr) / (width / 2.0)))
+    c = rng.uniform(0.3, 0.5, shape)
+    q = quats[:, label]
+    return phi, c, q

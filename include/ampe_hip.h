@@ -1,0 +1,312 @@
+/*
+ * ampe_hip.h -- C ABI of the MI355X (gfx950) implementation of AMPE's
+ * per-timestep hot path: phase / orientation / composition right-hand sides,
+ * per-cell KKS phase concentrations, the FAC (red-black Gauss-Seidel)
+ * preconditioner blocks, CVODE-style vector reductions, AMR gradient tagging
+ * and a BDF Newton-Krylov time integrator that drives them.
+ *
+ * Conventions (all entry points):
+ *   - return 0 on success, >0 for a recoverable condition (e.g. a solver that
+ *     did not converge), <0 for a fatal error; ampe_last_error() gives text.
+ *     No entry point aborts the process.
+ *   - "device" pointers are HIP device allocations owned by the caller;
+ *     no entry point allocates on the hot path except the *_create calls.
+ *   - stream is an opaque hipStream_t (NULL = default stream).  Entry points
+ *     enqueue work and do not synchronise the device, except those that return
+ *     a host scalar (reductions, KKS iteration counts), which synchronise the
+ *     given stream.
+ *
+ * Data layout on a level (SAMRAI CellData/SideData semantics, i fastest):
+ *   cell field   : [component][patch][ghost box]  ghost box = box grown by g
+ *                  (in 2-D the k extent is 1 and carries no ghosts)
+ *   side field   : [direction][patch][faces]      faces of direction d have
+ *                  extent n_d + 1 along d and no ghosts
+ *   patches of one level are stored back to back; per-patch offsets are
+ *   available from ampe_level_patch_offsets().
+ *
+ * Each entry cites the reference interface it replaces (file:line in
+ * LLNL/AMPE).
+ */
+#ifndef AMPE_HIP_H
+#define AMPE_HIP_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef void* ampe_stream_t;
+
+/* Cell-centred index box, inclusive bounds (SAMRAI hier::Box). */
+typedef struct ampe_box {
+   int lo[3];
+   int hi[3];
+} ampe_box;
+
+enum ampe_flux_type {
+   AMPE_FLUX_SIMPLE = 0,      /* PhaseFluxStrategySimple -> gradient_flux   */
+   AMPE_FLUX_ISOTROPIC = 1,   /* PhaseFluxStrategyIsotropic (2-D only)      */
+   AMPE_FLUX_ANISOTROPIC = 2  /* PhaseFluxStrategyAnisotropy                */
+};
+
+enum ampe_free_energy {
+   AMPE_FE_QUADRATIC = 0,      /* QuadraticFreeEnergyStrategy              */
+   AMPE_FE_CALPHAD_BINARY = 1  /* CALPHADFreeEnergyStrategyBinary          */
+};
+
+/* Binary CALPHAD phase description evaluated at the (uniform) temperature:
+ * G(c) = c*gA + (1-c)*gB + RT[c ln c + (1-c) ln(1-c)]
+ *        + c(1-c) * sum_i L[i] (2c-1)^i                      [J/mol]
+ * c is the mole fraction of species A (first species of the database). */
+typedef struct ampe_calphad_phase {
+   double gA, gB;
+   double L[4];
+} ampe_calphad_phase;
+
+typedef struct ampe_model_params {
+   int ndim;              /* 2 or 3 */
+   double dx[3];          /* mesh spacing */
+   double temperature;    /* uniform temperature [K] */
+
+   /* phase field */
+   double epsilon_phi;    /* gradient energy coefficient epsilon_phi */
+   double phi_well_scale; /* w: double-well height */
+   double phi_mobility;   /* M_phi (uniform, SimpleQuatMobilityStrategy) */
+   char phi_well_type;    /* 'd' double well, 's' single well */
+   char energy_interp;    /* h(phi): 'p' pbg, 'h' harmonic, 'l' linear, ... */
+   char conc_interp;      /* h(phi) used for c = (1-h) cL + h cS */
+   char avg_type;         /* face average: 'a' arithmetic, 'h' harmonic */
+   int flux_type;         /* enum ampe_flux_type */
+   double aniso_nu;       /* 2-D: nu (epsilon_anisotropy); 3-D: eps4 */
+   int aniso_knumber;     /* 2-D anisotropy fold k */
+
+   /* orientation (quaternion) field */
+   int evolve_quat;       /* 0/1 */
+   int qlen;              /* quaternion components (4) */
+   double H_parameter;    /* misorientation factor is 2*H */
+   double epsilon_q;
+   double quat_mobility;  /* M_q max */
+   double min_quat_mobility;
+   double quat_mobility_scale; /* alt_scale_factor of quatmobility */
+   char quat_mobility_type;    /* 'p' pbg, 'e' exp, 'i' inverse */
+   char orient_interp1;   /* p1(phi) of the 2H T p1 |grad q| term */
+   char orient_interp2;   /* p2(phi) of the eps_q^2/2 p2 |grad q|^2 term */
+   char floor_type;       /* 'm' max, 't' tanh, 's' sqrt */
+   double gradient_floor;
+
+   /* composition */
+   int with_conc;         /* 0/1 */
+   double D_liquid, Q0_liquid, D_solid, Q0_solid; /* D exp(-Q/RT) */
+   double conc_mobility;  /* M_c multiplying div J */
+
+   /* free energy / KKS */
+   int fe_model;          /* enum ampe_free_energy */
+   double vm_L, vm_A;     /* molar volumes [m^3/mol] (energy factor 1e-6/vm) */
+   double quad_A[2];      /* quadratic well curvature, [0]=liquid [1]=solid */
+   double quad_ceq[2];    /* quadratic equilibrium composition */
+   double quad_m[2];      /* d ceq / dT */
+   double quad_Tref;
+   ampe_calphad_phase calphad[2]; /* [0]=liquid [1]=solid A */
+   int newton_max_its;    /* KKS Newton iteration cap */
+   double newton_tol;     /* KKS Newton residual tolerance */
+} ampe_model_params;
+
+enum ampe_precision { AMPE_FP64 = 0, AMPE_FAC_FP32 = 1 };
+enum ampe_kks_guess { AMPE_GUESS_PREVIOUS = 0, AMPE_GUESS_SURROGATE = 1 };
+
+typedef struct ampe_solver_params {
+   double dt;             /* fixed BDF step (initial = max step) */
+   int max_order;         /* 1 or 2 */
+   double rtol, atol;     /* CVODE scalar tolerances */
+   int maxl;              /* SPGMR Krylov dimension (max_krylov_dimension) */
+   int max_newton;        /* Newton iterations per attempt (CVODE maxcor) */
+   double nlscoef;        /* Newton convergence coefficient */
+   double eplifac;        /* linear / nonlinear tolerance ratio */
+   int msbp;              /* steps between preconditioner setups */
+   int lag_quat_sidegrad; /* lag |grad q| face norms in Jv evaluations */
+   int fac_max_cycles;    /* phase & composition FAC cycles */
+   int fac_sweeps;        /* RBGS sweeps per cycle (coarse_solver_max_iterations) */
+   double fac_tol;        /* RBGS early exit (coarse_solver_tolerance, <0: off) */
+   int quat_max_cycles;   /* orientation block cycles */
+   int quat_sweeps;       /* orientation block RBGS sweeps per cycle */
+   int precision;         /* enum ampe_precision */
+   int kks_guess;         /* enum ampe_kks_guess */
+} ampe_solver_params;
+
+/* Per-run counters (mirror QuatIntegrator.cc:2161-2170 nni/nli/nfe/npe/nps). */
+typedef struct ampe_stats {
+   long long nst, nni, nli, nfe, npe, nps, ncfn, nlcf;
+   long long kks_calls, kks_cells, kks_iters, kks_fallbacks;
+   double last_acnrm; /* WRMS norm of the last step's Newton correction */
+   double t;          /* simulated time reached */
+   double h;          /* next step size (dt unless a failure cut it) */
+} ampe_stats;
+
+/* ---------------------------------------------------------------- runtime */
+const char* ampe_last_error(void);
+int ampe_version(void);
+
+/* --------------------------------------------------------------- levels */
+/* A patch level: uneven boxes, one ghost width, optional periodic domain.
+ * Replaces the SAMRAI PatchLevel + ghost RefineSchedule role used by
+ * QuatIntegrator::fillScratch (source/QuatIntegrator.cc:2873-2957). */
+typedef struct ampe_level ampe_level;
+int ampe_level_create(int ndim, const ampe_box* boxes, int npatch, int ghost,
+                      const ampe_box* periodic_domain, ampe_level** out);
+int ampe_level_destroy(ampe_level* level);
+/* ncell_ghost: cells of one component incl. ghosts; ncell: interior cells;
+ * nside[d]: faces of direction d (all patches). */
+int ampe_level_sizes(const ampe_level* level, long long* ncell_ghost,
+                     long long* ncell, long long nside[3]);
+/* Host copy of per-patch offsets: cell_off[npatch], side_off[3*npatch]. */
+int ampe_level_patch_offsets(const ampe_level* level, long long* cell_off,
+                             long long* side_off);
+/* Fill ghost cells of ncomp component blocks from neighbours / periodic
+ * images (SAMRAI RefineSchedule::fillData on one level). */
+int ampe_ghost_fill(const ampe_level* level, double* field, int ncomp,
+                    ampe_stream_t stream);
+
+/* ------------------------------------------------------------ RHS kernels */
+/* |grad q| at cells (quat_grad_modulus_type "cells") and, if face_norm2 is
+ * not NULL, |grad q|^2 on every face from the full side gradient.
+ * Replaces QuatGradStrategy::computeDiffs/computeGradCell/computeGradSide/
+ * computeGradModulus (source/QuatGradStrategy.h:32-73; F77 quatdiffs
+ * source/fortran/2d/quatdiffs.m4:56, quatgrad_cell quatgrad.m4:12,
+ * quatgrad_side quatgrad.m4:157, quatgrad_modulus quatgrad.m4:882;
+ * face norm as in compute_face_coef2d quatfacops.m4:83-92). q needs filled
+ * ghosts; gmod is a cell field (interior written). */
+int ampe_quat_grad(const ampe_level* level, const ampe_model_params* p,
+                   const double* q, double* gmod, double* face_norm2,
+                   ampe_stream_t stream);
+
+/* Phase RHS: flux (Simple / Isotropic / Anisotropy), div, well, orientation
+ * terms, KKS driving force, times M_phi -- fused.  Replaces
+ * PhaseRHSStrategyWithQ::evaluateRHS (source/PhaseRHSStrategyWithQ.cc:90-312)
+ * with PhaseFluxStrategy::computeFluxes (source/PhaseFluxStrategy.h:26) and
+ * FreeEnergyStrategy::addDrivingForce (source/FreeEnergyStrategy.h:44-60).
+ * phi, q need filled ghosts; cl, cs are cell fields; rhs: cell field. */
+int ampe_phase_rhs(const ampe_level* level, const ampe_model_params* p,
+                   const double* phi, const double* q, const double* cl,
+                   const double* cs, double* rhs, ampe_stream_t stream);
+
+/* Orientation RHS with projection: face coefficients from lagged face
+ * |grad q|^2, flux fc * dq/dx, divergence, lambda, -M_q(...).  Replaces
+ * QuatSysSolver::evaluateRHS -> QuatFACOps::evaluateRHS
+ * (source/QuatFACOps.cc:1861-1890, 1958-2136; F77 compute_face_coef2d,
+ * compute_flux2d_from_gradq, compute_lambda_flux2d, add_quat_proj_op2d). */
+int ampe_quat_rhs(const ampe_level* level, const ampe_model_params* p,
+                  const double* phi, const double* q,
+                  const double* face_norm2, double* rhs, ampe_stream_t stream);
+
+/* CVODE projection of q onto |q| = 1 (QuatIntegrator::applyProjection,
+ * source/QuatIntegrator.cc:3911-3965; F77 project2d quatfacops.m4:940). */
+int ampe_quat_project(const ampe_level* level, const ampe_model_params* p,
+                      const double* q, double* corr, double* err,
+                      ampe_stream_t stream);
+
+/* Composition face coefficients D_c (pfm diffusion) and D_phi (phase
+ * coupling).  Replaces KKSCompositionRHSStrategy::setDiffusionCoeff
+ * (source/KKSCompositionRHSStrategy.cc:72-373; F77
+ * concentration_pfmdiffusion concentrationdiffusion.m4:14). */
+int ampe_conc_coeffs(const ampe_level* level, const ampe_model_params* p,
+                     const double* phi, const double* cl, const double* cs,
+                     double* Dc, double* Dphi, ampe_stream_t stream);
+
+/* Composition RHS: J = (Dc dc + Dphi dphi)/dx, rhs = M_c div J.  Replaces
+ * KKSCompositionRHSStrategy::computeFluxOnPatch + computerhsconcentration
+ * (source/KKSCompositionRHSStrategy.cc:377-441, QuatIntegrator.cc:2578-2690). */
+int ampe_conc_rhs(const ampe_level* level, const ampe_model_params* p,
+                  const double* c, const double* phi, const double* Dc,
+                  const double* Dphi, double* rhs, ampe_stream_t stream);
+
+/* Per-cell KKS phase concentrations over the ghost box, Newton from
+ * (cl_ref, cs_ref) (or the closed form for quadratic wells).  Replaces
+ * PhaseConcentrationsStrategy::computePhaseConcentrations
+ * (source/PhaseConcentrationsStrategy.cc:26-127) with the CALPHAD binary and
+ * quadratic OnPatch strategies.  *max_iters (host) receives the largest
+ * Newton iteration count, or -1 if a cell did not converge. */
+int ampe_kks_solve(const ampe_level* level, const ampe_model_params* p,
+                   const double* phi, const double* c, const double* cl_ref,
+                   const double* cs_ref, double* cl, double* cs,
+                   int* max_iters, ampe_stream_t stream);
+
+/* AMR gradient detector: tags = 1 where |grad phi * dx|^2 >= tol^2, else 0
+ * (QuatModel::tagGradientDetectorCells source/QuatModel.cc:3476-3602,
+ * F77 tagfromgrads quattag.m4:9).  tags: int, interior cells per patch. */
+int ampe_tag_cells(const ampe_level* level, const ampe_model_params* p,
+                   const double* phi, double tol, int* tags,
+                   ampe_stream_t stream);
+
+/* -------------------------------------------------------- FAC operators */
+/* Elliptic operator A u = M div(D grad u) + C u on one level
+ * (EllipticFACOps, source/EllipticFACOps.h:103).  Each coefficient is a
+ * field or, when the pointer is NULL, the constant. */
+typedef struct ampe_elliptic_op {
+   const double* D;  /* side field or NULL */
+   double D_const;
+   const double* C;  /* cell field (interior) or NULL */
+   double C_const;
+   const double* M;  /* cell field (interior) or NULL */
+   double M_const;
+   double dx[3];     /* mesh spacing */
+} ampe_elliptic_op;
+
+/* PhaseFACOps::setOperatorCoefficients / setC (source/PhaseFACOps.cc:33-183):
+ * C = 1 + gamma*M_phi*w*g''(phi) written to C (cell field). */
+int ampe_fac_phase_coeffs(const ampe_level* level, const ampe_model_params* p,
+                          double gamma, const double* phi, double* C,
+                          ampe_stream_t stream);
+
+/* num_sweeps red-black Gauss-Seidel sweeps on A u = rhs (u ghosted, ghost
+ * fill before each colour), with early exit when max|res| <= tol (tol >= 0).
+ * EllipticFACOps::smoothErrorByRedBlack (source/EllipticFACOps.cc:1067-1230)
+ * with efo_rbgswithfluxmax* (source/fortran/2d/ellipticfacops.m4:60-345).
+ * *maxres (host, may be NULL) receives the last sweep's max |residual|. */
+int ampe_fac_smooth(const ampe_level* level, const ampe_elliptic_op* op,
+                    const double* rhs, double* u, int num_sweeps, double tol,
+                    double* maxres, ampe_stream_t stream);
+
+/* res = rhs - A u (efo_compres*sca, source/fortran/2d/ellipticfacops.m4:346)
+ * after a ghost fill of u; *rms (host, may be NULL) = RMS norm of res. */
+int ampe_fac_residual(const ampe_level* level, const ampe_elliptic_op* op,
+                      const double* rhs, double* u, double* res, double* rms,
+                      ampe_stream_t stream);
+
+/* ----------------------------------------------------- vector reductions */
+/* Dot product of two ncomp-component cell fields over interior cells
+ * (SAMRAIVectorReal::dot as reached through Sundials_SAMRAIVector::dotWith,
+ * source/samrai/Sundials_SAMRAIVector.cc:156, without the cell-volume factor,
+ * which the caller applies).  Deterministic fixed-order reduction. */
+int ampe_vec_dot(const ampe_level* level, int ncomp, const double* x,
+                 const double* y, double* out, ampe_stream_t stream);
+/* Weighted RMS norm sqrt(sum (x w)^2 / N) (Sundials_SAMRAIVector::
+ * weightedRMSNorm, source/samrai/Sundials_SAMRAIVector.cc:177). */
+int ampe_vec_wrms(const ampe_level* level, int ncomp, const double* x,
+                  const double* w, double* out, ampe_stream_t stream);
+
+/* ------------------------------------------------------------- integrator */
+/* BDF (order <= 2, fixed step) Newton-Krylov integrator with SPGMR (MGS,
+ * left block preconditioner, difference-quotient Jv) driving the kernels
+ * above -- the QuatIntegrator/CVODESolver callback loop
+ * (source/QuatIntegrator.cc:2096-2233, 3134-3771;
+ * source/samrai/CVODESolver.cc:150-247). */
+typedef struct ampe_sim ampe_sim;
+int ampe_sim_create(const ampe_model_params* p, const ampe_solver_params* s,
+                    const ampe_box* boxes, int npatch,
+                    const ampe_box* periodic_domain, ampe_sim** out);
+int ampe_sim_destroy(ampe_sim* sim);
+/* Host arrays of interior cells, patch after patch, i fastest; q holds qlen
+ * component blocks.  c / q may be NULL when the model has no such field. */
+int ampe_sim_set_state(ampe_sim* sim, const double* phi, const double* c,
+                       const double* q);
+int ampe_sim_get_state(ampe_sim* sim, double* phi, double* c, double* q,
+                       double* cl, double* cs);
+/* Advance nsteps accepted BDF steps; >0 if a Newton solve failed. */
+int ampe_sim_advance(ampe_sim* sim, int nsteps, ampe_stats* stats);
+/* Device time (ms) of the last ampe_sim_advance, from HIP events. */
+int ampe_sim_last_time_ms(const ampe_sim* sim, double* ms);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

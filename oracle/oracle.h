@@ -1,0 +1,120 @@
+/*
+ * Oracle: a plain-C FP64 restatement of AMPE's per-timestep hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This code is the checker the HIP path is compared
+ * against and the CPU baseline that bench.py times; it is never linked into,
+ * called by or shipped with the product library (ampe_amd/).
+ *
+ * Every function cites the reference file:line it restates.  Arrays follow the
+ * same layout as the product (one patch): a cell field is the patch box grown
+ * by g ghost cells (no ghosts along k in 2-D), i fastest; a side field holds the
+ * x faces ((nx+1) ny nz), then the y faces (nx (ny+1) nz), then the z faces.
+ *
+ * Parity status (details in each function header):
+ *   pinned by closed forms / reference unit-test vectors: flux kernels,
+ *     computerhspbg, quaternion gradients, face coefficients, quaternion RHS,
+ *     projection, composition coefficients/flux/RHS, RBGS/residual, tagging,
+ *     quadratic KKS, CALPHAD free energy derivatives and KKS invariants.
+ *   parity unpinned: exact Thermo4PFM Newton iterate sequence; CVODE step
+ *     heuristics (this is a fixed-step BDF1/2 restatement); hypre PCG/PFMG
+ *     replaced by red-black Gauss-Seidel on the same symmetrised matrix.
+ */
+#ifndef AMPE_ORACLE_H
+#define AMPE_ORACLE_H
+
+#include "../include/ampe_hip.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct OGeom {
+   int ndim, nx, ny, nz, g, gz;
+   long sy, sz, ncg;       /* strides and size of a ghosted cell array */
+   long nfx, nfy, nfz;     /* face counts per direction */
+   double dx[3];
+} OGeom;
+
+void o_geom_init(OGeom* G, int ndim, int nx, int ny, int nz, int g,
+                 const double dx[3]);
+
+/* functions.f helpers */
+double o_interp_func(double phi, char type);
+double o_deriv_interp_func(double phi, char type);
+double o_second_deriv_interp_func(double phi, char type);
+double o_well_func(double phi, char type);
+double o_deriv_well_func(double phi, char type);
+double o_second_deriv_well_func(double phi, char type);
+double o_average_func(double a, double b, char type);
+double o_eval_grad_normi(double gn2, char floor_type, double floor2,
+                         double max_gni);
+
+/* free energies */
+double o_fenergy(const ampe_model_params* p, double c, int phase);
+double o_dfenergy(const ampe_model_params* p, double c, int phase);
+double o_d2fenergy(const ampe_model_params* p, double c, int phase);
+int o_kks_cell(const ampe_model_params* p, double c, double phi, double* cl,
+               double* cs);
+
+void o_ghost_fill_periodic(const OGeom* G, double* f, int ncomp);
+
+void o_phase_flux(const OGeom* G, const ampe_model_params* p,
+                  const double* phi, const double* q, double* flux);
+void o_quat_grad_modulus(const OGeom* G, const ampe_model_params* p,
+                         const double* q, double* gmod);
+void o_quat_face_norm2(const OGeom* G, const ampe_model_params* p,
+                       const double* q, double* n2);
+void o_phase_rhs(const OGeom* G, const ampe_model_params* p,
+                 const double* phi, const double* q, const double* cl,
+                 const double* cs, double* rhs);
+void o_quat_mobility(const OGeom* G, const ampe_model_params* p,
+                     const double* phi, double* mq);
+void o_quat_face_coef(const OGeom* G, const ampe_model_params* p,
+                      const double* phi, const double* n2, double* fc);
+void o_quat_rhs(const OGeom* G, const ampe_model_params* p, const double* phi,
+                const double* q, const double* n2, double* rhs);
+void o_quat_project(const OGeom* G, const ampe_model_params* p,
+                    const double* q, double* corr, double* err);
+void o_conc_coeffs(const OGeom* G, const ampe_model_params* p,
+                   const double* phi, const double* cl, const double* cs,
+                   double* Dc, double* Dphi);
+void o_conc_rhs(const OGeom* G, const ampe_model_params* p, const double* c,
+                const double* phi, const double* Dc, const double* Dphi,
+                double* rhs);
+int o_kks_solve(const OGeom* G, const ampe_model_params* p, const double* phi,
+                const double* c, const double* clr, const double* csr,
+                double* cl, double* cs);
+void o_tag_cells(const OGeom* G, const ampe_model_params* p,
+                 const double* phi, double tol, int* tags);
+
+/* elliptic FAC pieces */
+void o_phase_fac_C(const OGeom* G, const ampe_model_params* p, double gamma,
+                   const double* phi, double* C);
+double o_rbgs_color(const OGeom* G, const ampe_elliptic_op* op,
+                    const double* rhs, double* u, int color, double* flux);
+double o_fac_smooth(const OGeom* G, const ampe_elliptic_op* op,
+                    const double* rhs, double* u, int sweeps, double tol,
+                    double* flux);
+double o_fac_residual(const OGeom* G, const ampe_elliptic_op* op,
+                      const double* rhs, double* u, double* res,
+                      double* flux);
+
+/* vector reductions (interior cells) */
+double o_dot(const OGeom* G, int ncomp, const double* x, const double* y);
+double o_wrms(const OGeom* G, int ncomp, const double* x, const double* w);
+
+/* integrator */
+typedef struct OSim OSim;
+OSim* o_sim_create(const ampe_model_params* p, const ampe_solver_params* s,
+                   int nx, int ny, int nz);
+void o_sim_destroy(OSim* S);
+void o_sim_set_state(OSim* S, const double* phi, const double* c,
+                     const double* q);
+void o_sim_get_state(OSim* S, double* phi, double* c, double* q, double* cl,
+                     double* cs);
+int o_sim_advance(OSim* S, int nsteps, ampe_stats* st);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

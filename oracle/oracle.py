@@ -1,0 +1,187 @@
+"""ctypes binding of the FP64 C oracle (oracle/build/liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and
+bench.py's cpu_baseline leg, never by the product package.
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+from ampe_amd.params import ModelParams, SolverParams, Stats
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = os.path.join(_HERE, "build", "liboracle.so")
+_lib = None
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB):
+            build()
+        _lib = C.CDLL(_LIB)
+        _setup(_lib)
+    return _lib
+
+
+class OGeom(C.Structure):
+    _fields_ = [("ndim", C.c_int), ("nx", C.c_int), ("ny", C.c_int),
+                ("nz", C.c_int), ("g", C.c_int), ("gz", C.c_int),
+                ("sy", C.c_long), ("sz", C.c_long), ("ncg", C.c_long),
+                ("nfx", C.c_long), ("nfy", C.c_long), ("nfz", C.c_long),
+                ("dx", C.c_double * 3)]
+
+
+class EllipticOp(C.Structure):
+    _fields_ = [("D", C.c_void_p), ("D_const", C.c_double),
+                ("C", C.c_void_p), ("C_const", C.c_double),
+                ("M", C.c_void_p), ("M_const", C.c_double),
+                ("dx", C.c_double * 3)]
+
+
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int)
+_G = C.POINTER(OGeom)
+_P = C.POINTER(ModelParams)
+
+
+def _setup(L):
+    sig = {
+        "o_geom_init": (None, [_G, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, _dp]),
+        "o_interp_func": (C.c_double, [C.c_double, C.c_char]),
+        "o_deriv_interp_func": (C.c_double, [C.c_double, C.c_char]),
+        "o_second_deriv_interp_func": (C.c_double, [C.c_double, C.c_char]),
+        "o_well_func": (C.c_double, [C.c_double, C.c_char]),
+        "o_deriv_well_func": (C.c_double, [C.c_double, C.c_char]),
+        "o_second_deriv_well_func": (C.c_double, [C.c_double, C.c_char]),
+        "o_average_func": (C.c_double, [C.c_double, C.c_double, C.c_char]),
+        "o_eval_grad_normi": (C.c_double, [C.c_double, C.c_char, C.c_double, C.c_double]),
+        "o_fenergy": (C.c_double, [_P, C.c_double, C.c_int]),
+        "o_dfenergy": (C.c_double, [_P, C.c_double, C.c_int]),
+        "o_d2fenergy": (C.c_double, [_P, C.c_double, C.c_int]),
+        "o_kks_cell": (C.c_int, [_P, C.c_double, C.c_double, _dp, _dp]),
+        "o_ghost_fill_periodic": (None, [_G, _dp, C.c_int]),
+        "o_phase_flux": (None, [_G, _P, _dp, _dp, _dp]),
+        "o_quat_grad_modulus": (None, [_G, _P, _dp, _dp]),
+        "o_quat_face_norm2": (None, [_G, _P, _dp, _dp]),
+        "o_phase_rhs": (None, [_G, _P, _dp, _dp, _dp, _dp, _dp]),
+        "o_quat_mobility": (None, [_G, _P, _dp, _dp]),
+        "o_quat_face_coef": (None, [_G, _P, _dp, _dp, _dp]),
+        "o_quat_rhs": (None, [_G, _P, _dp, _dp, _dp, _dp]),
+        "o_quat_project": (None, [_G, _P, _dp, _dp, _dp]),
+        "o_conc_coeffs": (None, [_G, _P, _dp, _dp, _dp, _dp, _dp]),
+        "o_conc_rhs": (None, [_G, _P, _dp, _dp, _dp, _dp, _dp]),
+        "o_kks_solve": (C.c_int, [_G, _P, _dp, _dp, _dp, _dp, _dp, _dp]),
+        "o_tag_cells": (None, [_G, _P, _dp, C.c_double, _ip]),
+        "o_phase_fac_C": (None, [_G, _P, C.c_double, _dp, _dp]),
+        "o_fac_smooth": (C.c_double, [_G, C.POINTER(EllipticOp), _dp, _dp, C.c_int, C.c_double, _dp]),
+        "o_fac_residual": (C.c_double, [_G, C.POINTER(EllipticOp), _dp, _dp, _dp, _dp]),
+        "o_dot": (C.c_double, [_G, C.c_int, _dp, _dp]),
+        "o_wrms": (C.c_double, [_G, C.c_int, _dp, _dp]),
+        "o_sim_create": (C.c_void_p, [_P, C.POINTER(SolverParams), C.c_int, C.c_int, C.c_int]),
+        "o_sim_destroy": (None, [C.c_void_p]),
+        "o_sim_set_state": (None, [C.c_void_p, _dp, _dp, _dp]),
+        "o_sim_get_state": (None, [C.c_void_p, _dp, _dp, _dp, _dp, _dp]),
+        "o_sim_advance": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(Stats)]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+
+
+def ptr(a):
+    if a is None:
+        return None
+    assert a.dtype == np.float64 and a.flags.c_contiguous
+    return a.ctypes.data_as(_dp)
+
+
+def iptr(a):
+    assert a.dtype == np.int32 and a.flags.c_contiguous
+    return a.ctypes.data_as(_ip)
+
+
+class Patch:
+    """One periodic patch of the oracle layout (cell arrays with g ghosts)."""
+
+    def __init__(self, ndim, n, g=1, dx=(1.0, 1.0, 1.0)):
+        self.G = OGeom()
+        nx, ny, nz = (n if isinstance(n, (tuple, list)) else (n,) * 3)
+        dxa = (C.c_double * 3)(*dx)
+        lib().o_geom_init(C.byref(self.G), ndim, nx, ny, nz if ndim == 3 else 1,
+                          g, dxa)
+        G = self.G
+        self.ndim, self.nx, self.ny, self.nz, self.g = ndim, G.nx, G.ny, G.nz, g
+        self.gz = G.gz
+        self.ncg = G.ncg
+        self.nf = G.nfx + G.nfy + G.nfz
+        self.N = G.nx * G.ny * G.nz
+
+    @property
+    def gshape(self):
+        """numpy shape of one ghosted cell array (k, j, i)."""
+        return (self.nz + 2 * self.gz, self.ny + 2 * self.g, self.nx + 2 * self.g)
+
+    def cells(self, ncomp=1):
+        return np.zeros((ncomp,) + self.gshape)
+
+    def sides(self, ncomp=1):
+        return np.zeros((ncomp, self.nf))
+
+    def interior(self, a):
+        g, gz = self.g, self.gz
+        return a[..., gz:gz + self.nz, g:g + self.ny, g:g + self.nx]
+
+    def load(self, a, interior_vals):
+        """interior_vals: (..., [nz,] ny, nx) -> ghosted array a (periodic
+        ghost fill afterwards)."""
+        v = np.asarray(interior_vals, dtype=np.float64)
+        if self.ndim == 2 and (v.ndim < 3 or v.shape[-3] != 1 or
+                               v.shape[-3:] != (1, self.ny, self.nx)):
+            v = v.reshape(v.shape[:-2] + (1,) + v.shape[-2:])
+        self.interior(a)[...] = v
+        ncomp = int(np.prod(a.shape[:-3])) if a.ndim > 3 else 1
+        lib().o_ghost_fill_periodic(C.byref(self.G), ptr(a), ncomp)
+        return a
+
+
+class Sim:
+    """Oracle integrator handle."""
+
+    def __init__(self, p, s, n):
+        nx, ny, nz = (n if isinstance(n, (tuple, list)) else (n,) * 3)
+        self.p, self.s = p, s
+        self.h = lib().o_sim_create(C.byref(p), C.byref(s), nx, ny, nz)
+        self.shape = (ny, nx) if p.ndim == 2 else (nz, ny, nx)
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().o_sim_destroy(self.h)
+            self.h = None
+
+    def set_state(self, phi, c, q):
+        phi = np.ascontiguousarray(phi, dtype=np.float64)
+        c = np.ascontiguousarray(c, dtype=np.float64) if c is not None else None
+        q = np.ascontiguousarray(q, dtype=np.float64) if q is not None else None
+        lib().o_sim_set_state(self.h, ptr(phi), ptr(c), ptr(q))
+
+    def get_state(self):
+        phi = np.zeros(self.shape)
+        c = np.zeros(self.shape)
+        q = np.zeros((self.p.qlen,) + self.shape)
+        cl = np.zeros(self.shape)
+        cs = np.zeros(self.shape)
+        lib().o_sim_get_state(self.h, ptr(phi), ptr(c), ptr(q), ptr(cl), ptr(cs))
+        return phi, c, q, cl, cs
+
+    def advance(self, nsteps):
+        st = Stats()
+        ret = lib().o_sim_advance(self.h, nsteps, C.byref(st))
+        return ret, st.as_dict()

@@ -1,0 +1,669 @@
+/*
+ * Oracle integrator: FP64 CPU restatement of QuatIntegrator's CVODE loop on
+ * one periodic patch.  TEST INFRASTRUCTURE ONLY (see oracle.h).
+ *
+ * Follows:
+ *   - RHS callback QuatIntegrator::evaluateRHSFunction
+ *     (source/QuatIntegrator.cc:3134-3298) with setCoefficients (:2994-3083),
+ *     lagged quaternion side gradients in Jv evaluations (:3189, :2805-2811)
+ *     and composition coefficients refreshed only when not lagged (:3245).
+ *   - Preconditioner setup CVSpgmrPrecondSet (:3300-3376) and solve
+ *     CVSpgmrPrecondSolve (:3666-3771): phase, orientation and composition
+ *     blocks; single-level FAC = FACPreconditioner::solveSystem
+ *     (source/samrai/FACPreconditioner.cc:267-429) with the "redblack"
+ *     coarse solve (source/EllipticFACOps.cc:1356-1385).  The orientation
+ *     block's hypre PCG/PFMG level solve (source/QuatLevelSolver.cc:1210-1339)
+ *     is replaced by red-black Gauss-Seidel on the same symmetrised matrix
+ *     (set_j_ij / set_stencil, source/fortran/2d/quatlevelsolver.m4:9-120):
+ *     parity unpinned against hypre.
+ *   - CVODE semantics as configured in source/samrai/CVODESolver.cc:150-247
+ *     and source/QuatIntegrator.cc:1571-1596: Newton-Krylov with SPGMR (MGS,
+ *     left preconditioning, ewt scaling, maxl, no restarts), difference
+ *     quotient Jv with sigma = 1/||v||_wrms, lsetup frequency msbp,
+ *     gamma-ratio test 0.3, Newton test del*min(1,crate) <= nlscoef,
+ *     linear tolerance eplifac*nlscoef*sqrt(sum cv), projection of q after
+ *     each step (applyProjection :3911-3965).  Step size and order are fixed
+ *     (BDF1 for the first step, BDF2 after): CVODE's adaptive heuristics are
+ *     parity unpinned.
+ */
+#include "oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#include <stdio.h>
+
+static int dbg(void)
+{
+   static int v = -1;
+   if (v < 0) v = getenv("ORACLE_DEBUG") ? 1 : 0;
+   return v;
+}
+
+struct OSim {
+   ampe_model_params p;
+   ampe_solver_params s;
+   OGeom G;
+   int nc, iq, ic;
+   long nv, nf;
+   double t, h, hprev, gamma_setup, crate;
+   long nst, nstlp;
+   int have_prev;
+   double *y, *yprev, *ypred, *ycur, *ahist, *fcur, *b, *del, *ewt, *work,
+       *work2, *jv, *xk, *corr, *acor;
+   double** V;
+   double *cl, *cs, *clr, *csr;
+   double *n2, *Dc, *Dphi;
+   double *Cphi, *Dconc, *fc, *sqrtm, *mq;
+   double *fac_u, *fac_e, *fac_r, *fac_flux;
+   double* H;
+   ampe_stats st;
+   int fail;
+};
+
+static double* vec(long n) { return (double*)calloc((size_t)n, sizeof(double)); }
+
+OSim* o_sim_create(const ampe_model_params* p, const ampe_solver_params* s,
+                   int nx, int ny, int nz)
+{
+   OSim* S = (OSim*)calloc(1, sizeof(OSim));
+   S->p = *p;
+   S->s = *s;
+   o_geom_init(&S->G, p->ndim, nx, ny, nz, 1, p->dx);
+   S->nc = 1 + (p->with_conc ? 1 : 0) + (p->evolve_quat ? p->qlen : 0);
+   S->ic = p->with_conc ? 1 : -1;
+   S->iq = p->evolve_quat ? 1 + (p->with_conc ? 1 : 0) : -1;
+   S->nv = (long)S->nc * S->G.ncg;
+   S->nf = S->G.nfx + S->G.nfy + S->G.nfz;
+   double** vs[] = {&S->y, &S->yprev, &S->ypred, &S->ycur, &S->ahist, &S->fcur,
+                    &S->b, &S->del, &S->ewt, &S->work, &S->work2, &S->jv,
+                    &S->xk, &S->corr, &S->acor};
+   for (size_t i = 0; i < sizeof(vs) / sizeof(vs[0]); i++) *vs[i] = vec(S->nv);
+   S->V = (double**)calloc((size_t)s->maxl + 1, sizeof(double*));
+   for (int i = 0; i <= s->maxl; i++) S->V[i] = vec(S->nv);
+   long ng = S->G.ncg;
+   S->cl = vec(ng);
+   S->cs = vec(ng);
+   S->clr = vec(ng);
+   S->csr = vec(ng);
+   S->Cphi = vec(ng);
+   S->sqrtm = vec(ng);
+   S->mq = vec(ng);
+   S->fac_u = vec(ng * (p->evolve_quat ? p->qlen : 1));
+   S->fac_e = vec(ng * (p->evolve_quat ? p->qlen : 1));
+   S->fac_r = vec(ng * (p->evolve_quat ? p->qlen : 1));
+   S->n2 = vec(S->nf);
+   S->Dc = vec(S->nf);
+   S->Dphi = vec(S->nf);
+   S->Dconc = vec(S->nf);
+   S->fc = vec(S->nf);
+   S->fac_flux = vec(S->nf);
+   S->H = vec((long)(s->maxl + 1) * (s->maxl + 1));
+   S->h = s->dt;
+   return S;
+}
+
+void o_sim_destroy(OSim* S)
+{
+   if (!S) return;
+   double* fs[] = {S->y, S->yprev, S->ypred, S->ycur, S->ahist, S->fcur, S->b,
+                   S->del, S->ewt, S->work, S->work2, S->jv, S->xk, S->corr,
+                   S->acor, S->cl, S->cs, S->clr, S->csr, S->Cphi, S->sqrtm,
+                   S->mq, S->fac_u, S->fac_e, S->fac_r, S->n2, S->Dc, S->Dphi,
+                   S->Dconc, S->fc, S->fac_flux, S->H};
+   for (size_t i = 0; i < sizeof(fs) / sizeof(fs[0]); i++) free(fs[i]);
+   for (int i = 0; i <= S->s.maxl; i++) free(S->V[i]);
+   free(S->V);
+   free(S);
+}
+
+#define CIG(G, i, j, k) \
+   ((long)((i) + (G)->g) + (long)((j) + (G)->g) * (G)->sy + (long)((k) + (G)->gz) * (G)->sz)
+
+static void load_field(const OGeom* G, double* dst, const double* src)
+{
+   long n = 0;
+   for (int k = 0; k < G->nz; k++)
+      for (int j = 0; j < G->ny; j++)
+         for (int i = 0; i < G->nx; i++) dst[CIG(G, i, j, k)] = src[n++];
+}
+
+static void store_field(const OGeom* G, double* dst, const double* src)
+{
+   long n = 0;
+   for (int k = 0; k < G->nz; k++)
+      for (int j = 0; j < G->ny; j++)
+         for (int i = 0; i < G->nx; i++) dst[n++] = src[CIG(G, i, j, k)];
+}
+
+static void update_ewt(OSim* S, const double* y)
+{
+   for (long n = 0; n < S->nv; n++)
+      S->ewt[n] = 1.0 / (S->s.rtol * fabs(y[n]) + S->s.atol);
+}
+
+void o_sim_set_state(OSim* S, const double* phi, const double* c,
+                     const double* q)
+{
+   const OGeom* G = &S->G;
+   memset(S->y, 0, sizeof(double) * S->nv);
+   load_field(G, S->y, phi);
+   if (S->ic >= 0) load_field(G, S->y + (long)S->ic * G->ncg, c);
+   if (S->iq >= 0)
+      for (int m = 0; m < S->p.qlen; m++)
+         load_field(G, S->y + (long)(S->iq + m) * G->ncg,
+                    q + (long)m * G->nx * G->ny * G->nz);
+   o_ghost_fill_periodic(G, S->y, S->nc);
+   if (S->ic >= 0) {
+      /* initial phase concentrations: KKS from the guess cL = cS = c */
+      const double* cc = S->y + (long)S->ic * G->ncg;
+      memcpy(S->clr, cc, sizeof(double) * G->ncg);
+      memcpy(S->csr, cc, sizeof(double) * G->ncg);
+      o_kks_solve(G, &S->p, S->y, cc, S->clr, S->csr, S->cl, S->cs);
+   }
+   S->t = 0.0;
+   S->nst = 0;
+   S->have_prev = 0;
+   memset(&S->st, 0, sizeof(S->st));
+   update_ewt(S, S->y);
+}
+
+void o_sim_get_state(OSim* S, double* phi, double* c, double* q, double* cl,
+                     double* cs)
+{
+   const OGeom* G = &S->G;
+   long N = (long)G->nx * G->ny * G->nz;
+   if (phi) store_field(G, phi, S->y);
+   if (c && S->ic >= 0) store_field(G, c, S->y + (long)S->ic * G->ncg);
+   if (q && S->iq >= 0)
+      for (int m = 0; m < S->p.qlen; m++)
+         store_field(G, q + (long)m * N, S->y + (long)(S->iq + m) * G->ncg);
+   if (cl && S->ic >= 0) store_field(G, cl, S->cl);
+   if (cs && S->ic >= 0) store_field(G, cs, S->cs);
+}
+
+/* evaluateRHSFunction (source/QuatIntegrator.cc:3134-3298).  y is ghost
+ * filled in place (the scratch copy of fillScratch is the vector itself). */
+static int rhs_eval(OSim* S, double* y, double* ydot, int fd)
+{
+   const OGeom* G = &S->G;
+   const ampe_model_params* p = &S->p;
+   long ng = G->ncg;
+   int recompute = (!fd) || (!S->s.lag_quat_sidegrad);
+   S->st.nfe++;
+   o_ghost_fill_periodic(G, y, S->nc);
+   const double* phi = y;
+   const double* q = (S->iq >= 0) ? y + (long)S->iq * ng : NULL;
+   const double* c = (S->ic >= 0) ? y + (long)S->ic * ng : NULL;
+   if (q && recompute) o_quat_face_norm2(G, p, q, S->n2);
+   if (c) {
+      int it = o_kks_solve(G, p, phi, c, S->clr, S->csr, S->cl, S->cs);
+      S->st.kks_calls++;
+      S->st.kks_cells += ng;
+      if (it < 0) return -1;
+      S->st.kks_iters += it;
+   }
+   o_phase_rhs(G, p, phi, q, S->cl, S->cs, ydot);
+   if (q) o_quat_rhs(G, p, phi, q, S->n2, ydot + (long)S->iq * ng);
+   if (c) {
+      if (recompute) o_conc_coeffs(G, p, phi, S->cl, S->cs, S->Dc, S->Dphi);
+      o_conc_rhs(G, p, c, phi, S->Dc, S->Dphi, ydot + (long)S->ic * ng);
+   }
+   return 0;
+}
+
+/* CVSpgmrPrecondSet (source/QuatIntegrator.cc:3300-3376) */
+static int psetup(OSim* S, double* y, double gamma)
+{
+   const OGeom* G = &S->G;
+   const ampe_model_params* p = &S->p;
+   long ng = G->ncg;
+   S->st.npe++;
+   o_ghost_fill_periodic(G, y, S->nc);
+   const double* phi = y;
+   const double* q = (S->iq >= 0) ? y + (long)S->iq * ng : NULL;
+   const double* c = (S->ic >= 0) ? y + (long)S->ic * ng : NULL;
+   if (q) o_quat_face_norm2(G, p, q, S->n2);
+   if (c) {
+      int it = o_kks_solve(G, p, phi, c, S->clr, S->csr, S->cl, S->cs);
+      S->st.kks_calls++;
+      S->st.kks_cells += ng;
+      if (it < 0) return -1;
+      S->st.kks_iters += it;
+   }
+   /* phase block: M = M_phi, C = 1 + gamma M w g'', D = -gamma eps^2
+    * (source/PhaseFACOps.cc:33-51) */
+   o_phase_fac_C(G, p, gamma, phi, S->Cphi);
+   /* composition block: D = -gamma D_c (lagged), C = 1, M = M_c
+    * (source/ConcFACOps.cc:28-61) */
+   if (c)
+      for (long f = 0; f < S->nf; f++) S->Dconc[f] = -gamma * S->Dc[f];
+   /* orientation block: face coefficients and sqrt(M_q) with ghosts
+    * (source/QuatFACOps.cc:735-829) */
+   if (q) {
+      o_quat_face_coef(G, p, phi, S->n2, S->fc);
+      o_quat_mobility(G, p, phi, S->mq);
+      for (long n = 0; n < ng; n++) S->sqrtm[n] = sqrt(S->mq[n]);
+   }
+   S->gamma_setup = gamma;
+   return 0;
+}
+
+/* Single-level FAC solve of A u = f with the redblack coarse solver
+ * (FACPreconditioner::solveSystem, source/samrai/FACPreconditioner.cc:267-429).
+ * Returns 0 when the RMS residual is below tol. */
+static int fac_solve(OSim* S, const ampe_elliptic_op* op, const double* f,
+                     double* u, double tol)
+{
+   const OGeom* G = &S->G;
+   long ng = G->ncg;
+   memset(u, 0, sizeof(double) * ng);
+   double norm = o_fac_residual(G, op, f, u, S->fac_r, S->fac_flux);
+   int iters = 0;
+   do {
+      memset(S->fac_e, 0, sizeof(double) * ng);
+      o_fac_smooth(G, op, S->fac_r, S->fac_e, S->s.fac_sweeps, S->s.fac_tol,
+                   S->fac_flux);
+      for (long n = 0; n < ng; n++) u[n] = u[n] + S->fac_e[n];
+      norm = o_fac_residual(G, op, f, u, S->fac_r, S->fac_flux);
+      iters++;
+   } while (norm > tol && iters < S->s.fac_max_cycles);
+   return norm < tol ? 0 : 1;
+}
+
+/* Orientation block operator A = I + gamma sqrt(M) L sqrt(M),
+ * L: off-diagonal fc/h^2, diagonal -sum fc/h^2 (set_j_ij2d/3d and
+ * set_stencil2d/3d, source/fortran/2d/quatlevelsolver.m4:9-120). */
+#define QFX(i, j, k) ((long)(i) + (long)(G->nx + 1) * ((long)(j) + (long)G->ny * (k)))
+#define QFY(i, j, k) (G->nfx + (long)(i) + (long)G->nx * ((long)(j) + (long)(G->ny + 1) * (k)))
+#define QFZ(i, j, k) (G->nfx + G->nfy + (long)(i) + (long)G->nx * ((long)(j) + (long)G->ny * (k)))
+
+static double quat_apply_row(const OSim* S, double gamma, const double* z,
+                             int i, int j, int k, double* center)
+{
+   const OGeom* G = &S->G;
+   const double* fc = S->fc;
+   const double* sm = S->sqrtm;
+   long c0 = CIG(G, i, j, k);
+   double hx = 1.0 / (G->dx[0] * G->dx[0]), hy = 1.0 / (G->dx[1] * G->dx[1]);
+   double ow = fc[QFX(i, j, k)] * hx, oe = fc[QFX(i + 1, j, k)] * hx;
+   double os = fc[QFY(i, j, k)] * hy, on = fc[QFY(i, j + 1, k)] * hy;
+   double diag = -ow - oe - os - on;
+   double ob = 0.0, ot = 0.0;
+   if (G->ndim == 3) {
+      double hz = 1.0 / (G->dx[2] * G->dx[2]);
+      ob = fc[QFZ(i, j, k)] * hz;
+      ot = fc[QFZ(i, j, k + 1)] * hz;
+      diag = diag - ob - ot;
+   }
+   double fac = gamma * sm[c0];
+   *center = fac * diag * sm[c0] + 1.0;
+   double r = *center * z[c0] + fac * ow * sm[c0 - 1] * z[c0 - 1] +
+              fac * oe * sm[c0 + 1] * z[c0 + 1] +
+              fac * os * sm[c0 - G->sy] * z[c0 - G->sy] +
+              fac * on * sm[c0 + G->sy] * z[c0 + G->sy];
+   if (G->ndim == 3)
+      r = r + fac * ob * sm[c0 - G->sz] * z[c0 - G->sz] +
+          fac * ot * sm[c0 + G->sz] * z[c0 + G->sz];
+   return r;
+}
+
+static double quat_rbgs_color(OSim* S, double gamma, const double* rhs,
+                              double* z, int color)
+{
+   const OGeom* G = &S->G;
+   double maxres = 0.0;
+#pragma omp parallel for reduction(max : maxres)
+   for (int k = 0; k < G->nz; k++)
+      for (int j = 0; j < G->ny; j++) {
+         int x = (G->ndim == 3) ? (j + k) : j;
+         int io = ((x - (x / 2) * 2) != color) ? 1 : 0;
+         for (int i = io; i < G->nx; i += 2) {
+            long c0 = CIG(G, i, j, k);
+            double center;
+            double az = quat_apply_row(S, gamma, z, i, j, k, &center);
+            double res = rhs[c0] - az;
+            z[c0] = z[c0] + res / center;
+            if (maxres < fabs(res)) maxres = fabs(res);
+         }
+      }
+   return maxres;
+}
+
+static double quat_residual(OSim* S, double gamma, const double* rhs,
+                            double* z, double* res)
+{
+   const OGeom* G = &S->G;
+   o_ghost_fill_periodic(G, z, 1);
+   double s = 0.0;
+   for (int k = 0; k < G->nz; k++)
+      for (int j = 0; j < G->ny; j++)
+         for (int i = 0; i < G->nx; i++) {
+            long c0 = CIG(G, i, j, k);
+            double center;
+            double r = rhs[c0] - quat_apply_row(S, gamma, z, i, j, k, &center);
+            res[c0] = r;
+            s += r * r;
+         }
+   return s;
+}
+
+/* QuatPrecondSolve (source/QuatIntegrator.cc:3594-3650) with
+ * QuatSysSolver::solveSystem (source/QuatSysSolver.cc:296-349):
+ * divide by sqrt(M), solve, multiply by sqrt(M). */
+static int quat_solve(OSim* S, const double* r, double* z, double tol)
+{
+   const OGeom* G = &S->G;
+   long ng = G->ncg, N = (long)G->nx * G->ny * G->nz;
+   int ql = S->p.qlen;
+   double gamma = S->gamma_setup;
+   double* f = S->fac_u;  /* scaled rhs, ql blocks */
+   double* res = S->fac_r;
+   double* e = S->fac_e;
+   for (int m = 0; m < ql; m++)
+      for (long n = 0; n < ng; n++)
+         f[(long)m * ng + n] = r[(long)m * ng + n] / S->sqrtm[n];
+   memset(z, 0, sizeof(double) * ng * ql);
+   double s = 0.0;
+   for (int m = 0; m < ql; m++)
+      s += quat_residual(S, gamma, f + (long)m * ng, z + (long)m * ng,
+                         res + (long)m * ng);
+   double norm = sqrt(s / ((double)ql * N));
+   int iters = 0;
+   do {
+      memset(e, 0, sizeof(double) * ng * ql);
+      for (int m = 0; m < ql; m++) {
+         double* em = e + (long)m * ng;
+         const double* rm = res + (long)m * ng;
+         o_ghost_fill_periodic(G, em, 1);
+         for (int sw = 0; sw < S->s.quat_sweeps; sw++) {
+            o_ghost_fill_periodic(G, em, 1);
+            quat_rbgs_color(S, gamma, rm, em, 0);
+            o_ghost_fill_periodic(G, em, 1);
+            quat_rbgs_color(S, gamma, rm, em, 1);
+            o_ghost_fill_periodic(G, em, 1);
+         }
+      }
+      for (long n = 0; n < ng * ql; n++) z[n] = z[n] + e[n];
+      s = 0.0;
+      for (int m = 0; m < ql; m++)
+         s += quat_residual(S, gamma, f + (long)m * ng, z + (long)m * ng,
+                            res + (long)m * ng);
+      norm = sqrt(s / ((double)ql * N));
+      iters++;
+   } while (norm > tol && iters < S->s.quat_max_cycles);
+   for (int m = 0; m < ql; m++)
+      for (long n = 0; n < ng; n++) z[(long)m * ng + n] *= S->sqrtm[n];
+   return norm < tol ? 0 : 1;
+}
+
+/* CVSpgmrPrecondSolve (source/QuatIntegrator.cc:3666-3771) */
+static int psolve(OSim* S, const double* r, double* z, double delta)
+{
+   const OGeom* G = &S->G;
+   const ampe_model_params* p = &S->p;
+   long ng = G->ncg;
+   int ret = 0;
+   S->st.nps++;
+   memset(z, 0, sizeof(double) * S->nv);
+   ampe_elliptic_op ph = {NULL, -S->gamma_setup * p->epsilon_phi * p->epsilon_phi,
+                          S->Cphi, 1.0, NULL, p->phi_mobility,
+                          {G->dx[0], G->dx[1], G->dx[2]}};
+   if (fac_solve(S, &ph, r, z, delta)) ret = 1;
+   if (S->iq >= 0) {
+      if (quat_solve(S, r + (long)S->iq * ng, z + (long)S->iq * ng, delta)) ret = 1;
+   }
+   if (S->ic >= 0) {
+      ampe_elliptic_op co = {S->Dconc, 0.0, NULL, 1.0, NULL, p->conc_mobility,
+                             {G->dx[0], G->dx[1], G->dx[2]}};
+      if (fac_solve(S, &co, r + (long)S->ic * ng, z + (long)S->ic * ng, delta))
+         ret = 1;
+   }
+   return ret;
+}
+
+static double wrms(const OSim* S, const double* x)
+{
+   return o_wrms(&S->G, S->nc, x, S->ewt);
+}
+
+/* Jv by difference quotient (CVODE cvLsDQJtimes) and atimes z = v - gamma Jv */
+static int atimes(OSim* S, double gamma, double* v, double* out)
+{
+   double sig = 1.0 / wrms(S, v);
+   for (long n = 0; n < S->nv; n++) S->work2[n] = sig * v[n] + S->ycur[n];
+   if (rhs_eval(S, S->work2, S->jv, 1)) return -1;
+   double siginv = 1.0 / sig;
+   for (long n = 0; n < S->nv; n++) {
+      double jvn = siginv * S->jv[n] + (-siginv) * S->fcur[n];
+      out[n] = v[n] + (-gamma) * jvn;
+   }
+   return 0;
+}
+
+/* SPGMR, left preconditioned, MGS, x0 = 0, s1 = s2 = ewt
+ * (SUNLinSol_SPGMR as used by CVODE).  Returns 0 converged, 1 residual
+ * reduced but not converged, 2 failure. */
+static int spgmr(OSim* S, double gamma, const double* bvec, double* x,
+                 double delta, int* nli_out)
+{
+   int maxl = S->s.maxl;
+   long nv = S->nv;
+   const OGeom* G = &S->G;
+   double** V = S->V;
+   double* Hm = S->H;
+   double givens[64], g[32], yv[32];
+   int ld = maxl + 1;
+#define HH(i, j) Hm[(long)(i) * ld + (j)]
+   memset(x, 0, sizeof(double) * nv);
+   if (psolve(S, bvec, S->work, delta) < 0) return 2;
+   for (long n = 0; n < nv; n++) V[0][n] = S->ewt[n] * S->work[n];
+   double rho = sqrt(o_dot(G, S->nc, V[0], V[0]));
+   double rho0 = rho;
+   *nli_out = 0;
+   if (rho <= delta) return 0;
+   double rinv = 1.0 / rho;
+   for (long n = 0; n < nv; n++) V[0][n] *= rinv;
+   memset(g, 0, sizeof(g));
+   g[0] = rho;
+   int l, converged = 0, lused = 0;
+   for (l = 0; l < maxl; l++) {
+      (*nli_out)++;
+      S->st.nli++;
+      for (long n = 0; n < nv; n++) S->xk[n] = V[l][n] / S->ewt[n];
+      if (atimes(S, gamma, S->xk, V[l + 1])) return 2;
+      if (psolve(S, V[l + 1], S->work, delta) < 0) return 2;
+      for (long n = 0; n < nv; n++) V[l + 1][n] = S->ewt[n] * S->work[n];
+      /* modified Gram-Schmidt */
+      for (int i = 0; i <= l; i++) {
+         double hij = o_dot(G, S->nc, V[i], V[l + 1]);
+         HH(i, l) = hij;
+         for (long n = 0; n < nv; n++) V[l + 1][n] = V[l + 1][n] + (-hij) * V[i][n];
+      }
+      double hn = sqrt(o_dot(G, S->nc, V[l + 1], V[l + 1]));
+      HH(l + 1, l) = hn;
+      /* apply previous Givens rotations to column l, then a new one */
+      for (int i = 0; i < l; i++) {
+         double c = givens[2 * i], s = givens[2 * i + 1];
+         double t1 = HH(i, l), t2 = HH(i + 1, l);
+         HH(i, l) = c * t1 - s * t2;
+         HH(i + 1, l) = s * t1 + c * t2;
+      }
+      double a = HH(l, l), bb = HH(l + 1, l), c, s;
+      if (bb == 0.0) {
+         c = 1.0;
+         s = 0.0;
+      } else if (fabs(bb) > fabs(a)) {
+         double t = a / bb;
+         s = -1.0 / sqrt(1.0 + t * t);
+         c = -s * t;
+      } else {
+         double t = bb / a;
+         c = 1.0 / sqrt(1.0 + t * t);
+         s = -c * t;
+      }
+      givens[2 * l] = c;
+      givens[2 * l + 1] = s;
+      HH(l, l) = c * a - s * bb;
+      HH(l + 1, l) = 0.0;
+      g[l + 1] = s * g[l];
+      g[l] = c * g[l];
+      rho = fabs(g[l + 1]);
+      lused = l + 1;
+      if (dbg()) fprintf(stderr, "  gmres l=%d rho=%.3e (rho0=%.3e delta=%.3e)\n", l, rho, rho0, delta);
+      if (rho <= delta) {
+         converged = 1;
+         break;
+      }
+      if (hn == 0.0) break;
+      double hinv = 1.0 / hn;
+      for (long n = 0; n < nv; n++) V[l + 1][n] *= hinv;
+   }
+   /* back substitution H y = g */
+   for (int i = lused - 1; i >= 0; i--) {
+      double sacc = g[i];
+      for (int j = i + 1; j < lused; j++) sacc -= HH(i, j) * yv[j];
+      yv[i] = sacc / HH(i, i);
+   }
+   for (int i = 0; i < lused; i++)
+      for (long n = 0; n < nv; n++) x[n] = x[n] + yv[i] * V[i][n];
+   for (long n = 0; n < nv; n++) x[n] = x[n] / S->ewt[n];
+#undef HH
+   if (converged) return 0;
+   return (rho < rho0) ? 1 : 2;
+}
+
+/* Newton iteration for one step attempt with step h: returns 0 converged,
+ * 1 convergence failure, -1 fatal.  *did_setup reports a fresh lsetup. */
+static int newton(OSim* S, double gamma, double delta, int force_setup,
+                  int* did_setup)
+{
+   long nv = S->nv;
+   double tq4 = S->s.nlscoef;
+   int call_setup = force_setup || S->nst == 0 ||
+                    fabs(gamma / S->gamma_setup - 1.0) > 0.3 ||
+                    S->nst >= S->nstlp + S->s.msbp;
+   *did_setup = 0;
+   memcpy(S->ycur, S->ypred, sizeof(double) * nv);
+   double delp = 0.0;
+   for (int m = 0; m < S->s.max_newton; m++) {
+      S->st.nni++;
+      if (rhs_eval(S, S->ycur, S->fcur, 0)) return -1;
+      for (long n = 0; n < nv; n++)
+         S->b[n] = gamma * S->fcur[n] - (S->ycur[n] - S->ahist[n]);
+      if (m == 0 && call_setup) {
+         if (psetup(S, S->ycur, gamma)) return -1;
+         S->nstlp = S->nst;
+         S->crate = 1.0;
+         *did_setup = 1;
+      }
+      int nli = 0;
+      int ls = spgmr(S, gamma, S->b, S->del, delta, &nli);
+      if (ls == 2 || (ls == 1 && m > 0)) {
+         S->st.nlcf++;
+         return 1;
+      }
+      double del = wrms(S, S->del);
+      for (long n = 0; n < nv; n++) S->ycur[n] = S->ycur[n] + S->del[n];
+      if (m > 0) {
+         double r = del / delp;
+         S->crate = (0.3 * S->crate > r) ? 0.3 * S->crate : r;
+      }
+      double dcon = del * (S->crate < 1.0 ? S->crate : 1.0) / tq4;
+      if (dbg()) fprintf(stderr, " newton m=%d ls=%d del=%.3e crate=%.3e\n", m, ls, del, S->crate);
+      if (dcon <= 1.0) return 0;
+      if (m >= 1 && del > 2.0 * delp) return 1;
+      delp = del;
+   }
+   return 1;
+}
+
+/* One accepted BDF step: BDF1 on the first step, variable-step BDF2 after
+ * (step ratio w = h/hprev: gamma = h(1+w)/(1+2w),
+ * a = ((1+w)^2 y_n - w^2 y_{n-1})/(1+2w), predictor y_n + w (y_n - y_{n-1})).
+ * A Newton failure with a fresh preconditioner cuts h by 4 (CVODE's
+ * convergence-failure factor); h then regrows by 2 per step up to dt.
+ * Returns 0 on success, 1 if h fell below dt*1e-8, -1 fatal. */
+static int bdf_step(OSim* S)
+{
+   const OGeom* G = &S->G;
+   long nv = S->nv, ng = G->ncg;
+   /* resetRefPhaseConcentrations (source/QuatModel.cc:5218-5235) */
+   if (S->ic >= 0) {
+      memcpy(S->clr, S->cl, sizeof(double) * ng);
+      memcpy(S->csr, S->cs, sizeof(double) * ng);
+   }
+   /* nrmfac = sqrt(sum of control volumes over all components)
+    * (CVodeSetLSNormFactor(-1): sqrt(N_VDotProd(1,1))) */
+   double cv = G->dx[0] * G->dx[1] * (G->ndim == 3 ? G->dx[2] : 1.0);
+   double nrmfac = sqrt((double)S->nc * G->nx * G->ny * G->nz * cv);
+   double delta = S->s.eplifac * S->s.nlscoef * nrmfac;
+   int force_setup = 0;
+   for (;;) {
+      double h = S->h, gamma;
+      if (!S->have_prev) {
+         /* zeroth-order predictor on the first step: the step is imposed,
+          * not chosen by CVODE's initial-step heuristic, so y0 + h f(y0) can
+          * be far outside the Newton ball for stiff relaxing modes */
+         memcpy(S->ypred, S->y, sizeof(double) * nv);
+      } else {
+         double w = h / S->hprev;
+         for (long n = 0; n < nv; n++)
+            S->ypred[n] = S->y[n] + w * (S->y[n] - S->yprev[n]);
+      }
+      if (!S->have_prev || S->s.max_order < 2) {
+         gamma = h;
+         memcpy(S->ahist, S->y, sizeof(double) * nv);
+      } else {
+         double w = h / S->hprev;
+         double den = 1.0 + 2.0 * w;
+         gamma = h * (1.0 + w) / den;
+         double c1 = (1.0 + w) * (1.0 + w) / den, c2 = w * w / den;
+         for (long n = 0; n < nv; n++) S->ahist[n] = c1 * S->y[n] - c2 * S->yprev[n];
+      }
+      int did_setup = 0;
+      int r = newton(S, gamma, delta, force_setup, &did_setup);
+      if (r < 0) return -1;
+      if (r == 0) {
+         for (long n = 0; n < nv; n++) S->acor[n] = S->ycur[n] - S->ypred[n];
+         if (S->iq >= 0 && S->p.qlen > 1) {
+            memset(S->corr, 0, sizeof(double) * nv);
+            o_quat_project(G, &S->p, S->ycur + (long)S->iq * ng,
+                           S->corr + (long)S->iq * ng, S->acor + (long)S->iq * ng);
+            for (long n = 0; n < nv; n++) S->ycur[n] = S->ycur[n] + S->corr[n];
+         }
+         S->st.last_acnrm = wrms(S, S->acor);
+         memcpy(S->yprev, S->y, sizeof(double) * nv);
+         memcpy(S->y, S->ycur, sizeof(double) * nv);
+         S->have_prev = 1;
+         S->hprev = h;
+         S->t += h;
+         S->nst++;
+         S->st.nst++;
+         S->h = (2.0 * h < S->s.dt) ? 2.0 * h : S->s.dt;
+         update_ewt(S, S->y);
+         return 0;
+      }
+      S->st.ncfn++;
+      if (!did_setup) {
+         force_setup = 1;
+         continue;
+      }
+      S->h = 0.25 * h;
+      force_setup = 1;
+      if (S->h < S->s.dt * 1.0e-8) return 1;
+   }
+}
+
+int o_sim_advance(OSim* S, int nsteps, ampe_stats* st)
+{
+   int ret = 0;
+   for (int n = 0; n < nsteps; n++) {
+      ret = bdf_step(S);
+      if (ret) break;
+   }
+   S->st.t = S->t;
+   S->st.h = S->h;
+   if (st) *st = S->st;
+   return ret;
+}
