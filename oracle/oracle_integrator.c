@@ -36,7 +36,10 @@
 static int dbg(void)
 {
    static int v = -1;
-   if (v < 0) v = getenv("ORACLE_DEBUG") ? 1 : 0;
+   if (v < 0) {
+      const char* e = getenv("ORACLE_DEBUG");
+      v = (e && e[0] && e[0] != '0') ? 1 : 0;
+   }
    return v;
 }
 

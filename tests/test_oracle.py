@@ -1,0 +1,316 @@
+"""CPU checks that pin the FP64 oracle: closed forms, the reference's own unit
+test cases (testFlux, testGradQ, testCALPHADbinaryKKS, testCALPHADFunctions
+analogues), the AuNi database file, conservation, symmetry and the BDF order
+of accuracy.  Each check is chosen so that a plausible restatement mistake (a
+dropped term, a wrong sign or index, a transposed operand) fails it."""
+import ctypes as C
+import json
+import os
+import re
+
+import numpy as np
+import pytest
+
+from ampe_amd import params as P
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+@pytest.fixture(scope="module")
+def O():
+    from oracle import oracle as O
+    O.lib()
+    return O
+
+
+def _linear_patch(O, ndim, n, dx, alpha, ncomp=1, g=1):
+    """Patch whose cells (ghosts included) hold sum_d alpha[d][m] * index_d."""
+    pt = O.Patch(ndim, n, g=g, dx=dx)
+    a = pt.cells(ncomp)
+    kk, jj, ii = np.meshgrid(np.arange(pt.nz + 2 * pt.gz) - pt.gz, np.arange(pt.ny + 2 * g) - g,
+                             np.arange(pt.nx + 2 * g) - g, indexing="ij")
+    idx = (ii, jj, kk)
+    for m in range(ncomp):
+        a[m] = sum(alpha[d][m] * idx[d] for d in range(ndim))
+    return pt, a
+
+
+def _face_blocks(pt, arr):
+    nfx = (pt.nx + 1) * pt.ny * pt.nz
+    nfy = pt.nx * (pt.ny + 1) * pt.nz
+    return [arr[:nfx], arr[nfx:nfx + nfy], arr[nfx + nfy:]]
+
+
+# ------------------------------------------------------------- helpers
+@pytest.mark.parametrize("t", ["p", "h", "q", "w", "m", "3"])
+def test_interp_derivative_consistency(O, t):
+    L = O.lib()
+    for phi in np.linspace(0.05, 0.95, 19):
+        h = 1e-6
+        fd = (L.o_interp_func(phi + h, t.encode()) - L.o_interp_func(phi - h, t.encode())) / (2 * h)
+        assert abs(fd - L.o_deriv_interp_func(phi, t.encode())) < 1e-7
+        if t in "phqw":
+            fd2 = (L.o_deriv_interp_func(phi + h, t.encode()) -
+                   L.o_deriv_interp_func(phi - h, t.encode())) / (2 * h)
+            assert abs(fd2 - L.o_second_deriv_interp_func(phi, t.encode())) < 1e-6
+
+
+def test_interp_endpoints(O):
+    L = O.lib()
+    for t in "phlm":
+        assert L.o_interp_func(0.0, t.encode()) == 0.0
+        assert abs(L.o_interp_func(1.0, t.encode()) - 1.0) < 1e-15
+        assert L.o_interp_func(-0.3, t.encode()) == 0.0
+    # pbg = 10 t^3 - 15 t^4 + 6 t^5 (functions.f:35-40)
+    x = 0.3
+    assert abs(L.o_interp_func(x, b"p") - (10 * x**3 - 15 * x**4 + 6 * x**5)) < 1e-15
+
+
+def test_well_functions(O):
+    L = O.lib()
+    for phi in np.linspace(0.02, 0.98, 25):
+        h = 1e-6
+        for t in (b"d", b"s"):
+            fd = (L.o_well_func(phi + h, t) - L.o_well_func(phi - h, t)) / (2 * h)
+            assert abs(fd - L.o_deriv_well_func(phi, t)) < 1e-6
+            fd2 = (L.o_deriv_well_func(phi + h, t) - L.o_deriv_well_func(phi - h, t)) / (2 * h)
+            assert abs(fd2 - L.o_second_deriv_well_func(phi, t)) < 1e-5
+    assert L.o_well_func(0.5, b"d") == 1.0  # 16 phi^2 (1-phi)^2 peaks at 1
+
+
+def test_average_and_floor(O):
+    L = O.lib()
+    assert L.o_average_func(0.2, 0.6, b"a") == pytest.approx(0.4)
+    assert L.o_average_func(0.2, 0.6, b"h") == pytest.approx(2 / (1 / 0.2 + 1 / 0.6))
+    assert L.o_average_func(0.0, 0.6, b"h") == 0.0
+    f, m = 1e-2, 1e2
+    assert L.o_eval_grad_normi(4.0, b"m", f * f, m) == pytest.approx(0.5)
+    assert L.o_eval_grad_normi(1e-6, b"m", f * f, m) == m
+    # tanh floor: continuous across its Taylor switch point gng2 = 0.01
+    g2 = 0.01 / (m * m)
+    assert L.o_eval_grad_normi(g2 * (1 - 1e-9), b"t", f * f, m) == pytest.approx(
+        L.o_eval_grad_normi(g2 * (1 + 1e-9), b"t", f * f, m), rel=1e-6)
+    assert L.o_eval_grad_normi(3.0, b"s", 1.0, m) == pytest.approx(0.5)
+
+
+# ------------------------------------------------------- closed forms
+@pytest.mark.parametrize("ndim", [2, 3])
+def test_linear_field_fluxes(O, ndim):
+    """testFlux.cc analogue: a linear field gives the exact flux eps^2 alpha/dx
+    on every face, for the simple and (2-D) isotropic stencils."""
+    dx = (0.2, 0.15, 0.1)
+    alpha = [[1.0], [2.0], [3.0]]
+    pt, phi = _linear_patch(O, ndim, (7, 10, 13), dx, alpha)
+    for ftype in ([P.FLUX_SIMPLE, P.FLUX_ISOTROPIC] if ndim == 2 else [P.FLUX_SIMPLE]):
+        p = P.two_grain_params(ndim=ndim, flux_type=ftype)
+        flux = np.zeros(pt.nf)
+        O.lib().o_phase_flux(C.byref(pt.G), C.byref(p), O.ptr(phi[0]), None, O.ptr(flux))
+        for d, blk in enumerate(_face_blocks(pt, flux)[:ndim]):
+            expect = p.epsilon_phi ** 2 * alpha[d][0] / dx[d]
+            assert np.allclose(blk, expect, rtol=1e-12, atol=0)
+
+
+@pytest.mark.parametrize("ndim", [2, 3])
+def test_linear_quaternion_gradients(O, ndim):
+    """testGradQ.cc analogue: q_m linear with slope (axis+1) + 0.1 m gives cell
+    gradients and side gradients alpha/dx, so |grad q| and the face norms
+    are sqrt / sums of (alpha/dx)^2."""
+    dx = (0.1, 0.11, 0.12)
+    alpha = [[(ax + 1) + 0.1 * m for m in range(4)] for ax in range(3)]
+    pt, q = _linear_patch(O, ndim, (6, 9, 12), dx, alpha, ncomp=4)
+    p = P.two_grain_params(ndim=ndim)
+    gm = pt.cells()
+    O.lib().o_quat_grad_modulus(C.byref(pt.G), C.byref(p), O.ptr(q), O.ptr(gm))
+    s = sum((alpha[d][m] / dx[d]) ** 2 for d in range(ndim) for m in range(4))
+    assert np.allclose(pt.interior(gm), np.sqrt(s), rtol=1e-12)
+    n2 = np.zeros(pt.nf)
+    O.lib().o_quat_face_norm2(C.byref(pt.G), C.byref(p), O.ptr(q), O.ptr(n2))
+    for blk in _face_blocks(pt, n2)[:ndim]:
+        assert np.allclose(blk, s, rtol=1e-12)
+
+
+def _random_patch(O, ndim, n, seed, dx=(1.0, 1.0, 1.0)):
+    rng = np.random.Generator(np.random.Philox(seed))
+    pt = O.Patch(ndim, n, dx=dx)
+    shape = (pt.nz, pt.ny, pt.nx)
+    phi = pt.load(pt.cells(), rng.uniform(0, 1, shape))
+    c = pt.load(pt.cells(), rng.uniform(0.3, 0.5, shape))
+    cl = pt.load(pt.cells(), rng.uniform(0.2, 0.6, shape))
+    cs = pt.load(pt.cells(), rng.uniform(0.2, 0.6, shape))
+    qv = rng.normal(size=(4,) + shape)
+    qv /= np.sqrt((qv ** 2).sum(0))
+    q = pt.load(pt.cells(4), qv)
+    return pt, phi, c, cl, cs, q
+
+
+@pytest.mark.parametrize("ndim", [2, 3])
+def test_composition_conservation(O, ndim):
+    """sum over a periodic patch of M div J vanishes (flux form)."""
+    pt, phi, c, cl, cs, q = _random_patch(O, ndim, (16, 12, 10), 3)
+    p = P.two_grain_params(ndim=ndim)
+    Dc, Dp = np.zeros(pt.nf), np.zeros(pt.nf)
+    O.lib().o_conc_coeffs(C.byref(pt.G), C.byref(p), O.ptr(phi), O.ptr(cl), O.ptr(cs),
+                          O.ptr(Dc), O.ptr(Dp))
+    r = pt.cells()
+    O.lib().o_conc_rhs(C.byref(pt.G), C.byref(p), O.ptr(c), O.ptr(phi), O.ptr(Dc), O.ptr(Dp),
+                       O.ptr(r))
+    ri = pt.interior(r)
+    assert abs(ri.sum()) < 1e-12 * np.abs(ri).sum()
+    assert np.abs(ri).max() > 0
+    # D_c is bounded by the liquid and solid diffusivities
+    dl = p.D_liquid * np.exp(-p.Q0_liquid / (P.R_GAS * p.temperature))
+    ds = p.D_solid * np.exp(-p.Q0_solid / (P.R_GAS * p.temperature))
+    lo, hi = min(dl, ds), max(dl, ds)
+    assert (Dc >= lo * (1 - 1e-12)).all() and (Dc <= hi * (1 + 1e-12)).all()
+
+
+def test_isotropic_stencil_rotation_symmetry(O):
+    """Rotating phi by 90 degrees rotates the isotropic-flux phase RHS."""
+    n = 24
+    rng = np.random.Generator(np.random.Philox(9))
+    f = rng.uniform(0, 1, (n, n))
+    p = P.two_grain_params(flux_type=P.FLUX_ISOTROPIC, evolve_quat=0, with_conc=0)
+    out = []
+    for field in (f, np.rot90(f).copy()):
+        pt = O.Patch(2, (n, n, 1))
+        phi = pt.load(pt.cells(), field)
+        r = pt.cells()
+        O.lib().o_phase_rhs(C.byref(pt.G), C.byref(p), O.ptr(phi), None, None, None, O.ptr(r))
+        out.append(pt.interior(r)[0, 0])
+    assert np.allclose(np.rot90(out[0]), out[1], rtol=0, atol=1e-9 * np.abs(out[0]).max())
+
+
+@pytest.mark.parametrize("variant", ["phase", "conc"])
+def test_rbgs_converges_to_manufactured_solution(O, variant):
+    """Red-black GS on A u = M div(D grad u) + C u reaches a manufactured u*
+    (f = A u* computed by the residual routine with rhs = 0)."""
+    ndim, n = 2, (20, 16, 1)
+    pt, phi, c, cl, cs, q = _random_patch(O, ndim, n, 5)
+    gamma = 1e-3
+    p = P.two_grain_params()
+    op = O.EllipticOp()
+    Cf = pt.cells()
+    if variant == "phase":
+        O.lib().o_phase_fac_C(C.byref(pt.G), C.byref(p), gamma, O.ptr(phi), O.ptr(Cf))
+        op.C = Cf.ctypes.data
+        op.D_const = -gamma * p.epsilon_phi ** 2
+        op.M_const = p.phi_mobility
+    else:
+        D = -gamma * np.random.Generator(np.random.Philox(2)).uniform(1e3, 1e4, pt.nf)
+        op.D = D.ctypes.data
+        op.C_const = 1.0
+        op.M_const = 1.0
+    ustar = pt.load(pt.cells(), np.random.Generator(np.random.Philox(4)).normal(size=(1, 16, 20)))
+    res, flux = pt.cells(), np.zeros(pt.nf)
+    zero = pt.cells()
+    O.lib().o_fac_residual(C.byref(pt.G), C.byref(op), O.ptr(zero), O.ptr(ustar), O.ptr(res),
+                           O.ptr(flux))
+    f = -res
+    u = pt.cells()
+    O.lib().o_fac_smooth(C.byref(pt.G), C.byref(op), O.ptr(f), O.ptr(u), 4000, -1.0, O.ptr(flux))
+    err = np.abs(pt.interior(u) - pt.interior(ustar)).max()
+    assert err < 1e-8 * np.abs(pt.interior(ustar)).max(), err
+
+
+# ------------------------------------------------------------- KKS
+def test_quadratic_kks_invariants(O):
+    p = P.two_grain_params()
+    rng = np.random.Generator(np.random.Philox(1))
+    for c, phi in zip(rng.uniform(0.0, 0.6, 50), rng.uniform(-0.1, 1.1, 50)):
+        cl, cs = C.c_double(0), C.c_double(0)
+        assert O.lib().o_kks_cell(C.byref(p), c, phi, C.byref(cl), C.byref(cs)) == 0
+        h = O.lib().o_interp_func(phi, p.conc_interp)
+        assert abs((1 - h) * cl.value + h * cs.value - c) < 1e-14
+        assert abs(O.lib().o_dfenergy(C.byref(p), cl.value, 0) -
+                   O.lib().o_dfenergy(C.byref(p), cs.value, 1)) < 1e-9
+
+
+def _parse_calphad_dat(path):
+    """Minimal reader of the SAMRAI-style database blocks in calphadAuNi.dat."""
+    text = re.sub(r"//[^\n]*", "", open(path).read())
+    out = {}
+    stack = []
+    for tok in re.finditer(r"(\w+)\s*\{|\}|(\w+)\s*=\s*([^\n]+)", text):
+        if tok.group(1):
+            stack.append(tok.group(1))
+        elif tok.group(0) == "}":
+            stack.pop()
+        else:
+            key = "/".join(stack + [tok.group(2)])
+            vals = tok.group(3).strip().rstrip(",")
+            if vals.startswith('"'):
+                out[key] = vals.strip('"')
+            else:
+                out[key] = [float(v) for v in vals.split(",") if v.strip()]
+    return out
+
+
+def test_auni_tables_match_reference_database(O):
+    db = _parse_calphad_dat(os.path.join(HERE, "golden", "calphadAuNi.dat"))
+    for (sp, ph), d in P._AUNI.items():
+        blk = f"Species{sp}/Phase{'L' if ph == 'L' else 'A'}"
+        for key, vals in d.items():
+            assert np.allclose(db[f"{blk}/{key}"][:len(vals)], vals), (blk, key)
+    for ph, blk in (("L", "LmixPhaseL"), ("S", "LmixPhaseA")):
+        for i, (a, b) in enumerate(P._AUNI_LMIX[ph]):
+            assert db[f"{blk}/L{i}"][:2] == [a, b]
+
+
+def test_calphad_derivatives(O):
+    """testCALPHADFunctions analogue: finite differences vs analytic
+    derivatives of the free energy and chemical potential."""
+    p = P.two_grain_params(fe_model=P.FE_CALPHAD_BINARY, temperature=1450.0)
+    p.calphad = P.auni_phases(1450.0)
+    L = O.lib()
+    for ph in (0, 1):
+        for c in np.linspace(0.05, 0.95, 19):
+            h = 1e-6
+            fd = (L.o_fenergy(C.byref(p), c + h, ph) - L.o_fenergy(C.byref(p), c - h, ph)) / (2 * h)
+            assert abs(fd - L.o_dfenergy(C.byref(p), c, ph)) < 1e-6 * (1 + abs(fd))
+            fd2 = (L.o_dfenergy(C.byref(p), c + h, ph) - L.o_dfenergy(C.byref(p), c - h, ph)) / (2 * h)
+            assert abs(fd2 - L.o_d2fenergy(C.byref(p), c, ph)) < 1e-5 * (1 + abs(fd2))
+
+
+def test_calphad_kks_reference_case(O):
+    """testCALPHADbinaryKKS.cc: AuNi, T = 1450 K, c = 0.3, phi = 0.5, pbg,
+    initial guess (0.5, 0.5): |dfL/dcL - dfS/dcS| < 1e-5 and mass balance."""
+    case = json.load(open(os.path.join(HERE, "golden", "reference_unit_tests.json")))[
+        "testCALPHADbinaryKKS"]
+    T = case["temperature"]
+    p = P.two_grain_params(fe_model=P.FE_CALPHAD_BINARY, temperature=T, energy_interp=b"p",
+                           conc_interp=b"p", newton_max_its=case["newton_max_its"])
+    p.calphad = P.auni_phases(T)
+    cl, cs = C.c_double(case["initial_guess"][0]), C.c_double(case["initial_guess"][1])
+    it = O.lib().o_kks_cell(C.byref(p), case["concentration"], case["phi"], C.byref(cl),
+                            C.byref(cs))
+    assert it > 0
+    dmu = O.lib().o_dfenergy(C.byref(p), cl.value, 0) - O.lib().o_dfenergy(C.byref(p), cs.value, 1)
+    assert abs(dmu) < case["max_abs_dmu"]
+    h = O.lib().o_interp_func(case["phi"], b"p")
+    assert abs((1 - h) * cl.value + h * cs.value - case["concentration"]) < 1e-12
+    assert cl.value != cs.value  # a two-phase solution, not the trivial one
+
+
+# ------------------------------------------------------------- BDF
+def test_bdf_second_order(O):
+    """Uniform phi: every cell obeys phi' = -M w g'(phi) exactly; the BDF2
+    integrator's global error must fall by ~4 when dt halves."""
+    from scipy.integrate import solve_ivp
+    from oracle import oracle as OO
+    p = P.two_grain_params(evolve_quat=0, with_conc=0)
+    k = p.phi_mobility * p.phi_well_scale
+    ode = lambda t, y: [-k * 32 * y[0] * (1 - y[0]) * (1 - 2 * y[0])]  # noqa: E731
+    T = 4e-3
+    ref = solve_ivp(ode, (0, T), [0.3], rtol=1e-13, atol=1e-15, method="DOP853").y[0, -1]
+    errs = []
+    for nsteps in (40, 80):
+        s = P.solver_defaults(dt=T / nsteps, atol=1e-13, rtol=1e-13, maxl=5, max_newton=8)
+        sim = OO.Sim(p, s, (8, 8, 1))
+        sim.set_state(np.full((8, 8), 0.3), None, None)
+        rc, st = sim.advance(nsteps)
+        assert rc == 0 and st["nst"] == nsteps
+        phi = sim.get_state()[0]
+        assert np.ptp(phi) < 1e-15
+        errs.append(abs(phi[0, 0] - ref))
+    ratio = errs[0] / errs[1]
+    assert 3.3 < ratio < 4.7, (errs, ratio)
