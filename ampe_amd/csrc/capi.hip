@@ -153,8 +153,8 @@ int ampe_kks_solve(const ampe_level* level, const ampe_model_params* p, const do
    AMPE_REQUIRE(level && phi && c && cl_ref && cs_ref && cl && cs, "null argument");
    if (check_params(p, *level->L)) return -1;
    int it = 0;
-   if (ampe::launch_kks(*level->L, *p, phi, c, cl_ref, cs_ref, cl, cs, &it, nullptr,
-                        hs(stream)))
+   if (ampe::launch_kks_sync(*level->L, *p, phi, c, cl_ref, cs_ref, cl, cs, &it, nullptr,
+                             hs(stream)))
       return -1;
    if (max_iters) *max_iters = it;
    return it < 0 ? 1 : 0;
@@ -223,7 +223,8 @@ int ampe_fac_residual(const ampe_level* level, const ampe_elliptic_op* op, const
    hipStream_t s = hs(stream);
    ampe::EllOp<double> o = to_op(op);
    if (ampe::ghost_fill(L, u, 1, s)) return -1;
-   if (ampe::launch_residual<double>(L, o, op->dx, rhs, u, res, rms != nullptr, s)) return -1;
+   if (ampe::launch_residual<double>(L, o, op->dx, rhs, u, res, rms ? L.partial : nullptr, s))
+      return -1;
    if (rms) {
       double ss = 0.0;
       if (ampe::reduce_partials(L, L.ncell_tiles, 1, 0, &ss, s)) return -1;

@@ -24,10 +24,11 @@ template <typename T>
 int launch_rbgs(const Level& L, const EllOp<T>& op, const double* dx, const T* rhs, T* u,
                 int color, bool want_max, hipStream_t s);
 
-// res = rhs - A u; with want_sum the per-tile sum res^2 lands in L.partial.
+// res = rhs - A u; if part is not null the per-tile sum res^2 lands in
+// part[0 .. L.ncell_tiles).
 template <typename T>
 int launch_residual(const Level& L, const EllOp<T>& op, const double* dx, const T* rhs,
-                    const T* u, T* res, bool want_sum, hipStream_t s);
+                    const T* u, T* res, double* part, hipStream_t s);
 
 template <typename T>
 int launch_quat_rbgs(const Level& L, const T* fc, const T* sm, double gamma, const double* dx,
@@ -36,7 +37,7 @@ int launch_quat_rbgs(const Level& L, const T* fc, const T* sm, double gamma, con
 template <typename T>
 int launch_quat_residual(const Level& L, const T* fc, const T* sm, double gamma,
                          const double* dx, int ncomp, const T* rhs, const T* z, T* res,
-                         bool want_sum, hipStream_t s);
+                         double* part, hipStream_t s);
 
 // ------------------------------------------------------------ vectors
 // Flat element-wise ops over n entries (whole ghosted vectors).

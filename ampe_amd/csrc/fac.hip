@@ -254,10 +254,9 @@ int launch_rbgs(const Level& L, const EllOp<T>& op, const double* dx, const T* r
 
 template <typename T>
 int launch_residual(const Level& L, const EllOp<T>& op, const double* dx, const T* rhs,
-                    const T* u, T* res, bool want_sum, hipStream_t s)
+                    const T* u, T* res, double* part, hipStream_t s)
 {
    OpArgs<T> oa{op.D, op.Dc, op.C, op.Cc, op.M, op.Mc};
-   double* part = want_sum ? L.partial : nullptr;
    if (L.ndim == 3)
       hipLaunchKernelGGL((k_residual<T, 3>), dim3(L.ncell_tiles), tblock(), 0, s, level_args(L),
                          oa, (T)dx[0], (T)dx[1], (T)dx[2], rhs, u, res, part);
@@ -287,9 +286,8 @@ int launch_quat_rbgs(const Level& L, const T* fc, const T* sm, double gamma, con
 template <typename T>
 int launch_quat_residual(const Level& L, const T* fc, const T* sm, double gamma,
                          const double* dx, int ncomp, const T* rhs, const T* z, T* res,
-                         bool want_sum, hipStream_t s)
+                         double* part, hipStream_t s)
 {
-   double* part = want_sum ? L.partial : nullptr;
    if (L.ndim == 3)
       hipLaunchKernelGGL((k_quat_residual<T, 3>), dim3(L.ncell_tiles), tblock(), 0, s,
                          level_args(L), fc, sm, (T)gamma, (T)dx[0], (T)dx[1], (T)dx[2], ncomp,
@@ -306,12 +304,12 @@ int launch_quat_residual(const Level& L, const T* fc, const T* sm, double gamma,
    template int launch_rbgs<T>(const Level&, const EllOp<T>&, const double*, const T*, T*, \
                                int, bool, hipStream_t);                                  \
    template int launch_residual<T>(const Level&, const EllOp<T>&, const double*, const T*, \
-                                   const T*, T*, bool, hipStream_t);                     \
+                                   const T*, T*, double*, hipStream_t);                  \
    template int launch_quat_rbgs<T>(const Level&, const T*, const T*, double, const double*, \
                                     int, const T*, T*, int, hipStream_t);                \
    template int launch_quat_residual<T>(const Level&, const T*, const T*, double,         \
-                                        const double*, int, const T*, const T*, T*, bool, \
-                                        hipStream_t);
+                                        const double*, int, const T*, const T*, T*,      \
+                                        double*, hipStream_t);
 AMPE_INST(double)
 AMPE_INST(float)
 

@@ -1,0 +1,34 @@
+// Fused Newton-Krylov vector kernels (see krylov.hip).  Host results of the
+// reductions are raw sums over interior cells (no cell-volume factor).
+#pragma once
+
+#include "kernels.h"
+
+namespace ampe {
+
+constexpr int kMaxKrylov = 32;
+
+struct VecList {
+   int n;
+   const double* v[kMaxKrylov];
+   double y[kMaxKrylov];
+};
+
+int kry_prep(const Level& L, int nc, double hinv, double* V, const double* ewt, double* xk,
+             double* sumsq, hipStream_t s);
+int kry_mid(const Level& L, int nc, double sig, const double* xk, const double* ycur,
+            double* work2, hipStream_t s);
+int kry_finish(const Level& L, int nc, double siginv, double gamma, const double* xk,
+               const double* jv, const double* fcur, double* out, hipStream_t s);
+int kry_scale_dot(const Level& L, int nc, const double* ewt, const double* work, double* out,
+                  const double* other, double* dot, hipStream_t s);
+int kry_mgs(const Level& L, int nc, double* w, const double* Vi, double h, const double* Vnext,
+            double* dot, hipStream_t s);
+int kry_lincomb(const Level& L, int nc, const VecList& vl, const double* ewt, double* x,
+                hipStream_t s);
+int kry_newton_rhs(const Level& L, int nc, double gamma, const double* f, const double* y,
+                   const double* ah, double* b, hipStream_t s);
+int kry_newton_update(const Level& L, int nc, const double* del, const double* ewt, double* y,
+                      double* sumsq, hipStream_t s);
+
+}  // namespace ampe

@@ -1,0 +1,230 @@
+// Fused vector kernels of the Newton-Krylov loop.  Each runs over the interior
+// cells of a level (tile list) and all nc component blocks, performs the
+// element-wise update of one or two steps of SPGMR / Newton in one pass and
+// writes one partial sum per tile for the reduction that follows, so a
+// Gram-Schmidt step costs one read of each operand instead of two or three.
+// The element-wise arithmetic is that of the unfused sequence (same order,
+// no contraction).  Ghost entries of Krylov vectors are not touched: every
+// consumer that reads a stencil ghost-fills its input first.
+#include "krylov.h"
+
+namespace ampe {
+
+__device__ __forceinline__ void tile_store_sum(double acc, double* __restrict__ partial)
+{
+   for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o);
+   __shared__ double sh[4];
+   int t = threadIdx.x + TX * threadIdx.y;
+   if ((t & 63) == 0) sh[t >> 6] = acc;
+   __syncthreads();
+   if (t == 0) partial[blockIdx.x] = (sh[0] + sh[1]) + (sh[2] + sh[3]);
+}
+
+// V = hinv V; xk = V / ewt; partial = sum (xk ewt)^2
+__global__ void __launch_bounds__(256) k_gmres_prep(LevelArgs a, int nc, double hinv,
+                                                    double* __restrict__ V,
+                                                    const double* __restrict__ ewt,
+                                                    double* __restrict__ xk,
+                                                    double* __restrict__ partial)
+{
+   CellPos c;
+   double acc = 0.0;
+   if (cell_pos(a, 0, c)) {
+      for (int m = 0; m < nc; m++) {
+         long long i = m * a.ncg + c.c0;
+         double v = V[i] * hinv;
+         V[i] = v;
+         double x = v / ewt[i];
+         xk[i] = x;
+         double t = x * ewt[i];
+         acc += t * t;
+      }
+   }
+   tile_store_sum(acc, partial);
+}
+
+// work2 = sig xk + ycur
+__global__ void __launch_bounds__(256) k_gmres_mid(LevelArgs a, int nc, double sig,
+                                                   const double* __restrict__ xk,
+                                                   const double* __restrict__ ycur,
+                                                   double* __restrict__ work2)
+{
+   CellPos c;
+   if (!cell_pos(a, 0, c)) return;
+   for (int m = 0; m < nc; m++) {
+      long long i = m * a.ncg + c.c0;
+      work2[i] = sig * xk[i] + ycur[i];
+   }
+}
+
+// out = xk + (-gamma) (siginv jv + (-siginv) fcur)
+__global__ void __launch_bounds__(256) k_gmres_finish(LevelArgs a, int nc, double siginv,
+                                                      double gamma,
+                                                      const double* __restrict__ xk,
+                                                      const double* __restrict__ jv,
+                                                      const double* __restrict__ fcur,
+                                                      double* __restrict__ out)
+{
+   CellPos c;
+   if (!cell_pos(a, 0, c)) return;
+   for (int m = 0; m < nc; m++) {
+      long long i = m * a.ncg + c.c0;
+      double jvn = siginv * jv[i] + (-siginv) * fcur[i];
+      out[i] = xk[i] + (-gamma) * jvn;
+   }
+}
+
+// out = ewt work; partial = sum other * out (other may alias out)
+__global__ void __launch_bounds__(256) k_scale_dot(LevelArgs a, int nc,
+                                                   const double* __restrict__ ewt,
+                                                   const double* __restrict__ work, double* out,
+                                                   const double* other,
+                                                   double* __restrict__ partial)
+{
+   CellPos c;
+   double acc = 0.0;
+   if (cell_pos(a, 0, c)) {
+      for (int m = 0; m < nc; m++) {
+         long long i = m * a.ncg + c.c0;
+         double v = ewt[i] * work[i];
+         out[i] = v;
+         double o = (other == out) ? v : other[i];
+         acc += o * v;
+      }
+   }
+   tile_store_sum(acc, partial);
+}
+
+// w = w + (-h) Vi; partial = sum Vnext * w (or w * w when Vnext is null)
+__global__ void __launch_bounds__(256) k_mgs(LevelArgs a, int nc, double* __restrict__ w,
+                                             const double* __restrict__ Vi, double h,
+                                             const double* __restrict__ Vnext,
+                                             double* __restrict__ partial)
+{
+   CellPos c;
+   double acc = 0.0;
+   if (cell_pos(a, 0, c)) {
+      for (int m = 0; m < nc; m++) {
+         long long i = m * a.ncg + c.c0;
+         double v = w[i] + (-h) * Vi[i];
+         w[i] = v;
+         acc += Vnext ? Vnext[i] * v : v * v;
+      }
+   }
+   tile_store_sum(acc, partial);
+}
+
+// x = (sum_i y_i V_i) / ewt, summed in index order from zero
+__global__ void __launch_bounds__(256) k_lincomb(LevelArgs a, int nc, VecList vl,
+                                                 const double* __restrict__ ewt,
+                                                 double* __restrict__ x)
+{
+   CellPos c;
+   if (!cell_pos(a, 0, c)) return;
+   for (int m = 0; m < nc; m++) {
+      long long i = m * a.ncg + c.c0;
+      double acc = 0.0;
+      for (int k = 0; k < vl.n; k++) acc = acc + vl.y[k] * vl.v[k][i];
+      x[i] = acc / ewt[i];
+   }
+}
+
+// b = gamma f - (y - a)
+__global__ void __launch_bounds__(256) k_newton_rhs(LevelArgs a, int nc, double gamma,
+                                                    const double* __restrict__ f,
+                                                    const double* __restrict__ y,
+                                                    const double* __restrict__ ah,
+                                                    double* __restrict__ b)
+{
+   CellPos c;
+   if (!cell_pos(a, 0, c)) return;
+   for (int m = 0; m < nc; m++) {
+      long long i = m * a.ncg + c.c0;
+      b[i] = gamma * f[i] - (y[i] - ah[i]);
+   }
+}
+
+// partial = sum (del ewt)^2; y = y + del
+__global__ void __launch_bounds__(256) k_newton_update(LevelArgs a, int nc,
+                                                       const double* __restrict__ del,
+                                                       const double* __restrict__ ewt,
+                                                       double* __restrict__ y,
+                                                       double* __restrict__ partial)
+{
+   CellPos c;
+   double acc = 0.0;
+   if (cell_pos(a, 0, c)) {
+      for (int m = 0; m < nc; m++) {
+         long long i = m * a.ncg + c.c0;
+         double t = del[i] * ewt[i];
+         acc += t * t;
+         y[i] = y[i] + del[i];
+      }
+   }
+   tile_store_sum(acc, partial);
+}
+
+#define AMPE_TILE_LAUNCH(KER, ...)                                                        \
+   do {                                                                                   \
+      hipLaunchKernelGGL(KER, dim3(L.ncell_tiles), dim3(TX, TY), 0, s, level_args(L),      \
+                         __VA_ARGS__);                                                    \
+      AMPE_HIP_CHECK(hipGetLastError());                                                  \
+   } while (0)
+
+int kry_prep(const Level& L, int nc, double hinv, double* V, const double* ewt, double* xk,
+             double* sumsq, hipStream_t s)
+{
+   AMPE_TILE_LAUNCH(k_gmres_prep, nc, hinv, V, ewt, xk, L.partial);
+   return reduce_partials(L, L.ncell_tiles, 1, 0, sumsq, s);
+}
+
+int kry_mid(const Level& L, int nc, double sig, const double* xk, const double* ycur,
+            double* work2, hipStream_t s)
+{
+   AMPE_TILE_LAUNCH(k_gmres_mid, nc, sig, xk, ycur, work2);
+   return 0;
+}
+
+int kry_finish(const Level& L, int nc, double siginv, double gamma, const double* xk,
+               const double* jv, const double* fcur, double* out, hipStream_t s)
+{
+   AMPE_TILE_LAUNCH(k_gmres_finish, nc, siginv, gamma, xk, jv, fcur, out);
+   return 0;
+}
+
+int kry_scale_dot(const Level& L, int nc, const double* ewt, const double* work, double* out,
+                  const double* other, double* dot, hipStream_t s)
+{
+   AMPE_TILE_LAUNCH(k_scale_dot, nc, ewt, work, out, other, L.partial);
+   return reduce_partials(L, L.ncell_tiles, 1, 0, dot, s);
+}
+
+int kry_mgs(const Level& L, int nc, double* w, const double* Vi, double h, const double* Vnext,
+            double* dot, hipStream_t s)
+{
+   AMPE_TILE_LAUNCH(k_mgs, nc, w, Vi, h, Vnext, L.partial);
+   return reduce_partials(L, L.ncell_tiles, 1, 0, dot, s);
+}
+
+int kry_lincomb(const Level& L, int nc, const VecList& vl, const double* ewt, double* x,
+                hipStream_t s)
+{
+   AMPE_TILE_LAUNCH(k_lincomb, nc, vl, ewt, x);
+   return 0;
+}
+
+int kry_newton_rhs(const Level& L, int nc, double gamma, const double* f, const double* y,
+                   const double* ah, double* b, hipStream_t s)
+{
+   AMPE_TILE_LAUNCH(k_newton_rhs, nc, gamma, f, y, ah, b);
+   return 0;
+}
+
+int kry_newton_update(const Level& L, int nc, const double* del, const double* ewt, double* y,
+                      double* sumsq, hipStream_t s)
+{
+   AMPE_TILE_LAUNCH(k_newton_update, nc, del, ewt, y, L.partial);
+   return reduce_partials(L, L.ncell_tiles, 1, 0, sumsq, s);
+}
+
+}  // namespace ampe

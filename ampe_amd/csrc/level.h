@@ -76,5 +76,8 @@ int ghost_fill(const Level& L, float* f, int ncomp, hipStream_t s);
 // interleaved), result to host.
 int reduce_partials(const Level& L, int nparts, int ncol, int op,
                     double* host_out, hipStream_t s);
+// Sums of nreg consecutive regions of nparts partials each (region r starts at
+// L.partial + r * nparts), one launch and one synchronisation.
+int reduce_regions(const Level& L, int nparts, int nreg, double* host_out, hipStream_t s);
 
 }  // namespace ampe

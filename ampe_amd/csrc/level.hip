@@ -268,4 +268,22 @@ int reduce_partials(const Level& L, int nparts, int ncol, int op,
    return 0;
 }
 
+int reduce_regions(const Level& L, int nparts, int nreg, double* host_out, hipStream_t s)
+{
+   if (nreg > 16) {
+      set_error("reduce_regions: at most 16 regions");
+      return -1;
+   }
+   for (int r = 0; r < nreg; r++) {
+      hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(1024), 0, s,
+                         L.partial + (size_t)r * nparts, nparts, 1, 0, L.dred + r);
+      AMPE_HIP_CHECK(hipGetLastError());
+   }
+   AMPE_HIP_CHECK(hipMemcpyAsync(L.hpinned, L.dred, sizeof(double) * nreg,
+                                 hipMemcpyDeviceToHost, s));
+   AMPE_HIP_CHECK(hipStreamSynchronize(s));
+   for (int r = 0; r < nreg; r++) host_out[r] = L.hpinned[r];
+   return 0;
+}
+
 }  // namespace ampe

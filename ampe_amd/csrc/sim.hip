@@ -78,20 +78,22 @@ int sim_create(const ampe_model_params& p, const ampe_solver_params& s, const am
    double** cells[] = {&S->cl, &S->cs, &S->clr, &S->csr, &S->Cphi, &S->sqrtm};
    for (auto v : cells) rc |= dalloc(v, S->ng);
    rc |= dalloc(&S->fac_u, S->ng * ql);
-   rc |= dalloc(&S->fac_e, S->ng * ql);
-   rc |= dalloc(&S->fac_r, S->ng * ql);
+   rc |= dalloc(&S->fac_e, S->nv);
+   rc |= dalloc(&S->fac_r, S->nv);
    double** sides[] = {&S->n2, &S->Dc, &S->Dphi, &S->Dconc, &S->fc};
    for (auto v : sides) rc |= dalloc(v, S->nf);
    if (s.precision == AMPE_FAC_FP32) {
-      rc |= falloc(&S->f_u, S->ng * ql);
-      rc |= falloc(&S->f_e, S->ng * ql);
-      rc |= falloc(&S->f_r, S->ng * ql);
-      rc |= falloc(&S->f_f, S->ng * ql);
+      rc |= falloc(&S->f_u, S->nv);
+      rc |= falloc(&S->f_e, S->nv);
+      rc |= falloc(&S->f_r, S->nv);
+      rc |= falloc(&S->f_f, S->nv);
       rc |= falloc(&S->f_Cphi, S->ng);
       rc |= falloc(&S->f_sqrtm, S->ng);
       rc |= falloc(&S->f_Dconc, S->nf);
       rc |= falloc(&S->f_fc, S->nf);
    }
+   if (!rc && hipMalloc(&S->kks_acc, sizeof(long long) * 4) != hipSuccess) rc = -1;
+   if (!rc && hipMemset(S->kks_acc, 0, sizeof(long long) * 4) != hipSuccess) rc = -1;
    if (rc || hipStreamCreate(&S->stream) != hipSuccess ||
        hipEventCreate(&S->ev0) != hipSuccess || hipEventCreate(&S->ev1) != hipSuccess) {
       if (!rc) set_error("sim_create: stream/event creation failed");
@@ -114,6 +116,7 @@ void sim_destroy(Sim* S)
    for (double* v : S->V) (void)hipFree(v);
    float* fall[] = {S->f_u, S->f_e, S->f_r, S->f_f, S->f_Cphi, S->f_sqrtm, S->f_Dconc, S->f_fc};
    for (float* v : fall) (void)hipFree(v);
+   (void)hipFree(S->kks_acc);
    if (S->stream) (void)hipStreamDestroy(S->stream);
    if (S->ev0) (void)hipEventDestroy(S->ev0);
    if (S->ev1) (void)hipEventDestroy(S->ev1);
@@ -157,20 +160,26 @@ static int gather_interior(Sim* S, const double* dev, double* host)
 
 static int kks(Sim* S, const double* y)
 {
-   const Level& L = *S->L;
-   int maxit = 0;
-   long long tot = 0;
-   const double* phi = y;
    const double* c = y + (long long)S->ic * S->ng;
-   if (launch_kks(L, S->p, phi, c, S->clr, S->csr, S->cl, S->cs, &maxit, &tot, S->stream))
-      return -1;
-   S->st.kks_calls++;
-   S->st.kks_cells += L.ncg;
-   if (maxit < 0) {
+   return launch_kks(*S->L, S->p, y, c, S->clr, S->csr, S->cl, S->cs, S->kks_acc, S->stream);
+}
+
+// Read the KKS device counters into the stats; -1 if a cell failed.
+static int kks_collect(Sim* S)
+{
+   if (S->ic < 0) return 0;
+   AMPE_HIP_CHECK(hipMemcpyAsync(S->L->hpinned, S->kks_acc, sizeof(long long) * 4,
+                                 hipMemcpyDeviceToHost, S->stream));
+   AMPE_HIP_CHECK(hipStreamSynchronize(S->stream));
+   long long h[4];
+   std::memcpy(h, S->L->hpinned, sizeof(h));
+   S->st.kks_iters = h[0];
+   S->st.kks_calls = h[3];
+   S->st.kks_cells = S->st.kks_calls * S->L->ncg;
+   if (h[2]) {
       set_error("KKS Newton did not converge in a cell");
       return -1;
    }
-   S->st.kks_iters += tot;
    return 0;
 }
 
@@ -205,8 +214,11 @@ int sim_set_state(Sim* S, const double* phi, const double* c, const double* q)
                                     hipMemcpyDeviceToDevice, st));
       AMPE_HIP_CHECK(hipMemcpyAsync(S->csr, cc, sizeof(double) * S->ng,
                                     hipMemcpyDeviceToDevice, st));
+      AMPE_HIP_CHECK(hipMemsetAsync(S->kks_acc, 0, sizeof(long long) * 4, st));
       if (kks(S, S->y)) return -1;
+      if (kks_collect(S)) return -1;
       std::memset(&S->st, 0, sizeof(S->st));
+      AMPE_HIP_CHECK(hipMemsetAsync(S->kks_acc, 0, sizeof(long long) * 4, st));
    }
    S->t = 0.0;
    S->nst = 0;
@@ -232,6 +244,7 @@ int sim_get_state(Sim* S, double* phi, double* c, double* q, double* cl, double*
    if (cs && S->ic >= 0 && gather_interior(S, S->cs, cs)) return -1;
    return 0;
 }
+
 
 // ------------------------------------------------------------ callbacks
 // evaluateRHSFunction (source/QuatIntegrator.cc:3134-3298)
@@ -292,180 +305,232 @@ static int psetup(Sim* S, double* y, double gamma)
    return 0;
 }
 
+// One preconditioner block: an elliptic FAC block (phase, composition) or the
+// orientation block, with its buffers.  comp = first component index.
 template <typename T>
-static int smooth(Sim* S, const EllOp<T>& op, const T* r, T* e)
+struct PBlock {
+   int kind;  // 0 elliptic, 1 orientation
+   int comp, ncomp;
+   EllOp<T> op;
+   const T* fc;
+   const T* sm;
+   const T* f;  // right-hand side
+   T* u;        // solution
+   T* e;        // cycle correction
+   T* r;        // residual
+   double norm;
+   int cycles;
+};
+
+template <typename T>
+static int block_smooth(Sim* S, PBlock<T>& B, const T* rhs, T* x)
 {
    const Level& L = *S->L;
    hipStream_t st = S->stream;
    const double* dx = S->p.dx;
-   bool want = S->s.fac_tol >= 0.0;
-   if (ghost_fill(L, e, 1, st)) return -1;
-   for (int sw = 0; sw < S->s.fac_sweeps; sw++) {
-      double mr = 0.0, mb = 0.0;
-      if (ghost_fill(L, e, 1, st)) return -1;
-      if (launch_rbgs<T>(L, op, dx, r, e, 0, want, st)) return -1;
-      if (want && reduce_partials(L, L.ncell_tiles, 1, 1, &mr, st)) return -1;
-      if (ghost_fill(L, e, 1, st)) return -1;
-      if (launch_rbgs<T>(L, op, dx, r, e, 1, want, st)) return -1;
-      if (want && reduce_partials(L, L.ncell_tiles, 1, 1, &mb, st)) return -1;
-      if (ghost_fill(L, e, 1, st)) return -1;
-      if (want && (mr > mb ? mr : mb) <= S->s.fac_tol) break;
+   if (B.kind == 0) {
+      bool want = S->s.fac_tol >= 0.0;
+      if (ghost_fill(L, x, 1, st)) return -1;
+      for (int sw = 0; sw < S->s.fac_sweeps; sw++) {
+         double mr = 0.0, mb = 0.0;
+         if (ghost_fill(L, x, 1, st)) return -1;
+         if (launch_rbgs<T>(L, B.op, dx, rhs, x, 0, want, st)) return -1;
+         if (want && reduce_partials(L, L.ncell_tiles, 1, 1, &mr, st)) return -1;
+         if (ghost_fill(L, x, 1, st)) return -1;
+         if (launch_rbgs<T>(L, B.op, dx, rhs, x, 1, want, st)) return -1;
+         if (want && reduce_partials(L, L.ncell_tiles, 1, 1, &mb, st)) return -1;
+         if (ghost_fill(L, x, 1, st)) return -1;
+         if (want && (mr > mb ? mr : mb) <= S->s.fac_tol) break;
+      }
+   } else {
+      int ql = B.ncomp;
+      if (ghost_fill(L, x, ql, st)) return -1;
+      for (int sw = 0; sw < S->s.quat_sweeps; sw++) {
+         if (ghost_fill(L, x, ql, st)) return -1;
+         if (launch_quat_rbgs<T>(L, B.fc, B.sm, S->gamma_setup, dx, ql, rhs, x, 0, st)) return -1;
+         if (ghost_fill(L, x, ql, st)) return -1;
+         if (launch_quat_rbgs<T>(L, B.fc, B.sm, S->gamma_setup, dx, ql, rhs, x, 1, st)) return -1;
+         if (ghost_fill(L, x, ql, st)) return -1;
+      }
    }
    return 0;
 }
 
-// Single-level FAC solve with the redblack coarse solver
-// (FACPreconditioner::solveSystem, source/samrai/FACPreconditioner.cc:267-429).
+// residual r = f - A u (ghost fill of u first), per-tile sums into region reg
 template <typename T>
-static int fac_solve(Sim* S, const EllOp<T>& op, const T* f, T* u, T* e, T* r, double tol)
+static int block_residual(Sim* S, PBlock<T>& B, int reg)
 {
    const Level& L = *S->L;
    hipStream_t st = S->stream;
-   const double* dx = S->p.dx;
-   AMPE_HIP_CHECK(hipMemsetAsync(u, 0, sizeof(T) * S->ng, st));
-   if (ghost_fill(L, u, 1, st)) return -1;
-   double ss = 0.0;
-   if (launch_residual<T>(L, op, dx, f, u, r, true, st)) return -1;
-   if (reduce_partials(L, L.ncell_tiles, 1, 0, &ss, st)) return -1;
-   double norm = std::sqrt(ss / (double)L.ncell);
-   int iters = 0;
-   do {
-      AMPE_HIP_CHECK(hipMemsetAsync(e, 0, sizeof(T) * S->ng, st));
-      if (smooth<T>(S, op, r, e)) return -1;
-      if (std::is_same<T, double>::value) {
-         if (vec_axpby(S->ng, 1.0, (const double*)u, 1.0, (const double*)e, (double*)u, st))
-            return -1;
-      } else {
-         if (vec_axpby_f((long long)S->ng, (float*)u, (const float*)e, st)) return -1;
-      }
-      if (ghost_fill(L, u, 1, st)) return -1;
-      if (launch_residual<T>(L, op, dx, f, u, r, true, st)) return -1;
-      if (reduce_partials(L, L.ncell_tiles, 1, 0, &ss, st)) return -1;
-      norm = std::sqrt(ss / (double)L.ncell);
-      iters++;
-   } while (norm > tol && iters < S->s.fac_max_cycles);
-   return norm < tol ? 0 : 1;
+   double* part = L.partial + (size_t)reg * L.ncell_tiles;
+   if (ghost_fill(L, B.u, B.ncomp, st)) return -1;
+   if (B.kind == 0) return launch_residual<T>(L, B.op, S->p.dx, B.f, B.u, B.r, part, st);
+   return launch_quat_residual<T>(L, B.fc, B.sm, S->gamma_setup, S->p.dx, B.ncomp, B.f, B.u,
+                                  B.r, part, st);
 }
 
-// Orientation block: divide by sqrt(M), red-black Gauss-Seidel cycles on
-// I + gamma sqrt(M) L sqrt(M), multiply by sqrt(M) (QuatSysSolver::
-// solveSystem, source/QuatSysSolver.cc:296-349; the hypre level solve of
-// QuatLevelSolver is replaced).
+// Each block runs the single-level FAC loop of FACPreconditioner::solveSystem
+// (source/samrai/FACPreconditioner.cc:267-429) with the redblack coarse solve:
+// cycle 1 starts from u = 0, so its correction is smoothed directly into u
+// (the residual of u = 0 is f); every cycle ends with the residual RMS.  The
+// blocks' norms are reduced together, one host synchronisation per cycle.
 template <typename T>
-static int quat_solve(Sim* S, const T* fc, const T* sm, const T* f, T* z, T* e, T* r,
-                      double tol)
+static int run_blocks(Sim* S, std::vector<PBlock<T>>& blocks, double tol)
 {
    const Level& L = *S->L;
    hipStream_t st = S->stream;
-   const double* dx = S->p.dx;
-   int ql = S->p.qlen;
-   double gamma = S->gamma_setup;
-   long long n = S->ng * ql;
-   AMPE_HIP_CHECK(hipMemsetAsync(z, 0, sizeof(T) * n, st));
-   if (ghost_fill(L, z, ql, st)) return -1;
-   double ss = 0.0;
-   if (launch_quat_residual<T>(L, fc, sm, gamma, dx, ql, f, z, r, true, st)) return -1;
-   if (reduce_partials(L, L.ncell_tiles, 1, 0, &ss, st)) return -1;
-   double norm = std::sqrt(ss / ((double)ql * L.ncell));
-   int iters = 0;
-   do {
-      AMPE_HIP_CHECK(hipMemsetAsync(e, 0, sizeof(T) * n, st));
-      if (ghost_fill(L, e, ql, st)) return -1;
-      for (int sw = 0; sw < S->s.quat_sweeps; sw++) {
-         if (ghost_fill(L, e, ql, st)) return -1;
-         if (launch_quat_rbgs<T>(L, fc, sm, gamma, dx, ql, r, e, 0, st)) return -1;
-         if (ghost_fill(L, e, ql, st)) return -1;
-         if (launch_quat_rbgs<T>(L, fc, sm, gamma, dx, ql, r, e, 1, st)) return -1;
-         if (ghost_fill(L, e, ql, st)) return -1;
+   int nb = (int)blocks.size();
+   for (int b = 0; b < nb; b++) {
+      PBlock<T>& B = blocks[b];
+      AMPE_HIP_CHECK(hipMemsetAsync(B.u, 0, sizeof(T) * S->ng * B.ncomp, st));
+      if (block_smooth<T>(S, B, B.f, B.u)) return -1;
+      if (block_residual<T>(S, B, b)) return -1;
+      B.cycles = 1;
+   }
+   double ss[8];
+   if (reduce_regions(L, L.ncell_tiles, nb, ss, st)) return -1;
+   for (int b = 0; b < nb; b++)
+      blocks[b].norm = std::sqrt(ss[b] / ((double)blocks[b].ncomp * L.ncell));
+   for (;;) {
+      std::vector<int> active;
+      for (int b = 0; b < nb; b++) {
+         PBlock<T>& B = blocks[b];
+         int maxc = B.kind == 0 ? S->s.fac_max_cycles : S->s.quat_max_cycles;
+         if (B.norm > tol && B.cycles < maxc) active.push_back(b);
       }
-      if (std::is_same<T, double>::value) {
-         if (vec_axpby(n, 1.0, (const double*)z, 1.0, (const double*)e, (double*)z, st))
-            return -1;
-      } else {
-         if (vec_axpby_f(n, (float*)z, (const float*)e, st)) return -1;
+      if (active.empty()) break;
+      for (size_t a = 0; a < active.size(); a++) {
+         PBlock<T>& B = blocks[active[a]];
+         AMPE_HIP_CHECK(hipMemsetAsync(B.e, 0, sizeof(T) * S->ng * B.ncomp, st));
+         if (block_smooth<T>(S, B, B.r, B.e)) return -1;
+         long long n = S->ng * B.ncomp;
+         if (std::is_same<T, double>::value) {
+            if (vec_axpby(n, 1.0, (const double*)B.u, 1.0, (const double*)B.e, (double*)B.u, st))
+               return -1;
+         } else {
+            if (vec_axpby_f(n, (float*)B.u, (const float*)B.e, st)) return -1;
+         }
+         if (block_residual<T>(S, B, (int)a)) return -1;
+         B.cycles++;
       }
-      if (ghost_fill(L, z, ql, st)) return -1;
-      if (launch_quat_residual<T>(L, fc, sm, gamma, dx, ql, f, z, r, true, st)) return -1;
-      if (reduce_partials(L, L.ncell_tiles, 1, 0, &ss, st)) return -1;
-      norm = std::sqrt(ss / ((double)ql * L.ncell));
-      iters++;
-   } while (norm > tol && iters < S->s.quat_max_cycles);
-   return norm < tol ? 0 : 1;
+      if (reduce_regions(L, L.ncell_tiles, (int)active.size(), ss, st)) return -1;
+      for (size_t a = 0; a < active.size(); a++) {
+         PBlock<T>& B = blocks[active[a]];
+         B.norm = std::sqrt(ss[a] / ((double)B.ncomp * L.ncell));
+      }
+   }
+   int ret = 0;
+   for (auto& B : blocks)
+      if (!(B.norm < tol)) ret = 1;
+   return ret;
 }
 
-// CVSpgmrPrecondSolve (source/QuatIntegrator.cc:3666-3771)
+// CVSpgmrPrecondSolve (source/QuatIntegrator.cc:3666-3771): phase block
+// (PhasePrecondSolve :3391-3449), orientation block (QuatPrecondSolve
+// :3594-3650, divide / multiply by sqrt(M) of QuatSysSolver::solveSystem,
+// source/QuatSysSolver.cc:296-349), composition block
+// (ConcentrationPrecondSolve :3547-3591).
 static int psolve(Sim* S, const double* r, double* z, double delta)
 {
    const ampe_model_params& p = S->p;
    hipStream_t st = S->stream;
    long long ng = S->ng;
-   int ret = 0, rc;
    S->st.nps++;
    AMPE_HIP_CHECK(hipMemsetAsync(z, 0, sizeof(double) * S->nv, st));
    double dphi = -S->gamma_setup * p.epsilon_phi * p.epsilon_phi;
    int ql = S->iq >= 0 ? p.qlen : 1;
+   if (S->iq >= 0 &&
+       vec_div_comp(S, r + (long long)S->iq * ng, S->sqrtm, S->fac_u, ql))  // FP64 division
+      return -1;
    if (S->s.precision == AMPE_FAC_FP32) {
-      EllOp<float> ph;
-      ph.Dc = (float)dphi;
-      ph.C = S->f_Cphi;
-      ph.Cc = 1.0f;
-      ph.Mc = (float)p.phi_mobility;
-      if (vec_to_f32(ng, r, S->f_f, st)) return -1;
-      rc = fac_solve<float>(S, ph, S->f_f, S->f_u, S->f_e, S->f_r, delta);
-      if (rc < 0) return -1;
-      ret |= rc;
-      if (vec_to_f64(ng, S->f_u, z, st)) return -1;
+      std::vector<PBlock<float>> bl;
+      PBlock<float> ph{};
+      ph.kind = 0;
+      ph.comp = 0;
+      ph.ncomp = 1;
+      ph.op.Dc = (float)dphi;
+      ph.op.C = S->f_Cphi;
+      ph.op.Cc = 1.0f;
+      ph.op.Mc = (float)p.phi_mobility;
+      bl.push_back(ph);
       if (S->iq >= 0) {
-         const double* rq = r + (long long)S->iq * ng;
-         // divide by sqrt(M) in FP64, then round
-         if (vec_div_comp(S, rq, S->sqrtm, S->fac_u, ql)) return -1;
-         if (vec_to_f32(ng * ql, S->fac_u, S->f_f, st)) return -1;
-         rc = quat_solve<float>(S, S->f_fc, S->f_sqrtm, S->f_f, S->f_u, S->f_e, S->f_r, delta);
-         if (rc < 0) return -1;
-         ret |= rc;
-         if (vec_to_f64(ng * ql, S->f_u, z + (long long)S->iq * ng, st)) return -1;
-         if (vec_mul_comp(S, z + (long long)S->iq * ng, S->sqrtm, ql)) return -1;
+         PBlock<float> qb{};
+         qb.kind = 1;
+         qb.comp = S->iq;
+         qb.ncomp = ql;
+         qb.fc = S->f_fc;
+         qb.sm = S->f_sqrtm;
+         bl.push_back(qb);
       }
       if (S->ic >= 0) {
-         EllOp<float> co;
-         co.D = S->f_Dconc;
-         co.Cc = 1.0f;
-         co.Mc = (float)p.conc_mobility;
-         if (vec_to_f32(ng, r + (long long)S->ic * ng, S->f_f, st)) return -1;
-         rc = fac_solve<float>(S, co, S->f_f, S->f_u, S->f_e, S->f_r, delta);
-         if (rc < 0) return -1;
-         ret |= rc;
-         if (vec_to_f64(ng, S->f_u, z + (long long)S->ic * ng, st)) return -1;
+         PBlock<float> cb{};
+         cb.kind = 0;
+         cb.comp = S->ic;
+         cb.ncomp = 1;
+         cb.op.D = S->f_Dconc;
+         cb.op.Cc = 1.0f;
+         cb.op.Mc = (float)p.conc_mobility;
+         bl.push_back(cb);
+      }
+      for (auto& B : bl) {
+         long long off = (long long)B.comp * ng;
+         B.f = S->f_f + off;
+         B.u = S->f_u + off;
+         B.e = S->f_e + off;
+         B.r = S->f_r + off;
+         const double* src = (B.kind == 1) ? S->fac_u : r + off;
+         if (vec_to_f32(ng * B.ncomp, src, S->f_f + off, st)) return -1;
+      }
+      int ret = run_blocks<float>(S, bl, delta);
+      if (ret < 0) return -1;
+      for (auto& B : bl) {
+         long long off = (long long)B.comp * ng;
+         if (vec_to_f64(ng * B.ncomp, B.u, z + off, st)) return -1;
+         if (B.kind == 1 && vec_mul_comp(S, z + off, S->sqrtm, ql)) return -1;
       }
       return ret;
    }
-   EllOp<double> ph;
-   ph.Dc = dphi;
-   ph.C = S->Cphi;
-   ph.Cc = 1.0;
-   ph.Mc = p.phi_mobility;
-   rc = fac_solve<double>(S, ph, r, z, S->fac_e, S->fac_r, delta);
-   if (rc < 0) return -1;
-   ret |= rc;
+   std::vector<PBlock<double>> bl;
+   PBlock<double> ph{};
+   ph.kind = 0;
+   ph.comp = 0;
+   ph.ncomp = 1;
+   ph.op.Dc = dphi;
+   ph.op.C = S->Cphi;
+   ph.op.Cc = 1.0;
+   ph.op.Mc = p.phi_mobility;
+   ph.f = r;
+   bl.push_back(ph);
    if (S->iq >= 0) {
-      const double* rq = r + (long long)S->iq * ng;
-      double* zq = z + (long long)S->iq * ng;
-      if (vec_div_comp(S, rq, S->sqrtm, S->fac_u, ql)) return -1;
-      rc = quat_solve<double>(S, S->fc, S->sqrtm, S->fac_u, zq, S->fac_e, S->fac_r, delta);
-      if (rc < 0) return -1;
-      ret |= rc;
-      if (vec_mul_comp(S, zq, S->sqrtm, ql)) return -1;
+      PBlock<double> qb{};
+      qb.kind = 1;
+      qb.comp = S->iq;
+      qb.ncomp = ql;
+      qb.fc = S->fc;
+      qb.sm = S->sqrtm;
+      qb.f = S->fac_u;
+      bl.push_back(qb);
    }
    if (S->ic >= 0) {
-      EllOp<double> co;
-      co.D = S->Dconc;
-      co.Cc = 1.0;
-      co.Mc = p.conc_mobility;
-      rc = fac_solve<double>(S, co, r + (long long)S->ic * ng, z + (long long)S->ic * ng,
-                             S->fac_e, S->fac_r, delta);
-      if (rc < 0) return -1;
-      ret |= rc;
+      PBlock<double> cb{};
+      cb.kind = 0;
+      cb.comp = S->ic;
+      cb.ncomp = 1;
+      cb.op.D = S->Dconc;
+      cb.op.Cc = 1.0;
+      cb.op.Mc = p.conc_mobility;
+      cb.f = r + (long long)S->ic * ng;
+      bl.push_back(cb);
    }
+   for (auto& B : bl) {
+      long long off = (long long)B.comp * ng;
+      B.u = z + off;
+      B.e = S->fac_e + off;
+      B.r = S->fac_r + off;
+   }
+   int ret = run_blocks<double>(S, bl, delta);
+   if (ret < 0) return -1;
+   if (S->iq >= 0 && vec_mul_comp(S, z + (long long)S->iq * ng, S->sqrtm, ql)) return -1;
    return ret;
 }
 
@@ -492,79 +557,57 @@ static double cellvol(const Sim* S)
    return dx[0] * dx[1] * (S->p.ndim == 3 ? dx[2] : 1.0);
 }
 
-static int dot(Sim* S, const double* x, const double* y, double* out)
-{
-   double v;
-   if (vec_dot(*S->L, S->nc, x, y, &v, S->stream)) return -1;
-   *out = v * cellvol(S);
-   return 0;
-}
-
-static int wrms(Sim* S, const double* x, double* out)
-{
-   double v;
-   if (vec_sumsq_w(*S->L, S->nc, x, S->ewt, &v, S->stream)) return -1;
-   *out = std::sqrt(v / ((double)S->nc * S->L->ncell));
-   return 0;
-}
-
-// Jv by difference quotient (sigma = 1/||v||_wrms), out = v - gamma Jv
-static int atimes(Sim* S, double gamma, double* v, double* out)
-{
-   hipStream_t st = S->stream;
-   double nv;
-   if (wrms(S, v, &nv)) return -1;
-   double sig = 1.0 / nv;
-   if (vec_axpby(S->nv, sig, v, 1.0, S->ycur, S->work2, st)) return -1;
-   if (rhs_eval(S, S->work2, S->jv, true)) return -1;
-   double siginv = 1.0 / sig;
-   if (vec_axpby(S->nv, siginv, S->jv, -siginv, S->fcur, S->jv, st)) return -1;
-   if (vec_axpby(S->nv, 1.0, v, -gamma, S->jv, out, st)) return -1;
-   return 0;
-}
-
-// SPGMR, left preconditioned, MGS, x0 = 0, scaling by ewt.
-// Returns 0 converged, 1 residual reduced, 2 failure, -1 fatal.
+// SPGMR, left preconditioned, MGS, x0 = 0, scaling by ewt, difference-quotient
+// Jv (sigma = 1/||v||_wrms) and A v = v - gamma Jv.  V[l] is normalised lazily
+// by the next iteration's first kernel.  Returns 0 converged, 1 residual
+// reduced, 2 failure, -1 fatal.
 static int spgmr(Sim* S, double gamma, const double* bvec, double* x, double delta)
 {
+   const Level& L = *S->L;
    hipStream_t st = S->stream;
-   int maxl = S->s.maxl;
-   long long nv = S->nv;
+   int maxl = S->s.maxl, nc = S->nc;
+   double cv = cellvol(S);
+   double wfac = 1.0 / ((double)nc * L.ncell);
    std::vector<double>& Hm = S->H;
    Hm.assign((size_t)(maxl + 1) * (maxl + 1), 0.0);
    int ld = maxl + 1;
 #define HH(i, j) Hm[(size_t)(i) * ld + (j)]
-   double givens[64], g[32], yv[32];
-   AMPE_HIP_CHECK(hipMemsetAsync(x, 0, sizeof(double) * nv, st));
-   int pr = psolve(S, bvec, S->work, delta);
-   if (pr < 0) return -1;
-   if (vec_mul(nv, S->ewt, S->work, S->V[0], st)) return -1;
+   double givens[2 * kMaxKrylov], g[kMaxKrylov + 1], yv[kMaxKrylov];
+   if (psolve(S, bvec, S->work, delta) < 0) return -1;
    double rho;
-   if (dot(S, S->V[0], S->V[0], &rho)) return -1;
-   rho = std::sqrt(rho);
+   if (kry_scale_dot(L, nc, S->ewt, S->work, S->V[0], S->V[0], &rho, st)) return -1;
+   rho = std::sqrt(rho * cv);
    double rho0 = rho;
-   if (rho <= delta) return 0;
-   double rinv = 1.0 / rho;
-   if (vec_axpby(nv, rinv, S->V[0], 0.0, S->V[0], S->V[0], st)) return -1;
+   if (rho <= delta) {
+      AMPE_HIP_CHECK(hipMemsetAsync(x, 0, sizeof(double) * S->nv, st));
+      return 0;
+   }
+   double hinv = 1.0 / rho;
    std::memset(g, 0, sizeof(g));
    g[0] = rho;
    int lused = 0;
    bool converged = false;
    for (int l = 0; l < maxl; l++) {
       S->st.nli++;
-      if (vec_div(nv, S->V[l], S->ewt, S->xk, st)) return -1;
-      if (atimes(S, gamma, S->xk, S->V[l + 1])) return -1;
+      double ss;
+      // V[l] *= hinv; xk = V[l] / ewt; ||xk||_wrms
+      if (kry_prep(L, nc, hinv, S->V[l], S->ewt, S->xk, &ss, st)) return -1;
+      double sig = 1.0 / std::sqrt(ss * wfac);
+      if (kry_mid(L, nc, sig, S->xk, S->ycur, S->work2, st)) return -1;
+      if (rhs_eval(S, S->work2, S->jv, true)) return -1;
+      double siginv = 1.0 / sig;
+      if (kry_finish(L, nc, siginv, gamma, S->xk, S->jv, S->fcur, S->V[l + 1], st)) return -1;
       if (psolve(S, S->V[l + 1], S->work, delta) < 0) return -1;
-      if (vec_mul(nv, S->ewt, S->work, S->V[l + 1], st)) return -1;
+      // V[l+1] = ewt work; modified Gram-Schmidt, each step fused with the next dot
+      double dot;
+      if (kry_scale_dot(L, nc, S->ewt, S->work, S->V[l + 1], S->V[0], &dot, st)) return -1;
       for (int i = 0; i <= l; i++) {
-         double hij;
-         if (dot(S, S->V[i], S->V[l + 1], &hij)) return -1;
+         double hij = dot * cv;
          HH(i, l) = hij;
-         if (vec_axpby(nv, 1.0, S->V[l + 1], -hij, S->V[i], S->V[l + 1], st)) return -1;
+         const double* nxt = (i < l) ? S->V[i + 1] : nullptr;
+         if (kry_mgs(L, nc, S->V[l + 1], S->V[i], hij, nxt, &dot, st)) return -1;
       }
-      double hn;
-      if (dot(S, S->V[l + 1], S->V[l + 1], &hn)) return -1;
-      hn = std::sqrt(hn);
+      double hn = std::sqrt(dot * cv);
       HH(l + 1, l) = hn;
       for (int i = 0; i < l; i++) {
          double c = givens[2 * i], s = givens[2 * i + 1];
@@ -598,17 +641,20 @@ static int spgmr(Sim* S, double gamma, const double* bvec, double* x, double del
          break;
       }
       if (hn == 0.0) break;
-      double hinv = 1.0 / hn;
-      if (vec_axpby(nv, hinv, S->V[l + 1], 0.0, S->V[l + 1], S->V[l + 1], st)) return -1;
+      hinv = 1.0 / hn;
    }
    for (int i = lused - 1; i >= 0; i--) {
       double sacc = g[i];
       for (int j = i + 1; j < lused; j++) sacc -= HH(i, j) * yv[j];
       yv[i] = sacc / HH(i, i);
    }
-   for (int i = 0; i < lused; i++)
-      if (vec_axpby(nv, 1.0, x, yv[i], S->V[i], x, st)) return -1;
-   if (vec_div(nv, x, S->ewt, x, st)) return -1;
+   VecList vl;
+   vl.n = lused;
+   for (int i = 0; i < lused; i++) {
+      vl.v[i] = S->V[i];
+      vl.y[i] = yv[i];
+   }
+   if (kry_lincomb(L, nc, vl, S->ewt, x, st)) return -1;
 #undef HH
    if (converged) return 0;
    return rho < rho0 ? 1 : 2;
@@ -617,9 +663,11 @@ static int spgmr(Sim* S, double gamma, const double* bvec, double* x, double del
 // ------------------------------------------------------------ BDF step
 static int newton(Sim* S, double gamma, double delta, bool force_setup, bool* did_setup)
 {
+   const Level& L = *S->L;
    hipStream_t st = S->stream;
    long long nv = S->nv;
    double tq4 = S->s.nlscoef;
+   double wfac = 1.0 / ((double)S->nc * L.ncell);
    bool call_setup = force_setup || S->nst == 0 ||
                      std::fabs(gamma / S->gamma_setup - 1.0) > 0.3 ||
                      S->nst >= S->nstlp + S->s.msbp;
@@ -630,9 +678,7 @@ static int newton(Sim* S, double gamma, double delta, bool force_setup, bool* di
    for (int m = 0; m < S->s.max_newton; m++) {
       S->st.nni++;
       if (rhs_eval(S, S->ycur, S->fcur, false)) return -1;
-      // b = gamma f - (y - a)
-      if (vec_axpby(nv, 1.0, S->ycur, -1.0, S->ahist, S->b, st)) return -1;
-      if (vec_axpby(nv, gamma, S->fcur, -1.0, S->b, S->b, st)) return -1;
+      if (kry_newton_rhs(L, S->nc, gamma, S->fcur, S->ycur, S->ahist, S->b, st)) return -1;
       if (m == 0 && call_setup) {
          if (psetup(S, S->ycur, gamma)) return -1;
          S->nstlp = S->nst;
@@ -645,9 +691,9 @@ static int newton(Sim* S, double gamma, double delta, bool force_setup, bool* di
          S->st.nlcf++;
          return 1;
       }
-      double del;
-      if (wrms(S, S->del, &del)) return -1;
-      if (vec_axpby(nv, 1.0, S->ycur, 1.0, S->del, S->ycur, st)) return -1;
+      double ss;
+      if (kry_newton_update(L, S->nc, S->del, S->ewt, S->ycur, &ss, st)) return -1;
+      double del = std::sqrt(ss * wfac);
       if (m > 0) {
          double r = del / delp;
          S->crate = (0.3 * S->crate > r) ? 0.3 * S->crate : r;
@@ -665,6 +711,7 @@ static int bdf_step(Sim* S)
    const Level& L = *S->L;
    hipStream_t st = S->stream;
    long long nv = S->nv, ng = S->ng;
+   double wfac = 1.0 / ((double)S->nc * L.ncell);
    if (S->ic >= 0) {
       // resetRefPhaseConcentrations (source/QuatModel.cc:5218-5235)
       AMPE_HIP_CHECK(hipMemcpyAsync(S->clr, S->cl, sizeof(double) * ng,
@@ -682,7 +729,7 @@ static int bdf_step(Sim* S)
                                        hipMemcpyDeviceToDevice, st));
       } else {
          double w = h / S->hprev;
-         // ypred = y + w (y - yprev) = (1 + w) y - w yprev, in the restated order
+         // ypred = y + w (y - yprev)
          if (vec_axpby(nv, 1.0, S->y, -1.0, S->yprev, S->work, st)) return -1;
          if (vec_axpby(nv, 1.0, S->y, w, S->work, S->ypred, st)) return -1;
       }
@@ -700,6 +747,7 @@ static int bdf_step(Sim* S)
       bool did_setup = false;
       int r = newton(S, gamma, delta, force_setup, &did_setup);
       if (r < 0) return -1;
+      if (kks_collect(S)) return -1;
       if (r == 0) {
          if (vec_axpby(nv, 1.0, S->ycur, -1.0, S->ypred, S->acor, st)) return -1;
          if (S->iq >= 0 && S->p.qlen > 1) {
@@ -710,7 +758,9 @@ static int bdf_step(Sim* S)
                return -1;
             if (vec_axpby(nv, 1.0, S->ycur, 1.0, S->corr, S->ycur, st)) return -1;
          }
-         if (wrms(S, S->acor, &S->st.last_acnrm)) return -1;
+         double ss;
+         if (vec_sumsq_w(L, S->nc, S->acor, S->ewt, &ss, st)) return -1;
+         S->st.last_acnrm = std::sqrt(ss * wfac);
          std::swap(S->yprev, S->y);
          std::swap(S->y, S->ycur);
          S->have_prev = true;

@@ -214,7 +214,7 @@ double o_dfenergy(const ampe_model_params* p, double c, int ph)
    double x = 2.0 * c - 1.0;
    double P = cp->L[0] + x * (cp->L[1] + x * (cp->L[2] + x * cp->L[3]));
    double dP = cp->L[1] + x * (2.0 * cp->L[2] + x * 3.0 * cp->L[3]);
-   return cp->gA - cp->gB + RT * (log(c) - log(1.0 - c)) +
+   return cp->gA - cp->gB + RT * log(c / (1.0 - c)) +
           (1.0 - 2.0 * c) * P + 2.0 * c * (1.0 - c) * dP;
 }
 
