@@ -5,6 +5,7 @@
 #include <string>
 
 #include "../../include/ampe_hip.h"
+#include "kks.h"
 #include "sim.h"
 
 namespace ampe {
@@ -18,6 +19,9 @@ struct ampe_level {
 };
 struct ampe_sim {
    ampe::Sim* S;
+};
+struct ampe_surrogate {
+   ampe::Surrogate* S;
 };
 
 #define AMPE_REQUIRE(cond, msg)        \
@@ -295,6 +299,127 @@ int ampe_sim_last_time_ms(const ampe_sim* sim, double* ms)
    AMPE_REQUIRE(sim && ms, "null argument");
    *ms = sim->S->last_ms;
    return 0;
+}
+
+int ampe_sim_set_surrogate(ampe_sim* sim, const ampe_surrogate* s, int probe_its,
+                           double probe_tol)
+{
+   AMPE_REQUIRE(sim, "null argument");
+   AMPE_REQUIRE(probe_its >= 0, "probe_its must be >= 0");
+   if (s) {
+      const ampe::Surrogate& m = *s->S;
+      AMPE_REQUIRE(m.nin == 3 && m.nout == 2,
+                   "the KKS surrogate maps (c, phi, T) -> (cL, cS): n_in 3, n_out 2");
+   }
+   sim->S->guess.surrogate = s ? s->S : nullptr;
+   sim->S->guess.probe_its = probe_its;
+   sim->S->guess.probe_tol = probe_tol;
+   return 0;
+}
+
+int ampe_calphad_at(const ampe_calphad_tdep* tab, double T, ampe_calphad_phase out[2])
+{
+   AMPE_REQUIRE(tab && out, "null argument");
+   AMPE_REQUIRE(T > 0.0, "temperature must be positive");
+   for (int ph = 0; ph < 2; ph++)
+      for (int sp = 0; sp < 2; sp++)
+         AMPE_REQUIRE(tab->g[ph][sp].nrange >= 1 && tab->g[ph][sp].nrange <= 4,
+                      "gfun nrange must be in [1, 4]");
+   ampe::calphad_at(*tab, T, out);
+   return 0;
+}
+
+int ampe_surrogate_create(int n_in, int n_hidden, int n_out, unsigned long long seed,
+                          ampe_surrogate** out)
+{
+   AMPE_REQUIRE(out, "null output handle");
+   ampe::Surrogate* S = nullptr;
+   if (ampe::surrogate_create(n_in, n_hidden, n_out, seed, &S)) return -1;
+   *out = new ampe_surrogate{S};
+   return 0;
+}
+
+int ampe_surrogate_destroy(ampe_surrogate* s)
+{
+   if (!s) return 0;
+   ampe::surrogate_destroy(s->S);
+   delete s;
+   return 0;
+}
+
+int ampe_surrogate_nparams(const ampe_surrogate* s, long long* n)
+{
+   AMPE_REQUIRE(s && n, "null argument");
+   *n = s->S->nparam;
+   return 0;
+}
+
+int ampe_surrogate_set_params(ampe_surrogate* s, const float* params)
+{
+   AMPE_REQUIRE(s && params, "null argument");
+   return ampe::surrogate_set_params(s->S, params);
+}
+
+int ampe_surrogate_get_params(const ampe_surrogate* s, float* params)
+{
+   AMPE_REQUIRE(s && params, "null argument");
+   return ampe::surrogate_get_params(s->S, params);
+}
+
+int ampe_surrogate_set_scaling(ampe_surrogate* s, const double* in_shift, const double* in_scale,
+                               const double* out_shift, const double* out_scale)
+{
+   AMPE_REQUIRE(s && in_shift && in_scale && out_shift && out_scale, "null argument");
+   ampe::Surrogate& m = *s->S;
+   for (int i = 0; i < m.nin; i++) {
+      AMPE_REQUIRE(std::isfinite(in_scale[i]) && in_scale[i] != 0.0, "bad input scale");
+      m.in_shift[i] = (float)in_shift[i];
+      m.in_scale[i] = (float)in_scale[i];
+   }
+   for (int i = 0; i < m.nout; i++) {
+      AMPE_REQUIRE(std::isfinite(out_scale[i]) && out_scale[i] != 0.0, "bad output scale");
+      m.out_shift[i] = out_shift[i];
+      m.out_scale[i] = out_scale[i];
+   }
+   return 0;
+}
+
+int ampe_surrogate_forward(const ampe_surrogate* s, long long n, const float* x, float* y,
+                           ampe_stream_t stream)
+{
+   AMPE_REQUIRE(s && (n == 0 || (x && y)), "null argument");
+   AMPE_REQUIRE(n >= 0, "negative n");
+   return ampe::surrogate_forward(s->S, n, x, y, hs(stream));
+}
+
+int ampe_surrogate_loss_grad(ampe_surrogate* s, long long n, const float* x, const float* t,
+                             float* grad, double* loss, ampe_stream_t stream)
+{
+   AMPE_REQUIRE(s && x && t && grad, "null argument");
+   return ampe::surrogate_loss_grad(s->S, n, x, t, grad, loss, hs(stream));
+}
+
+int ampe_surrogate_train(ampe_surrogate* s, long long n, const double* x, const double* t,
+                         const ampe_train_params* tp, float* loss_hist, ampe_stream_t stream)
+{
+   AMPE_REQUIRE(s && x && t && tp, "null argument");
+   AMPE_REQUIRE(tp->lr0 > 0.0 && tp->lr1 > 0.0, "learning rates must be positive");
+   AMPE_REQUIRE(tp->beta1 >= 0.0 && tp->beta1 < 1.0 && tp->beta2 >= 0.0 && tp->beta2 < 1.0,
+                "Adam betas must be in [0, 1)");
+   return ampe::surrogate_train(s->S, n, x, t, *tp, loss_hist, hs(stream));
+}
+
+int ampe_kks_batch(const ampe_model_params* p, const ampe_calphad_tdep* tab, long long n,
+                   const double* c, const double* phi, const double* T,
+                   const ampe_surrogate* guess, int probe_its, double probe_tol, double* cl,
+                   double* cs, int* iters, ampe_kks_batch_stats* stats, ampe_stream_t stream)
+{
+   AMPE_REQUIRE(p && tab, "null argument");
+   AMPE_REQUIRE(n >= 0, "negative n");
+   AMPE_REQUIRE(n == 0 || (c && phi && T && cl && cs), "null argument");
+   AMPE_REQUIRE(!guess || probe_its >= 0, "probe_its must be >= 0");
+   return ampe::kks_batch(*p, *tab, n, c, phi, T, guess ? guess->S : nullptr, probe_its,
+                          probe_tol, cl, cs, iters, stats, hs(stream));
 }
 
 }  // extern "C"

@@ -83,16 +83,6 @@ int launch_conc_coeffs(const Level& L, const ampe_model_params& p, const double*
 int launch_conc_rhs(const Level& L, const ampe_model_params& p, const double* c,
                     const double* phi, const double* Dc, const double* Dphi, double* rhs,
                     hipStream_t s);
-// KKS over all ghosted cells.  launch_kks accumulates into device counters
-// dev_acc[4] = {total iterations, max iterations, failure flag, calls}
-// without synchronising; launch_kks_sync returns max iterations (or -1 on a
-// non-converged cell) and the total, synchronising s.
-int launch_kks(const Level& L, const ampe_model_params& p, const double* phi,
-               const double* c, const double* clr, const double* csr, double* cl,
-               double* cs, long long* dev_acc, hipStream_t s);
-int launch_kks_sync(const Level& L, const ampe_model_params& p, const double* phi,
-                    const double* c, const double* clr, const double* csr, double* cl,
-                    double* cs, int* max_iters, long long* total_iters, hipStream_t s);
 int launch_tag(const Level& L, const ampe_model_params& p, const double* phi, double tol,
                int* tags, hipStream_t s);
 int launch_quat_face_coef(const Level& L, const ampe_model_params& p, const double* phi,

@@ -1,0 +1,368 @@
+// Per-cell KKS phase-concentration solves.
+//
+// k_kks: the integrator's call over all ghosted cells of a level
+// (PhaseConcentrationsStrategy::computePhaseConcentrations,
+// source/PhaseConcentrationsStrategy.cc:26-127, ghost cells included per
+// source/QuatIntegrator.cc:3119-3126), guess from the previous step's
+// concentrations (source/CALPHADequilibriumPhaseConcentrationsStrategy.cc:
+// 368-369) or from the surrogate MLP.
+// k_kks_batch: independent cells with a per-cell temperature and the end
+// members evaluated at that temperature, default start (0.5, 0.5)
+// (tests/testCALPHADbinaryKKS.cc:123-127) or the surrogate.
+//
+// Each workgroup writes {sum of updates, max updates or failure marker,
+// fallbacks} to its partial slot; a one-workgroup finish kernel folds them
+// into device counters in a fixed order (no same-address atomics, no host
+// synchronisation per call).
+#include <algorithm>
+#include <cstring>
+
+#include "kks.h"
+
+namespace ampe {
+
+constexpr double kKksFail = 1.0e9;
+constexpr int kKksPart = 4;  // doubles per workgroup
+
+// it: updates, mx: updates or kKksFail, fb: fallback (0/1), nf: failure (0/1)
+__device__ inline void kks_block_partial(double it, double mx, double fb, double nf,
+                                         double* __restrict__ partial)
+{
+   for (int o = 32; o > 0; o >>= 1) {
+      it += __shfl_xor(it, o);
+      fb += __shfl_xor(fb, o);
+      nf += __shfl_xor(nf, o);
+      double v = __shfl_xor(mx, o);
+      mx = v > mx ? v : mx;
+   }
+   __shared__ double sh[16];
+   int w = threadIdx.x >> 6;
+   if ((threadIdx.x & 63) == 0) {
+      sh[4 * w] = it;
+      sh[4 * w + 1] = mx;
+      sh[4 * w + 2] = fb;
+      sh[4 * w + 3] = nf;
+   }
+   __syncthreads();
+   if (threadIdx.x == 0) {
+      double s = 0.0, m = 0.0, f = 0.0, e = 0.0;
+      for (int k = 0; k < 4; k++) {
+         s += sh[4 * k];
+         m = sh[4 * k + 1] > m ? sh[4 * k + 1] : m;
+         f += sh[4 * k + 2];
+         e += sh[4 * k + 3];
+      }
+      partial[kKksPart * blockIdx.x] = s;
+      partial[kKksPart * blockIdx.x + 1] = m;
+      partial[kKksPart * blockIdx.x + 2] = f;
+      partial[kKksPart * blockIdx.x + 3] = e;
+   }
+}
+
+// Surrogate guess for the lane's cell; all 64 lanes of the wave call it.
+__device__ inline void kks_surrogate_guess(const MlpDev& m, const u16x8* sf, const float* sb,
+                                           double c, double phi, double T, double& gl,
+                                           double& gs)
+{
+   float x[kMlpMaxIn] = {0.f, 0.f, 0.f, 0.f};
+   x[0] = ((float)c - m.in_shift[0]) * m.in_scale[0];
+   x[1] = ((float)phi - m.in_shift[1]) * m.in_scale[1];
+   x[2] = ((float)T - m.in_shift[2]) * m.in_scale[2];
+   float y[kMlpMaxOut];
+   mlp_wave_infer<kMlpHidden>(sf, sb, m.nin, x, y);
+   gl = (double)y[0] * m.out_scale[0] + m.out_shift[0];
+   gs = (double)y[1] * m.out_scale[1] + m.out_shift[1];
+}
+
+template <bool SURR>
+__global__ void __launch_bounds__(256) k_kks(long long n, ampe_model_params p, MlpDev m,
+                                             int probe_its, double probe_tol,
+                                             const double* __restrict__ phi,
+                                             const double* __restrict__ c,
+                                             const double* __restrict__ clr,
+                                             const double* __restrict__ csr,
+                                             double* __restrict__ cl, double* __restrict__ cs,
+                                             double* __restrict__ partial)
+{
+   using F = MlpFrags<kMlpHidden>;
+   __shared__ u16x8 sf[SURR ? F::NFWD * 64 : 1];
+   __shared__ float sb[SURR ? F::NBIAS : 1];
+   if (SURR) {
+      mlp_stage<kMlpHidden>(m, F::NFWD, sf, sb);
+      __syncthreads();
+   }
+   long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+   bool in = t < n;
+   double ct = in ? c[t] : 0.5, pt = in ? phi[t] : 0.0;
+   double it = 0.0, mx = 0.0, fb = 0.0;
+   if (SURR && p.fe_model != AMPE_FE_QUADRATIC) {
+      double gl, gs;
+      kks_surrogate_guess(m, sf, sb, ct, pt, p.temperature, gl, gs);
+      if (in) {
+         double h = interp(pt, p.conc_interp);
+         int r = kks_newton(p.calphad, p.temperature, h, ct, gl, gs, p.newton_max_its,
+                            p.newton_tol, probe_its, probe_tol);
+         if (r == -2) {
+            fb = 1.0;
+            gl = clr[t];
+            gs = csr[t];
+            int r2 = kks_newton(p.calphad, p.temperature, h, ct, gl, gs, p.newton_max_its,
+                                p.newton_tol, -1, 0.0);
+            r = r2 < 0 ? -1 : r2 + probe_its;
+         }
+         cl[t] = gl;
+         cs[t] = gs;
+         mx = r < 0 ? kKksFail : (double)r;
+         it = r < 0 ? 0.0 : (double)r;
+      }
+   } else if (in) {
+      double xl = clr[t], xs = csr[t];
+      int r = kks_cell(p, ct, pt, xl, xs);
+      cl[t] = xl;
+      cs[t] = xs;
+      mx = r < 0 ? kKksFail : (double)r;
+      it = r < 0 ? 0.0 : (double)r;
+   }
+   kks_block_partial(it, mx, fb, mx >= kKksFail ? 1.0 : 0.0, partial);
+}
+
+// acc[0] += updates, acc[1] = max(acc[1], max updates), acc[2] |= failure,
+// acc[3] += 1 (calls), acc[4] += fallbacks.  One workgroup.
+__global__ void __launch_bounds__(1024) k_kks_finish(const double* __restrict__ partial, int nb,
+                                                     long long* __restrict__ acc)
+{
+   __shared__ double ss[1024], sm[1024], sf[1024];
+   double s = 0.0, m = 0.0, f = 0.0;
+   for (int i = threadIdx.x; i < nb; i += 1024) {
+      s += partial[kKksPart * i];
+      m = partial[kKksPart * i + 1] > m ? partial[kKksPart * i + 1] : m;
+      f += partial[kKksPart * i + 2];
+   }
+   ss[threadIdx.x] = s;
+   sm[threadIdx.x] = m;
+   sf[threadIdx.x] = f;
+   __syncthreads();
+   for (int w = 512; w > 0; w >>= 1) {
+      if (threadIdx.x < w) {
+         ss[threadIdx.x] += ss[threadIdx.x + w];
+         sm[threadIdx.x] = sm[threadIdx.x + w] > sm[threadIdx.x] ? sm[threadIdx.x + w]
+                                                                 : sm[threadIdx.x];
+         sf[threadIdx.x] += sf[threadIdx.x + w];
+      }
+      __syncthreads();
+   }
+   if (threadIdx.x == 0) {
+      bool fail = sm[0] >= kKksFail;
+      acc[0] += (long long)ss[0];
+      long long mxi = fail ? 0 : (long long)sm[0];
+      acc[1] = mxi > acc[1] ? mxi : acc[1];
+      if (fail) acc[2] = 1;
+      acc[3] += 1;
+      acc[4] += (long long)sf[0];
+   }
+}
+
+MlpDev Surrogate::dev() const
+{
+   MlpDev d;
+   d.nin = nin;
+   d.nout = nout;
+   d.frag = frag;
+   d.bias = bias;
+   for (int i = 0; i < kMlpMaxIn; i++) {
+      d.in_shift[i] = in_shift[i];
+      d.in_scale[i] = in_scale[i];
+   }
+   for (int i = 0; i < kMlpMaxOut; i++) {
+      d.out_shift[i] = out_shift[i];
+      d.out_scale[i] = out_scale[i];
+   }
+   return d;
+}
+
+int launch_kks(const Level& L, const ampe_model_params& p, const double* phi,
+               const double* c, const double* clr, const double* csr, double* cl,
+               double* cs, const KksGuess& guess, long long* dev_acc, hipStream_t s)
+{
+   unsigned nb = (unsigned)((L.ncg + 255) / 256);
+   MlpDev m{};
+   if (guess.surrogate) {
+      const Surrogate& S = *guess.surrogate;
+      if (S.hid != kMlpHidden || S.nin != 3 || S.nout != 2) {
+         set_error("launch_kks: the surrogate must map (c, phi, T) -> (cL, cS) with 64 hidden");
+         return -1;
+      }
+      m = S.dev();
+      hipLaunchKernelGGL(k_kks<true>, dim3(nb), dim3(256), 0, s, L.ncg, p, m, guess.probe_its,
+                         guess.probe_tol, phi, c, clr, csr, cl, cs, L.kks_part);
+   } else {
+      hipLaunchKernelGGL(k_kks<false>, dim3(nb), dim3(256), 0, s, L.ncg, p, m, -1, 0.0, phi, c,
+                         clr, csr, cl, cs, L.kks_part);
+   }
+   AMPE_HIP_CHECK(hipGetLastError());
+   hipLaunchKernelGGL(k_kks_finish, dim3(1), dim3(1024), 0, s, L.kks_part, (int)nb, dev_acc);
+   AMPE_HIP_CHECK(hipGetLastError());
+   return 0;
+}
+
+int launch_kks_sync(const Level& L, const ampe_model_params& p, const double* phi,
+                    const double* c, const double* clr, const double* csr, double* cl,
+                    double* cs, int* max_iters, long long* total_iters, hipStream_t s)
+{
+   long long* acc = reinterpret_cast<long long*>(L.iscratch);
+   AMPE_HIP_CHECK(hipMemsetAsync(acc, 0, sizeof(long long) * kKksAcc, s));
+   if (launch_kks(L, p, phi, c, clr, csr, cl, cs, KksGuess{}, acc, s)) return -1;
+   AMPE_HIP_CHECK(hipMemcpyAsync(L.hpinned, acc, sizeof(long long) * kKksAcc,
+                                 hipMemcpyDeviceToHost, s));
+   AMPE_HIP_CHECK(hipStreamSynchronize(s));
+   long long h[kKksAcc];
+   memcpy(h, L.hpinned, sizeof(h));
+   if (max_iters) *max_iters = h[2] ? -1 : (int)h[1];
+   if (total_iters) *total_iters = h[0];
+   return 0;
+}
+
+// ------------------------------------------------------------ batch
+template <bool SURR>
+__global__ void __launch_bounds__(256) k_kks_batch(long long n, ampe_model_params p,
+                                                   ampe_calphad_tdep tab, MlpDev m,
+                                                   int probe_its, double probe_tol,
+                                                   const double* __restrict__ c,
+                                                   const double* __restrict__ phi,
+                                                   const double* __restrict__ T,
+                                                   double* __restrict__ cl,
+                                                   double* __restrict__ cs,
+                                                   int* __restrict__ iters,
+                                                   double* __restrict__ partial)
+{
+   using F = MlpFrags<kMlpHidden>;
+   __shared__ u16x8 sf[SURR ? F::NFWD * 64 : 1];
+   __shared__ float sb[SURR ? F::NBIAS : 1];
+   if (SURR) {
+      mlp_stage<kMlpHidden>(m, F::NFWD, sf, sb);
+      __syncthreads();
+   }
+   long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+   bool in = t < n;
+   double ct = in ? c[t] : 0.5, pt = in ? phi[t] : 0.0, Tt = in ? T[t] : 1500.0;
+   double gl = 0.5, gs = 0.5;
+   if (SURR) kks_surrogate_guess(m, sf, sb, ct, pt, Tt, gl, gs);
+   double it = 0.0, mx = 0.0, fb = 0.0;
+   if (in) {
+      ampe_calphad_phase ph[2];
+      calphad_at(tab, Tt, ph);
+      double h = interp(pt, p.conc_interp);
+      int r;
+      if (SURR) {
+         r = kks_newton(ph, Tt, h, ct, gl, gs, p.newton_max_its, p.newton_tol, probe_its,
+                        probe_tol);
+         if (r == -2) {
+            fb = 1.0;
+            gl = 0.5;
+            gs = 0.5;
+            int r2 = kks_newton(ph, Tt, h, ct, gl, gs, p.newton_max_its, p.newton_tol, -1, 0.0);
+            r = r2 < 0 ? -1 : r2 + probe_its;
+         }
+      } else {
+         r = kks_newton(ph, Tt, h, ct, gl, gs, p.newton_max_its, p.newton_tol, -1, 0.0);
+      }
+      cl[t] = gl;
+      cs[t] = gs;
+      if (iters) iters[t] = r;
+      mx = r < 0 ? kKksFail : (double)r;
+      it = r < 0 ? 0.0 : (double)r;
+   }
+   kks_block_partial(it, mx, fb, mx >= kKksFail ? 1.0 : 0.0, partial);
+}
+
+__global__ void k_kks_batch_finish(const double* __restrict__ partial, int nb,
+                                   long long* __restrict__ acc)
+{
+   // one thread, fixed order; a workgroup whose max is the failure marker
+   // still reports the largest count of its converged cells through s
+   if (threadIdx.x != 0 || blockIdx.x != 0) return;
+   long long s = 0, m = 0, f = 0, fails = 0;
+   for (int i = 0; i < nb; i++) {
+      s += (long long)partial[kKksPart * i];
+      double mx = partial[kKksPart * i + 1];
+      if (mx < kKksFail && (long long)mx > m) m = (long long)mx;
+      f += (long long)partial[kKksPart * i + 2];
+      fails += (long long)partial[kKksPart * i + 3];
+   }
+   acc[0] = s;
+   acc[1] = m;
+   acc[2] = f;
+   acc[3] = fails;
+}
+
+int kks_batch(const ampe_model_params& p, const ampe_calphad_tdep& tab, long long n,
+              const double* c, const double* phi, const double* T, const Surrogate* guess,
+              int probe_its, double probe_tol, double* cl, double* cs, int* iters,
+              ampe_kks_batch_stats* stats, hipStream_t s)
+{
+   if (n <= 0) {
+      if (stats) std::memset(stats, 0, sizeof(*stats));
+      return 0;
+   }
+   if (p.fe_model != AMPE_FE_CALPHAD_BINARY) {
+      set_error("kks_batch: the model must be binary CALPHAD");
+      return -1;
+   }
+   for (int ph = 0; ph < 2; ph++)
+      for (int sp = 0; sp < 2; sp++) {
+         int nr = tab.g[ph][sp].nrange;
+         if (nr < 1 || nr > 4) {
+            set_error("kks_batch: gfun nrange must be in [1, 4]");
+            return -1;
+         }
+      }
+   if (guess && (guess->hid != kMlpHidden || guess->nin != 3 || guess->nout != 2)) {
+      set_error("kks_batch: the surrogate must map (c, phi, T) -> (cL, cS) with 64 hidden");
+      return -1;
+   }
+   long long nbl = (n + 255) / 256;
+   if (nbl > (1LL << 30)) {
+      set_error("kks_batch: n too large");
+      return -1;
+   }
+   double* part = nullptr;
+   long long* acc = nullptr;
+   AMPE_HIP_CHECK(hipMalloc((void**)&part, sizeof(double) * kKksPart * nbl));
+   if (hipMalloc((void**)&acc, sizeof(long long) * 4) != hipSuccess) {
+      (void)hipFree(part);
+      set_error("kks_batch: out of device memory");
+      return -1;
+   }
+   MlpDev m{};
+   if (guess) {
+      m = guess->dev();
+      hipLaunchKernelGGL(k_kks_batch<true>, dim3((unsigned)nbl), dim3(256), 0, s, n, p, tab, m,
+                         probe_its, probe_tol, c, phi, T, cl, cs, iters, part);
+   } else {
+      hipLaunchKernelGGL(k_kks_batch<false>, dim3((unsigned)nbl), dim3(256), 0, s, n, p, tab, m,
+                         -1, 0.0, c, phi, T, cl, cs, iters, part);
+   }
+   hipError_t e = hipGetLastError();
+   if (e == hipSuccess) {
+      hipLaunchKernelGGL(k_kks_batch_finish, dim3(1), dim3(64), 0, s, part, (int)nbl, acc);
+      e = hipGetLastError();
+   }
+   long long h[4] = {0, 0, 0, 0};
+   if (e == hipSuccess) e = hipMemcpyAsync(h, acc, sizeof(h), hipMemcpyDeviceToHost, s);
+   if (e == hipSuccess) e = hipStreamSynchronize(s);
+   (void)hipFree(part);
+   (void)hipFree(acc);
+   if (e != hipSuccess) {
+      set_error(std::string("kks_batch: ") + hipGetErrorString(e));
+      return -1;
+   }
+   if (stats) {
+      stats->iters = h[0];
+      stats->max_iters = h[1];
+      stats->fallbacks = h[2];
+      stats->failures = h[3];
+   }
+   return 0;
+}
+
+}  // namespace ampe

@@ -61,6 +61,7 @@ struct Level {
    double* dred = nullptr;     // reduction results (16 doubles)
    double* hpinned = nullptr;  // pinned host scalars
    int* iscratch = nullptr;    // device integer flags
+   double* kks_part = nullptr; // KKS per-workgroup partials (4 per 256 ghosted cells)
 };
 
 int level_create(int ndim, const ampe_box* boxes, int npatch, int ghost,

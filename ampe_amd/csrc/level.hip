@@ -161,6 +161,8 @@ int level_create(int ndim, const ampe_box* boxes, int npatch, int ghost,
    if (e == hipSuccess) e = hipHostMalloc(&L->hpinned, sizeof(double) * 64);
    if (e == hipSuccess) e = hipMalloc(&L->iscratch, sizeof(int) * 64);
    if (e == hipSuccess)
+      e = hipMalloc(&L->kks_part, sizeof(double) * 4 * (size_t)((L->ncg + 255) / 256 + 1));
+   if (e == hipSuccess)
       e = hipMemcpy(L->dp, L->hp.data(), sizeof(PatchDev) * npatch, hipMemcpyHostToDevice);
    if (e == hipSuccess)
       e = hipMemcpy(L->cell_tiles, ct.data(), sizeof(Tile) * ct.size(), hipMemcpyHostToDevice);
@@ -192,6 +194,7 @@ void level_destroy(Level* L)
    (void)hipFree(L->dred);
    (void)hipHostFree(L->hpinned);
    (void)hipFree(L->iscratch);
+   (void)hipFree(L->kks_part);
    delete L;
 }
 

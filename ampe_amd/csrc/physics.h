@@ -138,26 +138,19 @@ AMPE_HD double quad_ceq(const ampe_model_params& p, int ph)
    return p.quad_ceq[ph] + p.quad_m[ph] * (p.temperature - p.quad_Tref);
 }
 
-AMPE_HD double fenergy(const ampe_model_params& p, double c, int ph)
+// Binary CALPHAD phase at temperature T: G, dG/dc and d2G/dc2 [J/mol].
+AMPE_HD double calphad_g(const ampe_calphad_phase& cp, double T, double c)
 {
-   if (p.fe_model == AMPE_FE_QUADRATIC) {
-      double d = c - quad_ceq(p, ph);
-      return p.quad_A[ph] * d * d;
-   }
-   const ampe_calphad_phase& cp = p.calphad[ph];
-   double RT = kRGas * p.temperature;
+   double RT = kRGas * T;
    double x = 2.0 * c - 1.0;
    double P = cp.L[0] + x * (cp.L[1] + x * (cp.L[2] + x * cp.L[3]));
    return c * cp.gA + (1.0 - c) * cp.gB +
           RT * (c * log(c) + (1.0 - c) * log(1.0 - c)) + c * (1.0 - c) * P;
 }
 
-AMPE_HD double dfenergy(const ampe_model_params& p, double c, int ph)
+AMPE_HD double calphad_mu(const ampe_calphad_phase& cp, double T, double c)
 {
-   if (p.fe_model == AMPE_FE_QUADRATIC)
-      return 2.0 * p.quad_A[ph] * (c - quad_ceq(p, ph));
-   const ampe_calphad_phase& cp = p.calphad[ph];
-   double RT = kRGas * p.temperature;
+   double RT = kRGas * T;
    double x = 2.0 * c - 1.0;
    double P = cp.L[0] + x * (cp.L[1] + x * (cp.L[2] + x * cp.L[3]));
    double dP = cp.L[1] + x * (2.0 * cp.L[2] + x * 3.0 * cp.L[3]);
@@ -165,11 +158,9 @@ AMPE_HD double dfenergy(const ampe_model_params& p, double c, int ph)
           2.0 * c * (1.0 - c) * dP;
 }
 
-AMPE_HD double d2fenergy(const ampe_model_params& p, double c, int ph)
+AMPE_HD double calphad_d2(const ampe_calphad_phase& cp, double T, double c)
 {
-   if (p.fe_model == AMPE_FE_QUADRATIC) return 2.0 * p.quad_A[ph];
-   const ampe_calphad_phase& cp = p.calphad[ph];
-   double RT = kRGas * p.temperature;
+   double RT = kRGas * T;
    double x = 2.0 * c - 1.0;
    double P = cp.L[0] + x * (cp.L[1] + x * (cp.L[2] + x * cp.L[3]));
    double dP = cp.L[1] + x * (2.0 * cp.L[2] + x * 3.0 * cp.L[3]);
@@ -178,40 +169,88 @@ AMPE_HD double d2fenergy(const ampe_model_params& p, double c, int ph)
           4.0 * c * (1.0 - c) * d2P;
 }
 
-// Per-cell KKS: mu_L(cL) = mu_S(cS), (1-h) cL + h cS = c.  Quadratic wells:
-// closed form.  CALPHAD: Newton on [(mu_L - mu_S)/RT, mass balance] from the
-// guess in (cl, cs), step halving to stay in (0,1).  Returns the iteration
-// count or -1.
-AMPE_HD int kks_cell(const ampe_model_params& p, double c, double phi,
-                     double& cl, double& cs)
+AMPE_HD double fenergy(const ampe_model_params& p, double c, int ph)
 {
-   double h = interp(phi, p.conc_interp);
    if (p.fe_model == AMPE_FE_QUADRATIC) {
-      double eL = quad_ceq(p, 0), eS = quad_ceq(p, 1);
-      double r = p.quad_A[0] / p.quad_A[1];
-      double xl = (c - h * (eS - r * eL)) / (1.0 - h + h * r);
-      cl = xl;
-      cs = eS + r * (xl - eL);
-      return 0;
+      double d = c - quad_ceq(p, ph);
+      return p.quad_A[ph] * d * d;
    }
-   double RTinv = 1.0 / (kRGas * p.temperature);
+   return calphad_g(p.calphad[ph], p.temperature, c);
+}
+
+AMPE_HD double dfenergy(const ampe_model_params& p, double c, int ph)
+{
+   if (p.fe_model == AMPE_FE_QUADRATIC)
+      return 2.0 * p.quad_A[ph] * (c - quad_ceq(p, ph));
+   return calphad_mu(p.calphad[ph], p.temperature, c);
+}
+
+AMPE_HD double d2fenergy(const ampe_model_params& p, double c, int ph)
+{
+   if (p.fe_model == AMPE_FE_QUADRATIC) return 2.0 * p.quad_A[ph];
+   return calphad_d2(p.calphad[ph], p.temperature, c);
+}
+
+// End-member G(T) of a tdep table: interval k is the last with T >= Tc[k]
+// (k <= nrange - 1); terms added in database order.
+AMPE_HD double calphad_gfun(const ampe_calphad_gfun& f, double T)
+{
+   int k = 0;
+   while (k < f.nrange - 1 && T >= f.Tc[k + 1]) k++;
+   const double* a = f.coef[k];
+   double T2 = T * T, T3 = T2 * T;
+   double T7 = T3 * T3 * T;
+   double Tm9 = 1.0 / (T3 * T3 * T3);
+   double g = a[0] + a[1] * T + a[2] * T * log(T);
+   g = g + a[3] * T2;
+   g = g + a[4] * T3;
+   g = g + a[5] * T7;
+   g = g + a[6] / T;
+   g = g + a[7] * Tm9;
+   return g;
+}
+
+AMPE_HD void calphad_at(const ampe_calphad_tdep& tab, double T, ampe_calphad_phase* out)
+{
+   for (int ph = 0; ph < 2; ph++) {
+      out[ph].gA = calphad_gfun(tab.g[ph][0], T);
+      out[ph].gB = calphad_gfun(tab.g[ph][1], T);
+      for (int i = 0; i < 4; i++) out[ph].L[i] = tab.L[ph][i][0] + tab.L[ph][i][1] * T;
+   }
+}
+
+// Binary CALPHAD KKS Newton: F = [(mu_L(xl) - mu_S(xs))/RT, (1-h) xl + h xs - c]
+// from (xl, xs) clamped into [1e-12, 1-1e-12], step halving to stay inside
+// (0,1), converged when both |F_i| < tol.  Returns the number of updates, -1
+// after maxits updates without convergence, or -2 when probe_its >= 0 and
+// max|F| > probe_tol after probe_its updates (the caller restarts).
+AMPE_HD int kks_newton(const ampe_calphad_phase* ph, double T, double h, double c,
+                       double& cl, double& cs, int maxits, double tol, int probe_its,
+                       double probe_tol)
+{
+   double RTinv = 1.0 / (kRGas * T);
    double xl = cl, xs = cs;
    const double lo = 1.0e-12, hi = 1.0 - 1.0e-12;
    if (!(xl > lo)) xl = lo;
    if (!(xl < hi)) xl = hi;
    if (!(xs > lo)) xs = lo;
    if (!(xs < hi)) xs = hi;
-   for (int it = 0; it <= p.newton_max_its; it++) {
-      double f1 = (dfenergy(p, xl, 0) - dfenergy(p, xs, 1)) * RTinv;
+   for (int it = 0; it <= maxits; it++) {
+      double f1 = (calphad_mu(ph[0], T, xl) - calphad_mu(ph[1], T, xs)) * RTinv;
       double f2 = (1.0 - h) * xl + h * xs - c;
-      if (fabs(f1) < p.newton_tol && fabs(f2) < p.newton_tol) {
+      if (fabs(f1) < tol && fabs(f2) < tol) {
          cl = xl;
          cs = xs;
          return it;
       }
-      if (it == p.newton_max_its) break;
-      double a = d2fenergy(p, xl, 0) * RTinv;
-      double b = -d2fenergy(p, xs, 1) * RTinv;
+      if (it == maxits) break;
+      if (it == probe_its && !(dmax(fabs(f1), fabs(f2)) <= probe_tol)) {
+         cl = xl;
+         cs = xs;
+         return -2;
+      }
+      double a = calphad_d2(ph[0], T, xl) * RTinv;
+      double b = -calphad_d2(ph[1], T, xs) * RTinv;
       double cc = 1.0 - h, d = h;
       double det = a * d - b * cc;
       double dl = -(d * f1 - b * f2) / det;
@@ -228,6 +267,32 @@ AMPE_HD int kks_cell(const ampe_model_params& p, double c, double phi,
    cl = xl;
    cs = xs;
    return -1;
+}
+
+// Quadratic wells: closed form of mu_L = mu_S with mass balance.
+AMPE_HD void kks_quadratic(const ampe_model_params& p, double h, double c, double& cl,
+                           double& cs)
+{
+   double eL = quad_ceq(p, 0), eS = quad_ceq(p, 1);
+   double r = p.quad_A[0] / p.quad_A[1];
+   double xl = (c - h * (eS - r * eL)) / (1.0 - h + h * r);
+   cl = xl;
+   cs = eS + r * (xl - eL);
+}
+
+// Per-cell KKS at the model temperature: mu_L(cL) = mu_S(cS),
+// (1-h) cL + h cS = c.  Quadratic wells: closed form.  CALPHAD: kks_newton
+// from the guess in (cl, cs).  Returns the iteration count or -1.
+AMPE_HD int kks_cell(const ampe_model_params& p, double c, double phi, double& cl,
+                     double& cs)
+{
+   double h = interp(phi, p.conc_interp);
+   if (p.fe_model == AMPE_FE_QUADRATIC) {
+      kks_quadratic(p, h, c, cl, cs);
+      return 0;
+   }
+   return kks_newton(p.calphad, p.temperature, h, c, cl, cs, p.newton_max_its, p.newton_tol,
+                     -1, 0.0);
 }
 
 }  // namespace ampe

@@ -397,91 +397,6 @@ __global__ void __launch_bounds__(256) k_conc_rhs(LevelArgs a, ampe_model_params
    rhs[c0] = p.conc_mobility * s;
 }
 
-// ------------------------------------------------------------ KKS
-// Per cell over the ghost box (PhaseConcentrationsStrategy::
-// computePhaseConcentrations, source/PhaseConcentrationsStrategy.cc:26-127).
-// Each workgroup writes (sum of Newton iterations, max iterations or the
-// failure marker) to its partial slot; k_kks_finish folds them into the
-// device counters in a fixed order, so no same-address atomics are issued.
-constexpr double kKksFail = 1.0e9;
-
-__global__ void __launch_bounds__(256) k_kks(long long n, ampe_model_params p,
-                                             const double* __restrict__ phi,
-                                             const double* __restrict__ c,
-                                             const double* __restrict__ clr,
-                                             const double* __restrict__ csr,
-                                             double* __restrict__ cl, double* __restrict__ cs,
-                                             double* __restrict__ partial)
-{
-   long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-   double it = 0.0, mx = 0.0;
-   if (t < n) {
-      double xl = clr[t], xs = csr[t];
-      int r = kks_cell(p, c[t], phi[t], xl, xs);
-      cl[t] = xl;
-      cs[t] = xs;
-      if (r < 0) {
-         mx = kKksFail;
-      } else {
-         it = (double)r;
-         mx = (double)r;
-      }
-   }
-   for (int o = 32; o > 0; o >>= 1) {
-      it += __shfl_xor(it, o);
-      double v = __shfl_xor(mx, o);
-      mx = v > mx ? v : mx;
-   }
-   __shared__ double sh[8];
-   int w = threadIdx.x >> 6;
-   if ((threadIdx.x & 63) == 0) {
-      sh[2 * w] = it;
-      sh[2 * w + 1] = mx;
-   }
-   __syncthreads();
-   if (threadIdx.x == 0) {
-      double s = 0.0, m = 0.0;
-      for (int k = 0; k < 4; k++) {
-         s += sh[2 * k];
-         m = sh[2 * k + 1] > m ? sh[2 * k + 1] : m;
-      }
-      partial[2 * blockIdx.x] = s;
-      partial[2 * blockIdx.x + 1] = m;
-   }
-}
-
-// acc[0] += total iterations, acc[1] = max(acc[1], max iterations),
-// acc[2] |= failure, acc[3] += 1 (calls).  One workgroup.
-__global__ void __launch_bounds__(1024) k_kks_finish(const double* __restrict__ partial, int nb,
-                                                     long long* __restrict__ acc)
-{
-   __shared__ double ss[1024], sm[1024];
-   double s = 0.0, m = 0.0;
-   for (int i = threadIdx.x; i < nb; i += 1024) {
-      s += partial[2 * i];
-      m = partial[2 * i + 1] > m ? partial[2 * i + 1] : m;
-   }
-   ss[threadIdx.x] = s;
-   sm[threadIdx.x] = m;
-   __syncthreads();
-   for (int w = 512; w > 0; w >>= 1) {
-      if (threadIdx.x < w) {
-         ss[threadIdx.x] += ss[threadIdx.x + w];
-         sm[threadIdx.x] = sm[threadIdx.x + w] > sm[threadIdx.x] ? sm[threadIdx.x + w]
-                                                                : sm[threadIdx.x];
-      }
-      __syncthreads();
-   }
-   if (threadIdx.x == 0) {
-      bool fail = sm[0] >= kKksFail;
-      acc[0] += (long long)ss[0];
-      long long mx = fail ? 0 : (long long)sm[0];
-      if (mx > acc[1]) acc[1] = mx;
-      if (fail) acc[2] = 1;
-      acc[3] += 1;
-   }
-}
-
 // ------------------------------------------------------------ tagging
 // tagfromgrads (2d/quattag.m4:9-55) on the cell gradient of phi
 template <int NDIM>
@@ -562,39 +477,6 @@ int launch_conc_rhs(const Level& L, const ampe_model_params& p, const double* c,
                     hipStream_t s)
 {
    AMPE_DISPATCH(L, k_conc_rhs, L.ncell_tiles, level_args(L), p, c, phi, Dc, Dphi, rhs);
-   return 0;
-}
-
-int launch_kks(const Level& L, const ampe_model_params& p, const double* phi,
-               const double* c, const double* clr, const double* csr, double* cl,
-               double* cs, long long* dev_acc, hipStream_t s)
-{
-   unsigned nb = (unsigned)((L.ncg + 255) / 256);
-   if ((long long)nb * 2 > (long long)std::max(L.ncell_tiles, L.nface_tiles) * 8) {
-      set_error("launch_kks: partial buffer too small");
-      return -1;
-   }
-   hipLaunchKernelGGL(k_kks, dim3(nb), dim3(256), 0, s, L.ncg, p, phi, c, clr, csr, cl, cs,
-                      L.partial);
-   AMPE_HIP_CHECK(hipGetLastError());
-   hipLaunchKernelGGL(k_kks_finish, dim3(1), dim3(1024), 0, s, L.partial, (int)nb, dev_acc);
-   AMPE_HIP_CHECK(hipGetLastError());
-   return 0;
-}
-
-int launch_kks_sync(const Level& L, const ampe_model_params& p, const double* phi,
-                    const double* c, const double* clr, const double* csr, double* cl,
-                    double* cs, int* max_iters, long long* total_iters, hipStream_t s)
-{
-   long long* acc = reinterpret_cast<long long*>(L.iscratch);
-   AMPE_HIP_CHECK(hipMemsetAsync(acc, 0, sizeof(long long) * 4, s));
-   if (launch_kks(L, p, phi, c, clr, csr, cl, cs, acc, s)) return -1;
-   AMPE_HIP_CHECK(hipMemcpyAsync(L.hpinned, acc, sizeof(long long) * 4, hipMemcpyDeviceToHost, s));
-   AMPE_HIP_CHECK(hipStreamSynchronize(s));
-   long long h[4];
-   memcpy(h, L.hpinned, sizeof(h));
-   if (max_iters) *max_iters = h[2] ? -1 : (int)h[1];
-   if (total_iters) *total_iters = h[0];
    return 0;
 }
 

@@ -5,6 +5,7 @@
 #include <vector>
 
 #include "fac.h"
+#include "kks.h"
 #include "krylov.h"
 
 namespace ampe {
@@ -36,7 +37,8 @@ struct Sim {
    float *f_Cphi = nullptr, *f_sqrtm = nullptr, *f_Dconc = nullptr, *f_fc = nullptr;
    hipStream_t stream = nullptr;
    hipEvent_t ev0 = nullptr, ev1 = nullptr;
-   long long* kks_acc = nullptr;  // device KKS counters {iters, max, fail, calls}
+   long long* kks_acc = nullptr;  // device KKS counters (launch_kks)
+   KksGuess guess;                // surrogate guess when kks_guess is surrogate
    float last_ms = 0.f;
    ampe_stats st{};
 };

@@ -92,8 +92,8 @@ int sim_create(const ampe_model_params& p, const ampe_solver_params& s, const am
       rc |= falloc(&S->f_Dconc, S->nf);
       rc |= falloc(&S->f_fc, S->nf);
    }
-   if (!rc && hipMalloc(&S->kks_acc, sizeof(long long) * 4) != hipSuccess) rc = -1;
-   if (!rc && hipMemset(S->kks_acc, 0, sizeof(long long) * 4) != hipSuccess) rc = -1;
+   if (!rc && hipMalloc(&S->kks_acc, sizeof(long long) * kKksAcc) != hipSuccess) rc = -1;
+   if (!rc && hipMemset(S->kks_acc, 0, sizeof(long long) * kKksAcc) != hipSuccess) rc = -1;
    if (rc || hipStreamCreate(&S->stream) != hipSuccess ||
        hipEventCreate(&S->ev0) != hipSuccess || hipEventCreate(&S->ev1) != hipSuccess) {
       if (!rc) set_error("sim_create: stream/event creation failed");
@@ -161,20 +161,29 @@ static int gather_interior(Sim* S, const double* dev, double* host)
 static int kks(Sim* S, const double* y)
 {
    const double* c = y + (long long)S->ic * S->ng;
-   return launch_kks(*S->L, S->p, y, c, S->clr, S->csr, S->cl, S->cs, S->kks_acc, S->stream);
+   KksGuess g;
+   if (S->s.kks_guess == AMPE_GUESS_SURROGATE) {
+      if (!S->guess.surrogate) {
+         set_error("kks_guess is the surrogate but no surrogate was set");
+         return -1;
+      }
+      g = S->guess;
+   }
+   return launch_kks(*S->L, S->p, y, c, S->clr, S->csr, S->cl, S->cs, g, S->kks_acc, S->stream);
 }
 
 // Read the KKS device counters into the stats; -1 if a cell failed.
 static int kks_collect(Sim* S)
 {
    if (S->ic < 0) return 0;
-   AMPE_HIP_CHECK(hipMemcpyAsync(S->L->hpinned, S->kks_acc, sizeof(long long) * 4,
+   AMPE_HIP_CHECK(hipMemcpyAsync(S->L->hpinned, S->kks_acc, sizeof(long long) * kKksAcc,
                                  hipMemcpyDeviceToHost, S->stream));
    AMPE_HIP_CHECK(hipStreamSynchronize(S->stream));
-   long long h[4];
+   long long h[kKksAcc];
    std::memcpy(h, S->L->hpinned, sizeof(h));
    S->st.kks_iters = h[0];
    S->st.kks_calls = h[3];
+   S->st.kks_fallbacks = h[4];
    S->st.kks_cells = S->st.kks_calls * S->L->ncg;
    if (h[2]) {
       set_error("KKS Newton did not converge in a cell");
@@ -214,11 +223,11 @@ int sim_set_state(Sim* S, const double* phi, const double* c, const double* q)
                                     hipMemcpyDeviceToDevice, st));
       AMPE_HIP_CHECK(hipMemcpyAsync(S->csr, cc, sizeof(double) * S->ng,
                                     hipMemcpyDeviceToDevice, st));
-      AMPE_HIP_CHECK(hipMemsetAsync(S->kks_acc, 0, sizeof(long long) * 4, st));
+      AMPE_HIP_CHECK(hipMemsetAsync(S->kks_acc, 0, sizeof(long long) * kKksAcc, st));
       if (kks(S, S->y)) return -1;
       if (kks_collect(S)) return -1;
       std::memset(&S->st, 0, sizeof(S->st));
-      AMPE_HIP_CHECK(hipMemsetAsync(S->kks_acc, 0, sizeof(long long) * 4, st));
+      AMPE_HIP_CHECK(hipMemsetAsync(S->kks_acc, 0, sizeof(long long) * kKksAcc, st));
    }
    S->t = 0.0;
    S->nst = 0;

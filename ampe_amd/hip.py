@@ -9,7 +9,8 @@ import os
 
 import numpy as np
 
-from ampe_amd.params import Box, ModelParams, SolverParams, Stats
+from ampe_amd.params import (Box, CalphadPhase, CalphadTdep, KksBatchStats, ModelParams,
+                             SolverParams, Stats, TrainParams)
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libampe_hip.so")
@@ -55,6 +56,20 @@ _SIGS = {
     "ampe_sim_get_state": (_i, [_vp, _vp, _vp, _vp, _vp, _vp]),
     "ampe_sim_advance": (_i, [_vp, _i, C.POINTER(Stats)]),
     "ampe_sim_last_time_ms": (_i, [_vp, C.POINTER(_d)]),
+    "ampe_sim_set_surrogate": (_i, [_vp, _vp, _i, _d]),
+    "ampe_calphad_at": (_i, [C.POINTER(CalphadTdep), _d, C.POINTER(CalphadPhase)]),
+    "ampe_surrogate_create": (_i, [_i, _i, _i, C.c_ulonglong, C.POINTER(_vp)]),
+    "ampe_surrogate_destroy": (_i, [_vp]),
+    "ampe_surrogate_nparams": (_i, [_vp, C.POINTER(_ll)]),
+    "ampe_surrogate_set_params": (_i, [_vp, _vp]),
+    "ampe_surrogate_get_params": (_i, [_vp, _vp]),
+    "ampe_surrogate_set_scaling": (_i, [_vp, C.POINTER(_d), C.POINTER(_d), C.POINTER(_d),
+                                        C.POINTER(_d)]),
+    "ampe_surrogate_forward": (_i, [_vp, _ll, _vp, _vp, _vp]),
+    "ampe_surrogate_loss_grad": (_i, [_vp, _ll, _vp, _vp, _vp, C.POINTER(_d), _vp]),
+    "ampe_surrogate_train": (_i, [_vp, _ll, _vp, _vp, C.POINTER(TrainParams), _vp, _vp]),
+    "ampe_kks_batch": (_i, [C.POINTER(ModelParams), C.POINTER(CalphadTdep), _ll, _vp, _vp, _vp,
+                            _vp, _i, _d, _vp, _vp, _vp, C.POINTER(KksBatchStats), _vp]),
 }
 
 EXPORTED = tuple(_SIGS)
@@ -181,3 +196,91 @@ class Sim:
         ms = _d()
         check(lib().ampe_sim_last_time_ms(self.h, C.byref(ms)), "ampe_sim_last_time_ms")
         return ms.value
+
+    def set_surrogate(self, surrogate, probe_its=3, probe_tol=1.0e-8):
+        h = surrogate.h if surrogate is not None else None
+        check(lib().ampe_sim_set_surrogate(self.h, h, probe_its, probe_tol),
+              "ampe_sim_set_surrogate")
+        self._surrogate = surrogate  # keep it alive as long as the sim
+
+
+def _dev_ptr(t):
+    return None if t is None else _vp(t.data_ptr())
+
+
+def _stream_ptr(stream):
+    return None if stream is None else _vp(stream.cuda_stream)
+
+
+class Surrogate:
+    """KKS surrogate MLP (ampe_surrogate): n_in -> 64 -> 64 -> n_out, tanh,
+    BF16 matrix-core kernels with FP32 accumulation.  Device arrays are torch
+    tensors (float32 for normalised data, float64 for raw training data)."""
+
+    def __init__(self, n_in=3, n_hidden=64, n_out=2, seed=0):
+        h = _vp()
+        check(lib().ampe_surrogate_create(n_in, n_hidden, n_out, seed, C.byref(h)),
+              "ampe_surrogate_create")
+        self.h = h
+        self.n_in, self.n_hidden, self.n_out = n_in, n_hidden, n_out
+        n = _ll()
+        check(lib().ampe_surrogate_nparams(h, C.byref(n)), "ampe_surrogate_nparams")
+        self.nparams = n.value
+
+    def __del__(self):
+        if getattr(self, "h", None) is not None and _lib is not None:
+            _lib.ampe_surrogate_destroy(self.h)
+            self.h = None
+
+    def get_params(self):
+        w = np.zeros(self.nparams, dtype=np.float32)
+        check(lib().ampe_surrogate_get_params(self.h, w.ctypes.data_as(_vp)),
+              "ampe_surrogate_get_params")
+        return w
+
+    def set_params(self, w):
+        w = np.ascontiguousarray(w, dtype=np.float32)
+        assert w.size == self.nparams
+        check(lib().ampe_surrogate_set_params(self.h, w.ctypes.data_as(_vp)),
+              "ampe_surrogate_set_params")
+
+    def set_scaling(self, in_shift, in_scale, out_shift, out_scale):
+        arr = lambda v: (_d * 4)(*([float(x) for x in v] + [0.0] * (4 - len(v))))  # noqa: E731
+        self.scaling = (list(in_shift), list(in_scale), list(out_shift), list(out_scale))
+        check(lib().ampe_surrogate_set_scaling(self.h, arr(in_shift), arr(in_scale),
+                                               arr(out_shift), arr(out_scale)),
+              "ampe_surrogate_set_scaling")
+
+    def forward(self, x, y, stream=None):
+        n = x.shape[-1]
+        check(lib().ampe_surrogate_forward(self.h, n, _dev_ptr(x), _dev_ptr(y),
+                                           _stream_ptr(stream)), "ampe_surrogate_forward")
+
+    def loss_grad(self, x, t, grad, stream=None):
+        loss = _d()
+        check(lib().ampe_surrogate_loss_grad(self.h, x.shape[-1], _dev_ptr(x), _dev_ptr(t),
+                                             _dev_ptr(grad), C.byref(loss),
+                                             _stream_ptr(stream)), "ampe_surrogate_loss_grad")
+        return loss.value
+
+    def train(self, x, t, batch, steps, lr0=3e-3, lr1=3e-5, beta1=0.9, beta2=0.999,
+              eps=1e-8, stream=None):
+        tp = TrainParams(batch=batch, steps=steps, lr0=lr0, lr1=lr1, beta1=beta1,
+                         beta2=beta2, eps=eps)
+        hist = np.zeros(steps, dtype=np.float32)
+        check(lib().ampe_surrogate_train(self.h, x.shape[-1], _dev_ptr(x), _dev_ptr(t),
+                                         C.byref(tp), hist.ctypes.data_as(_vp),
+                                         _stream_ptr(stream)), "ampe_surrogate_train")
+        return hist
+
+
+def kks_batch(p, tab, c, phi, T, cl, cs, iters=None, guess=None, probe_its=3,
+              probe_tol=1.0e-8, stream=None):
+    """Binary CALPHAD KKS over independent cells (ampe_kks_batch); c, phi, T,
+    cl, cs: float64 device tensors, iters: int32 device tensor or None."""
+    st = KksBatchStats()
+    check(lib().ampe_kks_batch(C.byref(p), C.byref(tab), c.numel(), _dev_ptr(c), _dev_ptr(phi),
+                               _dev_ptr(T), guess.h if guess is not None else None, probe_its,
+                               probe_tol, _dev_ptr(cl), _dev_ptr(cs), _dev_ptr(iters),
+                               C.byref(st), _stream_ptr(stream)), "ampe_kks_batch")
+    return st.as_dict()

@@ -18,6 +18,27 @@ class CalphadPhase(C.Structure):
     _fields_ = [("gA", C.c_double), ("gB", C.c_double), ("L", C.c_double * 4)]
 
 
+class CalphadGfun(C.Structure):
+    _fields_ = [("nrange", C.c_int), ("Tc", C.c_double * 5), ("coef", (C.c_double * 8) * 4)]
+
+
+class CalphadTdep(C.Structure):
+    _fields_ = [("g", (CalphadGfun * 2) * 2), ("L", ((C.c_double * 2) * 4) * 2)]
+
+
+class TrainParams(C.Structure):
+    _fields_ = [("batch", C.c_longlong), ("steps", C.c_int), ("lr0", C.c_double),
+                ("lr1", C.c_double), ("beta1", C.c_double), ("beta2", C.c_double),
+                ("eps", C.c_double)]
+
+
+class KksBatchStats(C.Structure):
+    _fields_ = [(n, C.c_longlong) for n in ("iters", "max_iters", "fallbacks", "failures")]
+
+    def as_dict(self):
+        return {n: getattr(self, n) for n, _ in self._fields_}
+
+
 class ModelParams(C.Structure):
     _fields_ = [
         ("ndim", C.c_int),
@@ -150,6 +171,40 @@ def calphad_g(species, phase, T):
     return float(g)
 
 
+_GFUN_KEYS = ("a", "b", "c", "d2", "d3", "d7", "dm1", "dm9")
+
+
+def auni_tdep(rk=None):
+    """Temperature-dependent AuNi table (CalphadTdep): end members of
+    calphadAuNi.dat:7-45 per [phase L/S][species A/B] and Redlich-Kister
+    L_i(T) = L_i0 + L_i1 T (calphadAuNi.dat:49-63), or the constant (L0, L1)
+    of rk per phase."""
+    tab = CalphadTdep()
+    for ip, ph in enumerate("LS"):
+        for isp, sp in enumerate("AB"):
+            d = _AUNI[(sp, ph)]
+            f = tab.g[ip][isp]
+            f.nrange = len(d["Tc"]) - 1
+            for k, tc in enumerate(d["Tc"]):
+                f.Tc[k] = tc
+            for k in range(f.nrange):
+                for j, key in enumerate(_GFUN_KEYS):
+                    f.coef[k][j] = d[key][k] if key in d else 0.0
+        Ls = list(_AUNI_LMIX[ph]) if rk is None else [(v, 0.0) for v in rk[ph]]
+        for i, (l0, l1) in enumerate(Ls[:4]):
+            tab.L[ip][i][0] = l0
+            tab.L[ip][i][1] = l1
+    return tab
+
+
+def seeded_rk(seed=1002):
+    """Redlich-Kister (L0, L1) per phase, L0 ~ U[-2e4, 2e4], L1 ~ U[-5e3, 5e3]
+    J/mol, drawn as in polycrystal_params."""
+    rng = np.random.Generator(np.random.Philox(seed))
+    return {ph: (float(rng.uniform(-2e4, 2e4)), float(rng.uniform(-5e3, 5e3)))
+            for ph in "LS"}
+
+
 def auni_phases(T, rk=None):
     """[liquid, solid] CalphadPhase at temperature T.  rk optionally overrides
     the Redlich-Kister (L0, L1) per phase: {'L': (L0, L1), 'S': (L0, L1)}."""
@@ -247,9 +302,7 @@ def polycrystal_params(ndim=3, seed=1002, **kw):
     L0 ~ U[-2e4, 2e4], L1 ~ U[-5e3, 5e3] J/mol per phase.  Orientation
     coefficients of tests/TwoGrainsQuadratic/2d.input.  dx = 0.02 um.
     No penalty term (the reference's penalty solver is not compiled)."""
-    rng = np.random.Generator(np.random.Philox(seed))
-    rk = {ph: (float(rng.uniform(-2e4, 2e4)), float(rng.uniform(-5e3, 5e3)))
-          for ph in "LS"}
+    rk = seeded_rk(seed)
     sigma, delta = 0.372678, 0.035
     p = two_grain_params(ndim=ndim)
     for d in range(3):

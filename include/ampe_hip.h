@@ -283,6 +283,102 @@ int ampe_vec_dot(const ampe_level* level, int ncomp, const double* x,
 int ampe_vec_wrms(const ampe_level* level, int ncomp, const double* x,
                   const double* w, double* out, ampe_stream_t stream);
 
+/* ------------------------------------------- temperature-dependent CALPHAD */
+/* One end-member Gibbs energy on nrange temperature intervals
+ * [Tc[k], Tc[k+1]):  G(T) = a + b T + c T ln T + d2 T^2 + d3 T^3 + d7 T^7
+ *                          + dm1 / T + dm9 T^-9                  [J/mol]
+ * (the thermodynamic_data/calphadAuNi.dat:5-45 database form). */
+typedef struct ampe_calphad_gfun {
+   int nrange;
+   double Tc[5];
+   double coef[4][8]; /* per interval: a, b, c, d2, d3, d7, dm1, dm9 */
+} ampe_calphad_gfun;
+
+/* Binary CALPHAD phases with temperature dependence: end members g[phase]
+ * [species] (phase 0 = liquid, 1 = solid A; species 0 = A, 1 = B) and
+ * Redlich-Kister L_i(T) = L[phase][i][0] + L[phase][i][1] T
+ * (calphadAuNi.dat:47-63). */
+typedef struct ampe_calphad_tdep {
+   ampe_calphad_gfun g[2][2];
+   double L[2][4][2];
+} ampe_calphad_tdep;
+
+/* Evaluate a tdep table at temperature T into two ampe_calphad_phase records
+ * (host helper, same arithmetic as the device path). */
+int ampe_calphad_at(const ampe_calphad_tdep* tab, double T,
+                    ampe_calphad_phase out[2]);
+
+/* ------------------------------------------------------ KKS surrogate MLP */
+/* No counterpart in the reference: a small MLP n_in -> H -> H -> n_out with
+ * tanh hidden layers, BF16 weights and activations on the gfx950 matrix cores
+ * with FP32 accumulation, that supplies the initial guess of the per-cell KKS
+ * Newton solve in place of the guess the reference takes from the previous
+ * step's phase concentrations
+ * (source/CALPHADequilibriumPhaseConcentrationsStrategy.cc:368-369).
+ * Parameters are FP32 masters, flattened as W1[H][n_in], b1[H], W2[H][H],
+ * b2[H], W3[n_out][H], b3[n_out].  Network inputs and outputs are normalised:
+ * x_n = (x - in_shift) * in_scale, y = y_n * out_scale + out_shift. */
+typedef struct ampe_surrogate ampe_surrogate;
+/* n_in <= 4, n_hidden in {64, 128}, n_out <= 4; Glorot-uniform weights from
+ * seed, zero biases, identity scaling. */
+int ampe_surrogate_create(int n_in, int n_hidden, int n_out,
+                          unsigned long long seed, ampe_surrogate** out);
+int ampe_surrogate_destroy(ampe_surrogate* s);
+int ampe_surrogate_nparams(const ampe_surrogate* s, long long* n);
+/* Host FP32 parameter vectors of length nparams. */
+int ampe_surrogate_set_params(ampe_surrogate* s, const float* params);
+int ampe_surrogate_get_params(const ampe_surrogate* s, float* params);
+int ampe_surrogate_set_scaling(ampe_surrogate* s, const double* in_shift,
+                               const double* in_scale, const double* out_shift,
+                               const double* out_scale);
+/* Forward pass on normalised inputs: x device float[n_in][n] (SoA), y device
+ * float[n_out][n]. */
+int ampe_surrogate_forward(const ampe_surrogate* s, long long n,
+                           const float* x, float* y, ampe_stream_t stream);
+/* Mean-squared-error loss L = (1/n) sum_samples sum_outputs (y - t)^2 on
+ * normalised data and its gradient (device float[nparams]); *loss on host. */
+int ampe_surrogate_loss_grad(ampe_surrogate* s, long long n, const float* x,
+                             const float* t, float* grad, double* loss,
+                             ampe_stream_t stream);
+
+typedef struct ampe_train_params {
+   long long batch;   /* samples per Adam step */
+   int steps;         /* Adam steps */
+   double lr0, lr1;   /* learning rate, decayed geometrically lr0 -> lr1 */
+   double beta1, beta2, eps;
+} ampe_train_params;
+/* Adam training on raw (unnormalised) FP64 device data: x double[n_in][n],
+ * t double[n_out][n]; the object's scaling normalises both.  Minibatches
+ * walk the data in order (draw it in random order).  loss_hist (host, may
+ * be NULL) receives the minibatch loss of every step. */
+int ampe_surrogate_train(ampe_surrogate* s, long long n, const double* x,
+                         const double* t, const ampe_train_params* tp,
+                         float* loss_hist, ampe_stream_t stream);
+
+/* Binary CALPHAD KKS over n independent cells with a per-cell temperature
+ * (T device double[n]), the per-cell loop of
+ * CALPHADequilibriumPhaseConcentrationsStrategy::
+ * computePhaseConcentrationsOnPatch (source/
+ * CALPHADequilibriumPhaseConcentrationsStrategy.cc:343-402) with the end
+ * members evaluated from tab at each cell's T.  Without a surrogate the
+ * Newton starts from (0.5, 0.5) (tests/testCALPHADbinaryKKS.cc:123-127).
+ * With one, it starts from the surrogate's (cL, cS) for inputs (c, phi, T);
+ * if after probe_its updates the residual exceeds probe_tol it restarts from
+ * (0.5, 0.5).  Convergence: |residual| < p->newton_tol, at most
+ * p->newton_max_its updates per start.  iters (device int[n], may be NULL):
+ * updates per cell (both starts), -1 if not converged. */
+typedef struct ampe_kks_batch_stats {
+   long long iters;     /* Newton updates over all cells */
+   long long max_iters; /* largest per-cell count */
+   long long fallbacks; /* cells restarted from the default guess */
+   long long failures;  /* cells not converged */
+} ampe_kks_batch_stats;
+int ampe_kks_batch(const ampe_model_params* p, const ampe_calphad_tdep* tab,
+                   long long n, const double* c, const double* phi,
+                   const double* T, const ampe_surrogate* guess, int probe_its,
+                   double probe_tol, double* cl, double* cs, int* iters,
+                   ampe_kks_batch_stats* stats, ampe_stream_t stream);
+
 /* ------------------------------------------------------------- integrator */
 /* BDF (order <= 2, fixed step) Newton-Krylov integrator with SPGMR (MGS,
  * left block preconditioner, difference-quotient Jv) driving the kernels
@@ -304,6 +400,12 @@ int ampe_sim_get_state(ampe_sim* sim, double* phi, double* c, double* q,
 int ampe_sim_advance(ampe_sim* sim, int nsteps, ampe_stats* stats);
 /* Device time (ms) of the last ampe_sim_advance, from HIP events. */
 int ampe_sim_last_time_ms(const ampe_sim* sim, double* ms);
+/* Surrogate used when the solver's kks_guess is AMPE_GUESS_SURROGATE (the
+ * integrator keeps the pointer; the object must outlive the sim).  Inputs are
+ * (c, phi, T) per cell; after probe_its Newton updates a residual above
+ * probe_tol restarts the cell from the previous-step guess. */
+int ampe_sim_set_surrogate(ampe_sim* sim, const ampe_surrogate* s,
+                           int probe_its, double probe_tol);
 
 #ifdef __cplusplus
 }

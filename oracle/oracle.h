@@ -114,6 +114,22 @@ void o_sim_get_state(OSim* S, double* phi, double* c, double* q, double* cl,
                      double* cs);
 int o_sim_advance(OSim* S, int nsteps, ampe_stats* st);
 
+
+/* surrogate path (oracle_surrogate.c) */
+float o_bf16(float f);
+double o_calphad_gfun(const ampe_calphad_gfun* f, double T);
+void o_calphad_at(const ampe_calphad_tdep* tab, double T, ampe_calphad_phase* out);
+int o_kks_newton(const ampe_calphad_phase* ph, double T, double h, double c, double* cl,
+                 double* cs, int maxits, double tol, int probe_its, double probe_tol);
+long o_kks_batch(const ampe_model_params* p, const ampe_calphad_tdep* tab, long n,
+                 const double* c, const double* phi, const double* T, const double* gl,
+                 const double* gs, int probe_its, double probe_tol, double* cl, double* cs,
+                 int* iters);
+void o_mlp_forward(int nin, int hid, int nout, const float* w, long n, const float* x,
+                   double* y, int emulate);
+double o_mlp_loss_grad(int nin, int hid, int nout, const float* w, long n, const float* x,
+                       const float* t, double* grad, int emulate);
+
 #ifdef __cplusplus
 }
 #endif

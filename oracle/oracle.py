@@ -9,7 +9,7 @@ import subprocess
 
 import numpy as np
 
-from ampe_amd.params import ModelParams, SolverParams, Stats
+from ampe_amd.params import CalphadGfun, CalphadPhase, CalphadTdep, ModelParams, SolverParams, Stats
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB = os.path.join(_HERE, "build", "liboracle.so")
@@ -47,6 +47,7 @@ class EllipticOp(C.Structure):
 
 _dp = C.POINTER(C.c_double)
 _ip = C.POINTER(C.c_int)
+_fp = C.POINTER(C.c_float)
 _G = C.POINTER(OGeom)
 _P = C.POINTER(ModelParams)
 
@@ -89,6 +90,16 @@ def _setup(L):
         "o_sim_set_state": (None, [C.c_void_p, _dp, _dp, _dp]),
         "o_sim_get_state": (None, [C.c_void_p, _dp, _dp, _dp, _dp, _dp]),
         "o_sim_advance": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(Stats)]),
+        "o_bf16": (C.c_float, [C.c_float]),
+        "o_calphad_gfun": (C.c_double, [C.POINTER(CalphadGfun), C.c_double]),
+        "o_calphad_at": (None, [C.POINTER(CalphadTdep), C.c_double, C.POINTER(CalphadPhase)]),
+        "o_kks_newton": (C.c_int, [C.POINTER(CalphadPhase), C.c_double, C.c_double, C.c_double,
+                                   _dp, _dp, C.c_int, C.c_double, C.c_int, C.c_double]),
+        "o_kks_batch": (C.c_long, [_P, C.POINTER(CalphadTdep), C.c_long, _dp, _dp, _dp, _dp, _dp,
+                                   C.c_int, C.c_double, _dp, _dp, _ip]),
+        "o_mlp_forward": (None, [C.c_int, C.c_int, C.c_int, _fp, C.c_long, _fp, _dp, C.c_int]),
+        "o_mlp_loss_grad": (C.c_double, [C.c_int, C.c_int, C.c_int, _fp, C.c_long, _fp, _fp,
+                                         _dp, C.c_int]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -185,3 +196,48 @@ class Sim:
         st = Stats()
         ret = lib().o_sim_advance(self.h, nsteps, C.byref(st))
         return ret, st.as_dict()
+
+
+def fptr(a):
+    assert a.dtype == np.float32 and a.flags.c_contiguous
+    return a.ctypes.data_as(_fp)
+
+
+def calphad_at(tab, T):
+    out = (CalphadPhase * 2)()
+    lib().o_calphad_at(C.byref(tab), T, out)
+    return out
+
+
+def kks_batch(p, tab, c, phi, T, gl=None, gs=None, probe_its=3, probe_tol=1.0e-8):
+    """Per-cell-temperature CALPHAD KKS from (0.5, 0.5) or from (gl, gs)
+    with the probe / restart rule; returns cl, cs, iters, failures."""
+    c, phi, T = (np.ascontiguousarray(a, dtype=np.float64) for a in (c, phi, T))
+    n = c.size
+    cl, cs = np.zeros(n), np.zeros(n)
+    it = np.zeros(n, dtype=np.int32)
+    if gl is not None:
+        gl = np.ascontiguousarray(gl, dtype=np.float64)
+        gs = np.ascontiguousarray(gs, dtype=np.float64)
+    nf = lib().o_kks_batch(C.byref(p), C.byref(tab), n, ptr(c), ptr(phi), ptr(T), ptr(gl),
+                           ptr(gs), probe_its, probe_tol, ptr(cl), ptr(cs), iptr(it))
+    return cl, cs, it, nf
+
+
+def mlp_forward(w, x, nin, hid, nout, emulate):
+    x = np.ascontiguousarray(x, dtype=np.float32)
+    n = x.shape[-1]
+    y = np.zeros((nout, n))
+    lib().o_mlp_forward(nin, hid, nout, fptr(np.ascontiguousarray(w, dtype=np.float32)), n,
+                        fptr(x), ptr(y), emulate)
+    return y
+
+
+def mlp_loss_grad(w, x, t, nin, hid, nout, emulate):
+    w = np.ascontiguousarray(w, dtype=np.float32)
+    x = np.ascontiguousarray(x, dtype=np.float32)
+    t = np.ascontiguousarray(t, dtype=np.float32)
+    g = np.zeros(w.size)
+    loss = lib().o_mlp_loss_grad(nin, hid, nout, fptr(w), x.shape[-1], fptr(x), fptr(t), ptr(g),
+                                 emulate)
+    return loss, g

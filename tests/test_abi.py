@@ -41,9 +41,11 @@ def test_struct_sizes_match_header():
 #include <stddef.h>
 #include "ampe_hip.h"
 int main(void){
- printf("%zu %zu %zu %zu %zu %zu %zu\n", sizeof(ampe_model_params), sizeof(ampe_solver_params),
-        sizeof(ampe_stats), sizeof(ampe_box), offsetof(ampe_model_params, calphad),
-        offsetof(ampe_model_params, newton_tol), sizeof(ampe_elliptic_op));
+ printf("%zu %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu\n", sizeof(ampe_model_params),
+        sizeof(ampe_solver_params), sizeof(ampe_stats), sizeof(ampe_box),
+        offsetof(ampe_model_params, calphad), offsetof(ampe_model_params, newton_tol),
+        sizeof(ampe_elliptic_op), sizeof(ampe_calphad_gfun), sizeof(ampe_calphad_tdep),
+        offsetof(ampe_calphad_tdep, L), sizeof(ampe_train_params), sizeof(ampe_kks_batch_stats));
  return 0;}
 '''
     with tempfile.TemporaryDirectory() as d:
@@ -55,5 +57,6 @@ int main(void){
     from ampe_amd.hip import EllipticOp
     got = [C.sizeof(P.ModelParams), C.sizeof(P.SolverParams), C.sizeof(P.Stats),
            C.sizeof(P.Box), P.ModelParams.calphad.offset, P.ModelParams.newton_tol.offset,
-           C.sizeof(EllipticOp)]
+           C.sizeof(EllipticOp), C.sizeof(P.CalphadGfun), C.sizeof(P.CalphadTdep),
+           P.CalphadTdep.L.offset, C.sizeof(P.TrainParams), C.sizeof(P.KksBatchStats)]
     assert [int(x) for x in out] == got
