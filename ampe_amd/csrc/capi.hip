@@ -6,6 +6,7 @@
 
 #include "../../include/ampe_hip.h"
 #include "kks.h"
+#include "mg.h"
 #include "sim.h"
 
 namespace ampe {
@@ -299,6 +300,34 @@ int ampe_sim_last_time_ms(const ampe_sim* sim, double* ms)
    AMPE_REQUIRE(sim && ms, "null argument");
    *ms = sim->S->last_ms;
    return 0;
+}
+
+int ampe_coarsen_cell(const ampe_level* coarse, const ampe_level* fine, int ncomp,
+                      const double* f, double* c, ampe_stream_t stream)
+{
+   AMPE_REQUIRE(coarse && fine && f && c && ncomp >= 1, "bad argument");
+   AMPE_REQUIRE(ampe::is_coarsening(*coarse->L, *fine->L),
+                "coarse level is not the fine level halved");
+   return ampe::mg_restrict_cell<double>(*coarse->L, *fine->L, ncomp, f, c, hs(stream));
+}
+
+int ampe_coarsen_side(const ampe_level* coarse, const ampe_level* fine, const double* f,
+                      double* c, ampe_stream_t stream)
+{
+   AMPE_REQUIRE(coarse && fine && f && c, "bad argument");
+   AMPE_REQUIRE(ampe::is_coarsening(*coarse->L, *fine->L),
+                "coarse level is not the fine level halved");
+   return ampe::mg_restrict_side<double>(*coarse->L, *fine->L, f, c, hs(stream));
+}
+
+int ampe_refine_add(const ampe_level* fine, const ampe_level* coarse, int ncomp, int order,
+                    const double* c, double* f, ampe_stream_t stream)
+{
+   AMPE_REQUIRE(coarse && fine && f && c && ncomp >= 1, "bad argument");
+   AMPE_REQUIRE(order == 0 || order == 1, "order must be 0 or 1");
+   AMPE_REQUIRE(ampe::is_coarsening(*coarse->L, *fine->L),
+                "coarse level is not the fine level halved");
+   return ampe::mg_prolong_add<double>(*fine->L, *coarse->L, ncomp, order, c, f, hs(stream));
 }
 
 int ampe_sim_set_surrogate(ampe_sim* sim, const ampe_surrogate* s, int probe_its,

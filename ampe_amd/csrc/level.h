@@ -43,6 +43,8 @@ struct Tile {
 
 struct Level {
    int ndim = 2, g = 1, gz = 0, npatch = 0;
+   bool periodic = false;       // periodic domain (pbox) or physical boundaries
+   ampe_box pbox = {{0, 0, 0}, {0, 0, 0}};
    long long ncg = 0;         // ghosted cells per component
    long long ncell = 0;       // interior cells
    long long nside[3] = {0, 0, 0};

@@ -45,6 +45,10 @@ int level_create(int ndim, const ampe_box* boxes, int npatch, int ghost,
    L->g = ghost;
    L->gz = (ndim == 3) ? ghost : 0;
    L->npatch = npatch;
+   if (periodic) {
+      L->periodic = true;
+      L->pbox = *periodic;
+   }
    L->hp.resize(npatch);
    long long coff = 0, ioff = 0, soff[3] = {0, 0, 0};
    for (int p = 0; p < npatch; p++) {

@@ -1,0 +1,428 @@
+// Geometric multigrid below one patch level (see mg.h).
+#include <algorithm>
+#include <cstring>
+
+#include "mg.h"
+
+namespace ampe {
+
+// ------------------------------------------------------------ transfers
+// Coarse cell = mean of its 2^NDIM children (fixed summation order).
+template <typename T, int NDIM>
+__global__ void __launch_bounds__(256) k_restrict_cell(LevelArgs ca, LevelArgs fa, int ncomp,
+                                                       const T* __restrict__ f,
+                                                       T* __restrict__ c)
+{
+   CellPos p;
+   if (!cell_pos(ca, 0, p)) return;
+   const PatchDev fp = fa.P[ca.T[blockIdx.x].patch];
+   int fk = NDIM == 3 ? 2 * p.k : 0;
+   long long f0 = fp.cell_off + (2 * p.i + fa.g) + (long long)(2 * p.j + fa.g) * fp.sy +
+                  (long long)(fk + fa.gz) * fp.sz;
+   const long long sy = fp.sy, sz = fp.sz;
+   for (int m = 0; m < ncomp; m++) {
+      const T* fm = f + m * fa.ncg;
+      T s = (fm[f0] + fm[f0 + 1]) + (fm[f0 + sy] + fm[f0 + sy + 1]);
+      if (NDIM == 3) s = s + ((fm[f0 + sz] + fm[f0 + sz + 1]) + (fm[f0 + sz + sy] + fm[f0 + sz + sy + 1]));
+      c[m * ca.ncg + p.c0] = s * (NDIM == 3 ? T(0.125) : T(0.25));
+   }
+}
+
+// Coarse face = mean of the fine faces covering it.
+template <typename T, int NDIM>
+__global__ void __launch_bounds__(256) k_restrict_side(LevelArgs ca, LevelArgs fa,
+                                                       const T* __restrict__ f,
+                                                       T* __restrict__ c)
+{
+   CellPos p;
+   if (!cell_pos(ca, 1, p)) return;
+   if (NDIM == 2 && p.k > 0) return;
+   const PatchDev fp = fa.P[ca.T[blockIdx.x].patch];
+   for (int d = 0; d < NDIM; d++) {
+      bool ok = (p.i < p.pd.nx + (d == 0)) && (p.j < p.pd.ny + (d == 1)) &&
+                (NDIM == 2 || p.k < p.pd.nz + (d == 2));
+      if (!ok) continue;
+      int b[3] = {2 * p.i, 2 * p.j, NDIM == 3 ? 2 * p.k : 0};
+      int t1 = (d == 0) ? 1 : 0;  // first transverse direction
+      int t2 = (d == 2) ? 1 : 2;  // second transverse direction (3-D)
+      T s = 0;
+      for (int a = 0; a < 2; a++)
+         for (int e = 0; e < (NDIM == 3 ? 2 : 1); e++) {
+            int q[3] = {b[0], b[1], b[2]};
+            q[t1] += a;
+            if (NDIM == 3) q[t2] += e;
+            T v = f[face_idx(fa, fp, d, q[0], q[1], q[2])];
+            s = (a == 0 && e == 0) ? v : s + v;
+         }
+      c[face_idx(ca, p.pd, d, p.i, p.j, p.k)] = s * (NDIM == 3 ? T(0.25) : T(0.5));
+   }
+}
+
+// Fine cell += coarse correction: ORDER 1 tri/bilinear interpolation (weights
+// 3/4 toward the parent, 1/4 toward the coarse neighbour on the cell's side in
+// each direction; the coarse field needs filled ghosts), ORDER 0 the parent
+// value (constant refinement).
+template <typename T, int NDIM, int ORDER>
+__global__ void __launch_bounds__(256) k_prolong_add(LevelArgs fa, LevelArgs ca, int ncomp,
+                                                     const T* __restrict__ xc,
+                                                     T* __restrict__ xf)
+{
+   CellPos p;
+   if (!cell_pos(fa, 0, p)) return;
+   const PatchDev cp = ca.P[fa.T[blockIdx.x].patch];
+   int K = NDIM == 3 ? (p.k >> 1) : 0;
+   long long c0 = cp.cell_off + ((p.i >> 1) + ca.g) + (long long)((p.j >> 1) + ca.g) * cp.sy +
+                  (long long)(K + ca.gz) * cp.sz;
+   long long oi = (p.i & 1) ? 1 : -1;
+   long long oj = (p.j & 1) ? cp.sy : -cp.sy;
+   long long ok = (NDIM == 3) ? ((p.k & 1) ? cp.sz : -cp.sz) : 0;
+   for (int m = 0; m < ncomp; m++) {
+      const T* x = xc + m * ca.ncg;
+      T v;
+      if (ORDER == 0) {
+         v = x[c0];
+      } else if (NDIM == 2) {
+         v = T(9.0 / 16.0) * x[c0] + T(3.0 / 16.0) * (x[c0 + oi] + x[c0 + oj]) +
+             T(1.0 / 16.0) * x[c0 + oi + oj];
+      } else {
+         v = T(27.0 / 64.0) * x[c0] +
+             T(9.0 / 64.0) * ((x[c0 + oi] + x[c0 + oj]) + x[c0 + ok]) +
+             T(3.0 / 64.0) * ((x[c0 + oi + oj] + x[c0 + oi + ok]) + x[c0 + oj + ok]) +
+             T(1.0 / 64.0) * x[c0 + oi + oj + ok];
+      }
+      T* y = xf + m * fa.ncg;
+      y[p.c0] = y[p.c0] + v;
+   }
+}
+
+template <typename T>
+int mg_restrict_cell(const Level& coarse, const Level& fine, int ncomp, const T* f, T* c,
+                     hipStream_t s)
+{
+   if (coarse.ndim == 3)
+      hipLaunchKernelGGL((k_restrict_cell<T, 3>), dim3(coarse.ncell_tiles), dim3(TX, TY), 0, s,
+                         level_args(coarse), level_args(fine), ncomp, f, c);
+   else
+      hipLaunchKernelGGL((k_restrict_cell<T, 2>), dim3(coarse.ncell_tiles), dim3(TX, TY), 0, s,
+                         level_args(coarse), level_args(fine), ncomp, f, c);
+   AMPE_HIP_CHECK(hipGetLastError());
+   return 0;
+}
+
+template <typename T>
+int mg_restrict_side(const Level& coarse, const Level& fine, const T* f, T* c, hipStream_t s)
+{
+   if (coarse.ndim == 3)
+      hipLaunchKernelGGL((k_restrict_side<T, 3>), dim3(coarse.nface_tiles), dim3(TX, TY), 0, s,
+                         level_args(coarse, true), level_args(fine), f, c);
+   else
+      hipLaunchKernelGGL((k_restrict_side<T, 2>), dim3(coarse.nface_tiles), dim3(TX, TY), 0, s,
+                         level_args(coarse, true), level_args(fine), f, c);
+   AMPE_HIP_CHECK(hipGetLastError());
+   return 0;
+}
+
+template <typename T>
+int mg_prolong_add(const Level& fine, const Level& coarse, int ncomp, int order, const T* xc,
+                   T* xf, hipStream_t s)
+{
+   dim3 g(fine.ncell_tiles), b(TX, TY);
+   LevelArgs fa = level_args(fine), ca = level_args(coarse);
+   if (fine.ndim == 3) {
+      if (order == 0)
+         hipLaunchKernelGGL((k_prolong_add<T, 3, 0>), g, b, 0, s, fa, ca, ncomp, xc, xf);
+      else
+         hipLaunchKernelGGL((k_prolong_add<T, 3, 1>), g, b, 0, s, fa, ca, ncomp, xc, xf);
+   } else {
+      if (order == 0)
+         hipLaunchKernelGGL((k_prolong_add<T, 2, 0>), g, b, 0, s, fa, ca, ncomp, xc, xf);
+      else
+         hipLaunchKernelGGL((k_prolong_add<T, 2, 1>), g, b, 0, s, fa, ca, ncomp, xc, xf);
+   }
+   AMPE_HIP_CHECK(hipGetLastError());
+   return 0;
+}
+
+bool is_coarsening(const Level& coarse, const Level& fine)
+{
+   if (coarse.ndim != fine.ndim || coarse.npatch != fine.npatch) return false;
+   for (int p = 0; p < fine.npatch; p++) {
+      const PatchDev& f = fine.hp[p];
+      const PatchDev& c = coarse.hp[p];
+      int nf[3] = {f.nx, f.ny, f.nz}, nc[3] = {c.nx, c.ny, c.nz};
+      for (int a = 0; a < fine.ndim; a++)
+         if ((f.lo[a] & 1) || (nf[a] & 1) || c.lo[a] * 2 != f.lo[a] || nc[a] * 2 != nf[a])
+            return false;
+   }
+   return true;
+}
+
+// ------------------------------------------------------------ hierarchy
+static bool halvable(const Level& L, int min_cells)
+{
+   int nd = L.ndim;
+   for (const PatchDev& d : L.hp) {
+      int n[3] = {d.nx, d.ny, d.nz};
+      for (int a = 0; a < nd; a++) {
+         if ((d.lo[a] & 1) || (n[a] & 1) || n[a] / 2 < min_cells) return false;
+      }
+   }
+   if (L.periodic)
+      for (int a = 0; a < nd; a++) {
+         int ext = L.pbox.hi[a] - L.pbox.lo[a] + 1;
+         if ((L.pbox.lo[a] & 1) || (ext & 1)) return false;
+      }
+   return true;
+}
+
+template <typename T>
+static int talloc(T** p, size_t n)
+{
+   if (hipMalloc(p, sizeof(T) * std::max<size_t>(n, 1)) != hipSuccess) return -1;
+   return hipMemset(*p, 0, sizeof(T) * std::max<size_t>(n, 1)) == hipSuccess ? 0 : -1;
+}
+
+template <typename T>
+int mg_create(const Level& fine, const double* dx, int ncomp_max, int max_levels, int min_cells,
+              Mg<T>** out)
+{
+   Mg<T>* m = new Mg<T>();
+   m->ncomp_max = ncomp_max;
+   MgLevel<T> l0;
+   l0.L = const_cast<Level*>(&fine);
+   for (int a = 0; a < 3; a++) l0.dx[a] = dx[a];
+   int rc = talloc(&l0.r, (size_t)ncomp_max * fine.ncg);
+   m->lv.push_back(l0);
+   if (max_levels < 1) max_levels = 1;
+   while (!rc && (int)m->lv.size() < max_levels && halvable(*m->lv.back().L, min_cells)) {
+      const Level& F = *m->lv.back().L;
+      std::vector<ampe_box> boxes(F.npatch);
+      for (int p = 0; p < F.npatch; p++) {
+         const PatchDev& d = F.hp[p];
+         int n[3] = {d.nx, d.ny, d.nz};
+         for (int a = 0; a < 3; a++) {
+            boxes[p].lo[a] = a < F.ndim ? d.lo[a] / 2 : 0;
+            boxes[p].hi[a] = a < F.ndim ? d.lo[a] / 2 + n[a] / 2 - 1 : 0;
+         }
+      }
+      ampe_box pb;
+      for (int a = 0; a < 3; a++) {
+         pb.lo[a] = F.pbox.lo[a] / 2;
+         pb.hi[a] = (F.pbox.hi[a] + 1) / 2 - 1;
+      }
+      MgLevel<T> c;
+      if (level_create(F.ndim, boxes.data(), F.npatch, F.g, F.periodic ? &pb : nullptr, &c.L)) {
+         rc = -1;
+         break;
+      }
+      for (int a = 0; a < 3; a++) c.dx[a] = 2.0 * m->lv.back().dx[a];
+      size_t ncg = (size_t)c.L->ncg, ns = (size_t)c.L->nside_tot;
+      rc |= talloc(&c.C, ncg);
+      rc |= talloc(&c.D, ns);
+      rc |= talloc(&c.fc, ns);
+      rc |= talloc(&c.sm, ncg);
+      rc |= talloc(&c.x, (size_t)ncomp_max * ncg);
+      rc |= talloc(&c.b, (size_t)ncomp_max * ncg);
+      rc |= talloc(&c.r, (size_t)ncomp_max * ncg);
+      m->lv.push_back(c);
+   }
+   if (rc) {
+      set_error("mg_create: out of device memory");
+      mg_destroy(m);
+      return -1;
+   }
+   *out = m;
+   return 0;
+}
+
+template <typename T>
+void mg_destroy(Mg<T>* m)
+{
+   if (!m) return;
+   for (size_t l = 0; l < m->lv.size(); l++) {
+      MgLevel<T>& c = m->lv[l];
+      (void)hipFree(c.r);
+      if (l == 0) continue;
+      (void)hipFree(c.C);
+      (void)hipFree(c.D);
+      (void)hipFree(c.fc);
+      (void)hipFree(c.sm);
+      (void)hipFree(c.x);
+      (void)hipFree(c.b);
+      level_destroy(c.L);
+   }
+   delete m;
+}
+
+template <typename T>
+int mg_setup_phase(Mg<T>& m, const T* C0, hipStream_t s)
+{
+   for (int l = 1; l < m.nlev(); l++) {
+      const T* f = l == 1 ? C0 : m.lv[l - 1].C;
+      if (mg_restrict_cell<T>(*m.lv[l].L, *m.lv[l - 1].L, 1, f, m.lv[l].C, s)) return -1;
+   }
+   return 0;
+}
+
+template <typename T>
+int mg_setup_conc(Mg<T>& m, const T* D0, hipStream_t s)
+{
+   for (int l = 1; l < m.nlev(); l++) {
+      const T* f = l == 1 ? D0 : m.lv[l - 1].D;
+      if (mg_restrict_side<T>(*m.lv[l].L, *m.lv[l - 1].L, f, m.lv[l].D, s)) return -1;
+   }
+   return 0;
+}
+
+template <typename T>
+int mg_setup_quat(Mg<T>& m, const T* fc0, const T* sm0, hipStream_t s)
+{
+   for (int l = 1; l < m.nlev(); l++) {
+      const Level& C = *m.lv[l].L;
+      const Level& F = *m.lv[l - 1].L;
+      const T* ff = l == 1 ? fc0 : m.lv[l - 1].fc;
+      const T* fs = l == 1 ? sm0 : m.lv[l - 1].sm;
+      if (mg_restrict_side<T>(C, F, ff, m.lv[l].fc, s)) return -1;
+      if (mg_restrict_cell<T>(C, F, 1, fs, m.lv[l].sm, s)) return -1;
+      if (ghost_fill(C, m.lv[l].sm, 1, s)) return -1;
+   }
+   return 0;
+}
+
+// ------------------------------------------------------------ cycles
+template <typename T>
+static int smooth_ell(const Level& L, const EllOp<T>& op, const double* dx, const T* b, T* x,
+                      int sweeps, hipStream_t s)
+{
+   for (int sw = 0; sw < sweeps; sw++) {
+      if (ghost_fill(L, x, 1, s)) return -1;
+      if (launch_rbgs<T>(L, op, dx, b, x, 0, false, s)) return -1;
+      if (ghost_fill(L, x, 1, s)) return -1;
+      if (launch_rbgs<T>(L, op, dx, b, x, 1, false, s)) return -1;
+   }
+   return 0;
+}
+
+template <typename T>
+static int smooth_quat(const Level& L, const T* fc, const T* sm, double gamma, const double* dx,
+                       int ncomp, const T* b, T* x, int sweeps, hipStream_t s)
+{
+   for (int sw = 0; sw < sweeps; sw++) {
+      if (ghost_fill(L, x, ncomp, s)) return -1;
+      if (launch_quat_rbgs<T>(L, fc, sm, gamma, dx, ncomp, b, x, 0, s)) return -1;
+      if (ghost_fill(L, x, ncomp, s)) return -1;
+      if (launch_quat_rbgs<T>(L, fc, sm, gamma, dx, ncomp, b, x, 1, s)) return -1;
+   }
+   return 0;
+}
+
+template <typename T>
+static EllOp<T> level_op(const Mg<T>& m, const EllOp<T>& op0, int kind, int l)
+{
+   EllOp<T> op = op0;
+   if (l > 0) {
+      if (kind == 0) op.C = m.lv[l].C;
+      else op.D = m.lv[l].D;
+   }
+   return op;
+}
+
+template <typename T>
+int mg_vcycle_ell(Mg<T>& m, const EllOp<T>& op0, int kind, const MgCycle& cy, const T* b0,
+                  T* x0, hipStream_t s)
+{
+   if (op0.M) {
+      set_error("mg_vcycle_ell: a mobility field is not supported (constant M only)");
+      return -1;
+   }
+   int nl = m.nlev();
+   // descend
+   for (int l = 0; l < nl; l++) {
+      MgLevel<T>& c = m.lv[l];
+      const T* b = l == 0 ? b0 : c.b;
+      T* x = l == 0 ? x0 : c.x;
+      EllOp<T> op = level_op(m, op0, kind, l);
+      if (l == nl - 1) {
+         if (smooth_ell<T>(*c.L, op, c.dx, b, x, cy.ncoarse, s)) return -1;
+         break;
+      }
+      if (smooth_ell<T>(*c.L, op, c.dx, b, x, cy.nu1, s)) return -1;
+      if (ghost_fill(*c.L, x, 1, s)) return -1;
+      if (launch_residual<T>(*c.L, op, c.dx, b, x, c.r, nullptr, s)) return -1;
+      MgLevel<T>& n = m.lv[l + 1];
+      if (mg_restrict_cell<T>(*n.L, *c.L, 1, c.r, n.b, s)) return -1;
+      AMPE_HIP_CHECK(hipMemsetAsync(n.x, 0, sizeof(T) * n.L->ncg, s));
+   }
+   // ascend
+   for (int l = nl - 2; l >= 0; l--) {
+      MgLevel<T>& c = m.lv[l];
+      MgLevel<T>& n = m.lv[l + 1];
+      const T* b = l == 0 ? b0 : c.b;
+      T* x = l == 0 ? x0 : c.x;
+      if (ghost_fill(*n.L, n.x, 1, s)) return -1;
+      if (mg_prolong_add<T>(*c.L, *n.L, 1, 1, n.x, x, s)) return -1;
+      if (smooth_ell<T>(*c.L, level_op(m, op0, kind, l), c.dx, b, x, cy.nu2, s)) return -1;
+   }
+   return 0;
+}
+
+template <typename T>
+int mg_vcycle_quat(Mg<T>& m, const T* fc0, const T* sm0, double gamma, int ncomp,
+                   const MgCycle& cy, const T* b0, T* x0, hipStream_t s)
+{
+   if (ncomp > m.ncomp_max) {
+      set_error("mg_vcycle_quat: more components than the hierarchy holds");
+      return -1;
+   }
+   int nl = m.nlev();
+   for (int l = 0; l < nl; l++) {
+      MgLevel<T>& c = m.lv[l];
+      const T* b = l == 0 ? b0 : c.b;
+      T* x = l == 0 ? x0 : c.x;
+      const T* fc = l == 0 ? fc0 : c.fc;
+      const T* sm = l == 0 ? sm0 : c.sm;
+      if (l == nl - 1) {
+         if (smooth_quat<T>(*c.L, fc, sm, gamma, c.dx, ncomp, b, x, cy.ncoarse, s)) return -1;
+         break;
+      }
+      if (smooth_quat<T>(*c.L, fc, sm, gamma, c.dx, ncomp, b, x, cy.nu1, s)) return -1;
+      if (ghost_fill(*c.L, x, ncomp, s)) return -1;
+      if (launch_quat_residual<T>(*c.L, fc, sm, gamma, c.dx, ncomp, b, x, c.r, nullptr, s))
+         return -1;
+      MgLevel<T>& n = m.lv[l + 1];
+      if (mg_restrict_cell<T>(*n.L, *c.L, ncomp, c.r, n.b, s)) return -1;
+      AMPE_HIP_CHECK(hipMemsetAsync(n.x, 0, sizeof(T) * n.L->ncg * ncomp, s));
+   }
+   for (int l = nl - 2; l >= 0; l--) {
+      MgLevel<T>& c = m.lv[l];
+      MgLevel<T>& n = m.lv[l + 1];
+      const T* b = l == 0 ? b0 : c.b;
+      T* x = l == 0 ? x0 : c.x;
+      const T* fc = l == 0 ? fc0 : c.fc;
+      const T* sm = l == 0 ? sm0 : c.sm;
+      if (ghost_fill(*n.L, n.x, ncomp, s)) return -1;
+      if (mg_prolong_add<T>(*c.L, *n.L, ncomp, 1, n.x, x, s)) return -1;
+      if (smooth_quat<T>(*c.L, fc, sm, gamma, c.dx, ncomp, b, x, cy.nu2, s)) return -1;
+   }
+   return 0;
+}
+
+#define AMPE_MG_INST(T)                                                                      \
+   template int mg_create<T>(const Level&, const double*, int, int, int, Mg<T>**);          \
+   template void mg_destroy<T>(Mg<T>*);                                                      \
+   template int mg_setup_phase<T>(Mg<T>&, const T*, hipStream_t);                            \
+   template int mg_setup_conc<T>(Mg<T>&, const T*, hipStream_t);                             \
+   template int mg_setup_quat<T>(Mg<T>&, const T*, const T*, hipStream_t);                   \
+   template int mg_vcycle_ell<T>(Mg<T>&, const EllOp<T>&, int, const MgCycle&, const T*, T*, \
+                                 hipStream_t);                                               \
+   template int mg_vcycle_quat<T>(Mg<T>&, const T*, const T*, double, int, const MgCycle&,   \
+                                  const T*, T*, hipStream_t);                                \
+   template int mg_restrict_cell<T>(const Level&, const Level&, int, const T*, T*,           \
+                                    hipStream_t);                                            \
+   template int mg_restrict_side<T>(const Level&, const Level&, const T*, T*, hipStream_t);  \
+   template int mg_prolong_add<T>(const Level&, const Level&, int, int, const T*, T*,        \
+                                  hipStream_t);
+AMPE_MG_INST(float)
+AMPE_MG_INST(double)
+
+}  // namespace ampe

@@ -7,6 +7,7 @@
 #include "fac.h"
 #include "kks.h"
 #include "krylov.h"
+#include "mg.h"
 
 namespace ampe {
 
@@ -39,6 +40,8 @@ struct Sim {
    hipEvent_t ev0 = nullptr, ev1 = nullptr;
    long long* kks_acc = nullptr;  // device KKS counters (launch_kks)
    KksGuess guess;                // surrogate guess when kks_guess is surrogate
+   Mg<double>* mgd = nullptr;     // multigrid hierarchy (precond = MG, FP64)
+   Mg<float>* mgf = nullptr;      // multigrid hierarchy (precond = MG, FP32)
    float last_ms = 0.f;
    ampe_stats st{};
 };
@@ -50,7 +53,6 @@ int sim_set_state(Sim* S, const double* phi, const double* c, const double* q);
 int sim_get_state(Sim* S, double* phi, double* c, double* q, double* cl, double* cs);
 int sim_advance(Sim* S, int nsteps, ampe_stats* stats);
 
-int vec_div_comp(Sim* S, const double* x, const double* sm, double* y, int ncomp);
 int vec_mul_comp(Sim* S, double* x, const double* sm, int ncomp);
 
 }  // namespace ampe

@@ -92,6 +92,16 @@ int sim_create(const ampe_model_params& p, const ampe_solver_params& s, const am
       rc |= falloc(&S->f_Dconc, S->nf);
       rc |= falloc(&S->f_fc, S->nf);
    }
+   if (!rc && s.precond == AMPE_PRECOND_MG) {
+      int maxlev = s.mg_max_levels > 0 ? s.mg_max_levels : 32;
+      if (s.precision == AMPE_FAC_FP32)
+         rc = mg_create<float>(L, p.dx, ql, maxlev, 2, &S->mgf);
+      else
+         rc = mg_create<double>(L, p.dx, ql, maxlev, 2, &S->mgd);
+   } else if (s.precond != AMPE_PRECOND_REDBLACK) {
+      set_error("sim_create: unknown preconditioner");
+      rc = -1;
+   }
    if (!rc && hipMalloc(&S->kks_acc, sizeof(long long) * kKksAcc) != hipSuccess) rc = -1;
    if (!rc && hipMemset(S->kks_acc, 0, sizeof(long long) * kKksAcc) != hipSuccess) rc = -1;
    if (rc || hipStreamCreate(&S->stream) != hipSuccess ||
@@ -117,6 +127,8 @@ void sim_destroy(Sim* S)
    float* fall[] = {S->f_u, S->f_e, S->f_r, S->f_f, S->f_Cphi, S->f_sqrtm, S->f_Dconc, S->f_fc};
    for (float* v : fall) (void)hipFree(v);
    (void)hipFree(S->kks_acc);
+   mg_destroy(S->mgf);
+   mg_destroy(S->mgd);
    if (S->stream) (void)hipStreamDestroy(S->stream);
    if (S->ev0) (void)hipEventDestroy(S->ev0);
    if (S->ev1) (void)hipEventDestroy(S->ev1);
@@ -310,6 +322,16 @@ static int psetup(Sim* S, double* y, double gamma)
          if (vec_to_f32(S->ng, S->sqrtm, S->f_sqrtm, st)) return -1;
       }
    }
+   if (S->mgf) {
+      if (mg_setup_phase<float>(*S->mgf, S->f_Cphi, st)) return -1;
+      if (c && mg_setup_conc<float>(*S->mgf, S->f_Dconc, st)) return -1;
+      if (q && mg_setup_quat<float>(*S->mgf, S->f_fc, S->f_sqrtm, st)) return -1;
+   }
+   if (S->mgd) {
+      if (mg_setup_phase<double>(*S->mgd, S->Cphi, st)) return -1;
+      if (c && mg_setup_conc<double>(*S->mgd, S->Dconc, st)) return -1;
+      if (q && mg_setup_quat<double>(*S->mgd, S->fc, S->sqrtm, st)) return -1;
+   }
    S->gamma_setup = gamma;
    return 0;
 }
@@ -332,11 +354,33 @@ struct PBlock {
 };
 
 template <typename T>
+static Mg<T>* mg_of(Sim* S);
+template <>
+Mg<float>* mg_of<float>(Sim* S)
+{
+   return S->mgf;
+}
+template <>
+Mg<double>* mg_of<double>(Sim* S)
+{
+   return S->mgd;
+}
+
+template <typename T>
 static int block_smooth(Sim* S, PBlock<T>& B, const T* rhs, T* x)
 {
    const Level& L = *S->L;
    hipStream_t st = S->stream;
    const double* dx = S->p.dx;
+   Mg<T>* mg = mg_of<T>(S);
+   if (mg) {
+      // one multigrid V-cycle from x = 0 (x is zero on entry)
+      MgCycle cy;
+      cy.nu1 = cy.nu2 = B.kind == 0 ? S->s.fac_sweeps : S->s.quat_sweeps;
+      cy.ncoarse = S->s.mg_coarse_sweeps;
+      if (B.kind == 0) return mg_vcycle_ell<T>(*mg, B.op, B.comp == 0 ? 0 : 1, cy, rhs, x, st);
+      return mg_vcycle_quat<T>(*mg, B.fc, B.sm, S->gamma_setup, B.ncomp, cy, rhs, x, st);
+   }
    if (B.kind == 0) {
       bool want = S->s.fac_tol >= 0.0;
       if (ghost_fill(L, x, 1, st)) return -1;
@@ -389,13 +433,17 @@ static int run_blocks(Sim* S, std::vector<PBlock<T>>& blocks, double tol)
    const Level& L = *S->L;
    hipStream_t st = S->stream;
    int nb = (int)blocks.size();
+   bool multi = S->s.fac_max_cycles > 1 || S->s.quat_max_cycles > 1;
+   // cycle 1 from u = 0 (zeroed by psolve_in)
    for (int b = 0; b < nb; b++) {
       PBlock<T>& B = blocks[b];
-      AMPE_HIP_CHECK(hipMemsetAsync(B.u, 0, sizeof(T) * S->ng * B.ncomp, st));
       if (block_smooth<T>(S, B, B.f, B.u)) return -1;
-      if (block_residual<T>(S, B, b)) return -1;
       B.cycles = 1;
+      if (multi && block_residual<T>(S, B, b)) return -1;
    }
+   // With one cycle per block nothing depends on the residual norms (the
+   // return value is informational), so they are not computed.
+   if (!multi) return 0;
    double ss[8];
    if (reduce_regions(L, L.ncell_tiles, nb, ss, st)) return -1;
    for (int b = 0; b < nb; b++)
@@ -439,100 +487,75 @@ static int run_blocks(Sim* S, std::vector<PBlock<T>>& blocks, double tol)
 // :3594-3650, divide / multiply by sqrt(M) of QuatSysSolver::solveSystem,
 // source/QuatSysSolver.cc:296-349), composition block
 // (ConcentrationPrecondSolve :3547-3591).
+template <typename T>
+static std::vector<PBlock<T>> make_blocks(Sim* S, double dphi, const T* Cphi, const T* Dconc,
+                                          const T* fc, const T* sm)
+{
+   const ampe_model_params& p = S->p;
+   std::vector<PBlock<T>> bl;
+   PBlock<T> ph{};
+   ph.kind = 0;
+   ph.comp = 0;
+   ph.ncomp = 1;
+   ph.op.Dc = (T)dphi;
+   ph.op.C = Cphi;
+   ph.op.Cc = T(1);
+   ph.op.Mc = (T)p.phi_mobility;
+   bl.push_back(ph);
+   if (S->iq >= 0) {
+      PBlock<T> qb{};
+      qb.kind = 1;
+      qb.comp = S->iq;
+      qb.ncomp = p.qlen;
+      qb.fc = fc;
+      qb.sm = sm;
+      bl.push_back(qb);
+   }
+   if (S->ic >= 0) {
+      PBlock<T> cb{};
+      cb.kind = 0;
+      cb.comp = S->ic;
+      cb.ncomp = 1;
+      cb.op.D = Dconc;
+      cb.op.Cc = T(1);
+      cb.op.Mc = (T)p.conc_mobility;
+      bl.push_back(cb);
+   }
+   return bl;
+}
+
 static int psolve(Sim* S, const double* r, double* z, double delta)
 {
    const ampe_model_params& p = S->p;
    hipStream_t st = S->stream;
    long long ng = S->ng;
    S->st.nps++;
-   AMPE_HIP_CHECK(hipMemsetAsync(z, 0, sizeof(double) * S->nv, st));
    double dphi = -S->gamma_setup * p.epsilon_phi * p.epsilon_phi;
-   int ql = S->iq >= 0 ? p.qlen : 1;
-   if (S->iq >= 0 &&
-       vec_div_comp(S, r + (long long)S->iq * ng, S->sqrtm, S->fac_u, ql))  // FP64 division
-      return -1;
+   int ql = S->iq >= 0 ? p.qlen : 0;
+   int iq = S->iq >= 0 ? S->iq : S->nc;
    if (S->s.precision == AMPE_FAC_FP32) {
-      std::vector<PBlock<float>> bl;
-      PBlock<float> ph{};
-      ph.kind = 0;
-      ph.comp = 0;
-      ph.ncomp = 1;
-      ph.op.Dc = (float)dphi;
-      ph.op.C = S->f_Cphi;
-      ph.op.Cc = 1.0f;
-      ph.op.Mc = (float)p.phi_mobility;
-      bl.push_back(ph);
-      if (S->iq >= 0) {
-         PBlock<float> qb{};
-         qb.kind = 1;
-         qb.comp = S->iq;
-         qb.ncomp = ql;
-         qb.fc = S->f_fc;
-         qb.sm = S->f_sqrtm;
-         bl.push_back(qb);
-      }
-      if (S->ic >= 0) {
-         PBlock<float> cb{};
-         cb.kind = 0;
-         cb.comp = S->ic;
-         cb.ncomp = 1;
-         cb.op.D = S->f_Dconc;
-         cb.op.Cc = 1.0f;
-         cb.op.Mc = (float)p.conc_mobility;
-         bl.push_back(cb);
-      }
+      // f = r (orientation blocks / sqrt(M), FP64 division) in FP32, u = 0
+      if (psolve_in<float>(ng, S->nc, iq, ql, r, S->sqrtm, S->f_f, nullptr, S->f_u, st))
+         return -1;
+      auto bl = make_blocks<float>(S, dphi, S->f_Cphi, S->f_Dconc, S->f_fc, S->f_sqrtm);
       for (auto& B : bl) {
          long long off = (long long)B.comp * ng;
          B.f = S->f_f + off;
          B.u = S->f_u + off;
          B.e = S->f_e + off;
          B.r = S->f_r + off;
-         const double* src = (B.kind == 1) ? S->fac_u : r + off;
-         if (vec_to_f32(ng * B.ncomp, src, S->f_f + off, st)) return -1;
       }
       int ret = run_blocks<float>(S, bl, delta);
       if (ret < 0) return -1;
-      for (auto& B : bl) {
-         long long off = (long long)B.comp * ng;
-         if (vec_to_f64(ng * B.ncomp, B.u, z + off, st)) return -1;
-         if (B.kind == 1 && vec_mul_comp(S, z + off, S->sqrtm, ql)) return -1;
-      }
+      if (psolve_out<float>(ng, S->nc, iq, ql, S->f_u, S->sqrtm, z, st)) return -1;
       return ret;
    }
-   std::vector<PBlock<double>> bl;
-   PBlock<double> ph{};
-   ph.kind = 0;
-   ph.comp = 0;
-   ph.ncomp = 1;
-   ph.op.Dc = dphi;
-   ph.op.C = S->Cphi;
-   ph.op.Cc = 1.0;
-   ph.op.Mc = p.phi_mobility;
-   ph.f = r;
-   bl.push_back(ph);
-   if (S->iq >= 0) {
-      PBlock<double> qb{};
-      qb.kind = 1;
-      qb.comp = S->iq;
-      qb.ncomp = ql;
-      qb.fc = S->fc;
-      qb.sm = S->sqrtm;
-      qb.f = S->fac_u;
-      bl.push_back(qb);
-   }
-   if (S->ic >= 0) {
-      PBlock<double> cb{};
-      cb.kind = 0;
-      cb.comp = S->ic;
-      cb.ncomp = 1;
-      cb.op.D = S->Dconc;
-      cb.op.Cc = 1.0;
-      cb.op.Mc = p.conc_mobility;
-      cb.f = r + (long long)S->ic * ng;
-      bl.push_back(cb);
-   }
+   // FP64: z = 0, fac_u = orientation blocks of r / sqrt(M)
+   if (psolve_in<double>(ng, S->nc, iq, ql, r, S->sqrtm, nullptr, S->fac_u, z, st)) return -1;
+   auto bl = make_blocks<double>(S, dphi, S->Cphi, S->Dconc, S->fc, S->sqrtm);
    for (auto& B : bl) {
       long long off = (long long)B.comp * ng;
+      B.f = B.kind == 1 ? S->fac_u : r + off;
       B.u = z + off;
       B.e = S->fac_e + off;
       B.r = S->fac_r + off;
@@ -543,20 +566,9 @@ static int psolve(Sim* S, const double* r, double* z, double delta)
    return ret;
 }
 
-int vec_div_comp(Sim* S, const double* x, const double* sm, double* y, int ncomp)
-{
-   for (int m = 0; m < ncomp; m++)
-      if (vec_div(S->ng, x + (long long)m * S->ng, sm, y + (long long)m * S->ng, S->stream))
-         return -1;
-   return 0;
-}
-
 int vec_mul_comp(Sim* S, double* x, const double* sm, int ncomp)
 {
-   for (int m = 0; m < ncomp; m++)
-      if (vec_mul(S->ng, x + (long long)m * S->ng, sm, x + (long long)m * S->ng, S->stream))
-         return -1;
-   return 0;
+   return vec_scale_comp(S->ng, ncomp, sm, x, S->stream);
 }
 
 // ------------------------------------------------------------ Krylov

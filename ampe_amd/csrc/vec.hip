@@ -136,4 +136,96 @@ int vec_sumsq_w(const Level& L, int ncomp, const double* x, const double* w, dou
    return reduce_partials(L, L.ncell_tiles, 1, 0, out, s);
 }
 
+// Preconditioner input over all nc ghosted component blocks: u = 0 and
+// f = r, with the orientation blocks [iq, iq + ql) divided by sqrt(M) (FP64
+// division, then the conversion to T).  For T = double f may be null: then
+// only the orientation blocks are written, to fq (ql blocks).
+template <typename T>
+__global__ void __launch_bounds__(256) k_psolve_in(long long ng, int nc, int iq, int ql,
+                                                   const double* __restrict__ r,
+                                                   const double* __restrict__ sm,
+                                                   T* __restrict__ f, T* __restrict__ fq,
+                                                   T* __restrict__ u)
+{
+   long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+   if (i >= ng * nc) return;
+   long long m = i / ng, c = i - m * ng;
+   u[i] = T(0);
+   bool q = m >= iq && m < iq + ql;
+   if (q) {
+      double v = r[i] / sm[c];
+      if (f) f[i] = (T)v;
+      else fq[i - (long long)iq * ng] = (T)v;
+   } else if (f) {
+      f[i] = (T)r[i];
+   }
+}
+
+// Preconditioner output: z = u, orientation blocks multiplied by sqrt(M).
+template <typename T>
+__global__ void __launch_bounds__(256) k_psolve_out(long long ng, int nc, int iq, int ql,
+                                                    const T* __restrict__ u,
+                                                    const double* __restrict__ sm,
+                                                    double* __restrict__ z)
+{
+   long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+   if (i >= ng * nc) return;
+   long long m = i / ng, c = i - m * ng;
+   double v = (double)u[i];
+   if (m >= iq && m < iq + ql) v = v * sm[c];
+   z[i] = v;
+}
+
+template <typename T>
+int psolve_in(long long ng, int nc, int iq, int ql, const double* r, const double* sm, T* f,
+              T* fq, T* u, hipStream_t s)
+{
+   long long n = ng * nc;
+   hipLaunchKernelGGL(k_psolve_in<T>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, ng, nc,
+                      iq, ql, r, sm, f, fq, u);
+   AMPE_HIP_CHECK(hipGetLastError());
+   return 0;
+}
+
+template <typename T>
+int psolve_out(long long ng, int nc, int iq, int ql, const T* u, const double* sm, double* z,
+               hipStream_t s)
+{
+   long long n = ng * nc;
+   hipLaunchKernelGGL(k_psolve_out<T>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, ng,
+                      nc, iq, ql, u, sm, z);
+   AMPE_HIP_CHECK(hipGetLastError());
+   return 0;
+}
+
+// x[m][c] *= sm[c] for ncomp blocks of ng entries
+__global__ void __launch_bounds__(256) k_scale_comp(long long ng, int ncomp,
+                                                    const double* __restrict__ sm,
+                                                    double* __restrict__ x)
+{
+   long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+   if (i >= ng * ncomp) return;
+   long long m = i / ng;
+   x[i] = x[i] * sm[i - m * ng];
+}
+
+int vec_scale_comp(long long ng, int ncomp, const double* sm, double* x, hipStream_t s)
+{
+   long long n = ng * ncomp;
+   if (n <= 0) return 0;
+   hipLaunchKernelGGL(k_scale_comp, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, ng, ncomp,
+                      sm, x);
+   AMPE_HIP_CHECK(hipGetLastError());
+   return 0;
+}
+
+template int psolve_in<float>(long long, int, int, int, const double*, const double*, float*,
+                              float*, float*, hipStream_t);
+template int psolve_in<double>(long long, int, int, int, const double*, const double*, double*,
+                               double*, double*, hipStream_t);
+template int psolve_out<float>(long long, int, int, int, const float*, const double*, double*,
+                               hipStream_t);
+template int psolve_out<double>(long long, int, int, int, const double*, const double*,
+                                double*, hipStream_t);
+
 }  // namespace ampe

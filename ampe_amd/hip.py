@@ -57,6 +57,9 @@ _SIGS = {
     "ampe_sim_advance": (_i, [_vp, _i, C.POINTER(Stats)]),
     "ampe_sim_last_time_ms": (_i, [_vp, C.POINTER(_d)]),
     "ampe_sim_set_surrogate": (_i, [_vp, _vp, _i, _d]),
+    "ampe_coarsen_cell": (_i, [_vp, _vp, _i, _vp, _vp, _vp]),
+    "ampe_coarsen_side": (_i, [_vp, _vp, _vp, _vp, _vp]),
+    "ampe_refine_add": (_i, [_vp, _vp, _i, _i, _vp, _vp, _vp]),
     "ampe_calphad_at": (_i, [C.POINTER(CalphadTdep), _d, C.POINTER(CalphadPhase)]),
     "ampe_surrogate_create": (_i, [_i, _i, _i, C.c_ulonglong, C.POINTER(_vp)]),
     "ampe_surrogate_destroy": (_i, [_vp]),

@@ -104,6 +104,9 @@ class SolverParams(C.Structure):
         ("quat_sweeps", C.c_int),
         ("precision", C.c_int),
         ("kks_guess", C.c_int),
+        ("precond", C.c_int),
+        ("mg_max_levels", C.c_int),
+        ("mg_coarse_sweeps", C.c_int),
     ]
 
 
@@ -121,6 +124,7 @@ FLUX_SIMPLE, FLUX_ISOTROPIC, FLUX_ANISOTROPIC = 0, 1, 2
 FE_QUADRATIC, FE_CALPHAD_BINARY = 0, 1
 FP64, FAC_FP32 = 0, 1
 GUESS_PREVIOUS, GUESS_SURROGATE = 0, 1
+PRECOND_REDBLACK, PRECOND_MG = 0, 1
 
 R_GAS = 8.314472
 
@@ -230,7 +234,8 @@ def solver_defaults(**kw):
                      max_newton=3, nlscoef=0.1, eplifac=0.05, msbp=1,
                      lag_quat_sidegrad=1, fac_max_cycles=1, fac_sweeps=2,
                      fac_tol=-1.0, quat_max_cycles=1, quat_sweeps=2,
-                     precision=FP64, kks_guess=GUESS_PREVIOUS)
+                     precision=FP64, kks_guess=GUESS_PREVIOUS,
+                     precond=PRECOND_REDBLACK, mg_max_levels=0, mg_coarse_sweeps=16)
     for k, v in kw.items():
         setattr(s, k, v)
     return s

@@ -2,16 +2,27 @@
 
 Workload (default): 3-D 256^3 periodic polycrystal, 64 Voronoi grains with
 S3-uniform orientations, solid fraction 0.7, binary CALPHAD KKS at 1500 K with
-seeded Redlich-Kister parameters (physical units, dx = 0.02 um).  The same initial
-state is advanced by the plain FP64 path and by the mixed path (FP32 FAC
-preconditioner under the FP64 Newton-Krylov outer loop); value is the mixed
-path's throughput, with its per-field relative L2 error against the FP64 run
-after the same steps.  dt = 1e-5 s; the warmup steps carry the initial
-transient (white noise in c relaxes on ~1e-8 s, so the first steps are cut and
-h regrows to dt), the timed steps run at dt.
+seeded Redlich-Kister parameters (physical units, dx = 0.02 um), dt = 5e-6 s.
 
-Prints one JSON line (see the keys below).  Run with --gpus 1 only: this build
-targets a single MI355X.
+Protocol.  The initial state carries a fast transient (white noise in c relaxes
+on ~1e-8 s) during which the integrator cuts steps; it is relaxed once with
+the plain FP64 form (--relax steps, untimed).  Every form then starts from that
+same relaxed state, takes 3 untimed steps (BDF2 history, first KKS guesses) and
+--steps timed steps at dt.  All forms take identical steps (same t), so their
+fields are compared at the same simulated time.
+
+Forms:
+  fp64_reference : FP64 everywhere, red-black block solves (the reference's
+                   FAC "redblack" coarse solve), previous-step KKS guess -- the
+                   plain FP64 GPU form, parity-tested against the oracle;
+  fp64_mg        : FP64 everywhere, multigrid V-cycle block solves;
+  mixed          : FP32 multigrid preconditioner under the FP64 Newton-Krylov
+                   outer loop, previous-step KKS guess;
+  mixed_surrogate: mixed + surrogate-seeded KKS (the metric's form, value).
+The KKS surrogate is trained at start-up on 1e7 FP64 Newton solutions of the
+config-3 distribution and also reported in isolation on 2^24 cells.
+
+Prints one JSON line.  Run with --gpus 1 only: this build targets one MI355X.
 """
 import argparse
 import ctypes as C
@@ -66,22 +77,52 @@ def polycrystal_state_fast(n, seed=1002, ngrains=64, solid_fraction=0.7, width=3
     return phi, c, q
 
 
-def solver(prec, args):
-    return P.solver_defaults(dt=args.dt, maxl=args.maxl, max_newton=5, atol=args.atol,
-                             rtol=args.atol * 1e-2, precision=prec, fac_sweeps=2,
-                             quat_sweeps=2)
+FORMS = ("fp64_reference", "fp64_mg", "mixed", "mixed_surrogate")
 
 
-def run_gpu(p, s, state, n, warmup, steps):
+def solver(form, args):
+    kw = dict(dt=args.dt, maxl=args.maxl, max_newton=5, atol=args.atol, rtol=args.atol * 1e-2)
+    mg = dict(precond=P.PRECOND_MG, fac_sweeps=1, quat_sweeps=1, mg_max_levels=args.mg_levels,
+              mg_coarse_sweeps=args.mg_coarse_sweeps)
+    if form == "fp64_reference":
+        kw.update(precision=P.FP64, fac_sweeps=2, quat_sweeps=2)
+    elif form == "fp64_mg":
+        kw.update(precision=P.FP64, **mg)
+    elif form == "mixed":
+        kw.update(precision=P.FAC_FP32, **mg)
+    elif form == "mixed_surrogate":
+        kw.update(precision=P.FAC_FP32, kks_guess=P.GUESS_SURROGATE, **mg)
+    else:
+        raise ValueError(form)
+    return P.solver_defaults(**kw)
+
+
+def relax(p, args, state, n):
+    """Untimed relaxation of the initial transient with the plain FP64 form."""
+    from ampe_amd import hip as H
+    box = H.boxes_of((n, n, n), 3)
+    sim = H.Sim(p, solver("fp64_reference", args), [box], periodic=box)
+    sim.set_state(*state)
+    t0 = time.perf_counter()
+    rc, st = sim.advance(args.relax)
+    if rc:
+        raise RuntimeError("relaxation step failed")
+    log(f"[bench] relaxed {args.relax} steps to t={st['t']:.3e} in "
+        f"{time.perf_counter() - t0:.1f}s (step cuts {st['ncfn']})")
+    return sim.get_state((n, n, n))[:3]
+
+
+def run_gpu(p, s, state, n, steps, surrogate=None, pre=3):
     import torch
     from ampe_amd import hip as H
     box = H.boxes_of((n, n, n), 3)
     sim = H.Sim(p, s, [box], periodic=box)
+    if surrogate is not None:
+        sim.set_surrogate(surrogate, probe_its=3, probe_tol=1e-8)
     sim.set_state(*state)
-    if warmup:
-        rc, _ = sim.advance(warmup)
-        if rc:
-            raise RuntimeError("warmup step failed")
+    rc, s0 = sim.advance(pre)
+    if rc:
+        raise RuntimeError("untimed step failed")
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     rc, st = sim.advance(steps)
@@ -90,8 +131,40 @@ def run_gpu(p, s, state, n, warmup, steps):
     if rc:
         raise RuntimeError(f"timed steps failed: {st}")
     dev_ms = sim.last_time_ms()
+    delta = {k: st[k] - s0[k] for k in ("nst", "nni", "nli", "nfe", "npe", "nps", "ncfn",
+                                        "nlcf", "kks_calls", "kks_iters", "kks_fallbacks")}
+    delta["t"] = st["t"]
     fields = sim.get_state((n, n, n))
-    return dict(wall=wall, dev_ms=dev_ms, stats=st, fields=fields)
+    return dict(wall=wall, dev_ms=dev_ms, stats=delta, fields=fields)
+
+
+def surrogate_report(p, dev):
+    """Train the KKS surrogate (1e7 FP64 Newton samples) and measure it on
+    2^24 fresh cells against the default (0.5, 0.5) start."""
+    import torch
+    from ampe_amd import hip as H
+    from ampe_amd import surrogate as SG
+    tab = P.auni_tdep(P.seeded_rk())
+    s, info = SG.train_kks_surrogate(p, tab, dev)
+    err = SG.surrogate_error(s, p, tab, dev)
+    n = 1 << 24
+    x = torch.from_numpy(np.stack(SG.draw_cells(n, 2024))).to(dev)
+    out = {}
+    for name, guess in (("default", None), ("surrogate", s)):
+        cl = torch.zeros(n, dtype=torch.float64, device=dev)
+        cs = torch.zeros_like(cl)
+        H.kks_batch(p, tab, x[0], x[1], x[2], cl, cs, guess=guess)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        st = H.kks_batch(p, tab, x[0], x[1], x[2], cl, cs, guess=guess)
+        torch.cuda.synchronize()
+        out[name] = dict(ms=1e3 * (time.perf_counter() - t0), iters_per_cell=st["iters"] / n,
+                         fallbacks=st["fallbacks"], failures=st["failures"])
+    rep = dict(train=info, guess_error=err, cells=n, default=out["default"],
+               surrogate=out["surrogate"],
+               newton_iters_saved=1.0 - out["surrogate"]["iters_per_cell"]
+               / out["default"]["iters_per_cell"])
+    return s, rep
 
 
 def kernel_roofline(p, state, n, reps=20):
@@ -142,10 +215,10 @@ def cpu_baseline(args):
     from oracle import oracle as O
     n = args.cpu_n
     p, phi, c, q = make_workload(n, seed=1002)
-    s = solver(P.FP64, args)
+    s = solver("fp64_reference", args)
     sim = O.Sim(p, s, n)
     sim.set_state(phi, c, q)
-    rc, st0 = sim.advance(args.warmup)
+    rc, st0 = sim.advance(args.relax + 3)
     t0 = time.perf_counter()
     rc, st = sim.advance(args.cpu_steps)
     dt = time.perf_counter() - t0
@@ -153,9 +226,9 @@ def cpu_baseline(args):
     cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
     return dict(value=n ** 3 * steps / dt, unit="cells*steps/s", cores=cores,
                 kind="port",
-                sample=f"oracle FP64 integrator (OpenMP), same workload at {n}^3 after "
-                       f"{args.warmup} warmup steps: {steps} timed BDF steps in {dt:.1f}s, "
-                       f"rc={rc}")
+                sample=f"oracle FP64 integrator (OpenMP, red-black block solves), same "
+                       f"workload at {n}^3 after {args.relax + 3} untimed steps: {steps} timed "
+                       f"BDF steps in {dt:.1f}s, rc={rc}")
 
 
 def rel_l2(a, b):
@@ -166,14 +239,19 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
-    ap.add_argument("--warmup", type=int, default=8)
+    ap.add_argument("--warmup", type=int, default=3, help="untimed steps per form")
+    ap.add_argument("--relax", type=int, default=20)
     ap.add_argument("--n", type=int, default=256)
-    ap.add_argument("--dt", type=float, default=1.0e-5)
+    ap.add_argument("--dt", type=float, default=5.0e-6)
     ap.add_argument("--atol", type=float, default=1.0e-6)
     ap.add_argument("--maxl", type=int, default=10)
+    ap.add_argument("--mg-levels", type=int, default=3)
+    ap.add_argument("--mg-coarse-sweeps", type=int, default=2)
+    ap.add_argument("--forms", default=",".join(FORMS))
     ap.add_argument("--cpu-n", type=int, default=32)
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-surrogate", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU only (this build has no multi-GPU path); "
@@ -182,21 +260,38 @@ def main():
     import torch
     assert torch.cuda.is_available(), "bench needs a GPU"
     torch.cuda.init()
+    dev = torch.device("cuda:0")
+    forms = [f for f in args.forms.split(",") if f]
 
     n = args.n
     p, phi, c, q = make_workload(n)
-    state = (phi, c, q)
-    log("[bench] FP64 run")
-    r64 = run_gpu(p, solver(P.FP64, args), state, n, args.warmup, args.steps)
-    log(f"[bench] FP64: {r64['wall']:.2f}s {r64['stats']}")
-    log("[bench] mixed run")
-    rmx = run_gpu(p, solver(P.FAC_FP32, args), state, n, args.warmup, args.steps)
-    log(f"[bench] mixed: {rmx['wall']:.2f}s {rmx['stats']}")
+    surrogate, srep = None, None
+    if "mixed_surrogate" in forms and not args.no_surrogate:
+        log("[bench] training the KKS surrogate")
+        surrogate, srep = surrogate_report(p, dev)
+        log(f"[bench] surrogate {srep}")
+    elif "mixed_surrogate" in forms:
+        forms.remove("mixed_surrogate")
+    state = relax(p, args, (phi, c, q), n)
     cells = n ** 3
-    v64 = cells * args.steps / r64["wall"]
-    vmx = cells * args.steps / rmx["wall"]
-    errs = {name: rel_l2(a, b) for name, a, b in
-            zip(("phi", "c", "q"), rmx["fields"][:3], r64["fields"][:3])}
+    runs = {}
+    for f in forms:
+        runs[f] = run_gpu(p, solver(f, args), state, n, args.steps,
+                          surrogate if f == "mixed_surrogate" else None, pre=args.warmup)
+        r = runs[f]
+        r["value"] = cells * r["stats"]["nst"] / r["wall"]
+        log(f"[bench] {f}: {r['wall']:.3f}s value {r['value']:.4g} {r['stats']}")
+    head = "mixed_surrogate" if "mixed_surrogate" in runs else forms[-1]
+    H = runs[head]
+    comp = {}
+    for f, r in runs.items():
+        if f == head:
+            continue
+        comp[f] = dict(value=r["value"], speedup_of_headline=H["value"] / r["value"],
+                       rel_l2_headline_vs_this={
+                           name: rel_l2(a, b) for name, a, b in
+                           zip(("phi", "c", "q"), H["fields"][:3], r["fields"][:3])},
+                       same_t=r["stats"]["t"] == H["stats"]["t"], counters=r["stats"])
     roof = kernel_roofline(p, state, n)
     log(f"[bench] roofline {roof}")
     cpu = None
@@ -204,30 +299,27 @@ def main():
         log("[bench] cpu baseline")
         cpu = cpu_baseline(args)
         log(f"[bench] cpu {cpu}")
-    same_t = abs(r64["stats"]["t"] - rmx["stats"]["t"]) <= 1e-12 * max(r64["stats"]["t"], 1e-300)
     out = {
-        "metric": "cells*BDF-steps/s at <=1e-6 L2 vs FP64; mixed vs plain FP64, 1 GPU",
-        "value": vmx,
+        "metric": "cells*BDF-steps/s at <=1e-6 L2 vs FP64; mixed+surrogate vs plain FP64, 1 GPU",
+        "value": H["value"],
         "unit": "cells*steps/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": rmx["wall"] / args.steps * 1e3,
+        "ms_per_step": H["wall"] / H["stats"]["nst"] * 1e3,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64 (FAC preconditioner f32)",
+        "dtype": "f64 (preconditioner f32, surrogate bf16)",
         "data": "synthetic",
         "config": {"workload": f"3D {n}^3 periodic CALPHAD-KKS polycrystal, 64 Voronoi "
                                f"grains, solid fraction 0.7, T=1500K, dt={args.dt}, "
-                               f"atol={args.atol}, maxl={args.maxl}",
-                   "cells": cells, "parallelism": "single GPU"},
-        "fp64_value": v64,
-        "speedup_mixed_vs_fp64": vmx / v64,
-        "rel_l2_mixed_vs_fp64": errs,
-        "same_step_sequence": same_t,
-        "counters_fp64": r64["stats"],
-        "counters_mixed": rmx["stats"],
+                               f"atol={args.atol}, maxl={args.maxl}, relaxed {args.relax} "
+                               f"steps",
+                   "cells": cells, "form": head, "parallelism": "single GPU"},
+        "counters": H["stats"],
+        "compared_forms": comp,
+        "surrogate": srep,
         "roofline": {"bound": "hbm", "achieved": roof["achieved"], "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": roof["achieved"] / HBM_PEAK_GBS,
                      "traffic": None, "kernel": roof["kernel"],

@@ -112,6 +112,11 @@ typedef struct ampe_model_params {
 
 enum ampe_precision { AMPE_FP64 = 0, AMPE_FAC_FP32 = 1 };
 enum ampe_kks_guess { AMPE_GUESS_PREVIOUS = 0, AMPE_GUESS_SURROGATE = 1 };
+/* Block solver inside each FAC cycle: the reference's single-level red-black
+ * coarse solve (EllipticFACOps "redblack", source/EllipticFACOps.cc:1366-1370)
+ * or a geometric multigrid V-cycle below the level (the role of hypre PFMG,
+ * the reference's default coarse solver). */
+enum ampe_precond { AMPE_PRECOND_REDBLACK = 0, AMPE_PRECOND_MG = 1 };
 
 typedef struct ampe_solver_params {
    double dt;             /* fixed BDF step (initial = max step) */
@@ -130,6 +135,9 @@ typedef struct ampe_solver_params {
    int quat_sweeps;       /* orientation block RBGS sweeps per cycle */
    int precision;         /* enum ampe_precision */
    int kks_guess;         /* enum ampe_kks_guess */
+   int precond;           /* enum ampe_precond */
+   int mg_max_levels;     /* multigrid levels including the fine one (<= 0: all) */
+   int mg_coarse_sweeps;  /* red-black sweeps on the coarsest multigrid level */
 } ampe_solver_params;
 
 /* Per-run counters (mirror QuatIntegrator.cc:2161-2170 nni/nli/nfe/npe/nps). */
@@ -270,6 +278,22 @@ int ampe_fac_smooth(const ampe_level* level, const ampe_elliptic_op* op,
 int ampe_fac_residual(const ampe_level* level, const ampe_elliptic_op* op,
                       const double* rhs, double* u, double* res, double* rms,
                       ampe_stream_t stream);
+
+/* ------------------------------------------------ level transfer (ratio 2) */
+/* coarse must hold the boxes of fine halved, patch by patch (even lower
+ * corners and extents).  Cell fields of ncomp components; side fields of one.
+ * Replace EllipticFACOps::restrictSolution / restrictResidual /
+ * prolongErrorAndCorrect (source/EllipticFACOps.cc:948-1041): SAMRAI
+ * CONSERVATIVE_COARSEN on cells (mean of the 2^d children) and faces (mean of
+ * the fine faces covering a coarse face), and refinement added to the fine
+ * field: order 0 = CONSTANT_REFINE (the reference default), order 1 =
+ * cell-centred tri/bilinear interpolation (needs filled coarse ghosts). */
+int ampe_coarsen_cell(const ampe_level* coarse, const ampe_level* fine, int ncomp,
+                      const double* f, double* c, ampe_stream_t stream);
+int ampe_coarsen_side(const ampe_level* coarse, const ampe_level* fine, const double* f,
+                      double* c, ampe_stream_t stream);
+int ampe_refine_add(const ampe_level* fine, const ampe_level* coarse, int ncomp, int order,
+                    const double* c, double* f, ampe_stream_t stream);
 
 /* ----------------------------------------------------- vector reductions */
 /* Dot product of two ncomp-component cell fields over interior cells
