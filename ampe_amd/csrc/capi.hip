@@ -174,11 +174,22 @@ int ampe_tag_cells(const ampe_level* level, const ampe_model_params* p, const do
 }
 
 int ampe_fac_phase_coeffs(const ampe_level* level, const ampe_model_params* p, double gamma,
-                          const double* phi, double* C, ampe_stream_t stream)
+                          const double* phi, const double* M, double* C,
+                          ampe_stream_t stream)
 {
    AMPE_REQUIRE(level && phi && C, "null argument");
    if (check_params(p, *level->L)) return -1;
-   return ampe::launch_phase_C(*level->L, *p, gamma, phi, C, hs(stream));
+   return ampe::launch_phase_C(*level->L, *p, gamma, phi, M, C, hs(stream));
+}
+
+int ampe_phase_mobility(const ampe_level* level, const ampe_model_params* p, const double* cl,
+                        const double* cs, double* M, ampe_stream_t stream)
+{
+   AMPE_REQUIRE(level && M, "null argument");
+   if (check_params(p, *level->L)) return -1;
+   AMPE_REQUIRE(p->phi_mobility_type != AMPE_PHI_MOB_KIM || (cl && cs),
+                "Kim mobility needs the phase concentrations");
+   return ampe::launch_phase_mobility(*level->L, *p, cl, cs, M, hs(stream));
 }
 
 static ampe::EllOp<double> to_op(const ampe_elliptic_op* op)
@@ -331,7 +342,7 @@ int ampe_refine_add(const ampe_level* fine, const ampe_level* coarse, int ncomp,
 }
 
 int ampe_sim_set_surrogate(ampe_sim* sim, const ampe_surrogate* s, int probe_its,
-                           double probe_tol)
+                           double probe_tol, double switch_tol)
 {
    AMPE_REQUIRE(sim, "null argument");
    AMPE_REQUIRE(probe_its >= 0, "probe_its must be >= 0");
@@ -343,6 +354,7 @@ int ampe_sim_set_surrogate(ampe_sim* sim, const ampe_surrogate* s, int probe_its
    sim->S->guess.surrogate = s ? s->S : nullptr;
    sim->S->guess.probe_its = probe_its;
    sim->S->guess.probe_tol = probe_tol;
+   sim->S->guess.switch_tol = switch_tol;
    return 0;
 }
 

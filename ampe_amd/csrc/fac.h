@@ -17,7 +17,7 @@ struct EllOp {
 };
 
 int launch_phase_C(const Level& L, const ampe_model_params& p, double gamma,
-                   const double* phi, double* C, hipStream_t s);
+                   const double* phi, const double* M, double* C, hipStream_t s);
 
 // One colour sweep; with want_max the per-tile max |res| lands in L.partial.
 template <typename T>

@@ -10,12 +10,13 @@ namespace ampe {
 // PhaseFACOps::setCOnPatchPrivate (source/PhaseFACOps.cc:101-183)
 __global__ void __launch_bounds__(256) k_phase_C(LevelArgs a, ampe_model_params p, double gamma,
                                                  const double* __restrict__ phi,
+                                                 const double* __restrict__ M,
                                                  double* __restrict__ C)
 {
    CellPos c;
    if (!cell_pos(a, 0, c)) return;
    double g2 = d2well(phi[c.c0], p.phi_well_type);
-   double gm = gamma * p.phi_mobility;
+   double gm = gamma * (M ? M[c.c0] : p.phi_mobility);
    C[c.c0] = 1.0 + gm * p.phi_well_scale * g2;
 }
 
@@ -114,18 +115,18 @@ __global__ void __launch_bounds__(256) k_rbgs(LevelArgs a, OpArgs<T> op, T dx0, 
                                               const T* __restrict__ rhs, T* __restrict__ u,
                                               int color, double* __restrict__ partial)
 {
-   CellPos c;
-   bool ok = cell_pos(a, 0, c);
    T absres = 0;
-   if (ok && is_color<NDIM>(c, color)) {
-      T dinv[3] = {T(1) / dx0, T(1) / dx1, T(1) / dx2};
-      T diag;
-      T res = op_residual<T, NDIM>(a, c, op, rhs, u, dinv, &diag);
-      T dudr = T(1) / diag;
-      T du = -res * dudr;
-      u[c.c0] = u[c.c0] + du * T(1);
-      absres = res < 0 ? -res : res;
-   }
+   AMPE_COLUMN_BEGIN(a, c)
+   if (!is_color<NDIM>(c, color)) continue;
+   T dinv[3] = {T(1) / dx0, T(1) / dx1, T(1) / dx2};
+   T diag;
+   T res = op_residual<T, NDIM>(a, c, op, rhs, u, dinv, &diag);
+   T dudr = T(1) / diag;
+   T du = -res * dudr;
+   u[c.c0] = u[c.c0] + du * T(1);
+   T ar = res < 0 ? -res : res;
+   absres = ar > absres ? ar : absres;
+   AMPE_COLUMN_END
    if (partial) block_store_max<T>(absres, partial);
 }
 
@@ -135,15 +136,14 @@ __global__ void __launch_bounds__(256) k_residual(LevelArgs a, OpArgs<T> op, T d
                                                   const T* __restrict__ rhs, const T* __restrict__ u,
                                                   T* __restrict__ res, double* __restrict__ partial)
 {
-   CellPos c;
    double sq = 0.0;
-   if (cell_pos(a, 0, c)) {
-      T dinv[3] = {T(1) / dx0, T(1) / dx1, T(1) / dx2};
-      T diag;
-      T r = op_residual<T, NDIM>(a, c, op, rhs, u, dinv, &diag);
-      res[c.c0] = r;
-      sq = (double)r * (double)r;
-   }
+   AMPE_COLUMN_BEGIN(a, c)
+   T dinv[3] = {T(1) / dx0, T(1) / dx1, T(1) / dx2};
+   T diag;
+   T r = op_residual<T, NDIM>(a, c, op, rhs, u, dinv, &diag);
+   res[c.c0] = r;
+   sq += (double)r * (double)r;
+   AMPE_COLUMN_END
    if (partial) block_store_sum<T>(sq, partial);
 }
 
@@ -188,8 +188,8 @@ __global__ void __launch_bounds__(256) k_quat_rbgs(LevelArgs a, const T* __restr
                                                    T h2, int ncomp, const T* __restrict__ rhs,
                                                    T* __restrict__ z, int color)
 {
-   CellPos c;
-   if (!cell_pos(a, 0, c) || !is_color<NDIM>(c, color)) return;
+   AMPE_COLUMN_BEGIN(a, c)
+   if (!is_color<NDIM>(c, color)) continue;
    T hinv2[3] = {T(1) / (h0 * h0), T(1) / (h1 * h1), T(1) / (h2 * h2)};
    for (int m = 0; m < ncomp; m++) {
       const T* rm = rhs + m * a.ncg;
@@ -199,6 +199,7 @@ __global__ void __launch_bounds__(256) k_quat_rbgs(LevelArgs a, const T* __restr
       T res = rm[c.c0] - az;
       zm[c.c0] = zm[c.c0] + res / center;
    }
+   AMPE_COLUMN_END
 }
 
 template <typename T, int NDIM>
@@ -209,18 +210,17 @@ __global__ void __launch_bounds__(256) k_quat_residual(LevelArgs a, const T* __r
                                                        const T* __restrict__ z, T* __restrict__ res,
                                                        double* __restrict__ partial)
 {
-   CellPos c;
    double sq = 0.0;
-   if (cell_pos(a, 0, c)) {
-      T hinv2[3] = {T(1) / (h0 * h0), T(1) / (h1 * h1), T(1) / (h2 * h2)};
-      for (int m = 0; m < ncomp; m++) {
-         T center;
-         T az = quat_row<T, NDIM>(a, c, fc, sm, gamma, hinv2, z + m * a.ncg, &center);
-         T r = rhs[m * a.ncg + c.c0] - az;
-         res[m * a.ncg + c.c0] = r;
-         sq += (double)r * (double)r;
-      }
+   AMPE_COLUMN_BEGIN(a, c)
+   T hinv2[3] = {T(1) / (h0 * h0), T(1) / (h1 * h1), T(1) / (h2 * h2)};
+   for (int m = 0; m < ncomp; m++) {
+      T center;
+      T az = quat_row<T, NDIM>(a, c, fc, sm, gamma, hinv2, z + m * a.ncg, &center);
+      T r = rhs[m * a.ncg + c.c0] - az;
+      res[m * a.ncg + c.c0] = r;
+      sq += (double)r * (double)r;
    }
+   AMPE_COLUMN_END
    if (partial) block_store_sum<T>(sq, partial);
 }
 
@@ -228,12 +228,21 @@ __global__ void __launch_bounds__(256) k_quat_residual(LevelArgs a, const T* __r
 static dim3 tblock() { return dim3(TX, TY); }
 
 int launch_phase_C(const Level& L, const ampe_model_params& p, double gamma,
-                   const double* phi, double* C, hipStream_t s)
+                   const double* phi, const double* M, double* C, hipStream_t s)
 {
    hipLaunchKernelGGL(k_phase_C, dim3(L.ncell_tiles), tblock(), 0, s, level_args(L), p, gamma,
-                      phi, C);
+                      phi, M, C);
    AMPE_HIP_CHECK(hipGetLastError());
    return 0;
+}
+
+// Launch shape: column tiles (KZ planes per workgroup), or one-plane tiles
+// when a per-tile reduction is requested (its callers reduce ncell_tiles
+// partials).
+static void shape(const Level& L, bool reduce, LevelArgs* a, int* grid)
+{
+   *a = reduce ? level_args(L) : level_args_col(L);
+   *grid = reduce ? L.ncell_tiles : L.ncol_tiles;
 }
 
 template <typename T>
@@ -242,12 +251,15 @@ int launch_rbgs(const Level& L, const EllOp<T>& op, const double* dx, const T* r
 {
    OpArgs<T> oa{op.D, op.Dc, op.C, op.Cc, op.M, op.Mc};
    double* part = want_max ? L.partial : nullptr;
+   LevelArgs la;
+   int grid;
+   shape(L, want_max, &la, &grid);
    if (L.ndim == 3)
-      hipLaunchKernelGGL((k_rbgs<T, 3>), dim3(L.ncell_tiles), tblock(), 0, s, level_args(L), oa,
-                         (T)dx[0], (T)dx[1], (T)dx[2], rhs, u, color, part);
+      hipLaunchKernelGGL((k_rbgs<T, 3>), dim3(grid), tblock(), 0, s, la, oa, (T)dx[0],
+                         (T)dx[1], (T)dx[2], rhs, u, color, part);
    else
-      hipLaunchKernelGGL((k_rbgs<T, 2>), dim3(L.ncell_tiles), tblock(), 0, s, level_args(L), oa,
-                         (T)dx[0], (T)dx[1], (T)dx[2], rhs, u, color, part);
+      hipLaunchKernelGGL((k_rbgs<T, 2>), dim3(grid), tblock(), 0, s, la, oa, (T)dx[0],
+                         (T)dx[1], (T)dx[2], rhs, u, color, part);
    AMPE_HIP_CHECK(hipGetLastError());
    return 0;
 }
@@ -257,12 +269,15 @@ int launch_residual(const Level& L, const EllOp<T>& op, const double* dx, const 
                     const T* u, T* res, double* part, hipStream_t s)
 {
    OpArgs<T> oa{op.D, op.Dc, op.C, op.Cc, op.M, op.Mc};
+   LevelArgs la;
+   int grid;
+   shape(L, part != nullptr, &la, &grid);
    if (L.ndim == 3)
-      hipLaunchKernelGGL((k_residual<T, 3>), dim3(L.ncell_tiles), tblock(), 0, s, level_args(L),
-                         oa, (T)dx[0], (T)dx[1], (T)dx[2], rhs, u, res, part);
+      hipLaunchKernelGGL((k_residual<T, 3>), dim3(grid), tblock(), 0, s, la, oa, (T)dx[0],
+                         (T)dx[1], (T)dx[2], rhs, u, res, part);
    else
-      hipLaunchKernelGGL((k_residual<T, 2>), dim3(L.ncell_tiles), tblock(), 0, s, level_args(L),
-                         oa, (T)dx[0], (T)dx[1], (T)dx[2], rhs, u, res, part);
+      hipLaunchKernelGGL((k_residual<T, 2>), dim3(grid), tblock(), 0, s, la, oa, (T)dx[0],
+                         (T)dx[1], (T)dx[2], rhs, u, res, part);
    AMPE_HIP_CHECK(hipGetLastError());
    return 0;
 }
@@ -271,14 +286,15 @@ template <typename T>
 int launch_quat_rbgs(const Level& L, const T* fc, const T* sm, double gamma, const double* dx,
                      int ncomp, const T* rhs, T* z, int color, hipStream_t s)
 {
+   LevelArgs la;
+   int grid;
+   shape(L, false, &la, &grid);
    if (L.ndim == 3)
-      hipLaunchKernelGGL((k_quat_rbgs<T, 3>), dim3(L.ncell_tiles), tblock(), 0, s,
-                         level_args(L), fc, sm, (T)gamma, (T)dx[0], (T)dx[1], (T)dx[2], ncomp,
-                         rhs, z, color);
+      hipLaunchKernelGGL((k_quat_rbgs<T, 3>), dim3(grid), tblock(), 0, s, la, fc, sm, (T)gamma,
+                         (T)dx[0], (T)dx[1], (T)dx[2], ncomp, rhs, z, color);
    else
-      hipLaunchKernelGGL((k_quat_rbgs<T, 2>), dim3(L.ncell_tiles), tblock(), 0, s,
-                         level_args(L), fc, sm, (T)gamma, (T)dx[0], (T)dx[1], (T)dx[2], ncomp,
-                         rhs, z, color);
+      hipLaunchKernelGGL((k_quat_rbgs<T, 2>), dim3(grid), tblock(), 0, s, la, fc, sm, (T)gamma,
+                         (T)dx[0], (T)dx[1], (T)dx[2], ncomp, rhs, z, color);
    AMPE_HIP_CHECK(hipGetLastError());
    return 0;
 }
@@ -288,14 +304,15 @@ int launch_quat_residual(const Level& L, const T* fc, const T* sm, double gamma,
                          const double* dx, int ncomp, const T* rhs, const T* z, T* res,
                          double* part, hipStream_t s)
 {
+   LevelArgs la;
+   int grid;
+   shape(L, part != nullptr, &la, &grid);
    if (L.ndim == 3)
-      hipLaunchKernelGGL((k_quat_residual<T, 3>), dim3(L.ncell_tiles), tblock(), 0, s,
-                         level_args(L), fc, sm, (T)gamma, (T)dx[0], (T)dx[1], (T)dx[2], ncomp,
-                         rhs, z, res, part);
+      hipLaunchKernelGGL((k_quat_residual<T, 3>), dim3(grid), tblock(), 0, s, la, fc, sm,
+                         (T)gamma, (T)dx[0], (T)dx[1], (T)dx[2], ncomp, rhs, z, res, part);
    else
-      hipLaunchKernelGGL((k_quat_residual<T, 2>), dim3(L.ncell_tiles), tblock(), 0, s,
-                         level_args(L), fc, sm, (T)gamma, (T)dx[0], (T)dx[1], (T)dx[2], ncomp,
-                         rhs, z, res, part);
+      hipLaunchKernelGGL((k_quat_residual<T, 2>), dim3(grid), tblock(), 0, s, la, fc, sm,
+                         (T)gamma, (T)dx[0], (T)dx[1], (T)dx[2], ncomp, rhs, z, res, part);
    AMPE_HIP_CHECK(hipGetLastError());
    return 0;
 }

@@ -14,17 +14,29 @@ struct LevelArgs {
    int g, gz;
    long long ncg;
    long long sdo[3];  // direction-block offsets inside a side field
+   int nz_tile;       // planes per tile (1, or KZ for column tiles)
 };
+
 
 inline LevelArgs level_args(const Level& L, bool faces = false)
 {
    LevelArgs a;
    a.P = L.dp;
    a.T = faces ? L.face_tiles : L.cell_tiles;
+   a.nz_tile = 1;
    a.g = L.g;
    a.gz = L.gz;
    a.ncg = L.ncg;
    for (int d = 0; d < 3; d++) a.sdo[d] = L.side_dir_off[d];
+   return a;
+}
+
+// Column tile list (KZ planes per workgroup in 3-D).
+inline LevelArgs level_args_col(const Level& L)
+{
+   LevelArgs a = level_args(L);
+   a.T = L.col_tiles;
+   a.nz_tile = L.ndim == 3 ? KZ : 1;
    return a;
 }
 
@@ -52,6 +64,38 @@ __device__ __forceinline__ bool cell_pos(const LevelArgs& a, int extend, CellPos
           (long long)(c.k + a.gz) * c.pd.sz;
    return true;
 }
+
+// Cell of this thread at plane t.k + kk of a column tile; false when outside
+// the box (i, j or k).
+__device__ __forceinline__ bool cell_pos_col(const LevelArgs& a, int kk, CellPos& c)
+{
+   const Tile t = a.T[blockIdx.x];
+   c.pd = a.P[t.patch];
+   c.i = t.i0 + (int)threadIdx.x;
+   c.j = t.j0 + (int)threadIdx.y;
+   c.k = t.k + kk;
+   if (c.i >= c.pd.nx || c.j >= c.pd.ny || c.k >= c.pd.nz) return false;
+   c.st[0] = 1;
+   c.st[1] = c.pd.sy;
+   c.st[2] = c.pd.sz;
+   c.c0 = c.pd.cell_off + (c.i + a.g) + (long long)(c.j + a.g) * c.pd.sy +
+          (long long)(c.k + a.gz) * c.pd.sz;
+   return true;
+}
+
+// Per-plane loop of a column kernel: the body runs once per plane of the
+// thread's column, with `c` the cell; `continue` inside the body moves to
+// the next plane.
+#define AMPE_COLUMN_BEGIN(a, c)                      \
+   for (int kk_ = 0; kk_ < (a).nz_tile; kk_++) {     \
+      CellPos c;                                     \
+      if (!cell_pos_col((a), kk_, c)) break;         \
+      do {
+#define AMPE_COLUMN_END \
+   }                    \
+   while (0)            \
+      ;                 \
+   }
 
 // Index of face (i,j,k) of direction d of patch pd in a side field.
 __device__ __forceinline__ long long face_idx(const LevelArgs& a, const PatchDev& pd,
@@ -83,6 +127,8 @@ int launch_conc_coeffs(const Level& L, const ampe_model_params& p, const double*
 int launch_conc_rhs(const Level& L, const ampe_model_params& p, const double* c,
                     const double* phi, const double* Dc, const double* Dphi, double* rhs,
                     hipStream_t s);
+int launch_phase_mobility(const Level& L, const ampe_model_params& p, const double* cl,
+                          const double* cs, double* M, hipStream_t s);
 int launch_tag(const Level& L, const ampe_model_params& p, const double* phi, double tol,
                int* tags, hipStream_t s);
 int launch_quat_face_coef(const Level& L, const ampe_model_params& p, const double* phi,

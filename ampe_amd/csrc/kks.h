@@ -8,20 +8,22 @@
 namespace ampe {
 
 // Initial-guess policy of the integrator's KKS: surrogate == nullptr starts
-// every cell from (clr, csr) (the previous step's concentrations); otherwise
-// from the surrogate, restarting from (clr, csr) when the residual after
-// probe_its updates exceeds probe_tol.
+// every cell from (clr, csr) (the previous step's concentrations); otherwise a
+// cell whose residual at (clr, csr) exceeds switch_tol starts from the
+// surrogate, restarting from (clr, csr) when the residual after probe_its
+// updates exceeds probe_tol.
 struct KksGuess {
    const Surrogate* surrogate = nullptr;
    int probe_its = 3;
    double probe_tol = 1.0e-8;
+   double switch_tol = 1.0e-2;  // previous-guess residual above which the surrogate starts
 };
 
 // KKS over all ghosted cells of a level.  launch_kks accumulates into device
-// counters dev_acc[5] = {total iterations, max iterations, failure flag,
-// calls, fallbacks} without synchronising; launch_kks_sync returns max
+// counters dev_acc[6] = {total iterations, max iterations, failure flag,
+// calls, fallbacks, surrogate starts} without synchronising; launch_kks_sync returns max
 // iterations (or -1 on a non-converged cell) and the total, synchronising s.
-constexpr int kKksAcc = 5;
+constexpr int kKksAcc = 6;
 int launch_kks(const Level& L, const ampe_model_params& p, const double* phi,
                const double* c, const double* clr, const double* csr, double* cl,
                double* cs, const KksGuess& guess, long long* dev_acc, hipStream_t s);

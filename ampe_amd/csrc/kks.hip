@@ -11,7 +11,7 @@
 // (tests/testCALPHADbinaryKKS.cc:123-127) or the surrogate.
 //
 // Each workgroup writes {sum of updates, max updates or failure marker,
-// fallbacks} to its partial slot; a one-workgroup finish kernel folds them
+// fallbacks, failures, surrogate starts} to its partial slot; a one-workgroup finish kernel folds them
 // into device counters in a fixed order (no same-address atomics, no host
 // synchronisation per call).
 #include <algorithm>
@@ -22,40 +22,45 @@
 namespace ampe {
 
 constexpr double kKksFail = 1.0e9;
-constexpr int kKksPart = 4;  // doubles per workgroup
+constexpr int kKksPart = 5;  // doubles per workgroup
 
-// it: updates, mx: updates or kKksFail, fb: fallback (0/1), nf: failure (0/1)
-__device__ inline void kks_block_partial(double it, double mx, double fb, double nf,
+// it: updates, mx: updates or kKksFail, fb: fallback (0/1), nf: failure
+// (0/1), ss: surrogate start (0/1)
+__device__ inline void kks_block_partial(double it, double mx, double fb, double nf, double ss,
                                          double* __restrict__ partial)
 {
    for (int o = 32; o > 0; o >>= 1) {
       it += __shfl_xor(it, o);
       fb += __shfl_xor(fb, o);
       nf += __shfl_xor(nf, o);
+      ss += __shfl_xor(ss, o);
       double v = __shfl_xor(mx, o);
       mx = v > mx ? v : mx;
    }
-   __shared__ double sh[16];
+   __shared__ double sh[4 * kKksPart];
    int w = threadIdx.x >> 6;
    if ((threadIdx.x & 63) == 0) {
-      sh[4 * w] = it;
-      sh[4 * w + 1] = mx;
-      sh[4 * w + 2] = fb;
-      sh[4 * w + 3] = nf;
+      sh[kKksPart * w] = it;
+      sh[kKksPart * w + 1] = mx;
+      sh[kKksPart * w + 2] = fb;
+      sh[kKksPart * w + 3] = nf;
+      sh[kKksPart * w + 4] = ss;
    }
    __syncthreads();
    if (threadIdx.x == 0) {
-      double s = 0.0, m = 0.0, f = 0.0, e = 0.0;
+      double s = 0.0, m = 0.0, f = 0.0, e = 0.0, g = 0.0;
       for (int k = 0; k < 4; k++) {
-         s += sh[4 * k];
-         m = sh[4 * k + 1] > m ? sh[4 * k + 1] : m;
-         f += sh[4 * k + 2];
-         e += sh[4 * k + 3];
+         s += sh[kKksPart * k];
+         m = sh[kKksPart * k + 1] > m ? sh[kKksPart * k + 1] : m;
+         f += sh[kKksPart * k + 2];
+         e += sh[kKksPart * k + 3];
+         g += sh[kKksPart * k + 4];
       }
       partial[kKksPart * blockIdx.x] = s;
       partial[kKksPart * blockIdx.x + 1] = m;
       partial[kKksPart * blockIdx.x + 2] = f;
       partial[kKksPart * blockIdx.x + 3] = e;
+      partial[kKksPart * blockIdx.x + 4] = g;
    }
 }
 
@@ -76,7 +81,7 @@ __device__ inline void kks_surrogate_guess(const MlpDev& m, const u16x8* sf, con
 
 template <bool SURR>
 __global__ void __launch_bounds__(256) k_kks(long long n, ampe_model_params p, MlpDev m,
-                                             int probe_its, double probe_tol,
+                                             int probe_its, double probe_tol, double switch_tol,
                                              const double* __restrict__ phi,
                                              const double* __restrict__ c,
                                              const double* __restrict__ clr,
@@ -94,24 +99,40 @@ __global__ void __launch_bounds__(256) k_kks(long long n, ampe_model_params p, M
    long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
    bool in = t < n;
    double ct = in ? c[t] : 0.5, pt = in ? phi[t] : 0.0;
-   double it = 0.0, mx = 0.0, fb = 0.0;
-   if (SURR && p.fe_model != AMPE_FE_QUADRATIC) {
-      double gl, gs;
-      kks_surrogate_guess(m, sf, sb, ct, pt, p.temperature, gl, gs);
-      if (in) {
-         double h = interp(pt, p.conc_interp);
-         int r = kks_newton(p.calphad, p.temperature, h, ct, gl, gs, p.newton_max_its,
-                            p.newton_tol, probe_its, probe_tol);
-         if (r == -2) {
-            fb = 1.0;
-            gl = clr[t];
-            gs = csr[t];
-            int r2 = kks_newton(p.calphad, p.temperature, h, ct, gl, gs, p.newton_max_its,
-                                p.newton_tol, -1, 0.0);
-            r = r2 < 0 ? -1 : r2 + probe_its;
+   double it = 0.0, mx = 0.0, fb = 0.0, ss = 0.0;
+   if (SURR && p.fe_model == AMPE_FE_CALPHAD_BINARY) {
+      const double T = p.temperature;
+      double h = interp(pt, p.conc_interp);
+      double xl = in ? clr[t] : 0.5, xs = in ? csr[t] : 0.5;
+      kks_clamp(xl, xs);
+      double f1, f2;
+      kks_residual(p.calphad, T, 1.0 / (kRGas * T), h, ct, xl, xs, f1, f2);
+      bool bad = in && !(dmax(fabs(f1), fabs(f2)) <= switch_tol);
+      int r = 0;
+      if (__any(bad)) {
+         double gl, gs;
+         kks_surrogate_guess(m, sf, sb, ct, pt, T, gl, gs);
+         if (bad) {
+            ss = 1.0;
+            r = kks_newton(p.calphad, T, h, ct, gl, gs, p.newton_max_its, p.newton_tol,
+                           probe_its, probe_tol);
+            if (r == -2) {
+               fb = 1.0;
+               int r2 = kks_newton_from(p.calphad, T, h, ct, xl, xs, f1, f2, p.newton_max_its,
+                                        p.newton_tol, -1, 0.0);
+               r = r2 < 0 ? -1 : r2 + probe_its;
+            } else {
+               xl = gl;
+               xs = gs;
+            }
          }
-         cl[t] = gl;
-         cs[t] = gs;
+      }
+      if (in && !bad)
+         r = kks_newton_from(p.calphad, T, h, ct, xl, xs, f1, f2, p.newton_max_its,
+                             p.newton_tol, -1, 0.0);
+      if (in) {
+         cl[t] = xl;
+         cs[t] = xs;
          mx = r < 0 ? kKksFail : (double)r;
          it = r < 0 ? 0.0 : (double)r;
       }
@@ -123,24 +144,27 @@ __global__ void __launch_bounds__(256) k_kks(long long n, ampe_model_params p, M
       mx = r < 0 ? kKksFail : (double)r;
       it = r < 0 ? 0.0 : (double)r;
    }
-   kks_block_partial(it, mx, fb, mx >= kKksFail ? 1.0 : 0.0, partial);
+   kks_block_partial(it, mx, fb, mx >= kKksFail ? 1.0 : 0.0, ss, partial);
 }
 
 // acc[0] += updates, acc[1] = max(acc[1], max updates), acc[2] |= failure,
-// acc[3] += 1 (calls), acc[4] += fallbacks.  One workgroup.
+// acc[3] += 1 (calls), acc[4] += fallbacks, acc[5] += surrogate starts.
+// One workgroup.
 __global__ void __launch_bounds__(1024) k_kks_finish(const double* __restrict__ partial, int nb,
                                                      long long* __restrict__ acc)
 {
-   __shared__ double ss[1024], sm[1024], sf[1024];
-   double s = 0.0, m = 0.0, f = 0.0;
+   __shared__ double ss[1024], sm[1024], sf[1024], sg[1024];
+   double s = 0.0, m = 0.0, f = 0.0, g = 0.0;
    for (int i = threadIdx.x; i < nb; i += 1024) {
       s += partial[kKksPart * i];
       m = partial[kKksPart * i + 1] > m ? partial[kKksPart * i + 1] : m;
       f += partial[kKksPart * i + 2];
+      g += partial[kKksPart * i + 4];
    }
    ss[threadIdx.x] = s;
    sm[threadIdx.x] = m;
    sf[threadIdx.x] = f;
+   sg[threadIdx.x] = g;
    __syncthreads();
    for (int w = 512; w > 0; w >>= 1) {
       if (threadIdx.x < w) {
@@ -148,6 +172,7 @@ __global__ void __launch_bounds__(1024) k_kks_finish(const double* __restrict__ 
          sm[threadIdx.x] = sm[threadIdx.x + w] > sm[threadIdx.x] ? sm[threadIdx.x + w]
                                                                  : sm[threadIdx.x];
          sf[threadIdx.x] += sf[threadIdx.x + w];
+         sg[threadIdx.x] += sg[threadIdx.x + w];
       }
       __syncthreads();
    }
@@ -159,6 +184,7 @@ __global__ void __launch_bounds__(1024) k_kks_finish(const double* __restrict__ 
       if (fail) acc[2] = 1;
       acc[3] += 1;
       acc[4] += (long long)sf[0];
+      acc[5] += (long long)sg[0];
    }
 }
 
@@ -194,10 +220,11 @@ int launch_kks(const Level& L, const ampe_model_params& p, const double* phi,
       }
       m = S.dev();
       hipLaunchKernelGGL(k_kks<true>, dim3(nb), dim3(256), 0, s, L.ncg, p, m, guess.probe_its,
-                         guess.probe_tol, phi, c, clr, csr, cl, cs, L.kks_part);
+                         guess.probe_tol, guess.switch_tol, phi, c, clr, csr, cl, cs,
+                         L.kks_part);
    } else {
-      hipLaunchKernelGGL(k_kks<false>, dim3(nb), dim3(256), 0, s, L.ncg, p, m, -1, 0.0, phi, c,
-                         clr, csr, cl, cs, L.kks_part);
+      hipLaunchKernelGGL(k_kks<false>, dim3(nb), dim3(256), 0, s, L.ncg, p, m, -1, 0.0, 0.0,
+                         phi, c, clr, csr, cl, cs, L.kks_part);
    }
    AMPE_HIP_CHECK(hipGetLastError());
    hipLaunchKernelGGL(k_kks_finish, dim3(1), dim3(1024), 0, s, L.kks_part, (int)nb, dev_acc);
@@ -272,7 +299,7 @@ __global__ void __launch_bounds__(256) k_kks_batch(long long n, ampe_model_param
       mx = r < 0 ? kKksFail : (double)r;
       it = r < 0 ? 0.0 : (double)r;
    }
-   kks_block_partial(it, mx, fb, mx >= kKksFail ? 1.0 : 0.0, partial);
+   kks_block_partial(it, mx, fb, mx >= kKksFail ? 1.0 : 0.0, (SURR && in) ? 1.0 : 0.0, partial);
 }
 
 __global__ void k_kks_batch_finish(const double* __restrict__ partial, int nb,

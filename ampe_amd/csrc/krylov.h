@@ -28,6 +28,15 @@ int kry_lincomb(const Level& L, int nc, const VecList& vl, const double* ewt, do
                 hipStream_t s);
 int kry_newton_rhs(const Level& L, int nc, double gamma, const double* f, const double* y,
                    const double* ah, double* b, hipStream_t s);
+// Classical Gram-Schmidt (at most kMaxCgs dots per pass, so maxl <= 14).
+constexpr int kMaxCgs = 16;
+// out = ewt * work (ewt != null) or out as is; sums[k] = out . vl.v[k]
+// (k < vl.n) and sums[vl.n] = out . out, one pass and one synchronisation.
+int kry_cgs_dots(const Level& L, int nc, const double* ewt, const double* work, double* out,
+                 const VecList& vl, double* sums, hipStream_t s);
+// w = w + sum_k vl.y[k] vl.v[k] (k in order); *sumsq = w . w.
+int kry_cgs_update(const Level& L, int nc, double* w, const VecList& vl, double* sumsq,
+                   hipStream_t s);
 int kry_newton_update(const Level& L, int nc, const double* del, const double* ewt, double* y,
                       double* sumsq, hipStream_t s);
 

@@ -164,6 +164,74 @@ __global__ void __launch_bounds__(256) k_newton_update(LevelArgs a, int nc,
    tile_store_sum(acc, partial);
 }
 
+// Per-tile sums of n values (n <= kMaxCgs) to partial[k * ntiles + tile].
+__device__ __forceinline__ void tile_store_sums(const double* acc, int n, int ntiles,
+                                                double* __restrict__ partial)
+{
+   __shared__ double sh[4][kMaxCgs];
+   int t = threadIdx.x + TX * threadIdx.y;
+#pragma unroll
+   for (int k = 0; k < kMaxCgs; k++) {
+      if (k >= n) break;
+      double v = acc[k];
+      for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+      if ((t & 63) == 0) sh[t >> 6][k] = v;
+   }
+   __syncthreads();
+   if (t < n) partial[(long long)t * ntiles + blockIdx.x] = (sh[0][t] + sh[1][t]) + (sh[2][t] + sh[3][t]);
+}
+
+__global__ void __launch_bounds__(256) k_cgs_dots(LevelArgs a, int nc, int ntiles,
+                                                  const double* __restrict__ ewt,
+                                                  const double* __restrict__ work, double* out,
+                                                  VecList vl, double* __restrict__ partial)
+{
+   CellPos c;
+   double acc[kMaxCgs];
+#pragma unroll
+   for (int k = 0; k < kMaxCgs; k++) acc[k] = 0.0;
+   const int n = vl.n;
+   if (cell_pos(a, 0, c)) {
+      for (int m = 0; m < nc; m++) {
+         long long i = m * a.ncg + c.c0;
+         double v;
+         if (ewt) {
+            v = ewt[i] * work[i];
+            out[i] = v;
+         } else {
+            v = out[i];
+         }
+#pragma unroll
+         for (int k = 0; k < kMaxCgs - 1; k++)
+            if (k < n) acc[k] += v * vl.v[k][i];
+#pragma unroll
+         for (int k = 0; k < kMaxCgs; k++)
+            if (k == n) acc[k] += v * v;
+      }
+   }
+   tile_store_sums(acc, n + 1, ntiles, partial);
+}
+
+__global__ void __launch_bounds__(256) k_cgs_update(LevelArgs a, int nc, double* __restrict__ w,
+                                                    VecList vl, double* __restrict__ partial)
+{
+   CellPos c;
+   double acc = 0.0;
+   const int n = vl.n;
+   if (cell_pos(a, 0, c)) {
+      for (int m = 0; m < nc; m++) {
+         long long i = m * a.ncg + c.c0;
+         double v = w[i];
+#pragma unroll
+         for (int k = 0; k < kMaxCgs; k++)
+            if (k < n) v = v + vl.y[k] * vl.v[k][i];
+         w[i] = v;
+         acc += v * v;
+      }
+   }
+   tile_store_sum(acc, partial);
+}
+
 #define AMPE_TILE_LAUNCH(KER, ...)                                                        \
    do {                                                                                   \
       hipLaunchKernelGGL(KER, dim3(L.ncell_tiles), dim3(TX, TY), 0, s, level_args(L),      \
@@ -218,6 +286,24 @@ int kry_newton_rhs(const Level& L, int nc, double gamma, const double* f, const 
 {
    AMPE_TILE_LAUNCH(k_newton_rhs, nc, gamma, f, y, ah, b);
    return 0;
+}
+
+int kry_cgs_dots(const Level& L, int nc, const double* ewt, const double* work, double* out,
+                 const VecList& vl, double* sums, hipStream_t s)
+{
+   if (vl.n + 1 > kMaxCgs) {
+      set_error("kry_cgs_dots: too many vectors");
+      return -1;
+   }
+   AMPE_TILE_LAUNCH(k_cgs_dots, nc, L.ncell_tiles, ewt, work, out, vl, L.partial);
+   return reduce_regions(L, L.ncell_tiles, vl.n + 1, sums, s);
+}
+
+int kry_cgs_update(const Level& L, int nc, double* w, const VecList& vl, double* sumsq,
+                   hipStream_t s)
+{
+   AMPE_TILE_LAUNCH(k_cgs_update, nc, w, vl, L.partial);
+   return reduce_partials(L, L.ncell_tiles, 1, 0, sumsq, s);
 }
 
 int kry_newton_update(const Level& L, int nc, const double* del, const double* ewt, double* y,

@@ -26,6 +26,10 @@ void set_error(const std::string& msg);
 // Tile of cells processed by one workgroup: TX x TY cells of one (patch, k).
 constexpr int TX = 64;
 constexpr int TY = 4;
+// Planes per column tile: a workgroup of a column kernel walks KZ consecutive
+// k planes, so the k-neighbour loads of a plane are the previous plane's
+// centre loads (cache hits) instead of HBM reads.
+constexpr int KZ = 1;
 
 struct PatchDev {
    int nx, ny, nz;         // interior extents (nz = 1 in 2-D)
@@ -56,14 +60,16 @@ struct Level {
    int ncell_tiles = 0;
    Tile* face_tiles = nullptr;  // cells grown by one in each direction
    int nface_tiles = 0;
+   Tile* col_tiles = nullptr;   // interior cells in columns of KZ planes (3-D)
+   int ncol_tiles = 0;
    long long* ghost_dst = nullptr;  // ghost gather (periodic / neighbours)
    long long* ghost_src = nullptr;
    long long nghost = 0;
-   double* partial = nullptr;  // per-workgroup reduction partials (8 per tile)
+   double* partial = nullptr;  // per-workgroup reduction partials (16 per tile)
    double* dred = nullptr;     // reduction results (16 doubles)
    double* hpinned = nullptr;  // pinned host scalars
    int* iscratch = nullptr;    // device integer flags
-   double* kks_part = nullptr; // KKS per-workgroup partials (4 per 256 ghosted cells)
+   double* kks_part = nullptr; // KKS per-workgroup partials (8 per 256 ghosted cells)
 };
 
 int level_create(int ndim, const ampe_box* boxes, int npatch, int ghost,

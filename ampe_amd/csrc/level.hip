@@ -92,12 +92,15 @@ int level_create(int ndim, const ampe_box* boxes, int npatch, int ghost,
    L->nside_tot = soff[0] + soff[1] + soff[2];
 
    // work lists
-   std::vector<Tile> ct, ft;
+   std::vector<Tile> ct, ft, zt;
    for (int p = 0; p < npatch; p++) {
       const PatchDev& d = L->hp[p];
       for (int k = 0; k < d.nz; k++)
          for (int j0 = 0; j0 < d.ny; j0 += TY)
             for (int i0 = 0; i0 < d.nx; i0 += TX) ct.push_back({p, i0, j0, k});
+      for (int k = 0; k < d.nz; k += (ndim == 3 ? KZ : 1))
+         for (int j0 = 0; j0 < d.ny; j0 += TY)
+            for (int i0 = 0; i0 < d.nx; i0 += TX) zt.push_back({p, i0, j0, k});
       int fz = (ndim == 3) ? d.nz + 1 : 1;
       for (int k = 0; k < fz; k++)
          for (int j0 = 0; j0 < d.ny + 1; j0 += TY)
@@ -105,6 +108,7 @@ int level_create(int ndim, const ampe_box* boxes, int npatch, int ghost,
    }
    L->ncell_tiles = (int)ct.size();
    L->nface_tiles = (int)ft.size();
+   L->ncol_tiles = (int)zt.size();
 
    // ghost gather: each ghost cell copies the interior cell it images
    std::vector<long long> gd, gs;
@@ -156,22 +160,25 @@ int level_create(int ndim, const ampe_box* boxes, int npatch, int ghost,
    e = hipMalloc(&L->dp, sizeof(PatchDev) * npatch);
    if (e == hipSuccess) e = hipMalloc(&L->cell_tiles, sizeof(Tile) * ct.size());
    if (e == hipSuccess) e = hipMalloc(&L->face_tiles, sizeof(Tile) * ft.size());
+   if (e == hipSuccess) e = hipMalloc(&L->col_tiles, sizeof(Tile) * zt.size());
    if (e == hipSuccess && L->nghost > 0) {
       e = hipMalloc(&L->ghost_dst, sizeof(long long) * gd.size());
       if (e == hipSuccess) e = hipMalloc(&L->ghost_src, sizeof(long long) * gs.size());
    }
-   if (e == hipSuccess) e = hipMalloc(&L->partial, sizeof(double) * (size_t)maxparts * 8);
+   if (e == hipSuccess) e = hipMalloc(&L->partial, sizeof(double) * (size_t)maxparts * 16);
    if (e == hipSuccess) e = hipMalloc(&L->dred, sizeof(double) * 16);
    if (e == hipSuccess) e = hipHostMalloc(&L->hpinned, sizeof(double) * 64);
    if (e == hipSuccess) e = hipMalloc(&L->iscratch, sizeof(int) * 64);
    if (e == hipSuccess)
-      e = hipMalloc(&L->kks_part, sizeof(double) * 4 * (size_t)((L->ncg + 255) / 256 + 1));
+      e = hipMalloc(&L->kks_part, sizeof(double) * 8 * (size_t)((L->ncg + 255) / 256 + 1));
    if (e == hipSuccess)
       e = hipMemcpy(L->dp, L->hp.data(), sizeof(PatchDev) * npatch, hipMemcpyHostToDevice);
    if (e == hipSuccess)
       e = hipMemcpy(L->cell_tiles, ct.data(), sizeof(Tile) * ct.size(), hipMemcpyHostToDevice);
    if (e == hipSuccess)
       e = hipMemcpy(L->face_tiles, ft.data(), sizeof(Tile) * ft.size(), hipMemcpyHostToDevice);
+   if (e == hipSuccess)
+      e = hipMemcpy(L->col_tiles, zt.data(), sizeof(Tile) * zt.size(), hipMemcpyHostToDevice);
    if (e == hipSuccess && L->nghost > 0) {
       e = hipMemcpy(L->ghost_dst, gd.data(), sizeof(long long) * gd.size(), hipMemcpyHostToDevice);
       if (e == hipSuccess)
@@ -192,6 +199,7 @@ void level_destroy(Level* L)
    (void)hipFree(L->dp);
    (void)hipFree(L->cell_tiles);
    (void)hipFree(L->face_tiles);
+   (void)hipFree(L->col_tiles);
    (void)hipFree(L->ghost_dst);
    (void)hipFree(L->ghost_src);
    (void)hipFree(L->partial);
@@ -261,6 +269,24 @@ __global__ void __launch_bounds__(1024) k_reduce_partials(const double* __restri
    }
 }
 
+// One workgroup per region r: the sum of part[r * nparts .. + nparts) in the
+// fixed order of k_reduce_partials.
+__global__ void __launch_bounds__(1024) k_reduce_regions(const double* __restrict__ part,
+                                                         int nparts, double* __restrict__ out)
+{
+   __shared__ double sh[1024];
+   const double* p = part + (long long)blockIdx.x * nparts;
+   double acc = 0.0;
+   for (int i = threadIdx.x; i < nparts; i += 1024) acc = acc + p[i];
+   sh[threadIdx.x] = acc;
+   __syncthreads();
+   for (int w = 512; w > 0; w >>= 1) {
+      if (threadIdx.x < w) sh[threadIdx.x] = sh[threadIdx.x] + sh[threadIdx.x + w];
+      __syncthreads();
+   }
+   if (threadIdx.x == 0) out[blockIdx.x] = sh[0];
+}
+
 int reduce_partials(const Level& L, int nparts, int ncol, int op,
                     double* host_out, hipStream_t s)
 {
@@ -281,11 +307,9 @@ int reduce_regions(const Level& L, int nparts, int nreg, double* host_out, hipSt
       set_error("reduce_regions: at most 16 regions");
       return -1;
    }
-   for (int r = 0; r < nreg; r++) {
-      hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(1024), 0, s,
-                         L.partial + (size_t)r * nparts, nparts, 1, 0, L.dred + r);
-      AMPE_HIP_CHECK(hipGetLastError());
-   }
+   hipLaunchKernelGGL(k_reduce_regions, dim3(nreg), dim3(1024), 0, s, L.partial, nparts,
+                      L.dred);
+   AMPE_HIP_CHECK(hipGetLastError());
    AMPE_HIP_CHECK(hipMemcpyAsync(L.hpinned, L.dred, sizeof(double) * nreg,
                                  hipMemcpyDeviceToHost, s));
    AMPE_HIP_CHECK(hipStreamSynchronize(s));

@@ -23,6 +23,7 @@ struct MgLevel {
    Level* L = nullptr;  // owned for levels >= 1
    double dx[3] = {1, 1, 1};
    T* C = nullptr;      // phase block C (cell), levels >= 1
+   T* M = nullptr;      // phase block mobility (cell), levels >= 1
    T* D = nullptr;      // composition block D (side), levels >= 1
    T* fc = nullptr;     // orientation face coefficients (side), levels >= 1
    T* sm = nullptr;     // orientation sqrt(M) (cell, ghosted), levels >= 1
@@ -54,14 +55,14 @@ void mg_destroy(Mg<T>* m);
 
 // Coarse-level coefficients restricted from fine-level arrays.
 template <typename T>
-int mg_setup_phase(Mg<T>& m, const T* C0, hipStream_t s);
+int mg_setup_phase(Mg<T>& m, const T* C0, const T* M0, hipStream_t s);
 template <typename T>
 int mg_setup_conc(Mg<T>& m, const T* D0, hipStream_t s);
 template <typename T>
 int mg_setup_quat(Mg<T>& m, const T* fc0, const T* sm0, hipStream_t s);
 
 // One V-cycle on A x = b from x = 0 (x0 zero on entry), fine arrays b0, x0.
-// Elliptic blocks: kind 0 = phase (C field, D and M constants), kind 1 =
+// Elliptic blocks: kind 0 = phase (C field, D constant, M constant or field), kind 1 =
 // composition (D field, C and M constants); op0 is the fine operator.
 template <typename T>
 int mg_vcycle_ell(Mg<T>& m, const EllOp<T>& op0, int kind, const MgCycle& cy, const T* b0,

@@ -218,6 +218,7 @@ int mg_create(const Level& fine, const double* dx, int ncomp_max, int max_levels
       for (int a = 0; a < 3; a++) c.dx[a] = 2.0 * m->lv.back().dx[a];
       size_t ncg = (size_t)c.L->ncg, ns = (size_t)c.L->nside_tot;
       rc |= talloc(&c.C, ncg);
+      rc |= talloc(&c.M, ncg);
       rc |= talloc(&c.D, ns);
       rc |= talloc(&c.fc, ns);
       rc |= talloc(&c.sm, ncg);
@@ -244,6 +245,7 @@ void mg_destroy(Mg<T>* m)
       (void)hipFree(c.r);
       if (l == 0) continue;
       (void)hipFree(c.C);
+      (void)hipFree(c.M);
       (void)hipFree(c.D);
       (void)hipFree(c.fc);
       (void)hipFree(c.sm);
@@ -255,11 +257,15 @@ void mg_destroy(Mg<T>* m)
 }
 
 template <typename T>
-int mg_setup_phase(Mg<T>& m, const T* C0, hipStream_t s)
+int mg_setup_phase(Mg<T>& m, const T* C0, const T* M0, hipStream_t s)
 {
    for (int l = 1; l < m.nlev(); l++) {
       const T* f = l == 1 ? C0 : m.lv[l - 1].C;
       if (mg_restrict_cell<T>(*m.lv[l].L, *m.lv[l - 1].L, 1, f, m.lv[l].C, s)) return -1;
+      if (M0) {
+         const T* fm = l == 1 ? M0 : m.lv[l - 1].M;
+         if (mg_restrict_cell<T>(*m.lv[l].L, *m.lv[l - 1].L, 1, fm, m.lv[l].M, s)) return -1;
+      }
    }
    return 0;
 }
@@ -321,8 +327,12 @@ static EllOp<T> level_op(const Mg<T>& m, const EllOp<T>& op0, int kind, int l)
 {
    EllOp<T> op = op0;
    if (l > 0) {
-      if (kind == 0) op.C = m.lv[l].C;
-      else op.D = m.lv[l].D;
+      if (kind == 0) {
+         op.C = m.lv[l].C;
+         if (op0.M) op.M = m.lv[l].M;
+      } else {
+         op.D = m.lv[l].D;
+      }
    }
    return op;
 }
@@ -331,8 +341,8 @@ template <typename T>
 int mg_vcycle_ell(Mg<T>& m, const EllOp<T>& op0, int kind, const MgCycle& cy, const T* b0,
                   T* x0, hipStream_t s)
 {
-   if (op0.M) {
-      set_error("mg_vcycle_ell: a mobility field is not supported (constant M only)");
+   if (op0.M && kind != 0) {
+      set_error("mg_vcycle_ell: a mobility field is supported for the phase block only");
       return -1;
    }
    int nl = m.nlev();
@@ -410,7 +420,7 @@ int mg_vcycle_quat(Mg<T>& m, const T* fc0, const T* sm0, double gamma, int ncomp
 #define AMPE_MG_INST(T)                                                                      \
    template int mg_create<T>(const Level&, const double*, int, int, int, Mg<T>**);          \
    template void mg_destroy<T>(Mg<T>*);                                                      \
-   template int mg_setup_phase<T>(Mg<T>&, const T*, hipStream_t);                            \
+   template int mg_setup_phase<T>(Mg<T>&, const T*, const T*, hipStream_t);                  \
    template int mg_setup_conc<T>(Mg<T>&, const T*, hipStream_t);                             \
    template int mg_setup_quat<T>(Mg<T>&, const T*, const T*, hipStream_t);                   \
    template int mg_vcycle_ell<T>(Mg<T>&, const EllOp<T>&, int, const MgCycle&, const T*, T*, \

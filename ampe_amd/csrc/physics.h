@@ -57,6 +57,7 @@ AMPE_HD double dinterp(double phi, char type)
          t = dmax(0.0, dmin(1.0, phi));
          return 6.0 * t * (1.0 - t);
       case 'w': t = dmax(0.0, phi); return t * (4.0 - 3.0 * t);
+      // the reference tests phi > 0 .or. phi < 1 (functions.f:136-137), true for all phi
       case 'l': return (phi > 0.0 || phi < 1.0) ? 1.0 : 0.0;
       case 'm':
          t = dmax(0.0, dmin(1.0, phi));
@@ -169,11 +170,29 @@ AMPE_HD double calphad_d2(const ampe_calphad_phase& cp, double T, double c)
           4.0 * c * (1.0 - c) * d2P;
 }
 
+// Dilute binary solution (c = solute fraction):
+//   f_L(c) = RT (c ln c - c),
+//   f_S(c) = RT (c ln c - c) - RT ln(k_e) c + dG(T),
+//   dG(T)  = RT (k_e - 1)(T - T_m) / m,
+// so mu_L = mu_S gives c_S = k_e c_L and the two-phase equilibrium lies on
+// the linear liquidus c_L = (T - T_m) / m.
+AMPE_HD double dilute_dG(const ampe_model_params& p)
+{
+   double RT = kRGas * p.temperature;
+   return RT * (p.dilute_keq - 1.0) * (p.temperature - p.dilute_Tm) / p.dilute_m;
+}
+
 AMPE_HD double fenergy(const ampe_model_params& p, double c, int ph)
 {
    if (p.fe_model == AMPE_FE_QUADRATIC) {
       double d = c - quad_ceq(p, ph);
       return p.quad_A[ph] * d * d;
+   }
+   if (p.fe_model == AMPE_FE_KKS_DILUTE) {
+      double RT = kRGas * p.temperature;
+      double f = RT * (c * log(c) - c);
+      if (ph == 1) f = f - RT * log(p.dilute_keq) * c + dilute_dG(p);
+      return f;
    }
    return calphad_g(p.calphad[ph], p.temperature, c);
 }
@@ -182,13 +201,50 @@ AMPE_HD double dfenergy(const ampe_model_params& p, double c, int ph)
 {
    if (p.fe_model == AMPE_FE_QUADRATIC)
       return 2.0 * p.quad_A[ph] * (c - quad_ceq(p, ph));
+   if (p.fe_model == AMPE_FE_KKS_DILUTE) {
+      double RT = kRGas * p.temperature;
+      double mu = RT * log(c);
+      if (ph == 1) mu = mu - RT * log(p.dilute_keq);
+      return mu;
+   }
    return calphad_mu(p.calphad[ph], p.temperature, c);
 }
 
 AMPE_HD double d2fenergy(const ampe_model_params& p, double c, int ph)
 {
    if (p.fe_model == AMPE_FE_QUADRATIC) return 2.0 * p.quad_A[ph];
+   if (p.fe_model == AMPE_FE_KKS_DILUTE) return kRGas * p.temperature / c;
    return calphad_d2(p.calphad[ph], p.temperature, c);
+}
+
+// Phase mobility at one cell: the uniform M_phi, or Kim's KKS mobility
+// (KimMobilityStrategyInfMob::evaluateMobility, source/
+// KimMobilityStrategyInfMob.cc:54-83; FiniteMob, KimMobilityStrategyFiniteMob.h
+// :29-38; FiniteMobAntiTrap, .cc:54-80) with kks_mobility_factor
+// (source/KKStools.cc:13-30) and zeta = (cL - cS) f_L''(cL) (cL - cS) / D_L(T)
+// converted by 1e-6 / v_m.
+AMPE_HD double phase_mobility(const ampe_model_params& p, double cl, double cs)
+{
+   if (p.phi_mobility_type != AMPE_PHI_MOB_KIM) return p.phi_mobility;
+   double eps = p.epsilon_phi, w = p.phi_well_scale;
+   double a2 = (p.energy_interp == 'p') ? 47. / 60. : 0.5;
+   double factor = 16. * w / (3. * eps * eps * a2);
+   double alpha = 0.0;
+   bool finite = p.kim_interface_mobility > 0.0;
+   if (finite) {
+      double xi = eps / sqrt(16. * w);
+      alpha = 3. * sqrt(2.) * xi / p.kim_interface_mobility;
+      if (!p.kim_antitrapping) return 1. / alpha;
+   }
+   double T = p.temperature;
+   double DL = p.D_liquid * exp(-p.Q0_liquid / (kRGas * T));
+   double d2 = d2fenergy(p, cl, 0);
+   double zeta = 0.;
+   zeta += (cl - cs) * d2 * (cl - cs);
+   double zf = zeta / DL;
+   zf *= (1.e-6 / p.vm_L);
+   if (finite) return 1. / (alpha + (1. / factor) * zf);
+   return factor / zf;
 }
 
 // End-member G(T) of a tdep table: interval k is the last with T >= Tc[k]
@@ -219,25 +275,38 @@ AMPE_HD void calphad_at(const ampe_calphad_tdep& tab, double T, ampe_calphad_pha
    }
 }
 
-// Binary CALPHAD KKS Newton: F = [(mu_L(xl) - mu_S(xs))/RT, (1-h) xl + h xs - c]
-// from (xl, xs) clamped into [1e-12, 1-1e-12], step halving to stay inside
-// (0,1), converged when both |F_i| < tol.  Returns the number of updates, -1
-// after maxits updates without convergence, or -2 when probe_its >= 0 and
-// max|F| > probe_tol after probe_its updates (the caller restarts).
-AMPE_HD int kks_newton(const ampe_calphad_phase* ph, double T, double h, double c,
-                       double& cl, double& cs, int maxits, double tol, int probe_its,
-                       double probe_tol)
+// Residual of the binary CALPHAD KKS system at (xl, xs):
+// F = [(mu_L(xl) - mu_S(xs))/RT, (1-h) xl + h xs - c].
+AMPE_HD void kks_residual(const ampe_calphad_phase* ph, double T, double RTinv, double h,
+                          double c, double xl, double xs, double& f1, double& f2)
 {
-   double RTinv = 1.0 / (kRGas * T);
-   double xl = cl, xs = cs;
+   f1 = (calphad_mu(ph[0], T, xl) - calphad_mu(ph[1], T, xs)) * RTinv;
+   f2 = (1.0 - h) * xl + h * xs - c;
+}
+
+// Start of a KKS Newton: the guess clamped into [1e-12, 1-1e-12].
+AMPE_HD void kks_clamp(double& xl, double& xs)
+{
    const double lo = 1.0e-12, hi = 1.0 - 1.0e-12;
    if (!(xl > lo)) xl = lo;
    if (!(xl < hi)) xl = hi;
    if (!(xs > lo)) xs = lo;
    if (!(xs < hi)) xs = hi;
+}
+
+// Newton iterations from a clamped start (xl, xs) whose residual (f1, f2) is
+// given; step halving keeps the iterate inside (0,1); converged when both
+// |F_i| < tol.  Returns the number of updates, -1 after maxits updates
+// without convergence, or -2 when probe_its >= 0 and max|F| > probe_tol after
+// probe_its updates (the caller restarts).
+AMPE_HD int kks_newton_from(const ampe_calphad_phase* ph, double T, double h, double c,
+                            double& cl, double& cs, double f1, double f2, int maxits,
+                            double tol, int probe_its, double probe_tol)
+{
+   double RTinv = 1.0 / (kRGas * T);
+   double xl = cl, xs = cs;
    for (int it = 0; it <= maxits; it++) {
-      double f1 = (calphad_mu(ph[0], T, xl) - calphad_mu(ph[1], T, xs)) * RTinv;
-      double f2 = (1.0 - h) * xl + h * xs - c;
+      if (it > 0) kks_residual(ph, T, RTinv, h, c, xl, xs, f1, f2);
       if (fabs(f1) < tol && fabs(f2) < tol) {
          cl = xl;
          cs = xs;
@@ -269,6 +338,16 @@ AMPE_HD int kks_newton(const ampe_calphad_phase* ph, double T, double h, double 
    return -1;
 }
 
+// Binary CALPHAD KKS Newton from the guess (cl, cs) (see kks_newton_from).
+AMPE_HD int kks_newton(const ampe_calphad_phase* ph, double T, double h, double c, double& cl,
+                       double& cs, int maxits, double tol, int probe_its, double probe_tol)
+{
+   kks_clamp(cl, cs);
+   double f1, f2;
+   kks_residual(ph, T, 1.0 / (kRGas * T), h, c, cl, cs, f1, f2);
+   return kks_newton_from(ph, T, h, c, cl, cs, f1, f2, maxits, tol, probe_its, probe_tol);
+}
+
 // Quadratic wells: closed form of mu_L = mu_S with mass balance.
 AMPE_HD void kks_quadratic(const ampe_model_params& p, double h, double c, double& cl,
                            double& cs)
@@ -281,14 +360,22 @@ AMPE_HD void kks_quadratic(const ampe_model_params& p, double h, double c, doubl
 }
 
 // Per-cell KKS at the model temperature: mu_L(cL) = mu_S(cS),
-// (1-h) cL + h cS = c.  Quadratic wells: closed form.  CALPHAD: kks_newton
-// from the guess in (cl, cs).  Returns the iteration count or -1.
+// (1-h) cL + h cS = c.  Quadratic wells and the dilute solution: closed
+// forms.  CALPHAD: kks_newton from the guess in (cl, cs).  Returns the
+// iteration count or -1.
 AMPE_HD int kks_cell(const ampe_model_params& p, double c, double phi, double& cl,
                      double& cs)
 {
    double h = interp(phi, p.conc_interp);
    if (p.fe_model == AMPE_FE_QUADRATIC) {
       kks_quadratic(p, h, c, cl, cs);
+      return 0;
+   }
+   if (p.fe_model == AMPE_FE_KKS_DILUTE) {
+      double k = p.dilute_keq;
+      double xl = c / (1.0 - h + h * k);
+      cl = xl;
+      cs = k * xl;
       return 0;
    }
    return kks_newton(p.calphad, p.temperature, h, c, cl, cs, p.newton_max_its, p.newton_tol,

@@ -191,15 +191,13 @@ __global__ void __launch_bounds__(256) k_quat_face_norm2(LevelArgs a, ampe_model
 // 2d/quatrhs.m4:264-405), KKS driving force (QuadraticFreeEnergyStrategy.cc
 // :393-511 / CALPHADFreeEnergyStrategyBinary.cc:521-651), times M_phi.
 template <int NDIM>
-__global__ void __launch_bounds__(256) k_phase_rhs(LevelArgs a, ampe_model_params p,
-                                                   const double* __restrict__ phi,
-                                                   const double* __restrict__ q,
-                                                   const double* __restrict__ cl,
-                                                   const double* __restrict__ cs,
-                                                   double* __restrict__ rhs)
+__device__ __forceinline__ void phase_rhs_cell(const LevelArgs& a, const ampe_model_params& p,
+                                               const CellPos& c, const double* __restrict__ phi,
+                                               const double* __restrict__ q,
+                                               const double* __restrict__ cl,
+                                               const double* __restrict__ cs,
+                                               double* __restrict__ rhs)
 {
-   CellPos c;
-   if (!cell_pos(a, 0, c)) return;
    const long long c0 = c.c0;
    double r = 0.0;
 #pragma unroll
@@ -229,7 +227,27 @@ __global__ void __launch_bounds__(256) k_phase_rhs(LevelArgs a, ampe_model_param
       double hp = dinterp(ph, p.energy_interp);
       r += hp * ((f_l - f_a) - mu * (c_l - c_a));
    }
-   rhs[c0] = p.phi_mobility * r;
+   double mob = (p.phi_mobility_type == AMPE_PHI_MOB_KIM) ? phase_mobility(p, cl[c0], cs[c0])
+                                                          : p.phi_mobility;
+   rhs[c0] = mob * r;
+}
+
+template <int NDIM>
+__global__ void __launch_bounds__(256) k_phase_rhs(LevelArgs a, ampe_model_params p,
+                                                   const double* __restrict__ phi,
+                                                   const double* __restrict__ q,
+                                                   const double* __restrict__ cl,
+                                                   const double* __restrict__ cs,
+                                                   double* __restrict__ rhs)
+{
+   for (int kk = 0; kk < a.nz_tile; kk++) {
+      CellPos c;
+      if (!cell_pos_col(a, kk, c)) {
+         if (c.i >= c.pd.nx || c.j >= c.pd.ny) return;
+         break;
+      }
+      phase_rhs_cell<NDIM>(a, p, c, phi, q, cl, cs, rhs);
+   }
 }
 
 // ------------------------------------------------------------ quat RHS
@@ -260,8 +278,7 @@ __global__ void __launch_bounds__(256) k_quat_rhs(LevelArgs a, ampe_model_params
                                                   const double* __restrict__ n2,
                                                   double* __restrict__ rhs)
 {
-   CellPos c;
-   if (!cell_pos(a, 0, c)) return;
+   AMPE_COLUMN_BEGIN(a, c)
    const long long c0 = c.c0, ncg = a.ncg;
    const int ql = p.qlen;
    double fclo[3], fchi[3];
@@ -305,6 +322,7 @@ __global__ void __launch_bounds__(256) k_quat_rhs(LevelArgs a, ampe_model_params
       double qm = q[m * ncg + c0];
       rhs[m * ncg + c0] = 0.0 - mq * (dv[m] + 2.0 * qm * lam);
    }
+AMPE_COLUMN_END
 }
 
 // project2d/3d (2d/quatfacops.m4:940-997)
@@ -376,8 +394,7 @@ __global__ void __launch_bounds__(256) k_conc_rhs(LevelArgs a, ampe_model_params
                                                   const double* __restrict__ Dphi,
                                                   double* __restrict__ rhs)
 {
-   CellPos c;
-   if (!cell_pos(a, 0, c)) return;
+   AMPE_COLUMN_BEGIN(a, c)
    const long long c0 = c.c0;
    const int ijk[3] = {c.i, c.j, c.k};
    double s = 0.0;
@@ -395,6 +412,28 @@ __global__ void __launch_bounds__(256) k_conc_rhs(LevelArgs a, ampe_model_params
       s = (d == 0) ? t : s + t;
    }
    rhs[c0] = p.conc_mobility * s;
+AMPE_COLUMN_END
+}
+
+// QuatMobilityStrategy::computePhaseMobility over interior cells
+// (QuatIntegrator::computeMobilities, source/QuatIntegrator.cc:2959-2992)
+__global__ void __launch_bounds__(256) k_phase_mobility(LevelArgs a, ampe_model_params p,
+                                                        const double* __restrict__ cl,
+                                                        const double* __restrict__ cs,
+                                                        double* __restrict__ M)
+{
+   CellPos c;
+   if (!cell_pos(a, 0, c)) return;
+   M[c.c0] = phase_mobility(p, cl ? cl[c.c0] : 0.0, cs ? cs[c.c0] : 0.0);
+}
+
+int launch_phase_mobility(const Level& L, const ampe_model_params& p, const double* cl,
+                          const double* cs, double* M, hipStream_t s)
+{
+   hipLaunchKernelGGL(k_phase_mobility, dim3(L.ncell_tiles), dim3(TX, TY), 0, s, level_args(L),
+                      p, cl, cs, M);
+   AMPE_HIP_CHECK(hipGetLastError());
+   return 0;
 }
 
 // ------------------------------------------------------------ tagging
@@ -444,14 +483,14 @@ int launch_phase_rhs(const Level& L, const ampe_model_params& p, const double* p
                      const double* q, const double* cl, const double* cs, double* rhs,
                      hipStream_t s)
 {
-   AMPE_DISPATCH(L, k_phase_rhs, L.ncell_tiles, level_args(L), p, phi, q, cl, cs, rhs);
+   AMPE_DISPATCH(L, k_phase_rhs, L.ncol_tiles, level_args_col(L), p, phi, q, cl, cs, rhs);
    return 0;
 }
 
 int launch_quat_rhs(const Level& L, const ampe_model_params& p, const double* phi,
                     const double* q, const double* n2, double* rhs, hipStream_t s)
 {
-   AMPE_DISPATCH(L, k_quat_rhs, L.ncell_tiles, level_args(L), p, phi, q, n2, rhs);
+   AMPE_DISPATCH(L, k_quat_rhs, L.ncol_tiles, level_args_col(L), p, phi, q, n2, rhs);
    return 0;
 }
 
@@ -476,7 +515,7 @@ int launch_conc_rhs(const Level& L, const ampe_model_params& p, const double* c,
                     const double* phi, const double* Dc, const double* Dphi, double* rhs,
                     hipStream_t s)
 {
-   AMPE_DISPATCH(L, k_conc_rhs, L.ncell_tiles, level_args(L), p, c, phi, Dc, Dphi, rhs);
+   AMPE_DISPATCH(L, k_conc_rhs, L.ncol_tiles, level_args_col(L), p, c, phi, Dc, Dphi, rhs);
    return 0;
 }
 

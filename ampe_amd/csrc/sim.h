@@ -31,11 +31,12 @@ struct Sim {
    double *cl = nullptr, *cs = nullptr, *clr = nullptr, *csr = nullptr;
    // lagged face data and preconditioner coefficients
    double *n2 = nullptr, *Dc = nullptr, *Dphi = nullptr, *Dconc = nullptr, *fc = nullptr;
-   double *Cphi = nullptr, *sqrtm = nullptr;
+   double *Cphi = nullptr, *sqrtm = nullptr, *Mphi = nullptr;
    double *fac_u = nullptr, *fac_e = nullptr, *fac_r = nullptr;
    // FP32 preconditioner copies (mixed precision)
    float *f_u = nullptr, *f_e = nullptr, *f_r = nullptr, *f_f = nullptr;
-   float *f_Cphi = nullptr, *f_sqrtm = nullptr, *f_Dconc = nullptr, *f_fc = nullptr;
+   float *f_Cphi = nullptr, *f_sqrtm = nullptr, *f_Dconc = nullptr, *f_fc = nullptr,
+         *f_Mphi = nullptr;
    hipStream_t stream = nullptr;
    hipEvent_t ev0 = nullptr, ev1 = nullptr;
    long long* kks_acc = nullptr;  // device KKS counters (launch_kks)

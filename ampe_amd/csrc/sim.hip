@@ -49,8 +49,16 @@ int sim_create(const ampe_model_params& p, const ampe_solver_params& s, const am
       set_error("sim_create: maxl must be in [1, 30]");
       return -1;
    }
+   if (s.gstype != AMPE_GS_MODIFIED && (s.gstype != AMPE_GS_CLASSICAL || s.maxl > kMaxCgs - 2)) {
+      set_error("sim_create: gstype must be modified, or classical with maxl <= 14");
+      return -1;
+   }
    if (p.evolve_quat && p.qlen != 4) {
       set_error("sim_create: qlen must be 4");
+      return -1;
+   }
+   if (p.phi_mobility_type == AMPE_PHI_MOB_KIM && !p.with_conc) {
+      set_error("sim_create: Kim's phase mobility needs a composition field");
       return -1;
    }
    Sim* S = new Sim();
@@ -75,7 +83,7 @@ int sim_create(const ampe_model_params& p, const ampe_solver_params& s, const am
    for (auto v : vecs) rc |= dalloc(v, S->nv);
    S->V.assign(s.maxl + 1, nullptr);
    for (auto& v : S->V) rc |= dalloc(&v, S->nv);
-   double** cells[] = {&S->cl, &S->cs, &S->clr, &S->csr, &S->Cphi, &S->sqrtm};
+   double** cells[] = {&S->cl, &S->cs, &S->clr, &S->csr, &S->Cphi, &S->sqrtm, &S->Mphi};
    for (auto v : cells) rc |= dalloc(v, S->ng);
    rc |= dalloc(&S->fac_u, S->ng * ql);
    rc |= dalloc(&S->fac_e, S->nv);
@@ -88,6 +96,7 @@ int sim_create(const ampe_model_params& p, const ampe_solver_params& s, const am
       rc |= falloc(&S->f_r, S->nv);
       rc |= falloc(&S->f_f, S->nv);
       rc |= falloc(&S->f_Cphi, S->ng);
+      rc |= falloc(&S->f_Mphi, S->ng);
       rc |= falloc(&S->f_sqrtm, S->ng);
       rc |= falloc(&S->f_Dconc, S->nf);
       rc |= falloc(&S->f_fc, S->nf);
@@ -120,11 +129,13 @@ void sim_destroy(Sim* S)
    if (!S) return;
    double* all[] = {S->y, S->yprev, S->ypred, S->ycur, S->ahist, S->fcur, S->b, S->del,
                     S->ewt, S->work, S->work2, S->jv, S->xk, S->corr, S->acor, S->cl, S->cs,
-                    S->clr, S->csr, S->Cphi, S->sqrtm, S->fac_u, S->fac_e, S->fac_r, S->n2,
+                    S->clr, S->csr, S->Cphi, S->sqrtm, S->Mphi, S->fac_u, S->fac_e, S->fac_r,
+                    S->n2,
                     S->Dc, S->Dphi, S->Dconc, S->fc};
    for (double* v : all) (void)hipFree(v);
    for (double* v : S->V) (void)hipFree(v);
-   float* fall[] = {S->f_u, S->f_e, S->f_r, S->f_f, S->f_Cphi, S->f_sqrtm, S->f_Dconc, S->f_fc};
+   float* fall[] = {S->f_u,    S->f_e,     S->f_r,     S->f_f,   S->f_Cphi,
+                    S->f_sqrtm, S->f_Dconc, S->f_fc, S->f_Mphi};
    for (float* v : fall) (void)hipFree(v);
    (void)hipFree(S->kks_acc);
    mg_destroy(S->mgf);
@@ -196,6 +207,7 @@ static int kks_collect(Sim* S)
    S->st.kks_iters = h[0];
    S->st.kks_calls = h[3];
    S->st.kks_fallbacks = h[4];
+   S->st.kks_surrogate_starts = h[5];
    S->st.kks_cells = S->st.kks_calls * S->L->ncg;
    if (h[2]) {
       set_error("KKS Newton did not converge in a cell");
@@ -306,7 +318,11 @@ static int psetup(Sim* S, double* y, double gamma)
    if (q && launch_quat_grad(L, p, q, nullptr, S->n2, st)) return -1;
    if (c && kks(S, y)) return -1;
    // phase block: C = 1 + gamma M w g'' (source/PhaseFACOps.cc:33-51)
-   if (launch_phase_C(L, p, gamma, phi, S->Cphi, st)) return -1;
+   // phase mobility after the phase concentrations (QuatIntegrator::
+   // setCoefficients -> computeMobilities, source/QuatIntegrator.cc:3078-3080)
+   bool kim = p.phi_mobility_type == AMPE_PHI_MOB_KIM;
+   if (kim && launch_phase_mobility(L, p, S->cl, S->cs, S->Mphi, st)) return -1;
+   if (launch_phase_C(L, p, gamma, phi, kim ? S->Mphi : nullptr, S->Cphi, st)) return -1;
    // composition block: D = -gamma D_c (source/ConcFACOps.cc:28-61)
    if (c && vec_axpby(S->nf, -gamma, S->Dc, 0.0, S->Dc, S->Dconc, st)) return -1;
    // orientation block (source/QuatFACOps.cc:735-829)
@@ -316,6 +332,7 @@ static int psetup(Sim* S, double* y, double gamma)
    }
    if (S->s.precision == AMPE_FAC_FP32) {
       if (vec_to_f32(S->ng, S->Cphi, S->f_Cphi, st)) return -1;
+      if (kim && vec_to_f32(S->ng, S->Mphi, S->f_Mphi, st)) return -1;
       if (c && vec_to_f32(S->nf, S->Dconc, S->f_Dconc, st)) return -1;
       if (q) {
          if (vec_to_f32(S->nf, S->fc, S->f_fc, st)) return -1;
@@ -323,12 +340,12 @@ static int psetup(Sim* S, double* y, double gamma)
       }
    }
    if (S->mgf) {
-      if (mg_setup_phase<float>(*S->mgf, S->f_Cphi, st)) return -1;
+      if (mg_setup_phase<float>(*S->mgf, S->f_Cphi, kim ? S->f_Mphi : nullptr, st)) return -1;
       if (c && mg_setup_conc<float>(*S->mgf, S->f_Dconc, st)) return -1;
       if (q && mg_setup_quat<float>(*S->mgf, S->f_fc, S->f_sqrtm, st)) return -1;
    }
    if (S->mgd) {
-      if (mg_setup_phase<double>(*S->mgd, S->Cphi, st)) return -1;
+      if (mg_setup_phase<double>(*S->mgd, S->Cphi, kim ? S->Mphi : nullptr, st)) return -1;
       if (c && mg_setup_conc<double>(*S->mgd, S->Dconc, st)) return -1;
       if (q && mg_setup_quat<double>(*S->mgd, S->fc, S->sqrtm, st)) return -1;
    }
@@ -488,8 +505,8 @@ static int run_blocks(Sim* S, std::vector<PBlock<T>>& blocks, double tol)
 // source/QuatSysSolver.cc:296-349), composition block
 // (ConcentrationPrecondSolve :3547-3591).
 template <typename T>
-static std::vector<PBlock<T>> make_blocks(Sim* S, double dphi, const T* Cphi, const T* Dconc,
-                                          const T* fc, const T* sm)
+static std::vector<PBlock<T>> make_blocks(Sim* S, double dphi, const T* Cphi, const T* Mphi,
+                                          const T* Dconc, const T* fc, const T* sm)
 {
    const ampe_model_params& p = S->p;
    std::vector<PBlock<T>> bl;
@@ -501,6 +518,7 @@ static std::vector<PBlock<T>> make_blocks(Sim* S, double dphi, const T* Cphi, co
    ph.op.C = Cphi;
    ph.op.Cc = T(1);
    ph.op.Mc = (T)p.phi_mobility;
+   if (p.phi_mobility_type == AMPE_PHI_MOB_KIM) ph.op.M = Mphi;
    bl.push_back(ph);
    if (S->iq >= 0) {
       PBlock<T> qb{};
@@ -537,7 +555,8 @@ static int psolve(Sim* S, const double* r, double* z, double delta)
       // f = r (orientation blocks / sqrt(M), FP64 division) in FP32, u = 0
       if (psolve_in<float>(ng, S->nc, iq, ql, r, S->sqrtm, S->f_f, nullptr, S->f_u, st))
          return -1;
-      auto bl = make_blocks<float>(S, dphi, S->f_Cphi, S->f_Dconc, S->f_fc, S->f_sqrtm);
+      auto bl = make_blocks<float>(S, dphi, S->f_Cphi, S->f_Mphi, S->f_Dconc, S->f_fc,
+                                   S->f_sqrtm);
       for (auto& B : bl) {
          long long off = (long long)B.comp * ng;
          B.f = S->f_f + off;
@@ -552,7 +571,7 @@ static int psolve(Sim* S, const double* r, double* z, double delta)
    }
    // FP64: z = 0, fac_u = orientation blocks of r / sqrt(M)
    if (psolve_in<double>(ng, S->nc, iq, ql, r, S->sqrtm, nullptr, S->fac_u, z, st)) return -1;
-   auto bl = make_blocks<double>(S, dphi, S->Cphi, S->Dconc, S->fc, S->sqrtm);
+   auto bl = make_blocks<double>(S, dphi, S->Cphi, S->Mphi, S->Dconc, S->fc, S->sqrtm);
    for (auto& B : bl) {
       long long off = (long long)B.comp * ng;
       B.f = B.kind == 1 ? S->fac_u : r + off;
@@ -619,16 +638,46 @@ static int spgmr(Sim* S, double gamma, const double* bvec, double* x, double del
       double siginv = 1.0 / sig;
       if (kry_finish(L, nc, siginv, gamma, S->xk, S->jv, S->fcur, S->V[l + 1], st)) return -1;
       if (psolve(S, S->V[l + 1], S->work, delta) < 0) return -1;
-      // V[l+1] = ewt work; modified Gram-Schmidt, each step fused with the next dot
-      double dot;
-      if (kry_scale_dot(L, nc, S->ewt, S->work, S->V[l + 1], S->V[0], &dot, st)) return -1;
-      for (int i = 0; i <= l; i++) {
-         double hij = dot * cv;
-         HH(i, l) = hij;
-         const double* nxt = (i < l) ? S->V[i + 1] : nullptr;
-         if (kry_mgs(L, nc, S->V[l + 1], S->V[i], hij, nxt, &dot, st)) return -1;
+      double hn;
+      if (S->s.gstype == AMPE_GS_CLASSICAL) {
+         // V[l+1] = ewt work with all its dots in one pass, one linear
+         // combination, re-orthogonalised when the norm dropped by > 1000
+         // (SUNClassicalGS)
+         VecList vl;
+         vl.n = l + 1;
+         for (int i = 0; i <= l; i++) vl.v[i] = S->V[i];
+         double sums[kMaxCgs];
+         if (kry_cgs_dots(L, nc, S->ewt, S->work, S->V[l + 1], vl, sums, st)) return -1;
+         double vk_norm = std::sqrt(sums[l + 1] * cv);
+         for (int i = 0; i <= l; i++) {
+            HH(i, l) = sums[i] * cv;
+            vl.y[i] = -HH(i, l);
+         }
+         double ss;
+         if (kry_cgs_update(L, nc, S->V[l + 1], vl, &ss, st)) return -1;
+         hn = std::sqrt(ss * cv);
+         if (1000.0 * hn < vk_norm) {
+            if (kry_cgs_dots(L, nc, nullptr, nullptr, S->V[l + 1], vl, sums, st)) return -1;
+            for (int i = 0; i <= l; i++) {
+               double d = sums[i] * cv;
+               HH(i, l) += d;
+               vl.y[i] = -d;
+            }
+            if (kry_cgs_update(L, nc, S->V[l + 1], vl, &ss, st)) return -1;
+            hn = std::sqrt(ss * cv);
+         }
+      } else {
+         // V[l+1] = ewt work; modified Gram-Schmidt, each step fused with the next dot
+         double dot;
+         if (kry_scale_dot(L, nc, S->ewt, S->work, S->V[l + 1], S->V[0], &dot, st)) return -1;
+         for (int i = 0; i <= l; i++) {
+            double hij = dot * cv;
+            HH(i, l) = hij;
+            const double* nxt = (i < l) ? S->V[i + 1] : nullptr;
+            if (kry_mgs(L, nc, S->V[l + 1], S->V[i], hij, nxt, &dot, st)) return -1;
+         }
+         hn = std::sqrt(dot * cv);
       }
-      double hn = std::sqrt(dot * cv);
       HH(l + 1, l) = hn;
       for (int i = 0; i < l; i++) {
          double c = givens[2 * i], s = givens[2 * i + 1];

@@ -44,7 +44,8 @@ _SIGS = {
     "ampe_kks_solve": (_i, [_vp, C.POINTER(ModelParams), _vp, _vp, _vp, _vp, _vp, _vp,
                             C.POINTER(_i), _vp]),
     "ampe_tag_cells": (_i, [_vp, C.POINTER(ModelParams), _vp, _d, _vp, _vp]),
-    "ampe_fac_phase_coeffs": (_i, [_vp, C.POINTER(ModelParams), _d, _vp, _vp, _vp]),
+    "ampe_fac_phase_coeffs": (_i, [_vp, C.POINTER(ModelParams), _d, _vp, _vp, _vp, _vp]),
+    "ampe_phase_mobility": (_i, [_vp, C.POINTER(ModelParams), _vp, _vp, _vp, _vp]),
     "ampe_fac_smooth": (_i, [_vp, C.POINTER(EllipticOp), _vp, _vp, _i, _d, C.POINTER(_d), _vp]),
     "ampe_fac_residual": (_i, [_vp, C.POINTER(EllipticOp), _vp, _vp, _vp, C.POINTER(_d), _vp]),
     "ampe_vec_dot": (_i, [_vp, _i, _vp, _vp, C.POINTER(_d), _vp]),
@@ -56,7 +57,7 @@ _SIGS = {
     "ampe_sim_get_state": (_i, [_vp, _vp, _vp, _vp, _vp, _vp]),
     "ampe_sim_advance": (_i, [_vp, _i, C.POINTER(Stats)]),
     "ampe_sim_last_time_ms": (_i, [_vp, C.POINTER(_d)]),
-    "ampe_sim_set_surrogate": (_i, [_vp, _vp, _i, _d]),
+    "ampe_sim_set_surrogate": (_i, [_vp, _vp, _i, _d, _d]),
     "ampe_coarsen_cell": (_i, [_vp, _vp, _i, _vp, _vp, _vp]),
     "ampe_coarsen_side": (_i, [_vp, _vp, _vp, _vp, _vp]),
     "ampe_refine_add": (_i, [_vp, _vp, _i, _i, _vp, _vp, _vp]),
@@ -200,9 +201,9 @@ class Sim:
         check(lib().ampe_sim_last_time_ms(self.h, C.byref(ms)), "ampe_sim_last_time_ms")
         return ms.value
 
-    def set_surrogate(self, surrogate, probe_its=3, probe_tol=1.0e-8):
+    def set_surrogate(self, surrogate, probe_its=3, probe_tol=1.0e-8, switch_tol=1.0e-2):
         h = surrogate.h if surrogate is not None else None
-        check(lib().ampe_sim_set_surrogate(self.h, h, probe_its, probe_tol),
+        check(lib().ampe_sim_set_surrogate(self.h, h, probe_its, probe_tol, switch_tol),
               "ampe_sim_set_surrogate")
         self._surrogate = surrogate  # keep it alive as long as the sim
 

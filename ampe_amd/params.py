@@ -82,6 +82,12 @@ class ModelParams(C.Structure):
         ("calphad", CalphadPhase * 2),
         ("newton_max_its", C.c_int),
         ("newton_tol", C.c_double),
+        ("dilute_keq", C.c_double),
+        ("dilute_m", C.c_double),
+        ("dilute_Tm", C.c_double),
+        ("phi_mobility_type", C.c_int),
+        ("kim_interface_mobility", C.c_double),
+        ("kim_antitrapping", C.c_int),
     ]
 
 
@@ -107,13 +113,15 @@ class SolverParams(C.Structure):
         ("precond", C.c_int),
         ("mg_max_levels", C.c_int),
         ("mg_coarse_sweeps", C.c_int),
+        ("gstype", C.c_int),
     ]
 
 
 class Stats(C.Structure):
     _fields_ = [(n, C.c_longlong) for n in
                 ("nst", "nni", "nli", "nfe", "npe", "nps", "ncfn", "nlcf",
-                 "kks_calls", "kks_cells", "kks_iters", "kks_fallbacks")] + [
+                 "kks_calls", "kks_cells", "kks_iters", "kks_fallbacks",
+                 "kks_surrogate_starts")] + [
         ("last_acnrm", C.c_double), ("t", C.c_double), ("h", C.c_double)]
 
     def as_dict(self):
@@ -121,10 +129,12 @@ class Stats(C.Structure):
 
 
 FLUX_SIMPLE, FLUX_ISOTROPIC, FLUX_ANISOTROPIC = 0, 1, 2
-FE_QUADRATIC, FE_CALPHAD_BINARY = 0, 1
+FE_QUADRATIC, FE_CALPHAD_BINARY, FE_KKS_DILUTE = 0, 1, 2
+PHI_MOB_CONSTANT, PHI_MOB_KIM = 0, 1
 FP64, FAC_FP32 = 0, 1
 GUESS_PREVIOUS, GUESS_SURROGATE = 0, 1
 PRECOND_REDBLACK, PRECOND_MG = 0, 1
+GS_MODIFIED, GS_CLASSICAL = 0, 1
 
 R_GAS = 8.314472
 
@@ -235,7 +245,8 @@ def solver_defaults(**kw):
                      lag_quat_sidegrad=1, fac_max_cycles=1, fac_sweeps=2,
                      fac_tol=-1.0, quat_max_cycles=1, quat_sweeps=2,
                      precision=FP64, kks_guess=GUESS_PREVIOUS,
-                     precond=PRECOND_REDBLACK, mg_max_levels=0, mg_coarse_sweeps=16)
+                     precond=PRECOND_REDBLACK, mg_max_levels=0, mg_coarse_sweeps=16,
+                     gstype=GS_MODIFIED)
     for k, v in kw.items():
         setattr(s, k, v)
     return s
@@ -332,6 +343,36 @@ def polycrystal_params(ndim=3, seed=1002, **kw):
     for k, v in kw.items():
         setattr(p, k, v)
     return p
+
+
+def dilute_params(ndim=2, **kw):
+    """Dilute binary solution (KKSdiluteBinary) with the CuNi values of
+    tests/KKSdiluteBinaryKKS/test.input and tests/DiluteCuNi/2d.input:
+    k_e = 0.7965, liquidus slope -310.9 K, T_m = 1728 K, T = 1710 K, linear
+    concentration interpolation, on the polycrystal's interface and
+    orientation parameters."""
+    p = polycrystal_params(ndim=ndim)
+    p.fe_model = FE_KKS_DILUTE
+    p.temperature = 1710.0
+    p.dilute_keq = 0.7965
+    p.dilute_m = -310.9
+    p.dilute_Tm = 1728.0
+    p.conc_interp = b"l"
+    p.vm_L = p.vm_A = 6.8812e-6
+    for k, v in kw.items():
+        setattr(p, k, v)
+    return p
+
+
+def kim_interface_mobility(vm_liquid, Tm, liquidus_slope, keq, beta):
+    """Interface mobility [um^3/(pJ s)] from the kinetics coefficient beta [s/m]
+    (kinetics_coefficient), source/QuatModelParameters.cc:1156-1166."""
+    mu = vm_liquid / (R_GAS * Tm)
+    mu *= 1.0e6
+    mu *= -1.0 * liquidus_slope
+    mu /= 1.0 - keq
+    mu /= beta
+    return mu
 
 
 def random_unit_quaternions(rng, n):

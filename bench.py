@@ -15,10 +15,12 @@ Forms:
   fp64_reference : FP64 everywhere, red-black block solves (the reference's
                    FAC "redblack" coarse solve), previous-step KKS guess -- the
                    plain FP64 GPU form, parity-tested against the oracle;
-  fp64_mg        : FP64 everywhere, multigrid V-cycle block solves;
+  fp64_mg        : FP64 everywhere, multigrid V-cycle block solves, classical
+                   Gram-Schmidt (one pass for all dots);
   mixed          : FP32 multigrid preconditioner under the FP64 Newton-Krylov
-                   outer loop, previous-step KKS guess;
-  mixed_surrogate: mixed + surrogate-seeded KKS (the metric's form, value).
+                   outer loop (classical Gram-Schmidt), previous-step KKS guess;
+  mixed_surrogate: mixed + surrogate-seeded KKS where a cell's previous-step
+                   guess has a KKS residual above 1e-2 (the metric's form, value).
 The KKS surrogate is trained at start-up on 1e7 FP64 Newton solutions of the
 config-3 distribution and also reported in isolation on 2^24 cells.
 
@@ -83,7 +85,7 @@ FORMS = ("fp64_reference", "fp64_mg", "mixed", "mixed_surrogate")
 def solver(form, args):
     kw = dict(dt=args.dt, maxl=args.maxl, max_newton=5, atol=args.atol, rtol=args.atol * 1e-2)
     mg = dict(precond=P.PRECOND_MG, fac_sweeps=1, quat_sweeps=1, mg_max_levels=args.mg_levels,
-              mg_coarse_sweeps=args.mg_coarse_sweeps)
+              mg_coarse_sweeps=args.mg_coarse_sweeps, gstype=P.GS_CLASSICAL)
     if form == "fp64_reference":
         kw.update(precision=P.FP64, fac_sweeps=2, quat_sweeps=2)
     elif form == "fp64_mg":
@@ -132,7 +134,8 @@ def run_gpu(p, s, state, n, steps, surrogate=None, pre=3):
         raise RuntimeError(f"timed steps failed: {st}")
     dev_ms = sim.last_time_ms()
     delta = {k: st[k] - s0[k] for k in ("nst", "nni", "nli", "nfe", "npe", "nps", "ncfn",
-                                        "nlcf", "kks_calls", "kks_iters", "kks_fallbacks")}
+                                        "nlcf", "kks_calls", "kks_iters", "kks_fallbacks",
+                                        "kks_surrogate_starts")}
     delta["t"] = st["t"]
     fields = sim.get_state((n, n, n))
     return dict(wall=wall, dev_ms=dev_ms, stats=delta, fields=fields)

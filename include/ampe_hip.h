@@ -50,7 +50,8 @@ enum ampe_flux_type {
 
 enum ampe_free_energy {
    AMPE_FE_QUADRATIC = 0,      /* QuadraticFreeEnergyStrategy              */
-   AMPE_FE_CALPHAD_BINARY = 1  /* CALPHADFreeEnergyStrategyBinary          */
+   AMPE_FE_CALPHAD_BINARY = 1, /* CALPHADFreeEnergyStrategyBinary          */
+   AMPE_FE_KKS_DILUTE = 2      /* KKSdiluteBinary (dilute binary solution) */
 };
 
 /* Binary CALPHAD phase description evaluated at the (uniform) temperature:
@@ -108,8 +109,23 @@ typedef struct ampe_model_params {
    ampe_calphad_phase calphad[2]; /* [0]=liquid [1]=solid A */
    int newton_max_its;    /* KKS Newton iteration cap */
    double newton_tol;     /* KKS Newton residual tolerance */
+   /* dilute binary solution (AMPE_FE_KKS_DILUTE, ConcentrationModel keys
+    * keq, liquidus_slope, meltingT of tests/KKSdiluteBinaryKKS/test.input) */
+   double dilute_keq;     /* partition coefficient k_e */
+   double dilute_m;       /* liquidus slope [K per mole fraction], < 0 */
+   double dilute_Tm;      /* melting temperature of the solvent [K] */
+   /* phase mobility (QuatMobilityStrategy::computePhaseMobility) */
+   int phi_mobility_type;         /* enum ampe_phi_mobility */
+   double kim_interface_mobility; /* Kim: <= 0 infinite interface mobility */
+   int kim_antitrapping;          /* Kim, finite mobility: anti-trapping form */
 } ampe_model_params;
 
+/* Phase mobility: the uniform M_phi (SimpleQuatMobilityStrategy,
+ * source/SimpleQuatMobilityStrategy.cc:46) or Kim's KKS mobility from the
+ * liquid's d2f/dc2 (Kim, Acta Mater. 55 (2007) 4391; KimMobilityStrategy
+ * InfMob / FiniteMob / FiniteMobAntiTrap, source/KimMobilityStrategy*.cc,
+ * factor of source/KKStools.cc:13-30), evaluated per cell from (cL, cS). */
+enum ampe_phi_mobility { AMPE_PHI_MOB_CONSTANT = 0, AMPE_PHI_MOB_KIM = 1 };
 enum ampe_precision { AMPE_FP64 = 0, AMPE_FAC_FP32 = 1 };
 enum ampe_kks_guess { AMPE_GUESS_PREVIOUS = 0, AMPE_GUESS_SURROGATE = 1 };
 /* Block solver inside each FAC cycle: the reference's single-level red-black
@@ -117,6 +133,10 @@ enum ampe_kks_guess { AMPE_GUESS_PREVIOUS = 0, AMPE_GUESS_SURROGATE = 1 };
  * or a geometric multigrid V-cycle below the level (the role of hypre PFMG,
  * the reference's default coarse solver). */
 enum ampe_precond { AMPE_PRECOND_REDBLACK = 0, AMPE_PRECOND_MG = 1 };
+/* SPGMR orthogonalisation (SUNLinSol_SPGMR gstype): modified Gram-Schmidt
+ * (the reference's choice, source/QuatIntegrator.cc:1571-1596) or classical
+ * Gram-Schmidt with one conditional re-orthogonalisation (SUNClassicalGS). */
+enum ampe_gstype { AMPE_GS_MODIFIED = 0, AMPE_GS_CLASSICAL = 1 };
 
 typedef struct ampe_solver_params {
    double dt;             /* fixed BDF step (initial = max step) */
@@ -138,12 +158,14 @@ typedef struct ampe_solver_params {
    int precond;           /* enum ampe_precond */
    int mg_max_levels;     /* multigrid levels including the fine one (<= 0: all) */
    int mg_coarse_sweeps;  /* red-black sweeps on the coarsest multigrid level */
+   int gstype;            /* enum ampe_gstype (classical: maxl <= 14) */
 } ampe_solver_params;
 
 /* Per-run counters (mirror QuatIntegrator.cc:2161-2170 nni/nli/nfe/npe/nps). */
 typedef struct ampe_stats {
    long long nst, nni, nli, nfe, npe, nps, ncfn, nlcf;
    long long kks_calls, kks_cells, kks_iters, kks_fallbacks;
+   long long kks_surrogate_starts; /* cells whose Newton started from the surrogate */
    double last_acnrm; /* WRMS norm of the last step's Newton correction */
    double t;          /* simulated time reached */
    double h;          /* next step size (dt unless a failure cut it) */
@@ -259,10 +281,20 @@ typedef struct ampe_elliptic_op {
 } ampe_elliptic_op;
 
 /* PhaseFACOps::setOperatorCoefficients / setC (source/PhaseFACOps.cc:33-183):
- * C = 1 + gamma*M_phi*w*g''(phi) written to C (cell field). */
+ * C = 1 + gamma*M*w*g''(phi) written to C (cell field); M is the phase
+ * mobility field, or NULL for the uniform M_phi. */
 int ampe_fac_phase_coeffs(const ampe_level* level, const ampe_model_params* p,
-                          double gamma, const double* phi, double* C,
-                          ampe_stream_t stream);
+                          double gamma, const double* phi, const double* M,
+                          double* C, ampe_stream_t stream);
+
+/* Phase mobility field over interior cells from the phase concentrations
+ * (QuatMobilityStrategy::computePhaseMobility as called by
+ * QuatIntegrator::computeMobilities, source/QuatIntegrator.cc:2959-2992; Kim
+ * forms of source/KimMobilityStrategy*.cc).  cl, cs may be NULL for the
+ * uniform mobility. */
+int ampe_phase_mobility(const ampe_level* level, const ampe_model_params* p,
+                        const double* cl, const double* cs, double* M,
+                        ampe_stream_t stream);
 
 /* num_sweeps red-black Gauss-Seidel sweeps on A u = rhs (u ghosted, ghost
  * fill before each colour), with early exit when max|res| <= tol (tol >= 0).
@@ -425,11 +457,13 @@ int ampe_sim_advance(ampe_sim* sim, int nsteps, ampe_stats* stats);
 /* Device time (ms) of the last ampe_sim_advance, from HIP events. */
 int ampe_sim_last_time_ms(const ampe_sim* sim, double* ms);
 /* Surrogate used when the solver's kks_guess is AMPE_GUESS_SURROGATE (the
- * integrator keeps the pointer; the object must outlive the sim).  Inputs are
- * (c, phi, T) per cell; after probe_its Newton updates a residual above
- * probe_tol restarts the cell from the previous-step guess. */
+ * integrator keeps the pointer; the object must outlive the sim).  A cell
+ * whose previous-step guess has a KKS residual above switch_tol starts from
+ * the surrogate's (cL, cS) for inputs (c, phi, T) instead; after probe_its
+ * Newton updates a residual above probe_tol restarts it from the
+ * previous-step guess.  switch_tol = 0 starts every cell from the surrogate. */
 int ampe_sim_set_surrogate(ampe_sim* sim, const ampe_surrogate* s,
-                           int probe_its, double probe_tol);
+                           int probe_its, double probe_tol, double switch_tol);
 
 #ifdef __cplusplus
 }

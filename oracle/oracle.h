@@ -89,7 +89,10 @@ void o_tag_cells(const OGeom* G, const ampe_model_params* p,
 
 /* elliptic FAC pieces */
 void o_phase_fac_C(const OGeom* G, const ampe_model_params* p, double gamma,
-                   const double* phi, double* C);
+                   const double* phi, const double* M, double* C);
+double o_phase_mobility(const ampe_model_params* p, double cl, double cs);
+void o_phase_mobility_field(const OGeom* G, const ampe_model_params* p, const double* cl,
+                            const double* cs, double* M);
 double o_rbgs_color(const OGeom* G, const ampe_elliptic_op* op,
                     const double* rhs, double* u, int color, double* flux);
 double o_fac_smooth(const OGeom* G, const ampe_elliptic_op* op,

@@ -80,7 +80,9 @@ def _setup(L):
         "o_conc_rhs": (None, [_G, _P, _dp, _dp, _dp, _dp, _dp]),
         "o_kks_solve": (C.c_int, [_G, _P, _dp, _dp, _dp, _dp, _dp, _dp]),
         "o_tag_cells": (None, [_G, _P, _dp, C.c_double, _ip]),
-        "o_phase_fac_C": (None, [_G, _P, C.c_double, _dp, _dp]),
+        "o_phase_fac_C": (None, [_G, _P, C.c_double, _dp, _dp, _dp]),
+        "o_phase_mobility": (C.c_double, [_P, C.c_double, C.c_double]),
+        "o_phase_mobility_field": (None, [_G, _P, _dp, _dp, _dp]),
         "o_fac_smooth": (C.c_double, [_G, C.POINTER(EllipticOp), _dp, _dp, C.c_int, C.c_double, _dp]),
         "o_fac_residual": (C.c_double, [_G, C.POINTER(EllipticOp), _dp, _dp, _dp, _dp]),
         "o_dot": (C.c_double, [_G, C.c_int, _dp, _dp]),

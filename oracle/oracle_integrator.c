@@ -57,7 +57,7 @@ struct OSim {
    double** V;
    double *cl, *cs, *clr, *csr;
    double *n2, *Dc, *Dphi;
-   double *Cphi, *Dconc, *fc, *sqrtm, *mq;
+   double *Cphi, *Dconc, *fc, *sqrtm, *mq, *Mphi;
    double *fac_u, *fac_e, *fac_r, *fac_flux;
    double* H;
    ampe_stats st;
@@ -90,6 +90,7 @@ OSim* o_sim_create(const ampe_model_params* p, const ampe_solver_params* s,
    S->clr = vec(ng);
    S->csr = vec(ng);
    S->Cphi = vec(ng);
+   S->Mphi = vec(ng);
    S->sqrtm = vec(ng);
    S->mq = vec(ng);
    S->fac_u = vec(ng * (p->evolve_quat ? p->qlen : 1));
@@ -111,7 +112,7 @@ void o_sim_destroy(OSim* S)
    if (!S) return;
    double* fs[] = {S->y, S->yprev, S->ypred, S->ycur, S->ahist, S->fcur, S->b,
                    S->del, S->ewt, S->work, S->work2, S->jv, S->xk, S->corr,
-                   S->acor, S->cl, S->cs, S->clr, S->csr, S->Cphi, S->sqrtm,
+                   S->acor, S->cl, S->cs, S->clr, S->csr, S->Cphi, S->Mphi, S->sqrtm,
                    S->mq, S->fac_u, S->fac_e, S->fac_r, S->n2, S->Dc, S->Dphi,
                    S->Dconc, S->fc, S->fac_flux, S->H};
    for (size_t i = 0; i < sizeof(fs) / sizeof(fs[0]); i++) free(fs[i]);
@@ -236,7 +237,9 @@ static int psetup(OSim* S, double* y, double gamma)
    }
    /* phase block: M = M_phi, C = 1 + gamma M w g'', D = -gamma eps^2
     * (source/PhaseFACOps.cc:33-51) */
-   o_phase_fac_C(G, p, gamma, phi, S->Cphi);
+   int kim = p->phi_mobility_type == AMPE_PHI_MOB_KIM;
+   if (kim) o_phase_mobility_field(G, p, S->cl, S->cs, S->Mphi);
+   o_phase_fac_C(G, p, gamma, phi, kim ? S->Mphi : NULL, S->Cphi);
    /* composition block: D = -gamma D_c (lagged), C = 1, M = M_c
     * (source/ConcFACOps.cc:28-61) */
    if (c)
@@ -410,7 +413,9 @@ static int psolve(OSim* S, const double* r, double* z, double delta)
    S->st.nps++;
    memset(z, 0, sizeof(double) * S->nv);
    ampe_elliptic_op ph = {NULL, -S->gamma_setup * p->epsilon_phi * p->epsilon_phi,
-                          S->Cphi, 1.0, NULL, p->phi_mobility,
+                          S->Cphi, 1.0,
+                          p->phi_mobility_type == AMPE_PHI_MOB_KIM ? S->Mphi : NULL,
+                          p->phi_mobility,
                           {G->dx[0], G->dx[1], G->dx[2]}};
    if (fac_solve(S, &ph, r, z, delta)) ret = 1;
    if (S->iq >= 0) {
@@ -444,8 +449,8 @@ static int atimes(OSim* S, double gamma, double* v, double* out)
    return 0;
 }
 
-/* SPGMR, left preconditioned, MGS, x0 = 0, s1 = s2 = ewt
- * (SUNLinSol_SPGMR as used by CVODE).  Returns 0 converged, 1 residual
+/* SPGMR, left preconditioned, MGS (or classical GS when gstype says so),
+ * x0 = 0, s1 = s2 = ewt (SUNLinSol_SPGMR as used by CVODE).  Returns 0 converged, 1 residual
  * reduced but not converged, 2 failure. */
 static int spgmr(OSim* S, double gamma, const double* bvec, double* x,
                  double delta, int* nli_out)
@@ -477,13 +482,41 @@ static int spgmr(OSim* S, double gamma, const double* bvec, double* x,
       if (atimes(S, gamma, S->xk, V[l + 1])) return 2;
       if (psolve(S, V[l + 1], S->work, delta) < 0) return 2;
       for (long n = 0; n < nv; n++) V[l + 1][n] = S->ewt[n] * S->work[n];
-      /* modified Gram-Schmidt */
-      for (int i = 0; i <= l; i++) {
-         double hij = o_dot(G, S->nc, V[i], V[l + 1]);
-         HH(i, l) = hij;
-         for (long n = 0; n < nv; n++) V[l + 1][n] = V[l + 1][n] + (-hij) * V[i][n];
+      double hn;
+      if (S->s.gstype == AMPE_GS_CLASSICAL) {
+         /* classical Gram-Schmidt with one conditional re-orthogonalisation
+          * (SUNClassicalGS, SUNDIALS 5.x sundials_iterative.c: all dots of the
+          * new vector, one linear combination, re-orthogonalise when the
+          * norm dropped by more than a factor 1000) */
+         double st[34];
+         for (int i = 0; i <= l + 1; i++) st[i] = o_dot(G, S->nc, V[l + 1], V[i]);
+         double vk_norm = sqrt(st[l + 1]);
+         for (int i = 0; i <= l; i++) HH(i, l) = st[i];
+         for (long n = 0; n < nv; n++) {
+            double acc = V[l + 1][n];
+            for (int i = 0; i <= l; i++) acc = acc + (-st[i]) * V[i][n];
+            V[l + 1][n] = acc;
+         }
+         hn = sqrt(o_dot(G, S->nc, V[l + 1], V[l + 1]));
+         if (1000.0 * hn < vk_norm) {
+            for (int i = 0; i <= l; i++) st[i] = o_dot(G, S->nc, V[l + 1], V[i]);
+            for (int i = 0; i <= l; i++) HH(i, l) += st[i];
+            for (long n = 0; n < nv; n++) {
+               double acc = V[l + 1][n];
+               for (int i = 0; i <= l; i++) acc = acc + (-st[i]) * V[i][n];
+               V[l + 1][n] = acc;
+            }
+            hn = sqrt(o_dot(G, S->nc, V[l + 1], V[l + 1]));
+         }
+      } else {
+         /* modified Gram-Schmidt */
+         for (int i = 0; i <= l; i++) {
+            double hij = o_dot(G, S->nc, V[i], V[l + 1]);
+            HH(i, l) = hij;
+            for (long n = 0; n < nv; n++) V[l + 1][n] = V[l + 1][n] + (-hij) * V[i][n];
+         }
+         hn = sqrt(o_dot(G, S->nc, V[l + 1], V[l + 1]));
       }
-      double hn = sqrt(o_dot(G, S->nc, V[l + 1], V[l + 1]));
       HH(l + 1, l) = hn;
       /* apply previous Givens rotations to column l, then a new one */
       for (int i = 0; i < l; i++) {

@@ -190,11 +190,27 @@ static double quad_ceq(const ampe_model_params* p, int ph)
    return p->quad_ceq[ph] + p->quad_m[ph] * (p->temperature - p->quad_Tref);
 }
 
+/* Dilute binary solution (restatement of the published dilute model; the
+ * Thermo4PFM KKSFreeEnergyFunctionDiluteBinary energy is not available, parity
+ * of the energy itself unpinned): f_L = RT (c ln c - c),
+ * f_S = f_L - RT ln(k_e) c + dG, dG = RT (k_e - 1)(T - T_m)/m. */
+static double dilute_dG(const ampe_model_params* p)
+{
+   double RT = R_GAS * p->temperature;
+   return RT * (p->dilute_keq - 1.0) * (p->temperature - p->dilute_Tm) / p->dilute_m;
+}
+
 double o_fenergy(const ampe_model_params* p, double c, int ph)
 {
    if (p->fe_model == AMPE_FE_QUADRATIC) {
       double d = c - quad_ceq(p, ph);
       return p->quad_A[ph] * d * d;
+   }
+   if (p->fe_model == AMPE_FE_KKS_DILUTE) {
+      double RT = R_GAS * p->temperature;
+      double f = RT * (c * log(c) - c);
+      if (ph == 1) f = f - RT * log(p->dilute_keq) * c + dilute_dG(p);
+      return f;
    }
    const ampe_calphad_phase* cp = &p->calphad[ph];
    double RT = R_GAS * p->temperature;
@@ -209,6 +225,12 @@ double o_dfenergy(const ampe_model_params* p, double c, int ph)
    if (p->fe_model == AMPE_FE_QUADRATIC) {
       return 2.0 * p->quad_A[ph] * (c - quad_ceq(p, ph));
    }
+   if (p->fe_model == AMPE_FE_KKS_DILUTE) {
+      double RT = R_GAS * p->temperature;
+      double mu = RT * log(c);
+      if (ph == 1) mu = mu - RT * log(p->dilute_keq);
+      return mu;
+   }
    const ampe_calphad_phase* cp = &p->calphad[ph];
    double RT = R_GAS * p->temperature;
    double x = 2.0 * c - 1.0;
@@ -221,6 +243,7 @@ double o_dfenergy(const ampe_model_params* p, double c, int ph)
 double o_d2fenergy(const ampe_model_params* p, double c, int ph)
 {
    if (p->fe_model == AMPE_FE_QUADRATIC) return 2.0 * p->quad_A[ph];
+   if (p->fe_model == AMPE_FE_KKS_DILUTE) return R_GAS * p->temperature / c;
    const ampe_calphad_phase* cp = &p->calphad[ph];
    double RT = R_GAS * p->temperature;
    double x = 2.0 * c - 1.0;
@@ -229,6 +252,38 @@ double o_d2fenergy(const ampe_model_params* p, double c, int ph)
    double d2P = 2.0 * cp->L[2] + x * 6.0 * cp->L[3];
    return RT / (c * (1.0 - c)) - 2.0 * P + 4.0 * (1.0 - 2.0 * c) * dP +
           4.0 * c * (1.0 - c) * d2P;
+}
+
+/* Phase mobility at one cell.  Constant M_phi (SimpleQuatMobilityStrategy)
+ * or Kim's KKS mobility (KimMobilityStrategyInfMob::evaluateMobility,
+ * source/KimMobilityStrategyInfMob.cc:54-83; FiniteMob, header :29-38;
+ * FiniteMobAntiTrap :54-80) with the factor of kks_mobility_factor
+ * (source/KKStools.cc:13-30): zeta = (cL - cS) f_L''(cL) (cL - cS) / D_L(T),
+ * converted by 1e-6 / v_m; InfMob: factor / zeta; FiniteMob: 1 / alpha;
+ * FiniteMobAntiTrap: 1 / (alpha + zeta / factor), alpha = 3 sqrt(2) xi / mu,
+ * xi = eps / sqrt(16 w). */
+double o_phase_mobility(const ampe_model_params* p, double cl, double cs)
+{
+   if (p->phi_mobility_type != AMPE_PHI_MOB_KIM) return p->phi_mobility;
+   double eps = p->epsilon_phi, w = p->phi_well_scale;
+   double a2 = (p->energy_interp == 'p') ? 47. / 60. : 0.5;
+   double factor = 16. * w / (3. * eps * eps * a2);
+   double alpha = 0.0;
+   int finite = p->kim_interface_mobility > 0.0;
+   if (finite) {
+      double xi = eps / sqrt(16. * w);
+      alpha = 3. * sqrt(2.) * xi / p->kim_interface_mobility;
+      if (!p->kim_antitrapping) return 1. / alpha;
+   }
+   double T = p->temperature;
+   double DL = p->D_liquid * exp(-p->Q0_liquid / (R_GAS * T));
+   double d2 = o_d2fenergy(p, cl, 0);
+   double zeta = 0.;
+   zeta += (cl - cs) * d2 * (cl - cs);
+   double zf = zeta / DL;
+   zf *= (1.e-6 / p->vm_L);
+   if (finite) return 1. / (alpha + (1. / factor) * zf);
+   return factor / zf;
 }
 
 /* Per-cell KKS: find (cL, cS) with mu_L(cL) = mu_S(cS) and
@@ -249,6 +304,14 @@ int o_kks_cell(const ampe_model_params* p, double c, double phi, double* cl,
       double xl = (c - h * (eS - r * eL)) / (1.0 - h + h * r);
       *cl = xl;
       *cs = eS + r * (xl - eL);
+      return 0;
+   }
+   if (p->fe_model == AMPE_FE_KKS_DILUTE) {
+      /* mu_L = mu_S  <=>  cS = k_e cL; mass balance gives cL */
+      double k = p->dilute_keq;
+      double xl = c / (1.0 - h + h * k);
+      *cl = xl;
+      *cs = k * xl;
       return 0;
    }
    double RTinv = 1.0 / (R_GAS * p->temperature);
@@ -306,6 +369,19 @@ int o_kks_solve(const OGeom* G, const ampe_model_params* p, const double* phi,
       else if (it > maxit) maxit = it;
    }
    return fail ? -1 : maxit;
+}
+
+/* Phase mobility field over interior cells (QuatIntegrator::computeMobilities,
+ * source/QuatIntegrator.cc:2959-2992, after the phase concentrations). */
+void o_phase_mobility_field(const OGeom* G, const ampe_model_params* p, const double* cl,
+                            const double* cs, double* M)
+{
+   for (int k = 0; k < G->nz; k++)
+      for (int j = 0; j < G->ny; j++)
+         for (int i = 0; i < G->nx; i++) {
+            long c0 = CI(i, j, k);
+            M[c0] = o_phase_mobility(p, cl[c0], cs[c0]);
+         }
 }
 
 /* ------------------------------------------------------------------------
@@ -673,7 +749,10 @@ void o_phase_rhs(const OGeom* G, const ampe_model_params* p,
                double hp = o_deriv_interp_func(ph, p->energy_interp);
                r += hp * ((f_l - f_a) - mu * (c_l - c_a));
             }
-            rhs[c0] = p->phi_mobility * r;
+            double mob = (p->phi_mobility_type == AMPE_PHI_MOB_KIM)
+                             ? o_phase_mobility(p, cl[c0], cs[c0])
+                             : p->phi_mobility;
+            rhs[c0] = mob * r;
          }
    free(flux);
    free(gmod);
@@ -916,14 +995,14 @@ void o_tag_cells(const OGeom* G, const ampe_model_params* p,
 
 /* PhaseFACOps::setCOnPatchPrivate (source/PhaseFACOps.cc:101-183) */
 void o_phase_fac_C(const OGeom* G, const ampe_model_params* p, double gamma,
-                   const double* phi, double* C)
+                   const double* phi, const double* M, double* C)
 {
    for (int k = 0; k < G->nz; k++)
       for (int j = 0; j < G->ny; j++)
          for (int i = 0; i < G->nx; i++) {
             long c0 = CI(i, j, k);
             double g2 = o_second_deriv_well_func(phi[c0], p->phi_well_type);
-            double gm = gamma * p->phi_mobility;
+            double gm = gamma * (M ? M[c0] : p->phi_mobility);
             C[c0] = 1.0 + gm * p->phi_well_scale * g2;
          }
 }

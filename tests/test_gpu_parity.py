@@ -67,6 +67,8 @@ def _rel(a, b):
 def _params(ndim, kind):
     if kind == "calphad":
         return P.polycrystal_params(ndim=ndim)
+    if kind == "dilute":
+        return P.dilute_params(ndim=ndim)
     p = P.two_grain_params(ndim=ndim)
     if kind == "iso":
         p.flux_type = P.FLUX_ISOTROPIC
@@ -95,7 +97,8 @@ def test_quat_grad(dev, ndim):
 
 
 @pytest.mark.parametrize("ndim,kind", [(2, "simple"), (2, "iso"), (2, "aniso"), (3, "simple"),
-                                       (3, "aniso"), (2, "calphad"), (3, "calphad")])
+                                       (3, "aniso"), (2, "calphad"), (3, "calphad"),
+                                       (2, "dilute"), (3, "dilute")])
 def test_phase_rhs(dev, ndim, kind):
     p = _params(ndim, kind)
     O, H, pt, lev = _setup(ndim, p)
@@ -106,7 +109,7 @@ def test_phase_rhs(dev, ndim, kind):
     ds = [_dev(a, dev) for a in (phi, q, cl, cs)]
     r = _dev(np.zeros_like(r_o), dev)
     H.check(H.lib().ampe_phase_rhs(lev.h, C.byref(p), *[_p(t) for t in ds], _p(r), None))
-    tol = 1e-12 if kind in ("aniso", "calphad") else 1e-13
+    tol = 1e-12 if kind in ("aniso", "calphad", "dilute") else 1e-13
     assert _rel(pt.interior(r.cpu().numpy()), pt.interior(r_o)) < tol
 
 
@@ -158,7 +161,7 @@ def test_conc(dev, ndim):
     assert _rel(pt.interior(r.cpu().numpy()), pt.interior(r_o)) < 1e-13
 
 
-@pytest.mark.parametrize("ndim,kind", [(2, "simple"), (3, "calphad")])
+@pytest.mark.parametrize("ndim,kind", [(2, "simple"), (3, "calphad"), (3, "dilute")])
 def test_kks(dev, ndim, kind):
     p = _params(ndim, kind)
     O, H, pt, lev = _setup(ndim, p)
@@ -193,6 +196,45 @@ def test_tags_bit_exact(dev, ndim):
     assert 0 < tags_o.sum() < pt.N
 
 
+@pytest.mark.parametrize("variant", ["inf", "finite", "antitrap"])
+def test_kim_phase_mobility(dev, variant):
+    """Kim's KKS phase mobility (three reference variants): the mobility field,
+    the phase preconditioner coefficient C built from it and the phase RHS that
+    multiplies by it, GPU vs oracle (dilute CuNi model, 3-D)."""
+    p = P.dilute_params(ndim=3, phi_mobility_type=P.PHI_MOB_KIM, D_liquid=1.0e3,
+                        Q0_liquid=0.0, energy_interp=b"h")
+    if variant != "inf":
+        p.kim_interface_mobility = P.kim_interface_mobility(p.vm_L, p.dilute_Tm, p.dilute_m,
+                                                            p.dilute_keq, 2.0e-5)
+        p.kim_antitrapping = 1 if variant == "antitrap" else 0
+    O, H, pt, lev = _setup(3, p)
+    phi, q, c, cl, cs = _fields(pt, seed=41)
+    M_o = pt.cells()
+    O.lib().o_phase_mobility_field(C.byref(pt.G), C.byref(p), O.ptr(cl), O.ptr(cs), O.ptr(M_o))
+    cld, csd, phd = _dev(cl, dev), _dev(cs, dev), _dev(phi, dev)
+    Md = _dev(pt.cells(), dev)
+    H.check(H.lib().ampe_phase_mobility(lev.h, C.byref(p), _p(cld), _p(csd), _p(Md), None))
+    Mg = pt.interior(Md.cpu().numpy())
+    assert _rel(Mg, pt.interior(M_o)) < 1e-14
+    assert np.all(np.isfinite(Mg)) and np.all(Mg > 0)
+    if variant == "finite":
+        assert np.ptp(Mg) == 0.0  # 1 / alpha, independent of the concentrations
+    gamma = 2.0e-6
+    C_o = pt.cells()
+    O.lib().o_phase_fac_C(C.byref(pt.G), C.byref(p), gamma, O.ptr(phi), O.ptr(M_o), O.ptr(C_o))
+    Cd = _dev(pt.cells(), dev)
+    H.check(H.lib().ampe_fac_phase_coeffs(lev.h, C.byref(p), gamma, _p(phd), _p(Md), _p(Cd),
+                                          None))
+    assert _rel(pt.interior(Cd.cpu().numpy()), pt.interior(C_o)) < 1e-14
+    r_o = pt.cells()
+    O.lib().o_phase_rhs(C.byref(pt.G), C.byref(p), O.ptr(phi), O.ptr(q), O.ptr(cl), O.ptr(cs),
+                        O.ptr(r_o))
+    ds = [_dev(a, dev) for a in (phi, q, cl, cs)]
+    r = _dev(np.zeros_like(r_o), dev)
+    H.check(H.lib().ampe_phase_rhs(lev.h, C.byref(p), *[_p(t) for t in ds], _p(r), None))
+    assert _rel(pt.interior(r.cpu().numpy()), pt.interior(r_o)) < 1e-12
+
+
 @pytest.mark.parametrize("ndim", [2, 3])
 def test_fac_smooth_residual(dev, ndim):
     p = _params(ndim, "simple")
@@ -200,10 +242,11 @@ def test_fac_smooth_residual(dev, ndim):
     phi, q, c, cl, cs = _fields(pt, seed=29)
     gamma = 1.0e-3
     C_o = pt.cells()
-    O.lib().o_phase_fac_C(C.byref(pt.G), C.byref(p), gamma, O.ptr(phi), O.ptr(C_o))
+    O.lib().o_phase_fac_C(C.byref(pt.G), C.byref(p), gamma, O.ptr(phi), None, O.ptr(C_o))
     phd = _dev(phi, dev)
     Cd = _dev(pt.cells(), dev)
-    H.check(H.lib().ampe_fac_phase_coeffs(lev.h, C.byref(p), gamma, _p(phd), _p(Cd), None))
+    H.check(H.lib().ampe_fac_phase_coeffs(lev.h, C.byref(p), gamma, _p(phd), None, _p(Cd),
+                                          None))
     assert _rel(pt.interior(Cd.cpu().numpy()), pt.interior(C_o)) < 1e-14
     rhs = pt.load(pt.cells(), _rng(31).normal(size=(pt.nz, pt.ny, pt.nx)))
     Dside = -gamma * _rng(37).uniform(1.0, 5.0, pt.nf)
@@ -339,12 +382,13 @@ def _two_grain(n, ndim=2):
     return p, phi, c, q
 
 
-def test_integrator_fp64_matches_oracle(dev):
+@pytest.mark.parametrize("gstype", [P.GS_MODIFIED, P.GS_CLASSICAL])
+def test_integrator_fp64_matches_oracle(dev, gstype):
     from oracle import oracle as O
     from ampe_amd import hip as H
     n = 48
     p, phi, c, q = _two_grain(n)
-    s = P.solver_defaults(maxl=10, max_newton=5)
+    s = P.solver_defaults(maxl=10, max_newton=5, gstype=gstype)
     osim = O.Sim(p, s, n)
     osim.set_state(phi, c, q)
     box = H.boxes_of((n, n), 2)
@@ -357,6 +401,43 @@ def test_integrator_fp64_matches_oracle(dev):
         fo = osim.get_state()
         fg = gsim.get_state((n, n))
         for name, a, b in zip(("phi", "c", "q"), fg[:3], fo[:3]):
+            e = _rel(a, b)
+            assert e < 1e-10, (step, name, e, sto, stg)
+        assert sto["nni"] == stg["nni"] and sto["nli"] == stg["nli"], (sto, stg)
+
+
+@pytest.mark.parametrize("model,gstype", [("calphad", P.GS_MODIFIED),
+                                          ("calphad", P.GS_CLASSICAL),
+                                          ("dilute_kim", P.GS_MODIFIED)])
+def test_integrator_3d_kks_matches_oracle(dev, model, gstype):
+    """3-D polycrystal with a KKS composition model, FP64 GPU integrator against
+    the oracle step by step: binary CALPHAD (Newton inside every RHS
+    evaluation) with both Gram-Schmidt variants, and the dilute solution with
+    Kim's phase mobility (mobility field in the RHS and the preconditioner)."""
+    from oracle import oracle as O
+    from ampe_amd import hip as H
+    n = 16
+    if model == "calphad":
+        p = P.polycrystal_params(seed=1002)
+        dt = 1e-7
+    else:
+        p = P.dilute_params(ndim=3, phi_mobility_type=P.PHI_MOB_KIM, D_liquid=1.0e3,
+                            Q0_liquid=0.0)
+        dt = 1e-9
+    phi, c, q = P.polycrystal_state(n=n, seed=1002, ngrains=4)
+    s = P.solver_defaults(dt=dt, maxl=10, max_newton=5, atol=1e-6, rtol=1e-8, gstype=gstype)
+    osim = O.Sim(p, s, n)
+    osim.set_state(phi, c, q)
+    box = H.boxes_of((n, n, n), 3)
+    gsim = H.Sim(p, s, [box], periodic=box)
+    gsim.set_state(phi, c, q)
+    for step in range(3):
+        ro, sto = osim.advance(1)
+        rg, stg = gsim.advance(1)
+        assert ro == rg == 0
+        fo = osim.get_state()
+        fg = gsim.get_state((n, n, n))
+        for name, a, b in zip(("phi", "c", "q", "cl", "cs"), fg, fo):
             e = _rel(a, b)
             assert e < 1e-10, (step, name, e, sto, stg)
         assert sto["nni"] == stg["nni"] and sto["nli"] == stg["nli"], (sto, stg)

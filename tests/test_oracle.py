@@ -191,7 +191,7 @@ def test_rbgs_converges_to_manufactured_solution(O, variant):
     op = O.EllipticOp()
     Cf = pt.cells()
     if variant == "phase":
-        O.lib().o_phase_fac_C(C.byref(pt.G), C.byref(p), gamma, O.ptr(phi), O.ptr(Cf))
+        O.lib().o_phase_fac_C(C.byref(pt.G), C.byref(p), gamma, O.ptr(phi), None, O.ptr(Cf))
         op.C = Cf.ctypes.data
         op.D_const = -gamma * p.epsilon_phi ** 2
         op.M_const = p.phi_mobility
