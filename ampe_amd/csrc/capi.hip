@@ -142,6 +142,23 @@ int ampe_conc_coeffs(const ampe_level* level, const ampe_model_params* p, const 
    return ampe::launch_conc_coeffs(*level->L, *p, phi, cl, cs, Dc, Dphi, hs(stream));
 }
 
+int ampe_conc_coeffs_ebs(const ampe_level* level, const ampe_model_params* p,
+                         const double* phi, double* DL, double* DA, ampe_stream_t stream)
+{
+   AMPE_REQUIRE(level && phi && DL && DA, "null argument");
+   if (check_params(p, *level->L)) return -1;
+   return ampe::launch_ebs_coeffs(*level->L, *p, phi, DL, DA, hs(stream));
+}
+
+int ampe_conc_rhs_ebs(const ampe_level* level, const ampe_model_params* p, const double* cl,
+                      const double* cs, const double* DL, const double* DA, double* rhs,
+                      ampe_stream_t stream)
+{
+   AMPE_REQUIRE(level && cl && cs && DL && DA && rhs, "null argument");
+   if (check_params(p, *level->L)) return -1;
+   return ampe::launch_conc_rhs_ebs(*level->L, *p, cl, cs, DL, DA, rhs, hs(stream));
+}
+
 int ampe_conc_rhs(const ampe_level* level, const ampe_model_params* p, const double* c,
                   const double* phi, const double* Dc, const double* Dphi, double* rhs,
                   ampe_stream_t stream)

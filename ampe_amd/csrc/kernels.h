@@ -127,6 +127,11 @@ int launch_conc_coeffs(const Level& L, const ampe_model_params& p, const double*
 int launch_conc_rhs(const Level& L, const ampe_model_params& p, const double* c,
                     const double* phi, const double* Dc, const double* Dphi, double* rhs,
                     hipStream_t s);
+int launch_ebs_coeffs(const Level& L, const ampe_model_params& p, const double* phi, double* DL,
+                      double* DA, hipStream_t s);
+int launch_conc_rhs_ebs(const Level& L, const ampe_model_params& p, const double* cl,
+                        const double* cs, const double* DL, const double* DA, double* rhs,
+                        hipStream_t s);
 int launch_phase_mobility(const Level& L, const ampe_model_params& p, const double* cl,
                           const double* cs, double* M, hipStream_t s);
 int launch_tag(const Level& L, const ampe_model_params& p, const double* phi, double tol,

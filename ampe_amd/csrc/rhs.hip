@@ -415,6 +415,72 @@ __global__ void __launch_bounds__(256) k_conc_rhs(LevelArgs a, ampe_model_params
 AMPE_COLUMN_END
 }
 
+// EBS face diffusivities of the two phases:
+// concentration_pfmdiffusion_of_temperature (2d/concentrationdiffusion.m4:341-428)
+// as TbasedCompositionDiffusionStrategy::setDiffusion calls it
+// (source/TbasedCompositionDiffusionStrategy.cc:305-322), uniform temperature
+template <int NDIM>
+__global__ void __launch_bounds__(256) k_ebs_coeffs(LevelArgs a, ampe_model_params p,
+                                                    const double* __restrict__ phi,
+                                                    double* __restrict__ DL,
+                                                    double* __restrict__ DA)
+{
+   CellPos c;
+   if (!cell_pos(a, 1, c)) return;
+   if (NDIM == 2 && c.k > 0) return;
+   double qlr = p.Q0_liquid / kRGas, qsr = p.Q0_solid / kRGas;
+   for (int d = 0; d < NDIM; d++) {
+      bool ok = (c.i < c.pd.nx + (d == 0)) && (c.j < c.pd.ny + (d == 1)) &&
+                (NDIM == 2 || c.k < c.pd.nz + (d == 2));
+      if (!ok) continue;
+      long long c0 = c.c0, cm = c0 - c.st[d];
+      long long fi = face_idx(a, c.pd, d, c.i, c.j, c.k);
+      double vphi = face_avg(phi[cm], phi[c0], p.avg_type);
+      double hphi = interp(vphi, p.diff_interp);
+      double invT = 2.0 / (p.temperature + p.temperature);
+      double dl = p.D_liquid * exp(-qlr * invT);
+      double ds = p.D_solid * exp(-qsr * invT);
+      DL[fi] = (1.0 - hphi) * dl;
+      DA[fi] = hphi * ds;
+   }
+}
+
+// EBSCompositionRHSStrategy::computeFluxOnPatch (source/EBSCompositionRHSStrategy.cc
+// :183-328): add_flux (2d/flux.m4:11-66) of the liquid then the solid phase
+// onto a zero flux, then computerhsconcentration (2d/concentrationrhs.m4:326-367)
+template <int NDIM>
+__global__ void __launch_bounds__(256) k_conc_rhs_ebs(LevelArgs a, ampe_model_params p,
+                                                      const double* __restrict__ cl,
+                                                      const double* __restrict__ cs,
+                                                      const double* __restrict__ DL,
+                                                      const double* __restrict__ DA,
+                                                      double* __restrict__ rhs)
+{
+   AMPE_COLUMN_BEGIN(a, c)
+   const long long c0 = c.c0;
+   const int ijk[3] = {c.i, c.j, c.k};
+   double s = 0.0;
+#pragma unroll
+   for (int d = 0; d < NDIM; d++) {
+      double dinv = 1.0 / p.dx[d];
+      int lo[3] = {ijk[0], ijk[1], ijk[2]};
+      long long flo = face_idx(a, c.pd, d, lo[0], lo[1], lo[2]);
+      lo[d] += 1;
+      long long fhi = face_idx(a, c.pd, d, lo[0], lo[1], lo[2]);
+      long long cm = c0 - c.st[d], cp = c0 + c.st[d];
+      double jl = 0.0;
+      jl = jl + dinv * DL[flo] * (cl[c0] - cl[cm]);
+      jl = jl + dinv * DA[flo] * (cs[c0] - cs[cm]);
+      double jh = 0.0;
+      jh = jh + dinv * DL[fhi] * (cl[cp] - cl[c0]);
+      jh = jh + dinv * DA[fhi] * (cs[cp] - cs[c0]);
+      double t = dinv * (jh - jl);
+      s = (d == 0) ? t : s + t;
+   }
+   rhs[c0] = p.conc_mobility * s;
+AMPE_COLUMN_END
+}
+
 // QuatMobilityStrategy::computePhaseMobility over interior cells
 // (QuatIntegrator::computeMobilities, source/QuatIntegrator.cc:2959-2992)
 __global__ void __launch_bounds__(256) k_phase_mobility(LevelArgs a, ampe_model_params p,
@@ -516,6 +582,21 @@ int launch_conc_rhs(const Level& L, const ampe_model_params& p, const double* c,
                     hipStream_t s)
 {
    AMPE_DISPATCH(L, k_conc_rhs, L.ncol_tiles, level_args_col(L), p, c, phi, Dc, Dphi, rhs);
+   return 0;
+}
+
+int launch_ebs_coeffs(const Level& L, const ampe_model_params& p, const double* phi, double* DL,
+                      double* DA, hipStream_t s)
+{
+   AMPE_DISPATCH(L, k_ebs_coeffs, L.nface_tiles, level_args(L, true), p, phi, DL, DA);
+   return 0;
+}
+
+int launch_conc_rhs_ebs(const Level& L, const ampe_model_params& p, const double* cl,
+                        const double* cs, const double* DL, const double* DA, double* rhs,
+                        hipStream_t s)
+{
+   AMPE_DISPATCH(L, k_conc_rhs_ebs, L.ncol_tiles, level_args_col(L), p, cl, cs, DL, DA, rhs);
    return 0;
 }
 

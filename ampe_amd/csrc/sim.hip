@@ -61,6 +61,10 @@ int sim_create(const ampe_model_params& p, const ampe_solver_params& s, const am
       set_error("sim_create: Kim's phase mobility needs a composition field");
       return -1;
    }
+   if (p.conc_rhs_form != AMPE_CONC_RHS_KKS && p.conc_rhs_form != AMPE_CONC_RHS_EBS) {
+      set_error("sim_create: unknown composition rhs form");
+      return -1;
+   }
    Sim* S = new Sim();
    S->p = p;
    S->s = s;
@@ -296,7 +300,13 @@ static int rhs_eval(Sim* S, double* y, double* ydot, bool fd)
    if (c && kks(S, y)) return -1;
    if (launch_phase_rhs(L, p, phi, q, S->cl, S->cs, ydot, st)) return -1;
    if (q && launch_quat_rhs(L, p, phi, q, S->n2, ydot + (long long)S->iq * S->ng, st)) return -1;
-   if (c) {
+   if (c && p.conc_rhs_form == AMPE_CONC_RHS_EBS) {
+      // Dc, Dphi hold the liquid and solid face diffusivities
+      if (recompute && launch_ebs_coeffs(L, p, phi, S->Dc, S->Dphi, st)) return -1;
+      if (launch_conc_rhs_ebs(L, p, S->cl, S->cs, S->Dc, S->Dphi,
+                              ydot + (long long)S->ic * S->ng, st))
+         return -1;
+   } else if (c) {
       if (recompute && launch_conc_coeffs(L, p, phi, S->cl, S->cs, S->Dc, S->Dphi, st)) return -1;
       if (launch_conc_rhs(L, p, c, phi, S->Dc, S->Dphi, ydot + (long long)S->ic * S->ng, st))
          return -1;
@@ -323,8 +333,15 @@ static int psetup(Sim* S, double* y, double gamma)
    bool kim = p.phi_mobility_type == AMPE_PHI_MOB_KIM;
    if (kim && launch_phase_mobility(L, p, S->cl, S->cs, S->Mphi, st)) return -1;
    if (launch_phase_C(L, p, gamma, phi, kim ? S->Mphi : nullptr, S->Cphi, st)) return -1;
-   // composition block: D = -gamma D_c (source/ConcFACOps.cc:28-61)
-   if (c && vec_axpby(S->nf, -gamma, S->Dc, 0.0, S->Dc, S->Dconc, st)) return -1;
+   // composition block: D = -gamma D_c (source/ConcFACOps.cc:28-61); for EBS
+   // D_c = D_L + D_A (EBSCompositionRHSStrategy::
+   // setDiffusionCoeffForPreconditionerOnPatch)
+   if (c && p.conc_rhs_form == AMPE_CONC_RHS_EBS) {
+      if (vec_axpby(S->nf, 1.0, S->Dc, 1.0, S->Dphi, S->Dconc, st)) return -1;
+      if (vec_axpby(S->nf, -gamma, S->Dconc, 0.0, S->Dconc, S->Dconc, st)) return -1;
+   } else if (c && vec_axpby(S->nf, -gamma, S->Dc, 0.0, S->Dc, S->Dconc, st)) {
+      return -1;
+   }
    // orientation block (source/QuatFACOps.cc:735-829)
    if (q) {
       if (launch_quat_face_coef(L, p, phi, S->n2, S->fc, st)) return -1;

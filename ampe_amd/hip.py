@@ -43,6 +43,8 @@ _SIGS = {
     "ampe_conc_rhs": (_i, [_vp, C.POINTER(ModelParams), _vp, _vp, _vp, _vp, _vp, _vp]),
     "ampe_kks_solve": (_i, [_vp, C.POINTER(ModelParams), _vp, _vp, _vp, _vp, _vp, _vp,
                             C.POINTER(_i), _vp]),
+    "ampe_conc_coeffs_ebs": (_i, [_vp, C.POINTER(ModelParams), _vp, _vp, _vp, _vp]),
+    "ampe_conc_rhs_ebs": (_i, [_vp, C.POINTER(ModelParams), _vp, _vp, _vp, _vp, _vp, _vp]),
     "ampe_tag_cells": (_i, [_vp, C.POINTER(ModelParams), _vp, _d, _vp, _vp]),
     "ampe_fac_phase_coeffs": (_i, [_vp, C.POINTER(ModelParams), _d, _vp, _vp, _vp, _vp]),
     "ampe_phase_mobility": (_i, [_vp, C.POINTER(ModelParams), _vp, _vp, _vp, _vp]),

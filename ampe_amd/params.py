@@ -88,6 +88,8 @@ class ModelParams(C.Structure):
         ("phi_mobility_type", C.c_int),
         ("kim_interface_mobility", C.c_double),
         ("kim_antitrapping", C.c_int),
+        ("conc_rhs_form", C.c_int),
+        ("diff_interp", C.c_char),
     ]
 
 
@@ -131,6 +133,7 @@ class Stats(C.Structure):
 FLUX_SIMPLE, FLUX_ISOTROPIC, FLUX_ANISOTROPIC = 0, 1, 2
 FE_QUADRATIC, FE_CALPHAD_BINARY, FE_KKS_DILUTE = 0, 1, 2
 PHI_MOB_CONSTANT, PHI_MOB_KIM = 0, 1
+CONC_RHS_KKS, CONC_RHS_EBS = 0, 1
 FP64, FAC_FP32 = 0, 1
 GUESS_PREVIOUS, GUESS_SURROGATE = 0, 1
 PRECOND_REDBLACK, PRECOND_MG = 0, 1
@@ -301,6 +304,7 @@ def two_grain_params(ndim=2, **kw):
     p.calphad = auni_phases(873.0)
     p.newton_max_its = 50
     p.newton_tol = 1.0e-12
+    p.diff_interp = b"l"
     for k, v in kw.items():
         setattr(p, k, v)
     return p

@@ -118,6 +118,10 @@ typedef struct ampe_model_params {
    int phi_mobility_type;         /* enum ampe_phi_mobility */
    double kim_interface_mobility; /* Kim: <= 0 infinite interface mobility */
    int kim_antitrapping;          /* Kim, finite mobility: anti-trapping form */
+   /* composition right-hand side (ConcentrationModel rhs_form) */
+   int conc_rhs_form;     /* enum ampe_conc_rhs_form */
+   char diff_interp;      /* EBS phase weights h(phi) of the diffusivities
+                           * (diffusion_interp_func_type, 'l' linear, 'p' pbg) */
 } ampe_model_params;
 
 /* Phase mobility: the uniform M_phi (SimpleQuatMobilityStrategy,
@@ -126,6 +130,10 @@ typedef struct ampe_model_params {
  * InfMob / FiniteMob / FiniteMobAntiTrap, source/KimMobilityStrategy*.cc,
  * factor of source/KKStools.cc:13-30), evaluated per cell from (cL, cS). */
 enum ampe_phi_mobility { AMPE_PHI_MOB_CONSTANT = 0, AMPE_PHI_MOB_KIM = 1 };
+/* Composition RHS: KKS (KKSCompositionRHSStrategy, flux from D_c grad c and
+ * D_phi grad phi) or EBS (EBSCompositionRHSStrategy, flux sum over phases of
+ * D_phase grad c_phase, source/EBSCompositionRHSStrategy.cc:183-328). */
+enum ampe_conc_rhs_form { AMPE_CONC_RHS_KKS = 0, AMPE_CONC_RHS_EBS = 1 };
 enum ampe_precision { AMPE_FP64 = 0, AMPE_FAC_FP32 = 1 };
 enum ampe_kks_guess { AMPE_GUESS_PREVIOUS = 0, AMPE_GUESS_SURROGATE = 1 };
 /* Block solver inside each FAC cycle: the reference's single-level red-black
@@ -258,6 +266,22 @@ int ampe_kks_solve(const ampe_level* level, const ampe_model_params* p,
                    const double* phi, const double* c, const double* cl_ref,
                    const double* cs_ref, double* cl, double* cs,
                    int* max_iters, ampe_stream_t stream);
+
+/* EBS composition model, two phases: face diffusivities D_L = (1-h) D_L0
+ * exp(-Q_L/RT), D_A = h D_S0 exp(-Q_S/RT), h = diff_interp(avg(phi)) (the
+ * temperature-dependent strategy's concentration_pfmdiffusion_of_temperature,
+ * source/fortran/2d/concentrationdiffusion.m4:341-428 via
+ * TbasedCompositionDiffusionStrategy::setDiffusion), then
+ * rhs = M_c div(D_L grad cL + D_A grad cS) (EBSCompositionRHSStrategy::
+ * computeFluxOnPatch with add_flux, source/fortran/2d/flux.m4:11-66, and
+ * computerhsconcentration).  cl, cs need filled ghosts (the KKS solve
+ * covers them); DL, DA are side fields. */
+int ampe_conc_coeffs_ebs(const ampe_level* level, const ampe_model_params* p,
+                         const double* phi, double* DL, double* DA,
+                         ampe_stream_t stream);
+int ampe_conc_rhs_ebs(const ampe_level* level, const ampe_model_params* p,
+                      const double* cl, const double* cs, const double* DL,
+                      const double* DA, double* rhs, ampe_stream_t stream);
 
 /* AMR gradient detector: tags = 1 where |grad phi * dx|^2 >= tol^2, else 0
  * (QuatModel::tagGradientDetectorCells source/QuatModel.cc:3476-3602,

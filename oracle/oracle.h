@@ -56,6 +56,10 @@ double o_d2fenergy(const ampe_model_params* p, double c, int phase);
 int o_kks_cell(const ampe_model_params* p, double c, double phi, double* cl,
                double* cs);
 
+void o_ebs_coeffs(const OGeom* G, const ampe_model_params* p, const double* phi, double* DL,
+                  double* DA);
+void o_conc_rhs_ebs(const OGeom* G, const ampe_model_params* p, const double* cl,
+                    const double* cs, const double* DL, const double* DA, double* rhs);
 void o_ghost_fill_periodic(const OGeom* G, double* f, int ncomp);
 
 void o_phase_flux(const OGeom* G, const ampe_model_params* p,

@@ -79,6 +79,8 @@ def _setup(L):
         "o_conc_coeffs": (None, [_G, _P, _dp, _dp, _dp, _dp, _dp]),
         "o_conc_rhs": (None, [_G, _P, _dp, _dp, _dp, _dp, _dp]),
         "o_kks_solve": (C.c_int, [_G, _P, _dp, _dp, _dp, _dp, _dp, _dp]),
+        "o_ebs_coeffs": (None, [_G, _P, _dp, _dp, _dp]),
+        "o_conc_rhs_ebs": (None, [_G, _P, _dp, _dp, _dp, _dp, _dp]),
         "o_tag_cells": (None, [_G, _P, _dp, C.c_double, _ip]),
         "o_phase_fac_C": (None, [_G, _P, C.c_double, _dp, _dp, _dp]),
         "o_phase_mobility": (C.c_double, [_P, C.c_double, C.c_double]),

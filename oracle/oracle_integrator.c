@@ -210,8 +210,14 @@ static int rhs_eval(OSim* S, double* y, double* ydot, int fd)
    o_phase_rhs(G, p, phi, q, S->cl, S->cs, ydot);
    if (q) o_quat_rhs(G, p, phi, q, S->n2, ydot + (long)S->iq * ng);
    if (c) {
-      if (recompute) o_conc_coeffs(G, p, phi, S->cl, S->cs, S->Dc, S->Dphi);
-      o_conc_rhs(G, p, c, phi, S->Dc, S->Dphi, ydot + (long)S->ic * ng);
+      if (p->conc_rhs_form == AMPE_CONC_RHS_EBS) {
+         /* Dc, Dphi hold the liquid and solid face diffusivities */
+         if (recompute) o_ebs_coeffs(G, p, phi, S->Dc, S->Dphi);
+         o_conc_rhs_ebs(G, p, S->cl, S->cs, S->Dc, S->Dphi, ydot + (long)S->ic * ng);
+      } else {
+         if (recompute) o_conc_coeffs(G, p, phi, S->cl, S->cs, S->Dc, S->Dphi);
+         o_conc_rhs(G, p, c, phi, S->Dc, S->Dphi, ydot + (long)S->ic * ng);
+      }
    }
    return 0;
 }
@@ -243,7 +249,13 @@ static int psetup(OSim* S, double* y, double gamma)
    /* composition block: D = -gamma D_c (lagged), C = 1, M = M_c
     * (source/ConcFACOps.cc:28-61) */
    if (c)
-      for (long f = 0; f < S->nf; f++) S->Dconc[f] = -gamma * S->Dc[f];
+      for (long f = 0; f < S->nf; f++) {
+         /* EBS: D_precond = 0 + D_L + D_A (EBSCompositionRHSStrategy::
+          * setDiffusionCoeffForPreconditionerOnPatch) */
+         double d = (p->conc_rhs_form == AMPE_CONC_RHS_EBS) ? (0.0 + S->Dc[f]) + S->Dphi[f]
+                                                              : S->Dc[f];
+         S->Dconc[f] = -gamma * d;
+      }
    /* orientation block: face coefficients and sqrt(M_q) with ghosts
     * (source/QuatFACOps.cc:735-829) */
    if (q) {

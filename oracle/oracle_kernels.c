@@ -965,6 +965,71 @@ void o_conc_rhs(const OGeom* G, const ampe_model_params* p, const double* c,
    free(fl);
 }
 
+/* EBS face diffusivities, two phases, uniform temperature:
+ * concentration_pfmdiffusion_of_temperature (2d/concentrationdiffusion.m4
+ * :341-428) as called by TbasedCompositionDiffusionStrategy::setDiffusion
+ * (source/TbasedCompositionDiffusionStrategy.cc:305-322). */
+void o_ebs_coeffs(const OGeom* G, const ampe_model_params* p, const double* phi, double* DL,
+                  double* DA)
+{
+   long st[3] = {1, G->sy, G->sz};
+   double qlr = p->Q0_liquid / R_GAS, qsr = p->Q0_solid / R_GAS;
+   for (int dir = 0; dir < G->ndim; dir++) {
+      int ex = G->nx + (dir == 0), ey = G->ny + (dir == 1), ez = G->nz + (dir == 2);
+      long off = (dir == 0 ? 0 : dir == 1 ? YOFF : ZOFF);
+      for (int k = 0; k < ez; k++)
+         for (int j = 0; j < ey; j++)
+            for (int i = 0; i < ex; i++) {
+               long c0 = CI(i, j, k), cm = c0 - st[dir];
+               long fi = off + (long)i + (long)ex * ((long)j + (long)ey * k);
+               double vphi = o_average_func(phi[cm], phi[c0], p->avg_type);
+               double hphi = o_interp_func(vphi, p->diff_interp);
+               double invT = 2.0 / (p->temperature + p->temperature);
+               double dl = p->D_liquid * exp(-qlr * invT);
+               double ds = p->D_solid * exp(-qsr * invT);
+               DL[fi] = (1.0 - hphi) * dl;
+               DA[fi] = hphi * ds;
+            }
+   }
+}
+
+/* EBSCompositionRHSStrategy::computeFluxOnPatch (source/EBSCompositionRHSStrategy.cc
+ * :183-328) with add_flux (2d/flux.m4:11-66, the liquid then the solid
+ * contribution added to a zeroed flux) and computerhsconcentration
+ * (2d/concentrationrhs.m4:326-367). */
+void o_conc_rhs_ebs(const OGeom* G, const ampe_model_params* p, const double* cl,
+                    const double* cs, const double* DL, const double* DA, double* rhs)
+{
+   long st[3] = {1, G->sy, G->sz};
+   double dinv[3] = {1.0 / G->dx[0], 1.0 / G->dx[1], 1.0 / G->dx[2]};
+   long nf = G->nfx + G->nfy + G->nfz;
+   double* fl = (double*)malloc(sizeof(double) * nf);
+   for (int dir = 0; dir < G->ndim; dir++) {
+      int ex = G->nx + (dir == 0), ey = G->ny + (dir == 1), ez = G->nz + (dir == 2);
+      long off = (dir == 0 ? 0 : dir == 1 ? YOFF : ZOFF);
+      for (int k = 0; k < ez; k++)
+         for (int j = 0; j < ey; j++)
+            for (int i = 0; i < ex; i++) {
+               long c0 = CI(i, j, k), cm = c0 - st[dir];
+               long fi = off + (long)i + (long)ex * ((long)j + (long)ey * k);
+               double f = 0.0;
+               f = f + dinv[dir] * DL[fi] * (cl[c0] - cl[cm]);
+               f = f + dinv[dir] * DA[fi] * (cs[c0] - cs[cm]);
+               fl[fi] = f;
+            }
+   }
+   for (int k = 0; k < G->nz; k++)
+      for (int j = 0; j < G->ny; j++)
+         for (int i = 0; i < G->nx; i++) {
+            double s = dinv[0] * (fl[FX(i + 1, j, k)] - fl[FX(i, j, k)]) +
+                       dinv[1] * (fl[YOFF + FY(i, j + 1, k)] - fl[YOFF + FY(i, j, k)]);
+            if (G->ndim == 3)
+               s = s + dinv[2] * (fl[ZOFF + FZ(i, j, k + 1)] - fl[ZOFF + FZ(i, j, k)]);
+            rhs[CI(i, j, k)] = p->conc_mobility * s;
+         }
+   free(fl);
+}
+
 /* tagfromgrads (2d/quattag.m4:9-55) on the cell gradient of phi
  * (QuatModel::computePhaseGradCell: diffs + cell average, quatgrad_cell
  * formula).  tags: interior cells, i fastest, no ghosts. */

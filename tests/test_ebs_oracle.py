@@ -1,0 +1,138 @@
+"""CPU checks of the EBS composition right-hand side in the oracle
+(EBSCompositionRHSStrategy: flux = sum over phases of D_phase grad c_phase,
+with D_phase carrying the phase weight).  Pinned by closed forms: the discrete
+divergence of the flux of a quadratic phase composition, conservation on a
+periodic patch, and the reduction to the KKS-form right-hand side when both
+phases share the composition and phi is uniform (then D_L + D_A is the KKS
+mixture diffusivity and the phi-gradient flux vanishes)."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from ampe_amd import params as P
+from oracle import oracle as O
+
+
+def _ebs(pt, p, phi, cl, cs):
+    DL, DA = np.zeros(pt.nf), np.zeros(pt.nf)
+    O.lib().o_ebs_coeffs(C.byref(pt.G), C.byref(p), O.ptr(phi), O.ptr(DL), O.ptr(DA))
+    r = pt.cells()
+    O.lib().o_conc_rhs_ebs(C.byref(pt.G), C.byref(p), O.ptr(cl), O.ptr(cs), O.ptr(DL),
+                           O.ptr(DA), O.ptr(r))
+    return DL, DA, r
+
+
+def _diffusivities(p):
+    dl = p.D_liquid * np.exp(-p.Q0_liquid / (P.R_GAS * p.temperature))
+    ds = p.D_solid * np.exp(-p.Q0_solid / (P.R_GAS * p.temperature))
+    return dl, ds
+
+
+def _centres(pt, dx):
+    """cell-centre coordinates over the whole ghosted array."""
+    gz = pt.gz
+    k, j, i = np.meshgrid(*[np.arange(s) for s in pt.gshape], indexing="ij")
+    return ((i - pt.g + 0.5) * dx[0], (j - pt.g + 0.5) * dx[1], (k - gz + 0.5) * dx[2])
+
+
+@pytest.mark.parametrize("ndim", [2, 3])
+@pytest.mark.parametrize("phase", ["solid", "liquid"])
+def test_single_phase_quadratic(ndim, phase):
+    """phi = 1 (solid) or 0 (liquid) everywhere: only that phase's diffusivity
+    survives, and for c_phase = a x^2 + b y^2 (+ e z^2) the discrete flux
+    divergence is 2 (a + b (+ e)) exactly, so rhs = M D_phase 2 (a + b + e)."""
+    dx = (0.5, 0.25, 0.125)
+    p = P.two_grain_params(ndim=ndim, dx=dx)
+    pt = O.Patch(ndim, (12, 10, 8), dx=dx)
+    x, y, z = _centres(pt, dx)
+    a, b, e = 0.3, -0.7, 1.1
+    quad = a * x ** 2 + b * y ** 2 + (e * z ** 2 if ndim == 3 else 0.0)
+    lin = 0.2 + 0.1 * x - 0.3 * y
+    phival = 1.0 if phase == "solid" else 0.0
+    phi = np.full(pt.gshape, phival)[None]
+    cl = (lin if phase == "solid" else quad)[None].copy()
+    cs = (quad if phase == "solid" else lin)[None].copy()
+    DL, DA, r = _ebs(pt, p, phi, cl, cs)
+    dl, ds = _diffusivities(p)
+    if phase == "solid":
+        assert np.all(DL == 0.0) and np.allclose(DA, ds, rtol=1e-15, atol=0)
+        d = ds
+    else:
+        assert np.all(DA == 0.0) and np.allclose(DL, dl, rtol=1e-15, atol=0)
+        d = dl
+    want = p.conc_mobility * d * 2.0 * (a + b + (e if ndim == 3 else 0.0))
+    ri = pt.interior(r)
+    assert np.max(np.abs(ri - want)) < 1e-9 * abs(want)
+
+
+@pytest.mark.parametrize("ndim", [2, 3])
+def test_conservation_periodic(ndim):
+    rng = np.random.Generator(np.random.Philox(31))
+    pt = O.Patch(ndim, (16, 12, 10))
+    p = P.two_grain_params(ndim=ndim)
+    shape = (pt.nz, pt.ny, pt.nx)
+    phi = pt.load(pt.cells(), rng.uniform(0, 1, shape))
+    cl = pt.load(pt.cells(), rng.uniform(0.2, 0.6, shape))
+    cs = pt.load(pt.cells(), rng.uniform(0.2, 0.6, shape))
+    DL, DA, r = _ebs(pt, p, phi, cl, cs)
+    ri = pt.interior(r)
+    assert abs(ri.sum()) < 1e-12 * np.abs(ri).sum()
+    assert np.abs(ri).max() > 0
+    dl, ds = _diffusivities(p)
+    # linear weights: D_L / dl + D_A / ds = 1 face by face
+    assert np.max(np.abs(DL / dl + DA / ds - 1.0)) < 1e-14
+
+
+@pytest.mark.parametrize("ndim", [2, 3])
+@pytest.mark.parametrize("interp", ["l", "p"])
+def test_reduces_to_kks_form(ndim, interp):
+    """cl = cs = c and uniform phi: EBS rhs = KKS-form rhs when the diffusion
+    and energy interpolations agree."""
+    rng = np.random.Generator(np.random.Philox(37))
+    pt = O.Patch(ndim, (14, 12, 8))
+    p = P.two_grain_params(ndim=ndim)
+    p.energy_interp = interp.encode()
+    p.diff_interp = interp.encode()
+    shape = (pt.nz, pt.ny, pt.nx)
+    phi = pt.load(pt.cells(), np.full(shape, 0.37))
+    c = pt.load(pt.cells(), rng.uniform(0.2, 0.6, shape))
+    DL, DA, r = _ebs(pt, p, phi, c, c)
+    Dc, Dp = np.zeros(pt.nf), np.zeros(pt.nf)
+    O.lib().o_conc_coeffs(C.byref(pt.G), C.byref(p), O.ptr(phi), O.ptr(c), O.ptr(c),
+                          O.ptr(Dc), O.ptr(Dp))
+    rk = pt.cells()
+    O.lib().o_conc_rhs(C.byref(pt.G), C.byref(p), O.ptr(c), O.ptr(phi), O.ptr(Dc), O.ptr(Dp),
+                       O.ptr(rk))
+    assert np.max(np.abs(DL + DA - Dc)) < 1e-15 * np.abs(Dc).max()
+    a, b = pt.interior(r), pt.interior(rk)
+    assert np.linalg.norm(a - b) < 1e-12 * np.linalg.norm(b)
+
+
+def test_diffusion_interpolation_is_its_own_parameter():
+    """h of the diffusivities follows diff_interp, not energy_interp."""
+    pt = O.Patch(2, (8, 8, 1))
+    p = P.two_grain_params(ndim=2)
+    phi = pt.load(pt.cells(), np.full((1, 8, 8), 0.25))
+    dl, ds = _diffusivities(p)
+    for ch, h in ((b"l", 0.25), (b"p", 0.25 ** 3 * (10 - 15 * 0.25 + 6 * 0.25 ** 2))):
+        p.diff_interp = ch
+        DL, DA, _ = _ebs(pt, p, phi, phi, phi)
+        assert np.allclose(DA, h * ds, rtol=1e-14, atol=0)
+        assert np.allclose(DL, (1 - h) * dl, rtol=1e-14, atol=0)
+
+
+def test_integrator_ebs_conserves_composition():
+    """A few oracle integrator steps with the EBS form on a periodic 2-D
+    two-grain state keep the mean composition."""
+    n = 32
+    p = P.two_grain_params(ndim=2, quad_ceq=(0.40, 0.45), conc_rhs_form=P.CONC_RHS_EBS)
+    phi, c, q = P.two_grain_state(n=n, radius=10.0)
+    s = P.solver_defaults(dt=1e-3, maxl=10, max_newton=5, atol=1e-6, rtol=1e-8)
+    sim = O.Sim(p, s, n)
+    sim.set_state(phi, c, q)
+    rc, st = sim.advance(3)
+    assert rc == 0
+    _, c1, _, cl, cs = sim.get_state()
+    assert abs(c1.mean() - c.mean()) < 1e-12
+    assert np.abs(c1 - c).max() > 0

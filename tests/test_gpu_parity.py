@@ -161,6 +161,31 @@ def test_conc(dev, ndim):
     assert _rel(pt.interior(r.cpu().numpy()), pt.interior(r_o)) < 1e-13
 
 
+@pytest.mark.parametrize("ndim", [2, 3])
+@pytest.mark.parametrize("interp", ["l", "p"])
+def test_conc_ebs(dev, ndim, interp):
+    """EBS composition form: phase-weighted face diffusivities and the
+    divergence of D_L grad cL + D_A grad cS."""
+    p = _params(ndim, "simple")
+    p.diff_interp = interp.encode()
+    O, H, pt, lev = _setup(ndim, p)
+    phi, q, c, cl, cs = _fields(pt, seed=29)
+    DL_o, DA_o = np.zeros(pt.nf), np.zeros(pt.nf)
+    O.lib().o_ebs_coeffs(C.byref(pt.G), C.byref(p), O.ptr(phi), O.ptr(DL_o), O.ptr(DA_o))
+    r_o = pt.cells()
+    O.lib().o_conc_rhs_ebs(C.byref(pt.G), C.byref(p), O.ptr(cl), O.ptr(cs), O.ptr(DL_o),
+                           O.ptr(DA_o), O.ptr(r_o))
+    phd, cld, csd = (_dev(a, dev) for a in (phi, cl, cs))
+    DL, DA = _dev(np.zeros(pt.nf), dev), _dev(np.zeros(pt.nf), dev)
+    H.check(H.lib().ampe_conc_coeffs_ebs(lev.h, C.byref(p), _p(phd), _p(DL), _p(DA), None))
+    assert _rel(DL.cpu().numpy(), DL_o) < 1e-13
+    assert _rel(DA.cpu().numpy(), DA_o) < 1e-13
+    r = _dev(np.zeros_like(r_o), dev)
+    H.check(H.lib().ampe_conc_rhs_ebs(lev.h, C.byref(p), _p(cld), _p(csd), _p(DL), _p(DA),
+                                      _p(r), None))
+    assert _rel(pt.interior(r.cpu().numpy()), pt.interior(r_o)) < 1e-13
+
+
 @pytest.mark.parametrize("ndim,kind", [(2, "simple"), (3, "calphad"), (3, "dilute")])
 def test_kks(dev, ndim, kind):
     p = _params(ndim, kind)
@@ -408,17 +433,21 @@ def test_integrator_fp64_matches_oracle(dev, gstype):
 
 @pytest.mark.parametrize("model,gstype", [("calphad", P.GS_MODIFIED),
                                           ("calphad", P.GS_CLASSICAL),
-                                          ("dilute_kim", P.GS_MODIFIED)])
+                                          ("dilute_kim", P.GS_MODIFIED),
+                                          ("calphad_ebs", P.GS_MODIFIED)])
 def test_integrator_3d_kks_matches_oracle(dev, model, gstype):
     """3-D polycrystal with a KKS composition model, FP64 GPU integrator against
     the oracle step by step: binary CALPHAD (Newton inside every RHS
-    evaluation) with both Gram-Schmidt variants, and the dilute solution with
-    Kim's phase mobility (mobility field in the RHS and the preconditioner)."""
+    evaluation) with both Gram-Schmidt variants and with the EBS composition
+    flux, and the dilute solution with Kim's phase mobility (mobility field in
+    the RHS and the preconditioner)."""
     from oracle import oracle as O
     from ampe_amd import hip as H
     n = 16
-    if model == "calphad":
+    if model.startswith("calphad"):
         p = P.polycrystal_params(seed=1002)
+        if model == "calphad_ebs":
+            p.conc_rhs_form = P.CONC_RHS_EBS
         dt = 1e-7
     else:
         p = P.dilute_params(ndim=3, phi_mobility_type=P.PHI_MOB_KIM, D_liquid=1.0e3,
