@@ -97,6 +97,19 @@ __device__ __forceinline__ bool cell_pos_col(const LevelArgs& a, int kk, CellPos
       ;                 \
    }
 
+// Launch-uniform constants of the right-hand-side kernels (built on the host
+// from the model parameters by rhs_const).
+struct RhsConst {
+   double dxinv[3];   // 1/dx
+   double hdxinv[3];  // 0.5/dx
+   double e2dx[3];    // eps_phi^2/dx
+   double e2dx12[3];  // eps_phi^2/(12 dx)
+   double convL, convA;  // 1e-6/v_m of the liquid and the solid
+   double maxgni;     // 1/gradient_floor
+   double Dl, Ds;     // D_L0 exp(-Q_L/RT), D_S0 exp(-Q_S/RT)
+};
+RhsConst rhs_const(const ampe_model_params& p);
+
 // Index of face (i,j,k) of direction d of patch pd in a side field.
 __device__ __forceinline__ long long face_idx(const LevelArgs& a, const PatchDev& pd,
                                               int d, int i, int j, int k)

@@ -107,7 +107,7 @@ __global__ void __launch_bounds__(256) k_kks(long long n, ampe_model_params p, M
       kks_clamp(xl, xs);
       double f1, f2;
       kks_residual(p.calphad, T, 1.0 / (kRGas * T), h, ct, xl, xs, f1, f2);
-      bool bad = in && !(dmax(fabs(f1), fabs(f2)) <= switch_tol);
+      bool bad = in && (switch_tol <= 0.0 || !(dmax(fabs(f1), fabs(f2)) <= switch_tol));
       int r = 0;
       if (__any(bad)) {
          double gl, gs;

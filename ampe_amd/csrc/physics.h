@@ -94,11 +94,13 @@ AMPE_HD double face_avg(double a, double b, char type)
    return 0.5 * (a + b);
 }
 
-// eval_grad_normi (source/fortran/quat.f:1497-1537)
+// eval_grad_normi (source/fortran/quat.f:1497-1537).  x**(-0.5) is evaluated
+// as 1/sqrt(x) (correctly rounded sqrt and divide: within 2 ulp of the power)
+// instead of a general pow, which costs a log and an exp per face.
 AMPE_HD double grad_normi(double gn2, char floor_type, double floor2,
                           double max_gni)
 {
-   if (floor_type == 'm') return (gn2 > floor2) ? pow(gn2, -0.5) : max_gni;
+   if (floor_type == 'm') return (gn2 > floor2) ? 1.0 / sqrt(gn2) : max_gni;
    if (floor_type == 't') {
       double gng2 = gn2 * max_gni * max_gni;
       if (gng2 > 0.01) {
@@ -107,7 +109,7 @@ AMPE_HD double grad_normi(double gn2, char floor_type, double floor2,
       }
       return max_gni * (1.0 - gng2 * (5.0 - 2.0 * gng2) / 15.0);
    }
-   return pow(gn2 + floor2, -0.5);
+   return 1.0 / sqrt(gn2 + floor2);
 }
 
 // quatmobility (source/fortran/2d/mobility.m4:13-90)
@@ -182,13 +184,17 @@ AMPE_HD double dilute_dG(const ampe_model_params& p)
    return RT * (p.dilute_keq - 1.0) * (p.temperature - p.dilute_Tm) / p.dilute_m;
 }
 
-AMPE_HD double fenergy(const ampe_model_params& p, double c, int ph)
+// The energy functions take the model as a template argument FE when a kernel
+// is specialised for it; FE < 0 reads p.fe_model at run time.
+template <int FE>
+AMPE_HD double fenergy_t(const ampe_model_params& p, double c, int ph)
 {
-   if (p.fe_model == AMPE_FE_QUADRATIC) {
+   const int fe = (FE < 0) ? p.fe_model : FE;
+   if (fe == AMPE_FE_QUADRATIC) {
       double d = c - quad_ceq(p, ph);
       return p.quad_A[ph] * d * d;
    }
-   if (p.fe_model == AMPE_FE_KKS_DILUTE) {
+   if (fe == AMPE_FE_KKS_DILUTE) {
       double RT = kRGas * p.temperature;
       double f = RT * (c * log(c) - c);
       if (ph == 1) f = f - RT * log(p.dilute_keq) * c + dilute_dG(p);
@@ -197,11 +203,12 @@ AMPE_HD double fenergy(const ampe_model_params& p, double c, int ph)
    return calphad_g(p.calphad[ph], p.temperature, c);
 }
 
-AMPE_HD double dfenergy(const ampe_model_params& p, double c, int ph)
+template <int FE>
+AMPE_HD double dfenergy_t(const ampe_model_params& p, double c, int ph)
 {
-   if (p.fe_model == AMPE_FE_QUADRATIC)
-      return 2.0 * p.quad_A[ph] * (c - quad_ceq(p, ph));
-   if (p.fe_model == AMPE_FE_KKS_DILUTE) {
+   const int fe = (FE < 0) ? p.fe_model : FE;
+   if (fe == AMPE_FE_QUADRATIC) return 2.0 * p.quad_A[ph] * (c - quad_ceq(p, ph));
+   if (fe == AMPE_FE_KKS_DILUTE) {
       double RT = kRGas * p.temperature;
       double mu = RT * log(c);
       if (ph == 1) mu = mu - RT * log(p.dilute_keq);
@@ -210,11 +217,26 @@ AMPE_HD double dfenergy(const ampe_model_params& p, double c, int ph)
    return calphad_mu(p.calphad[ph], p.temperature, c);
 }
 
+template <int FE>
+AMPE_HD double d2fenergy_t(const ampe_model_params& p, double c, int ph)
+{
+   const int fe = (FE < 0) ? p.fe_model : FE;
+   if (fe == AMPE_FE_QUADRATIC) return 2.0 * p.quad_A[ph];
+   if (fe == AMPE_FE_KKS_DILUTE) return kRGas * p.temperature / c;
+   return calphad_d2(p.calphad[ph], p.temperature, c);
+}
+
+AMPE_HD double fenergy(const ampe_model_params& p, double c, int ph)
+{
+   return fenergy_t<-1>(p, c, ph);
+}
+AMPE_HD double dfenergy(const ampe_model_params& p, double c, int ph)
+{
+   return dfenergy_t<-1>(p, c, ph);
+}
 AMPE_HD double d2fenergy(const ampe_model_params& p, double c, int ph)
 {
-   if (p.fe_model == AMPE_FE_QUADRATIC) return 2.0 * p.quad_A[ph];
-   if (p.fe_model == AMPE_FE_KKS_DILUTE) return kRGas * p.temperature / c;
-   return calphad_d2(p.calphad[ph], p.temperature, c);
+   return d2fenergy_t<-1>(p, c, ph);
 }
 
 // Phase mobility at one cell: the uniform M_phi, or Kim's KKS mobility
@@ -223,7 +245,8 @@ AMPE_HD double d2fenergy(const ampe_model_params& p, double c, int ph)
 // :29-38; FiniteMobAntiTrap, .cc:54-80) with kks_mobility_factor
 // (source/KKStools.cc:13-30) and zeta = (cL - cS) f_L''(cL) (cL - cS) / D_L(T)
 // converted by 1e-6 / v_m.
-AMPE_HD double phase_mobility(const ampe_model_params& p, double cl, double cs)
+template <int FE>
+AMPE_HD double phase_mobility_t(const ampe_model_params& p, double cl, double cs)
 {
    if (p.phi_mobility_type != AMPE_PHI_MOB_KIM) return p.phi_mobility;
    double eps = p.epsilon_phi, w = p.phi_well_scale;
@@ -238,13 +261,18 @@ AMPE_HD double phase_mobility(const ampe_model_params& p, double cl, double cs)
    }
    double T = p.temperature;
    double DL = p.D_liquid * exp(-p.Q0_liquid / (kRGas * T));
-   double d2 = d2fenergy(p, cl, 0);
+   double d2 = d2fenergy_t<FE>(p, cl, 0);
    double zeta = 0.;
    zeta += (cl - cs) * d2 * (cl - cs);
    double zf = zeta / DL;
    zf *= (1.e-6 / p.vm_L);
    if (finite) return 1. / (alpha + (1. / factor) * zf);
    return factor / zf;
+}
+
+AMPE_HD double phase_mobility(const ampe_model_params& p, double cl, double cs)
+{
+   return phase_mobility_t<-1>(p, cl, cs);
 }
 
 // End-member G(T) of a tdep table: interval k is the last with T >= Tc[k]

@@ -9,6 +9,7 @@
 #include "kernels.h"
 
 #include <algorithm>
+#include <cmath>
 #include <cstring>
 
 namespace ampe {
@@ -16,6 +17,29 @@ namespace ampe {
 // Single precision literal 1.e-12 of anisotropic_gradient_flux promoted to
 // double (source/fortran/2d/quatrhs.m4:193, 3d/quatrhs.m4:184).
 #define AMPE_ANISO_THRESHOLD ((double)1.e-12f)
+
+// Launch-uniform quantities derived from the model on the host: IEEE division
+// gives the device's value, and the kernels do no per-cell division or exp
+// for them.
+RhsConst rhs_const(const ampe_model_params& p)
+{
+   RhsConst k;
+   double e2 = p.epsilon_phi * p.epsilon_phi;
+   for (int d = 0; d < 3; d++) {
+      k.dxinv[d] = 1.0 / p.dx[d];
+      k.hdxinv[d] = 0.5 / p.dx[d];
+      k.e2dx[d] = e2 / p.dx[d];
+      k.e2dx12[d] = (1.0 / 12.0) * e2 / p.dx[d];
+   }
+   k.convL = 1.0e-6 / p.vm_L;
+   k.convA = 1.0e-6 / p.vm_A;
+   k.maxgni = 1.0 / p.gradient_floor;
+   // concentration_pfmdiffusion at the uniform temperature (face T = (T+T)/2)
+   double invT = 2.0 / (p.temperature + p.temperature);
+   k.Dl = p.D_liquid * std::exp(-(p.Q0_liquid / kRGas) * invT);
+   k.Ds = p.D_solid * std::exp(-(p.Q0_solid / kRGas) * invT);
+   return k;
+}
 
 // ------------------------------------------------------------ phase flux
 // Face between cells cm = c0 - st[d] and c0, direction d.
@@ -31,18 +55,20 @@ __device__ __forceinline__ void quatmult4(const double* a, const double* b, doub
    r[3] = a[0] * b[3] + a[3] * b[0] + a[1] * b[2] - a[2] * b[1];
 }
 
-template <int NDIM>
-__device__ double phase_face_flux(const ampe_model_params& p, const double* __restrict__ ph,
-                                  const double* __restrict__ q, long long ncg,
-                                  const long long* st, int d, long long cm, long long c0)
+// FLUX < 0 reads p.flux_type at run time.
+template <int NDIM, int FLUX = -1>
+__device__ double phase_face_flux(const ampe_model_params& p, const RhsConst& k,
+                                  const double* __restrict__ ph, const double* __restrict__ q,
+                                  long long ncg, const long long* st, int d, long long cm,
+                                  long long c0)
 {
-   const double* dx = p.dx;
-   if (p.flux_type == AMPE_FLUX_ANISOTROPIC) {
+   const int ft = (FLUX < 0) ? p.flux_type : FLUX;
+   if (ft == AMPE_FLUX_ANISOTROPIC) {
       if (NDIM == 2) {
          double eps = p.epsilon_phi, nu = p.aniso_nu;
          int kn = p.aniso_knumber;
          double pi = 4.0 * atan(1.0);
-         double dinv0 = 1.0 / dx[0], dinv1 = 1.0 / dx[1];
+         double dinv0 = k.dxinv[0], dinv1 = k.dxinv[1];
          int t = 1 - d;  // transverse direction
          double gn = (ph[c0] - ph[cm]) * (d == 0 ? dinv0 : dinv1);
          double gt = 0.25 * (ph[cm + st[t]] - ph[cm - st[t]] + ph[c0 + st[t]] - ph[c0 - st[t]]) *
@@ -60,7 +86,7 @@ __device__ double phase_face_flux(const ampe_model_params& p, const double* __re
          return epst * epst * dphidy + epst * deps * dphidx;
       } else {
          double eps = p.epsilon_phi, eps4 = p.aniso_nu;
-         double dinv[3] = {1.0 / dx[0], 1.0 / dx[1], 1.0 / dx[2]};
+         const double* dinv = k.dxinv;
          double factor = 4.0 * eps4 / (1.0 - 3.0 * eps4);
          double gr[3];
 #pragma unroll
@@ -96,24 +122,22 @@ __device__ double phase_face_flux(const ampe_model_params& p, const double* __re
          return gamma * gamma * gr[d] + 16.0 * eps * gamma * eps4 * sqrt(gphi2) * dg[d + 1];
       }
    }
-   double e2 = p.epsilon_phi * p.epsilon_phi;
-   if (p.flux_type == AMPE_FLUX_ISOTROPIC && NDIM == 2) {
-      double dinv = (1.0 / 12.0) * e2 / dx[d];
+   if (ft == AMPE_FLUX_ISOTROPIC && NDIM == 2) {
+      double dinv = k.e2dx12[d];
       long long t = st[1 - d];
       return dinv * ((ph[c0 - t] - ph[cm - t]) + (ph[c0] - ph[cm]) * 10.0 + (ph[c0 + t] - ph[cm + t]));
    }
-   double dinv = e2 / dx[d];
-   return (ph[c0] - ph[cm]) * dinv;
+   return (ph[c0] - ph[cm]) * k.e2dx[d];
 }
 
 // |grad q| from cell-centred gradients: quatdiffs -> quatgrad_cell ->
 // quatgrad_modulus (2d/quatgrad.m4:12-60, 882-915).
 template <int NDIM>
-__device__ __forceinline__ double quat_cell_modulus(const ampe_model_params& p,
+__device__ __forceinline__ double quat_cell_modulus(const ampe_model_params& p, const RhsConst& k,
                                                     const double* __restrict__ q, long long ncg,
                                                     const long long* st, long long c0)
 {
-   double p5[3] = {0.5 / p.dx[0], 0.5 / p.dx[1], 0.5 / p.dx[2]};
+   const double* p5 = k.hdxinv;
    double s = 0.0;
    for (int m = 0; m < p.qlen; m++) {
       const double* a = q + m * ncg;
@@ -132,25 +156,25 @@ __device__ __forceinline__ double quat_cell_modulus(const ampe_model_params& p,
 // ------------------------------------------------------------ quat grads
 template <int NDIM>
 __global__ void __launch_bounds__(256) k_quat_modulus(LevelArgs a, ampe_model_params p,
-                                                      const double* __restrict__ q,
+                                                      RhsConst k, const double* __restrict__ q,
                                                       double* __restrict__ gmod)
 {
    CellPos c;
    if (!cell_pos(a, 0, c)) return;
-   gmod[c.c0] = quat_cell_modulus<NDIM>(p, q, a.ncg, c.st, c.c0);
+   gmod[c.c0] = quat_cell_modulus<NDIM>(p, k, q, a.ncg, c.st, c.c0);
 }
 
 // |grad q|^2 on faces from the full side gradient (quatgrad_side
 // 2d/quatgrad.m4:157-229, summed as compute_face_coef2d quatfacops.m4:83-92).
 template <int NDIM>
-__device__ __forceinline__ double quat_face_norm2(const ampe_model_params& p,
+__device__ __forceinline__ double quat_face_norm2(const ampe_model_params& p, const RhsConst& k,
                                                   const double* __restrict__ q, long long ncg,
                                                   const long long* st, int d, long long cm,
                                                   long long c0)
 {
    double s = 0.0;
    for (int n = 0; n < NDIM; n++) {
-      double dinv = 1.0 / p.dx[n];
+      double dinv = k.dxinv[n];
       double p25 = 0.25 * dinv;
       for (int m = 0; m < p.qlen; m++) {
          const double* aa = q + m * ncg;
@@ -170,7 +194,7 @@ __device__ __forceinline__ double quat_face_norm2(const ampe_model_params& p,
 
 template <int NDIM>
 __global__ void __launch_bounds__(256) k_quat_face_norm2(LevelArgs a, ampe_model_params p,
-                                                         const double* __restrict__ q,
+                                                         RhsConst k, const double* __restrict__ q,
                                                          double* __restrict__ n2)
 {
    CellPos c;
@@ -181,7 +205,7 @@ __global__ void __launch_bounds__(256) k_quat_face_norm2(LevelArgs a, ampe_model
                 (NDIM == 2 || c.k < c.pd.nz + (d == 2));
       if (!ok) continue;
       n2[face_idx(a, c.pd, d, c.i, c.j, c.k)] =
-          quat_face_norm2<NDIM>(p, q, a.ncg, c.st, d, c.c0 - c.st[d], c.c0);
+          quat_face_norm2<NDIM>(p, k, q, a.ncg, c.st, d, c.c0 - c.st[d], c.c0);
    }
 }
 
@@ -190,9 +214,14 @@ __global__ void __launch_bounds__(256) k_quat_face_norm2(LevelArgs a, ampe_model
 // :151-312): flux divergence, double well, orientation terms (computerhspbg,
 // 2d/quatrhs.m4:264-405), KKS driving force (QuadraticFreeEnergyStrategy.cc
 // :393-511 / CALPHADFreeEnergyStrategyBinary.cc:521-651), times M_phi.
-template <int NDIM>
+// Specialised on the flux type FLUX and the free-energy model FE (kFeNone: no
+// composition field) so that each instance holds only its own path's registers.
+constexpr int kFeNone = 3;
+
+template <int NDIM, int FLUX, int FE>
 __device__ __forceinline__ void phase_rhs_cell(const LevelArgs& a, const ampe_model_params& p,
-                                               const CellPos& c, const double* __restrict__ phi,
+                                               const RhsConst& k, const CellPos& c,
+                                               const double* __restrict__ phi,
                                                const double* __restrict__ q,
                                                const double* __restrict__ cl,
                                                const double* __restrict__ cs,
@@ -202,9 +231,9 @@ __device__ __forceinline__ void phase_rhs_cell(const LevelArgs& a, const ampe_mo
    double r = 0.0;
 #pragma unroll
    for (int d = 0; d < NDIM; d++) {
-      double flo = phase_face_flux<NDIM>(p, phi, q, a.ncg, c.st, d, c0 - c.st[d], c0);
-      double fhi = phase_face_flux<NDIM>(p, phi, q, a.ncg, c.st, d, c0, c0 + c.st[d]);
-      double dt = (fhi - flo) * (1.0 / p.dx[d]);
+      double flo = phase_face_flux<NDIM, FLUX>(p, k, phi, q, a.ncg, c.st, d, c0 - c.st[d], c0);
+      double fhi = phase_face_flux<NDIM, FLUX>(p, k, phi, q, a.ncg, c.st, d, c0, c0 + c.st[d]);
+      double dt = (fhi - flo) * k.dxinv[d];
       r = (d == 0) ? dt : r + dt;
    }
    double ph = phi[c0];
@@ -214,26 +243,33 @@ __device__ __forceinline__ void phase_rhs_cell(const LevelArgs& a, const ampe_mo
       double eq2 = 0.5 * p.epsilon_q * p.epsilon_q;
       double p1 = dinterp(ph, p.orient_interp1);
       double p2 = dinterp(ph, p.orient_interp2);
-      double gm = quat_cell_modulus<NDIM>(p, q, a.ncg, c.st, c0);
+      double gm = quat_cell_modulus<NDIM>(p, k, q, a.ncg, c.st, c0);
       r = r - mf * T * p1 * gm - p2 * eq2 * gm * gm;
    }
-   if (p.with_conc) {
-      double convL = 1.0e-6 / p.vm_L, convA = 1.0e-6 / p.vm_A;
+   double mob = p.phi_mobility;
+   if (FE != kFeNone) {
+      double convL = k.convL, convA = k.convA;
       double c_l = cl[c0], c_a = cs[c0];
-      double f_l = fenergy(p, c_l, 0) * convL;
-      double f_a = fenergy(p, c_a, 1) * convA;
-      double mu = (p.fe_model == AMPE_FE_QUADRATIC) ? dfenergy(p, c_l, 0) * convL
-                                                    : dfenergy(p, c_a, 1) * convA;
+      double f_l = fenergy_t<FE>(p, c_l, 0) * convL;
+      double f_a = fenergy_t<FE>(p, c_a, 1) * convA;
+      double mu = (FE == AMPE_FE_QUADRATIC) ? dfenergy_t<FE>(p, c_l, 0) * convL
+                                            : dfenergy_t<FE>(p, c_a, 1) * convA;
       double hp = dinterp(ph, p.energy_interp);
       r += hp * ((f_l - f_a) - mu * (c_l - c_a));
+      if (p.phi_mobility_type == AMPE_PHI_MOB_KIM) mob = phase_mobility_t<FE>(p, c_l, c_a);
    }
-   double mob = (p.phi_mobility_type == AMPE_PHI_MOB_KIM) ? phase_mobility(p, cl[c0], cs[c0])
-                                                          : p.phi_mobility;
    rhs[c0] = mob * r;
 }
 
-template <int NDIM>
-__global__ void __launch_bounds__(256) k_phase_rhs(LevelArgs a, ampe_model_params p,
+// Minimum waves per SIMD the register allocation of k_phase_rhs must allow.
+#ifndef AMPE_PHASE_RHS_WAVES
+#define AMPE_PHASE_RHS_WAVES 1
+#endif
+
+template <int NDIM, int FLUX, int FE>
+__global__ void __launch_bounds__(256)
+    __attribute__((amdgpu_waves_per_eu(AMPE_PHASE_RHS_WAVES)))
+    k_phase_rhs(LevelArgs a, ampe_model_params p, RhsConst k,
                                                    const double* __restrict__ phi,
                                                    const double* __restrict__ q,
                                                    const double* __restrict__ cl,
@@ -246,17 +282,17 @@ __global__ void __launch_bounds__(256) k_phase_rhs(LevelArgs a, ampe_model_param
          if (c.i >= c.pd.nx || c.j >= c.pd.ny) return;
          break;
       }
-      phase_rhs_cell<NDIM>(a, p, c, phi, q, cl, cs, rhs);
+      phase_rhs_cell<NDIM, FLUX, FE>(a, p, k, c, phi, q, cl, cs, rhs);
    }
 }
 
 // ------------------------------------------------------------ quat RHS
 // compute_face_coef2d/3d (2d/quatfacops.m4:14-122) from the face |grad q|^2
-__device__ __forceinline__ double quat_face_coef(const ampe_model_params& p, double phm,
-                                                 double ph0, double n2f)
+__device__ __forceinline__ double quat_face_coef(const ampe_model_params& p, const RhsConst& k,
+                                                 double phm, double ph0, double n2f)
 {
    double floor2 = p.gradient_floor * p.gradient_floor;
-   double maxgni = 1.0 / p.gradient_floor;
+   double maxgni = k.maxgni;
    double eps2 = p.epsilon_q * p.epsilon_q;
    double mf = 2.0 * p.H_parameter;
    double phia = face_avg(phm, ph0, p.avg_type);
@@ -272,7 +308,7 @@ __device__ __forceinline__ double quat_face_coef(const ampe_model_params& p, dou
 // (compute_flux2d_from_gradq), lambda (compute_lambda_flux2d/3d),
 // rhs = -M_q (div + 2 q lambda) (add_quat_proj_op2d/3d).
 template <int NDIM>
-__global__ void __launch_bounds__(256) k_quat_rhs(LevelArgs a, ampe_model_params p,
+__global__ void __launch_bounds__(256) k_quat_rhs(LevelArgs a, ampe_model_params p, RhsConst k,
                                                   const double* __restrict__ phi,
                                                   const double* __restrict__ q,
                                                   const double* __restrict__ n2,
@@ -289,11 +325,11 @@ __global__ void __launch_bounds__(256) k_quat_rhs(LevelArgs a, ampe_model_params
       long long flo = face_idx(a, c.pd, d, lo[0], lo[1], lo[2]);
       lo[d] += 1;
       long long fhi = face_idx(a, c.pd, d, lo[0], lo[1], lo[2]);
-      fclo[d] = quat_face_coef(p, phi[c0 - c.st[d]], phi[c0], n2[flo]);
-      fchi[d] = quat_face_coef(p, phi[c0], phi[c0 + c.st[d]], n2[fhi]);
+      fclo[d] = quat_face_coef(p, k, phi[c0 - c.st[d]], phi[c0], n2[flo]);
+      fchi[d] = quat_face_coef(p, k, phi[c0], phi[c0 + c.st[d]], n2[fhi]);
    }
-   double dinv[3] = {1.0 / p.dx[0], 1.0 / p.dx[1], 1.0 / p.dx[2]};
-   double fac[3] = {0.5 / p.dx[0], 0.5 / p.dx[1], 0.5 / p.dx[2]};
+   const double* dinv = k.dxinv;
+   const double* fac = k.hdxinv;
    double lam = 0.0, sumq2 = 0.0;
    double dv[4];
    for (int m = 0; m < ql; m++) {
@@ -352,7 +388,7 @@ __global__ void __launch_bounds__(256) k_quat_project(LevelArgs a, int ql,
 // concentration_pfmdiffusion (2d/concentrationdiffusion.m4:14-96) and
 // setDiffCoeffForPhaseOnPatch (source/KKSCompositionRHSStrategy.cc:210-373)
 template <int NDIM>
-__global__ void __launch_bounds__(256) k_conc_coeffs(LevelArgs a, ampe_model_params p,
+__global__ void __launch_bounds__(256) k_conc_coeffs(LevelArgs a, ampe_model_params p, RhsConst k,
                                                      const double* __restrict__ phi,
                                                      const double* __restrict__ cl,
                                                      const double* __restrict__ cs,
@@ -362,7 +398,6 @@ __global__ void __launch_bounds__(256) k_conc_coeffs(LevelArgs a, ampe_model_par
    CellPos c;
    if (!cell_pos(a, 1, c)) return;
    if (NDIM == 2 && c.k > 0) return;
-   double qlr = p.Q0_liquid / kRGas, qsr = p.Q0_solid / kRGas;
    for (int d = 0; d < NDIM; d++) {
       bool ok = (c.i < c.pd.nx + (d == 0)) && (c.j < c.pd.ny + (d == 1)) &&
                 (NDIM == 2 || c.k < c.pd.nz + (d == 2));
@@ -371,9 +406,7 @@ __global__ void __launch_bounds__(256) k_conc_coeffs(LevelArgs a, ampe_model_par
       long long fi = face_idx(a, c.pd, d, c.i, c.j, c.k);
       double vphi = face_avg(phi[cm], phi[c0], p.avg_type);
       double hphi = interp(vphi, p.energy_interp);
-      double invT = 2.0 / (p.temperature + p.temperature);
-      double dl = p.D_liquid * exp(-qlr * invT);
-      double ds = p.D_solid * exp(-qsr * invT);
+      double dl = k.Dl, ds = k.Ds;
       double dd = (1.0 - hphi) * dl + hphi * ds;
       Dc[fi] = dd;
       double ph = face_avg(phi[c0], phi[cm], p.avg_type);
@@ -387,7 +420,7 @@ __global__ void __launch_bounds__(256) k_conc_coeffs(LevelArgs a, ampe_model_par
 // concentrationflux (2d/concentrationrhs.m4:15-82) + computerhsconcentration
 // (2d/concentrationrhs.m4:326-367)
 template <int NDIM>
-__global__ void __launch_bounds__(256) k_conc_rhs(LevelArgs a, ampe_model_params p,
+__global__ void __launch_bounds__(256) k_conc_rhs(LevelArgs a, ampe_model_params p, RhsConst k,
                                                   const double* __restrict__ cc,
                                                   const double* __restrict__ phi,
                                                   const double* __restrict__ Dc,
@@ -400,7 +433,7 @@ __global__ void __launch_bounds__(256) k_conc_rhs(LevelArgs a, ampe_model_params
    double s = 0.0;
 #pragma unroll
    for (int d = 0; d < NDIM; d++) {
-      double dinv = 1.0 / p.dx[d];
+      double dinv = k.dxinv[d];
       int lo[3] = {ijk[0], ijk[1], ijk[2]};
       long long flo = face_idx(a, c.pd, d, lo[0], lo[1], lo[2]);
       lo[d] += 1;
@@ -420,7 +453,7 @@ AMPE_COLUMN_END
 // as TbasedCompositionDiffusionStrategy::setDiffusion calls it
 // (source/TbasedCompositionDiffusionStrategy.cc:305-322), uniform temperature
 template <int NDIM>
-__global__ void __launch_bounds__(256) k_ebs_coeffs(LevelArgs a, ampe_model_params p,
+__global__ void __launch_bounds__(256) k_ebs_coeffs(LevelArgs a, ampe_model_params p, RhsConst k,
                                                     const double* __restrict__ phi,
                                                     double* __restrict__ DL,
                                                     double* __restrict__ DA)
@@ -428,7 +461,6 @@ __global__ void __launch_bounds__(256) k_ebs_coeffs(LevelArgs a, ampe_model_para
    CellPos c;
    if (!cell_pos(a, 1, c)) return;
    if (NDIM == 2 && c.k > 0) return;
-   double qlr = p.Q0_liquid / kRGas, qsr = p.Q0_solid / kRGas;
    for (int d = 0; d < NDIM; d++) {
       bool ok = (c.i < c.pd.nx + (d == 0)) && (c.j < c.pd.ny + (d == 1)) &&
                 (NDIM == 2 || c.k < c.pd.nz + (d == 2));
@@ -437,9 +469,7 @@ __global__ void __launch_bounds__(256) k_ebs_coeffs(LevelArgs a, ampe_model_para
       long long fi = face_idx(a, c.pd, d, c.i, c.j, c.k);
       double vphi = face_avg(phi[cm], phi[c0], p.avg_type);
       double hphi = interp(vphi, p.diff_interp);
-      double invT = 2.0 / (p.temperature + p.temperature);
-      double dl = p.D_liquid * exp(-qlr * invT);
-      double ds = p.D_solid * exp(-qsr * invT);
+      double dl = k.Dl, ds = k.Ds;
       DL[fi] = (1.0 - hphi) * dl;
       DA[fi] = hphi * ds;
    }
@@ -449,7 +479,7 @@ __global__ void __launch_bounds__(256) k_ebs_coeffs(LevelArgs a, ampe_model_para
 // :183-328): add_flux (2d/flux.m4:11-66) of the liquid then the solid phase
 // onto a zero flux, then computerhsconcentration (2d/concentrationrhs.m4:326-367)
 template <int NDIM>
-__global__ void __launch_bounds__(256) k_conc_rhs_ebs(LevelArgs a, ampe_model_params p,
+__global__ void __launch_bounds__(256) k_conc_rhs_ebs(LevelArgs a, ampe_model_params p, RhsConst k,
                                                       const double* __restrict__ cl,
                                                       const double* __restrict__ cs,
                                                       const double* __restrict__ DL,
@@ -462,7 +492,7 @@ __global__ void __launch_bounds__(256) k_conc_rhs_ebs(LevelArgs a, ampe_model_pa
    double s = 0.0;
 #pragma unroll
    for (int d = 0; d < NDIM; d++) {
-      double dinv = 1.0 / p.dx[d];
+      double dinv = k.dxinv[d];
       int lo[3] = {ijk[0], ijk[1], ijk[2]};
       long long flo = face_idx(a, c.pd, d, lo[0], lo[1], lo[2]);
       lo[d] += 1;
@@ -505,7 +535,7 @@ int launch_phase_mobility(const Level& L, const ampe_model_params& p, const doub
 // ------------------------------------------------------------ tagging
 // tagfromgrads (2d/quattag.m4:9-55) on the cell gradient of phi
 template <int NDIM>
-__global__ void __launch_bounds__(256) k_tag(LevelArgs a, ampe_model_params p,
+__global__ void __launch_bounds__(256) k_tag(LevelArgs a, ampe_model_params p, RhsConst k,
                                              const double* __restrict__ phi, double tol2,
                                              int* __restrict__ tags)
 {
@@ -515,7 +545,7 @@ __global__ void __launch_bounds__(256) k_tag(LevelArgs a, ampe_model_params p,
    double g = 0.0;
 #pragma unroll
    for (int d = 0; d < NDIM; d++) {
-      double p5 = 0.5 / p.dx[d];
+      double p5 = k.hdxinv[d];
       long long s = c.st[d];
       double gd = ((phi[c0 + s] - phi[c0]) + (phi[c0] - phi[c0 - s])) * p5;
       double gh = gd * p.dx[d];
@@ -539,24 +569,65 @@ static dim3 tile_block() { return dim3(TX, TY); }
 int launch_quat_grad(const Level& L, const ampe_model_params& p, const double* q,
                      double* gmod, double* face_norm2, hipStream_t s)
 {
-   if (gmod) AMPE_DISPATCH(L, k_quat_modulus, L.ncell_tiles, level_args(L), p, q, gmod);
+   if (gmod) AMPE_DISPATCH(L, k_quat_modulus, L.ncell_tiles, level_args(L), p, rhs_const(p), q, gmod);
    if (face_norm2)
-      AMPE_DISPATCH(L, k_quat_face_norm2, L.nface_tiles, level_args(L, true), p, q, face_norm2);
+      AMPE_DISPATCH(L, k_quat_face_norm2, L.nface_tiles, level_args(L, true), p, rhs_const(p), q, face_norm2);
    return 0;
+}
+
+template <int NDIM, int FLUX>
+static void launch_phase_rhs_fe(const Level& L, const ampe_model_params& p, int fe,
+                                const double* phi, const double* q, const double* cl,
+                                const double* cs, double* rhs, hipStream_t s)
+{
+   dim3 g(L.ncol_tiles), b = tile_block();
+   LevelArgs a = level_args_col(L);
+   RhsConst k = rhs_const(p);
+   switch (fe) {
+   case AMPE_FE_QUADRATIC:
+      hipLaunchKernelGGL((k_phase_rhs<NDIM, FLUX, AMPE_FE_QUADRATIC>), g, b, 0, s, a, p, k, phi, q,
+                         cl, cs, rhs);
+      break;
+   case AMPE_FE_CALPHAD_BINARY:
+      hipLaunchKernelGGL((k_phase_rhs<NDIM, FLUX, AMPE_FE_CALPHAD_BINARY>), g, b, 0, s, a, p, k, phi,
+                         q, cl, cs, rhs);
+      break;
+   case AMPE_FE_KKS_DILUTE:
+      hipLaunchKernelGGL((k_phase_rhs<NDIM, FLUX, AMPE_FE_KKS_DILUTE>), g, b, 0, s, a, p, k, phi, q,
+                         cl, cs, rhs);
+      break;
+   default:
+      hipLaunchKernelGGL((k_phase_rhs<NDIM, FLUX, kFeNone>), g, b, 0, s, a, p, k, phi, q, cl, cs,
+                         rhs);
+   }
 }
 
 int launch_phase_rhs(const Level& L, const ampe_model_params& p, const double* phi,
                      const double* q, const double* cl, const double* cs, double* rhs,
                      hipStream_t s)
 {
-   AMPE_DISPATCH(L, k_phase_rhs, L.ncol_tiles, level_args_col(L), p, phi, q, cl, cs, rhs);
+   int fe = p.with_conc ? p.fe_model : kFeNone;
+   if (L.ndim == 3) {
+      if (p.flux_type == AMPE_FLUX_ANISOTROPIC)
+         launch_phase_rhs_fe<3, AMPE_FLUX_ANISOTROPIC>(L, p, fe, phi, q, cl, cs, rhs, s);
+      else
+         launch_phase_rhs_fe<3, AMPE_FLUX_SIMPLE>(L, p, fe, phi, q, cl, cs, rhs, s);
+   } else {
+      if (p.flux_type == AMPE_FLUX_ANISOTROPIC)
+         launch_phase_rhs_fe<2, AMPE_FLUX_ANISOTROPIC>(L, p, fe, phi, q, cl, cs, rhs, s);
+      else if (p.flux_type == AMPE_FLUX_ISOTROPIC)
+         launch_phase_rhs_fe<2, AMPE_FLUX_ISOTROPIC>(L, p, fe, phi, q, cl, cs, rhs, s);
+      else
+         launch_phase_rhs_fe<2, AMPE_FLUX_SIMPLE>(L, p, fe, phi, q, cl, cs, rhs, s);
+   }
+   AMPE_HIP_CHECK(hipGetLastError());
    return 0;
 }
 
 int launch_quat_rhs(const Level& L, const ampe_model_params& p, const double* phi,
                     const double* q, const double* n2, double* rhs, hipStream_t s)
 {
-   AMPE_DISPATCH(L, k_quat_rhs, L.ncol_tiles, level_args_col(L), p, phi, q, n2, rhs);
+   AMPE_DISPATCH(L, k_quat_rhs, L.ncol_tiles, level_args_col(L), p, rhs_const(p), phi, q, n2, rhs);
    return 0;
 }
 
@@ -573,7 +644,7 @@ int launch_conc_coeffs(const Level& L, const ampe_model_params& p, const double*
                        const double* cl, const double* cs, double* Dc, double* Dphi,
                        hipStream_t s)
 {
-   AMPE_DISPATCH(L, k_conc_coeffs, L.nface_tiles, level_args(L, true), p, phi, cl, cs, Dc, Dphi);
+   AMPE_DISPATCH(L, k_conc_coeffs, L.nface_tiles, level_args(L, true), p, rhs_const(p), phi, cl, cs, Dc, Dphi);
    return 0;
 }
 
@@ -581,14 +652,14 @@ int launch_conc_rhs(const Level& L, const ampe_model_params& p, const double* c,
                     const double* phi, const double* Dc, const double* Dphi, double* rhs,
                     hipStream_t s)
 {
-   AMPE_DISPATCH(L, k_conc_rhs, L.ncol_tiles, level_args_col(L), p, c, phi, Dc, Dphi, rhs);
+   AMPE_DISPATCH(L, k_conc_rhs, L.ncol_tiles, level_args_col(L), p, rhs_const(p), c, phi, Dc, Dphi, rhs);
    return 0;
 }
 
 int launch_ebs_coeffs(const Level& L, const ampe_model_params& p, const double* phi, double* DL,
                       double* DA, hipStream_t s)
 {
-   AMPE_DISPATCH(L, k_ebs_coeffs, L.nface_tiles, level_args(L, true), p, phi, DL, DA);
+   AMPE_DISPATCH(L, k_ebs_coeffs, L.nface_tiles, level_args(L, true), p, rhs_const(p), phi, DL, DA);
    return 0;
 }
 
@@ -596,14 +667,14 @@ int launch_conc_rhs_ebs(const Level& L, const ampe_model_params& p, const double
                         const double* cs, const double* DL, const double* DA, double* rhs,
                         hipStream_t s)
 {
-   AMPE_DISPATCH(L, k_conc_rhs_ebs, L.ncol_tiles, level_args_col(L), p, cl, cs, DL, DA, rhs);
+   AMPE_DISPATCH(L, k_conc_rhs_ebs, L.ncol_tiles, level_args_col(L), p, rhs_const(p), cl, cs, DL, DA, rhs);
    return 0;
 }
 
 int launch_tag(const Level& L, const ampe_model_params& p, const double* phi, double tol,
                int* tags, hipStream_t s)
 {
-   AMPE_DISPATCH(L, k_tag, L.ncell_tiles, level_args(L), p, phi, tol * tol, tags);
+   AMPE_DISPATCH(L, k_tag, L.ncell_tiles, level_args(L), p, rhs_const(p), phi, tol * tol, tags);
    return 0;
 }
 
@@ -615,7 +686,7 @@ namespace ampe {
 // preconditioner; QuatFACOps::setOperatorCoefficients,
 // source/QuatFACOps.cc:735-829).
 template <int NDIM>
-__global__ void __launch_bounds__(256) k_quat_face_coef(LevelArgs a, ampe_model_params p,
+__global__ void __launch_bounds__(256) k_quat_face_coef(LevelArgs a, ampe_model_params p, RhsConst k,
                                                         const double* __restrict__ phi,
                                                         const double* __restrict__ n2,
                                                         double* __restrict__ fc)
@@ -628,7 +699,7 @@ __global__ void __launch_bounds__(256) k_quat_face_coef(LevelArgs a, ampe_model_
                 (NDIM == 2 || c.k < c.pd.nz + (d == 2));
       if (!ok) continue;
       long long fi = face_idx(a, c.pd, d, c.i, c.j, c.k);
-      fc[fi] = quat_face_coef(p, phi[c.c0 - c.st[d]], phi[c.c0], n2[fi]);
+      fc[fi] = quat_face_coef(p, k, phi[c.c0 - c.st[d]], phi[c.c0], n2[fi]);
    }
 }
 
@@ -645,7 +716,7 @@ __global__ void __launch_bounds__(256) k_sqrt_quat_mobility(long long n, ampe_mo
 int launch_quat_face_coef(const Level& L, const ampe_model_params& p, const double* phi,
                           const double* n2, double* fc, hipStream_t s)
 {
-   AMPE_DISPATCH(L, k_quat_face_coef, L.nface_tiles, level_args(L, true), p, phi, n2, fc);
+   AMPE_DISPATCH(L, k_quat_face_coef, L.nface_tiles, level_args(L, true), p, rhs_const(p), phi, n2, fc);
    return 0;
 }
 

@@ -485,7 +485,7 @@ int ampe_sim_last_time_ms(const ampe_sim* sim, double* ms);
  * whose previous-step guess has a KKS residual above switch_tol starts from
  * the surrogate's (cL, cS) for inputs (c, phi, T) instead; after probe_its
  * Newton updates a residual above probe_tol restarts it from the
- * previous-step guess.  switch_tol = 0 starts every cell from the surrogate. */
+ * previous-step guess.  switch_tol <= 0 starts every cell from the surrogate. */
 int ampe_sim_set_surrogate(ampe_sim* sim, const ampe_surrogate* s,
                            int probe_its, double probe_tol, double switch_tol);
 
