@@ -142,6 +142,16 @@ int ampe_conc_coeffs(const ampe_level* level, const ampe_model_params* p, const 
    return ampe::launch_conc_coeffs(*level->L, *p, phi, cl, cs, Dc, Dphi, hs(stream));
 }
 
+int ampe_kernel_timer_enable(int kernel, int on)
+{
+   return ampe::KernelTimer::enable(kernel, on != 0);
+}
+
+int ampe_kernel_timer_read(int kernel, double* ms, long long* launches, double* bytes)
+{
+   return ampe::KernelTimer::read(kernel, ms, launches, bytes);
+}
+
 int ampe_conc_coeffs_ebs(const ampe_level* level, const ampe_model_params* p,
                          const double* phi, double* DL, double* DA, ampe_stream_t stream)
 {
@@ -367,6 +377,20 @@ int ampe_sim_set_surrogate(ampe_sim* sim, const ampe_surrogate* s, int probe_its
       const ampe::Surrogate& m = *s->S;
       AMPE_REQUIRE(m.nin == 3 && m.nout == 2,
                    "the KKS surrogate maps (c, phi, T) -> (cL, cS): n_in 3, n_out 2");
+   }
+   if (s && !sim->S->guess.list) {
+      ampe::KksGuess& g = sim->S->guess;
+      hipError_t e = hipMalloc(&g.list, sizeof(int) * (size_t)sim->S->L->ncg);
+      if (e == hipSuccess) e = hipMalloc(&g.count, sizeof(unsigned));
+      if (e == hipSuccess) e = hipMemset(g.count, 0, sizeof(unsigned));
+      if (e != hipSuccess) {
+         (void)hipFree(g.list);
+         (void)hipFree(g.count);
+         g.list = nullptr;
+         g.count = nullptr;
+         ampe::set_error(std::string("ampe_sim_set_surrogate: ") + hipGetErrorString(e));
+         return -1;
+      }
    }
    sim->S->guess.surrogate = s ? s->S : nullptr;
    sim->S->guess.probe_its = probe_its;

@@ -111,14 +111,14 @@ __device__ __forceinline__ void block_store_sum(double v, double* partial)
 // One colour of efo_rbgswithfluxmax{var,con}dc{var,con}sf
 // (source/fortran/2d/ellipticfacops.m4:60-345): u += -res * dudr.
 template <typename T, int NDIM>
-__global__ void __launch_bounds__(256) k_rbgs(LevelArgs a, OpArgs<T> op, T dx0, T dx1, T dx2,
+__global__ void __launch_bounds__(256) k_rbgs(LevelArgs a, OpArgs<T> op, T di0, T di1, T di2,
                                               const T* __restrict__ rhs, T* __restrict__ u,
                                               int color, double* __restrict__ partial)
 {
    T absres = 0;
    AMPE_COLUMN_BEGIN(a, c)
    if (!is_color<NDIM>(c, color)) continue;
-   T dinv[3] = {T(1) / dx0, T(1) / dx1, T(1) / dx2};
+   T dinv[3] = {di0, di1, di2};
    T diag;
    T res = op_residual<T, NDIM>(a, c, op, rhs, u, dinv, &diag);
    T dudr = T(1) / diag;
@@ -132,13 +132,13 @@ __global__ void __launch_bounds__(256) k_rbgs(LevelArgs a, OpArgs<T> op, T dx0, 
 
 // efo_compres*sca (source/fortran/2d/ellipticfacops.m4:346-446) + sum res^2
 template <typename T, int NDIM>
-__global__ void __launch_bounds__(256) k_residual(LevelArgs a, OpArgs<T> op, T dx0, T dx1, T dx2,
+__global__ void __launch_bounds__(256) k_residual(LevelArgs a, OpArgs<T> op, T di0, T di1, T di2,
                                                   const T* __restrict__ rhs, const T* __restrict__ u,
                                                   T* __restrict__ res, double* __restrict__ partial)
 {
    double sq = 0.0;
    AMPE_COLUMN_BEGIN(a, c)
-   T dinv[3] = {T(1) / dx0, T(1) / dx1, T(1) / dx2};
+   T dinv[3] = {di0, di1, di2};
    T diag;
    T r = op_residual<T, NDIM>(a, c, op, rhs, u, dinv, &diag);
    res[c.c0] = r;
@@ -184,13 +184,13 @@ __device__ __forceinline__ T quat_row(const LevelArgs& a, const CellPos& c, cons
 
 template <typename T, int NDIM>
 __global__ void __launch_bounds__(256) k_quat_rbgs(LevelArgs a, const T* __restrict__ fc,
-                                                   const T* __restrict__ sm, T gamma, T h0, T h1,
-                                                   T h2, int ncomp, const T* __restrict__ rhs,
+                                                   const T* __restrict__ sm, T gamma, T hi0, T hi1,
+                                                   T hi2, int ncomp, const T* __restrict__ rhs,
                                                    T* __restrict__ z, int color)
 {
    AMPE_COLUMN_BEGIN(a, c)
    if (!is_color<NDIM>(c, color)) continue;
-   T hinv2[3] = {T(1) / (h0 * h0), T(1) / (h1 * h1), T(1) / (h2 * h2)};
+   T hinv2[3] = {hi0, hi1, hi2};
    for (int m = 0; m < ncomp; m++) {
       const T* rm = rhs + m * a.ncg;
       T* zm = z + m * a.ncg;
@@ -204,15 +204,15 @@ __global__ void __launch_bounds__(256) k_quat_rbgs(LevelArgs a, const T* __restr
 
 template <typename T, int NDIM>
 __global__ void __launch_bounds__(256) k_quat_residual(LevelArgs a, const T* __restrict__ fc,
-                                                       const T* __restrict__ sm, T gamma, T h0,
-                                                       T h1, T h2, int ncomp,
+                                                       const T* __restrict__ sm, T gamma, T hi0,
+                                                       T hi1, T hi2, int ncomp,
                                                        const T* __restrict__ rhs,
                                                        const T* __restrict__ z, T* __restrict__ res,
                                                        double* __restrict__ partial)
 {
    double sq = 0.0;
    AMPE_COLUMN_BEGIN(a, c)
-   T hinv2[3] = {T(1) / (h0 * h0), T(1) / (h1 * h1), T(1) / (h2 * h2)};
+   T hinv2[3] = {hi0, hi1, hi2};
    for (int m = 0; m < ncomp; m++) {
       T center;
       T az = quat_row<T, NDIM>(a, c, fc, sm, gamma, hinv2, z + m * a.ncg, &center);
@@ -226,6 +226,22 @@ __global__ void __launch_bounds__(256) k_quat_residual(LevelArgs a, const T* __r
 
 // ------------------------------------------------------------ launchers
 static dim3 tblock() { return dim3(TX, TY); }
+
+// 1/h and 1/h^2 in the operator's precision, computed once per launch on the
+// host (IEEE division: the device's value).
+template <typename T>
+static void recip(const double* dx, T* di)
+{
+   for (int d = 0; d < 3; d++) di[d] = T(1) / (T)dx[d];
+}
+template <typename T>
+static void recip2(const double* dx, T* di)
+{
+   for (int d = 0; d < 3; d++) {
+      T h = (T)dx[d];
+      di[d] = T(1) / (h * h);
+   }
+}
 
 int launch_phase_C(const Level& L, const ampe_model_params& p, double gamma,
                    const double* phi, const double* M, double* C, hipStream_t s)
@@ -254,12 +270,14 @@ int launch_rbgs(const Level& L, const EllOp<T>& op, const double* dx, const T* r
    LevelArgs la;
    int grid;
    shape(L, want_max, &la, &grid);
+   T di[3];
+   recip(dx, di);
    if (L.ndim == 3)
-      hipLaunchKernelGGL((k_rbgs<T, 3>), dim3(grid), tblock(), 0, s, la, oa, (T)dx[0],
-                         (T)dx[1], (T)dx[2], rhs, u, color, part);
+      hipLaunchKernelGGL((k_rbgs<T, 3>), dim3(grid), tblock(), 0, s, la, oa, di[0], di[1], di[2],
+                         rhs, u, color, part);
    else
-      hipLaunchKernelGGL((k_rbgs<T, 2>), dim3(grid), tblock(), 0, s, la, oa, (T)dx[0],
-                         (T)dx[1], (T)dx[2], rhs, u, color, part);
+      hipLaunchKernelGGL((k_rbgs<T, 2>), dim3(grid), tblock(), 0, s, la, oa, di[0], di[1], di[2],
+                         rhs, u, color, part);
    AMPE_HIP_CHECK(hipGetLastError());
    return 0;
 }
@@ -272,12 +290,14 @@ int launch_residual(const Level& L, const EllOp<T>& op, const double* dx, const 
    LevelArgs la;
    int grid;
    shape(L, part != nullptr, &la, &grid);
+   T di[3];
+   recip(dx, di);
    if (L.ndim == 3)
-      hipLaunchKernelGGL((k_residual<T, 3>), dim3(grid), tblock(), 0, s, la, oa, (T)dx[0],
-                         (T)dx[1], (T)dx[2], rhs, u, res, part);
+      hipLaunchKernelGGL((k_residual<T, 3>), dim3(grid), tblock(), 0, s, la, oa, di[0], di[1],
+                         di[2], rhs, u, res, part);
    else
-      hipLaunchKernelGGL((k_residual<T, 2>), dim3(grid), tblock(), 0, s, la, oa, (T)dx[0],
-                         (T)dx[1], (T)dx[2], rhs, u, res, part);
+      hipLaunchKernelGGL((k_residual<T, 2>), dim3(grid), tblock(), 0, s, la, oa, di[0], di[1],
+                         di[2], rhs, u, res, part);
    AMPE_HIP_CHECK(hipGetLastError());
    return 0;
 }
@@ -289,12 +309,15 @@ int launch_quat_rbgs(const Level& L, const T* fc, const T* sm, double gamma, con
    LevelArgs la;
    int grid;
    shape(L, false, &la, &grid);
+   T hi[3];
+   recip2(dx, hi);
+   KernelTimer::Span span(KernelTimer::QUAT_RBGS, s, quat_rbgs_bytes(L, ncomp, sizeof(T)));
    if (L.ndim == 3)
       hipLaunchKernelGGL((k_quat_rbgs<T, 3>), dim3(grid), tblock(), 0, s, la, fc, sm, (T)gamma,
-                         (T)dx[0], (T)dx[1], (T)dx[2], ncomp, rhs, z, color);
+                         hi[0], hi[1], hi[2], ncomp, rhs, z, color);
    else
       hipLaunchKernelGGL((k_quat_rbgs<T, 2>), dim3(grid), tblock(), 0, s, la, fc, sm, (T)gamma,
-                         (T)dx[0], (T)dx[1], (T)dx[2], ncomp, rhs, z, color);
+                         hi[0], hi[1], hi[2], ncomp, rhs, z, color);
    AMPE_HIP_CHECK(hipGetLastError());
    return 0;
 }
@@ -307,12 +330,14 @@ int launch_quat_residual(const Level& L, const T* fc, const T* sm, double gamma,
    LevelArgs la;
    int grid;
    shape(L, part != nullptr, &la, &grid);
+   T hi[3];
+   recip2(dx, hi);
    if (L.ndim == 3)
       hipLaunchKernelGGL((k_quat_residual<T, 3>), dim3(grid), tblock(), 0, s, la, fc, sm,
-                         (T)gamma, (T)dx[0], (T)dx[1], (T)dx[2], ncomp, rhs, z, res, part);
+                         (T)gamma, hi[0], hi[1], hi[2], ncomp, rhs, z, res, part);
    else
       hipLaunchKernelGGL((k_quat_residual<T, 2>), dim3(grid), tblock(), 0, s, la, fc, sm,
-                         (T)gamma, (T)dx[0], (T)dx[1], (T)dx[2], ncomp, rhs, z, res, part);
+                         (T)gamma, hi[0], hi[1], hi[2], ncomp, rhs, z, res, part);
    AMPE_HIP_CHECK(hipGetLastError());
    return 0;
 }

@@ -47,11 +47,15 @@ struct CellPos {
    PatchDev pd;
 };
 
+// Tile of this workgroup: the tile lists are stored in dispatch order, already
+// arranged per XCD on the host (level_create).
+__device__ __forceinline__ int tile_index(const LevelArgs&) { return (int)blockIdx.x; }
+
 // Cell of this thread in the tile list; false when outside the box
 // (extend = 1 for face tiles: boxes grown by one in each direction).
 __device__ __forceinline__ bool cell_pos(const LevelArgs& a, int extend, CellPos& c)
 {
-   const Tile t = a.T[blockIdx.x];
+   const Tile t = a.T[tile_index(a)];
    c.pd = a.P[t.patch];
    c.i = t.i0 + (int)threadIdx.x;
    c.j = t.j0 + (int)threadIdx.y;
@@ -69,7 +73,7 @@ __device__ __forceinline__ bool cell_pos(const LevelArgs& a, int extend, CellPos
 // the box (i, j or k).
 __device__ __forceinline__ bool cell_pos_col(const LevelArgs& a, int kk, CellPos& c)
 {
-   const Tile t = a.T[blockIdx.x];
+   const Tile t = a.T[tile_index(a)];
    c.pd = a.P[t.patch];
    c.i = t.i0 + (int)threadIdx.x;
    c.j = t.j0 + (int)threadIdx.y;

@@ -17,7 +17,15 @@ struct KksGuess {
    int probe_its = 3;
    double probe_tol = 1.0e-8;
    double switch_tol = 1.0e-2;  // previous-guess residual above which the surrogate starts
+   // device work list of the cells that start from the surrogate (ncg
+   // entries) and its counter (zero between calls)
+   int* list = nullptr;
+   unsigned* count = nullptr;
 };
+
+// Workgroups of the surrogate pass (each stages the network once and walks
+// the work list with a grid stride).
+constexpr int kKksSurrGrid = 256;
 
 // KKS over all ghosted cells of a level.  launch_kks accumulates into device
 // counters dev_acc[6] = {total iterations, max iterations, failure flag,

@@ -79,9 +79,8 @@ __device__ inline void kks_surrogate_guess(const MlpDev& m, const u16x8* sf, con
    gs = (double)y[1] * m.out_scale[1] + m.out_shift[1];
 }
 
-template <bool SURR>
-__global__ void __launch_bounds__(256) k_kks(long long n, ampe_model_params p, MlpDev m,
-                                             int probe_its, double probe_tol, double switch_tol,
+// Every cell from the previous-step guess (kks_cell).
+__global__ void __launch_bounds__(256) k_kks(long long n, ampe_model_params p,
                                              const double* __restrict__ phi,
                                              const double* __restrict__ c,
                                              const double* __restrict__ clr,
@@ -89,60 +88,130 @@ __global__ void __launch_bounds__(256) k_kks(long long n, ampe_model_params p, M
                                              double* __restrict__ cl, double* __restrict__ cs,
                                              double* __restrict__ partial)
 {
-   using F = MlpFrags<kMlpHidden>;
-   __shared__ u16x8 sf[SURR ? F::NFWD * 64 : 1];
-   __shared__ float sb[SURR ? F::NBIAS : 1];
-   if (SURR) {
-      mlp_stage<kMlpHidden>(m, F::NFWD, sf, sb);
-      __syncthreads();
-   }
    long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-   bool in = t < n;
-   double ct = in ? c[t] : 0.5, pt = in ? phi[t] : 0.0;
-   double it = 0.0, mx = 0.0, fb = 0.0, ss = 0.0;
-   if (SURR && p.fe_model == AMPE_FE_CALPHAD_BINARY) {
+   double it = 0.0, mx = 0.0;
+   if (t < n) {
+      double xl = clr[t], xs = csr[t];
+      int r = kks_cell(p, c[t], phi[t], xl, xs);
+      cl[t] = xl;
+      cs[t] = xs;
+      mx = r < 0 ? kKksFail : (double)r;
+      it = r < 0 ? 0.0 : (double)r;
+   }
+   kks_block_partial(it, mx, 0.0, mx >= kKksFail ? 1.0 : 0.0, 0.0, partial);
+}
+
+// Switch policy, first pass (binary CALPHAD): a cell whose residual at the
+// clamped previous-step guess is at most switch_tol (and switch_tol > 0) runs
+// the Newton from that guess; the others are appended to the work list (one
+// atomic per wave) and left to k_kks_surr.  Kept free of the network so that
+// it runs at the occupancy of the plain solve.
+__global__ void __launch_bounds__(256) k_kks_defer(long long n, ampe_model_params p,
+                                                   double switch_tol,
+                                                   const double* __restrict__ phi,
+                                                   const double* __restrict__ c,
+                                                   const double* __restrict__ clr,
+                                                   const double* __restrict__ csr,
+                                                   double* __restrict__ cl,
+                                                   double* __restrict__ cs,
+                                                   int* __restrict__ list,
+                                                   unsigned* __restrict__ count,
+                                                   double* __restrict__ partial)
+{
+   long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+   double it = 0.0, mx = 0.0;
+   bool bad = false;
+   if (t < n) {
       const double T = p.temperature;
-      double h = interp(pt, p.conc_interp);
-      double xl = in ? clr[t] : 0.5, xs = in ? csr[t] : 0.5;
+      double ct = c[t];
+      double h = interp(phi[t], p.conc_interp);
+      double xl = clr[t], xs = csr[t];
       kks_clamp(xl, xs);
       double f1, f2;
       kks_residual(p.calphad, T, 1.0 / (kRGas * T), h, ct, xl, xs, f1, f2);
-      bool bad = in && (switch_tol <= 0.0 || !(dmax(fabs(f1), fabs(f2)) <= switch_tol));
-      int r = 0;
-      if (__any(bad)) {
-         double gl, gs;
-         kks_surrogate_guess(m, sf, sb, ct, pt, T, gl, gs);
-         if (bad) {
-            ss = 1.0;
-            r = kks_newton(p.calphad, T, h, ct, gl, gs, p.newton_max_its, p.newton_tol,
-                           probe_its, probe_tol);
-            if (r == -2) {
-               fb = 1.0;
-               int r2 = kks_newton_from(p.calphad, T, h, ct, xl, xs, f1, f2, p.newton_max_its,
-                                        p.newton_tol, -1, 0.0);
-               r = r2 < 0 ? -1 : r2 + probe_its;
-            } else {
-               xl = gl;
-               xs = gs;
-            }
-         }
-      }
-      if (in && !bad)
-         r = kks_newton_from(p.calphad, T, h, ct, xl, xs, f1, f2, p.newton_max_its,
-                             p.newton_tol, -1, 0.0);
-      if (in) {
+      bad = switch_tol <= 0.0 || !(dmax(fabs(f1), fabs(f2)) <= switch_tol);
+      if (!bad) {
+         int r = kks_newton_from(p.calphad, T, h, ct, xl, xs, f1, f2, p.newton_max_its,
+                                 p.newton_tol, -1, 0.0);
          cl[t] = xl;
          cs[t] = xs;
          mx = r < 0 ? kKksFail : (double)r;
          it = r < 0 ? 0.0 : (double)r;
       }
-   } else if (in) {
-      double xl = clr[t], xs = csr[t];
-      int r = kks_cell(p, ct, pt, xl, xs);
-      cl[t] = xl;
-      cs[t] = xs;
-      mx = r < 0 ? kKksFail : (double)r;
-      it = r < 0 ? 0.0 : (double)r;
+   }
+   unsigned long long mask = __ballot(bad);
+   if (mask) {
+      const int lane = threadIdx.x & 63;
+      const int first = __ffsll((long long)mask) - 1;
+      unsigned base = 0;
+      if (lane == first) base = atomicAdd(count, (unsigned)__popcll(mask));
+      base = __shfl(base, first);
+      if (bad) list[base + __popcll(mask & ((1ull << lane) - 1ull))] = (int)t;
+   }
+   kks_block_partial(it, mx, 0.0, mx >= kKksFail ? 1.0 : 0.0, 0.0, partial);
+}
+
+// Switch policy, second pass: the listed cells start from the surrogate and
+// restart from the previous-step guess when the residual after probe_its
+// updates exceeds probe_tol.  Grid-stride over the list, whole waves at a
+// time (the network's MFMA inference needs all 64 lanes).
+__global__ void __launch_bounds__(256) k_kks_surr(ampe_model_params p, MlpDev m, int probe_its,
+                                                  double probe_tol,
+                                                  const double* __restrict__ phi,
+                                                  const double* __restrict__ c,
+                                                  const double* __restrict__ clr,
+                                                  const double* __restrict__ csr,
+                                                  double* __restrict__ cl,
+                                                  double* __restrict__ cs,
+                                                  const int* __restrict__ list,
+                                                  const unsigned* __restrict__ count,
+                                                  double* __restrict__ partial)
+{
+   using F = MlpFrags<kMlpHidden>;
+   __shared__ u16x8 sf[F::NFWD * 64];
+   __shared__ float sb[F::NBIAS];
+   const long long nl = (long long)*count;
+   double it = 0.0, mx = 0.0, fb = 0.0, ss = 0.0;
+   if ((long long)blockIdx.x * blockDim.x < nl) {
+      mlp_stage<kMlpHidden>(m, F::NFWD, sf, sb);
+   }
+   __syncthreads();
+   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+   const double T = p.temperature;
+   for (long long base = ((long long)blockIdx.x * 4 + wave) * 64; base < nl;
+        base += (long long)gridDim.x * blockDim.x) {
+      long long i = base + lane;
+      bool in = i < nl;
+      long long t = in ? (long long)list[i] : 0;
+      double ct = in ? c[t] : 0.5, pt = in ? phi[t] : 0.0;
+      double gl, gs;
+      kks_surrogate_guess(m, sf, sb, ct, pt, T, gl, gs);
+      if (in) {
+         double h = interp(pt, p.conc_interp);
+         int r = kks_newton(p.calphad, T, h, ct, gl, gs, p.newton_max_its, p.newton_tol,
+                            probe_its, probe_tol);
+         if (r == -2) {
+            fb += 1.0;
+            double xl = clr[t], xs = csr[t];
+            kks_clamp(xl, xs);
+            double f1, f2;
+            kks_residual(p.calphad, T, 1.0 / (kRGas * T), h, ct, xl, xs, f1, f2);
+            int r2 = kks_newton_from(p.calphad, T, h, ct, xl, xs, f1, f2, p.newton_max_its,
+                                     p.newton_tol, -1, 0.0);
+            r = r2 < 0 ? -1 : r2 + probe_its;
+            gl = xl;
+            gs = xs;
+         }
+         cl[t] = gl;
+         cs[t] = gs;
+         ss += 1.0;
+         if (r < 0) {
+            mx = kKksFail;
+         } else {
+            it += (double)r;
+            mx = (double)r > mx ? (double)r : mx;
+         }
+      }
    }
    kks_block_partial(it, mx, fb, mx >= kKksFail ? 1.0 : 0.0, ss, partial);
 }
@@ -151,7 +220,8 @@ __global__ void __launch_bounds__(256) k_kks(long long n, ampe_model_params p, M
 // acc[3] += 1 (calls), acc[4] += fallbacks, acc[5] += surrogate starts.
 // One workgroup.
 __global__ void __launch_bounds__(1024) k_kks_finish(const double* __restrict__ partial, int nb,
-                                                     long long* __restrict__ acc)
+                                                     long long* __restrict__ acc,
+                                                     unsigned* __restrict__ count)
 {
    __shared__ double ss[1024], sm[1024], sf[1024], sg[1024];
    double s = 0.0, m = 0.0, f = 0.0, g = 0.0;
@@ -185,6 +255,7 @@ __global__ void __launch_bounds__(1024) k_kks_finish(const double* __restrict__ 
       acc[3] += 1;
       acc[4] += (long long)sf[0];
       acc[5] += (long long)sg[0];
+      if (count) *count = 0u;  // the work list is empty again for the next call
    }
 }
 
@@ -211,23 +282,35 @@ int launch_kks(const Level& L, const ampe_model_params& p, const double* phi,
                double* cs, const KksGuess& guess, long long* dev_acc, hipStream_t s)
 {
    unsigned nb = (unsigned)((L.ncg + 255) / 256);
-   MlpDev m{};
-   if (guess.surrogate) {
+   unsigned nparts = nb;
+   unsigned* count = nullptr;
+   if (guess.surrogate && p.fe_model == AMPE_FE_CALPHAD_BINARY) {
       const Surrogate& S = *guess.surrogate;
       if (S.hid != kMlpHidden || S.nin != 3 || S.nout != 2) {
          set_error("launch_kks: the surrogate must map (c, phi, T) -> (cL, cS) with 64 hidden");
          return -1;
       }
-      m = S.dev();
-      hipLaunchKernelGGL(k_kks<true>, dim3(nb), dim3(256), 0, s, L.ncg, p, m, guess.probe_its,
-                         guess.probe_tol, guess.switch_tol, phi, c, clr, csr, cl, cs,
-                         L.kks_part);
+      if (!guess.list || !guess.count) {
+         set_error("launch_kks: the surrogate start needs a work list");
+         return -1;
+      }
+      MlpDev m = S.dev();
+      count = guess.count;
+      hipLaunchKernelGGL(k_kks_defer, dim3(nb), dim3(256), 0, s, L.ncg, p, guess.switch_tol,
+                         phi, c, clr, csr, cl, cs, guess.list, count, L.kks_part);
+      AMPE_HIP_CHECK(hipGetLastError());
+      unsigned g = std::min<unsigned>(kKksSurrGrid, nb);
+      hipLaunchKernelGGL(k_kks_surr, dim3(g), dim3(256), 0, s, p, m, guess.probe_its,
+                         guess.probe_tol, phi, c, clr, csr, cl, cs, guess.list, count,
+                         L.kks_part + (size_t)kKksPart * nb);
+      nparts = nb + g;
    } else {
-      hipLaunchKernelGGL(k_kks<false>, dim3(nb), dim3(256), 0, s, L.ncg, p, m, -1, 0.0, 0.0,
-                         phi, c, clr, csr, cl, cs, L.kks_part);
+      hipLaunchKernelGGL(k_kks, dim3(nb), dim3(256), 0, s, L.ncg, p, phi, c, clr, csr, cl, cs,
+                         L.kks_part);
    }
    AMPE_HIP_CHECK(hipGetLastError());
-   hipLaunchKernelGGL(k_kks_finish, dim3(1), dim3(1024), 0, s, L.kks_part, (int)nb, dev_acc);
+   hipLaunchKernelGGL(k_kks_finish, dim3(1), dim3(1024), 0, s, L.kks_part, (int)nparts, dev_acc,
+                      count);
    AMPE_HIP_CHECK(hipGetLastError());
    return 0;
 }

@@ -89,4 +89,26 @@ int reduce_partials(const Level& L, int nparts, int ncol, int op,
 // L.partial + r * nparts), one launch and one synchronisation.
 int reduce_regions(const Level& L, int nparts, int nreg, double* host_out, hipStream_t s);
 
+// Optional event timing of selected kernels, process-wide.  While a kernel
+// class is enabled, each of its launches is bracketed by two events recorded
+// on its launch stream, with the launch's algorithmic bytes; read() waits for
+// the recorded launches and returns their total duration and bytes.  Not
+// thread-safe; launches under hipGraph capture must not be timed.
+struct KernelTimer {
+   enum Id { QUAT_RBGS = 0, COUNT = 1 };
+   struct Span {
+      Span(int id, hipStream_t s, double bytes);
+      ~Span();
+      int slot = -1;
+      hipStream_t stream = nullptr;
+   };
+   static int enable(int id, bool on);
+   static int read(int id, double* ms, long long* launches, double* bytes);
+};
+
+// Algorithmic bytes of one colour of the orientation red-black sweep: all z
+// read, the colour's z written and rhs read, every face coefficient and the
+// cell sqrt(M) read once.
+double quat_rbgs_bytes(const Level& L, int ncomp, int elem);
+
 }  // namespace ampe

@@ -29,6 +29,41 @@ static int wrapi(int i, int lo, int n)
    return lo + r;
 }
 
+// Dispatch order of a tile list.  Workgroups are dispatched round-robin over
+// the 8 XCDs (workgroup b to XCD b % 8), each XCD with its own L2.  Every
+// plane (patch, k) of the list is cut into 8 contiguous chunks of rows, chunk
+// x of every plane going to XCD x in plane order: an XCD sweeps one band of
+// rows through all planes, so the x/y halos of its tiles and the previous
+// planes it reads are in its own L2.  The per-XCD sequences are concatenated
+// and dealt out so that workgroup b takes the (b / 8)-th tile of XCD b % 8's
+// share.
+static void xcd_order(std::vector<Tile>& tiles)
+{
+   constexpr int X = 8;
+   const size_t n = tiles.size();
+   if (n == 0) return;
+   std::vector<std::vector<Tile>> seq(X);
+   size_t p0 = 0;
+   while (p0 < n) {
+      size_t p1 = p0;
+      while (p1 < n && tiles[p1].patch == tiles[p0].patch && tiles[p1].k == tiles[p0].k) p1++;
+      size_t tp = p1 - p0;
+      for (int x = 0; x < X; x++) {
+         size_t a = p0 + tp * x / X, b = p0 + tp * (x + 1) / X;
+         seq[x].insert(seq[x].end(), tiles.begin() + a, tiles.begin() + b);
+      }
+      p0 = p1;
+   }
+   std::vector<Tile> q;
+   q.reserve(n);
+   for (int x = 0; x < X; x++) q.insert(q.end(), seq[x].begin(), seq[x].end());
+   const size_t qq = n / X, r = n % X;
+   for (size_t b = 0; b < n; b++) {
+      size_t x = b % X, slot = b / X;
+      tiles[b] = q[x * qq + (x < r ? x : r) + slot];
+   }
+}
+
 int level_create(int ndim, const ampe_box* boxes, int npatch, int ghost,
                  const ampe_box* periodic, Level** out)
 {
@@ -106,6 +141,9 @@ int level_create(int ndim, const ampe_box* boxes, int npatch, int ghost,
          for (int j0 = 0; j0 < d.ny + 1; j0 += TY)
             for (int i0 = 0; i0 < d.nx + 1; i0 += TX) ft.push_back({p, i0, j0, k});
    }
+   xcd_order(ct);
+   xcd_order(ft);
+   xcd_order(zt);
    L->ncell_tiles = (int)ct.size();
    L->nface_tiles = (int)ft.size();
    L->ncol_tiles = (int)zt.size();
@@ -170,7 +208,10 @@ int level_create(int ndim, const ampe_box* boxes, int npatch, int ghost,
    if (e == hipSuccess) e = hipHostMalloc(&L->hpinned, sizeof(double) * 64);
    if (e == hipSuccess) e = hipMalloc(&L->iscratch, sizeof(int) * 64);
    if (e == hipSuccess)
-      e = hipMalloc(&L->kks_part, sizeof(double) * 8 * (size_t)((L->ncg + 255) / 256 + 1));
+      // KKS partials: one slot per 256 ghosted cells plus 256 slots of the
+      // surrogate pass
+      e = hipMalloc(&L->kks_part,
+                    sizeof(double) * 8 * ((size_t)((L->ncg + 255) / 256 + 1) + 256));
    if (e == hipSuccess)
       e = hipMemcpy(L->dp, L->hp.data(), sizeof(PatchDev) * npatch, hipMemcpyHostToDevice);
    if (e == hipSuccess)
@@ -315,6 +356,99 @@ int reduce_regions(const Level& L, int nparts, int nreg, double* host_out, hipSt
    AMPE_HIP_CHECK(hipStreamSynchronize(s));
    for (int r = 0; r < nreg; r++) host_out[r] = L.hpinned[r];
    return 0;
+}
+
+// ------------------------------------------------------------ kernel timer
+namespace {
+struct TimerRec {
+   int id;
+   hipEvent_t a, b;
+   double bytes;
+};
+bool g_timer_on[KernelTimer::COUNT] = {};
+std::vector<TimerRec> g_timer_recs;
+std::vector<hipEvent_t> g_timer_pool;
+
+hipEvent_t timer_event()
+{
+   if (!g_timer_pool.empty()) {
+      hipEvent_t e = g_timer_pool.back();
+      g_timer_pool.pop_back();
+      return e;
+   }
+   hipEvent_t e = nullptr;
+   if (hipEventCreate(&e) != hipSuccess) return nullptr;
+   return e;
+}
+}  // namespace
+
+KernelTimer::Span::Span(int id, hipStream_t s, double bytes)
+{
+   if (id < 0 || id >= COUNT || !g_timer_on[id]) return;
+   TimerRec r{id, timer_event(), timer_event(), bytes};
+   if (!r.a || !r.b || hipEventRecord(r.a, s) != hipSuccess) {
+      if (r.a) g_timer_pool.push_back(r.a);
+      if (r.b) g_timer_pool.push_back(r.b);
+      return;
+   }
+   g_timer_recs.push_back(r);
+   slot = (int)g_timer_recs.size() - 1;
+   stream = s;
+}
+
+KernelTimer::Span::~Span()
+{
+   if (slot >= 0) (void)hipEventRecord(g_timer_recs[slot].b, stream);
+}
+
+int KernelTimer::enable(int id, bool on)
+{
+   if (id < 0 || id >= COUNT) {
+      set_error("kernel timer: unknown kernel id");
+      return -1;
+   }
+   g_timer_on[id] = on;
+   return 0;
+}
+
+int KernelTimer::read(int id, double* ms, long long* launches, double* bytes)
+{
+   if (id < 0 || id >= COUNT) {
+      set_error("kernel timer: unknown kernel id");
+      return -1;
+   }
+   double tot = 0.0, by = 0.0;
+   long long n = 0;
+   std::vector<TimerRec> keep;
+   for (const TimerRec& r : g_timer_recs) {
+      if (r.id != id) {
+         keep.push_back(r);
+         continue;
+      }
+      float t = 0.0f;
+      AMPE_HIP_CHECK(hipEventSynchronize(r.b));
+      AMPE_HIP_CHECK(hipEventElapsedTime(&t, r.a, r.b));
+      tot += t;
+      by += r.bytes;
+      n++;
+      g_timer_pool.push_back(r.a);
+      g_timer_pool.push_back(r.b);
+   }
+   g_timer_recs.swap(keep);
+   if (ms) *ms = tot;
+   if (launches) *launches = n;
+   if (bytes) *bytes = by;
+   return 0;
+}
+
+double quat_rbgs_bytes(const Level& L, int ncomp, int elem)
+{
+   double n = (double)L.ncell;
+   return (double)elem * (ncomp * n            // z, all cells
+                          + ncomp * 0.5 * n    // rhs, this colour
+                          + ncomp * 0.5 * n    // z written, this colour
+                          + (double)L.nside_tot  // face coefficients
+                          + n);                // sqrt(M)
 }
 
 }  // namespace ampe

@@ -15,7 +15,7 @@ __global__ void __launch_bounds__(256) k_restrict_cell(LevelArgs ca, LevelArgs f
 {
    CellPos p;
    if (!cell_pos(ca, 0, p)) return;
-   const PatchDev fp = fa.P[ca.T[blockIdx.x].patch];
+   const PatchDev fp = fa.P[ca.T[tile_index(ca)].patch];
    int fk = NDIM == 3 ? 2 * p.k : 0;
    long long f0 = fp.cell_off + (2 * p.i + fa.g) + (long long)(2 * p.j + fa.g) * fp.sy +
                   (long long)(fk + fa.gz) * fp.sz;
@@ -37,7 +37,7 @@ __global__ void __launch_bounds__(256) k_restrict_side(LevelArgs ca, LevelArgs f
    CellPos p;
    if (!cell_pos(ca, 1, p)) return;
    if (NDIM == 2 && p.k > 0) return;
-   const PatchDev fp = fa.P[ca.T[blockIdx.x].patch];
+   const PatchDev fp = fa.P[ca.T[tile_index(ca)].patch];
    for (int d = 0; d < NDIM; d++) {
       bool ok = (p.i < p.pd.nx + (d == 0)) && (p.j < p.pd.ny + (d == 1)) &&
                 (NDIM == 2 || p.k < p.pd.nz + (d == 2));
@@ -69,7 +69,7 @@ __global__ void __launch_bounds__(256) k_prolong_add(LevelArgs fa, LevelArgs ca,
 {
    CellPos p;
    if (!cell_pos(fa, 0, p)) return;
-   const PatchDev cp = ca.P[fa.T[blockIdx.x].patch];
+   const PatchDev cp = ca.P[fa.T[tile_index(fa)].patch];
    int K = NDIM == 3 ? (p.k >> 1) : 0;
    long long c0 = cp.cell_off + ((p.i >> 1) + ca.g) + (long long)((p.j >> 1) + ca.g) * cp.sy +
                   (long long)(K + ca.gz) * cp.sz;

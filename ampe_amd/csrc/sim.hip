@@ -142,6 +142,8 @@ void sim_destroy(Sim* S)
                     S->f_sqrtm, S->f_Dconc, S->f_fc, S->f_Mphi};
    for (float* v : fall) (void)hipFree(v);
    (void)hipFree(S->kks_acc);
+   (void)hipFree(S->guess.list);
+   (void)hipFree(S->guess.count);
    mg_destroy(S->mgf);
    mg_destroy(S->mgd);
    if (S->stream) (void)hipStreamDestroy(S->stream);

@@ -43,6 +43,9 @@ _SIGS = {
     "ampe_conc_rhs": (_i, [_vp, C.POINTER(ModelParams), _vp, _vp, _vp, _vp, _vp, _vp]),
     "ampe_kks_solve": (_i, [_vp, C.POINTER(ModelParams), _vp, _vp, _vp, _vp, _vp, _vp,
                             C.POINTER(_i), _vp]),
+    "ampe_kernel_timer_enable": (_i, [C.c_int, C.c_int]),
+    "ampe_kernel_timer_read": (_i, [C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_longlong),
+                                    C.POINTER(C.c_double)]),
     "ampe_conc_coeffs_ebs": (_i, [_vp, C.POINTER(ModelParams), _vp, _vp, _vp, _vp]),
     "ampe_conc_rhs_ebs": (_i, [_vp, C.POINTER(ModelParams), _vp, _vp, _vp, _vp, _vp, _vp]),
     "ampe_tag_cells": (_i, [_vp, C.POINTER(ModelParams), _vp, _d, _vp, _vp]),

@@ -283,6 +283,16 @@ int ampe_conc_rhs_ebs(const ampe_level* level, const ampe_model_params* p,
                       const double* cl, const double* cs, const double* DL,
                       const double* DA, double* rhs, ampe_stream_t stream);
 
+/* Event timing of one kernel class inside whatever runs while it is enabled
+ * (used by the benchmark to measure a kernel live inside the timed steps).
+ * kernel: enum ampe_timed_kernel.  ampe_kernel_timer_read waits for the
+ * recorded launches, returns their total duration [ms], count and algorithmic
+ * bytes, and clears them.  No reference counterpart (the reference times whole
+ * phases with SAMRAI timers); process-wide, not thread-safe. */
+enum ampe_timed_kernel { AMPE_TIMED_QUAT_RBGS = 0 };
+int ampe_kernel_timer_enable(int kernel, int on);
+int ampe_kernel_timer_read(int kernel, double* ms, long long* launches, double* bytes);
+
 /* AMR gradient detector: tags = 1 where |grad phi * dx|^2 >= tol^2, else 0
  * (QuatModel::tagGradientDetectorCells source/QuatModel.cc:3476-3602,
  * F77 tagfromgrads quattag.m4:9).  tags: int, interior cells per patch. */
