@@ -49,10 +49,11 @@ import numpy as np  # noqa: E402
 from ampe_amd import params as P  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
-# HBM bytes per launch of k_quat_rbgs<float,3> averaged over the launches of
-# the headline form (FETCH_SIZE x 2 + WRITE_SIZE, scaled by a 1 GiB copy),
-# measured with rocprofv3 --pmc; None until measured.
-QUAT_RBGS_TRAFFIC = None
+# HBM bytes per launch of k_quat_rbgs<float,3>, mean over the three
+# multigrid levels of the headline form: rocprofv3 --pmc FETCH_SIZE and
+# WRITE_SIZE per dispatch, scaled by a 1 GiB device copy (2.0 x FETCH_SIZE +
+# 1.0 x WRITE_SIZE); profiles/quat_rbgs_f32_polycrystal256_traffic.txt.
+QUAT_RBGS_TRAFFIC = 428.0e6
 
 
 def log(*a):
