@@ -57,12 +57,13 @@ int vec_sqrt(long long n, const double* x, double* y, hipStream_t s);
 // Preconditioner input / output over nc component blocks (orientation blocks
 // [iq, iq+ql) scaled by 1/sqrt(M) on input, sqrt(M) on output); psolve_in
 // also zeroes u.  With f null (FP64) only the orientation blocks go to fq.
+// psolve_out multiplies the result by ewt when ewt is not null.
 template <typename T>
 int psolve_in(long long ng, int nc, int iq, int ql, const double* r, const double* sm, T* f,
               T* fq, T* u, hipStream_t s);
 template <typename T>
-int psolve_out(long long ng, int nc, int iq, int ql, const T* u, const double* sm, double* z,
-               hipStream_t s);
+int psolve_out(long long ng, int nc, int iq, int ql, const T* u, const double* sm,
+               const double* ewt, double* z, hipStream_t s);
 int vec_scale_comp(long long ng, int ncomp, const double* sm, double* x, hipStream_t s);
 // Reductions over interior cells of ncomp component blocks (host result,
 // synchronises s).

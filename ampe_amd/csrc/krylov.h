@@ -18,6 +18,13 @@ int kry_prep(const Level& L, int nc, double hinv, double* V, const double* ewt, 
              double* sumsq, hipStream_t s);
 int kry_mid(const Level& L, int nc, double sig, const double* xk, const double* ycur,
             double* work2, hipStream_t s);
+// The same with the direction (V h) / ewt computed on the fly from an
+// unnormalised basis vector V and its scale h (no xk array).
+int kry_mid_scaled(const Level& L, int nc, double sig, const double* V, double h,
+                   const double* ewt, const double* ycur, double* work2, hipStream_t s);
+int kry_finish_scaled(const Level& L, int nc, double siginv, double gamma, const double* V,
+                      double h, const double* ewt, const double* jv, const double* fcur,
+                      double* out, hipStream_t s);
 int kry_finish(const Level& L, int nc, double siginv, double gamma, const double* xk,
                const double* jv, const double* fcur, double* out, hipStream_t s);
 int kry_scale_dot(const Level& L, int nc, const double* ewt, const double* work, double* out,

@@ -43,9 +43,22 @@ __global__ void __launch_bounds__(256) k_gmres_prep(LevelArgs a, int nc, double 
    tile_store_sum(acc, partial);
 }
 
-// work2 = sig xk + ycur
-__global__ void __launch_bounds__(256) k_gmres_mid(LevelArgs a, int nc, double sig,
-                                                   const double* __restrict__ xk,
+// Krylov direction x = V / ewt: the stored xk, or (V h) / ewt from an
+// unnormalised basis vector V with scale h (xk == nullptr).
+struct KryDir {
+   const double* xk;
+   const double* V;
+   double h;
+   const double* ewt;
+};
+
+__device__ __forceinline__ double kry_dir(const KryDir& d, long long i)
+{
+   return d.xk ? d.xk[i] : (d.V[i] * d.h) / d.ewt[i];
+}
+
+// work2 = sig x + ycur
+__global__ void __launch_bounds__(256) k_gmres_mid(LevelArgs a, int nc, double sig, KryDir x,
                                                    const double* __restrict__ ycur,
                                                    double* __restrict__ work2)
 {
@@ -53,14 +66,13 @@ __global__ void __launch_bounds__(256) k_gmres_mid(LevelArgs a, int nc, double s
    if (!cell_pos(a, 0, c)) return;
    for (int m = 0; m < nc; m++) {
       long long i = m * a.ncg + c.c0;
-      work2[i] = sig * xk[i] + ycur[i];
+      work2[i] = sig * kry_dir(x, i) + ycur[i];
    }
 }
 
-// out = xk + (-gamma) (siginv jv + (-siginv) fcur)
+// out = x + (-gamma) (siginv jv + (-siginv) fcur)
 __global__ void __launch_bounds__(256) k_gmres_finish(LevelArgs a, int nc, double siginv,
-                                                      double gamma,
-                                                      const double* __restrict__ xk,
+                                                      double gamma, KryDir x,
                                                       const double* __restrict__ jv,
                                                       const double* __restrict__ fcur,
                                                       double* __restrict__ out)
@@ -70,7 +82,7 @@ __global__ void __launch_bounds__(256) k_gmres_finish(LevelArgs a, int nc, doubl
    for (int m = 0; m < nc; m++) {
       long long i = m * a.ncg + c.c0;
       double jvn = siginv * jv[i] + (-siginv) * fcur[i];
-      out[i] = xk[i] + (-gamma) * jvn;
+      out[i] = kry_dir(x, i) + (-gamma) * jvn;
    }
 }
 
@@ -249,14 +261,31 @@ int kry_prep(const Level& L, int nc, double hinv, double* V, const double* ewt, 
 int kry_mid(const Level& L, int nc, double sig, const double* xk, const double* ycur,
             double* work2, hipStream_t s)
 {
-   AMPE_TILE_LAUNCH(k_gmres_mid, nc, sig, xk, ycur, work2);
+   AMPE_TILE_LAUNCH(k_gmres_mid, nc, sig, KryDir{xk, nullptr, 0.0, nullptr}, ycur, work2);
+   return 0;
+}
+
+int kry_mid_scaled(const Level& L, int nc, double sig, const double* V, double h,
+                   const double* ewt, const double* ycur, double* work2, hipStream_t s)
+{
+   AMPE_TILE_LAUNCH(k_gmres_mid, nc, sig, KryDir{nullptr, V, h, ewt}, ycur, work2);
    return 0;
 }
 
 int kry_finish(const Level& L, int nc, double siginv, double gamma, const double* xk,
                const double* jv, const double* fcur, double* out, hipStream_t s)
 {
-   AMPE_TILE_LAUNCH(k_gmres_finish, nc, siginv, gamma, xk, jv, fcur, out);
+   AMPE_TILE_LAUNCH(k_gmres_finish, nc, siginv, gamma, KryDir{xk, nullptr, 0.0, nullptr}, jv,
+                    fcur, out);
+   return 0;
+}
+
+int kry_finish_scaled(const Level& L, int nc, double siginv, double gamma, const double* V,
+                      double h, const double* ewt, const double* jv, const double* fcur,
+                      double* out, hipStream_t s)
+{
+   AMPE_TILE_LAUNCH(k_gmres_finish, nc, siginv, gamma, KryDir{nullptr, V, h, ewt}, jv, fcur,
+                    out);
    return 0;
 }
 

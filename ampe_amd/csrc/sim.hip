@@ -561,7 +561,10 @@ static std::vector<PBlock<T>> make_blocks(Sim* S, double dphi, const T* Cphi, co
    return bl;
 }
 
-static int psolve(Sim* S, const double* r, double* z, double delta)
+// z = P^-1 r; with ewt_out (FP32 preconditioner only) z = ewt_out (P^-1 r),
+// and z may alias r.
+static int psolve(Sim* S, const double* r, double* z, double delta,
+                  const double* ewt_out = nullptr)
 {
    const ampe_model_params& p = S->p;
    hipStream_t st = S->stream;
@@ -585,8 +588,12 @@ static int psolve(Sim* S, const double* r, double* z, double delta)
       }
       int ret = run_blocks<float>(S, bl, delta);
       if (ret < 0) return -1;
-      if (psolve_out<float>(ng, S->nc, iq, ql, S->f_u, S->sqrtm, z, st)) return -1;
+      if (psolve_out<float>(ng, S->nc, iq, ql, S->f_u, S->sqrtm, ewt_out, z, st)) return -1;
       return ret;
+   }
+   if (ewt_out) {
+      set_error("psolve: the scaled output is for the FP32 preconditioner");
+      return -1;
    }
    // FP64: z = 0, fac_u = orientation blocks of r / sqrt(M)
    if (psolve_in<double>(ng, S->nc, iq, ql, r, S->sqrtm, nullptr, S->fac_u, z, st)) return -1;
@@ -632,6 +639,14 @@ static int spgmr(Sim* S, double gamma, const double* bvec, double* x, double del
    int ld = maxl + 1;
 #define HH(i, j) Hm[(size_t)(i) * ld + (j)]
    double givens[2 * kMaxKrylov], g[kMaxKrylov + 1], yv[kMaxKrylov];
+   // Classical Gram-Schmidt keeps the basis unnormalised: basis vector i is
+   // V[i] hs[i].  The difference-quotient increment then needs no norm pass:
+   // the normalised vector has sum(v^2) cv = 1, so its WRMS norm over ewt is
+   // sqrt(wfac / cv) and sigma = sqrt(cv / wfac).  With the FP32
+   // preconditioner, its output is scaled by ewt in the conversion kernel.
+   const bool cgs = S->s.gstype == AMPE_GS_CLASSICAL;
+   const bool fuse_out = cgs && S->s.precision == AMPE_FAC_FP32;
+   double hs[kMaxKrylov + 1];
    if (psolve(S, bvec, S->work, delta) < 0) return -1;
    double rho;
    if (kry_scale_dot(L, nc, S->ewt, S->work, S->V[0], S->V[0], &rho, st)) return -1;
@@ -642,21 +657,39 @@ static int spgmr(Sim* S, double gamma, const double* bvec, double* x, double del
       return 0;
    }
    double hinv = 1.0 / rho;
+   hs[0] = hinv;
    std::memset(g, 0, sizeof(g));
    g[0] = rho;
    int lused = 0;
    bool converged = false;
    for (int l = 0; l < maxl; l++) {
       S->st.nli++;
-      double ss;
-      // V[l] *= hinv; xk = V[l] / ewt; ||xk||_wrms
-      if (kry_prep(L, nc, hinv, S->V[l], S->ewt, S->xk, &ss, st)) return -1;
-      double sig = 1.0 / std::sqrt(ss * wfac);
-      if (kry_mid(L, nc, sig, S->xk, S->ycur, S->work2, st)) return -1;
-      if (rhs_eval(S, S->work2, S->jv, true)) return -1;
-      double siginv = 1.0 / sig;
-      if (kry_finish(L, nc, siginv, gamma, S->xk, S->jv, S->fcur, S->V[l + 1], st)) return -1;
-      if (psolve(S, S->V[l + 1], S->work, delta) < 0) return -1;
+      if (cgs) {
+         double sig = std::sqrt(cv / wfac);
+         if (kry_mid_scaled(L, nc, sig, S->V[l], hs[l], S->ewt, S->ycur, S->work2, st))
+            return -1;
+         if (rhs_eval(S, S->work2, S->jv, true)) return -1;
+         double siginv = 1.0 / sig;
+         if (kry_finish_scaled(L, nc, siginv, gamma, S->V[l], hs[l], S->ewt, S->jv, S->fcur,
+                               S->V[l + 1], st))
+            return -1;
+      } else {
+         double ss;
+         // V[l] *= hinv; xk = V[l] / ewt; ||xk||_wrms
+         if (kry_prep(L, nc, hinv, S->V[l], S->ewt, S->xk, &ss, st)) return -1;
+         double sig = 1.0 / std::sqrt(ss * wfac);
+         if (kry_mid(L, nc, sig, S->xk, S->ycur, S->work2, st)) return -1;
+         if (rhs_eval(S, S->work2, S->jv, true)) return -1;
+         double siginv = 1.0 / sig;
+         if (kry_finish(L, nc, siginv, gamma, S->xk, S->jv, S->fcur, S->V[l + 1], st))
+            return -1;
+      }
+      if (fuse_out) {
+         // V[l+1] = ewt (P^-1 V[l+1]) in place
+         if (psolve(S, S->V[l + 1], S->V[l + 1], delta, S->ewt) < 0) return -1;
+      } else if (psolve(S, S->V[l + 1], S->work, delta) < 0) {
+         return -1;
+      }
       double hn;
       if (S->s.gstype == AMPE_GS_CLASSICAL) {
          // V[l+1] = ewt work with all its dots in one pass, one linear
@@ -666,11 +699,15 @@ static int spgmr(Sim* S, double gamma, const double* bvec, double* x, double del
          vl.n = l + 1;
          for (int i = 0; i <= l; i++) vl.v[i] = S->V[i];
          double sums[kMaxCgs];
-         if (kry_cgs_dots(L, nc, S->ewt, S->work, S->V[l + 1], vl, sums, st)) return -1;
+         if (fuse_out) {
+            if (kry_cgs_dots(L, nc, nullptr, nullptr, S->V[l + 1], vl, sums, st)) return -1;
+         } else if (kry_cgs_dots(L, nc, S->ewt, S->work, S->V[l + 1], vl, sums, st)) {
+            return -1;
+         }
          double vk_norm = std::sqrt(sums[l + 1] * cv);
          for (int i = 0; i <= l; i++) {
-            HH(i, l) = sums[i] * cv;
-            vl.y[i] = -HH(i, l);
+            HH(i, l) = (sums[i] * hs[i]) * cv;
+            vl.y[i] = -HH(i, l) * hs[i];
          }
          double ss;
          if (kry_cgs_update(L, nc, S->V[l + 1], vl, &ss, st)) return -1;
@@ -678,9 +715,9 @@ static int spgmr(Sim* S, double gamma, const double* bvec, double* x, double del
          if (1000.0 * hn < vk_norm) {
             if (kry_cgs_dots(L, nc, nullptr, nullptr, S->V[l + 1], vl, sums, st)) return -1;
             for (int i = 0; i <= l; i++) {
-               double d = sums[i] * cv;
+               double d = (sums[i] * hs[i]) * cv;
                HH(i, l) += d;
-               vl.y[i] = -d;
+               vl.y[i] = -d * hs[i];
             }
             if (kry_cgs_update(L, nc, S->V[l + 1], vl, &ss, st)) return -1;
             hn = std::sqrt(ss * cv);
@@ -731,6 +768,7 @@ static int spgmr(Sim* S, double gamma, const double* bvec, double* x, double del
       }
       if (hn == 0.0) break;
       hinv = 1.0 / hn;
+      hs[l + 1] = hinv;
    }
    for (int i = lused - 1; i >= 0; i--) {
       double sacc = g[i];
@@ -741,7 +779,7 @@ static int spgmr(Sim* S, double gamma, const double* bvec, double* x, double del
    vl.n = lused;
    for (int i = 0; i < lused; i++) {
       vl.v[i] = S->V[i];
-      vl.y[i] = yv[i];
+      vl.y[i] = cgs ? yv[i] * hs[i] : yv[i];
    }
    if (kry_lincomb(L, nc, vl, S->ewt, x, st)) return -1;
 #undef HH

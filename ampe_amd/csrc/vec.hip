@@ -161,11 +161,13 @@ __global__ void __launch_bounds__(256) k_psolve_in(long long ng, int nc, int iq,
    }
 }
 
-// Preconditioner output: z = u, orientation blocks multiplied by sqrt(M).
+// Preconditioner output: z = u, orientation blocks multiplied by sqrt(M),
+// then by ewt when given (z = ewt u).
 template <typename T>
 __global__ void __launch_bounds__(256) k_psolve_out(long long ng, int nc, int iq, int ql,
                                                     const T* __restrict__ u,
                                                     const double* __restrict__ sm,
+                                                    const double* __restrict__ ewt,
                                                     double* __restrict__ z)
 {
    long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -173,6 +175,7 @@ __global__ void __launch_bounds__(256) k_psolve_out(long long ng, int nc, int iq
    long long m = i / ng, c = i - m * ng;
    double v = (double)u[i];
    if (m >= iq && m < iq + ql) v = v * sm[c];
+   if (ewt) v = ewt[i] * v;
    z[i] = v;
 }
 
@@ -188,12 +191,12 @@ int psolve_in(long long ng, int nc, int iq, int ql, const double* r, const doubl
 }
 
 template <typename T>
-int psolve_out(long long ng, int nc, int iq, int ql, const T* u, const double* sm, double* z,
-               hipStream_t s)
+int psolve_out(long long ng, int nc, int iq, int ql, const T* u, const double* sm,
+               const double* ewt, double* z, hipStream_t s)
 {
    long long n = ng * nc;
    hipLaunchKernelGGL(k_psolve_out<T>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, ng,
-                      nc, iq, ql, u, sm, z);
+                      nc, iq, ql, u, sm, ewt, z);
    AMPE_HIP_CHECK(hipGetLastError());
    return 0;
 }
@@ -223,9 +226,9 @@ template int psolve_in<float>(long long, int, int, int, const double*, const dou
                               float*, float*, hipStream_t);
 template int psolve_in<double>(long long, int, int, int, const double*, const double*, double*,
                                double*, double*, hipStream_t);
-template int psolve_out<float>(long long, int, int, int, const float*, const double*, double*,
-                               hipStream_t);
+template int psolve_out<float>(long long, int, int, int, const float*, const double*,
+                               const double*, double*, hipStream_t);
 template int psolve_out<double>(long long, int, int, int, const double*, const double*,
-                                double*, hipStream_t);
+                                const double*, double*, hipStream_t);
 
 }  // namespace ampe
