@@ -10,6 +10,8 @@
 
 #define AMPE_HD __host__ __device__ __forceinline__
 
+#include "mathfn.h"
+
 namespace ampe {
 
 constexpr double kRGas = 8.314472;  // J/(K mol)
@@ -148,7 +150,7 @@ AMPE_HD double calphad_g(const ampe_calphad_phase& cp, double T, double c)
    double x = 2.0 * c - 1.0;
    double P = cp.L[0] + x * (cp.L[1] + x * (cp.L[2] + x * cp.L[3]));
    return c * cp.gA + (1.0 - c) * cp.gB +
-          RT * (c * log(c) + (1.0 - c) * log(1.0 - c)) + c * (1.0 - c) * P;
+          RT * (c * log_pos(c) + (1.0 - c) * log_pos(1.0 - c)) + c * (1.0 - c) * P;
 }
 
 AMPE_HD double calphad_mu(const ampe_calphad_phase& cp, double T, double c)
@@ -157,7 +159,7 @@ AMPE_HD double calphad_mu(const ampe_calphad_phase& cp, double T, double c)
    double x = 2.0 * c - 1.0;
    double P = cp.L[0] + x * (cp.L[1] + x * (cp.L[2] + x * cp.L[3]));
    double dP = cp.L[1] + x * (2.0 * cp.L[2] + x * 3.0 * cp.L[3]);
-   return cp.gA - cp.gB + RT * log(c / (1.0 - c)) + (1.0 - 2.0 * c) * P +
+   return cp.gA - cp.gB + RT * log_pos(c / (1.0 - c)) + (1.0 - 2.0 * c) * P +
           2.0 * c * (1.0 - c) * dP;
 }
 
@@ -196,8 +198,8 @@ AMPE_HD double fenergy_t(const ampe_model_params& p, double c, int ph)
    }
    if (fe == AMPE_FE_KKS_DILUTE) {
       double RT = kRGas * p.temperature;
-      double f = RT * (c * log(c) - c);
-      if (ph == 1) f = f - RT * log(p.dilute_keq) * c + dilute_dG(p);
+      double f = RT * (c * log_pos(c) - c);
+      if (ph == 1) f = f - RT * log_pos(p.dilute_keq) * c + dilute_dG(p);
       return f;
    }
    return calphad_g(p.calphad[ph], p.temperature, c);
@@ -210,8 +212,8 @@ AMPE_HD double dfenergy_t(const ampe_model_params& p, double c, int ph)
    if (fe == AMPE_FE_QUADRATIC) return 2.0 * p.quad_A[ph] * (c - quad_ceq(p, ph));
    if (fe == AMPE_FE_KKS_DILUTE) {
       double RT = kRGas * p.temperature;
-      double mu = RT * log(c);
-      if (ph == 1) mu = mu - RT * log(p.dilute_keq);
+      double mu = RT * log_pos(c);
+      if (ph == 1) mu = mu - RT * log_pos(p.dilute_keq);
       return mu;
    }
    return calphad_mu(p.calphad[ph], p.temperature, c);
@@ -285,7 +287,7 @@ AMPE_HD double calphad_gfun(const ampe_calphad_gfun& f, double T)
    double T2 = T * T, T3 = T2 * T;
    double T7 = T3 * T3 * T;
    double Tm9 = 1.0 / (T3 * T3 * T3);
-   double g = a[0] + a[1] * T + a[2] * T * log(T);
+   double g = a[0] + a[1] * T + a[2] * T * log_pos(T);
    g = g + a[3] * T2;
    g = g + a[4] * T3;
    g = g + a[5] * T7;
