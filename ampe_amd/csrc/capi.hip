@@ -142,6 +142,22 @@ int ampe_conc_coeffs(const ampe_level* level, const ampe_model_params* p, const 
    return ampe::launch_conc_coeffs(*level->L, *p, phi, cl, cs, Dc, Dphi, hs(stream));
 }
 
+int ampe_quat_face_coefs(const ampe_level* level, const ampe_model_params* p, const double* phi,
+                         const double* n2, double* fc, ampe_stream_t stream)
+{
+   AMPE_REQUIRE(level && phi && n2 && fc, "null argument");
+   if (check_params(p, *level->L)) return -1;
+   return ampe::launch_quat_face_coef(*level->L, *p, phi, n2, fc, hs(stream));
+}
+
+int ampe_quat_rhs_fc(const ampe_level* level, const ampe_model_params* p, const double* phi,
+                     const double* q, const double* fc, double* rhs, ampe_stream_t stream)
+{
+   AMPE_REQUIRE(level && phi && q && fc && rhs, "null argument");
+   if (check_params(p, *level->L)) return -1;
+   return ampe::launch_quat_rhs_fc(*level->L, *p, phi, q, fc, rhs, hs(stream));
+}
+
 int ampe_kernel_timer_enable(int kernel, int on)
 {
    return ampe::KernelTimer::enable(kernel, on != 0);

@@ -136,6 +136,9 @@ int launch_phase_rhs(const Level& L, const ampe_model_params& p, const double* p
                      hipStream_t s);
 int launch_quat_rhs(const Level& L, const ampe_model_params& p, const double* phi,
                     const double* q, const double* n2, double* rhs, hipStream_t s);
+// The same from face coefficients computed beforehand by launch_quat_face_coef.
+int launch_quat_rhs_fc(const Level& L, const ampe_model_params& p, const double* phi,
+                       const double* q, const double* fc, double* rhs, hipStream_t s);
 int launch_quat_project(const Level& L, const ampe_model_params& p, const double* q,
                         double* corr, double* err, hipStream_t s);
 int launch_conc_coeffs(const Level& L, const ampe_model_params& p, const double* phi,

@@ -307,11 +307,14 @@ __device__ __forceinline__ double quat_face_coef(const ampe_model_params& p, con
 // (source/QuatFACOps.cc:1861-1890, 2089-2108): flux = fc dq/dh
 // (compute_flux2d_from_gradq), lambda (compute_lambda_flux2d/3d),
 // rhs = -M_q (div + 2 q lambda) (add_quat_proj_op2d/3d).
-template <int NDIM>
+// FC: `face` holds the face coefficients (one evaluation per face by
+// k_quat_face_coef); otherwise it holds the face |grad q|^2 and each cell
+// evaluates the coefficients of its six faces.
+template <int NDIM, bool FC>
 __global__ void __launch_bounds__(256) k_quat_rhs(LevelArgs a, ampe_model_params p, RhsConst k,
                                                   const double* __restrict__ phi,
                                                   const double* __restrict__ q,
-                                                  const double* __restrict__ n2,
+                                                  const double* __restrict__ face,
                                                   double* __restrict__ rhs)
 {
    AMPE_COLUMN_BEGIN(a, c)
@@ -325,8 +328,13 @@ __global__ void __launch_bounds__(256) k_quat_rhs(LevelArgs a, ampe_model_params
       long long flo = face_idx(a, c.pd, d, lo[0], lo[1], lo[2]);
       lo[d] += 1;
       long long fhi = face_idx(a, c.pd, d, lo[0], lo[1], lo[2]);
-      fclo[d] = quat_face_coef(p, k, phi[c0 - c.st[d]], phi[c0], n2[flo]);
-      fchi[d] = quat_face_coef(p, k, phi[c0], phi[c0 + c.st[d]], n2[fhi]);
+      if (FC) {
+         fclo[d] = face[flo];
+         fchi[d] = face[fhi];
+      } else {
+         fclo[d] = quat_face_coef(p, k, phi[c0 - c.st[d]], phi[c0], face[flo]);
+         fchi[d] = quat_face_coef(p, k, phi[c0], phi[c0 + c.st[d]], face[fhi]);
+      }
    }
    const double* dinv = k.dxinv;
    const double* fac = k.hdxinv;
@@ -627,7 +635,30 @@ int launch_phase_rhs(const Level& L, const ampe_model_params& p, const double* p
 int launch_quat_rhs(const Level& L, const ampe_model_params& p, const double* phi,
                     const double* q, const double* n2, double* rhs, hipStream_t s)
 {
-   AMPE_DISPATCH(L, k_quat_rhs, L.ncol_tiles, level_args_col(L), p, rhs_const(p), phi, q, n2, rhs);
+   RhsConst k = rhs_const(p);
+   dim3 g(L.ncol_tiles), b = tile_block();
+   if (L.ndim == 3)
+      hipLaunchKernelGGL((k_quat_rhs<3, false>), g, b, 0, s, level_args_col(L), p, k, phi, q, n2,
+                         rhs);
+   else
+      hipLaunchKernelGGL((k_quat_rhs<2, false>), g, b, 0, s, level_args_col(L), p, k, phi, q, n2,
+                         rhs);
+   AMPE_HIP_CHECK(hipGetLastError());
+   return 0;
+}
+
+int launch_quat_rhs_fc(const Level& L, const ampe_model_params& p, const double* phi,
+                       const double* q, const double* fc, double* rhs, hipStream_t s)
+{
+   RhsConst k = rhs_const(p);
+   dim3 g(L.ncol_tiles), b = tile_block();
+   if (L.ndim == 3)
+      hipLaunchKernelGGL((k_quat_rhs<3, true>), g, b, 0, s, level_args_col(L), p, k, phi, q, fc,
+                         rhs);
+   else
+      hipLaunchKernelGGL((k_quat_rhs<2, true>), g, b, 0, s, level_args_col(L), p, k, phi, q, fc,
+                         rhs);
+   AMPE_HIP_CHECK(hipGetLastError());
    return 0;
 }
 

@@ -31,6 +31,7 @@ struct Sim {
    double *cl = nullptr, *cs = nullptr, *clr = nullptr, *csr = nullptr;
    // lagged face data and preconditioner coefficients
    double *n2 = nullptr, *Dc = nullptr, *Dphi = nullptr, *Dconc = nullptr, *fc = nullptr;
+   double* fcq = nullptr;  // orientation face coefficients of the current RHS evaluation
    double *Cphi = nullptr, *sqrtm = nullptr, *Mphi = nullptr;
    double *fac_u = nullptr, *fac_e = nullptr, *fac_r = nullptr;
    // FP32 preconditioner copies (mixed precision)

@@ -92,7 +92,7 @@ int sim_create(const ampe_model_params& p, const ampe_solver_params& s, const am
    rc |= dalloc(&S->fac_u, S->ng * ql);
    rc |= dalloc(&S->fac_e, S->nv);
    rc |= dalloc(&S->fac_r, S->nv);
-   double** sides[] = {&S->n2, &S->Dc, &S->Dphi, &S->Dconc, &S->fc};
+   double** sides[] = {&S->n2, &S->Dc, &S->Dphi, &S->Dconc, &S->fc, &S->fcq};
    for (auto v : sides) rc |= dalloc(v, S->nf);
    if (s.precision == AMPE_FAC_FP32) {
       rc |= falloc(&S->f_u, S->nv);
@@ -135,7 +135,7 @@ void sim_destroy(Sim* S)
                     S->ewt, S->work, S->work2, S->jv, S->xk, S->corr, S->acor, S->cl, S->cs,
                     S->clr, S->csr, S->Cphi, S->sqrtm, S->Mphi, S->fac_u, S->fac_e, S->fac_r,
                     S->n2,
-                    S->Dc, S->Dphi, S->Dconc, S->fc};
+                    S->Dc, S->Dphi, S->Dconc, S->fc, S->fcq};
    for (double* v : all) (void)hipFree(v);
    for (double* v : S->V) (void)hipFree(v);
    float* fall[] = {S->f_u,    S->f_e,     S->f_r,     S->f_f,   S->f_Cphi,
@@ -301,7 +301,10 @@ static int rhs_eval(Sim* S, double* y, double* ydot, bool fd)
    if (q && recompute && launch_quat_grad(L, p, q, nullptr, S->n2, st)) return -1;
    if (c && kks(S, y)) return -1;
    if (launch_phase_rhs(L, p, phi, q, S->cl, S->cs, ydot, st)) return -1;
-   if (q && launch_quat_rhs(L, p, phi, q, S->n2, ydot + (long long)S->iq * S->ng, st)) return -1;
+   // face coefficients once per face, then the orientation RHS from them
+   if (q && launch_quat_face_coef(L, p, phi, S->n2, S->fcq, st)) return -1;
+   if (q && launch_quat_rhs_fc(L, p, phi, q, S->fcq, ydot + (long long)S->iq * S->ng, st))
+      return -1;
    if (c && p.conc_rhs_form == AMPE_CONC_RHS_EBS) {
       // Dc, Dphi hold the liquid and solid face diffusivities
       if (recompute && launch_ebs_coeffs(L, p, phi, S->Dc, S->Dphi, st)) return -1;

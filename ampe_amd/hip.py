@@ -43,6 +43,8 @@ _SIGS = {
     "ampe_conc_rhs": (_i, [_vp, C.POINTER(ModelParams), _vp, _vp, _vp, _vp, _vp, _vp]),
     "ampe_kks_solve": (_i, [_vp, C.POINTER(ModelParams), _vp, _vp, _vp, _vp, _vp, _vp,
                             C.POINTER(_i), _vp]),
+    "ampe_quat_face_coefs": (_i, [_vp, C.POINTER(ModelParams), _vp, _vp, _vp, _vp]),
+    "ampe_quat_rhs_fc": (_i, [_vp, C.POINTER(ModelParams), _vp, _vp, _vp, _vp, _vp]),
     "ampe_kernel_timer_enable": (_i, [C.c_int, C.c_int]),
     "ampe_kernel_timer_read": (_i, [C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_longlong),
                                     C.POINTER(C.c_double)]),

@@ -235,6 +235,18 @@ int ampe_quat_rhs(const ampe_level* level, const ampe_model_params* p,
                   const double* phi, const double* q,
                   const double* face_norm2, double* rhs, ampe_stream_t stream);
 
+/* The same in two passes, as QuatFACOps does it: the face coefficients of
+ * every face from the face |grad q|^2 (QuatFACOps::computeFaceCoefs,
+ * source/QuatFACOps.cc:735-829, F77 compute_face_coef2d/3d), then the RHS
+ * from them (evaluateRHS with the coefficients, source/QuatFACOps.cc
+ * :1861-1890).  Same values as ampe_quat_rhs; fc is a side field. */
+int ampe_quat_face_coefs(const ampe_level* level, const ampe_model_params* p,
+                         const double* phi, const double* n2, double* fc,
+                         ampe_stream_t stream);
+int ampe_quat_rhs_fc(const ampe_level* level, const ampe_model_params* p,
+                     const double* phi, const double* q, const double* fc, double* rhs,
+                     ampe_stream_t stream);
+
 /* CVODE projection of q onto |q| = 1 (QuatIntegrator::applyProjection,
  * source/QuatIntegrator.cc:3911-3965; F77 project2d quatfacops.m4:940). */
 int ampe_quat_project(const ampe_level* level, const ampe_model_params* p,

@@ -127,6 +127,12 @@ def test_quat_rhs_and_project(dev, ndim):
     r = _dev(np.zeros_like(r_o), dev)
     H.check(H.lib().ampe_quat_rhs(lev.h, C.byref(p), _p(phd), _p(qd), _p(n2d), _p(r), None))
     assert _rel(pt.interior(r.cpu().numpy()), pt.interior(r_o)) < 1e-12
+    # two passes (face coefficients, then the RHS from them): the same values
+    fcd = _dev(np.zeros(pt.nf), dev)
+    r2 = _dev(np.zeros_like(r_o), dev)
+    H.check(H.lib().ampe_quat_face_coefs(lev.h, C.byref(p), _p(phd), _p(n2d), _p(fcd), None))
+    H.check(H.lib().ampe_quat_rhs_fc(lev.h, C.byref(p), _p(phd), _p(qd), _p(fcd), _p(r2), None))
+    assert np.array_equal(pt.interior(r2.cpu().numpy()), pt.interior(r.cpu().numpy()))
     # projection
     err = pt.cells(4) + _rng(5).normal(size=pt.cells(4).shape)
     corr_o, err_o = pt.cells(4), err.copy()
@@ -429,6 +435,40 @@ def test_integrator_fp64_matches_oracle(dev, gstype):
             e = _rel(a, b)
             assert e < 1e-10, (step, name, e, sto, stg)
         assert sto["nni"] == stg["nni"] and sto["nli"] == stg["nli"], (sto, stg)
+
+
+def test_config1_two_grain_20_steps_matches_oracle(dev):
+    """The plain-method case run in full: 2-D 128^2 periodic, two grains
+    (radius 32, tanh width 4), quadratic KKS with c = 0.40 / 0.45 plus noise,
+    isotropic phase flux, 20 BDF steps at dt = 1e-3, FP64 red-black block
+    solves with 2 sweeps.  The step is cut on convergence failures here, and the
+    GPU must take the same cuts: per step, every field within 1e-10 relative L2
+    of the oracle, identical Newton / Krylov / failure counters and time."""
+    from oracle import oracle as O
+    from ampe_amd import hip as H
+    n = 128
+    p = P.two_grain_params(ndim=2, flux_type=P.FLUX_ISOTROPIC)
+    p.quad_ceq[0], p.quad_ceq[1] = 0.40, 0.45
+    phi, c, q = P.two_grain_state(n=n)
+    s = P.solver_defaults(dt=1e-3, maxl=10, max_newton=5, atol=1e-6, rtol=1e-8,
+                          fac_sweeps=2, quat_sweeps=2)
+    osim = O.Sim(p, s, n)
+    osim.set_state(phi, c, q)
+    box = H.boxes_of((n, n), 2)
+    gsim = H.Sim(p, s, [box], periodic=box)
+    gsim.set_state(phi, c, q)
+    for step in range(20):
+        ro, sto = osim.advance(1)
+        rg, stg = gsim.advance(1)
+        assert ro == rg == 0, (step, sto, stg)
+        fo = osim.get_state()
+        fg = gsim.get_state((n, n))
+        for name, a, b in zip(("phi", "c", "q"), fg[:3], fo[:3]):
+            e = _rel(a, b)
+            assert e < 1e-10, (step, name, e, sto, stg)
+        for k in ("nst", "nni", "nli", "ncfn", "nlcf"):
+            assert sto[k] == stg[k], (step, k, sto, stg)
+        assert sto["t"] == stg["t"], (step, sto["t"], stg["t"])
 
 
 @pytest.mark.parametrize("model,gstype", [("calphad", P.GS_MODIFIED),
