@@ -33,6 +33,31 @@ __device__ __forceinline__ bool is_color(const CellPos& c, int color)
    return ((c.i - io) & 1) == 0 && c.i >= io;
 }
 
+// Colour-compact mapping: a workgroup of TX/2 x TY threads covers one tile,
+// each thread the one cell of the colour in its pair (2 tx, 2 tx + 1) of the
+// row, so every lane updates a cell (the full-tile mapping leaves half of
+// them idle).  Plane t.k + kk; false when outside the box.
+template <int NDIM>
+__device__ __forceinline__ bool cell_pos_color(const LevelArgs& a, int kk, int color,
+                                               CellPos& c)
+{
+   const Tile t = a.T[tile_index(a)];
+   c.pd = a.P[t.patch];
+   c.j = t.j0 + (int)threadIdx.y;
+   c.k = t.k + kk;
+   int gj = c.pd.lo[1] + c.j;
+   int x = (NDIM == 3) ? c.pd.lo[0] + gj + (c.pd.lo[2] + c.k) : c.pd.lo[1] + gj;
+   int io = ((x - (x / 2) * 2) != color) ? 1 : 0;
+   c.i = t.i0 + 2 * (int)threadIdx.x + io;  // t.i0 is a multiple of TX (even)
+   if (c.i >= c.pd.nx || c.j >= c.pd.ny || c.k >= c.pd.nz) return false;
+   c.st[0] = 1;
+   c.st[1] = c.pd.sy;
+   c.st[2] = c.pd.sz;
+   c.c0 = c.pd.cell_off + (c.i + a.g) + (long long)(c.j + a.g) * c.pd.sy +
+          (long long)(c.k + a.gz) * c.pd.sz;
+   return true;
+}
+
 template <typename T>
 struct OpArgs {
    const T* D;
@@ -111,23 +136,43 @@ __device__ __forceinline__ void block_store_sum(double v, double* partial)
 // One colour of efo_rbgswithfluxmax{var,con}dc{var,con}sf
 // (source/fortran/2d/ellipticfacops.m4:60-345): u += -res * dudr.
 template <typename T, int NDIM>
-__global__ void __launch_bounds__(256) k_rbgs(LevelArgs a, OpArgs<T> op, T di0, T di1, T di2,
-                                              const T* __restrict__ rhs, T* __restrict__ u,
-                                              int color, double* __restrict__ partial)
+__device__ __forceinline__ T rbgs_cell(const LevelArgs& a, const CellPos& c, const OpArgs<T>& op,
+                                       const T* dinv, const T* __restrict__ rhs, T* __restrict__ u)
 {
-   T absres = 0;
-   AMPE_COLUMN_BEGIN(a, c)
-   if (!is_color<NDIM>(c, color)) continue;
-   T dinv[3] = {di0, di1, di2};
    T diag;
    T res = op_residual<T, NDIM>(a, c, op, rhs, u, dinv, &diag);
    T dudr = T(1) / diag;
    T du = -res * dudr;
    u[c.c0] = u[c.c0] + du * T(1);
-   T ar = res < 0 ? -res : res;
+   return res < 0 ? -res : res;
+}
+
+template <typename T, int NDIM>
+__global__ void __launch_bounds__(256) k_rbgs(LevelArgs a, OpArgs<T> op, T di0, T di1, T di2,
+                                              const T* __restrict__ rhs, T* __restrict__ u,
+                                              int color, double* __restrict__ partial)
+{
+   T absres = 0;
+   const T dinv[3] = {di0, di1, di2};
+   AMPE_COLUMN_BEGIN(a, c)
+   if (!is_color<NDIM>(c, color)) continue;
+   T ar = rbgs_cell<T, NDIM>(a, c, op, dinv, rhs, u);
    absres = ar > absres ? ar : absres;
    AMPE_COLUMN_END
    if (partial) block_store_max<T>(absres, partial);
+}
+
+// The same on the colour-compact mapping (no max-residual partials).
+template <typename T, int NDIM>
+__global__ void __launch_bounds__(128) k_rbgs_c(LevelArgs a, OpArgs<T> op, T di0, T di1, T di2,
+                                                const T* __restrict__ rhs, T* __restrict__ u,
+                                                int color)
+{
+   const T dinv[3] = {di0, di1, di2};
+   for (int kk = 0; kk < a.nz_tile; kk++) {
+      CellPos c;
+      if (cell_pos_color<NDIM>(a, kk, color, c)) (void)rbgs_cell<T, NDIM>(a, c, op, dinv, rhs, u);
+   }
 }
 
 // efo_compres*sca (source/fortran/2d/ellipticfacops.m4:346-446) + sum res^2
@@ -183,14 +228,11 @@ __device__ __forceinline__ T quat_row(const LevelArgs& a, const CellPos& c, cons
 }
 
 template <typename T, int NDIM>
-__global__ void __launch_bounds__(256) k_quat_rbgs(LevelArgs a, const T* __restrict__ fc,
-                                                   const T* __restrict__ sm, T gamma, T hi0, T hi1,
-                                                   T hi2, int ncomp, const T* __restrict__ rhs,
-                                                   T* __restrict__ z, int color)
+__device__ __forceinline__ void quat_rbgs_cell(const LevelArgs& a, const CellPos& c,
+                                               const T* __restrict__ fc, const T* __restrict__ sm,
+                                               T gamma, const T* hinv2, int ncomp,
+                                               const T* __restrict__ rhs, T* __restrict__ z)
 {
-   AMPE_COLUMN_BEGIN(a, c)
-   if (!is_color<NDIM>(c, color)) continue;
-   T hinv2[3] = {hi0, hi1, hi2};
    for (int m = 0; m < ncomp; m++) {
       const T* rm = rhs + m * a.ncg;
       T* zm = z + m * a.ncg;
@@ -199,7 +241,35 @@ __global__ void __launch_bounds__(256) k_quat_rbgs(LevelArgs a, const T* __restr
       T res = rm[c.c0] - az;
       zm[c.c0] = zm[c.c0] + res / center;
    }
+}
+
+template <typename T, int NDIM>
+__global__ void __launch_bounds__(256) k_quat_rbgs(LevelArgs a, const T* __restrict__ fc,
+                                                   const T* __restrict__ sm, T gamma, T hi0, T hi1,
+                                                   T hi2, int ncomp, const T* __restrict__ rhs,
+                                                   T* __restrict__ z, int color)
+{
+   const T hinv2[3] = {hi0, hi1, hi2};
+   AMPE_COLUMN_BEGIN(a, c)
+   if (!is_color<NDIM>(c, color)) continue;
+   quat_rbgs_cell<T, NDIM>(a, c, fc, sm, gamma, hinv2, ncomp, rhs, z);
    AMPE_COLUMN_END
+}
+
+// The same on the colour-compact mapping.
+template <typename T, int NDIM>
+__global__ void __launch_bounds__(128) k_quat_rbgs_c(LevelArgs a, const T* __restrict__ fc,
+                                                     const T* __restrict__ sm, T gamma, T hi0,
+                                                     T hi1, T hi2, int ncomp,
+                                                     const T* __restrict__ rhs,
+                                                     T* __restrict__ z, int color)
+{
+   const T hinv2[3] = {hi0, hi1, hi2};
+   for (int kk = 0; kk < a.nz_tile; kk++) {
+      CellPos c;
+      if (cell_pos_color<NDIM>(a, kk, color, c))
+         quat_rbgs_cell<T, NDIM>(a, c, fc, sm, gamma, hinv2, ncomp, rhs, z);
+   }
 }
 
 template <typename T, int NDIM>
@@ -226,6 +296,7 @@ __global__ void __launch_bounds__(256) k_quat_residual(LevelArgs a, const T* __r
 
 // ------------------------------------------------------------ launchers
 static dim3 tblock() { return dim3(TX, TY); }
+static dim3 cblock() { return dim3(TX / 2, TY); }
 
 // 1/h and 1/h^2 in the operator's precision, computed once per launch on the
 // host (IEEE division: the device's value).
@@ -272,12 +343,20 @@ int launch_rbgs(const Level& L, const EllOp<T>& op, const double* dx, const T* r
    shape(L, want_max, &la, &grid);
    T di[3];
    recip(dx, di);
-   if (L.ndim == 3)
+   if (!want_max) {
+      if (L.ndim == 3)
+         hipLaunchKernelGGL((k_rbgs_c<T, 3>), dim3(grid), cblock(), 0, s, la, oa, di[0], di[1],
+                            di[2], rhs, u, color);
+      else
+         hipLaunchKernelGGL((k_rbgs_c<T, 2>), dim3(grid), cblock(), 0, s, la, oa, di[0], di[1],
+                            di[2], rhs, u, color);
+   } else if (L.ndim == 3) {
       hipLaunchKernelGGL((k_rbgs<T, 3>), dim3(grid), tblock(), 0, s, la, oa, di[0], di[1], di[2],
                          rhs, u, color, part);
-   else
+   } else {
       hipLaunchKernelGGL((k_rbgs<T, 2>), dim3(grid), tblock(), 0, s, la, oa, di[0], di[1], di[2],
                          rhs, u, color, part);
+   }
    AMPE_HIP_CHECK(hipGetLastError());
    return 0;
 }
@@ -313,10 +392,10 @@ int launch_quat_rbgs(const Level& L, const T* fc, const T* sm, double gamma, con
    recip2(dx, hi);
    KernelTimer::Span span(KernelTimer::QUAT_RBGS, s, quat_rbgs_bytes(L, ncomp, sizeof(T)));
    if (L.ndim == 3)
-      hipLaunchKernelGGL((k_quat_rbgs<T, 3>), dim3(grid), tblock(), 0, s, la, fc, sm, (T)gamma,
+      hipLaunchKernelGGL((k_quat_rbgs_c<T, 3>), dim3(grid), cblock(), 0, s, la, fc, sm, (T)gamma,
                          hi[0], hi[1], hi[2], ncomp, rhs, z, color);
    else
-      hipLaunchKernelGGL((k_quat_rbgs<T, 2>), dim3(grid), tblock(), 0, s, la, fc, sm, (T)gamma,
+      hipLaunchKernelGGL((k_quat_rbgs_c<T, 2>), dim3(grid), cblock(), 0, s, la, fc, sm, (T)gamma,
                          hi[0], hi[1], hi[2], ncomp, rhs, z, color);
    AMPE_HIP_CHECK(hipGetLastError());
    return 0;
