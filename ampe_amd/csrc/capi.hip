@@ -158,6 +158,14 @@ int ampe_quat_rhs_fc(const ampe_level* level, const ampe_model_params* p, const 
    return ampe::launch_quat_rhs_fc(*level->L, *p, phi, q, fc, rhs, hs(stream));
 }
 
+int ampe_grains_label(const ampe_level* level, const double* phase, double threshold,
+                      int* labels, int* ngrains, ampe_stream_t stream)
+{
+   AMPE_REQUIRE(level && phase && labels, "null argument");
+   AMPE_REQUIRE(threshold > 0.0, "the grain threshold must be positive");
+   return ampe::grains_label(*level->L, phase, threshold, labels, ngrains, nullptr, hs(stream));
+}
+
 int ampe_kernel_timer_enable(int kernel, int on)
 {
    return ampe::KernelTimer::enable(kernel, on != 0);

@@ -154,6 +154,10 @@ int launch_conc_rhs_ebs(const Level& L, const ampe_model_params& p, const double
                         hipStream_t s);
 int launch_phase_mobility(const Level& L, const ampe_model_params& p, const double* cl,
                           const double* cs, double* M, hipStream_t s);
+// Grain labels of the level's interior cells (interior-compact order, -1
+// outside grains); synchronises s.
+int grains_label(const Level& L, const double* phase, double thr, int* labels, int* ngrains,
+                 int* passes, hipStream_t s);
 int launch_tag(const Level& L, const ampe_model_params& p, const double* phi, double tol,
                int* tags, hipStream_t s);
 int launch_quat_face_coef(const Level& L, const ampe_model_params& p, const double* phi,

@@ -79,6 +79,7 @@ void level_destroy(Level* L);
 // Ghost gather for ncomp component blocks (double or float fields).
 int ghost_fill(const Level& L, double* f, int ncomp, hipStream_t s);
 int ghost_fill(const Level& L, float* f, int ncomp, hipStream_t s);
+int ghost_fill(const Level& L, int* f, int ncomp, hipStream_t s);
 
 // Fixed-order device reduction of the per-tile partials written by a cell
 // kernel: sum (op = 0) or max (op = 1) of nparts values (ncol columns

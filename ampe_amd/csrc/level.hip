@@ -282,6 +282,10 @@ int ghost_fill(const Level& L, float* f, int ncomp, hipStream_t s)
 {
    return ghost_fill_t(L, f, ncomp, s);
 }
+int ghost_fill(const Level& L, int* f, int ncomp, hipStream_t s)
+{
+   return ghost_fill_t(L, f, ncomp, s);
+}
 
 // Single-workgroup fixed-order reduction of nparts x ncol partials
 // (column-interleaved).  Each lane walks a fixed stride, then a fixed tree.

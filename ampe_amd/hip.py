@@ -45,6 +45,7 @@ _SIGS = {
                             C.POINTER(_i), _vp]),
     "ampe_quat_face_coefs": (_i, [_vp, C.POINTER(ModelParams), _vp, _vp, _vp, _vp]),
     "ampe_quat_rhs_fc": (_i, [_vp, C.POINTER(ModelParams), _vp, _vp, _vp, _vp, _vp]),
+    "ampe_grains_label": (_i, [_vp, _vp, C.c_double, _vp, C.POINTER(C.c_int), _vp]),
     "ampe_kernel_timer_enable": (_i, [C.c_int, C.c_int]),
     "ampe_kernel_timer_read": (_i, [C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_longlong),
                                     C.POINTER(C.c_double)]),
@@ -146,6 +147,48 @@ class Level:
         check(lib().ampe_level_patch_offsets(h, co, so), "ampe_level_patch_offsets")
         self.cell_off = list(co)
         self.side_off = [list(so[3 * p:3 * p + 3]) for p in range(np_)]
+
+    def _patch_ext(self, b):
+        n = [b.hi[a] - b.lo[a] + 1 for a in range(3)]
+        if self.ndim == 2:
+            n[2] = 1
+        return n
+
+    def to_ghosted(self, a, ghost=1):
+        """Domain array ([nz,] ny, nx), domain corner at the patches' lowest
+        corner, to the level's ghosted cell layout (ghost entries zero)."""
+        a = np.asarray(a, dtype=np.float64)
+        a3 = a if self.ndim == 3 else a[None]
+        org = [min(b.lo[d] for b in self.boxes) for d in range(3)]
+        out = np.zeros(self.ncg)
+        g, gz = ghost, (ghost if self.ndim == 3 else 0)
+        for p, b in enumerate(self.boxes):
+            n = self._patch_ext(b)
+            blk = np.zeros((n[2] + 2 * gz, n[1] + 2 * g, n[0] + 2 * g))
+            x0, y0, z0 = (b.lo[d] - org[d] for d in range(3))
+            blk[gz:gz + n[2], g:g + n[1], g:g + n[0]] = \
+                a3[z0:z0 + n[2], y0:y0 + n[1], x0:x0 + n[0]]
+            out[self.cell_off[p]:self.cell_off[p] + blk.size] = blk.ravel()
+        return out
+
+    def from_interior(self, v):
+        """Interior-compact per-patch array (tags, grain labels) to the domain
+        array ([nz,] ny, nx)."""
+        org = [min(b.lo[d] for b in self.boxes) for d in range(3)]
+        top = [max(b.hi[d] for b in self.boxes) for d in range(3)]
+        shape = [top[d] - org[d] + 1 for d in range(3)]
+        if self.ndim == 2:
+            shape[2] = 1
+        out = np.zeros((shape[2], shape[1], shape[0]), dtype=v.dtype)
+        off = 0
+        for b in self.boxes:
+            n = self._patch_ext(b)
+            x0, y0, z0 = (b.lo[d] - org[d] for d in range(3))
+            cnt = n[0] * n[1] * n[2]
+            out[z0:z0 + n[2], y0:y0 + n[1], x0:x0 + n[0]] = \
+                v[off:off + cnt].reshape(n[2], n[1], n[0])
+            off += cnt
+        return out if self.ndim == 3 else out[0]
 
     def __del__(self):
         if getattr(self, "h", None) is not None and _lib is not None:

@@ -305,6 +305,15 @@ enum ampe_timed_kernel { AMPE_TIMED_QUAT_RBGS = 0 };
 int ampe_kernel_timer_enable(int kernel, int on);
 int ampe_kernel_timer_read(int kernel, double* ms, long long* launches, double* bytes);
 
+/* Grain identification (Grains::findAndNumberGrains, source/Grains.cc
+ * :263-645): cells with phase >= threshold, 4-connected (2-D) or 6-connected
+ * (3-D) across patch and periodic boundaries, numbered 0, 1, ... in the order
+ * of each grain's smallest index in the level's bounding box (x fastest);
+ * labels: int, interior cells per patch as in ampe_tag_cells, -1 outside
+ * grains.  Synchronises the stream; *ngrains = number of grains. */
+int ampe_grains_label(const ampe_level* level, const double* phase, double threshold,
+                      int* labels, int* ngrains, ampe_stream_t stream);
+
 /* AMR gradient detector: tags = 1 where |grad phi * dx|^2 >= tol^2, else 0
  * (QuatModel::tagGradientDetectorCells source/QuatModel.cc:3476-3602,
  * F77 tagfromgrads quattag.m4:9).  tags: int, interior cells per patch. */

@@ -1030,6 +1030,90 @@ void o_conc_rhs_ebs(const OGeom* G, const ampe_model_params* p, const double* cl
    free(fl);
 }
 
+/* Grains::findAndNumberGrains (source/Grains.cc:263-645) on one periodic
+ * patch covering the domain (cell weights all positive): number the cells
+ * with phase >= thr by their box index (x fastest), -1 elsewhere and in the
+ * ghosts; then sweep the cells in box order, lowering each labelled cell to a
+ * face neighbour's label smaller than the label it had at the start of its
+ * visit (the last such neighbour in the order -x, +x, -y, +y, -z, +z wins, as
+ * in the reference), with a periodic ghost fill before every sweep, until a
+ * sweep changes nothing or the reference's iteration cap (twice twice the
+ * largest extent) is reached; finally renumber the surviving labels 0, 1, ...
+ * in increasing order.  labels: interior cells, x fastest.  Returns the
+ * number of grains; *sweeps = sweeps done. */
+int o_grains_label(const OGeom* G, const double* phase, double thr, int* labels, int* sweeps)
+{
+   long st[3] = {1, G->sy, G->sz};
+   int* g = (int*)malloc(sizeof(int) * G->ncg);
+   for (long c = 0; c < G->ncg; c++) g[c] = -1;
+   for (int k = 0; k < G->nz; k++)
+      for (int j = 0; j < G->ny; j++)
+         for (int i = 0; i < G->nx; i++) {
+            long c0 = CI(i, j, k);
+            if (phase[c0] >= thr) g[c0] = i + G->nx * (j + G->ny * k);
+         }
+   int width = G->nx > G->ny ? G->nx : G->ny;
+   if (G->ndim == 3 && G->nz > width) width = G->nz;
+   int max_count = 2 * (2 * width);
+   int counter = 0;
+   while (counter < max_count) {
+      int changed = 0;
+      for (int k = -G->gz; k < G->nz + G->gz; k++)
+         for (int j = -G->g; j < G->ny + G->g; j++)
+            for (int i = -G->g; i < G->nx + G->g; i++) {
+               if (i >= 0 && i < G->nx && j >= 0 && j < G->ny && k >= 0 && k < G->nz) continue;
+               g[CI(i, j, k)] = g[CI(wrap(i, G->nx), wrap(j, G->ny), wrap(k, G->nz))];
+            }
+      for (int k = 0; k < G->nz; k++)
+         for (int j = 0; j < G->ny; j++)
+            for (int i = 0; i < G->nx; i++) {
+               long c0 = CI(i, j, k);
+               const int n = g[c0];
+               if (n < 0) continue;
+               for (int d = 0; d < G->ndim; d++) {
+                  int nm = g[c0 - st[d]];
+                  if (nm >= 0 && nm < n) {
+                     g[c0] = nm;
+                     changed = 1;
+                  }
+                  int np = g[c0 + st[d]];
+                  if (np >= 0 && np < n) {
+                     g[c0] = np;
+                     changed = 1;
+                  }
+               }
+            }
+      counter++;
+      if (!changed) break;
+   }
+   if (sweeps) *sweeps = counter;
+   /* renumber: sorted distinct labels -> 0, 1, ... (labels are box indices) */
+   long nbox = (long)G->nx * G->ny * G->nz;
+   int* rank = (int*)malloc(sizeof(int) * nbox);
+   for (long b = 0; b < nbox; b++) rank[b] = 0;
+   for (int k = 0; k < G->nz; k++)
+      for (int j = 0; j < G->ny; j++)
+         for (int i = 0; i < G->nx; i++) {
+            int n = g[CI(i, j, k)];
+            if (n >= 0) rank[n] = 1;
+         }
+   int ng = 0;
+   for (long b = 0; b < nbox; b++) {
+      int f = rank[b];
+      rank[b] = ng;
+      ng += f;
+   }
+   for (int k = 0; k < G->nz; k++)
+      for (int j = 0; j < G->ny; j++)
+         for (int i = 0; i < G->nx; i++) {
+            int n = g[CI(i, j, k)];
+            labels[(long)i + (long)G->nx * ((long)j + (long)G->ny * k)] = n >= 0 ? rank[n] : -1;
+         }
+   free(rank);
+   free(g);
+   return ng;
+}
+
 /* tagfromgrads (2d/quattag.m4:9-55) on the cell gradient of phi
  * (QuatModel::computePhaseGradCell: diffs + cell average, quatgrad_cell
  * formula).  tags: interior cells, i fastest, no ghosts. */
