@@ -56,7 +56,8 @@ double o_d2fenergy(const ampe_model_params* p, double c, int phase);
 int o_kks_cell(const ampe_model_params* p, double c, double phi, double* cl,
                double* cs);
 
-int o_grains_label(const OGeom* G, const double* phase, double thr, int* labels, int* sweeps);
+int o_grains_label(const OGeom* G, const double* phase, double thr, int periodic, int* labels,
+                   int* sweeps);
 void o_ebs_coeffs(const OGeom* G, const ampe_model_params* p, const double* phi, double* DL,
                   double* DA);
 void o_conc_rhs_ebs(const OGeom* G, const ampe_model_params* p, const double* cl,

@@ -79,7 +79,7 @@ def _setup(L):
         "o_conc_coeffs": (None, [_G, _P, _dp, _dp, _dp, _dp, _dp]),
         "o_conc_rhs": (None, [_G, _P, _dp, _dp, _dp, _dp, _dp]),
         "o_kks_solve": (C.c_int, [_G, _P, _dp, _dp, _dp, _dp, _dp, _dp]),
-        "o_grains_label": (C.c_int, [_G, _dp, C.c_double, _ip, _ip]),
+        "o_grains_label": (C.c_int, [_G, _dp, C.c_double, C.c_int, _ip, _ip]),
         "o_ebs_coeffs": (None, [_G, _P, _dp, _dp, _dp]),
         "o_conc_rhs_ebs": (None, [_G, _P, _dp, _dp, _dp, _dp, _dp]),
         "o_tag_cells": (None, [_G, _P, _dp, C.c_double, _ip]),

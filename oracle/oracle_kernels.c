@@ -1030,18 +1030,20 @@ void o_conc_rhs_ebs(const OGeom* G, const ampe_model_params* p, const double* cl
    free(fl);
 }
 
-/* Grains::findAndNumberGrains (source/Grains.cc:263-645) on one periodic
- * patch covering the domain (cell weights all positive): number the cells
+/* Grains::findAndNumberGrains (source/Grains.cc:263-645) on one patch
+ * covering the domain (cell weights all positive): number the cells
  * with phase >= thr by their box index (x fastest), -1 elsewhere and in the
  * ghosts; then sweep the cells in box order, lowering each labelled cell to a
  * face neighbour's label smaller than the label it had at the start of its
  * visit (the last such neighbour in the order -x, +x, -y, +y, -z, +z wins, as
- * in the reference), with a periodic ghost fill before every sweep, until a
+ * in the reference), with a periodic ghost fill before every sweep (ghosts
+ * stay -1 on a non-periodic domain: no physical-boundary fill), until a
  * sweep changes nothing or the reference's iteration cap (twice twice the
  * largest extent) is reached; finally renumber the surviving labels 0, 1, ...
  * in increasing order.  labels: interior cells, x fastest.  Returns the
  * number of grains; *sweeps = sweeps done. */
-int o_grains_label(const OGeom* G, const double* phase, double thr, int* labels, int* sweeps)
+int o_grains_label(const OGeom* G, const double* phase, double thr, int periodic, int* labels,
+                   int* sweeps)
 {
    long st[3] = {1, G->sy, G->sz};
    int* g = (int*)malloc(sizeof(int) * G->ncg);
@@ -1058,7 +1060,7 @@ int o_grains_label(const OGeom* G, const double* phase, double thr, int* labels,
    int counter = 0;
    while (counter < max_count) {
       int changed = 0;
-      for (int k = -G->gz; k < G->nz + G->gz; k++)
+      for (int k = -G->gz; k < G->nz + G->gz && periodic; k++)
          for (int j = -G->g; j < G->ny + G->g; j++)
             for (int i = -G->g; i < G->nx + G->g; i++) {
                if (i >= 0 && i < G->nx && j >= 0 && j < G->ny && k >= 0 && k < G->nz) continue;

@@ -13,7 +13,7 @@ from ampe_amd import params as P
 pytestmark = pytest.mark.gpu
 
 
-def _oracle(phase, thr):
+def _oracle(phase, thr, periodic=True):
     from oracle import oracle as O
     ndim = phase.ndim
     shape = phase.shape if ndim == 3 else (1,) + phase.shape
@@ -21,16 +21,17 @@ def _oracle(phase, thr):
     a = pt.load(pt.cells(), phase.reshape(shape))
     lab = np.zeros(pt.N, dtype=np.int32)
     sw = C.c_int(0)
-    ng = O.lib().o_grains_label(C.byref(pt.G), O.ptr(a), thr, O.iptr(lab), C.byref(sw))
+    ng = O.lib().o_grains_label(C.byref(pt.G), O.ptr(a), thr, int(periodic), O.iptr(lab),
+                                C.byref(sw))
     return lab.reshape(phase.shape), ng
 
 
-def _gpu(phase, thr, boxes=None):
+def _gpu(phase, thr, boxes=None, periodic=True):
     import torch
     from ampe_amd import hip as H
     ndim = phase.ndim
     full = H.boxes_of(phase.shape, ndim)
-    lev = H.Level(ndim, boxes or [full], 1, periodic=full)
+    lev = H.Level(ndim, boxes or [full], 1, periodic=full if periodic else None)
     f = torch.from_numpy(lev.to_ghosted(phase)).to("cuda")
     lab = torch.full((lev.ncell,), -7, dtype=torch.int32, device="cuda")
     ng = C.c_int(-1)
@@ -52,6 +53,21 @@ def test_labels_bit_exact(dev, shape, seed, thr):
     lg, ng = _gpu(f, thr)
     assert ng == no and no > 3
     assert np.array_equal(lg, lo)
+
+
+@pytest.mark.parametrize("ndim", [2, 3])
+def test_non_periodic_domain(dev, ndim):
+    """Physical boundaries (no ghost fill from the other side), as in the
+    reference's ComputeNumberOfGrains deck (periodic_dimension = 0, 0)."""
+    shape = (30, 40) if ndim == 2 else (20, 24, 28)
+    f = _field(shape, 21)
+    thr = 0.6 if ndim == 2 else 0.66
+    lo, no = _oracle(f, thr, periodic=False)
+    lg, ng = _gpu(f, thr, periodic=False)
+    lp, npp = _oracle(f, thr, periodic=True)
+    assert ng == no and no > 2
+    assert np.array_equal(lg, lo)
+    assert npp <= no
 
 
 def test_polycrystal_state(dev):

@@ -13,7 +13,7 @@ from scipy import ndimage
 from oracle import oracle as O
 
 
-def _label(phase, thr=0.5):
+def _label(phase, thr=0.5, periodic=True):
     """phase: (nz, ny, nx) or (ny, nx); returns (labels, ngrains, sweeps)."""
     ndim = phase.ndim
     shape = phase.shape if ndim == 3 else (1,) + phase.shape
@@ -21,7 +21,8 @@ def _label(phase, thr=0.5):
     a = pt.load(pt.cells(), phase.reshape(shape))
     lab = np.zeros(pt.N, dtype=np.int32)
     sw = C.c_int(0)
-    ng = O.lib().o_grains_label(C.byref(pt.G), O.ptr(a), thr, O.iptr(lab), C.byref(sw))
+    ng = O.lib().o_grains_label(C.byref(pt.G), O.ptr(a), thr, int(periodic), O.iptr(lab),
+                                C.byref(sw))
     return lab.reshape(shape if ndim == 3 else shape[1:]), ng, sw.value
 
 
@@ -76,6 +77,8 @@ def test_periodic_wrap_joins():
     f[4:7, 10:12] = 1.0
     lab, ng, _ = _label(f)
     assert ng == 1
+    _, ng_np, _ = _label(f, periodic=False)  # without the wrap: two grains
+    assert ng_np == 2
 
 
 def test_corner_contact_is_not_connected():
