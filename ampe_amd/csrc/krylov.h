@@ -8,6 +8,24 @@ namespace ampe {
 
 constexpr int kMaxKrylov = 32;
 
+// FP32 preconditioner right-hand side written next to a Krylov vector r:
+// f = r, orientation components [iq, iq + ql) divided by sqrt(M) = sm
+// (f == nullptr: none).
+struct PrecIn {
+   float* f = nullptr;
+   const double* sm = nullptr;
+   int iq = 0, ql = 0;
+};
+
+// A Krylov vector read from the FP32 preconditioner solution u: v = ewt u,
+// orientation components multiplied by sm first (u == nullptr: read v).
+struct PrecOut {
+   const float* u = nullptr;
+   const double* sm = nullptr;
+   const double* ewt = nullptr;
+   int iq = 0, ql = 0;
+};
+
 struct VecList {
    int n;
    const double* v[kMaxKrylov];
@@ -24,7 +42,7 @@ int kry_mid_scaled(const Level& L, int nc, double sig, const double* V, double h
                    const double* ewt, const double* ycur, double* work2, hipStream_t s);
 int kry_finish_scaled(const Level& L, int nc, double siginv, double gamma, const double* V,
                       double h, const double* ewt, const double* jv, const double* fcur,
-                      double* out, hipStream_t s);
+                      double* out, const PrecIn& pin, hipStream_t s);
 int kry_finish(const Level& L, int nc, double siginv, double gamma, const double* xk,
                const double* jv, const double* fcur, double* out, hipStream_t s);
 int kry_scale_dot(const Level& L, int nc, const double* ewt, const double* work, double* out,
@@ -41,6 +59,9 @@ constexpr int kMaxCgs = 16;
 // (k < vl.n) and sums[vl.n] = out . out, one pass and one synchronisation.
 int kry_cgs_dots(const Level& L, int nc, const double* ewt, const double* work, double* out,
                  const VecList& vl, double* sums, hipStream_t s);
+// The same with out = ewt u formed from the FP32 preconditioner solution.
+int kry_cgs_dots_from(const Level& L, int nc, const PrecOut& src, double* out,
+                      const VecList& vl, double* sums, hipStream_t s);
 // w = w + sum_k vl.y[k] vl.v[k] (k in order); *sumsq = w . w.
 int kry_cgs_update(const Level& L, int nc, double* w, const VecList& vl, double* sumsq,
                    hipStream_t s);

@@ -70,19 +70,25 @@ __global__ void __launch_bounds__(256) k_gmres_mid(LevelArgs a, int nc, double s
    }
 }
 
-// out = x + (-gamma) (siginv jv + (-siginv) fcur)
+// out = x + (-gamma) (siginv jv + (-siginv) fcur); with pin.f also the FP32
+// preconditioner right-hand side of out (the arithmetic of k_psolve_in)
 __global__ void __launch_bounds__(256) k_gmres_finish(LevelArgs a, int nc, double siginv,
                                                       double gamma, KryDir x,
                                                       const double* __restrict__ jv,
                                                       const double* __restrict__ fcur,
-                                                      double* __restrict__ out)
+                                                      double* __restrict__ out, PrecIn pin)
 {
    CellPos c;
    if (!cell_pos(a, 0, c)) return;
    for (int m = 0; m < nc; m++) {
       long long i = m * a.ncg + c.c0;
       double jvn = siginv * jv[i] + (-siginv) * fcur[i];
-      out[i] = kry_dir(x, i) + (-gamma) * jvn;
+      double v = kry_dir(x, i) + (-gamma) * jvn;
+      out[i] = v;
+      if (pin.f) {
+         bool q = m >= pin.iq && m < pin.iq + pin.ql;
+         pin.f[i] = q ? (float)(v / pin.sm[c.c0]) : (float)v;
+      }
    }
 }
 
@@ -196,7 +202,8 @@ __device__ __forceinline__ void tile_store_sums(const double* acc, int n, int nt
 __global__ void __launch_bounds__(256) k_cgs_dots(LevelArgs a, int nc, int ntiles,
                                                   const double* __restrict__ ewt,
                                                   const double* __restrict__ work, double* out,
-                                                  VecList vl, double* __restrict__ partial)
+                                                  PrecOut src, VecList vl,
+                                                  double* __restrict__ partial)
 {
    CellPos c;
    double acc[kMaxCgs];
@@ -207,7 +214,13 @@ __global__ void __launch_bounds__(256) k_cgs_dots(LevelArgs a, int nc, int ntile
       for (int m = 0; m < nc; m++) {
          long long i = m * a.ncg + c.c0;
          double v;
-         if (ewt) {
+         if (src.u) {
+            // k_psolve_out's arithmetic, then the ewt scaling
+            v = (double)src.u[i];
+            if (m >= src.iq && m < src.iq + src.ql) v = v * src.sm[c.c0];
+            v = src.ewt[i] * v;
+            out[i] = v;
+         } else if (ewt) {
             v = ewt[i] * work[i];
             out[i] = v;
          } else {
@@ -276,16 +289,16 @@ int kry_finish(const Level& L, int nc, double siginv, double gamma, const double
                const double* jv, const double* fcur, double* out, hipStream_t s)
 {
    AMPE_TILE_LAUNCH(k_gmres_finish, nc, siginv, gamma, KryDir{xk, nullptr, 0.0, nullptr}, jv,
-                    fcur, out);
+                    fcur, out, PrecIn{});
    return 0;
 }
 
 int kry_finish_scaled(const Level& L, int nc, double siginv, double gamma, const double* V,
                       double h, const double* ewt, const double* jv, const double* fcur,
-                      double* out, hipStream_t s)
+                      double* out, const PrecIn& pin, hipStream_t s)
 {
    AMPE_TILE_LAUNCH(k_gmres_finish, nc, siginv, gamma, KryDir{nullptr, V, h, ewt}, jv, fcur,
-                    out);
+                    out, pin);
    return 0;
 }
 
@@ -324,7 +337,18 @@ int kry_cgs_dots(const Level& L, int nc, const double* ewt, const double* work, 
       set_error("kry_cgs_dots: too many vectors");
       return -1;
    }
-   AMPE_TILE_LAUNCH(k_cgs_dots, nc, L.ncell_tiles, ewt, work, out, vl, L.partial);
+   AMPE_TILE_LAUNCH(k_cgs_dots, nc, L.ncell_tiles, ewt, work, out, PrecOut{}, vl, L.partial);
+   return reduce_regions(L, L.ncell_tiles, vl.n + 1, sums, s);
+}
+
+int kry_cgs_dots_from(const Level& L, int nc, const PrecOut& src, double* out,
+                      const VecList& vl, double* sums, hipStream_t s)
+{
+   if (vl.n + 1 > kMaxCgs) {
+      set_error("kry_cgs_dots: too many vectors");
+      return -1;
+   }
+   AMPE_TILE_LAUNCH(k_cgs_dots, nc, L.ncell_tiles, nullptr, nullptr, out, src, vl, L.partial);
    return reduce_regions(L, L.ncell_tiles, vl.n + 1, sums, s);
 }
 

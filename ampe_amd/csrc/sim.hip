@@ -564,10 +564,14 @@ static std::vector<PBlock<T>> make_blocks(Sim* S, double dphi, const T* Cphi, co
    return bl;
 }
 
-// z = P^-1 r; with ewt_out (FP32 preconditioner only) z = ewt_out (P^-1 r),
-// and z may alias r.
-static int psolve(Sim* S, const double* r, double* z, double delta,
-                  const double* ewt_out = nullptr)
+// z = P^-1 r.  FP32 preconditioner only: f_ready = the FP32 right-hand side
+// S->f_f was already formed from r (by the Krylov kernel that produced r);
+// keep = the solution is left in S->f_u for the caller (z untouched).
+struct PsolveOpts {
+   bool f_ready = false;
+   bool keep = false;
+};
+static int psolve(Sim* S, const double* r, double* z, double delta, PsolveOpts o = {})
 {
    const ampe_model_params& p = S->p;
    hipStream_t st = S->stream;
@@ -578,8 +582,12 @@ static int psolve(Sim* S, const double* r, double* z, double delta,
    int iq = S->iq >= 0 ? S->iq : S->nc;
    if (S->s.precision == AMPE_FAC_FP32) {
       // f = r (orientation blocks / sqrt(M), FP64 division) in FP32, u = 0
-      if (psolve_in<float>(ng, S->nc, iq, ql, r, S->sqrtm, S->f_f, nullptr, S->f_u, st))
+      if (o.f_ready) {
+         AMPE_HIP_CHECK(hipMemsetAsync(S->f_u, 0, sizeof(float) * S->nv, st));
+      } else if (psolve_in<float>(ng, S->nc, iq, ql, r, S->sqrtm, S->f_f, nullptr, S->f_u,
+                                  st)) {
          return -1;
+      }
       auto bl = make_blocks<float>(S, dphi, S->f_Cphi, S->f_Mphi, S->f_Dconc, S->f_fc,
                                    S->f_sqrtm);
       for (auto& B : bl) {
@@ -591,11 +599,12 @@ static int psolve(Sim* S, const double* r, double* z, double delta,
       }
       int ret = run_blocks<float>(S, bl, delta);
       if (ret < 0) return -1;
-      if (psolve_out<float>(ng, S->nc, iq, ql, S->f_u, S->sqrtm, ewt_out, z, st)) return -1;
+      if (!o.keep && psolve_out<float>(ng, S->nc, iq, ql, S->f_u, S->sqrtm, nullptr, z, st))
+         return -1;
       return ret;
    }
-   if (ewt_out) {
-      set_error("psolve: the scaled output is for the FP32 preconditioner");
+   if (o.f_ready || o.keep) {
+      set_error("psolve: fused input/output is for the FP32 preconditioner");
       return -1;
    }
    // FP64: z = 0, fac_u = orientation blocks of r / sqrt(M)
@@ -646,9 +655,23 @@ static int spgmr(Sim* S, double gamma, const double* bvec, double* x, double del
    // V[i] hs[i].  The difference-quotient increment then needs no norm pass:
    // the normalised vector has sum(v^2) cv = 1, so its WRMS norm over ewt is
    // sqrt(wfac / cv) and sigma = sqrt(cv / wfac).  With the FP32
-   // preconditioner, its output is scaled by ewt in the conversion kernel.
+   // preconditioner, the kernel that forms A v also writes the FP32
+   // preconditioner input, and the dot kernel reads the FP32 solution.
    const bool cgs = S->s.gstype == AMPE_GS_CLASSICAL;
    const bool fuse_out = cgs && S->s.precision == AMPE_FAC_FP32;
+   PrecIn pin;
+   PrecOut pout;
+   if (fuse_out) {
+      pin.f = S->f_f;
+      pin.sm = S->sqrtm;
+      pin.iq = S->iq >= 0 ? S->iq : nc;
+      pin.ql = S->iq >= 0 ? S->p.qlen : 0;
+      pout.u = S->f_u;
+      pout.sm = S->sqrtm;
+      pout.ewt = S->ewt;
+      pout.iq = pin.iq;
+      pout.ql = pin.ql;
+   }
    double hs[kMaxKrylov + 1];
    if (psolve(S, bvec, S->work, delta) < 0) return -1;
    double rho;
@@ -674,7 +697,7 @@ static int spgmr(Sim* S, double gamma, const double* bvec, double* x, double del
          if (rhs_eval(S, S->work2, S->jv, true)) return -1;
          double siginv = 1.0 / sig;
          if (kry_finish_scaled(L, nc, siginv, gamma, S->V[l], hs[l], S->ewt, S->jv, S->fcur,
-                               S->V[l + 1], st))
+                               S->V[l + 1], pin, st))
             return -1;
       } else {
          double ss;
@@ -688,8 +711,11 @@ static int spgmr(Sim* S, double gamma, const double* bvec, double* x, double del
             return -1;
       }
       if (fuse_out) {
-         // V[l+1] = ewt (P^-1 V[l+1]) in place
-         if (psolve(S, S->V[l + 1], S->V[l + 1], delta, S->ewt) < 0) return -1;
+         // input formed by the finish kernel, solution left in f_u
+         PsolveOpts o;
+         o.f_ready = true;
+         o.keep = true;
+         if (psolve(S, S->V[l + 1], nullptr, delta, o) < 0) return -1;
       } else if (psolve(S, S->V[l + 1], S->work, delta) < 0) {
          return -1;
       }
@@ -703,7 +729,8 @@ static int spgmr(Sim* S, double gamma, const double* bvec, double* x, double del
          for (int i = 0; i <= l; i++) vl.v[i] = S->V[i];
          double sums[kMaxCgs];
          if (fuse_out) {
-            if (kry_cgs_dots(L, nc, nullptr, nullptr, S->V[l + 1], vl, sums, st)) return -1;
+            // V[l+1] = ewt P^-1 (...) from the FP32 solution, with its dots
+            if (kry_cgs_dots_from(L, nc, pout, S->V[l + 1], vl, sums, st)) return -1;
          } else if (kry_cgs_dots(L, nc, S->ewt, S->work, S->V[l + 1], vl, sums, st)) {
             return -1;
          }
