@@ -197,46 +197,82 @@ def surrogate_report(p, dev):
     return s, rep
 
 
-def kernel_roofline(p, state, n, reps=20):
-    """Average duration of the fused phase-RHS kernel on the workload's level,
-    timed with events on the stream it is launched on."""
+def stencil_rooflines(p, state, n, reps=20):
+    """Average launch duration of each right-hand-side stencil kernel on the
+    workload's level and state (events on the launch stream), with its
+    algorithmic bytes per cell (each array read once, each output written
+    once, FP64, 3-D)."""
     import torch
     from ampe_amd import hip as H
     dev = torch.device("cuda:0")
     box = H.boxes_of((n, n, n), 3)
     lev = H.Level(3, [box], 1, periodic=box)
-    g = n + 2
-
-    def ghosted(a, ncomp=1):
-        out = np.zeros((ncomp, g, g, g))
-        out[:, 1:-1, 1:-1, 1:-1] = np.asarray(a).reshape(ncomp, n, n, n)
-        return torch.from_numpy(out).to(dev)
-
     phi, c, q = state
-    tphi, tq = ghosted(phi), ghosted(q, 4)
-    tcl, tcs = ghosted(c), ghosted(c)
+    tphi = torch.from_numpy(lev.to_ghosted(phi)).to(dev)
+    tc = torch.from_numpy(lev.to_ghosted(c)).to(dev)
+    tq = torch.from_numpy(np.concatenate([lev.to_ghosted(q[m]) for m in range(4)])).to(dev)
+    tcl, tcs = tc.clone(), tc.clone()
     rhs = torch.zeros_like(tphi)
+    rq = torch.zeros_like(tq)
+    side = lambda: torch.zeros(lev.nside_tot, dtype=torch.float64, device=dev)  # noqa: E731
+    n2, fc, Dc, Dp = side(), side(), side(), side()
     stream = torch.cuda.current_stream()
     sp = C.c_void_p(stream.cuda_stream)
     ptr = lambda t: C.c_void_p(t.data_ptr())  # noqa: E731
-    H.check(H.lib().ampe_ghost_fill(lev.h, ptr(tphi), 1, sp))
-    H.check(H.lib().ampe_ghost_fill(lev.h, ptr(tq), 4, sp))
-    for _ in range(3):
-        H.check(H.lib().ampe_phase_rhs(lev.h, C.byref(p), ptr(tphi), ptr(tq), ptr(tcl),
-                                       ptr(tcs), ptr(rhs), sp))
+    L, pp = H.lib(), C.byref(p)
+    for t, k in ((tphi, 1), (tq, 4), (tc, 1), (tcl, 1), (tcs, 1)):
+        H.check(L.ampe_ghost_fill(lev.h, ptr(t), k, sp))
+    H.check(L.ampe_quat_grad(lev.h, pp, ptr(tq), None, ptr(n2), sp))
+    H.check(L.ampe_conc_coeffs(lev.h, pp, ptr(tphi), ptr(tcl), ptr(tcs), ptr(Dc), ptr(Dp), sp))
+    calls = [
+        ("k_phase_rhs<3>", 8 * 7 + 8,
+         lambda: L.ampe_phase_rhs(lev.h, pp, ptr(tphi), ptr(tq), ptr(tcl), ptr(tcs), ptr(rhs),
+                                  sp)),
+        ("k_quat_face_norm2<3>", 8 * 4 + 8 * 3,
+         lambda: L.ampe_quat_grad(lev.h, pp, ptr(tq), None, ptr(n2), sp)),
+        ("k_quat_face_coef<3>", 8 + 8 * 3 + 8 * 3,
+         lambda: L.ampe_quat_face_coefs(lev.h, pp, ptr(tphi), ptr(n2), ptr(fc), sp)),
+        ("k_quat_rhs<3,FC>", 8 + 8 * 4 + 8 * 3 + 8 * 4,
+         lambda: L.ampe_quat_rhs_fc(lev.h, pp, ptr(tphi), ptr(tq), ptr(fc), ptr(rq), sp)),
+        ("k_conc_rhs<3>", 8 * 2 + 8 * 6 + 8,
+         lambda: L.ampe_conc_rhs(lev.h, pp, ptr(tc), ptr(tphi), ptr(Dc), ptr(Dp), ptr(rhs), sp)),
+        ("k_conc_rhs_ebs<3>", 8 * 2 + 8 * 6 + 8,
+         lambda: L.ampe_conc_rhs_ebs(lev.h, pp, ptr(tcl), ptr(tcs), ptr(Dc), ptr(Dp), ptr(rhs),
+                                     sp)),
+    ]
+    out = {}
+    for name, bpc, f in calls:
+        for _ in range(3):
+            H.check(f())
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(reps):
+            H.check(f())
+        e1.record(stream)
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / reps
+        gbs = bpc * n ** 3 / (ms * 1e-3) / 1e9
+        out[name] = dict(ms=ms, bytes_per_cell=bpc, achieved=gbs, frac=gbs / HBM_PEAK_GBS)
+    return out
+
+
+def copy_bandwidth(nbytes=1 << 30, reps=10):
+    """Device-to-device copy rate (read + write bytes / time): the practical
+    streaming ceiling the kernels are compared with besides the nominal peak."""
+    import torch
+    a = torch.ones(nbytes // 8, dtype=torch.float64, device="cuda")
+    b = torch.empty_like(a)
+    b.copy_(a)
+    s = torch.cuda.current_stream()
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    e0.record(stream)
+    e0.record(s)
     for _ in range(reps):
-        H.check(H.lib().ampe_phase_rhs(lev.h, C.byref(p), ptr(tphi), ptr(tq), ptr(tcl),
-                                       ptr(tcs), ptr(rhs), sp))
-    e1.record(stream)
+        b.copy_(a)
+    e1.record(s)
     torch.cuda.synchronize()
     ms = e0.elapsed_time(e1) / reps
-    # algorithmic bytes per cell: read phi, q(4), cL, cS; write rhs (FP64)
-    bytes_per_cell = 8 * (1 + 4 + 2) + 8
-    achieved = bytes_per_cell * n ** 3 / (ms * 1e-3) / 1e9
-    return dict(kernel="k_phase_rhs<3>", ms=ms, bytes_per_cell=bytes_per_cell,
-                achieved=achieved)
+    del a, b
+    return 2.0 * nbytes / (ms * 1e-3) / 1e9
 
 
 def grains_report(phi, n, thr=0.5, reps=3):
@@ -352,10 +388,13 @@ def main():
                      avg_ms=kt["ms"] / kt["launches"],
                      bytes_per_launch=kt["bytes"] / kt["launches"], achieved=quat_gbs)
     log(f"[bench] roofline {roof_quat}")
-    roof = kernel_roofline(p, state, n)
-    log(f"[bench] phase rhs {roof}")
+    stencils = stencil_rooflines(p, state, n)
+    for k, v in stencils.items():
+        log(f"[bench] {k}: {v['ms']:.3f} ms {v['achieved']:.0f} GB/s ({100 * v['frac']:.0f} %)")
     grains = grains_report(H["fields"][0], n)
     log(f"[bench] grains {grains}")
+    copy_gbs = copy_bandwidth()
+    log(f"[bench] device copy {copy_gbs:.0f} GB/s")
     cpu = None
     if not args.no_cpu:
         log("[bench] cpu baseline")
@@ -384,12 +423,9 @@ def main():
         "surrogate": srep,
         "roofline": {"bound": "hbm", "achieved": quat_gbs, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": quat_gbs / HBM_PEAK_GBS,
-                     "traffic": QUAT_RBGS_TRAFFIC, **roof_quat},
-        "phase_rhs_roofline": {"bound": "hbm", "achieved": roof["achieved"],
-                               "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                               "frac": roof["achieved"] / HBM_PEAK_GBS,
-                               "kernel": roof["kernel"], "kernel_ms": roof["ms"],
-                               "bytes_per_cell": roof["bytes_per_cell"]},
+                     "traffic": QUAT_RBGS_TRAFFIC, "copy_gbs": copy_gbs,
+                     "frac_of_copy": quat_gbs / copy_gbs, **roof_quat},
+        "stencil_rooflines": stencils,
         "cpu_baseline": cpu,
         "grains": grains,
     }
