@@ -166,17 +166,22 @@ __global__ void __launch_bounds__(256) k_quat_modulus(LevelArgs a, ampe_model_pa
 
 // |grad q|^2 on faces from the full side gradient (quatgrad_side
 // 2d/quatgrad.m4:157-229, summed as compute_face_coef2d quatfacops.m4:83-92).
-template <int NDIM>
+// QL > 0: the component count as a compile-time constant (loops unrolled);
+// QL = 0 reads p.qlen.
+template <int NDIM, int QL = 0>
 __device__ __forceinline__ double quat_face_norm2(const ampe_model_params& p, const RhsConst& k,
                                                   const double* __restrict__ q, long long ncg,
                                                   const long long* st, int d, long long cm,
                                                   long long c0)
 {
    double s = 0.0;
+   const int ql = QL > 0 ? QL : p.qlen;
+#pragma unroll
    for (int n = 0; n < NDIM; n++) {
       double dinv = k.dxinv[n];
       double p25 = 0.25 * dinv;
-      for (int m = 0; m < p.qlen; m++) {
+#pragma unroll
+      for (int m = 0; m < ql; m++) {
          const double* aa = q + m * ncg;
          double gv;
          if (n == d) {
@@ -192,7 +197,7 @@ __device__ __forceinline__ double quat_face_norm2(const ampe_model_params& p, co
    return s;
 }
 
-template <int NDIM>
+template <int NDIM, int QL>
 __global__ void __launch_bounds__(256) k_quat_face_norm2(LevelArgs a, ampe_model_params p,
                                                          RhsConst k, const double* __restrict__ q,
                                                          double* __restrict__ n2)
@@ -205,7 +210,7 @@ __global__ void __launch_bounds__(256) k_quat_face_norm2(LevelArgs a, ampe_model
                 (NDIM == 2 || c.k < c.pd.nz + (d == 2));
       if (!ok) continue;
       n2[face_idx(a, c.pd, d, c.i, c.j, c.k)] =
-          quat_face_norm2<NDIM>(p, k, q, a.ncg, c.st, d, c.c0 - c.st[d], c.c0);
+          quat_face_norm2<NDIM, QL>(p, k, q, a.ncg, c.st, d, c.c0 - c.st[d], c.c0);
    }
 }
 
@@ -578,8 +583,20 @@ int launch_quat_grad(const Level& L, const ampe_model_params& p, const double* q
                      double* gmod, double* face_norm2, hipStream_t s)
 {
    if (gmod) AMPE_DISPATCH(L, k_quat_modulus, L.ncell_tiles, level_args(L), p, rhs_const(p), q, gmod);
-   if (face_norm2)
-      AMPE_DISPATCH(L, k_quat_face_norm2, L.nface_tiles, level_args(L, true), p, rhs_const(p), q, face_norm2);
+   if (face_norm2) {
+      dim3 g(L.nface_tiles), b = tile_block();
+      LevelArgs a = level_args(L, true);
+      RhsConst k = rhs_const(p);
+      if (L.ndim == 3 && p.qlen == 4)
+         hipLaunchKernelGGL((k_quat_face_norm2<3, 4>), g, b, 0, s, a, p, k, q, face_norm2);
+      else if (L.ndim == 3)
+         hipLaunchKernelGGL((k_quat_face_norm2<3, 0>), g, b, 0, s, a, p, k, q, face_norm2);
+      else if (p.qlen == 4)
+         hipLaunchKernelGGL((k_quat_face_norm2<2, 4>), g, b, 0, s, a, p, k, q, face_norm2);
+      else
+         hipLaunchKernelGGL((k_quat_face_norm2<2, 0>), g, b, 0, s, a, p, k, q, face_norm2);
+      AMPE_HIP_CHECK(hipGetLastError());
+   }
    return 0;
 }
 
