@@ -282,6 +282,45 @@ int ampe_fac_smooth(const ampe_level* level, const ampe_elliptic_op* op, const d
    return 0;
 }
 
+int ampe_quat_smooth(const ampe_level* level, const double* fc, const double* sm, double gamma,
+                     const double dx[3], int ncomp, const double* rhs, double* z,
+                     int num_sweeps, int fused, ampe_stream_t stream)
+{
+   AMPE_REQUIRE(level && fc && sm && rhs && z && dx, "null argument");
+   AMPE_REQUIRE(ncomp >= 1 && ncomp <= 4, "ncomp must be in [1, 4]");
+   const Level& L = *level->L;
+   hipStream_t s = hs(stream);
+   if (!fused) {
+      for (int sw = 0; sw < num_sweeps; sw++) {
+         if (ampe::ghost_fill(L, z, ncomp, s)) return -1;
+         if (ampe::launch_quat_rbgs<double>(L, fc, sm, gamma, dx, ncomp, rhs, z, 0, s)) return -1;
+         if (ampe::ghost_fill(L, z, ncomp, s)) return -1;
+         if (ampe::launch_quat_rbgs<double>(L, fc, sm, gamma, dx, ncomp, rhs, z, 1, s)) return -1;
+      }
+      return ampe::ghost_fill(L, z, ncomp, s);
+   }
+   double* t = nullptr;
+   size_t bytes = sizeof(double) * (size_t)ncomp * (size_t)L.ncg;
+   AMPE_HIP_CHECK(hipMalloc((void**)&t, bytes));
+   // the second buffer starts as a copy, so ghosts without a gather source
+   // hold z's values in both
+   int rc = hipMemcpyAsync(t, z, bytes, hipMemcpyDeviceToDevice, s) == hipSuccess ? 0 : -1;
+   double *cur = z, *oth = t;
+   for (int sw = 0; sw < num_sweeps && !rc; sw++) {
+      rc = ampe::ghost_fill(L, cur, ncomp, s);
+      if (!rc)
+         rc = ampe::launch_quat_rbgs_fused<double>(L, fc, sm, gamma, dx, ncomp, rhs, cur, oth, s);
+      std::swap(cur, oth);
+   }
+   if (!rc) rc = ampe::ghost_fill(L, cur, ncomp, s);
+   if (!rc && cur != z && hipMemcpyAsync(z, cur, bytes, hipMemcpyDeviceToDevice, s) != hipSuccess)
+      rc = -1;
+   if (hipStreamSynchronize(s) != hipSuccess) rc = -1;
+   (void)hipFree(t);
+   if (rc) ampe::set_error(std::string("ampe_quat_smooth: ") + ampe::last_error());
+   return rc;
+}
+
 int ampe_fac_residual(const ampe_level* level, const ampe_elliptic_op* op, const double* rhs,
                       double* u, double* res, double* rms, ampe_stream_t stream)
 {

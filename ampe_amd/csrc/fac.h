@@ -41,6 +41,13 @@ int launch_quat_residual(const Level& L, const T* fc, const T* sm, double gamma,
 
 // ------------------------------------------------------------ vectors
 // Flat element-wise ops over n entries (whole ghosted vectors).
+// One red-black sweep of the orientation block from zin (ghosts filled) into
+// zout in one pass (plus a ghost fill of zout and a pass over the patches'
+// outer layer): the values of the red launch followed by the black launch.
+template <typename T>
+int launch_quat_rbgs_fused(const Level& L, const T* fc, const T* sm, double gamma,
+                           const double* dx, int ncomp, const T* rhs, const T* zin, T* zout,
+                           hipStream_t s);
 int vec_axpby(long long n, double a, const double* x, double b, const double* y, double* z,
               hipStream_t s);
 int vec_fill(long long n, double v, double* z, hipStream_t s);

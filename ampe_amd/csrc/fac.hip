@@ -200,6 +200,44 @@ __global__ void __launch_bounds__(256) k_residual(LevelArgs a, OpArgs<T> op, T d
 template <typename T, int NDIM>
 __device__ __forceinline__ T quat_row(const LevelArgs& a, const CellPos& c, const T* __restrict__ fc,
                                       const T* __restrict__ sm, T gamma, const T* hinv2,
+                                      const T* __restrict__ z, T* center);
+
+// The same row with the z values supplied by zv(d, side): the cell itself for
+// d < 0, else its neighbour across the low (side -1) or high (+1) face of
+// direction d.  Same operations in the same order as quat_row.
+template <typename T, int NDIM, class ZV>
+__device__ __forceinline__ T quat_row_zv(const LevelArgs& a, const CellPos& c,
+                                         const T* __restrict__ fc, const T* __restrict__ sm,
+                                         T gamma, const T* hinv2, const ZV& zv, T* center)
+{
+   const long long c0 = c.c0;
+   const int ijk[3] = {c.i, c.j, c.k};
+   T olo[3], ohi[3];
+   T diag = 0;
+#pragma unroll
+   for (int d = 0; d < NDIM; d++) {
+      int lo[3] = {ijk[0], ijk[1], ijk[2]};
+      long long flo = face_idx(a, c.pd, d, lo[0], lo[1], lo[2]);
+      lo[d] += 1;
+      long long fhi = face_idx(a, c.pd, d, lo[0], lo[1], lo[2]);
+      olo[d] = fc[flo] * hinv2[d];
+      ohi[d] = fc[fhi] * hinv2[d];
+      diag = (d == 0) ? -olo[0] - ohi[0] : diag - olo[d] - ohi[d];
+   }
+   T fac = gamma * sm[c0];
+   *center = fac * diag * sm[c0] + T(1);
+   T r = *center * zv(-1, 0);
+#pragma unroll
+   for (int d = 0; d < NDIM; d++) {
+      long long s = c.st[d];
+      r = r + fac * olo[d] * sm[c0 - s] * zv(d, -1) + fac * ohi[d] * sm[c0 + s] * zv(d, 1);
+   }
+   return r;
+}
+
+template <typename T, int NDIM>
+__device__ __forceinline__ T quat_row(const LevelArgs& a, const CellPos& c, const T* __restrict__ fc,
+                                      const T* __restrict__ sm, T gamma, const T* hinv2,
                                       const T* __restrict__ z, T* center)
 {
    const long long c0 = c.c0;
@@ -269,6 +307,127 @@ __global__ void __launch_bounds__(128) k_quat_rbgs_c(LevelArgs a, const T* __res
       CellPos c;
       if (cell_pos_color<NDIM>(a, kk, color, c))
          quat_rbgs_cell<T, NDIM>(a, c, fc, sm, gamma, hinv2, ncomp, rhs, z);
+   }
+}
+
+
+// ------------------------------------------------------------ fused sweep
+// One red-black sweep of the orientation block in one pass, zin -> zout
+// (zin ghost-filled, not written).  A workgroup first updates every red cell
+// of its tile grown by one cell in each direction (neighbouring tiles'
+// cells included, from zin only) into LDS, then the black cells of its tile
+// from those red values: the same operations as the red launch followed by
+// the black launch, so the same values, with zin read once instead of twice.
+// Black cells on a patch's outer layer need values of ghost cells; they keep
+// their previous value here and are updated by k_quat_rbgs_shell after a
+// ghost fill of zout.
+constexpr int kRX = TX + 2, kRY = TY + 2;
+
+template <int NDIM>
+__device__ __forceinline__ void make_cell(const LevelArgs& a, const PatchDev& pd, int i, int j,
+                                          int k, CellPos& c)
+{
+   c.pd = pd;
+   c.i = i;
+   c.j = j;
+   c.k = k;
+   c.st[0] = 1;
+   c.st[1] = pd.sy;
+   c.st[2] = pd.sz;
+   c.c0 = pd.cell_off + (i + a.g) + (long long)(j + a.g) * pd.sy + (long long)(k + a.gz) * pd.sz;
+}
+
+template <int NDIM>
+__device__ __forceinline__ bool outer_layer(const CellPos& c)
+{
+   return c.i == 0 || c.i == c.pd.nx - 1 || c.j == 0 || c.j == c.pd.ny - 1 ||
+          (NDIM == 3 && (c.k == 0 || c.k == c.pd.nz - 1));
+}
+
+template <typename T, int NDIM>
+__global__ void __launch_bounds__(256) k_quat_rbgs_fused(LevelArgs a, const T* __restrict__ fc,
+                                                         const T* __restrict__ sm, T gamma,
+                                                         T hi0, T hi1, T hi2, int ncomp,
+                                                         const T* __restrict__ rhs,
+                                                         const T* __restrict__ zin,
+                                                         T* __restrict__ zout)
+{
+   constexpr int RZ = NDIM == 3 ? 3 : 1;
+   constexpr int NR = kRX * kRY * RZ;
+   __shared__ T red[4][NR];
+   const T hinv2[3] = {hi0, hi1, hi2};
+   const Tile t = a.T[tile_index(a)];
+   const PatchDev pd = a.P[t.patch];
+   const int tid = (int)threadIdx.x + TX * (int)threadIdx.y;
+   for (int r = tid; r < NR; r += TX * TY) {
+      const int ri = r % kRX - 1, rj = (r / kRX) % kRY - 1;
+      const int rk = (NDIM == 3) ? r / (kRX * kRY) - 1 : 0;
+      const int i = t.i0 + ri, j = t.j0 + rj, k = t.k + rk;
+      if (i < 0 || i >= pd.nx || j < 0 || j >= pd.ny || k < 0 || k >= pd.nz) continue;
+      CellPos c;
+      make_cell<NDIM>(a, pd, i, j, k, c);
+      if (!is_color<NDIM>(c, 0)) continue;
+      const bool own = rk == 0 && ri >= 0 && ri < TX && rj >= 0 && rj < TY;
+      for (int m = 0; m < ncomp; m++) {
+         const T* zm = zin + m * a.ncg;
+         T center;
+         T az = quat_row<T, NDIM>(a, c, fc, sm, gamma, hinv2, zm, &center);
+         T res = rhs[m * a.ncg + c.c0] - az;
+         T v = zm[c.c0] + res / center;
+         red[m][r] = v;
+         if (own) zout[m * a.ncg + c.c0] = v;
+      }
+   }
+   __syncthreads();
+   CellPos c;
+   if (!cell_pos(a, 0, c)) return;
+   if (is_color<NDIM>(c, 0)) return;
+   if (outer_layer<NDIM>(c)) {
+      // previous value for now: the ghost fill of zout then gives every ghost
+      // cell the value the black launch would read (red updated, black not;
+      // ghost sources lie on their patches' outer layers, and in 2-D the
+      // colouring is per patch, so a neighbour across a patch face can be
+      // black)
+      for (int m = 0; m < ncomp; m++) zout[m * a.ncg + c.c0] = zin[m * a.ncg + c.c0];
+      return;
+   }
+   const int r0 = ((int)threadIdx.x + 1) + kRX * (((int)threadIdx.y + 1) + kRY * (RZ / 2));
+   const int rst[3] = {1, kRX, kRX * kRY};
+   for (int m = 0; m < ncomp; m++) {
+      const T* zm = zin + m * a.ncg;
+      const T* rm = red[m];
+      auto zv = [&](int d, int side) -> T { return d < 0 ? zm[c.c0] : rm[r0 + side * rst[d]]; };
+      T center;
+      T az = quat_row_zv<T, NDIM>(a, c, fc, sm, gamma, hinv2, zv, &center);
+      T res = rhs[m * a.ncg + c.c0] - az;
+      zout[m * a.ncg + c.c0] = zm[c.c0] + res / center;
+   }
+}
+
+// Black cells on the outer layer: neighbours from zout (ghost-filled: red
+// cells updated, black cells still previous), own previous value from zin.
+template <typename T, int NDIM>
+__global__ void __launch_bounds__(256) k_quat_rbgs_shell(LevelArgs a, const T* __restrict__ fc,
+                                                         const T* __restrict__ sm, T gamma,
+                                                         T hi0, T hi1, T hi2, int ncomp,
+                                                         const T* __restrict__ rhs,
+                                                         const T* __restrict__ zin,
+                                                         T* __restrict__ zout)
+{
+   CellPos c;
+   if (!cell_pos(a, 0, c)) return;
+   if (is_color<NDIM>(c, 0) || !outer_layer<NDIM>(c)) return;
+   const T hinv2[3] = {hi0, hi1, hi2};
+   for (int m = 0; m < ncomp; m++) {
+      const T* zm = zin + m * a.ncg;
+      const T* zo = zout + m * a.ncg;
+      auto zv = [&](int d, int side) -> T {
+         return d < 0 ? zm[c.c0] : zo[c.c0 + side * c.st[d]];
+      };
+      T center;
+      T az = quat_row_zv<T, NDIM>(a, c, fc, sm, gamma, hinv2, zv, &center);
+      T res = rhs[m * a.ncg + c.c0] - az;
+      zout[m * a.ncg + c.c0] = zm[c.c0] + res / center;
    }
 }
 
@@ -402,6 +561,40 @@ int launch_quat_rbgs(const Level& L, const T* fc, const T* sm, double gamma, con
 }
 
 template <typename T>
+int launch_quat_rbgs_fused(const Level& L, const T* fc, const T* sm, double gamma,
+                           const double* dx, int ncomp, const T* rhs, const T* zin, T* zout,
+                           hipStream_t s)
+{
+   if (ncomp > 4) {
+      set_error("launch_quat_rbgs_fused: at most four components");
+      return -1;
+   }
+   T hi[3];
+   recip2(dx, hi);
+   LevelArgs la = level_args(L);
+   dim3 g(L.ncell_tiles);
+   {
+      KernelTimer::Span span(KernelTimer::QUAT_RBGS, s, quat_rbgs_fused_bytes(L, ncomp, sizeof(T)));
+      if (L.ndim == 3)
+         hipLaunchKernelGGL((k_quat_rbgs_fused<T, 3>), g, tblock(), 0, s, la, fc, sm, (T)gamma,
+                            hi[0], hi[1], hi[2], ncomp, rhs, zin, zout);
+      else
+         hipLaunchKernelGGL((k_quat_rbgs_fused<T, 2>), g, tblock(), 0, s, la, fc, sm, (T)gamma,
+                            hi[0], hi[1], hi[2], ncomp, rhs, zin, zout);
+      AMPE_HIP_CHECK(hipGetLastError());
+   }
+   if (ghost_fill(L, zout, ncomp, s)) return -1;
+   if (L.ndim == 3)
+      hipLaunchKernelGGL((k_quat_rbgs_shell<T, 3>), g, tblock(), 0, s, la, fc, sm, (T)gamma,
+                         hi[0], hi[1], hi[2], ncomp, rhs, zin, zout);
+   else
+      hipLaunchKernelGGL((k_quat_rbgs_shell<T, 2>), g, tblock(), 0, s, la, fc, sm, (T)gamma,
+                         hi[0], hi[1], hi[2], ncomp, rhs, zin, zout);
+   AMPE_HIP_CHECK(hipGetLastError());
+   return 0;
+}
+
+template <typename T>
 int launch_quat_residual(const Level& L, const T* fc, const T* sm, double gamma,
                          const double* dx, int ncomp, const T* rhs, const T* z, T* res,
                          double* part, hipStream_t s)
@@ -430,7 +623,10 @@ int launch_quat_residual(const Level& L, const T* fc, const T* sm, double gamma,
                                     int, const T*, T*, int, hipStream_t);                \
    template int launch_quat_residual<T>(const Level&, const T*, const T*, double,         \
                                         const double*, int, const T*, const T*, T*,      \
-                                        double*, hipStream_t);
+                                        double*, hipStream_t);                           \
+   template int launch_quat_rbgs_fused<T>(const Level&, const T*, const T*, double,       \
+                                          const double*, int, const T*, const T*, T*,    \
+                                          hipStream_t);
 AMPE_INST(double)
 AMPE_INST(float)
 

@@ -111,5 +111,8 @@ struct KernelTimer {
 // read, the colour's z written and rhs read, every face coefficient and the
 // cell sqrt(M) read once.
 double quat_rbgs_bytes(const Level& L, int ncomp, int elem);
+// The same for one fused sweep (both colours): z read and written, rhs, face
+// coefficients and sqrt(M) read once.
+double quat_rbgs_fused_bytes(const Level& L, int ncomp, int elem);
 
 }  // namespace ampe

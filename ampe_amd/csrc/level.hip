@@ -445,6 +445,12 @@ int KernelTimer::read(int id, double* ms, long long* launches, double* bytes)
    return 0;
 }
 
+double quat_rbgs_fused_bytes(const Level& L, int ncomp, int elem)
+{
+   double n = (double)L.ncell;
+   return (double)elem * (3.0 * ncomp * n + (double)L.nside_tot + n);
+}
+
 double quat_rbgs_bytes(const Level& L, int ncomp, int elem)
 {
    double n = (double)L.ncell;

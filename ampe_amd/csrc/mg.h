@@ -30,6 +30,7 @@ struct MgLevel {
    T* x = nullptr;      // correction, ncomp_max blocks, levels >= 1
    T* b = nullptr;      // right-hand side, ncomp_max blocks, levels >= 1
    T* r = nullptr;      // residual, ncomp_max blocks, all levels
+   T* t = nullptr;      // second buffer of the fused orientation sweeps, all levels
 };
 
 template <typename T>
@@ -42,6 +43,11 @@ struct Mg {
 struct MgCycle {
    int nu1 = 2, nu2 = 2;  // pre / post red-black sweeps
    int ncoarse = 16;      // sweeps on the coarsest level
+   // orientation block: one-pass sweeps (same values; measured slower on the
+   // 256^3 workload, 0.40 ms against 2 x 0.10 ms per sweep averaged over the
+   // levels, because each one-plane tile recomputes the red cells of the
+   // planes above and below)
+   bool fused = false;
 };
 
 // Build the hierarchy below fine while every patch box and the periodic

@@ -192,6 +192,10 @@ int mg_create(const Level& fine, const double* dx, int ncomp_max, int max_levels
    l0.L = const_cast<Level*>(&fine);
    for (int a = 0; a < 3; a++) l0.dx[a] = dx[a];
    int rc = talloc(&l0.r, (size_t)ncomp_max * fine.ncg);
+   rc |= talloc(&l0.t, (size_t)ncomp_max * fine.ncg);
+   // ghost cells without a gather source are never written: keep them zero,
+   // as in the arrays the sweeps alternate with
+   if (!rc && hipMemset(l0.t, 0, sizeof(T) * (size_t)ncomp_max * fine.ncg) != hipSuccess) rc = -1;
    m->lv.push_back(l0);
    if (max_levels < 1) max_levels = 1;
    while (!rc && (int)m->lv.size() < max_levels && halvable(*m->lv.back().L, min_cells)) {
@@ -225,6 +229,8 @@ int mg_create(const Level& fine, const double* dx, int ncomp_max, int max_levels
       rc |= talloc(&c.x, (size_t)ncomp_max * ncg);
       rc |= talloc(&c.b, (size_t)ncomp_max * ncg);
       rc |= talloc(&c.r, (size_t)ncomp_max * ncg);
+      rc |= talloc(&c.t, (size_t)ncomp_max * ncg);
+      if (!rc && hipMemset(c.t, 0, sizeof(T) * (size_t)ncomp_max * ncg) != hipSuccess) rc = -1;
       m->lv.push_back(c);
    }
    if (rc) {
@@ -243,6 +249,7 @@ void mg_destroy(Mg<T>* m)
    for (size_t l = 0; l < m->lv.size(); l++) {
       MgLevel<T>& c = m->lv[l];
       (void)hipFree(c.r);
+      (void)hipFree(c.t);
       if (l == 0) continue;
       (void)hipFree(c.C);
       (void)hipFree(c.M);
@@ -322,6 +329,20 @@ static int smooth_quat(const Level& L, const T* fc, const T* sm, double gamma, c
    return 0;
 }
 
+// nsweeps fused sweeps alternating between *cur and *oth; on return *cur
+// holds the result.
+template <typename T>
+static int smooth_quat_pp(const Level& L, const T* fc, const T* sm, double gamma, const double* dx,
+                          int ncomp, const T* b, T** cur, T** oth, int sweeps, hipStream_t s)
+{
+   for (int sw = 0; sw < sweeps; sw++) {
+      if (ghost_fill(L, *cur, ncomp, s)) return -1;
+      if (launch_quat_rbgs_fused<T>(L, fc, sm, gamma, dx, ncomp, b, *cur, *oth, s)) return -1;
+      std::swap(*cur, *oth);
+   }
+   return 0;
+}
+
 template <typename T>
 static EllOp<T> level_op(const Mg<T>& m, const EllOp<T>& op0, int kind, int l)
 {
@@ -385,6 +406,47 @@ int mg_vcycle_quat(Mg<T>& m, const T* fc0, const T* sm0, double gamma, int ncomp
       return -1;
    }
    int nl = m.nlev();
+   if (cy.fused) {
+      // one-pass sweeps alternate between x and t on every level; cur[l] is
+      // the level's current iterate
+      std::vector<T*> cur(nl), oth(nl);
+      for (int l = 0; l < nl; l++) {
+         cur[l] = l == 0 ? x0 : m.lv[l].x;
+         oth[l] = m.lv[l].t;
+      }
+      for (int l = 0; l < nl; l++) {
+         MgLevel<T>& c = m.lv[l];
+         const T* b = l == 0 ? b0 : c.b;
+         const T* fc = l == 0 ? fc0 : c.fc;
+         const T* sm = l == 0 ? sm0 : c.sm;
+         int nsw = (l == nl - 1) ? cy.ncoarse : cy.nu1;
+         if (smooth_quat_pp<T>(*c.L, fc, sm, gamma, c.dx, ncomp, b, &cur[l], &oth[l], nsw, s))
+            return -1;
+         if (l == nl - 1) break;
+         if (ghost_fill(*c.L, cur[l], ncomp, s)) return -1;
+         if (launch_quat_residual<T>(*c.L, fc, sm, gamma, c.dx, ncomp, b, cur[l], c.r, nullptr,
+                                     s))
+            return -1;
+         MgLevel<T>& n = m.lv[l + 1];
+         if (mg_restrict_cell<T>(*n.L, *c.L, ncomp, c.r, n.b, s)) return -1;
+         AMPE_HIP_CHECK(hipMemsetAsync(cur[l + 1], 0, sizeof(T) * n.L->ncg * ncomp, s));
+      }
+      for (int l = nl - 2; l >= 0; l--) {
+         MgLevel<T>& c = m.lv[l];
+         MgLevel<T>& n = m.lv[l + 1];
+         const T* b = l == 0 ? b0 : c.b;
+         const T* fc = l == 0 ? fc0 : c.fc;
+         const T* sm = l == 0 ? sm0 : c.sm;
+         if (ghost_fill(*n.L, cur[l + 1], ncomp, s)) return -1;
+         if (mg_prolong_add<T>(*c.L, *n.L, ncomp, 1, cur[l + 1], cur[l], s)) return -1;
+         if (smooth_quat_pp<T>(*c.L, fc, sm, gamma, c.dx, ncomp, b, &cur[l], &oth[l], cy.nu2, s))
+            return -1;
+      }
+      if (cur[0] != x0)
+         AMPE_HIP_CHECK(hipMemcpyAsync(x0, cur[0], sizeof(T) * m.lv[0].L->ncg * ncomp,
+                                       hipMemcpyDeviceToDevice, s));
+      return 0;
+   }
    for (int l = 0; l < nl; l++) {
       MgLevel<T>& c = m.lv[l];
       const T* b = l == 0 ? b0 : c.b;

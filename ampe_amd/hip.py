@@ -45,6 +45,8 @@ _SIGS = {
                             C.POINTER(_i), _vp]),
     "ampe_quat_face_coefs": (_i, [_vp, C.POINTER(ModelParams), _vp, _vp, _vp, _vp]),
     "ampe_quat_rhs_fc": (_i, [_vp, C.POINTER(ModelParams), _vp, _vp, _vp, _vp, _vp]),
+    "ampe_quat_smooth": (_i, [_vp, _vp, _vp, C.c_double, C.POINTER(C.c_double), C.c_int, _vp,
+                              _vp, C.c_int, C.c_int, _vp]),
     "ampe_grains_label": (_i, [_vp, _vp, C.c_double, _vp, C.POINTER(C.c_int), _vp]),
     "ampe_kernel_timer_enable": (_i, [C.c_int, C.c_int]),
     "ampe_kernel_timer_read": (_i, [C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_longlong),

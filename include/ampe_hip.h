@@ -362,6 +362,18 @@ int ampe_fac_smooth(const ampe_level* level, const ampe_elliptic_op* op,
 
 /* res = rhs - A u (efo_compres*sca, source/fortran/2d/ellipticfacops.m4:346)
  * after a ghost fill of u; *rms (host, may be NULL) = RMS norm of res. */
+/* num_sweeps red-black sweeps of the orientation block
+ * (I + gamma sqrt(M) L sqrt(M)) z = rhs on ncomp (<= 4) component blocks:
+ * fc face coefficients (side field), sm = sqrt(M) (cell field, ghosts
+ * filled), dx the level spacing; z ghosted, updated in place.  fused = 0: a
+ * red then a black launch per sweep with ghost fills between; fused = 1: the
+ * one-pass sweep (same values).  The role of QuatFACOps::smoothError /
+ * QuatLevelSolver (source/QuatFACOps.cc:1009-1091), which the reference
+ * gives to hypre; this build smooths the same symmetrised matrix. */
+int ampe_quat_smooth(const ampe_level* level, const double* fc, const double* sm,
+                     double gamma, const double dx[3], int ncomp, const double* rhs,
+                     double* z, int num_sweeps, int fused, ampe_stream_t stream);
+
 int ampe_fac_residual(const ampe_level* level, const ampe_elliptic_op* op,
                       const double* rhs, double* u, double* res, double* rms,
                       ampe_stream_t stream);
