@@ -352,55 +352,79 @@ __global__ void __launch_bounds__(256) k_quat_rbgs_fused(LevelArgs a, const T* _
                                                          const T* __restrict__ zin,
                                                          T* __restrict__ zout)
 {
-   constexpr int RZ = NDIM == 3 ? 3 : 1;
-   constexpr int NR = kRX * kRY * RZ;
-   __shared__ T red[4][NR];
+   // a column of the tile's planes [t.k, kend): the red values of the tile
+   // grown by one cell in x and y for planes k-1, k, k+1 live in a ring of
+   // three LDS slots (slot = plane % 3); each step adds plane k+1, then
+   // updates the black cells of plane k
+   constexpr int NP = kRX * kRY;
+   constexpr int RING = NDIM == 3 ? 3 : 1;
+   __shared__ T red[RING][4][NP];
    const T hinv2[3] = {hi0, hi1, hi2};
    const Tile t = a.T[tile_index(a)];
    const PatchDev pd = a.P[t.patch];
+   const int kend = NDIM == 3 ? min(t.k + a.nz_tile, pd.nz) : 1;
    const int tid = (int)threadIdx.x + TX * (int)threadIdx.y;
-   for (int r = tid; r < NR; r += TX * TY) {
-      const int ri = r % kRX - 1, rj = (r / kRX) % kRY - 1;
-      const int rk = (NDIM == 3) ? r / (kRX * kRY) - 1 : 0;
-      const int i = t.i0 + ri, j = t.j0 + rj, k = t.k + rk;
-      if (i < 0 || i >= pd.nx || j < 0 || j >= pd.ny || k < 0 || k >= pd.nz) continue;
-      CellPos c;
-      make_cell<NDIM>(a, pd, i, j, k, c);
-      if (!is_color<NDIM>(c, 0)) continue;
-      const bool own = rk == 0 && ri >= 0 && ri < TX && rj >= 0 && rj < TY;
-      for (int m = 0; m < ncomp; m++) {
-         const T* zm = zin + m * a.ncg;
-         T center;
-         T az = quat_row<T, NDIM>(a, c, fc, sm, gamma, hinv2, zm, &center);
-         T res = rhs[m * a.ncg + c.c0] - az;
-         T v = zm[c.c0] + res / center;
-         red[m][r] = v;
-         if (own) zout[m * a.ncg + c.c0] = v;
+   auto red_plane = [&](int p) {
+      if (p < 0 || p >= pd.nz) return;
+      const int slot = NDIM == 3 ? p % 3 : 0;
+      for (int r = tid; r < NP; r += TX * TY) {
+         const int ri = r % kRX - 1, rj = r / kRX - 1;
+         const int i = t.i0 + ri, j = t.j0 + rj;
+         if (i < 0 || i >= pd.nx || j < 0 || j >= pd.ny) continue;
+         CellPos c;
+         make_cell<NDIM>(a, pd, i, j, p, c);
+         if (!is_color<NDIM>(c, 0)) continue;
+         const bool own = ri >= 0 && ri < TX && rj >= 0 && rj < TY && p >= t.k && p < kend;
+         for (int m = 0; m < ncomp; m++) {
+            const T* zm = zin + m * a.ncg;
+            T center;
+            T az = quat_row<T, NDIM>(a, c, fc, sm, gamma, hinv2, zm, &center);
+            T res = rhs[m * a.ncg + c.c0] - az;
+            T v = zm[c.c0] + res / center;
+            red[slot][m][r] = v;
+            if (own) zout[m * a.ncg + c.c0] = v;
+         }
       }
-   }
-   __syncthreads();
-   CellPos c;
-   if (!cell_pos(a, 0, c)) return;
-   if (is_color<NDIM>(c, 0)) return;
-   if (outer_layer<NDIM>(c)) {
-      // previous value for now: the ghost fill of zout then gives every ghost
-      // cell the value the black launch would read (red updated, black not;
-      // ghost sources lie on their patches' outer layers, and in 2-D the
-      // colouring is per patch, so a neighbour across a patch face can be
-      // black)
-      for (int m = 0; m < ncomp; m++) zout[m * a.ncg + c.c0] = zin[m * a.ncg + c.c0];
-      return;
-   }
-   const int r0 = ((int)threadIdx.x + 1) + kRX * (((int)threadIdx.y + 1) + kRY * (RZ / 2));
-   const int rst[3] = {1, kRX, kRX * kRY};
-   for (int m = 0; m < ncomp; m++) {
-      const T* zm = zin + m * a.ncg;
-      const T* rm = red[m];
-      auto zv = [&](int d, int side) -> T { return d < 0 ? zm[c.c0] : rm[r0 + side * rst[d]]; };
-      T center;
-      T az = quat_row_zv<T, NDIM>(a, c, fc, sm, gamma, hinv2, zv, &center);
-      T res = rhs[m * a.ncg + c.c0] - az;
-      zout[m * a.ncg + c.c0] = zm[c.c0] + res / center;
+   };
+   if (NDIM == 3) red_plane(t.k - 1);
+   red_plane(t.k);
+   const int tx = (int)threadIdx.x, ty = (int)threadIdx.y;
+   const int i = t.i0 + tx, j = t.j0 + ty;
+   const int r0 = (tx + 1) + kRX * (ty + 1);
+   for (int k = t.k; k < kend; k++) {
+      if (NDIM == 3) red_plane(k + 1);
+      __syncthreads();
+      if (i < pd.nx && j < pd.ny) {
+         CellPos c;
+         make_cell<NDIM>(a, pd, i, j, k, c);
+         if (!is_color<NDIM>(c, 0)) {
+            if (outer_layer<NDIM>(c)) {
+               // previous value for now: the ghost fill of zout then gives
+               // every ghost cell the value the black launch would read (red
+               // updated, black not; ghost sources lie on their patches' outer
+               // layers, and in 2-D the colouring is per patch, so a neighbour
+               // across a patch face can be black)
+               for (int m = 0; m < ncomp; m++) zout[m * a.ncg + c.c0] = zin[m * a.ncg + c.c0];
+            } else {
+               const int s0 = NDIM == 3 ? k % 3 : 0;
+               const int sm1 = NDIM == 3 ? (k + 2) % 3 : 0, sp1 = NDIM == 3 ? (k + 1) % 3 : 0;
+               for (int m = 0; m < ncomp; m++) {
+                  const T* zm = zin + m * a.ncg;
+                  auto zv = [&](int d, int side) -> T {
+                     if (d < 0) return zm[c.c0];
+                     if (d == 0) return red[s0][m][r0 + side];
+                     if (d == 1) return red[s0][m][r0 + side * kRX];
+                     return red[side < 0 ? sm1 : sp1][m][r0];
+                  };
+                  T center;
+                  T az = quat_row_zv<T, NDIM>(a, c, fc, sm, gamma, hinv2, zv, &center);
+                  T res = rhs[m * a.ncg + c.c0] - az;
+                  zout[m * a.ncg + c.c0] = zm[c.c0] + res / center;
+               }
+            }
+         }
+      }
+      __syncthreads();
    }
 }
 
@@ -575,11 +599,13 @@ int launch_quat_rbgs_fused(const Level& L, const T* fc, const T* sm, double gamm
    dim3 g(L.ncell_tiles);
    {
       KernelTimer::Span span(KernelTimer::QUAT_RBGS, s, quat_rbgs_fused_bytes(L, ncomp, sizeof(T)));
+      LevelArgs lm = level_args_march(L);
+      dim3 gm(L.nmarch_tiles);
       if (L.ndim == 3)
-         hipLaunchKernelGGL((k_quat_rbgs_fused<T, 3>), g, tblock(), 0, s, la, fc, sm, (T)gamma,
+         hipLaunchKernelGGL((k_quat_rbgs_fused<T, 3>), gm, tblock(), 0, s, lm, fc, sm, (T)gamma,
                             hi[0], hi[1], hi[2], ncomp, rhs, zin, zout);
       else
-         hipLaunchKernelGGL((k_quat_rbgs_fused<T, 2>), g, tblock(), 0, s, la, fc, sm, (T)gamma,
+         hipLaunchKernelGGL((k_quat_rbgs_fused<T, 2>), gm, tblock(), 0, s, lm, fc, sm, (T)gamma,
                             hi[0], hi[1], hi[2], ncomp, rhs, zin, zout);
       AMPE_HIP_CHECK(hipGetLastError());
    }

@@ -40,6 +40,15 @@ inline LevelArgs level_args_col(const Level& L)
    return a;
 }
 
+// Column list of the marching kernels (KM planes per workgroup in 3-D).
+inline LevelArgs level_args_march(const Level& L)
+{
+   LevelArgs a = level_args(L);
+   a.T = L.march_tiles;
+   a.nz_tile = L.ndim == 3 ? KM : 1;
+   return a;
+}
+
 struct CellPos {
    int i, j, k;
    long long c0;  // ghosted cell index inside one component block

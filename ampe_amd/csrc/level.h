@@ -30,6 +30,9 @@ constexpr int TY = 4;
 // k planes, so the k-neighbour loads of a plane are the previous plane's
 // centre loads (cache hits) instead of HBM reads.
 constexpr int KZ = 1;
+// Planes per column of the marching kernels (the one-pass red-black sweep):
+// a workgroup walks KM planes keeping three planes of results in LDS.
+constexpr int KM = 16;
 
 struct PatchDev {
    int nx, ny, nz;         // interior extents (nz = 1 in 2-D)
@@ -62,6 +65,8 @@ struct Level {
    int nface_tiles = 0;
    Tile* col_tiles = nullptr;   // interior cells in columns of KZ planes (3-D)
    int ncol_tiles = 0;
+   Tile* march_tiles = nullptr;  // interior cells in columns of KM planes (3-D)
+   int nmarch_tiles = 0;
    long long* ghost_dst = nullptr;  // ghost gather (periodic / neighbours)
    long long* ghost_src = nullptr;
    long long nghost = 0;

@@ -127,7 +127,7 @@ int level_create(int ndim, const ampe_box* boxes, int npatch, int ghost,
    L->nside_tot = soff[0] + soff[1] + soff[2];
 
    // work lists
-   std::vector<Tile> ct, ft, zt;
+   std::vector<Tile> ct, ft, zt, mt;
    for (int p = 0; p < npatch; p++) {
       const PatchDev& d = L->hp[p];
       for (int k = 0; k < d.nz; k++)
@@ -136,6 +136,9 @@ int level_create(int ndim, const ampe_box* boxes, int npatch, int ghost,
       for (int k = 0; k < d.nz; k += (ndim == 3 ? KZ : 1))
          for (int j0 = 0; j0 < d.ny; j0 += TY)
             for (int i0 = 0; i0 < d.nx; i0 += TX) zt.push_back({p, i0, j0, k});
+      for (int k = 0; k < d.nz; k += (ndim == 3 ? KM : 1))
+         for (int j0 = 0; j0 < d.ny; j0 += TY)
+            for (int i0 = 0; i0 < d.nx; i0 += TX) mt.push_back({p, i0, j0, k});
       int fz = (ndim == 3) ? d.nz + 1 : 1;
       for (int k = 0; k < fz; k++)
          for (int j0 = 0; j0 < d.ny + 1; j0 += TY)
@@ -144,6 +147,8 @@ int level_create(int ndim, const ampe_box* boxes, int npatch, int ghost,
    xcd_order(ct);
    xcd_order(ft);
    xcd_order(zt);
+   xcd_order(mt);
+   L->nmarch_tiles = (int)mt.size();
    L->ncell_tiles = (int)ct.size();
    L->nface_tiles = (int)ft.size();
    L->ncol_tiles = (int)zt.size();
@@ -199,6 +204,7 @@ int level_create(int ndim, const ampe_box* boxes, int npatch, int ghost,
    if (e == hipSuccess) e = hipMalloc(&L->cell_tiles, sizeof(Tile) * ct.size());
    if (e == hipSuccess) e = hipMalloc(&L->face_tiles, sizeof(Tile) * ft.size());
    if (e == hipSuccess) e = hipMalloc(&L->col_tiles, sizeof(Tile) * zt.size());
+   if (e == hipSuccess) e = hipMalloc(&L->march_tiles, sizeof(Tile) * mt.size());
    if (e == hipSuccess && L->nghost > 0) {
       e = hipMalloc(&L->ghost_dst, sizeof(long long) * gd.size());
       if (e == hipSuccess) e = hipMalloc(&L->ghost_src, sizeof(long long) * gs.size());
@@ -220,6 +226,8 @@ int level_create(int ndim, const ampe_box* boxes, int npatch, int ghost,
       e = hipMemcpy(L->face_tiles, ft.data(), sizeof(Tile) * ft.size(), hipMemcpyHostToDevice);
    if (e == hipSuccess)
       e = hipMemcpy(L->col_tiles, zt.data(), sizeof(Tile) * zt.size(), hipMemcpyHostToDevice);
+   if (e == hipSuccess)
+      e = hipMemcpy(L->march_tiles, mt.data(), sizeof(Tile) * mt.size(), hipMemcpyHostToDevice);
    if (e == hipSuccess && L->nghost > 0) {
       e = hipMemcpy(L->ghost_dst, gd.data(), sizeof(long long) * gd.size(), hipMemcpyHostToDevice);
       if (e == hipSuccess)
@@ -241,6 +249,7 @@ void level_destroy(Level* L)
    (void)hipFree(L->cell_tiles);
    (void)hipFree(L->face_tiles);
    (void)hipFree(L->col_tiles);
+   (void)hipFree(L->march_tiles);
    (void)hipFree(L->ghost_dst);
    (void)hipFree(L->ghost_src);
    (void)hipFree(L->partial);

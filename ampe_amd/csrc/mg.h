@@ -44,9 +44,9 @@ struct MgCycle {
    int nu1 = 2, nu2 = 2;  // pre / post red-black sweeps
    int ncoarse = 16;      // sweeps on the coarsest level
    // orientation block: one-pass sweeps (same values; measured slower on the
-   // 256^3 workload, 0.40 ms against 2 x 0.10 ms per sweep averaged over the
-   // levels, because each one-plane tile recomputes the red cells of the
-   // planes above and below)
+   // 256^3 workload: 0.32 ms per sweep against 2 x 0.10 ms, averaged over the
+   // levels -- each workgroup walks its column of planes with two barriers per
+   // plane, which leaves too few loads in flight)
    bool fused = false;
 };
 
