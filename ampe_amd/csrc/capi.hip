@@ -91,6 +91,17 @@ int ampe_level_patch_offsets(const ampe_level* level, long long* cell_off, long 
    return 0;
 }
 
+int ampe_level_patch_strides(const ampe_level* level, long long* sy, long long* sz)
+{
+   AMPE_REQUIRE(level, "null level");
+   const Level& L = *level->L;
+   for (int p = 0; p < L.npatch; p++) {
+      if (sy) sy[p] = L.hp[p].sy;
+      if (sz) sz[p] = L.hp[p].sz;
+   }
+   return 0;
+}
+
 int ampe_ghost_fill(const ampe_level* level, double* field, int ncomp, ampe_stream_t stream)
 {
    AMPE_REQUIRE(level && field, "null argument");

@@ -3,6 +3,8 @@
 // the orientation block's symmetrised operator I + gamma sqrt(M) L sqrt(M).
 // Templated on the storage/compute type: double for the FP64 path, float for
 // the mixed-precision preconditioner.
+#include <type_traits>
+
 #include "fac.h"
 
 namespace ampe {
@@ -265,37 +267,110 @@ __device__ __forceinline__ T quat_row(const LevelArgs& a, const CellPos& c, cons
    return r;
 }
 
+// The row's coefficients at one cell, shared by every component: centre
+// gamma sm^2 diag + 1 and the neighbour weights gamma sm fc/h^2 sm_nbr (the
+// products quat_row forms, in its order).
+template <typename T>
+struct QuatRowCoef {
+   T center;
+   T wlo[3], whi[3];
+};
+
 template <typename T, int NDIM>
+__device__ __forceinline__ QuatRowCoef<T> quat_row_coef(const LevelArgs& a, const CellPos& c,
+                                                        const T* __restrict__ fc,
+                                                        const T* __restrict__ sm, T gamma,
+                                                        const T* hinv2)
+{
+   const long long c0 = c.c0;
+   const int ijk[3] = {c.i, c.j, c.k};
+   T olo[3], ohi[3];
+   T diag = 0;
+#pragma unroll
+   for (int d = 0; d < NDIM; d++) {
+      int lo[3] = {ijk[0], ijk[1], ijk[2]};
+      long long flo = face_idx(a, c.pd, d, lo[0], lo[1], lo[2]);
+      lo[d] += 1;
+      long long fhi = face_idx(a, c.pd, d, lo[0], lo[1], lo[2]);
+      olo[d] = fc[flo] * hinv2[d];
+      ohi[d] = fc[fhi] * hinv2[d];
+      diag = (d == 0) ? -olo[0] - ohi[0] : diag - olo[d] - ohi[d];
+   }
+   QuatRowCoef<T> w;
+   T fac = gamma * sm[c0];
+   w.center = fac * diag * sm[c0] + T(1);
+#pragma unroll
+   for (int d = 0; d < NDIM; d++) {
+      w.wlo[d] = fac * olo[d] * sm[c0 - c.st[d]];
+      w.whi[d] = fac * ohi[d] * sm[c0 + c.st[d]];
+   }
+   return w;
+}
+
+// Component values of one cell's row: the cell and its face neighbours.
+template <typename T, int NDIM>
+struct QuatRowVals {
+   T zc, zl[NDIM], zh[NDIM];
+};
+
+template <typename T, int NDIM>
+__device__ __forceinline__ void quat_row_load(const CellPos& c, const T* __restrict__ zm,
+                                              QuatRowVals<T, NDIM>& v)
+{
+   v.zc = zm[c.c0];
+#pragma unroll
+   for (int d = 0; d < NDIM; d++) {
+      v.zl[d] = zm[c.c0 - c.st[d]];
+      v.zh[d] = zm[c.c0 + c.st[d]];
+   }
+}
+
+// A z from the loaded values: the sum quat_row forms, in its order.
+template <typename T, int NDIM>
+__device__ __forceinline__ T quat_row_apply(const QuatRowCoef<T>& w, const QuatRowVals<T, NDIM>& v)
+{
+   T r = w.center * v.zc;
+#pragma unroll
+   for (int d = 0; d < NDIM; d++) r = r + w.wlo[d] * v.zl[d] + w.whi[d] * v.zh[d];
+   return r;
+}
+
+// Gauss-Seidel update of one cell, every component.  QL > 0: QL components,
+// all loads issued before the first update (the updates write only this
+// cell, which no other load of the cell reads); QL = 0: ncomp components one
+// after the other.
+template <typename T, int NDIM, int QL>
 __device__ __forceinline__ void quat_rbgs_cell(const LevelArgs& a, const CellPos& c,
                                                const T* __restrict__ fc, const T* __restrict__ sm,
                                                T gamma, const T* hinv2, int ncomp,
                                                const T* __restrict__ rhs, T* __restrict__ z)
 {
-   for (int m = 0; m < ncomp; m++) {
-      const T* rm = rhs + m * a.ncg;
-      T* zm = z + m * a.ncg;
-      T center;
-      T az = quat_row<T, NDIM>(a, c, fc, sm, gamma, hinv2, zm, &center);
-      T res = rm[c.c0] - az;
-      zm[c.c0] = zm[c.c0] + res / center;
+   const QuatRowCoef<T> w = quat_row_coef<T, NDIM>(a, c, fc, sm, gamma, hinv2);
+   if (QL > 0) {
+      QuatRowVals<T, NDIM> v[QL > 0 ? QL : 1];
+      T rv[QL > 0 ? QL : 1];
+#pragma unroll
+      for (int m = 0; m < QL; m++) {
+         quat_row_load<T, NDIM>(c, z + m * a.ncg, v[m]);
+         rv[m] = rhs[m * a.ncg + c.c0];
+      }
+#pragma unroll
+      for (int m = 0; m < QL; m++) {
+         T res = rv[m] - quat_row_apply<T, NDIM>(w, v[m]);
+         z[m * a.ncg + c.c0] = v[m].zc + res / w.center;
+      }
+   } else {
+      for (int m = 0; m < ncomp; m++) {
+         QuatRowVals<T, NDIM> v;
+         quat_row_load<T, NDIM>(c, z + m * a.ncg, v);
+         T res = rhs[m * a.ncg + c.c0] - quat_row_apply<T, NDIM>(w, v);
+         z[m * a.ncg + c.c0] = v.zc + res / w.center;
+      }
    }
 }
 
-template <typename T, int NDIM>
-__global__ void __launch_bounds__(256) k_quat_rbgs(LevelArgs a, const T* __restrict__ fc,
-                                                   const T* __restrict__ sm, T gamma, T hi0, T hi1,
-                                                   T hi2, int ncomp, const T* __restrict__ rhs,
-                                                   T* __restrict__ z, int color)
-{
-   const T hinv2[3] = {hi0, hi1, hi2};
-   AMPE_COLUMN_BEGIN(a, c)
-   if (!is_color<NDIM>(c, color)) continue;
-   quat_rbgs_cell<T, NDIM>(a, c, fc, sm, gamma, hinv2, ncomp, rhs, z);
-   AMPE_COLUMN_END
-}
-
-// The same on the colour-compact mapping.
-template <typename T, int NDIM>
+// One colour of the sweep on the colour-compact mapping.
+template <typename T, int NDIM, int QL>
 __global__ void __launch_bounds__(128) k_quat_rbgs_c(LevelArgs a, const T* __restrict__ fc,
                                                      const T* __restrict__ sm, T gamma, T hi0,
                                                      T hi1, T hi2, int ncomp,
@@ -306,7 +381,7 @@ __global__ void __launch_bounds__(128) k_quat_rbgs_c(LevelArgs a, const T* __res
    for (int kk = 0; kk < a.nz_tile; kk++) {
       CellPos c;
       if (cell_pos_color<NDIM>(a, kk, color, c))
-         quat_rbgs_cell<T, NDIM>(a, c, fc, sm, gamma, hinv2, ncomp, rhs, z);
+         quat_rbgs_cell<T, NDIM, QL>(a, c, fc, sm, gamma, hinv2, ncomp, rhs, z);
    }
 }
 
@@ -455,7 +530,7 @@ __global__ void __launch_bounds__(256) k_quat_rbgs_shell(LevelArgs a, const T* _
    }
 }
 
-template <typename T, int NDIM>
+template <typename T, int NDIM, int QL>
 __global__ void __launch_bounds__(256) k_quat_residual(LevelArgs a, const T* __restrict__ fc,
                                                        const T* __restrict__ sm, T gamma, T hi0,
                                                        T hi1, T hi2, int ncomp,
@@ -465,13 +540,30 @@ __global__ void __launch_bounds__(256) k_quat_residual(LevelArgs a, const T* __r
 {
    double sq = 0.0;
    AMPE_COLUMN_BEGIN(a, c)
-   T hinv2[3] = {hi0, hi1, hi2};
-   for (int m = 0; m < ncomp; m++) {
-      T center;
-      T az = quat_row<T, NDIM>(a, c, fc, sm, gamma, hinv2, z + m * a.ncg, &center);
-      T r = rhs[m * a.ncg + c.c0] - az;
-      res[m * a.ncg + c.c0] = r;
-      sq += (double)r * (double)r;
+   const T hinv2[3] = {hi0, hi1, hi2};
+   const QuatRowCoef<T> w = quat_row_coef<T, NDIM>(a, c, fc, sm, gamma, hinv2);
+   if (QL > 0) {
+      QuatRowVals<T, NDIM> v[QL > 0 ? QL : 1];
+      T rv[QL > 0 ? QL : 1];
+#pragma unroll
+      for (int m = 0; m < QL; m++) {
+         quat_row_load<T, NDIM>(c, z + m * a.ncg, v[m]);
+         rv[m] = rhs[m * a.ncg + c.c0];
+      }
+#pragma unroll
+      for (int m = 0; m < QL; m++) {
+         T r = rv[m] - quat_row_apply<T, NDIM>(w, v[m]);
+         res[m * a.ncg + c.c0] = r;
+         sq += (double)r * (double)r;
+      }
+   } else {
+      for (int m = 0; m < ncomp; m++) {
+         QuatRowVals<T, NDIM> v;
+         quat_row_load<T, NDIM>(c, z + m * a.ncg, v);
+         T r = rhs[m * a.ncg + c.c0] - quat_row_apply<T, NDIM>(w, v);
+         res[m * a.ncg + c.c0] = r;
+         sq += (double)r * (double)r;
+      }
    }
    AMPE_COLUMN_END
    if (partial) block_store_sum<T>(sq, partial);
@@ -480,6 +572,27 @@ __global__ void __launch_bounds__(256) k_quat_residual(LevelArgs a, const T* __r
 // ------------------------------------------------------------ launchers
 static dim3 tblock() { return dim3(TX, TY); }
 static dim3 cblock() { return dim3(TX / 2, TY); }
+
+// Calls f(ndim, ql) with both as std::integral_constant: ql = ncomp for one,
+// two or four components (all loads of a cell issued together), else 0.
+template <class F>
+static void quat_dispatch(int ndim, int ncomp, F&& f)
+{
+   using I2 = std::integral_constant<int, 2>;
+   using I3 = std::integral_constant<int, 3>;
+   auto by_ql = [&](auto nd) {
+      switch (ncomp) {
+      case 1: f(nd, std::integral_constant<int, 1>()); break;
+      case 2: f(nd, std::integral_constant<int, 2>()); break;
+      case 4: f(nd, std::integral_constant<int, 4>()); break;
+      default: f(nd, std::integral_constant<int, 0>()); break;
+      }
+   };
+   if (ndim == 3)
+      by_ql(I3());
+   else
+      by_ql(I2());
+}
 
 // 1/h and 1/h^2 in the operator's precision, computed once per launch on the
 // host (IEEE division: the device's value).
@@ -574,12 +687,12 @@ int launch_quat_rbgs(const Level& L, const T* fc, const T* sm, double gamma, con
    T hi[3];
    recip2(dx, hi);
    KernelTimer::Span span(KernelTimer::QUAT_RBGS, s, quat_rbgs_bytes(L, ncomp, sizeof(T)));
-   if (L.ndim == 3)
-      hipLaunchKernelGGL((k_quat_rbgs_c<T, 3>), dim3(grid), cblock(), 0, s, la, fc, sm, (T)gamma,
-                         hi[0], hi[1], hi[2], ncomp, rhs, z, color);
-   else
-      hipLaunchKernelGGL((k_quat_rbgs_c<T, 2>), dim3(grid), cblock(), 0, s, la, fc, sm, (T)gamma,
-                         hi[0], hi[1], hi[2], ncomp, rhs, z, color);
+   auto go = [&](auto ndim, auto ql) {
+      hipLaunchKernelGGL((k_quat_rbgs_c<T, decltype(ndim)::value, decltype(ql)::value>),
+                         dim3(grid), cblock(), 0, s, la, fc, sm, (T)gamma, hi[0], hi[1], hi[2],
+                         ncomp, rhs, z, color);
+   };
+   quat_dispatch(L.ndim, ncomp, go);
    AMPE_HIP_CHECK(hipGetLastError());
    return 0;
 }
@@ -630,12 +743,12 @@ int launch_quat_residual(const Level& L, const T* fc, const T* sm, double gamma,
    shape(L, part != nullptr, &la, &grid);
    T hi[3];
    recip2(dx, hi);
-   if (L.ndim == 3)
-      hipLaunchKernelGGL((k_quat_residual<T, 3>), dim3(grid), tblock(), 0, s, la, fc, sm,
-                         (T)gamma, hi[0], hi[1], hi[2], ncomp, rhs, z, res, part);
-   else
-      hipLaunchKernelGGL((k_quat_residual<T, 2>), dim3(grid), tblock(), 0, s, la, fc, sm,
-                         (T)gamma, hi[0], hi[1], hi[2], ncomp, rhs, z, res, part);
+   auto go = [&](auto ndim, auto ql) {
+      hipLaunchKernelGGL((k_quat_residual<T, decltype(ndim)::value, decltype(ql)::value>),
+                         dim3(grid), tblock(), 0, s, la, fc, sm, (T)gamma, hi[0], hi[1], hi[2],
+                         ncomp, rhs, z, res, part);
+   };
+   quat_dispatch(L.ndim, ncomp, go);
    AMPE_HIP_CHECK(hipGetLastError());
    return 0;
 }

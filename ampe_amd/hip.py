@@ -34,6 +34,7 @@ _SIGS = {
     "ampe_level_destroy": (_i, [_vp]),
     "ampe_level_sizes": (_i, [_vp, C.POINTER(_ll), C.POINTER(_ll), C.POINTER(_ll)]),
     "ampe_level_patch_offsets": (_i, [_vp, C.POINTER(_ll), C.POINTER(_ll)]),
+    "ampe_level_patch_strides": (_i, [_vp, C.POINTER(_ll), C.POINTER(_ll)]),
     "ampe_ghost_fill": (_i, [_vp, _vp, _i, _vp]),
     "ampe_quat_grad": (_i, [_vp, C.POINTER(ModelParams), _vp, _vp, _vp, _vp]),
     "ampe_phase_rhs": (_i, [_vp, C.POINTER(ModelParams), _vp, _vp, _vp, _vp, _vp, _vp]),
@@ -149,6 +150,21 @@ class Level:
         check(lib().ampe_level_patch_offsets(h, co, so), "ampe_level_patch_offsets")
         self.cell_off = list(co)
         self.side_off = [list(so[3 * p:3 * p + 3]) for p in range(np_)]
+        sy, sz = (_ll * np_)(), (_ll * np_)()
+        check(lib().ampe_level_patch_strides(h, sy, sz), "ampe_level_patch_strides")
+        self.sy, self.sz = list(sy), list(sz)
+        self.ghost = ghost
+
+    def ghost_view(self, buf, p, ghost=None):
+        """View of patch p's ghosted block in the flat array buf (one
+        component): shape ([nz + 2g,] ny + 2g, nx + 2g), row padding dropped."""
+        g = self.ghost if ghost is None else ghost
+        gz = g if self.ndim == 3 else 0
+        n = self._patch_ext(self.boxes[p])
+        rows = (n[2] + 2 * gz) * (n[1] + 2 * g)
+        blk = buf[self.cell_off[p]:self.cell_off[p] + rows * self.sy[p]]
+        blk = blk.reshape(n[2] + 2 * gz, n[1] + 2 * g, self.sy[p])[:, :, :n[0] + 2 * g]
+        return blk if self.ndim == 3 else blk[0]
 
     def _patch_ext(self, b):
         n = [b.hi[a] - b.lo[a] + 1 for a in range(3)]
@@ -166,11 +182,11 @@ class Level:
         g, gz = ghost, (ghost if self.ndim == 3 else 0)
         for p, b in enumerate(self.boxes):
             n = self._patch_ext(b)
-            blk = np.zeros((n[2] + 2 * gz, n[1] + 2 * g, n[0] + 2 * g))
             x0, y0, z0 = (b.lo[d] - org[d] for d in range(3))
+            blk = self.ghost_view(out, p, ghost)
+            blk = blk if self.ndim == 3 else blk[None]
             blk[gz:gz + n[2], g:g + n[1], g:g + n[0]] = \
                 a3[z0:z0 + n[2], y0:y0 + n[1], x0:x0 + n[0]]
-            out[self.cell_off[p]:self.cell_off[p] + blk.size] = blk.ravel()
         return out
 
     def from_interior(self, v):

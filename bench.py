@@ -16,7 +16,7 @@ within 1e-6 relative L2 per field of the plain FP64 form on this workload
 the mixed forms differ from FP64 with the same solver by ~1e-13).
 
 Roofline: the headline form's dominant kernel, the FP32 orientation red-black
-sweep k_quat_rbgs<float,3> (all MG levels), is event-timed on the integrator's
+sweep k_quat_rbgs_c<float,3,4> (all MG levels), is event-timed on the integrator's
 stream during the timed steps; achieved = its algorithmic bytes / its time.
 
 Forms:
@@ -49,7 +49,7 @@ import numpy as np  # noqa: E402
 from ampe_amd import params as P  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
-# HBM bytes per launch of k_quat_rbgs<float,3>, mean over the three
+# HBM bytes per launch of k_quat_rbgs_c<float,3,4>, mean over the three
 # multigrid levels of the headline form: rocprofv3 --pmc FETCH_SIZE and
 # WRITE_SIZE per dispatch, scaled by a 1 GiB device copy (2.0 x FETCH_SIZE +
 # 1.0 x WRITE_SIZE); profiles/quat_rbgs_f32_polycrystal256_traffic.txt.
@@ -384,7 +384,7 @@ def main():
     # sweep, event-timed on the integrator's stream over the timed steps
     kt = H["ktime"]
     quat_gbs = kt["bytes"] / (kt["ms"] * 1e-3) / 1e9
-    roof_quat = dict(kernel="k_quat_rbgs<float,3>", launches=kt["launches"],
+    roof_quat = dict(kernel="k_quat_rbgs_c<float,3,4>", launches=kt["launches"],
                      avg_ms=kt["ms"] / kt["launches"],
                      bytes_per_launch=kt["bytes"] / kt["launches"], achieved=quat_gbs)
     log(f"[bench] roofline {roof_quat}")

@@ -198,6 +198,11 @@ int ampe_level_sizes(const ampe_level* level, long long* ncell_ghost,
 /* Host copy of per-patch offsets: cell_off[npatch], side_off[3*npatch]. */
 int ampe_level_patch_offsets(const ampe_level* level, long long* cell_off,
                              long long* side_off);
+/* Strides of each patch's ghosted cell block: cell (i, j, k), interior
+ * indices from 0, ghost width g (gz = g in 3-D, 0 in 2-D), sits at
+ * cell_off + (i + g) + (j + g) * sy + (k + gz) * sz.  sy >= nx + 2g (rows may
+ * be padded); sy[npatch], sz[npatch]. */
+int ampe_level_patch_strides(const ampe_level* level, long long* sy, long long* sz);
 /* Fill ghost cells of ncomp component blocks from neighbours / periodic
  * images (SAMRAI RefineSchedule::fillData on one level). */
 int ampe_ghost_fill(const ampe_level* level, double* field, int ncomp,
