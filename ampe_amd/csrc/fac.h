@@ -58,6 +58,9 @@ int vec_bdf2_hist(long long n, double c1, const double* y, double c2, const doub
                   double* out, hipStream_t s);
 int vec_to_f32(long long n, const double* x, float* y, hipStream_t s);
 int vec_axpby_f(long long n, float* u, const float* e, hipStream_t s);  // u += e
+// z[0, n) = 0 by a kernel (usable under HIP graph capture)
+template <typename T>
+int fill_zero(T* z, long long n, hipStream_t s);
 int vec_to_f64(long long n, const float* x, double* y, hipStream_t s);
 int vec_div_f(long long n, const double* x, const double* sm, double* y, hipStream_t s);
 int vec_sqrt(long long n, const double* x, double* y, hipStream_t s);

@@ -398,6 +398,10 @@ hipEvent_t timer_event()
 KernelTimer::Span::Span(int id, hipStream_t s, double bytes)
 {
    if (id < 0 || id >= COUNT || !g_timer_on[id]) return;
+   // a launch recorded into a HIP graph is not timed (its events would never
+   // complete as such)
+   hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+   if (hipStreamIsCapturing(s, &cs) != hipSuccess || cs != hipStreamCaptureStatusNone) return;
    TimerRec r{id, timer_event(), timer_event(), bytes};
    if (!r.a || !r.b || hipEventRecord(r.a, s) != hipSuccess) {
       if (r.a) g_timer_pool.push_back(r.a);

@@ -382,7 +382,7 @@ int mg_vcycle_ell(Mg<T>& m, const EllOp<T>& op0, int kind, const MgCycle& cy, co
       if (launch_residual<T>(*c.L, op, c.dx, b, x, c.r, nullptr, s)) return -1;
       MgLevel<T>& n = m.lv[l + 1];
       if (mg_restrict_cell<T>(*n.L, *c.L, 1, c.r, n.b, s)) return -1;
-      AMPE_HIP_CHECK(hipMemsetAsync(n.x, 0, sizeof(T) * n.L->ncg, s));
+      if (fill_zero<T>(n.x, n.L->ncg, s)) return -1;
    }
    // ascend
    for (int l = nl - 2; l >= 0; l--) {
@@ -429,7 +429,7 @@ int mg_vcycle_quat(Mg<T>& m, const T* fc0, const T* sm0, double gamma, int ncomp
             return -1;
          MgLevel<T>& n = m.lv[l + 1];
          if (mg_restrict_cell<T>(*n.L, *c.L, ncomp, c.r, n.b, s)) return -1;
-         AMPE_HIP_CHECK(hipMemsetAsync(cur[l + 1], 0, sizeof(T) * n.L->ncg * ncomp, s));
+         if (fill_zero<T>(cur[l + 1], n.L->ncg * ncomp, s)) return -1;
       }
       for (int l = nl - 2; l >= 0; l--) {
          MgLevel<T>& c = m.lv[l];
@@ -463,7 +463,7 @@ int mg_vcycle_quat(Mg<T>& m, const T* fc0, const T* sm0, double gamma, int ncomp
          return -1;
       MgLevel<T>& n = m.lv[l + 1];
       if (mg_restrict_cell<T>(*n.L, *c.L, ncomp, c.r, n.b, s)) return -1;
-      AMPE_HIP_CHECK(hipMemsetAsync(n.x, 0, sizeof(T) * n.L->ncg * ncomp, s));
+      if (fill_zero<T>(n.x, n.L->ncg * ncomp, s)) return -1;
    }
    for (int l = nl - 2; l >= 0; l--) {
       MgLevel<T>& c = m.lv[l];

@@ -223,19 +223,44 @@ __global__ void __launch_bounds__(256) k_mlp_grad(MlpDev m, long long n, long lo
    if (lane == 0) row[ncol - 1] = lsum;
 }
 
-// Column sums of nrows partial rows in row order (FP64): grad[j] for
-// j < ncol - 1; the last column gives the loss (times inv_n).
-__global__ void __launch_bounds__(256) k_mlp_reduce(const float* __restrict__ partial, int nrows,
-                                                    int ncol, double inv_n,
-                                                    float* __restrict__ grad,
-                                                    double* __restrict__ loss)
+// Column sums of nrows partial rows (FP64): grad[j] for j < ncol - 1; the
+// last column gives the loss (times inv_n).  A workgroup owns 64 columns
+// (one per lane, so every row read is one coalesced line) and its kRedWaves
+// waves split the rows round robin, eight independent loads in flight per
+// lane; the waves' sums are then added in wave order, so the result does not
+// depend on timing.
+constexpr int kRedWaves = 16;
+
+__global__ void __launch_bounds__(64 * kRedWaves) k_mlp_reduce(const float* __restrict__ partial,
+                                                              int nrows, int ncol, double inv_n,
+                                                              float* __restrict__ grad,
+                                                              double* __restrict__ loss)
 {
-   int j = blockIdx.x * blockDim.x + threadIdx.x;
-   if (j >= ncol) return;
+   __shared__ double sh[kRedWaves][64];
+   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+   const int j = blockIdx.x * 64 + lane;
    double s = 0.0;
-   for (int i = 0; i < nrows; i++) s += (double)partial[(long long)i * ncol + j];
-   if (j < ncol - 1) grad[j] = (float)s;
-   else *loss = s * inv_n;
+   if (j < ncol) {
+      int i = w;
+      for (; i + 7 * kRedWaves < nrows; i += 8 * kRedWaves) {
+         float v[8];
+#pragma unroll
+         for (int u = 0; u < 8; u++) v[u] = partial[(long long)(i + u * kRedWaves) * ncol + j];
+#pragma unroll
+         for (int u = 0; u < 8; u++) s += (double)v[u];
+      }
+      for (; i < nrows; i += kRedWaves) s += (double)partial[(long long)i * ncol + j];
+   }
+   sh[w][lane] = s;
+   __syncthreads();
+   if (w == 0 && j < ncol) {
+      double t = 0.0;
+      for (int k = 0; k < kRedWaves; k++) t += sh[k][lane];
+      if (j < ncol - 1)
+         grad[j] = (float)t;
+      else
+         *loss = t * inv_n;
+   }
 }
 
 // Adam (Kingma & Ba 2015) with bias corrections bc1 = 1 - beta1^t,
@@ -404,8 +429,8 @@ static int grad_launch(const Surrogate* S, const GradPlan& g, long long n, long 
                       S->dev(), n, base, ld, g.tpw, x, t, (float)(1.0 / (double)n), partial,
                       g.ncol);
    AMPE_HIP_CHECK(hipGetLastError());
-   hipLaunchKernelGGL(k_mlp_reduce, dim3((g.ncol + 255) / 256), dim3(256), 0, st, partial,
-                      (int)g.waves, g.ncol, 1.0 / (double)n, grad, dloss);
+   hipLaunchKernelGGL(k_mlp_reduce, dim3((g.ncol + 63) / 64), dim3(64 * kRedWaves), 0, st,
+                      partial, (int)g.waves, g.ncol, 1.0 / (double)n, grad, dloss);
    AMPE_HIP_CHECK(hipGetLastError());
    return 0;
 }

@@ -46,6 +46,10 @@ struct Sim {
    Mg<float>* mgf = nullptr;      // multigrid hierarchy (precond = MG, FP32)
    float last_ms = 0.f;
    ampe_stats st{};
+   // captured FP32 preconditioner solve (graph_psolve) and the gamma its
+   // kernel arguments hold
+   hipGraphExec_t pgraph = nullptr;
+   double pgraph_gamma = 0.0;
 };
 
 int sim_create(const ampe_model_params& p, const ampe_solver_params& s, const ampe_box* boxes,

@@ -146,6 +146,7 @@ void sim_destroy(Sim* S)
    (void)hipFree(S->guess.count);
    mg_destroy(S->mgf);
    mg_destroy(S->mgd);
+   if (S->pgraph) (void)hipGraphExecDestroy(S->pgraph);
    if (S->stream) (void)hipStreamDestroy(S->stream);
    if (S->ev0) (void)hipEventDestroy(S->ev0);
    if (S->ev1) (void)hipEventDestroy(S->ev1);
@@ -497,7 +498,7 @@ static int run_blocks(Sim* S, std::vector<PBlock<T>>& blocks, double tol)
       if (active.empty()) break;
       for (size_t a = 0; a < active.size(); a++) {
          PBlock<T>& B = blocks[active[a]];
-         AMPE_HIP_CHECK(hipMemsetAsync(B.e, 0, sizeof(T) * S->ng * B.ncomp, st));
+         if (fill_zero<T>(B.e, S->ng * B.ncomp, st)) return -1;
          if (block_smooth<T>(S, B, B.r, B.e)) return -1;
          long long n = S->ng * B.ncomp;
          if (std::is_same<T, double>::value) {
@@ -564,6 +565,69 @@ static std::vector<PBlock<T>> make_blocks(Sim* S, double dphi, const T* Cphi, co
    return bl;
 }
 
+// f_u = P^-1 f_f on the FP32 buffers (f_u zeroed first when zero_u).
+static int psolve_f32(Sim* S, double delta, bool zero_u)
+{
+   long long ng = S->ng;
+   if (zero_u && fill_zero<float>(S->f_u, S->nv, S->stream)) return -1;
+   double dphi = -S->gamma_setup * S->p.epsilon_phi * S->p.epsilon_phi;
+   auto bl = make_blocks<float>(S, dphi, S->f_Cphi, S->f_Mphi, S->f_Dconc, S->f_fc,
+                                S->f_sqrtm);
+   for (auto& B : bl) {
+      long long off = (long long)B.comp * ng;
+      B.f = S->f_f + off;
+      B.u = S->f_u + off;
+      B.e = S->f_e + off;
+      B.r = S->f_r + off;
+   }
+   return run_blocks<float>(S, bl, delta);
+}
+
+// The solve above from a zero start has a fixed launch sequence on fixed
+// buffers when every block runs one cycle (no residual norms, no host
+// synchronisation: multigrid, or red-black sweeps without the early exit);
+// its only launch arguments that change between solves are
+// gamma's (the orientation operator and the phase block's D = -gamma eps^2).
+static bool psolve_graph_ok(const Sim* S)
+{
+   return S->s.graph_psolve && S->s.precision == AMPE_FAC_FP32 && S->s.fac_max_cycles <= 1 &&
+          S->s.quat_max_cycles <= 1 && (S->mgf || S->s.fac_tol < 0.0);
+}
+
+// psolve_f32(zero_u) captured into a HIP graph (again when gamma changed) and
+// replayed: one launch instead of one per kernel of every V-cycle.  The
+// coefficient buffers are read at replay, so a preconditioner setup at the
+// same gamma needs no new capture.
+static int psolve_f32_graph(Sim* S)
+{
+   hipStream_t st = S->stream;
+   if (S->pgraph && S->pgraph_gamma != S->gamma_setup) {
+      (void)hipGraphExecDestroy(S->pgraph);
+      S->pgraph = nullptr;
+   }
+   if (!S->pgraph) {
+      hipGraph_t g = nullptr;
+      AMPE_HIP_CHECK(hipStreamBeginCapture(st, hipStreamCaptureModeRelaxed));
+      int rc = psolve_f32(S, 0.0, true);
+      hipError_t e = hipStreamEndCapture(st, &g);
+      if (rc < 0 || e != hipSuccess || !g) {
+         if (g) (void)hipGraphDestroy(g);
+         if (rc >= 0) set_error(std::string("psolve graph capture: ") + hipGetErrorString(e));
+         return -1;
+      }
+      e = hipGraphInstantiate(&S->pgraph, g, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(g);
+      if (e != hipSuccess) {
+         S->pgraph = nullptr;
+         set_error(std::string("psolve graph instantiate: ") + hipGetErrorString(e));
+         return -1;
+      }
+      S->pgraph_gamma = S->gamma_setup;
+   }
+   AMPE_HIP_CHECK(hipGraphLaunch(S->pgraph, st));
+   return 0;
+}
+
 // z = P^-1 r.  FP32 preconditioner only: f_ready = the FP32 right-hand side
 // S->f_f was already formed from r (by the Krylov kernel that produced r);
 // keep = the solution is left in S->f_u for the caller (z untouched).
@@ -582,22 +646,14 @@ static int psolve(Sim* S, const double* r, double* z, double delta, PsolveOpts o
    int iq = S->iq >= 0 ? S->iq : S->nc;
    if (S->s.precision == AMPE_FAC_FP32) {
       // f = r (orientation blocks / sqrt(M), FP64 division) in FP32, u = 0
+      int ret;
       if (o.f_ready) {
-         AMPE_HIP_CHECK(hipMemsetAsync(S->f_u, 0, sizeof(float) * S->nv, st));
-      } else if (psolve_in<float>(ng, S->nc, iq, ql, r, S->sqrtm, S->f_f, nullptr, S->f_u,
-                                  st)) {
-         return -1;
+         ret = psolve_graph_ok(S) ? psolve_f32_graph(S) : psolve_f32(S, delta, true);
+      } else {
+         if (psolve_in<float>(ng, S->nc, iq, ql, r, S->sqrtm, S->f_f, nullptr, S->f_u, st))
+            return -1;
+         ret = psolve_f32(S, delta, false);
       }
-      auto bl = make_blocks<float>(S, dphi, S->f_Cphi, S->f_Mphi, S->f_Dconc, S->f_fc,
-                                   S->f_sqrtm);
-      for (auto& B : bl) {
-         long long off = (long long)B.comp * ng;
-         B.f = S->f_f + off;
-         B.u = S->f_u + off;
-         B.e = S->f_e + off;
-         B.r = S->f_r + off;
-      }
-      int ret = run_blocks<float>(S, bl, delta);
       if (ret < 0) return -1;
       if (!o.keep && psolve_out<float>(ng, S->nc, iq, ql, S->f_u, S->sqrtm, nullptr, z, st))
          return -1;

@@ -38,6 +38,26 @@ AMPE_FLAT_KERNEL(k_to_f64, (long long n, const float* __restrict__ x, double* __
                  y[t] = (double)x[t])
 AMPE_FLAT_KERNEL(k_add_f, (long long n, float* __restrict__ u, const float* __restrict__ e),
                  u[t] = u[t] + e[t])
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_zero(long long n, T* __restrict__ z)
+{
+   long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+   if (t < n) z[t] = T(0);
+}
+
+// A kernel, not hipMemsetAsync: the preconditioner solve may be captured
+// into a HIP graph, and its buffers are cleared by kernel nodes there.
+template <typename T>
+int fill_zero(T* z, long long n, hipStream_t s)
+{
+   if (n <= 0) return 0;
+   hipLaunchKernelGGL(k_zero<T>, dim3(nblk(n)), dim3(256), 0, s, n, z);
+   AMPE_HIP_CHECK(hipGetLastError());
+   return 0;
+}
+template int fill_zero<float>(float*, long long, hipStream_t);
+template int fill_zero<double>(double*, long long, hipStream_t);
 AMPE_FLAT_KERNEL(k_sqrt, (long long n, const double* __restrict__ x, double* __restrict__ y),
                  y[t] = sqrt(x[t]))
 

@@ -116,6 +116,7 @@ class SolverParams(C.Structure):
         ("mg_max_levels", C.c_int),
         ("mg_coarse_sweeps", C.c_int),
         ("gstype", C.c_int),
+        ("graph_psolve", C.c_int),
     ]
 
 
@@ -249,7 +250,7 @@ def solver_defaults(**kw):
                      fac_tol=-1.0, quat_max_cycles=1, quat_sweeps=2,
                      precision=FP64, kks_guess=GUESS_PREVIOUS,
                      precond=PRECOND_REDBLACK, mg_max_levels=0, mg_coarse_sweeps=16,
-                     gstype=GS_MODIFIED)
+                     gstype=GS_MODIFIED, graph_psolve=0)
     for k, v in kw.items():
         setattr(s, k, v)
     return s

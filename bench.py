@@ -104,9 +104,10 @@ def solver(form, args):
     elif form == "fp64_mg":
         kw.update(precision=P.FP64, **mg)
     elif form == "mixed":
-        kw.update(precision=P.FAC_FP32, **mg)
+        kw.update(precision=P.FAC_FP32, graph_psolve=args.graphs, **mg)
     elif form == "mixed_surrogate":
-        kw.update(precision=P.FAC_FP32, kks_guess=P.GUESS_SURROGATE, **mg)
+        kw.update(precision=P.FAC_FP32, kks_guess=P.GUESS_SURROGATE, graph_psolve=args.graphs,
+                  **mg)
     else:
         raise ValueError(form)
     return P.solver_defaults(**kw)
@@ -339,6 +340,8 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-surrogate", action="store_true")
+    ap.add_argument("--graphs", type=int, default=0,
+                    help="1: FP32 preconditioner solves replayed from a captured HIP graph")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU only (this build has no multi-GPU path); "

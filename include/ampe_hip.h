@@ -167,6 +167,12 @@ typedef struct ampe_solver_params {
    int mg_max_levels;     /* multigrid levels including the fine one (<= 0: all) */
    int mg_coarse_sweeps;  /* red-black sweeps on the coarsest multigrid level */
    int gstype;            /* enum ampe_gstype (classical: maxl <= 14) */
+   int graph_psolve;      /* 1: the FP32 preconditioner solve of the Krylov loop (the
+                           * multigrid V-cycles or red-black sweeps of every block) is
+                           * captured once per gamma into a HIP graph and replayed with
+                           * one launch per solve -- the role of a captured V-cycle; needs
+                           * precision = FP32, gstype = classical and one cycle per
+                           * block (fac_max_cycles = quat_max_cycles = 1), else ignored */
 } ampe_solver_params;
 
 /* Per-run counters (mirror QuatIntegrator.cc:2161-2170 nni/nli/nfe/npe/nps). */
