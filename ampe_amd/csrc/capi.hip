@@ -388,6 +388,13 @@ int ampe_sim_destroy(ampe_sim* sim)
    return 0;
 }
 
+int ampe_sim_set_temperature_ramp(ampe_sim* sim, double T0, double dTdt, double T_target,
+                                  const ampe_calphad_tdep* tab)
+{
+   AMPE_REQUIRE(sim, "null argument");
+   return ampe::sim_set_temperature_ramp(sim->S, T0, dTdt, T_target, tab);
+}
+
 int ampe_sim_set_state(ampe_sim* sim, const double* phi, const double* c, const double* q)
 {
    AMPE_REQUIRE(sim && phi, "null argument");

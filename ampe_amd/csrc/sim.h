@@ -50,6 +50,10 @@ struct Sim {
    // kernel arguments hold
    hipGraphExec_t pgraph = nullptr;
    double pgraph_gamma = 0.0;
+   // scalar temperature ramp (sim_set_temperature_ramp)
+   bool tramp = false, has_tab = false;
+   double T0 = 0.0, dTdt = 0.0, Ttarget = 0.0;
+   ampe_calphad_tdep tab{};
 };
 
 int sim_create(const ampe_model_params& p, const ampe_solver_params& s, const ampe_box* boxes,
@@ -58,6 +62,8 @@ void sim_destroy(Sim* S);
 int sim_set_state(Sim* S, const double* phi, const double* c, const double* q);
 int sim_get_state(Sim* S, double* phi, double* c, double* q, double* cl, double* cs);
 int sim_advance(Sim* S, int nsteps, ampe_stats* stats);
+int sim_set_temperature_ramp(Sim* S, double T0, double dTdt, double T_target,
+                             const ampe_calphad_tdep* tab);
 
 int vec_mul_comp(Sim* S, double* x, const double* sm, int ncomp);
 

@@ -42,6 +42,12 @@ static int falloc(float** p, long long n)
    return e == hipSuccess ? 0 : -1;
 }
 
+// First step: dt_init when 0 < dt_init < dt, else dt.
+static double first_step(const ampe_solver_params& s)
+{
+   return (s.dt_init > 0.0 && s.dt_init < s.dt) ? s.dt_init : s.dt;
+}
+
 int sim_create(const ampe_model_params& p, const ampe_solver_params& s, const ampe_box* boxes,
                int npatch, const ampe_box* periodic, Sim** out)
 {
@@ -123,7 +129,7 @@ int sim_create(const ampe_model_params& p, const ampe_solver_params& s, const am
       sim_destroy(S);
       return -1;
    }
-   S->h = s.dt;
+   S->h = first_step(s);
    *out = S;
    return 0;
 }
@@ -264,7 +270,7 @@ int sim_set_state(Sim* S, const double* phi, const double* c, const double* q)
    S->nst = 0;
    S->nstlp = 0;
    S->have_prev = false;
-   S->h = S->s.dt;
+   S->h = first_step(S->s);
    if (vec_ewt(S->nv, S->y, S->s.rtol, S->s.atol, S->ewt, st)) return -1;
    AMPE_HIP_CHECK(hipStreamSynchronize(st));
    return 0;
@@ -919,6 +925,34 @@ static int newton(Sim* S, double gamma, double delta, bool force_setup, bool* di
    return 1;
 }
 
+// ScalarTemperatureStrategy::getCurrentTemperature
+// (source/ScalarTemperatureStrategy.cc:56-72)
+static double ramp_temperature(const Sim* S, double t)
+{
+   double T = S->T0 + S->dTdt * t;
+   if (S->dTdt < 0.0 && T < S->Ttarget)
+      T = S->Ttarget;
+   else if (S->Ttarget > 0.0 && S->dTdt > 0.0 && T > S->Ttarget)
+      T = S->Ttarget;
+   return T;
+}
+
+int sim_set_temperature_ramp(Sim* S, double T0, double dTdt, double T_target,
+                             const ampe_calphad_tdep* tab)
+{
+   if (!(T0 > 0.0)) {
+      set_error("sim_set_temperature_ramp: T0 must be positive");
+      return -1;
+   }
+   S->tramp = true;
+   S->T0 = T0;
+   S->dTdt = dTdt;
+   S->Ttarget = T_target;
+   S->has_tab = tab != nullptr;
+   if (tab) S->tab = *tab;
+   return 0;
+}
+
 static int bdf_step(Sim* S)
 {
    const Level& L = *S->L;
@@ -937,6 +971,12 @@ static int bdf_step(Sim* S)
    bool force_setup = false;
    for (;;) {
       double h = S->h, gamma;
+      // the temperature field is set at the time of every RHS evaluation
+      // (source/QuatIntegrator.cc:3191); a step attempt evaluates at t + h
+      if (S->tramp && S->dTdt != 0.0) {
+         S->p.temperature = ramp_temperature(S, S->t + h);
+         if (S->has_tab) calphad_at(S->tab, S->p.temperature, S->p.calphad);
+      }
       if (!S->have_prev) {
          AMPE_HIP_CHECK(hipMemcpyAsync(S->ypred, S->y, sizeof(double) * nv,
                                        hipMemcpyDeviceToDevice, st));

@@ -35,6 +35,7 @@ _SIGS = {
     "ampe_level_sizes": (_i, [_vp, C.POINTER(_ll), C.POINTER(_ll), C.POINTER(_ll)]),
     "ampe_level_patch_offsets": (_i, [_vp, C.POINTER(_ll), C.POINTER(_ll)]),
     "ampe_level_patch_strides": (_i, [_vp, C.POINTER(_ll), C.POINTER(_ll)]),
+    "ampe_sim_set_temperature_ramp": (_i, [_vp, _d, _d, _d, _vp]),
     "ampe_ghost_fill": (_i, [_vp, _vp, _i, _vp]),
     "ampe_quat_grad": (_i, [_vp, C.POINTER(ModelParams), _vp, _vp, _vp, _vp]),
     "ampe_phase_rhs": (_i, [_vp, C.POINTER(ModelParams), _vp, _vp, _vp, _vp, _vp, _vp]),
@@ -258,6 +259,15 @@ class Sim:
         check(lib().ampe_sim_get_state(self.h, self._ptr(phi), self._ptr(c), self._ptr(q),
                                        self._ptr(cl), self._ptr(cs)), "ampe_sim_get_state")
         return phi, c, q, cl, cs
+
+    def set_temperature_ramp(self, T0, dTdt, T_target, tab=None):
+        """T(t) = T0 + dTdt t, held at T_target; CALPHAD phases re-evaluated
+        from the CalphadTdep table tab at every step."""
+        self._tab = tab
+        check(lib().ampe_sim_set_temperature_ramp(
+            self.h, T0, dTdt, T_target,
+            C.cast(C.byref(tab), _vp) if tab is not None else None),
+            "ampe_sim_set_temperature_ramp")
 
     def advance(self, nsteps):
         st = Stats()

@@ -1,0 +1,94 @@
+"""Initial conditions as the reference's generator writes them
+(utils/make_nuclei.py): spherical nuclei of phase 1 in a phase-0 matrix,
+interior / exterior compositions with the exterior one set so that the
+nominal composition is met, and single-precision fields (the generator's
+default, 'f' NetCDF variables) that the reference reads back as doubles.
+
+Restated behaviour, with the generator's quirks kept:
+  * centres: one nucleus at --center0 (the z coordinate defaults to 1) or at
+    the box centre; two nuclei at (nx/3, ny/3, nz/2) and (2nx/3, 2ny/3, nz/2)
+    with radii r and 1.5 r (make_nuclei.py:520-560);
+  * distances are plain Euclidean: the periodic-image loops compare the
+    option strings with the integer 1 (make_nuclei.py:238-252) and so never
+    run;
+  * a cell centre (i + 0.5, j + 0.5, k + 0.5) gets phase_in where its
+    distance is below the radius (width 0), or the tanh profile
+    phase_in (1 + tanh(-d / 2w)) / 2 where d < 8 w (make_nuclei.py:626-660);
+    vs accumulates the stored (single-precision) values, vl = nx ny nz - vs;
+  * with a nominal and an interior composition the exterior one is
+    (c vol - c_in vs) / vl, stored in single precision
+    (make_nuclei.py:715-721); c = c_in phi + c_out (1 - phi).
+Random orientations and more than two nuclei are not restated."""
+import numpy as np
+
+
+def make_nuclei(nx, ny, nz=1, radius=None, ngrains=1, phase_in=1.0, phase_out=0.0,
+                nomconc=None, conc_in=None, conc_out=None, width=0.0, center0=None,
+                temperature_in=None, temperature_out=None, double_precision=False):
+    if ngrains not in (1, 2):
+        raise NotImplementedError("make_nuclei: one or two nuclei")
+    dt = np.float64 if double_precision else np.float32
+    if radius is None:
+        radius = nx / 10
+    if ngrains == 1:
+        r = [radius]
+        if center0 is not None:
+            cx, cy = [center0[0]], [center0[1]]
+            cz = [center0[2] if len(center0) > 2 else 1]
+        else:
+            cx, cy = [nx / 2], [ny / 2]
+            cz = [nz / 2 if nz > 1 else 1]
+    else:
+        cx, cy, cz = [nx / 3, 2 * nx / 3], [ny / 3, 2 * ny / 3], [nz / 2, nz / 2]
+        r = [radius, radius * 1.5]
+    phase = np.zeros((nz, ny, nx), dt)
+    if phase_out:
+        phase[...] = phase_out
+    z = np.arange(nz) + 0.5
+    y = np.arange(ny) + 0.5
+    x = np.arange(nx) + 0.5
+    vs = 0.0
+    for g in range(ngrains):
+        thr = (r[g] + 5.0 * width) ** 2
+        for k in range(nz):
+            if (z[k] - cz[g]) ** 2 >= thr:
+                continue
+            for j in range(ny):
+                if (y[j] - cy[g]) ** 2 >= thr:
+                    continue
+                d2 = (x - cx[g]) ** 2 + (y[j] - cy[g]) ** 2 + (z[k] - cz[g]) ** 2
+                d = np.sqrt(d2) - np.sqrt(r[g] ** 2)
+                if width > 0.0:
+                    sel = d < 8.0 * width
+                    phase[k, j, sel] = phase_in * 0.5 * (1.0 + np.tanh(-1.0 * d[sel] / (2.0 * width)))
+                else:
+                    sel = d < 0.0
+                    phase[k, j, sel] = phase_in
+                # vs sums the stored values cell by cell, i ascending
+                for v in phase[k, j, sel]:
+                    vs = vs + float(v)
+    vol = nx * ny * nz
+    vl = vol - vs
+    out = dict(phase=phase.astype(np.float64), vs=vs, vl=vl)
+    if nomconc is not None or conc_in is not None:
+        c = list(nomconc) if nomconc is not None else None
+        ci = list(conc_in) if conc_in is not None else list(c)
+        ns = len(set(c)) if c is not None else len(ci)
+        co = np.zeros(ns, dt) if conc_out is None else np.asarray(conc_out, np.float64)
+        if conc_in is not None and c is not None and vl > 0:
+            for s in range(ns):
+                co[s] = (c[s] * vol - ci[s] * vs) / vl
+        conc = []
+        ph = phase.astype(np.float64)
+        for s in range(ns):
+            conc.append((ci[s] * ph + float(co[s]) * (1.0 - ph)).astype(dt).astype(np.float64))
+        out["conc"] = conc
+        out["conc_out"] = [float(v) for v in co]
+    if temperature_in is not None:
+        T = np.full((nz, ny, nx), temperature_out, dt)
+        for g in range(ngrains):
+            d2 = ((x[None, None, :] - cx[g]) ** 2 + (y[None, :, None] - cy[g]) ** 2 +
+                  (z[:, None, None] - cz[g]) ** 2)
+            T[d2 - r[g] ** 2 < 0.0] = temperature_in
+        out["temperature"] = T.astype(np.float64)
+    return out

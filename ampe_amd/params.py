@@ -117,6 +117,7 @@ class SolverParams(C.Structure):
         ("mg_coarse_sweeps", C.c_int),
         ("gstype", C.c_int),
         ("graph_psolve", C.c_int),
+        ("dt_init", C.c_double),
     ]
 
 
@@ -437,3 +438,58 @@ def polycrystal_state(n=256, seed=1002, ngrains=64, solid_fraction=0.7,
     c = rng.uniform(0.3, 0.5, shape)
     q = quats[:, label]
     return phi, c, q
+
+
+def kks_composition_deck():
+    """The reference's KKScomposition regression (tests/KKScomposition/
+    2d.input, tests/KKScomposition/test2d.py): 2-D 64 x 64 cells on
+    [0, 1.8]^2 um with zero-slope boundaries (periodic_dimension = 0, 0;
+    "slope" 0 conditions), AuNi CALPHAD KKS (calphadAuNi.dat, Redlich-Kister
+    terms of the database), sigma = 0.372678 J/m^2, delta = 0.035 um,
+    M_phi = 6.4, pbg interpolation, arithmetic face averages, D_S = 0.125,
+    D_L = 1224.23 um^2/s, molar volume 7.68e-6 m^3/mol, KKS Newton max_its 50,
+    no orientation field; temperature 1450 K cooled at -200 K/s towards
+    1220 K; end_time 0.3 s, atol 1e-4 (rtol = atol 1e-2), PhaseSysSolver
+    red-black with 5 sweeps and no early exit.  Initial state of test2d.py:
+    make_nuclei.py --nx 64 --ny 64 --nz 1 -r 16 --center0 "0, 0, 0" -c 0.25
+    --concentration-in 0.096.  The test passes when the integral of c stays
+    within 1e-4 of its initial value and the solid volume fraction at the
+    end time is 0.32 +- 0.01.
+
+    Returns (model params, ramp dict, initial state, checks dict)."""
+    from . import initial
+    sigma, delta = 0.372678, 0.035
+    n = 64
+    p = two_grain_params(ndim=2)
+    for d in range(3):
+        p.dx[d] = 1.8 / n
+    p.temperature = 1450.0
+    p.epsilon_phi = float(np.sqrt(6.0 * sigma * delta))
+    p.phi_well_scale = (3.0 * sigma / delta) / 16.0
+    p.phi_mobility = 6.4
+    p.energy_interp = b"p"
+    p.conc_interp = b"p"
+    p.avg_type = b"a"
+    p.evolve_quat = 0
+    p.fe_model = FE_CALPHAD_BINARY
+    p.vm_L = p.vm_A = 7.68e-6
+    p.calphad = auni_phases(1450.0)
+    p.D_liquid, p.Q0_liquid = 1224.23, 0.0
+    p.D_solid, p.Q0_solid = 0.125, 0.0
+    p.newton_max_its = 50
+    ramp = dict(T0=1450.0, dTdt=-200.0, T_target=1220.0, tab=auni_tdep())
+    ic = initial.make_nuclei(nx=n, ny=n, nz=1, radius=16, center0=(0.0, 0.0, 0.0),
+                             nomconc=[0.25], conc_in=[0.096])
+    state = (ic["phase"][0], ic["conc"][0][0], None)
+    checks = dict(end_time=0.3, conc_integral_tol=1.0e-4, solid_fraction=0.32,
+                  solid_fraction_tol=1.0e-2, max_cycles=900)
+    return p, ramp, state, checks
+
+
+def kks_composition_solver(dt=1.0e-3, **kw):
+    """Solver settings of the KKScomposition deck on the fixed-step BDF
+    integrator (the reference's CVODE chooses its own steps up to end_time)."""
+    base = dict(dt=dt, dt_init=1.0e-6, atol=1.0e-4, rtol=1.0e-6, maxl=10, max_newton=5, fac_sweeps=5,
+                fac_max_cycles=1, fac_tol=-1.0)
+    base.update(kw)
+    return solver_defaults(**base)

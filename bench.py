@@ -97,7 +97,8 @@ FORMS = ("fp64_reference", "fp64_mg", "mixed", "mixed_surrogate")
 
 def solver(form, args):
     kw = dict(dt=args.dt, maxl=args.maxl, max_newton=5, atol=args.atol, rtol=args.atol * 1e-2)
-    mg = dict(precond=P.PRECOND_MG, fac_sweeps=1, quat_sweeps=1, mg_max_levels=args.mg_levels,
+    mg = dict(precond=P.PRECOND_MG, fac_sweeps=args.mg_sweeps, quat_sweeps=args.mg_sweeps,
+              mg_max_levels=args.mg_levels,
               mg_coarse_sweeps=args.mg_coarse_sweeps, gstype=P.GS_CLASSICAL)
     if form == "fp64_reference":
         kw.update(precision=P.FP64, fac_sweeps=2, quat_sweeps=2)
@@ -335,6 +336,7 @@ def main():
     ap.add_argument("--maxl", type=int, default=10)
     ap.add_argument("--mg-levels", type=int, default=3)
     ap.add_argument("--mg-coarse-sweeps", type=int, default=2)
+    ap.add_argument("--mg-sweeps", type=int, default=1, help="pre- and post-smoothing sweeps")
     ap.add_argument("--forms", default=",".join(FORMS))
     ap.add_argument("--cpu-n", type=int, default=32)
     ap.add_argument("--cpu-steps", type=int, default=2)

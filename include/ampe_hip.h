@@ -173,6 +173,11 @@ typedef struct ampe_solver_params {
                            * one launch per solve -- the role of a captured V-cycle; needs
                            * precision = FP32, gstype = classical and one cycle per
                            * block (fac_max_cycles = quat_max_cycles = 1), else ignored */
+   double dt_init;        /* first step (0 < dt_init < dt; else dt); the step doubles
+                           * after every accepted step up to dt, as after a cut --
+                           * the small automatic first step CVODE takes when the
+                           * initial step size is 0 (QuatIntegrator's default
+                           * d_previous_timestep, source/QuatIntegrator.cc:135,1575) */
 } ampe_solver_params;
 
 /* Per-run counters (mirror QuatIntegrator.cc:2161-2170 nni/nli/nfe/npe/nps). */
@@ -532,6 +537,17 @@ int ampe_sim_get_state(ampe_sim* sim, double* phi, double* c, double* q,
                        double* cl, double* cs);
 /* Advance nsteps accepted BDF steps; >0 if a Newton solve failed. */
 int ampe_sim_advance(ampe_sim* sim, int nsteps, ampe_stats* stats);
+/* Scalar temperature T(t) = T0 + dTdt t, held at T_target once reached
+ * (ScalarTemperatureStrategy::getCurrentTemperature,
+ * source/ScalarTemperatureStrategy.cc:56-72; Temperature { temperature,
+ * dtemperaturedt, target_temperature } of the input deck), set at the time of
+ * every step attempt as evaluateRHSFunction sets the temperature field at the
+ * evaluation time (source/QuatIntegrator.cc:3191).  With tab (not NULL) the
+ * CALPHAD phases are re-evaluated at that temperature
+ * (ampe_calphad_at).  The model's own temperature applies before the first
+ * step. */
+int ampe_sim_set_temperature_ramp(ampe_sim* sim, double T0, double dTdt, double T_target,
+                                  const ampe_calphad_tdep* tab);
 /* Device time (ms) of the last ampe_sim_advance, from HIP events. */
 int ampe_sim_last_time_ms(const ampe_sim* sim, double* ms);
 /* Surrogate used when the solver's kks_guess is AMPE_GUESS_SURROGATE (the

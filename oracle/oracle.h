@@ -33,6 +33,7 @@ typedef struct OGeom {
    long sy, sz, ncg;       /* strides and size of a ghosted cell array */
    long nfx, nfy, nfz;     /* face counts per direction */
    double dx[3];
+   int periodic;           /* 1: periodic box; 0: zero-slope ghosts (clamped) */
 } OGeom;
 
 void o_geom_init(OGeom* G, int ndim, int nx, int ny, int nz, int g,
@@ -63,6 +64,10 @@ void o_ebs_coeffs(const OGeom* G, const ampe_model_params* p, const double* phi,
 void o_conc_rhs_ebs(const OGeom* G, const ampe_model_params* p, const double* cl,
                     const double* cs, const double* DL, const double* DA, double* rhs);
 void o_ghost_fill_periodic(const OGeom* G, double* f, int ncomp);
+/* ghosts from the periodic image (G->periodic) or from the nearest interior
+ * cell, each index clamped: the zero-slope boundary of the reference's
+ * "slope 0" conditions, as the device level's physical boundaries */
+void o_ghost_fill(const OGeom* G, double* f, int ncomp);
 
 void o_phase_flux(const OGeom* G, const ampe_model_params* p,
                   const double* phi, const double* q, double* flux);
@@ -122,6 +127,17 @@ void o_sim_set_state(OSim* S, const double* phi, const double* c,
 void o_sim_get_state(OSim* S, double* phi, double* c, double* q, double* cl,
                      double* cs);
 int o_sim_advance(OSim* S, int nsteps, ampe_stats* st);
+/* physical (zero-slope) boundaries instead of the periodic box; call before
+ * o_sim_set_state */
+void o_sim_set_periodic(OSim* S, int periodic);
+/* Scalar temperature T(t) = T0 + dTdt t, held at T_target once reached
+ * (ScalarTemperatureStrategy::getCurrentTemperature,
+ * source/ScalarTemperatureStrategy.cc), applied at the time of every step
+ * attempt (evaluateRHSFunction sets the field at the evaluation time,
+ * source/QuatIntegrator.cc:3191); with tab, the CALPHAD phases are
+ * re-evaluated at that temperature. */
+void o_sim_set_temperature_ramp(OSim* S, double T0, double dTdt, double T_target,
+                                const ampe_calphad_tdep* tab);
 
 
 /* surrogate path (oracle_surrogate.c) */

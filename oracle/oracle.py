@@ -35,7 +35,7 @@ class OGeom(C.Structure):
                 ("nz", C.c_int), ("g", C.c_int), ("gz", C.c_int),
                 ("sy", C.c_long), ("sz", C.c_long), ("ncg", C.c_long),
                 ("nfx", C.c_long), ("nfy", C.c_long), ("nfz", C.c_long),
-                ("dx", C.c_double * 3)]
+                ("dx", C.c_double * 3), ("periodic", C.c_int)]
 
 
 class EllipticOp(C.Structure):
@@ -68,6 +68,7 @@ def _setup(L):
         "o_d2fenergy": (C.c_double, [_P, C.c_double, C.c_int]),
         "o_kks_cell": (C.c_int, [_P, C.c_double, C.c_double, _dp, _dp]),
         "o_ghost_fill_periodic": (None, [_G, _dp, C.c_int]),
+        "o_ghost_fill": (None, [_G, _dp, C.c_int]),
         "o_phase_flux": (None, [_G, _P, _dp, _dp, _dp]),
         "o_quat_grad_modulus": (None, [_G, _P, _dp, _dp]),
         "o_quat_face_norm2": (None, [_G, _P, _dp, _dp]),
@@ -95,6 +96,9 @@ def _setup(L):
         "o_sim_set_state": (None, [C.c_void_p, _dp, _dp, _dp]),
         "o_sim_get_state": (None, [C.c_void_p, _dp, _dp, _dp, _dp, _dp]),
         "o_sim_advance": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(Stats)]),
+        "o_sim_set_periodic": (None, [C.c_void_p, C.c_int]),
+        "o_sim_set_temperature_ramp": (None, [C.c_void_p, C.c_double, C.c_double, C.c_double,
+                                              C.c_void_p]),
         "o_bf16": (C.c_float, [C.c_float]),
         "o_calphad_gfun": (C.c_double, [C.POINTER(CalphadGfun), C.c_double]),
         "o_calphad_at": (None, [C.POINTER(CalphadTdep), C.c_double, C.POINTER(CalphadPhase)]),
@@ -171,11 +175,20 @@ class Patch:
 class Sim:
     """Oracle integrator handle."""
 
-    def __init__(self, p, s, n):
+    def __init__(self, p, s, n, periodic=True):
         nx, ny, nz = (n if isinstance(n, (tuple, list)) else (n,) * 3)
         self.p, self.s = p, s
         self.h = lib().o_sim_create(C.byref(p), C.byref(s), nx, ny, nz)
         self.shape = (ny, nx) if p.ndim == 2 else (nz, ny, nx)
+        if not periodic:
+            lib().o_sim_set_periodic(self.h, 0)
+
+    def set_temperature_ramp(self, T0, dTdt, T_target, tab=None):
+        """T(t) = T0 + dTdt t held at T_target; CALPHAD phases from tab."""
+        self._tab = tab
+        lib().o_sim_set_temperature_ramp(self.h, T0, dTdt, T_target,
+                                         C.cast(C.byref(tab), C.c_void_p) if tab is not None
+                                         else None)
 
     def __del__(self):
         if getattr(self, "h", None):

@@ -1,6 +1,7 @@
 /*
  * Oracle integrator: FP64 CPU restatement of QuatIntegrator's CVODE loop on
- * one periodic patch.  TEST INFRASTRUCTURE ONLY (see oracle.h).
+ * one patch, periodic or with zero-slope boundaries.  TEST INFRASTRUCTURE
+ * ONLY (see oracle.h).
  *
  * Follows:
  *   - RHS callback QuatIntegrator::evaluateRHSFunction
@@ -62,9 +63,44 @@ struct OSim {
    double* H;
    ampe_stats st;
    int fail;
+   /* scalar temperature ramp (o_sim_set_temperature_ramp) */
+   int tramp, has_tab;
+   double T0, dTdt, Ttarget;
+   ampe_calphad_tdep tab;
 };
 
+void o_sim_set_periodic(OSim* S, int periodic) { S->G.periodic = periodic ? 1 : 0; }
+
+void o_sim_set_temperature_ramp(OSim* S, double T0, double dTdt, double T_target,
+                                const ampe_calphad_tdep* tab)
+{
+   S->tramp = 1;
+   S->T0 = T0;
+   S->dTdt = dTdt;
+   S->Ttarget = T_target;
+   S->has_tab = tab != NULL;
+   if (tab) S->tab = *tab;
+}
+
+/* ScalarTemperatureStrategy::getCurrentTemperature
+ * (source/ScalarTemperatureStrategy.cc:56-72) */
+static double ramp_temperature(const OSim* S, double t)
+{
+   double T = S->T0 + S->dTdt * t;
+   if (S->dTdt < 0.0 && T < S->Ttarget)
+      T = S->Ttarget;
+   else if (S->Ttarget > 0.0 && S->dTdt > 0.0 && T > S->Ttarget)
+      T = S->Ttarget;
+   return T;
+}
+
 static double* vec(long n) { return (double*)calloc((size_t)n, sizeof(double)); }
+
+/* first step: dt_init when 0 < dt_init < dt, else dt */
+static double first_step(const ampe_solver_params* s)
+{
+   return (s->dt_init > 0.0 && s->dt_init < s->dt) ? s->dt_init : s->dt;
+}
 
 OSim* o_sim_create(const ampe_model_params* p, const ampe_solver_params* s,
                    int nx, int ny, int nz)
@@ -103,7 +139,7 @@ OSim* o_sim_create(const ampe_model_params* p, const ampe_solver_params* s,
    S->fc = vec(S->nf);
    S->fac_flux = vec(S->nf);
    S->H = vec((long)(s->maxl + 1) * (s->maxl + 1));
-   S->h = s->dt;
+   S->h = first_step(s);
    return S;
 }
 
@@ -157,7 +193,7 @@ void o_sim_set_state(OSim* S, const double* phi, const double* c,
       for (int m = 0; m < S->p.qlen; m++)
          load_field(G, S->y + (long)(S->iq + m) * G->ncg,
                     q + (long)m * G->nx * G->ny * G->nz);
-   o_ghost_fill_periodic(G, S->y, S->nc);
+   o_ghost_fill(G, S->y, S->nc);
    if (S->ic >= 0) {
       /* initial phase concentrations: KKS from the guess cL = cS = c */
       const double* cc = S->y + (long)S->ic * G->ncg;
@@ -168,6 +204,7 @@ void o_sim_set_state(OSim* S, const double* phi, const double* c,
    S->t = 0.0;
    S->nst = 0;
    S->have_prev = 0;
+   S->h = first_step(&S->s);
    memset(&S->st, 0, sizeof(S->st));
    update_ewt(S, S->y);
 }
@@ -195,7 +232,7 @@ static int rhs_eval(OSim* S, double* y, double* ydot, int fd)
    long ng = G->ncg;
    int recompute = (!fd) || (!S->s.lag_quat_sidegrad);
    S->st.nfe++;
-   o_ghost_fill_periodic(G, y, S->nc);
+   o_ghost_fill(G, y, S->nc);
    const double* phi = y;
    const double* q = (S->iq >= 0) ? y + (long)S->iq * ng : NULL;
    const double* c = (S->ic >= 0) ? y + (long)S->ic * ng : NULL;
@@ -229,7 +266,7 @@ static int psetup(OSim* S, double* y, double gamma)
    const ampe_model_params* p = &S->p;
    long ng = G->ncg;
    S->st.npe++;
-   o_ghost_fill_periodic(G, y, S->nc);
+   o_ghost_fill(G, y, S->nc);
    const double* phi = y;
    const double* q = (S->iq >= 0) ? y + (long)S->iq * ng : NULL;
    const double* c = (S->ic >= 0) ? y + (long)S->ic * ng : NULL;
@@ -352,7 +389,7 @@ static double quat_residual(OSim* S, double gamma, const double* rhs,
                             double* z, double* res)
 {
    const OGeom* G = &S->G;
-   o_ghost_fill_periodic(G, z, 1);
+   o_ghost_fill(G, z, 1);
    double s = 0.0;
    for (int k = 0; k < G->nz; k++)
       for (int j = 0; j < G->ny; j++)
@@ -393,13 +430,13 @@ static int quat_solve(OSim* S, const double* r, double* z, double tol)
       for (int m = 0; m < ql; m++) {
          double* em = e + (long)m * ng;
          const double* rm = res + (long)m * ng;
-         o_ghost_fill_periodic(G, em, 1);
+         o_ghost_fill(G, em, 1);
          for (int sw = 0; sw < S->s.quat_sweeps; sw++) {
-            o_ghost_fill_periodic(G, em, 1);
+            o_ghost_fill(G, em, 1);
             quat_rbgs_color(S, gamma, rm, em, 0);
-            o_ghost_fill_periodic(G, em, 1);
+            o_ghost_fill(G, em, 1);
             quat_rbgs_color(S, gamma, rm, em, 1);
-            o_ghost_fill_periodic(G, em, 1);
+            o_ghost_fill(G, em, 1);
          }
       }
       for (long n = 0; n < ng * ql; n++) z[n] = z[n] + e[n];
@@ -649,6 +686,10 @@ static int bdf_step(OSim* S)
    int force_setup = 0;
    for (;;) {
       double h = S->h, gamma;
+      if (S->tramp && S->dTdt != 0.0) {
+         S->p.temperature = ramp_temperature(S, S->t + h);
+         if (S->has_tab) o_calphad_at(&S->tab, S->p.temperature, S->p.calphad);
+      }
       if (!S->have_prev) {
          /* zeroth-order predictor on the first step: the step is imposed,
           * not chosen by CVODE's initial-step heuristic, so y0 + h f(y0) can

@@ -35,6 +35,7 @@ void o_geom_init(OGeom* G, int ndim, int nx, int ny, int nz, int g,
    G->dx[0] = dx[0];
    G->dx[1] = dx[1];
    G->dx[2] = (ndim == 3) ? dx[2] : 1.0;
+   G->periodic = 1;
 }
 
 #define CI(i, j, k) \
@@ -404,6 +405,29 @@ void o_ghost_fill_periodic(const OGeom* G, double* f, int ncomp)
                   continue;
                a[CI(i, j, k)] =
                    a[CI(wrap(i, G->nx), wrap(j, G->ny), wrap(k, G->nz))];
+            }
+   }
+}
+
+static int clampi(int i, int n) { return i < 0 ? 0 : (i >= n ? n - 1 : i); }
+
+void o_ghost_fill(const OGeom* G, double* f, int ncomp)
+{
+   if (G->periodic) {
+      o_ghost_fill_periodic(G, f, ncomp);
+      return;
+   }
+   int g = G->g, gz = G->gz;
+   for (int m = 0; m < ncomp; m++) {
+      double* a = f + (long)m * G->ncg;
+      for (int k = -gz; k < G->nz + gz; k++)
+         for (int j = -g; j < G->ny + g; j++)
+            for (int i = -g; i < G->nx + g; i++) {
+               if (i >= 0 && i < G->nx && j >= 0 && j < G->ny && k >= 0 &&
+                   k < G->nz)
+                  continue;
+               a[CI(i, j, k)] =
+                   a[CI(clampi(i, G->nx), clampi(j, G->ny), clampi(k, G->nz))];
             }
    }
 }
@@ -1238,13 +1262,13 @@ double o_fac_smooth(const OGeom* G, const ampe_elliptic_op* op,
 {
    double maxres = 0.0;
    int not_converged = 1;
-   o_ghost_fill_periodic(G, u, 1);
+   o_ghost_fill(G, u, 1);
    for (int s = 0; s < sweeps && not_converged; s++) {
-      o_ghost_fill_periodic(G, u, 1);
+      o_ghost_fill(G, u, 1);
       double r = o_rbgs_color(G, op, rhs, u, 0, flux);
-      o_ghost_fill_periodic(G, u, 1);
+      o_ghost_fill(G, u, 1);
       double b = o_rbgs_color(G, op, rhs, u, 1, flux);
-      o_ghost_fill_periodic(G, u, 1);
+      o_ghost_fill(G, u, 1);
       if (tol >= 0.0) {
          maxres = r > b ? r : b;
          not_converged = maxres > tol;
@@ -1261,7 +1285,7 @@ double o_fac_residual(const OGeom* G, const ampe_elliptic_op* op,
                       const double* rhs, double* u, double* res, double* flux)
 {
    double dinv[3] = {1.0 / G->dx[0], 1.0 / G->dx[1], 1.0 / G->dx[2]};
-   o_ghost_fill_periodic(G, u, 1);
+   o_ghost_fill(G, u, 1);
    fac_flux(G, op, u, flux);
    double s = 0.0;
 #pragma omp parallel for reduction(+ : s)

@@ -1,0 +1,69 @@
+"""Oracle integrator on the KKScomposition deck's features: zero-slope
+(non-periodic) boundaries conserve the composition integral to round-off at
+tight Newton tolerances, and a temperature ramp held at its target from the
+first step gives the constant-temperature run at the target."""
+import numpy as np
+import pytest
+
+from ampe_amd import params as P
+
+O = pytest.importorskip("oracle.oracle")
+
+
+def _deck_small(n=32):
+    from ampe_amd import initial
+    p, ramp, _, _ = P.kks_composition_deck()
+    for d in range(3):
+        p.dx[d] = 1.8 / 64
+    ic = initial.make_nuclei(nx=n, ny=n, nz=1, radius=8, center0=(0.0, 0.0, 0.0),
+                             nomconc=[0.25], conc_in=[0.096])
+    return p, ramp, (ic["phase"][0], ic["conc"][0][0], None)
+
+
+def test_zero_slope_ghosts_are_clamped_interior():
+    import ctypes as C
+    P2 = O.Patch(2, (5, 4, 1))
+    P2.G.periodic = 0
+    a = P2.cells()
+    a[...] = np.nan
+    vals = np.arange(20.0).reshape(4, 5)
+    P2.interior(a)[...] = vals
+    O.lib().o_ghost_fill(C.byref(P2.G), O.ptr(a), 1)
+    full = a.reshape(6, 7)
+    assert np.array_equal(full[0, 1:-1], vals[0]) and np.array_equal(full[-1, 1:-1], vals[-1])
+    assert np.array_equal(full[1:-1, 0], vals[:, 0]) and np.array_equal(full[1:-1, -1], vals[:, -1])
+    assert full[0, 0] == vals[0, 0] and full[-1, -1] == vals[-1, -1]
+
+
+def test_composition_integral_conserved_with_zero_slope_boundaries():
+    p, ramp, st = _deck_small()
+    s = P.kks_composition_solver(dt=1e-3, atol=1e-10, rtol=1e-12)
+    sim = O.Sim(p, s, (32, 32, 1), periodic=False)
+    sim.set_temperature_ramp(ramp["T0"], ramp["dTdt"], ramp["T_target"], ramp["tab"])
+    sim.set_state(*st)
+    c0 = sim.get_state()[1]
+    rc, stt = sim.advance(12)
+    assert rc == 0
+    c = sim.get_state()[1]
+    assert abs(c.sum() - c0.sum()) / c0.sum() < 1e-11
+    assert not np.array_equal(c, c0)
+
+
+def test_ramp_held_at_target_equals_constant_temperature():
+    p, ramp, st = _deck_small()
+    s = P.kks_composition_solver(dt=1e-3, atol=1e-9, rtol=1e-11)
+    a = O.Sim(p, s, (32, 32, 1), periodic=False)
+    a.set_temperature_ramp(1450.0, -1.0e9, 1400.0, ramp["tab"])
+    a.set_state(*st)
+    q = P.kks_composition_deck()[0]
+    for d in range(3):
+        q.dx[d] = p.dx[d]
+    q.temperature = 1400.0
+    q.calphad = O.calphad_at(ramp["tab"], 1400.0)
+    b = O.Sim(q, s, (32, 32, 1), periodic=False)
+    b.set_state(*st)
+    ra, _ = a.advance(6)
+    rb, _ = b.advance(6)
+    assert ra == 0 and rb == 0
+    for x, y in zip(a.get_state()[:2], b.get_state()[:2]):
+        assert np.linalg.norm(x - y) / np.linalg.norm(y) < 1e-9
