@@ -493,3 +493,50 @@ def kks_composition_solver(dt=1.0e-3, **kw):
                 fac_max_cycles=1, fac_tol=-1.0)
     base.update(kw)
     return solver_defaults(**base)
+
+
+def one_grain_quadratic_deck():
+    """The reference's OneGrainQuadratic regression (tests/OneGrainQuadratic/
+    2d.input, test2d.py): 2-D 64 x 64 periodic cells on [0, 3.2]^2 um,
+    quadratic KKS wells (A = 1e4, c_eq = 0.05 / 0.10, m = 0, T_ref = 1000 K),
+    EBS composition flux with temperature-dependent diffusion (D_S = 1.3e8,
+    Q_S = 156377; D_L = 5.6e4, Q_L = 55329; linear phase weights),
+    sigma = 0.1 J/m^2, delta = 0.045 um, M_phi = 200, hbsm interpolation,
+    arithmetic averages, molar volumes 1.5e-5 m^3/mol, no orientation field;
+    temperature 873 K cooled at -20 K/s towards 573 K; end_time 0.25 s,
+    atol 1e-4.  Initial state of test2d.py: make_nuclei.py --nx 64 --ny 64
+    --nz 1 -r 8 --concentration-in 0.1 --concentration-out 0.06 --ngrains 1.
+    The test passes when the solid volume fraction at the end time is
+    0.21 +- 0.01 (test2d.py:39-46).
+
+    Returns (model params, ramp dict, initial state, checks dict)."""
+    from . import initial
+    sigma, delta = 0.1, 0.045
+    n = 64
+    p = two_grain_params(ndim=2)
+    for d in range(3):
+        p.dx[d] = 3.2 / n
+    p.temperature = 873.0
+    p.epsilon_phi = float(np.sqrt(6.0 * sigma * delta))
+    p.phi_well_scale = (3.0 * sigma / delta) / 16.0
+    p.phi_mobility = 200.0
+    p.energy_interp = b"h"
+    p.conc_interp = b"h"
+    p.avg_type = b"a"
+    p.evolve_quat = 0
+    p.fe_model = FE_QUADRATIC
+    p.vm_L = p.vm_A = 1.5e-5
+    p.quad_A[0], p.quad_A[1] = 1.0e4, 1.0e4
+    p.quad_ceq[0], p.quad_ceq[1] = 0.05, 0.10
+    p.quad_m[0], p.quad_m[1] = 0.0, 0.0
+    p.quad_Tref = 1000.0
+    p.D_liquid, p.Q0_liquid = 5.6e4, 55329.0
+    p.D_solid, p.Q0_solid = 1.3e8, 156377.0
+    p.conc_rhs_form = CONC_RHS_EBS
+    p.diff_interp = b"l"
+    ramp = dict(T0=873.0, dTdt=-20.0, T_target=573.0, tab=None)
+    ic = initial.make_nuclei(nx=n, ny=n, nz=1, radius=8, conc_in=[0.1], conc_out=[0.06])
+    state = (ic["phase"][0], ic["conc"][0][0], None)
+    checks = dict(end_time=0.25, solid_fraction=0.21, solid_fraction_tol=1.0e-2,
+                  max_steps=220)
+    return p, ramp, state, checks

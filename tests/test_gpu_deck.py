@@ -75,3 +75,32 @@ def test_kks_composition_regression(dev):
           "Krylov", st["nli"])
     assert abs(fs - ck["solid_fraction"]) <= ck["solid_fraction_tol"]
     assert t - ck["end_time"] < 0.05
+
+
+def test_one_grain_quadratic_regression(dev):
+    """tests/OneGrainQuadratic (test2d.py:39-46): solid volume fraction
+    0.21 +- 0.01 at t = 0.25 s; the device run also matches the oracle's run
+    of the same deck at the end time."""
+    from ampe_amd import hip as H
+    O = pytest.importorskip("oracle.oracle")
+    p, ramp, st, ck = P.one_grain_quadratic_deck()
+    s = P.kks_composition_solver(dt=1e-3, atol=1e-6, rtol=1e-8)
+    box = H.boxes_of((N, N), 2)
+    g = H.Sim(p, s, [box], periodic=box)
+    g.set_temperature_ramp(ramp["T0"], ramp["dTdt"], ramp["T_target"], ramp["tab"])
+    g.set_state(*st)
+    o = O.Sim(p, s, (N, N, 1))
+    o.set_temperature_ramp(ramp["T0"], ramp["dTdt"], ramp["T_target"], ramp["tab"])
+    o.set_state(*st)
+    t = 0.0
+    while t < ck["end_time"]:
+        rg, sg = g.advance(25)
+        ro, so = o.advance(25)
+        assert rg == 0 and ro == 0 and sg["t"] == so["t"]
+        t = sg["t"]
+    phi, c = g.get_state((N, N))[:2]
+    fs = np.abs(phi).mean()
+    print("t", sg["t"], "solid fraction", fs, "steps", sg["nst"], "Newton", sg["nni"])
+    assert abs(fs - ck["solid_fraction"]) <= ck["solid_fraction_tol"]
+    for a, b in zip((phi, c), o.get_state()[:2]):
+        assert np.linalg.norm(a - b) / np.linalg.norm(b) < 1e-8

@@ -67,3 +67,26 @@ def test_ramp_held_at_target_equals_constant_temperature():
     assert ra == 0 and rb == 0
     for x, y in zip(a.get_state()[:2], b.get_state()[:2]):
         assert np.linalg.norm(x - y) / np.linalg.norm(y) < 1e-9
+
+
+def test_one_grain_quadratic_regression():
+    """tests/OneGrainQuadratic (2d.input, test2d.py:39-46): solid volume
+    fraction 0.21 +- 0.01 at t = 0.25 s, on the oracle integrator (periodic
+    box, quadratic KKS, EBS flux with temperature-dependent diffusion,
+    cooling at 20 K/s)."""
+    p, ramp, st, ck = P.one_grain_quadratic_deck()
+    s = P.kks_composition_solver(dt=2e-3, atol=1e-6, rtol=1e-8)
+    sim = O.Sim(p, s, (64, 64, 1))
+    sim.set_temperature_ramp(ramp["T0"], ramp["dTdt"], ramp["T_target"], ramp["tab"])
+    sim.set_state(*st)
+    c0 = sim.get_state()[1].mean()
+    t = 0.0
+    while t < ck["end_time"]:
+        rc, stt = sim.advance(25)
+        assert rc == 0
+        t = stt["t"]
+    phi, c = sim.get_state()[:2]
+    fs = np.abs(phi).mean()
+    assert abs(fs - ck["solid_fraction"]) <= ck["solid_fraction_tol"], fs
+    assert abs(c.mean() - c0) < 1e-5
+    assert t - ck["end_time"] < 0.05
