@@ -18,13 +18,44 @@ Restated behaviour, with the generator's quirks kept:
   * with a nominal and an interior composition the exterior one is
     (c vol - c_in vs) / vl, stored in single precision
     (make_nuclei.py:715-721); c = c_in phi + c_out (1 - phi).
-Random orientations and more than two nuclei are not restated."""
+  * orientations (qlen 4): after random.seed(21361) each nucleus draws
+    t = pi U(-1, 1) and Shoemake's (u1, u2, u3), each u floored to a multiple
+    of 0.1 / 0.05 / 0.05 (make_nuclei.py:175-230, utils/quat.py:522-545,
+    Python's Mersenne Twister); every cell takes the quaternion of its current
+    nucleus, which changes only when the cell is farther than 2 r from it
+    (squared distance above (4 r)^2 / 4), then to the first nearer nucleus,
+    the search stopping once within 2 r (make_nuclei.py:673-700).
+More than two nuclei are not restated."""
+import math
+import random
+
 import numpy as np
+
+
+def _random_quats(n):
+    """Orientations of make_nuclei.py's setRandomQinSpheres for qlen 4."""
+    rng = random.Random(21361)
+    out = []
+    for _ in range(n):
+        rng.uniform(-1, 1.)  # the discrete angle t, used only for qlen 1 / 2
+        u1, u2, u3 = rng.uniform(0, 1.), rng.uniform(0, 1.), rng.uniform(0, 1.)
+        h1 = 0.1
+        h2 = 0.5 * h1
+        h3 = h2
+        u1 = math.floor(u1 / h1) * h1
+        u2 = math.floor(u2 / h2) * h2
+        u3 = math.floor(u3 / h3) * h3
+        out.append([math.sqrt(1. - u1) * math.sin(2. * math.pi * u2),
+                    math.sqrt(1. - u1) * math.cos(2. * math.pi * u2),
+                    math.sqrt(u1) * math.sin(2. * math.pi * u3),
+                    math.sqrt(u1) * math.cos(2. * math.pi * u3)])
+    return out
 
 
 def make_nuclei(nx, ny, nz=1, radius=None, ngrains=1, phase_in=1.0, phase_out=0.0,
                 nomconc=None, conc_in=None, conc_out=None, width=0.0, center0=None,
-                temperature_in=None, temperature_out=None, double_precision=False):
+                temperature_in=None, temperature_out=None, double_precision=False,
+                qlen=0):
     if ngrains not in (1, 2):
         raise NotImplementedError("make_nuclei: one or two nuclei")
     dt = np.float64 if double_precision else np.float32
@@ -84,6 +115,30 @@ def make_nuclei(nx, ny, nz=1, radius=None, ngrains=1, phase_in=1.0, phase_out=0.
             conc.append((ci[s] * ph + float(co[s]) * (1.0 - ph)).astype(dt).astype(np.float64))
         out["conc"] = conc
         out["conc_out"] = [float(v) for v in co]
+    if qlen:
+        if qlen != 4:
+            raise NotImplementedError("make_nuclei: qlen 4 orientations")
+        qs = _random_quats(ngrains)
+        mind2 = 0.25 * (4.0 * radius) ** 2
+        quat = np.zeros((4, nz, ny, nx), dt)
+        gmin = 0
+        for i in range(nx):
+            for j in range(ny):
+                for k in range(nz):
+                    d2min = ((x[i] - cx[gmin]) ** 2 + (y[j] - cy[gmin]) ** 2 +
+                             (z[k] - cz[gmin]) ** 2)
+                    if d2min > mind2:
+                        for g in range(ngrains):
+                            d2 = ((x[i] - cx[g]) ** 2 + (y[j] - cy[g]) ** 2 +
+                                  (z[k] - cz[g]) ** 2)
+                            if d2 < d2min:
+                                d2min = d2
+                                gmin = g
+                                if d2min < mind2:
+                                    break
+                    quat[:, k, j, i] = qs[gmin]
+        out["quat"] = quat.astype(np.float64)
+        out["grain_quats"] = qs
     if temperature_in is not None:
         T = np.full((nz, ny, nx), temperature_out, dt)
         for g in range(ngrains):

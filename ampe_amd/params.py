@@ -540,3 +540,43 @@ def one_grain_quadratic_deck():
     checks = dict(end_time=0.25, solid_fraction=0.21, solid_fraction_tol=1.0e-2,
                   max_steps=220)
     return p, ramp, state, checks
+
+
+def two_grains_quadratic_deck():
+    """The reference's TwoGrainsQuadratic regression (tests/TwoGrainsQuadratic/
+    2d.input, test2d.py): the OneGrainQuadratic setting with an orientation
+    field (H = 0.001, epsilon_orient = 0.1, orient_mobility = 200, the
+    reference's defaults for the rest), epsilon_phi = 0.165,
+    phi_well_scale = 0.4125, the KKS composition flux, T_ref = 873 K; two
+    nuclei of radii 8 and 12 cells with random orientations (make_nuclei.py
+    --nx 64 --ny 64 --nz 1 -r 8 --concentration-in 0.1
+    --concentration-out 0.06 --ngrains 2 -q 4).  The test passes when the
+    solid volume fraction at t = 0.25 s is 0.27 +- 0.01 and the grain volumes
+    reported every 0.05 s (cells with phi >= 0.85, GrainDiagnostics) range
+    from 0.3025 to 1.9175 um^2, each +- 0.01 (test2d.py:30-75).
+
+    Returns (model params, ramp dict, initial state, checks dict)."""
+    from . import initial
+    n = 64
+    p = two_grain_params(ndim=2)
+    for d in range(3):
+        p.dx[d] = 3.2 / n
+    p.temperature = 873.0
+    p.epsilon_phi = 0.165
+    p.phi_well_scale = 0.4125
+    p.phi_mobility = 200.0
+    p.H_parameter = 0.001
+    p.epsilon_q = 0.1
+    p.quat_mobility = 200.0
+    p.gradient_floor = 1.0e-2
+    p.D_liquid, p.Q0_liquid = 5.6e4, 55329.0
+    p.D_solid, p.Q0_solid = 1.3e8, 156377.0
+    p.quad_Tref = 873.0
+    ramp = dict(T0=873.0, dTdt=-20.0, T_target=573.0, tab=None)
+    ic = initial.make_nuclei(nx=n, ny=n, nz=1, radius=8, ngrains=2, conc_in=[0.1],
+                             conc_out=[0.06], qlen=4)
+    state = (ic["phase"][0], ic["conc"][0][0], ic["quat"][:, 0])
+    checks = dict(end_time=0.25, solid_fraction=0.27, solid_fraction_tol=1.0e-2,
+                  grain_threshold=0.85, grain_interval=0.05, grain_vmax=1.9175,
+                  grain_vmin=0.3025, grain_tol=1.0e-2)
+    return p, ramp, state, checks

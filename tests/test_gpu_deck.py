@@ -104,3 +104,47 @@ def test_one_grain_quadratic_regression(dev):
     assert abs(fs - ck["solid_fraction"]) <= ck["solid_fraction_tol"]
     for a, b in zip((phi, c), o.get_state()[:2]):
         assert np.linalg.norm(a - b) / np.linalg.norm(b) < 1e-8
+
+
+def _gpu_grain_volumes(phi, thr, dv):
+    import ctypes as C
+    import torch
+    from ampe_amd import hip as H
+    full = H.boxes_of(phi.shape, 2)
+    lev = H.Level(2, [full], 1, periodic=full)
+    f = torch.from_numpy(lev.to_ghosted(phi)).to("cuda")
+    lab = torch.full((lev.ncell,), -7, dtype=torch.int32, device="cuda")
+    ng = C.c_int(-1)
+    H.check(H.lib().ampe_grains_label(lev.h, C.c_void_p(f.data_ptr()), thr,
+                                      C.c_void_p(lab.data_ptr()), C.byref(ng), None))
+    lab = lab.cpu().numpy()
+    return [float(np.sum(lab == g)) * dv for g in range(ng.value)]
+
+
+def test_two_grains_quadratic_regression(dev):
+    """tests/TwoGrainsQuadratic (test2d.py:30-75) on the device integrator,
+    grains numbered by the device labelling: solid fraction 0.27 +- 0.01 at
+    t = 0.25 s, grain volumes (phi >= 0.85, every 0.05 s) from 0.3025 to
+    1.9175 um^2 (+- 0.01).  Exercises the orientation RHS and block."""
+    from ampe_amd import hip as H
+    p, ramp, st, ck = P.two_grains_quadratic_deck()
+    s = P.kks_composition_solver(dt=1e-3, atol=1e-5, rtol=1e-7, quat_sweeps=5)
+    box = H.boxes_of((N, N), 2)
+    sim = H.Sim(p, s, [box], periodic=box)
+    sim.set_temperature_ramp(ramp["T0"], ramp["dTdt"], ramp["T_target"], ramp["tab"])
+    sim.set_state(*st)
+    dv = p.dx[0] * p.dx[1]
+    t, k, vols = 0.0, 1, []
+    while t < ck["end_time"] - 1e-12:
+        rc, stt = sim.advance(1)
+        assert rc == 0
+        t = stt["t"]
+        if t >= k * ck["grain_interval"] - 1e-12:
+            vols += _gpu_grain_volumes(sim.get_state((N, N))[0], ck["grain_threshold"], dv)
+            k += 1
+    fs = np.abs(sim.get_state((N, N))[0]).mean()
+    print("t", t, "solid fraction", fs, "grain volumes", vols, "steps", stt["nst"])
+    assert k == 6
+    assert abs(fs - ck["solid_fraction"]) <= ck["solid_fraction_tol"]
+    assert abs(max(vols) - ck["grain_vmax"]) <= ck["grain_tol"]
+    assert abs(min(vols) - ck["grain_vmin"]) <= ck["grain_tol"]

@@ -45,3 +45,20 @@ def test_two_nuclei():
     # centres (20, 20) and (40, 40), radii 5 and 7.5, z offset 0.5 - 0.5 = 0
     assert phi[19, 19] == 1.0 and phi[39, 39] == 1.0
     assert phi[39, 39 + 7] == 1.0 and phi[19, 19 + 6] == 0.0
+
+
+def test_random_orientations_two_nuclei():
+    """make_nuclei.py -q 4 --ngrains 2: Shoemake quaternions on the generator's
+    discrete grid (u1 in steps of 0.1, u2 and u3 in steps of 0.05), unit
+    norm, and every cell carries one of the two."""
+    ic = initial.make_nuclei(nx=64, ny=64, nz=1, radius=8, ngrains=2, conc_in=[0.1],
+                             conc_out=[0.06], qlen=4)
+    qs = np.array(ic["grain_quats"])
+    assert np.allclose((qs ** 2).sum(axis=1), 1.0)
+    for qv in qs:
+        u1 = qv[2] ** 2 + qv[3] ** 2
+        assert abs(u1 * 10 - round(u1 * 10)) < 1e-9
+    q = ic["quat"][:, 0]
+    g0 = np.all(q == qs[0].astype(np.float32).astype(np.float64)[:, None, None], axis=0)
+    g1 = np.all(q == qs[1].astype(np.float32).astype(np.float64)[:, None, None], axis=0)
+    assert np.all(g0 | g1) and g0[20, 20] and g1[43, 43]

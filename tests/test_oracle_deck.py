@@ -90,3 +90,39 @@ def test_one_grain_quadratic_regression():
     assert abs(fs - ck["solid_fraction"]) <= ck["solid_fraction_tol"], fs
     assert abs(c.mean() - c0) < 1e-5
     assert t - ck["end_time"] < 0.05
+
+
+def _oracle_grain_volumes(phi, thr, dv):
+    import ctypes as C
+    pt = O.Patch(2, (phi.shape[1], phi.shape[0], 1))
+    a = pt.load(pt.cells(), phi)
+    lab = np.zeros(pt.N, dtype=np.int32)
+    sw = C.c_int(0)
+    ng = O.lib().o_grains_label(C.byref(pt.G), O.ptr(a), thr, 1, O.iptr(lab), C.byref(sw))
+    return [float(np.sum(lab == g)) * dv for g in range(ng)]
+
+
+def test_two_grains_quadratic_regression():
+    """tests/TwoGrainsQuadratic (test2d.py:30-75): solid fraction 0.27 +- 0.01
+    at t = 0.25 s and grain volumes (cells with phi >= 0.85, every 0.05 s)
+    from 0.3025 to 1.9175 um^2 (+- 0.01), on the oracle integrator: two
+    randomly oriented nuclei, quadratic KKS, orientation field, cooling."""
+    p, ramp, st, ck = P.two_grains_quadratic_deck()
+    s = P.kks_composition_solver(dt=1e-3, atol=1e-5, rtol=1e-7, quat_sweeps=5)
+    sim = O.Sim(p, s, (64, 64, 1))
+    sim.set_temperature_ramp(ramp["T0"], ramp["dTdt"], ramp["T_target"], ramp["tab"])
+    sim.set_state(*st)
+    dv = p.dx[0] * p.dx[1]
+    t, k, vols = 0.0, 1, []
+    while t < ck["end_time"] - 1e-12:
+        rc, stt = sim.advance(1)
+        assert rc == 0
+        t = stt["t"]
+        if t >= k * ck["grain_interval"] - 1e-12:
+            vols += _oracle_grain_volumes(sim.get_state()[0], ck["grain_threshold"], dv)
+            k += 1
+    fs = np.abs(sim.get_state()[0]).mean()
+    assert k == 6
+    assert abs(fs - ck["solid_fraction"]) <= ck["solid_fraction_tol"], fs
+    assert abs(max(vols) - ck["grain_vmax"]) <= ck["grain_tol"], vols
+    assert abs(min(vols) - ck["grain_vmin"]) <= ck["grain_tol"], vols
