@@ -495,7 +495,7 @@ def kks_composition_solver(dt=1.0e-3, **kw):
     return solver_defaults(**base)
 
 
-def one_grain_quadratic_deck():
+def one_grain_quadratic_deck(ndim=2):
     """The reference's OneGrainQuadratic regression (tests/OneGrainQuadratic/
     2d.input, test2d.py): 2-D 64 x 64 periodic cells on [0, 3.2]^2 um,
     quadratic KKS wells (A = 1e4, c_eq = 0.05 / 0.10, m = 0, T_ref = 1000 K),
@@ -507,15 +507,17 @@ def one_grain_quadratic_deck():
     atol 1e-4.  Initial state of test2d.py: make_nuclei.py --nx 64 --ny 64
     --nz 1 -r 8 --concentration-in 0.1 --concentration-out 0.06 --ngrains 1.
     The test passes when the solid volume fraction at the end time is
-    0.21 +- 0.01 (test2d.py:39-46).
+    0.21 +- 0.01 (test2d.py:39-46).  ndim = 3: 48^3 cells on [0, 2.4]^3 um,
+    make_nuclei.py --nx 48 --ny 48 --nz 48 -r 8, end_time 0.15 s, solid
+    fraction 0.14 +- 0.01 (3d.input, test3d.py:36-46).
 
     Returns (model params, ramp dict, initial state, checks dict)."""
     from . import initial
     sigma, delta = 0.1, 0.045
-    n = 64
-    p = two_grain_params(ndim=2)
+    n = 64 if ndim == 2 else 48
+    p = two_grain_params(ndim=ndim)
     for d in range(3):
-        p.dx[d] = 3.2 / n
+        p.dx[d] = 0.05
     p.temperature = 873.0
     p.epsilon_phi = float(np.sqrt(6.0 * sigma * delta))
     p.phi_well_scale = (3.0 * sigma / delta) / 16.0
@@ -535,14 +537,19 @@ def one_grain_quadratic_deck():
     p.conc_rhs_form = CONC_RHS_EBS
     p.diff_interp = b"l"
     ramp = dict(T0=873.0, dTdt=-20.0, T_target=573.0, tab=None)
-    ic = initial.make_nuclei(nx=n, ny=n, nz=1, radius=8, conc_in=[0.1], conc_out=[0.06])
-    state = (ic["phase"][0], ic["conc"][0][0], None)
-    checks = dict(end_time=0.25, solid_fraction=0.21, solid_fraction_tol=1.0e-2,
-                  max_steps=220)
+    ic = initial.make_nuclei(nx=n, ny=n, nz=1 if ndim == 2 else n, radius=8, conc_in=[0.1],
+                             conc_out=[0.06])
+    if ndim == 2:
+        state = (ic["phase"][0], ic["conc"][0][0], None)
+        checks = dict(end_time=0.25, solid_fraction=0.21, solid_fraction_tol=1.0e-2)
+    else:
+        state = (ic["phase"], ic["conc"][0], None)
+        checks = dict(end_time=0.15, solid_fraction=0.14, solid_fraction_tol=1.0e-2)
+    checks["shape"] = state[0].shape
     return p, ramp, state, checks
 
 
-def two_grains_quadratic_deck():
+def two_grains_quadratic_deck(ndim=2):
     """The reference's TwoGrainsQuadratic regression (tests/TwoGrainsQuadratic/
     2d.input, test2d.py): the OneGrainQuadratic setting with an orientation
     field (H = 0.001, epsilon_orient = 0.1, orient_mobility = 200, the
@@ -553,14 +560,19 @@ def two_grains_quadratic_deck():
     --concentration-out 0.06 --ngrains 2 -q 4).  The test passes when the
     solid volume fraction at t = 0.25 s is 0.27 +- 0.01 and the grain volumes
     reported every 0.05 s (cells with phi >= 0.85, GrainDiagnostics) range
-    from 0.3025 to 1.9175 um^2, each +- 0.01 (test2d.py:30-75).
+    from 0.3025 to 1.9175 um^2, each +- 0.01 (test2d.py:30-75).  ndim = 3:
+    64 x 64 x 48 cells on [0, 3.2] x [0, 3.2] x [0, 2.4] um, end_time 0.08 s,
+    diagnostics every 0.01 s, solid fraction 0.13 +- 0.01, grain volumes
+    up to 2.0 +- 0.01 and down to 0.179 +- 0.001 um^3 (3d.input,
+    test3d.py:28-70).
 
     Returns (model params, ramp dict, initial state, checks dict)."""
     from . import initial
     n = 64
-    p = two_grain_params(ndim=2)
+    nz = 1 if ndim == 2 else 48
+    p = two_grain_params(ndim=ndim)
     for d in range(3):
-        p.dx[d] = 3.2 / n
+        p.dx[d] = 0.05
     p.temperature = 873.0
     p.epsilon_phi = 0.165
     p.phi_well_scale = 0.4125
@@ -573,10 +585,17 @@ def two_grains_quadratic_deck():
     p.D_solid, p.Q0_solid = 1.3e8, 156377.0
     p.quad_Tref = 873.0
     ramp = dict(T0=873.0, dTdt=-20.0, T_target=573.0, tab=None)
-    ic = initial.make_nuclei(nx=n, ny=n, nz=1, radius=8, ngrains=2, conc_in=[0.1],
+    ic = initial.make_nuclei(nx=n, ny=n, nz=nz, radius=8, ngrains=2, conc_in=[0.1],
                              conc_out=[0.06], qlen=4)
-    state = (ic["phase"][0], ic["conc"][0][0], ic["quat"][:, 0])
-    checks = dict(end_time=0.25, solid_fraction=0.27, solid_fraction_tol=1.0e-2,
-                  grain_threshold=0.85, grain_interval=0.05, grain_vmax=1.9175,
-                  grain_vmin=0.3025, grain_tol=1.0e-2)
+    if ndim == 2:
+        state = (ic["phase"][0], ic["conc"][0][0], ic["quat"][:, 0])
+        checks = dict(end_time=0.25, solid_fraction=0.27, solid_fraction_tol=1.0e-2,
+                      grain_threshold=0.85, grain_interval=0.05, grain_vmax=1.9175,
+                      grain_vmin=0.3025, grain_tol=1.0e-2, grain_tol_min=1.0e-2)
+    else:
+        state = (ic["phase"], ic["conc"][0], ic["quat"])
+        checks = dict(end_time=0.08, solid_fraction=0.13, solid_fraction_tol=1.0e-2,
+                      grain_threshold=0.85, grain_interval=0.01, grain_vmax=2.0,
+                      grain_vmin=0.179, grain_tol=1.0e-2, grain_tol_min=1.0e-3)
+    checks["shape"] = state[0].shape
     return p, ramp, state, checks

@@ -77,41 +77,49 @@ def test_kks_composition_regression(dev):
     assert t - ck["end_time"] < 0.05
 
 
-def test_one_grain_quadratic_regression(dev):
-    """tests/OneGrainQuadratic (test2d.py:39-46): solid volume fraction
-    0.21 +- 0.01 at t = 0.25 s; the device run also matches the oracle's run
-    of the same deck at the end time."""
+@pytest.mark.parametrize("ndim", [2, 3])
+def test_one_grain_quadratic_regression(dev, ndim):
+    """tests/OneGrainQuadratic: solid volume fraction 0.21 +- 0.01 at
+    t = 0.25 s in 2-D (test2d.py:39-46), 0.14 +- 0.01 at t = 0.15 s on 48^3
+    cells in 3-D (test3d.py:36-46); in 2-D the device run also matches the
+    oracle's run of the same deck at the end time."""
     from ampe_amd import hip as H
     O = pytest.importorskip("oracle.oracle")
-    p, ramp, st, ck = P.one_grain_quadratic_deck()
+    p, ramp, st, ck = P.one_grain_quadratic_deck(ndim)
+    shape = ck["shape"]
     s = P.kks_composition_solver(dt=1e-3, atol=1e-6, rtol=1e-8)
-    box = H.boxes_of((N, N), 2)
+    box = H.boxes_of(shape, ndim)
     g = H.Sim(p, s, [box], periodic=box)
     g.set_temperature_ramp(ramp["T0"], ramp["dTdt"], ramp["T_target"], ramp["tab"])
     g.set_state(*st)
-    o = O.Sim(p, s, (N, N, 1))
-    o.set_temperature_ramp(ramp["T0"], ramp["dTdt"], ramp["T_target"], ramp["tab"])
-    o.set_state(*st)
+    o = None
+    if ndim == 2:
+        o = O.Sim(p, s, (shape[1], shape[0], 1))
+        o.set_temperature_ramp(ramp["T0"], ramp["dTdt"], ramp["T_target"], ramp["tab"])
+        o.set_state(*st)
     t = 0.0
     while t < ck["end_time"]:
         rg, sg = g.advance(25)
-        ro, so = o.advance(25)
-        assert rg == 0 and ro == 0 and sg["t"] == so["t"]
+        assert rg == 0
+        if o is not None:
+            ro, so = o.advance(25)
+            assert ro == 0 and sg["t"] == so["t"]
         t = sg["t"]
-    phi, c = g.get_state((N, N))[:2]
+    phi, c = g.get_state(shape)[:2]
     fs = np.abs(phi).mean()
-    print("t", sg["t"], "solid fraction", fs, "steps", sg["nst"], "Newton", sg["nni"])
+    print(ndim, "t", sg["t"], "solid fraction", fs, "steps", sg["nst"], "Newton", sg["nni"])
     assert abs(fs - ck["solid_fraction"]) <= ck["solid_fraction_tol"]
-    for a, b in zip((phi, c), o.get_state()[:2]):
-        assert np.linalg.norm(a - b) / np.linalg.norm(b) < 1e-8
+    if o is not None:
+        for a, b in zip((phi, c), o.get_state()[:2]):
+            assert np.linalg.norm(a - b) / np.linalg.norm(b) < 1e-8
 
 
 def _gpu_grain_volumes(phi, thr, dv):
     import ctypes as C
     import torch
     from ampe_amd import hip as H
-    full = H.boxes_of(phi.shape, 2)
-    lev = H.Level(2, [full], 1, periodic=full)
+    full = H.boxes_of(phi.shape, phi.ndim)
+    lev = H.Level(phi.ndim, [full], 1, periodic=full)
     f = torch.from_numpy(lev.to_ghosted(phi)).to("cuda")
     lab = torch.full((lev.ncell,), -7, dtype=torch.int32, device="cuda")
     ng = C.c_int(-1)
@@ -121,30 +129,36 @@ def _gpu_grain_volumes(phi, thr, dv):
     return [float(np.sum(lab == g)) * dv for g in range(ng.value)]
 
 
-def test_two_grains_quadratic_regression(dev):
-    """tests/TwoGrainsQuadratic (test2d.py:30-75) on the device integrator,
-    grains numbered by the device labelling: solid fraction 0.27 +- 0.01 at
-    t = 0.25 s, grain volumes (phi >= 0.85, every 0.05 s) from 0.3025 to
-    1.9175 um^2 (+- 0.01).  Exercises the orientation RHS and block."""
+@pytest.mark.parametrize("ndim", [2, 3])
+def test_two_grains_quadratic_regression(dev, ndim):
+    """tests/TwoGrainsQuadratic on the device integrator, grains numbered by
+    the device labelling.  2-D (test2d.py:30-75): solid fraction 0.27 +- 0.01
+    at t = 0.25 s, grain volumes (phi >= 0.85, every 0.05 s) from 0.3025 to
+    1.9175 um^2 (+- 0.01).  3-D, 64 x 64 x 48 cells (test3d.py:28-70): solid
+    fraction 0.13 +- 0.01 at t = 0.08 s, grain volumes (every 0.01 s) from
+    0.179 (+- 0.001) to 2.0 (+- 0.01) um^3.  Exercises the orientation RHS
+    and block."""
     from ampe_amd import hip as H
-    p, ramp, st, ck = P.two_grains_quadratic_deck()
+    p, ramp, st, ck = P.two_grains_quadratic_deck(ndim)
+    shape = ck["shape"]
     s = P.kks_composition_solver(dt=1e-3, atol=1e-5, rtol=1e-7, quat_sweeps=5)
-    box = H.boxes_of((N, N), 2)
+    box = H.boxes_of(shape, ndim)
     sim = H.Sim(p, s, [box], periodic=box)
     sim.set_temperature_ramp(ramp["T0"], ramp["dTdt"], ramp["T_target"], ramp["tab"])
     sim.set_state(*st)
-    dv = p.dx[0] * p.dx[1]
+    dv = p.dx[0] * p.dx[1] * (p.dx[2] if ndim == 3 else 1.0)
     t, k, vols = 0.0, 1, []
+    nk = int(round(ck["end_time"] / ck["grain_interval"]))
     while t < ck["end_time"] - 1e-12:
         rc, stt = sim.advance(1)
         assert rc == 0
         t = stt["t"]
         if t >= k * ck["grain_interval"] - 1e-12:
-            vols += _gpu_grain_volumes(sim.get_state((N, N))[0], ck["grain_threshold"], dv)
+            vols += _gpu_grain_volumes(sim.get_state(shape)[0], ck["grain_threshold"], dv)
             k += 1
-    fs = np.abs(sim.get_state((N, N))[0]).mean()
-    print("t", t, "solid fraction", fs, "grain volumes", vols, "steps", stt["nst"])
-    assert k == 6
+    fs = np.abs(sim.get_state(shape)[0]).mean()
+    print(ndim, "t", t, "solid fraction", fs, "grain volumes", vols, "steps", stt["nst"])
+    assert k == nk + 1
     assert abs(fs - ck["solid_fraction"]) <= ck["solid_fraction_tol"]
     assert abs(max(vols) - ck["grain_vmax"]) <= ck["grain_tol"]
-    assert abs(min(vols) - ck["grain_vmin"]) <= ck["grain_tol"]
+    assert abs(min(vols) - ck["grain_vmin"]) <= ck["grain_tol_min"]
