@@ -235,6 +235,21 @@ int ampe_tag_cells(const ampe_level* level, const ampe_model_params* p, const do
    return ampe::launch_tag(*level->L, *p, phi, tol, tags, hs(stream));
 }
 
+int ampe_cluster_mask(int ndim, const ampe_box* domain, const unsigned char* mask,
+                      const ampe_cluster_params* cp, ampe_box* boxes, int max_boxes,
+                      int* nboxes)
+{
+   AMPE_REQUIRE(domain && mask && cp && boxes && nboxes, "null argument");
+   return ampe::cluster_mask(ndim, *domain, mask, *cp, boxes, max_boxes, nboxes);
+}
+
+int ampe_cluster_tags(const ampe_level* level, const int* tags, const ampe_cluster_params* cp,
+                      ampe_box* boxes, int max_boxes, int* nboxes, ampe_stream_t stream)
+{
+   AMPE_REQUIRE(level && tags && cp && boxes && nboxes, "null argument");
+   return ampe::cluster_tags(*level->L, tags, *cp, boxes, max_boxes, nboxes, hs(stream));
+}
+
 int ampe_fac_phase_coeffs(const ampe_level* level, const ampe_model_params* p, double gamma,
                           const double* phi, const double* M, double* C,
                           ampe_stream_t stream)

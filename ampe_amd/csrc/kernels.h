@@ -169,6 +169,12 @@ int grains_label(const Level& L, const double* phase, double thr, int* labels, i
                  int* passes, hipStream_t s);
 int launch_tag(const Level& L, const ampe_model_params& p, const double* phi, double tol,
                int* tags, hipStream_t s);
+// Berger-Rigoutsos clustering of a host mask over the box dom, or of a
+// level's device tags (cluster.hip).
+int cluster_mask(int ndim, const ampe_box& dom, const unsigned char* mask,
+                 const ampe_cluster_params& cp, ampe_box* boxes, int max_boxes, int* nboxes);
+int cluster_tags(const Level& L, const int* tags, const ampe_cluster_params& cp,
+                 ampe_box* boxes, int max_boxes, int* nboxes, hipStream_t s);
 int launch_quat_face_coef(const Level& L, const ampe_model_params& p, const double* phi,
                           const double* n2, double* fc, hipStream_t s);
 int launch_sqrt_quat_mobility(const Level& L, const ampe_model_params& p, const double* phi,

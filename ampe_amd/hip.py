@@ -36,6 +36,8 @@ _SIGS = {
     "ampe_level_patch_offsets": (_i, [_vp, C.POINTER(_ll), C.POINTER(_ll)]),
     "ampe_level_patch_strides": (_i, [_vp, C.POINTER(_ll), C.POINTER(_ll)]),
     "ampe_sim_set_temperature_ramp": (_i, [_vp, _d, _d, _d, _vp]),
+    "ampe_cluster_mask": (_i, [_i, _vp, _vp, _vp, _vp, _i, _vp]),
+    "ampe_cluster_tags": (_i, [_vp, _vp, _vp, _vp, _i, _vp, _vp]),
     "ampe_ghost_fill": (_i, [_vp, _vp, _i, _vp]),
     "ampe_quat_grad": (_i, [_vp, C.POINTER(ModelParams), _vp, _vp, _vp, _vp]),
     "ampe_phase_rhs": (_i, [_vp, C.POINTER(ModelParams), _vp, _vp, _vp, _vp, _vp, _vp]),
@@ -366,3 +368,20 @@ def kks_batch(p, tab, c, phi, T, cl, cs, iters=None, guess=None, probe_its=3,
                                probe_tol, _dev_ptr(cl), _dev_ptr(cs), _dev_ptr(iters),
                                C.byref(st), _stream_ptr(stream)), "ampe_kks_batch")
     return st.as_dict()
+
+
+def cluster_mask(mask, cp, lo=(0, 0, 0), max_boxes=4096):
+    """Boxes (list of (lo, hi) inclusive, 3 ints each) clustering the nonzero
+    cells of mask ([nz,] ny, nx) whose first cell sits at lo (ampe_cluster_mask)."""
+    m = np.ascontiguousarray(np.asarray(mask) != 0, dtype=np.uint8)
+    ndim = m.ndim
+    shape3 = m.shape[::-1] + (1,) * (3 - ndim)
+    dom = Box()
+    for d in range(3):
+        dom.lo[d] = lo[d] if d < ndim else 0
+        dom.hi[d] = dom.lo[d] + shape3[d] - 1
+    out = (Box * max_boxes)()
+    n = C.c_int(0)
+    check(lib().ampe_cluster_mask(ndim, C.byref(dom), m.ctypes.data_as(_vp), C.byref(cp), out,
+                                  max_boxes, C.byref(n)), "ampe_cluster_mask")
+    return [(tuple(out[b].lo), tuple(out[b].hi)) for b in range(n.value)]

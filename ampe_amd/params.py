@@ -93,6 +93,11 @@ class ModelParams(C.Structure):
     ]
 
 
+class ClusterParams(C.Structure):
+    _fields_ = [("efficiency_tol", C.c_double), ("min_size", C.c_int * 3),
+                ("max_size", C.c_int * 3), ("tag_buffer", C.c_int)]
+
+
 class SolverParams(C.Structure):
     _fields_ = [
         ("dt", C.c_double),

@@ -337,6 +337,29 @@ int ampe_tag_cells(const ampe_level* level, const ampe_model_params* p,
                    const double* phi, double tol, int* tags,
                    ampe_stream_t stream);
 
+/* Box generation from tags: the role of SAMRAI's BergerRigoutsos in
+ * GriddingAlgorithm::regridAllFinerLevels (source/PFModel.cc:258,565-590),
+ * with the tag buffering of bufferTagsOnLevel (source/PFModel.cc:336).  The
+ * clustering is Berger & Rigoutsos (1991) as restated in
+ * ampe_amd/csrc/cluster.hip (SAMRAI's own variant is not reproduced); boxes
+ * are disjoint, cover every buffered tag and lie in the level's bounding box. */
+typedef struct ampe_cluster_params {
+   double efficiency_tol; /* accept a box when tagged / cells >= this
+                           * (BergerRigoutsos efficiency_tolerance) */
+   int min_size[3];       /* smallest extent of a cut piece (smallest_patch_size) */
+   int max_size[3];       /* accepted boxes are chopped to at most this extent
+                           * (largest_patch_size; <= 0: no limit) */
+   int tag_buffer;        /* tags grown by this many cells in every direction */
+} ampe_cluster_params;
+/* Host mask (x fastest over the domain box, nonzero = tagged). */
+int ampe_cluster_mask(int ndim, const ampe_box* domain, const unsigned char* mask,
+                      const ampe_cluster_params* cp, ampe_box* boxes, int max_boxes,
+                      int* nboxes);
+/* Device tags of a level (as ampe_tag_cells writes them); synchronises the
+ * stream. */
+int ampe_cluster_tags(const ampe_level* level, const int* tags, const ampe_cluster_params* cp,
+                      ampe_box* boxes, int max_boxes, int* nboxes, ampe_stream_t stream);
+
 /* -------------------------------------------------------- FAC operators */
 /* Elliptic operator A u = M div(D grad u) + C u on one level
  * (EllipticFACOps, source/EllipticFACOps.h:103).  Each coefficient is a

@@ -97,3 +97,49 @@ def test_uneven_patches_same_labels(dev, ndim):
     two, n2 = _gpu(f, thr, bs)
     assert n1 == n2 and n1 > 2
     assert np.array_equal(one, two)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("ndim", [2, 3])
+def test_cluster_tags_of_tagged_level(dev, ndim):
+    """Device tags from the gradient detector (ampe_tag_cells) of a level of
+    two patches, clustered by ampe_cluster_tags, give the boxes of the
+    oracle's clustering of the same tags."""
+    import torch
+    from ampe_amd import hip as H, params as P
+    from oracle import cluster as OC
+    n = 64 if ndim == 2 else 24
+    shape = (n,) * ndim
+    idx = np.indices(shape).astype(float) + 0.5
+    r = np.sqrt(sum((x - 0.4 * n) ** 2 for x in idx))
+    phi = 0.5 * (1 - np.tanh((r - 0.25 * n) / 2.0))
+    full = H.boxes_of(shape, ndim)
+    half = n // 2
+    b0, b1 = H.Box(), H.Box()
+    for d in range(3):
+        b0.lo[d], b0.hi[d] = full.lo[d], full.hi[d]
+        b1.lo[d], b1.hi[d] = full.lo[d], full.hi[d]
+    b0.hi[0] = half - 1
+    b1.lo[0] = half
+    lev = H.Level(ndim, [b0, b1], 1, periodic=full)
+    p = P.two_grain_params(ndim=ndim)
+    f = torch.from_numpy(lev.to_ghosted(phi)).to("cuda")
+    H.check(H.lib().ampe_ghost_fill(lev.h, C.c_void_p(f.data_ptr()), 1, None))
+    tags = torch.zeros(lev.ncell, dtype=torch.int32, device="cuda")
+    H.check(H.lib().ampe_tag_cells(lev.h, C.byref(p), C.c_void_p(f.data_ptr()), 0.1,
+                                   C.c_void_p(tags.data_ptr()), None))
+    cp = P.ClusterParams()
+    cp.efficiency_tol = 0.7
+    for d in range(3):
+        cp.min_size[d] = 2
+        cp.max_size[d] = 16
+    cp.tag_buffer = 1
+    out = (H.Box * 4096)()
+    nb = C.c_int(0)
+    H.check(H.lib().ampe_cluster_tags(lev.h, C.c_void_p(tags.data_ptr()), C.byref(cp), out,
+                                      4096, C.byref(nb), None))
+    got = [(tuple(out[b].lo)[:ndim], tuple(out[b].hi)[:ndim]) for b in range(nb.value)]
+    mask = lev.from_interior(tags.cpu().numpy())
+    assert mask.sum() > 0
+    ref = OC.cluster(mask, 0.7, (2,) * 3, (16,) * 3, 1)
+    assert got == ref and len(got) > 1
