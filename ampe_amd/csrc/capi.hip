@@ -445,6 +445,24 @@ int ampe_coarsen_cell(const ampe_level* coarse, const ampe_level* fine, int ncom
    return ampe::mg_restrict_cell<double>(*coarse->L, *fine->L, ncomp, f, c, hs(stream));
 }
 
+int ampe_quat_coarsen(const ampe_level* coarse, const ampe_level* fine, const double* dx_fine,
+                      int qlen, const double* qf, double* qc, ampe_stream_t stream)
+{
+   AMPE_REQUIRE(coarse && fine && dx_fine && qf && qc, "bad argument");
+   AMPE_REQUIRE(ampe::is_coarsening(*coarse->L, *fine->L),
+                "coarse level is not the fine level halved");
+   return ampe::quat_coarsen(*coarse->L, *fine->L, dx_fine, qlen, qf, qc, hs(stream));
+}
+
+int ampe_quat_refine(const ampe_level* fine, const ampe_level* coarse, const double* dx_fine,
+                     int qlen, const double* qc, double* qf, ampe_stream_t stream)
+{
+   AMPE_REQUIRE(coarse && fine && dx_fine && qf && qc, "bad argument");
+   AMPE_REQUIRE(ampe::is_coarsening(*coarse->L, *fine->L),
+                "coarse level is not the fine level halved");
+   return ampe::quat_refine(*fine->L, *coarse->L, dx_fine, qlen, qc, qf, hs(stream));
+}
+
 int ampe_coarsen_side(const ampe_level* coarse, const ampe_level* fine, const double* f,
                       double* c, ampe_stream_t stream)
 {

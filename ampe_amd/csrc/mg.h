@@ -91,4 +91,12 @@ int mg_prolong_add(const Level& fine, const Level& coarse, int ncomp, int order,
 // coarse holds the boxes of fine halved, patch by patch (even corners/extents)
 bool is_coarsening(const Level& coarse, const Level& fine);
 
+// Orientation field (ql <= 4 components) coarsened by the normalised volume
+// average, or refined by the normalised (bi/tri)linear interpolation (coarse
+// ghosts filled); dxf = fine mesh spacing (quat_transfer.hip).
+int quat_coarsen(const Level& coarse, const Level& fine, const double* dxf, int ql,
+                 const double* qf, double* qc, hipStream_t s);
+int quat_refine(const Level& fine, const Level& coarse, const double* dxf, int ql,
+                const double* qc, double* qf, hipStream_t s);
+
 }  // namespace ampe

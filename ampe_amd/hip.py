@@ -37,6 +37,8 @@ _SIGS = {
     "ampe_level_patch_strides": (_i, [_vp, C.POINTER(_ll), C.POINTER(_ll)]),
     "ampe_sim_set_temperature_ramp": (_i, [_vp, _d, _d, _d, _vp]),
     "ampe_cluster_mask": (_i, [_i, _vp, _vp, _vp, _vp, _i, _vp]),
+    "ampe_quat_coarsen": (_i, [_vp, _vp, _vp, _i, _vp, _vp, _vp]),
+    "ampe_quat_refine": (_i, [_vp, _vp, _vp, _i, _vp, _vp, _vp]),
     "ampe_cluster_tags": (_i, [_vp, _vp, _vp, _vp, _i, _vp, _vp]),
     "ampe_ghost_fill": (_i, [_vp, _vp, _i, _vp]),
     "ampe_quat_grad": (_i, [_vp, C.POINTER(ModelParams), _vp, _vp, _vp, _vp]),

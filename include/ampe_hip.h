@@ -432,6 +432,18 @@ int ampe_coarsen_side(const ampe_level* coarse, const ampe_level* fine, const do
                       double* c, ampe_stream_t stream);
 int ampe_refine_add(const ampe_level* fine, const ampe_level* coarse, int ncomp, int order,
                     const double* c, double* f, ampe_stream_t stream);
+/* Orientation field of qlen <= 4 components: coarsened by QuatWeightedAverage
+ * (source/QuatWeightedAverage.cc:100-171, F77 quatcoarsen: volume average of
+ * the children, then q/|q|) and refined (set, not added) by QuatLinearRefine
+ * (source/QuatLinearRefine.cc:107-174, F77 quatlinrefine: bi/trilinear
+ * interpolation of the coarse cells, then q/|q|; coarse ghosts filled).  The
+ * reference's symmetry rotation with index 0 is applied as the identity (its
+ * qlen 1 / 2 behaviour; for qlen 4 the reference stops).  dx_fine: fine mesh
+ * spacing. */
+int ampe_quat_coarsen(const ampe_level* coarse, const ampe_level* fine, const double* dx_fine,
+                      int qlen, const double* qf, double* qc, ampe_stream_t stream);
+int ampe_quat_refine(const ampe_level* fine, const ampe_level* coarse, const double* dx_fine,
+                     int qlen, const double* qc, double* qf, ampe_stream_t stream);
 
 /* ----------------------------------------------------- vector reductions */
 /* Dot product of two ncomp-component cell fields over interior cells

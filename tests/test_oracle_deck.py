@@ -126,3 +126,24 @@ def test_two_grains_quadratic_regression():
     assert abs(fs - ck["solid_fraction"]) <= ck["solid_fraction_tol"], fs
     assert abs(max(vols) - ck["grain_vmax"]) <= ck["grain_tol"], vols
     assert abs(min(vols) - ck["grain_vmin"]) <= ck["grain_tol"], vols
+
+
+def test_quat_transfer_oracle_closed_forms():
+    """oracle/quat_transfer.py: a uniform unit quaternion is unchanged by
+    coarsening and refinement; the refinement weights are quatlinrefine's
+    3/4 (own coarse cell) and 1/4 (neighbour) per direction."""
+    OQ = pytest.importorskip("oracle.quat_transfer")
+    q0 = np.array([0.5, -0.5, 0.5, 0.5])
+    for shape in [(8, 12), (4, 6, 8)]:
+        qf = np.broadcast_to(q0.reshape((4,) + (1,) * len(shape)), (4,) + shape).copy()
+        c = OQ.coarsen(qf, (0.1, 0.1, 0.1))
+        assert np.max(np.abs(c - q0.reshape((4,) + (1,) * len(shape)))) == 0.0
+        assert np.max(np.abs(OQ.refine(c, (0.1, 0.1, 0.1)) - qf)) == 0.0
+    # two components varying along x only: fine cell 2i + 1 sits 1/4 of the
+    # way from coarse cell i to i + 1
+    qc = np.zeros((2, 4, 6))
+    qc[0] = np.cos(0.2 * np.arange(6))[None, :]
+    qc[1] = np.sin(0.2 * np.arange(6))[None, :]
+    r = OQ.refine(qc, (1.0, 1.0))
+    raw = 0.75 * qc[:, 0, 2] + 0.25 * qc[:, 0, 3]
+    assert np.allclose(r[:, 0, 5], raw / np.sqrt((raw ** 2).sum()), rtol=0, atol=1e-15)
