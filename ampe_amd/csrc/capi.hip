@@ -204,6 +204,15 @@ int ampe_conc_rhs_ebs(const ampe_level* level, const ampe_model_params* p, const
    return ampe::launch_conc_rhs_ebs(*level->L, *p, cl, cs, DL, DA, rhs, hs(stream));
 }
 
+int ampe_conc_rhs_ebs_iso(const ampe_level* level, const ampe_model_params* p,
+                          const double* phi_d, const double* cl, const double* cs,
+                          double* rhs, ampe_stream_t stream)
+{
+   AMPE_REQUIRE(level && phi_d && cl && cs && rhs, "null argument");
+   if (check_params(p, *level->L)) return -1;
+   return ampe::launch_conc_rhs_ebs_iso(*level->L, *p, phi_d, cl, cs, rhs, hs(stream));
+}
+
 int ampe_conc_rhs(const ampe_level* level, const ampe_model_params* p, const double* c,
                   const double* phi, const double* Dc, const double* Dphi, double* rhs,
                   ampe_stream_t stream)

@@ -524,6 +524,92 @@ __global__ void __launch_bounds__(256) k_conc_rhs_ebs(LevelArgs a, ampe_model_pa
 AMPE_COLUMN_END
 }
 
+// The same with add_flux_iso (2d/flux.m4:69-135, 3d/flux.m4:103-232): each face
+// flux adds, per phase, its transverse neighbour faces (weight 1/12) and the
+// face itself (10/12 in 2-D, 8/12 in 3-D) in the reference's order -- 2-D
+// +t, 0, -t as (w D) dc; 3-D w (D dc) over x: -y, -z, +y, +z; y: -x, +x, -z, +z;
+// z: -x, +x, -y, +y; then the face -- with every face diffusivity evaluated
+// from phid (the arithmetic of k_ebs_coeffs).
+template <int NDIM>
+__device__ __forceinline__ double ebs_iso_flux(const ampe_model_params& p, const RhsConst& k,
+                                               const CellPos& c, int d, long long h0,
+                                               double w1, double wc,
+                                               const double* __restrict__ phid,
+                                               const double* __restrict__ cl,
+                                               const double* __restrict__ cs)
+{
+   long long to[4];
+   if (NDIM == 2) {
+      long long t = c.st[1 - d];
+      to[0] = t;
+      to[1] = 0;
+      to[2] = -t;
+   } else {
+      const int a = d == 0 ? 1 : 0, b = d == 2 ? 1 : 2;
+      if (d == 0) {
+         to[0] = -c.st[a];
+         to[1] = -c.st[b];
+         to[2] = c.st[a];
+         to[3] = c.st[b];
+      } else {
+         to[0] = -c.st[a];
+         to[1] = c.st[a];
+         to[2] = -c.st[b];
+         to[3] = c.st[b];
+      }
+   }
+   double f = 0.0;
+#pragma unroll
+   for (int ph = 0; ph < 2; ph++) {
+      const double* cc = ph == 0 ? cl : cs;
+      if (NDIM == 2) {
+#pragma unroll
+         for (int t = 0; t < 3; t++) {
+            long long h = h0 + to[t], m = h - c.st[d];
+            double hphi = interp(face_avg(phid[m], phid[h], p.avg_type), p.diff_interp);
+            double dd = ph == 0 ? (1.0 - hphi) * k.Dl : hphi * k.Ds;
+            double w = t == 1 ? wc : w1;
+            f = f + w * dd * (cc[h] - cc[m]);
+         }
+      } else {
+#pragma unroll
+         for (int t = 0; t < 5; t++) {
+            long long h = h0 + (t < 4 ? to[t] : 0), m = h - c.st[d];
+            double hphi = interp(face_avg(phid[m], phid[h], p.avg_type), p.diff_interp);
+            double dd = ph == 0 ? (1.0 - hphi) * k.Dl : hphi * k.Ds;
+            double w = t < 4 ? w1 : wc;
+            f = f + w * (dd * (cc[h] - cc[m]));
+         }
+      }
+   }
+   return f;
+}
+
+template <int NDIM>
+__global__ void __launch_bounds__(256) k_conc_rhs_ebs_iso(LevelArgs a, ampe_model_params p,
+                                                          RhsConst k,
+                                                          const double* __restrict__ phid,
+                                                          const double* __restrict__ cl,
+                                                          const double* __restrict__ cs,
+                                                          double* __restrict__ rhs)
+{
+   AMPE_COLUMN_BEGIN(a, c)
+   const long long c0 = c.c0;
+   double s = 0.0;
+#pragma unroll
+   for (int d = 0; d < NDIM; d++) {
+      double dinv = k.dxinv[d];
+      double w1 = 1.0 / (12.0 * p.dx[d]);
+      double wc = (NDIM == 2 ? 10.0 : 8.0) / (12.0 * p.dx[d]);
+      double jl = ebs_iso_flux<NDIM>(p, k, c, d, c0, w1, wc, phid, cl, cs);
+      double jh = ebs_iso_flux<NDIM>(p, k, c, d, c0 + c.st[d], w1, wc, phid, cl, cs);
+      double t = dinv * (jh - jl);
+      s = (d == 0) ? t : s + t;
+   }
+   rhs[c0] = p.conc_mobility * s;
+   AMPE_COLUMN_END
+}
+
 // QuatMobilityStrategy::computePhaseMobility over interior cells
 // (QuatIntegrator::computeMobilities, source/QuatIntegrator.cc:2959-2992)
 __global__ void __launch_bounds__(256) k_phase_mobility(LevelArgs a, ampe_model_params p,
@@ -716,6 +802,14 @@ int launch_conc_rhs_ebs(const Level& L, const ampe_model_params& p, const double
                         hipStream_t s)
 {
    AMPE_DISPATCH(L, k_conc_rhs_ebs, L.ncol_tiles, level_args_col(L), p, rhs_const(p), cl, cs, DL, DA, rhs);
+   return 0;
+}
+
+int launch_conc_rhs_ebs_iso(const Level& L, const ampe_model_params& p, const double* phid,
+                            const double* cl, const double* cs, double* rhs, hipStream_t s)
+{
+   AMPE_DISPATCH(L, k_conc_rhs_ebs_iso, L.ncol_tiles, level_args_col(L), p, rhs_const(p), phid,
+                 cl, cs, rhs);
    return 0;
 }
 

@@ -33,6 +33,7 @@ struct Sim {
    double *n2 = nullptr, *Dc = nullptr, *Dphi = nullptr, *Dconc = nullptr, *fc = nullptr;
    double* fcq = nullptr;  // orientation face coefficients of the current RHS evaluation
    double *Cphi = nullptr, *sqrtm = nullptr, *Mphi = nullptr;
+   double* phid = nullptr;  // phase field of the EBS isotropic stencil's diffusivities
    double *fac_u = nullptr, *fac_e = nullptr, *fac_r = nullptr;
    // FP32 preconditioner copies (mixed precision)
    float *f_u = nullptr, *f_e = nullptr, *f_r = nullptr, *f_f = nullptr;

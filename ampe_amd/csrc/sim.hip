@@ -71,6 +71,13 @@ int sim_create(const ampe_model_params& p, const ampe_solver_params& s, const am
       set_error("sim_create: unknown composition rhs form");
       return -1;
    }
+   if (p.conc_rhs_form == AMPE_CONC_RHS_EBS && p.ebs_stencil != AMPE_EBS_REGULAR &&
+       p.ebs_stencil != AMPE_EBS_ISOTROPIC) {
+      set_error(p.ebs_stencil == AMPE_EBS_ORDER4
+                    ? "sim_create: the order4 EBS stencil needs two ghost cells (not supported)"
+                    : "sim_create: unknown EBS stencil");
+      return -1;
+   }
    Sim* S = new Sim();
    S->p = p;
    S->s = s;
@@ -93,7 +100,8 @@ int sim_create(const ampe_model_params& p, const ampe_solver_params& s, const am
    for (auto v : vecs) rc |= dalloc(v, S->nv);
    S->V.assign(s.maxl + 1, nullptr);
    for (auto& v : S->V) rc |= dalloc(&v, S->nv);
-   double** cells[] = {&S->cl, &S->cs, &S->clr, &S->csr, &S->Cphi, &S->sqrtm, &S->Mphi};
+   double** cells[] = {&S->cl, &S->cs, &S->clr, &S->csr, &S->Cphi, &S->sqrtm, &S->Mphi,
+                       &S->phid};
    for (auto v : cells) rc |= dalloc(v, S->ng);
    rc |= dalloc(&S->fac_u, S->ng * ql);
    rc |= dalloc(&S->fac_e, S->nv);
@@ -141,7 +149,7 @@ void sim_destroy(Sim* S)
                     S->ewt, S->work, S->work2, S->jv, S->xk, S->corr, S->acor, S->cl, S->cs,
                     S->clr, S->csr, S->Cphi, S->sqrtm, S->Mphi, S->fac_u, S->fac_e, S->fac_r,
                     S->n2,
-                    S->Dc, S->Dphi, S->Dconc, S->fc, S->fcq};
+                    S->Dc, S->Dphi, S->Dconc, S->fc, S->fcq, S->phid};
    for (double* v : all) (void)hipFree(v);
    for (double* v : S->V) (void)hipFree(v);
    float* fall[] = {S->f_u,    S->f_e,     S->f_r,     S->f_f,   S->f_Cphi,
@@ -315,6 +323,17 @@ static int rhs_eval(Sim* S, double* y, double* ydot, bool fd)
    if (c && p.conc_rhs_form == AMPE_CONC_RHS_EBS) {
       // Dc, Dphi hold the liquid and solid face diffusivities
       if (recompute && launch_ebs_coeffs(L, p, phi, S->Dc, S->Dphi, st)) return -1;
+      if (p.ebs_stencil == AMPE_EBS_ISOTROPIC) {
+         // the diffusivities follow the lagging of the face data: the
+         // isotropic stencil evaluates them from the phase field of the last
+         // full evaluation, kept in phid (ghosts filled)
+         if (recompute)
+            AMPE_HIP_CHECK(hipMemcpyAsync(S->phid, phi, sizeof(double) * S->ng,
+                                          hipMemcpyDeviceToDevice, st));
+         if (launch_conc_rhs_ebs_iso(L, p, S->phid, S->cl, S->cs,
+                                     ydot + (long long)S->ic * S->ng, st))
+            return -1;
+      } else
       if (launch_conc_rhs_ebs(L, p, S->cl, S->cs, S->Dc, S->Dphi,
                               ydot + (long long)S->ic * S->ng, st))
          return -1;

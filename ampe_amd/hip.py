@@ -38,6 +38,7 @@ _SIGS = {
     "ampe_sim_set_temperature_ramp": (_i, [_vp, _d, _d, _d, _vp]),
     "ampe_cluster_mask": (_i, [_i, _vp, _vp, _vp, _vp, _i, _vp]),
     "ampe_quat_coarsen": (_i, [_vp, _vp, _vp, _i, _vp, _vp, _vp]),
+    "ampe_conc_rhs_ebs_iso": (_i, [_vp, C.POINTER(ModelParams), _vp, _vp, _vp, _vp, _vp]),
     "ampe_quat_refine": (_i, [_vp, _vp, _vp, _i, _vp, _vp, _vp]),
     "ampe_cluster_tags": (_i, [_vp, _vp, _vp, _vp, _i, _vp, _vp]),
     "ampe_ghost_fill": (_i, [_vp, _vp, _i, _vp]),

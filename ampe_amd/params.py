@@ -90,6 +90,7 @@ class ModelParams(C.Structure):
         ("kim_antitrapping", C.c_int),
         ("conc_rhs_form", C.c_int),
         ("diff_interp", C.c_char),
+        ("ebs_stencil", C.c_int),
     ]
 
 
@@ -141,6 +142,7 @@ FLUX_SIMPLE, FLUX_ISOTROPIC, FLUX_ANISOTROPIC = 0, 1, 2
 FE_QUADRATIC, FE_CALPHAD_BINARY, FE_KKS_DILUTE = 0, 1, 2
 PHI_MOB_CONSTANT, PHI_MOB_KIM = 0, 1
 CONC_RHS_KKS, CONC_RHS_EBS = 0, 1
+EBS_REGULAR, EBS_ISOTROPIC, EBS_ORDER4 = 0, 1, 2
 FP64, FAC_FP32 = 0, 1
 GUESS_PREVIOUS, GUESS_SURROGATE = 0, 1
 PRECOND_REDBLACK, PRECOND_MG = 0, 1

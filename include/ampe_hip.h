@@ -122,6 +122,7 @@ typedef struct ampe_model_params {
    int conc_rhs_form;     /* enum ampe_conc_rhs_form */
    char diff_interp;      /* EBS phase weights h(phi) of the diffusivities
                            * (diffusion_interp_func_type, 'l' linear, 'p' pbg) */
+   int ebs_stencil;       /* enum ampe_ebs_stencil (ConcentrationModel ebs_stencil) */
 } ampe_model_params;
 
 /* Phase mobility: the uniform M_phi (SimpleQuatMobilityStrategy,
@@ -134,6 +135,13 @@ enum ampe_phi_mobility { AMPE_PHI_MOB_CONSTANT = 0, AMPE_PHI_MOB_KIM = 1 };
  * D_phi grad phi) or EBS (EBSCompositionRHSStrategy, flux sum over phases of
  * D_phase grad c_phase, source/EBSCompositionRHSStrategy.cc:183-328). */
 enum ampe_conc_rhs_form { AMPE_CONC_RHS_KKS = 0, AMPE_CONC_RHS_EBS = 1 };
+/* EBS flux stencil (EBSCompositionRHSStrategy, source/EBSCompositionRHSStrategy.cc
+ * :149-154; QuatModelParameters ebs_stencil): "regular" add_flux
+ * (2d/flux.m4:11-66), "isotropic" add_flux_iso (2d/flux.m4:69-135, 3d/flux.m4
+ * :103-232; transverse neighbours weighted 1/12, the face itself 10/12 in 2-D,
+ * 8/12 in 3-D).  "order4" (add_flux_4th) needs two ghost cells and is not
+ * supported (the levels here have one). */
+enum ampe_ebs_stencil { AMPE_EBS_REGULAR = 0, AMPE_EBS_ISOTROPIC = 1, AMPE_EBS_ORDER4 = 2 };
 enum ampe_precision { AMPE_FP64 = 0, AMPE_FAC_FP32 = 1 };
 enum ampe_kks_guess { AMPE_GUESS_PREVIOUS = 0, AMPE_GUESS_SURROGATE = 1 };
 /* Block solver inside each FAC cycle: the reference's single-level red-black
@@ -310,6 +318,13 @@ int ampe_conc_coeffs_ebs(const ampe_level* level, const ampe_model_params* p,
 int ampe_conc_rhs_ebs(const ampe_level* level, const ampe_model_params* p,
                       const double* cl, const double* cs, const double* DL,
                       const double* DA, double* rhs, ampe_stream_t stream);
+/* The same with the isotropic stencil (add_flux_iso): the face diffusivities,
+ * D_L = (1 - h(phi_f)) D_L0 e^{-Q_L/RT}, D_A = h(phi_f) D_S0 e^{-Q_S/RT} with
+ * phi_f the face average of phi_d, are evaluated for every face the stencil
+ * touches, including those beside the patch (phi_d needs filled ghosts). */
+int ampe_conc_rhs_ebs_iso(const ampe_level* level, const ampe_model_params* p,
+                          const double* phi_d, const double* cl, const double* cs,
+                          double* rhs, ampe_stream_t stream);
 
 /* Event timing of one kernel class inside whatever runs while it is enabled
  * (used by the benchmark to measure a kernel live inside the timed steps).

@@ -58,7 +58,7 @@ struct OSim {
    double** V;
    double *cl, *cs, *clr, *csr;
    double *n2, *Dc, *Dphi;
-   double *Cphi, *Dconc, *fc, *sqrtm, *mq, *Mphi;
+   double *Cphi, *Dconc, *fc, *sqrtm, *mq, *Mphi, *phid;
    double *fac_u, *fac_e, *fac_r, *fac_flux;
    double* H;
    ampe_stats st;
@@ -127,6 +127,7 @@ OSim* o_sim_create(const ampe_model_params* p, const ampe_solver_params* s,
    S->csr = vec(ng);
    S->Cphi = vec(ng);
    S->Mphi = vec(ng);
+   S->phid = vec(ng);
    S->sqrtm = vec(ng);
    S->mq = vec(ng);
    S->fac_u = vec(ng * (p->evolve_quat ? p->qlen : 1));
@@ -150,7 +151,7 @@ void o_sim_destroy(OSim* S)
                    S->del, S->ewt, S->work, S->work2, S->jv, S->xk, S->corr,
                    S->acor, S->cl, S->cs, S->clr, S->csr, S->Cphi, S->Mphi, S->sqrtm,
                    S->mq, S->fac_u, S->fac_e, S->fac_r, S->n2, S->Dc, S->Dphi,
-                   S->Dconc, S->fc, S->fac_flux, S->H};
+                   S->Dconc, S->fc, S->fac_flux, S->H, S->phid};
    for (size_t i = 0; i < sizeof(fs) / sizeof(fs[0]); i++) free(fs[i]);
    for (int i = 0; i <= S->s.maxl; i++) free(S->V[i]);
    free(S->V);
@@ -250,7 +251,13 @@ static int rhs_eval(OSim* S, double* y, double* ydot, int fd)
       if (p->conc_rhs_form == AMPE_CONC_RHS_EBS) {
          /* Dc, Dphi hold the liquid and solid face diffusivities */
          if (recompute) o_ebs_coeffs(G, p, phi, S->Dc, S->Dphi);
-         o_conc_rhs_ebs(G, p, S->cl, S->cs, S->Dc, S->Dphi, ydot + (long)S->ic * ng);
+         if (p->ebs_stencil == AMPE_EBS_ISOTROPIC) {
+            /* diffusivities from the phase field of the last full evaluation */
+            if (recompute) memcpy(S->phid, phi, sizeof(double) * ng);
+            o_conc_rhs_ebs_iso(G, p, S->phid, S->cl, S->cs, ydot + (long)S->ic * ng);
+         } else {
+            o_conc_rhs_ebs(G, p, S->cl, S->cs, S->Dc, S->Dphi, ydot + (long)S->ic * ng);
+         }
       } else {
          if (recompute) o_conc_coeffs(G, p, phi, S->cl, S->cs, S->Dc, S->Dphi);
          o_conc_rhs(G, p, c, phi, S->Dc, S->Dphi, ydot + (long)S->ic * ng);

@@ -1054,6 +1054,102 @@ void o_conc_rhs_ebs(const OGeom* G, const ampe_model_params* p, const double* cl
    free(fl);
 }
 
+/* Liquid and solid diffusivities of the face between cells cm and c0
+ * (the arithmetic of o_ebs_coeffs). */
+static void ebs_face_d(const ampe_model_params* p, double phm, double ph0, double* dl,
+                       double* ds)
+{
+   double qlr = p->Q0_liquid / R_GAS, qsr = p->Q0_solid / R_GAS;
+   double vphi = o_average_func(phm, ph0, p->avg_type);
+   double hphi = o_interp_func(vphi, p->diff_interp);
+   double invT = 2.0 / (p->temperature + p->temperature);
+   *dl = (1.0 - hphi) * (p->D_liquid * exp(-qlr * invT));
+   *ds = hphi * (p->D_solid * exp(-qsr * invT));
+}
+
+/* EBS flux with add_flux_iso (2d/flux.m4:69-135: the face and its two
+ * transverse neighbour faces weighted 10/12 and 1/12, terms in the order
+ * +t, 0, -t as (w D) dc; 3d/flux.m4:103-232: the four transverse neighbours
+ * weighted 1/12 and the face 8/12, terms w (D dc) in the order x: -y, -z,
+ * +y, +z; y: -x, +x, -z, +z; z: -x, +x, -y, +y; then the face), liquid then
+ * solid added to a zero flux; diffusivities of every face from phid (ghosts
+ * filled), then computerhsconcentration as o_conc_rhs_ebs. */
+void o_conc_rhs_ebs_iso(const OGeom* G, const ampe_model_params* p, const double* phid,
+                        const double* cl, const double* cs, double* rhs)
+{
+   long st[3] = {1, G->sy, G->sz};
+   double dinv[3] = {1.0 / G->dx[0], 1.0 / G->dx[1], 1.0 / G->dx[2]};
+   long nf = G->nfx + G->nfy + G->nfz;
+   double* fl = (double*)malloc(sizeof(double) * nf);
+   for (int dir = 0; dir < G->ndim; dir++) {
+      int ex = G->nx + (dir == 0), ey = G->ny + (dir == 1), ez = G->nz + (dir == 2);
+      long off = (dir == 0 ? 0 : dir == 1 ? YOFF : ZOFF);
+      double w1 = 1.0 / (12.0 * G->dx[dir]);
+      double wc = (G->ndim == 2 ? 10.0 : 8.0) / (12.0 * G->dx[dir]);
+      /* transverse offsets in the reference's order */
+      long to[4];
+      int nt;
+      if (G->ndim == 2) {
+         long t = st[1 - dir];
+         to[0] = t;
+         to[1] = 0;
+         to[2] = -t;
+         nt = 3;
+      } else {
+         int a = dir == 0 ? 1 : 0, b = dir == 2 ? 1 : 2;
+         if (dir == 0) {
+            to[0] = -st[a]; to[1] = -st[b]; to[2] = st[a]; to[3] = st[b];
+         } else {
+            to[0] = -st[a]; to[1] = st[a]; to[2] = -st[b]; to[3] = st[b];
+         }
+         nt = 4;
+      }
+      for (int k = 0; k < ez; k++)
+         for (int j = 0; j < ey; j++)
+            for (int i = 0; i < ex; i++) {
+               long c0 = CI(i, j, k);
+               long fi = off + (long)i + (long)ex * ((long)j + (long)ey * k);
+               double f = 0.0;
+               for (int ph = 0; ph < 2; ph++) {
+                  const double* cc = ph == 0 ? cl : cs;
+                  double acc = f;
+                  if (G->ndim == 2) {
+                     for (int t = 0; t < nt; t++) {
+                        long h = c0 + to[t], m = h - st[dir];
+                        double dl, ds;
+                        ebs_face_d(p, phid[m], phid[h], &dl, &ds);
+                        double dd = ph == 0 ? dl : ds;
+                        double w = to[t] == 0 ? wc : w1;
+                        acc = acc + w * dd * (cc[h] - cc[m]);
+                     }
+                  } else {
+                     for (int t = 0; t < 5; t++) {
+                        long o = t < 4 ? to[t] : 0;
+                        long h = c0 + o, m = h - st[dir];
+                        double dl, ds;
+                        ebs_face_d(p, phid[m], phid[h], &dl, &ds);
+                        double dd = ph == 0 ? dl : ds;
+                        double w = t < 4 ? w1 : wc;
+                        acc = acc + w * (dd * (cc[h] - cc[m]));
+                     }
+                  }
+                  f = acc;
+               }
+               fl[fi] = f;
+            }
+   }
+   for (int k = 0; k < G->nz; k++)
+      for (int j = 0; j < G->ny; j++)
+         for (int i = 0; i < G->nx; i++) {
+            double s = dinv[0] * (fl[FX(i + 1, j, k)] - fl[FX(i, j, k)]) +
+                       dinv[1] * (fl[YOFF + FY(i, j + 1, k)] - fl[YOFF + FY(i, j, k)]);
+            if (G->ndim == 3)
+               s = s + dinv[2] * (fl[ZOFF + FZ(i, j, k + 1)] - fl[ZOFF + FZ(i, j, k)]);
+            rhs[CI(i, j, k)] = p->conc_mobility * s;
+         }
+   free(fl);
+}
+
 /* Grains::findAndNumberGrains (source/Grains.cc:263-645) on one patch
  * covering the domain (cell weights all positive): number the cells
  * with phase >= thr by their box index (x fastest), -1 elsewhere and in the

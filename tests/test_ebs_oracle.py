@@ -136,3 +136,47 @@ def test_integrator_ebs_conserves_composition():
     _, c1, _, cl, cs = sim.get_state()
     assert abs(c1.mean() - c.mean()) < 1e-12
     assert np.abs(c1 - c).max() > 0
+
+
+def _ebs_iso(pt, p, phi, cl, cs):
+    r = pt.cells()
+    O.lib().o_conc_rhs_ebs_iso(C.byref(pt.G), C.byref(p), O.ptr(phi), O.ptr(cl), O.ptr(cs),
+                               O.ptr(r))
+    return r
+
+
+@pytest.mark.parametrize("ndim", [2, 3])
+def test_isotropic_stencil_reduces_to_regular_on_one_dimensional_fields(ndim):
+    """add_flux_iso's weights sum to one (10/12 + 2/12 in 2-D, 8/12 + 4/12 in
+    3-D): on fields varying along x only every transverse face carries the
+    same diffusivity and difference, so the isotropic and regular stencils
+    agree to round-off; with variation along y as well they differ."""
+    dx = (0.5, 0.25, 0.125)
+    p = P.two_grain_params(ndim=ndim, dx=dx, conc_rhs_form=P.CONC_RHS_EBS, diff_interp=b"p")
+    pt = O.Patch(ndim, (12, 10, 8), dx=dx)
+    rng = np.random.default_rng(3)
+    prof = lambda: rng.uniform(0.1, 0.9, pt.gshape[-1])[None, None, :]
+    phi = np.broadcast_to(prof(), pt.gshape).copy()[None]
+    cl = np.broadcast_to(prof(), pt.gshape).copy()[None]
+    cs = np.broadcast_to(prof(), pt.gshape).copy()[None]
+    for a in (phi, cl, cs):
+        O.lib().o_ghost_fill_periodic(C.byref(pt.G), O.ptr(a), 1)
+    r_reg = _ebs(pt, p, phi, cl, cs)[2]
+    r_iso = _ebs_iso(pt, p, phi, cl, cs)
+    err = np.abs(pt.interior(r_iso - r_reg)).max() / np.abs(pt.interior(r_reg)).max()
+    assert err < 1e-13
+    cl2 = cl + 0.05 * rng.normal(size=cl.shape)
+    O.lib().o_ghost_fill_periodic(C.byref(pt.G), O.ptr(cl2), 1)
+    assert np.abs(pt.interior(_ebs_iso(pt, p, phi, cl2, cs) - _ebs(pt, p, phi, cl2, cs)[2])).max() > 1e-8
+
+
+@pytest.mark.parametrize("ndim", [2, 3])
+def test_isotropic_stencil_conserves(ndim):
+    """Periodic patch: the flux divergence sums to zero."""
+    dx = (0.5, 0.25, 0.125)
+    p = P.two_grain_params(ndim=ndim, dx=dx, conc_rhs_form=P.CONC_RHS_EBS, diff_interp=b"p")
+    pt = O.Patch(ndim, (12, 10, 8), dx=dx)
+    rng = np.random.default_rng(5)
+    phi, cl, cs = (pt.load(pt.cells(), rng.uniform(0, 1, (pt.nz, pt.ny, pt.nx))) for _ in range(3))
+    r = _ebs_iso(pt, p, phi, cl, cs)
+    assert abs(pt.interior(r).sum()) < 1e-12 * np.abs(pt.interior(r)).sum()

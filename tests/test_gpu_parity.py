@@ -192,6 +192,26 @@ def test_conc_ebs(dev, ndim, interp):
     assert _rel(pt.interior(r.cpu().numpy()), pt.interior(r_o)) < 1e-13
 
 
+@pytest.mark.parametrize("ndim", [2, 3])
+def test_conc_ebs_isotropic(dev, ndim):
+    """EBS composition form with the isotropic stencil (add_flux_iso): the
+    device kernel, which evaluates every face diffusivity it touches from phi,
+    against the oracle's restatement."""
+    p = _params(ndim, "simple")
+    p.diff_interp = b"p"
+    p.ebs_stencil = P.EBS_ISOTROPIC
+    O, H, pt, lev = _setup(ndim, p)
+    phi, q, c, cl, cs = _fields(pt, seed=31)
+    r_o = pt.cells()
+    O.lib().o_conc_rhs_ebs_iso(C.byref(pt.G), C.byref(p), O.ptr(phi), O.ptr(cl), O.ptr(cs),
+                               O.ptr(r_o))
+    phd, cld, csd = (_dev(a, dev) for a in (phi, cl, cs))
+    r = _dev(np.zeros_like(r_o), dev)
+    H.check(H.lib().ampe_conc_rhs_ebs_iso(lev.h, C.byref(p), _p(phd), _p(cld), _p(csd), _p(r),
+                                          None))
+    assert _rel(pt.interior(r.cpu().numpy()), pt.interior(r_o)) < 1e-13
+
+
 @pytest.mark.parametrize("ndim,kind", [(2, "simple"), (3, "calphad"), (3, "dilute")])
 def test_kks(dev, ndim, kind):
     p = _params(ndim, kind)
@@ -474,7 +494,8 @@ def test_config1_two_grain_20_steps_matches_oracle(dev):
 @pytest.mark.parametrize("model,gstype", [("calphad", P.GS_MODIFIED),
                                           ("calphad", P.GS_CLASSICAL),
                                           ("dilute_kim", P.GS_MODIFIED),
-                                          ("calphad_ebs", P.GS_MODIFIED)])
+                                          ("calphad_ebs", P.GS_MODIFIED),
+                                          ("calphad_ebs_iso", P.GS_MODIFIED)])
 def test_integrator_3d_kks_matches_oracle(dev, model, gstype):
     """3-D polycrystal with a KKS composition model, FP64 GPU integrator against
     the oracle step by step: binary CALPHAD (Newton inside every RHS
@@ -486,8 +507,10 @@ def test_integrator_3d_kks_matches_oracle(dev, model, gstype):
     n = 16
     if model.startswith("calphad"):
         p = P.polycrystal_params(seed=1002)
-        if model == "calphad_ebs":
+        if model.startswith("calphad_ebs"):
             p.conc_rhs_form = P.CONC_RHS_EBS
+        if model == "calphad_ebs_iso":
+            p.ebs_stencil = P.EBS_ISOTROPIC
         dt = 1e-7
     else:
         p = P.dilute_params(ndim=3, phi_mobility_type=P.PHI_MOB_KIM, D_liquid=1.0e3,
