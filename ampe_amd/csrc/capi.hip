@@ -204,6 +204,15 @@ int ampe_conc_rhs_ebs(const ampe_level* level, const ampe_model_params* p, const
    return ampe::launch_conc_rhs_ebs(*level->L, *p, cl, cs, DL, DA, rhs, hs(stream));
 }
 
+int ampe_conc_rhs_ebs_4th(const ampe_level* level, const ampe_model_params* p, const double* cl,
+                          const double* cs, const double* DL, const double* DA, double* rhs,
+                          ampe_stream_t stream)
+{
+   AMPE_REQUIRE(level && cl && cs && DL && DA && rhs, "null argument");
+   if (check_params(p, *level->L)) return -1;
+   return ampe::launch_conc_rhs_ebs_4th(*level->L, *p, cl, cs, DL, DA, rhs, hs(stream));
+}
+
 int ampe_conc_rhs_ebs_iso(const ampe_level* level, const ampe_model_params* p,
                           const double* phi_d, const double* cl, const double* cs,
                           double* rhs, ampe_stream_t stream)

@@ -161,6 +161,9 @@ int launch_ebs_coeffs(const Level& L, const ampe_model_params& p, const double* 
 int launch_conc_rhs_ebs(const Level& L, const ampe_model_params& p, const double* cl,
                         const double* cs, const double* DL, const double* DA, double* rhs,
                         hipStream_t s);
+int launch_conc_rhs_ebs_4th(const Level& L, const ampe_model_params& p, const double* cl,
+                            const double* cs, const double* DL, const double* DA, double* rhs,
+                            hipStream_t s);
 int launch_conc_rhs_ebs_iso(const Level& L, const ampe_model_params& p, const double* phid,
                             const double* cl, const double* cs, double* rhs, hipStream_t s);
 int launch_phase_mobility(const Level& L, const ampe_model_params& p, const double* cl,

@@ -42,6 +42,8 @@ struct PatchDev {
    long long side_off[3];  // patch faces offset inside direction block d
    long long int_off;      // compact interior offset (host I/O, tags)
    int lo[3];              // box lower corner (global index space)
+   int physb;              // bit 2d / 2d+1: lower / upper side in direction d lies on a
+                           // physical (non-periodic) boundary of the level's bounding box
 };
 
 struct Tile {

@@ -118,6 +118,32 @@ int level_create(int ndim, const ampe_box* boxes, int npatch, int ghost,
                        (n[2] + (a == 2));
       }
    }
+   // sides on the physical boundary (SAMRAI's getTouchesRegularBoundary): the
+   // level's bounding box stands for the domain, as every level built here
+   // covers it; none on a periodic domain
+   {
+      int blo[3], bhi[3];
+      for (int a = 0; a < 3; a++) {
+         blo[a] = L->hp[0].lo[a];
+         bhi[a] = blo[a];
+      }
+      for (const PatchDev& d : L->hp) {
+         int n[3] = {d.nx, d.ny, d.nz};
+         for (int a = 0; a < 3; a++) {
+            blo[a] = std::min(blo[a], d.lo[a]);
+            bhi[a] = std::max(bhi[a], d.lo[a] + n[a]);
+         }
+      }
+      for (PatchDev& d : L->hp) {
+         int n[3] = {d.nx, d.ny, d.nz};
+         d.physb = 0;
+         if (periodic) continue;
+         for (int a = 0; a < ndim; a++) {
+            if (d.lo[a] == blo[a]) d.physb |= 1 << (2 * a);
+            if (d.lo[a] + n[a] == bhi[a]) d.physb |= 1 << (2 * a + 1);
+         }
+      }
+   }
    L->ncg = coff;
    L->ncell = ioff;
    for (int a = 0; a < 3; a++) L->nside[a] = soff[a];

@@ -524,6 +524,62 @@ __global__ void __launch_bounds__(256) k_conc_rhs_ebs(LevelArgs a, ampe_model_pa
 AMPE_COLUMN_END
 }
 
+// The same with add_flux_4th (2d/flux.m4:139-242, 3d/flux.m4:235-420): each
+// face flux adds, per phase, D (15/12 (c_h - c_m) - 1/12 (c_hh - c_mm)) / dx over
+// the two cells on either side of the face (the divergence is then the
+// fourth-order Laplacian (-1, 16, -30, 16, -1) / 12 dx^2); a face on a
+// physical boundary (the patch's first or last face on a side whose physb bit
+// is set) keeps the second-order D (c_h - c_m) / dx.  Needs two ghost cells.
+__device__ __forceinline__ double ebs_4th_flux(double dxi, double w15, double w1, long long h,
+                                               long long st, bool bnd, double dl, double da,
+                                               const double* __restrict__ cl,
+                                               const double* __restrict__ cs)
+{
+   const long long m = h - st;
+   double f = 0.0;
+   if (bnd) {
+      f = f + dl * dxi * (cl[h] - cl[m]);
+      f = f + da * dxi * (cs[h] - cs[m]);
+   } else {
+      f = f + dl * (w15 * (cl[h] - cl[m]) - w1 * (cl[h + st] - cl[m - st]));
+      f = f + da * (w15 * (cs[h] - cs[m]) - w1 * (cs[h + st] - cs[m - st]));
+   }
+   return f;
+}
+
+template <int NDIM>
+__global__ void __launch_bounds__(256) k_conc_rhs_ebs_4th(LevelArgs a, ampe_model_params p,
+                                                          RhsConst k,
+                                                          const double* __restrict__ cl,
+                                                          const double* __restrict__ cs,
+                                                          const double* __restrict__ DL,
+                                                          const double* __restrict__ DA,
+                                                          double* __restrict__ rhs)
+{
+   AMPE_COLUMN_BEGIN(a, c)
+   const long long c0 = c.c0;
+   const int ijk[3] = {c.i, c.j, c.k};
+   const int n[3] = {c.pd.nx, c.pd.ny, c.pd.nz};
+   double s = 0.0;
+#pragma unroll
+   for (int d = 0; d < NDIM; d++) {
+      int lo[3] = {ijk[0], ijk[1], ijk[2]};
+      long long flo = face_idx(a, c.pd, d, lo[0], lo[1], lo[2]);
+      lo[d] += 1;
+      long long fhi = face_idx(a, c.pd, d, lo[0], lo[1], lo[2]);
+      const bool blo = ijk[d] == 0 && (c.pd.physb >> (2 * d) & 1);
+      const bool bhi = ijk[d] == n[d] - 1 && (c.pd.physb >> (2 * d + 1) & 1);
+      const double w15 = 15.0 / (12.0 * p.dx[d]), w1 = 1.0 / (12.0 * p.dx[d]);
+      double jl = ebs_4th_flux(k.dxinv[d], w15, w1, c0, c.st[d], blo, DL[flo], DA[flo], cl, cs);
+      double jh = ebs_4th_flux(k.dxinv[d], w15, w1, c0 + c.st[d], c.st[d], bhi, DL[fhi],
+                               DA[fhi], cl, cs);
+      double t = k.dxinv[d] * (jh - jl);
+      s = (d == 0) ? t : s + t;
+   }
+   rhs[c0] = p.conc_mobility * s;
+   AMPE_COLUMN_END
+}
+
 // The same with add_flux_iso (2d/flux.m4:69-135, 3d/flux.m4:103-232): each face
 // flux adds, per phase, its transverse neighbour faces (weight 1/12) and the
 // face itself (10/12 in 2-D, 8/12 in 3-D) in the reference's order -- 2-D
@@ -802,6 +858,19 @@ int launch_conc_rhs_ebs(const Level& L, const ampe_model_params& p, const double
                         hipStream_t s)
 {
    AMPE_DISPATCH(L, k_conc_rhs_ebs, L.ncol_tiles, level_args_col(L), p, rhs_const(p), cl, cs, DL, DA, rhs);
+   return 0;
+}
+
+int launch_conc_rhs_ebs_4th(const Level& L, const ampe_model_params& p, const double* cl,
+                            const double* cs, const double* DL, const double* DA, double* rhs,
+                            hipStream_t s)
+{
+   if (L.g < 2) {
+      set_error("conc_rhs_ebs_4th: the level needs two ghost cells");
+      return -1;
+   }
+   AMPE_DISPATCH(L, k_conc_rhs_ebs_4th, L.ncol_tiles, level_args_col(L), p, rhs_const(p), cl, cs,
+                 DL, DA, rhs);
    return 0;
 }
 

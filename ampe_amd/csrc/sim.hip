@@ -72,16 +72,20 @@ int sim_create(const ampe_model_params& p, const ampe_solver_params& s, const am
       return -1;
    }
    if (p.conc_rhs_form == AMPE_CONC_RHS_EBS && p.ebs_stencil != AMPE_EBS_REGULAR &&
-       p.ebs_stencil != AMPE_EBS_ISOTROPIC) {
-      set_error(p.ebs_stencil == AMPE_EBS_ORDER4
-                    ? "sim_create: the order4 EBS stencil needs two ghost cells (not supported)"
-                    : "sim_create: unknown EBS stencil");
+       p.ebs_stencil != AMPE_EBS_ISOTROPIC && p.ebs_stencil != AMPE_EBS_ORDER4) {
+      set_error("sim_create: unknown EBS stencil");
       return -1;
    }
    Sim* S = new Sim();
    S->p = p;
    S->s = s;
-   if (level_create(p.ndim, boxes, npatch, 1, periodic, &S->L)) {
+   // two ghost cells for the fourth-order composition stencil, one otherwise
+   // (QuatModelParameters::nghosts_required, source/QuatModelParameters.h:305-311)
+   const int ghost =
+       (p.with_conc && p.conc_rhs_form == AMPE_CONC_RHS_EBS && p.ebs_stencil == AMPE_EBS_ORDER4)
+           ? 2
+           : 1;
+   if (level_create(p.ndim, boxes, npatch, ghost, periodic, &S->L)) {
       delete S;
       return -1;
    }
@@ -333,10 +337,14 @@ static int rhs_eval(Sim* S, double* y, double* ydot, bool fd)
          if (launch_conc_rhs_ebs_iso(L, p, S->phid, S->cl, S->cs,
                                      ydot + (long long)S->ic * S->ng, st))
             return -1;
-      } else
-      if (launch_conc_rhs_ebs(L, p, S->cl, S->cs, S->Dc, S->Dphi,
-                              ydot + (long long)S->ic * S->ng, st))
+      } else if (p.ebs_stencil == AMPE_EBS_ORDER4) {
+         if (launch_conc_rhs_ebs_4th(L, p, S->cl, S->cs, S->Dc, S->Dphi,
+                                     ydot + (long long)S->ic * S->ng, st))
+            return -1;
+      } else if (launch_conc_rhs_ebs(L, p, S->cl, S->cs, S->Dc, S->Dphi,
+                                     ydot + (long long)S->ic * S->ng, st)) {
          return -1;
+      }
    } else if (c) {
       if (recompute && launch_conc_coeffs(L, p, phi, S->cl, S->cs, S->Dc, S->Dphi, st)) return -1;
       if (launch_conc_rhs(L, p, c, phi, S->Dc, S->Dphi, ydot + (long long)S->ic * S->ng, st))

@@ -139,8 +139,10 @@ enum ampe_conc_rhs_form { AMPE_CONC_RHS_KKS = 0, AMPE_CONC_RHS_EBS = 1 };
  * :149-154; QuatModelParameters ebs_stencil): "regular" add_flux
  * (2d/flux.m4:11-66), "isotropic" add_flux_iso (2d/flux.m4:69-135, 3d/flux.m4
  * :103-232; transverse neighbours weighted 1/12, the face itself 10/12 in 2-D,
- * 8/12 in 3-D).  "order4" (add_flux_4th) needs two ghost cells and is not
- * supported (the levels here have one). */
+ * 8/12 in 3-D), "order4" add_flux_4th (2d/flux.m4:139-242, 3d/flux.m4:235-420;
+ * fourth-order face gradients, second order on physical-boundary faces; the
+ * integrator's level then has two ghost cells, as nghosts_required gives,
+ * source/QuatModelParameters.h:305-311). */
 enum ampe_ebs_stencil { AMPE_EBS_REGULAR = 0, AMPE_EBS_ISOTROPIC = 1, AMPE_EBS_ORDER4 = 2 };
 enum ampe_precision { AMPE_FP64 = 0, AMPE_FAC_FP32 = 1 };
 enum ampe_kks_guess { AMPE_GUESS_PREVIOUS = 0, AMPE_GUESS_SURROGATE = 1 };
@@ -318,6 +320,15 @@ int ampe_conc_coeffs_ebs(const ampe_level* level, const ampe_model_params* p,
 int ampe_conc_rhs_ebs(const ampe_level* level, const ampe_model_params* p,
                       const double* cl, const double* cs, const double* DL,
                       const double* DA, double* rhs, ampe_stream_t stream);
+/* The same with the fourth-order stencil (add_flux_4th, 2d/flux.m4:139-242,
+ * 3d/flux.m4:235-420): face flux D (15 (c_h - c_m) - (c_hh - c_mm)) / (12 dx),
+ * but D (c_h - c_m) / dx on faces of the physical boundary (the sides of the
+ * level's bounding box of a non-periodic level: computeFluxOnPatch's physbc,
+ * source/EBSCompositionRHSStrategy.cc:236-255).  The level needs ghost >= 2
+ * and cl, cs two filled ghost layers. */
+int ampe_conc_rhs_ebs_4th(const ampe_level* level, const ampe_model_params* p,
+                          const double* cl, const double* cs, const double* DL,
+                          const double* DA, double* rhs, ampe_stream_t stream);
 /* The same with the isotropic stencil (add_flux_iso): the face diffusivities,
  * D_L = (1 - h(phi_f)) D_L0 e^{-Q_L/RT}, D_A = h(phi_f) D_S0 e^{-Q_S/RT} with
  * phi_f the face average of phi_d, are evaluated for every face the stencil

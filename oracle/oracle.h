@@ -61,6 +61,8 @@ int o_grains_label(const OGeom* G, const double* phase, double thr, int periodic
                    int* sweeps);
 void o_ebs_coeffs(const OGeom* G, const ampe_model_params* p, const double* phi, double* DL,
                   double* DA);
+void o_conc_rhs_ebs_4th(const OGeom* G, const ampe_model_params* p, const double* cl,
+                        const double* cs, const double* DL, const double* DA, double* rhs);
 void o_conc_rhs_ebs_iso(const OGeom* G, const ampe_model_params* p, const double* phid,
                         const double* cl, const double* cs, double* rhs);
 void o_conc_rhs_ebs(const OGeom* G, const ampe_model_params* p, const double* cl,

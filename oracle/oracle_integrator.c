@@ -108,7 +108,13 @@ OSim* o_sim_create(const ampe_model_params* p, const ampe_solver_params* s,
    OSim* S = (OSim*)calloc(1, sizeof(OSim));
    S->p = *p;
    S->s = *s;
-   o_geom_init(&S->G, p->ndim, nx, ny, nz, 1, p->dx);
+   /* two ghost cells for the fourth-order composition stencil
+    * (QuatModelParameters::nghosts_required, source/QuatModelParameters.h:305-311) */
+   int ghost = (p->with_conc && p->conc_rhs_form == AMPE_CONC_RHS_EBS &&
+                p->ebs_stencil == AMPE_EBS_ORDER4)
+                   ? 2
+                   : 1;
+   o_geom_init(&S->G, p->ndim, nx, ny, nz, ghost, p->dx);
    S->nc = 1 + (p->with_conc ? 1 : 0) + (p->evolve_quat ? p->qlen : 0);
    S->ic = p->with_conc ? 1 : -1;
    S->iq = p->evolve_quat ? 1 + (p->with_conc ? 1 : 0) : -1;
@@ -255,6 +261,8 @@ static int rhs_eval(OSim* S, double* y, double* ydot, int fd)
             /* diffusivities from the phase field of the last full evaluation */
             if (recompute) memcpy(S->phid, phi, sizeof(double) * ng);
             o_conc_rhs_ebs_iso(G, p, S->phid, S->cl, S->cs, ydot + (long)S->ic * ng);
+         } else if (p->ebs_stencil == AMPE_EBS_ORDER4) {
+            o_conc_rhs_ebs_4th(G, p, S->cl, S->cs, S->Dc, S->Dphi, ydot + (long)S->ic * ng);
          } else {
             o_conc_rhs_ebs(G, p, S->cl, S->cs, S->Dc, S->Dphi, ydot + (long)S->ic * ng);
          }

@@ -1054,6 +1054,55 @@ void o_conc_rhs_ebs(const OGeom* G, const ampe_model_params* p, const double* cl
    free(fl);
 }
 
+/* The same with add_flux_4th (2d/flux.m4:139-242, 3d/flux.m4:235-420):
+ * interior faces add d (15/12 (c_h - c_m) - 1/12 (c_hh - c_mm)) / dx per
+ * phase, the first and last face of a direction on a physical boundary
+ * (physb, source/EBSCompositionRHSStrategy.cc:236-255; every side of the
+ * single non-periodic patch) add d (c_h - c_m) / dx.  Needs G->g >= 2. */
+void o_conc_rhs_ebs_4th(const OGeom* G, const ampe_model_params* p, const double* cl,
+                        const double* cs, const double* DL, const double* DA, double* rhs)
+{
+   long st[3] = {1, G->sy, G->sz};
+   double dinv[3] = {1.0 / G->dx[0], 1.0 / G->dx[1], 1.0 / G->dx[2]};
+   long nf = G->nfx + G->nfy + G->nfz;
+   double* fl = (double*)malloc(sizeof(double) * nf);
+   for (int dir = 0; dir < G->ndim; dir++) {
+      int ex = G->nx + (dir == 0), ey = G->ny + (dir == 1), ez = G->nz + (dir == 2);
+      int nd = dir == 0 ? G->nx : dir == 1 ? G->ny : G->nz;
+      long off = (dir == 0 ? 0 : dir == 1 ? YOFF : ZOFF);
+      double d15 = 15.0 / (12.0 * G->dx[dir]), d1 = 1.0 / (12.0 * G->dx[dir]);
+      for (int k = 0; k < ez; k++)
+         for (int j = 0; j < ey; j++)
+            for (int i = 0; i < ex; i++) {
+               int id = dir == 0 ? i : dir == 1 ? j : k;
+               int bnd = !G->periodic && (id == 0 || id == nd);
+               long c0 = CI(i, j, k), s1 = st[dir];
+               long fi = off + (long)i + (long)ex * ((long)j + (long)ey * k);
+               double f = 0.0;
+               for (int ph = 0; ph < 2; ph++) {
+                  const double* c = ph == 0 ? cl : cs;
+                  double d = ph == 0 ? DL[fi] : DA[fi];
+                  if (bnd)
+                     f = f + d * dinv[dir] * (c[c0] - c[c0 - s1]);
+                  else
+                     f = f + d * (d15 * (c[c0] - c[c0 - s1]) -
+                                  d1 * (c[c0 + s1] - c[c0 - 2 * s1]));
+               }
+               fl[fi] = f;
+            }
+   }
+   for (int k = 0; k < G->nz; k++)
+      for (int j = 0; j < G->ny; j++)
+         for (int i = 0; i < G->nx; i++) {
+            double s = dinv[0] * (fl[FX(i + 1, j, k)] - fl[FX(i, j, k)]) +
+                       dinv[1] * (fl[YOFF + FY(i, j + 1, k)] - fl[YOFF + FY(i, j, k)]);
+            if (G->ndim == 3)
+               s = s + dinv[2] * (fl[ZOFF + FZ(i, j, k + 1)] - fl[ZOFF + FZ(i, j, k)]);
+            rhs[CI(i, j, k)] = p->conc_mobility * s;
+         }
+   free(fl);
+}
+
 /* Liquid and solid diffusivities of the face between cells cm and c0
  * (the arithmetic of o_ebs_coeffs). */
 static void ebs_face_d(const ampe_model_params* p, double phm, double ph0, double* dl,

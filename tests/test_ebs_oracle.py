@@ -180,3 +180,94 @@ def test_isotropic_stencil_conserves(ndim):
     phi, cl, cs = (pt.load(pt.cells(), rng.uniform(0, 1, (pt.nz, pt.ny, pt.nx))) for _ in range(3))
     r = _ebs_iso(pt, p, phi, cl, cs)
     assert abs(pt.interior(r).sum()) < 1e-12 * np.abs(pt.interior(r)).sum()
+
+
+def _ebs_4th(pt, p, phi, cl, cs):
+    DL, DA = np.zeros(pt.nf), np.zeros(pt.nf)
+    O.lib().o_ebs_coeffs(C.byref(pt.G), C.byref(p), O.ptr(phi), O.ptr(DL), O.ptr(DA))
+    r = pt.cells()
+    O.lib().o_conc_rhs_ebs_4th(C.byref(pt.G), C.byref(p), O.ptr(cl), O.ptr(cs), O.ptr(DL),
+                               O.ptr(DA), O.ptr(r))
+    return r
+
+
+@pytest.mark.parametrize("ndim", [2, 3])
+def test_order4_stencil_fourier_symbol(ndim):
+    """add_flux_4th on a periodic patch: the divergence of the face fluxes is
+    the fourth-order Laplacian (-1, 16, -30, 16, -1) / 12 dx^2, whose symbol on
+    cos(k x) is (32 cos(k dx) - 2 cos(2 k dx) - 30) / (12 dx^2).  phi = 0
+    leaves the liquid's uniform diffusivity only, so for cL = sum_d cos(k_d x_d)
+    rhs = M D_L sum_d symbol_d cos(k_d x_d) (cS, with D_A = 0, drops out)."""
+    dx = (0.5, 0.25, 0.125)
+    p = P.two_grain_params(ndim=ndim, dx=dx, conc_rhs_form=P.CONC_RHS_EBS)
+    p.ebs_stencil = P.EBS_ORDER4
+    pt = O.Patch(ndim, (12, 10, 8), g=2, dx=dx)
+    x = _centres(pt, dx)
+    n = (pt.nx, pt.ny, pt.nz)
+    rng = np.random.default_rng(11)
+    cl = np.zeros(pt.gshape)
+    expect = np.zeros(pt.gshape)
+    for d in range(ndim):
+        k = 2 * np.pi * (d + 1) / (n[d] * dx[d])
+        cl += np.cos(k * x[d])
+        sym = (32 * np.cos(k * dx[d]) - 2 * np.cos(2 * k * dx[d]) - 30) / (12 * dx[d] ** 2)
+        expect += sym * np.cos(k * x[d])
+    phi = np.zeros((1,) + pt.gshape)
+    cs = rng.uniform(0, 1, (1,) + pt.gshape)
+    r = _ebs_4th(pt, p, phi, cl[None].copy(), cs)
+    dl, _ = _diffusivities(p)
+    want = p.conc_mobility * dl * expect
+    err = np.abs(pt.interior(r[0]) - pt.interior(want)).max() / np.abs(pt.interior(want)).max()
+    assert err < 1e-12
+
+
+@pytest.mark.parametrize("ndim", [2, 3])
+def test_order4_physical_boundary(ndim):
+    """Non-periodic patch with zero-slope ghosts: the boundary faces keep the
+    second-order difference (zero there), so the divergence sums to zero; a
+    fourth-order difference across those faces would not (it reads the
+    second cell inside).  A quartic profile in x gets its exact Laplacian
+    12 a x^2 in the cells whose faces are all interior (the stencil is exact
+    to degree five)."""
+    dx = (0.5, 0.25, 0.125)
+    p = P.two_grain_params(ndim=ndim, dx=dx, conc_rhs_form=P.CONC_RHS_EBS)
+    p.ebs_stencil = P.EBS_ORDER4
+    pt = O.Patch(ndim, (12, 10, 8), g=2, dx=dx)
+    pt.G.periodic = 0
+    rng = np.random.default_rng(13)
+    phi, cl, cs = (rng.uniform(0, 1, (1,) + pt.gshape) for _ in range(3))
+    for a in (phi, cl, cs):
+        O.lib().o_ghost_fill(C.byref(pt.G), O.ptr(a), 1)
+    r = _ebs_4th(pt, p, phi, cl, cs)
+    ri = pt.interior(r)
+    assert abs(ri.sum()) < 1e-12 * np.abs(ri).sum()
+    # quartic in x, liquid only
+    x = _centres(pt, dx)[0]
+    a = 0.7
+    phi0 = np.zeros((1,) + pt.gshape)
+    q = (a * (x - 2.0) ** 4)[None].copy()
+    O.lib().o_ghost_fill(C.byref(pt.G), O.ptr(q), 1)
+    r = _ebs_4th(pt, p, phi0, q, q.copy())
+    dl, _ = _diffusivities(p)
+    want = p.conc_mobility * dl * 12 * a * (x - 2.0) ** 2
+    ri, wi = pt.interior(r[0]), pt.interior(want)
+    inner = ri[..., 2:-2], wi[..., 2:-2]
+    assert np.abs(inner[0] - inner[1]).max() < 1e-9 * np.abs(inner[1]).max()
+    assert np.abs(ri[..., :2] - wi[..., :2]).max() > 1e-6
+
+
+def test_integrator_order4_conserves_composition():
+    """Oracle integrator with the order4 stencil (two ghost cells) on a
+    non-periodic 2-D two-grain state keeps the mean composition."""
+    n = 32
+    p = P.two_grain_params(ndim=2, quad_ceq=(0.40, 0.45), conc_rhs_form=P.CONC_RHS_EBS)
+    p.ebs_stencil = P.EBS_ORDER4
+    phi, c, q = P.two_grain_state(n=n, radius=10.0)
+    s = P.solver_defaults(dt=1e-3, maxl=10, max_newton=5, atol=1e-6, rtol=1e-8)
+    sim = O.Sim(p, s, n, periodic=False)
+    sim.set_state(phi, c, q)
+    rc, st = sim.advance(3)
+    assert rc == 0
+    _, c1, _, cl, cs = sim.get_state()
+    assert abs(c1.mean() - c.mean()) < 1e-12
+    assert np.abs(c1 - c).max() > 0

@@ -24,9 +24,17 @@ def _gpu(p, s, ramp, state):
     return sim
 
 
-def test_deck_boundaries_and_ramp_match_oracle(dev):
+@pytest.mark.parametrize("form", ["kks", "ebs_order4"])
+def test_deck_boundaries_and_ramp_match_oracle(dev, form):
+    """The deck's zero-slope boundaries and temperature ramp, step by step
+    against the oracle; also with the EBS composition flux and its
+    fourth-order stencil (two ghost cells, second-order faces on the
+    physical boundary)."""
     O = pytest.importorskip("oracle.oracle")
     p, ramp, state, _ = P.kks_composition_deck()
+    if form == "ebs_order4":
+        p.conc_rhs_form = P.CONC_RHS_EBS
+        p.ebs_stencil = P.EBS_ORDER4
     s = P.kks_composition_solver(dt=1e-3, atol=1e-8, rtol=1e-10)
     g = _gpu(p, s, ramp, state)
     o = O.Sim(p, s, (N, N, 1), periodic=False)

@@ -212,6 +212,74 @@ def test_conc_ebs_isotropic(dev, ndim):
     assert _rel(pt.interior(r.cpu().numpy()), pt.interior(r_o)) < 1e-13
 
 
+@pytest.mark.parametrize("ndim", [2, 3])
+@pytest.mark.parametrize("layout", ["periodic", "bounded", "bounded_two_patches"])
+def test_conc_ebs_order4(dev, ndim, layout):
+    """EBS composition form with the fourth-order stencil (add_flux_4th) on
+    levels with two ghost cells: periodic, and bounded (zero-slope ghosts,
+    second-order faces on the physical boundary) as one patch or as two
+    patches split along x, whose shared faces must stay fourth order."""
+    from oracle import oracle as O
+    from ampe_amd import hip as H
+    p = _params(ndim, "simple")
+    p.diff_interp = b"p"
+    p.conc_rhs_form = P.CONC_RHS_EBS
+    p.ebs_stencil = P.EBS_ORDER4
+    nx, ny, nz = GEOMS[ndim]
+    pt = O.Patch(ndim, (nx, ny, nz), g=2, dx=tuple(p.dx))
+    pt.G.periodic = 1 if layout == "periodic" else 0
+    rng = _rng(37)
+    shape = (pt.nz, pt.ny, pt.nx)
+    vals = [rng.uniform(0.0, 1.0, shape), rng.uniform(0.2, 0.6, shape),
+            rng.uniform(0.2, 0.6, shape)]
+    phi, cl, cs = (pt.cells() for _ in range(3))
+    for a, v in zip((phi, cl, cs), vals):
+        pt.interior(a)[...] = v[None] if ndim == 3 else v[:1][None]
+        O.lib().o_ghost_fill(C.byref(pt.G), O.ptr(a), 1)
+    DL_o, DA_o = np.zeros(pt.nf), np.zeros(pt.nf)
+    O.lib().o_ebs_coeffs(C.byref(pt.G), C.byref(p), O.ptr(phi), O.ptr(DL_o), O.ptr(DA_o))
+    r_o = pt.cells()
+    O.lib().o_conc_rhs_ebs_4th(C.byref(pt.G), C.byref(p), O.ptr(cl), O.ptr(cs), O.ptr(DL_o),
+                               O.ptr(DA_o), O.ptr(r_o))
+    hi = [nx - 1, ny - 1, nz - 1 if ndim == 3 else 0]
+    if layout == "bounded_two_patches":
+        b0, b1 = P.Box(), P.Box()
+        b0.lo[:] = [0, 0, 0]
+        b0.hi[:] = [nx // 2 - 1] + hi[1:]
+        b1.lo[:] = [nx // 2, 0, 0]
+        b1.hi[:] = hi
+        boxes = [b0, b1]
+    else:
+        b = P.Box()
+        b.lo[:] = [0, 0, 0]
+        b.hi[:] = hi
+        boxes = [b]
+    lev = H.Level(ndim, boxes, 2, periodic=boxes[0] if layout == "periodic" else None)
+    dom = [v if ndim == 3 else v[0] for v in vals]
+    phd, cld, csd = (_dev(lev.to_ghosted(v, ghost=2), dev) for v in dom)
+    for t in (phd, cld, csd):
+        H.check(H.lib().ampe_ghost_fill(lev.h, _p(t), 1, None))
+    DL = _dev(np.zeros(lev.nside_tot), dev)
+    DA = _dev(np.zeros(lev.nside_tot), dev)
+    H.check(H.lib().ampe_conc_coeffs_ebs(lev.h, C.byref(p), _p(phd), _p(DL), _p(DA), None))
+    r = _dev(np.zeros(lev.ncg), dev)
+    H.check(H.lib().ampe_conc_rhs_ebs_4th(lev.h, C.byref(p), _p(cld), _p(csd), _p(DL), _p(DA),
+                                          _p(r), None))
+    rh = r.cpu().numpy()
+    got = np.zeros(shape)
+    for q, bx in enumerate(boxes):
+        blk = lev.ghost_view(rh, q)
+        blk = blk if ndim == 3 else blk[None]
+        n = [bx.hi[d] - bx.lo[d] + 1 for d in range(3)]
+        got[bx.lo[2]:bx.lo[2] + n[2], :, bx.lo[0]:bx.lo[0] + n[0]] = \
+            blk[2 * (ndim == 3):2 * (ndim == 3) + n[2], 2:2 + n[1], 2:2 + n[0]]
+    assert _rel(got, pt.interior(r_o)[0]) < 1e-13
+    # a level with one ghost cell is refused
+    lev1 = H.Level(ndim, boxes, 1, periodic=boxes[0] if layout == "periodic" else None)
+    assert H.lib().ampe_conc_rhs_ebs_4th(lev1.h, C.byref(p), _p(cld), _p(csd), _p(DL), _p(DA),
+                                         _p(r), None) != 0
+
+
 @pytest.mark.parametrize("ndim,kind", [(2, "simple"), (3, "calphad"), (3, "dilute")])
 def test_kks(dev, ndim, kind):
     p = _params(ndim, kind)
@@ -495,7 +563,8 @@ def test_config1_two_grain_20_steps_matches_oracle(dev):
                                           ("calphad", P.GS_CLASSICAL),
                                           ("dilute_kim", P.GS_MODIFIED),
                                           ("calphad_ebs", P.GS_MODIFIED),
-                                          ("calphad_ebs_iso", P.GS_MODIFIED)])
+                                          ("calphad_ebs_iso", P.GS_MODIFIED),
+                                          ("calphad_ebs_4th", P.GS_MODIFIED)])
 def test_integrator_3d_kks_matches_oracle(dev, model, gstype):
     """3-D polycrystal with a KKS composition model, FP64 GPU integrator against
     the oracle step by step: binary CALPHAD (Newton inside every RHS
@@ -511,6 +580,8 @@ def test_integrator_3d_kks_matches_oracle(dev, model, gstype):
             p.conc_rhs_form = P.CONC_RHS_EBS
         if model == "calphad_ebs_iso":
             p.ebs_stencil = P.EBS_ISOTROPIC
+        if model == "calphad_ebs_4th":
+            p.ebs_stencil = P.EBS_ORDER4
         dt = 1e-7
     else:
         p = P.dilute_params(ndim=3, phi_mobility_type=P.PHI_MOB_KIM, D_liquid=1.0e3,
