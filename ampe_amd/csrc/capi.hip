@@ -204,6 +204,23 @@ int ampe_conc_rhs_ebs(const ampe_level* level, const ampe_model_params* p, const
    return ampe::launch_conc_rhs_ebs(*level->L, *p, cl, cs, DL, DA, rhs, hs(stream));
 }
 
+int ampe_phase_rhs_multiorder(const ampe_level* level, const ampe_model_params* p, int norder,
+                              const double* phi, const double* mobility, double* rhs,
+                              ampe_stream_t stream)
+{
+   AMPE_REQUIRE(level && phi && rhs, "null argument");
+   if (check_params(p, *level->L)) return -1;
+   return ampe::launch_phase_rhs_multiorder(*level->L, *p, norder, phi, mobility, rhs,
+                                            hs(stream));
+}
+
+int ampe_project_phases(const ampe_level* level, int norder, const double* phi, double* corr,
+                        double* err, ampe_stream_t stream)
+{
+   AMPE_REQUIRE(level && phi && corr && err, "null argument");
+   return ampe::launch_project_phases(*level->L, norder, phi, corr, err, hs(stream));
+}
+
 int ampe_conc_rhs_ebs_4th(const ampe_level* level, const ampe_model_params* p, const double* cl,
                           const double* cs, const double* DL, const double* DA, double* rhs,
                           ampe_stream_t stream)

@@ -161,6 +161,11 @@ int launch_ebs_coeffs(const Level& L, const ampe_model_params& p, const double* 
 int launch_conc_rhs_ebs(const Level& L, const ampe_model_params& p, const double* cl,
                         const double* cs, const double* DL, const double* DA, double* rhs,
                         hipStream_t s);
+int launch_phase_rhs_multiorder(const Level& L, const ampe_model_params& p, int norder,
+                                const double* phi, const double* mobility, double* rhs,
+                                hipStream_t s);
+int launch_project_phases(const Level& L, int norder, const double* phi, double* corr,
+                          double* err, hipStream_t s);
 int launch_conc_rhs_ebs_4th(const Level& L, const ampe_model_params& p, const double* cl,
                             const double* cs, const double* DL, const double* DA, double* rhs,
                             hipStream_t s);

@@ -320,6 +320,27 @@ int ampe_conc_coeffs_ebs(const ampe_level* level, const ampe_model_params* p,
 int ampe_conc_rhs_ebs(const ampe_level* level, const ampe_model_params* p,
                       const double* cl, const double* cs, const double* DL,
                       const double* DA, double* rhs, ampe_stream_t stream);
+/* Multi-order phase field (norderp > 1, two phases): MultiOrderRHSStrategy::
+ * evaluateRHS (source/MultiOrderRHSStrategy.cc:146-259) with the simple
+ * gradient flux of epsilon_phi / sqrt(2) (source/PhaseFluxStrategyFactory.h
+ * :19-22) and computerhsmultiorder (source/fortran/2d/multiorder.m4:16-105,
+ * 3d/multiorder.m4): for each order p,
+ *   rhs_p = M [div(eps^2 grad phi_p) - m (phi_p^3 - phi_p
+ *           + 2 gamma phi_p (sum_q phi_q^2 - phi_p))],
+ * m = 8 phi_well_scale, gamma = 1.5 (the reference subtracts phi_p, not
+ * phi_p^2, inside the sum).  phi and rhs hold norder components of the
+ * level's ghosted size (phi's ghosts filled); mobility is a cell field or
+ * NULL for the uniform phi_mobility.  The free-energy driving force of
+ * three-phase models is not included.  Needs flux_type = AMPE_FLUX_SIMPLE. */
+int ampe_phase_rhs_multiorder(const ampe_level* level, const ampe_model_params* p,
+                              int norder, const double* phi, const double* mobility,
+                              double* rhs, ampe_stream_t stream);
+/* Projection onto sum_p phi_p = 1 (projectphi, source/fortran/2d/phase.m4
+ * :51-110, via MultiOrderRHSStrategy::projectPhases, source/MultiOrderRHSStrategy.cc
+ * :262-300): corr_p = (phi_p - (sum phi - 1)/norder) - phi_p, and err loses
+ * its mean over the orders, per interior cell. */
+int ampe_project_phases(const ampe_level* level, int norder, const double* phi,
+                        double* corr, double* err, ampe_stream_t stream);
 /* The same with the fourth-order stencil (add_flux_4th, 2d/flux.m4:139-242,
  * 3d/flux.m4:235-420): face flux D (15 (c_h - c_m) - (c_hh - c_mm)) / (12 dx),
  * but D (c_h - c_m) / dx on faces of the physical boundary (the sides of the

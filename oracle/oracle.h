@@ -61,6 +61,10 @@ int o_grains_label(const OGeom* G, const double* phase, double thr, int periodic
                    int* sweeps);
 void o_ebs_coeffs(const OGeom* G, const ampe_model_params* p, const double* phi, double* DL,
                   double* DA);
+void o_phase_rhs_multiorder(const OGeom* G, const ampe_model_params* p, int norder,
+                            const double* phi, const double* M, double* rhs);
+void o_project_phases(const OGeom* G, int norder, const double* phi, double* corr,
+                      double* err);
 void o_conc_rhs_ebs_4th(const OGeom* G, const ampe_model_params* p, const double* cl,
                         const double* cs, const double* DL, const double* DA, double* rhs);
 void o_conc_rhs_ebs_iso(const OGeom* G, const ampe_model_params* p, const double* phid,

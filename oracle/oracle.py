@@ -98,6 +98,8 @@ def _setup(L):
         "o_sim_advance": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(Stats)]),
         "o_sim_set_periodic": (None, [C.c_void_p, C.c_int]),
         "o_conc_rhs_ebs_iso": (None, [_G, _P, _dp, _dp, _dp, _dp]),
+        "o_phase_rhs_multiorder": (None, [_G, _P, C.c_int, _dp, _dp, _dp]),
+        "o_project_phases": (None, [_G, C.c_int, _dp, _dp, _dp]),
         "o_conc_rhs_ebs_4th": (None, [_G, _P, _dp, _dp, _dp, _dp, _dp]),
         "o_sim_set_temperature_ramp": (None, [C.c_void_p, C.c_double, C.c_double, C.c_double,
                                               C.c_void_p]),

@@ -1103,6 +1103,83 @@ void o_conc_rhs_ebs_4th(const OGeom* G, const ampe_model_params* p, const double
    free(fl);
 }
 
+/* Multi-order phase RHS: MultiOrderRHSStrategy::evaluateRHS
+ * (source/MultiOrderRHSStrategy.cc:146-259): gradient_flux of every order
+ * with epsilon / sqrt(2) (source/PhaseFluxStrategyFactory.h:19-22), then
+ * computerhsmultiorder (source/fortran/2d/multiorder.m4:16-105): phi2sum
+ * summed over the orders in order, rhs = div(flux) - m g', g' = phi^3 - phi +
+ * 2 phi gamma (phi2sum - phi), m = 8 w (:51), gamma = 1.5
+ * (source/QuatModelParameters.cc:40); then rhs *= M (the mobility cell field,
+ * or the uniform M when M == NULL). */
+void o_phase_rhs_multiorder(const OGeom* G, const ampe_model_params* p, int norder,
+                            const double* phi, const double* M, double* rhs)
+{
+   long ng = G->ncg;
+   long nf = G->nfx + G->nfy + G->nfz;
+   double* fl = (double*)malloc(sizeof(double) * nf * norder);
+   double eps = p->epsilon_phi / sqrt(2.);
+   for (int ip = 0; ip < norder; ip++) gradient_flux(G, eps, phi + ip * ng, fl + ip * nf);
+   double dinv[3] = {1.0 / G->dx[0], 1.0 / G->dx[1], 1.0 / G->dx[2]};
+   double m = 8. * p->phi_well_scale, gamma = 1.5;
+   double* s2 = (double*)malloc(sizeof(double) * ng);
+   for (int k = 0; k < G->nz; k++)
+      for (int j = 0; j < G->ny; j++)
+         for (int i = 0; i < G->nx; i++) {
+            long c = CI(i, j, k);
+            s2[c] = phi[c] * phi[c];
+         }
+   for (int ip = 1; ip < norder; ip++)
+      for (int k = 0; k < G->nz; k++)
+         for (int j = 0; j < G->ny; j++)
+            for (int i = 0; i < G->nx; i++) {
+               long c = CI(i, j, k);
+               s2[c] = s2[c] + phi[ip * ng + c] * phi[ip * ng + c];
+            }
+   for (int ip = 0; ip < norder; ip++) {
+      const double* f = fl + ip * nf;
+      for (int k = 0; k < G->nz; k++)
+         for (int j = 0; j < G->ny; j++)
+            for (int i = 0; i < G->nx; i++) {
+               long c = CI(i, j, k);
+               double dt = (f[FX(i + 1, j, k)] - f[FX(i, j, k)]) * dinv[0] +
+                           (f[YOFF + FY(i, j + 1, k)] - f[YOFF + FY(i, j, k)]) * dinv[1];
+               if (G->ndim == 3)
+                  dt = dt + dinv[2] * (f[ZOFF + FZ(i, j, k + 1)] - f[ZOFF + FZ(i, j, k)]);
+               double v = phi[ip * ng + c];
+               double gp = v * v * v - v;
+               double dsum = s2[c] - v;
+               gp = gp + 2. * v * gamma * dsum;
+               double r = dt - m * gp;
+               rhs[ip * ng + c] = r * (M ? M[c] : p->phi_mobility);
+            }
+   }
+   free(s2);
+   free(fl);
+}
+
+/* projectphi (source/fortran/2d/phase.m4:51-110) over the interior cells. */
+void o_project_phases(const OGeom* G, int norder, const double* phi, double* corr,
+                      double* err)
+{
+   long ng = G->ncg;
+   double avg = 1.0 / norder;
+   for (int k = 0; k < G->nz; k++)
+      for (int j = 0; j < G->ny; j++)
+         for (int i = 0; i < G->nx; i++) {
+            long c = CI(i, j, k);
+            double fac = 0.0;
+            for (int m = 0; m < norder; m++) fac = fac + phi[m * ng + c];
+            fac = (fac - 1.0) * avg;
+            for (int m = 0; m < norder; m++) corr[m * ng + c] = phi[m * ng + c] - fac;
+            fac = 0.0;
+            for (int m = 0; m < norder; m++) fac = fac + err[m * ng + c];
+            fac = fac * avg;
+            for (int m = 0; m < norder; m++) err[m * ng + c] = err[m * ng + c] - fac;
+            for (int m = 0; m < norder; m++)
+               corr[m * ng + c] = corr[m * ng + c] - phi[m * ng + c];
+         }
+}
+
 /* Liquid and solid diffusivities of the face between cells cm and c0
  * (the arithmetic of o_ebs_coeffs). */
 static void ebs_face_d(const ampe_model_params* p, double phm, double ph0, double* dl,
