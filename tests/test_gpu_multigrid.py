@@ -180,8 +180,11 @@ def _rel(a, b):
     return np.linalg.norm(a - b) / np.linalg.norm(b)
 
 
-@pytest.mark.parametrize("case", ["2d-quadratic", "3d-calphad"])
+@pytest.mark.parametrize("case", ["2d-quadratic", "3d-calphad", "3d-calphad-ebs-order4"])
 def test_multigrid_preconditioner(dev, case):
+    """Multigrid block solves in FP64 and FP32 against red-black; the order4
+    EBS case runs every level of the hierarchy, and its FP32 mirror, with two
+    ghost cells."""
     if case == "2d-quadratic":
         shape, steps = (128, 128), 4
         p = P.two_grain_params(ndim=2, quad_ceq=(0.40, 0.45))
@@ -190,6 +193,9 @@ def test_multigrid_preconditioner(dev, case):
     else:
         shape, steps = (32, 32, 32), 3
         p = P.polycrystal_params(seed=1002)
+        if case == "3d-calphad-ebs-order4":
+            p.conc_rhs_form = P.CONC_RHS_EBS
+            p.ebs_stencil = P.EBS_ORDER4
         state = P.polycrystal_state(n=32, seed=1002, ngrains=8)
         base = dict(dt=5e-8, maxl=10, max_newton=5, atol=1e-6, rtol=1e-8)
     ref, sr = _run(p, P.solver_defaults(**base), state, shape, steps)
@@ -202,7 +208,7 @@ def test_multigrid_preconditioner(dev, case):
         assert max(errs) < 1e-6
         # the 2-D quadratic case is limited by the phase-composition coupling the
         # block-diagonal preconditioner leaves out, not by the block solves
-        if case == "3d-calphad":
+        if case.startswith("3d-calphad"):
             assert sg["nli"] < sr["nli"]
         else:
             assert sg["nli"] <= sr["nli"]
