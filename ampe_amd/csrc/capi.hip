@@ -489,6 +489,15 @@ int ampe_quat_coarsen(const ampe_level* coarse, const ampe_level* fine, const do
    return ampe::quat_coarsen(*coarse->L, *fine->L, dx_fine, qlen, qf, qc, hs(stream));
 }
 
+int ampe_linear_refine(const ampe_level* fine, const ampe_level* coarse, const double* dx_fine,
+                       int ncomp, const double* c, double* f, ampe_stream_t stream)
+{
+   AMPE_REQUIRE(coarse && fine && dx_fine && c && f, "bad argument");
+   AMPE_REQUIRE(ampe::is_coarsening(*coarse->L, *fine->L),
+                "coarse level is not the fine level halved");
+   return ampe::linear_refine(*fine->L, *coarse->L, dx_fine, ncomp, c, f, hs(stream));
+}
+
 int ampe_quat_refine(const ampe_level* fine, const ampe_level* coarse, const double* dx_fine,
                      int qlen, const double* qc, double* qf, ampe_stream_t stream)
 {

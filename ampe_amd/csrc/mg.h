@@ -96,6 +96,8 @@ bool is_coarsening(const Level& coarse, const Level& fine);
 // ghosts filled); dxf = fine mesh spacing (quat_transfer.hip).
 int quat_coarsen(const Level& coarse, const Level& fine, const double* dxf, int ql,
                  const double* qf, double* qc, hipStream_t s);
+int linear_refine(const Level& fine, const Level& coarse, const double* dxf, int ncomp,
+                  const double* c, double* f, hipStream_t s);
 int quat_refine(const Level& fine, const Level& coarse, const double* dxf, int ql,
                 const double* qc, double* qf, hipStream_t s);
 

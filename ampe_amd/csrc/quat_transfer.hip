@@ -69,7 +69,10 @@ __device__ __forceinline__ void lin_index(int gf, double dxf, double dxc, int& g
    gc = ic;
 }
 
-template <int NDIM>
+// NORM: quatlinrefine (interpolate, then q/|q|, ql <= kQl); otherwise the plain
+// cell-centred linear refinement of every component (the interpolation
+// without the normalisation).
+template <int NDIM, bool NORM>
 __global__ void __launch_bounds__(256) k_quat_refine(LevelArgs fa, LevelArgs ca, int ql,
                                                      double dxf0, double dxf1, double dxf2,
                                                      double dxc0, double dxc1, double dxc2,
@@ -92,19 +95,47 @@ __global__ void __launch_bounds__(256) k_quat_refine(LevelArgs fa, LevelArgs ca,
    double q[kQl];
    for (int m = 0; m < ql; m++) {
       const double* a = qc + m * ca.ncg;
+      double v;
       double q1 = a[at(0, 0, 0)], q2 = a[at(0, 1, 0)], q3 = a[at(1, 0, 0)],
              q4 = a[at(1, 1, 0)];
       if (NDIM == 2) {
-         q[m] = (q1 + (q3 - q1) * x) * (1.0 - y) + (q2 + (q4 - q2) * x) * y;
+         v = (q1 + (q3 - q1) * x) * (1.0 - y) + (q2 + (q4 - q2) * x) * y;
       } else {
          double q5 = a[at(0, 0, 1)], q6 = a[at(0, 1, 1)], q7 = a[at(1, 0, 1)],
                 q8 = a[at(1, 1, 1)];
-         q[m] = ((q1 + (q3 - q1) * x) * (1.0 - y) + (q2 + (q4 - q2) * x) * y) * (1.0 - z) +
-                ((q5 + (q7 - q5) * x) * (1.0 - y) + (q6 + (q8 - q6) * x) * y) * z;
+         v = ((q1 + (q3 - q1) * x) * (1.0 - y) + (q2 + (q4 - q2) * x) * y) * (1.0 - z) +
+             ((q5 + (q7 - q5) * x) * (1.0 - y) + (q6 + (q8 - q6) * x) * y) * z;
       }
+      if (NORM)
+         q[m < kQl ? m : kQl - 1] = v;
+      else
+         qf[m * fa.ncg + p.c0] = v;
    }
-   qnorm(q, ql);
-   for (int m = 0; m < ql; m++) qf[m * fa.ncg + p.c0] = q[m];
+   if (NORM) {
+      qnorm(q, ql);
+      for (int m = 0; m < ql; m++) qf[m * fa.ncg + p.c0] = q[m];
+   }
+}
+
+}  // namespace
+
+namespace {
+
+int refine_launch(const Level& fine, const Level& coarse, const double* dxf, int ql, bool norm,
+                  const double* qc, double* qf, hipStream_t s)
+{
+   double c[3] = {2.0 * dxf[0], 2.0 * dxf[1], 2.0 * dxf[2]};
+   auto go = [&](auto kern) {
+      hipLaunchKernelGGL(kern, dim3(fine.ncell_tiles), dim3(TX, TY), 0, s, level_args(fine),
+                         level_args(coarse), ql, dxf[0], dxf[1], dxf[2], c[0], c[1], c[2], qc,
+                         qf);
+   };
+   if (fine.ndim == 3)
+      norm ? go(k_quat_refine<3, true>) : go(k_quat_refine<3, false>);
+   else
+      norm ? go(k_quat_refine<2, true>) : go(k_quat_refine<2, false>);
+   AMPE_HIP_CHECK(hipGetLastError());
+   return 0;
 }
 
 }  // namespace
@@ -139,17 +170,17 @@ int quat_refine(const Level& fine, const Level& coarse, const double* dxf, int q
       set_error("quat_refine: qlen must be 1 to 4");
       return -1;
    }
-   double c[3] = {2.0 * dxf[0], 2.0 * dxf[1], 2.0 * dxf[2]};
-   if (fine.ndim == 3)
-      hipLaunchKernelGGL(k_quat_refine<3>, dim3(fine.ncell_tiles), dim3(TX, TY), 0, s,
-                         level_args(fine), level_args(coarse), ql, dxf[0], dxf[1], dxf[2], c[0],
-                         c[1], c[2], qc, qf);
-   else
-      hipLaunchKernelGGL(k_quat_refine<2>, dim3(fine.ncell_tiles), dim3(TX, TY), 0, s,
-                         level_args(fine), level_args(coarse), ql, dxf[0], dxf[1], dxf[2], c[0],
-                         c[1], c[2], qc, qf);
-   AMPE_HIP_CHECK(hipGetLastError());
-   return 0;
+   return refine_launch(fine, coarse, dxf, ql, true, qc, qf, s);
+}
+
+int linear_refine(const Level& fine, const Level& coarse, const double* dxf, int ncomp,
+                  const double* c, double* f, hipStream_t s)
+{
+   if (ncomp < 1) {
+      set_error("linear_refine: ncomp must be >= 1");
+      return -1;
+   }
+   return refine_launch(fine, coarse, dxf, ncomp, false, c, f, s);
 }
 
 }  // namespace ampe

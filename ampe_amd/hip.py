@@ -39,6 +39,7 @@ _SIGS = {
     "ampe_cluster_mask": (_i, [_i, _vp, _vp, _vp, _vp, _i, _vp]),
     "ampe_quat_coarsen": (_i, [_vp, _vp, _vp, _i, _vp, _vp, _vp]),
     "ampe_conc_rhs_ebs_iso": (_i, [_vp, C.POINTER(ModelParams), _vp, _vp, _vp, _vp, _vp]),
+    "ampe_linear_refine": (_i, [_vp, _vp, _vp, _i, _vp, _vp, _vp]),
     "ampe_phase_rhs_multiorder": (_i, [_vp, C.POINTER(ModelParams), _i, _vp, _vp, _vp, _vp]),
     "ampe_project_phases": (_i, [_vp, _i, _vp, _vp, _vp, _vp]),
     "ampe_conc_rhs_ebs_4th": (_i, [_vp, C.POINTER(ModelParams), _vp, _vp, _vp, _vp, _vp, _vp]),

@@ -491,6 +491,15 @@ int ampe_quat_coarsen(const ampe_level* coarse, const ampe_level* fine, const do
                       int qlen, const double* qf, double* qc, ampe_stream_t stream);
 int ampe_quat_refine(const ampe_level* fine, const ampe_level* coarse, const double* dx_fine,
                      int qlen, const double* qc, double* qf, ampe_stream_t stream);
+/* Cell-centred linear refinement (set, not added) of ncomp scalar components:
+ * the (bi/tri)linear interpolation of ampe_quat_refine without the
+ * normalisation, the role of SAMRAI's "LINEAR_REFINE" that QuatModel registers
+ * for the phase, composition and temperature (source/QuatModel.cc:924-955) when
+ * a new fine level is filled from the coarse one (SAMRAI is absent: its
+ * operator is restated from quatlinrefine's interpolation).  Coarse ghosts
+ * filled; fine and coarse paired patch by patch as for ampe_quat_refine. */
+int ampe_linear_refine(const ampe_level* fine, const ampe_level* coarse, const double* dx_fine,
+                       int ncomp, const double* c, double* f, ampe_stream_t stream);
 
 /* ----------------------------------------------------- vector reductions */
 /* Dot product of two ncomp-component cell fields over interior cells

@@ -56,8 +56,9 @@ def _lin(n, dxf, dxc):
     return ic, x
 
 
-def refine(qc, dxf):
-    """qc: (ql, [nz,] ny, nx) periodic -> (ql, [2nz,] 2ny, 2nx)."""
+def refine(qc, dxf, normalize=True):
+    """qc: (ql, [nz,] ny, nx) periodic -> (ql, [2nz,] 2ny, 2nx); without
+    normalize, the plain cell-centred linear refinement of each component."""
     ndim = qc.ndim - 1
     cs = qc.shape[1:]
     fs = tuple(2 * s for s in cs)
@@ -88,4 +89,4 @@ def refine(qc, dxf):
         q5, q6, q7, q8 = a(0, 0, 1), a(0, 1, 1), a(1, 0, 1), a(1, 1, 1)
         q = (((q1 + (q3 - q1) * X) * (1.0 - Y) + (q2 + (q4 - q2) * X) * Y) * (1.0 - Z) +
              ((q5 + (q7 - q5) * X) * (1.0 - Y) + (q6 + (q8 - q6) * X) * Y) * Z)
-    return _norm(q)
+    return _norm(q) if normalize else q

@@ -116,3 +116,27 @@ def test_uniform_quaternion_unchanged(dev):
                                      C.c_void_p(tf2.data_ptr()), None))
     got = _host(fine, tf2, 4, shape)
     assert np.max(np.abs(got - qf)) < 1e-15
+
+
+@pytest.mark.parametrize("shape", [(32, 48), (16, 24, 20)])
+@pytest.mark.parametrize("split", [False, True])
+@pytest.mark.parametrize("ncomp", [1, 6])
+def test_linear_refine_matches_oracle(dev, shape, split, ncomp):
+    """ampe_linear_refine (scalar fields: the interpolation without the
+    normalisation) against the oracle, any number of components."""
+    import torch
+    from ampe_amd import hip as H
+    print("case", shape, split, ncomp, flush=True)
+    fine, coarse = _levels(shape, split)
+    cshape = tuple(s // 2 for s in shape)
+    rng = np.random.default_rng(7 + ncomp)
+    vc = rng.uniform(-1.0, 2.0, (ncomp,) + cshape)
+    tc = _dev(coarse, vc)
+    H.check(H.lib().ampe_ghost_fill(coarse.h, C.c_void_p(tc.data_ptr()), ncomp, None))
+    tf = torch.zeros(ncomp * fine.ncg, dtype=torch.float64, device="cuda")
+    dx = (C.c_double * 3)(0.05, 0.1, 0.2)
+    H.check(H.lib().ampe_linear_refine(fine.h, coarse.h, dx, ncomp, C.c_void_p(tc.data_ptr()),
+                                       C.c_void_p(tf.data_ptr()), None))
+    got = _host(fine, tf, ncomp, shape)
+    ref = OQ.refine(vc, (0.05, 0.1, 0.2), normalize=False)
+    assert np.max(np.abs(got - ref)) <= 1e-15

@@ -147,3 +147,23 @@ def test_quat_transfer_oracle_closed_forms():
     r = OQ.refine(qc, (1.0, 1.0))
     raw = 0.75 * qc[:, 0, 2] + 0.25 * qc[:, 0, 3]
     assert np.allclose(r[:, 0, 5], raw / np.sqrt((raw ** 2).sum()), rtol=0, atol=1e-15)
+
+
+@pytest.mark.parametrize("ndim", [2, 3])
+def test_linear_refine_reproduces_linear_fields(ndim):
+    """The oracle's cell-centred linear refinement (quatlinrefine without the
+    normalisation) reproduces a linear field at every fine cell whose coarse
+    stencil does not wrap round the periodic box, and keeps constants."""
+    from oracle import quat_transfer as OQ
+    cs = (6, 8, 10)[:ndim][::-1]
+    dxf = (0.05, 0.1, 0.2)
+    idx = np.indices(cs).astype(float)  # (ndim, [nz,] ny, nx), axis 0 = slowest
+    coef = (0.7, -1.3, 2.1)
+    c = 0.4 + sum(coef[d] * (idx[ndim - 1 - d] + 0.5) * 2 * dxf[d] for d in range(ndim))
+    f = OQ.refine(np.stack([c, np.full(cs, 3.25)]), dxf, normalize=False)
+    fidx = np.indices(tuple(2 * s for s in cs)).astype(float)
+    want = 0.4 + sum(coef[d] * (fidx[ndim - 1 - d] + 0.5) * dxf[d] for d in range(ndim))
+    inner = tuple(slice(1, -1) for _ in range(ndim))
+    assert np.abs(f[0][inner] - want[inner]).max() < 1e-13
+    assert np.abs(f[0] - want).max() > 1e-3  # the wrapped cells differ
+    assert np.all(f[1] == 3.25)
