@@ -15,16 +15,49 @@ pytestmark = pytest.mark.gpu
 N = 64
 
 
-def _gpu(p, s, ramp, state):
+def _gpu(p, s, ramp, state, split=False):
     from ampe_amd import hip as H
     box = H.boxes_of((N, N), 2)
-    sim = H.Sim(p, s, [box], periodic=None)
+    boxes = [box]
+    if split:
+        # two patches side by side: their shared faces are interior ones
+        a, b = H.Box(), H.Box()
+        for d in range(3):
+            a.lo[d], a.hi[d] = box.lo[d], box.hi[d]
+            b.lo[d], b.hi[d] = box.lo[d], box.hi[d]
+        a.hi[0] = N // 2 - 3
+        b.lo[0] = N // 2 - 2
+        boxes = [a, b]
+    sim = H.Sim(p, s, boxes, periodic=None)
+    sim.boxes = boxes
     sim.set_temperature_ramp(ramp["T0"], ramp["dTdt"], ramp["T_target"], ramp["tab"])
-    sim.set_state(*state)
+    if split:
+        phi, c, q = state
+        qc = None if q is None else np.stack([_compact(qm, boxes) for qm in q])
+        sim.set_state(_compact(phi, boxes), _compact(c, boxes), qc)
+    else:
+        sim.set_state(*state)
     return sim
 
 
-@pytest.mark.parametrize("form", ["kks", "ebs_order4"])
+def _compact(a, boxes):
+    """Domain array (ny, nx) -> the sim's host layout: patch by patch, each
+    patch's cells row by row."""
+    flat = np.concatenate([a[b.lo[1]:b.hi[1] + 1, b.lo[0]:b.hi[0] + 1].ravel() for b in boxes])
+    return flat.reshape(a.shape)
+
+
+def _domain(v, boxes):
+    out = np.zeros(v.shape)
+    flat, n = v.ravel(), 0
+    for b in boxes:
+        ny, nx = b.hi[1] - b.lo[1] + 1, b.hi[0] - b.lo[0] + 1
+        out[b.lo[1]:b.hi[1] + 1, b.lo[0]:b.hi[0] + 1] = flat[n:n + ny * nx].reshape(ny, nx)
+        n += ny * nx
+    return out
+
+
+@pytest.mark.parametrize("form", ["kks", "ebs_order4", "ebs_order4_two_patches"])
 def test_deck_boundaries_and_ramp_match_oracle(dev, form):
     """The deck's zero-slope boundaries and temperature ramp, step by step
     against the oracle; also with the EBS composition flux and its
@@ -32,11 +65,11 @@ def test_deck_boundaries_and_ramp_match_oracle(dev, form):
     physical boundary)."""
     O = pytest.importorskip("oracle.oracle")
     p, ramp, state, _ = P.kks_composition_deck()
-    if form == "ebs_order4":
+    if form.startswith("ebs_order4"):
         p.conc_rhs_form = P.CONC_RHS_EBS
         p.ebs_stencil = P.EBS_ORDER4
     s = P.kks_composition_solver(dt=1e-3, atol=1e-8, rtol=1e-10)
-    g = _gpu(p, s, ramp, state)
+    g = _gpu(p, s, ramp, state, split=form.endswith("two_patches"))
     o = O.Sim(p, s, (N, N, 1), periodic=False)
     o.set_temperature_ramp(ramp["T0"], ramp["dTdt"], ramp["T_target"], ramp["tab"])
     o.set_state(*state)
@@ -50,6 +83,8 @@ def test_deck_boundaries_and_ramp_match_oracle(dev, form):
         for name, a, b in zip(("phi", "c", "q", "cl", "cs"), fg, fo):
             if name == "q":
                 continue
+            if len(g.boxes) > 1:
+                a = _domain(a, g.boxes)
             err = np.linalg.norm(a - b) / np.linalg.norm(b)
             assert err < 1e-10, (step, name, err)
     print("t", sg["t"], "nni", sg["nni"], "nli", sg["nli"])
