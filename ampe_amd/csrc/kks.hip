@@ -333,6 +333,11 @@ int launch_kks_sync(const Level& L, const ampe_model_params& p, const double* ph
 }
 
 // ------------------------------------------------------------ batch
+// Grid of the batch kernel: at most this many workgroups, each walking its
+// cells with a grid stride, so the per-workgroup partials stay bounded
+// whatever the batch size (a 2^24-cell batch gives 8 cells per thread).
+constexpr int kKksBatchMaxBlocks = 8192;
+
 template <bool SURR>
 __global__ void __launch_bounds__(256) k_kks_batch(long long n, ampe_model_params p,
                                                    ampe_calphad_tdep tab, MlpDev m,
@@ -352,22 +357,29 @@ __global__ void __launch_bounds__(256) k_kks_batch(long long n, ampe_model_param
       mlp_stage<kMlpHidden>(m, F::NFWD, sf, sb);
       __syncthreads();
    }
-   long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-   bool in = t < n;
-   double ct = in ? c[t] : 0.5, pt = in ? phi[t] : 0.0, Tt = in ? T[t] : 1500.0;
-   double gl = 0.5, gs = 0.5;
-   if (SURR) kks_surrogate_guess(m, sf, sb, ct, pt, Tt, gl, gs);
-   double it = 0.0, mx = 0.0, fb = 0.0;
-   if (in) {
+   // it: updates of converged cells, mx: largest count of a converged cell,
+   // fb: fallbacks, nf: failures, ss: surrogate starts
+   double it = 0.0, mx = 0.0, fb = 0.0, nf = 0.0, ss = 0.0;
+   const long long stride = (long long)gridDim.x * blockDim.x;
+   // the trip count is uniform over the workgroup, so every wave calls the
+   // surrogate's wave-wide inference the same number of times
+   for (long long base = (long long)blockIdx.x * blockDim.x; base < n; base += stride) {
+      long long t = base + threadIdx.x;
+      bool in = t < n;
+      double ct = in ? c[t] : 0.5, pt = in ? phi[t] : 0.0, Tt = in ? T[t] : 1500.0;
+      double gl = 0.5, gs = 0.5;
+      if (SURR) kks_surrogate_guess(m, sf, sb, ct, pt, Tt, gl, gs);
+      if (!in) continue;
       ampe_calphad_phase ph[2];
       calphad_at(tab, Tt, ph);
       double h = interp(pt, p.conc_interp);
       int r;
       if (SURR) {
+         ss += 1.0;
          r = kks_newton(ph, Tt, h, ct, gl, gs, p.newton_max_its, p.newton_tol, probe_its,
                         probe_tol);
          if (r == -2) {
-            fb = 1.0;
+            fb += 1.0;
             gl = 0.5;
             gs = 0.5;
             int r2 = kks_newton(ph, Tt, h, ct, gl, gs, p.newton_max_its, p.newton_tol, -1, 0.0);
@@ -379,40 +391,59 @@ __global__ void __launch_bounds__(256) k_kks_batch(long long n, ampe_model_param
       cl[t] = gl;
       cs[t] = gs;
       if (iters) iters[t] = r;
-      mx = r < 0 ? kKksFail : (double)r;
-      it = r < 0 ? 0.0 : (double)r;
+      if (r < 0) {
+         nf += 1.0;
+      } else {
+         it += (double)r;
+         mx = (double)r > mx ? (double)r : mx;
+      }
    }
-   kks_block_partial(it, mx, fb, mx >= kKksFail ? 1.0 : 0.0, (SURR && in) ? 1.0 : 0.0, partial);
+   kks_block_partial(it, mx, fb, nf, ss, partial);
 }
 
-__global__ void k_kks_batch_finish(const double* __restrict__ partial, int nb,
-                                   long long* __restrict__ acc)
+// Fixed-order fold of nb workgroup partials into {updates, largest count,
+// fallbacks, failures}: each of 1024 lanes sums a fixed stride, then a fixed
+// tree (the pattern of k_kks_finish).
+__global__ void __launch_bounds__(1024) k_kks_batch_finish(const double* __restrict__ partial,
+                                                           int nb, long long* __restrict__ acc)
 {
-   // one thread, fixed order; a workgroup whose max is the failure marker
-   // still reports the largest count of its converged cells through s
-   if (threadIdx.x != 0 || blockIdx.x != 0) return;
-   long long s = 0, m = 0, f = 0, fails = 0;
-   for (int i = 0; i < nb; i++) {
-      s += (long long)partial[kKksPart * i];
-      double mx = partial[kKksPart * i + 1];
-      if (mx < kKksFail && (long long)mx > m) m = (long long)mx;
-      f += (long long)partial[kKksPart * i + 2];
-      fails += (long long)partial[kKksPart * i + 3];
+   __shared__ double ss[1024], sm[1024], sf[1024], se[1024];
+   double s = 0.0, m = 0.0, f = 0.0, e = 0.0;
+   for (int i = threadIdx.x; i < nb; i += 1024) {
+      s += partial[kKksPart * i];
+      m = partial[kKksPart * i + 1] > m ? partial[kKksPart * i + 1] : m;
+      f += partial[kKksPart * i + 2];
+      e += partial[kKksPart * i + 3];
    }
-   acc[0] = s;
-   acc[1] = m;
-   acc[2] = f;
-   acc[3] = fails;
+   ss[threadIdx.x] = s;
+   sm[threadIdx.x] = m;
+   sf[threadIdx.x] = f;
+   se[threadIdx.x] = e;
+   __syncthreads();
+   for (int w = 512; w > 0; w >>= 1) {
+      if ((int)threadIdx.x < w) {
+         ss[threadIdx.x] += ss[threadIdx.x + w];
+         sm[threadIdx.x] = sm[threadIdx.x + w] > sm[threadIdx.x] ? sm[threadIdx.x + w]
+                                                                 : sm[threadIdx.x];
+         sf[threadIdx.x] += sf[threadIdx.x + w];
+         se[threadIdx.x] += se[threadIdx.x + w];
+      }
+      __syncthreads();
+   }
+   if (threadIdx.x == 0) {
+      acc[0] = (long long)ss[0];
+      acc[1] = (long long)sm[0];
+      acc[2] = (long long)sf[0];
+      acc[3] = (long long)se[0];
+   }
 }
 
-int kks_batch(const ampe_model_params& p, const ampe_calphad_tdep& tab, long long n,
-              const double* c, const double* phi, const double* T, const Surrogate* guess,
-              int probe_its, double probe_tol, double* cl, double* cs, int* iters,
-              ampe_kks_batch_stats* stats, hipStream_t s)
+int kks_batch_check(const ampe_model_params& p, const ampe_calphad_tdep& tab, long long n,
+                    const Surrogate* guess)
 {
-   if (n <= 0) {
-      if (stats) std::memset(stats, 0, sizeof(*stats));
-      return 0;
+   if (n < 0) {
+      set_error("kks_batch: n must be >= 0");
+      return -1;
    }
    if (p.fe_model != AMPE_FE_CALPHAD_BINARY) {
       set_error("kks_batch: the model must be binary CALPHAD");
@@ -430,48 +461,78 @@ int kks_batch(const ampe_model_params& p, const ampe_calphad_tdep& tab, long lon
       set_error("kks_batch: the surrogate must map (c, phi, T) -> (cL, cS) with 64 hidden");
       return -1;
    }
+   return 0;
+}
+
+int kks_batch_blocks(long long n)
+{
    long long nbl = (n + 255) / 256;
-   if (nbl > (1LL << 30)) {
-      set_error("kks_batch: n too large");
-      return -1;
+   return (int)std::min<long long>(nbl, kKksBatchMaxBlocks);
+}
+
+int kks_batch_launch(const ampe_model_params& p, const ampe_calphad_tdep& tab, long long n,
+                     const double* c, const double* phi, const double* T, const Surrogate* guess,
+                     int probe_its, double probe_tol, double* cl, double* cs, int* iters,
+                     double* part, long long* acc, hipStream_t s)
+{
+   const int nb = kks_batch_blocks(n);
+   MlpDev m{};
+   if (guess) {
+      m = guess->dev();
+      hipLaunchKernelGGL(k_kks_batch<true>, dim3((unsigned)nb), dim3(256), 0, s, n, p, tab, m,
+                         probe_its, probe_tol, c, phi, T, cl, cs, iters, part);
+   } else {
+      hipLaunchKernelGGL(k_kks_batch<false>, dim3((unsigned)nb), dim3(256), 0, s, n, p, tab, m,
+                         -1, 0.0, c, phi, T, cl, cs, iters, part);
    }
+   AMPE_HIP_CHECK(hipGetLastError());
+   hipLaunchKernelGGL(k_kks_batch_finish, dim3(1), dim3(1024), 0, s, part, nb, acc);
+   AMPE_HIP_CHECK(hipGetLastError());
+   return 0;
+}
+
+static void kks_batch_stats_of(const long long h[4], ampe_kks_batch_stats* stats)
+{
+   if (!stats) return;
+   stats->iters = h[0];
+   stats->max_iters = h[1];
+   stats->fallbacks = h[2];
+   stats->failures = h[3];
+}
+
+int kks_batch(const ampe_model_params& p, const ampe_calphad_tdep& tab, long long n,
+              const double* c, const double* phi, const double* T, const Surrogate* guess,
+              int probe_its, double probe_tol, double* cl, double* cs, int* iters,
+              ampe_kks_batch_stats* stats, hipStream_t s)
+{
+   if (kks_batch_check(p, tab, n, guess)) return -1;
+   if (n == 0) {
+      if (stats) std::memset(stats, 0, sizeof(*stats));
+      return 0;
+   }
+   const int nb = kks_batch_blocks(n);
    double* part = nullptr;
    long long* acc = nullptr;
-   AMPE_HIP_CHECK(hipMalloc((void**)&part, sizeof(double) * kKksPart * nbl));
+   AMPE_HIP_CHECK(hipMalloc((void**)&part, sizeof(double) * kKksPart * nb));
    if (hipMalloc((void**)&acc, sizeof(long long) * 4) != hipSuccess) {
       (void)hipFree(part);
       set_error("kks_batch: out of device memory");
       return -1;
    }
-   MlpDev m{};
-   if (guess) {
-      m = guess->dev();
-      hipLaunchKernelGGL(k_kks_batch<true>, dim3((unsigned)nbl), dim3(256), 0, s, n, p, tab, m,
-                         probe_its, probe_tol, c, phi, T, cl, cs, iters, part);
-   } else {
-      hipLaunchKernelGGL(k_kks_batch<false>, dim3((unsigned)nbl), dim3(256), 0, s, n, p, tab, m,
-                         -1, 0.0, c, phi, T, cl, cs, iters, part);
-   }
-   hipError_t e = hipGetLastError();
-   if (e == hipSuccess) {
-      hipLaunchKernelGGL(k_kks_batch_finish, dim3(1), dim3(64), 0, s, part, (int)nbl, acc);
-      e = hipGetLastError();
-   }
+   int rc = kks_batch_launch(p, tab, n, c, phi, T, guess, probe_its, probe_tol, cl, cs, iters,
+                             part, acc, s);
    long long h[4] = {0, 0, 0, 0};
-   if (e == hipSuccess) e = hipMemcpyAsync(h, acc, sizeof(h), hipMemcpyDeviceToHost, s);
-   if (e == hipSuccess) e = hipStreamSynchronize(s);
+   hipError_t e = hipSuccess;
+   if (rc == 0) e = hipMemcpyAsync(h, acc, sizeof(h), hipMemcpyDeviceToHost, s);
+   if (rc == 0 && e == hipSuccess) e = hipStreamSynchronize(s);
    (void)hipFree(part);
    (void)hipFree(acc);
+   if (rc) return rc;
    if (e != hipSuccess) {
       set_error(std::string("kks_batch: ") + hipGetErrorString(e));
       return -1;
    }
-   if (stats) {
-      stats->iters = h[0];
-      stats->max_iters = h[1];
-      stats->fallbacks = h[2];
-      stats->failures = h[3];
-   }
+   kks_batch_stats_of(h, stats);
    return 0;
 }
 

@@ -121,8 +121,8 @@ int level_create(int ndim, const ampe_box* boxes, int npatch, int ghost,
    // sides on the physical boundary (SAMRAI's getTouchesRegularBoundary): the
    // level's bounding box stands for the domain, as every level built here
    // covers it; none on a periodic domain
+   int blo[3], bhi[3];  // bounding box, bhi exclusive
    {
-      int blo[3], bhi[3];
       for (int a = 0; a < 3; a++) {
          blo[a] = L->hp[0].lo[a];
          bhi[a] = blo[a];
@@ -194,6 +194,12 @@ int level_create(int ndim, const ampe_box* boxes, int npatch, int ghost,
                   for (int a = 0; a < ndim; a++)
                      gi[a] = wrapi(gi[a], periodic->lo[a],
                                    periodic->hi[a] - periodic->lo[a] + 1);
+               } else {
+                  // physical boundary: zero-slope image inside the bounding
+                  // box, then whichever patch owns it, so a ghost takes the
+                  // same value however the domain is cut into patches
+                  for (int a = 0; a < ndim; a++)
+                     gi[a] = std::min(std::max(gi[a], blo[a]), bhi[a] - 1);
                }
                int src_p = -1, si = 0, sj = 0, sk = 0;
                for (int q = 0; q < npatch && src_p < 0; q++) {
@@ -209,7 +215,8 @@ int level_create(int ndim, const ampe_box* boxes, int npatch, int ghost,
                   }
                }
                if (src_p < 0) {
-                  // physical (non-periodic) boundary: zero-slope extension
+                  // image not covered by any patch (a level that does not
+                  // fill its bounding box): zero-slope extension of this patch
                   src_p = p;
                   si = std::min(std::max(i, 0), d.nx - 1);
                   sj = std::min(std::max(j, 0), d.ny - 1);

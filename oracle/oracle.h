@@ -10,14 +10,26 @@
  * by g ghost cells (no ghosts along k in 2-D), i fastest; a side field holds the
  * x faces ((nx+1) ny nz), then the y faces (nx (ny+1) nz), then the z faces.
  *
- * Parity status (details in each function header):
- *   pinned by closed forms / reference unit-test vectors: flux kernels,
- *     computerhspbg, quaternion gradients, face coefficients, quaternion RHS,
- *     projection, composition coefficients/flux/RHS, RBGS/residual, tagging,
- *     quadratic KKS, CALPHAD free energy derivatives and KKS invariants.
+ * Parity status.  Each function below is checked by a named CPU test against
+ * mathematics the function itself does not restate (closed forms, invariants,
+ * numpy / scipy computations, the reference's unit-test vectors):
+ *   functions.f helpers, free energies, KKS (quadratic, CALPHAD, dilute):
+ *     tests/test_oracle.py, tests/test_dilute_oracle.py,
+ *     tests/test_surrogate_oracle.py;
+ *   simple / isotropic fluxes, quaternion gradients, composition kernels,
+ *     RBGS / residual, computerhspbg (rotation symmetry): tests/test_oracle.py;
+ *   anisotropic fluxes (2-D Kobayashi form, 3-D cubic form with a scipy
+ *     rotation), orientation face coefficients (energy derivative),
+ *     orientation mobility, orientation RHS (tangency, steady twist,
+ *     continuum convergence, rotation equivariance), CVODE projection,
+ *     tagging, dot / WRMS: tests/test_oracle_pins.py;
+ *   EBS stencils: tests/test_ebs_oracle.py; multi-order RHS and projection:
+ *     tests/test_multiorder_oracle.py; grain labels: tests/test_grains_oracle.py;
+ *   integrator: BDF order of accuracy (tests/test_oracle.py) and the
+ *     reference's regression decks end to end (tests/test_oracle_deck.py).
  *   parity unpinned: exact Thermo4PFM Newton iterate sequence; CVODE step
- *     heuristics (this is a fixed-step BDF1/2 restatement); hypre PCG/PFMG
- *     replaced by red-black Gauss-Seidel on the same symmetrised matrix.
+ *     heuristics (this is a BDF1/2 restatement); hypre PCG/PFMG replaced by
+ *     red-black Gauss-Seidel on the same symmetrised matrix.
  */
 #ifndef AMPE_ORACLE_H
 #define AMPE_ORACLE_H
