@@ -11,6 +11,21 @@
 
 namespace ampe {
 
+// Step and order state of the adaptive BDF (CVODE's fixed-leading-coefficient
+// formulation on a Nordsieck history, orders 1..2): q the order, L = q + 1,
+// qwait steps before an order change is considered, tau[i] the i-th previous
+// step, l / tq the method and error-test coefficients of the current step,
+// hscale the step the history is scaled to, eta / hprime / qprime the choice
+// for the next step, etamax its growth bound.
+struct BdfCtl {
+   int q = 1, qprime = 1, L = 2, qwait = 2;
+   double tau[4] = {0.0, 0.0, 0.0, 0.0};
+   double l[3] = {1.0, 1.0, 0.0};
+   double tq[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+   double hscale = 0.0, hprime = 0.0, eta = 1.0, etamax = 1.0e4, saved_tq5 = 0.0;
+   bool started = false;
+};
+
 struct Sim {
    ampe_model_params p;
    ampe_solver_params s;
@@ -26,6 +41,10 @@ struct Sim {
           *work = nullptr, *work2 = nullptr, *jv = nullptr, *xk = nullptr, *corr = nullptr,
           *acor = nullptr;
    std::vector<double*> V;
+   // adaptive mode: Nordsieck columns h y' and h^2 y''/2 (the state y is the
+   // zeroth column); z2 holds the saved correction while the order is 1
+   double *z1 = nullptr, *z2 = nullptr;
+   BdfCtl bdf;
    std::vector<double> H;
    // phase concentrations and their per-step Newton guesses (ghosted)
    double *cl = nullptr, *cs = nullptr, *clr = nullptr, *csr = nullptr;

@@ -51,6 +51,10 @@ static double first_step(const ampe_solver_params& s)
 int sim_create(const ampe_model_params& p, const ampe_solver_params& s, const ampe_box* boxes,
                int npatch, const ampe_box* periodic, Sim** out)
 {
+   if (s.adaptive && (s.max_order < 1 || s.max_order > 2)) {
+      set_error("sim_create: the adaptive BDF needs max_order 1 or 2");
+      return -1;
+   }
    if (s.maxl < 1 || s.maxl > 30) {
       set_error("sim_create: maxl must be in [1, 30]");
       return -1;
@@ -102,6 +106,10 @@ int sim_create(const ampe_model_params& p, const ampe_solver_params& s, const am
                       &S->del, &S->ewt, &S->work, &S->work2, &S->jv, &S->xk, &S->corr,
                       &S->acor};
    for (auto v : vecs) rc |= dalloc(v, S->nv);
+   if (s.adaptive) {
+      rc |= dalloc(&S->z1, S->nv);
+      rc |= dalloc(&S->z2, S->nv);
+   }
    S->V.assign(s.maxl + 1, nullptr);
    for (auto& v : S->V) rc |= dalloc(&v, S->nv);
    double** cells[] = {&S->cl, &S->cs, &S->clr, &S->csr, &S->Cphi, &S->sqrtm, &S->Mphi,
@@ -153,7 +161,7 @@ void sim_destroy(Sim* S)
                     S->ewt, S->work, S->work2, S->jv, S->xk, S->corr, S->acor, S->cl, S->cs,
                     S->clr, S->csr, S->Cphi, S->sqrtm, S->Mphi, S->fac_u, S->fac_e, S->fac_r,
                     S->n2,
-                    S->Dc, S->Dphi, S->Dconc, S->fc, S->fcq, S->phid};
+                    S->Dc, S->Dphi, S->Dconc, S->fc, S->fcq, S->phid, S->z1, S->z2};
    for (double* v : all) (void)hipFree(v);
    for (double* v : S->V) (void)hipFree(v);
    float* fall[] = {S->f_u,    S->f_e,     S->f_r,     S->f_f,   S->f_Cphi,
@@ -283,6 +291,7 @@ int sim_set_state(Sim* S, const double* phi, const double* c, const double* q)
    S->nstlp = 0;
    S->have_prev = false;
    S->h = first_step(S->s);
+   S->bdf = BdfCtl{};
    if (vec_ewt(S->nv, S->y, S->s.rtol, S->s.atol, S->ewt, st)) return -1;
    AMPE_HIP_CHECK(hipStreamSynchronize(st));
    return 0;
@@ -907,12 +916,14 @@ static int spgmr(Sim* S, double gamma, const double* bvec, double* x, double del
 }
 
 // ------------------------------------------------------------ BDF step
-static int newton(Sim* S, double gamma, double delta, bool force_setup, bool* did_setup)
+// tq4: the Newton convergence bound (CVODE's tq[4] = nlscoef / tq[2]; nlscoef
+// in the imposed-step mode).
+static int newton(Sim* S, double gamma, double delta, double tq4, bool force_setup,
+                  bool* did_setup)
 {
    const Level& L = *S->L;
    hipStream_t st = S->stream;
    long long nv = S->nv;
-   double tq4 = S->s.nlscoef;
    double wfac = 1.0 / ((double)S->nc * L.ncell);
    bool call_setup = force_setup || S->nst == 0 ||
                      std::fabs(gamma / S->gamma_setup - 1.0) > 0.3 ||
@@ -1025,7 +1036,7 @@ static int bdf_step(Sim* S)
          if (vec_bdf2_hist(nv, c1, S->y, c2, S->yprev, S->ahist, st)) return -1;
       }
       bool did_setup = false;
-      int r = newton(S, gamma, delta, force_setup, &did_setup);
+      int r = newton(S, gamma, delta, S->s.nlscoef, force_setup, &did_setup);
       if (r < 0) return -1;
       if (kks_collect(S)) return -1;
       if (r == 0) {
@@ -1063,12 +1074,331 @@ static int bdf_step(Sim* S)
    }
 }
 
+// ------------------------------------------------------------ adaptive BDF
+// CVODE's BDF as QuatIntegrator configures it (source/QuatIntegrator.cc
+// :1571-1596; SUNDIALS CVODE 5.x, orders 1..2): the Nordsieck history
+// z = (y, h y', h^2 y''/2) is predicted by the Pascal matrix, the corrector
+// solves y_n = y_pred + acor with acor = gamma f(y_n) - z1 / l1, gamma = h / l1,
+// the local error test is ||acor||_wrms tq2 <= 1, and after each step the next
+// step and order follow from eta_q = 1 / ((6 dsm)^(1/(q+1)) + 1e-6) and its
+// neighbours of order q - 1 (6 ||z_q|| tq1) and q + 1 (10 ||acor - c z_saved||
+// tq3), changed only when eta >= 1.5.  Error-test failures cut h by
+// max(0.1, eta_q) (at most 0.2 from the second failure; after three, order
+// one and h / 10 with a fresh history); convergence failures by 4.
+
+// Method and error-test coefficients of order q for the step h (CVODE's
+// fixed-leading-coefficient BDF with tau[1] the previous step).
+static void bdf_coeffs(BdfCtl& B, double h, double nlscoef)
+{
+   const int q = B.q;
+   double l0 = 1.0, l1 = 1.0, l2 = 0.0;
+   double alpha0 = -1.0, alpha0_hat = -1.0, xi_inv = 1.0, xistar_inv = 1.0;
+   double hsum = h;
+   if (q == 2) {
+      alpha0 -= 1.0 / 2.0;
+      xistar_inv = -l1 - alpha0;
+      hsum += B.tau[1];
+      xi_inv = h / hsum;
+      alpha0_hat = -l1 - xi_inv;
+      l2 += l1 * xistar_inv;
+      l1 += l0 * xistar_inv;
+   }
+   B.l[0] = l0;
+   B.l[1] = l1;
+   B.l[2] = l2;
+   const double lq = (q == 2) ? l2 : l1;
+   double A1 = 1.0 - alpha0_hat + alpha0;
+   double A2 = 1.0 + q * A1;
+   B.tq[2] = std::fabs(A1 / (alpha0 * A2));
+   B.tq[5] = std::fabs(A2 * xistar_inv / (lq * xi_inv));
+   if (B.qwait == 1) {
+      if (q > 1) {
+         double Cc = xistar_inv / lq;
+         double A3 = alpha0 + 1.0 / q;
+         double A4 = alpha0_hat + xi_inv;
+         double Cpinv = (1.0 - A4 + A3) / A3;
+         B.tq[1] = std::fabs(Cc * Cpinv);
+      } else {
+         B.tq[1] = 1.0;
+      }
+      hsum += B.tau[q];
+      xi_inv = h / hsum;
+      double A5 = alpha0 - 1.0 / (q + 1);
+      double A6 = alpha0_hat - xi_inv;
+      double Cppinv = (1.0 - A6 + A5) / A2;
+      B.tq[3] = std::fabs(Cppinv / (xi_inv * (q + 2) * A5));
+   }
+   B.tq[4] = nlscoef / B.tq[2];
+}
+
+static double bdf_eta_of(double bias, double err, int p)
+{
+   return 1.0 / (std::pow(bias * err, 1.0 / p) + 1.0e-6);
+}
+
+// z1, z2 of the history (the order-q columns; z0 is S->y)
+static double* zcol(Sim* S, int j) { return j == 0 ? S->y : (j == 1 ? S->z1 : S->z2); }
+
+// Pascal-matrix prediction of the history in place, and its inverse.
+static int bdf_predict(Sim* S, double sign)
+{
+   const int q = S->bdf.q;
+   for (int k = 1; k <= q; k++)
+      for (int j = q; j >= k; j--)
+         if (vec_axpby(S->nv, 1.0, zcol(S, j - 1), sign, zcol(S, j), zcol(S, j - 1), S->stream))
+            return -1;
+   return 0;
+}
+
+// Column j of the history times eta^j, h = hscale eta.
+static int bdf_rescale(Sim* S, double eta)
+{
+   BdfCtl& B = S->bdf;
+   double factor = eta;
+   for (int j = 1; j <= B.q; j++) {
+      if (vec_axpby(S->nv, factor, zcol(S, j), 0.0, zcol(S, j), zcol(S, j), S->stream))
+         return -1;
+      factor *= eta;
+   }
+   S->h = B.hscale * eta;
+   B.hscale = S->h;
+   return 0;
+}
+
+// WRMS norm of a state vector against the current error weights.
+static int bdf_wrms(Sim* S, const double* x, double* out)
+{
+   double ss;
+   if (vec_sumsq_w(*S->L, S->nc, x, S->ewt, &ss, S->stream)) return -1;
+   *out = std::sqrt(ss / ((double)S->nc * S->L->ncell));
+   return 0;
+}
+
+static void bdf_set_eta(BdfCtl& B, double h, double hmax)
+{
+   if (B.eta < 1.5) {
+      B.eta = 1.0;
+      B.hprime = h;
+   } else {
+      B.eta = std::min(B.eta, B.etamax);
+      B.eta /= std::max(1.0, std::fabs(h) * (1.0 / hmax) * B.eta);
+      B.hprime = h * B.eta;
+   }
+}
+
+// Step and order for the next step after an accepted one (acor: the step's
+// correction).
+static int bdf_prepare_next(Sim* S, double dsm, const double* acor, int qmax)
+{
+   BdfCtl& B = S->bdf;
+   const double h = S->h;
+   if (B.etamax == 1.0) {
+      B.qwait = std::max(B.qwait, 2);
+      B.qprime = B.q;
+      B.hprime = h;
+      B.eta = 1.0;
+      return 0;
+   }
+   double etaq = bdf_eta_of(6.0, dsm, B.L);
+   if (B.qwait != 0) {
+      B.eta = etaq;
+      B.qprime = B.q;
+      bdf_set_eta(B, h, S->s.dt);
+      return 0;
+   }
+   B.qwait = 2;
+   double etaqm1 = 0.0, etaqp1 = 0.0;
+   if (B.q > 1) {
+      double n;
+      if (bdf_wrms(S, zcol(S, B.q), &n)) return -1;
+      etaqm1 = bdf_eta_of(6.0, n * B.tq[1], B.q);
+   }
+   if (B.q != qmax && B.saved_tq5 != 0.0) {
+      double ratio = h / B.tau[2], pw = 1.0;
+      for (int i = 0; i < B.L; i++) pw *= ratio;
+      double cquot = (B.tq[5] / B.saved_tq5) * pw;
+      if (vec_axpby(S->nv, -cquot, S->z2, 1.0, acor, S->work2, S->stream)) return -1;
+      double n;
+      if (bdf_wrms(S, S->work2, &n)) return -1;
+      etaqp1 = bdf_eta_of(10.0, n * B.tq[3], B.L + 1);
+   }
+   double etam = std::max(etaqm1, std::max(etaq, etaqp1));
+   if (etam < 1.5) {
+      B.eta = 1.0;
+      B.qprime = B.q;
+   } else if (etam == etaq) {
+      B.eta = etaq;
+      B.qprime = B.q;
+   } else if (etam == etaqm1) {
+      B.eta = etaqm1;
+      B.qprime = B.q - 1;
+   } else {
+      B.eta = etaqp1;
+      B.qprime = B.q + 1;
+      AMPE_HIP_CHECK(hipMemcpyAsync(S->z2, acor, sizeof(double) * S->nv,
+                                    hipMemcpyDeviceToDevice, S->stream));
+   }
+   bdf_set_eta(B, h, S->s.dt);
+   return 0;
+}
+
+// Order change of the history (CVODE's cvAdjustOrder for BDF with q <= 2:
+// raising 1 -> 2 starts the new column at zero, lowering 2 -> 1 drops it).
+static int bdf_adjust_order(Sim* S, int qnew)
+{
+   BdfCtl& B = S->bdf;
+   if (qnew > B.q && vec_fill(S->nv, 0.0, S->z2, S->stream)) return -1;
+   B.q = qnew;
+   B.L = qnew + 1;
+   B.qwait = B.L;
+   return 0;
+}
+
+// z1 = h f(t, y) (the initial history, and its reload after repeated error
+// test failures at order one)
+static int bdf_load_z1(Sim* S, double t)
+{
+   if (S->tramp && S->dTdt != 0.0) {
+      S->p.temperature = ramp_temperature(S, t);
+      if (S->has_tab) calphad_at(S->tab, S->p.temperature, S->p.calphad);
+   }
+   if (rhs_eval(S, S->y, S->fcur, false)) return -1;
+   return vec_axpby(S->nv, S->h, S->fcur, 0.0, S->fcur, S->z1, S->stream);
+}
+
+static int bdf_adaptive_step(Sim* S)
+{
+   const Level& L = *S->L;
+   hipStream_t st = S->stream;
+   long long nv = S->nv, ng = S->ng;
+   BdfCtl& B = S->bdf;
+   const int qmax = S->s.max_order;
+   const double hmin = S->s.dt * 1.0e-8;
+   if (S->ic >= 0) {
+      // resetRefPhaseConcentrations (source/QuatModel.cc:5218-5235)
+      AMPE_HIP_CHECK(hipMemcpyAsync(S->clr, S->cl, sizeof(double) * ng,
+                                    hipMemcpyDeviceToDevice, st));
+      AMPE_HIP_CHECK(hipMemcpyAsync(S->csr, S->cs, sizeof(double) * ng,
+                                    hipMemcpyDeviceToDevice, st));
+   }
+   if (!B.started) {
+      B = BdfCtl{};
+      B.hscale = S->h;
+      B.hprime = S->h;
+      if (bdf_load_z1(S, S->t)) return -1;
+      B.started = true;
+   } else if (B.hprime != S->h) {
+      if (B.qprime != B.q && bdf_adjust_order(S, B.qprime)) return -1;
+      if (bdf_rescale(S, B.eta)) return -1;
+   }
+   double nrmfac = std::sqrt((double)S->nc * L.ncell * cellvol(S));
+   int nef = 0, ncf = 0;
+   bool force_setup = false, predicted = false;
+   double gamma = 0.0, delta = 0.0;
+   for (;;) {
+      const double h = S->h;
+      if (!predicted) {
+         if (S->tramp && S->dTdt != 0.0) {
+            S->p.temperature = ramp_temperature(S, S->t + h);
+            if (S->has_tab) calphad_at(S->tab, S->p.temperature, S->p.calphad);
+         }
+         if (bdf_predict(S, 1.0)) return -1;
+         AMPE_HIP_CHECK(hipMemcpyAsync(S->ypred, S->y, sizeof(double) * nv,
+                                       hipMemcpyDeviceToDevice, st));
+         bdf_coeffs(B, h, S->s.nlscoef);
+         gamma = h / B.l[1];
+         // the corrector y - y_pred + z1 / l1 - gamma f(y) = 0 in the form
+         // y - a - gamma f(y) = 0
+         if (vec_axpby(nv, 1.0, S->ypred, -1.0 / B.l[1], S->z1, S->ahist, st)) return -1;
+         delta = S->s.eplifac * B.tq[4] * nrmfac;
+         predicted = true;
+      }
+      bool did_setup = false;
+      int r = newton(S, gamma, delta, B.tq[4], force_setup, &did_setup);
+      if (r < 0) return -1;
+      if (kks_collect(S)) return -1;
+      if (r != 0) {
+         S->st.ncfn++;
+         if (!did_setup) {
+            // retry this prediction with a fresh preconditioner
+            force_setup = true;
+            continue;
+         }
+         if (bdf_predict(S, -1.0)) return -1;
+         predicted = false;
+         ncf++;
+         B.etamax = 1.0;
+         if (ncf >= 10 || h <= hmin) return 1;
+         if (bdf_rescale(S, 0.25)) return -1;
+         force_setup = true;
+         continue;
+      }
+      // converged: acor = y - y_pred, projected onto |q| = 1 with the error
+      if (vec_axpby(nv, 1.0, S->ycur, -1.0, S->ypred, S->acor, st)) return -1;
+      if (S->iq >= 0 && S->p.qlen > 1) {
+         AMPE_HIP_CHECK(hipMemsetAsync(S->corr, 0, sizeof(double) * nv, st));
+         if (launch_quat_project(L, S->p, S->ycur + (long long)S->iq * ng,
+                                 S->corr + (long long)S->iq * ng,
+                                 S->acor + (long long)S->iq * ng, st))
+            return -1;
+         if (vec_axpby(nv, 1.0, S->ycur, 1.0, S->corr, S->ycur, st)) return -1;
+      }
+      double acnrm;
+      if (bdf_wrms(S, S->acor, &acnrm)) return -1;
+      S->st.last_acnrm = acnrm;
+      const double dsm = acnrm * B.tq[2];
+      if (dsm > 1.0) {
+         if (bdf_predict(S, -1.0)) return -1;
+         predicted = false;
+         nef++;
+         S->st.netf++;
+         B.etamax = 1.0;
+         if (nef >= 7 || h <= hmin) return 1;
+         if (nef <= 3) {
+            double eta = std::max(0.1, bdf_eta_of(6.0, dsm, B.L));
+            if (nef >= 2) eta = std::min(eta, 0.2);
+            if (bdf_rescale(S, eta)) return -1;
+         } else if (B.q > 1) {
+            if (bdf_adjust_order(S, B.q - 1)) return -1;
+            if (bdf_rescale(S, 0.1)) return -1;
+         } else {
+            S->h = 0.1 * h;
+            B.hscale = S->h;
+            B.qwait = 10;
+            if (bdf_load_z1(S, S->t)) return -1;
+         }
+         continue;
+      }
+      // accepted: the history takes the projected correction y_n - y_pred
+      if (vec_axpby(nv, 1.0, S->ycur, -1.0, S->ypred, S->work, st)) return -1;
+      S->nst++;
+      S->st.nst++;
+      for (int i = B.q; i >= 2; i--) B.tau[i] = B.tau[i - 1];
+      if (B.q == 1 && S->nst > 1) B.tau[2] = B.tau[1];
+      B.tau[1] = h;
+      for (int j = 0; j <= B.q; j++)
+         if (vec_axpby(nv, B.l[j], S->work, 1.0, zcol(S, j), zcol(S, j), st)) return -1;
+      B.qwait--;
+      if (B.qwait == 1 && B.q != qmax) {
+         AMPE_HIP_CHECK(hipMemcpyAsync(S->z2, S->work, sizeof(double) * nv,
+                                       hipMemcpyDeviceToDevice, st));
+         B.saved_tq5 = B.tq[5];
+      }
+      S->t += h;
+      S->st.q = B.q;
+      if (bdf_prepare_next(S, dsm, S->work, qmax)) return -1;
+      B.etamax = 10.0;
+      if (vec_ewt(nv, S->y, S->s.rtol, S->s.atol, S->ewt, st)) return -1;
+      return 0;
+   }
+}
+
 int sim_advance(Sim* S, int nsteps, ampe_stats* stats)
 {
    int ret = 0;
    AMPE_HIP_CHECK(hipEventRecord(S->ev0, S->stream));
    for (int n = 0; n < nsteps; n++) {
-      ret = bdf_step(S);
+      ret = S->s.adaptive ? bdf_adaptive_step(S) : bdf_step(S);
       if (ret) break;
    }
    AMPE_HIP_CHECK(hipEventRecord(S->ev1, S->stream));
@@ -1077,7 +1407,8 @@ int sim_advance(Sim* S, int nsteps, ampe_stats* stats)
    AMPE_HIP_CHECK(hipEventElapsedTime(&ms, S->ev0, S->ev1));
    S->last_ms = ms;
    S->st.t = S->t;
-   S->st.h = S->h;
+   S->st.h = S->s.adaptive ? S->bdf.hprime : S->h;
+   if (!S->s.adaptive) S->st.q = S->have_prev && S->s.max_order >= 2 ? 2 : 1;
    if (stats) *stats = S->st;
    return ret;
 }

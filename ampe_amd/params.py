@@ -124,6 +124,7 @@ class SolverParams(C.Structure):
         ("gstype", C.c_int),
         ("graph_psolve", C.c_int),
         ("dt_init", C.c_double),
+        ("adaptive", C.c_int),
     ]
 
 
@@ -132,7 +133,8 @@ class Stats(C.Structure):
                 ("nst", "nni", "nli", "nfe", "npe", "nps", "ncfn", "nlcf",
                  "kks_calls", "kks_cells", "kks_iters", "kks_fallbacks",
                  "kks_surrogate_starts")] + [
-        ("last_acnrm", C.c_double), ("t", C.c_double), ("h", C.c_double)]
+        ("last_acnrm", C.c_double), ("t", C.c_double), ("h", C.c_double),
+        ("netf", C.c_longlong), ("q", C.c_longlong)]
 
     def as_dict(self):
         return {n: getattr(self, n) for n, _ in self._fields_}
@@ -258,7 +260,7 @@ def solver_defaults(**kw):
                      fac_tol=-1.0, quat_max_cycles=1, quat_sweeps=2,
                      precision=FP64, kks_guess=GUESS_PREVIOUS,
                      precond=PRECOND_REDBLACK, mg_max_levels=0, mg_coarse_sweeps=16,
-                     gstype=GS_MODIFIED, graph_psolve=0)
+                     gstype=GS_MODIFIED, graph_psolve=0, adaptive=0)
     for k, v in kw.items():
         setattr(s, k, v)
     return s

@@ -188,6 +188,14 @@ typedef struct ampe_solver_params {
                            * the small automatic first step CVODE takes when the
                            * initial step size is 0 (QuatIntegrator's default
                            * d_previous_timestep, source/QuatIntegrator.cc:135,1575) */
+   int adaptive;          /* 0: the imposed step above (BDF1, then variable-step BDF2).
+                           * 1: CVODE's adaptive BDF as QuatIntegrator configures it
+                           * (source/QuatIntegrator.cc:1571-1596): fixed-leading-
+                           * coefficient BDF of order 1..max_order (<= 2) on a
+                           * Nordsieck history, local error test on the projected
+                           * correction (dsm = ||acor||_wrms tq2 <= 1), step and
+                           * order selection after each step, first step dt_init
+                           * (else dt), largest step dt (max_step_size) */
 } ampe_solver_params;
 
 /* Per-run counters (mirror QuatIntegrator.cc:2161-2170 nni/nli/nfe/npe/nps). */
@@ -198,6 +206,8 @@ typedef struct ampe_stats {
    double last_acnrm; /* WRMS norm of the last step's Newton correction */
    double t;          /* simulated time reached */
    double h;          /* next step size (dt unless a failure cut it) */
+   long long netf;    /* local error test failures (adaptive mode) */
+   long long q;       /* order of the last accepted step */
 } ampe_stats;
 
 /* ---------------------------------------------------------------- runtime */

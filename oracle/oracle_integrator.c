@@ -63,6 +63,11 @@ struct OSim {
    double* H;
    ampe_stats st;
    int fail;
+   /* adaptive BDF (s.adaptive): Nordsieck columns h y', h^2 y''/2 and the
+    * step / order controller (see bdf_adaptive_step) */
+   double *z1, *z2;
+   int q, qprime, L, qwait, started;
+   double tau[4], l[3], tq[6], hscale, hprime, eta, etamax, saved_tq5;
    /* scalar temperature ramp (o_sim_set_temperature_ramp) */
    int tramp, has_tab;
    double T0, dTdt, Ttarget;
@@ -146,6 +151,8 @@ OSim* o_sim_create(const ampe_model_params* p, const ampe_solver_params* s,
    S->fc = vec(S->nf);
    S->fac_flux = vec(S->nf);
    S->H = vec((long)(s->maxl + 1) * (s->maxl + 1));
+   S->z1 = vec(S->nv);
+   S->z2 = vec(S->nv);
    S->h = first_step(s);
    return S;
 }
@@ -157,7 +164,7 @@ void o_sim_destroy(OSim* S)
                    S->del, S->ewt, S->work, S->work2, S->jv, S->xk, S->corr,
                    S->acor, S->cl, S->cs, S->clr, S->csr, S->Cphi, S->Mphi, S->sqrtm,
                    S->mq, S->fac_u, S->fac_e, S->fac_r, S->n2, S->Dc, S->Dphi,
-                   S->Dconc, S->fc, S->fac_flux, S->H, S->phid};
+                   S->Dconc, S->fc, S->fac_flux, S->H, S->phid, S->z1, S->z2};
    for (size_t i = 0; i < sizeof(fs) / sizeof(fs[0]); i++) free(fs[i]);
    for (int i = 0; i <= S->s.maxl; i++) free(S->V[i]);
    free(S->V);
@@ -212,6 +219,7 @@ void o_sim_set_state(OSim* S, const double* phi, const double* c,
    S->nst = 0;
    S->have_prev = 0;
    S->h = first_step(&S->s);
+   S->started = 0;
    memset(&S->st, 0, sizeof(S->st));
    update_ewt(S, S->y);
 }
@@ -635,11 +643,10 @@ static int spgmr(OSim* S, double gamma, const double* bvec, double* x,
 
 /* Newton iteration for one step attempt with step h: returns 0 converged,
  * 1 convergence failure, -1 fatal.  *did_setup reports a fresh lsetup. */
-static int newton(OSim* S, double gamma, double delta, int force_setup,
+static int newton(OSim* S, double gamma, double delta, double tq4, int force_setup,
                   int* did_setup)
 {
    long nv = S->nv;
-   double tq4 = S->s.nlscoef;
    int call_setup = force_setup || S->nst == 0 ||
                     fabs(gamma / S->gamma_setup - 1.0) > 0.3 ||
                     S->nst >= S->nstlp + S->s.msbp;
@@ -726,7 +733,7 @@ static int bdf_step(OSim* S)
          for (long n = 0; n < nv; n++) S->ahist[n] = c1 * S->y[n] - c2 * S->yprev[n];
       }
       int did_setup = 0;
-      int r = newton(S, gamma, delta, force_setup, &did_setup);
+      int r = newton(S, gamma, delta, S->s.nlscoef, force_setup, &did_setup);
       if (r < 0) return -1;
       if (r == 0) {
          for (long n = 0; n < nv; n++) S->acor[n] = S->ycur[n] - S->ypred[n];
@@ -759,15 +766,310 @@ static int bdf_step(OSim* S)
    }
 }
 
+/* ------------------------------------------------------------------------
+ * Adaptive BDF (s.adaptive = 1): SUNDIALS CVODE 5.x as QuatIntegrator
+ * configures it (source/QuatIntegrator.cc:1571-1596), restated from CVODE's
+ * published algorithm (Hindmarsh et al., ACM TOMS 31 (2005) 363; CVODE user
+ * guide, "Mathematical considerations") for orders 1..2:
+ *   - Nordsieck history z = (y, h y', h^2 y''/2), predicted by the Pascal
+ *     matrix (cvPredict) and restored by its inverse after a failure;
+ *   - fixed-leading-coefficient BDF coefficients l, error constants tq
+ *     (cvSetBDF / cvSetTqBDF) from the step history tau;
+ *   - corrector y_n = y_pred + acor, acor = gamma f(y_n) - z1 / l1,
+ *     gamma = h / l1; Newton test del min(1, crate) <= tq4 = nlscoef / tq2,
+ *     linear tolerance eplifac tq4 sqrt(sum cv);
+ *   - projection of q after convergence, the error estimate projected with it;
+ *   - error test dsm = ||acor||_wrms tq2 <= 1; failure: eta = max(0.1,
+ *     1/((6 dsm)^(1/L) + 1e-6)), at most 0.2 from the second failure, order
+ *     reduction (or reload of z1 = h f at order 1) after the third, give up
+ *     after 7; convergence failure: eta = 1/4, give up after 10;
+ *   - next step (cvPrepareNextStep): eta_q, and every q + 1 steps eta_{q-1}
+ *     (6 ||z_q|| tq1) and eta_{q+1} (10 ||acor - c z_saved|| tq3), the largest
+ *     taken when >= 1.5, bounded by etamax (1e4 on the first step, 10 after,
+ *     1 after a failed attempt) and h <= dt.
+ * Parity with SUNDIALS itself is unpinned (the library is absent); the device
+ * integrator implements the same algorithm.
+ * ---------------------------------------------------------------------- */
+static void axpby(long n, double a, const double* x, double b, const double* y, double* z)
+{
+   for (long i = 0; i < n; i++) z[i] = a * x[i] + b * y[i];
+}
+
+static double* zcol(OSim* S, int j) { return j == 0 ? S->y : (j == 1 ? S->z1 : S->z2); }
+
+static void bdf_coeffs(OSim* S, double h)
+{
+   int q = S->q;
+   double l0 = 1.0, l1 = 1.0, l2 = 0.0;
+   double alpha0 = -1.0, alpha0_hat = -1.0, xi_inv = 1.0, xistar_inv = 1.0;
+   double hsum = h;
+   if (q == 2) {
+      alpha0 -= 1.0 / 2.0;
+      xistar_inv = -l1 - alpha0;
+      hsum += S->tau[1];
+      xi_inv = h / hsum;
+      alpha0_hat = -l1 - xi_inv;
+      l2 += l1 * xistar_inv;
+      l1 += l0 * xistar_inv;
+   }
+   S->l[0] = l0;
+   S->l[1] = l1;
+   S->l[2] = l2;
+   double lq = (q == 2) ? l2 : l1;
+   double A1 = 1.0 - alpha0_hat + alpha0;
+   double A2 = 1.0 + q * A1;
+   S->tq[2] = fabs(A1 / (alpha0 * A2));
+   S->tq[5] = fabs(A2 * xistar_inv / (lq * xi_inv));
+   if (S->qwait == 1) {
+      if (q > 1) {
+         double Cc = xistar_inv / lq;
+         double A3 = alpha0 + 1.0 / q;
+         double A4 = alpha0_hat + xi_inv;
+         double Cpinv = (1.0 - A4 + A3) / A3;
+         S->tq[1] = fabs(Cc * Cpinv);
+      } else {
+         S->tq[1] = 1.0;
+      }
+      hsum += S->tau[q];
+      xi_inv = h / hsum;
+      double A5 = alpha0 - 1.0 / (q + 1);
+      double A6 = alpha0_hat - xi_inv;
+      double Cppinv = (1.0 - A6 + A5) / A2;
+      S->tq[3] = fabs(Cppinv / (xi_inv * (q + 2) * A5));
+   }
+   S->tq[4] = S->s.nlscoef / S->tq[2];
+}
+
+static double eta_of(double bias, double err, int p)
+{
+   return 1.0 / (pow(bias * err, 1.0 / p) + 1.0e-6);
+}
+
+static void bdf_predict(OSim* S, double sign)
+{
+   for (int k = 1; k <= S->q; k++)
+      for (int j = S->q; j >= k; j--) axpby(S->nv, 1.0, zcol(S, j - 1), sign, zcol(S, j), zcol(S, j - 1));
+}
+
+static void bdf_rescale(OSim* S, double eta)
+{
+   double factor = eta;
+   for (int j = 1; j <= S->q; j++) {
+      axpby(S->nv, factor, zcol(S, j), 0.0, zcol(S, j), zcol(S, j));
+      factor *= eta;
+   }
+   S->h = S->hscale * eta;
+   S->hscale = S->h;
+}
+
+static void bdf_set_eta(OSim* S, double h)
+{
+   if (S->eta < 1.5) {
+      S->eta = 1.0;
+      S->hprime = h;
+   } else {
+      double hmax = S->s.dt;
+      S->eta = S->eta < S->etamax ? S->eta : S->etamax;
+      double d = fabs(h) * (1.0 / hmax) * S->eta;
+      S->eta /= (d > 1.0 ? d : 1.0);
+      S->hprime = h * S->eta;
+   }
+}
+
+static void bdf_prepare_next(OSim* S, double dsm, const double* acor)
+{
+   double h = S->h;
+   int qmax = S->s.max_order;
+   if (S->etamax == 1.0) {
+      S->qwait = S->qwait > 2 ? S->qwait : 2;
+      S->qprime = S->q;
+      S->hprime = h;
+      S->eta = 1.0;
+      return;
+   }
+   double etaq = eta_of(6.0, dsm, S->L);
+   if (S->qwait != 0) {
+      S->eta = etaq;
+      S->qprime = S->q;
+      bdf_set_eta(S, h);
+      return;
+   }
+   S->qwait = 2;
+   double etaqm1 = 0.0, etaqp1 = 0.0;
+   if (S->q > 1) etaqm1 = eta_of(6.0, wrms(S, zcol(S, S->q)) * S->tq[1], S->q);
+   if (S->q != qmax && S->saved_tq5 != 0.0) {
+      double ratio = h / S->tau[2], pw = 1.0;
+      for (int i = 0; i < S->L; i++) pw *= ratio;
+      double cquot = (S->tq[5] / S->saved_tq5) * pw;
+      axpby(S->nv, -cquot, S->z2, 1.0, acor, S->work2);
+      etaqp1 = eta_of(10.0, wrms(S, S->work2) * S->tq[3], S->L + 1);
+   }
+   double etam = etaq > etaqp1 ? etaq : etaqp1;
+   etam = etaqm1 > etam ? etaqm1 : etam;
+   if (etam < 1.5) {
+      S->eta = 1.0;
+      S->qprime = S->q;
+   } else if (etam == etaq) {
+      S->eta = etaq;
+      S->qprime = S->q;
+   } else if (etam == etaqm1) {
+      S->eta = etaqm1;
+      S->qprime = S->q - 1;
+   } else {
+      S->eta = etaqp1;
+      S->qprime = S->q + 1;
+      memcpy(S->z2, acor, sizeof(double) * S->nv);
+   }
+   bdf_set_eta(S, h);
+}
+
+static void bdf_adjust_order(OSim* S, int qnew)
+{
+   if (qnew > S->q) memset(S->z2, 0, sizeof(double) * S->nv);
+   S->q = qnew;
+   S->L = qnew + 1;
+   S->qwait = S->L;
+}
+
+static int bdf_load_z1(OSim* S, double t)
+{
+   if (S->tramp && S->dTdt != 0.0) {
+      S->p.temperature = ramp_temperature(S, t);
+      if (S->has_tab) o_calphad_at(&S->tab, S->p.temperature, S->p.calphad);
+   }
+   if (rhs_eval(S, S->y, S->fcur, 0)) return -1;
+   axpby(S->nv, S->h, S->fcur, 0.0, S->fcur, S->z1);
+   return 0;
+}
+
+static int bdf_adaptive_step(OSim* S)
+{
+   const OGeom* G = &S->G;
+   long nv = S->nv, ng = G->ncg;
+   int qmax = S->s.max_order;
+   double hmin = S->s.dt * 1.0e-8;
+   if (S->ic >= 0) {
+      memcpy(S->clr, S->cl, sizeof(double) * ng);
+      memcpy(S->csr, S->cs, sizeof(double) * ng);
+   }
+   if (!S->started) {
+      S->q = S->qprime = 1;
+      S->L = 2;
+      S->qwait = 2;
+      for (int i = 0; i < 4; i++) S->tau[i] = 0.0;
+      S->hscale = S->hprime = S->h;
+      S->eta = 1.0;
+      S->etamax = 1.0e4;
+      S->saved_tq5 = 0.0;
+      if (bdf_load_z1(S, S->t)) return -1;
+      S->started = 1;
+   } else if (S->hprime != S->h) {
+      if (S->qprime != S->q) bdf_adjust_order(S, S->qprime);
+      bdf_rescale(S, S->eta);
+   }
+   double cv = G->dx[0] * G->dx[1] * (G->ndim == 3 ? G->dx[2] : 1.0);
+   double nrmfac = sqrt((double)S->nc * G->nx * G->ny * G->nz * cv);
+   int nef = 0, ncf = 0, force_setup = 0, predicted = 0;
+   double gamma = 0.0, delta = 0.0;
+   for (;;) {
+      double h = S->h;
+      if (!predicted) {
+         if (S->tramp && S->dTdt != 0.0) {
+            S->p.temperature = ramp_temperature(S, S->t + h);
+            if (S->has_tab) o_calphad_at(&S->tab, S->p.temperature, S->p.calphad);
+         }
+         bdf_predict(S, 1.0);
+         memcpy(S->ypred, S->y, sizeof(double) * nv);
+         bdf_coeffs(S, h);
+         gamma = h / S->l[1];
+         axpby(nv, 1.0, S->ypred, -1.0 / S->l[1], S->z1, S->ahist);
+         delta = S->s.eplifac * S->tq[4] * nrmfac;
+         predicted = 1;
+      }
+      int did_setup = 0;
+      int r = newton(S, gamma, delta, S->tq[4], force_setup, &did_setup);
+      if (r < 0) return -1;
+      if (r != 0) {
+         S->st.ncfn++;
+         if (!did_setup) {
+            force_setup = 1;
+            continue;
+         }
+         bdf_predict(S, -1.0);
+         predicted = 0;
+         ncf++;
+         S->etamax = 1.0;
+         if (ncf >= 10 || h <= hmin) return 1;
+         bdf_rescale(S, 0.25);
+         force_setup = 1;
+         continue;
+      }
+      axpby(nv, 1.0, S->ycur, -1.0, S->ypred, S->acor);
+      if (S->iq >= 0 && S->p.qlen > 1) {
+         memset(S->corr, 0, sizeof(double) * nv);
+         o_quat_project(G, &S->p, S->ycur + (long)S->iq * ng, S->corr + (long)S->iq * ng,
+                        S->acor + (long)S->iq * ng);
+         axpby(nv, 1.0, S->ycur, 1.0, S->corr, S->ycur);
+      }
+      double acnrm = wrms(S, S->acor);
+      S->st.last_acnrm = acnrm;
+      double dsm = acnrm * S->tq[2];
+      if (dbg())
+         fprintf(stderr, "adaptive t=%.6e h=%.6e q=%d dsm=%.4e\n", S->t, h, S->q, dsm);
+      if (dsm > 1.0) {
+         bdf_predict(S, -1.0);
+         predicted = 0;
+         nef++;
+         S->st.netf++;
+         S->etamax = 1.0;
+         if (nef >= 7 || h <= hmin) return 1;
+         if (nef <= 3) {
+            double eta = eta_of(6.0, dsm, S->L);
+            eta = eta > 0.1 ? eta : 0.1;
+            if (nef >= 2 && eta > 0.2) eta = 0.2;
+            bdf_rescale(S, eta);
+         } else if (S->q > 1) {
+            bdf_adjust_order(S, S->q - 1);
+            bdf_rescale(S, 0.1);
+         } else {
+            S->h = 0.1 * h;
+            S->hscale = S->h;
+            S->qwait = 10;
+            if (bdf_load_z1(S, S->t)) return -1;
+         }
+         continue;
+      }
+      axpby(nv, 1.0, S->ycur, -1.0, S->ypred, S->work);
+      S->nst++;
+      S->st.nst++;
+      for (int i = S->q; i >= 2; i--) S->tau[i] = S->tau[i - 1];
+      if (S->q == 1 && S->nst > 1) S->tau[2] = S->tau[1];
+      S->tau[1] = h;
+      for (int j = 0; j <= S->q; j++) axpby(nv, S->l[j], S->work, 1.0, zcol(S, j), zcol(S, j));
+      S->qwait--;
+      if (S->qwait == 1 && S->q != qmax) {
+         memcpy(S->z2, S->work, sizeof(double) * nv);
+         S->saved_tq5 = S->tq[5];
+      }
+      S->t += h;
+      S->st.q = S->q;
+      bdf_prepare_next(S, dsm, S->work);
+      S->etamax = 10.0;
+      update_ewt(S, S->y);
+      return 0;
+   }
+}
+
 int o_sim_advance(OSim* S, int nsteps, ampe_stats* st)
 {
    int ret = 0;
    for (int n = 0; n < nsteps; n++) {
-      ret = bdf_step(S);
+      ret = S->s.adaptive ? bdf_adaptive_step(S) : bdf_step(S);
       if (ret) break;
    }
    S->st.t = S->t;
-   S->st.h = S->h;
+   S->st.h = S->s.adaptive ? S->hprime : S->h;
+   if (!S->s.adaptive) S->st.q = (S->have_prev && S->s.max_order >= 2) ? 2 : 1;
    if (st) *st = S->st;
    return ret;
 }

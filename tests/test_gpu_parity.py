@@ -623,3 +623,49 @@ def test_integrator_mixed_precision_close_to_fp64(dev):
     assert abs(sa["t"] - sb["t"]) < 1e-15
     for name, x, y in zip(("phi", "c", "q"), a[:3], b[:3]):
         assert _rel(y, x) < 1e-6, (name, _rel(y, x))
+
+
+@pytest.mark.parametrize("case", ["two_grain_2d", "calphad_3d"])
+def test_adaptive_integrator_matches_oracle(dev, case):
+    """Adaptive BDF (local error test, step and order selection, CVODE's
+    fixed-leading-coefficient form) on the device against the oracle step by
+    step: identical step sizes, orders, error-test and convergence failures
+    and Newton / Krylov counters, every field within 1e-10."""
+    from oracle import oracle as O
+    from ampe_amd import hip as H
+    if case == "two_grain_2d":
+        n = 48
+        p, phi, c, q = _two_grain(n)
+        shape = (n, n)
+        s = P.solver_defaults(dt=1e-2, dt_init=1e-4, maxl=10, max_newton=4, atol=1e-5,
+                              rtol=1e-7, adaptive=1)
+        nsteps = 12
+    else:
+        n = 16
+        p = P.polycrystal_params(seed=1002)
+        phi, c, q = P.polycrystal_state(n=n, seed=1002, ngrains=4)
+        shape = (n, n, n)
+        s = P.solver_defaults(dt=1e-6, dt_init=1e-9, maxl=10, max_newton=4, atol=1e-6,
+                              rtol=1e-8, adaptive=1)
+        nsteps = 8
+    osim = O.Sim(p, s, shape[::-1] if len(shape) == 3 else (n, n, 1))
+    osim.set_state(phi, c, q)
+    box = H.boxes_of(shape, len(shape))
+    gsim = H.Sim(p, s, [box], periodic=box)
+    gsim.set_state(phi, c, q)
+    qs = set()
+    for step in range(nsteps):
+        ro, sto = osim.advance(1)
+        rg, stg = gsim.advance(1)
+        assert ro == rg == 0, (step, sto, stg)
+        for k in ("nst", "nni", "nli", "ncfn", "netf", "q"):
+            assert sto[k] == stg[k], (step, k, sto, stg)
+        assert abs(sto["t"] - stg["t"]) <= 1e-14 * sto["t"], (step, sto["t"], stg["t"])
+        assert abs(sto["h"] - stg["h"]) <= 1e-12 * sto["h"], (step, sto["h"], stg["h"])
+        fo = osim.get_state()
+        fg = gsim.get_state(shape)
+        for name, a, b in zip(("phi", "c", "q"), fg[:3], fo[:3]):
+            e = _rel(a, b)
+            assert e < 1e-10, (step, name, e, sto, stg)
+        qs.add(stg["q"])
+    assert qs == {1, 2}, qs

@@ -314,3 +314,46 @@ def test_bdf_second_order(O):
         errs.append(abs(phi[0, 0] - ref))
     ratio = errs[0] / errs[1]
     assert 3.3 < ratio < 4.7, (errs, ratio)
+
+
+def test_adaptive_bdf_error_control(O):
+    """Adaptive BDF (CVODE's local error test and step / order selection) on
+    the uniform-phi ODE: the global error tracks the tolerance, the step count
+    grows like tol^(-1/3) as for a second-order method, the order rises to 2,
+    and rejected steps are retried (error-test failures > 0 at some tolerance)."""
+    from scipy.integrate import solve_ivp
+    from oracle import oracle as OO
+    p = P.two_grain_params(evolve_quat=0, with_conc=0)
+    k = p.phi_mobility * p.phi_well_scale
+    ode = lambda t, y: [-k * 32 * y[0] * (1 - y[0]) * (1 - 2 * y[0])]  # noqa: E731
+    T = 4e-2
+    ref = solve_ivp(ode, (0, 2 * T), [0.3], rtol=1e-13, atol=1e-15, method="DOP853",
+                    dense_output=True)
+    res = {}
+    netf = 0
+    for tol in (1e-4, 1e-7):
+        s = P.solver_defaults(dt=1e-2, dt_init=1e-5, atol=tol, rtol=tol, maxl=5, max_newton=4,
+                              adaptive=1)
+        sim = OO.Sim(p, s, (8, 8, 1))
+        sim.set_state(np.full((8, 8), 0.3), None, None)
+        n, q2 = 0, 0
+        while True:
+            rc, st = sim.advance(1)
+            assert rc == 0, st
+            n += 1
+            q2 += st["q"] == 2
+            if st["t"] >= T:
+                break
+        phi = sim.get_state()[0]
+        # the inexact red-black preconditioned solve leaves a red / black
+        # difference of the order of the Newton tolerance
+        assert np.ptp(phi) < 10 * tol
+        err = abs(phi.mean() - ref.sol(st["t"])[0])
+        assert err < 10 * tol, (tol, err)
+        assert q2 > n // 2
+        netf += st["netf"]
+        res[tol] = (n, err)
+    (n4, e4), (n7, e7) = res[1e-4], res[1e-7]
+    assert e7 < 1e-2 * e4
+    assert 5 < n7 / n4 < 12, (n4, n7)
+    assert netf > 0
