@@ -17,9 +17,11 @@ using ampe::Level;
 
 struct ampe_level {
    Level* L;
+   bool owned = true;  // false: a view of a sim's level (ampe_sim_level)
 };
 struct ampe_sim {
    ampe::Sim* S;
+   ampe_level view{nullptr, false};
 };
 struct ampe_surrogate {
    ampe::Surrogate* S;
@@ -62,6 +64,7 @@ int ampe_level_create(int ndim, const ampe_box* boxes, int npatch, int ghost,
 int ampe_level_destroy(ampe_level* level)
 {
    if (!level) return 0;
+   AMPE_REQUIRE(level->owned, "ampe_level_destroy: the level belongs to a sim");
    ampe::level_destroy(level->L);
    delete level;
    return 0;
@@ -427,7 +430,85 @@ int ampe_sim_create(const ampe_model_params* p, const ampe_solver_params* s,
    ampe::Sim* S = nullptr;
    if (ampe::sim_create(*p, *s, boxes, npatch, periodic_domain, &S)) return -1;
    *out = new ampe_sim{S};
+   (*out)->view.L = S->L;
    return 0;
+}
+
+int ampe_sim_vector_size(const ampe_sim* sim, long long* n, int* ncomp)
+{
+   AMPE_REQUIRE(sim, "null argument");
+   if (n) *n = sim->S->nv;
+   if (ncomp) *ncomp = sim->S->nc;
+   return 0;
+}
+
+int ampe_sim_level(ampe_sim* sim, ampe_level** level)
+{
+   AMPE_REQUIRE(sim && level, "null argument");
+   *level = &sim->view;
+   return 0;
+}
+
+int ampe_sim_stream(const ampe_sim* sim, ampe_stream_t* stream)
+{
+   AMPE_REQUIRE(sim && stream, "null argument");
+   *stream = reinterpret_cast<ampe_stream_t>(sim->S->stream);
+   return 0;
+}
+
+int ampe_rhs_eval(ampe_sim* sim, double t, double* y, double* ydot, int fd_flag)
+{
+   AMPE_REQUIRE(sim && y && ydot, "null argument");
+   return ampe::sim_cb_rhs(sim->S, t, y, ydot, fd_flag);
+}
+
+int ampe_precond_setup(ampe_sim* sim, double t, double* y, double gamma, int* jcur)
+{
+   AMPE_REQUIRE(sim && y, "null argument");
+   AMPE_REQUIRE(gamma > 0.0, "gamma must be positive");
+   if (ampe::sim_cb_psetup(sim->S, t, y, gamma)) return -1;
+   if (jcur) *jcur = 1;
+   return 0;
+}
+
+int ampe_precond_solve(ampe_sim* sim, double t, const double* r, double* z, double gamma,
+                       double delta)
+{
+   // the block operators were built by the last setup with its gamma, which
+   // the solve uses (CVODE reuses a setup while gamma stays within 30 %)
+   (void)t;
+   (void)gamma;
+   AMPE_REQUIRE(sim && r && z, "null argument");
+   return ampe::sim_cb_psolve(sim->S, r, z, delta);
+}
+
+int ampe_apply_projection(ampe_sim* sim, const double* y, double* corr, double* err)
+{
+   AMPE_REQUIRE(sim && y && corr && err, "null argument");
+   return ampe::sim_cb_project(sim->S, y, corr, err);
+}
+
+int ampe_sim_reset_ref_concentrations(ampe_sim* sim)
+{
+   AMPE_REQUIRE(sim, "null argument");
+   return ampe::sim_cb_reset_ref(sim->S);
+}
+
+int ampe_sim_kks_status(ampe_sim* sim, ampe_stats* stats)
+{
+   AMPE_REQUIRE(sim, "null argument");
+   int rc = ampe::sim_cb_kks_check(sim->S);
+   if (stats) *stats = sim->S->st;
+   return rc;
+}
+
+int ampe_vec_axpby(long long n, double a, const double* x, double b, const double* y,
+                   double* z, ampe_stream_t stream)
+{
+   AMPE_REQUIRE(n >= 0, "negative n");
+   AMPE_REQUIRE(n == 0 || (x && y && z), "null argument");
+   if (n == 0) return 0;
+   return ampe::vec_axpby(n, a, x, b, y, z, hs(stream));
 }
 
 int ampe_sim_destroy(ampe_sim* sim)

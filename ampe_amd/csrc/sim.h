@@ -87,4 +87,13 @@ int sim_set_temperature_ramp(Sim* S, double T0, double dTdt, double T_target,
 
 int vec_mul_comp(Sim* S, double* x, const double* sm, int ncomp);
 
+// Integrator callbacks on caller-owned state vectors (nv doubles, the sim's
+// layout), enqueued on the sim's stream.
+int sim_cb_rhs(Sim* S, double t, double* y, double* ydot, int fd);
+int sim_cb_psetup(Sim* S, double t, double* y, double gamma);
+int sim_cb_psolve(Sim* S, const double* r, double* z, double delta);
+int sim_cb_project(Sim* S, const double* y, double* corr, double* err);
+int sim_cb_reset_ref(Sim* S);
+int sim_cb_kks_check(Sim* S);
+
 }  // namespace ampe

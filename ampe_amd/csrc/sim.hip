@@ -1393,6 +1393,61 @@ static int bdf_adaptive_step(Sim* S)
    }
 }
 
+// ------------------------------------------------------------ callbacks
+// The integrator callbacks on caller-owned state vectors (nc ghosted
+// component blocks), for a host that keeps its own time integrator.
+
+static void set_time(Sim* S, double t)
+{
+   if (S->tramp && S->dTdt != 0.0) {
+      S->p.temperature = ramp_temperature(S, t);
+      if (S->has_tab) calphad_at(S->tab, S->p.temperature, S->p.calphad);
+   }
+}
+
+int sim_cb_rhs(Sim* S, double t, double* y, double* ydot, int fd)
+{
+   set_time(S, t);
+   return rhs_eval(S, y, ydot, fd != 0);
+}
+
+int sim_cb_psetup(Sim* S, double t, double* y, double gamma)
+{
+   set_time(S, t);
+   if (psetup(S, y, gamma)) return -1;
+   if (S->pgraph && S->pgraph_gamma != gamma) {
+      (void)hipGraphExecDestroy(S->pgraph);
+      S->pgraph = nullptr;
+   }
+   return 0;
+}
+
+int sim_cb_psolve(Sim* S, const double* r, double* z, double delta)
+{
+   return psolve(S, r, z, delta);
+}
+
+int sim_cb_project(Sim* S, const double* y, double* corr, double* err)
+{
+   hipStream_t st = S->stream;
+   AMPE_HIP_CHECK(hipMemsetAsync(corr, 0, sizeof(double) * S->nv, st));
+   if (S->iq < 0 || S->p.qlen <= 1) return 0;
+   long long off = (long long)S->iq * S->ng;
+   return launch_quat_project(*S->L, S->p, y + off, corr + off, err + off, st);
+}
+
+int sim_cb_reset_ref(Sim* S)
+{
+   if (S->ic < 0) return 0;
+   AMPE_HIP_CHECK(hipMemcpyAsync(S->clr, S->cl, sizeof(double) * S->ng,
+                                 hipMemcpyDeviceToDevice, S->stream));
+   AMPE_HIP_CHECK(hipMemcpyAsync(S->csr, S->cs, sizeof(double) * S->ng,
+                                 hipMemcpyDeviceToDevice, S->stream));
+   return 0;
+}
+
+int sim_cb_kks_check(Sim* S) { return kks_collect(S); }
+
 int sim_advance(Sim* S, int nsteps, ampe_stats* stats)
 {
    int ret = 0;

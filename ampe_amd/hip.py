@@ -95,6 +95,16 @@ _SIGS = {
     "ampe_surrogate_train": (_i, [_vp, _ll, _vp, _vp, C.POINTER(TrainParams), _vp, _vp]),
     "ampe_kks_batch": (_i, [C.POINTER(ModelParams), C.POINTER(CalphadTdep), _ll, _vp, _vp, _vp,
                             _vp, _i, _d, _vp, _vp, _vp, C.POINTER(KksBatchStats), _vp]),
+    "ampe_sim_vector_size": (_i, [_vp, C.POINTER(_ll), C.POINTER(_i)]),
+    "ampe_sim_level": (_i, [_vp, C.POINTER(_vp)]),
+    "ampe_sim_stream": (_i, [_vp, C.POINTER(_vp)]),
+    "ampe_rhs_eval": (_i, [_vp, _d, _vp, _vp, _i]),
+    "ampe_precond_setup": (_i, [_vp, _d, _vp, _d, C.POINTER(_i)]),
+    "ampe_precond_solve": (_i, [_vp, _d, _vp, _vp, _d, _d]),
+    "ampe_apply_projection": (_i, [_vp, _vp, _vp, _vp]),
+    "ampe_sim_reset_ref_concentrations": (_i, [_vp]),
+    "ampe_sim_kks_status": (_i, [_vp, C.POINTER(Stats)]),
+    "ampe_vec_axpby": (_i, [_ll, _d, _vp, _d, _vp, _vp, _vp]),
 }
 
 EXPORTED = tuple(_SIGS)

@@ -660,6 +660,61 @@ int ampe_sim_last_time_ms(const ampe_sim* sim, double* ms);
 int ampe_sim_set_surrogate(ampe_sim* sim, const ampe_surrogate* s,
                            int probe_its, double probe_tol, double switch_tol);
 
+/* ------------------------------------------------- integrator callbacks */
+/* The callbacks of CVODEAbstractFunctions (source/samrai/
+ * CVODEAbstractFunctions.h:61-88) as QuatIntegrator implements them, for a
+ * host that keeps its own CVODE (or other BDF / Newton-Krylov loop) and hands
+ * the library caller-owned device state vectors.  A state vector holds the
+ * sim's nc components (phi, c, q0..q3, as the model has them), each a ghosted
+ * cell field of the sim's level, ampe_sim_vector_size() doubles in all; ghost
+ * entries are scratch, filled by the callee.  Dots and WRMS norms over such
+ * vectors: ampe_vec_dot / ampe_vec_wrms on ampe_sim_level() with ncomp = nc
+ * (interior cells only; multiply dots by the cell volume for SAMRAI's
+ * control-volume weighting).  The sim keeps the lagged face data, the phase
+ * concentrations and the preconditioner operators between calls, as
+ * QuatIntegrator does.  Work is enqueued on ampe_sim_stream(); run the
+ * caller's own vector operations on that stream. */
+int ampe_sim_vector_size(const ampe_sim* sim, long long* n, int* ncomp);
+/* Borrowed handle of the sim's level (valid while the sim lives; not to be
+ * passed to ampe_level_destroy). */
+int ampe_sim_level(ampe_sim* sim, ampe_level** level);
+int ampe_sim_stream(const ampe_sim* sim, ampe_stream_t* stream);
+/* evaluateRHSFunction(t, y, ydot, fd_flag) (source/QuatIntegrator.cc
+ * :3134-3298): ydot = f(t, y); fd_flag = 1 for the difference-quotient Jv
+ * evaluations, which reuse the orientation face gradients and composition
+ * face coefficients of the last fd_flag = 0 call (lag_quat_sidegrad).
+ * Temperature at t when a ramp is set.  The per-cell KKS solve runs inside;
+ * ampe_sim_kks_status() reports a failed cell. */
+int ampe_rhs_eval(ampe_sim* sim, double t, double* y, double* ydot, int fd_flag);
+/* CVSpgmrPrecondSet(t, y, fy, jok, jcur, gamma) (source/QuatIntegrator.cc
+ * :3300-3376): coefficients of the phase, composition and orientation blocks
+ * of I - gamma J at y (and the multigrid hierarchy when precond = MG);
+ * *jcur = 1 (may be NULL). */
+int ampe_precond_setup(ampe_sim* sim, double t, double* y, double gamma, int* jcur);
+/* CVSpgmrPrecondSolve(t, y, fy, r, z, gamma, delta, lr) (source/
+ * QuatIntegrator.cc:3666-3771, left preconditioning): z ~ P^-1 r with the
+ * blocks of the last setup (its gamma; the gamma argument is accepted for
+ * the CVODE signature).  0: every block's FAC loop reached the RMS tolerance
+ * delta; 1: not (recoverable, as in the reference); < 0: error. */
+int ampe_precond_solve(ampe_sim* sim, double t, const double* r, double* z, double gamma,
+                       double delta);
+/* applyProjection (source/QuatIntegrator.cc:3911-3965): corr = q/|q| - q on
+ * the orientation components (0 elsewhere); err loses its component along
+ * q/|q| there. */
+int ampe_apply_projection(ampe_sim* sim, const double* y, double* corr, double* err);
+/* resetRefPhaseConcentrations (source/QuatModel.cc:5218-5235): the last
+ * phase concentrations become the KKS Newton guesses; call once per step. */
+int ampe_sim_reset_ref_concentrations(ampe_sim* sim);
+/* Synchronises the sim's stream; stats (may be NULL) receives the counters
+ * of the calls so far; -1 if a KKS cell failed to converge. */
+int ampe_sim_kks_status(ampe_sim* sim, ampe_stats* stats);
+
+/* z = a x + b y over n doubles (N_VLinearSum; SAMRAIVectorReal::linearSum
+ * as reached through Sundials_SAMRAIVector, source/samrai/
+ * Sundials_SAMRAIVector.cc:106-203). */
+int ampe_vec_axpby(long long n, double a, const double* x, double b, const double* y,
+                   double* z, ampe_stream_t stream);
+
 #ifdef __cplusplus
 }
 #endif
