@@ -135,7 +135,6 @@ class Case:
             o_off.append(off)
             o_shape.append(sh)
             off += int(np.prod(sh))
-        dir_off = [sum(self.lev.nside[:d]) for d in range(3)]
         blocks = []
         for q, b in enumerate(self.boxes):
             n = [b.hi[a] - b.lo[a] + 1 for a in range(3)]
@@ -143,7 +142,8 @@ class Case:
                 n[2] = 1
             for d in range(self.ndim):
                 sh = (n[2] + (d == 2), n[1] + (d == 1), n[0] + (d == 0))
-                blocks.append((d, dir_off[d] + self.lev.side_off[q][d], sh,
+                # side_off is the patch's offset in the whole face array
+                blocks.append((d, self.lev.side_off[q][d], sh,
                                (b.lo[2], b.lo[1], b.lo[0])))
         return o_off, o_shape, blocks
 
