@@ -743,4 +743,18 @@ int ampe_kks_batch(const ampe_model_params* p, const ampe_calphad_tdep* tab, lon
                           probe_tol, cl, cs, iters, stats, hs(stream));
 }
 
+int ampe_kks_batch_noalloc(const ampe_model_params* p, const ampe_calphad_tdep* tab,
+                           long long n, const double* c, const double* phi, const double* T,
+                           const ampe_surrogate* guess, int probe_its, double probe_tol,
+                           double* cl, double* cs, int* iters, ampe_kks_batch_stats* stats_dev,
+                           ampe_stream_t stream)
+{
+   AMPE_REQUIRE(p && tab, "null argument");
+   AMPE_REQUIRE(n >= 0, "negative n");
+   AMPE_REQUIRE(n == 0 || (c && phi && T && cl && cs), "null argument");
+   AMPE_REQUIRE(!guess || probe_its >= 0, "probe_its must be >= 0");
+   return ampe::kks_batch_noalloc(*p, *tab, n, c, phi, T, guess ? guess->S : nullptr,
+                                  probe_its, probe_tol, cl, cs, iters, stats_dev, hs(stream));
+}
+
 }  // extern "C"

@@ -43,5 +43,11 @@ int kks_batch(const ampe_model_params& p, const ampe_calphad_tdep& tab, long lon
               const double* c, const double* phi, const double* T, const Surrogate* guess,
               int probe_its, double probe_tol, double* cl, double* cs, int* iters,
               ampe_kks_batch_stats* stats, hipStream_t s);
+// The same without device allocation or host synchronisation: workgroup
+// partials in a file-scope device array; statistics to device memory.
+int kks_batch_noalloc(const ampe_model_params& p, const ampe_calphad_tdep& tab, long long n,
+                      const double* c, const double* phi, const double* T,
+                      const Surrogate* guess, int probe_its, double probe_tol, double* cl,
+                      double* cs, int* iters, ampe_kks_batch_stats* stats_dev, hipStream_t s);
 
 }  // namespace ampe

@@ -16,6 +16,7 @@
 // synchronisation per call).
 #include <algorithm>
 #include <cstring>
+#include <mutex>
 
 #include "kks.h"
 
@@ -534,6 +535,67 @@ int kks_batch(const ampe_model_params& p, const ampe_calphad_tdep& tab, long lon
    }
    kks_batch_stats_of(h, stats);
    return 0;
+}
+
+// ------------------------------------------------------------ batch without allocation
+// The workgroup partials of the batch live in one file-scope device array per
+// GPU: kKksPart doubles for each of at most kKksBatchMaxBlocks workgroups (the
+// batch grid never has more, whatever n), then four 8-byte counters that
+// stand in for the caller's statistics when it passes none.  In host code the
+// array's name is not a device address; the address comes from
+// hipGetSymbolAddress, once per device with that device current.
+constexpr long long kKksBatchScratchDoubles = (long long)kKksPart * kKksBatchMaxBlocks + 4;
+static __device__ __align__(256) double scratch[kKksBatchScratchDoubles];
+
+constexpr int kKksScratchMaxDevices = 64;
+static std::mutex g_kks_scratch_mu;
+static double* g_kks_scratch[kKksScratchMaxDevices] = {};
+
+static int kks_batch_scratch(double** out)
+{
+   int dev = -1;
+   AMPE_HIP_CHECK(hipGetDevice(&dev));
+   if (dev < 0 || dev >= kKksScratchMaxDevices) {
+      set_error("kks_batch_noalloc: device ordinal out of range");
+      return -1;
+   }
+   std::lock_guard<std::mutex> lk(g_kks_scratch_mu);
+   if (!g_kks_scratch[dev]) {
+      void* ptr = nullptr;
+      hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+      if (e != hipSuccess || !ptr) {
+         set_error(std::string("kks_batch_noalloc: hipGetSymbolAddress: ") +
+                   hipGetErrorString(e));
+         return -1;
+      }
+      g_kks_scratch[dev] = static_cast<double*>(ptr);
+   }
+   *out = g_kks_scratch[dev];
+   return 0;
+}
+
+int kks_batch_noalloc(const ampe_model_params& p, const ampe_calphad_tdep& tab, long long n,
+                      const double* c, const double* phi, const double* T,
+                      const Surrogate* guess, int probe_its, double probe_tol, double* cl,
+                      double* cs, int* iters, ampe_kks_batch_stats* stats_dev, hipStream_t s)
+{
+   if (kks_batch_check(p, tab, n, guess)) return -1;
+   if (kks_batch_blocks(n) > kKksBatchMaxBlocks) {
+      set_error("kks_batch_noalloc: grid exceeds the scratch partials");
+      return -1;
+   }
+   double* base = nullptr;
+   if (kks_batch_scratch(&base)) return -1;
+   long long* acc = stats_dev ? reinterpret_cast<long long*>(stats_dev)
+                              : reinterpret_cast<long long*>(base + kKksPart * kKksBatchMaxBlocks);
+   if (n == 0) {
+      // zero counters: the finish kernel over no partials writes zeros
+      hipLaunchKernelGGL(k_kks_batch_finish, dim3(1), dim3(1024), 0, s, base, 0, acc);
+      AMPE_HIP_CHECK(hipGetLastError());
+      return 0;
+   }
+   return kks_batch_launch(p, tab, n, c, phi, T, guess, probe_its, probe_tol, cl, cs, iters,
+                           base, acc, s);
 }
 
 }  // namespace ampe

@@ -95,6 +95,8 @@ _SIGS = {
     "ampe_surrogate_train": (_i, [_vp, _ll, _vp, _vp, C.POINTER(TrainParams), _vp, _vp]),
     "ampe_kks_batch": (_i, [C.POINTER(ModelParams), C.POINTER(CalphadTdep), _ll, _vp, _vp, _vp,
                             _vp, _i, _d, _vp, _vp, _vp, C.POINTER(KksBatchStats), _vp]),
+    "ampe_kks_batch_noalloc": (_i, [C.POINTER(ModelParams), C.POINTER(CalphadTdep), _ll, _vp,
+                                    _vp, _vp, _vp, _i, _d, _vp, _vp, _vp, _vp, _vp]),
     "ampe_sim_vector_size": (_i, [_vp, C.POINTER(_ll), C.POINTER(_i)]),
     "ampe_sim_level": (_i, [_vp, C.POINTER(_vp)]),
     "ampe_sim_stream": (_i, [_vp, C.POINTER(_vp)]),
@@ -385,6 +387,19 @@ def kks_batch(p, tab, c, phi, T, cl, cs, iters=None, guess=None, probe_its=3,
                                probe_tol, _dev_ptr(cl), _dev_ptr(cs), _dev_ptr(iters),
                                C.byref(st), _stream_ptr(stream)), "ampe_kks_batch")
     return st.as_dict()
+
+
+def kks_batch_noalloc(p, tab, c, phi, T, cl, cs, stats_dev, iters=None, guess=None,
+                      probe_its=3, probe_tol=1.0e-8, stream=None):
+    """ampe_kks_batch_noalloc: as kks_batch, statistics written to stats_dev
+    (int64 device tensor of 4: updates, largest count, fallbacks, failures,
+    or None); nothing is synchronised."""
+    check(lib().ampe_kks_batch_noalloc(C.byref(p), C.byref(tab), c.numel(), _dev_ptr(c),
+                                       _dev_ptr(phi), _dev_ptr(T),
+                                       guess.h if guess is not None else None, probe_its,
+                                       probe_tol, _dev_ptr(cl), _dev_ptr(cs), _dev_ptr(iters),
+                                       _dev_ptr(stats_dev), _stream_ptr(stream)),
+          "ampe_kks_batch_noalloc")
 
 
 def cluster_mask(mask, cp, lo=(0, 0, 0), max_boxes=4096):

@@ -619,6 +619,23 @@ int ampe_kks_batch(const ampe_model_params* p, const ampe_calphad_tdep* tab,
                    double probe_tol, double* cl, double* cs, int* iters,
                    ampe_kks_batch_stats* stats, ampe_stream_t stream);
 
+/* ampe_kks_batch without allocation: no counterpart in the reference.  For
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  Same arguments and
+ * results as ampe_kks_batch, except that the statistics go to DEVICE memory
+ * (stats_dev: one ampe_kks_batch_stats, may be NULL) and nothing is copied to
+ * the host or synchronised.  The workgroup partials live in a file-scope
+ * device array of 327,712 bytes per GPU (5 doubles for each of at most 8192
+ * workgroups, plus 4 counters), held until the process ends; the batch grid
+ * walks any n >= 0 with a grid stride, so no n exceeds it.  Every call on one
+ * GPU shares that array: calls that overlap on one GPU (different streams or
+ * threads) race, so serialise them. */
+int ampe_kks_batch_noalloc(const ampe_model_params* p, const ampe_calphad_tdep* tab,
+                           long long n, const double* c, const double* phi,
+                           const double* T, const ampe_surrogate* guess, int probe_its,
+                           double probe_tol, double* cl, double* cs, int* iters,
+                           ampe_kks_batch_stats* stats_dev, ampe_stream_t stream);
+
 /* ------------------------------------------------------------- integrator */
 /* BDF (order <= 2, fixed step) Newton-Krylov integrator with SPGMR (MGS,
  * left block preconditioner, difference-quotient Jv) driving the kernels
