@@ -496,7 +496,10 @@ int ampe_refine_add(const ampe_level* fine, const ampe_level* coarse, int ncomp,
  * interpolation of the coarse cells, then q/|q|; coarse ghosts filled).  The
  * reference's symmetry rotation with index 0 is applied as the identity (its
  * qlen 1 / 2 behaviour; for qlen 4 the reference stops).  dx_fine: fine mesh
- * spacing. */
+ * spacing.  These are the symmetry-aware operators (symmetry_aware = true);
+ * with the reference's default (false, source/QuatModel.cc:505, 946-948) an
+ * orientation field is refined by plain LINEAR_REFINE, ampe_linear_refine with
+ * ncomp = qlen, which does not normalise -- the only qlen 4 case. */
 int ampe_quat_coarsen(const ampe_level* coarse, const ampe_level* fine, const double* dx_fine,
                       int qlen, const double* qf, double* qc, ampe_stream_t stream);
 int ampe_quat_refine(const ampe_level* fine, const ampe_level* coarse, const double* dx_fine,
