@@ -46,6 +46,12 @@ static int check_params(const ampe_model_params* p, const Level& L)
    return 0;
 }
 
+namespace ampe {
+// A non-owning handle on a level held elsewhere (the levels of a hierarchy).
+ampe_level* level_view(Level* L) { return new ampe_level{L, false}; }
+void level_view_free(ampe_level* v) { delete v; }
+}  // namespace ampe
+
 extern "C" {
 
 const char* ampe_last_error(void) { return ampe::last_error(); }

@@ -190,4 +190,38 @@ int launch_quat_face_coef(const Level& L, const ampe_model_params& p, const doub
 int launch_sqrt_quat_mobility(const Level& L, const ampe_model_params& p, const double* phi,
                               double* sm, hipStream_t s);
 
+// ------------------------------------------------------------ hierarchy
+// A coarse-fine face of a fine patch: the face (composite side index), the
+// cells below and above it along d (composite indices; one is the patch's
+// ghost), the patch's strides, the ghost's coarse parent and the interior
+// cell's mirror across its coarse cell (Ewing's diametrically opposite cell).
+struct CfFace {
+   long long face, cm, c0, sy, sz, parent, mirror;
+   int d, side;  // side 0: the patch's lower side along d (the ghost is cm)
+};
+// A coarse cell next to coarse-fine faces: for each such face, the coarse
+// face, its two cells and the first of its 2^(ndim-1) fine faces in the
+// finer level's CfFace list, in the order of the side-data coarsening.
+struct CfCorr {
+   long long cell, sy, sz;
+   long long face[6], cm[6], c0[6];
+   int d[6], side[6], fine[6];
+   int nface;
+};
+// Composite right-hand sides: the coarse level's fused kernel has run; each
+// listed coarse cell's divergence takes the finer level's fluxes (their
+// area-weighted average, af / ac) on its coarse-fine faces.
+int launch_hier_phase_corr(int ndim, const CfCorr* cc, int ncorr, const CfFace* ff,
+                           const ampe_model_params& pc, const ampe_model_params& pf,
+                           long long ncg, const double* phi, const double* q, const double* cl,
+                           const double* cs, double af, double ac, double* rhs, hipStream_t s);
+int launch_hier_conc_corr(int ndim, const CfCorr* cc, int ncorr, const CfFace* ff,
+                          const ampe_model_params& pc, const ampe_model_params& pf,
+                          const double* c, const double* phi, const double* Dc,
+                          const double* Dphi, double af, double ac, double* rhs, hipStream_t s);
+int launch_hier_quat_corr(int ndim, const CfCorr* cc, int ncorr, const CfFace* ff,
+                          const ampe_model_params& pc, const ampe_model_params& pf,
+                          long long ncg, const double* phi, const double* q, const double* fc,
+                          double af, double ac, double* rhs, hipStream_t s);
+
 }  // namespace ampe

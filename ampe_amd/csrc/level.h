@@ -83,6 +83,19 @@ int level_create(int ndim, const ampe_box* boxes, int npatch, int ghost,
                  const ampe_box* periodic, Level** out);
 void level_destroy(Level* L);
 
+// One level of a hierarchy placed in composite fields: its patches' ghosted
+// blocks from cell_base inside each component block of ncg_total cells, its
+// faces from side_base[d] inside direction block d of a side field with
+// nside_total[d] faces per direction, interior (compact) offsets from
+// int_base; ghost width 1; gd / gs: its ghost gather (composite indices).
+struct LevelPlacement {
+   long long cell_base = 0, ncg_total = 0, int_base = 0, ncell_total = 0;
+   long long side_base[3] = {0, 0, 0}, nside_total[3] = {0, 0, 0};
+};
+int level_create_placed(int ndim, const ampe_box* boxes, int npatch, const LevelPlacement& pl,
+                        const std::vector<long long>& gd, const std::vector<long long>& gs,
+                        Level** out);
+
 // Ghost gather for ncomp component blocks (double or float fields).
 int ghost_fill(const Level& L, double* f, int ncomp, hipStream_t s);
 int ghost_fill(const Level& L, float* f, int ncomp, hipStream_t s);

@@ -64,6 +64,86 @@ static void xcd_order(std::vector<Tile>& tiles)
    }
 }
 
+// Work lists, device copies of the patch table and the ghost gather, and
+// the reduction buffers of a level whose patch table L->hp is complete.
+static int level_upload(Level* L, const std::vector<long long>& gd,
+                        const std::vector<long long>& gs, Level** out)
+{
+   const int ndim = L->ndim, npatch = L->npatch;
+   // work lists
+   std::vector<Tile> ct, ft, zt, mt;
+   for (int p = 0; p < npatch; p++) {
+      const PatchDev& d = L->hp[p];
+      for (int k = 0; k < d.nz; k++)
+         for (int j0 = 0; j0 < d.ny; j0 += TY)
+            for (int i0 = 0; i0 < d.nx; i0 += TX) ct.push_back({p, i0, j0, k});
+      for (int k = 0; k < d.nz; k += (ndim == 3 ? KZ : 1))
+         for (int j0 = 0; j0 < d.ny; j0 += TY)
+            for (int i0 = 0; i0 < d.nx; i0 += TX) zt.push_back({p, i0, j0, k});
+      for (int k = 0; k < d.nz; k += (ndim == 3 ? KM : 1))
+         for (int j0 = 0; j0 < d.ny; j0 += TY)
+            for (int i0 = 0; i0 < d.nx; i0 += TX) mt.push_back({p, i0, j0, k});
+      int fz = (ndim == 3) ? d.nz + 1 : 1;
+      for (int k = 0; k < fz; k++)
+         for (int j0 = 0; j0 < d.ny + 1; j0 += TY)
+            for (int i0 = 0; i0 < d.nx + 1; i0 += TX) ft.push_back({p, i0, j0, k});
+   }
+   xcd_order(ct);
+   xcd_order(ft);
+   xcd_order(zt);
+   xcd_order(mt);
+   L->nmarch_tiles = (int)mt.size();
+   L->ncell_tiles = (int)ct.size();
+   L->nface_tiles = (int)ft.size();
+   L->ncol_tiles = (int)zt.size();
+
+   L->nghost = (long long)gd.size();
+
+   int maxparts = std::max(L->ncell_tiles, L->nface_tiles);
+   hipError_t e = hipSuccess;
+   e = hipMalloc(&L->dp, sizeof(PatchDev) * npatch);
+   if (e == hipSuccess) e = hipMalloc(&L->cell_tiles, sizeof(Tile) * ct.size());
+   if (e == hipSuccess) e = hipMalloc(&L->face_tiles, sizeof(Tile) * ft.size());
+   if (e == hipSuccess) e = hipMalloc(&L->col_tiles, sizeof(Tile) * zt.size());
+   if (e == hipSuccess) e = hipMalloc(&L->march_tiles, sizeof(Tile) * mt.size());
+   if (e == hipSuccess && L->nghost > 0) {
+      e = hipMalloc(&L->ghost_dst, sizeof(long long) * gd.size());
+      if (e == hipSuccess) e = hipMalloc(&L->ghost_src, sizeof(long long) * gs.size());
+   }
+   if (e == hipSuccess) e = hipMalloc(&L->partial, sizeof(double) * (size_t)maxparts * 16);
+   if (e == hipSuccess) e = hipMalloc(&L->dred, sizeof(double) * 16);
+   if (e == hipSuccess) e = hipHostMalloc(&L->hpinned, sizeof(double) * 64);
+   if (e == hipSuccess) e = hipMalloc(&L->iscratch, sizeof(int) * 64);
+   if (e == hipSuccess)
+      // KKS partials: one slot per 256 ghosted cells plus 256 slots of the
+      // surrogate pass
+      e = hipMalloc(&L->kks_part,
+                    sizeof(double) * 8 * ((size_t)((L->ncg + 255) / 256 + 1) + 256));
+   if (e == hipSuccess)
+      e = hipMemcpy(L->dp, L->hp.data(), sizeof(PatchDev) * npatch, hipMemcpyHostToDevice);
+   if (e == hipSuccess)
+      e = hipMemcpy(L->cell_tiles, ct.data(), sizeof(Tile) * ct.size(), hipMemcpyHostToDevice);
+   if (e == hipSuccess)
+      e = hipMemcpy(L->face_tiles, ft.data(), sizeof(Tile) * ft.size(), hipMemcpyHostToDevice);
+   if (e == hipSuccess)
+      e = hipMemcpy(L->col_tiles, zt.data(), sizeof(Tile) * zt.size(), hipMemcpyHostToDevice);
+   if (e == hipSuccess)
+      e = hipMemcpy(L->march_tiles, mt.data(), sizeof(Tile) * mt.size(), hipMemcpyHostToDevice);
+   if (e == hipSuccess && L->nghost > 0) {
+      e = hipMemcpy(L->ghost_dst, gd.data(), sizeof(long long) * gd.size(), hipMemcpyHostToDevice);
+      if (e == hipSuccess)
+         e = hipMemcpy(L->ghost_src, gs.data(), sizeof(long long) * gs.size(), hipMemcpyHostToDevice);
+   }
+   if (e != hipSuccess) {
+      set_error(std::string("level_create: ") + hipGetErrorString(e));
+      level_destroy(L);
+      return -1;
+   }
+   *out = L;
+   return 0;
+}
+
+
 int level_create(int ndim, const ampe_box* boxes, int npatch, int ghost,
                  const ampe_box* periodic, Level** out)
 {
@@ -152,33 +232,6 @@ int level_create(int ndim, const ampe_box* boxes, int npatch, int ghost,
    L->side_dir_off[2] = soff[0] + soff[1];
    L->nside_tot = soff[0] + soff[1] + soff[2];
 
-   // work lists
-   std::vector<Tile> ct, ft, zt, mt;
-   for (int p = 0; p < npatch; p++) {
-      const PatchDev& d = L->hp[p];
-      for (int k = 0; k < d.nz; k++)
-         for (int j0 = 0; j0 < d.ny; j0 += TY)
-            for (int i0 = 0; i0 < d.nx; i0 += TX) ct.push_back({p, i0, j0, k});
-      for (int k = 0; k < d.nz; k += (ndim == 3 ? KZ : 1))
-         for (int j0 = 0; j0 < d.ny; j0 += TY)
-            for (int i0 = 0; i0 < d.nx; i0 += TX) zt.push_back({p, i0, j0, k});
-      for (int k = 0; k < d.nz; k += (ndim == 3 ? KM : 1))
-         for (int j0 = 0; j0 < d.ny; j0 += TY)
-            for (int i0 = 0; i0 < d.nx; i0 += TX) mt.push_back({p, i0, j0, k});
-      int fz = (ndim == 3) ? d.nz + 1 : 1;
-      for (int k = 0; k < fz; k++)
-         for (int j0 = 0; j0 < d.ny + 1; j0 += TY)
-            for (int i0 = 0; i0 < d.nx + 1; i0 += TX) ft.push_back({p, i0, j0, k});
-   }
-   xcd_order(ct);
-   xcd_order(ft);
-   xcd_order(zt);
-   xcd_order(mt);
-   L->nmarch_tiles = (int)mt.size();
-   L->ncell_tiles = (int)ct.size();
-   L->nface_tiles = (int)ft.size();
-   L->ncol_tiles = (int)zt.size();
-
    // ghost gather: each ghost cell copies the interior cell it images
    std::vector<long long> gd, gs;
    for (int p = 0; p < npatch; p++) {
@@ -229,50 +282,70 @@ int level_create(int ndim, const ampe_box* boxes, int npatch, int ghost,
                             (long long)(sk + gz) * e.sz);
             }
    }
-   L->nghost = (long long)gd.size();
+   return level_upload(L, gd, gs, out);
+}
 
-   int maxparts = std::max(L->ncell_tiles, L->nface_tiles);
-   hipError_t e = hipSuccess;
-   e = hipMalloc(&L->dp, sizeof(PatchDev) * npatch);
-   if (e == hipSuccess) e = hipMalloc(&L->cell_tiles, sizeof(Tile) * ct.size());
-   if (e == hipSuccess) e = hipMalloc(&L->face_tiles, sizeof(Tile) * ft.size());
-   if (e == hipSuccess) e = hipMalloc(&L->col_tiles, sizeof(Tile) * zt.size());
-   if (e == hipSuccess) e = hipMalloc(&L->march_tiles, sizeof(Tile) * mt.size());
-   if (e == hipSuccess && L->nghost > 0) {
-      e = hipMalloc(&L->ghost_dst, sizeof(long long) * gd.size());
-      if (e == hipSuccess) e = hipMalloc(&L->ghost_src, sizeof(long long) * gs.size());
-   }
-   if (e == hipSuccess) e = hipMalloc(&L->partial, sizeof(double) * (size_t)maxparts * 16);
-   if (e == hipSuccess) e = hipMalloc(&L->dred, sizeof(double) * 16);
-   if (e == hipSuccess) e = hipHostMalloc(&L->hpinned, sizeof(double) * 64);
-   if (e == hipSuccess) e = hipMalloc(&L->iscratch, sizeof(int) * 64);
-   if (e == hipSuccess)
-      // KKS partials: one slot per 256 ghosted cells plus 256 slots of the
-      // surrogate pass
-      e = hipMalloc(&L->kks_part,
-                    sizeof(double) * 8 * ((size_t)((L->ncg + 255) / 256 + 1) + 256));
-   if (e == hipSuccess)
-      e = hipMemcpy(L->dp, L->hp.data(), sizeof(PatchDev) * npatch, hipMemcpyHostToDevice);
-   if (e == hipSuccess)
-      e = hipMemcpy(L->cell_tiles, ct.data(), sizeof(Tile) * ct.size(), hipMemcpyHostToDevice);
-   if (e == hipSuccess)
-      e = hipMemcpy(L->face_tiles, ft.data(), sizeof(Tile) * ft.size(), hipMemcpyHostToDevice);
-   if (e == hipSuccess)
-      e = hipMemcpy(L->col_tiles, zt.data(), sizeof(Tile) * zt.size(), hipMemcpyHostToDevice);
-   if (e == hipSuccess)
-      e = hipMemcpy(L->march_tiles, mt.data(), sizeof(Tile) * mt.size(), hipMemcpyHostToDevice);
-   if (e == hipSuccess && L->nghost > 0) {
-      e = hipMemcpy(L->ghost_dst, gd.data(), sizeof(long long) * gd.size(), hipMemcpyHostToDevice);
-      if (e == hipSuccess)
-         e = hipMemcpy(L->ghost_src, gs.data(), sizeof(long long) * gs.size(), hipMemcpyHostToDevice);
-   }
-   if (e != hipSuccess) {
-      set_error(std::string("level_create: ") + hipGetErrorString(e));
-      level_destroy(L);
+int level_create_placed(int ndim, const ampe_box* boxes, int npatch, const LevelPlacement& pl,
+                        const std::vector<long long>& gd, const std::vector<long long>& gs,
+                        Level** out)
+{
+   if ((ndim != 2 && ndim != 3) || npatch <= 0 || !boxes) {
+      set_error("level_create_placed: bad arguments");
       return -1;
    }
-   *out = L;
-   return 0;
+   Level* L = new Level();
+   L->ndim = ndim;
+   L->g = 1;
+   L->gz = (ndim == 3) ? 1 : 0;
+   L->npatch = npatch;
+   L->hp.resize(npatch);
+   long long coff = pl.cell_base, ioff = pl.int_base;
+   long long soff[3] = {pl.side_base[0], pl.side_base[1], pl.side_base[2]};
+   for (int p = 0; p < npatch; p++) {
+      PatchDev& d = L->hp[p];
+      int n[3];
+      for (int a = 0; a < 3; a++) {
+         n[a] = boxes[p].hi[a] - boxes[p].lo[a] + 1;
+         d.lo[a] = boxes[p].lo[a];
+      }
+      if (ndim == 2) {
+         n[2] = 1;
+         d.lo[2] = 0;
+      }
+      if (n[0] <= 0 || n[1] <= 0 || n[2] <= 0) {
+         set_error("level_create_placed: empty box");
+         delete L;
+         return -1;
+      }
+      d.nx = n[0];
+      d.ny = n[1];
+      d.nz = n[2];
+      d.sy = n[0] + 2;
+      d.sz = (long long)d.sy * (n[1] + 2);
+      d.cell_off = coff;
+      coff += d.sz * (n[2] + 2 * L->gz);
+      d.int_off = ioff;
+      ioff += (long long)n[0] * n[1] * n[2];
+      d.physb = 0;
+      for (int a = 0; a < 3; a++) {
+         d.side_off[a] = soff[a];
+         if (a < ndim)
+            soff[a] += (long long)(n[0] + (a == 0)) * (n[1] + (a == 1)) * (n[2] + (a == 2));
+      }
+   }
+   L->ncg = pl.ncg_total;
+   L->ncell = pl.ncell_total;
+   for (int a = 0; a < 3; a++) L->nside[a] = pl.nside_total[a];
+   L->side_dir_off[0] = 0;
+   L->side_dir_off[1] = pl.nside_total[0];
+   L->side_dir_off[2] = pl.nside_total[0] + pl.nside_total[1];
+   L->nside_tot = pl.nside_total[0] + pl.nside_total[1] + pl.nside_total[2];
+   if (coff > pl.ncg_total || ioff > pl.ncell_total) {
+      set_error("level_create_placed: patches exceed the composite field");
+      delete L;
+      return -1;
+   }
+   return level_upload(L, gd, gs, out);
 }
 
 void level_destroy(Level* L)

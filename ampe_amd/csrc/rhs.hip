@@ -940,4 +940,190 @@ int launch_sqrt_quat_mobility(const Level& L, const ampe_model_params& p, const 
    return 0;
 }
 
+// ------------------------------------------------------------ hierarchy
+// Composite fluxes of PhaseRHSStrategyWithQ::evaluateRHS (source/
+// PhaseRHSStrategyWithQ.cc:126-144), QuatIntegrator::
+// evaluateConcentrationRHS (source/QuatIntegrator.cc:2610-2620) and
+// QuatFACOps::accumulateOperatorOnLevel (source/QuatFACOps.cc:2040-2100):
+// the coarse face on the coarse-fine boundary carries the side-data
+// CONSERVATIVE_COARSEN of the fine fluxes (sum of fine flux * fine face area,
+// divided by the coarse face area).  The fused coarse kernel used the coarse
+// flux there; one thread per listed coarse cell replaces it in the cell's
+// divergence, with the same face-flux functions as the fused kernels.
+template <int NDIM>
+__global__ void __launch_bounds__(256) k_hier_phase_corr(const CfCorr* __restrict__ cc, int n,
+                                                         const CfFace* __restrict__ ff,
+                                                         ampe_model_params pc, RhsConst kc,
+                                                         ampe_model_params pf, RhsConst kf,
+                                                         long long ncg,
+                                                         const double* __restrict__ phi,
+                                                         const double* __restrict__ q,
+                                                         const double* __restrict__ cl,
+                                                         const double* __restrict__ cs, double af,
+                                                         double ac, double* __restrict__ rhs)
+{
+   int t = blockIdx.x * blockDim.x + threadIdx.x;
+   if (t >= n) return;
+   const CfCorr& e = cc[t];
+   const long long stc[3] = {1, e.sy, e.sz};
+   const int nf = NDIM == 3 ? 4 : 2;
+   double delta = 0.0;
+   for (int f = 0; f < e.nface; f++) {
+      const int d = e.d[f];
+      double fold = phase_face_flux<NDIM>(pc, kc, phi, q, ncg, stc, d, e.cm[f], e.c0[f]);
+      double acc = 0.0;
+      for (int r = 0; r < nf; r++) {
+         const CfFace& g = ff[e.fine[f] + r];
+         const long long stf[3] = {1, g.sy, g.sz};
+         acc = acc + phase_face_flux<NDIM>(pf, kf, phi, q, ncg, stf, d, g.cm, g.c0) * af;
+      }
+      double fnew = acc / ac;
+      double t1 = (fnew - fold) * kc.dxinv[d];
+      delta = delta + (e.side[f] ? t1 : -t1);
+   }
+   double mob = pc.phi_mobility;
+   if (pc.with_conc && pc.phi_mobility_type == AMPE_PHI_MOB_KIM)
+      mob = phase_mobility(pc, cl[e.cell], cs[e.cell]);
+   rhs[e.cell] = rhs[e.cell] + mob * delta;
+}
+
+template <int NDIM>
+__global__ void __launch_bounds__(256) k_hier_conc_corr(const CfCorr* __restrict__ cc, int n,
+                                                        const CfFace* __restrict__ ff,
+                                                        ampe_model_params pc, RhsConst kc,
+                                                        RhsConst kf,
+                                                        const double* __restrict__ c,
+                                                        const double* __restrict__ phi,
+                                                        const double* __restrict__ Dc,
+                                                        const double* __restrict__ Dphi,
+                                                        double af, double ac,
+                                                        double* __restrict__ rhs)
+{
+   int t = blockIdx.x * blockDim.x + threadIdx.x;
+   if (t >= n) return;
+   const CfCorr& e = cc[t];
+   const int nf = NDIM == 3 ? 4 : 2;
+   double delta = 0.0;
+   for (int f = 0; f < e.nface; f++) {
+      const int d = e.d[f];
+      const long long fo = e.face[f], m0 = e.cm[f], p0 = e.c0[f];
+      double fold = kc.dxinv[d] * (Dc[fo] * (c[p0] - c[m0]) + Dphi[fo] * (phi[p0] - phi[m0]));
+      double acc = 0.0;
+      for (int r = 0; r < nf; r++) {
+         const CfFace& g = ff[e.fine[f] + r];
+         double jf = kf.dxinv[d] * (Dc[g.face] * (c[g.c0] - c[g.cm]) +
+                                    Dphi[g.face] * (phi[g.c0] - phi[g.cm]));
+         acc = acc + jf * af;
+      }
+      double fnew = acc / ac;
+      double t1 = kc.dxinv[d] * (fnew - fold);
+      delta = delta + (e.side[f] ? t1 : -t1);
+   }
+   rhs[e.cell] = rhs[e.cell] + pc.conc_mobility * delta;
+}
+
+template <int NDIM>
+__global__ void __launch_bounds__(256) k_hier_quat_corr(const CfCorr* __restrict__ cc, int n,
+                                                        const CfFace* __restrict__ ff,
+                                                        ampe_model_params pc, RhsConst kc,
+                                                        RhsConst kf, long long ncg,
+                                                        const double* __restrict__ phi,
+                                                        const double* __restrict__ q,
+                                                        const double* __restrict__ fc,
+                                                        double af, double ac,
+                                                        double* __restrict__ rhs)
+{
+   int t = blockIdx.x * blockDim.x + threadIdx.x;
+   if (t >= n) return;
+   const CfCorr& e = cc[t];
+   const int nf = NDIM == 3 ? 4 : 2;
+   const int ql = pc.qlen;
+   const long long c0 = e.cell;
+   double ddv[4] = {0.0, 0.0, 0.0, 0.0}, dlam = 0.0, sumq2 = 0.0;
+   for (int m = 0; m < ql; m++) {
+      const double* aa = q + m * ncg;
+      double qm = aa[c0];
+      double dl = 0.0;
+      for (int f = 0; f < e.nface; f++) {
+         const int d = e.d[f];
+         double fold = fc[e.face[f]] * (kc.dxinv[d] * (aa[e.c0[f]] - aa[e.cm[f]]));
+         double acc = 0.0;
+         for (int r = 0; r < nf; r++) {
+            const CfFace& g = ff[e.fine[f] + r];
+            acc = acc + fc[g.face] * (kf.dxinv[d] * (aa[g.c0] - aa[g.cm])) * af;
+         }
+         double fnew = acc / ac;
+         double dif = e.side[f] ? (fnew - fold) : -(fnew - fold);
+         ddv[m] = ddv[m] + dif * kc.dxinv[d];
+         dl = dl + dif * kc.hdxinv[d];
+      }
+      dlam = dlam - qm * dl;
+      sumq2 = sumq2 + qm * qm;
+   }
+   dlam = dlam / sumq2;
+   double mq = quat_mobility(pc, phi[c0]);
+   for (int m = 0; m < ql; m++) {
+      double qm = q[m * ncg + c0];
+      rhs[m * ncg + c0] = rhs[m * ncg + c0] - mq * (ddv[m] + 2.0 * qm * dlam);
+   }
+}
+
+int launch_hier_phase_corr(int ndim, const CfCorr* cc, int ncorr, const CfFace* ff,
+                           const ampe_model_params& pc, const ampe_model_params& pf,
+                           long long ncg, const double* phi, const double* q, const double* cl,
+                           const double* cs, double af, double ac, double* rhs, hipStream_t s)
+{
+   if (ncorr <= 0) return 0;
+   RhsConst kc = rhs_const(pc), kf = rhs_const(pf);
+   dim3 g((ncorr + 255) / 256), b(256);
+   if (ndim == 3)
+      hipLaunchKernelGGL(k_hier_phase_corr<3>, g, b, 0, s, cc, ncorr, ff, pc, kc, pf, kf, ncg,
+                         phi, q, cl, cs, af, ac, rhs);
+   else
+      hipLaunchKernelGGL(k_hier_phase_corr<2>, g, b, 0, s, cc, ncorr, ff, pc, kc, pf, kf, ncg,
+                         phi, q, cl, cs, af, ac, rhs);
+   AMPE_HIP_CHECK(hipGetLastError());
+   return 0;
+}
+
+int launch_hier_conc_corr(int ndim, const CfCorr* cc, int ncorr, const CfFace* ff,
+                          const ampe_model_params& pc, const ampe_model_params& pf,
+                          const double* c, const double* phi, const double* Dc,
+                          const double* Dphi, double af, double ac, double* rhs, hipStream_t s)
+{
+   if (ncorr <= 0) return 0;
+   RhsConst kc = rhs_const(pc), kf = rhs_const(pf);
+   dim3 g((ncorr + 255) / 256), b(256);
+   if (ndim == 3)
+      hipLaunchKernelGGL(k_hier_conc_corr<3>, g, b, 0, s, cc, ncorr, ff, pc, kc, kf, c, phi, Dc,
+                         Dphi, af, ac, rhs);
+   else
+      hipLaunchKernelGGL(k_hier_conc_corr<2>, g, b, 0, s, cc, ncorr, ff, pc, kc, kf, c, phi, Dc,
+                         Dphi, af, ac, rhs);
+   AMPE_HIP_CHECK(hipGetLastError());
+   return 0;
+}
+
+int launch_hier_quat_corr(int ndim, const CfCorr* cc, int ncorr, const CfFace* ff,
+                          const ampe_model_params& pc, const ampe_model_params& pf,
+                          long long ncg, const double* phi, const double* q, const double* fc,
+                          double af, double ac, double* rhs, hipStream_t s)
+{
+   if (ncorr <= 0) return 0;
+   if (pc.qlen > 4) {
+      set_error("launch_hier_quat_corr: qlen must be <= 4");
+      return -1;
+   }
+   RhsConst kc = rhs_const(pc), kf = rhs_const(pf);
+   dim3 g((ncorr + 255) / 256), b(256);
+   if (ndim == 3)
+      hipLaunchKernelGGL(k_hier_quat_corr<3>, g, b, 0, s, cc, ncorr, ff, pc, kc, kf, ncg, phi, q,
+                         fc, af, ac, rhs);
+   else
+      hipLaunchKernelGGL(k_hier_quat_corr<2>, g, b, 0, s, cc, ncorr, ff, pc, kc, kf, ncg, phi, q,
+                         fc, af, ac, rhs);
+   AMPE_HIP_CHECK(hipGetLastError());
+   return 0;
+}
+
 }  // namespace ampe

@@ -22,6 +22,18 @@ _d = C.c_double
 _ll = C.c_longlong
 
 
+class HierBlock(C.Structure):
+    """ampe_hier_block"""
+    _fields_ = [("kind", _i), ("D", _vp), ("D_const", _d), ("C", _vp), ("C_const", _d),
+                ("M", _vp), ("M_const", _d), ("S", _vp), ("gamma", _d), ("ewing", _i)]
+
+
+class FacParams(C.Structure):
+    """ampe_fac_params"""
+    _fields_ = [("max_cycles", _i), ("pre", _i), ("post", _i), ("coarse_its", _i),
+                ("coarse_tol", _d)]
+
+
 class EllipticOp(C.Structure):
     _fields_ = [("D", _vp), ("D_const", _d), ("C", _vp), ("C_const", _d),
                 ("M", _vp), ("M_const", _d), ("dx", _d * 3)]
@@ -98,6 +110,21 @@ _SIGS = {
     "ampe_kks_batch_noalloc": (_i, [C.POINTER(ModelParams), C.POINTER(CalphadTdep), _ll, _vp,
                                     _vp, _vp, _vp, _i, _d, _vp, _vp, _vp, _vp, _vp]),
     "ampe_sim_vector_size": (_i, [_vp, C.POINTER(_ll), C.POINTER(_i)]),
+    "ampe_hier_create": (_i, [_i, _i, _vp, _vp, _vp, _vp, _i, C.POINTER(_vp)]),
+    "ampe_hier_destroy": (_i, [_vp]),
+    "ampe_hier_sizes": (_i, [_vp, C.POINTER(_ll), C.POINTER(_ll), C.POINTER(_ll)]),
+    "ampe_hier_level": (_i, [_vp, _i, C.POINTER(_vp)]),
+    "ampe_hier_composite": (_i, [_vp, C.POINTER(_vp)]),
+    "ampe_hier_weights": (_i, [_vp, _vp, _vp]),
+    "ampe_hier_ghost_fill": (_i, [_vp, _vp, _i, _i, _vp]),
+    "ampe_hier_coarsen": (_i, [_vp, _vp, _i, _i, _vp]),
+    "ampe_hier_phase_rhs": (_i, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "ampe_hier_quat_face_norm2": (_i, [_vp, _vp, _vp, _vp, _vp]),
+    "ampe_hier_quat_rhs": (_i, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "ampe_hier_conc_coeffs": (_i, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "ampe_hier_conc_rhs": (_i, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "ampe_hier_fac_solve": (_i, [_vp, _vp, _vp, _vp, _vp, _i, _d, C.POINTER(_d), C.POINTER(_i),
+                                 _vp]),
     "ampe_sim_level": (_i, [_vp, C.POINTER(_vp)]),
     "ampe_sim_stream": (_i, [_vp, C.POINTER(_vp)]),
     "ampe_rhs_eval": (_i, [_vp, _d, _vp, _vp, _i]),
@@ -417,3 +444,48 @@ def cluster_mask(mask, cp, lo=(0, 0, 0), max_boxes=4096):
     check(lib().ampe_cluster_mask(ndim, C.byref(dom), m.ctypes.data_as(_vp), C.byref(cp), out,
                                   max_boxes, C.byref(n)), "ampe_cluster_mask")
     return [(tuple(out[b].lo), tuple(out[b].hi)) for b in range(n.value)]
+
+
+class Hier:
+    """A patch hierarchy on the device (ampe_hier).  levels: list over levels
+    of lists of (lo, hi) inclusive boxes in each level's index space; dom0:
+    level-0 cells (nx, ny[, nz])."""
+
+    def __init__(self, ndim, levels, dom0, dx0=(1.0, 1.0, 1.0), periodic=True):
+        self.ndim = ndim
+        self.levels = [[(tuple(lo), tuple(hi)) for lo, hi in lev] for lev in levels]
+        flat = [b for lev in self.levels for b in lev]
+        arr = (Box * len(flat))()
+        for i, (lo, hi) in enumerate(flat):
+            arr[i].lo[:] = list(lo) + [0] * (3 - len(lo))
+            arr[i].hi[:] = list(hi) + [0] * (3 - len(hi))
+        npat = (_i * len(levels))(*[len(v) for v in levels])
+        d0 = (_i * 3)(*(list(dom0) + [1] * (3 - len(dom0))))
+        dx = (_d * 3)(*dx0)
+        h = _vp()
+        check(lib().ampe_hier_create(ndim, len(levels), npat, arr, d0, dx, 1 if periodic else 0,
+                                     C.byref(h)), "ampe_hier_create")
+        self.h = h
+        ncg, nc = _ll(), _ll()
+        ns = (_ll * 3)()
+        check(lib().ampe_hier_sizes(h, C.byref(ncg), C.byref(nc), ns), "ampe_hier_sizes")
+        self.ncg, self.ncell = ncg.value, nc.value
+        self.nside = [ns[0], ns[1], ns[2]]
+        self.nside_tot = sum(self.nside)
+        self.boxes = flat
+        self.patch_level = [l for l, lev in enumerate(self.levels) for _ in lev]
+
+    def level(self, l):
+        v = _vp()
+        check(lib().ampe_hier_level(self.h, l, C.byref(v)), "ampe_hier_level")
+        return v
+
+    def composite(self):
+        v = _vp()
+        check(lib().ampe_hier_composite(self.h, C.byref(v)), "ampe_hier_composite")
+        return v
+
+    def __del__(self):
+        if getattr(self, "h", None) is not None and _lib is not None:
+            _lib.ampe_hier_destroy(self.h)
+            self.h = None

@@ -735,6 +735,107 @@ int ampe_sim_kks_status(ampe_sim* sim, ampe_stats* stats);
 int ampe_vec_axpby(long long n, double a, const double* x, double b, const double* y,
                    double* z, ampe_stream_t stream);
 
+
+/* ------------------------------------------------------------ hierarchy */
+/* A patch hierarchy of nlev levels, refinement ratio 2 (SAMRAI
+ * PatchHierarchy as QuatModel builds it): level 0 covers the domain dom0
+ * (cells), level l+1's boxes (in its own index space, aligned with level-l
+ * cells) lie properly nested in level l; boxes[] lists npatch[0] boxes of
+ * level 0, then level 1's, ...; dx0: level-0 spacing, equal in every
+ * direction.  Fields on a hierarchy are composite arrays: per component,
+ * every patch's ghosted block (ghost width 1), level by level; side fields
+ * per direction, every patch's faces.  Each level is an ampe_level placed in
+ * those arrays (ampe_hier_level), so every per-level entry point above runs
+ * on it; ampe_hier_composite covers all patches (vector operations, KKS). */
+typedef struct ampe_hier ampe_hier;
+int ampe_hier_create(int ndim, int nlev, const int* npatch, const ampe_box* boxes,
+                     const int dom0[3], const double dx0[3], int periodic, ampe_hier** out);
+int ampe_hier_destroy(ampe_hier* h);
+int ampe_hier_sizes(const ampe_hier* h, long long* ncell_ghost, long long* ncell,
+                    long long nside[3]);
+int ampe_hier_level(ampe_hier* h, int l, ampe_level** level);
+int ampe_hier_composite(ampe_hier* h, ampe_level** level);
+/* Composite weights over ghosted cells: a cell's volume relative to a
+ * level-0 cell, zero under a finer level and on ghosts (the control volumes
+ * of QuatModel::computeVectorWeights, source/QuatModel.cc:4710-4750). */
+int ampe_hier_weights(const ampe_hier* h, double* w, ampe_stream_t stream);
+/* Ghost cells of every level, coarsest first: from the level (periodic
+ * images; zero-slope images across the physical boundary), and on the
+ * coarse-fine boundary from the next coarser level: cf = 0 leaves them,
+ * 1 CONSTANT_REFINE (the coarse parent), 2 LINEAR_REFINE (quatlinrefine's
+ * interpolation without normalisation) -- the refine schedules of
+ * QuatIntegrator::fillScratch (source/QuatIntegrator.cc:2873-2957). */
+int ampe_hier_ghost_fill(const ampe_hier* h, double* f, int ncomp, int cf,
+                         ampe_stream_t stream);
+/* CONSERVATIVE_COARSEN of ncomp components from every finer level onto the
+ * cells it covers, finest first (QuatIntegrator::coarsenData, source/
+ * QuatIntegrator.cc:2289-2331); quatnorm: then q/|q| (QuatWeightedAverage,
+ * at most four components). */
+int ampe_hier_coarsen(const ampe_hier* h, double* f, int ncomp, int quatnorm,
+                      ampe_stream_t stream);
+/* Composite right-hand sides, ghosts of the inputs filled on every level:
+ * each level's fused kernel, then the finer level's fluxes coarsened onto
+ * the coarse-fine faces (PhaseRHSStrategyWithQ::evaluateRHS, source/
+ * PhaseRHSStrategyWithQ.cc:90-149; QuatIntegrator::evaluateConcentrationRHS,
+ * :2578-2690; QuatFACOps::evaluateRHS, source/QuatFACOps.cc:1861-1890).
+ * p0 carries the level-0 spacing.  Only cells not under a finer level are
+ * defined (the others are overwritten by coarsening in the reference). */
+int ampe_hier_phase_rhs(const ampe_hier* h, const ampe_model_params* p0, const double* phi,
+                        const double* q, const double* cl, const double* cs, double* rhs,
+                        ampe_stream_t stream);
+int ampe_hier_quat_face_norm2(const ampe_hier* h, const ampe_model_params* p0, const double* q,
+                              double* n2, ampe_stream_t stream);
+/* fc: side-field workspace (face coefficients) */
+int ampe_hier_quat_rhs(const ampe_hier* h, const ampe_model_params* p0, const double* phi,
+                       const double* q, const double* n2, double* fc, double* rhs,
+                       ampe_stream_t stream);
+/* KKS-form D_c, D_phi on every level's faces, then coarsened onto the
+ * coarser faces (source/QuatIntegrator.cc:2395-2428). */
+int ampe_hier_conc_coeffs(const ampe_hier* h, const ampe_model_params* p0, const double* phi,
+                          const double* cl, const double* cs, double* Dc, double* Dphi,
+                          ampe_stream_t stream);
+int ampe_hier_conc_rhs(const ampe_hier* h, const ampe_model_params* p0, const double* c,
+                       const double* phi, const double* Dc, const double* Dphi, double* rhs,
+                       ampe_stream_t stream);
+
+/* One block of the FAC preconditioner on the hierarchy.  kind 0: elliptic
+ * A u = M div(D grad u) + C u (EllipticFACOps; Ewing's coarse-fine flux fix
+ * when ewing != 0, the reference's default cf_discretization); kind 1: the
+ * orientation block u + gamma S div(D grad(S u)) with S = sqrt(M_q) (cells,
+ * ghosts filled) and D the face coefficients, constant-refined coarse-fine
+ * ghosts (QuatFACOps stops on "Ewing" above level 0, source/QuatFACOps.cc:
+ * 1093-1105, so the choice that runs is taken), red-black level smoothing
+ * in the role of its hypre level solves. */
+typedef struct ampe_hier_block {
+   int kind;
+   const double* D;
+   double D_const;
+   const double* C;
+   double C_const;
+   const double* M;
+   double M_const;
+   const double* S;
+   double gamma;
+   int ewing;
+} ampe_hier_block;
+typedef struct ampe_fac_params {
+   int max_cycles;   /* FACPreconditioner max_cycles */
+   int pre, post;    /* pre- and post-smoothing sweeps */
+   int coarse_its;   /* coarse_solver_max_iterations ("redblack") */
+   double coarse_tol; /* coarse_solver_tolerance (< 0: none) */
+} ampe_fac_params;
+/* FACPreconditioner::solveSystem / facCycle_Recursive (source/samrai/
+ * FACPreconditioner.cc:267-474, 661-680) with EllipticFACOps' smoothing,
+ * composite residual, restriction and CONSTANT_REFINE prolongation (source/
+ * EllipticFACOps.cc:948-1241, 1480-1790): cycles from u (ghosted, ncomp
+ * components: 1 for kind 0) until the weighted RMS composite residual is
+ * below tol.  0: converged, 1: not (recoverable), < 0: error; *norm and
+ * *cycles (host, may be NULL) report the last norm and the cycles.
+ * Synchronises the stream once per cycle (the norm). */
+int ampe_hier_fac_solve(ampe_hier* h, const ampe_hier_block* b, const ampe_fac_params* fp,
+                        const double* f, double* u, int ncomp, double tol, double* norm,
+                        int* cycles, ampe_stream_t stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -98,6 +98,16 @@ void o_quat_face_norm2(const OGeom* G, const ampe_model_params* p,
 void o_phase_rhs(const OGeom* G, const ampe_model_params* p,
                  const double* phi, const double* q, const double* cl,
                  const double* cs, double* rhs);
+void o_phase_rhs_fx(const OGeom* G, const ampe_model_params* p, const double* flux,
+                    const double* phi, const double* q, const double* cl,
+                    const double* cs, double* rhs);
+void o_quat_flux(const OGeom* G, const ampe_model_params* p, const double* phi,
+                 const double* q, const double* n2, double* fl);
+void o_quat_rhs_fx(const OGeom* G, const ampe_model_params* p, const double* phi,
+                   const double* q, const double* fl, double* rhs);
+void o_conc_flux(const OGeom* G, const ampe_model_params* p, const double* c,
+                 const double* phi, const double* Dc, const double* Dphi, double* fl);
+void o_conc_rhs_fx(const OGeom* G, const ampe_model_params* p, const double* fl, double* rhs);
 void o_quat_mobility(const OGeom* G, const ampe_model_params* p,
                      const double* phi, double* mq);
 void o_quat_face_coef(const OGeom* G, const ampe_model_params* p,
@@ -174,6 +184,61 @@ void o_mlp_forward(int nin, int hid, int nout, const float* w, long n, const flo
                    double* y, int emulate);
 double o_mlp_loss_grad(int nin, int hid, int nout, const float* w, long n, const float* x,
                        const float* t, double* grad, int emulate);
+
+/* ---------------------------------------------------------------------------
+ * Multi-level hierarchy (oracle_amr.c): levels of ratio 2, level 0 covering
+ * the domain, ghost width 1; composite cell fields (per component, every
+ * patch's ghosted block, level by level) and side fields (every patch's x, y,
+ * z faces in a row).
+ * ------------------------------------------------------------------------- */
+typedef struct OHier OHier;
+typedef struct OHBlock {
+   int kind;                /* 0 elliptic (EllipticFACOps), 1 orientation block */
+   const double* D;         /* side field or NULL */
+   double D_const;
+   const double* C;         /* cell field or NULL (elliptic) */
+   double C_const;
+   const double* M;         /* cell field or NULL (elliptic) */
+   double M_const;
+   const double* S;         /* orientation: sqrt(M_q), cells with ghosts */
+   double gamma;            /* orientation: operator u + gamma S div(D grad(S u)) */
+   int ewing;               /* Ewing's coarse-fine flux fix (elliptic blocks) */
+} OHBlock;
+typedef struct OHFacParams {
+   int max_cycles, pre, post, coarse_its;
+   double coarse_tol;
+} OHFacParams;
+OHier* o_hier_create(int ndim, int nlev, const int* npatch, const ampe_box* boxes,
+                     const int dom0[3], const double dx0[3], int periodic);
+void o_hier_destroy(OHier* H);
+int o_hier_nlev(const OHier* H);
+int o_hier_npatch(const OHier* H);
+long o_hier_ncg(const OHier* H);
+long o_hier_nf(const OHier* H);
+void o_hier_offsets(const OHier* H, long* coff, long* foff);
+const OGeom* o_hier_geom(const OHier* H, int p);
+void o_hier_ghost_fill_level(const OHier* H, int l, double* f, int ncomp, int cf);
+void o_hier_ghost_fill(const OHier* H, double* f, int ncomp, int cf);
+void o_hier_weights(const OHier* H, double* w);
+void o_hier_coarsen_level(const OHier* H, int l, double* f, int ncomp, int quatnorm);
+void o_hier_coarsen(const OHier* H, double* f, int ncomp, int quatnorm);
+void o_hier_coarsen_faces_to(const OHier* H, int l, const double* src, double* dst, int ncomp,
+                             long stride);
+void o_hier_coarsen_faces(const OHier* H, int l, double* s, int ncomp, long stride);
+void o_hier_phase_rhs(const OHier* H, const ampe_model_params* p0, const double* phi,
+                      const double* q, const double* cl, const double* cs, double* rhs);
+void o_hier_quat_face_norm2(const OHier* H, const ampe_model_params* p0, const double* q,
+                            double* n2);
+void o_hier_quat_rhs(const OHier* H, const ampe_model_params* p0, const double* phi,
+                     const double* q, const double* n2, double* rhs);
+void o_hier_conc_coeffs(const OHier* H, const ampe_model_params* p0, const double* phi,
+                        const double* cl, const double* cs, double* Dc, double* Dphi);
+void o_hier_conc_rhs(const OHier* H, const ampe_model_params* p0, const double* c,
+                     const double* phi, const double* Dc, const double* Dphi, double* rhs);
+double o_hier_comp_residual(const OHier* H, const OHBlock* B, double* u, const double* f,
+                            double* r, int ncomp);
+int o_hier_fac_solve(const OHier* H, const OHBlock* B, const OHFacParams* P, const double* f,
+                     double* u, int ncomp, double tol, double* norm, int* cycles);
 
 #ifdef __cplusplus
 }

@@ -733,8 +733,19 @@ void o_phase_rhs(const OGeom* G, const ampe_model_params* p,
                  const double* cs, double* rhs)
 {
    double* flux = (double*)malloc(sizeof(double) * (G->nfx + G->nfy + G->nfz));
-   double* gmod = NULL;
    o_phase_flux(G, p, phi, q, flux);
+   o_phase_rhs_fx(G, p, flux, phi, q, cl, cs, rhs);
+   free(flux);
+}
+
+/* computerhspbg from a given side flux (the composite flux of a level whose
+ * coarse-fine faces hold the coarsened fine fluxes,
+ * source/PhaseRHSStrategyWithQ.cc:126-144). */
+void o_phase_rhs_fx(const OGeom* G, const ampe_model_params* p, const double* flux,
+                    const double* phi, const double* q, const double* cl,
+                    const double* cs, double* rhs)
+{
+   double* gmod = NULL;
    if (p->evolve_quat) {
       gmod = (double*)calloc(G->ncg, sizeof(double));
       o_quat_grad_modulus(G, p, q, gmod);
@@ -778,7 +789,6 @@ void o_phase_rhs(const OGeom* G, const ampe_model_params* p,
                              : p->phi_mobility;
             rhs[c0] = mob * r;
          }
-   free(flux);
    free(gmod);
 }
 
@@ -826,11 +836,19 @@ void o_quat_rhs(const OGeom* G, const ampe_model_params* p, const double* phi,
                 const double* q, const double* n2, double* rhs)
 {
    long nf = G->nfx + G->nfy + G->nfz;
-   double* fc = (double*)malloc(sizeof(double) * nf);
    double* fl = (double*)malloc(sizeof(double) * nf * p->qlen);
-   double* mq = (double*)malloc(sizeof(double) * G->ncg);
+   o_quat_flux(G, p, phi, q, n2, fl);
+   o_quat_rhs_fx(G, p, phi, q, fl, rhs);
+   free(fl);
+}
+
+/* compute_flux2d/3d: face flux fc * dq_m/dx of every component, fl[m][face] */
+void o_quat_flux(const OGeom* G, const ampe_model_params* p, const double* phi,
+                 const double* q, const double* n2, double* fl)
+{
+   long nf = G->nfx + G->nfy + G->nfz;
+   double* fc = (double*)malloc(sizeof(double) * nf);
    o_quat_face_coef(G, p, phi, n2, fc);
-   o_quat_mobility(G, p, phi, mq);
    long cs[3] = {1, G->sy, G->sz};
    double dinv[3] = {1.0 / G->dx[0], 1.0 / G->dx[1], 1.0 / G->dx[2]};
    for (int m = 0; m < p->qlen; m++) {
@@ -849,6 +867,17 @@ void o_quat_rhs(const OGeom* G, const ampe_model_params* p, const double* phi,
                }
       }
    }
+   free(fc);
+}
+
+/* add_quat_op + add_quat_proj_op from given component fluxes fl[m][face] */
+void o_quat_rhs_fx(const OGeom* G, const ampe_model_params* p, const double* phi,
+                   const double* q, const double* fl, double* rhs)
+{
+   long nf = G->nfx + G->nfy + G->nfz;
+   double* mq = (double*)malloc(sizeof(double) * G->ncg);
+   o_quat_mobility(G, p, phi, mq);
+   double dinv[3] = {1.0 / G->dx[0], 1.0 / G->dx[1], 1.0 / G->dx[2]};
    double fac[3] = {0.5 / G->dx[0], 0.5 / G->dx[1], 0.5 / G->dx[2]};
 #pragma omp parallel for
    for (int k = 0; k < G->nz; k++)
@@ -880,8 +909,6 @@ void o_quat_rhs(const OGeom* G, const ampe_model_params* p, const double* phi,
                rhs[(long)m * G->ncg + c0] = 0.0 - mq[c0] * (dv[m] + 2.0 * qm * lam);
             }
          }
-   free(fc);
-   free(fl);
    free(mq);
 }
 
@@ -959,10 +986,20 @@ void o_conc_rhs(const OGeom* G, const ampe_model_params* p, const double* c,
                 const double* phi, const double* Dc, const double* Dphi,
                 double* rhs)
 {
-   long st[3] = {1, G->sy, G->sz};
-   double dinv[3] = {1.0 / G->dx[0], 1.0 / G->dx[1], 1.0 / G->dx[2]};
    long nf = G->nfx + G->nfy + G->nfz;
    double* fl = (double*)malloc(sizeof(double) * nf);
+   o_conc_flux(G, p, c, phi, Dc, Dphi, fl);
+   o_conc_rhs_fx(G, p, fl, rhs);
+   free(fl);
+}
+
+/* concentrationflux: J = (D_c dc + D_phi dphi) / dx on every face */
+void o_conc_flux(const OGeom* G, const ampe_model_params* p, const double* c,
+                 const double* phi, const double* Dc, const double* Dphi, double* fl)
+{
+   (void)p;
+   long st[3] = {1, G->sy, G->sz};
+   double dinv[3] = {1.0 / G->dx[0], 1.0 / G->dx[1], 1.0 / G->dx[2]};
    for (int dir = 0; dir < G->ndim; dir++) {
       int ex = G->nx + (dir == 0), ey = G->ny + (dir == 1), ez = G->nz + (dir == 2);
       long off = (dir == 0 ? 0 : dir == 1 ? YOFF : ZOFF);
@@ -976,6 +1013,12 @@ void o_conc_rhs(const OGeom* G, const ampe_model_params* p, const double* c,
                                      Dphi[fi] * (phi[c0] - phi[cm]));
             }
    }
+}
+
+/* computerhsconcentration from a given flux */
+void o_conc_rhs_fx(const OGeom* G, const ampe_model_params* p, const double* fl, double* rhs)
+{
+   double dinv[3] = {1.0 / G->dx[0], 1.0 / G->dx[1], 1.0 / G->dx[2]};
 #pragma omp parallel for
    for (int k = 0; k < G->nz; k++)
       for (int j = 0; j < G->ny; j++)
@@ -986,7 +1029,6 @@ void o_conc_rhs(const OGeom* G, const ampe_model_params* p, const double* c,
                s = s + dinv[2] * (fl[ZOFF + FZ(i, j, k + 1)] - fl[ZOFF + FZ(i, j, k)]);
             rhs[CI(i, j, k)] = p->conc_mobility * s;
          }
-   free(fl);
 }
 
 /* EBS face diffusivities, two phases, uniform temperature:
@@ -1051,7 +1093,6 @@ void o_conc_rhs_ebs(const OGeom* G, const ampe_model_params* p, const double* cl
                s = s + dinv[2] * (fl[ZOFF + FZ(i, j, k + 1)] - fl[ZOFF + FZ(i, j, k)]);
             rhs[CI(i, j, k)] = p->conc_mobility * s;
          }
-   free(fl);
 }
 
 /* The same with add_flux_4th (2d/flux.m4:139-242, 3d/flux.m4:235-420):
@@ -1100,7 +1141,6 @@ void o_conc_rhs_ebs_4th(const OGeom* G, const ampe_model_params* p, const double
                s = s + dinv[2] * (fl[ZOFF + FZ(i, j, k + 1)] - fl[ZOFF + FZ(i, j, k)]);
             rhs[CI(i, j, k)] = p->conc_mobility * s;
          }
-   free(fl);
 }
 
 /* Multi-order phase RHS: MultiOrderRHSStrategy::evaluateRHS
@@ -1273,7 +1313,6 @@ void o_conc_rhs_ebs_iso(const OGeom* G, const ampe_model_params* p, const double
                s = s + dinv[2] * (fl[ZOFF + FZ(i, j, k + 1)] - fl[ZOFF + FZ(i, j, k)]);
             rhs[CI(i, j, k)] = p->conc_mobility * s;
          }
-   free(fl);
 }
 
 /* Grains::findAndNumberGrains (source/Grains.cc:263-645) on one patch

@@ -1,7 +1,12 @@
 import os
 import sys
 
-import pytest
+# The oracle's OpenMP loops run on grids of a few thousand cells in these
+# tests: one thread each, so that parallel test workers do not oversubscribe
+# the cores (an explicit OMP_NUM_THREADS is kept).
+os.environ.setdefault("OMP_NUM_THREADS", "1")
+
+import pytest  # noqa: E402
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 if ROOT not in sys.path:

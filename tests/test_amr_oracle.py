@@ -1,0 +1,215 @@
+"""Pins of the oracle's hierarchy restatement (oracle/oracle_amr.c) that do
+not retype its formulas:
+
+- linear fields: linear refinement of the coarse-fine ghosts and the
+  conservative coarsening are exact, constant refinement takes the parent;
+- the control-volume weights add up to the level-0 cell count;
+- composite conservation: with the fine fluxes coarsened onto the coarse-fine
+  faces the composition right-hand side integrates to zero on a periodic
+  hierarchy, and it does not when the levels are evaluated separately;
+- a fine level covering the whole domain reduces the composite right-hand
+  side to the single-level one on the fine grid (the single-patch oracle,
+  another code path);
+- Ewing's coarse-fine fix makes the composite operator exact on linear
+  fields (its weights 2/(1+r), 1/(1+r) and the mirror cell; the reference's
+  single-precision weights leave ~1e-8), while constant refinement alone
+  leaves an O(1) residual next to the coarse-fine boundary;
+- the FAC cycles over levels converge, and with the fine level covering the
+  domain reach the fine single-level solution.
+"""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from ampe_amd import params as P
+from oracle import oracle as O
+
+LEV0 = [((0, 0), (15, 31)), ((16, 0), (31, 31))]
+LEV1 = [((16, 16), (31, 39)), ((32, 20), (47, 35))]
+
+
+def _lin(n, h, a=0.3, b=0.7, c=-0.2):
+    y, x = np.mgrid[0:n, 0:n]
+    return a + b * (x + 0.5) * h + c * (y + 0.5) * h
+
+
+def _fill_levels(H, fn):
+    a = H.cells()
+    for lev in range(len(H.levels)):
+        n = H.dom0[0] << lev
+        H.set_level(a, lev, fn(lev, n))
+    return a
+
+
+def test_weights_and_linear_transfers():
+    H = O.Hier(2, [LEV0, LEV1], (32, 32), periodic=False)
+    w = H.weights()
+    assert w.sum() == 32 * 32
+    a = _fill_levels(H, lambda l, n: _lin(n, 1.0 / (1 << l)))
+    b = a.copy()
+    O.lib().o_hier_ghost_fill(H.h, O.ptr(b), 1, 2)
+    c = a.copy()
+    O.lib().o_hier_ghost_fill(H.h, O.ptr(c), 1, 1)
+    for p in range(H.np):
+        if H.patch_level[p] != 1:
+            continue
+        (lo, hi) = H.boxes[p]
+        gv, gc = H.ghost_view(b, p), H.ghost_view(c, p)
+        for j in range(gv.shape[0]):
+            for i in range(gv.shape[1]):
+                gi, gj = lo[0] + i - 1, lo[1] + j - 1
+                if not (0 <= gi < 64 and 0 <= gj < 64):
+                    continue
+                fine_on_level = any(l0[0] <= gi <= h0[0] and l0[1] <= gj <= h0[1]
+                                    for (l0, h0) in H.levels[1])
+                exact = 0.3 + 0.7 * (gi + 0.5) * 0.5 - 0.2 * (gj + 0.5) * 0.5
+                assert abs(gv[j, i] - exact) < 1e-14
+                if not fine_on_level:
+                    parent = 0.3 + 0.7 * (gi // 2 + 0.5) - 0.2 * (gj // 2 + 0.5)
+                    assert abs(gc[j, i] - parent) < 1e-14
+    d = a.copy()
+    O.lib().o_hier_coarsen(H.h, O.ptr(d), 1, 0)
+    assert np.max(np.abs(H.get_level(d, 0) - _lin(32, 1.0))) < 1e-14
+
+
+def _conc_setup(H, seed):
+    p = P.two_grain_params()
+    rng = np.random.default_rng(seed)
+    c = _fill_levels(H, lambda l, n: 0.4 + 0.05 * rng.standard_normal((n, n)))
+    ph = _fill_levels(H, lambda l, n: rng.uniform(0, 1, (n, n)))
+    for f in (c, ph):
+        O.lib().o_hier_coarsen(H.h, O.ptr(f), 1, 0)
+        O.lib().o_hier_ghost_fill(H.h, O.ptr(f), 1, 2)
+    cl, cs = c.copy(), 1.1 * c
+    return p, c, ph, cl, cs
+
+
+def test_composite_conservation_needs_flux_coarsening():
+    H = O.Hier(2, [LEV0, LEV1], (32, 32), periodic=True)
+    p, c, ph, cl, cs = _conc_setup(H, 1)
+    Dc, Dp = H.sides(), H.sides()
+    O.lib().o_hier_conc_coeffs(H.h, C.byref(p), O.ptr(ph), O.ptr(cl), O.ptr(cs), O.ptr(Dc),
+                               O.ptr(Dp))
+    r = H.cells()
+    O.lib().o_hier_conc_rhs(H.h, C.byref(p), O.ptr(c), O.ptr(ph), O.ptr(Dc), O.ptr(Dp), O.ptr(r))
+    w = H.weights()
+    scale = np.abs(w * r).sum()
+    assert abs((w * r).sum()) < 1e-13 * scale
+    # the same levels evaluated one by one (no coarsened fine fluxes)
+    r2 = H.cells()
+    for pp in range(H.np):
+        l = H.patch_level[pp]
+        pl = P.two_grain_params()
+        pl.dx[:] = [1.0 / (1 << l)] * 3
+        G = H.geoms[pp]
+        o, fo = H.coff[pp], H.foff[pp]
+        sl = slice(o, o + G.ncg)
+        blk = np.zeros(G.ncg)
+        O.lib().o_conc_rhs(C.byref(G), C.byref(pl), O.ptr(np.ascontiguousarray(c[sl])),
+                           O.ptr(np.ascontiguousarray(ph[sl])), O.ptr(Dc[fo:]), O.ptr(Dp[fo:]),
+                           O.ptr(blk))
+        r2[sl] = blk
+    assert abs((w * r2).sum()) > 1e-6 * scale
+
+
+def test_full_fine_level_reduces_to_single_level():
+    """Level 1 covering the periodic domain: its composite RHS is the fine
+    single-level RHS of the single-patch oracle."""
+    lev1 = [((0, 0), (31, 63)), ((32, 0), (63, 63))]
+    H = O.Hier(2, [LEV0, lev1], (32, 32), periodic=True)
+    p = P.two_grain_params()
+    p.flux_type = P.FLUX_ANISOTROPIC
+    p.aniso_nu = 0.05
+    p.aniso_knumber = 4
+    rng = np.random.default_rng(5)
+    phi1 = rng.uniform(0, 1, (64, 64))
+    q1 = rng.normal(size=(4, 64, 64))
+    q1 /= np.sqrt((q1 ** 2).sum(0))
+    phi = H.cells()
+    q = H.cells(4)
+    H.set_level(phi, 1, phi1)
+    for m in range(4):
+        H.set_level(q, 1, q1[m], comp=m)
+    O.lib().o_hier_coarsen(H.h, O.ptr(phi), 1, 0)
+    O.lib().o_hier_coarsen(H.h, O.ptr(q), 4, 1)
+    O.lib().o_hier_ghost_fill(H.h, O.ptr(phi), 1, 2)
+    O.lib().o_hier_ghost_fill(H.h, O.ptr(q), 4, 2)
+    r = H.cells()
+    p.with_conc = 0
+    O.lib().o_hier_phase_rhs(H.h, C.byref(p), O.ptr(phi), O.ptr(q), None, None, O.ptr(r))
+    got = H.get_level(r, 1)
+    # single patch, fine spacing 1/2
+    pf = P.two_grain_params()
+    pf.flux_type, pf.aniso_nu, pf.aniso_knumber, pf.with_conc = P.FLUX_ANISOTROPIC, 0.05, 4, 0
+    pf.dx[:] = [0.5, 0.5, 0.5]
+    pt = O.Patch(2, (64, 64, 1), g=1, dx=(0.5, 0.5, 1.0))
+    a = pt.cells()
+    pt.interior(a)[...] = phi1
+    qa = pt.cells(4)
+    pt.interior(qa)[...] = q1.reshape(pt.interior(qa).shape)
+    O.lib().o_ghost_fill(C.byref(pt.G), O.ptr(a), 1)
+    O.lib().o_ghost_fill(C.byref(pt.G), O.ptr(qa), 4)
+    ro = pt.cells()
+    O.lib().o_phase_rhs(C.byref(pt.G), C.byref(pf), O.ptr(a), O.ptr(qa), None, None, O.ptr(ro))
+    ref = pt.interior(ro).reshape(64, 64)
+    assert np.max(np.abs(got - ref)) < 1e-12 * np.max(np.abs(ref))
+
+
+@pytest.mark.parametrize("ewing", [1, 0])
+def test_ewing_fix_exact_on_linear_fields(ewing):
+    H = O.Hier(2, [LEV0, LEV1], (32, 32), periodic=False)
+    u = _fill_levels(H, lambda l, n: _lin(n, 1.0 / (1 << l), 0.0, 1.0, 0.5))
+    f = H.cells()
+    B = O.HierBlock(kind=0, D=None, D_const=-1.0, C=None, C_const=0.0, M=None, M_const=1.0,
+                    S=None, gamma=0.0, ewing=ewing)
+    r = H.cells()
+    O.lib().o_hier_comp_residual(H.h, C.byref(B), O.ptr(u), O.ptr(f), O.ptr(r), 1)
+    # fine cells next to the coarse-fine boundary, away from the domain boundary
+    res = H.get_level(r, 1, fill=0.0)
+    inner = np.abs(res[2:-2, 2:-2])
+    coarse = np.abs(H.get_level(r, 0, fill=0.0)[2:-2, 2:-2] * (H.uncovered(0)[2:-2, 2:-2]))
+    if ewing:
+        assert inner.max() < 1e-6 and coarse.max() < 1e-6, (inner.max(), coarse.max())
+    else:
+        assert inner.max() > 1e-2
+
+
+def _fac(H, B, f, cycles, ncomp=1):
+    FP = O.HierFacParams(max_cycles=cycles, pre=1, post=1, coarse_its=2, coarse_tol=-1.0)
+    u = H.cells(ncomp)
+    nrm, cyc = C.c_double(), C.c_int()
+    O.lib().o_hier_fac_solve(H.h, C.byref(B), C.byref(FP), O.ptr(f), O.ptr(u), ncomp, 1e-30,
+                             C.byref(nrm), C.byref(cyc))
+    return u, nrm.value
+
+
+def test_fac_converges_and_matches_fine_solution():
+    gamma = 1e-2
+    B = O.HierBlock(kind=0, D=None, D_const=-gamma * 2.0, C=None, C_const=1.0, M=None,
+                    M_const=1.0, S=None, gamma=0.0, ewing=1)
+    H = O.Hier(2, [LEV0, LEV1], (32, 32), periodic=True)
+    rng = np.random.default_rng(7)
+    f = _fill_levels(H, lambda l, n: rng.standard_normal((n, n)))
+    norms = [_fac(H, B, f, k)[1] for k in (1, 2, 3, 4)]
+    for a, b in zip(norms[:-1], norms[1:]):
+        assert b < 0.1 * a
+    # fine level over the whole domain: the fine single-level solution
+    lev1 = [((0, 0), (31, 63)), ((32, 0), (63, 63))]
+    H2 = O.Hier(2, [LEV0, lev1], (32, 32), periodic=True)
+    f1 = rng.standard_normal((64, 64))
+    f2 = H2.cells()
+    H2.set_level(f2, 1, f1)
+    u, nrm = _fac(H2, B, f2, 12)
+    assert nrm < 1e-11
+    pt = O.Patch(2, (64, 64, 1), g=1, dx=(0.5, 0.5, 1.0))
+    op = O.EllipticOp()
+    op.D, op.D_const, op.C, op.C_const, op.M, op.M_const = None, -gamma * 2.0, None, 1.0, None, 1.0
+    op.dx = (C.c_double * 3)(0.5, 0.5, 1.0)
+    rhs = pt.cells()
+    pt.interior(rhs)[...] = f1
+    us = pt.cells()
+    flux = np.zeros(pt.nf)
+    O.lib().o_fac_smooth(C.byref(pt.G), C.byref(op), O.ptr(rhs), O.ptr(us), 400, -1.0, O.ptr(flux))
+    ref = pt.interior(us).reshape(64, 64)
+    assert np.max(np.abs(H2.get_level(u, 1) - ref)) < 1e-9 * np.max(np.abs(ref))
