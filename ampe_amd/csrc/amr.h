@@ -1,0 +1,43 @@
+// Patch hierarchy interface (see amr.hip).
+#pragma once
+
+#include "kernels.h"
+
+namespace ampe {
+
+struct Hier;
+// the hierarchy of a C-ABI handle
+Hier* hier_of(ampe_hier* h);
+
+int hier_create(int ndim, int nlev, const int* npatch, const ampe_box* boxes, const int dom0[3],
+                const double dx0[3], int periodic, Hier** out);
+void hier_destroy(Hier* H);
+Level* hier_composite(Hier* H);
+int hier_weight_sum(Hier* H, double* sum);
+int hier_nlev(const Hier* H);
+// coarsenData of an integrator state (phase, composition: CONSERVATIVE_COARSEN;
+// orientation: QuatWeightedAverage) and its ghosts by LINEAR_REFINE
+int hier_sim_sync_fill(Hier* H, double* y, int nc, int ic, int iq, int qlen, hipStream_t s);
+int hier_ghost_fill(const Hier& H, double* f, int ncomp, int cf, hipStream_t s);
+int hier_coarsen(const Hier& H, double* f, int ncomp, int quatnorm, hipStream_t s);
+int hier_phase_rhs(const Hier& H, const ampe_model_params& p0, const double* phi,
+                   const double* q, const double* cl, const double* cs, double* rhs,
+                   hipStream_t s);
+int hier_quat_face_norm2(const Hier& H, const ampe_model_params& p0, const double* q,
+                         double* n2, hipStream_t s);
+int hier_quat_rhs(const Hier& H, const ampe_model_params& p0, const double* phi, const double* q,
+                  const double* n2, double* fc, double* rhs, hipStream_t s);
+int hier_conc_coeffs(const Hier& H, const ampe_model_params& p0, const double* phi,
+                     const double* cl, const double* cs, double* Dc, double* Dphi, hipStream_t s);
+int hier_conc_rhs(const Hier& H, const ampe_model_params& p0, const double* c, const double* phi,
+                  const double* Dc, const double* Dphi, double* rhs, hipStream_t s);
+int hier_fac_solve(Hier& H, const ampe_hier_block& B, const ampe_fac_params& P, const double* f,
+                   double* u, int ncomp, double tol, double* norm, int* cycles, hipStream_t s);
+// Per-level coefficient passes of the preconditioner setup: phase C and
+// mobility, orientation face coefficients and sqrt(M_q), each level with its
+// own spacing.
+int hier_precond_coeffs(const Hier& H, const ampe_model_params& p0, double gamma,
+                        const double* phi, const double* cl, const double* cs, const double* n2,
+                        double* Mphi, double* Cphi, double* fc, double* sqrtm, hipStream_t s);
+
+}  // namespace ampe

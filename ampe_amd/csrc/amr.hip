@@ -31,6 +31,7 @@
 #include <cstring>
 #include <vector>
 
+#include "amr.h"
 #include "fac.h"
 #include "kernels.h"
 
@@ -80,6 +81,12 @@ struct Hier {
    // coarse-fine faces of level l (l > 0) and corrections of level l (< nlev-1)
    DevVec<CfFace> cff[kMaxLev];
    DevVec<CfCorr> cfc[kMaxLev];
+   // the FAC composite residual replaces every coarse face on a fine patch
+   // boundary: fine faces of level l on fine-fine patch boundaries (saved
+   // after cff[l]) and the covered coarse cells of level l next to a fine
+   // patch boundary of level l+1
+   DevVec<CfFace> fff[kMaxLev];
+   DevVec<CfCorr> cfi[kMaxLev];
    // prolongation of level l (l > 0): fine interior cell, coarse parent
    DevVec<long long> pro_dst[kMaxLev], pro_src[kMaxLev];
    double* wt = nullptr;     // composite weights (interior cells), device
@@ -176,6 +183,8 @@ void hier_destroy(Hier* H)
       H->fcrs_dir[l].release();
       H->cff[l].release();
       H->cfc[l].release();
+      H->fff[l].release();
+      H->cfi[l].release();
       H->pro_dst[l].release();
       H->pro_src[l].release();
    }
@@ -445,14 +454,14 @@ int hier_create(int ndim, int nlev, const int* npatch, const ampe_box* boxes, co
    // each coarse face's 2^(ndim-1) fine faces are consecutive in the
    // coarsening order; then the coarse cells next to them
    for (int l = 1; l < nlev; l++) {
-      std::vector<CfFace> faces;
+      std::vector<CfFace> faces, ffaces;
       struct Key {
          long long cell;
          int d, side;
          long long cface, cm, c0;
          int first;
       };
-      std::vector<Key> keys;
+      std::vector<Key> keys, ikeys;
       for (int p = H->pbeg[l]; p < H->pbeg[l + 1]; p++) {
          const ampe_box& b = H->boxes[p];
          int n[3] = {box_n(b, 0, ndim), box_n(b, 1, ndim), box_n(b, 2, ndim)};
@@ -477,35 +486,11 @@ int hier_create(int ndim, int nlev, const int* npatch, const ampe_box* boxes, co
                      if (outside && !H->periodic) continue;
                      int gf[3] = {gout[0], gout[1], gout[2]};
                      canon(*H, l, gf);
-                     if (find_cell(*H, l, gf) >= 0) continue;  // same level, not coarse-fine
-                     // coarse cell outside and the covered coarse cell inside
-                     int co[3] = {fdiv2(gf[0]), fdiv2(gf[1]), ndim == 3 ? fdiv2(gf[2]) : 0};
-                     int gin[3] = {fdiv2(b.lo[0] + I0[0]), fdiv2(b.lo[1] + I0[1]),
-                                   ndim == 3 ? fdiv2(b.lo[2] + I0[2]) : 0};
-                     int cpatch = -1;
-                     long long ccell = find_cell(*H, l - 1, co, &cpatch);
-                     if (ccell < 0) return hier_fail(H, "a level is not nested in the coarser one");
-                     // the coarse face on ccell's side facing the fine patch
-                     const ampe_box& cb = H->boxes[cpatch];
-                     int cl[3] = {co[0] - cb.lo[0], co[1] - cb.lo[1], ndim == 3 ? co[2] - cb.lo[2] : 0};
-                     int cside = side ? 0 : 1;  // fine patch's hi side -> coarse cell's lo face
-                     int cfl[3] = {cl[0], cl[1], cl[2]};
-                     if (cside) cfl[d] += 1;
-                     long long cface = fidx(*H, cpatch, d, cfl[0], cfl[1], cfl[2]);
-                     // neighbour across the face inside the coarse patch's ghost box
-                     long long cbs = box_n(cb, 0, ndim) + 2;
-                     long long cst[3] = {1, cbs, cbs * (box_n(cb, 1, ndim) + 2)};
-                     long long cnb = cside ? ccell + cst[d] : ccell - cst[d];
-                     (void)gin;
-                     Key key;
-                     key.cell = ccell;
-                     key.d = d;
-                     key.side = cside;
-                     key.cface = cface;
-                     key.cm = cside ? ccell : cnb;
-                     key.c0 = cside ? cnb : ccell;
-                     key.first = (int)faces.size();
-                     keys.push_back(key);
+                     // a fine-fine patch boundary: its fine faces still
+                     // replace the covered coarse faces in the FAC residual
+                     bool fine_fine = find_cell(*H, l, gf) >= 0;
+                     std::vector<CfFace>& out = fine_fine ? ffaces : faces;
+                     int first = fine_fine ? -1 - (int)ffaces.size() : (int)faces.size();
                      // fine faces of this coarse face, coarsening order
                      for (int r2 = 0; r2 < (ndim == 3 ? 2 : 1); r2++)
                         for (int r1 = 0; r1 < 2; r1++) {
@@ -535,48 +520,108 @@ int hier_create(int ndim, int nlev, const int* npatch, const ampe_box* boxes, co
                            f.mirror = cidx(*H, p, Iq[0], Iq[1], Iq[2]);
                            f.d = d;
                            f.side = side;
-                           faces.push_back(f);
+                           out.push_back(f);
                         }
+                     // the covered coarse cell inside: its face on this side
+                     int gin[3] = {fdiv2(b.lo[0] + I0[0]), fdiv2(b.lo[1] + I0[1]),
+                                   ndim == 3 ? fdiv2(b.lo[2] + I0[2]) : 0};
+                     {
+                        int ip = -1;
+                        long long icell = find_cell(*H, l - 1, gin, &ip);
+                        if (icell < 0) return hier_fail(H, "a level is not nested in the coarser one");
+                        const ampe_box& ib = H->boxes[ip];
+                        int il[3] = {gin[0] - ib.lo[0], gin[1] - ib.lo[1],
+                                     ndim == 3 ? gin[2] - ib.lo[2] : 0};
+                        int ifl[3] = {il[0], il[1], il[2]};
+                        if (side) ifl[d] += 1;
+                        long long ibs = box_n(ib, 0, ndim) + 2;
+                        long long ist[3] = {1, ibs, ibs * (box_n(ib, 1, ndim) + 2)};
+                        long long inb = side ? icell + ist[d] : icell - ist[d];
+                        Key key;
+                        key.cell = icell;
+                        key.d = d;
+                        key.side = side;
+                        key.cface = fidx(*H, ip, d, ifl[0], ifl[1], ifl[2]);
+                        key.cm = side ? icell : inb;
+                        key.c0 = side ? inb : icell;
+                        key.first = first;
+                        ikeys.push_back(key);
+                     }
+                     if (fine_fine) continue;
+                     // the uncovered coarse cell outside
+                     int co[3] = {fdiv2(gf[0]), fdiv2(gf[1]), ndim == 3 ? fdiv2(gf[2]) : 0};
+                     int cpatch = -1;
+                     long long ccell = find_cell(*H, l - 1, co, &cpatch);
+                     if (ccell < 0) return hier_fail(H, "a level is not nested in the coarser one");
+                     // the coarse face on ccell's side facing the fine patch
+                     const ampe_box& cb = H->boxes[cpatch];
+                     int cl[3] = {co[0] - cb.lo[0], co[1] - cb.lo[1], ndim == 3 ? co[2] - cb.lo[2] : 0};
+                     int cside = side ? 0 : 1;  // fine patch's hi side -> coarse cell's lo face
+                     int cfl[3] = {cl[0], cl[1], cl[2]};
+                     if (cside) cfl[d] += 1;
+                     long long cface = fidx(*H, cpatch, d, cfl[0], cfl[1], cfl[2]);
+                     // neighbour across the face inside the coarse patch's ghost box
+                     long long cbs = box_n(cb, 0, ndim) + 2;
+                     long long cst[3] = {1, cbs, cbs * (box_n(cb, 1, ndim) + 2)};
+                     long long cnb = cside ? ccell + cst[d] : ccell - cst[d];
+                     Key key;
+                     key.cell = ccell;
+                     key.d = d;
+                     key.side = cside;
+                     key.cface = cface;
+                     key.cm = cside ? ccell : cnb;
+                     key.c0 = cside ? cnb : ccell;
+                     key.first = first;
+                     keys.push_back(key);
                   }
             }
       }
+      // fine-fine faces follow the coarse-fine ones in the saved fluxes
+      for (Key& k : ikeys)
+         if (k.first < 0) k.first = (int)faces.size() + (-1 - k.first);
       // corrections of level l-1: one record per coarse cell
-      std::sort(keys.begin(), keys.end(), [](const Key& a, const Key& b) {
-         return a.cell != b.cell ? a.cell < b.cell : (a.d != b.d ? a.d < b.d : a.side < b.side);
-      });
-      std::vector<CfCorr> corr;
-      for (size_t i = 0; i < keys.size();) {
-         CfCorr c;
-         std::memset(&c, 0, sizeof(c));
-         c.cell = keys[i].cell;
-         int cp = -1;
-         for (int q = H->pbeg[l - 1]; q < H->pbeg[l] && cp < 0; q++) {
-            const ampe_box& qb = H->boxes[q];
-            long long qn = (long long)(box_n(qb, 0, ndim) + 2) * (box_n(qb, 1, ndim) + 2) *
-                           (box_n(qb, 2, ndim) + (ndim == 3 ? 2 : 0));
-            if (c.cell >= H->coff[q] && c.cell < H->coff[q] + qn) cp = q;
+      auto records = [&](std::vector<Key>& ks, std::vector<CfCorr>& corr) -> int {
+         std::sort(ks.begin(), ks.end(), [](const Key& a, const Key& b) {
+            return a.cell != b.cell ? a.cell < b.cell
+                                    : (a.d != b.d ? a.d < b.d : a.side < b.side);
+         });
+         for (size_t i = 0; i < ks.size();) {
+            CfCorr c;
+            std::memset(&c, 0, sizeof(c));
+            c.cell = ks[i].cell;
+            int cp = -1;
+            for (int q = H->pbeg[l - 1]; q < H->pbeg[l] && cp < 0; q++) {
+               const ampe_box& qb = H->boxes[q];
+               long long qn = (long long)(box_n(qb, 0, ndim) + 2) * (box_n(qb, 1, ndim) + 2) *
+                              (box_n(qb, 2, ndim) + (ndim == 3 ? 2 : 0));
+               if (c.cell >= H->coff[q] && c.cell < H->coff[q] + qn) cp = q;
+            }
+            if (cp < 0) return hier_fail(H, "coarse cell outside its level");
+            const ampe_box& cb = H->boxes[cp];
+            c.sy = box_n(cb, 0, ndim) + 2;
+            c.sz = c.sy * (box_n(cb, 1, ndim) + 2);
+            c.nface = 0;
+            size_t j = i;
+            while (j < ks.size() && ks[j].cell == c.cell) {
+               if (c.nface >= 6) return hier_fail(H, "more than six coarse-fine faces on a cell");
+               int f = c.nface++;
+               c.face[f] = ks[j].cface;
+               c.cm[f] = ks[j].cm;
+               c.c0[f] = ks[j].c0;
+               c.d[f] = ks[j].d;
+               c.side[f] = ks[j].side;
+               c.fine[f] = ks[j].first;
+               j++;
+            }
+            corr.push_back(c);
+            i = j;
          }
-         if (cp < 0) return hier_fail(H, "coarse cell outside its level");
-         const ampe_box& cb = H->boxes[cp];
-         c.sy = box_n(cb, 0, ndim) + 2;
-         c.sz = c.sy * (box_n(cb, 1, ndim) + 2);
-         c.nface = 0;
-         size_t j = i;
-         while (j < keys.size() && keys[j].cell == c.cell) {
-            if (c.nface >= 6) return hier_fail(H, "more than six coarse-fine faces on a cell");
-            int f = c.nface++;
-            c.face[f] = keys[j].cface;
-            c.cm[f] = keys[j].cm;
-            c.c0[f] = keys[j].c0;
-            c.d[f] = keys[j].d;
-            c.side[f] = keys[j].side;
-            c.fine[f] = keys[j].first;
-            j++;
-         }
-         corr.push_back(c);
-         i = j;
-      }
-      if (H->cff[l].upload(faces) || H->cfc[l - 1].upload(corr))
+         return 0;
+      };
+      std::vector<CfCorr> corr, icorr;
+      if (records(keys, corr) || records(ikeys, icorr)) return -1;
+      if (H->cff[l].upload(faces) || H->cfc[l - 1].upload(corr) || H->fff[l].upload(ffaces) ||
+          H->cfi[l - 1].upload(icorr))
          return hier_fail(H, "upload failed");
    }
 
@@ -607,6 +652,7 @@ int hier_create(int ndim, int nlev, const int* npatch, const ampe_box* boxes, co
               hipSuccess)
          return hier_fail(H, "upload failed");
    }
+   H->comp->wt = H->wt;
    *out = H;
    return 0;
 }
@@ -1016,7 +1062,7 @@ static int fac_grow(Hier& H, int ncomp)
    (void)hipFree(H.fgsave);
    H.fe = H.fr = H.ftmp = H.foflux = H.fgsave = nullptr;
    long long nface = 0;
-   for (int l = 0; l < H.nlev; l++) nface = std::max(nface, H.cff[l].n);
+   for (int l = 0; l < H.nlev; l++) nface = std::max(nface, H.cff[l].n + H.fff[l].n);
    AMPE_HIP_CHECK(hipMalloc((void**)&H.fe, sizeof(double) * need));
    AMPE_HIP_CHECK(hipMalloc((void**)&H.fr, sizeof(double) * need));
    AMPE_HIP_CHECK(hipMalloc((void**)&H.ftmp, sizeof(double) * need));
@@ -1148,19 +1194,26 @@ static int fac_comp_residual(const FacCtx& X, int l, double* res, double* u, con
          if (launch_residual<double>(L, op, H.dx[l], fm, um, rm, nullptr, X.s)) return -1;
       }
       double di[3] = {1.0 / H.dx[l][0], 1.0 / H.dx[l][1], 1.0 / H.dx[l][2]};
-      if (l + 1 < H.nlev && H.cfc[l].n > 0) {
+      // every coarse face on a fine patch boundary takes the coarsened saved
+      // fine fluxes: the uncovered cells next to coarse-fine faces, then the
+      // covered cells inside the fine patches
+      for (int t = 0; t < 2 && l + 1 < H.nlev; t++) {
+         const DevVec<CfCorr>& tab = t == 0 ? H.cfc[l] : H.cfi[l];
+         if (tab.n == 0) continue;
          double af, ac;
          corr_areas(H, l, af, ac);
-         hipLaunchKernelGGL(k_fac_corr, grid1(H.cfc[l].n), dim3(256), 0, X.s, rm, um,
+         hipLaunchKernelGGL(k_fac_corr, grid1(tab.n), dim3(256), 0, X.s, rm, um,
                             B.kind == 1 ? B.S : nullptr, B.D, B.D_const, B.M, B.M_const, B.gamma,
-                            B.kind, H.cfc[l].p, H.cfc[l].n, oflux_of(X, l + 1, m),
-                            H.ndim == 3 ? 4 : 2, di[0], di[1], di[2], af, ac);
+                            B.kind, tab.p, tab.n, oflux_of(X, l + 1, m), H.ndim == 3 ? 4 : 2,
+                            di[0], di[1], di[2], af, ac);
          AMPE_HIP_CHECK(hipGetLastError());
       }
-      if (l > 0 && H.cff[l].n > 0) {
-         hipLaunchKernelGGL(k_oflux_save, grid1(H.cff[l].n), dim3(256), 0, X.s, um,
-                            B.kind == 1 ? B.S : nullptr, B.D, B.D_const, H.cff[l].p, H.cff[l].n,
-                            di[0], di[1], di[2], oflux_of(X, l, m));
+      for (int t = 0; t < 2 && l > 0; t++) {
+         const DevVec<CfFace>& tab = t == 0 ? H.cff[l] : H.fff[l];
+         if (tab.n == 0) continue;
+         hipLaunchKernelGGL(k_oflux_save, grid1(tab.n), dim3(256), 0, X.s, um,
+                            B.kind == 1 ? B.S : nullptr, B.D, B.D_const, tab.p, tab.n, di[0],
+                            di[1], di[2], oflux_of(X, l, m) + (t == 0 ? 0 : H.cff[l].n));
          AMPE_HIP_CHECK(hipGetLastError());
       }
    }
@@ -1269,7 +1322,7 @@ int hier_fac_solve(Hier& H, const ampe_hier_block& B, const ampe_fac_params& P, 
    }
    if (fac_grow(H, ncomp)) return -1;
    long long nface = 0;
-   for (int l = 0; l < H.nlev; l++) nface = std::max(nface, H.cff[l].n);
+   for (int l = 0; l < H.nlev; l++) nface = std::max(nface, H.cff[l].n + H.fff[l].n);
    FacCtx X{&H, &B, &P, ncomp, nface + 1, s};
    long long n = H.ncg * ncomp;
    if (fill_zero<double>(H.fr, n, s)) return -1;
@@ -1291,6 +1344,45 @@ int hier_fac_solve(Hier& H, const ampe_hier_block& B, const ampe_fac_params& P, 
    if (norm) *norm = nrm;
    if (cycles) *cycles = it;
    return nrm < tol ? 0 : 1;
+}
+
+Level* hier_composite(Hier* H) { return H->comp; }
+
+int hier_nlev(const Hier* H) { return H->nlev; }
+
+int hier_weight_sum(Hier* H, double* sum)
+{
+   std::vector<double> w((size_t)H->ncg);
+   AMPE_HIP_CHECK(hipMemcpy(w.data(), H->wt, sizeof(double) * w.size(), hipMemcpyDeviceToHost));
+   double s = 0.0;
+   for (double v : w) s += v;
+   *sum = s;
+   return 0;
+}
+
+int hier_sim_sync_fill(Hier* H, double* y, int nc, int ic, int iq, int qlen, hipStream_t s)
+{
+   if (hier_coarsen(*H, y, 1, 0, s)) return -1;
+   if (ic >= 0 && hier_coarsen(*H, y + (long long)ic * H->ncg, 1, 0, s)) return -1;
+   if (iq >= 0 && hier_coarsen(*H, y + (long long)iq * H->ncg, qlen, 1, s)) return -1;
+   return hier_ghost_fill(*H, y, nc, 2, s);
+}
+
+int hier_precond_coeffs(const Hier& H, const ampe_model_params& p0, double gamma,
+                        const double* phi, const double* cl, const double* cs, const double* n2,
+                        double* Mphi, double* Cphi, double* fc, double* sqrtm, hipStream_t s)
+{
+   bool kim = p0.phi_mobility_type == AMPE_PHI_MOB_KIM;
+   for (int l = 0; l < H.nlev; l++) {
+      ampe_model_params p = lev_params(H, p0, l);
+      const Level& L = *H.lev[l];
+      if (kim && launch_phase_mobility(L, p, cl, cs, Mphi, s)) return -1;
+      if (launch_phase_C(L, p, gamma, phi, kim ? Mphi : nullptr, Cphi, s)) return -1;
+      if (fc && launch_quat_face_coef(L, p, phi, n2, fc, s)) return -1;
+   }
+   // sqrt(M_q) of every ghosted entry (cell-wise, no spacing)
+   if (sqrtm && launch_sqrt_quat_mobility(*H.comp, p0, phi, sqrtm, s)) return -1;
+   return 0;
 }
 
 }  // namespace ampe
@@ -1315,6 +1407,10 @@ struct ampe_hier {
    } while (0)
 
 static hipStream_t hstream(ampe_stream_t s) { return reinterpret_cast<hipStream_t>(s); }
+
+namespace ampe {
+Hier* hier_of(ampe_hier* h) { return h ? h->H : nullptr; }
+}  // namespace ampe
 
 extern "C" {
 

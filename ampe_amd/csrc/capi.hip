@@ -440,6 +440,29 @@ int ampe_sim_create(const ampe_model_params* p, const ampe_solver_params* s,
    return 0;
 }
 
+int ampe_sim_create_hier(const ampe_model_params* p, const ampe_solver_params* s,
+                         ampe_hier* hier, ampe_sim** out)
+{
+   AMPE_REQUIRE(p && s && hier && out, "null argument");
+   ampe::Sim* S = nullptr;
+   if (ampe::sim_create_hier(*p, *s, ampe::hier_of(hier), &S)) return -1;
+   *out = new ampe_sim{S};
+   (*out)->view.L = S->L;
+   return 0;
+}
+
+int ampe_sim_set_state_composite(ampe_sim* sim, const double* y)
+{
+   AMPE_REQUIRE(sim && y, "null argument");
+   return ampe::sim_set_state_composite(sim->S, y);
+}
+
+int ampe_sim_get_state_composite(ampe_sim* sim, double* y)
+{
+   AMPE_REQUIRE(sim && y, "null argument");
+   return ampe::sim_get_state_composite(sim->S, y);
+}
+
 int ampe_sim_vector_size(const ampe_sim* sim, long long* n, int* ncomp)
 {
    AMPE_REQUIRE(sim, "null argument");

@@ -15,6 +15,7 @@ struct LevelArgs {
    long long ncg;
    long long sdo[3];  // direction-block offsets inside a side field
    int nz_tile;       // planes per tile (1, or KZ for column tiles)
+   const double* wt;  // cell weights of the reductions (null: 1)
 };
 
 
@@ -28,6 +29,7 @@ inline LevelArgs level_args(const Level& L, bool faces = false)
    a.gz = L.gz;
    a.ncg = L.ncg;
    for (int d = 0; d < 3; d++) a.sdo[d] = L.side_dir_off[d];
+   a.wt = L.wt;
    return a;
 }
 
@@ -129,6 +131,12 @@ __device__ __forceinline__ long long face_idx(const LevelArgs& a, const PatchDev
 {
    int ex = pd.nx + (d == 0), ey = pd.ny + (d == 1);
    return a.sdo[d] + pd.side_off[d] + i + (long long)ex * (j + (long long)ey * k);
+}
+
+// Weight of cell c0 in a reduction (1 without weights).
+__device__ __forceinline__ double red_w(const LevelArgs& a, long long c0)
+{
+   return a.wt ? a.wt[c0] : 1.0;
 }
 
 // Compact interior index (host I/O layout).

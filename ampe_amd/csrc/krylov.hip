@@ -30,6 +30,7 @@ __global__ void __launch_bounds__(256) k_gmres_prep(LevelArgs a, int nc, double 
    CellPos c;
    double acc = 0.0;
    if (cell_pos(a, 0, c)) {
+      const double w = red_w(a, c.c0);
       for (int m = 0; m < nc; m++) {
          long long i = m * a.ncg + c.c0;
          double v = V[i] * hinv;
@@ -37,7 +38,7 @@ __global__ void __launch_bounds__(256) k_gmres_prep(LevelArgs a, int nc, double 
          double x = v / ewt[i];
          xk[i] = x;
          double t = x * ewt[i];
-         acc += t * t;
+         acc += (t * t) * w;
       }
    }
    tile_store_sum(acc, partial);
@@ -102,12 +103,13 @@ __global__ void __launch_bounds__(256) k_scale_dot(LevelArgs a, int nc,
    CellPos c;
    double acc = 0.0;
    if (cell_pos(a, 0, c)) {
+      const double w = red_w(a, c.c0);
       for (int m = 0; m < nc; m++) {
          long long i = m * a.ncg + c.c0;
          double v = ewt[i] * work[i];
          out[i] = v;
          double o = (other == out) ? v : other[i];
-         acc += o * v;
+         acc += (o * v) * w;
       }
    }
    tile_store_sum(acc, partial);
@@ -122,11 +124,12 @@ __global__ void __launch_bounds__(256) k_mgs(LevelArgs a, int nc, double* __rest
    CellPos c;
    double acc = 0.0;
    if (cell_pos(a, 0, c)) {
+      const double wc = red_w(a, c.c0);
       for (int m = 0; m < nc; m++) {
          long long i = m * a.ncg + c.c0;
          double v = w[i] + (-h) * Vi[i];
          w[i] = v;
-         acc += Vnext ? Vnext[i] * v : v * v;
+         acc += (Vnext ? Vnext[i] * v : v * v) * wc;
       }
    }
    tile_store_sum(acc, partial);
@@ -172,10 +175,11 @@ __global__ void __launch_bounds__(256) k_newton_update(LevelArgs a, int nc,
    CellPos c;
    double acc = 0.0;
    if (cell_pos(a, 0, c)) {
+      const double w = red_w(a, c.c0);
       for (int m = 0; m < nc; m++) {
          long long i = m * a.ncg + c.c0;
          double t = del[i] * ewt[i];
-         acc += t * t;
+         acc += (t * t) * w;
          y[i] = y[i] + del[i];
       }
    }
@@ -211,6 +215,7 @@ __global__ void __launch_bounds__(256) k_cgs_dots(LevelArgs a, int nc, int ntile
    for (int k = 0; k < kMaxCgs; k++) acc[k] = 0.0;
    const int n = vl.n;
    if (cell_pos(a, 0, c)) {
+      const double w = red_w(a, c.c0);
       for (int m = 0; m < nc; m++) {
          long long i = m * a.ncg + c.c0;
          double v;
@@ -228,10 +233,10 @@ __global__ void __launch_bounds__(256) k_cgs_dots(LevelArgs a, int nc, int ntile
          }
 #pragma unroll
          for (int k = 0; k < kMaxCgs - 1; k++)
-            if (k < n) acc[k] += v * vl.v[k][i];
+            if (k < n) acc[k] += (v * vl.v[k][i]) * w;
 #pragma unroll
          for (int k = 0; k < kMaxCgs; k++)
-            if (k == n) acc[k] += v * v;
+            if (k == n) acc[k] += (v * v) * w;
       }
    }
    tile_store_sums(acc, n + 1, ntiles, partial);
@@ -244,6 +249,7 @@ __global__ void __launch_bounds__(256) k_cgs_update(LevelArgs a, int nc, double*
    double acc = 0.0;
    const int n = vl.n;
    if (cell_pos(a, 0, c)) {
+      const double wc = red_w(a, c.c0);
       for (int m = 0; m < nc; m++) {
          long long i = m * a.ncg + c.c0;
          double v = w[i];
@@ -251,7 +257,7 @@ __global__ void __launch_bounds__(256) k_cgs_update(LevelArgs a, int nc, double*
          for (int k = 0; k < kMaxCgs; k++)
             if (k < n) v = v + vl.y[k] * vl.v[k][i];
          w[i] = v;
-         acc += v * v;
+         acc += (v * v) * wc;
       }
    }
    tile_store_sum(acc, partial);

@@ -77,6 +77,9 @@ struct Level {
    double* hpinned = nullptr;  // pinned host scalars
    int* iscratch = nullptr;    // device integer flags
    double* kks_part = nullptr; // KKS per-workgroup partials (8 per 256 ghosted cells)
+   // control-volume weights of the cells in dot products and norms (a
+   // hierarchy's composite level: zero under a finer level); null: 1
+   const double* wt = nullptr;
 };
 
 int level_create(int ndim, const ampe_box* boxes, int npatch, int ghost,

@@ -4,6 +4,7 @@
 #include <type_traits>
 #include <vector>
 
+#include "amr.h"
 #include "fac.h"
 #include "kks.h"
 #include "krylov.h"
@@ -30,6 +31,8 @@ struct Sim {
    ampe_model_params p;
    ampe_solver_params s;
    Level* L = nullptr;
+   Hier* hier = nullptr;  // patch hierarchy (L is then its composite level)
+   double wsum = 0.0;  // summed control-volume weights of all components (hierarchy)
    int nc = 0, iq = -1, ic = -1;
    long long ng = 0, nv = 0, nf = 0;
    double t = 0, h = 0, hprev = 0, gamma_setup = 1, crate = 1;
@@ -78,7 +81,10 @@ struct Sim {
 
 int sim_create(const ampe_model_params& p, const ampe_solver_params& s, const ampe_box* boxes,
                int npatch, const ampe_box* periodic, Sim** out);
+int sim_create_hier(const ampe_model_params& p, const ampe_solver_params& s, Hier* H, Sim** out);
 void sim_destroy(Sim* S);
+int sim_set_state_composite(Sim* S, const double* y_host);
+int sim_get_state_composite(Sim* S, double* y_host);
 int sim_set_state(Sim* S, const double* phi, const double* c, const double* q);
 int sim_get_state(Sim* S, double* phi, double* c, double* q, double* cl, double* cs);
 int sim_advance(Sim* S, int nsteps, ampe_stats* stats);

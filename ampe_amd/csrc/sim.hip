@@ -48,8 +48,9 @@ static double first_step(const ampe_solver_params& s)
    return (s.dt_init > 0.0 && s.dt_init < s.dt) ? s.dt_init : s.dt;
 }
 
-int sim_create(const ampe_model_params& p, const ampe_solver_params& s, const ampe_box* boxes,
-               int npatch, const ampe_box* periodic, Sim** out)
+static int sim_finish(Sim* S, Sim** out);
+
+static int sim_check(const ampe_model_params& p, const ampe_solver_params& s)
 {
    if (s.adaptive && (s.max_order < 1 || s.max_order > 2)) {
       set_error("sim_create: the adaptive BDF needs max_order 1 or 2");
@@ -80,6 +81,13 @@ int sim_create(const ampe_model_params& p, const ampe_solver_params& s, const am
       set_error("sim_create: unknown EBS stencil");
       return -1;
    }
+   return 0;
+}
+
+int sim_create(const ampe_model_params& p, const ampe_solver_params& s, const ampe_box* boxes,
+               int npatch, const ampe_box* periodic, Sim** out)
+{
+   if (sim_check(p, s)) return -1;
    Sim* S = new Sim();
    S->p = p;
    S->s = s;
@@ -93,6 +101,15 @@ int sim_create(const ampe_model_params& p, const ampe_solver_params& s, const am
       delete S;
       return -1;
    }
+   return sim_finish(S, out);
+}
+
+// Vectors, work arrays, the preconditioner's hierarchy and the stream of a
+// sim whose level S->L is set.
+static int sim_finish(Sim* S, Sim** out)
+{
+   const ampe_model_params& p = S->p;
+   const ampe_solver_params& s = S->s;
    const Level& L = *S->L;
    S->nc = 1 + (p.with_conc ? 1 : 0) + (p.evolve_quat ? p.qlen : 0);
    S->ic = p.with_conc ? 1 : -1;
@@ -154,6 +171,40 @@ int sim_create(const ampe_model_params& p, const ampe_solver_params& s, const am
    return 0;
 }
 
+int sim_create_hier(const ampe_model_params& p, const ampe_solver_params& s, Hier* H, Sim** out)
+{
+   if (sim_check(p, s)) return -1;
+   if (!H) {
+      set_error("sim_create_hier: null hierarchy");
+      return -1;
+   }
+   if (s.precond != AMPE_PRECOND_REDBLACK || s.precision != AMPE_FP64 || s.graph_psolve) {
+      set_error("sim_create_hier: the hierarchy runs the FP64 FAC preconditioner over levels "
+                "(precond redblack, precision fp64, no graph capture)");
+      return -1;
+   }
+   if (p.with_conc && p.conc_rhs_form != AMPE_CONC_RHS_KKS) {
+      set_error("sim_create_hier: the hierarchy supports the KKS composition form");
+      return -1;
+   }
+   if (s.kks_guess == AMPE_GUESS_SURROGATE) {
+      set_error("sim_create_hier: the previous-step KKS guess only");
+      return -1;
+   }
+   Sim* S = new Sim();
+   S->p = p;
+   S->s = s;
+   S->hier = H;
+   S->L = hier_composite(H);
+   double sw = 0.0;
+   if (hier_weight_sum(H, &sw)) {
+      delete S;
+      return -1;
+   }
+   S->wsum = sw * (double)(1 + (p.with_conc ? 1 : 0) + (p.evolve_quat ? p.qlen : 0));
+   return sim_finish(S, out);
+}
+
 void sim_destroy(Sim* S)
 {
    if (!S) return;
@@ -176,7 +227,7 @@ void sim_destroy(Sim* S)
    if (S->stream) (void)hipStreamDestroy(S->stream);
    if (S->ev0) (void)hipEventDestroy(S->ev0);
    if (S->ev1) (void)hipEventDestroy(S->ev1);
-   level_destroy(S->L);
+   if (!S->hier) level_destroy(S->L);
    delete S;
 }
 
@@ -297,6 +348,64 @@ int sim_set_state(Sim* S, const double* phi, const double* c, const double* q)
    return 0;
 }
 
+int sim_set_state_composite(Sim* S, const double* y_host)
+{
+   if (!S->hier) {
+      set_error("set_state_composite: not a hierarchy sim");
+      return -1;
+   }
+   hipStream_t st = S->stream;
+   AMPE_HIP_CHECK(hipMemcpy(S->y, y_host, sizeof(double) * S->nv, hipMemcpyHostToDevice));
+   int ql = S->iq >= 0 ? S->p.qlen : 0;
+   if (hier_sim_sync_fill(S->hier, S->y, S->nc, S->ic, S->iq, ql, st)) return -1;
+   std::memset(&S->st, 0, sizeof(S->st));
+   if (S->ic >= 0) {
+      const double* cc = S->y + (long long)S->ic * S->ng;
+      AMPE_HIP_CHECK(hipMemcpyAsync(S->clr, cc, sizeof(double) * S->ng,
+                                    hipMemcpyDeviceToDevice, st));
+      AMPE_HIP_CHECK(hipMemcpyAsync(S->csr, cc, sizeof(double) * S->ng,
+                                    hipMemcpyDeviceToDevice, st));
+      AMPE_HIP_CHECK(hipMemsetAsync(S->kks_acc, 0, sizeof(long long) * kKksAcc, st));
+      if (kks(S, S->y)) return -1;
+      if (kks_collect(S)) return -1;
+      std::memset(&S->st, 0, sizeof(S->st));
+      AMPE_HIP_CHECK(hipMemsetAsync(S->kks_acc, 0, sizeof(long long) * kKksAcc, st));
+   }
+   S->t = 0.0;
+   S->nst = 0;
+   S->nstlp = 0;
+   S->have_prev = false;
+   S->h = first_step(S->s);
+   S->bdf = BdfCtl{};
+   if (vec_ewt(S->nv, S->y, S->s.rtol, S->s.atol, S->ewt, st)) return -1;
+   AMPE_HIP_CHECK(hipStreamSynchronize(st));
+   return 0;
+}
+
+// The state with every finer level coarsened onto the cells it covers (the
+// output QuatIntegrator coarsens after each step, source/QuatIntegrator.cc:
+// 2190-2196); the integrator's own state is left as it is.
+int sim_get_state_composite(Sim* S, double* y_host)
+{
+   if (!S->hier) {
+      set_error("get_state_composite: not a hierarchy sim");
+      return -1;
+   }
+   hipStream_t st = S->stream;
+   AMPE_HIP_CHECK(hipMemcpyAsync(S->work, S->y, sizeof(double) * S->nv, hipMemcpyDeviceToDevice,
+                                 st));
+   if (hier_coarsen(*S->hier, S->work, 1, 0, st)) return -1;
+   if (S->ic >= 0 && hier_coarsen(*S->hier, S->work + (long long)S->ic * S->ng, 1, 0, st))
+      return -1;
+   if (S->iq >= 0 &&
+       hier_coarsen(*S->hier, S->work + (long long)S->iq * S->ng, S->p.qlen, 1, st))
+      return -1;
+   AMPE_HIP_CHECK(hipMemcpyAsync(y_host, S->work, sizeof(double) * S->nv, hipMemcpyDeviceToHost,
+                                 st));
+   AMPE_HIP_CHECK(hipStreamSynchronize(st));
+   return 0;
+}
+
 int sim_get_state(Sim* S, double* phi, double* c, double* q, double* cl, double* cs)
 {
    const Level& L = *S->L;
@@ -315,8 +424,101 @@ int sim_get_state(Sim* S, double* phi, double* c, double* q, double* cl, double*
 
 // ------------------------------------------------------------ callbacks
 // evaluateRHSFunction (source/QuatIntegrator.cc:3134-3298)
+// ------------------------------------------------------------ hierarchy
+// The callbacks over all levels of a hierarchy: setCoefficients coarsens the
+// iterate and fillScratch fills every level's ghosts (LINEAR_REFINE on the
+// coarse-fine boundary), then the composite right-hand sides
+// (source/QuatIntegrator.cc:2994-3083, 2873-2957, 3134-3298); the
+// preconditioner blocks are FAC cycles over the levels.
+static int hier_rhs_eval(Sim* S, double* y, double* ydot, bool fd)
+{
+   const ampe_model_params& p = S->p;
+   hipStream_t st = S->stream;
+   bool recompute = !fd || !S->s.lag_quat_sidegrad;
+   S->st.nfe++;
+   int ql = S->iq >= 0 ? p.qlen : 0;
+   if (hier_sim_sync_fill(S->hier, y, S->nc, S->ic, S->iq, ql, st)) return -1;
+   const double* phi = y;
+   const double* q = S->iq >= 0 ? y + (long long)S->iq * S->ng : nullptr;
+   const double* c = S->ic >= 0 ? y + (long long)S->ic * S->ng : nullptr;
+   if (q && recompute && hier_quat_face_norm2(*S->hier, p, q, S->n2, st)) return -1;
+   if (c && kks(S, y)) return -1;
+   if (hier_phase_rhs(*S->hier, p, phi, q, S->cl, S->cs, ydot, st)) return -1;
+   if (q && hier_quat_rhs(*S->hier, p, phi, q, S->n2, S->fcq, ydot + (long long)S->iq * S->ng,
+                          st))
+      return -1;
+   if (c) {
+      if (recompute && hier_conc_coeffs(*S->hier, p, phi, S->cl, S->cs, S->Dc, S->Dphi, st))
+         return -1;
+      if (hier_conc_rhs(*S->hier, p, c, phi, S->Dc, S->Dphi, ydot + (long long)S->ic * S->ng,
+                        st))
+         return -1;
+   }
+   return 0;
+}
+
+static int hier_psetup(Sim* S, double* y, double gamma)
+{
+   const ampe_model_params& p = S->p;
+   hipStream_t st = S->stream;
+   S->st.npe++;
+   int ql = S->iq >= 0 ? p.qlen : 0;
+   if (hier_sim_sync_fill(S->hier, y, S->nc, S->ic, S->iq, ql, st)) return -1;
+   const double* phi = y;
+   const double* q = S->iq >= 0 ? y + (long long)S->iq * S->ng : nullptr;
+   const double* c = S->ic >= 0 ? y + (long long)S->ic * S->ng : nullptr;
+   if (q && hier_quat_face_norm2(*S->hier, p, q, S->n2, st)) return -1;
+   if (c && kks(S, y)) return -1;
+   if (hier_precond_coeffs(*S->hier, p, gamma, phi, S->cl, S->cs, S->n2, S->Mphi, S->Cphi,
+                           q ? S->fc : nullptr, q ? S->sqrtm : nullptr, st))
+      return -1;
+   if (c && vec_axpby(S->nf, -gamma, S->Dc, 0.0, S->Dc, S->Dconc, st)) return -1;
+   S->gamma_setup = gamma;
+   return 0;
+}
+
+static int hier_psolve(Sim* S, const double* r, double* z, double delta)
+{
+   const ampe_model_params& p = S->p;
+   hipStream_t st = S->stream;
+   long long ng = S->ng;
+   S->st.nps++;
+   int ql = S->iq >= 0 ? p.qlen : 0;
+   int iq = S->iq >= 0 ? S->iq : S->nc;
+   // z = 0, fac_u = orientation blocks of r / sqrt(M)
+   if (psolve_in<double>(ng, S->nc, iq, ql, r, S->sqrtm, nullptr, S->fac_u, z, st)) return -1;
+   int ret = 0;
+   ampe_fac_params fp{S->s.fac_max_cycles, 1, 1, S->s.fac_sweeps, S->s.fac_tol};
+   ampe_hier_block ph{0,       nullptr, -S->gamma_setup * p.epsilon_phi * p.epsilon_phi,
+                      S->Cphi, 1.0,
+                      p.phi_mobility_type == AMPE_PHI_MOB_KIM ? S->Mphi : nullptr,
+                      p.phi_mobility, nullptr, 0.0, 1};
+   int r0 = hier_fac_solve(*S->hier, ph, fp, r, z, 1, delta, nullptr, nullptr, st);
+   if (r0 < 0) return -1;
+   ret |= r0;
+   if (S->iq >= 0) {
+      ampe_fac_params qp{S->s.quat_max_cycles, 1, 1, S->s.quat_sweeps, -1.0};
+      ampe_hier_block qb{1, S->fc, 0.0, nullptr, 1.0, nullptr, 1.0, S->sqrtm, S->gamma_setup, 0};
+      int r1 = hier_fac_solve(*S->hier, qb, qp, S->fac_u, z + (long long)S->iq * ng, ql, delta,
+                              nullptr, nullptr, st);
+      if (r1 < 0) return -1;
+      ret |= r1;
+      if (vec_mul_comp(S, z + (long long)S->iq * ng, S->sqrtm, ql)) return -1;
+   }
+   if (S->ic >= 0) {
+      ampe_hier_block co{0, S->Dconc, 0.0, nullptr, 1.0, nullptr, p.conc_mobility, nullptr, 0.0,
+                         1};
+      int r2 = hier_fac_solve(*S->hier, co, fp, r + (long long)S->ic * ng,
+                              z + (long long)S->ic * ng, 1, delta, nullptr, nullptr, st);
+      if (r2 < 0) return -1;
+      ret |= r2;
+   }
+   return ret;
+}
+
 static int rhs_eval(Sim* S, double* y, double* ydot, bool fd)
 {
+   if (S->hier) return hier_rhs_eval(S, y, ydot, fd);
    const Level& L = *S->L;
    const ampe_model_params& p = S->p;
    hipStream_t st = S->stream;
@@ -365,6 +567,7 @@ static int rhs_eval(Sim* S, double* y, double* ydot, bool fd)
 // CVSpgmrPrecondSet (source/QuatIntegrator.cc:3300-3376)
 static int psetup(Sim* S, double* y, double gamma)
 {
+   if (S->hier) return hier_psetup(S, y, gamma);
    const Level& L = *S->L;
    const ampe_model_params& p = S->p;
    hipStream_t st = S->stream;
@@ -677,8 +880,10 @@ struct PsolveOpts {
    bool f_ready = false;
    bool keep = false;
 };
+static int hier_psolve(Sim* S, const double* r, double* z, double delta);
 static int psolve(Sim* S, const double* r, double* z, double delta, PsolveOpts o = {})
 {
+   if (S->hier) return hier_psolve(S, r, z, delta);
    const ampe_model_params& p = S->p;
    hipStream_t st = S->stream;
    long long ng = S->ng;
@@ -727,6 +932,14 @@ int vec_mul_comp(Sim* S, double* x, const double* sm, int ncomp)
 }
 
 // ------------------------------------------------------------ Krylov
+// Summed control-volume weights of all components in level-0 cells: the
+// cell count times the components on one level; on a hierarchy the weights
+// (zero under a finer level) summed.
+static double wcount(const Sim* S)
+{
+   return S->hier ? S->wsum : (double)S->nc * S->L->ncell;
+}
+
 static double cellvol(const Sim* S)
 {
    const double* dx = S->p.dx;
@@ -743,7 +956,7 @@ static int spgmr(Sim* S, double gamma, const double* bvec, double* x, double del
    hipStream_t st = S->stream;
    int maxl = S->s.maxl, nc = S->nc;
    double cv = cellvol(S);
-   double wfac = 1.0 / ((double)nc * L.ncell);
+   double wfac = 1.0 / wcount(S);
    std::vector<double>& Hm = S->H;
    Hm.assign((size_t)(maxl + 1) * (maxl + 1), 0.0);
    int ld = maxl + 1;
@@ -924,7 +1137,7 @@ static int newton(Sim* S, double gamma, double delta, double tq4, bool force_set
    const Level& L = *S->L;
    hipStream_t st = S->stream;
    long long nv = S->nv;
-   double wfac = 1.0 / ((double)S->nc * L.ncell);
+   double wfac = 1.0 / wcount(S);
    bool call_setup = force_setup || S->nst == 0 ||
                      std::fabs(gamma / S->gamma_setup - 1.0) > 0.3 ||
                      S->nst >= S->nstlp + S->s.msbp;
@@ -996,7 +1209,7 @@ static int bdf_step(Sim* S)
    const Level& L = *S->L;
    hipStream_t st = S->stream;
    long long nv = S->nv, ng = S->ng;
-   double wfac = 1.0 / ((double)S->nc * L.ncell);
+   double wfac = 1.0 / wcount(S);
    if (S->ic >= 0) {
       // resetRefPhaseConcentrations (source/QuatModel.cc:5218-5235)
       AMPE_HIP_CHECK(hipMemcpyAsync(S->clr, S->cl, sizeof(double) * ng,
@@ -1004,7 +1217,7 @@ static int bdf_step(Sim* S)
       AMPE_HIP_CHECK(hipMemcpyAsync(S->csr, S->cs, sizeof(double) * ng,
                                     hipMemcpyDeviceToDevice, st));
    }
-   double nrmfac = std::sqrt((double)S->nc * L.ncell * cellvol(S));
+   double nrmfac = std::sqrt(wcount(S) * cellvol(S));
    double delta = S->s.eplifac * S->s.nlscoef * nrmfac;
    bool force_setup = false;
    for (;;) {
@@ -1170,7 +1383,7 @@ static int bdf_wrms(Sim* S, const double* x, double* out)
 {
    double ss;
    if (vec_sumsq_w(*S->L, S->nc, x, S->ewt, &ss, S->stream)) return -1;
-   *out = std::sqrt(ss / ((double)S->nc * S->L->ncell));
+   *out = std::sqrt(ss / wcount(S));
    return 0;
 }
 
@@ -1291,7 +1504,7 @@ static int bdf_adaptive_step(Sim* S)
       if (B.qprime != B.q && bdf_adjust_order(S, B.qprime)) return -1;
       if (bdf_rescale(S, B.eta)) return -1;
    }
-   double nrmfac = std::sqrt((double)S->nc * L.ncell * cellvol(S));
+   double nrmfac = std::sqrt(wcount(S) * cellvol(S));
    int nef = 0, ncf = 0;
    bool force_setup = false, predicted = false;
    double gamma = 0.0, delta = 0.0;

@@ -123,10 +123,11 @@ __global__ void __launch_bounds__(256) k_dot_partial(LevelArgs a, int ncomp, int
    CellPos c;
    double acc = 0.0;
    if (cell_pos(a, 0, c)) {
+      const double w = red_w(a, c.c0);
       for (int m = 0; m < ncomp; m++) {
          long long i = m * a.ncg + c.c0;
          double v = (mode == 0) ? x[i] * y[i] : (x[i] * y[i]) * (x[i] * y[i]);
-         acc += v;
+         acc += v * w;
       }
    }
    // wave reduction then across the 4 waves of the tile

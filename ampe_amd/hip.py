@@ -110,6 +110,10 @@ _SIGS = {
     "ampe_kks_batch_noalloc": (_i, [C.POINTER(ModelParams), C.POINTER(CalphadTdep), _ll, _vp,
                                     _vp, _vp, _vp, _i, _d, _vp, _vp, _vp, _vp, _vp]),
     "ampe_sim_vector_size": (_i, [_vp, C.POINTER(_ll), C.POINTER(_i)]),
+    "ampe_sim_create_hier": (_i, [C.POINTER(ModelParams), C.POINTER(SolverParams), _vp,
+                                  C.POINTER(_vp)]),
+    "ampe_sim_set_state_composite": (_i, [_vp, _vp]),
+    "ampe_sim_get_state_composite": (_i, [_vp, _vp]),
     "ampe_hier_create": (_i, [_i, _i, _vp, _vp, _vp, _vp, _i, C.POINTER(_vp)]),
     "ampe_hier_destroy": (_i, [_vp]),
     "ampe_hier_sizes": (_i, [_vp, C.POINTER(_ll), C.POINTER(_ll), C.POINTER(_ll)]),
@@ -488,4 +492,40 @@ class Hier:
     def __del__(self):
         if getattr(self, "h", None) is not None and _lib is not None:
             _lib.ampe_hier_destroy(self.h)
+            self.h = None
+
+
+class HierSim:
+    """Device integrator on a hierarchy (ampe_sim_create_hier); states are
+    host composite arrays (nc x the hierarchy's ghosted cells)."""
+
+    def __init__(self, p, s, hier):
+        h = _vp()
+        check(lib().ampe_sim_create_hier(C.byref(p), C.byref(s), hier.h, C.byref(h)),
+              "ampe_sim_create_hier")
+        self.h = h
+        self.hier = hier  # outlives the sim
+        self.p, self.s = p, s
+        self.nc = 1 + (1 if p.with_conc else 0) + (p.qlen if p.evolve_quat else 0)
+
+    def set_state(self, y):
+        y = np.ascontiguousarray(y, dtype=np.float64)
+        assert y.size == self.nc * self.hier.ncg
+        check(lib().ampe_sim_set_state_composite(self.h, y.ctypes.data_as(_vp)),
+              "ampe_sim_set_state_composite")
+
+    def get_state(self):
+        y = np.zeros(self.nc * self.hier.ncg)
+        check(lib().ampe_sim_get_state_composite(self.h, y.ctypes.data_as(_vp)),
+              "ampe_sim_get_state_composite")
+        return y
+
+    def advance(self, nsteps):
+        st = Stats()
+        rc = check(lib().ampe_sim_advance(self.h, nsteps, C.byref(st)), "ampe_sim_advance")
+        return rc, st.as_dict()
+
+    def __del__(self):
+        if getattr(self, "h", None) is not None and _lib is not None:
+            _lib.ampe_sim_destroy(self.h)
             self.h = None

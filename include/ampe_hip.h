@@ -836,6 +836,26 @@ int ampe_hier_fac_solve(ampe_hier* h, const ampe_hier_block* b, const ampe_fac_p
                         const double* f, double* u, int ncomp, double tol, double* norm,
                         int* cycles, ampe_stream_t stream);
 
+/* The integrator on a hierarchy (QuatIntegrator over all levels of the
+ * PatchHierarchy): state vectors are composite fields of nc components
+ * (phase, composition, orientation) whose control volumes are the
+ * hierarchy's weights; every right-hand-side evaluation coarsens the iterate
+ * and fills the ghosts by linear refinement (setCoefficients / fillScratch,
+ * source/QuatIntegrator.cc:2873-3083); the preconditioner blocks are FAC
+ * cycles over the levels (ampe_hier_fac_solve; fac_max_cycles, fac_sweeps as
+ * the coarse red-black iterations, fac_tol, one pre- and post-smoothing
+ * sweep; quat_max_cycles, quat_sweeps for the orientation block).  Requires
+ * precond redblack, precision fp64, the KKS composition form and the
+ * previous-step KKS guess.  The hierarchy must outlive the sim. */
+int ampe_sim_create_hier(const ampe_model_params* p, const ampe_solver_params* s,
+                         ampe_hier* hier, ampe_sim** out);
+/* host composite state (nc x the hierarchy's ghosted cells; ghost entries
+ * ignored): coarsened, ghosts filled, initial phase concentrations */
+int ampe_sim_set_state_composite(ampe_sim* sim, const double* y);
+/* host composite state with the finer levels coarsened onto the cells they
+ * cover (the output the reference coarsens after each step) */
+int ampe_sim_get_state_composite(ampe_sim* sim, double* y);
+
 #ifdef __cplusplus
 }
 #endif

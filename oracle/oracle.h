@@ -237,6 +237,10 @@ void o_hier_conc_rhs(const OHier* H, const ampe_model_params* p0, const double* 
                      const double* phi, const double* Dc, const double* Dphi, double* rhs);
 double o_hier_comp_residual(const OHier* H, const OHBlock* B, double* u, const double* f,
                             double* r, int ncomp);
+typedef struct OSim OSim;
+OSim* o_sim_create_hier(const ampe_model_params* p, const ampe_solver_params* s, OHier* H);
+void o_sim_set_state_composite(OSim* S, const double* y);
+void o_sim_get_state_composite(OSim* S, double* y);
 int o_hier_fac_solve(const OHier* H, const OHBlock* B, const OHFacParams* P, const double* f,
                      double* u, int ncomp, double tol, double* norm, int* cycles);
 

@@ -132,6 +132,9 @@ def _setup(L):
         "o_hier_quat_rhs": (None, [C.c_void_p, _P, _dp, _dp, _dp, _dp]),
         "o_hier_conc_coeffs": (None, [C.c_void_p, _P, _dp, _dp, _dp, _dp, _dp]),
         "o_hier_conc_rhs": (None, [C.c_void_p, _P, _dp, _dp, _dp, _dp, _dp]),
+        "o_sim_create_hier": (C.c_void_p, [_P, C.POINTER(SolverParams), C.c_void_p]),
+        "o_sim_set_state_composite": (None, [C.c_void_p, _dp]),
+        "o_sim_get_state_composite": (None, [C.c_void_p, _dp]),
         "o_hier_comp_residual": (C.c_double, [C.c_void_p, C.c_void_p, _dp, _dp, _dp, C.c_int]),
         "o_hier_fac_solve": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, _dp, _dp, C.c_int,
                                        C.c_double, _dp, _ip]),
@@ -288,6 +291,38 @@ def mlp_loss_grad(w, x, t, nin, hid, nout, emulate):
 
 
 # ---------------------------------------------------------------- hierarchy
+class HierSim:
+    """Oracle integrator on a hierarchy (composite state vectors)."""
+
+    def __init__(self, p, s, hier):
+        self.hier = hier
+        self.p, self.s = p, s
+        self.h = lib().o_sim_create_hier(C.byref(p), C.byref(s), hier.h)
+        if not self.h:
+            raise ValueError("o_sim_create_hier rejected the model")
+        self.nc = 1 + (1 if p.with_conc else 0) + (p.qlen if p.evolve_quat else 0)
+
+    def set_state(self, y):
+        y = np.ascontiguousarray(y, dtype=np.float64)
+        assert y.size == self.nc * self.hier.ncg
+        lib().o_sim_set_state_composite(self.h, ptr(y))
+
+    def get_state(self):
+        y = np.zeros(self.nc * self.hier.ncg)
+        lib().o_sim_get_state_composite(self.h, ptr(y))
+        return y
+
+    def advance(self, nsteps):
+        st = Stats()
+        ret = lib().o_sim_advance(self.h, nsteps, C.byref(st))
+        return ret, st.as_dict()
+
+    def __del__(self):
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.o_sim_destroy(self.h)
+            self.h = None
+
+
 class HierBlock(C.Structure):
     """One preconditioner block of the hierarchy FAC (OHBlock)."""
     _fields_ = [("kind", C.c_int), ("D", C.c_void_p), ("D_const", C.c_double),

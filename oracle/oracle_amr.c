@@ -447,8 +447,13 @@ static long fidx(const OHier* H, int p, int d, int i, int j, int k)
  * level-l faces they cover (every face of a fine patch's coarsened box):
  * sum of fine face * fine face area, divided by the coarse face area.  A
  * coarse face shared by two coarse patches is written in both. */
-void o_hier_coarsen_faces_to(const OHier* H, int l, const double* src, double* dst, int ncomp,
-                             long stride)
+/* coarse faces under the level-(l+1) patches take the area-weighted average
+ * of their fine faces; outer != 0 restricts this to the coarse faces on the
+ * fine patch boundaries (an OutersideData source: the FAC composite residual,
+ * EllipticFACOps.cc:1524-1528), outer == 0 takes every face under a fine patch
+ * (a SideData source: the right-hand sides' flux coarsening) */
+static void coarsen_faces(const OHier* H, int l, const double* src, double* dst, int ncomp,
+                          long stride, int outer)
 {
    if (l + 1 >= H->nlev) return;
    int nd = H->ndim;
@@ -473,6 +478,7 @@ void o_hier_coarsen_faces_to(const OHier* H, int l, const double* src, double* d
             for (int cj = 0; cj < e[1]; cj++)
                for (int ci = 0; ci < e[0]; ci++) {
                   int cf[3] = {ci, cj, ck};
+                  if (outer && cf[d] != 0 && cf[d] != cn[d]) continue;
                   int gcf[3] = {clo[0] + ci, clo[1] + cj, clo[2] + ck};
                   for (int m = 0; m < ncomp; m++) {
                      const double* a = src + (long)m * stride;
@@ -508,9 +514,15 @@ void o_hier_coarsen_faces_to(const OHier* H, int l, const double* src, double* d
    }
 }
 
+void o_hier_coarsen_faces_to(const OHier* H, int l, const double* src, double* dst, int ncomp,
+                             long stride)
+{
+   coarsen_faces(H, l, src, dst, ncomp, stride, 0);
+}
+
 void o_hier_coarsen_faces(const OHier* H, int l, double* s, int ncomp, long stride)
 {
-   o_hier_coarsen_faces_to(H, l, s, s, ncomp, stride);
+   coarsen_faces(H, l, s, s, ncomp, stride, 0);
 }
 
 /* per-level copy of the model parameters with the level's mesh spacing */
@@ -889,7 +901,7 @@ static void level_comp_residual(const OHier* H, const OHBlock* B, int l, int lmi
       double* om = W->oflux + (long)m * H->nf;
       o_hier_ghost_fill_level(H, l, um, 1, l > lmin ? 1 : 0);
       level_flux(H, B, l, lmin, um, W->flux);
-      if (l < lmax) o_hier_coarsen_faces_to(H, l, om, W->flux, 1, H->nf);
+      if (l < lmax) coarsen_faces(H, l, om, W->flux, 1, H->nf, 1);
       for (int p = H->pbeg[l]; p < H->pbeg[l + 1]; p++) {
          const OGeom* G = &H->G[p];
          for (int k = 0; k < G->nz; k++)

@@ -68,6 +68,11 @@ struct OSim {
    double *z1, *z2;
    int q, qprime, L, qwait, started;
    double tau[4], l[3], tq[6], hscale, hprime, eta, etamax, saved_tq5;
+   /* patch hierarchy (o_sim_create_hier): composite vectors, weights */
+   OHier* Hh;
+   double* wt;
+   double wsum;
+   unsigned char* interior;
    /* scalar temperature ramp (o_sim_set_temperature_ramp) */
    int tramp, has_tab;
    double T0, dTdt, Ttarget;
@@ -168,6 +173,8 @@ void o_sim_destroy(OSim* S)
    for (size_t i = 0; i < sizeof(fs) / sizeof(fs[0]); i++) free(fs[i]);
    for (int i = 0; i <= S->s.maxl; i++) free(S->V[i]);
    free(S->V);
+   free(S->wt);
+   free(S->interior);
    free(S);
 }
 
@@ -240,8 +247,14 @@ void o_sim_get_state(OSim* S, double* phi, double* c, double* q, double* cl,
 
 /* evaluateRHSFunction (source/QuatIntegrator.cc:3134-3298).  y is ghost
  * filled in place (the scratch copy of fillScratch is the vector itself). */
+static int rhs_eval_hier(OSim* S, double* y, double* ydot, int fd);
+static int psetup_hier(OSim* S, double* y, double gamma);
+static int psolve_hier(OSim* S, const double* r, double* z, double delta);
+static void project_q(OSim* S, const double* q, double* corr, double* err);
+
 static int rhs_eval(OSim* S, double* y, double* ydot, int fd)
 {
+   if (S->Hh) return rhs_eval_hier(S, y, ydot, fd);
    const OGeom* G = &S->G;
    const ampe_model_params* p = &S->p;
    long ng = G->ncg;
@@ -285,6 +298,7 @@ static int rhs_eval(OSim* S, double* y, double* ydot, int fd)
 /* CVSpgmrPrecondSet (source/QuatIntegrator.cc:3300-3376) */
 static int psetup(OSim* S, double* y, double gamma)
 {
+   if (S->Hh) return psetup_hier(S, y, gamma);
    const OGeom* G = &S->G;
    const ampe_model_params* p = &S->p;
    long ng = G->ncg;
@@ -478,6 +492,7 @@ static int quat_solve(OSim* S, const double* r, double* z, double tol)
 /* CVSpgmrPrecondSolve (source/QuatIntegrator.cc:3666-3771) */
 static int psolve(OSim* S, const double* r, double* z, double delta)
 {
+   if (S->Hh) return psolve_hier(S, r, z, delta);
    const OGeom* G = &S->G;
    const ampe_model_params* p = &S->p;
    long ng = G->ncg;
@@ -502,9 +517,52 @@ static int psolve(OSim* S, const double* r, double* z, double delta)
    return ret;
 }
 
+/* SAMRAIVectorReal::dot / weightedRMSNorm with the control volumes of the
+ * weight field (level-0 cell volume times the relative weight, zero under a
+ * finer level) on a hierarchy; the single-level forms otherwise */
+static double hdot(const OSim* S, const double* x, const double* y)
+{
+   if (!S->Hh) return o_dot(&S->G, S->nc, x, y);
+   const OHier* H = S->Hh;
+   long ng = o_hier_ncg(H);
+   double s = 0.0;
+   for (int m = 0; m < S->nc; m++)
+      for (long n = 0; n < ng; n++) {
+         double w = S->wt[n];
+         if (w != 0.0) s += (x[(long)m * ng + n] * y[(long)m * ng + n]) * w;
+      }
+   const OGeom* G0 = o_hier_geom(H, 0);
+   double cv = G0->dx[0] * G0->dx[1] * (G0->ndim == 3 ? G0->dx[2] : 1.0);
+   return s * cv;
+}
+
 static double wrms(const OSim* S, const double* x)
 {
-   return o_wrms(&S->G, S->nc, x, S->ewt);
+   if (!S->Hh) return o_wrms(&S->G, S->nc, x, S->ewt);
+   long ng = o_hier_ncg(S->Hh);
+   double s = 0.0;
+   for (int m = 0; m < S->nc; m++)
+      for (long n = 0; n < ng; n++) {
+         double w = S->wt[n];
+         if (w == 0.0) continue;
+         double v = x[(long)m * ng + n] * S->ewt[(long)m * ng + n];
+         s += (v * v) * w;
+      }
+   return sqrt(s / S->wsum);
+}
+
+/* sqrt of the summed control volumes over all components (CVODE's
+ * LSNormFactor(-1) norm factor) */
+static double norm_factor(const OSim* S)
+{
+   if (!S->Hh) {
+      const OGeom* G = &S->G;
+      double cv = G->dx[0] * G->dx[1] * (G->ndim == 3 ? G->dx[2] : 1.0);
+      return sqrt((double)S->nc * G->nx * G->ny * G->nz * cv);
+   }
+   const OGeom* G0 = o_hier_geom(S->Hh, 0);
+   double cv = G0->dx[0] * G0->dx[1] * (G0->ndim == 3 ? G0->dx[2] : 1.0);
+   return sqrt(S->wsum * cv);
 }
 
 /* Jv by difference quotient (CVODE cvLsDQJtimes) and atimes z = v - gamma Jv */
@@ -529,7 +587,6 @@ static int spgmr(OSim* S, double gamma, const double* bvec, double* x,
 {
    int maxl = S->s.maxl;
    long nv = S->nv;
-   const OGeom* G = &S->G;
    double** V = S->V;
    double* Hm = S->H;
    double givens[64], g[32], yv[32];
@@ -538,7 +595,7 @@ static int spgmr(OSim* S, double gamma, const double* bvec, double* x,
    memset(x, 0, sizeof(double) * nv);
    if (psolve(S, bvec, S->work, delta) < 0) return 2;
    for (long n = 0; n < nv; n++) V[0][n] = S->ewt[n] * S->work[n];
-   double rho = sqrt(o_dot(G, S->nc, V[0], V[0]));
+   double rho = sqrt(hdot(S, V[0], V[0]));
    double rho0 = rho;
    *nli_out = 0;
    if (rho <= delta) return 0;
@@ -561,7 +618,7 @@ static int spgmr(OSim* S, double gamma, const double* bvec, double* x,
           * new vector, one linear combination, re-orthogonalise when the
           * norm dropped by more than a factor 1000) */
          double st[34];
-         for (int i = 0; i <= l + 1; i++) st[i] = o_dot(G, S->nc, V[l + 1], V[i]);
+         for (int i = 0; i <= l + 1; i++) st[i] = hdot(S, V[l + 1], V[i]);
          double vk_norm = sqrt(st[l + 1]);
          for (int i = 0; i <= l; i++) HH(i, l) = st[i];
          for (long n = 0; n < nv; n++) {
@@ -569,25 +626,25 @@ static int spgmr(OSim* S, double gamma, const double* bvec, double* x,
             for (int i = 0; i <= l; i++) acc = acc + (-st[i]) * V[i][n];
             V[l + 1][n] = acc;
          }
-         hn = sqrt(o_dot(G, S->nc, V[l + 1], V[l + 1]));
+         hn = sqrt(hdot(S, V[l + 1], V[l + 1]));
          if (1000.0 * hn < vk_norm) {
-            for (int i = 0; i <= l; i++) st[i] = o_dot(G, S->nc, V[l + 1], V[i]);
+            for (int i = 0; i <= l; i++) st[i] = hdot(S, V[l + 1], V[i]);
             for (int i = 0; i <= l; i++) HH(i, l) += st[i];
             for (long n = 0; n < nv; n++) {
                double acc = V[l + 1][n];
                for (int i = 0; i <= l; i++) acc = acc + (-st[i]) * V[i][n];
                V[l + 1][n] = acc;
             }
-            hn = sqrt(o_dot(G, S->nc, V[l + 1], V[l + 1]));
+            hn = sqrt(hdot(S, V[l + 1], V[l + 1]));
          }
       } else {
          /* modified Gram-Schmidt */
          for (int i = 0; i <= l; i++) {
-            double hij = o_dot(G, S->nc, V[i], V[l + 1]);
+            double hij = hdot(S, V[i], V[l + 1]);
             HH(i, l) = hij;
             for (long n = 0; n < nv; n++) V[l + 1][n] = V[l + 1][n] + (-hij) * V[i][n];
          }
-         hn = sqrt(o_dot(G, S->nc, V[l + 1], V[l + 1]));
+         hn = sqrt(hdot(S, V[l + 1], V[l + 1]));
       }
       HH(l + 1, l) = hn;
       /* apply previous Givens rotations to column l, then a new one */
@@ -702,8 +759,7 @@ static int bdf_step(OSim* S)
    }
    /* nrmfac = sqrt(sum of control volumes over all components)
     * (CVodeSetLSNormFactor(-1): sqrt(N_VDotProd(1,1))) */
-   double cv = G->dx[0] * G->dx[1] * (G->ndim == 3 ? G->dx[2] : 1.0);
-   double nrmfac = sqrt((double)S->nc * G->nx * G->ny * G->nz * cv);
+   double nrmfac = norm_factor(S);
    double delta = S->s.eplifac * S->s.nlscoef * nrmfac;
    int force_setup = 0;
    for (;;) {
@@ -739,8 +795,8 @@ static int bdf_step(OSim* S)
          for (long n = 0; n < nv; n++) S->acor[n] = S->ycur[n] - S->ypred[n];
          if (S->iq >= 0 && S->p.qlen > 1) {
             memset(S->corr, 0, sizeof(double) * nv);
-            o_quat_project(G, &S->p, S->ycur + (long)S->iq * ng,
-                           S->corr + (long)S->iq * ng, S->acor + (long)S->iq * ng);
+            project_q(S, S->ycur + (long)S->iq * ng, S->corr + (long)S->iq * ng,
+                      S->acor + (long)S->iq * ng);
             for (long n = 0; n < nv; n++) S->ycur[n] = S->ycur[n] + S->corr[n];
          }
          S->st.last_acnrm = wrms(S, S->acor);
@@ -967,8 +1023,7 @@ static int bdf_adaptive_step(OSim* S)
       if (S->qprime != S->q) bdf_adjust_order(S, S->qprime);
       bdf_rescale(S, S->eta);
    }
-   double cv = G->dx[0] * G->dx[1] * (G->ndim == 3 ? G->dx[2] : 1.0);
-   double nrmfac = sqrt((double)S->nc * G->nx * G->ny * G->nz * cv);
+   double nrmfac = norm_factor(S);
    int nef = 0, ncf = 0, force_setup = 0, predicted = 0;
    double gamma = 0.0, delta = 0.0;
    for (;;) {
@@ -1007,8 +1062,8 @@ static int bdf_adaptive_step(OSim* S)
       axpby(nv, 1.0, S->ycur, -1.0, S->ypred, S->acor);
       if (S->iq >= 0 && S->p.qlen > 1) {
          memset(S->corr, 0, sizeof(double) * nv);
-         o_quat_project(G, &S->p, S->ycur + (long)S->iq * ng, S->corr + (long)S->iq * ng,
-                        S->acor + (long)S->iq * ng);
+         project_q(S, S->ycur + (long)S->iq * ng, S->corr + (long)S->iq * ng,
+                   S->acor + (long)S->iq * ng);
          axpby(nv, 1.0, S->ycur, 1.0, S->corr, S->ycur);
       }
       double acnrm = wrms(S, S->acor);
@@ -1058,6 +1113,268 @@ static int bdf_adaptive_step(OSim* S)
       update_ewt(S, S->y);
       return 0;
    }
+}
+
+
+/* ------------------------------------------------------------------------
+ * The integrator on a patch hierarchy: composite state vectors, the
+ * callbacks of QuatIntegrator over all levels (setCoefficients ->
+ * coarsenData of the iterate, fillScratch with LINEAR_REFINE coarse-fine
+ * ghosts, source/QuatIntegrator.cc:2994-3083, 2873-2957), composite
+ * right-hand sides, FAC preconditioner blocks over levels, and the
+ * control-volume weighted dot products and norms.
+ * ---------------------------------------------------------------------- */
+static void hier_sync_fill(OSim* S, double* y)
+{
+   const OHier* H = S->Hh;
+   long ng = o_hier_ncg(H);
+   o_hier_coarsen(H, y, 1, 0);
+   if (S->ic >= 0) o_hier_coarsen(H, y + (long)S->ic * ng, 1, 0);
+   if (S->iq >= 0) o_hier_coarsen(H, y + (long)S->iq * ng, S->p.qlen, 1);
+   o_hier_ghost_fill(H, y, S->nc, 2);
+}
+
+static int hier_kks(OSim* S, const double* y)
+{
+   const OHier* H = S->Hh;
+   long ng = o_hier_ncg(H);
+   int np = o_hier_npatch(H);
+   long* coff = (long*)malloc(sizeof(long) * np);
+   o_hier_offsets(H, coff, NULL);
+   const double* c = y + (long)S->ic * ng;
+   long tot = 0;
+   int fail = 0;
+   for (int p = 0; p < np; p++) {
+      const OGeom* G = o_hier_geom(H, p);
+      long o = coff[p];
+      int it = o_kks_solve(G, &S->p, y + o, c + o, S->clr + o, S->csr + o, S->cl + o, S->cs + o);
+      if (it < 0) fail = 1;
+      else tot += it;
+   }
+   free(coff);
+   S->st.kks_calls++;
+   S->st.kks_cells += ng;
+   if (fail) return -1;
+   S->st.kks_iters += tot;
+   return 0;
+}
+
+static int rhs_eval_hier(OSim* S, double* y, double* ydot, int fd)
+{
+   const OHier* H = S->Hh;
+   const ampe_model_params* p = &S->p;
+   long ng = o_hier_ncg(H);
+   int recompute = (!fd) || (!S->s.lag_quat_sidegrad);
+   S->st.nfe++;
+   hier_sync_fill(S, y);
+   const double* phi = y;
+   const double* q = (S->iq >= 0) ? y + (long)S->iq * ng : NULL;
+   const double* c = (S->ic >= 0) ? y + (long)S->ic * ng : NULL;
+   if (q && recompute) o_hier_quat_face_norm2(H, p, q, S->n2);
+   if (c && hier_kks(S, y)) return -1;
+   o_hier_phase_rhs(H, p, phi, q, S->cl, S->cs, ydot);
+   if (q) o_hier_quat_rhs(H, p, phi, q, S->n2, ydot + (long)S->iq * ng);
+   if (c) {
+      if (recompute) o_hier_conc_coeffs(H, p, phi, S->cl, S->cs, S->Dc, S->Dphi);
+      o_hier_conc_rhs(H, p, c, phi, S->Dc, S->Dphi, ydot + (long)S->ic * ng);
+   }
+   return 0;
+}
+
+static int psetup_hier(OSim* S, double* y, double gamma)
+{
+   const OHier* H = S->Hh;
+   const ampe_model_params* p0 = &S->p;
+   long ng = o_hier_ncg(H);
+   int np = o_hier_npatch(H);
+   S->st.npe++;
+   hier_sync_fill(S, y);
+   const double* phi = y;
+   const double* q = (S->iq >= 0) ? y + (long)S->iq * ng : NULL;
+   const double* c = (S->ic >= 0) ? y + (long)S->ic * ng : NULL;
+   if (q) o_hier_quat_face_norm2(H, p0, q, S->n2);
+   if (c && hier_kks(S, y)) return -1;
+   long* coff = (long*)malloc(sizeof(long) * np);
+   long* foff = (long*)malloc(sizeof(long) * np);
+   o_hier_offsets(H, coff, foff);
+   int kim = p0->phi_mobility_type == AMPE_PHI_MOB_KIM;
+   for (int pp = 0; pp < np; pp++) {
+      const OGeom* G = o_hier_geom(H, pp);
+      ampe_model_params p = *p0;
+      for (int a = 0; a < 3; a++) p.dx[a] = G->dx[a];
+      long o = coff[pp];
+      if (kim) o_phase_mobility_field(G, &p, S->cl + o, S->cs + o, S->Mphi + o);
+      o_phase_fac_C(G, &p, gamma, phi + o, kim ? S->Mphi + o : NULL, S->Cphi + o);
+      if (q) {
+         o_quat_face_coef(G, &p, phi + o, S->n2 + foff[pp], S->fc + foff[pp]);
+         o_quat_mobility(G, &p, phi + o, S->mq + o);
+      }
+   }
+   free(coff);
+   free(foff);
+   if (c)
+      for (long f = 0; f < S->nf; f++) S->Dconc[f] = -gamma * S->Dc[f];
+   if (q)
+      for (long n = 0; n < ng; n++) S->sqrtm[n] = sqrt(S->mq[n]);
+   S->gamma_setup = gamma;
+   return 0;
+}
+
+/* CVSpgmrPrecondSolve with the FAC blocks cycled over the levels
+ * (PhaseFACSolver / ConcFACSolver: EllipticFACOps with Ewing's fix;
+ * QuatSysSolver: divide by sqrt(M), QuatFACOps, multiply) */
+static int psolve_hier(OSim* S, const double* r, double* z, double delta)
+{
+   const OHier* H = S->Hh;
+   const ampe_model_params* p = &S->p;
+   long ng = o_hier_ncg(H);
+   int ret = 0;
+   S->st.nps++;
+   memset(z, 0, sizeof(double) * S->nv);
+   OHFacParams fp = {S->s.fac_max_cycles, 1, 1, S->s.fac_sweeps, S->s.fac_tol};
+   OHBlock ph = {0, NULL, -S->gamma_setup * p->epsilon_phi * p->epsilon_phi, S->Cphi, 1.0,
+                 p->phi_mobility_type == AMPE_PHI_MOB_KIM ? S->Mphi : NULL, p->phi_mobility,
+                 NULL, 0.0, 1};
+   if (o_hier_fac_solve(H, &ph, &fp, r, z, 1, delta, NULL, NULL)) ret = 1;
+   if (S->iq >= 0) {
+      int ql = p->qlen;
+      double* f = S->fac_u;
+      for (int m = 0; m < ql; m++)
+         for (long n = 0; n < ng; n++)
+            f[(long)m * ng + n] = r[(long)(S->iq + m) * ng + n] / S->sqrtm[n];
+      double* u = z + (long)S->iq * ng;
+      OHFacParams qp = {S->s.quat_max_cycles, 1, 1, S->s.quat_sweeps, -1.0};
+      OHBlock qb = {1, S->fc, 0.0, NULL, 1.0, NULL, 1.0, S->sqrtm, S->gamma_setup, 0};
+      if (o_hier_fac_solve(H, &qb, &qp, f, u, ql, delta, NULL, NULL)) ret = 1;
+      for (int m = 0; m < ql; m++)
+         for (long n = 0; n < ng; n++) u[(long)m * ng + n] *= S->sqrtm[n];
+   }
+   if (S->ic >= 0) {
+      OHBlock co = {0, S->Dconc, 0.0, NULL, 1.0, NULL, p->conc_mobility, NULL, 0.0, 1};
+      if (o_hier_fac_solve(H, &co, &fp, r + (long)S->ic * ng, z + (long)S->ic * ng, 1, delta,
+                           NULL, NULL))
+         ret = 1;
+   }
+   return ret;
+}
+
+/* project2d/3d on every interior cell (all patches) */
+static void project_q(OSim* S, const double* q, double* corr, double* err)
+{
+   if (!S->Hh) {
+      o_quat_project(&S->G, &S->p, q, corr, err);
+      return;
+   }
+   const OHier* H = S->Hh;
+   long ng = o_hier_ncg(H);
+   int ql = S->p.qlen;
+   for (long n = 0; n < ng; n++) {
+      if (!S->interior[n]) continue;
+      double qv[4], cv[4], fac = 0.0;
+      for (int m = 0; m < ql; m++) {
+         qv[m] = q[(long)m * ng + n];
+         fac = fac + qv[m] * qv[m];
+      }
+      fac = 1.0 / sqrt(fac);
+      for (int m = 0; m < ql; m++) cv[m] = qv[m] * fac;
+      fac = 0.0;
+      for (int m = 0; m < ql; m++) fac = fac + cv[m] * err[(long)m * ng + n];
+      for (int m = 0; m < ql; m++) err[(long)m * ng + n] = err[(long)m * ng + n] - cv[m] * fac;
+      for (int m = 0; m < ql; m++) corr[(long)m * ng + n] = cv[m] - qv[m];
+   }
+}
+
+OSim* o_sim_create_hier(const ampe_model_params* p, const ampe_solver_params* s, OHier* H)
+{
+   if (p->with_conc && p->conc_rhs_form != AMPE_CONC_RHS_KKS) return NULL;
+   if (p->evolve_quat && p->qlen > 4) return NULL;
+   OSim* S = o_sim_create(p, s, 1, 1, 1);
+   /* resize every field to the composite */
+   long ng = o_hier_ncg(H), nf = o_hier_nf(H);
+   S->Hh = H;
+   S->G.ncg = ng;
+   S->nv = (long)S->nc * ng;
+   S->nf = nf;
+   double** vs[] = {&S->y, &S->yprev, &S->ypred, &S->ycur, &S->ahist, &S->fcur,
+                    &S->b, &S->del, &S->ewt, &S->work, &S->work2, &S->jv,
+                    &S->xk, &S->corr, &S->acor, &S->z1, &S->z2};
+   for (size_t i = 0; i < sizeof(vs) / sizeof(vs[0]); i++) {
+      free(*vs[i]);
+      *vs[i] = vec(S->nv);
+   }
+   for (int i = 0; i <= s->maxl; i++) {
+      free(S->V[i]);
+      S->V[i] = vec(S->nv);
+   }
+   double** cs[] = {&S->cl, &S->cs, &S->clr, &S->csr, &S->Cphi, &S->Mphi, &S->phid,
+                    &S->sqrtm, &S->mq};
+   for (size_t i = 0; i < sizeof(cs) / sizeof(cs[0]); i++) {
+      free(*cs[i]);
+      *cs[i] = vec(ng);
+   }
+   int ql = p->evolve_quat ? p->qlen : 1;
+   free(S->fac_u);
+   free(S->fac_e);
+   free(S->fac_r);
+   S->fac_u = vec(ng * ql);
+   S->fac_e = vec(ng * ql);
+   S->fac_r = vec(ng * ql);
+   double** fs[] = {&S->n2, &S->Dc, &S->Dphi, &S->Dconc, &S->fc, &S->fac_flux};
+   for (size_t i = 0; i < sizeof(fs) / sizeof(fs[0]); i++) {
+      free(*fs[i]);
+      *fs[i] = vec(nf);
+   }
+   S->wt = vec(ng);
+   o_hier_weights(H, S->wt);
+   double sw = 0.0;
+   for (long n = 0; n < ng; n++) sw += S->wt[n];
+   S->wsum = (double)S->nc * sw;
+   /* interior mask of the composite */
+   S->interior = (unsigned char*)calloc((size_t)ng, 1);
+   int np = o_hier_npatch(H);
+   long* coff = (long*)malloc(sizeof(long) * np);
+   o_hier_offsets(H, coff, NULL);
+   for (int pp = 0; pp < np; pp++) {
+      const OGeom* G = o_hier_geom(H, pp);
+      for (int k = 0; k < G->nz; k++)
+         for (int j = 0; j < G->ny; j++)
+            for (int i = 0; i < G->nx; i++) S->interior[coff[pp] + CIG(G, i, j, k)] = 1;
+   }
+   free(coff);
+   return S;
+}
+
+/* composite state (nc blocks of the composite cell layout; ghosts ignored):
+ * coarsened, ghosts filled, initial phase concentrations from cL = cS = c */
+void o_sim_set_state_composite(OSim* S, const double* y)
+{
+   memcpy(S->y, y, sizeof(double) * S->nv);
+   hier_sync_fill(S, S->y);
+   if (S->ic >= 0) {
+      long ng = o_hier_ncg(S->Hh);
+      const double* cc = S->y + (long)S->ic * ng;
+      memcpy(S->clr, cc, sizeof(double) * ng);
+      memcpy(S->csr, cc, sizeof(double) * ng);
+      hier_kks(S, S->y);
+   }
+   S->t = 0.0;
+   S->nst = 0;
+   S->have_prev = 0;
+   S->h = first_step(&S->s);
+   S->started = 0;
+   memset(&S->st, 0, sizeof(S->st));
+   update_ewt(S, S->y);
+}
+
+/* the state with every finer level coarsened onto the cells it covers (the
+ * output QuatIntegrator coarsens after each step, :2190-2196) */
+void o_sim_get_state_composite(OSim* S, double* y)
+{
+   memcpy(y, S->y, sizeof(double) * S->nv);
+   long ng = o_hier_ncg(S->Hh);
+   o_hier_coarsen(S->Hh, y, 1, 0);
+   if (S->ic >= 0) o_hier_coarsen(S->Hh, y + (long)S->ic * ng, 1, 0);
+   if (S->iq >= 0) o_hier_coarsen(S->Hh, y + (long)S->iq * ng, S->p.qlen, 1);
 }
 
 int o_sim_advance(OSim* S, int nsteps, ampe_stats* st)
