@@ -31,6 +31,12 @@ int hier_conc_coeffs(const Hier& H, const ampe_model_params& p0, const double* p
                      const double* cl, const double* cs, double* Dc, double* Dphi, hipStream_t s);
 int hier_conc_rhs(const Hier& H, const ampe_model_params& p0, const double* c, const double* phi,
                   const double* Dc, const double* Dphi, double* rhs, hipStream_t s);
+// regrid: data of a new hierarchy from an old one, and new level boxes from
+// the gradient detector (amr.hip)
+int hier_transfer(const Hier& Ho, const Hier& Hn, const double* fo, double* fn, int ncomp,
+                  hipStream_t s);
+int hier_regrid(const Hier& Ho, const double* phi_o, const ampe_model_params& p0, double tol,
+                const ampe_cluster_params& cp, int max_levels, Hier** out, hipStream_t s);
 int hier_fac_solve(Hier& H, const ampe_hier_block& B, const ampe_fac_params& P, const double* f,
                    double* u, int ncomp, double tol, double* norm, int* cycles, hipStream_t s);
 // Per-level coefficient passes of the preconditioner setup: phase C and

@@ -1385,6 +1385,246 @@ int hier_precond_coeffs(const Hier& H, const ampe_model_params& p0, double gamma
    return 0;
 }
 
+// ------------------------------------------------------------ regrid
+namespace {
+
+__global__ void __launch_bounds__(256) k_copy_cells(const double* __restrict__ fo, long long ncg_o,
+                                                    double* __restrict__ fn, long long ncg_n,
+                                                    int ncomp, const long long* __restrict__ dst,
+                                                    const long long* __restrict__ src, long long n)
+{
+   long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+   if (e >= n) return;
+   for (int m = 0; m < ncomp; m++) fn[m * ncg_n + dst[e]] = fo[m * ncg_o + src[e]];
+}
+
+bool same_domain(const Hier& A, const Hier& B)
+{
+   if (A.ndim != B.ndim || A.periodic != B.periodic) return false;
+   for (int a = 0; a < 3; a++)
+      if (A.dom[0][a] != B.dom[0][a] || A.dx[0][a] != B.dx[0][a]) return false;
+   return true;
+}
+
+}  // namespace
+
+// The regrid fill of QuatModel::Regrid (source/QuatModel.cc:2733-2790; the
+// refine schedule of initializeLevelData, :2997-3004): a cell of the new
+// hierarchy's level l takes the old level l's value where an old patch holds
+// it, else LINEAR_REFINE of the new level l-1 (the coarse-fine ghost
+// interpolation), coarsest level first.  Interior cells only.
+int hier_transfer(const Hier& Ho, const Hier& Hn, const double* fo, double* fn, int ncomp,
+                  hipStream_t s)
+{
+   if (!same_domain(Ho, Hn) || ncomp < 1) {
+      set_error("hier_transfer: the hierarchies must share dimension, level-0 domain, spacing "
+                "and periodicity");
+      return -1;
+   }
+   const int ndim = Hn.ndim;
+   for (int l = 0; l < Hn.nlev; l++) {
+      std::vector<long long> cd, cs, rd, rs;
+      std::vector<double> rw;
+      for (int p = Hn.pbeg[l]; p < Hn.pbeg[l + 1]; p++) {
+         const ampe_box& b = Hn.boxes[p];
+         int n[3] = {box_n(b, 0, ndim), box_n(b, 1, ndim), box_n(b, 2, ndim)};
+         for (int k = 0; k < n[2]; k++)
+            for (int j = 0; j < n[1]; j++)
+               for (int i = 0; i < n[0]; i++) {
+                  int g[3] = {b.lo[0] + i, b.lo[1] + j, ndim == 3 ? b.lo[2] + k : 0};
+                  long long dst = cidx(Hn, p, i, j, k);
+                  long long src = l < Ho.nlev ? find_cell(Ho, l, g) : -1;
+                  if (src >= 0) {
+                     cd.push_back(dst);
+                     cs.push_back(src);
+                     continue;
+                  }
+                  if (l == 0) {
+                     set_error("hier_transfer: the old level 0 does not cover the new one");
+                     return -1;
+                  }
+                  int cp[3] = {fdiv2(g[0]), fdiv2(g[1]), ndim == 3 ? fdiv2(g[2]) : 0};
+                  canon(Hn, l - 1, cp);
+                  long long sc = find_cell(Hn, l - 1, cp);
+                  if (sc < 0) {
+                     set_error("hier_transfer: a level is not nested in the coarser one");
+                     return -1;
+                  }
+                  rd.push_back(dst);
+                  rs.push_back(sc);
+                  int gc[3] = {0, 0, 0};
+                  double w[3] = {0.0, 0.0, 0.0};
+                  for (int a = 0; a < ndim; a++)
+                     lin_index(g[a], Hn.dx[l][a], Hn.dx[l - 1][a], gc[a], w[a]);
+                  for (int a = 0; a < 3; a++) rw.push_back(w[a]);
+                  static const int off[8][3] = {{0, 0, 0}, {0, 1, 0}, {1, 0, 0}, {1, 1, 0},
+                                                {0, 0, 1}, {0, 1, 1}, {1, 0, 1}, {1, 1, 1}};
+                  for (int c = 0; c < 8; c++) {
+                     if (c >= (ndim == 3 ? 8 : 4)) {
+                        rs.push_back(sc);
+                        continue;
+                     }
+                     int cc[3] = {gc[0] + off[c][0], gc[1] + off[c][1],
+                                  ndim == 3 ? gc[2] + off[c][2] : 0};
+                     canon(Hn, l - 1, cc);
+                     long long sv = find_cell(Hn, l - 1, cc);
+                     if (sv < 0) {
+                        set_error("hier_transfer: a level is not nested in the coarser one");
+                        return -1;
+                     }
+                     rs.push_back(sv);
+                  }
+               }
+      }
+      DevVec<long long> dcd, dcs, drd, drs;
+      DevVec<double> drw;
+      auto release = [&]() {
+         dcd.release();
+         dcs.release();
+         drd.release();
+         drs.release();
+         drw.release();
+      };
+      if (dcd.upload(cd) || dcs.upload(cs) || drd.upload(rd) || drs.upload(rs) || drw.upload(rw)) {
+         release();
+         return -1;
+      }
+      if (dcd.n > 0)
+         hipLaunchKernelGGL(k_copy_cells, grid1(dcd.n), dim3(256), 0, s, fo, Ho.ncg, fn, Hn.ncg,
+                            ncomp, dcd.p, dcs.p, dcd.n);
+      if (drd.n > 0)
+         hipLaunchKernelGGL(k_cf_fill, grid1(drd.n), dim3(256), 0, s, fn, Hn.ncg, ncomp, drd.p,
+                            drs.p, drw.p, drd.n, 2, ndim);
+      hipError_t e = hipGetLastError();
+      if (e == hipSuccess) e = hipStreamSynchronize(s);
+      release();
+      if (e != hipSuccess) {
+         set_error(std::string("hier_transfer: ") + hipGetErrorString(e));
+         return -1;
+      }
+   }
+   return 0;
+}
+
+// New level boxes from the gradient detector, coarsest first (the role of
+// GriddingAlgorithm::regridAllFinerLevels in PFModel::Regrid, source/
+// PFModel.cc:565-590, with the tags of QuatModel::applyGradientDetector,
+// source/QuatModel.cc:3437-3602).  Level 0 is kept.  For k = 0 .. max_levels-2
+// the tag field is carried onto the levels 0..k built so far (hier_transfer),
+// its ghosts filled by linear refinement, level k tagged, and the tags
+// clustered (cluster_mask): over the level-0 domain for k = 0; above, per
+// patch of level k inside the patch shrunk by one cell on every side that is
+// not a non-periodic physical boundary, so that every new box is properly
+// nested.  Clustered level-k boxes refined by 2 are level k+1; no tags end
+// the hierarchy.
+int hier_regrid(const Hier& Ho, const double* phi_o, const ampe_model_params& p0, double tol,
+                const ampe_cluster_params& cp, int max_levels, Hier** out, hipStream_t s)
+{
+   if (max_levels < 1 || max_levels > kMaxLev) {
+      set_error("hier_regrid: max_levels must be 1.." + std::to_string(kMaxLev));
+      return -1;
+   }
+   const int ndim = Ho.ndim;
+   std::vector<std::vector<ampe_box>> lev(1);
+   lev[0].assign(Ho.boxes.begin() + Ho.pbeg[0], Ho.boxes.begin() + Ho.pbeg[1]);
+   int dom0[3] = {Ho.dom[0][0], Ho.dom[0][1], Ho.dom[0][2]};
+   double dx0[3] = {Ho.dx[0][0], Ho.dx[0][1], Ho.dx[0][2]};
+   auto build = [&](int nl, Hier** H) {
+      std::vector<int> np;
+      std::vector<ampe_box> flat;
+      for (int l = 0; l < nl; l++) {
+         np.push_back((int)lev[l].size());
+         flat.insert(flat.end(), lev[l].begin(), lev[l].end());
+      }
+      return hier_create(ndim, nl, np.data(), flat.data(), dom0, dx0, Ho.periodic, H);
+   };
+   for (int k = 0; k + 1 < max_levels; k++) {
+      Hier* Hk = nullptr;
+      if (build(k + 1, &Hk)) return -1;
+      double* phi = nullptr;
+      int* tags = nullptr;
+      std::vector<int> ht((size_t)Hk->ncell);
+      int rc = 0;
+      if (hipMalloc((void**)&phi, sizeof(double) * (size_t)Hk->ncg) != hipSuccess ||
+          hipMalloc((void**)&tags, sizeof(int) * (size_t)Hk->ncell) != hipSuccess) {
+         set_error("hier_regrid: out of device memory");
+         rc = -1;
+      }
+      if (!rc && fill_zero<double>(phi, Hk->ncg, s)) rc = -1;
+      if (!rc && hier_transfer(Ho, *Hk, phi_o, phi, 1, s)) rc = -1;
+      if (!rc && hier_ghost_fill(*Hk, phi, 1, 2, s)) rc = -1;
+      if (!rc && launch_tag(*Hk->lev[k], lev_params(*Hk, p0, k), phi, tol, tags, s)) rc = -1;
+      if (!rc && (hipMemcpyAsync(ht.data(), tags, sizeof(int) * ht.size(), hipMemcpyDeviceToHost,
+                                 s) != hipSuccess ||
+                  hipStreamSynchronize(s) != hipSuccess)) {
+         set_error("hier_regrid: tag copy failed");
+         rc = -1;
+      }
+      (void)hipFree(phi);
+      (void)hipFree(tags);
+      std::vector<ampe_box> nb;
+      for (int p = Hk->pbeg[k]; !rc && p < Hk->pbeg[k + 1]; p++) {
+         // the region clustered: the level-0 domain, or the shrunk patch
+         ampe_box r;
+         if (k == 0) {
+            if (p > Hk->pbeg[0]) break;
+            for (int a = 0; a < 3; a++) {
+               r.lo[a] = 0;
+               r.hi[a] = a < ndim ? Hk->dom[0][a] - 1 : 0;
+            }
+         } else {
+            r = Hk->boxes[p];
+            bool empty = false;
+            for (int a = 0; a < ndim; a++) {
+               if (Ho.periodic || r.lo[a] > 0) r.lo[a] += 1;
+               if (Ho.periodic || r.hi[a] < Hk->dom[k][a] - 1) r.hi[a] -= 1;
+               if (r.hi[a] < r.lo[a]) empty = true;
+            }
+            if (empty) continue;
+         }
+         int rn[3] = {box_n(r, 0, ndim), box_n(r, 1, ndim), box_n(r, 2, ndim)};
+         std::vector<unsigned char> mask((size_t)rn[0] * rn[1] * rn[2], 0);
+         for (int q = Hk->pbeg[k]; q < Hk->pbeg[k + 1]; q++) {
+            if (k > 0 && q != p) continue;
+            const PatchDev& pd = Hk->lev[k]->hp[q - Hk->pbeg[k]];
+            for (int kk = 0; kk < pd.nz; kk++)
+               for (int j = 0; j < pd.ny; j++)
+                  for (int i = 0; i < pd.nx; i++) {
+                     int g[3] = {pd.lo[0] + i, pd.lo[1] + j, pd.lo[2] + kk};
+                     bool in = true;
+                     for (int a = 0; a < ndim; a++)
+                        if (g[a] < r.lo[a] || g[a] > r.hi[a]) in = false;
+                     if (!in) continue;
+                     long long c =
+                         pd.int_off + i + (long long)pd.nx * (j + (long long)pd.ny * kk);
+                     if (!ht[(size_t)c]) continue;
+                     size_t m = (size_t)(g[0] - r.lo[0]) +
+                                (size_t)rn[0] * ((size_t)(g[1] - r.lo[1]) +
+                                                 (size_t)rn[1] * (size_t)(ndim == 3 ? g[2] - r.lo[2] : 0));
+                     mask[m] = 1;
+                  }
+         }
+         std::vector<ampe_box> bx(mask.size());
+         int nbx = 0;
+         if (cluster_mask(ndim, r, mask.data(), cp, bx.data(), (int)bx.size(), &nbx)) {
+            rc = -1;
+            break;
+         }
+         nb.insert(nb.end(), bx.begin(), bx.begin() + nbx);
+      }
+      hier_destroy(Hk);
+      if (rc) return -1;
+      if (nb.empty()) break;
+      for (ampe_box& b : nb)
+         for (int a = 0; a < ndim; a++) {
+            b.lo[a] = 2 * b.lo[a];
+            b.hi[a] = 2 * b.hi[a] + 1;
+         }
+      lev.push_back(nb);
+   }
+   return build((int)lev.size(), out);
+}
+
 }  // namespace ampe
 
 // ------------------------------------------------------------ C ABI
@@ -1414,18 +1654,58 @@ Hier* hier_of(ampe_hier* h) { return h ? h->H : nullptr; }
 
 extern "C" {
 
+static ampe_hier* hier_wrap(ampe::Hier* H)
+{
+   ampe_hier* h = new ampe_hier();
+   h->H = H;
+   for (int l = 0; l <= ampe::kMaxLev; l++) h->views[l] = nullptr;
+   for (int l = 0; l < H->nlev; l++) h->views[l] = ampe::level_view(H->lev[l]);
+   h->views[ampe::kMaxLev] = ampe::level_view(H->comp);
+   return h;
+}
+
 int ampe_hier_create(int ndim, int nlev, const int* npatch, const ampe_box* boxes,
                      const int dom0[3], const double dx0[3], int periodic, ampe_hier** out)
 {
    AMPE_HREQ(out, "null argument");
    ampe::Hier* H = nullptr;
    if (ampe::hier_create(ndim, nlev, npatch, boxes, dom0, dx0, periodic, &H)) return -1;
-   ampe_hier* h = new ampe_hier();
-   h->H = H;
-   for (int l = 0; l <= ampe::kMaxLev; l++) h->views[l] = nullptr;
-   for (int l = 0; l < nlev; l++) h->views[l] = ampe::level_view(H->lev[l]);
-   h->views[ampe::kMaxLev] = ampe::level_view(H->comp);
-   *out = h;
+   *out = hier_wrap(H);
+   return 0;
+}
+
+int ampe_hier_boxes(const ampe_hier* h, int* nlev, int npatch[AMPE_MAX_LEVELS], ampe_box* boxes,
+                    int max_boxes)
+{
+   AMPE_HREQ(h && nlev && npatch, "null argument");
+   const ampe::Hier& H = *h->H;
+   *nlev = H.nlev;
+   for (int l = 0; l < AMPE_MAX_LEVELS; l++)
+      npatch[l] = l < H.nlev ? H.pbeg[l + 1] - H.pbeg[l] : 0;
+   AMPE_HREQ((int)H.boxes.size() <= max_boxes || !boxes, "ampe_hier_boxes: max_boxes too small");
+   if (boxes)
+      for (size_t b = 0; b < H.boxes.size(); b++) boxes[b] = H.boxes[b];
+   return 0;
+}
+
+int ampe_hier_transfer(const ampe_hier* old_h, const ampe_hier* new_h, const double* f_old,
+                       double* f_new, int ncomp, ampe_stream_t stream)
+{
+   AMPE_HREQ(old_h && new_h && f_old && f_new, "null argument");
+   return ampe::hier_transfer(*old_h->H, *new_h->H, f_old, f_new, ncomp, hstream(stream));
+}
+
+int ampe_hier_regrid(const ampe_hier* old_h, const double* phi, const ampe_model_params* p0,
+                     double tol, const ampe_cluster_params* cp, int max_levels, ampe_hier** out,
+                     ampe_stream_t stream)
+{
+   AMPE_HREQ(old_h && phi && p0 && cp && out, "null argument");
+   AMPE_HREQ(cp->efficiency_tol > 0.0 && cp->efficiency_tol <= 1.0 && cp->tag_buffer >= 0,
+             "ampe_hier_regrid: need 0 < efficiency_tol <= 1 and tag_buffer >= 0");
+   ampe::Hier* H = nullptr;
+   if (ampe::hier_regrid(*old_h->H, phi, *p0, tol, *cp, max_levels, &H, hstream(stream)))
+      return -1;
+   *out = hier_wrap(H);
    return 0;
 }
 

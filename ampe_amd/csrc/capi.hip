@@ -457,6 +457,12 @@ int ampe_sim_set_state_composite(ampe_sim* sim, const double* y)
    return ampe::sim_set_state_composite(sim->S, y);
 }
 
+int ampe_sim_set_time(ampe_sim* sim, double t, double h0)
+{
+   AMPE_REQUIRE(sim, "null argument");
+   return ampe::sim_set_time(sim->S, t, h0);
+}
+
 int ampe_sim_get_state_composite(ampe_sim* sim, double* y)
 {
    AMPE_REQUIRE(sim && y, "null argument");

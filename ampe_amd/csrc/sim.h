@@ -85,6 +85,7 @@ int sim_create_hier(const ampe_model_params& p, const ampe_solver_params& s, Hie
 void sim_destroy(Sim* S);
 int sim_set_state_composite(Sim* S, const double* y_host);
 int sim_get_state_composite(Sim* S, double* y_host);
+int sim_set_time(Sim* S, double t, double h0);
 int sim_set_state(Sim* S, const double* phi, const double* c, const double* q);
 int sim_get_state(Sim* S, double* phi, double* c, double* q, double* cl, double* cs);
 int sim_advance(Sim* S, int nsteps, ampe_stats* stats);

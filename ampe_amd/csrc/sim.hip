@@ -382,6 +382,21 @@ int sim_set_state_composite(Sim* S, const double* y_host)
    return 0;
 }
 
+// The integrator restarted at time t after a set_state (a regrid:
+// QuatIntegrator::resetIntegrator, source/QuatIntegrator.cc:1676-1730, with
+// setSundialsOptions' initial time and step, :1571-1580); h0 > 0 is the
+// first step, else the solver's own first step.
+int sim_set_time(Sim* S, double t, double h0)
+{
+   if (!(t >= 0.0) || h0 != h0) {
+      set_error("sim_set_time: t must be >= 0");
+      return -1;
+   }
+   S->t = t;
+   if (h0 > 0.0) S->h = h0;
+   return 0;
+}
+
 // The state with every finer level coarsened onto the cells it covers (the
 // output QuatIntegrator coarsens after each step, source/QuatIntegrator.cc:
 // 2190-2196); the integrator's own state is left as it is.

@@ -115,6 +115,11 @@ _SIGS = {
     "ampe_sim_set_state_composite": (_i, [_vp, _vp]),
     "ampe_sim_get_state_composite": (_i, [_vp, _vp]),
     "ampe_hier_create": (_i, [_i, _i, _vp, _vp, _vp, _vp, _i, C.POINTER(_vp)]),
+    "ampe_hier_boxes": (_i, [_vp, C.POINTER(_i), _vp, _vp, _i]),
+    "ampe_hier_transfer": (_i, [_vp, _vp, _vp, _vp, _i, _vp]),
+    "ampe_hier_regrid": (_i, [_vp, _vp, C.POINTER(ModelParams), _d, _vp, _i, C.POINTER(_vp),
+                              _vp]),
+    "ampe_sim_set_time": (_i, [_vp, _d, _d]),
     "ampe_hier_destroy": (_i, [_vp]),
     "ampe_hier_sizes": (_i, [_vp, C.POINTER(_ll), C.POINTER(_ll), C.POINTER(_ll)]),
     "ampe_hier_level": (_i, [_vp, _i, C.POINTER(_vp)]),
@@ -455,20 +460,25 @@ class Hier:
     of lists of (lo, hi) inclusive boxes in each level's index space; dom0:
     level-0 cells (nx, ny[, nz])."""
 
-    def __init__(self, ndim, levels, dom0, dx0=(1.0, 1.0, 1.0), periodic=True):
+    def __init__(self, ndim, levels, dom0, dx0=(1.0, 1.0, 1.0), periodic=True, _handle=None):
         self.ndim = ndim
+        self.dom0, self.dx0, self.periodic = tuple(dom0), tuple(dx0), bool(periodic)
+        if _handle is not None:
+            h = _handle
+            levels = self._levels_of(h, ndim)
         self.levels = [[(tuple(lo), tuple(hi)) for lo, hi in lev] for lev in levels]
         flat = [b for lev in self.levels for b in lev]
-        arr = (Box * len(flat))()
-        for i, (lo, hi) in enumerate(flat):
-            arr[i].lo[:] = list(lo) + [0] * (3 - len(lo))
-            arr[i].hi[:] = list(hi) + [0] * (3 - len(hi))
-        npat = (_i * len(levels))(*[len(v) for v in levels])
-        d0 = (_i * 3)(*(list(dom0) + [1] * (3 - len(dom0))))
-        dx = (_d * 3)(*dx0)
-        h = _vp()
-        check(lib().ampe_hier_create(ndim, len(levels), npat, arr, d0, dx, 1 if periodic else 0,
-                                     C.byref(h)), "ampe_hier_create")
+        if _handle is None:
+            arr = (Box * len(flat))()
+            for i, (lo, hi) in enumerate(flat):
+                arr[i].lo[:] = list(lo) + [0] * (3 - len(lo))
+                arr[i].hi[:] = list(hi) + [0] * (3 - len(hi))
+            npat = (_i * len(levels))(*[len(v) for v in levels])
+            d0 = (_i * 3)(*(list(dom0) + [1] * (3 - len(dom0))))
+            dx = (_d * 3)(*dx0)
+            h = _vp()
+            check(lib().ampe_hier_create(ndim, len(levels), npat, arr, d0, dx,
+                                         1 if periodic else 0, C.byref(h)), "ampe_hier_create")
         self.h = h
         ncg, nc = _ll(), _ll()
         ns = (_ll * 3)()
@@ -478,6 +488,36 @@ class Hier:
         self.nside_tot = sum(self.nside)
         self.boxes = flat
         self.patch_level = [l for l, lev in enumerate(self.levels) for _ in lev]
+
+    @staticmethod
+    def _levels_of(h, ndim):
+        nlev = _i()
+        npat = (_i * 4)()
+        check(lib().ampe_hier_boxes(h, C.byref(nlev), npat, None, 0), "ampe_hier_boxes")
+        nb = sum(npat[l] for l in range(nlev.value))
+        arr = (Box * nb)()
+        check(lib().ampe_hier_boxes(h, C.byref(nlev), npat, arr, nb), "ampe_hier_boxes")
+        out, b = [], 0
+        for l in range(nlev.value):
+            lev = []
+            for _ in range(npat[l]):
+                lev.append((tuple(arr[b].lo[:ndim]), tuple(arr[b].hi[:ndim])))
+                b += 1
+            out.append(lev)
+        return out
+
+    def regrid(self, phi_ptr, p0, tol, cp, max_levels, stream=None):
+        """A new hierarchy from the gradient detector on the composite tag
+        field at device address phi_ptr (ampe_hier_regrid)."""
+        h = _vp()
+        check(lib().ampe_hier_regrid(self.h, phi_ptr, C.byref(p0), tol, C.byref(cp), max_levels,
+                                     C.byref(h), stream), "ampe_hier_regrid")
+        return Hier(self.ndim, None, self.dom0, self.dx0, self.periodic, _handle=h)
+
+    def transfer_to(self, new, f_old_ptr, f_new_ptr, ncomp, stream=None):
+        """Interior cells of a field on the new hierarchy (ampe_hier_transfer)."""
+        check(lib().ampe_hier_transfer(self.h, new.h, f_old_ptr, f_new_ptr, ncomp, stream),
+              "ampe_hier_transfer")
 
     def level(self, l):
         v = _vp()
@@ -519,6 +559,9 @@ class HierSim:
         check(lib().ampe_sim_get_state_composite(self.h, y.ctypes.data_as(_vp)),
               "ampe_sim_get_state_composite")
         return y
+
+    def set_time(self, t, h0=0.0):
+        check(lib().ampe_sim_set_time(self.h, t, h0), "ampe_sim_set_time")
 
     def advance(self, nsteps):
         st = Stats()

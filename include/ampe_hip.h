@@ -748,12 +748,17 @@ int ampe_vec_axpby(long long n, double a, const double* x, double b, const doubl
  * those arrays (ampe_hier_level), so every per-level entry point above runs
  * on it; ampe_hier_composite covers all patches (vector operations, KKS). */
 typedef struct ampe_hier ampe_hier;
+#define AMPE_MAX_LEVELS 4
 int ampe_hier_create(int ndim, int nlev, const int* npatch, const ampe_box* boxes,
                      const int dom0[3], const double dx0[3], int periodic, ampe_hier** out);
 int ampe_hier_destroy(ampe_hier* h);
 int ampe_hier_sizes(const ampe_hier* h, long long* ncell_ghost, long long* ncell,
                     long long nside[3]);
 int ampe_hier_level(ampe_hier* h, int l, ampe_level** level);
+/* The hierarchy's levels: *nlev, npatch[l] (0 above the finest level) and
+ * the boxes in ampe_hier_create's order (boxes may be NULL to query). */
+int ampe_hier_boxes(const ampe_hier* h, int* nlev, int npatch[AMPE_MAX_LEVELS], ampe_box* boxes,
+                    int max_boxes);
 int ampe_hier_composite(ampe_hier* h, ampe_level** level);
 /* Composite weights over ghosted cells: a cell's volume relative to a
  * level-0 cell, zero under a finer level and on ghosts (the control volumes
@@ -767,6 +772,31 @@ int ampe_hier_weights(const ampe_hier* h, double* w, ampe_stream_t stream);
  * QuatIntegrator::fillScratch (source/QuatIntegrator.cc:2873-2957). */
 int ampe_hier_ghost_fill(const ampe_hier* h, double* f, int ncomp, int cf,
                          ampe_stream_t stream);
+/* Regrid (PFModel::Regrid, source/PFModel.cc:565-590, the refine schedule
+ * of QuatModel::Regrid and initializeLevelData, source/QuatModel.cc:
+ * 2733-2790, 2997-3004).
+ * ampe_hier_regrid: a new hierarchy (the caller destroys it) with the old
+ * level 0 and finer levels from the gradient detector, coarsest first: for
+ * k = 0 .. max_levels-2 the tag field phi (one composite component on the
+ * old hierarchy) is carried onto the new levels 0..k as ampe_hier_transfer
+ * does, its ghosts filled by linear refinement, level k tagged as
+ * ampe_tag_cells does (tol, the level's own spacing), and the tags
+ * clustered as ampe_cluster_mask does (cp): over the level-0 domain for
+ * k = 0, above per patch of level k inside the patch shrunk by one cell on
+ * every side that is not a non-periodic physical boundary (proper nesting);
+ * the clustered boxes refined by 2 form level k+1, and a level without tags
+ * ends the hierarchy.  Synchronises the stream.
+ * ampe_hier_transfer: interior cells of new-hierarchy field f_new (ncomp
+ * components) from f_old on old_h: copied where a patch of the old
+ * hierarchy's same level holds the cell, else LINEAR_REFINE of the new
+ * level l-1 (quatlinrefine's interpolation, as the coarse-fine ghost fill),
+ * coarsest level first; level 0 must be covered by the old level 0.
+ * Ghosts are left as they are.  Synchronises the stream. */
+int ampe_hier_regrid(const ampe_hier* old_h, const double* phi, const ampe_model_params* p0,
+                     double tol, const ampe_cluster_params* cp, int max_levels, ampe_hier** out,
+                     ampe_stream_t stream);
+int ampe_hier_transfer(const ampe_hier* old_h, const ampe_hier* new_h, const double* f_old,
+                       double* f_new, int ncomp, ampe_stream_t stream);
 /* CONSERVATIVE_COARSEN of ncomp components from every finer level onto the
  * cells it covers, finest first (QuatIntegrator::coarsenData, source/
  * QuatIntegrator.cc:2289-2331); quatnorm: then q/|q| (QuatWeightedAverage,
@@ -852,6 +882,12 @@ int ampe_sim_create_hier(const ampe_model_params* p, const ampe_solver_params* s
 /* host composite state (nc x the hierarchy's ghosted cells; ghost entries
  * ignored): coarsened, ghosts filled, initial phase concentrations */
 int ampe_sim_set_state_composite(ampe_sim* sim, const double* y);
+/* After a set_state, restart the integrator at time t (a regrid: the
+ * reference resets CVODE at the current time with the previous step as its
+ * first, QuatIntegrator::resetIntegrator / setSundialsOptions, source/
+ * QuatIntegrator.cc:1571-1580, 1676-1730); h0 > 0 is the first step, else
+ * the solver's own.  Works for single-level sims too. */
+int ampe_sim_set_time(ampe_sim* sim, double t, double h0);
 /* host composite state with the finer levels coarsened onto the cells they
  * cover (the output the reference coarsens after each step) */
 int ampe_sim_get_state_composite(ampe_sim* sim, double* y);

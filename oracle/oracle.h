@@ -222,6 +222,7 @@ void o_hier_ghost_fill(const OHier* H, double* f, int ncomp, int cf);
 void o_hier_weights(const OHier* H, double* w);
 void o_hier_coarsen_level(const OHier* H, int l, double* f, int ncomp, int quatnorm);
 void o_hier_coarsen(const OHier* H, double* f, int ncomp, int quatnorm);
+int o_hier_transfer(const OHier* Ho, const OHier* Hn, const double* fo, double* fn, int ncomp);
 void o_hier_coarsen_faces_to(const OHier* H, int l, const double* src, double* dst, int ncomp,
                              long stride);
 void o_hier_coarsen_faces(const OHier* H, int l, double* s, int ncomp, long stride);
@@ -240,6 +241,7 @@ double o_hier_comp_residual(const OHier* H, const OHBlock* B, double* u, const d
 typedef struct OSim OSim;
 OSim* o_sim_create_hier(const ampe_model_params* p, const ampe_solver_params* s, OHier* H);
 void o_sim_set_state_composite(OSim* S, const double* y);
+void o_sim_set_time(OSim* S, double t, double h0);
 void o_sim_get_state_composite(OSim* S, double* y);
 int o_hier_fac_solve(const OHier* H, const OHBlock* B, const OHFacParams* P, const double* f,
                      double* u, int ncomp, double tol, double* norm, int* cycles);

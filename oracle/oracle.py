@@ -123,6 +123,7 @@ def _setup(L):
         "o_hier_geom": (_G, [C.c_void_p, C.c_int]),
         "o_hier_ghost_fill_level": (None, [C.c_void_p, C.c_int, _dp, C.c_int, C.c_int]),
         "o_hier_ghost_fill": (None, [C.c_void_p, _dp, C.c_int, C.c_int]),
+        "o_hier_transfer": (C.c_int, [C.c_void_p, C.c_void_p, _dp, _dp, C.c_int]),
         "o_hier_weights": (None, [C.c_void_p, _dp]),
         "o_hier_coarsen_level": (None, [C.c_void_p, C.c_int, _dp, C.c_int, C.c_int]),
         "o_hier_coarsen": (None, [C.c_void_p, _dp, C.c_int, C.c_int]),
@@ -134,6 +135,7 @@ def _setup(L):
         "o_hier_conc_rhs": (None, [C.c_void_p, _P, _dp, _dp, _dp, _dp, _dp]),
         "o_sim_create_hier": (C.c_void_p, [_P, C.POINTER(SolverParams), C.c_void_p]),
         "o_sim_set_state_composite": (None, [C.c_void_p, _dp]),
+        "o_sim_set_time": (None, [C.c_void_p, C.c_double, C.c_double]),
         "o_sim_get_state_composite": (None, [C.c_void_p, _dp]),
         "o_hier_comp_residual": (C.c_double, [C.c_void_p, C.c_void_p, _dp, _dp, _dp, C.c_int]),
         "o_hier_fac_solve": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, _dp, _dp, C.c_int,
@@ -312,6 +314,9 @@ class HierSim:
         lib().o_sim_get_state_composite(self.h, ptr(y))
         return y
 
+    def set_time(self, t, h0=0.0):
+        lib().o_sim_set_time(self.h, t, h0)
+
     def advance(self, nsteps):
         st = Stats()
         ret = lib().o_sim_advance(self.h, nsteps, C.byref(st))
@@ -367,6 +372,8 @@ class Hier:
         self.patch_level = [l for l, lev in enumerate(self.levels) for _ in lev]
         self.boxes = flat
         self.dom0 = tuple(dom0)
+        self.dx0 = tuple(dx0)
+        self.periodic = bool(periodic)
 
     def __del__(self):
         if getattr(self, "h", None) and _lib is not None:
@@ -431,3 +438,82 @@ class Hier:
         w = self.weights()
         m = self.get_level(w, lev, fill=0.0)
         return m > 0
+
+
+def hier_transfer(Ho, Hn, fo, ncomp=1):
+    """Interior cells of a field on Hn from fo on Ho (o_hier_transfer):
+    copied from the old same level, else linear refinement of the new
+    coarser level."""
+    fo = np.ascontiguousarray(fo, dtype=np.float64)
+    fn = Hn.cells(ncomp)
+    if lib().o_hier_transfer(Ho.h, Hn.h, ptr(fo), ptr(fn), ncomp):
+        raise ValueError("o_hier_transfer: hierarchies do not match or are not nested")
+    return fn
+
+
+def hier_level_tags(H, f, lev, p, tol):
+    """Gradient-detector tags (o_tag_cells) of level lev over the level
+    domain ([nz,] ny, nx), -1 off the level's patches; f: composite field
+    with ghosts filled."""
+    n = [H.dom0[d] << lev for d in range(H.ndim)]
+    shape = tuple(n[::-1])
+    tags = np.full(shape if H.ndim == 3 else (1,) + shape, -1, dtype=np.int32)
+    for pp in range(H.np):
+        if H.patch_level[pp] != lev:
+            continue
+        G = H.geoms[pp]
+        blk = np.ascontiguousarray(f[H.coff[pp]:H.coff[pp] + G.ncg])
+        t = np.zeros(G.nx * G.ny * G.nz, dtype=np.int32)
+        lib().o_tag_cells(C.byref(G), C.byref(p), ptr(blk), tol, iptr(t))
+        lo, _ = H.boxes[pp]
+        z0 = lo[2] if H.ndim == 3 else 0
+        tags[z0:z0 + G.nz, lo[1]:lo[1] + G.ny, lo[0]:lo[0] + G.nx] = t.reshape(G.nz, G.ny, G.nx)
+    return tags if H.ndim == 3 else tags[0]
+
+
+def regrid_levels(Ho, phi, p, tol, efficiency_tol, min_size, max_size, tag_buffer, max_levels):
+    """New level boxes from the gradient detector, coarsest first (the role
+    of GriddingAlgorithm::regridAllFinerLevels in PFModel::Regrid,
+    source/PFModel.cc:565-590; tags of QuatModel::applyGradientDetector,
+    source/QuatModel.cc:3437-3602): level 0 kept; for k = 0 .. max_levels-2
+    the tag field phi (composite on Ho) is carried onto the levels 0..k
+    built so far, its ghosts filled by linear refinement, level k tagged and
+    clustered (cluster.cluster): over the level-0 domain for k = 0, above per
+    patch of level k inside the patch shrunk by one cell on every side that
+    is not a non-periodic physical boundary; the boxes refined by 2 are
+    level k+1, and a level without tags ends the hierarchy."""
+    from oracle import cluster as CL
+    nd = Ho.ndim
+    levels = [list(Ho.levels[0])]
+    for k in range(max_levels - 1):
+        Hk = Hier(nd, levels, Ho.dom0, Ho.dx0, Ho.periodic)
+        f = hier_transfer(Ho, Hk, phi)
+        lib().o_hier_ghost_fill(Hk.h, ptr(f), 1, 2)
+        tags = hier_level_tags(Hk, f, k, p, tol)
+        n = [Ho.dom0[d] << k for d in range(nd)]
+        if k == 0:
+            regions = [(tuple([0] * nd), tuple(x - 1 for x in n))]
+        else:
+            regions = []
+            for (lo, hi) in levels[k]:
+                lo, hi = list(lo), list(hi)
+                for a in range(nd):
+                    if Ho.periodic or lo[a] > 0:
+                        lo[a] += 1
+                    if Ho.periodic or hi[a] < n[a] - 1:
+                        hi[a] -= 1
+                if all(hi[a] >= lo[a] for a in range(nd)):
+                    regions.append((tuple(lo), tuple(hi)))
+        boxes = []
+        for lo, hi in regions:
+            sl = tuple(slice(lo[a], hi[a] + 1) for a in reversed(range(nd)))
+            mask = (tags[sl] > 0).astype(np.uint8)
+            for blo, bhi in CL.cluster(mask, efficiency_tol, min_size, max_size, tag_buffer):
+                boxes.append((tuple(blo[a] + lo[a] for a in range(nd)),
+                              tuple(bhi[a] + lo[a] for a in range(nd))))
+        if not boxes:
+            break
+        levels.append([(tuple(2 * x for x in blo), tuple(2 * x + 1 for x in bhi))
+                       for blo, bhi in boxes])
+    return levels
+

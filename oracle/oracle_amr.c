@@ -291,6 +291,69 @@ OHier* o_hier_create(int ndim, int nlev, const int* npatch, const ampe_box* boxe
    return H;
 }
 
+/* The regrid fill of QuatModel::Regrid (source/QuatModel.cc:2733-2790; the
+ * refine schedule of initializeLevelData, :2997-3004): a cell of the new
+ * hierarchy's level l takes the old level l's value where an old patch holds
+ * it, else LINEAR_REFINE of the new level l-1 (quatlinrefine's interpolation
+ * of the coarse-fine ghost fill), coarsest level first.  Interior cells only;
+ * returns -1 when the hierarchies differ in dimension, level-0 domain or
+ * periodicity, or a cell finds no source. */
+int o_hier_transfer(const OHier* Ho, const OHier* Hn, const double* fo, double* fn, int ncomp)
+{
+   int nd = Hn->ndim;
+   if (Ho->ndim != nd || Ho->periodic != Hn->periodic) return -1;
+   for (int a = 0; a < 3; a++)
+      if (Ho->dom[0][a] != Hn->dom[0][a] || Ho->dx[0][a] != Hn->dx[0][a]) return -1;
+   const int off[8][3] = {{0, 0, 0}, {0, 1, 0}, {1, 0, 0}, {1, 1, 0},
+                          {0, 0, 1}, {0, 1, 1}, {1, 0, 1}, {1, 1, 1}};
+   for (int l = 0; l < Hn->nlev; l++)
+      for (int p = Hn->pbeg[l]; p < Hn->pbeg[l + 1]; p++) {
+         const OGeom* G = &Hn->G[p];
+         for (int k = 0; k < G->nz; k++)
+            for (int j = 0; j < G->ny; j++)
+               for (int i = 0; i < G->nx; i++) {
+                  int g[3] = {Hn->lo[p][0] + i, Hn->lo[p][1] + j, Hn->lo[p][2] + k};
+                  long dst = cidx(Hn, p, i, j, k);
+                  long src = l < Ho->nlev ? find(Ho, l, g) : -1;
+                  if (src >= 0) {
+                     for (int m = 0; m < ncomp; m++)
+                        fn[(long)m * Hn->ncg + dst] = fo[(long)m * Ho->ncg + src];
+                     continue;
+                  }
+                  if (l == 0) return -1;
+                  int gc[3] = {0, 0, 0};
+                  double w[3] = {0.0, 0.0, 0.0};
+                  for (int a = 0; a < nd; a++)
+                     lin_index(g[a], Hn->dx[l][a], Hn->dx[l - 1][a], &gc[a], &w[a]);
+                  long sv[8];
+                  int nc = nd == 3 ? 8 : 4;
+                  for (int c = 0; c < nc; c++) {
+                     int cc[3] = {gc[0] + off[c][0], gc[1] + off[c][1],
+                                  nd == 3 ? gc[2] + off[c][2] : 0};
+                     canon(Hn, l - 1, cc);
+                     sv[c] = find(Hn, l - 1, cc);
+                     if (sv[c] < 0) return -1;
+                  }
+                  for (int m = 0; m < ncomp; m++) {
+                     const double* a = fn + (long)m * Hn->ncg;
+                     double x = w[0], y = w[1], z = w[2];
+                     double q1 = a[sv[0]], q2 = a[sv[1]], q3 = a[sv[2]], q4 = a[sv[3]];
+                     double v;
+                     if (nd == 2) {
+                        v = (q1 + (q3 - q1) * x) * (1.0 - y) + (q2 + (q4 - q2) * x) * y;
+                     } else {
+                        double q5 = a[sv[4]], q6 = a[sv[5]], q7 = a[sv[6]], q8 = a[sv[7]];
+                        v = ((q1 + (q3 - q1) * x) * (1.0 - y) + (q2 + (q4 - q2) * x) * y) *
+                                (1.0 - z) +
+                            ((q5 + (q7 - q5) * x) * (1.0 - y) + (q6 + (q8 - q6) * x) * y) * z;
+                     }
+                     fn[(long)m * Hn->ncg + dst] = v;
+                  }
+               }
+      }
+   return 0;
+}
+
 int o_hier_nlev(const OHier* H) { return H->nlev; }
 int o_hier_npatch(const OHier* H) { return H->np; }
 long o_hier_ncg(const OHier* H) { return H->ncg; }

@@ -1366,6 +1366,16 @@ void o_sim_set_state_composite(OSim* S, const double* y)
    update_ewt(S, S->y);
 }
 
+/* the integrator restarted at time t after a set_state (a regrid:
+ * QuatIntegrator::resetIntegrator, source/QuatIntegrator.cc:1676-1730, with
+ * setSundialsOptions' initial time and step, :1571-1580); h0 > 0 is the first
+ * step, else the solver's own first step */
+void o_sim_set_time(OSim* S, double t, double h0)
+{
+   S->t = t;
+   if (h0 > 0.0) S->h = h0;
+}
+
 /* the state with every finer level coarsened onto the cells it covers (the
  * output QuatIntegrator coarsens after each step, :2190-2196) */
 void o_sim_get_state_composite(OSim* S, double* y)

@@ -293,3 +293,81 @@ def test_hier_integrator_full_fine_level_is_the_fine_integrator():
         assert np.abs(H.get_level(yh, 1, comp=1) - cc).max() < 1e-10
         for m in range(len(q)):
             assert np.abs(H.get_level(yh, 1, comp=2 + m) - qq[m]).max() < 1e-10
+
+
+def test_transfer_copies_old_cells_and_refines_new_ones():
+    """o_hier_transfer: identity onto the same hierarchy; a new fine patch
+    takes the old fine values where the old level holds the cell and the
+    exact linear values elsewhere on a linear field (linear refinement is
+    exact away from the clamped boundary); a non-linear old fine value
+    survives the transfer."""
+    Ho = O.Hier(2, [LEV0, LEV1], (32, 32), periodic=False)
+    f = _fill_levels(Ho, lambda l, n: _lin(n, 1.0 / (1 << l)))
+    # mark one old fine cell with a value linear refinement cannot produce
+    lv1 = Ho.get_level(f, 1, fill=0.0)
+    lv1[20, 18] = 7.0
+    Ho.set_level(f, 1, lv1)
+    same = O.hier_transfer(Ho, Ho, f)
+    for l in range(2):
+        a, b = Ho.get_level(same, l), Ho.get_level(f, l)
+        m = ~np.isnan(b)
+        assert np.array_equal(a[m], b[m])
+    new1 = [((12, 14), (27, 29)), ((40, 40), (51, 55))]
+    Hn = O.Hier(2, [LEV0, new1], (32, 32), periodic=False)
+    g = O.hier_transfer(Ho, Hn, f)
+    got = Hn.get_level(g, 1)
+    exact = _lin(64, 0.5)
+    old = ~np.isnan(Ho.get_level(f, 1))
+    for (lo, hi) in new1:
+        for j in range(lo[1], hi[1] + 1):
+            for i in range(lo[0], hi[0] + 1):
+                want = lv1[j, i] if old[j, i] else exact[j, i]
+                assert abs(got[j, i] - want) < 1e-14, (i, j)
+    assert got[20, 18] == 7.0
+    assert np.array_equal(Hn.get_level(g, 0), Ho.get_level(f, 0))
+
+
+def _disk_phi(n, r, w):
+    y, x = np.mgrid[0:n, 0:n] + 0.5
+    d = np.sqrt((x - n / 2) ** 2 + (y - n / 2) ** 2)
+    return 0.5 * (1.0 - np.tanh((d - r) / (w / 2.0)))
+
+
+@pytest.mark.parametrize("periodic", [True, False])
+def test_regrid_covers_tags_and_nests(periodic):
+    """regrid_levels: every buffered tag of level k lies in a level-(k+1)
+    box, boxes are disjoint and properly nested (the oracle hierarchy accepts
+    them), level-k+1 boxes of k >= 1 stay one cell inside their level-k
+    patch, and a field without gradients keeps level 0 alone."""
+    n = 32
+    Ho = O.Hier(2, [[((0, 0), (19, 31)), ((20, 0), (31, 31))]], (n, n), periodic=periodic)
+    f = Ho.cells()
+    Ho.set_level(f, 0, _disk_phi(n, 6.0, 3.0))
+    p = P.two_grain_params()
+    tol, buf = 0.05, 1
+    levels = O.regrid_levels(Ho, f, p, tol, 0.7, (2, 2, 2), (8, 8, 8), buf, 3)
+    assert len(levels) == 3
+    Hn = O.Hier(2, levels, (n, n), periodic=periodic)  # nesting checked by the constructor
+    O.lib().o_hier_ghost_fill(Ho.h, O.ptr(f), 1, 2)
+    tags0 = O.hier_level_tags(Ho, f, 0, p, tol)
+    cov = np.zeros((2 * n, 2 * n), bool)
+    for (lo, hi) in levels[1]:
+        assert not cov[lo[1]:hi[1] + 1, lo[0]:hi[0] + 1].any()
+        cov[lo[1]:hi[1] + 1, lo[0]:hi[0] + 1] = True
+    for j, i in zip(*np.nonzero(tags0 > 0)):
+        for dj in range(-buf, buf + 1):
+            for di in range(-buf, buf + 1):
+                jj, ii = j + dj, i + di
+                if 0 <= jj < n and 0 <= ii < n:
+                    assert cov[2 * jj, 2 * ii], (ii, jj)
+    for (lo, hi) in levels[2]:
+        clo, chi = [x // 2 for x in lo], [x // 2 for x in hi]
+        inside = [b for b in levels[1]
+                  if all(b[0][a] + 1 <= clo[a] and chi[a] <= b[1][a] - 1 for a in range(2))]
+        assert inside, (lo, hi)
+    # the fine levels follow the field: the transferred disk tags level 1
+    g = O.hier_transfer(Ho, Hn, f)
+    O.lib().o_hier_ghost_fill(Hn.h, O.ptr(g), 1, 2)
+    assert (O.hier_level_tags(Hn, g, 1, p, tol) > 0).any()
+    flat = Ho.cells() + 0.25
+    assert len(O.regrid_levels(Ho, flat, p, tol, 0.7, (2, 2, 2), (8, 8, 8), buf, 3)) == 1
