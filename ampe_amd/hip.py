@@ -300,6 +300,10 @@ class Sim:
         assert a.dtype == np.float64 and a.flags.c_contiguous
         return a.ctypes.data_as(_vp)
 
+    def set_time(self, t, h0=0.0):
+        """Restart at time t with first step h0 (ampe_sim_set_time)."""
+        check(lib().ampe_sim_set_time(self.h, t, h0), "ampe_sim_set_time")
+
     def set_state(self, phi, c, q):
         phi = np.ascontiguousarray(phi, dtype=np.float64)
         c = None if c is None else np.ascontiguousarray(c, dtype=np.float64)

@@ -640,9 +640,10 @@ int ampe_kks_batch_noalloc(const ampe_model_params* p, const ampe_calphad_tdep* 
                            ampe_kks_batch_stats* stats_dev, ampe_stream_t stream);
 
 /* ------------------------------------------------------------- integrator */
-/* BDF (order <= 2, fixed step) Newton-Krylov integrator with SPGMR (MGS,
- * left block preconditioner, difference-quotient Jv) driving the kernels
- * above -- the QuatIntegrator/CVODESolver callback loop
+/* BDF (order <= 2; imposed step, or CVODE's adaptive step and order with
+ * solver.adaptive) Newton-Krylov integrator with SPGMR (modified or classical
+ * Gram-Schmidt, left block preconditioner, difference-quotient Jv) driving
+ * the kernels above -- the QuatIntegrator/CVODESolver callback loop
  * (source/QuatIntegrator.cc:2096-2233, 3134-3771;
  * source/samrai/CVODESolver.cc:150-247). */
 typedef struct ampe_sim ampe_sim;
