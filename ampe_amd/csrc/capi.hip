@@ -223,6 +223,41 @@ int ampe_phase_rhs_multiorder(const ampe_level* level, const ampe_model_params* 
                                             hs(stream));
 }
 
+int ampe_kks_ternary3(const ampe_level* level, const ampe_model_params* p, const double* phi,
+                      const double* c, double* ph, ampe_stream_t stream)
+{
+   AMPE_REQUIRE(level && phi && c && ph, "null argument");
+   if (check_params(p, *level->L)) return -1;
+   return ampe::launch_kks_tern3(*level->L, *p, p->norderp, phi, c, ph, hs(stream));
+}
+
+int ampe_phase_rhs_ternary3(const ampe_level* level, const ampe_model_params* p,
+                            const double* phi, const double* ph, const double* mobility,
+                            double* rhs, ampe_stream_t stream)
+{
+   AMPE_REQUIRE(level && phi && ph && rhs, "null argument");
+   if (check_params(p, *level->L)) return -1;
+   return ampe::launch_phase_rhs_tern3(*level->L, *p, p->norderp, phi, ph, mobility, rhs,
+                                       hs(stream));
+}
+
+int ampe_ebs3_coeffs(const ampe_level* level, const ampe_model_params* p, const double* phi,
+                     double* DL, double* DA, double* DB, ampe_stream_t stream)
+{
+   AMPE_REQUIRE(level && phi && DL && DA && DB, "null argument");
+   if (check_params(p, *level->L)) return -1;
+   return ampe::launch_ebs3_coeffs(*level->L, *p, p->norderp, phi, DL, DA, DB, hs(stream));
+}
+
+int ampe_conc_rhs_ebs3(const ampe_level* level, const ampe_model_params* p, const double* ph,
+                       const double* DL, const double* DA, const double* DB, double* rhs,
+                       ampe_stream_t stream)
+{
+   AMPE_REQUIRE(level && ph && DL && DA && DB && rhs, "null argument");
+   if (check_params(p, *level->L)) return -1;
+   return ampe::launch_conc_rhs_ebs3(*level->L, *p, ph, DL, DA, DB, rhs, hs(stream));
+}
+
 int ampe_project_phases(const ampe_level* level, int norder, const double* phi, double* corr,
                         double* err, ampe_stream_t stream)
 {

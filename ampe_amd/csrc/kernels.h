@@ -174,6 +174,19 @@ int launch_phase_rhs_multiorder(const Level& L, const ampe_model_params& p, int 
                                 hipStream_t s);
 int launch_project_phases(const Level& L, int norder, const double* phi, double* corr,
                           double* err, hipStream_t s);
+// three phases on norderp > 1 orders, ternary quadratic (ternary.hip); ph:
+// six ghosted fields cL0, cL1, cA0, cA1, cB0, cB1; c and the composition rhs:
+// two species
+int launch_kks_tern3(const Level& L, const ampe_model_params& p, int norder, const double* phi,
+                     const double* c, double* ph, hipStream_t s);
+int launch_phase_rhs_tern3(const Level& L, const ampe_model_params& p, int norder,
+                           const double* phi, const double* ph, const double* mobility,
+                           double* rhs, hipStream_t s);
+int launch_ebs3_coeffs(const Level& L, const ampe_model_params& p, int norder, const double* phi,
+                       double* DL, double* DA, double* DB, hipStream_t s);
+int launch_conc_rhs_ebs3(const Level& L, const ampe_model_params& p, const double* ph,
+                         const double* DL, const double* DA, const double* DB, double* rhs,
+                         hipStream_t s);
 int launch_conc_rhs_ebs_4th(const Level& L, const ampe_model_params& p, const double* cl,
                             const double* cs, const double* DL, const double* DA, double* rhs,
                             hipStream_t s);

@@ -34,6 +34,12 @@ struct Sim {
    Hier* hier = nullptr;  // patch hierarchy (L is then its composite level)
    double wsum = 0.0;  // summed control-volume weights of all components (hierarchy)
    int nc = 0, iq = -1, ic = -1;
+   // phase orders (norderp > 1: MultiOrderRHSStrategy) and species (two for
+   // the three-phase ternary model, tern)
+   int nord = 1, ncs = 0;
+   bool tern = false;
+   double* ph6 = nullptr;  // ternary phase concentrations cL0, cL1, cA0, cA1, cB0, cB1
+   double* Db = nullptr;   // solid-B face diffusivity (ternary EBS)
    long long ng = 0, nv = 0, nf = 0;
    double t = 0, h = 0, hprev = 0, gamma_setup = 1, crate = 1;
    long long nst = 0, nstlp = 0;

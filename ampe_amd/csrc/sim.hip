@@ -81,6 +81,25 @@ static int sim_check(const ampe_model_params& p, const ampe_solver_params& s)
       set_error("sim_create: unknown EBS stencil");
       return -1;
    }
+   const bool tern = p.fe_model == AMPE_FE_QUADRATIC_TERNARY_3PH;
+   if (tern && (p.norderp < 3 || !p.with_conc || p.evolve_quat ||
+                p.conc_rhs_form != AMPE_CONC_RHS_EBS || p.ebs_stencil != AMPE_EBS_REGULAR ||
+                p.norderp_A < 1 || p.norderp_A > p.norderp - 1)) {
+      set_error("sim_create: the three-phase ternary model needs norderp >= 3, 1 <= norderp_A "
+                "<= norderp - 1, a composition field with the regular EBS form and no "
+                "orientation field");
+      return -1;
+   }
+   if (p.norderp > 1 && !tern && (p.with_conc || p.evolve_quat)) {
+      set_error("sim_create: several order parameters need the three-phase ternary model or "
+                "no composition and orientation fields");
+      return -1;
+   }
+   if (p.norderp > 1 && (s.precond != AMPE_PRECOND_REDBLACK || p.norderp + 2 > 8)) {
+      set_error("sim_create: several order parameters run the red-black preconditioner, "
+                "norderp <= 6");
+      return -1;
+   }
    return 0;
 }
 
@@ -111,9 +130,12 @@ static int sim_finish(Sim* S, Sim** out)
    const ampe_model_params& p = S->p;
    const ampe_solver_params& s = S->s;
    const Level& L = *S->L;
-   S->nc = 1 + (p.with_conc ? 1 : 0) + (p.evolve_quat ? p.qlen : 0);
-   S->ic = p.with_conc ? 1 : -1;
-   S->iq = p.evolve_quat ? 1 + (p.with_conc ? 1 : 0) : -1;
+   S->nord = p.norderp > 1 ? p.norderp : 1;
+   S->tern = p.fe_model == AMPE_FE_QUADRATIC_TERNARY_3PH;
+   S->ncs = p.with_conc ? (S->tern ? 2 : 1) : 0;
+   S->nc = S->nord + S->ncs + (p.evolve_quat ? p.qlen : 0);
+   S->ic = p.with_conc ? S->nord : -1;
+   S->iq = p.evolve_quat ? S->nord + S->ncs : -1;
    S->ng = L.ncg;
    S->nv = (long long)S->nc * S->ng;
    S->nf = L.nside_tot;
@@ -137,6 +159,10 @@ static int sim_finish(Sim* S, Sim** out)
    rc |= dalloc(&S->fac_r, S->nv);
    double** sides[] = {&S->n2, &S->Dc, &S->Dphi, &S->Dconc, &S->fc, &S->fcq};
    for (auto v : sides) rc |= dalloc(v, S->nf);
+   if (S->tern) {
+      rc |= dalloc(&S->ph6, 6 * S->ng);
+      rc |= dalloc(&S->Db, S->nf);
+   }
    if (s.precision == AMPE_FAC_FP32) {
       rc |= falloc(&S->f_u, S->nv);
       rc |= falloc(&S->f_e, S->nv);
@@ -187,6 +213,10 @@ int sim_create_hier(const ampe_model_params& p, const ampe_solver_params& s, Hie
       set_error("sim_create_hier: the hierarchy supports the KKS composition form");
       return -1;
    }
+   if (p.norderp > 1) {
+      set_error("sim_create_hier: one order parameter");
+      return -1;
+   }
    if (s.kks_guess == AMPE_GUESS_SURROGATE) {
       set_error("sim_create_hier: the previous-step KKS guess only");
       return -1;
@@ -212,7 +242,8 @@ void sim_destroy(Sim* S)
                     S->ewt, S->work, S->work2, S->jv, S->xk, S->corr, S->acor, S->cl, S->cs,
                     S->clr, S->csr, S->Cphi, S->sqrtm, S->Mphi, S->fac_u, S->fac_e, S->fac_r,
                     S->n2,
-                    S->Dc, S->Dphi, S->Dconc, S->fc, S->fcq, S->phid, S->z1, S->z2};
+                    S->Dc, S->Dphi, S->Dconc, S->fc, S->fcq, S->phid, S->z1, S->z2,
+                    S->ph6, S->Db};
    for (double* v : all) (void)hipFree(v);
    for (double* v : S->V) (void)hipFree(v);
    float* fall[] = {S->f_u,    S->f_e,     S->f_r,     S->f_f,   S->f_Cphi,
@@ -268,6 +299,7 @@ static int gather_interior(Sim* S, const double* dev, double* host)
 static int kks(Sim* S, const double* y)
 {
    const double* c = y + (long long)S->ic * S->ng;
+   if (S->tern) return launch_kks_tern3(*S->L, S->p, S->nord, y, c, S->ph6, S->stream);
    KksGuess g;
    if (S->s.kks_guess == AMPE_GUESS_SURROGATE) {
       if (!S->guess.surrogate) {
@@ -305,13 +337,18 @@ int sim_set_state(Sim* S, const double* phi, const double* c, const double* q)
    const Level& L = *S->L;
    hipStream_t st = S->stream;
    AMPE_HIP_CHECK(hipMemset(S->y, 0, sizeof(double) * S->nv));
-   if (scatter_interior(S, phi, S->y)) return -1;
+   for (int m = 0; m < S->nord; m++)
+      if (scatter_interior(S, phi + (long long)m * L.ncell, S->y + (long long)m * S->ng))
+         return -1;
    if (S->ic >= 0) {
       if (!c) {
          set_error("set_state: model has a composition field");
          return -1;
       }
-      if (scatter_interior(S, c, S->y + (long long)S->ic * S->ng)) return -1;
+      for (int m = 0; m < S->ncs; m++)
+         if (scatter_interior(S, c + (long long)m * L.ncell,
+                              S->y + (long long)(S->ic + m) * S->ng))
+            return -1;
    }
    if (S->iq >= 0) {
       if (!q) {
@@ -425,14 +462,22 @@ int sim_get_state(Sim* S, double* phi, double* c, double* q, double* cl, double*
 {
    const Level& L = *S->L;
    AMPE_HIP_CHECK(hipStreamSynchronize(S->stream));
-   if (phi && gather_interior(S, S->y, phi)) return -1;
-   if (c && S->ic >= 0 && gather_interior(S, S->y + (long long)S->ic * S->ng, c)) return -1;
+   for (int m = 0; phi && m < S->nord; m++)
+      if (gather_interior(S, S->y + (long long)m * S->ng, phi + (long long)m * L.ncell)) return -1;
+   for (int m = 0; c && m < S->ncs; m++)
+      if (gather_interior(S, S->y + (long long)(S->ic + m) * S->ng, c + (long long)m * L.ncell))
+         return -1;
    if (q && S->iq >= 0)
       for (int m = 0; m < S->p.qlen; m++)
          if (gather_interior(S, S->y + (long long)(S->iq + m) * S->ng, q + (long long)m * L.ncell))
             return -1;
-   if (cl && S->ic >= 0 && gather_interior(S, S->cl, cl)) return -1;
-   if (cs && S->ic >= 0 && gather_interior(S, S->cs, cs)) return -1;
+   // the three-phase model: cl = liquid, cs = solid A, two species each
+   for (int m = 0; S->ic >= 0 && m < S->ncs; m++) {
+      const double* a = S->tern ? S->ph6 + (long long)m * S->ng : S->cl;
+      const double* b = S->tern ? S->ph6 + (long long)(2 + m) * S->ng : S->cs;
+      if (cl && gather_interior(S, a, cl + (long long)m * L.ncell)) return -1;
+      if (cs && gather_interior(S, b, cs + (long long)m * L.ncell)) return -1;
+   }
    return 0;
 }
 
@@ -543,6 +588,17 @@ static int rhs_eval(Sim* S, double* y, double* ydot, bool fd)
    const double* phi = y;
    const double* q = S->iq >= 0 ? y + (long long)S->iq * S->ng : nullptr;
    const double* c = S->ic >= 0 ? y + (long long)S->ic * S->ng : nullptr;
+   if (S->tern) {
+      // three phases, two species: closed-form KKS, the multi-order phase RHS
+      // with the driving force, the EBS composition RHS of the three phases
+      if (kks(S, y)) return -1;
+      if (launch_phase_rhs_tern3(L, p, S->nord, phi, S->ph6, nullptr, ydot, st)) return -1;
+      if (recompute && launch_ebs3_coeffs(L, p, S->nord, phi, S->Dc, S->Dphi, S->Db, st))
+         return -1;
+      return launch_conc_rhs_ebs3(L, p, S->ph6, S->Dc, S->Dphi, S->Db,
+                                  ydot + (long long)S->ic * S->ng, st);
+   }
+   if (S->nord > 1) return launch_phase_rhs_multiorder(L, p, S->nord, phi, nullptr, ydot, st);
    if (q && recompute && launch_quat_grad(L, p, q, nullptr, S->n2, st)) return -1;
    if (c && kks(S, y)) return -1;
    if (launch_phase_rhs(L, p, phi, q, S->cl, S->cs, ydot, st)) return -1;
@@ -598,12 +654,19 @@ static int psetup(Sim* S, double* y, double gamma)
    // setCoefficients -> computeMobilities, source/QuatIntegrator.cc:3078-3080)
    bool kim = p.phi_mobility_type == AMPE_PHI_MOB_KIM;
    if (kim && launch_phase_mobility(L, p, S->cl, S->cs, S->Mphi, st)) return -1;
-   if (launch_phase_C(L, p, gamma, phi, kim ? S->Mphi : nullptr, S->Cphi, st)) return -1;
+   {
+      // several orders: the well scale passed is 0 (source/QuatIntegrator.cc
+      // :3321-3329), so C = 1
+      ampe_model_params pw = p;
+      if (S->nord > 1) pw.phi_well_scale = 0.0;
+      if (launch_phase_C(L, pw, gamma, phi, kim ? S->Mphi : nullptr, S->Cphi, st)) return -1;
+   }
    // composition block: D = -gamma D_c (source/ConcFACOps.cc:28-61); for EBS
-   // D_c = D_L + D_A (EBSCompositionRHSStrategy::
+   // D_c = D_L + D_A (+ D_B) (EBSCompositionRHSStrategy::
    // setDiffusionCoeffForPreconditionerOnPatch)
    if (c && p.conc_rhs_form == AMPE_CONC_RHS_EBS) {
       if (vec_axpby(S->nf, 1.0, S->Dc, 1.0, S->Dphi, S->Dconc, st)) return -1;
+      if (S->tern && vec_axpby(S->nf, 1.0, S->Dconc, 1.0, S->Db, S->Dconc, st)) return -1;
       if (vec_axpby(S->nf, -gamma, S->Dconc, 0.0, S->Dconc, S->Dconc, st)) return -1;
    } else if (c && vec_axpby(S->nf, -gamma, S->Dc, 0.0, S->Dc, S->Dconc, st)) {
       return -1;
@@ -793,16 +856,20 @@ static std::vector<PBlock<T>> make_blocks(Sim* S, double dphi, const T* Cphi, co
 {
    const ampe_model_params& p = S->p;
    std::vector<PBlock<T>> bl;
-   PBlock<T> ph{};
-   ph.kind = 0;
-   ph.comp = 0;
-   ph.ncomp = 1;
-   ph.op.Dc = (T)dphi;
-   ph.op.C = Cphi;
-   ph.op.Cc = T(1);
-   ph.op.Mc = (T)p.phi_mobility;
-   if (p.phi_mobility_type == AMPE_PHI_MOB_KIM) ph.op.M = Mphi;
-   bl.push_back(ph);
+   // one phase block per order (PhasePrecondSolve, source/QuatIntegrator.cc
+   // :3427-3444)
+   for (int m = 0; m < S->nord; m++) {
+      PBlock<T> ph{};
+      ph.kind = 0;
+      ph.comp = m;
+      ph.ncomp = 1;
+      ph.op.Dc = (T)dphi;
+      ph.op.C = Cphi;
+      ph.op.Cc = T(1);
+      ph.op.Mc = (T)p.phi_mobility;
+      if (p.phi_mobility_type == AMPE_PHI_MOB_KIM) ph.op.M = Mphi;
+      bl.push_back(ph);
+   }
    if (S->iq >= 0) {
       PBlock<T> qb{};
       qb.kind = 1;
@@ -812,10 +879,12 @@ static std::vector<PBlock<T>> make_blocks(Sim* S, double dphi, const T* Cphi, co
       qb.sm = sm;
       bl.push_back(qb);
    }
-   if (S->ic >= 0) {
+   // one composition block per species (ConcFACOps depth ic, source/
+   // ConcFACOps.cc:38-56; the ternary diagonals are equal)
+   for (int m = 0; m < S->ncs; m++) {
       PBlock<T> cb{};
       cb.kind = 0;
-      cb.comp = S->ic;
+      cb.comp = S->ic + m;
       cb.ncomp = 1;
       cb.op.D = Dconc;
       cb.op.Cc = T(1);

@@ -9,6 +9,7 @@ import os
 
 import numpy as np
 
+from ampe_amd import params as P
 from ampe_amd.params import (Box, CalphadPhase, CalphadTdep, KksBatchStats, ModelParams,
                              SolverParams, Stats, TrainParams)
 
@@ -116,6 +117,10 @@ _SIGS = {
     "ampe_sim_get_state_composite": (_i, [_vp, _vp]),
     "ampe_hier_create": (_i, [_i, _i, _vp, _vp, _vp, _vp, _i, C.POINTER(_vp)]),
     "ampe_hier_boxes": (_i, [_vp, C.POINTER(_i), _vp, _vp, _i]),
+    "ampe_kks_ternary3": (_i, [_vp, C.POINTER(ModelParams), _vp, _vp, _vp, _vp]),
+    "ampe_phase_rhs_ternary3": (_i, [_vp, C.POINTER(ModelParams), _vp, _vp, _vp, _vp, _vp]),
+    "ampe_ebs3_coeffs": (_i, [_vp, C.POINTER(ModelParams), _vp, _vp, _vp, _vp, _vp]),
+    "ampe_conc_rhs_ebs3": (_i, [_vp, C.POINTER(ModelParams), _vp, _vp, _vp, _vp, _vp, _vp]),
     "ampe_hier_transfer": (_i, [_vp, _vp, _vp, _vp, _i, _vp]),
     "ampe_hier_regrid": (_i, [_vp, _vp, C.POINTER(ModelParams), _d, _vp, _i, C.POINTER(_vp),
                               _vp]),
@@ -312,11 +317,18 @@ class Sim:
               "ampe_sim_set_state")
 
     def get_state(self, shape):
-        phi = np.zeros(shape)
-        c = np.zeros(shape)
-        q = np.zeros((self.p.qlen,) + tuple(shape))
-        cl = np.zeros(shape)
-        cs = np.zeros(shape)
+        """phi, c, q, cl, cs; with several orders phi is (norderp, *shape), with
+        the three-phase ternary model c, cl (liquid) and cs (solid A) are
+        (2, *shape)."""
+        shape = tuple(shape)
+        nord = self.p.norderp if self.p.norderp > 1 else 1
+        ncs = 2 if self.p.fe_model == P.FE_QUADRATIC_TERNARY_3PH else 1
+        lead = lambda k: (k,) if k > 1 else ()  # noqa: E731
+        phi = np.zeros(lead(nord) + shape)
+        c = np.zeros(lead(ncs) + shape)
+        q = np.zeros((self.p.qlen,) + shape)
+        cl = np.zeros(lead(ncs) + shape)
+        cs = np.zeros(lead(ncs) + shape)
         check(lib().ampe_sim_get_state(self.h, self._ptr(phi), self._ptr(c), self._ptr(q),
                                        self._ptr(cl), self._ptr(cs)), "ampe_sim_get_state")
         return phi, c, q, cl, cs

@@ -147,3 +147,39 @@ def make_nuclei(nx, ny, nz=1, radius=None, ngrains=1, phase_in=1.0, phase_out=0.
             T[d2 - r[g] ** 2 < 0.0] = temperature_in
         out["temperature"] = T.astype(np.float64)
     return out
+
+
+def make_multi_spheres(nx, ny, spheres, conc_a, conc_b, conc_out, nz=1, width=0.0):
+    """Restatement of utils/make_multi_spheres.py: one order parameter per
+    sphere (x, y, z, radius, phase 'A' or 'B' of the csv rows, cell units),
+    1 inside (distance to the centre below the radius), else 0, or
+    0.5 (1 + tanh(-d / 2w)) within 8w of the surface when width > 0; the
+    composition takes the sphere phase's values inside and conc_out elsewhere.
+    As in the generator, the periodicity option never takes effect (it
+    compares text with a number), so distances are plain Euclidean ones, and
+    for nz = 1 the centre sits at z = 1 while the cells sit at z = 0.5.
+    Single precision fields, as the generator writes them.
+
+    Returns dict(phase=[nspheres, nz, ny, nx], conc=[nspecies, nz, ny, nx])."""
+    nspecies = len(set(conc_a))
+    conc = np.zeros((nspecies, nz, ny, nx), np.float32)
+    phase = np.zeros((len(spheres), nz, ny, nx), np.float32)
+    for s in range(nspecies):
+        conc[s] = conc_out[s]
+    z = (np.arange(nz) + 0.5)[:, None, None]
+    y = (np.arange(ny) + 0.5)[None, :, None]
+    x = (np.arange(nx) + 0.5)[None, None, :]
+    for g, (cx, cy, cz, r, ph) in enumerate(spheres):
+        if nz == 1:
+            cz = 1
+        cs = conc_a if ph == "A" else conc_b
+        d2 = (x - cx) ** 2 + (y - cy) ** 2 + (z - cz) ** 2
+        d = np.sqrt(d2) - np.sqrt(float(r) ** 2)
+        inside = d < 0.0
+        phase[g][inside] = 1.0
+        for s in range(nspecies):
+            conc[s][inside] = cs[s]
+        if width > 0.0:
+            band = np.abs(d) < 8.0 * width
+            phase[g][band] = (0.5 * (1.0 + np.tanh(-1.0 * d / (2.0 * width))))[band]
+    return dict(phase=phase, conc=conc)

@@ -91,6 +91,13 @@ class ModelParams(C.Structure):
         ("conc_rhs_form", C.c_int),
         ("diff_interp", C.c_char),
         ("ebs_stencil", C.c_int),
+        ("norderp", C.c_int),
+        ("norderp_A", C.c_int),
+        ("vm_B", C.c_double),
+        ("tern_A", (C.c_double * 2) * 3),
+        ("tern_ceq", (C.c_double * 2) * 3),
+        ("D_solid_B", C.c_double),
+        ("Q0_solid_B", C.c_double),
     ]
 
 
@@ -141,7 +148,7 @@ class Stats(C.Structure):
 
 
 FLUX_SIMPLE, FLUX_ISOTROPIC, FLUX_ANISOTROPIC = 0, 1, 2
-FE_QUADRATIC, FE_CALPHAD_BINARY, FE_KKS_DILUTE = 0, 1, 2
+FE_QUADRATIC, FE_CALPHAD_BINARY, FE_KKS_DILUTE, FE_QUADRATIC_TERNARY_3PH = 0, 1, 2, 3
 PHI_MOB_CONSTANT, PHI_MOB_KIM = 0, 1
 CONC_RHS_KKS, CONC_RHS_EBS = 0, 1
 EBS_REGULAR, EBS_ISOTROPIC, EBS_ORDER4 = 0, 1, 2
@@ -608,3 +615,67 @@ def two_grains_quadratic_deck(ndim=2):
                       grain_vmin=0.179, grain_tol=1.0e-2, grain_tol_min=1.0e-3)
     checks["shape"] = state[0].shape
     return p, ramp, state, checks
+
+
+def sintering_deck():
+    """The reference's Sintering regression (tests/Sintering/2d.input,
+    test2d.py): 2-D 64 x 64 cells on [0, 3.2]^2 um, norderp = 5 order
+    parameters -- two grains of solid A, two of solid B, the liquid (pores)
+    last -- with the ternary three-phase quadratic model (A = 1e4 J/mol for
+    every phase and species, c_eq L = (0, 0), A = (1, 0), B = (0, 1)), EBS
+    composition flux with temperature-dependent diffusion (D_A = D_B = 1e4,
+    D_L = 0, Q = 0, linear weights), sigma = 0.1 J/m^2, delta = 0.045 um,
+    M_phi = 200, arithmetic averages, molar volumes 1.5e-5 m^3/mol,
+    T = 1000 K, periodic; atol 1e-4 (rtol atol/100), end_time 0.006 s in at
+    most 150 steps.  Initial state: make_multi_spheres.py --nx 64 --ny 64
+    --nz 1 --concentration-A 1.,0. --concentration-B 0.,1.
+    --concentration-out 0.,0. --spheres 4spheres.csv (spheres of radius 12
+    at (20, 20), (44, 20) [A] and (20, 44), (44, 44) [B]); the liquid order
+    parameter is 1 - the sum of the others (FieldsInitializer,
+    source/FieldsInitializer.cc:441-456).  The test passes when, at the end
+    time, the largest and smallest volume fractions over the five order
+    parameters are 0.586 and 0.118 (+- 0.001, test2d.py:59-69).
+
+    Returns (model params, initial state (phi[5], c[2]), checks dict)."""
+    from . import initial
+    sigma, delta = 0.1, 0.045
+    n = 64
+    p = two_grain_params(ndim=2)
+    for d in range(3):
+        p.dx[d] = 0.05
+    p.temperature = 1000.0
+    p.epsilon_phi = float(np.sqrt(6.0 * sigma * delta))
+    p.phi_well_scale = (3.0 * sigma / delta) / 16.0
+    p.phi_mobility = 200.0
+    p.flux_type = FLUX_SIMPLE
+    p.energy_interp = b"p"
+    p.conc_interp = b"l"
+    p.avg_type = b"a"
+    p.evolve_quat = 0
+    p.with_conc = 1
+    p.fe_model = FE_QUADRATIC_TERNARY_3PH
+    p.norderp, p.norderp_A = 5, 2
+    p.vm_L = p.vm_A = p.vm_B = 1.5e-5
+    for a in range(3):
+        for k in range(2):
+            p.tern_A[a][k] = 1.0e4
+    ceq = ((0.0, 0.0), (1.0, 0.0), (0.0, 1.0))
+    for a in range(3):
+        for k in range(2):
+            p.tern_ceq[a][k] = ceq[a][k]
+    p.D_liquid, p.Q0_liquid = 0.0, 0.0
+    p.D_solid, p.Q0_solid = 1.0e4, 0.0
+    p.D_solid_B, p.Q0_solid_B = 1.0e4, 0.0
+    p.conc_rhs_form = CONC_RHS_EBS
+    p.diff_interp = b"l"
+    p.ebs_stencil = EBS_REGULAR
+    spheres = [(20, 20, 16, 12, "A"), (44, 20, 16, 12, "A"), (20, 44, 16, 12, "B"),
+               (44, 44, 16, 12, "B")]
+    ic = initial.make_multi_spheres(n, n, spheres, conc_a=(1.0, 0.0), conc_b=(0.0, 1.0),
+                                    conc_out=(0.0, 0.0))
+    grains = ic["phase"][:, 0].astype(np.float64)
+    phi = np.concatenate([grains, (1.0 - grains.sum(axis=0))[None]])
+    c = ic["conc"][:, 0].astype(np.float64)
+    checks = dict(end_time=0.006, max_steps=150, vmax=0.586, vmin=0.118, tol=1.0e-3)
+    return p, (phi, c), checks
+

@@ -51,7 +51,10 @@ enum ampe_flux_type {
 enum ampe_free_energy {
    AMPE_FE_QUADRATIC = 0,      /* QuadraticFreeEnergyStrategy              */
    AMPE_FE_CALPHAD_BINARY = 1, /* CALPHADFreeEnergyStrategyBinary          */
-   AMPE_FE_KKS_DILUTE = 2      /* KKSdiluteBinary (dilute binary solution) */
+   AMPE_FE_KKS_DILUTE = 2,     /* KKSdiluteBinary (dilute binary solution) */
+   AMPE_FE_QUADRATIC_TERNARY_3PH = 3 /* QuadraticFreeEnergyStrategyMultiOrderTernaryThreePhase:
+                                      * three phases (liquid, solid A, solid B) on
+                                      * norderp > 1 order parameters, two species */
 };
 
 /* Binary CALPHAD phase description evaluated at the (uniform) temperature:
@@ -123,6 +126,18 @@ typedef struct ampe_model_params {
    char diff_interp;      /* EBS phase weights h(phi) of the diffusivities
                            * (diffusion_interp_func_type, 'l' linear, 'p' pbg) */
    int ebs_stencil;       /* enum ampe_ebs_stencil (ConcentrationModel ebs_stencil) */
+   /* multi-order phase field with three phases and a ternary quadratic model
+    * (norderp, norderp_A of ModelParameters; ConcentrationModel { nspecies = 3,
+    * model = "quadratic", Quadratic { A_liquid, Ceq_liquid, A_solid_A, ... } },
+    * source/QuadraticFreeEnergyStrategyMultiOrderTernaryThreePhase.cc:24-80):
+    * f_phase(c) = sum_k A[k] (c_k - ceq[k])^2 [J/mol] for phases L, A, B. */
+   int norderp;           /* phase order parameters (> 1: multi-order; the last is the liquid) */
+   int norderp_A;         /* orders 0 .. norderp_A-1 are grains of solid A, norderp_A ..
+                           * norderp-2 grains of solid B */
+   double vm_B;           /* molar volume of solid B [m^3/mol] */
+   double tern_A[3][2];   /* curvatures per phase (L, A, B) and species */
+   double tern_ceq[3][2]; /* equilibrium compositions per phase and species */
+   double D_solid_B, Q0_solid_B; /* D exp(-Q/RT) of solid B (D_solid, Q0_solid: solid A) */
 } ampe_model_params;
 
 /* Phase mobility: the uniform M_phi (SimpleQuatMobilityStrategy,
@@ -345,6 +360,40 @@ int ampe_conc_rhs_ebs(const ampe_level* level, const ampe_model_params* p,
 int ampe_phase_rhs_multiorder(const ampe_level* level, const ampe_model_params* p,
                               int norder, const double* phi, const double* mobility,
                               double* rhs, ampe_stream_t stream);
+/* Three phases (liquid = the last order, solid A = orders 0..norderp_A-1,
+ * solid B = the rest) with the ternary quadratic model
+ * (fe_model = AMPE_FE_QUADRATIC_TERNARY_3PH, p->norderp >= 3 orders, two
+ * species); phase weights h = (phi_L^2, sum_A phi^2, sum_B phi^2) / sum phi^2.
+ * ph: six ghosted fields cL0, cL1, cA0, cA1, cB0, cB1.
+ * ampe_kks_ternary3: QuadraticEquilibriumThreePhasesTernaryStrategyMultiOrder::
+ * computePhaseConcentrationsOnPatch (source/QuadraticEquilibriumThreePhases
+ * TernaryStrategyMultiOrder.cc:49-181) over the ghost box: for
+ * f_a(c) = sum_k A_ak (c_k - e_ak)^2 the KKS conditions (one potential
+ * mu_k = 2 A_ak (x_ak - e_ak), c_k = sum_a g(h_a) x_ak, g = conc_interp) in
+ * closed form; c holds two fields.
+ * ampe_phase_rhs_ternary3: ampe_phase_rhs_multiorder's terms minus Moelans'
+ * driving force 2 phi_p [(1 - h_a) w_a - sum_(b != a) h_b w_b] / sum phi^2,
+ * w = f - mu . c (QuadraticFreeEnergyStrategyMultiOrderTernaryThreePhase::
+ * addDrivingForceOnPatch, source/QuadraticFreeEnergyStrategyMultiOrder
+ * TernaryThreePhase.cc:327-520), then times the mobility.
+ * ampe_ebs3_coeffs: face diffusivities D_a = g_diff(sum over a's orders of
+ * the face average) D0_a exp(-Q_a/RT) (concentration_pfmdiffusion_of_
+ * temperature_threephases, source/fortran/2d/concentrationdiffusion.m4
+ * :523-640; D_solid/Q0_solid for A, D_solid_B/Q0_solid_B for B).
+ * ampe_conc_rhs_ebs3: rhs_k = M_c div(D_L grad cL_k + D_A grad cA_k +
+ * D_B grad cB_k) for both species (EBSCompositionRHSStrategy::
+ * computeFluxOnPatch with add_flux, source/EBSCompositionRHSStrategy.cc
+ * :183-328); rhs holds two fields. */
+int ampe_kks_ternary3(const ampe_level* level, const ampe_model_params* p, const double* phi,
+                      const double* c, double* ph, ampe_stream_t stream);
+int ampe_phase_rhs_ternary3(const ampe_level* level, const ampe_model_params* p,
+                            const double* phi, const double* ph, const double* mobility,
+                            double* rhs, ampe_stream_t stream);
+int ampe_ebs3_coeffs(const ampe_level* level, const ampe_model_params* p, const double* phi,
+                     double* DL, double* DA, double* DB, ampe_stream_t stream);
+int ampe_conc_rhs_ebs3(const ampe_level* level, const ampe_model_params* p, const double* ph,
+                       const double* DL, const double* DA, const double* DB, double* rhs,
+                       ampe_stream_t stream);
 /* Projection onto sum_p phi_p = 1 (projectphi, source/fortran/2d/phase.m4
  * :51-110, via MultiOrderRHSStrategy::projectPhases, source/MultiOrderRHSStrategy.cc
  * :262-300): corr_p = (phi_p - (sum phi - 1)/norder) - phi_p, and err loses
@@ -652,7 +701,11 @@ int ampe_sim_create(const ampe_model_params* p, const ampe_solver_params* s,
                     const ampe_box* periodic_domain, ampe_sim** out);
 int ampe_sim_destroy(ampe_sim* sim);
 /* Host arrays of interior cells, patch after patch, i fastest; q holds qlen
- * component blocks.  c / q may be NULL when the model has no such field. */
+ * component blocks.  c / q may be NULL when the model has no such field.
+ * With norderp > 1 phi holds norderp component blocks (the phase RHS is then
+ * MultiOrderRHSStrategy's, with one phase preconditioner block per order);
+ * with the three-phase ternary model c holds two (and get_state's cl, cs the
+ * liquid's and solid A's two species). */
 int ampe_sim_set_state(ampe_sim* sim, const double* phi, const double* c,
                        const double* q);
 int ampe_sim_get_state(ampe_sim* sim, double* phi, double* c, double* q,

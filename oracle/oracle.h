@@ -73,6 +73,19 @@ int o_grains_label(const OGeom* G, const double* phase, double thr, int periodic
                    int* sweeps);
 void o_ebs_coeffs(const OGeom* G, const ampe_model_params* p, const double* phi, double* DL,
                   double* DA);
+/* three phases on norderp > 1 orders, ternary quadratic (oracle_ternary.c) */
+void o_tern3_phase_conc(const ampe_model_params* p, const double c[2], const double h[3],
+                        double x[6]);
+void o_kks_tern3(const OGeom* G, const ampe_model_params* p, int norder, const double* phi,
+                 const double* c, double* ph);
+double o_tern3_driving_force(const ampe_model_params* p, int norder, const double* phi_cell,
+                             const double ph_cell[6], int ip);
+void o_phase_rhs_tern3(const OGeom* G, const ampe_model_params* p, int norder,
+                       const double* phi, const double* ph, const double* M, double* rhs);
+void o_ebs3_coeffs(const OGeom* G, const ampe_model_params* p, int norder, const double* phi,
+                   double* DL, double* DA, double* DB);
+void o_conc_rhs_ebs3(const OGeom* G, const ampe_model_params* p, const double* ph,
+                     const double* DL, const double* DA, const double* DB, double* rhs);
 void o_phase_rhs_multiorder(const OGeom* G, const ampe_model_params* p, int norder,
                             const double* phi, const double* M, double* rhs);
 void o_project_phases(const OGeom* G, int norder, const double* phi, double* corr,

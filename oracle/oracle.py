@@ -10,6 +10,7 @@ import subprocess
 import numpy as np
 
 from ampe_amd.params import CalphadGfun, CalphadPhase, CalphadTdep, ModelParams, SolverParams, Stats
+from ampe_amd.params import FE_QUADRATIC_TERNARY_3PH as FE_TERN3
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB = os.path.join(_HERE, "build", "liboracle.so")
@@ -124,6 +125,12 @@ def _setup(L):
         "o_hier_ghost_fill_level": (None, [C.c_void_p, C.c_int, _dp, C.c_int, C.c_int]),
         "o_hier_ghost_fill": (None, [C.c_void_p, _dp, C.c_int, C.c_int]),
         "o_hier_transfer": (C.c_int, [C.c_void_p, C.c_void_p, _dp, _dp, C.c_int]),
+        "o_tern3_phase_conc": (None, [_P, _dp, _dp, _dp]),
+        "o_kks_tern3": (None, [_G, _P, C.c_int, _dp, _dp, _dp]),
+        "o_tern3_driving_force": (C.c_double, [_P, C.c_int, _dp, _dp, C.c_int]),
+        "o_phase_rhs_tern3": (None, [_G, _P, C.c_int, _dp, _dp, _dp, _dp]),
+        "o_ebs3_coeffs": (None, [_G, _P, C.c_int, _dp, _dp, _dp, _dp]),
+        "o_conc_rhs_ebs3": (None, [_G, _P, _dp, _dp, _dp, _dp, _dp]),
         "o_hier_weights": (None, [C.c_void_p, _dp]),
         "o_hier_coarsen_level": (None, [C.c_void_p, C.c_int, _dp, C.c_int, C.c_int]),
         "o_hier_coarsen": (None, [C.c_void_p, _dp, C.c_int, C.c_int]),
@@ -210,7 +217,12 @@ class Sim:
         nx, ny, nz = (n if isinstance(n, (tuple, list)) else (n,) * 3)
         self.p, self.s = p, s
         self.h = lib().o_sim_create(C.byref(p), C.byref(s), nx, ny, nz)
+        if not self.h:
+            raise ValueError("o_sim_create rejected the model")
         self.shape = (ny, nx) if p.ndim == 2 else (nz, ny, nx)
+        # phase orders and species (norderp > 1; the three-phase ternary model)
+        self.nord = p.norderp if p.norderp > 1 else 1
+        self.ncs = (2 if p.fe_model == FE_TERN3 else 1) if p.with_conc else 0
         if not periodic:
             lib().o_sim_set_periodic(self.h, 0)
 
@@ -233,11 +245,14 @@ class Sim:
         lib().o_sim_set_state(self.h, ptr(phi), ptr(c), ptr(q))
 
     def get_state(self):
-        phi = np.zeros(self.shape)
-        c = np.zeros(self.shape)
+        """phi, c, q, cl, cs; with several orders phi is (norderp, ...), with the
+        three-phase ternary model c, cl (liquid) and cs (solid A) are (2, ...)."""
+        lead = lambda k: (k,) if k > 1 else ()  # noqa: E731
+        phi = np.zeros(lead(self.nord) + self.shape)
+        c = np.zeros(lead(self.ncs) + self.shape)
         q = np.zeros((self.p.qlen,) + self.shape)
-        cl = np.zeros(self.shape)
-        cs = np.zeros(self.shape)
+        cl = np.zeros(lead(self.ncs) + self.shape)
+        cs = np.zeros(lead(self.ncs) + self.shape)
         lib().o_sim_get_state(self.h, ptr(phi), ptr(c), ptr(q), ptr(cl), ptr(cs))
         return phi, c, q, cl, cs
 

@@ -49,6 +49,7 @@ struct OSim {
    ampe_solver_params s;
    OGeom G;
    int nc, iq, ic;
+   int nord, ncs, tern; /* phase orders, species, three-phase ternary model */
    long nv, nf;
    double t, h, hprev, gamma_setup, crate;
    long nst, nstlp;
@@ -57,7 +58,8 @@ struct OSim {
        *work2, *jv, *xk, *corr, *acor;
    double** V;
    double *cl, *cs, *clr, *csr;
-   double *n2, *Dc, *Dphi;
+   double* ph6; /* ternary three-phase concentrations cL0, cL1, cA0, cA1, cB0, cB1 */
+   double *n2, *Dc, *Dphi, *Db;
    double *Cphi, *Dconc, *fc, *sqrtm, *mq, *Mphi, *phid;
    double *fac_u, *fac_e, *fac_r, *fac_flux;
    double* H;
@@ -125,9 +127,21 @@ OSim* o_sim_create(const ampe_model_params* p, const ampe_solver_params* s,
                    ? 2
                    : 1;
    o_geom_init(&S->G, p->ndim, nx, ny, nz, ghost, p->dx);
-   S->nc = 1 + (p->with_conc ? 1 : 0) + (p->evolve_quat ? p->qlen : 0);
-   S->ic = p->with_conc ? 1 : -1;
-   S->iq = p->evolve_quat ? 1 + (p->with_conc ? 1 : 0) : -1;
+   /* norderp > 1: the phase field has norderp order parameters
+    * (MultiOrderRHSStrategy); the three-phase ternary model carries two
+    * species */
+   S->nord = p->norderp > 1 ? p->norderp : 1;
+   S->tern = p->fe_model == AMPE_FE_QUADRATIC_TERNARY_3PH;
+   S->ncs = p->with_conc ? (S->tern ? 2 : 1) : 0;
+   if ((S->tern && (S->nord < 3 || !p->with_conc || p->evolve_quat ||
+                    p->conc_rhs_form != AMPE_CONC_RHS_EBS)) ||
+       (S->nord > 1 && !S->tern && (p->with_conc || p->evolve_quat))) {
+      free(S);
+      return NULL;
+   }
+   S->nc = S->nord + S->ncs + (p->evolve_quat ? p->qlen : 0);
+   S->ic = p->with_conc ? S->nord : -1;
+   S->iq = p->evolve_quat ? S->nord + S->ncs : -1;
    S->nv = (long)S->nc * S->G.ncg;
    S->nf = S->G.nfx + S->G.nfy + S->G.nfz;
    double** vs[] = {&S->y, &S->yprev, &S->ypred, &S->ycur, &S->ahist, &S->fcur,
@@ -152,6 +166,8 @@ OSim* o_sim_create(const ampe_model_params* p, const ampe_solver_params* s,
    S->n2 = vec(S->nf);
    S->Dc = vec(S->nf);
    S->Dphi = vec(S->nf);
+   S->Db = vec(S->nf);
+   S->ph6 = vec(6 * ng);
    S->Dconc = vec(S->nf);
    S->fc = vec(S->nf);
    S->fac_flux = vec(S->nf);
@@ -169,7 +185,7 @@ void o_sim_destroy(OSim* S)
                    S->del, S->ewt, S->work, S->work2, S->jv, S->xk, S->corr,
                    S->acor, S->cl, S->cs, S->clr, S->csr, S->Cphi, S->Mphi, S->sqrtm,
                    S->mq, S->fac_u, S->fac_e, S->fac_r, S->n2, S->Dc, S->Dphi,
-                   S->Dconc, S->fc, S->fac_flux, S->H, S->phid, S->z1, S->z2};
+                   S->Dconc, S->fc, S->fac_flux, S->H, S->phid, S->z1, S->z2, S->Db, S->ph6};
    for (size_t i = 0; i < sizeof(fs) / sizeof(fs[0]); i++) free(fs[i]);
    for (int i = 0; i <= S->s.maxl; i++) free(S->V[i]);
    free(S->V);
@@ -207,15 +223,19 @@ void o_sim_set_state(OSim* S, const double* phi, const double* c,
                      const double* q)
 {
    const OGeom* G = &S->G;
+   long N = (long)G->nx * G->ny * G->nz;
    memset(S->y, 0, sizeof(double) * S->nv);
-   load_field(G, S->y, phi);
-   if (S->ic >= 0) load_field(G, S->y + (long)S->ic * G->ncg, c);
+   for (int m = 0; m < S->nord; m++) load_field(G, S->y + (long)m * G->ncg, phi + (long)m * N);
+   for (int m = 0; m < S->ncs; m++)
+      load_field(G, S->y + (long)(S->ic + m) * G->ncg, c + (long)m * N);
    if (S->iq >= 0)
       for (int m = 0; m < S->p.qlen; m++)
          load_field(G, S->y + (long)(S->iq + m) * G->ncg,
                     q + (long)m * G->nx * G->ny * G->nz);
    o_ghost_fill(G, S->y, S->nc);
-   if (S->ic >= 0) {
+   if (S->tern) {
+      o_kks_tern3(G, &S->p, S->nord, S->y, S->y + (long)S->ic * G->ncg, S->ph6);
+   } else if (S->ic >= 0) {
       /* initial phase concentrations: KKS from the guess cL = cS = c */
       const double* cc = S->y + (long)S->ic * G->ncg;
       memcpy(S->clr, cc, sizeof(double) * G->ncg);
@@ -236,13 +256,19 @@ void o_sim_get_state(OSim* S, double* phi, double* c, double* q, double* cl,
 {
    const OGeom* G = &S->G;
    long N = (long)G->nx * G->ny * G->nz;
-   if (phi) store_field(G, phi, S->y);
-   if (c && S->ic >= 0) store_field(G, c, S->y + (long)S->ic * G->ncg);
+   if (phi)
+      for (int m = 0; m < S->nord; m++) store_field(G, phi + (long)m * N, S->y + (long)m * G->ncg);
+   if (c)
+      for (int m = 0; m < S->ncs; m++)
+         store_field(G, c + (long)m * N, S->y + (long)(S->ic + m) * G->ncg);
    if (q && S->iq >= 0)
       for (int m = 0; m < S->p.qlen; m++)
          store_field(G, q + (long)m * N, S->y + (long)(S->iq + m) * G->ncg);
-   if (cl && S->ic >= 0) store_field(G, cl, S->cl);
-   if (cs && S->ic >= 0) store_field(G, cs, S->cs);
+   /* the three-phase model: cl = liquid, cs = solid A, two species each */
+   for (int m = 0; m < (S->tern ? 2 : 1) && S->ic >= 0; m++) {
+      if (cl) store_field(G, cl + (long)m * N, S->tern ? S->ph6 + (long)m * G->ncg : S->cl);
+      if (cs) store_field(G, cs + (long)m * N, S->tern ? S->ph6 + (long)(2 + m) * G->ncg : S->cs);
+   }
 }
 
 /* evaluateRHSFunction (source/QuatIntegrator.cc:3134-3298).  y is ghost
@@ -264,6 +290,21 @@ static int rhs_eval(OSim* S, double* y, double* ydot, int fd)
    const double* phi = y;
    const double* q = (S->iq >= 0) ? y + (long)S->iq * ng : NULL;
    const double* c = (S->ic >= 0) ? y + (long)S->ic * ng : NULL;
+   if (S->tern) {
+      /* three phases, two species: closed-form KKS, multi-order phase RHS with
+       * the driving force, EBS composition RHS of the three phases */
+      o_kks_tern3(G, p, S->nord, phi, c, S->ph6);
+      S->st.kks_calls++;
+      S->st.kks_cells += ng;
+      o_phase_rhs_tern3(G, p, S->nord, phi, S->ph6, NULL, ydot);
+      if (recompute) o_ebs3_coeffs(G, p, S->nord, phi, S->Dc, S->Dphi, S->Db);
+      o_conc_rhs_ebs3(G, p, S->ph6, S->Dc, S->Dphi, S->Db, ydot + (long)S->ic * ng);
+      return 0;
+   }
+   if (S->nord > 1) {
+      o_phase_rhs_multiorder(G, p, S->nord, phi, NULL, ydot);
+      return 0;
+   }
    if (q && recompute) o_quat_face_norm2(G, p, q, S->n2);
    if (c) {
       int it = o_kks_solve(G, p, phi, c, S->clr, S->csr, S->cl, S->cs);
@@ -308,7 +349,11 @@ static int psetup(OSim* S, double* y, double gamma)
    const double* q = (S->iq >= 0) ? y + (long)S->iq * ng : NULL;
    const double* c = (S->ic >= 0) ? y + (long)S->ic * ng : NULL;
    if (q) o_quat_face_norm2(G, p, q, S->n2);
-   if (c) {
+   if (S->tern) {
+      o_kks_tern3(G, p, S->nord, phi, c, S->ph6);
+      S->st.kks_calls++;
+      S->st.kks_cells += ng;
+   } else if (c) {
       int it = o_kks_solve(G, p, phi, c, S->clr, S->csr, S->cl, S->cs);
       S->st.kks_calls++;
       S->st.kks_cells += ng;
@@ -316,10 +361,15 @@ static int psetup(OSim* S, double* y, double gamma)
       S->st.kks_iters += it;
    }
    /* phase block: M = M_phi, C = 1 + gamma M w g'', D = -gamma eps^2
-    * (source/PhaseFACOps.cc:33-51) */
+    * (source/PhaseFACOps.cc:33-51); with several orders the well scale
+    * passed is 0 (source/QuatIntegrator.cc:3321-3329), so C = 1 */
    int kim = p->phi_mobility_type == AMPE_PHI_MOB_KIM;
    if (kim) o_phase_mobility_field(G, p, S->cl, S->cs, S->Mphi);
-   o_phase_fac_C(G, p, gamma, phi, kim ? S->Mphi : NULL, S->Cphi);
+   {
+      ampe_model_params pw = *p;
+      if (S->nord > 1) pw.phi_well_scale = 0.0;
+      o_phase_fac_C(G, &pw, gamma, phi, kim ? S->Mphi : NULL, S->Cphi);
+   }
    /* composition block: D = -gamma D_c (lagged), C = 1, M = M_c
     * (source/ConcFACOps.cc:28-61) */
    if (c)
@@ -328,6 +378,7 @@ static int psetup(OSim* S, double* y, double gamma)
           * setDiffusionCoeffForPreconditionerOnPatch) */
          double d = (p->conc_rhs_form == AMPE_CONC_RHS_EBS) ? (0.0 + S->Dc[f]) + S->Dphi[f]
                                                               : S->Dc[f];
+         if (S->tern) d = d + S->Db[f];
          S->Dconc[f] = -gamma * d;
       }
    /* orientation block: face coefficients and sqrt(M_q) with ghosts
@@ -504,14 +555,18 @@ static int psolve(OSim* S, const double* r, double* z, double delta)
                           p->phi_mobility_type == AMPE_PHI_MOB_KIM ? S->Mphi : NULL,
                           p->phi_mobility,
                           {G->dx[0], G->dx[1], G->dx[2]}};
-   if (fac_solve(S, &ph, r, z, delta)) ret = 1;
+   /* one phase solve per order (PhasePrecondSolve, :3427-3444) */
+   for (int m = 0; m < S->nord; m++)
+      if (fac_solve(S, &ph, r + (long)m * ng, z + (long)m * ng, delta)) ret = 1;
    if (S->iq >= 0) {
       if (quat_solve(S, r + (long)S->iq * ng, z + (long)S->iq * ng, delta)) ret = 1;
    }
-   if (S->ic >= 0) {
+   /* composition: one block per species with its own diffusion (ConcFACOps
+    * depth ic, source/ConcFACOps.cc:38-56; the ternary diagonals are equal) */
+   for (int m = 0; m < S->ncs; m++) {
       ampe_elliptic_op co = {S->Dconc, 0.0, NULL, 1.0, NULL, p->conc_mobility,
                              {G->dx[0], G->dx[1], G->dx[2]}};
-      if (fac_solve(S, &co, r + (long)S->ic * ng, z + (long)S->ic * ng, delta))
+      if (fac_solve(S, &co, r + (long)(S->ic + m) * ng, z + (long)(S->ic + m) * ng, delta))
          ret = 1;
    }
    return ret;
@@ -1288,7 +1343,9 @@ OSim* o_sim_create_hier(const ampe_model_params* p, const ampe_solver_params* s,
 {
    if (p->with_conc && p->conc_rhs_form != AMPE_CONC_RHS_KKS) return NULL;
    if (p->evolve_quat && p->qlen > 4) return NULL;
+   if (p->norderp > 1) return NULL;
    OSim* S = o_sim_create(p, s, 1, 1, 1);
+   if (!S) return NULL;
    /* resize every field to the composite */
    long ng = o_hier_ncg(H), nf = o_hier_nf(H);
    S->Hh = H;
