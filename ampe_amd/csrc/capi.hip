@@ -677,6 +677,31 @@ int ampe_refine_add(const ampe_level* fine, const ampe_level* coarse, int ncomp,
    return ampe::mg_prolong_add<double>(*fine->L, *coarse->L, ncomp, order, c, f, hs(stream));
 }
 
+int ampe_quat_residual_restrict(const ampe_level* coarse, const ampe_level* fine,
+                                const double* fc, const double* sm, double gamma,
+                                const double dx[3], int ncomp, const double* rhs, double* z,
+                                double* res, double* b, int red_only, ampe_stream_t stream)
+{
+   AMPE_REQUIRE(coarse && fine && fc && sm && rhs && z && b && dx, "null argument");
+   AMPE_REQUIRE(ncomp >= 1 && ncomp <= 4, "ncomp must be in [1, 4]");
+   AMPE_REQUIRE(ampe::is_coarsening(*coarse->L, *fine->L),
+                "coarse level is not the fine level halved");
+   const Level& F = *fine->L;
+   hipStream_t s = hs(stream);
+   if (ampe::ghost_fill(F, z, ncomp, s)) return -1;
+   if (red_only) {
+      AMPE_REQUIRE(ampe::red_residual_ok(F, ncomp),
+                   "red_only needs a periodic fine level with even, non-negative patch "
+                   "corners and one, two or four components");
+      return ampe::launch_quat_resrestrict<double>(*coarse->L, F, fc, sm, gamma, dx, ncomp, rhs,
+                                                   z, b, s);
+   }
+   AMPE_REQUIRE(res, "null residual buffer");
+   if (ampe::launch_quat_residual<double>(F, fc, sm, gamma, dx, ncomp, rhs, z, res, nullptr, s))
+      return -1;
+   return ampe::mg_restrict_cell<double>(*coarse->L, F, ncomp, res, b, s);
+}
+
 int ampe_sim_set_surrogate(ampe_sim* sim, const ampe_surrogate* s, int probe_its,
                            double probe_tol, double switch_tol)
 {

@@ -30,14 +30,27 @@ template <typename T>
 int launch_residual(const Level& L, const EllOp<T>& op, const double* dx, const T* rhs,
                     const T* u, T* res, double* part, hipStream_t s);
 
+// Orientation block: one colour of the red-black sweep, and the residual.
+// split: the component blocks of rhs, z and res are stored in the row
+// parity-split layout (split_col; fc and sm stay natural).
 template <typename T>
 int launch_quat_rbgs(const Level& L, const T* fc, const T* sm, double gamma, const double* dx,
-                     int ncomp, const T* rhs, T* z, int color, hipStream_t s);
+                     int ncomp, const T* rhs, T* z, int color, hipStream_t s, bool split = false);
 
 template <typename T>
 int launch_quat_residual(const Level& L, const T* fc, const T* sm, double gamma,
                          const double* dx, int ncomp, const T* rhs, const T* z, T* res,
-                         double* part, hipStream_t s);
+                         double* part, hipStream_t s, bool split = false);
+
+// b (on coarse, the fine boxes halved) = the restriction of the residual
+// rhs - A z on fine when z comes out of red-black sweeps ending with the black
+// colour: the red cells' residuals alone (the black ones vanish), summed per
+// coarse cell.  One, two or four components; valid on periodic levels whose
+// colouring is one checkerboard (the caller checks).
+template <typename T>
+int launch_quat_resrestrict(const Level& coarse, const Level& fine, const T* fc, const T* sm,
+                            double gamma, const double* dx, int ncomp, const T* rhs, const T* z,
+                            T* b, hipStream_t s, bool split = false);
 
 // ------------------------------------------------------------ vectors
 // Flat element-wise ops over n entries (whole ghosted vectors).
@@ -67,13 +80,15 @@ int vec_sqrt(long long n, const double* x, double* y, hipStream_t s);
 // Preconditioner input / output over nc component blocks (orientation blocks
 // [iq, iq+ql) scaled by 1/sqrt(M) on input, sqrt(M) on output); psolve_in
 // also zeroes u.  With f null (FP64) only the orientation blocks go to fq.
-// psolve_out multiplies the result by ewt when ewt is not null.
+// psolve_out multiplies the result by ewt when ewt is not null.  split (a
+// level): the orientation blocks of f / fq / u are in that level's row
+// parity-split layout.
 template <typename T>
 int psolve_in(long long ng, int nc, int iq, int ql, const double* r, const double* sm, T* f,
-              T* fq, T* u, hipStream_t s);
+              T* fq, T* u, hipStream_t s, const Level* split = nullptr);
 template <typename T>
 int psolve_out(long long ng, int nc, int iq, int ql, const T* u, const double* sm,
-               const double* ewt, double* z, hipStream_t s);
+               const double* ewt, double* z, hipStream_t s, const Level* split = nullptr);
 int vec_scale_comp(long long ng, int ncomp, const double* sm, double* x, hipStream_t s);
 // Reductions over interior cells of ncomp component blocks (host result,
 // synchronises s).

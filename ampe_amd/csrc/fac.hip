@@ -313,15 +313,31 @@ struct QuatRowVals {
    T zc, zl[NDIM], zh[NDIM];
 };
 
-template <typename T, int NDIM>
-__device__ __forceinline__ void quat_row_load(const CellPos& c, const T* __restrict__ zm,
-                                              QuatRowVals<T, NDIM>& v)
+// Where one cell's component values sit: the cell and its two neighbours in
+// the row (rows above and below, planes before and after: the cell's index
+// -+ the row / plane stride).  SPLIT: the row parity-split layout.
+struct QuatIdx {
+   long long c, xl, xh;
+};
+
+template <bool SPLIT>
+__device__ __forceinline__ QuatIdx quat_idx(const LevelArgs& a, const CellPos& c)
 {
-   v.zc = zm[c.c0];
+   if (SPLIT) return {split_at(a, c, 0), split_at(a, c, -1), split_at(a, c, 1)};
+   return {c.c0, c.c0 - 1, c.c0 + 1};
+}
+
+template <typename T, int NDIM>
+__device__ __forceinline__ void quat_row_load(const CellPos& c, const QuatIdx& q,
+                                              const T* __restrict__ zm, QuatRowVals<T, NDIM>& v)
+{
+   v.zc = zm[q.c];
+   v.zl[0] = zm[q.xl];
+   v.zh[0] = zm[q.xh];
 #pragma unroll
-   for (int d = 0; d < NDIM; d++) {
-      v.zl[d] = zm[c.c0 - c.st[d]];
-      v.zh[d] = zm[c.c0 + c.st[d]];
+   for (int d = 1; d < NDIM; d++) {
+      v.zl[d] = zm[q.c - c.st[d]];
+      v.zh[d] = zm[q.c + c.st[d]];
    }
 }
 
@@ -338,39 +354,41 @@ __device__ __forceinline__ T quat_row_apply(const QuatRowCoef<T>& w, const QuatR
 // Gauss-Seidel update of one cell, every component.  QL > 0: QL components,
 // all loads issued before the first update (the updates write only this
 // cell, which no other load of the cell reads); QL = 0: ncomp components one
-// after the other.
-template <typename T, int NDIM, int QL>
+// after the other.  SPLIT: z and rhs in the row parity-split layout (fc and
+// sm natural).
+template <typename T, int NDIM, int QL, bool SPLIT>
 __device__ __forceinline__ void quat_rbgs_cell(const LevelArgs& a, const CellPos& c,
                                                const T* __restrict__ fc, const T* __restrict__ sm,
                                                T gamma, const T* hinv2, int ncomp,
                                                const T* __restrict__ rhs, T* __restrict__ z)
 {
    const QuatRowCoef<T> w = quat_row_coef<T, NDIM>(a, c, fc, sm, gamma, hinv2);
+   const QuatIdx q = quat_idx<SPLIT>(a, c);
    if (QL > 0) {
       QuatRowVals<T, NDIM> v[QL > 0 ? QL : 1];
       T rv[QL > 0 ? QL : 1];
 #pragma unroll
       for (int m = 0; m < QL; m++) {
-         quat_row_load<T, NDIM>(c, z + m * a.ncg, v[m]);
-         rv[m] = rhs[m * a.ncg + c.c0];
+         quat_row_load<T, NDIM>(c, q, z + m * a.ncg, v[m]);
+         rv[m] = rhs[m * a.ncg + q.c];
       }
 #pragma unroll
       for (int m = 0; m < QL; m++) {
          T res = rv[m] - quat_row_apply<T, NDIM>(w, v[m]);
-         z[m * a.ncg + c.c0] = v[m].zc + res / w.center;
+         z[m * a.ncg + q.c] = v[m].zc + res / w.center;
       }
    } else {
       for (int m = 0; m < ncomp; m++) {
          QuatRowVals<T, NDIM> v;
-         quat_row_load<T, NDIM>(c, z + m * a.ncg, v);
-         T res = rhs[m * a.ncg + c.c0] - quat_row_apply<T, NDIM>(w, v);
-         z[m * a.ncg + c.c0] = v.zc + res / w.center;
+         quat_row_load<T, NDIM>(c, q, z + m * a.ncg, v);
+         T res = rhs[m * a.ncg + q.c] - quat_row_apply<T, NDIM>(w, v);
+         z[m * a.ncg + q.c] = v.zc + res / w.center;
       }
    }
 }
 
 // One colour of the sweep on the colour-compact mapping.
-template <typename T, int NDIM, int QL>
+template <typename T, int NDIM, int QL, bool SPLIT>
 __global__ void __launch_bounds__(128) k_quat_rbgs_c(LevelArgs a, const T* __restrict__ fc,
                                                      const T* __restrict__ sm, T gamma, T hi0,
                                                      T hi1, T hi2, int ncomp,
@@ -381,7 +399,7 @@ __global__ void __launch_bounds__(128) k_quat_rbgs_c(LevelArgs a, const T* __res
    for (int kk = 0; kk < a.nz_tile; kk++) {
       CellPos c;
       if (cell_pos_color<NDIM>(a, kk, color, c))
-         quat_rbgs_cell<T, NDIM, QL>(a, c, fc, sm, gamma, hinv2, ncomp, rhs, z);
+         quat_rbgs_cell<T, NDIM, QL, SPLIT>(a, c, fc, sm, gamma, hinv2, ncomp, rhs, z);
    }
 }
 
@@ -530,7 +548,47 @@ __global__ void __launch_bounds__(256) k_quat_rbgs_shell(LevelArgs a, const T* _
    }
 }
 
-template <typename T, int NDIM, int QL>
+// Residual of one cell, every component (QL as in quat_rbgs_cell); returns
+// the sum of the squared residuals (double).
+template <typename T, int NDIM, int QL, bool SPLIT>
+__device__ __forceinline__ double quat_residual_cell(const LevelArgs& a, const CellPos& c,
+                                                     const T* __restrict__ fc,
+                                                     const T* __restrict__ sm, T gamma,
+                                                     const T* hinv2, int ncomp,
+                                                     const T* __restrict__ rhs,
+                                                     const T* __restrict__ z,
+                                                     T* __restrict__ res)
+{
+   double sq = 0.0;
+   const QuatRowCoef<T> w = quat_row_coef<T, NDIM>(a, c, fc, sm, gamma, hinv2);
+   const QuatIdx q = quat_idx<SPLIT>(a, c);
+   if (QL > 0) {
+      QuatRowVals<T, NDIM> v[QL > 0 ? QL : 1];
+      T rv[QL > 0 ? QL : 1];
+#pragma unroll
+      for (int m = 0; m < QL; m++) {
+         quat_row_load<T, NDIM>(c, q, z + m * a.ncg, v[m]);
+         rv[m] = rhs[m * a.ncg + q.c];
+      }
+#pragma unroll
+      for (int m = 0; m < QL; m++) {
+         T r = rv[m] - quat_row_apply<T, NDIM>(w, v[m]);
+         res[m * a.ncg + q.c] = r;
+         sq += (double)r * (double)r;
+      }
+   } else {
+      for (int m = 0; m < ncomp; m++) {
+         QuatRowVals<T, NDIM> v;
+         quat_row_load<T, NDIM>(c, q, z + m * a.ncg, v);
+         T r = rhs[m * a.ncg + q.c] - quat_row_apply<T, NDIM>(w, v);
+         res[m * a.ncg + q.c] = r;
+         sq += (double)r * (double)r;
+      }
+   }
+   return sq;
+}
+
+template <typename T, int NDIM, int QL, bool SPLIT>
 __global__ void __launch_bounds__(256) k_quat_residual(LevelArgs a, const T* __restrict__ fc,
                                                        const T* __restrict__ sm, T gamma, T hi0,
                                                        T hi1, T hi2, int ncomp,
@@ -539,34 +597,93 @@ __global__ void __launch_bounds__(256) k_quat_residual(LevelArgs a, const T* __r
                                                        double* __restrict__ partial)
 {
    double sq = 0.0;
-   AMPE_COLUMN_BEGIN(a, c)
    const T hinv2[3] = {hi0, hi1, hi2};
-   const QuatRowCoef<T> w = quat_row_coef<T, NDIM>(a, c, fc, sm, gamma, hinv2);
-   if (QL > 0) {
-      QuatRowVals<T, NDIM> v[QL > 0 ? QL : 1];
-      T rv[QL > 0 ? QL : 1];
-#pragma unroll
-      for (int m = 0; m < QL; m++) {
-         quat_row_load<T, NDIM>(c, z + m * a.ncg, v[m]);
-         rv[m] = rhs[m * a.ncg + c.c0];
-      }
-#pragma unroll
-      for (int m = 0; m < QL; m++) {
-         T r = rv[m] - quat_row_apply<T, NDIM>(w, v[m]);
-         res[m * a.ncg + c.c0] = r;
-         sq += (double)r * (double)r;
-      }
-   } else {
-      for (int m = 0; m < ncomp; m++) {
-         QuatRowVals<T, NDIM> v;
-         quat_row_load<T, NDIM>(c, z + m * a.ncg, v);
-         T r = rhs[m * a.ncg + c.c0] - quat_row_apply<T, NDIM>(w, v);
-         res[m * a.ncg + c.c0] = r;
-         sq += (double)r * (double)r;
-      }
-   }
+   AMPE_COLUMN_BEGIN(a, c)
+   sq += quat_residual_cell<T, NDIM, QL, SPLIT>(a, c, fc, sm, gamma, hinv2, ncomp, rhs, z, res);
    AMPE_COLUMN_END
    if (partial) block_store_sum<T>(sq, partial);
+}
+
+// The residual in the split layout without the reduction, on the pair
+// mapping: a half-wave's cells have one column parity, so its split-layout
+// loads and stores cover contiguous runs.
+template <typename T, int NDIM, int QL>
+__global__ void __launch_bounds__(128) k_quat_residual_p(LevelArgs a, const T* __restrict__ fc,
+                                                         const T* __restrict__ sm, T gamma,
+                                                         T hi0, T hi1, T hi2, int ncomp,
+                                                         const T* __restrict__ rhs,
+                                                         const T* __restrict__ z,
+                                                         T* __restrict__ res)
+{
+   const T hinv2[3] = {hi0, hi1, hi2};
+   for (int kk = 0; kk < a.nz_tile; kk++)
+      for (int io = 0; io < 2; io++) {
+         CellPos c;
+         if (cell_pos_pair(a, kk, io, c))
+            (void)quat_residual_cell<T, NDIM, QL, true>(a, c, fc, sm, gamma, hinv2, ncomp, rhs,
+                                                        z, res);
+      }
+}
+
+// Red residuals restricted.  After a pre-smoothing that ends with the black
+// colour, the residual vanishes on black cells in exact arithmetic (a black
+// update zeroes its own row's residual, and a black row couples its cell to
+// red cells only), so the coarse right-hand side is the restriction of the red
+// residuals: per coarse cell, its 2^(NDIM-1) red children's residuals in the
+// restriction's summation order (each pair of x-children sums to its red
+// member), without storing the fine residual.  Needs one checkerboard across
+// patches and periodic ghosts (a zero-slope ghost would copy a black cell's
+// new value into its own row); the host checks both.
+template <typename T, int NDIM, int QL, bool SPLIT>
+__global__ void __launch_bounds__(256) k_quat_resrestrict(LevelArgs ca, LevelArgs fa,
+                                                          const T* __restrict__ fc,
+                                                          const T* __restrict__ sm, T gamma,
+                                                          T hi0, T hi1, T hi2,
+                                                          const T* __restrict__ rhs,
+                                                          const T* __restrict__ z,
+                                                          T* __restrict__ b)
+{
+   CellPos p;
+   if (!cell_pos(ca, 0, p)) return;
+   const T hinv2[3] = {hi0, hi1, hi2};
+   const PatchDev fp = fa.P[ca.T[tile_index(ca)].patch];
+   T pr[QL][4];
+#pragma unroll
+   for (int dz = 0; dz < (NDIM == 3 ? 2 : 1); dz++)
+#pragma unroll
+      for (int dy = 0; dy < 2; dy++) {
+         CellPos c;
+         c.pd = fp;
+         c.j = 2 * p.j + dy;
+         c.k = NDIM == 3 ? 2 * p.k + dz : 0;
+         int gj = fp.lo[1] + c.j;
+         int x = (NDIM == 3) ? fp.lo[0] + gj + (fp.lo[2] + c.k) : fp.lo[1] + gj;
+         int io = ((x - (x / 2) * 2) != 0) ? 1 : 0;  // red cells of the row (cell_pos_color)
+         c.i = 2 * p.i + io;
+         c.st[0] = 1;
+         c.st[1] = fp.sy;
+         c.st[2] = fp.sz;
+         c.c0 = fp.cell_off + (c.i + fa.g) + (long long)(c.j + fa.g) * fp.sy +
+                (long long)(c.k + fa.gz) * fp.sz;
+         const QuatRowCoef<T> w = quat_row_coef<T, NDIM>(fa, c, fc, sm, gamma, hinv2);
+         const QuatIdx q = quat_idx<SPLIT>(fa, c);
+         QuatRowVals<T, NDIM> v[QL];
+         T rv[QL];
+#pragma unroll
+         for (int m = 0; m < QL; m++) {
+            quat_row_load<T, NDIM>(c, q, z + m * fa.ncg, v[m]);
+            rv[m] = rhs[m * fa.ncg + q.c];
+         }
+#pragma unroll
+         for (int m = 0; m < QL; m++) pr[m][dy + 2 * dz] = rv[m] - quat_row_apply<T, NDIM>(w, v[m]);
+      }
+   const long long co = SPLIT ? split_at(ca, p, 0) : p.c0;
+#pragma unroll
+   for (int m = 0; m < QL; m++) {
+      T s = pr[m][0] + pr[m][1];
+      if (NDIM == 3) s = s + (pr[m][2] + pr[m][3]);
+      b[m * ca.ncg + co] = s * (NDIM == 3 ? T(0.125) : T(0.25));
+   }
 }
 
 // ------------------------------------------------------------ launchers
@@ -679,7 +796,7 @@ int launch_residual(const Level& L, const EllOp<T>& op, const double* dx, const 
 
 template <typename T>
 int launch_quat_rbgs(const Level& L, const T* fc, const T* sm, double gamma, const double* dx,
-                     int ncomp, const T* rhs, T* z, int color, hipStream_t s)
+                     int ncomp, const T* rhs, T* z, int color, hipStream_t s, bool split)
 {
    LevelArgs la;
    int grid;
@@ -688,9 +805,13 @@ int launch_quat_rbgs(const Level& L, const T* fc, const T* sm, double gamma, con
    recip2(dx, hi);
    KernelTimer::Span span(KernelTimer::QUAT_RBGS, s, quat_rbgs_bytes(L, ncomp, sizeof(T)));
    auto go = [&](auto ndim, auto ql) {
-      hipLaunchKernelGGL((k_quat_rbgs_c<T, decltype(ndim)::value, decltype(ql)::value>),
-                         dim3(grid), cblock(), 0, s, la, fc, sm, (T)gamma, hi[0], hi[1], hi[2],
-                         ncomp, rhs, z, color);
+      constexpr int ND = decltype(ndim)::value, Q = decltype(ql)::value;
+      if (split)
+         hipLaunchKernelGGL((k_quat_rbgs_c<T, ND, Q, true>), dim3(grid), cblock(), 0, s, la, fc,
+                            sm, (T)gamma, hi[0], hi[1], hi[2], ncomp, rhs, z, color);
+      else
+         hipLaunchKernelGGL((k_quat_rbgs_c<T, ND, Q, false>), dim3(grid), cblock(), 0, s, la, fc,
+                            sm, (T)gamma, hi[0], hi[1], hi[2], ncomp, rhs, z, color);
    };
    quat_dispatch(L.ndim, ncomp, go);
    AMPE_HIP_CHECK(hipGetLastError());
@@ -736,7 +857,7 @@ int launch_quat_rbgs_fused(const Level& L, const T* fc, const T* sm, double gamm
 template <typename T>
 int launch_quat_residual(const Level& L, const T* fc, const T* sm, double gamma,
                          const double* dx, int ncomp, const T* rhs, const T* z, T* res,
-                         double* part, hipStream_t s)
+                         double* part, hipStream_t s, bool split)
 {
    LevelArgs la;
    int grid;
@@ -744,11 +865,48 @@ int launch_quat_residual(const Level& L, const T* fc, const T* sm, double gamma,
    T hi[3];
    recip2(dx, hi);
    auto go = [&](auto ndim, auto ql) {
-      hipLaunchKernelGGL((k_quat_residual<T, decltype(ndim)::value, decltype(ql)::value>),
-                         dim3(grid), tblock(), 0, s, la, fc, sm, (T)gamma, hi[0], hi[1], hi[2],
-                         ncomp, rhs, z, res, part);
+      constexpr int ND = decltype(ndim)::value, Q = decltype(ql)::value;
+      if (split && !part)
+         hipLaunchKernelGGL((k_quat_residual_p<T, ND, Q>), dim3(grid), cblock(), 0, s, la, fc,
+                            sm, (T)gamma, hi[0], hi[1], hi[2], ncomp, rhs, z, res);
+      else if (split)
+         hipLaunchKernelGGL((k_quat_residual<T, ND, Q, true>), dim3(grid), tblock(), 0, s, la, fc,
+                            sm, (T)gamma, hi[0], hi[1], hi[2], ncomp, rhs, z, res, part);
+      else
+         hipLaunchKernelGGL((k_quat_residual<T, ND, Q, false>), dim3(grid), tblock(), 0, s, la,
+                            fc, sm, (T)gamma, hi[0], hi[1], hi[2], ncomp, rhs, z, res, part);
    };
    quat_dispatch(L.ndim, ncomp, go);
+   AMPE_HIP_CHECK(hipGetLastError());
+   return 0;
+}
+
+template <typename T>
+int launch_quat_resrestrict(const Level& coarse, const Level& fine, const T* fc, const T* sm,
+                            double gamma, const double* dx, int ncomp, const T* rhs, const T* z,
+                            T* b, hipStream_t s, bool split)
+{
+   if (ncomp != 1 && ncomp != 2 && ncomp != 4) {
+      set_error("launch_quat_resrestrict: one, two or four components");
+      return -1;
+   }
+   T hi[3];
+   recip2(dx, hi);
+   LevelArgs ca = level_args(coarse), fa = level_args(fine);
+   auto go = [&](auto ndim, auto ql) {
+      constexpr int ND = decltype(ndim)::value, Q = decltype(ql)::value;
+      if constexpr (Q > 0) {
+         if (split)
+            hipLaunchKernelGGL((k_quat_resrestrict<T, ND, Q, true>), dim3(coarse.ncell_tiles),
+                               tblock(), 0, s, ca, fa, fc, sm, (T)gamma, hi[0], hi[1], hi[2],
+                               rhs, z, b);
+         else
+            hipLaunchKernelGGL((k_quat_resrestrict<T, ND, Q, false>), dim3(coarse.ncell_tiles),
+                               tblock(), 0, s, ca, fa, fc, sm, (T)gamma, hi[0], hi[1], hi[2],
+                               rhs, z, b);
+      }
+   };
+   quat_dispatch(fine.ndim, ncomp, go);
    AMPE_HIP_CHECK(hipGetLastError());
    return 0;
 }
@@ -759,10 +917,13 @@ int launch_quat_residual(const Level& L, const T* fc, const T* sm, double gamma,
    template int launch_residual<T>(const Level&, const EllOp<T>&, const double*, const T*, \
                                    const T*, T*, double*, hipStream_t);                  \
    template int launch_quat_rbgs<T>(const Level&, const T*, const T*, double, const double*, \
-                                    int, const T*, T*, int, hipStream_t);                \
+                                    int, const T*, T*, int, hipStream_t, bool);          \
    template int launch_quat_residual<T>(const Level&, const T*, const T*, double,         \
                                         const double*, int, const T*, const T*, T*,      \
-                                        double*, hipStream_t);                           \
+                                        double*, hipStream_t, bool);                     \
+   template int launch_quat_resrestrict<T>(const Level&, const Level&, const T*, const T*, \
+                                           double, const double*, int, const T*,         \
+                                           const T*, T*, hipStream_t, bool);             \
    template int launch_quat_rbgs_fused<T>(const Level&, const T*, const T*, double,       \
                                           const double*, int, const T*, const T*, T*,    \
                                           hipStream_t);

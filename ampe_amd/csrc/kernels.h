@@ -98,6 +98,63 @@ __device__ __forceinline__ bool cell_pos_col(const LevelArgs& a, int kk, CellPos
    return true;
 }
 
+// Pair mapping of a tile onto a workgroup of TX/2 x TY threads: thread
+// (tx, ty) takes the cell 2 tx + io of its row, so the lanes of a half-wave
+// hold cells of one column parity (io = 0 and 1 in turn cover the tile).
+// Plane t.k + kk; false when outside the box.
+__device__ __forceinline__ bool cell_pos_pair(const LevelArgs& a, int kk, int io, CellPos& c)
+{
+   const Tile t = a.T[tile_index(a)];
+   c.pd = a.P[t.patch];
+   c.i = t.i0 + 2 * (int)threadIdx.x + io;  // t.i0 is a multiple of TX (even)
+   c.j = t.j0 + (int)threadIdx.y;
+   c.k = t.k + kk;
+   if (c.i >= c.pd.nx || c.j >= c.pd.ny || c.k >= c.pd.nz) return false;
+   c.st[0] = 1;
+   c.st[1] = c.pd.sy;
+   c.st[2] = c.pd.sz;
+   c.c0 = c.pd.cell_off + (c.i + a.g) + (long long)(c.j + a.g) * c.pd.sy +
+          (long long)(c.k + a.gz) * c.pd.sz;
+   return true;
+}
+
+// Row parity-split layout of a ghosted cell block: inside every row of sy
+// cells the even ghosted columns come first, then the odd ones, so the cells
+// of one red-black colour in a row are contiguous (a colour sweep writes and
+// reads its own cells in whole lines).  Rows, planes and patches stay where
+// the natural layout has them; only the position inside a row changes.
+// Position of ghosted column ig in its row:
+__host__ __device__ __forceinline__ int split_col(int ig, int sy)
+{
+   return (ig & 1) ? ((sy + 1) >> 1) + (ig >> 1) : (ig >> 1);
+}
+
+// Split-layout index of the cell di columns (-1, 0, 1) beside c.
+__device__ __forceinline__ long long split_at(const LevelArgs& a, const CellPos& c, int di)
+{
+   const int ig = c.i + a.g;
+   return c.c0 - ig + split_col(ig + di, c.pd.sy);
+}
+
+// Split-layout index of ghosted cell n of one component block (natural
+// index), the patch found by bisection over the ascending cell offsets.
+__device__ __forceinline__ long long split_index(const PatchDev* __restrict__ P, int np,
+                                                 long long n)
+{
+   int lo = 0, hi = np - 1;
+   while (lo < hi) {
+      int mid = (lo + hi + 1) >> 1;
+      if (P[mid].cell_off <= n)
+         lo = mid;
+      else
+         hi = mid - 1;
+   }
+   const long long loc = n - P[lo].cell_off;
+   const int sy = P[lo].sy;
+   const int ig = (int)(loc % sy);
+   return n - ig + split_col(ig, sy);
+}
+
 // Per-plane loop of a column kernel: the body runs once per plane of the
 // thread's column, with `c` the cell; `continue` inside the body moves to
 // the next plane.

@@ -15,6 +15,7 @@ struct PrecIn {
    float* f = nullptr;
    const double* sm = nullptr;
    int iq = 0, ql = 0;
+   int split = 0;  // orientation components in the row parity-split layout
 };
 
 // A Krylov vector read from the FP32 preconditioner solution u: v = ewt u,
@@ -24,6 +25,7 @@ struct PrecOut {
    const double* sm = nullptr;
    const double* ewt = nullptr;
    int iq = 0, ql = 0;
+   int split = 0;  // orientation components in the row parity-split layout
 };
 
 struct VecList {

@@ -88,7 +88,10 @@ __global__ void __launch_bounds__(256) k_gmres_finish(LevelArgs a, int nc, doubl
       out[i] = v;
       if (pin.f) {
          bool q = m >= pin.iq && m < pin.iq + pin.ql;
-         pin.f[i] = q ? (float)(v / pin.sm[c.c0]) : (float)v;
+         if (q)
+            pin.f[pin.split ? m * a.ncg + split_at(a, c, 0) : i] = (float)(v / pin.sm[c.c0]);
+         else
+            pin.f[i] = (float)v;
       }
    }
 }
@@ -221,8 +224,12 @@ __global__ void __launch_bounds__(256) k_cgs_dots(LevelArgs a, int nc, int ntile
          double v;
          if (src.u) {
             // k_psolve_out's arithmetic, then the ewt scaling
-            v = (double)src.u[i];
-            if (m >= src.iq && m < src.iq + src.ql) v = v * src.sm[c.c0];
+            if (m >= src.iq && m < src.iq + src.ql) {
+               v = (double)src.u[src.split ? m * a.ncg + split_at(a, c, 0) : i];
+               v = v * src.sm[c.c0];
+            } else {
+               v = (double)src.u[i];
+            }
             v = src.ewt[i] * v;
             out[i] = v;
          } else if (ewt) {

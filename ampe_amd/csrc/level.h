@@ -72,6 +72,10 @@ struct Level {
    long long* ghost_dst = nullptr;  // ghost gather (periodic / neighbours)
    long long* ghost_src = nullptr;
    long long nghost = 0;
+   // the same gather in the row parity-split layout (built on request by
+   // level_split_tables)
+   long long* ghost_dst_s = nullptr;
+   long long* ghost_src_s = nullptr;
    double* partial = nullptr;  // per-workgroup reduction partials (16 per tile)
    double* dred = nullptr;     // reduction results (16 doubles)
    double* hpinned = nullptr;  // pinned host scalars
@@ -99,10 +103,14 @@ int level_create_placed(int ndim, const ampe_box* boxes, int npatch, const Level
                         const std::vector<long long>& gd, const std::vector<long long>& gs,
                         Level** out);
 
-// Ghost gather for ncomp component blocks (double or float fields).
-int ghost_fill(const Level& L, double* f, int ncomp, hipStream_t s);
-int ghost_fill(const Level& L, float* f, int ncomp, hipStream_t s);
-int ghost_fill(const Level& L, int* f, int ncomp, hipStream_t s);
+// Ghost gather for ncomp component blocks (double or float fields); split:
+// the blocks are stored in the row parity-split layout (split_col), which
+// needs level_split_tables first.
+int ghost_fill(const Level& L, double* f, int ncomp, hipStream_t s, bool split = false);
+int ghost_fill(const Level& L, float* f, int ncomp, hipStream_t s, bool split = false);
+int ghost_fill(const Level& L, int* f, int ncomp, hipStream_t s, bool split = false);
+// Builds the split-layout ghost gather of L (once; later calls return 0).
+int level_split_tables(Level& L, hipStream_t s);
 
 // Fixed-order device reduction of the per-tile partials written by a cell
 // kernel: sum (op = 0) or max (op = 1) of nparts values (ncol columns

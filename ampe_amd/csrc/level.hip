@@ -1,5 +1,6 @@
 // Level layout, ghost gather and deterministic reductions.
 #include "level.h"
+#include "kernels.h"
 
 #include <algorithm>
 #include <cstring>
@@ -358,6 +359,8 @@ void level_destroy(Level* L)
    (void)hipFree(L->march_tiles);
    (void)hipFree(L->ghost_dst);
    (void)hipFree(L->ghost_src);
+   (void)hipFree(L->ghost_dst_s);
+   (void)hipFree(L->ghost_src_s);
    (void)hipFree(L->partial);
    (void)hipFree(L->dred);
    (void)hipHostFree(L->hpinned);
@@ -379,27 +382,60 @@ __global__ void __launch_bounds__(256) k_ghost_gather(T* __restrict__ f, long lo
 }
 
 template <typename T>
-static int ghost_fill_t(const Level& L, T* f, int ncomp, hipStream_t s)
+static int ghost_fill_t(const Level& L, T* f, int ncomp, hipStream_t s, bool split)
 {
    if (L.nghost == 0 || ncomp <= 0) return 0;
+   if (split && !L.ghost_dst_s) {
+      set_error("ghost_fill: the level has no split-layout gather");
+      return -1;
+   }
    dim3 grid((unsigned)((L.nghost + 255) / 256), (unsigned)ncomp);
    hipLaunchKernelGGL(k_ghost_gather<T>, grid, dim3(256), 0, s, f, L.ncg,
-                      L.ghost_dst, L.ghost_src, L.nghost);
+                      split ? L.ghost_dst_s : L.ghost_dst, split ? L.ghost_src_s : L.ghost_src,
+                      L.nghost);
    AMPE_HIP_CHECK(hipGetLastError());
    return 0;
 }
 
-int ghost_fill(const Level& L, double* f, int ncomp, hipStream_t s)
+int ghost_fill(const Level& L, double* f, int ncomp, hipStream_t s, bool split)
 {
-   return ghost_fill_t(L, f, ncomp, s);
+   return ghost_fill_t(L, f, ncomp, s, split);
 }
-int ghost_fill(const Level& L, float* f, int ncomp, hipStream_t s)
+int ghost_fill(const Level& L, float* f, int ncomp, hipStream_t s, bool split)
 {
-   return ghost_fill_t(L, f, ncomp, s);
+   return ghost_fill_t(L, f, ncomp, s, split);
 }
-int ghost_fill(const Level& L, int* f, int ncomp, hipStream_t s)
+int ghost_fill(const Level& L, int* f, int ncomp, hipStream_t s, bool split)
 {
-   return ghost_fill_t(L, f, ncomp, s);
+   return ghost_fill_t(L, f, ncomp, s, split);
+}
+
+__global__ void __launch_bounds__(256) k_split_map(const PatchDev* __restrict__ P, int np,
+                                                   const long long* __restrict__ in,
+                                                   long long* __restrict__ out, long long n)
+{
+   long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+   if (t < n) out[t] = split_index(P, np, in[t]);
+}
+
+int level_split_tables(Level& L, hipStream_t s)
+{
+   if (L.ghost_dst_s || L.nghost == 0) return 0;
+   for (int p = 1; p < L.npatch; p++)
+      if (L.hp[p].cell_off <= L.hp[p - 1].cell_off) {
+         set_error("level_split_tables: patch offsets are not ascending");
+         return -1;
+      }
+   size_t bytes = sizeof(long long) * (size_t)L.nghost;
+   AMPE_HIP_CHECK(hipMalloc(&L.ghost_dst_s, bytes));
+   AMPE_HIP_CHECK(hipMalloc(&L.ghost_src_s, bytes));
+   dim3 grid((unsigned)((L.nghost + 255) / 256));
+   hipLaunchKernelGGL(k_split_map, grid, dim3(256), 0, s, L.dp, L.npatch, L.ghost_dst,
+                      L.ghost_dst_s, L.nghost);
+   hipLaunchKernelGGL(k_split_map, grid, dim3(256), 0, s, L.dp, L.npatch, L.ghost_src,
+                      L.ghost_src_s, L.nghost);
+   AMPE_HIP_CHECK(hipGetLastError());
+   return 0;
 }
 
 // Single-workgroup fixed-order reduction of nparts x ncol partials

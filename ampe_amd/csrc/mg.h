@@ -37,6 +37,7 @@ template <typename T>
 struct Mg {
    std::vector<MgLevel<T>> lv;  // lv[0] is the fine level (not owned)
    int ncomp_max = 1;
+   bool qsplit = false;         // orientation vectors in the row parity-split layout
    int nlev() const { return (int)lv.size(); }
 };
 
@@ -78,18 +79,32 @@ template <typename T>
 int mg_vcycle_quat(Mg<T>& m, const T* fc0, const T* sm0, double gamma, int ncomp,
                    const MgCycle& cy, const T* b0, T* x0, hipStream_t s);
 
-// Transfer operators (exposed for tests through the C ABI).
+// Store the orientation block's vectors (b, x, r of every level, and the
+// fine-level b0 / x0 the caller passes) in the row parity-split layout
+// (split_col): one colour of a sweep then reads and writes whole lines.
+// Builds the levels' split ghost gathers; the one-pass sweeps are natural
+// only.
+template <typename T>
+int mg_enable_qsplit(Mg<T>& m, hipStream_t s);
+
+// Transfer operators (exposed for tests through the C ABI); split: f / c, xc
+// / xf in the row parity-split layout.
 template <typename T>
 int mg_restrict_cell(const Level& coarse, const Level& fine, int ncomp, const T* f, T* c,
-                     hipStream_t s);
+                     hipStream_t s, bool split = false);
 template <typename T>
 int mg_restrict_side(const Level& coarse, const Level& fine, const T* f, T* c, hipStream_t s);
 // order 0: constant refinement, 1: tri/bilinear (coarse ghosts filled).
 template <typename T>
 int mg_prolong_add(const Level& fine, const Level& coarse, int ncomp, int order, const T* xc,
-                   T* xf, hipStream_t s);
+                   T* xf, hipStream_t s, bool split = false);
 // coarse holds the boxes of fine halved, patch by patch (even corners/extents)
 bool is_coarsening(const Level& coarse, const Level& fine);
+// After red-black sweeps ending with the black colour, the orientation
+// residual vanishes on black cells when every black cell's neighbours are
+// red: one checkerboard over the patches (non-negative, even patch corners)
+// and periodic ghosts; one, two or four components.
+bool red_residual_ok(const Level& L, int ncomp);
 
 // Orientation field (ql <= 4 components) coarsened by the normalised volume
 // average, or refined by the normalised (bi/tri)linear interpolation (coarse

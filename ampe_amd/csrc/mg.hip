@@ -1,6 +1,7 @@
 // Geometric multigrid below one patch level (see mg.h).
 #include <algorithm>
 #include <cstring>
+#include <type_traits>
 
 #include "mg.h"
 
@@ -8,7 +9,9 @@ namespace ampe {
 
 // ------------------------------------------------------------ transfers
 // Coarse cell = mean of its 2^NDIM children (fixed summation order).
-template <typename T, int NDIM>
+// SPLIT: f and c in the row parity-split layout (the two children of a row
+// sit at the split positions of their columns; rows and planes as natural).
+template <typename T, int NDIM, bool SPLIT>
 __global__ void __launch_bounds__(256) k_restrict_cell(LevelArgs ca, LevelArgs fa, int ncomp,
                                                        const T* __restrict__ f,
                                                        T* __restrict__ c)
@@ -17,14 +20,18 @@ __global__ void __launch_bounds__(256) k_restrict_cell(LevelArgs ca, LevelArgs f
    if (!cell_pos(ca, 0, p)) return;
    const PatchDev fp = fa.P[ca.T[tile_index(ca)].patch];
    int fk = NDIM == 3 ? 2 * p.k : 0;
-   long long f0 = fp.cell_off + (2 * p.i + fa.g) + (long long)(2 * p.j + fa.g) * fp.sy +
-                  (long long)(fk + fa.gz) * fp.sz;
+   const int ig = 2 * p.i + fa.g;
+   const long long row = fp.cell_off + (long long)(2 * p.j + fa.g) * fp.sy +
+                         (long long)(fk + fa.gz) * fp.sz;
+   const long long f0 = SPLIT ? row + split_col(ig, fp.sy) : row + ig;
+   const long long f1 = SPLIT ? row + split_col(ig + 1, fp.sy) : row + ig + 1;
+   const long long co = SPLIT ? split_at(ca, p, 0) : p.c0;
    const long long sy = fp.sy, sz = fp.sz;
    for (int m = 0; m < ncomp; m++) {
       const T* fm = f + m * fa.ncg;
-      T s = (fm[f0] + fm[f0 + 1]) + (fm[f0 + sy] + fm[f0 + sy + 1]);
-      if (NDIM == 3) s = s + ((fm[f0 + sz] + fm[f0 + sz + 1]) + (fm[f0 + sz + sy] + fm[f0 + sz + sy + 1]));
-      c[m * ca.ncg + p.c0] = s * (NDIM == 3 ? T(0.125) : T(0.25));
+      T s = (fm[f0] + fm[f1]) + (fm[f0 + sy] + fm[f1 + sy]);
+      if (NDIM == 3) s = s + ((fm[f0 + sz] + fm[f1 + sz]) + (fm[f0 + sz + sy] + fm[f1 + sz + sy]));
+      c[m * ca.ncg + co] = s * (NDIM == 3 ? T(0.125) : T(0.25));
    }
 }
 
@@ -61,50 +68,123 @@ __global__ void __launch_bounds__(256) k_restrict_side(LevelArgs ca, LevelArgs f
 // Fine cell += coarse correction: ORDER 1 tri/bilinear interpolation (weights
 // 3/4 toward the parent, 1/4 toward the coarse neighbour on the cell's side in
 // each direction; the coarse field needs filled ghosts), ORDER 0 the parent
-// value (constant refinement).
-template <typename T, int NDIM, int ORDER>
-__global__ void __launch_bounds__(256) k_prolong_add(LevelArgs fa, LevelArgs ca, int ncomp,
-                                                     const T* __restrict__ xc,
-                                                     T* __restrict__ xf)
+// value (constant refinement).  SPLIT: xc and xf in the row parity-split
+// layout.
+template <typename T, int NDIM, int ORDER, bool SPLIT>
+__device__ __forceinline__ void prolong_cell(const LevelArgs& fa, const LevelArgs& ca,
+                                             const CellPos& p, int ncomp,
+                                             const T* __restrict__ xc, T* __restrict__ xf)
 {
-   CellPos p;
-   if (!cell_pos(fa, 0, p)) return;
    const PatchDev cp = ca.P[fa.T[tile_index(fa)].patch];
    int K = NDIM == 3 ? (p.k >> 1) : 0;
-   long long c0 = cp.cell_off + ((p.i >> 1) + ca.g) + (long long)((p.j >> 1) + ca.g) * cp.sy +
-                  (long long)(K + ca.gz) * cp.sz;
-   long long oi = (p.i & 1) ? 1 : -1;
+   const int ig = (p.i >> 1) + ca.g;
+   const long long row = cp.cell_off + (long long)((p.j >> 1) + ca.g) * cp.sy +
+                         (long long)(K + ca.gz) * cp.sz;
+   const int di = (p.i & 1) ? 1 : -1;
+   // the parent and its neighbour across the cell's side in the row
+   const long long c0 = SPLIT ? row + split_col(ig, cp.sy) : row + ig;
+   const long long c1 = SPLIT ? row + split_col(ig + di, cp.sy) : row + ig + di;
    long long oj = (p.j & 1) ? cp.sy : -cp.sy;
    long long ok = (NDIM == 3) ? ((p.k & 1) ? cp.sz : -cp.sz) : 0;
+   const long long fo = SPLIT ? split_at(fa, p, 0) : p.c0;
    for (int m = 0; m < ncomp; m++) {
       const T* x = xc + m * ca.ncg;
       T v;
       if (ORDER == 0) {
          v = x[c0];
       } else if (NDIM == 2) {
-         v = T(9.0 / 16.0) * x[c0] + T(3.0 / 16.0) * (x[c0 + oi] + x[c0 + oj]) +
-             T(1.0 / 16.0) * x[c0 + oi + oj];
+         v = T(9.0 / 16.0) * x[c0] + T(3.0 / 16.0) * (x[c1] + x[c0 + oj]) +
+             T(1.0 / 16.0) * x[c1 + oj];
       } else {
          v = T(27.0 / 64.0) * x[c0] +
-             T(9.0 / 64.0) * ((x[c0 + oi] + x[c0 + oj]) + x[c0 + ok]) +
-             T(3.0 / 64.0) * ((x[c0 + oi + oj] + x[c0 + oi + ok]) + x[c0 + oj + ok]) +
-             T(1.0 / 64.0) * x[c0 + oi + oj + ok];
+             T(9.0 / 64.0) * ((x[c1] + x[c0 + oj]) + x[c0 + ok]) +
+             T(3.0 / 64.0) * ((x[c1 + oj] + x[c1 + ok]) + x[c0 + oj + ok]) +
+             T(1.0 / 64.0) * x[c1 + oj + ok];
       }
       T* y = xf + m * fa.ncg;
-      y[p.c0] = y[p.c0] + v;
+      y[fo] = y[fo] + v;
+   }
+}
+
+template <typename T, int NDIM, int ORDER, bool SPLIT>
+__global__ void __launch_bounds__(256) k_prolong_add(LevelArgs fa, LevelArgs ca, int ncomp,
+                                                     const T* __restrict__ xc,
+                                                     T* __restrict__ xf)
+{
+   CellPos p;
+   if (cell_pos(fa, 0, p)) prolong_cell<T, NDIM, ORDER, SPLIT>(fa, ca, p, ncomp, xc, xf);
+}
+
+// The split-layout prolongation per coarse cell: a thread adds the
+// correction to the 2^NDIM children of its coarse cell (consecutive lanes:
+// consecutive coarse cells, so for each child offset the fine updates are
+// contiguous in the split layout), each child's value by the formula and
+// summation order of prolong_cell.
+template <typename T, int NDIM, int ORDER>
+__global__ void __launch_bounds__(256) k_prolong_add_cc(LevelArgs fa, LevelArgs ca, int ncomp,
+                                                        const T* __restrict__ xc,
+                                                        T* __restrict__ xf)
+{
+   CellPos p;
+   if (!cell_pos(ca, 0, p)) return;
+   const PatchDev fp = fa.P[ca.T[tile_index(ca)].patch];
+   const int sy = p.pd.sy;
+   const long long row = p.c0 - (p.i + ca.g);
+   const int ig = p.i + ca.g;
+   const long long c0 = row + split_col(ig, sy);
+   const long long cm = row + split_col(ig - 1, sy), cp = row + split_col(ig + 1, sy);
+   for (int m = 0; m < ncomp; m++) {
+      const T* x = xc + m * ca.ncg;
+      T* y = xf + m * fa.ncg;
+#pragma unroll
+      for (int dz = 0; dz < (NDIM == 3 ? 2 : 1); dz++)
+#pragma unroll
+         for (int dy = 0; dy < 2; dy++)
+#pragma unroll
+            for (int dx = 0; dx < 2; dx++) {
+               const long long c1 = dx ? cp : cm;
+               const long long oj = dy ? sy : -(long long)sy;
+               const long long ok = NDIM == 3 ? (dz ? p.pd.sz : -p.pd.sz) : 0;
+               T v;
+               if (ORDER == 0) {
+                  v = x[c0];
+               } else if (NDIM == 2) {
+                  v = T(9.0 / 16.0) * x[c0] + T(3.0 / 16.0) * (x[c1] + x[c0 + oj]) +
+                      T(1.0 / 16.0) * x[c1 + oj];
+               } else {
+                  v = T(27.0 / 64.0) * x[c0] +
+                      T(9.0 / 64.0) * ((x[c1] + x[c0 + oj]) + x[c0 + ok]) +
+                      T(3.0 / 64.0) * ((x[c1 + oj] + x[c1 + ok]) + x[c0 + oj + ok]) +
+                      T(1.0 / 64.0) * x[c1 + oj + ok];
+               }
+               const int fi = 2 * p.i + dx, fj = 2 * p.j + dy;
+               const int fk = NDIM == 3 ? 2 * p.k + dz : 0;
+               const long long fo = fp.cell_off + split_col(fi + fa.g, fp.sy) +
+                                    (long long)(fj + fa.g) * fp.sy +
+                                    (long long)(fk + fa.gz) * fp.sz;
+               y[fo] = y[fo] + v;
+            }
    }
 }
 
 template <typename T>
 int mg_restrict_cell(const Level& coarse, const Level& fine, int ncomp, const T* f, T* c,
-                     hipStream_t s)
+                     hipStream_t s, bool split)
 {
+   dim3 g(coarse.ncell_tiles), b(TX, TY);
+   LevelArgs ca = level_args(coarse), fa = level_args(fine);
+   auto go = [&](auto nd, auto sp) {
+      hipLaunchKernelGGL((k_restrict_cell<T, decltype(nd)::value, decltype(sp)::value>), g, b, 0,
+                         s, ca, fa, ncomp, f, c);
+   };
+   using I2 = std::integral_constant<int, 2>;
+   using I3 = std::integral_constant<int, 3>;
+   using ST = std::true_type;
+   using SF = std::false_type;
    if (coarse.ndim == 3)
-      hipLaunchKernelGGL((k_restrict_cell<T, 3>), dim3(coarse.ncell_tiles), dim3(TX, TY), 0, s,
-                         level_args(coarse), level_args(fine), ncomp, f, c);
+      split ? go(I3(), ST()) : go(I3(), SF());
    else
-      hipLaunchKernelGGL((k_restrict_cell<T, 2>), dim3(coarse.ncell_tiles), dim3(TX, TY), 0, s,
-                         level_args(coarse), level_args(fine), ncomp, f, c);
+      split ? go(I2(), ST()) : go(I2(), SF());
    AMPE_HIP_CHECK(hipGetLastError());
    return 0;
 }
@@ -124,21 +204,31 @@ int mg_restrict_side(const Level& coarse, const Level& fine, const T* f, T* c, h
 
 template <typename T>
 int mg_prolong_add(const Level& fine, const Level& coarse, int ncomp, int order, const T* xc,
-                   T* xf, hipStream_t s)
+                   T* xf, hipStream_t s, bool split)
 {
    dim3 g(fine.ncell_tiles), b(TX, TY);
    LevelArgs fa = level_args(fine), ca = level_args(coarse);
-   if (fine.ndim == 3) {
-      if (order == 0)
-         hipLaunchKernelGGL((k_prolong_add<T, 3, 0>), g, b, 0, s, fa, ca, ncomp, xc, xf);
+   auto go = [&](auto nd, auto sp) {
+      constexpr int ND = decltype(nd)::value;
+      constexpr bool SP = decltype(sp)::value;
+      dim3 gc(coarse.ncell_tiles);
+      if (SP && order == 0)
+         hipLaunchKernelGGL((k_prolong_add_cc<T, ND, 0>), gc, b, 0, s, fa, ca, ncomp, xc, xf);
+      else if (SP)
+         hipLaunchKernelGGL((k_prolong_add_cc<T, ND, 1>), gc, b, 0, s, fa, ca, ncomp, xc, xf);
+      else if (order == 0)
+         hipLaunchKernelGGL((k_prolong_add<T, ND, 0, false>), g, b, 0, s, fa, ca, ncomp, xc, xf);
       else
-         hipLaunchKernelGGL((k_prolong_add<T, 3, 1>), g, b, 0, s, fa, ca, ncomp, xc, xf);
-   } else {
-      if (order == 0)
-         hipLaunchKernelGGL((k_prolong_add<T, 2, 0>), g, b, 0, s, fa, ca, ncomp, xc, xf);
-      else
-         hipLaunchKernelGGL((k_prolong_add<T, 2, 1>), g, b, 0, s, fa, ca, ncomp, xc, xf);
-   }
+         hipLaunchKernelGGL((k_prolong_add<T, ND, 1, false>), g, b, 0, s, fa, ca, ncomp, xc, xf);
+   };
+   using I2 = std::integral_constant<int, 2>;
+   using I3 = std::integral_constant<int, 3>;
+   using ST = std::true_type;
+   using SF = std::false_type;
+   if (fine.ndim == 3)
+      split ? go(I3(), ST()) : go(I3(), SF());
+   else
+      split ? go(I2(), ST()) : go(I2(), SF());
    AMPE_HIP_CHECK(hipGetLastError());
    return 0;
 }
@@ -318,13 +408,13 @@ static int smooth_ell(const Level& L, const EllOp<T>& op, const double* dx, cons
 
 template <typename T>
 static int smooth_quat(const Level& L, const T* fc, const T* sm, double gamma, const double* dx,
-                       int ncomp, const T* b, T* x, int sweeps, hipStream_t s)
+                       int ncomp, const T* b, T* x, int sweeps, hipStream_t s, bool split)
 {
    for (int sw = 0; sw < sweeps; sw++) {
-      if (ghost_fill(L, x, ncomp, s)) return -1;
-      if (launch_quat_rbgs<T>(L, fc, sm, gamma, dx, ncomp, b, x, 0, s)) return -1;
-      if (ghost_fill(L, x, ncomp, s)) return -1;
-      if (launch_quat_rbgs<T>(L, fc, sm, gamma, dx, ncomp, b, x, 1, s)) return -1;
+      if (ghost_fill(L, x, ncomp, s, split)) return -1;
+      if (launch_quat_rbgs<T>(L, fc, sm, gamma, dx, ncomp, b, x, 0, s, split)) return -1;
+      if (ghost_fill(L, x, ncomp, s, split)) return -1;
+      if (launch_quat_rbgs<T>(L, fc, sm, gamma, dx, ncomp, b, x, 1, s, split)) return -1;
    }
    return 0;
 }
@@ -397,6 +487,24 @@ int mg_vcycle_ell(Mg<T>& m, const EllOp<T>& op0, int kind, const MgCycle& cy, co
    return 0;
 }
 
+// The residual after red-black sweeps ending with the black colour vanishes
+// on black cells when every black cell's neighbours are red: one checkerboard
+// over the patches (non-negative, even patch corners: the colour rule of the
+// sweeps then depends on global indices only) and periodic ghosts (zero-slope
+// ghosts copy a cell's own value into its row).  One, two or four components.
+bool red_residual_ok(const Level& L, int ncomp)
+{
+   if (!L.periodic || (ncomp != 1 && ncomp != 2 && ncomp != 4)) return false;
+   for (int a = 0; a < L.ndim; a++) {
+      int ext = L.pbox.hi[a] - L.pbox.lo[a] + 1;
+      if (L.pbox.lo[a] < 0 || (L.pbox.lo[a] & 1) || (ext & 1)) return false;
+   }
+   for (const PatchDev& d : L.hp)
+      for (int a = 0; a < L.ndim; a++)
+         if (d.lo[a] < 0 || (d.lo[a] & 1)) return false;
+   return true;
+}
+
 template <typename T>
 int mg_vcycle_quat(Mg<T>& m, const T* fc0, const T* sm0, double gamma, int ncomp,
                    const MgCycle& cy, const T* b0, T* x0, hipStream_t s)
@@ -406,6 +514,10 @@ int mg_vcycle_quat(Mg<T>& m, const T* fc0, const T* sm0, double gamma, int ncomp
       return -1;
    }
    int nl = m.nlev();
+   if (cy.fused && m.qsplit) {
+      set_error("mg_vcycle_quat: the one-pass sweeps use the natural layout");
+      return -1;
+   }
    if (cy.fused) {
       // one-pass sweeps alternate between x and t on every level; cur[l] is
       // the level's current iterate
@@ -447,6 +559,7 @@ int mg_vcycle_quat(Mg<T>& m, const T* fc0, const T* sm0, double gamma, int ncomp
                                        hipMemcpyDeviceToDevice, s));
       return 0;
    }
+   const bool sp = m.qsplit;
    for (int l = 0; l < nl; l++) {
       MgLevel<T>& c = m.lv[l];
       const T* b = l == 0 ? b0 : c.b;
@@ -454,15 +567,22 @@ int mg_vcycle_quat(Mg<T>& m, const T* fc0, const T* sm0, double gamma, int ncomp
       const T* fc = l == 0 ? fc0 : c.fc;
       const T* sm = l == 0 ? sm0 : c.sm;
       if (l == nl - 1) {
-         if (smooth_quat<T>(*c.L, fc, sm, gamma, c.dx, ncomp, b, x, cy.ncoarse, s)) return -1;
+         if (smooth_quat<T>(*c.L, fc, sm, gamma, c.dx, ncomp, b, x, cy.ncoarse, s, sp)) return -1;
          break;
       }
-      if (smooth_quat<T>(*c.L, fc, sm, gamma, c.dx, ncomp, b, x, cy.nu1, s)) return -1;
-      if (ghost_fill(*c.L, x, ncomp, s)) return -1;
-      if (launch_quat_residual<T>(*c.L, fc, sm, gamma, c.dx, ncomp, b, x, c.r, nullptr, s))
-         return -1;
+      if (smooth_quat<T>(*c.L, fc, sm, gamma, c.dx, ncomp, b, x, cy.nu1, s, sp)) return -1;
+      if (ghost_fill(*c.L, x, ncomp, s, sp)) return -1;
       MgLevel<T>& n = m.lv[l + 1];
-      if (mg_restrict_cell<T>(*n.L, *c.L, ncomp, c.r, n.b, s)) return -1;
+      if (cy.nu1 >= 1 && red_residual_ok(*c.L, ncomp)) {
+         // the sweeps end with the black colour: restrict the red residuals
+         if (launch_quat_resrestrict<T>(*n.L, *c.L, fc, sm, gamma, c.dx, ncomp, b, x, n.b, s, sp))
+            return -1;
+      } else {
+         if (launch_quat_residual<T>(*c.L, fc, sm, gamma, c.dx, ncomp, b, x, c.r, nullptr, s,
+                                     sp))
+            return -1;
+         if (mg_restrict_cell<T>(*n.L, *c.L, ncomp, c.r, n.b, s, sp)) return -1;
+      }
       if (fill_zero<T>(n.x, n.L->ncg * ncomp, s)) return -1;
    }
    for (int l = nl - 2; l >= 0; l--) {
@@ -472,10 +592,19 @@ int mg_vcycle_quat(Mg<T>& m, const T* fc0, const T* sm0, double gamma, int ncomp
       T* x = l == 0 ? x0 : c.x;
       const T* fc = l == 0 ? fc0 : c.fc;
       const T* sm = l == 0 ? sm0 : c.sm;
-      if (ghost_fill(*n.L, n.x, ncomp, s)) return -1;
-      if (mg_prolong_add<T>(*c.L, *n.L, ncomp, 1, n.x, x, s)) return -1;
-      if (smooth_quat<T>(*c.L, fc, sm, gamma, c.dx, ncomp, b, x, cy.nu2, s)) return -1;
+      if (ghost_fill(*n.L, n.x, ncomp, s, sp)) return -1;
+      if (mg_prolong_add<T>(*c.L, *n.L, ncomp, 1, n.x, x, s, sp)) return -1;
+      if (smooth_quat<T>(*c.L, fc, sm, gamma, c.dx, ncomp, b, x, cy.nu2, s, sp)) return -1;
    }
+   return 0;
+}
+
+template <typename T>
+int mg_enable_qsplit(Mg<T>& m, hipStream_t s)
+{
+   for (auto& c : m.lv)
+      if (level_split_tables(*c.L, s)) return -1;
+   m.qsplit = true;
    return 0;
 }
 
@@ -490,10 +619,11 @@ int mg_vcycle_quat(Mg<T>& m, const T* fc0, const T* sm0, double gamma, int ncomp
    template int mg_vcycle_quat<T>(Mg<T>&, const T*, const T*, double, int, const MgCycle&,   \
                                   const T*, T*, hipStream_t);                                \
    template int mg_restrict_cell<T>(const Level&, const Level&, int, const T*, T*,           \
-                                    hipStream_t);                                            \
+                                    hipStream_t, bool);                                      \
+   template int mg_enable_qsplit<T>(Mg<T>&, hipStream_t);                                    \
    template int mg_restrict_side<T>(const Level&, const Level&, const T*, T*, hipStream_t);  \
    template int mg_prolong_add<T>(const Level&, const Level&, int, int, const T*, T*,        \
-                                  hipStream_t);
+                                  hipStream_t, bool);
 AMPE_MG_INST(float)
 AMPE_MG_INST(double)
 

@@ -176,9 +176,13 @@ static int sim_finish(Sim* S, Sim** out)
    }
    if (!rc && s.precond == AMPE_PRECOND_MG) {
       int maxlev = s.mg_max_levels > 0 ? s.mg_max_levels : 32;
-      if (s.precision == AMPE_FAC_FP32)
+      if (s.precision == AMPE_FAC_FP32) {
          rc = mg_create<float>(L, p.dx, ql, maxlev, 2, &S->mgf);
-      else
+         if (!rc && p.evolve_quat && !s.mg_quat_natural) {
+            rc = mg_enable_qsplit<float>(*S->mgf, nullptr);
+            if (!rc && hipDeviceSynchronize() != hipSuccess) rc = -1;
+         }
+      } else
          rc = mg_create<double>(L, p.dx, ql, maxlev, 2, &S->mgd);
    } else if (s.precond != AMPE_PRECOND_REDBLACK) {
       set_error("sim_create: unknown preconditioner");
@@ -779,10 +783,12 @@ static int block_residual(Sim* S, PBlock<T>& B, int reg)
    const Level& L = *S->L;
    hipStream_t st = S->stream;
    double* part = L.partial + (size_t)reg * L.ncell_tiles;
-   if (ghost_fill(L, B.u, B.ncomp, st)) return -1;
+   Mg<T>* mg = mg_of<T>(S);
+   const bool split = B.kind == 1 && mg && mg->qsplit;
+   if (ghost_fill(L, B.u, B.ncomp, st, split)) return -1;
    if (B.kind == 0) return launch_residual<T>(L, B.op, S->p.dx, B.f, B.u, B.r, part, st);
    return launch_quat_residual<T>(L, B.fc, B.sm, S->gamma_setup, S->p.dx, B.ncomp, B.f, B.u,
-                                  B.r, part, st);
+                                  B.r, part, st, split);
 }
 
 // Each block runs the single-level FAC loop of FACPreconditioner::solveSystem
@@ -957,6 +963,13 @@ static int psolve_f32_graph(Sim* S)
    return 0;
 }
 
+// The level whose row parity-split layout the FP32 orientation vectors
+// f_f / f_u use (null: natural layout).
+static const Level* qsplit_level(const Sim* S)
+{
+   return S->mgf && S->mgf->qsplit ? S->L : nullptr;
+}
+
 // z = P^-1 r.  FP32 preconditioner only: f_ready = the FP32 right-hand side
 // S->f_f was already formed from r (by the Krylov kernel that produced r);
 // keep = the solution is left in S->f_u for the caller (z untouched).
@@ -981,12 +994,14 @@ static int psolve(Sim* S, const double* r, double* z, double delta, PsolveOpts o
       if (o.f_ready) {
          ret = psolve_graph_ok(S) ? psolve_f32_graph(S) : psolve_f32(S, delta, true);
       } else {
-         if (psolve_in<float>(ng, S->nc, iq, ql, r, S->sqrtm, S->f_f, nullptr, S->f_u, st))
+         if (psolve_in<float>(ng, S->nc, iq, ql, r, S->sqrtm, S->f_f, nullptr, S->f_u, st,
+                              qsplit_level(S)))
             return -1;
          ret = psolve_f32(S, delta, false);
       }
       if (ret < 0) return -1;
-      if (!o.keep && psolve_out<float>(ng, S->nc, iq, ql, S->f_u, S->sqrtm, nullptr, z, st))
+      if (!o.keep && psolve_out<float>(ng, S->nc, iq, ql, S->f_u, S->sqrtm, nullptr, z, st,
+                                       qsplit_level(S)))
          return -1;
       return ret;
    }
@@ -1066,6 +1081,7 @@ static int spgmr(Sim* S, double gamma, const double* bvec, double* x, double del
       pout.ewt = S->ewt;
       pout.iq = pin.iq;
       pout.ql = pin.ql;
+      pin.split = pout.split = qsplit_level(S) ? 1 : 0;
    }
    double hs[kMaxKrylov + 1];
    if (psolve(S, bvec, S->work, delta) < 0) return -1;

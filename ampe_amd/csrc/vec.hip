@@ -160,13 +160,15 @@ int vec_sumsq_w(const Level& L, int ncomp, const double* x, const double* w, dou
 // Preconditioner input over all nc ghosted component blocks: u = 0 and
 // f = r, with the orientation blocks [iq, iq + ql) divided by sqrt(M) (FP64
 // division, then the conversion to T).  For T = double f may be null: then
-// only the orientation blocks are written, to fq (ql blocks).
+// only the orientation blocks are written, to fq (ql blocks).  np > 0: the
+// orientation blocks are stored in the row parity-split layout of the level
+// whose patch table is P (np patches).
 template <typename T>
 __global__ void __launch_bounds__(256) k_psolve_in(long long ng, int nc, int iq, int ql,
                                                    const double* __restrict__ r,
                                                    const double* __restrict__ sm,
                                                    T* __restrict__ f, T* __restrict__ fq,
-                                                   T* __restrict__ u)
+                                                   T* __restrict__ u, const PatchDev* P, int np)
 {
    long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
    if (i >= ng * nc) return;
@@ -175,8 +177,9 @@ __global__ void __launch_bounds__(256) k_psolve_in(long long ng, int nc, int iq,
    bool q = m >= iq && m < iq + ql;
    if (q) {
       double v = r[i] / sm[c];
-      if (f) f[i] = (T)v;
-      else fq[i - (long long)iq * ng] = (T)v;
+      long long o = np > 0 ? m * ng + split_index(P, np, c) : i;
+      if (f) f[o] = (T)v;
+      else fq[o - (long long)iq * ng] = (T)v;
    } else if (f) {
       f[i] = (T)r[i];
    }
@@ -189,35 +192,43 @@ __global__ void __launch_bounds__(256) k_psolve_out(long long ng, int nc, int iq
                                                     const T* __restrict__ u,
                                                     const double* __restrict__ sm,
                                                     const double* __restrict__ ewt,
-                                                    double* __restrict__ z)
+                                                    double* __restrict__ z, const PatchDev* P,
+                                                    int np)
 {
    long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
    if (i >= ng * nc) return;
    long long m = i / ng, c = i - m * ng;
-   double v = (double)u[i];
-   if (m >= iq && m < iq + ql) v = v * sm[c];
+   double v;
+   if (m >= iq && m < iq + ql) {
+      v = (double)u[np > 0 ? m * ng + split_index(P, np, c) : i];
+      v = v * sm[c];
+   } else {
+      v = (double)u[i];
+   }
    if (ewt) v = ewt[i] * v;
    z[i] = v;
 }
 
 template <typename T>
 int psolve_in(long long ng, int nc, int iq, int ql, const double* r, const double* sm, T* f,
-              T* fq, T* u, hipStream_t s)
+              T* fq, T* u, hipStream_t s, const Level* split)
 {
    long long n = ng * nc;
    hipLaunchKernelGGL(k_psolve_in<T>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, ng, nc,
-                      iq, ql, r, sm, f, fq, u);
+                      iq, ql, r, sm, f, fq, u, split ? split->dp : nullptr,
+                      split ? split->npatch : 0);
    AMPE_HIP_CHECK(hipGetLastError());
    return 0;
 }
 
 template <typename T>
 int psolve_out(long long ng, int nc, int iq, int ql, const T* u, const double* sm,
-               const double* ewt, double* z, hipStream_t s)
+               const double* ewt, double* z, hipStream_t s, const Level* split)
 {
    long long n = ng * nc;
    hipLaunchKernelGGL(k_psolve_out<T>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, ng,
-                      nc, iq, ql, u, sm, ewt, z);
+                      nc, iq, ql, u, sm, ewt, z, split ? split->dp : nullptr,
+                      split ? split->npatch : 0);
    AMPE_HIP_CHECK(hipGetLastError());
    return 0;
 }
@@ -244,12 +255,12 @@ int vec_scale_comp(long long ng, int ncomp, const double* sm, double* x, hipStre
 }
 
 template int psolve_in<float>(long long, int, int, int, const double*, const double*, float*,
-                              float*, float*, hipStream_t);
+                              float*, float*, hipStream_t, const Level*);
 template int psolve_in<double>(long long, int, int, int, const double*, const double*, double*,
-                               double*, double*, hipStream_t);
+                               double*, double*, hipStream_t, const Level*);
 template int psolve_out<float>(long long, int, int, int, const float*, const double*,
-                               const double*, double*, hipStream_t);
+                               const double*, double*, hipStream_t, const Level*);
 template int psolve_out<double>(long long, int, int, int, const double*, const double*,
-                                const double*, double*, hipStream_t);
+                                const double*, double*, hipStream_t, const Level*);
 
 }  // namespace ampe

@@ -132,6 +132,7 @@ class SolverParams(C.Structure):
         ("graph_psolve", C.c_int),
         ("dt_init", C.c_double),
         ("adaptive", C.c_int),
+        ("mg_quat_natural", C.c_int),
     ]
 
 

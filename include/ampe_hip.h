@@ -211,6 +211,11 @@ typedef struct ampe_solver_params {
                            * correction (dsm = ||acor||_wrms tq2 <= 1), step and
                            * order selection after each step, first step dt_init
                            * (else dt), largest step dt (max_step_size) */
+   int mg_quat_natural;   /* FP32 multigrid: 0 (default) stores the orientation block's
+                           * multigrid vectors row parity-split (inside each ghosted row
+                           * the even columns, then the odd ones), so one red-black
+                           * colour reads and writes whole lines; 1 keeps the natural
+                           * layout.  Same values either way. */
 } ampe_solver_params;
 
 /* Per-run counters (mirror QuatIntegrator.cc:2161-2170 nni/nli/nfe/npe/nps). */
@@ -538,6 +543,23 @@ int ampe_coarsen_side(const ampe_level* coarse, const ampe_level* fine, const do
                       double* c, ampe_stream_t stream);
 int ampe_refine_add(const ampe_level* fine, const ampe_level* coarse, int ncomp, int order,
                     const double* c, double* f, ampe_stream_t stream);
+
+/* Coarse right-hand side of the orientation block's multigrid: b (on coarse,
+ * the fine boxes halved) = the mean over each coarse cell's children of the
+ * fine residual rhs - A z, A = I + gamma sqrt(M) L sqrt(M) as in
+ * ampe_quat_smooth (z ghost-filled here).  The residual and restriction steps
+ * of the orientation FAC cycle, QuatFACOps::computeCompositeResidualOnLevel
+ * and restrictResidual (source/QuatFACOps.cc:1588-1754, 939-950).
+ * red_only = 0: the residual of every cell goes to res (ncomp fine component
+ * blocks), then the restriction.  red_only = 1: z comes out of red-black
+ * sweeps ending with the black colour, whose residual vanishes on black
+ * cells, so only the red cells' residuals are formed, per coarse cell, and
+ * res is not used; needs a periodic fine level with even, non-negative patch
+ * corners (one checkerboard) and ncomp 1, 2 or 4. */
+int ampe_quat_residual_restrict(const ampe_level* coarse, const ampe_level* fine,
+                                const double* fc, const double* sm, double gamma,
+                                const double dx[3], int ncomp, const double* rhs, double* z,
+                                double* res, double* b, int red_only, ampe_stream_t stream);
 /* Orientation field of qlen <= 4 components: coarsened by QuatWeightedAverage
  * (source/QuatWeightedAverage.cc:100-171, F77 quatcoarsen: volume average of
  * the children, then q/|q|) and refined (set, not added) by QuatLinearRefine
