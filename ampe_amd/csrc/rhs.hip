@@ -214,6 +214,69 @@ __global__ void __launch_bounds__(256) k_quat_face_norm2(LevelArgs a, ampe_model
    }
 }
 
+// The same face norms in 3-D with the workgroup's q values staged in LDS: the
+// tile's cells grown by one in each direction (planes k-1, k, k+1) are read
+// once, coalesced along i, and every face of the tile takes its values from
+// LDS in quat_face_norm2's order (the global-memory form reads up to ten
+// neighbours per component per face).  Halo cells outside the ghost box are
+// only needed by faces that do not exist; they are set to zero.
+template <int QL>
+__global__ void __launch_bounds__(256) k_quat_face_norm2_lds(LevelArgs a, ampe_model_params p,
+                                                             RhsConst k,
+                                                             const double* __restrict__ q,
+                                                             double* __restrict__ n2)
+{
+   constexpr int HX = TX + 2, HY = TY + 2;
+   __shared__ double sq[QL][3][HY][HX];
+   const Tile t = a.T[tile_index(a)];
+   const PatchDev pd = a.P[t.patch];
+   const int tid = (int)threadIdx.y * TX + (int)threadIdx.x;
+   for (int e = tid; e < 3 * HY * HX; e += TX * TY) {
+      const int ii = e % HX, r = e / HX;
+      const int jj = r % HY, kk = r / HY;
+      const int ci = t.i0 - 1 + ii, cj = t.j0 - 1 + jj, ck = t.k - 1 + kk;
+      const bool ok = ci >= -a.g && ci < pd.nx + a.g && cj >= -a.g && cj < pd.ny + a.g &&
+                      ck >= -a.gz && ck < pd.nz + a.gz;
+      const long long idx = pd.cell_off + (ci + a.g) + (long long)(cj + a.g) * pd.sy +
+                            (long long)(ck + a.gz) * pd.sz;
+#pragma unroll
+      for (int m = 0; m < QL; m++) sq[m][kk][jj][ii] = ok ? q[m * a.ncg + idx] : 0.0;
+   }
+   __syncthreads();
+   const int i = t.i0 + (int)threadIdx.x, j = t.j0 + (int)threadIdx.y, kc = t.k;
+   if (i >= pd.nx + 1 || j >= pd.ny + 1) return;
+   const int x0 = (int)threadIdx.x + 1, y0 = (int)threadIdx.y + 1;
+#pragma unroll
+   for (int d = 0; d < 3; d++) {
+      const bool ok = (i < pd.nx + (d == 0)) && (j < pd.ny + (d == 1)) && (kc < pd.nz + (d == 2));
+      if (!ok) continue;
+      // cm = the cell across the low face of direction d
+      const int mx = x0 - (d == 0), my = y0 - (d == 1), mz = 1 - (d == 2);
+      double s = 0.0;
+#pragma unroll
+      for (int n = 0; n < 3; n++) {
+         const double dinv = k.dxinv[n];
+         const double p25 = 0.25 * dinv;
+         const int tx = (n == 0), ty = (n == 1), tz = (n == 2);
+#pragma unroll
+         for (int m = 0; m < QL; m++) {
+            const double c0v = sq[m][1][y0][x0], cmv = sq[m][mz][my][mx];
+            double gv;
+            if (n == d) {
+               gv = dinv * (c0v - cmv);
+            } else {
+               gv = p25 * ((sq[m][mz + tz][my + ty][mx + tx] - cmv) +
+                           (cmv - sq[m][mz - tz][my - ty][mx - tx]) +
+                           (sq[m][1 + tz][y0 + ty][x0 + tx] - c0v) +
+                           (c0v - sq[m][1 - tz][y0 - ty][x0 - tx]));
+            }
+            s = s + gv * gv;
+         }
+      }
+      n2[face_idx(a, pd, d, i, j, kc)] = s;
+   }
+}
+
 // ------------------------------------------------------------ phase RHS
 // PhaseRHSStrategyWithQ::evaluateRHS per patch (source/PhaseRHSStrategyWithQ.cc
 // :151-312): flux divergence, double well, orientation terms (computerhspbg,
@@ -730,7 +793,7 @@ int launch_quat_grad(const Level& L, const ampe_model_params& p, const double* q
       LevelArgs a = level_args(L, true);
       RhsConst k = rhs_const(p);
       if (L.ndim == 3 && p.qlen == 4)
-         hipLaunchKernelGGL((k_quat_face_norm2<3, 4>), g, b, 0, s, a, p, k, q, face_norm2);
+         hipLaunchKernelGGL((k_quat_face_norm2_lds<4>), g, b, 0, s, a, p, k, q, face_norm2);
       else if (L.ndim == 3)
          hipLaunchKernelGGL((k_quat_face_norm2<3, 0>), g, b, 0, s, a, p, k, q, face_norm2);
       else if (p.qlen == 4)

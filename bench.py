@@ -452,7 +452,8 @@ def main():
     ap.add_argument("--truth-atol", type=float, default=1.0e-8,
                     help="Newton tolerance of the converged FP64 reference run (0: none)")
     ap.add_argument("--maxl", type=int, default=10)
-    ap.add_argument("--mg-levels", type=int, default=3)
+    ap.add_argument("--mg-levels", type=int, default=4,
+                    help="multigrid levels (256^3 .. 32^3 at the default size)")
     ap.add_argument("--mg-coarse-sweeps", type=int, default=2)
     ap.add_argument("--mg-sweeps", type=int, default=1, help="pre- and post-smoothing sweeps")
     ap.add_argument("--forms", default=",".join(FORMS))
@@ -461,7 +462,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-surrogate", action="store_true")
     ap.add_argument("--no-stencils", action="store_true")
-    ap.add_argument("--graphs", type=int, default=0,
+    ap.add_argument("--graphs", type=int, default=1,
                     help="1: FP32 preconditioner solves replayed from a captured HIP graph")
     args = ap.parse_args()
     if args.gpus != 1:
