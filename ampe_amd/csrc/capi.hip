@@ -204,6 +204,15 @@ int ampe_conc_coeffs_ebs(const ampe_level* level, const ampe_model_params* p,
    return ampe::launch_ebs_coeffs(*level->L, *p, phi, DL, DA, hs(stream));
 }
 
+int ampe_conc_coeffs_ebs_mobility(const ampe_level* level, const ampe_model_params* p,
+                                  const double* phi, const double* cl, const double* cs,
+                                  double* DL, double* DA, ampe_stream_t stream)
+{
+   AMPE_REQUIRE(level && phi && cl && cs && DL && DA, "null argument");
+   if (check_params(p, *level->L)) return -1;
+   return ampe::launch_ebs_coeffs_mob(*level->L, *p, phi, cl, cs, DL, DA, hs(stream));
+}
+
 int ampe_conc_rhs_ebs(const ampe_level* level, const ampe_model_params* p, const double* cl,
                       const double* cs, const double* DL, const double* DA, double* rhs,
                       ampe_stream_t stream)

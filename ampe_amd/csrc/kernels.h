@@ -221,6 +221,10 @@ int launch_conc_coeffs(const Level& L, const ampe_model_params& p, const double*
 int launch_conc_rhs(const Level& L, const ampe_model_params& p, const double* c,
                     const double* phi, const double* Dc, const double* Dphi, double* rhs,
                     hipStream_t s);
+// EBS face diffusivities of the composition-dependent (CALPHAD mobility) model.
+int launch_ebs_coeffs_mob(const Level& L, const ampe_model_params& p, const double* phi,
+                          const double* cl, const double* cs, double* DL, double* DA,
+                          hipStream_t s);
 int launch_ebs_coeffs(const Level& L, const ampe_model_params& p, const double* phi, double* DL,
                       double* DA, hipStream_t s);
 int launch_conc_rhs_ebs(const Level& L, const ampe_model_params& p, const double* cl,

@@ -6,6 +6,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <cmath>
+
 #include "../../include/ampe_hip.h"
 
 #define AMPE_HD __host__ __device__ __forceinline__
@@ -184,6 +186,60 @@ AMPE_HD double dilute_dG(const ampe_model_params& p)
 {
    double RT = kRGas * p.temperature;
    return RT * (p.dilute_keq - 1.0) * (p.temperature - p.dilute_Tm) / p.dilute_m;
+}
+
+// Temperature-dependent parts of the CALPHAD atomic mobilities, per phase
+// (0 liquid, 1 solid A) and species: Q_A, Q_B and the Redlich-Kister QQ_n of
+// CALPHADMobility::getQ / getQQn (source/CALPHADMobility.h:42-103), formed on
+// the host once per launch.
+struct MobConst {
+   double RT;
+   double QA[2][2], QB[2][2];
+   double QQ[2][2][4];
+};
+
+inline MobConst mob_const(const ampe_model_params& p)
+{
+   MobConst m;
+   const double T = p.temperature;
+   m.RT = kRGas * T;
+   for (int ph = 0; ph < 2; ph++)
+      for (int sp = 0; sp < 2; sp++) {
+         const ampe_calphad_mobility& mb = p.mobility[ph][sp];
+         double q[2];
+         for (int x = 0; x < 2; x++) {
+            q[x] = mb.q[x][0] + m.RT * std::log(mb.q[x][1]);
+            if (mb.has_extra) q[x] += mb.qx[x][0] * T * std::log(T) + mb.qx[x][1] / T;
+         }
+         m.QA[ph][sp] = q[0];
+         m.QB[ph][sp] = q[1];
+         for (int n = 0; n < 4; n++)
+            m.QQ[ph][sp][n] = mb.qq[n][0] + m.RT * std::log(mb.qq[n][1]);
+      }
+   return m;
+}
+
+// CALPHADMobility::getAtomicMobility for a binary (getDeltaG,
+// source/CALPHADMobility.cc:155-166): exp(dG / RT) / RT.
+AMPE_HD double atomic_mobility(const MobConst& m, int ph, int sp, double c0, double c1)
+{
+   const double dc = c0 - c1;
+   const double* qq = m.QQ[ph][sp];
+   const double dG = c0 * m.QA[ph][sp] + c1 * m.QB[ph][sp] +
+                     c0 * c1 * (qq[0] + dc * (qq[1] + dc * (qq[2] + dc * qq[3])));
+   const double rtinv = 1.0 / m.RT;
+   return exp(dG * rtinv) * rtinv;
+}
+
+// computeDiffusionMobilityBinaryPhase (source/CALPHADMobility.cc:200-218):
+// c0 c1 (c0 m1 + c1 m0), in um^2 (the 1e12 of m^2).
+AMPE_HD double phase_diffusion_mobility(const MobConst& m, int ph, double c0)
+{
+   const double c1 = 1.0 - c0;
+   const double m0 = atomic_mobility(m, ph, 0, c0, c1);
+   const double m1 = atomic_mobility(m, ph, 1, c0, c1);
+   const double mm = c0 * m1 + c1 * m0;
+   return c0 * c1 * mm * 1.0e12;
 }
 
 // The energy functions take the model as a template argument FE when a kernel

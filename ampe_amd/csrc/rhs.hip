@@ -551,6 +551,41 @@ __global__ void __launch_bounds__(256) k_ebs_coeffs(LevelArgs a, ampe_model_para
    }
 }
 
+// MobilityCompositionDiffusionStrategy::setDiffusion
+// (source/MobilityCompositionDiffusionStrategy.cc:97-150): per face, the
+// phase concentrations averaged arithmetically (setDiffCoeffInEachPhaseOnPatch,
+// :194-329), each phase's M(c) d2f/dc2(c) (computeLocalDiffusionMatrixL/A,
+// :57-95, one species: the 1 x 1 product), then the phase weights
+// (setPFMDiffOnPatch, :331-611); uniform temperature.
+template <int NDIM>
+__global__ void __launch_bounds__(256) k_ebs_coeffs_mob(LevelArgs a, ampe_model_params p,
+                                                        MobConst mc,
+                                                        const double* __restrict__ phi,
+                                                        const double* __restrict__ cl,
+                                                        const double* __restrict__ cs,
+                                                        double* __restrict__ DL,
+                                                        double* __restrict__ DA)
+{
+   CellPos c;
+   if (!cell_pos(a, 1, c)) return;
+   if (NDIM == 2 && c.k > 0) return;
+   for (int d = 0; d < NDIM; d++) {
+      bool ok = (c.i < c.pd.nx + (d == 0)) && (c.j < c.pd.ny + (d == 1)) &&
+                (NDIM == 2 || c.k < c.pd.nz + (d == 2));
+      if (!ok) continue;
+      long long c0 = c.c0, cm = c0 - c.st[d];
+      long long fi = face_idx(a, c.pd, d, c.i, c.j, c.k);
+      const double cL = 0.5 * (cl[c0] + cl[cm]);
+      const double cA = 0.5 * (cs[c0] + cs[cm]);
+      const double dl = phase_diffusion_mobility(mc, 0, cL) * d2fenergy_t<-1>(p, cL, 0);
+      const double da = phase_diffusion_mobility(mc, 1, cA) * d2fenergy_t<-1>(p, cA, 1);
+      double vphi = face_avg(phi[cm], phi[c0], p.avg_type);
+      double hphi = interp(vphi, p.diff_interp);
+      DL[fi] = (1.0 - hphi) * dl;
+      DA[fi] = hphi * da;
+   }
+}
+
 // EBSCompositionRHSStrategy::computeFluxOnPatch (source/EBSCompositionRHSStrategy.cc
 // :183-328): add_flux (2d/flux.m4:11-66) of the liquid then the solid phase
 // onto a zero flux, then computerhsconcentration (2d/concentrationrhs.m4:326-367)
@@ -913,6 +948,15 @@ int launch_ebs_coeffs(const Level& L, const ampe_model_params& p, const double* 
                       double* DA, hipStream_t s)
 {
    AMPE_DISPATCH(L, k_ebs_coeffs, L.nface_tiles, level_args(L, true), p, rhs_const(p), phi, DL, DA);
+   return 0;
+}
+
+int launch_ebs_coeffs_mob(const Level& L, const ampe_model_params& p, const double* phi,
+                          const double* cl, const double* cs, double* DL, double* DA,
+                          hipStream_t s)
+{
+   AMPE_DISPATCH(L, k_ebs_coeffs_mob, L.nface_tiles, level_args(L, true), p, mob_const(p), phi,
+                 cl, cs, DL, DA);
    return 0;
 }
 

@@ -81,6 +81,14 @@ static int sim_check(const ampe_model_params& p, const ampe_solver_params& s)
       set_error("sim_create: unknown EBS stencil");
       return -1;
    }
+   if (p.conc_diffusion_type != AMPE_CONC_DIFF_TEMPERATURE &&
+       (p.conc_diffusion_type != AMPE_CONC_DIFF_COMPOSITION ||
+        p.conc_rhs_form != AMPE_CONC_RHS_EBS || p.ebs_stencil == AMPE_EBS_ISOTROPIC ||
+        p.fe_model == AMPE_FE_QUADRATIC_TERNARY_3PH)) {
+      set_error("sim_create: composition-dependent diffusion needs the two-phase EBS form "
+                "with the regular or order4 stencil");
+      return -1;
+   }
    const bool tern = p.fe_model == AMPE_FE_QUADRATIC_TERNARY_3PH;
    if (tern && (p.norderp < 3 || !p.with_conc || p.evolve_quat ||
                 p.conc_rhs_form != AMPE_CONC_RHS_EBS || p.ebs_stencil != AMPE_EBS_REGULAR ||
@@ -612,7 +620,13 @@ static int rhs_eval(Sim* S, double* y, double* ydot, bool fd)
       return -1;
    if (c && p.conc_rhs_form == AMPE_CONC_RHS_EBS) {
       // Dc, Dphi hold the liquid and solid face diffusivities
-      if (recompute && launch_ebs_coeffs(L, p, phi, S->Dc, S->Dphi, st)) return -1;
+      if (recompute) {
+         if (p.conc_diffusion_type == AMPE_CONC_DIFF_COMPOSITION) {
+            if (launch_ebs_coeffs_mob(L, p, phi, S->cl, S->cs, S->Dc, S->Dphi, st)) return -1;
+         } else if (launch_ebs_coeffs(L, p, phi, S->Dc, S->Dphi, st)) {
+            return -1;
+         }
+      }
       if (p.ebs_stencil == AMPE_EBS_ISOTROPIC) {
          // the diffusivities follow the lagging of the face data: the
          // isotropic stencil evaluates them from the phase field of the last

@@ -95,6 +95,7 @@ _SIGS = {
     "ampe_coarsen_cell": (_i, [_vp, _vp, _i, _vp, _vp, _vp]),
     "ampe_coarsen_side": (_i, [_vp, _vp, _vp, _vp, _vp]),
     "ampe_refine_add": (_i, [_vp, _vp, _i, _i, _vp, _vp, _vp]),
+    "ampe_conc_coeffs_ebs_mobility": (_i, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "ampe_quat_residual_restrict": (_i, [_vp, _vp, _vp, _vp, C.c_double, _vp, _i, _vp, _vp,
                                          _vp, _vp, _i, _vp]),
     "ampe_calphad_at": (_i, [C.POINTER(CalphadTdep), _d, C.POINTER(CalphadPhase)]),

@@ -39,6 +39,11 @@ class KksBatchStats(C.Structure):
         return {n: getattr(self, n) for n, _ in self._fields_}
 
 
+class CalphadMobility(C.Structure):
+    _fields_ = [("q", (C.c_double * 2) * 2), ("qx", (C.c_double * 2) * 2),
+                ("has_extra", C.c_int), ("qq", (C.c_double * 2) * 4)]
+
+
 class ModelParams(C.Structure):
     _fields_ = [
         ("ndim", C.c_int),
@@ -98,6 +103,8 @@ class ModelParams(C.Structure):
         ("tern_ceq", (C.c_double * 2) * 3),
         ("D_solid_B", C.c_double),
         ("Q0_solid_B", C.c_double),
+        ("conc_diffusion_type", C.c_int),
+        ("mobility", (CalphadMobility * 2) * 2),
     ]
 
 
@@ -152,6 +159,7 @@ FLUX_SIMPLE, FLUX_ISOTROPIC, FLUX_ANISOTROPIC = 0, 1, 2
 FE_QUADRATIC, FE_CALPHAD_BINARY, FE_KKS_DILUTE, FE_QUADRATIC_TERNARY_3PH = 0, 1, 2, 3
 PHI_MOB_CONSTANT, PHI_MOB_KIM = 0, 1
 CONC_RHS_KKS, CONC_RHS_EBS = 0, 1
+CONC_DIFF_TEMPERATURE, CONC_DIFF_COMPOSITION = 0, 1
 EBS_REGULAR, EBS_ISOTROPIC, EBS_ORDER4 = 0, 1, 2
 FP64, FAC_FP32 = 0, 1
 GUESS_PREVIOUS, GUESS_SURROGATE = 0, 1
@@ -500,6 +508,71 @@ def kks_composition_deck():
     state = (ic["phase"][0], ic["conc"][0][0], None)
     checks = dict(end_time=0.3, conc_integral_tol=1.0e-4, solid_fraction=0.32,
                   solid_fraction_tol=1.0e-2, max_cycles=900)
+    return p, ramp, state, checks
+
+
+# CALPHAD atomic mobilities of Au (species 0) and Ni (species 1) in the
+# liquid and in fcc (thermodynamic_data/calphadAuNi.dat:66-106,
+# MobilityParameters): (value, prefactor) of qA, qB, q0AB, q1AB; the absent
+# q2AB, q3AB are (0, 1).
+_AUNI_MOB = {
+    ("L", 0): dict(qA=(-35921.87, 3.1514e-8), qB=(-46353.23, 5.9576e-8),
+                   q0AB=(0.0, 1.0), q1AB=(0.0, 1.0)),
+    ("A", 0): dict(qA=(-176600.0, 1.002878e-05), qB=(-306500.0, 4.49132e-2),
+                   q0AB=(232300.0, 7.6881435e-08), q1AB=(15900.0, 1.0)),
+    ("L", 1): dict(qA=(-35921.87, 3.6471e-8), qB=(-46353.23, 6.8948e-8),
+                   q0AB=(0.0, 1.0), q1AB=(0.0, 1.0)),
+    ("A", 1): dict(qA=(-188800.0, 1.702455873e-05), qB=(-287000.0, 2.25994e-4),
+                   q0AB=(136200.0, 1.0), q1AB=(-42800.0, 1.0)),
+}
+
+
+def set_mobility(m, qA, qB, q0AB=(0.0, 1.0), q1AB=(0.0, 1.0), q2AB=(0.0, 1.0),
+                 q3AB=(0.0, 1.0), extraA=None, extraB=None):
+    """Fill one CalphadMobility record (CALPHADMobility::initialize,
+    source/CALPHADMobility.cc:12-153: absent Redlich-Kister terms (0, 1);
+    four-value qA/qB entries carry the T ln T and 1/T coefficients)."""
+    for x, v in enumerate((qA, qB)):
+        m.q[x][0], m.q[x][1] = v
+    m.has_extra = 1 if (extraA is not None or extraB is not None) else 0
+    for x, v in enumerate((extraA, extraB)):
+        m.qx[x][0], m.qx[x][1] = v if v is not None else (0.0, 0.0)
+    for n, v in enumerate((q0AB, q1AB, q2AB, q3AB)):
+        m.qq[n][0], m.qq[n][1] = v
+
+
+def auni_mobility(p):
+    """p.mobility[phase][species] from calphadAuNi.dat's MobilityParameters
+    (PhaseL -> 0, PhaseA -> 1; Species0 = Au, Species1 = Ni)."""
+    for ip, ph in enumerate("LA"):
+        for sp in range(2):
+            set_mobility(p.mobility[ip][sp], **_AUNI_MOB[(ph, sp)])
+    return p
+
+
+def single_grain_growth_auni_deck():
+    """The reference's SingleGrainGrowthAuNi regression (tests/
+    SingleGrainGrowthAuNi/2d.input, test2d.py): the KKScomposition set-up
+    (2-D 64 x 64 cells on [0, 1.8]^2 um, zero-slope boundaries, AuNi CALPHAD
+    KKS, M_phi = 6.4, pbg interpolation, arithmetic averages, molar volume
+    7.68e-6, KKS Newton max_its 50, 1450 K cooled at -200 K/s towards
+    1220 K, make_nuclei.py -r 16 --center0 "0, 0, 0" -c 0.25
+    --concentration-in 0.096) with epsilon_phi = 0.25 (pJ/um)^1/2,
+    phi_well_scale = 2.5 pJ/um^3, the EBS composition form and its default
+    composition-dependent diffusion (the database's atomic mobilities,
+    linear diffusion weights, source/QuatModelParameters.cc:238-250, 1014-1016),
+    atol 1e-5.  test2d.py: the integral of c within 1e-4 of its initial value,
+    solid volume fraction 0.32 +- 0.01 after t = 0.3 s.
+
+    Returns (model params, ramp dict, initial state, checks dict)."""
+    p, ramp, state, checks = kks_composition_deck()
+    p.epsilon_phi = 0.25
+    p.phi_well_scale = 2.5
+    p.conc_rhs_form = CONC_RHS_EBS
+    p.ebs_stencil = EBS_REGULAR
+    p.diff_interp = b"l"
+    p.conc_diffusion_type = CONC_DIFF_COMPOSITION
+    auni_mobility(p)
     return p, ramp, state, checks
 
 

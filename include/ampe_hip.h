@@ -66,6 +66,21 @@ typedef struct ampe_calphad_phase {
    double L[4];
 } ampe_calphad_phase;
 
+/* CALPHAD atomic mobility of one species in one phase (CALPHADMobility,
+ * source/CALPHADMobility.h:31-160 and .cc:12-196, database block
+ * MobilityParameters { Species<k> { Phase<L|A> { qA, qB, q0AB..q3AB } } }):
+ *   Q_X(T)  = q[X][0] + R T ln q[X][1]  (+ qx[X][0] T ln T + qx[X][1] / T
+ *             when has_extra), X = A (index 0), B (1);
+ *   QQ_n(T) = qq[n][0] + R T ln qq[n][1]  (absent terms: (0, 1));
+ *   dG = c0 Q_A + c1 Q_B + c0 c1 (QQ_0 + dc (QQ_1 + dc (QQ_2 + dc QQ_3))),
+ *   dc = c0 - c1, c1 = 1 - c0;  mobility = exp(dG / RT) / RT. */
+typedef struct ampe_calphad_mobility {
+   double q[2][2];
+   double qx[2][2];
+   int has_extra;
+   double qq[4][2];
+} ampe_calphad_mobility;
+
 typedef struct ampe_model_params {
    int ndim;              /* 2 or 3 */
    double dx[3];          /* mesh spacing */
@@ -138,7 +153,22 @@ typedef struct ampe_model_params {
    double tern_A[3][2];   /* curvatures per phase (L, A, B) and species */
    double tern_ceq[3][2]; /* equilibrium compositions per phase and species */
    double D_solid_B, Q0_solid_B; /* D exp(-Q/RT) of solid B (D_solid, Q0_solid: solid A) */
+   /* EBS diffusivities (ConcentrationModel diffusion_type): enum
+    * ampe_conc_diffusion.  Composition dependent: the atomic mobilities of the
+    * two species in the liquid and in solid A, [phase][species]. */
+   int conc_diffusion_type;
+   ampe_calphad_mobility mobility[2][2];
 } ampe_model_params;
+
+/* EBS face diffusivities: temperature dependent (TbasedCompositionDiffusion-
+ * Strategy: D exp(-Q/RT) per phase) or composition dependent
+ * (MobilityCompositionDiffusionStrategy, the reference's default for the EBS
+ * form: per phase the CALPHAD mobility times d2f/dc2 at the face-averaged
+ * phase concentration, source/QuatModelParameters.cc:238-250). */
+enum ampe_conc_diffusion {
+   AMPE_CONC_DIFF_TEMPERATURE = 0,
+   AMPE_CONC_DIFF_COMPOSITION = 1
+};
 
 /* Phase mobility: the uniform M_phi (SimpleQuatMobilityStrategy,
  * source/SimpleQuatMobilityStrategy.cc:46) or Kim's KKS mobility from the
@@ -344,6 +374,20 @@ int ampe_kks_solve(const ampe_level* level, const ampe_model_params* p,
  * computeFluxOnPatch with add_flux, source/fortran/2d/flux.m4:11-66, and
  * computerhsconcentration).  cl, cs need filled ghosts (the KKS solve
  * covers them); DL, DA are side fields. */
+/* EBS face diffusivities of the composition-dependent model
+ * (MobilityCompositionDiffusionStrategy::setDiffusion,
+ * source/MobilityCompositionDiffusionStrategy.cc:97-150, 194-329 and
+ * setPFMDiffOnPatch :331-611; computeDiffusionMobilityBinaryPhase,
+ * source/CALPHADMobility.cc:200-218): on each face the phase concentrations
+ * are averaged arithmetically, c = (c_i + c_i-1) / 2, the phase's diffusivity
+ * is M(c) d2f/dc2(c) with M = c (1 - c) (c m_1 + (1 - c) m_0) 1e12 um^2 from
+ * the two species' atomic mobilities m_k (ampe_calphad_mobility, model
+ * p->mobility) and d2f/dc2 in J/mol, and D_L = (1 - h) D_L(c_L),
+ * D_A = h D_A(c_A), h = diff_interp(avg(phi)).  cl, cs with filled ghosts. */
+int ampe_conc_coeffs_ebs_mobility(const ampe_level* level, const ampe_model_params* p,
+                                  const double* phi, const double* cl, const double* cs,
+                                  double* DL, double* DA, ampe_stream_t stream);
+
 int ampe_conc_coeffs_ebs(const ampe_level* level, const ampe_model_params* p,
                          const double* phi, double* DL, double* DA,
                          ampe_stream_t stream);

@@ -71,6 +71,11 @@ int o_kks_cell(const ampe_model_params* p, double c, double phi, double* cl,
 
 int o_grains_label(const OGeom* G, const double* phase, double thr, int periodic, int* labels,
                    int* sweeps);
+/* composition-dependent EBS diffusivities (CALPHAD mobilities) */
+double o_atomic_mobility(const ampe_calphad_mobility* m, double c0, double c1, double T);
+double o_phase_diffusion_mobility(const ampe_calphad_mobility m[2], double c0, double T);
+void o_ebs_coeffs_mob(const OGeom* G, const ampe_model_params* p, const double* phi,
+                      const double* cl, const double* cs, double* DL, double* DA);
 void o_ebs_coeffs(const OGeom* G, const ampe_model_params* p, const double* phi, double* DL,
                   double* DA);
 /* three phases on norderp > 1 orders, ternary quadratic (oracle_ternary.c) */

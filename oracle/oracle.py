@@ -83,6 +83,9 @@ def _setup(L):
         "o_kks_solve": (C.c_int, [_G, _P, _dp, _dp, _dp, _dp, _dp, _dp]),
         "o_grains_label": (C.c_int, [_G, _dp, C.c_double, C.c_int, _ip, _ip]),
         "o_ebs_coeffs": (None, [_G, _P, _dp, _dp, _dp]),
+        "o_ebs_coeffs_mob": (None, [_G, _P, _dp, _dp, _dp, _dp, _dp]),
+        "o_atomic_mobility": (C.c_double, [C.c_void_p, C.c_double, C.c_double, C.c_double]),
+        "o_phase_diffusion_mobility": (C.c_double, [C.c_void_p, C.c_double, C.c_double]),
         "o_conc_rhs_ebs": (None, [_G, _P, _dp, _dp, _dp, _dp, _dp]),
         "o_tag_cells": (None, [_G, _P, _dp, C.c_double, _ip]),
         "o_phase_fac_C": (None, [_G, _P, C.c_double, _dp, _dp, _dp]),

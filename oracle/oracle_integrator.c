@@ -318,7 +318,12 @@ static int rhs_eval(OSim* S, double* y, double* ydot, int fd)
    if (c) {
       if (p->conc_rhs_form == AMPE_CONC_RHS_EBS) {
          /* Dc, Dphi hold the liquid and solid face diffusivities */
-         if (recompute) o_ebs_coeffs(G, p, phi, S->Dc, S->Dphi);
+         if (recompute) {
+            if (p->conc_diffusion_type == AMPE_CONC_DIFF_COMPOSITION)
+               o_ebs_coeffs_mob(G, p, phi, S->cl, S->cs, S->Dc, S->Dphi);
+            else
+               o_ebs_coeffs(G, p, phi, S->Dc, S->Dphi);
+         }
          if (p->ebs_stencil == AMPE_EBS_ISOTROPIC) {
             /* diffusivities from the phase field of the last full evaluation */
             if (recompute) memcpy(S->phid, phi, sizeof(double) * ng);

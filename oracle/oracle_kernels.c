@@ -1059,6 +1059,72 @@ void o_ebs_coeffs(const OGeom* G, const ampe_model_params* p, const double* phi,
    }
 }
 
+/* CALPHADMobility::getQ (source/CALPHADMobility.h:42-55): value + R T ln(prefactor),
+ * plus the T ln T and 1/T terms of a four-value entry. */
+static double o_mob_q(const double q[2], const double qx[2], int has_extra, double T)
+{
+   double val = q[0] + R_GAS * T * log(q[1]);
+   if (has_extra) val += qx[0] * T * log(T) + qx[1] / T;
+   return val;
+}
+
+/* CALPHADMobility::getAtomicMobility with the binary getDeltaG
+ * (source/CALPHADMobility.h:105-150, .cc:155-166): exp(dG / RT) / RT. */
+double o_atomic_mobility(const ampe_calphad_mobility* m, double c0, double c1, double T)
+{
+   const double qa = o_mob_q(m->q[0], m->qx[0], m->has_extra, T);
+   const double qb = o_mob_q(m->q[1], m->qx[1], m->has_extra, T);
+   double qq[4];
+   for (int n = 0; n < 4; n++) qq[n] = m->qq[n][0] + R_GAS * T * log(m->qq[n][1]);
+   const double dc = c0 - c1;
+   const double dG = c0 * qa + c1 * qb + c0 * c1 * (qq[0] + dc * (qq[1] + dc * (qq[2] + dc * qq[3])));
+   const double rtinv = 1.0 / (R_GAS * T);
+   return exp(dG * rtinv) * rtinv;
+}
+
+/* computeDiffusionMobilityBinaryPhase (source/CALPHADMobility.cc:200-218):
+ * c0 c1 (c0 m1 + c1 m0) in um^2, m_k the atomic mobility of species k. */
+double o_phase_diffusion_mobility(const ampe_calphad_mobility m[2], double c0, double T)
+{
+   const double c1 = 1.0 - c0;
+   const double m0 = o_atomic_mobility(&m[0], c0, c1, T);
+   const double m1 = o_atomic_mobility(&m[1], c0, c1, T);
+   const double mm = c0 * m1 + c1 * m0;
+   return c0 * c1 * mm * 1.0e12;
+}
+
+/* MobilityCompositionDiffusionStrategy::setDiffusion
+ * (source/MobilityCompositionDiffusionStrategy.cc:97-150): per face the phase
+ * concentrations and the temperature averaged arithmetically
+ * (setDiffCoeffInEachPhaseOnPatch, :194-329), each phase's mobility times
+ * d2f/dc2 in J/mol (computeLocalDiffusionMatrixL/A, :57-95), then the phase
+ * weights (setPFMDiffOnPatch, :331-611). */
+void o_ebs_coeffs_mob(const OGeom* G, const ampe_model_params* p, const double* phi,
+                      const double* cl, const double* cs, double* DL, double* DA)
+{
+   long st[3] = {1, G->sy, G->sz};
+   for (int dir = 0; dir < G->ndim; dir++) {
+      int ex = G->nx + (dir == 0), ey = G->ny + (dir == 1), ez = G->nz + (dir == 2);
+      long off = (dir == 0 ? 0 : dir == 1 ? YOFF : ZOFF);
+      for (int k = 0; k < ez; k++)
+         for (int j = 0; j < ey; j++)
+            for (int i = 0; i < ex; i++) {
+               long c0 = CI(i, j, k), cm = c0 - st[dir];
+               long fi = off + (long)i + (long)ex * ((long)j + (long)ey * k);
+               double T = 0.5 * (p->temperature + p->temperature);
+               double cL = 0.5 * (cl[c0] + cl[cm]), cA = 0.5 * (cs[c0] + cs[cm]);
+               double dl = o_phase_diffusion_mobility(p->mobility[0], cL, T) *
+                           o_d2fenergy(p, cL, 0);
+               double da = o_phase_diffusion_mobility(p->mobility[1], cA, T) *
+                           o_d2fenergy(p, cA, 1);
+               double vphi = o_average_func(phi[cm], phi[c0], p->avg_type);
+               double hphi = o_interp_func(vphi, p->diff_interp);
+               DL[fi] = (1.0 - hphi) * dl;
+               DA[fi] = hphi * da;
+            }
+   }
+}
+
 /* EBSCompositionRHSStrategy::computeFluxOnPatch (source/EBSCompositionRHSStrategy.cc
  * :183-328) with add_flux (2d/flux.m4:11-66, the liquid then the solid
  * contribution added to a zeroed flux) and computerhsconcentration
