@@ -42,6 +42,7 @@ int kry_mid(const Level& L, int nc, double sig, const double* xk, const double* 
 // unnormalised basis vector V and its scale h (no xk array).
 int kry_mid_scaled(const Level& L, int nc, double sig, const double* V, double h,
                    const double* ewt, const double* ycur, double* work2, hipStream_t s);
+// (out may be null when pin.f receives the FP32 preconditioner input.)
 int kry_finish_scaled(const Level& L, int nc, double siginv, double gamma, const double* V,
                       double h, const double* ewt, const double* jv, const double* fcur,
                       double* out, const PrecIn& pin, hipStream_t s);

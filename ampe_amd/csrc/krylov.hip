@@ -72,7 +72,9 @@ __global__ void __launch_bounds__(256) k_gmres_mid(LevelArgs a, int nc, double s
 }
 
 // out = x + (-gamma) (siginv jv + (-siginv) fcur); with pin.f also the FP32
-// preconditioner right-hand side of out (the arithmetic of k_psolve_in)
+// preconditioner right-hand side of out (the arithmetic of k_psolve_in), and
+// out may then be null (the FP64 vector is not needed: the preconditioned
+// vector replaces it)
 __global__ void __launch_bounds__(256) k_gmres_finish(LevelArgs a, int nc, double siginv,
                                                       double gamma, KryDir x,
                                                       const double* __restrict__ jv,
@@ -85,7 +87,7 @@ __global__ void __launch_bounds__(256) k_gmres_finish(LevelArgs a, int nc, doubl
       long long i = m * a.ncg + c.c0;
       double jvn = siginv * jv[i] + (-siginv) * fcur[i];
       double v = kry_dir(x, i) + (-gamma) * jvn;
-      out[i] = v;
+      if (out) out[i] = v;
       if (pin.f) {
          bool q = m >= pin.iq && m < pin.iq + pin.ql;
          if (q)

@@ -286,6 +286,15 @@ __global__ void __launch_bounds__(256) k_quat_face_norm2_lds(LevelArgs a, ampe_m
 // composition field) so that each instance holds only its own path's registers.
 constexpr int kFeNone = 3;
 
+template <int NDIM, int FE>
+__device__ __forceinline__ void phase_rhs_tail(const LevelArgs& a, const ampe_model_params& p,
+                                               const RhsConst& k, const CellPos& c, double r,
+                                               const double* __restrict__ phi,
+                                               const double* __restrict__ q,
+                                               const double* __restrict__ cl,
+                                               const double* __restrict__ cs,
+                                               double* __restrict__ rhs);
+
 template <int NDIM, int FLUX, int FE>
 __device__ __forceinline__ void phase_rhs_cell(const LevelArgs& a, const ampe_model_params& p,
                                                const RhsConst& k, const CellPos& c,
@@ -304,6 +313,21 @@ __device__ __forceinline__ void phase_rhs_cell(const LevelArgs& a, const ampe_mo
       double dt = (fhi - flo) * k.dxinv[d];
       r = (d == 0) ? dt : r + dt;
    }
+   phase_rhs_tail<NDIM, FE>(a, p, k, c, r, phi, q, cl, cs, rhs);
+}
+
+// The phase RHS of one cell after the flux divergence r: double well,
+// orientation terms, driving force, times the mobility.
+template <int NDIM, int FE>
+__device__ __forceinline__ void phase_rhs_tail(const LevelArgs& a, const ampe_model_params& p,
+                                               const RhsConst& k, const CellPos& c, double r,
+                                               const double* __restrict__ phi,
+                                               const double* __restrict__ q,
+                                               const double* __restrict__ cl,
+                                               const double* __restrict__ cs,
+                                               double* __restrict__ rhs)
+{
+   const long long c0 = c.c0;
    double ph = phi[c0];
    r = r - p.phi_well_scale * dwell(ph, p.phi_well_type);
    if (p.evolve_quat) {
@@ -352,6 +376,64 @@ __global__ void __launch_bounds__(256)
       }
       phase_rhs_cell<NDIM, FLUX, FE>(a, p, k, c, phi, q, cl, cs, rhs);
    }
+}
+
+// The phase RHS with each x and y face flux evaluated once: a workgroup's
+// tile (one plane) puts the fluxes of its cells' low faces, and of the high
+// face closing each row and column, in LDS, then every cell takes the
+// divergence from there (z faces, 3-D: both per cell).  The per-cell form
+// evaluates every face twice, once from each side; the anisotropic flux has
+// about four transcendentals per face.  Same values as phase_rhs_cell.
+template <int NDIM, int FLUX, int FE>
+__global__ void __launch_bounds__(256) k_phase_rhs_faces(LevelArgs a, ampe_model_params p,
+                                                         RhsConst k,
+                                                         const double* __restrict__ phi,
+                                                         const double* __restrict__ q,
+                                                         const double* __restrict__ cl,
+                                                         const double* __restrict__ cs,
+                                                         double* __restrict__ rhs)
+{
+   __shared__ double fx[TY][TX + 1];
+   __shared__ double fy[TY + 1][TX];
+   const Tile t = a.T[tile_index(a)];
+   CellPos c;
+   c.pd = a.P[t.patch];
+   const int tx = (int)threadIdx.x, ty = (int)threadIdx.y;
+   const int nloc = min(TX, c.pd.nx - t.i0), mloc = min(TY, c.pd.ny - t.j0);
+   c.i = t.i0 + tx;
+   c.j = t.j0 + ty;
+   c.k = t.k;
+   c.st[0] = 1;
+   c.st[1] = c.pd.sy;
+   c.st[2] = c.pd.sz;
+   c.c0 = c.pd.cell_off + (c.i + a.g) + (long long)(c.j + a.g) * c.pd.sy +
+          (long long)(c.k + a.gz) * c.pd.sz;
+   const long long c0 = c.c0, sy = c.st[1];
+   if (ty < mloc) {
+      if (tx < nloc) fx[ty][tx] = phase_face_flux<NDIM, FLUX>(p, k, phi, q, a.ncg, c.st, 0, c0 - 1, c0);
+      if (tx == 0) {
+         const long long e = c0 + nloc;
+         fx[ty][nloc] = phase_face_flux<NDIM, FLUX>(p, k, phi, q, a.ncg, c.st, 0, e - 1, e);
+      }
+   }
+   if (tx < nloc) {
+      if (ty < mloc) fy[ty][tx] = phase_face_flux<NDIM, FLUX>(p, k, phi, q, a.ncg, c.st, 1, c0 - sy, c0);
+      if (ty == 0) {
+         const long long e = c0 + (long long)mloc * sy;
+         fy[mloc][tx] = phase_face_flux<NDIM, FLUX>(p, k, phi, q, a.ncg, c.st, 1, e - sy, e);
+      }
+   }
+   __syncthreads();
+   if (tx >= nloc || ty >= mloc) return;
+   double r = (fx[ty][tx + 1] - fx[ty][tx]) * k.dxinv[0];
+   r = r + (fy[ty + 1][tx] - fy[ty][tx]) * k.dxinv[1];
+   if (NDIM == 3) {
+      const long long sz = c.st[2];
+      double flo = phase_face_flux<NDIM, FLUX>(p, k, phi, q, a.ncg, c.st, 2, c0 - sz, c0);
+      double fhi = phase_face_flux<NDIM, FLUX>(p, k, phi, q, a.ncg, c.st, 2, c0, c0 + sz);
+      r = r + (fhi - flo) * k.dxinv[2];
+   }
+   phase_rhs_tail<NDIM, FE>(a, p, k, c, r, phi, q, cl, cs, rhs);
 }
 
 // ------------------------------------------------------------ quat RHS
@@ -845,25 +927,48 @@ static void launch_phase_rhs_fe(const Level& L, const ampe_model_params& p, int 
                                 const double* phi, const double* q, const double* cl,
                                 const double* cs, double* rhs, hipStream_t s)
 {
-   dim3 g(L.ncol_tiles), b = tile_block();
-   LevelArgs a = level_args_col(L);
    RhsConst k = rhs_const(p);
-   switch (fe) {
-   case AMPE_FE_QUADRATIC:
-      hipLaunchKernelGGL((k_phase_rhs<NDIM, FLUX, AMPE_FE_QUADRATIC>), g, b, 0, s, a, p, k, phi, q,
-                         cl, cs, rhs);
-      break;
-   case AMPE_FE_CALPHAD_BINARY:
-      hipLaunchKernelGGL((k_phase_rhs<NDIM, FLUX, AMPE_FE_CALPHAD_BINARY>), g, b, 0, s, a, p, k, phi,
-                         q, cl, cs, rhs);
-      break;
-   case AMPE_FE_KKS_DILUTE:
-      hipLaunchKernelGGL((k_phase_rhs<NDIM, FLUX, AMPE_FE_KKS_DILUTE>), g, b, 0, s, a, p, k, phi, q,
-                         cl, cs, rhs);
-      break;
-   default:
-      hipLaunchKernelGGL((k_phase_rhs<NDIM, FLUX, kFeNone>), g, b, 0, s, a, p, k, phi, q, cl, cs,
-                         rhs);
+   if constexpr (FLUX == AMPE_FLUX_ANISOTROPIC) {
+      // each face flux once, through LDS
+      dim3 g(L.ncell_tiles), b = tile_block();
+      LevelArgs a = level_args(L);
+      switch (fe) {
+      case AMPE_FE_QUADRATIC:
+         hipLaunchKernelGGL((k_phase_rhs_faces<NDIM, FLUX, AMPE_FE_QUADRATIC>), g, b, 0, s, a, p,
+                            k, phi, q, cl, cs, rhs);
+         break;
+      case AMPE_FE_CALPHAD_BINARY:
+         hipLaunchKernelGGL((k_phase_rhs_faces<NDIM, FLUX, AMPE_FE_CALPHAD_BINARY>), g, b, 0, s, a,
+                            p, k, phi, q, cl, cs, rhs);
+         break;
+      case AMPE_FE_KKS_DILUTE:
+         hipLaunchKernelGGL((k_phase_rhs_faces<NDIM, FLUX, AMPE_FE_KKS_DILUTE>), g, b, 0, s, a, p,
+                            k, phi, q, cl, cs, rhs);
+         break;
+      default:
+         hipLaunchKernelGGL((k_phase_rhs_faces<NDIM, FLUX, kFeNone>), g, b, 0, s, a, p, k, phi, q,
+                            cl, cs, rhs);
+      }
+   } else {
+      dim3 g(L.ncol_tiles), b = tile_block();
+      LevelArgs a = level_args_col(L);
+      switch (fe) {
+      case AMPE_FE_QUADRATIC:
+         hipLaunchKernelGGL((k_phase_rhs<NDIM, FLUX, AMPE_FE_QUADRATIC>), g, b, 0, s, a, p, k, phi, q,
+                            cl, cs, rhs);
+         break;
+      case AMPE_FE_CALPHAD_BINARY:
+         hipLaunchKernelGGL((k_phase_rhs<NDIM, FLUX, AMPE_FE_CALPHAD_BINARY>), g, b, 0, s, a, p, k, phi,
+                            q, cl, cs, rhs);
+         break;
+      case AMPE_FE_KKS_DILUTE:
+         hipLaunchKernelGGL((k_phase_rhs<NDIM, FLUX, AMPE_FE_KKS_DILUTE>), g, b, 0, s, a, p, k, phi, q,
+                            cl, cs, rhs);
+         break;
+      default:
+         hipLaunchKernelGGL((k_phase_rhs<NDIM, FLUX, kFeNone>), g, b, 0, s, a, p, k, phi, q, cl, cs,
+                            rhs);
+      }
    }
 }
 

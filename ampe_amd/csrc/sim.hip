@@ -1121,8 +1121,10 @@ static int spgmr(Sim* S, double gamma, const double* bvec, double* x, double del
             return -1;
          if (rhs_eval(S, S->work2, S->jv, true)) return -1;
          double siginv = 1.0 / sig;
+         // with the fused FP32 preconditioner input the FP64 (I - gamma J) v is
+         // not stored: the dot kernel writes V[l+1] from the preconditioned vector
          if (kry_finish_scaled(L, nc, siginv, gamma, S->V[l], hs[l], S->ewt, S->jv, S->fcur,
-                               S->V[l + 1], pin, st))
+                               fuse_out ? nullptr : S->V[l + 1], pin, st))
             return -1;
       } else {
          double ss;
