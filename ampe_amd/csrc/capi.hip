@@ -453,6 +453,27 @@ int ampe_fac_residual(const ampe_level* level, const ampe_elliptic_op* op, const
    return 0;
 }
 
+int ampe_fac_residual_restrict(const ampe_level* coarse, const ampe_level* fine,
+                               const ampe_elliptic_op* op, const double* rhs, double* u,
+                               double* res, double* b, int red_only, ampe_stream_t stream)
+{
+   AMPE_REQUIRE(coarse && fine && op && rhs && u && b, "null argument");
+   AMPE_REQUIRE(ampe::is_coarsening(*coarse->L, *fine->L),
+                "coarse level is not the fine level halved");
+   const Level& F = *fine->L;
+   hipStream_t s = hs(stream);
+   ampe::EllOp<double> o = to_op(op);
+   if (ampe::ghost_fill(F, u, 1, s)) return -1;
+   if (red_only) {
+      AMPE_REQUIRE(ampe::red_residual_ok(F, 1),
+                   "red_only needs a periodic fine level with even, non-negative patch corners");
+      return ampe::launch_ell_resrestrict<double>(*coarse->L, F, o, op->dx, rhs, u, b, s);
+   }
+   AMPE_REQUIRE(res, "null residual buffer");
+   if (ampe::launch_residual<double>(F, o, op->dx, rhs, u, res, nullptr, s)) return -1;
+   return ampe::mg_restrict_cell<double>(*coarse->L, F, 1, res, b, s);
+}
+
 int ampe_vec_dot(const ampe_level* level, int ncomp, const double* x, const double* y,
                  double* out, ampe_stream_t stream)
 {

@@ -42,6 +42,14 @@ int launch_quat_residual(const Level& L, const T* fc, const T* sm, double gamma,
                          const double* dx, int ncomp, const T* rhs, const T* z, T* res,
                          double* part, hipStream_t s, bool split = false);
 
+// b (on coarse, the fine boxes halved) = the restriction of the elliptic
+// residual rhs - A u on fine after red-black sweeps ending with the black
+// colour: the red cells' residuals alone, summed per coarse cell (periodic
+// levels with one checkerboard; the caller checks).
+template <typename T>
+int launch_ell_resrestrict(const Level& coarse, const Level& fine, const EllOp<T>& op,
+                           const double* dx, const T* rhs, const T* u, T* b, hipStream_t s);
+
 // b (on coarse, the fine boxes halved) = the restriction of the residual
 // rhs - A z on fine when z comes out of red-black sweeps ending with the black
 // colour: the red cells' residuals alone (the black ones vanish), summed per

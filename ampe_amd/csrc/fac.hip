@@ -194,6 +194,49 @@ __global__ void __launch_bounds__(256) k_residual(LevelArgs a, OpArgs<T> op, T d
    if (partial) block_store_sum<T>(sq, partial);
 }
 
+// The elliptic residual restricted from the red cells only: after red-black
+// sweeps ending with the black colour each black row's residual vanishes (its
+// update zeroes it, and the row couples the cell to red neighbours only), so
+// the coarse right-hand side is, per coarse cell, the mean of its red
+// children's residuals in k_restrict_cell's summation order.  Periodic levels
+// with one checkerboard (the caller checks).
+template <typename T, int NDIM>
+__global__ void __launch_bounds__(256) k_ell_resrestrict(LevelArgs ca, LevelArgs fa,
+                                                         OpArgs<T> op, T di0, T di1, T di2,
+                                                         const T* __restrict__ rhs,
+                                                         const T* __restrict__ u,
+                                                         T* __restrict__ b)
+{
+   CellPos p;
+   if (!cell_pos(ca, 0, p)) return;
+   const T dinv[3] = {di0, di1, di2};
+   const PatchDev fp = fa.P[ca.T[tile_index(ca)].patch];
+   T pr[4];
+#pragma unroll
+   for (int dz = 0; dz < (NDIM == 3 ? 2 : 1); dz++)
+#pragma unroll
+      for (int dy = 0; dy < 2; dy++) {
+         CellPos c;
+         c.pd = fp;
+         c.j = 2 * p.j + dy;
+         c.k = NDIM == 3 ? 2 * p.k + dz : 0;
+         int gj = fp.lo[1] + c.j;
+         int x = (NDIM == 3) ? fp.lo[0] + gj + (fp.lo[2] + c.k) : fp.lo[1] + gj;
+         int io = ((x - (x / 2) * 2) != 0) ? 1 : 0;  // red cells of the row (cell_pos_color)
+         c.i = 2 * p.i + io;
+         c.st[0] = 1;
+         c.st[1] = fp.sy;
+         c.st[2] = fp.sz;
+         c.c0 = fp.cell_off + (c.i + fa.g) + (long long)(c.j + fa.g) * fp.sy +
+                (long long)(c.k + fa.gz) * fp.sz;
+         T diag;
+         pr[dy + 2 * dz] = op_residual<T, NDIM>(fa, c, op, rhs, u, dinv, &diag);
+      }
+   T sum = pr[0] + pr[1];
+   if (NDIM == 3) sum = sum + (pr[2] + pr[3]);
+   b[p.c0] = sum * (NDIM == 3 ? T(0.125) : T(0.25));
+}
+
 // ------------------------------------------------------------ orientation
 // Row of A = I + gamma sqrt(M) L sqrt(M), L from face coefficients:
 // off-diagonal fc/h^2, diagonal -sum fc/h^2 (set_j_ij2d/3d, set_stencil2d/3d,
@@ -795,6 +838,24 @@ int launch_residual(const Level& L, const EllOp<T>& op, const double* dx, const 
 }
 
 template <typename T>
+int launch_ell_resrestrict(const Level& coarse, const Level& fine, const EllOp<T>& op,
+                           const double* dx, const T* rhs, const T* u, T* b, hipStream_t s)
+{
+   OpArgs<T> oa{op.D, op.Dc, op.C, op.Cc, op.M, op.Mc};
+   T di[3];
+   recip(dx, di);
+   LevelArgs ca = level_args(coarse), fa = level_args(fine);
+   if (fine.ndim == 3)
+      hipLaunchKernelGGL((k_ell_resrestrict<T, 3>), dim3(coarse.ncell_tiles), tblock(), 0, s, ca,
+                         fa, oa, di[0], di[1], di[2], rhs, u, b);
+   else
+      hipLaunchKernelGGL((k_ell_resrestrict<T, 2>), dim3(coarse.ncell_tiles), tblock(), 0, s, ca,
+                         fa, oa, di[0], di[1], di[2], rhs, u, b);
+   AMPE_HIP_CHECK(hipGetLastError());
+   return 0;
+}
+
+template <typename T>
 int launch_quat_rbgs(const Level& L, const T* fc, const T* sm, double gamma, const double* dx,
                      int ncomp, const T* rhs, T* z, int color, hipStream_t s, bool split)
 {
@@ -921,6 +982,8 @@ int launch_quat_resrestrict(const Level& coarse, const Level& fine, const T* fc,
    template int launch_quat_residual<T>(const Level&, const T*, const T*, double,         \
                                         const double*, int, const T*, const T*, T*,      \
                                         double*, hipStream_t, bool);                     \
+   template int launch_ell_resrestrict<T>(const Level&, const Level&, const EllOp<T>&,     \
+                                          const double*, const T*, const T*, T*, hipStream_t); \
    template int launch_quat_resrestrict<T>(const Level&, const Level&, const T*, const T*, \
                                            double, const double*, int, const T*,         \
                                            const T*, T*, hipStream_t, bool);             \

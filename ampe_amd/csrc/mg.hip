@@ -469,9 +469,14 @@ int mg_vcycle_ell(Mg<T>& m, const EllOp<T>& op0, int kind, const MgCycle& cy, co
       }
       if (smooth_ell<T>(*c.L, op, c.dx, b, x, cy.nu1, s)) return -1;
       if (ghost_fill(*c.L, x, 1, s)) return -1;
-      if (launch_residual<T>(*c.L, op, c.dx, b, x, c.r, nullptr, s)) return -1;
       MgLevel<T>& n = m.lv[l + 1];
-      if (mg_restrict_cell<T>(*n.L, *c.L, 1, c.r, n.b, s)) return -1;
+      if (cy.nu1 >= 1 && red_residual_ok(*c.L, 1)) {
+         // the sweeps end with the black colour: restrict the red residuals
+         if (launch_ell_resrestrict<T>(*n.L, *c.L, op, c.dx, b, x, n.b, s)) return -1;
+      } else {
+         if (launch_residual<T>(*c.L, op, c.dx, b, x, c.r, nullptr, s)) return -1;
+         if (mg_restrict_cell<T>(*n.L, *c.L, 1, c.r, n.b, s)) return -1;
+      }
       if (fill_zero<T>(n.x, n.L->ncg, s)) return -1;
    }
    // ascend

@@ -95,6 +95,8 @@ _SIGS = {
     "ampe_coarsen_cell": (_i, [_vp, _vp, _i, _vp, _vp, _vp]),
     "ampe_coarsen_side": (_i, [_vp, _vp, _vp, _vp, _vp]),
     "ampe_refine_add": (_i, [_vp, _vp, _i, _i, _vp, _vp, _vp]),
+    "ampe_fac_residual_restrict": (_i, [_vp, _vp, C.POINTER(EllipticOp), _vp, _vp, _vp, _vp, _i,
+                                        _vp]),
     "ampe_conc_coeffs_ebs_mobility": (_i, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "ampe_quat_residual_restrict": (_i, [_vp, _vp, _vp, _vp, C.c_double, _vp, _i, _vp, _vp,
                                          _vp, _vp, _i, _vp]),

@@ -572,6 +572,19 @@ int ampe_fac_residual(const ampe_level* level, const ampe_elliptic_op* op,
                       const double* rhs, double* u, double* res, double* rms,
                       ampe_stream_t stream);
 
+/* Coarse right-hand side of a FAC block's multigrid: b (on coarse, the fine
+ * boxes halved) = the mean over each coarse cell's children of the fine
+ * residual rhs - A u (u ghost-filled here): computeCompositeResidualOnLevel
+ * followed by restrictResidual (source/EllipticFACOps.cc:1593, 2145-2276,
+ * 948-1041).  red_only = 0: every cell's residual to res, then the
+ * restriction.  red_only = 1: u comes out of red-black sweeps ending with the
+ * black colour, whose residual vanishes on black cells, so only the red
+ * cells' residuals are formed, per coarse cell (res unused); needs a periodic
+ * fine level with even, non-negative patch corners. */
+int ampe_fac_residual_restrict(const ampe_level* coarse, const ampe_level* fine,
+                               const ampe_elliptic_op* op, const double* rhs, double* u,
+                               double* res, double* b, int red_only, ampe_stream_t stream);
+
 /* ------------------------------------------------ level transfer (ratio 2) */
 /* coarse must hold the boxes of fine halved, patch by patch (even lower
  * corners and extents).  Cell fields of ncomp components; side fields of one.

@@ -178,3 +178,53 @@ def test_red_residual_restriction(dev, shape, split):
     assert H.lib().ampe_quat_residual_restrict(coarse_b.h, fine_b.h, ptr(fc), ptr(sm), 0.3, dx,
                                                ncomp, ptr(rhs), ptr(z), None, ptr(b_red), 1,
                                                None) != 0
+
+
+@pytest.mark.parametrize("shape,split", [((48, 64), 20), ((16, 24, 32), None), ((16, 24, 32), 12)])
+@pytest.mark.parametrize("variant", ["phase", "conc"])
+def test_red_residual_restriction_elliptic(dev, shape, split, variant):
+    """The same for the phase block (C and M cell fields, constant D) and the
+    composition block (D face field): after red-black sweeps the black
+    residuals are rounding noise, and the red-only coarse right-hand side
+    (ampe_fac_residual_restrict, red_only = 1) matches the full one."""
+    import ctypes as C
+    import torch
+    from ampe_amd import hip as H
+    ndim = len(shape)
+    ext = list(shape[::-1]) + [1] * (3 - ndim)
+    cuts = [(0, ext[0] - 1)] if split is None else [(0, split - 1), (split, ext[0] - 1)]
+    boxes = [_box((lo, 0, 0), (hi, ext[1] - 1, ext[2] - 1)) for lo, hi in cuts]
+    full = H.boxes_of(shape, ndim)
+    fine = H.Level(ndim, boxes, 1, periodic=full)
+    coarse = H.Level(ndim, _halved(boxes), 1, periodic=_halved([full])[0])
+    rng = np.random.default_rng(9)
+    op = H.EllipticOp()
+    keep = []
+    if variant == "phase":
+        Cf = torch.from_numpy(fine.to_ghosted(rng.uniform(1.0, 2.0, shape))).to(dev)
+        Mf = torch.from_numpy(fine.to_ghosted(rng.uniform(0.5, 1.5, shape))).to(dev)
+        op.C, op.M = Cf.data_ptr(), Mf.data_ptr()
+        op.D_const, op.C_const, op.M_const = -0.7, 1.0, 1.0
+        keep += [Cf, Mf]
+    else:
+        Df = torch.from_numpy(-rng.uniform(0.2, 2.0, fine.nside_tot)).to(dev)
+        op.D = Df.data_ptr()
+        op.D_const, op.C_const, op.M_const = 0.0, 1.0, 1.3
+        keep.append(Df)
+    op.dx = (C.c_double * 3)(0.5, 0.25, 0.75)
+    rhs = torch.from_numpy(fine.to_ghosted(rng.normal(size=shape))).to(dev)
+    u = torch.from_numpy(fine.to_ghosted(rng.normal(size=shape))).to(dev)
+    ptr = lambda t: C.c_void_p(t.data_ptr())  # noqa: E731
+    H.check(H.lib().ampe_fac_smooth(fine.h, C.byref(op), ptr(rhs), ptr(u), 1, -1.0, None, None))
+    res = torch.zeros_like(u)
+    b_full = torch.zeros(coarse.ncg, dtype=torch.float64, device=dev)
+    b_red = torch.zeros_like(b_full)
+    H.check(H.lib().ampe_fac_residual_restrict(coarse.h, fine.h, C.byref(op), ptr(rhs), ptr(u),
+                                               ptr(res), ptr(b_full), 0, None))
+    H.check(H.lib().ampe_fac_residual_restrict(coarse.h, fine.h, C.byref(op), ptr(rhs), ptr(u),
+                                               None, ptr(b_red), 1, None))
+    inter = _domain(fine, res.cpu().numpy(), shape)
+    black = (np.indices(shape).sum(axis=0) % 2) == 1
+    assert np.abs(inter[black]).max() < 1e-12 * np.abs(inter[~black]).max()
+    bf, br = b_full.cpu().numpy(), b_red.cpu().numpy()
+    assert np.abs(br - bf).max() < 1e-12 * np.abs(bf).max()
