@@ -567,7 +567,9 @@ class HierSim:
         self.h = h
         self.hier = hier  # outlives the sim
         self.p, self.s = p, s
-        self.nc = 1 + (1 if p.with_conc else 0) + (p.qlen if p.evolve_quat else 0)
+        nord = p.norderp if p.norderp > 1 else 1
+        ncs = (2 if p.fe_model == P.FE_QUADRATIC_TERNARY_3PH else 1) if p.with_conc else 0
+        self.nc = nord + ncs + (p.qlen if p.evolve_quat else 0)
 
     def set_state(self, y):
         y = np.ascontiguousarray(y, dtype=np.float64)

@@ -91,6 +91,18 @@ void o_ebs3_coeffs(const OGeom* G, const ampe_model_params* p, int norder, const
                    double* DL, double* DA, double* DB);
 void o_conc_rhs_ebs3(const OGeom* G, const ampe_model_params* p, const double* ph,
                      const double* DL, const double* DA, const double* DB, double* rhs);
+void o_phase_flux_multiorder(const OGeom* G, const ampe_model_params* p, int norder,
+                             const double* phi, double* fl);
+void o_phase_rhs_multiorder_fx(const OGeom* G, const ampe_model_params* p, int norder,
+                               const double* fl, const double* phi, const double* M,
+                               double* rhs);
+void o_phase_rhs_tern3_fx(const OGeom* G, const ampe_model_params* p, int norder,
+                          const double* fl, const double* phi, const double* ph,
+                          const double* M, double* rhs);
+void o_ebs3_flux(const OGeom* G, const ampe_model_params* p, const double* ph, const double* DL,
+                 const double* DA, const double* DB, double* fl2);
+void o_conc_rhs_ebs3_fx(const OGeom* G, const ampe_model_params* p, const double* fl2,
+                        double* rhs);
 void o_phase_rhs_multiorder(const OGeom* G, const ampe_model_params* p, int norder,
                             const double* phi, const double* M, double* rhs);
 void o_project_phases(const OGeom* G, int norder, const double* phi, double* corr,
@@ -252,6 +264,12 @@ void o_hier_quat_rhs(const OHier* H, const ampe_model_params* p0, const double* 
                      const double* q, const double* n2, double* rhs);
 void o_hier_conc_coeffs(const OHier* H, const ampe_model_params* p0, const double* phi,
                         const double* cl, const double* cs, double* Dc, double* Dphi);
+void o_hier_phase_rhs_tern3(const OHier* H, const ampe_model_params* p0, int norder,
+                            const double* phi, const double* ph6, double* rhs);
+void o_hier_ebs3_coeffs(const OHier* H, const ampe_model_params* p0, int norder,
+                        const double* phi, double* DL, double* DA, double* DB);
+void o_hier_conc_rhs_ebs3(const OHier* H, const ampe_model_params* p0, const double* ph6,
+                          const double* DL, const double* DA, const double* DB, double* rhs);
 void o_hier_conc_rhs(const OHier* H, const ampe_model_params* p0, const double* c,
                      const double* phi, const double* Dc, const double* Dphi, double* rhs);
 double o_hier_comp_residual(const OHier* H, const OHBlock* B, double* u, const double* f,

@@ -143,6 +143,9 @@ def _setup(L):
         "o_hier_quat_rhs": (None, [C.c_void_p, _P, _dp, _dp, _dp, _dp]),
         "o_hier_conc_coeffs": (None, [C.c_void_p, _P, _dp, _dp, _dp, _dp, _dp]),
         "o_hier_conc_rhs": (None, [C.c_void_p, _P, _dp, _dp, _dp, _dp, _dp]),
+        "o_hier_phase_rhs_tern3": (None, [C.c_void_p, _P, C.c_int, _dp, _dp, _dp]),
+        "o_hier_ebs3_coeffs": (None, [C.c_void_p, _P, C.c_int, _dp, _dp, _dp, _dp]),
+        "o_hier_conc_rhs_ebs3": (None, [C.c_void_p, _P, _dp, _dp, _dp, _dp, _dp]),
         "o_sim_create_hier": (C.c_void_p, [_P, C.POINTER(SolverParams), C.c_void_p]),
         "o_sim_set_state_composite": (None, [C.c_void_p, _dp]),
         "o_sim_set_time": (None, [C.c_void_p, C.c_double, C.c_double]),
@@ -320,7 +323,9 @@ class HierSim:
         self.h = lib().o_sim_create_hier(C.byref(p), C.byref(s), hier.h)
         if not self.h:
             raise ValueError("o_sim_create_hier rejected the model")
-        self.nc = 1 + (1 if p.with_conc else 0) + (p.qlen if p.evolve_quat else 0)
+        nord = p.norderp if p.norderp > 1 else 1
+        ncs = (2 if p.fe_model == FE_TERN3 else 1) if p.with_conc else 0
+        self.nc = nord + ncs + (p.qlen if p.evolve_quat else 0)
 
     def set_state(self, y):
         y = np.ascontiguousarray(y, dtype=np.float64)

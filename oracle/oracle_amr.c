@@ -761,6 +761,96 @@ void o_hier_conc_rhs(const OHier* H, const ampe_model_params* p0, const double* 
    free(flux);
 }
 
+/* Composite three-phase multi-order phase RHS (ghosts of the orders filled on
+ * every level, ph6 the phase concentrations of every cell): finest to
+ * coarsest, every order's side fluxes of the level, the finer level's fluxes
+ * coarsened onto it (the norder-deep flux coarsening of
+ * MultiOrderRHSStrategy::evaluateRHS, source/MultiOrderRHSStrategy.cc:85-138),
+ * then the RHS per patch. */
+void o_hier_phase_rhs_tern3(const OHier* H, const ampe_model_params* p0, int norder,
+                            const double* phi, const double* ph6, double* rhs)
+{
+   double* flux = (double*)calloc((size_t)H->nf * norder, sizeof(double));
+   for (int l = H->nlev - 1; l >= 0; l--) {
+      ampe_model_params p = lev_params(H, p0, l);
+      for (int pp = H->pbeg[l]; pp < H->pbeg[l + 1]; pp++) {
+         double* fb_phi = pgather(H, pp, phi, norder);
+         double* fb = (double*)malloc(sizeof(double) * pnf(&H->G[pp]) * norder);
+         o_phase_flux_multiorder(&H->G[pp], &p, norder, fb_phi, fb);
+         fscatter(H, pp, fb, flux, norder);
+         free(fb);
+         free(fb_phi);
+      }
+      o_hier_coarsen_faces(H, l, flux, norder, H->nf);
+      for (int pp = H->pbeg[l]; pp < H->pbeg[l + 1]; pp++) {
+         double* pb = pgather(H, pp, phi, norder);
+         double* cb = pgather(H, pp, ph6, 6);
+         double* fb = fgather(H, pp, flux, norder);
+         double* rb = pgather(H, pp, rhs, norder);
+         o_phase_rhs_tern3_fx(&H->G[pp], &p, norder, fb, pb, cb, NULL, rb);
+         pscatter(H, pp, rb, rhs, norder);
+         free(rb);
+         free(fb);
+         free(cb);
+         free(pb);
+      }
+   }
+   free(flux);
+}
+
+/* Three-phase face diffusivities on every patch, then the finer levels'
+ * coarsened onto the coarser faces (as o_hier_conc_coeffs). */
+void o_hier_ebs3_coeffs(const OHier* H, const ampe_model_params* p0, int norder,
+                        const double* phi, double* DL, double* DA, double* DB)
+{
+   for (int l = 0; l < H->nlev; l++) {
+      ampe_model_params p = lev_params(H, p0, l);
+      for (int pp = H->pbeg[l]; pp < H->pbeg[l + 1]; pp++) {
+         double* pb = pgather(H, pp, phi, norder);
+         o_ebs3_coeffs(&H->G[pp], &p, norder, pb, DL + H->foff[pp], DA + H->foff[pp],
+                       DB + H->foff[pp]);
+         free(pb);
+      }
+   }
+   for (int l = H->nlev - 2; l >= 0; l--) {
+      o_hier_coarsen_faces(H, l, DL, 1, H->nf);
+      o_hier_coarsen_faces(H, l, DA, 1, H->nf);
+      o_hier_coarsen_faces(H, l, DB, 1, H->nf);
+   }
+}
+
+/* Composite three-phase EBS composition RHS of the two species: finest to
+ * coarsest, the level's fluxes, the finer fluxes coarsened onto it, then
+ * M_c div J per patch (QuatIntegrator::evaluateConcentrationRHS,
+ * source/QuatIntegrator.cc:2578-2690). */
+void o_hier_conc_rhs_ebs3(const OHier* H, const ampe_model_params* p0, const double* ph6,
+                          const double* DL, const double* DA, const double* DB, double* rhs)
+{
+   double* flux = (double*)calloc((size_t)H->nf * 2, sizeof(double));
+   for (int l = H->nlev - 1; l >= 0; l--) {
+      ampe_model_params p = lev_params(H, p0, l);
+      for (int pp = H->pbeg[l]; pp < H->pbeg[l + 1]; pp++) {
+         double* cb = pgather(H, pp, ph6, 6);
+         double* fb = (double*)malloc(sizeof(double) * pnf(&H->G[pp]) * 2);
+         o_ebs3_flux(&H->G[pp], &p, cb, DL + H->foff[pp], DA + H->foff[pp], DB + H->foff[pp],
+                     fb);
+         fscatter(H, pp, fb, flux, 2);
+         free(fb);
+         free(cb);
+      }
+      o_hier_coarsen_faces(H, l, flux, 2, H->nf);
+      for (int pp = H->pbeg[l]; pp < H->pbeg[l + 1]; pp++) {
+         double* fb = fgather(H, pp, flux, 2);
+         double* rb = pgather(H, pp, rhs, 2);
+         o_conc_rhs_ebs3_fx(&H->G[pp], &p, fb, rb);
+         pscatter(H, pp, rb, rhs, 2);
+         free(rb);
+         free(fb);
+      }
+   }
+   free(flux);
+}
+
 /* ---------------------------------------------------------------- FAC */
 
 /* the per-level operator of one preconditioner block */

@@ -1188,8 +1188,8 @@ static void hier_sync_fill(OSim* S, double* y)
 {
    const OHier* H = S->Hh;
    long ng = o_hier_ncg(H);
-   o_hier_coarsen(H, y, 1, 0);
-   if (S->ic >= 0) o_hier_coarsen(H, y + (long)S->ic * ng, 1, 0);
+   o_hier_coarsen(H, y, S->nord, 0);
+   if (S->ic >= 0) o_hier_coarsen(H, y + (long)S->ic * ng, S->ncs, 0);
    if (S->iq >= 0) o_hier_coarsen(H, y + (long)S->iq * ng, S->p.qlen, 1);
    o_hier_ghost_fill(H, y, S->nc, 2);
 }
@@ -1198,6 +1198,15 @@ static int hier_kks(OSim* S, const double* y)
 {
    const OHier* H = S->Hh;
    long ng = o_hier_ncg(H);
+   if (S->tern) {
+      /* closed form, cell by cell over the composite (ghosts included) */
+      OGeom Gc = *o_hier_geom(H, 0);
+      Gc.ncg = ng;
+      o_kks_tern3(&Gc, &S->p, S->nord, y, y + (long)S->ic * ng, S->ph6);
+      S->st.kks_calls++;
+      S->st.kks_cells += ng;
+      return 0;
+   }
    int np = o_hier_npatch(H);
    long* coff = (long*)malloc(sizeof(long) * np);
    o_hier_offsets(H, coff, NULL);
@@ -1230,6 +1239,13 @@ static int rhs_eval_hier(OSim* S, double* y, double* ydot, int fd)
    const double* phi = y;
    const double* q = (S->iq >= 0) ? y + (long)S->iq * ng : NULL;
    const double* c = (S->ic >= 0) ? y + (long)S->ic * ng : NULL;
+   if (S->tern) {
+      if (hier_kks(S, y)) return -1;
+      o_hier_phase_rhs_tern3(H, p, S->nord, phi, S->ph6, ydot);
+      if (recompute) o_hier_ebs3_coeffs(H, p, S->nord, phi, S->Dc, S->Dphi, S->Db);
+      o_hier_conc_rhs_ebs3(H, p, S->ph6, S->Dc, S->Dphi, S->Db, ydot + (long)S->ic * ng);
+      return 0;
+   }
    if (q && recompute) o_hier_quat_face_norm2(H, p, q, S->n2);
    if (c && hier_kks(S, y)) return -1;
    o_hier_phase_rhs(H, p, phi, q, S->cl, S->cs, ydot);
@@ -1263,6 +1279,8 @@ static int psetup_hier(OSim* S, double* y, double gamma)
       ampe_model_params p = *p0;
       for (int a = 0; a < 3; a++) p.dx[a] = G->dx[a];
       long o = coff[pp];
+      /* several orders: the well scale passed is 0 (C = 1) */
+      if (S->nord > 1) p.phi_well_scale = 0.0;
       if (kim) o_phase_mobility_field(G, &p, S->cl + o, S->cs + o, S->Mphi + o);
       o_phase_fac_C(G, &p, gamma, phi + o, kim ? S->Mphi + o : NULL, S->Cphi + o);
       if (q) {
@@ -1273,7 +1291,10 @@ static int psetup_hier(OSim* S, double* y, double gamma)
    free(coff);
    free(foff);
    if (c)
-      for (long f = 0; f < S->nf; f++) S->Dconc[f] = -gamma * S->Dc[f];
+      for (long f = 0; f < S->nf; f++) {
+         double d = S->tern ? ((0.0 + S->Dc[f]) + S->Dphi[f]) + S->Db[f] : S->Dc[f];
+         S->Dconc[f] = -gamma * d;
+      }
    if (q)
       for (long n = 0; n < ng; n++) S->sqrtm[n] = sqrt(S->mq[n]);
    S->gamma_setup = gamma;
@@ -1295,7 +1316,11 @@ static int psolve_hier(OSim* S, const double* r, double* z, double delta)
    OHBlock ph = {0, NULL, -S->gamma_setup * p->epsilon_phi * p->epsilon_phi, S->Cphi, 1.0,
                  p->phi_mobility_type == AMPE_PHI_MOB_KIM ? S->Mphi : NULL, p->phi_mobility,
                  NULL, 0.0, 1};
-   if (o_hier_fac_solve(H, &ph, &fp, r, z, 1, delta, NULL, NULL)) ret = 1;
+   /* one phase block per order (PhasePrecondSolve, source/QuatIntegrator.cc:3427-3444) */
+   for (int m = 0; m < S->nord; m++)
+      if (o_hier_fac_solve(H, &ph, &fp, r + (long)m * ng, z + (long)m * ng, 1, delta, NULL,
+                           NULL))
+         ret = 1;
    if (S->iq >= 0) {
       int ql = p->qlen;
       double* f = S->fac_u;
@@ -1309,10 +1334,11 @@ static int psolve_hier(OSim* S, const double* r, double* z, double delta)
       for (int m = 0; m < ql; m++)
          for (long n = 0; n < ng; n++) u[(long)m * ng + n] *= S->sqrtm[n];
    }
-   if (S->ic >= 0) {
+   /* one composition block per species */
+   for (int m = 0; m < S->ncs; m++) {
       OHBlock co = {0, S->Dconc, 0.0, NULL, 1.0, NULL, p->conc_mobility, NULL, 0.0, 1};
-      if (o_hier_fac_solve(H, &co, &fp, r + (long)S->ic * ng, z + (long)S->ic * ng, 1, delta,
-                           NULL, NULL))
+      if (o_hier_fac_solve(H, &co, &fp, r + (long)(S->ic + m) * ng, z + (long)(S->ic + m) * ng,
+                           1, delta, NULL, NULL))
          ret = 1;
    }
    return ret;
@@ -1346,9 +1372,10 @@ static void project_q(OSim* S, const double* q, double* corr, double* err)
 
 OSim* o_sim_create_hier(const ampe_model_params* p, const ampe_solver_params* s, OHier* H)
 {
-   if (p->with_conc && p->conc_rhs_form != AMPE_CONC_RHS_KKS) return NULL;
+   const int tern = p->fe_model == AMPE_FE_QUADRATIC_TERNARY_3PH;
+   if (p->with_conc && !tern && p->conc_rhs_form != AMPE_CONC_RHS_KKS) return NULL;
    if (p->evolve_quat && p->qlen > 4) return NULL;
-   if (p->norderp > 1) return NULL;
+   if (p->norderp > 1 && !tern) return NULL;
    OSim* S = o_sim_create(p, s, 1, 1, 1);
    if (!S) return NULL;
    /* resize every field to the composite */
@@ -1381,7 +1408,9 @@ OSim* o_sim_create_hier(const ampe_model_params* p, const ampe_solver_params* s,
    S->fac_u = vec(ng * ql);
    S->fac_e = vec(ng * ql);
    S->fac_r = vec(ng * ql);
-   double** fs[] = {&S->n2, &S->Dc, &S->Dphi, &S->Dconc, &S->fc, &S->fac_flux};
+   free(S->ph6);
+   S->ph6 = vec(6 * ng);
+   double** fs[] = {&S->n2, &S->Dc, &S->Dphi, &S->Dconc, &S->fc, &S->fac_flux, &S->Db};
    for (size_t i = 0; i < sizeof(fs) / sizeof(fs[0]); i++) {
       free(*fs[i]);
       *fs[i] = vec(nf);
@@ -1412,7 +1441,9 @@ void o_sim_set_state_composite(OSim* S, const double* y)
 {
    memcpy(S->y, y, sizeof(double) * S->nv);
    hier_sync_fill(S, S->y);
-   if (S->ic >= 0) {
+   if (S->tern) {
+      hier_kks(S, S->y);
+   } else if (S->ic >= 0) {
       long ng = o_hier_ncg(S->Hh);
       const double* cc = S->y + (long)S->ic * ng;
       memcpy(S->clr, cc, sizeof(double) * ng);
@@ -1444,8 +1475,8 @@ void o_sim_get_state_composite(OSim* S, double* y)
 {
    memcpy(y, S->y, sizeof(double) * S->nv);
    long ng = o_hier_ncg(S->Hh);
-   o_hier_coarsen(S->Hh, y, 1, 0);
-   if (S->ic >= 0) o_hier_coarsen(S->Hh, y + (long)S->ic * ng, 1, 0);
+   o_hier_coarsen(S->Hh, y, S->nord, 0);
+   if (S->ic >= 0) o_hier_coarsen(S->Hh, y + (long)S->ic * ng, S->ncs, 0);
    if (S->iq >= 0) o_hier_coarsen(S->Hh, y + (long)S->iq * ng, S->p.qlen, 1);
 }
 

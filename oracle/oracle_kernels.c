@@ -1217,14 +1217,24 @@ void o_conc_rhs_ebs_4th(const OGeom* G, const ampe_model_params* p, const double
  * 2 phi gamma (phi2sum - phi), m = 8 w (:51), gamma = 1.5
  * (source/QuatModelParameters.cc:40); then rhs *= M (the mobility cell field,
  * or the uniform M when M == NULL). */
-void o_phase_rhs_multiorder(const OGeom* G, const ampe_model_params* p, int norder,
-                            const double* phi, const double* M, double* rhs)
+/* The side fluxes of every order: the simple gradient flux with eps / sqrt(2)
+ * (source/PhaseFluxStrategyFactory.h:19-22), norder blocks of nf faces. */
+void o_phase_flux_multiorder(const OGeom* G, const ampe_model_params* p, int norder,
+                             const double* phi, double* fl)
 {
    long ng = G->ncg;
    long nf = G->nfx + G->nfy + G->nfz;
-   double* fl = (double*)malloc(sizeof(double) * nf * norder);
    double eps = p->epsilon_phi / sqrt(2.);
    for (int ip = 0; ip < norder; ip++) gradient_flux(G, eps, phi + ip * ng, fl + ip * nf);
+}
+
+/* computerhsmultiorder from given side fluxes (norder blocks of nf faces). */
+void o_phase_rhs_multiorder_fx(const OGeom* G, const ampe_model_params* p, int norder,
+                               const double* fl, const double* phi, const double* M,
+                               double* rhs)
+{
+   long ng = G->ncg;
+   long nf = G->nfx + G->nfy + G->nfz;
    double dinv[3] = {1.0 / G->dx[0], 1.0 / G->dx[1], 1.0 / G->dx[2]};
    double m = 8. * p->phi_well_scale, gamma = 1.5;
    double* s2 = (double*)malloc(sizeof(double) * ng);
@@ -1260,6 +1270,15 @@ void o_phase_rhs_multiorder(const OGeom* G, const ampe_model_params* p, int nord
             }
    }
    free(s2);
+}
+
+void o_phase_rhs_multiorder(const OGeom* G, const ampe_model_params* p, int norder,
+                            const double* phi, const double* M, double* rhs)
+{
+   long nf = G->nfx + G->nfy + G->nfz;
+   double* fl = (double*)malloc(sizeof(double) * nf * norder);
+   o_phase_flux_multiorder(G, p, norder, phi, fl);
+   o_phase_rhs_multiorder_fx(G, p, norder, fl, phi, M, rhs);
    free(fl);
 }
 

@@ -145,13 +145,16 @@ double o_tern3_driving_force(const ampe_model_params* p, int norder, const doubl
 /* MultiOrderRHSStrategy::evaluateRHS with the ternary three-phase free
  * energy: the interface terms of o_phase_rhs_multiorder, minus the driving
  * force, times the mobility (field M or the uniform phi_mobility) */
-void o_phase_rhs_tern3(const OGeom* G, const ampe_model_params* p, int norder,
-                       const double* phi, const double* ph, const double* M, double* rhs)
+/* The three-phase phase RHS from given per-order side fluxes (norder blocks
+ * of nf faces, the composite fluxes on a hierarchy). */
+void o_phase_rhs_tern3_fx(const OGeom* G, const ampe_model_params* p, int norder,
+                          const double* fl, const double* phi, const double* ph,
+                          const double* M, double* rhs)
 {
    long ng = G->ncg;
    ampe_model_params q = *p;
    q.phi_mobility = 1.0;
-   o_phase_rhs_multiorder(G, &q, norder, phi, NULL, rhs);
+   o_phase_rhs_multiorder_fx(G, &q, norder, fl, phi, NULL, rhs);
    for (int ip = 0; ip < norder; ip++)
       for (int k = 0; k < G->nz; k++)
          for (int j = 0; j < G->ny; j++)
@@ -161,6 +164,16 @@ void o_phase_rhs_tern3(const OGeom* G, const ampe_model_params* p, int norder,
                r = r - tern3_driving(p, norder, phi, ng, c, ph, ip);
                rhs[(long)ip * ng + c] = r * (M ? M[c] : p->phi_mobility);
             }
+}
+
+void o_phase_rhs_tern3(const OGeom* G, const ampe_model_params* p, int norder,
+                       const double* phi, const double* ph, const double* M, double* rhs)
+{
+   long nf = G->nfx + G->nfy + G->nfz;
+   double* fl = (double*)malloc(sizeof(double) * nf * norder);
+   o_phase_flux_multiorder(G, p, norder, phi, fl);
+   o_phase_rhs_tern3_fx(G, p, norder, fl, phi, ph, M, rhs);
+   free(fl);
 }
 
 /* concentration_pfmdiffusion_of_temperature_threephases: D_a on every face,
@@ -207,18 +220,21 @@ void o_ebs3_coeffs(const OGeom* G, const ampe_model_params* p, int norder, const
  * dcA_k) + dxinv D_B dcB_k (add_flux for L, A then B; the zero off-diagonal
  * entries of the ternary diffusion matrices add nothing), then
  * rhs_k = M_c div flux_k (rhs holds two fields) */
-void o_conc_rhs_ebs3(const OGeom* G, const ampe_model_params* p, const double* ph,
-                     const double* DL, const double* DA, const double* DB, double* rhs)
+/* The two species' side fluxes of the three-phase EBS form,
+ * J_k = sum over L, A, B of D_a grad c_ak (2 blocks of nf faces). */
+void o_ebs3_flux(const OGeom* G, const ampe_model_params* p, const double* ph, const double* DL,
+                 const double* DA, const double* DB, double* fl2)
 {
+   (void)p;
    long st[3] = {1, G->sy, G->sz};
    long ng = G->ncg;
    double dinv[3] = {1.0 / G->dx[0], 1.0 / G->dx[1], 1.0 / G->dx[2]};
    long nf = G->nfx + G->nfy + G->nfz;
-   double* fl = (double*)malloc(sizeof(double) * nf);
    for (int sp = 0; sp < 2; sp++) {
       const double* cl = ph + (long)sp * ng;
       const double* ca = ph + (long)(2 + sp) * ng;
       const double* cb = ph + (long)(4 + sp) * ng;
+      double* fl = fl2 + (long)sp * nf;
       for (int dir = 0; dir < G->ndim; dir++) {
          int ex = G->nx + (dir == 0), ey = G->ny + (dir == 1), ez = G->nz + (dir == 2);
          long off = (dir == 0 ? 0 : dir == 1 ? YOFF : ZOFF);
@@ -234,6 +250,18 @@ void o_conc_rhs_ebs3(const OGeom* G, const ampe_model_params* p, const double* p
                   fl[fi] = f;
                }
       }
+   }
+}
+
+/* M_c div J_k of the two species from given side fluxes. */
+void o_conc_rhs_ebs3_fx(const OGeom* G, const ampe_model_params* p, const double* fl2,
+                        double* rhs)
+{
+   long ng = G->ncg;
+   double dinv[3] = {1.0 / G->dx[0], 1.0 / G->dx[1], 1.0 / G->dx[2]};
+   long nf = G->nfx + G->nfy + G->nfz;
+   for (int sp = 0; sp < 2; sp++) {
+      const double* fl = fl2 + (long)sp * nf;
       for (int k = 0; k < G->nz; k++)
          for (int j = 0; j < G->ny; j++)
             for (int i = 0; i < G->nx; i++) {
@@ -244,5 +272,14 @@ void o_conc_rhs_ebs3(const OGeom* G, const ampe_model_params* p, const double* p
                rhs[(long)sp * ng + CI(i, j, k)] = p->conc_mobility * s;
             }
    }
+}
+
+void o_conc_rhs_ebs3(const OGeom* G, const ampe_model_params* p, const double* ph,
+                     const double* DL, const double* DA, const double* DB, double* rhs)
+{
+   long nf = G->nfx + G->nfy + G->nfz;
+   double* fl = (double*)malloc(sizeof(double) * nf * 2);
+   o_ebs3_flux(G, p, ph, DL, DA, DB, fl);
+   o_conc_rhs_ebs3_fx(G, p, fl, rhs);
    free(fl);
 }
