@@ -17,7 +17,8 @@ int hier_weight_sum(Hier* H, double* sum);
 int hier_nlev(const Hier* H);
 // coarsenData of an integrator state (phase, composition: CONSERVATIVE_COARSEN;
 // orientation: QuatWeightedAverage) and its ghosts by LINEAR_REFINE
-int hier_sim_sync_fill(Hier* H, double* y, int nc, int ic, int iq, int qlen, hipStream_t s);
+int hier_sim_sync_fill(Hier* H, double* y, int nc, int nord, int ic, int ncs, int iq, int qlen,
+                       hipStream_t s);
 int hier_ghost_fill(const Hier& H, double* f, int ncomp, int cf, hipStream_t s);
 int hier_coarsen(const Hier& H, double* f, int ncomp, int quatnorm, hipStream_t s);
 int hier_phase_rhs(const Hier& H, const ampe_model_params& p0, const double* phi,
@@ -31,6 +32,15 @@ int hier_conc_coeffs(const Hier& H, const ampe_model_params& p0, const double* p
                      const double* cl, const double* cs, double* Dc, double* Dphi, hipStream_t s);
 int hier_conc_rhs(const Hier& H, const ampe_model_params& p0, const double* c, const double* phi,
                   const double* Dc, const double* Dphi, double* rhs, hipStream_t s);
+// the three-phase multi-order model: composite phase RHS of every order,
+// the face diffusivities of the three phases and both species' EBS RHS
+int hier_phase_rhs_tern3(const Hier& H, const ampe_model_params& p0, int norder,
+                         const double* phi, const double* ph6, double* rhs, hipStream_t s);
+int hier_ebs3_coeffs(const Hier& H, const ampe_model_params& p0, int norder, const double* phi,
+                     double* DL, double* DA, double* DB, hipStream_t s);
+int hier_conc_rhs_ebs3(const Hier& H, const ampe_model_params& p0, const double* ph6,
+                       const double* DL, const double* DA, const double* DB, double* rhs,
+                       hipStream_t s);
 // regrid: data of a new hierarchy from an old one, and new level boxes from
 // the gradient detector (amr.hip)
 int hier_transfer(const Hier& Ho, const Hier& Hn, const double* fo, double* fn, int ncomp,

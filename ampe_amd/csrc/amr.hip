@@ -1050,6 +1050,62 @@ int hier_conc_rhs(const Hier& H, const ampe_model_params& p0, const double* c, c
    return 0;
 }
 
+// The three-phase multi-order model on the hierarchy (phase concentrations
+// ph6 of every cell already set): finest to coarsest, each level's fused
+// kernels, then the coarse-fine corrections with the finer level's fluxes;
+// the face diffusivities of the finer levels are coarsened onto the coarser
+// faces first, as for the KKS coefficients.
+int hier_phase_rhs_tern3(const Hier& H, const ampe_model_params& p0, int norder,
+                         const double* phi, const double* ph6, double* rhs, hipStream_t s)
+{
+   for (int l = H.nlev - 1; l >= 0; l--) {
+      ampe_model_params p = lev_params(H, p0, l);
+      if (launch_phase_rhs_tern3(*H.lev[l], p, norder, phi, ph6, nullptr, rhs, s)) return -1;
+      if (l + 1 < H.nlev) {
+         double af, ac;
+         corr_areas(H, l, af, ac);
+         if (launch_hier_tern3_phase_corr(H.ndim, H.cfc[l].p, (int)H.cfc[l].n, H.cff[l + 1].p,
+                                          p, lev_params(H, p0, l + 1), norder, H.ncg, phi, af,
+                                          ac, rhs, s))
+            return -1;
+      }
+   }
+   return 0;
+}
+
+int hier_ebs3_coeffs(const Hier& H, const ampe_model_params& p0, int norder, const double* phi,
+                     double* DL, double* DA, double* DB, hipStream_t s)
+{
+   for (int l = 0; l < H.nlev; l++)
+      if (launch_ebs3_coeffs(*H.lev[l], lev_params(H, p0, l), norder, phi, DL, DA, DB, s))
+         return -1;
+   for (int l = H.nlev - 2; l >= 0; l--) {
+      if (hier_coarsen_faces_level(H, l, DL, DL, 1, s)) return -1;
+      if (hier_coarsen_faces_level(H, l, DA, DA, 1, s)) return -1;
+      if (hier_coarsen_faces_level(H, l, DB, DB, 1, s)) return -1;
+   }
+   return 0;
+}
+
+int hier_conc_rhs_ebs3(const Hier& H, const ampe_model_params& p0, const double* ph6,
+                       const double* DL, const double* DA, const double* DB, double* rhs,
+                       hipStream_t s)
+{
+   for (int l = H.nlev - 1; l >= 0; l--) {
+      ampe_model_params p = lev_params(H, p0, l);
+      if (launch_conc_rhs_ebs3(*H.lev[l], p, ph6, DL, DA, DB, rhs, s)) return -1;
+      if (l + 1 < H.nlev) {
+         double af, ac;
+         corr_areas(H, l, af, ac);
+         if (launch_hier_ebs3_corr(H.ndim, H.cfc[l].p, (int)H.cfc[l].n, H.cff[l + 1].p, p,
+                                   lev_params(H, p0, l + 1), H.ncg, ph6, DL, DA, DB, af, ac,
+                                   rhs, s))
+            return -1;
+      }
+   }
+   return 0;
+}
+
 // ------------------------------------------------------------ FAC
 static int fac_grow(Hier& H, int ncomp)
 {
@@ -1360,10 +1416,11 @@ int hier_weight_sum(Hier* H, double* sum)
    return 0;
 }
 
-int hier_sim_sync_fill(Hier* H, double* y, int nc, int ic, int iq, int qlen, hipStream_t s)
+int hier_sim_sync_fill(Hier* H, double* y, int nc, int nord, int ic, int ncs, int iq, int qlen,
+                       hipStream_t s)
 {
-   if (hier_coarsen(*H, y, 1, 0, s)) return -1;
-   if (ic >= 0 && hier_coarsen(*H, y + (long long)ic * H->ncg, 1, 0, s)) return -1;
+   if (hier_coarsen(*H, y, nord, 0, s)) return -1;
+   if (ic >= 0 && hier_coarsen(*H, y + (long long)ic * H->ncg, ncs, 0, s)) return -1;
    if (iq >= 0 && hier_coarsen(*H, y + (long long)iq * H->ncg, qlen, 1, s)) return -1;
    return hier_ghost_fill(*H, y, nc, 2, s);
 }

@@ -297,6 +297,16 @@ int launch_hier_phase_corr(int ndim, const CfCorr* cc, int ncorr, const CfFace* 
                            const ampe_model_params& pc, const ampe_model_params& pf,
                            long long ncg, const double* phi, const double* q, const double* cl,
                            const double* cs, double af, double ac, double* rhs, hipStream_t s);
+// the three-phase multi-order model: every order's phase flux and both
+// species' EBS fluxes (phase mobility constant)
+int launch_hier_tern3_phase_corr(int ndim, const CfCorr* cc, int ncorr, const CfFace* ff,
+                                 const ampe_model_params& pc, const ampe_model_params& pf,
+                                 int norder, long long ncg, const double* phi, double af,
+                                 double ac, double* rhs, hipStream_t s);
+int launch_hier_ebs3_corr(int ndim, const CfCorr* cc, int ncorr, const CfFace* ff,
+                          const ampe_model_params& pc, const ampe_model_params& pf, long long ncg,
+                          const double* ph, const double* DL, const double* DA, const double* DB,
+                          double af, double ac, double* rhs, hipStream_t s);
 int launch_hier_conc_corr(int ndim, const CfCorr* cc, int ncorr, const CfFace* ff,
                           const ampe_model_params& pc, const ampe_model_params& pf,
                           const double* c, const double* phi, const double* Dc,

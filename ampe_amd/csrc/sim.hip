@@ -221,12 +221,15 @@ int sim_create_hier(const ampe_model_params& p, const ampe_solver_params& s, Hie
                 "(precond redblack, precision fp64, no graph capture)");
       return -1;
    }
-   if (p.with_conc && p.conc_rhs_form != AMPE_CONC_RHS_KKS) {
-      set_error("sim_create_hier: the hierarchy supports the KKS composition form");
+   const bool tern = p.fe_model == AMPE_FE_QUADRATIC_TERNARY_3PH;
+   if (p.with_conc && !tern && p.conc_rhs_form != AMPE_CONC_RHS_KKS) {
+      set_error("sim_create_hier: the hierarchy supports the KKS composition form and the "
+                "three-phase ternary EBS form");
       return -1;
    }
-   if (p.norderp > 1) {
-      set_error("sim_create_hier: one order parameter");
+   if (p.norderp > 1 && !tern) {
+      set_error("sim_create_hier: several order parameters with the three-phase ternary model "
+                "only");
       return -1;
    }
    if (s.kks_guess == AMPE_GUESS_SURROGATE) {
@@ -243,7 +246,9 @@ int sim_create_hier(const ampe_model_params& p, const ampe_solver_params& s, Hie
       delete S;
       return -1;
    }
-   S->wsum = sw * (double)(1 + (p.with_conc ? 1 : 0) + (p.evolve_quat ? p.qlen : 0));
+   const int nord = p.norderp > 1 ? p.norderp : 1;
+   const int ncs = p.with_conc ? (tern ? 2 : 1) : 0;
+   S->wsum = sw * (double)(nord + ncs + (p.evolve_quat ? p.qlen : 0));
    return sim_finish(S, out);
 }
 
@@ -406,7 +411,7 @@ int sim_set_state_composite(Sim* S, const double* y_host)
    hipStream_t st = S->stream;
    AMPE_HIP_CHECK(hipMemcpy(S->y, y_host, sizeof(double) * S->nv, hipMemcpyHostToDevice));
    int ql = S->iq >= 0 ? S->p.qlen : 0;
-   if (hier_sim_sync_fill(S->hier, S->y, S->nc, S->ic, S->iq, ql, st)) return -1;
+   if (hier_sim_sync_fill(S->hier, S->y, S->nc, S->nord, S->ic, S->ncs, S->iq, ql, st)) return -1;
    std::memset(&S->st, 0, sizeof(S->st));
    if (S->ic >= 0) {
       const double* cc = S->y + (long long)S->ic * S->ng;
@@ -458,8 +463,9 @@ int sim_get_state_composite(Sim* S, double* y_host)
    hipStream_t st = S->stream;
    AMPE_HIP_CHECK(hipMemcpyAsync(S->work, S->y, sizeof(double) * S->nv, hipMemcpyDeviceToDevice,
                                  st));
-   if (hier_coarsen(*S->hier, S->work, 1, 0, st)) return -1;
-   if (S->ic >= 0 && hier_coarsen(*S->hier, S->work + (long long)S->ic * S->ng, 1, 0, st))
+   if (hier_coarsen(*S->hier, S->work, S->nord, 0, st)) return -1;
+   if (S->ic >= 0 &&
+       hier_coarsen(*S->hier, S->work + (long long)S->ic * S->ng, S->ncs, 0, st))
       return -1;
    if (S->iq >= 0 &&
        hier_coarsen(*S->hier, S->work + (long long)S->iq * S->ng, S->p.qlen, 1, st))
@@ -509,10 +515,18 @@ static int hier_rhs_eval(Sim* S, double* y, double* ydot, bool fd)
    bool recompute = !fd || !S->s.lag_quat_sidegrad;
    S->st.nfe++;
    int ql = S->iq >= 0 ? p.qlen : 0;
-   if (hier_sim_sync_fill(S->hier, y, S->nc, S->ic, S->iq, ql, st)) return -1;
+   if (hier_sim_sync_fill(S->hier, y, S->nc, S->nord, S->ic, S->ncs, S->iq, ql, st)) return -1;
    const double* phi = y;
    const double* q = S->iq >= 0 ? y + (long long)S->iq * S->ng : nullptr;
    const double* c = S->ic >= 0 ? y + (long long)S->ic * S->ng : nullptr;
+   if (S->tern) {
+      if (kks(S, y)) return -1;
+      if (hier_phase_rhs_tern3(*S->hier, p, S->nord, phi, S->ph6, ydot, st)) return -1;
+      if (recompute && hier_ebs3_coeffs(*S->hier, p, S->nord, phi, S->Dc, S->Dphi, S->Db, st))
+         return -1;
+      return hier_conc_rhs_ebs3(*S->hier, p, S->ph6, S->Dc, S->Dphi, S->Db,
+                                ydot + (long long)S->ic * S->ng, st);
+   }
    if (q && recompute && hier_quat_face_norm2(*S->hier, p, q, S->n2, st)) return -1;
    if (c && kks(S, y)) return -1;
    if (hier_phase_rhs(*S->hier, p, phi, q, S->cl, S->cs, ydot, st)) return -1;
@@ -535,16 +549,26 @@ static int hier_psetup(Sim* S, double* y, double gamma)
    hipStream_t st = S->stream;
    S->st.npe++;
    int ql = S->iq >= 0 ? p.qlen : 0;
-   if (hier_sim_sync_fill(S->hier, y, S->nc, S->ic, S->iq, ql, st)) return -1;
+   if (hier_sim_sync_fill(S->hier, y, S->nc, S->nord, S->ic, S->ncs, S->iq, ql, st)) return -1;
    const double* phi = y;
    const double* q = S->iq >= 0 ? y + (long long)S->iq * S->ng : nullptr;
    const double* c = S->ic >= 0 ? y + (long long)S->ic * S->ng : nullptr;
    if (q && hier_quat_face_norm2(*S->hier, p, q, S->n2, st)) return -1;
    if (c && kks(S, y)) return -1;
-   if (hier_precond_coeffs(*S->hier, p, gamma, phi, S->cl, S->cs, S->n2, S->Mphi, S->Cphi,
+   // several orders: the well scale passed is 0, so C = 1
+   ampe_model_params pw = p;
+   if (S->nord > 1) pw.phi_well_scale = 0.0;
+   if (hier_precond_coeffs(*S->hier, pw, gamma, phi, S->cl, S->cs, S->n2, S->Mphi, S->Cphi,
                            q ? S->fc : nullptr, q ? S->sqrtm : nullptr, st))
       return -1;
-   if (c && vec_axpby(S->nf, -gamma, S->Dc, 0.0, S->Dc, S->Dconc, st)) return -1;
+   if (c && S->tern) {
+      // D_L + D_A + D_B of the coarsened face diffusivities
+      if (vec_axpby(S->nf, 1.0, S->Dc, 1.0, S->Dphi, S->Dconc, st)) return -1;
+      if (vec_axpby(S->nf, 1.0, S->Dconc, 1.0, S->Db, S->Dconc, st)) return -1;
+      if (vec_axpby(S->nf, -gamma, S->Dconc, 0.0, S->Dconc, S->Dconc, st)) return -1;
+   } else if (c && vec_axpby(S->nf, -gamma, S->Dc, 0.0, S->Dc, S->Dconc, st)) {
+      return -1;
+   }
    S->gamma_setup = gamma;
    return 0;
 }
@@ -565,9 +589,13 @@ static int hier_psolve(Sim* S, const double* r, double* z, double delta)
                       S->Cphi, 1.0,
                       p.phi_mobility_type == AMPE_PHI_MOB_KIM ? S->Mphi : nullptr,
                       p.phi_mobility, nullptr, 0.0, 1};
-   int r0 = hier_fac_solve(*S->hier, ph, fp, r, z, 1, delta, nullptr, nullptr, st);
-   if (r0 < 0) return -1;
-   ret |= r0;
+   // one phase block per order (PhasePrecondSolve, source/QuatIntegrator.cc:3427-3444)
+   for (int m = 0; m < S->nord; m++) {
+      int r0 = hier_fac_solve(*S->hier, ph, fp, r + (long long)m * ng, z + (long long)m * ng, 1,
+                              delta, nullptr, nullptr, st);
+      if (r0 < 0) return -1;
+      ret |= r0;
+   }
    if (S->iq >= 0) {
       ampe_fac_params qp{S->s.quat_max_cycles, 1, 1, S->s.quat_sweeps, -1.0};
       ampe_hier_block qb{1, S->fc, 0.0, nullptr, 1.0, nullptr, 1.0, S->sqrtm, S->gamma_setup, 0};
@@ -577,11 +605,12 @@ static int hier_psolve(Sim* S, const double* r, double* z, double delta)
       ret |= r1;
       if (vec_mul_comp(S, z + (long long)S->iq * ng, S->sqrtm, ql)) return -1;
    }
-   if (S->ic >= 0) {
+   // one composition block per species
+   for (int m = 0; m < S->ncs; m++) {
       ampe_hier_block co{0, S->Dconc, 0.0, nullptr, 1.0, nullptr, p.conc_mobility, nullptr, 0.0,
                          1};
-      int r2 = hier_fac_solve(*S->hier, co, fp, r + (long long)S->ic * ng,
-                              z + (long long)S->ic * ng, 1, delta, nullptr, nullptr, st);
+      const long long off = (long long)(S->ic + m) * ng;
+      int r2 = hier_fac_solve(*S->hier, co, fp, r + off, z + off, 1, delta, nullptr, nullptr, st);
       if (r2 < 0) return -1;
       ret |= r2;
    }

@@ -254,6 +254,93 @@ __global__ void __launch_bounds__(256) k_conc_rhs_ebs3(LevelArgs a, double dxi0,
    AMPE_COLUMN_END
 }
 
+// Composite fluxes on a patch hierarchy: the coarse level's fused kernel
+// used the coarse face flux on each coarse-fine face; one thread per listed
+// coarse cell replaces it in the divergence with the area-weighted average
+// of the finer level's fluxes (the side-data CONSERVATIVE_COARSEN of
+// MultiOrderRHSStrategy::evaluateRHS, source/MultiOrderRHSStrategy.cc:85-138,
+// and QuatIntegrator::evaluateConcentrationRHS, source/QuatIntegrator.cc:
+// 2610-2620), every order and both species.
+template <int NDIM>
+__global__ void __launch_bounds__(256) k_hier_tern3_phase_corr(
+    const CfCorr* __restrict__ cc, int n, const CfFace* __restrict__ ff, int norder,
+    long long ncg, double e2c0, double e2c1, double e2c2, double e2f0, double e2f1, double e2f2,
+    double dxi0, double dxi1, double dxi2, double mob, double af, double ac,
+    const double* __restrict__ phi, double* __restrict__ rhs)
+{
+   int t = blockIdx.x * blockDim.x + threadIdx.x;
+   if (t >= n) return;
+   const CfCorr& e = cc[t];
+   const int nf = NDIM == 3 ? 4 : 2;
+   const double e2c[3] = {e2c0, e2c1, e2c2}, e2f[3] = {e2f0, e2f1, e2f2};
+   const double dxi[3] = {dxi0, dxi1, dxi2};
+   for (int ip = 0; ip < norder; ip++) {
+      const double* f = phi + (long long)ip * ncg;
+      double delta = 0.0;
+      for (int k = 0; k < e.nface; k++) {
+         const int d = e.d[k];
+         const double fold = (f[e.c0[k]] - f[e.cm[k]]) * e2c[d];
+         double acc = 0.0;
+         for (int r = 0; r < nf; r++) {
+            const CfFace& g = ff[e.fine[k] + r];
+            acc = acc + (f[g.c0] - f[g.cm]) * e2f[d] * af;
+         }
+         const double t1 = (acc / ac - fold) * dxi[d];
+         delta = delta + (e.side[k] ? t1 : -t1);
+      }
+      rhs[(long long)ip * ncg + e.cell] = rhs[(long long)ip * ncg + e.cell] + mob * delta;
+   }
+}
+
+__device__ __forceinline__ double ebs3_face_flux(const double* __restrict__ cl,
+                                                 const double* __restrict__ ca,
+                                                 const double* __restrict__ cb, double dinv,
+                                                 double dl, double da, double db, long long c0,
+                                                 long long cm)
+{
+   double j = 0.0;
+   j = j + dinv * dl * (cl[c0] - cl[cm]);
+   j = j + dinv * da * (ca[c0] - ca[cm]);
+   j = j + dinv * db * (cb[c0] - cb[cm]);
+   return j;
+}
+
+template <int NDIM>
+__global__ void __launch_bounds__(256) k_hier_ebs3_corr(
+    const CfCorr* __restrict__ cc, int n, const CfFace* __restrict__ ff, long long ncg,
+    double dxc0, double dxc1, double dxc2, double dxf0, double dxf1, double dxf2, double mc,
+    double af, double ac, const double* __restrict__ ph, const double* __restrict__ DL,
+    const double* __restrict__ DA, const double* __restrict__ DB, double* __restrict__ rhs)
+{
+   int t = blockIdx.x * blockDim.x + threadIdx.x;
+   if (t >= n) return;
+   const CfCorr& e = cc[t];
+   const int nf = NDIM == 3 ? 4 : 2;
+   const double dic[3] = {dxc0, dxc1, dxc2}, dif[3] = {dxf0, dxf1, dxf2};
+   for (int sp = 0; sp < 2; sp++) {
+      const double* cl = ph + (long long)sp * ncg;
+      const double* ca = ph + (long long)(2 + sp) * ncg;
+      const double* cb = ph + (long long)(4 + sp) * ncg;
+      double delta = 0.0;
+      for (int k = 0; k < e.nface; k++) {
+         const int d = e.d[k];
+         const long long fo = e.face[k];
+         const double fold =
+             ebs3_face_flux(cl, ca, cb, dic[d], DL[fo], DA[fo], DB[fo], e.c0[k], e.cm[k]);
+         double acc = 0.0;
+         for (int r = 0; r < nf; r++) {
+            const CfFace& g = ff[e.fine[k] + r];
+            acc = acc + ebs3_face_flux(cl, ca, cb, dif[d], DL[g.face], DA[g.face], DB[g.face],
+                                       g.c0, g.cm) *
+                            af;
+         }
+         const double t1 = dic[d] * (acc / ac - fold);
+         delta = delta + (e.side[k] ? t1 : -t1);
+      }
+      rhs[(long long)sp * ncg + e.cell] = rhs[(long long)sp * ncg + e.cell] + mc * delta;
+   }
+}
+
 }  // namespace
 
 static int tern3_check(const ampe_model_params& p, int norder)
@@ -347,6 +434,63 @@ int launch_conc_rhs_ebs3(const Level& L, const ampe_model_params& p, const doubl
    else
       hipLaunchKernelGGL(k_conc_rhs_ebs3<2>, g, blk, 0, s, a, dxi[0], dxi[1], dxi[2],
                          p.conc_mobility, L.ncg, ph, DL, DA, DB, rhs);
+   AMPE_HIP_CHECK(hipGetLastError());
+   return 0;
+}
+
+static void tern3_e2dx(const ampe_model_params& p, int ndim, double e2dx[3], double dxi[3])
+{
+   const double eps = p.epsilon_phi / std::sqrt(2.);
+   const double e2 = eps * eps;
+   for (int d = 0; d < 3; d++) {
+      const double h = d < ndim ? p.dx[d] : 1.0;
+      e2dx[d] = e2 / h;
+      dxi[d] = 1.0 / h;
+   }
+}
+
+int launch_hier_tern3_phase_corr(int ndim, const CfCorr* cc, int ncorr, const CfFace* ff,
+                                 const ampe_model_params& pc, const ampe_model_params& pf,
+                                 int norder, long long ncg, const double* phi, double af,
+                                 double ac, double* rhs, hipStream_t s)
+{
+   if (ncorr <= 0) return 0;
+   double ec[3], ef[3], dc[3], df[3];
+   tern3_e2dx(pc, ndim, ec, dc);
+   tern3_e2dx(pf, ndim, ef, df);
+   dim3 g((ncorr + 255) / 256), b(256);
+   if (ndim == 3)
+      hipLaunchKernelGGL(k_hier_tern3_phase_corr<3>, g, b, 0, s, cc, ncorr, ff, norder, ncg,
+                         ec[0], ec[1], ec[2], ef[0], ef[1], ef[2], dc[0], dc[1], dc[2],
+                         pc.phi_mobility, af, ac, phi, rhs);
+   else
+      hipLaunchKernelGGL(k_hier_tern3_phase_corr<2>, g, b, 0, s, cc, ncorr, ff, norder, ncg,
+                         ec[0], ec[1], ec[2], ef[0], ef[1], ef[2], dc[0], dc[1], dc[2],
+                         pc.phi_mobility, af, ac, phi, rhs);
+   AMPE_HIP_CHECK(hipGetLastError());
+   return 0;
+}
+
+int launch_hier_ebs3_corr(int ndim, const CfCorr* cc, int ncorr, const CfFace* ff,
+                          const ampe_model_params& pc, const ampe_model_params& pf, long long ncg,
+                          const double* ph, const double* DL, const double* DA, const double* DB,
+                          double af, double ac, double* rhs, hipStream_t s)
+{
+   if (ncorr <= 0) return 0;
+   double dc[3], df[3];
+   for (int d = 0; d < 3; d++) {
+      dc[d] = 1.0 / pc.dx[d];
+      df[d] = 1.0 / pf.dx[d];
+   }
+   dim3 g((ncorr + 255) / 256), b(256);
+   if (ndim == 3)
+      hipLaunchKernelGGL(k_hier_ebs3_corr<3>, g, b, 0, s, cc, ncorr, ff, ncg, dc[0], dc[1],
+                         dc[2], df[0], df[1], df[2], pc.conc_mobility, af, ac, ph, DL, DA, DB,
+                         rhs);
+   else
+      hipLaunchKernelGGL(k_hier_ebs3_corr<2>, g, b, 0, s, cc, ncorr, ff, ncg, dc[0], dc[1],
+                         dc[2], df[0], df[1], df[2], pc.conc_mobility, af, ac, ph, DL, DA, DB,
+                         rhs);
    AMPE_HIP_CHECK(hipGetLastError());
    return 0;
 }

@@ -377,6 +377,21 @@ def _disk_state(n, ndim, r, w, c0, seed):
     return phi, c, q
 
 
+def _spheres3_state(n):
+    """Orders A0, A1, B and the liquid (1 - the others, clipped): spheres of
+    radius n/4 with tanh interfaces; c1 follows solid A, c2 solid B."""
+    x = np.arange(n) + 0.5
+    zz, yy, xx = np.meshgrid(x, x, x, indexing="ij")
+    phis = []
+    for cx, cy in ((0.33, 0.35), (0.69, 0.35), (0.5, 0.68)):
+        d = np.sqrt((xx - cx * n) ** 2 + (yy - cy * n) ** 2 + (zz - n / 2) ** 2)
+        phis.append(0.5 * (1.0 - np.tanh((d - n / 4) / 1.5)))
+    liq = np.clip(1.0 - phis[0] - phis[1] - phis[2], 0.0, 1.0)
+    c1 = 0.9 * (phis[0] + phis[1]) + 0.05
+    c2 = 0.9 * phis[2] + 0.05
+    return phis + [liq, c1, c2]
+
+
 INTEGRATOR_CASES = {
     # two grains, quadratic KKS, orientation field, periodic, one fine box
     "2d_two_grain": dict(ndim=2, n=32, periodic=True,
@@ -388,6 +403,18 @@ INTEGRATOR_CASES = {
                             levels=lambda n: [[((0, 0), (15, 31)), ((16, 0), (31, 31))],
                                               [((20, 20), (43, 35)), ((24, 36), (39, 43))]],
                             steps=4),
+    # Sintering: three-phase ternary, five order parameters, two species,
+    # periodic, two fine patches sharing an edge across the grain contacts
+    "2d_sintering": dict(ndim=2, n=32, periodic=True,
+                         levels=lambda n: [[((0, 0), (n - 1, n - 1))],
+                                           [((16, 8), (47, 31)), ((20, 32), (43, 55))]],
+                         steps=4),
+    # three-phase ternary in 3-D: two grains of solid A, one of B, the
+    # liquid, one fine box over the three contacts
+    "3d_ternary": dict(ndim=3, n=16, periodic=True,
+                       levels=lambda n: [[((0, 0, 0), (n - 1, n - 1, n - 1))],
+                                         [((6, 6, 8), (25, 25, 23))]],
+                       steps=2),
     # 3-D CALPHAD polycrystal, one fine box
     "3d_calphad": dict(ndim=3, n=16, periodic=True,
                        levels=lambda n: [[((0, 0, 0), (n - 1, n - 1, n - 1))],
@@ -420,6 +447,18 @@ def test_hier_integrator_matches_oracle(dev, case):
         s = P.solver_defaults(dt=1e-7, maxl=10, max_newton=5, atol=1e-6, rtol=1e-8)
         phi, c, q = _disk_state(n, 2, 6.0, 3.0, 0.05, 7)
         fields = [phi, c] + list(q)
+    elif case == "2d_sintering":
+        p, (phi, c), _ = P.sintering_deck()
+        p.dx[:] = [2 * d for d in p.dx]
+        s = P.solver_defaults(dt=1e-5, maxl=20, max_newton=8, atol=1e-6, rtol=1e-8)
+        fields = [0.25 * (f[0::2, 0::2] + f[1::2, 0::2] + f[0::2, 1::2] + f[1::2, 1::2])
+                  for f in list(phi) + list(c)]
+    elif case == "3d_ternary":
+        p = P.sintering_deck()[0]
+        p.ndim = 3
+        p.norderp, p.norderp_A = 4, 2
+        s = P.solver_defaults(dt=1e-5, maxl=20, max_newton=8, atol=1e-6, rtol=1e-8)
+        fields = _spheres3_state(n)
     else:
         p = P.polycrystal_params(ndim=3)
         s = P.solver_defaults(dt=1e-7, maxl=10, max_newton=5, atol=1e-6, rtol=1e-8)
