@@ -103,9 +103,8 @@ static int sim_check(const ampe_model_params& p, const ampe_solver_params& s)
                 "no composition and orientation fields");
       return -1;
    }
-   if (p.norderp > 1 && (s.precond != AMPE_PRECOND_REDBLACK || p.norderp + 2 > 8)) {
-      set_error("sim_create: several order parameters run the red-black preconditioner, "
-                "norderp <= 6");
+   if (p.norderp > 1 && s.precond != AMPE_PRECOND_REDBLACK) {
+      set_error("sim_create: several order parameters run the red-black preconditioner");
       return -1;
    }
    return 0;
@@ -114,6 +113,12 @@ static int sim_check(const ampe_model_params& p, const ampe_solver_params& s)
 int sim_create(const ampe_model_params& p, const ampe_solver_params& s, const ampe_box* boxes,
                int npatch, const ampe_box* periodic, Sim** out)
 {
+   // one level batches the preconditioner's blocks, at most eight
+   if (p.norderp + 2 > 8) {
+      set_error("sim_create: several order parameters on one level: norderp <= 6 (the "
+                "hierarchy integrator has no such limit)");
+      return -1;
+   }
    if (sim_check(p, s)) return -1;
    Sim* S = new Sim();
    S->p = p;

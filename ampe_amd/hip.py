@@ -510,6 +510,23 @@ class Hier:
         self.boxes = flat
         self.patch_level = [l for l, lev in enumerate(self.levels) for _ in lev]
 
+    def patch_blocks(self, y, comp=0):
+        """[(level, lo, hi, view)] per patch, in the composite order (levels,
+        then patches): view is the patch's interior cells ([nz,] ny, nx) of
+        component comp in the host composite array y, each patch a ghosted
+        block with one ghost cell per side along each direction of the
+        level."""
+        out, off = [], 0
+        for lev, (lo, hi) in zip(self.patch_level, self.boxes):
+            n = [hi[d] - lo[d] + 1 for d in range(self.ndim)]
+            sh = tuple(m + 2 for m in reversed(n))
+            size = int(np.prod(sh))
+            blk = y[comp * self.ncg + off:comp * self.ncg + off + size].reshape(sh)
+            out.append((lev, lo, hi, blk[(slice(1, -1),) * self.ndim]))
+            off += size
+        assert off == self.ncg, (off, self.ncg)
+        return out
+
     @staticmethod
     def _levels_of(h, ndim):
         nlev = _i()

@@ -375,6 +375,131 @@ def grains_report(phi, n, thr=0.5, reps=3):
     return dict(cells=n ** 3, threshold=thr, grains=ng.value, ms=best * 1e3)
 
 
+def amr_ternary_workload(n0, seed=1005):
+    """The shape of the 3-D two-level multi-order ternary workload, at level 0
+    = n0^3 (2 n0 finest-equivalent): eight Voronoi grains (four of solid A,
+    four of solid B, seeds uniform in the periodic domain) in liquid (solid
+    fraction 0.7, tanh interfaces of 3 level-0 cells), the liquid order
+    1 - the others; per grain and for the liquid a composition (c1, c2)
+    uniform on the slice c1 + c2 <= 0.6; three-phase quadratic wells with
+    curvatures A ~ U[1, 10] kJ/mol and equilibrium compositions ~ U[0.1, 0.5]
+    per phase and species; the Sintering deck's mobilities, interface energy
+    and EBS diffusion otherwise.  Level 1 is a fixed central region of
+    n0^3 fine cells (half the domain along each axis) in 2 x 2 x 2 patches.
+
+    Returns (model params, level boxes, composite state filler)."""
+    rng = np.random.Generator(np.random.Philox(seed))
+    p = P.sintering_deck()[0]
+    p.ndim = 3
+    p.norderp, p.norderp_A = 9, 4
+    for a in range(3):
+        for k in range(2):
+            p.tern_A[a][k] = rng.uniform(1.0, 10.0) * 1e3
+            p.tern_ceq[a][k] = rng.uniform(0.1, 0.5)
+    seeds = rng.uniform(0, n0, (8, 3))
+    comp = []
+    while len(comp) < 9:
+        c1, c2 = rng.uniform(0.0, 0.6, 2)
+        if c1 + c2 <= 0.6:
+            comp.append((c1, c2))
+    comp = np.array(comp)
+    h = n0 // 2
+    fine = [((h + i * h, h + j * h, h + k * h),
+             (h + i * h + h - 1, h + j * h + h - 1, h + k * h + h - 1))
+            for k in range(2) for j in range(2) for i in range(2)]
+    levels = [[((0, 0, 0), (n0 - 1, n0 - 1, n0 - 1))], fine]
+
+    def nearest(x, y, z):
+        best = np.full(x.shape, np.inf)
+        lab = np.zeros(x.shape, dtype=np.int64)
+        for g, (sx, sy, sz) in enumerate(seeds):
+            d2 = np.zeros(x.shape)
+            for v, sv in ((x, sx), (y, sy), (z, sz)):
+                dd = np.abs(v - sv)
+                dd = np.minimum(dd, n0 - dd)
+                d2 = d2 + dd * dd
+            m = d2 < best
+            best[m] = d2[m]
+            lab[m] = g
+        return np.sqrt(best), lab
+
+    # the distance threshold of solid fraction 0.7 on level 0
+    x0 = np.arange(n0) + 0.5
+    zz, yy, xx = np.meshgrid(x0, x0, x0, indexing="ij")
+    thr = float(np.quantile(nearest(xx, yy, zz)[0], 0.7))
+    del zz, yy, xx
+
+    def fill(hier, y):
+        nord = p.norderp
+        blocks = [hier.patch_blocks(y, m) for m in range(nord + 2)]
+        for b in range(len(blocks[0])):
+            lev, lo, hi = blocks[0][b][:3]
+            r = 0.5 ** lev
+            ax = [(np.arange(lo[d], hi[d] + 1) + 0.5) * r for d in range(3)]
+            z, yv, x = np.meshgrid(ax[2], ax[1], ax[0], indexing="ij")
+            dist, lab = nearest(x, yv, z)
+            solid = 0.5 * (1.0 - np.tanh((dist - thr) / 1.5))
+            tot = np.zeros(x.shape)
+            for g in range(8):
+                v = np.where(lab == g, solid, 0.0)
+                blocks[g][b][3][...] = v
+                tot += v
+            liq = 1.0 - tot
+            blocks[8][b][3][...] = liq
+            for k in range(2):
+                c = liq * comp[8, k]
+                for g in range(8):
+                    c = c + np.where(lab == g, solid, 0.0) * comp[g, k]
+                blocks[9 + k][b][3][...] = c
+
+    return p, levels, fill
+
+
+def amr_ternary_report(args, dev):
+    """Throughput of the FP64 hierarchy integrator on the two-level
+    multi-order ternary workload: cells = level-0 cells + level-1 cells."""
+    from ampe_amd import hip as H
+    import torch
+    n0 = args.amr_n
+    p, levels, fill = amr_ternary_workload(n0)
+    dx0 = tuple(2 * d for d in p.dx)
+    p.dx[:] = list(dx0)
+    hg = H.Hier(3, levels, (n0, n0, n0), dx0, True)
+    s = P.solver_defaults(dt=args.amr_dt, maxl=args.maxl, max_newton=5, atol=1e-4, rtol=1e-6)
+    sim = H.HierSim(p, s, hg)
+    y = np.zeros(sim.nc * hg.ncg)
+    t0 = time.perf_counter()
+    fill(hg, y)
+    sim.set_state(y)
+    del y
+    log(f"[bench] amr ternary: {len(levels[1])} fine patches, state in "
+        f"{time.perf_counter() - t0:.1f}s")
+    st = {"nst": 0}
+    for k in range(args.amr_warmup):
+        rc, st = sim.advance(1)
+        assert rc == 0, st
+        log(f"[bench] amr ternary warmup step {k}: {st}")
+    torch.cuda.synchronize()
+    n_before = st["nst"]
+    t0 = time.perf_counter()
+    for k in range(args.amr_steps):
+        rc, st = sim.advance(1)
+        assert rc == 0, st
+        log(f"[bench] amr ternary step {k}: t {st['t']:.3e} nni {st['nni']} nli {st['nli']}")
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    cells = n0 ** 3 + sum(int(np.prod([hi[d] - lo[d] + 1 for d in range(3)]))
+                          for lo, hi in levels[1])
+    nst = st["nst"] - n_before
+    return dict(workload=f"3-D two-level multi-order ternary: level 0 {n0}^3, level 1 "
+                         f"{len(levels[1])} patches ({cells - n0 ** 3} cells, "
+                         f"{2 * n0}^3 finest-equivalent), 9 orders, 2 species",
+                value=cells * nst / wall, unit="cells*steps/s", cells=cells, steps=nst,
+                ms_per_step=wall / max(nst, 1) * 1e3, dt=args.amr_dt, counters=st,
+                form="FP64 throughout (FAC over levels in FP64; the three-phase KKS is "
+                     "closed-form, so no surrogate)")
+
+
 def cpu_model():
     try:
         with open("/proc/cpuinfo") as f:
@@ -464,6 +589,14 @@ def main():
     ap.add_argument("--no-stencils", action="store_true")
     ap.add_argument("--graphs", type=int, default=1,
                     help="1: FP32 preconditioner solves replayed from a captured HIP graph")
+    ap.add_argument("--amr", action="store_true",
+                    help="also time the two-level multi-order ternary hierarchy (amr_ternary)")
+    ap.add_argument("--amr-only", action="store_true",
+                    help="time only the hierarchy workload and print its object")
+    ap.add_argument("--amr-n", type=int, default=256, help="level-0 cells per side")
+    ap.add_argument("--amr-steps", type=int, default=3)
+    ap.add_argument("--amr-warmup", type=int, default=1)
+    ap.add_argument("--amr-dt", type=float, default=1.0e-5)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU only (this build has no multi-GPU path); "
@@ -474,6 +607,10 @@ def main():
     torch.cuda.init()
     dev = torch.device("cuda:0")
     forms = [f for f in args.forms.split(",") if f]
+    if args.amr_only:
+        amr = amr_ternary_report(args, dev)
+        print(json.dumps({"amr_ternary": amr}), flush=True)
+        return
 
     n = args.n
     p, phi, c, q = make_workload(n)
@@ -563,6 +700,11 @@ def main():
     log(f"[bench] grains {grains}")
     copy_gbs = copy_bandwidth()
     log(f"[bench] device copy {copy_gbs:.0f} GB/s")
+    amr = None
+    if args.amr:
+        amr = amr_ternary_report(args, dev)
+        log(f"[bench] amr ternary {amr['value']:.4g} cells*steps/s, "
+            f"{amr['ms_per_step']:.1f} ms/step")
     cpu = None
     if not args.no_cpu:
         log("[bench] cpu baseline")
@@ -599,6 +741,7 @@ def main():
         "stencil_rooflines": stencils,
         "cpu_baseline": cpu,
         "grains": grains,
+        "amr_ternary": amr,
     }
     print(json.dumps(out), flush=True)
 
