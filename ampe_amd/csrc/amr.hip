@@ -73,7 +73,8 @@ struct Hier {
    // coarse-fine ghosts of level l (l > 0): dst, 9 sources, 3 weights
    DevVec<long long> cf_dst[kMaxLev], cf_src[kMaxLev];
    DevVec<double> cf_w[kMaxLev];
-   // cell coarsening onto level l (l < nlev-1): coarse cell, 8 fine cells
+   // cell coarsening onto level l (l < nlev-1): coarse cell; its first fine
+   // cell and the fine patch's row and plane strides (low / high 32 bits)
    DevVec<long long> crs_dst[kMaxLev], crs_src[kMaxLev];
    // face coarsening onto level l: coarse face, 4 fine faces, direction
    DevVec<long long> fcrs_dst[kMaxLev], fcrs_src[kMaxLev];
@@ -379,14 +380,14 @@ int hier_create(int ndim, int nlev, const int* npatch, const ampe_box* boxes, co
                   long long c0 = find_cell(*H, l, g);
                   if (c0 < 0) return hier_fail(H, "a level is not nested in the coarser one");
                   dst.push_back(c0);
-                  for (int r2 = 0; r2 < 2; r2++)
-                     for (int r1 = 0; r1 < 2; r1++)
-                        for (int r0 = 0; r0 < 2; r0++) {
-                           bool used = ndim == 3 || r2 == 0;
-                           src.push_back(used ? cidx(*H, p, 2 * ci + r0, 2 * cj + r1,
-                                                     ndim == 3 ? 2 * ck + r2 : 0)
-                                              : -1);
-                        }
+                  // the first child and the fine patch's strides: child
+                  // (r0, r1, r2) is base + r0 + r1 sy + r2 sz
+                  const long long base = cidx(*H, p, 2 * ci, 2 * cj, ndim == 3 ? 2 * ck : 0);
+                  const long long sy = cidx(*H, p, 2 * ci, 2 * cj + 1, ndim == 3 ? 2 * ck : 0) - base;
+                  const long long sz =
+                      ndim == 3 ? cidx(*H, p, 2 * ci, 2 * cj, 2 * ck + 1) - base : 0;
+                  src.push_back(base);
+                  src.push_back(sy | (sz << 32));
                }
          // faces of the coarsened box, written into every coarse patch holding them
          for (int d = 0; d < ndim; d++) {
@@ -698,12 +699,15 @@ __global__ void __launch_bounds__(256) k_coarsen(const double* __restrict__ fsrc
 {
    long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
    if (e >= n) return;
+   // children in x-fastest order from the first child and the fine strides
+   const long long base = src[2 * e], packed = src[2 * e + 1];
+   const long long sy = packed & 0xffffffffLL, sz = packed >> 32;
    double acc[4];
    for (int m = 0; m < ncomp; m++) {
-      const double* a = fsrc + m * ncg;
+      const double* a = fsrc + m * ncg + base;
       double s = 0.0;
       for (int c = 0; c < 8; c++)
-         if (c < nchild) s = s + a[src[8 * e + c]] * dvf;
+         if (c < nchild) s = s + a[(c & 1) + ((c >> 1) & 1) * sy + (c >> 2) * sz] * dvf;
       double v = s / dvc;
       if (m < 4) acc[m] = v;
       if (!quatnorm) f[m * ncg + dst[e]] = v;
