@@ -465,7 +465,8 @@ def amr_ternary_report(args, dev):
     dx0 = tuple(2 * d for d in p.dx)
     p.dx[:] = list(dx0)
     hg = H.Hier(3, levels, (n0, n0, n0), dx0, True)
-    s = P.solver_defaults(dt=args.amr_dt, maxl=args.maxl, max_newton=5, atol=1e-4, rtol=1e-6)
+    s = P.solver_defaults(dt=args.amr_dt, maxl=args.maxl, max_newton=5, atol=1e-4, rtol=1e-6,
+                          gstype=P.GS_CLASSICAL)
     sim = H.HierSim(p, s, hg)
     y = np.zeros(sim.nc * hg.ncg)
     t0 = time.perf_counter()
