@@ -24,6 +24,12 @@ sweep k_quat_rbgs_c<float,3,4> (all MG levels), is event-timed on the integrator
 stream during the timed steps; achieved = its algorithmic bytes / its time;
 traffic = its HBM bytes per launch from the committed rocprofv3 counter summary.
 
+AMR workloads (--amr 1, the default; amr_dendrite and amr_ternary in the
+line): the 2-D three-level dendrite with regrids every 10 steps (level 0
+512^2 in 81 uneven patches) and the 3-D two-level multi-order ternary
+hierarchy (level 0 256^3, 512^3 finest-equivalent), each timed with the
+FP64 hierarchy integrator; cells*steps/s over every level's cells.
+
 Forms:
   fp64_reference : FP64 everywhere, red-black block solves (the reference's
                    FAC "redblack" coarse solve), modified Gram-Schmidt,
@@ -501,6 +507,108 @@ def amr_ternary_report(args, dev):
                      "closed-form, so no surrogate)")
 
 
+def amr_dendrite_report(args, dev):
+    """The 2-D AMR dendrite workload: level 0 of 512^2 cells in 81 uneven
+    patches (40-64 cells a side), up to three levels (2048^2
+    finest-equivalent) built by the gradient detector and Berger-Rigoutsos
+    clustering (efficiency 0.7, patches 16-64 cells, tag buffer 2) at t = 0
+    and every --amr-dendrite-regrid steps; anisotropic phase flux (k = 4,
+    nu = 0.05), dilute binary KKS (k_e = 0.14), bounded domain; a solid disk
+    of radius 10 finest cells (tanh width 4 finest cells) at the centre with
+    N(0, 0.01^2) noise on phi inside the interface, c = 0.05 +- 0.002.  The
+    initial data are evaluated on every level of the first hierarchy; each
+    regrid then moves the state by copy and linear refinement and restarts
+    the integrator at the current time.  cells*steps counts every level's
+    cells of the hierarchy each step ran on; regrids are in the wall time."""
+    from ampe_amd import hip as H
+    import torch
+    n0, nlev, every = 512, 3, args.amr_dendrite_regrid
+    widths = [64, 48, 64, 56, 64, 40, 64, 48, 64]
+    edges = np.concatenate([[0], np.cumsum(widths)])
+    lev0 = [((int(edges[i]), int(edges[j])), (int(edges[i + 1] - 1), int(edges[j + 1] - 1)))
+            for j in range(len(widths)) for i in range(len(widths))]
+    p = P.dilute_params(ndim=2, dilute_keq=0.14)
+    p.flux_type = P.FLUX_ANISOTROPIC
+    p.aniso_nu = 0.05
+    p.aniso_knumber = 4
+    s = P.solver_defaults(dt=args.amr_dendrite_dt, maxl=10, max_newton=5, atol=1e-6, rtol=1e-8,
+                          gstype=P.GS_CLASSICAL)
+    cp = P.ClusterParams()
+    cp.efficiency_tol = 0.7
+    cp.min_size[:] = [16] * 3
+    cp.max_size[:] = [64] * 3
+    cp.tag_buffer = 2
+    tol = 0.05
+    rng = np.random.Generator(np.random.Philox(1004))
+    nc = 6
+
+    def fill(hier, y):
+        blocks = [hier.patch_blocks(y, m) for m in range(nc)]
+        for b in range(len(blocks[0])):
+            lev, lo, hi, _ = blocks[0][b]
+            r = 0.5 ** lev
+            xs = (np.arange(lo[0], hi[0] + 1) + 0.5) * r - n0 / 2
+            ys = (np.arange(lo[1], hi[1] + 1) + 0.5) * r - n0 / 2
+            yy, xx = np.meshgrid(ys, xs, indexing="ij")
+            d = np.sqrt(xx * xx + yy * yy) * 4.0   # finest cells
+            phi = 0.5 * (1.0 - np.tanh((d - 10.0) / 2.0))
+            mid = (phi > 0.05) & (phi < 0.95)
+            phi = np.clip(phi + 0.01 * rng.standard_normal(phi.shape) * mid, 0.0, 1.0)
+            blocks[0][b][3][...] = phi
+            blocks[1][b][3][...] = 0.05 + 0.002 * rng.uniform(-1, 1, phi.shape)
+            blocks[2][b][3][...] = 1.0
+            for m in (3, 4, 5):
+                blocks[m][b][3][...] = 0.0
+
+    def cells_of(hier):
+        return sum((hi[0] - lo[0] + 1) * (hi[1] - lo[1] + 1) for lo, hi in hier.boxes)
+
+    dx0 = tuple(p.dx)
+    h0 = H.Hier(2, [lev0], (n0, n0), dx0, False)
+    y = np.zeros(nc * h0.ncg)
+    fill(h0, y)
+    ty = torch.from_numpy(y).to(dev)
+    hg = h0.regrid(C.c_void_p(ty.data_ptr()), p, tol, cp, nlev)
+    y = np.zeros(nc * hg.ncg)
+    fill(hg, y)
+    log(f"[bench] amr dendrite: levels {[len(v) for v in hg.levels]} patches, "
+        f"{cells_of(hg)} cells")
+    sim = H.HierSim(p, s, hg)
+    sim.set_state(y)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    t_now, nst, cell_steps, nregrid = 0.0, 0, 0, 0
+    while nst < args.amr_dendrite_steps:
+        for _ in range(min(every, args.amr_dendrite_steps - nst)):
+            rc, st = sim.advance(1)
+            assert rc == 0, st
+            nst += 1
+            cell_steps += cells_of(hg)
+            t_now = st["t"]
+        log(f"[bench] amr dendrite: {nst} steps, t {t_now:.3e}, {cells_of(hg)} cells, "
+            f"levels {[len(v) for v in hg.levels]}")
+        if nst >= args.amr_dendrite_steps:
+            break
+        ty = torch.from_numpy(sim.get_state()).to(dev)
+        hn = hg.regrid(C.c_void_p(ty.data_ptr()), p, tol, cp, nlev)
+        tn = torch.zeros(nc * hn.ncg, dtype=torch.float64, device=dev)
+        hg.transfer_to(hn, C.c_void_p(ty.data_ptr()), C.c_void_p(tn.data_ptr()), nc)
+        hg = hn
+        sim = H.HierSim(p, s, hg)
+        sim.set_state(tn.cpu().numpy())
+        sim.set_time(t_now)
+        nregrid += 1
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    return dict(workload=f"2-D AMR dendrite: level 0 {n0}^2 in {len(lev0)} uneven patches, "
+                         f"{nlev} levels ({n0 << (nlev - 1)}^2 finest-equivalent), regrid every "
+                         f"{every} steps",
+                value=cell_steps / wall, unit="cells*steps/s", steps=nst, regrids=nregrid,
+                mean_cells=cell_steps / max(nst, 1), ms_per_step=wall / max(nst, 1) * 1e3,
+                dt=args.amr_dendrite_dt, final_levels=[len(v) for v in hg.levels],
+                form="FP64 throughout (FAC over levels in FP64)")
+
+
 def cpu_model():
     try:
         with open("/proc/cpuinfo") as f:
@@ -590,14 +698,17 @@ def main():
     ap.add_argument("--no-stencils", action="store_true")
     ap.add_argument("--graphs", type=int, default=1,
                     help="1: FP32 preconditioner solves replayed from a captured HIP graph")
-    ap.add_argument("--amr", action="store_true",
-                    help="also time the two-level multi-order ternary hierarchy (amr_ternary)")
+    ap.add_argument("--amr", type=int, default=1,
+                    help="1: also time the AMR workloads (amr_dendrite, amr_ternary)")
     ap.add_argument("--amr-only", action="store_true",
                     help="time only the hierarchy workload and print its object")
     ap.add_argument("--amr-n", type=int, default=256, help="level-0 cells per side")
     ap.add_argument("--amr-steps", type=int, default=3)
     ap.add_argument("--amr-warmup", type=int, default=1)
     ap.add_argument("--amr-dt", type=float, default=1.0e-5)
+    ap.add_argument("--amr-dendrite-steps", type=int, default=200)
+    ap.add_argument("--amr-dendrite-regrid", type=int, default=10)
+    ap.add_argument("--amr-dendrite-dt", type=float, default=1.0e-7)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU only (this build has no multi-GPU path); "
@@ -609,8 +720,10 @@ def main():
     dev = torch.device("cuda:0")
     forms = [f for f in args.forms.split(",") if f]
     if args.amr_only:
+        den = amr_dendrite_report(args, dev)
+        log(f"[bench] amr dendrite {den}")
         amr = amr_ternary_report(args, dev)
-        print(json.dumps({"amr_ternary": amr}), flush=True)
+        print(json.dumps({"amr_dendrite": den, "amr_ternary": amr}), flush=True)
         return
 
     n = args.n
@@ -701,11 +814,26 @@ def main():
     log(f"[bench] grains {grains}")
     copy_gbs = copy_bandwidth()
     log(f"[bench] device copy {copy_gbs:.0f} GB/s")
-    amr = None
+    amr, den = None, None
     if args.amr:
-        amr = amr_ternary_report(args, dev)
-        log(f"[bench] amr ternary {amr['value']:.4g} cells*steps/s, "
-            f"{amr['ms_per_step']:.1f} ms/step")
+        # the main forms' integrators are gone; a failure here is reported in
+        # the line, not fatal to it
+        import gc
+        gc.collect()
+        torch.cuda.empty_cache()
+        try:
+            den = amr_dendrite_report(args, dev)
+            log(f"[bench] amr dendrite {den['value']:.4g} cells*steps/s, "
+                f"{den['ms_per_step']:.1f} ms/step")
+        except Exception as e:  # noqa: BLE001
+            den = {"error": repr(e)}
+        gc.collect()
+        torch.cuda.empty_cache()
+        try:
+            amr = amr_ternary_report(args, dev)
+        except Exception as e:  # noqa: BLE001
+            amr = {"error": repr(e)}
+        log(f"[bench] amr ternary {amr}")
     cpu = None
     if not args.no_cpu:
         log("[bench] cpu baseline")
@@ -742,6 +870,7 @@ def main():
         "stencil_rooflines": stencils,
         "cpu_baseline": cpu,
         "grains": grains,
+        "amr_dendrite": den,
         "amr_ternary": amr,
     }
     print(json.dumps(out), flush=True)
