@@ -36,6 +36,8 @@ int hier_conc_rhs(const Hier& H, const ampe_model_params& p0, const double* c, c
 // the face diffusivities of the three phases and both species' EBS RHS
 int hier_phase_rhs_tern3(const Hier& H, const ampe_model_params& p0, int norder,
                          const double* phi, const double* ph6, double* rhs, hipStream_t s);
+int hier_phase_rhs_multiorder(const Hier& H, const ampe_model_params& p0, int norder,
+                              const double* phi, double* rhs, hipStream_t s);
 int hier_ebs3_coeffs(const Hier& H, const ampe_model_params& p0, int norder, const double* phi,
                      double* DL, double* DA, double* DB, hipStream_t s);
 int hier_conc_rhs_ebs3(const Hier& H, const ampe_model_params& p0, const double* ph6,

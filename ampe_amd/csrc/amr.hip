@@ -1077,6 +1077,26 @@ int hier_phase_rhs_tern3(const Hier& H, const ampe_model_params& p0, int norder,
    return 0;
 }
 
+// the multi-order model without composition (grain growth): the same
+// per-order gradient flux as the three-phase model, so the same correction
+int hier_phase_rhs_multiorder(const Hier& H, const ampe_model_params& p0, int norder,
+                              const double* phi, double* rhs, hipStream_t s)
+{
+   for (int l = H.nlev - 1; l >= 0; l--) {
+      ampe_model_params p = lev_params(H, p0, l);
+      if (launch_phase_rhs_multiorder(*H.lev[l], p, norder, phi, nullptr, rhs, s)) return -1;
+      if (l + 1 < H.nlev) {
+         double af, ac;
+         corr_areas(H, l, af, ac);
+         if (launch_hier_tern3_phase_corr(H.ndim, H.cfc[l].p, (int)H.cfc[l].n, H.cff[l + 1].p,
+                                          p, lev_params(H, p0, l + 1), norder, H.ncg, phi, af,
+                                          ac, rhs, s))
+            return -1;
+      }
+   }
+   return 0;
+}
+
 int hier_ebs3_coeffs(const Hier& H, const ampe_model_params& p0, int norder, const double* phi,
                      double* DL, double* DA, double* DB, hipStream_t s)
 {

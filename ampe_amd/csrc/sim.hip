@@ -232,11 +232,6 @@ int sim_create_hier(const ampe_model_params& p, const ampe_solver_params& s, Hie
                 "three-phase ternary EBS form");
       return -1;
    }
-   if (p.norderp > 1 && !tern) {
-      set_error("sim_create_hier: several order parameters with the three-phase ternary model "
-                "only");
-      return -1;
-   }
    if (s.kks_guess == AMPE_GUESS_SURROGATE) {
       set_error("sim_create_hier: the previous-step KKS guess only");
       return -1;
@@ -532,6 +527,7 @@ static int hier_rhs_eval(Sim* S, double* y, double* ydot, bool fd)
       return hier_conc_rhs_ebs3(*S->hier, p, S->ph6, S->Dc, S->Dphi, S->Db,
                                 ydot + (long long)S->ic * S->ng, st);
    }
+   if (S->nord > 1) return hier_phase_rhs_multiorder(*S->hier, p, S->nord, phi, ydot, st);
    if (q && recompute && hier_quat_face_norm2(*S->hier, p, q, S->n2, st)) return -1;
    if (c && kks(S, y)) return -1;
    if (hier_phase_rhs(*S->hier, p, phi, q, S->cl, S->cs, ydot, st)) return -1;

@@ -501,7 +501,9 @@ int ampe_tag_cells(const ampe_level* level, const ampe_model_params* p,
 typedef struct ampe_cluster_params {
    double efficiency_tol; /* accept a box when tagged / cells >= this
                            * (BergerRigoutsos efficiency_tolerance) */
-   int min_size[3];       /* smallest extent of a cut piece (smallest_patch_size) */
+   int min_size[3];       /* smallest extent of a cut piece when a box is split; accepted
+                             boxes are the tags' bounding boxes and may be smaller (SAMRAI's
+                             smallest_patch_size, which grows boxes, is not reproduced) */
    int max_size[3];       /* accepted boxes are chopped to at most this extent
                            * (largest_patch_size; <= 0: no limit) */
    int tag_buffer;        /* tags grown by this many cells in every direction */

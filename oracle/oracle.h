@@ -266,6 +266,8 @@ void o_hier_conc_coeffs(const OHier* H, const ampe_model_params* p0, const doubl
                         const double* cl, const double* cs, double* Dc, double* Dphi);
 void o_hier_phase_rhs_tern3(const OHier* H, const ampe_model_params* p0, int norder,
                             const double* phi, const double* ph6, double* rhs);
+void o_hier_phase_rhs_multiorder(const OHier* H, const ampe_model_params* p0, int norder,
+                                 const double* phi, double* rhs);
 void o_hier_ebs3_coeffs(const OHier* H, const ampe_model_params* p0, int norder,
                         const double* phi, double* DL, double* DA, double* DB);
 void o_hier_conc_rhs_ebs3(const OHier* H, const ampe_model_params* p0, const double* ph6,

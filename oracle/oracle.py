@@ -144,6 +144,7 @@ def _setup(L):
         "o_hier_conc_coeffs": (None, [C.c_void_p, _P, _dp, _dp, _dp, _dp, _dp]),
         "o_hier_conc_rhs": (None, [C.c_void_p, _P, _dp, _dp, _dp, _dp, _dp]),
         "o_hier_phase_rhs_tern3": (None, [C.c_void_p, _P, C.c_int, _dp, _dp, _dp]),
+    "o_hier_phase_rhs_multiorder": (None, [C.c_void_p, _P, C.c_int, _dp, _dp]),
         "o_hier_ebs3_coeffs": (None, [C.c_void_p, _P, C.c_int, _dp, _dp, _dp, _dp]),
         "o_hier_conc_rhs_ebs3": (None, [C.c_void_p, _P, _dp, _dp, _dp, _dp, _dp]),
         "o_sim_create_hier": (C.c_void_p, [_P, C.POINTER(SolverParams), C.c_void_p]),

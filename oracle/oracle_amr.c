@@ -798,6 +798,39 @@ void o_hier_phase_rhs_tern3(const OHier* H, const ampe_model_params* p0, int nor
    free(flux);
 }
 
+/* Composite multi-order phase RHS without composition (grain growth):
+ * the flux coarsening of o_hier_phase_rhs_tern3, then computerhsmultiorder
+ * per patch (MultiOrderRHSStrategy::evaluateRHS, source/
+ * MultiOrderRHSStrategy.cc:85-138). */
+void o_hier_phase_rhs_multiorder(const OHier* H, const ampe_model_params* p0, int norder,
+                                 const double* phi, double* rhs)
+{
+   double* flux = (double*)calloc((size_t)H->nf * norder, sizeof(double));
+   for (int l = H->nlev - 1; l >= 0; l--) {
+      ampe_model_params p = lev_params(H, p0, l);
+      for (int pp = H->pbeg[l]; pp < H->pbeg[l + 1]; pp++) {
+         double* pb = pgather(H, pp, phi, norder);
+         double* fb = (double*)malloc(sizeof(double) * pnf(&H->G[pp]) * norder);
+         o_phase_flux_multiorder(&H->G[pp], &p, norder, pb, fb);
+         fscatter(H, pp, fb, flux, norder);
+         free(fb);
+         free(pb);
+      }
+      o_hier_coarsen_faces(H, l, flux, norder, H->nf);
+      for (int pp = H->pbeg[l]; pp < H->pbeg[l + 1]; pp++) {
+         double* pb = pgather(H, pp, phi, norder);
+         double* fb = fgather(H, pp, flux, norder);
+         double* rb = pgather(H, pp, rhs, norder);
+         o_phase_rhs_multiorder_fx(&H->G[pp], &p, norder, fb, pb, NULL, rb);
+         pscatter(H, pp, rb, rhs, norder);
+         free(rb);
+         free(fb);
+         free(pb);
+      }
+   }
+   free(flux);
+}
+
 /* Three-phase face diffusivities on every patch, then the finer levels'
  * coarsened onto the coarser faces (as o_hier_conc_coeffs). */
 void o_hier_ebs3_coeffs(const OHier* H, const ampe_model_params* p0, int norder,

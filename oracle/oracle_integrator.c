@@ -1246,6 +1246,10 @@ static int rhs_eval_hier(OSim* S, double* y, double* ydot, int fd)
       o_hier_conc_rhs_ebs3(H, p, S->ph6, S->Dc, S->Dphi, S->Db, ydot + (long)S->ic * ng);
       return 0;
    }
+   if (S->nord > 1) {
+      o_hier_phase_rhs_multiorder(H, p, S->nord, phi, ydot);
+      return 0;
+   }
    if (q && recompute) o_hier_quat_face_norm2(H, p, q, S->n2);
    if (c && hier_kks(S, y)) return -1;
    o_hier_phase_rhs(H, p, phi, q, S->cl, S->cs, ydot);
@@ -1375,7 +1379,6 @@ OSim* o_sim_create_hier(const ampe_model_params* p, const ampe_solver_params* s,
    const int tern = p->fe_model == AMPE_FE_QUADRATIC_TERNARY_3PH;
    if (p->with_conc && !tern && p->conc_rhs_form != AMPE_CONC_RHS_KKS) return NULL;
    if (p->evolve_quat && p->qlen > 4) return NULL;
-   if (p->norderp > 1 && !tern) return NULL;
    OSim* S = o_sim_create(p, s, 1, 1, 1);
    if (!S) return NULL;
    /* resize every field to the composite */

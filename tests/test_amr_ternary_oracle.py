@@ -186,3 +186,71 @@ def test_hier_integrator_full_fine_level_is_the_fine_integrator():
             assert np.abs(H.get_level(yh, 1, comp=m) - ph[m]).max() < 1e-6
         for k in range(2):
             assert np.abs(H.get_level(yh, 1, comp=p.norderp + k) - cc[k]).max() < 1e-6
+
+
+# ---- the multi-order model without composition (grain growth) on a hierarchy
+
+def _grains_params():
+    p = P.two_grain_params(ndim=2)
+    p.norderp = 4
+    p.with_conc = 0
+    p.evolve_quat = 0
+    p.flux_type = P.FLUX_SIMPLE
+    return p
+
+
+def _grains_state(n, seed=3):
+    """Four smoothed Voronoi grains on a periodic n^2 grid."""
+    rng = np.random.default_rng(seed)
+    seeds = rng.uniform(0, n, (4, 2))
+    x = np.arange(n) + 0.5
+    yy, xx = np.meshgrid(x, x, indexing="ij")
+    d = []
+    for sx, sy in seeds:
+        dx = np.abs(xx - sx)
+        dy = np.abs(yy - sy)
+        dx, dy = np.minimum(dx, n - dx), np.minimum(dy, n - dy)
+        d.append(np.sqrt(dx * dx + dy * dy))
+    d = np.array(d)
+    w = np.exp(-(d - d.min(0)) / 1.5)
+    return list(w / w.sum(0))
+
+
+def test_multiorder_flux_part_is_conservative():
+    """With no wells (phi_well_scale = 0) the composite multi-order RHS is
+    M div(eps^2/2 grad phi_p): it integrates to zero over a periodic
+    hierarchy for every order, which needs the fine fluxes on the
+    coarse-fine faces."""
+    p = _grains_params()
+    p.phi_well_scale = 0.0
+    H = O.Hier(2, [LEV0, LEV1], (32, 32), dx0=(0.05, 0.05, 1.0), periodic=True)
+    phi, _ = _random_fields(H, np.random.default_rng(5), p.norderp)
+    r = H.cells(p.norderp)
+    O.lib().o_hier_phase_rhs_multiorder(H.h, C.byref(p), p.norderp, O.ptr(phi), O.ptr(r))
+    w = H.weights()
+    for m in range(p.norderp):
+        rm = r[m * H.ncg:(m + 1) * H.ncg]
+        assert np.abs(w * rm).sum() > 0
+        assert abs((w * rm).sum()) < 1e-12 * np.abs(w * rm).sum()
+
+
+def test_multiorder_hier_single_level_is_the_patch_integrator():
+    p = _grains_params()
+    n = 32
+    s = P.solver_defaults(dt=1e-3, maxl=10, max_newton=5, atol=1e-8, rtol=1e-10)
+    phi = _grains_state(n)
+    H = O.Hier(2, [[((0, 0), (n - 1, n - 1))]], (n, n), tuple(p.dx), True)
+    hs = O.HierSim(p, s, H)
+    hs.set_state(_hier_state(H, phi, 0))
+    ss = O.Sim(p, s, n, True)
+    ss.set_state(phi, None, None)
+    for _ in range(3):
+        r1, a = hs.advance(1)
+        r2, b = ss.advance(1)
+        assert r1 == r2 == 0
+        for k in ("nst", "nni", "nli", "nps", "ncfn", "nlcf"):
+            assert a[k] == b[k], (k, a, b)
+        yh = hs.get_state()
+        ph = ss.get_state()[0]
+        for m in range(p.norderp):
+            assert np.abs(H.get_level(yh, 0, comp=m) - ph[m]).max() < 1e-13

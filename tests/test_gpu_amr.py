@@ -415,6 +415,11 @@ INTEGRATOR_CASES = {
                              levels=lambda n: [[((0, 0), (n - 1, n - 1))],
                                                [((16, 8), (47, 31)), ((20, 32), (43, 55))]],
                              steps=4),
+    # grain growth: four order parameters, no composition or orientation
+    "2d_grains": dict(ndim=2, n=32, periodic=True,
+                      levels=lambda n: [[((0, 0), (n - 1, n - 1))],
+                                        [((8, 12), (39, 35)), ((40, 12), (55, 51))]],
+                      steps=4),
     # three-phase ternary in 3-D: two grains of solid A, one of B, the
     # liquid, one fine box over the three contacts
     "3d_ternary": dict(ndim=3, n=16, periodic=True,
@@ -461,6 +466,24 @@ def test_hier_integrator_matches_oracle(dev, case):
                               gstype=P.GS_CLASSICAL if case.endswith("cgs") else P.GS_MODIFIED)
         fields = [0.25 * (f[0::2, 0::2] + f[1::2, 0::2] + f[0::2, 1::2] + f[1::2, 1::2])
                   for f in list(phi) + list(c)]
+    elif case == "2d_grains":
+        p = P.two_grain_params(ndim=2)
+        p.norderp = 4
+        p.with_conc = 0
+        p.evolve_quat = 0
+        p.flux_type = P.FLUX_SIMPLE
+        s = P.solver_defaults(dt=1e-3, maxl=10, max_newton=5, atol=1e-8, rtol=1e-10)
+        rng = _rng(3)
+        x = np.arange(n) + 0.5
+        yy, xx = np.meshgrid(x, x, indexing="ij")
+        d = []
+        for sx, sy in rng.uniform(0, n, (4, 2)):
+            ddx, ddy = np.abs(xx - sx), np.abs(yy - sy)
+            ddx, ddy = np.minimum(ddx, n - ddx), np.minimum(ddy, n - ddy)
+            d.append(np.sqrt(ddx * ddx + ddy * ddy))
+        d = np.array(d)
+        w = np.exp(-(d - d.min(0)) / 1.5)
+        fields = list(w / w.sum(0))
     elif case == "3d_ternary":
         p = P.sintering_deck()[0]
         p.ndim = 3
