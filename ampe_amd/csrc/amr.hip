@@ -1325,7 +1325,7 @@ static int full_comp_residual(const FacCtx& X, double* r, double* u, const doubl
       if (fac_comp_residual(X, l, r, u, f)) return -1;
       if (hier_coarsen_level(H, l, r, X.ncomp, 0, X.s)) return -1;
    }
-   return fac_norm(X, r, norm);
+   return norm ? fac_norm(X, r, norm) : 0;
 }
 
 static int fac_cycle(const FacCtx& X, double* e, double* r, int l)
@@ -1406,8 +1406,10 @@ int hier_fac_solve(Hier& H, const ampe_hier_block& B, const ampe_fac_params& P, 
    FacCtx X{&H, &B, &P, ncomp, nface + 1, s};
    long long n = H.ncg * ncomp;
    if (fill_zero<double>(H.fr, n, s)) return -1;
+   // the initial residual drives the first cycle; its norm is not needed
+   // (the cycle loop tests the norm after each cycle)
    double nrm = 0.0;
-   if (full_comp_residual(X, H.fr, u, f, &nrm)) return -1;
+   if (full_comp_residual(X, H.fr, u, f, nullptr)) return -1;
    int it = 0;
    do {
       if (fill_zero<double>(H.fe, n, s)) return -1;
