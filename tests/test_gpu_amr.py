@@ -377,14 +377,16 @@ def _disk_state(n, ndim, r, w, c0, seed):
     return phi, c, q
 
 
-def _spheres3_state(n):
+def _spheres3_state(n, nx=None):
     """Orders A0, A1, B and the liquid (1 - the others, clipped): spheres of
-    radius n/4 with tanh interfaces; c1 follows solid A, c2 solid B."""
+    radius n/4 with tanh interfaces on an nx x n x n grid (nx = n by
+    default); c1 follows solid A, c2 solid B."""
+    nx = nx or n
     x = np.arange(n) + 0.5
-    zz, yy, xx = np.meshgrid(x, x, x, indexing="ij")
+    zz, yy, xx = np.meshgrid(x, x, np.arange(nx) + 0.5, indexing="ij")
     phis = []
     for cx, cy in ((0.33, 0.35), (0.69, 0.35), (0.5, 0.68)):
-        d = np.sqrt((xx - cx * n) ** 2 + (yy - cy * n) ** 2 + (zz - n / 2) ** 2)
+        d = np.sqrt((xx - cx * nx) ** 2 + (yy - cy * n) ** 2 + (zz - n / 2) ** 2)
         phis.append(0.5 * (1.0 - np.tanh((d - n / 4) / 1.5)))
     liq = np.clip(1.0 - phis[0] - phis[1] - phis[2], 0.0, 1.0)
     c1 = 0.9 * (phis[0] + phis[1]) + 0.05
@@ -426,6 +428,13 @@ INTEGRATOR_CASES = {
                        levels=lambda n: [[((0, 0, 0), (n - 1, n - 1, n - 1))],
                                          [((6, 6, 8), (25, 25, 23))]],
                        steps=2),
+    # the same wider than the 64-cell tiles: level 0 of 80 x 16 x 16 in two
+    # uneven patches, a fine patch of 144 x 16 x 16 (three tiles along x)
+    "3d_ternary_wide": dict(ndim=3, n=16, nx=80, periodic=True,
+                            levels=lambda n: [[((0, 0, 0), (47, n - 1, n - 1)),
+                                               ((48, 0, 0), (79, n - 1, n - 1))],
+                                              [((8, 8, 8), (151, 23, 23))]],
+                            steps=2),
     # 3-D CALPHAD polycrystal, one fine box
     "3d_calphad": dict(ndim=3, n=16, periodic=True,
                        levels=lambda n: [[((0, 0, 0), (n - 1, n - 1, n - 1))],
@@ -484,18 +493,18 @@ def test_hier_integrator_matches_oracle(dev, case):
         d = np.array(d)
         w = np.exp(-(d - d.min(0)) / 1.5)
         fields = list(w / w.sum(0))
-    elif case == "3d_ternary":
+    elif case.startswith("3d_ternary"):
         p = P.sintering_deck()[0]
         p.ndim = 3
         p.norderp, p.norderp_A = 4, 2
         s = P.solver_defaults(dt=1e-5, maxl=20, max_newton=8, atol=1e-6, rtol=1e-8)
-        fields = _spheres3_state(n)
+        fields = _spheres3_state(n, cfg.get("nx"))
     else:
         p = P.polycrystal_params(ndim=3)
         s = P.solver_defaults(dt=1e-7, maxl=10, max_newton=5, atol=1e-6, rtol=1e-8)
         phi, c, q = P.polycrystal_state(n=n, seed=1002, ngrains=4)
         fields = [phi, c] + list(q)
-    dom = (n,) * ndim
+    dom = (cfg.get("nx", n),) + (n,) * (ndim - 1)
     ho = O.Hier(ndim, levels, dom, tuple(p.dx), periodic)
     hg = H.Hier(ndim, levels, dom, tuple(p.dx), periodic)
     y0 = _hier_state(ho, fields, periodic)
