@@ -6,6 +6,38 @@
 namespace ampe {
 
 struct Hier;
+
+// FAC work arrays of one element type, grown on demand (amr.hip)
+template <typename T>
+struct FacWork {
+   T *e = nullptr, *r = nullptr, *tmp = nullptr, *oflux = nullptr, *gsave = nullptr;
+   long long cap = 0;
+   void release()
+   {
+      T* all[] = {e, r, tmp, oflux, gsave};
+      for (T* v : all) (void)hipFree(v);
+      e = r = tmp = oflux = gsave = nullptr;
+      cap = 0;
+   }
+};
+
+// One preconditioner block of the FAC solve over the levels in the element
+// type T: kind 0 elliptic (D on faces or D_const, C, M), kind 1 the
+// orientation block (face coefficients D, S = sqrt(M_q), gamma), as
+// ampe_hier_block.
+template <typename T>
+struct HBlock {
+   int kind;
+   const T* D;
+   T D_const;
+   const T* C;
+   T C_const;
+   const T* M;
+   T M_const;
+   const T* S;
+   T gamma;
+   int ewing;
+};
 // the hierarchy of a C-ABI handle
 Hier* hier_of(ampe_hier* h);
 
@@ -51,6 +83,10 @@ int hier_regrid(const Hier& Ho, const double* phi_o, const ampe_model_params& p0
                 const ampe_cluster_params& cp, int max_levels, Hier** out, hipStream_t s);
 int hier_fac_solve(Hier& H, const ampe_hier_block& B, const ampe_fac_params& P, const double* f,
                    double* u, int ncomp, double tol, double* norm, int* cycles, hipStream_t s);
+// the same block solve in FP32 (f, u, the coefficients and the cycle's work
+// arrays), for the FP32 preconditioner over a hierarchy
+int hier_fac_solve_f32(Hier& H, const HBlock<float>& B, const ampe_fac_params& P, const float* f,
+                       float* u, int ncomp, double tol, double* norm, int* cycles, hipStream_t s);
 // Per-level coefficient passes of the preconditioner setup: phase C and
 // mobility, orientation face coefficients and sqrt(M_q), each level with its
 // own spacing.

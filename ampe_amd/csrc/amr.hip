@@ -92,8 +92,8 @@ struct Hier {
    DevVec<long long> pro_dst[kMaxLev], pro_src[kMaxLev];
    double* wt = nullptr;     // composite weights (interior cells), device
    // FAC scratch (grown on demand)
-   double *fe = nullptr, *fr = nullptr, *ftmp = nullptr, *foflux = nullptr, *fgsave = nullptr;
-   long long fac_cap = 0;
+   FacWork<double> wd;  // FAC work arrays of the FP64 block solves
+   FacWork<float> wf;   // and of the FP32 ones
 };
 
 namespace {
@@ -191,11 +191,8 @@ void hier_destroy(Hier* H)
    }
    if (H->comp) level_destroy(H->comp);
    (void)hipFree(H->wt);
-   (void)hipFree(H->fe);
-   (void)hipFree(H->fr);
-   (void)hipFree(H->ftmp);
-   (void)hipFree(H->foflux);
-   (void)hipFree(H->fgsave);
+   H->wd.release();
+   H->wf.release();
    delete H;
 }
 
@@ -661,7 +658,8 @@ int hier_create(int ndim, int nlev, const int* npatch, const ampe_box* boxes, co
 // ------------------------------------------------------------ kernels
 namespace {
 
-__global__ void __launch_bounds__(256) k_cf_fill(double* __restrict__ f, long long ncg, int ncomp,
+template <typename T>
+__global__ void __launch_bounds__(256) k_cf_fill(T* __restrict__ f, long long ncg, int ncomp,
                                                  const long long* __restrict__ dst,
                                                  const long long* __restrict__ src,
                                                  const double* __restrict__ w, long long n,
@@ -671,17 +669,17 @@ __global__ void __launch_bounds__(256) k_cf_fill(double* __restrict__ f, long lo
    if (e >= n) return;
    const long long* s = src + 9 * e;
    for (int m = 0; m < ncomp; m++) {
-      double* a = f + m * ncg;
-      double v;
+      T* a = f + m * ncg;
+      T v;
       if (mode == 1) {
          v = a[s[0]];
       } else {
          double x = w[3 * e], y = w[3 * e + 1], z = w[3 * e + 2];
-         double q1 = a[s[1]], q2 = a[s[2]], q3 = a[s[3]], q4 = a[s[4]];
+         T q1 = a[s[1]], q2 = a[s[2]], q3 = a[s[3]], q4 = a[s[4]];
          if (ndim == 2) {
             v = (q1 + (q3 - q1) * x) * (1.0 - y) + (q2 + (q4 - q2) * x) * y;
          } else {
-            double q5 = a[s[5]], q6 = a[s[6]], q7 = a[s[7]], q8 = a[s[8]];
+            T q5 = a[s[5]], q6 = a[s[6]], q7 = a[s[7]], q8 = a[s[8]];
             v = ((q1 + (q3 - q1) * x) * (1.0 - y) + (q2 + (q4 - q2) * x) * y) * (1.0 - z) +
                 ((q5 + (q7 - q5) * x) * (1.0 - y) + (q6 + (q8 - q6) * x) * y) * z;
          }
@@ -690,8 +688,9 @@ __global__ void __launch_bounds__(256) k_cf_fill(double* __restrict__ f, long lo
    }
 }
 
-__global__ void __launch_bounds__(256) k_coarsen(const double* __restrict__ fsrc,
-                                                 double* __restrict__ f, long long ncg, int ncomp,
+template <typename T>
+__global__ void __launch_bounds__(256) k_coarsen(const T* __restrict__ fsrc,
+                                                 T* __restrict__ f, long long ncg, int ncomp,
                                                  const long long* __restrict__ dst,
                                                  const long long* __restrict__ src, long long n,
                                                  double dvf, double dvc, int nchild,
@@ -702,13 +701,13 @@ __global__ void __launch_bounds__(256) k_coarsen(const double* __restrict__ fsrc
    // children in x-fastest order from the first child and the fine strides
    const long long base = src[2 * e], packed = src[2 * e + 1];
    const long long sy = packed & 0xffffffffLL, sz = packed >> 32;
-   double acc[4];
+   T acc[4];
    for (int m = 0; m < ncomp; m++) {
-      const double* a = fsrc + m * ncg + base;
-      double s = 0.0;
+      const T* a = fsrc + m * ncg + base;
+      T s = 0.0;
       for (int c = 0; c < 8; c++)
          if (c < nchild) s = s + a[(c & 1) + ((c >> 1) & 1) * sy + (c >> 2) * sz] * dvf;
-      double v = s / dvc;
+      T v = s / (T)dvc;
       if (m < 4) acc[m] = v;
       if (!quatnorm) f[m * ncg + dst[e]] = v;
    }
@@ -744,7 +743,8 @@ __global__ void __launch_bounds__(256) k_face_coarsen(const double* __restrict__
    }
 }
 
-__global__ void __launch_bounds__(256) k_prolong_const_add(double* __restrict__ e, long long ncg,
+template <typename T>
+__global__ void __launch_bounds__(256) k_prolong_const_add(T* __restrict__ e, long long ncg,
                                                            int ncomp,
                                                            const long long* __restrict__ dst,
                                                            const long long* __restrict__ src,
@@ -756,9 +756,10 @@ __global__ void __launch_bounds__(256) k_prolong_const_add(double* __restrict__ 
 }
 
 // interior update u += e over the composite's tiles
+template <typename T>
 __global__ void __launch_bounds__(256) k_add_interior(LevelArgs a, int ncomp,
-                                                      double* __restrict__ u,
-                                                      const double* __restrict__ e)
+                                                      T* __restrict__ u,
+                                                      const T* __restrict__ e)
 {
    CellPos c;
    if (!cell_pos(a, 0, c)) return;
@@ -766,8 +767,9 @@ __global__ void __launch_bounds__(256) k_add_interior(LevelArgs a, int ncomp,
 }
 
 // per-tile sum of w r^2 over ncomp components (weights of interior cells)
+template <typename T>
 __global__ void __launch_bounds__(256) k_wsumsq(LevelArgs a, int ncomp,
-                                                const double* __restrict__ r,
+                                                const T* __restrict__ r,
                                                 const double* __restrict__ w,
                                                 double* __restrict__ partial)
 {
@@ -776,7 +778,7 @@ __global__ void __launch_bounds__(256) k_wsumsq(LevelArgs a, int ncomp,
    if (cell_pos(a, 0, c)) {
       double wc = w[c.c0];
       for (int m = 0; m < ncomp; m++) {
-         double x = r[m * a.ncg + c.c0];
+         double x = (double)r[m * a.ncg + c.c0];
          v += wc * x * x;
          sw += wc;
       }
@@ -805,83 +807,86 @@ __global__ void __launch_bounds__(256) k_wsumsq(LevelArgs a, int ncomp,
 // side (the mirror image on an upper side), with u_g the constant-refined
 // coarse value kept in gsave; longwt = 2 tranwt, tranwt = 1.0/(1+ratio) in
 // single precision as the reference writes it.
-__global__ void __launch_bounds__(256) k_ewing_ghosts(double* __restrict__ u,
+template <typename T>
+__global__ void __launch_bounds__(256) k_ewing_ghosts(T* __restrict__ u,
                                                       const CfFace* __restrict__ ff, long long n,
-                                                      const double* __restrict__ gsave,
-                                                      double tranwt, double longwt)
+                                                      const T* __restrict__ gsave,
+                                                      T tranwt, T longwt)
 {
    long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
    if (t >= n) return;
    const CfFace f = ff[t];
    long long I = f.side ? f.cm : f.c0, g = f.side ? f.c0 : f.cm;
-   double uI = u[I];
+   T uI = u[I];
    u[g] = uI - longwt * (uI - gsave[t]) - tranwt * (u[f.mirror] - uI);
 }
 
 // gsave[t] = coarse parent value of face t's ghost (CONSTANT_REFINE); with
 // write_ghost the ghost itself takes it too
-__global__ void __launch_bounds__(256) k_cf_save(double* __restrict__ u,
+template <typename T>
+__global__ void __launch_bounds__(256) k_cf_save(T* __restrict__ u,
                                                  const CfFace* __restrict__ ff, long long n,
-                                                 double* __restrict__ gsave, int write_ghost)
+                                                 T* __restrict__ gsave, int write_ghost)
 {
    long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
    if (t >= n) return;
    const CfFace f = ff[t];
-   double v = u[f.parent];
+   T v = u[f.parent];
    gsave[t] = v;
    if (write_ghost) u[f.side ? f.c0 : f.cm] = v;
 }
 
 // fine coarse-fine face fluxes D (w_c0 - w_cm) / h (w = u, or S u for the
 // orientation block), saved for the coarser level's composite residual
-__global__ void __launch_bounds__(256) k_oflux_save(const double* __restrict__ u,
-                                                    const double* __restrict__ S,
-                                                    const double* __restrict__ D, double Dc,
+template <typename T>
+__global__ void __launch_bounds__(256) k_oflux_save(const T* __restrict__ u,
+                                                    const T* __restrict__ S,
+                                                    const T* __restrict__ D, T Dc,
                                                     const CfFace* __restrict__ ff, long long n,
-                                                    double di0, double di1, double di2,
-                                                    double* __restrict__ oflux)
+                                                    T di0, T di1, T di2,
+                                                    T* __restrict__ oflux)
 {
    long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
    if (t >= n) return;
    const CfFace f = ff[t];
-   double dinv = f.d == 0 ? di0 : (f.d == 1 ? di1 : di2);
-   double dd = D ? D[f.face] : Dc;
-   double wc = S ? S[f.c0] * u[f.c0] : u[f.c0];
-   double wm = S ? S[f.cm] * u[f.cm] : u[f.cm];
+   T dinv = f.d == 0 ? di0 : (f.d == 1 ? di1 : di2);
+   T dd = D ? D[f.face] : Dc;
+   T wc = S ? S[f.c0] * u[f.c0] : u[f.c0];
+   T wm = S ? S[f.cm] * u[f.cm] : u[f.cm];
    oflux[t] = dinv * dd * (wc - wm);
 }
 
 // composite residual: the coarse flux on each coarse-fine face replaced by
 // the coarsened saved fine fluxes
-__global__ void __launch_bounds__(256) k_fac_corr(double* __restrict__ res,
-                                                  const double* __restrict__ u,
-                                                  const double* __restrict__ S,
-                                                  const double* __restrict__ D, double Dc,
-                                                  const double* __restrict__ M, double Mc,
-                                                  double gamma, int kind,
+template <typename T>
+__global__ void __launch_bounds__(256) k_fac_corr(T* __restrict__ res,
+                                                  const T* __restrict__ u,
+                                                  const T* __restrict__ S,
+                                                  const T* __restrict__ D, T Dc,
+                                                  const T* __restrict__ M, T Mc,
+                                                  T gamma, int kind,
                                                   const CfCorr* __restrict__ cc, long long n,
-                                                  const double* __restrict__ oflux, int nfine,
-                                                  double di0, double di1, double di2, double af,
-                                                  double ac)
+                                                  const T* __restrict__ oflux, int nfine,
+                                                  T di0, T di1, T di2, T af, T ac)
 {
    long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
    if (t >= n) return;
    const CfCorr& e = cc[t];
-   double delta = 0.0;
+   T delta = 0.0;
    for (int f = 0; f < e.nface; f++) {
       int d = e.d[f];
-      double dinv = d == 0 ? di0 : (d == 1 ? di1 : di2);
-      double dd = D ? D[e.face[f]] : Dc;
-      double wc = S ? S[e.c0[f]] * u[e.c0[f]] : u[e.c0[f]];
-      double wm = S ? S[e.cm[f]] * u[e.cm[f]] : u[e.cm[f]];
-      double fold = dinv * dd * (wc - wm);
-      double acc = 0.0;
+      T dinv = d == 0 ? di0 : (d == 1 ? di1 : di2);
+      T dd = D ? D[e.face[f]] : Dc;
+      T wc = S ? S[e.c0[f]] * u[e.c0[f]] : u[e.c0[f]];
+      T wm = S ? S[e.cm[f]] * u[e.cm[f]] : u[e.cm[f]];
+      T fold = dinv * dd * (wc - wm);
+      T acc = 0.0;
       for (int r = 0; r < nfine; r++) acc = acc + oflux[e.fine[f] + r] * af;
-      double fnew = acc / ac;
-      double t1 = dinv * (fnew - fold);
+      T fnew = acc / ac;
+      T t1 = dinv * (fnew - fold);
       delta = delta + (e.side[f] ? t1 : -t1);
    }
-   double m = kind == 1 ? gamma * S[e.cell] : (M ? M[e.cell] : Mc);
+   T m = kind == 1 ? gamma * S[e.cell] : (M ? M[e.cell] : Mc);
    res[e.cell] = res[e.cell] - m * delta;
 }
 
@@ -894,7 +899,7 @@ int hier_ghost_fill_level(const Hier& H, int l, double* f, int ncomp, int cf, hi
 {
    if (ghost_fill(*H.lev[l], f, ncomp, s)) return -1;
    if (l > 0 && cf > 0 && H.cf_dst[l].n > 0) {
-      hipLaunchKernelGGL(k_cf_fill, grid1(H.cf_dst[l].n), dim3(256), 0, s, f, H.ncg, ncomp,
+      hipLaunchKernelGGL(k_cf_fill<double>, grid1(H.cf_dst[l].n), dim3(256), 0, s, f, H.ncg, ncomp,
                          H.cf_dst[l].p, H.cf_src[l].p, H.cf_w[l].p, H.cf_dst[l].n, cf, H.ndim);
       AMPE_HIP_CHECK(hipGetLastError());
    }
@@ -920,7 +925,7 @@ int hier_coarsen_level(const Hier& H, int l, double* f, int ncomp, int quatnorm,
       set_error("hier_coarsen: at most four components with the normalisation");
       return -1;
    }
-   hipLaunchKernelGGL(k_coarsen, grid1(H.crs_dst[l].n), dim3(256), 0, s, f, f, H.ncg, ncomp,
+   hipLaunchKernelGGL(k_coarsen<double>, grid1(H.crs_dst[l].n), dim3(256), 0, s, f, f, H.ncg, ncomp,
                       H.crs_dst[l].p, H.crs_src[l].p, H.crs_dst[l].n, cell_vol(H, l + 1),
                       cell_vol(H, l), H.ndim == 3 ? 8 : 4, quatnorm);
    AMPE_HIP_CHECK(hipGetLastError());
@@ -1131,53 +1136,83 @@ int hier_conc_rhs_ebs3(const Hier& H, const ampe_model_params& p0, const double*
 }
 
 // ------------------------------------------------------------ FAC
+// The FAC work arrays of one element type (FP64 for the block solves of the
+// FP64 preconditioner, FP32 for the FP32 one), grown on demand.
+template <typename T>
+static FacWork<T>& fac_work(Hier& H);
+template <>
+FacWork<double>& fac_work<double>(Hier& H)
+{
+   return H.wd;
+}
+template <>
+FacWork<float>& fac_work<float>(Hier& H)
+{
+   return H.wf;
+}
+
+template <typename T>
 static int fac_grow(Hier& H, int ncomp)
 {
+   FacWork<T>& W = fac_work<T>(H);
    long long need = H.ncg * ncomp;
-   if (need <= H.fac_cap) return 0;
-   (void)hipFree(H.fe);
-   (void)hipFree(H.fr);
-   (void)hipFree(H.ftmp);
-   (void)hipFree(H.foflux);
-   (void)hipFree(H.fgsave);
-   H.fe = H.fr = H.ftmp = H.foflux = H.fgsave = nullptr;
+   if (need <= W.cap) return 0;
+   W.release();
    long long nface = 0;
    for (int l = 0; l < H.nlev; l++) nface = std::max(nface, H.cff[l].n + H.fff[l].n);
-   AMPE_HIP_CHECK(hipMalloc((void**)&H.fe, sizeof(double) * need));
-   AMPE_HIP_CHECK(hipMalloc((void**)&H.fr, sizeof(double) * need));
-   AMPE_HIP_CHECK(hipMalloc((void**)&H.ftmp, sizeof(double) * need));
+   AMPE_HIP_CHECK(hipMalloc((void**)&W.e, sizeof(T) * need));
+   AMPE_HIP_CHECK(hipMalloc((void**)&W.r, sizeof(T) * need));
+   AMPE_HIP_CHECK(hipMalloc((void**)&W.tmp, sizeof(T) * need));
    // saved fine fluxes per level and component
    AMPE_HIP_CHECK(
-       hipMalloc((void**)&H.foflux, sizeof(double) * (size_t)(nface + 1) * kMaxLev * ncomp));
-   AMPE_HIP_CHECK(hipMalloc((void**)&H.fgsave, sizeof(double) * (size_t)(nface + 1)));
-   H.fac_cap = need;
+       hipMalloc((void**)&W.oflux, sizeof(T) * (size_t)(nface + 1) * kMaxLev * ncomp));
+   AMPE_HIP_CHECK(hipMalloc((void**)&W.gsave, sizeof(T) * (size_t)(nface + 1)));
+   W.cap = need;
    return 0;
 }
 
+template <typename T>
 struct FacCtx {
    Hier* H;
-   const ampe_hier_block* B;
+   const HBlock<T>* B;
    const ampe_fac_params* P;
    int ncomp;
    long long oflux_stride;  // per (level, component) slot
    hipStream_t s;
+   FacWork<T>* W;
 };
 
-static double* oflux_of(const FacCtx& X, int l, int m)
+template <typename T>
+static T* oflux_of(const FacCtx<T>& X, int l, int m)
 {
-   return X.H->foflux + ((long long)l * X.ncomp + m) * X.oflux_stride;
+   return X.W->oflux + ((long long)l * X.ncomp + m) * X.oflux_stride;
 }
 
-static const double* comp_ptr(const FacCtx& X, const double* f, int m)
+template <typename T>
+static const T* comp_ptr(const FacCtx<T>& X, const T* f, int m)
 {
    return f + (long long)m * X.H->ncg;
 }
 
 static double tranwt_ref() { return (double)(1.0f / (float)(1 + 2)); }
 
+template <typename T>
+static EllOp<T> ell_op(const HBlock<T>& B)
+{
+   EllOp<T> op;
+   op.D = B.D;
+   op.Dc = B.D_const;
+   op.C = B.C;
+   op.Cc = B.C_const;
+   op.M = B.M;
+   op.Mc = B.M_const;
+   return op;
+}
+
 // coarse-fine ghosts of level l (> 0) for component array u: constant refine
 // saved per face; the elliptic blocks then take Ewing's effective ghosts
-static int fac_cf_ghosts(const FacCtx& X, int l, double* u, bool refresh)
+template <typename T>
+static int fac_cf_ghosts(const FacCtx<T>& X, int l, T* u, bool refresh)
 {
    const Hier& H = *X.H;
    if (l == 0) return 0;
@@ -1187,20 +1222,20 @@ static int fac_cf_ghosts(const FacCtx& X, int l, double* u, bool refresh)
       // the full coarse-fine fill by CONSTANT_REFINE of every ghost (corner
       // ghosts included), then the per-face saved values
       if (H.cf_dst[l].n > 0) {
-         hipLaunchKernelGGL(k_cf_fill, grid1(H.cf_dst[l].n), dim3(256), 0, X.s, u, H.ncg, 1,
+         hipLaunchKernelGGL(k_cf_fill<T>, grid1(H.cf_dst[l].n), dim3(256), 0, X.s, u, H.ncg, 1,
                             H.cf_dst[l].p, H.cf_src[l].p, H.cf_w[l].p, H.cf_dst[l].n, 1, H.ndim);
          AMPE_HIP_CHECK(hipGetLastError());
       }
       if (n > 0 && ewing) {
-         hipLaunchKernelGGL(k_cf_save, grid1(n), dim3(256), 0, X.s, u, H.cff[l].p, n, H.fgsave,
-                            0);
+         hipLaunchKernelGGL(k_cf_save<T>, grid1(n), dim3(256), 0, X.s, u, H.cff[l].p, n,
+                            X.W->gsave, 0);
          AMPE_HIP_CHECK(hipGetLastError());
       }
    }
    if (n > 0 && ewing) {
       double tw = tranwt_ref();
-      hipLaunchKernelGGL(k_ewing_ghosts, grid1(n), dim3(256), 0, X.s, u, H.cff[l].p, n, H.fgsave,
-                         tw, 2.0 * tw);
+      hipLaunchKernelGGL(k_ewing_ghosts<T>, grid1(n), dim3(256), 0, X.s, u, H.cff[l].p, n,
+                         X.W->gsave, (T)tw, (T)(2.0 * tw));
       AMPE_HIP_CHECK(hipGetLastError());
    }
    return 0;
@@ -1209,38 +1244,31 @@ static int fac_cf_ghosts(const FacCtx& X, int l, double* u, bool refresh)
 // The Ewing effective ghosts depend on the component being smoothed; with
 // several components (orientation block, no Ewing) the saved values are not
 // used.
-static int fac_colour(const FacCtx& X, int l, double* u, const double* f, int m, int color)
+template <typename T>
+static int fac_colour(const FacCtx<T>& X, int l, T* u, const T* f, int color)
 {
    const Hier& H = *X.H;
    const Level& L = *H.lev[l];
-   const ampe_hier_block& B = *X.B;
+   const HBlock<T>& B = *X.B;
    if (ghost_fill(L, u, 1, X.s)) return -1;
    if (l > 0 && fac_cf_ghosts(X, l, u, false)) return -1;
-   if (B.kind == 1) {
-      return launch_quat_rbgs<double>(L, B.D, B.S, B.gamma, H.dx[l], 1, f, u, color, X.s);
-   }
-   EllOp<double> op;
-   op.D = B.D;
-   op.Dc = B.D_const;
-   op.C = B.C;
-   op.Cc = B.C_const;
-   op.M = B.M;
-   op.Mc = B.M_const;
-   (void)m;
-   return launch_rbgs<double>(L, op, H.dx[l], f, u, color, false, X.s);
+   if (B.kind == 1)
+      return launch_quat_rbgs<T>(L, B.D, B.S, B.gamma, H.dx[l], 1, f, u, color, X.s);
+   return launch_rbgs<T>(L, ell_op(B), H.dx[l], f, u, color, false, X.s);
 }
 
-static int fac_smooth(const FacCtx& X, int l, const double* f, double* u, int sweeps)
+template <typename T>
+static int fac_smooth(const FacCtx<T>& X, int l, const T* f, T* u, int sweeps)
 {
    const Hier& H = *X.H;
    for (int m = 0; m < X.ncomp; m++) {
-      double* um = u + (long long)m * H.ncg;
-      const double* fm = comp_ptr(X, f, m);
+      T* um = u + (long long)m * H.ncg;
+      const T* fm = comp_ptr(X, f, m);
       if (ghost_fill(*H.lev[l], um, 1, X.s)) return -1;
       if (l > 0 && fac_cf_ghosts(X, l, um, true)) return -1;
       for (int sw = 0; sw < sweeps; sw++) {
-         if (fac_colour(X, l, um, fm, m, 0)) return -1;
-         if (fac_colour(X, l, um, fm, m, 1)) return -1;
+         if (fac_colour(X, l, um, fm, 0)) return -1;
+         if (fac_colour(X, l, um, fm, 1)) return -1;
          if (ghost_fill(*H.lev[l], um, 1, X.s)) return -1;
       }
    }
@@ -1248,32 +1276,25 @@ static int fac_smooth(const FacCtx& X, int l, const double* f, double* u, int sw
 }
 
 // composite residual on level l (EllipticFACOps::computeCompositeResidualOnLevel)
-static int fac_comp_residual(const FacCtx& X, int l, double* res, double* u, const double* f)
+template <typename T>
+static int fac_comp_residual(const FacCtx<T>& X, int l, T* res, T* u, const T* f)
 {
    const Hier& H = *X.H;
-   const ampe_hier_block& B = *X.B;
+   const HBlock<T>& B = *X.B;
    const Level& L = *H.lev[l];
    for (int m = 0; m < X.ncomp; m++) {
-      double* um = u + (long long)m * H.ncg;
-      const double* fm = comp_ptr(X, f, m);
-      double* rm = res + (long long)m * H.ncg;
+      T* um = u + (long long)m * H.ncg;
+      const T* fm = comp_ptr(X, f, m);
+      T* rm = res + (long long)m * H.ncg;
       if (ghost_fill(L, um, 1, X.s)) return -1;
       if (l > 0 && fac_cf_ghosts(X, l, um, true)) return -1;
       if (B.kind == 1) {
-         if (launch_quat_residual<double>(L, B.D, B.S, B.gamma, H.dx[l], 1, fm, um, rm, nullptr,
-                                          X.s))
+         if (launch_quat_residual<T>(L, B.D, B.S, B.gamma, H.dx[l], 1, fm, um, rm, nullptr, X.s))
             return -1;
-      } else {
-         EllOp<double> op;
-         op.D = B.D;
-         op.Dc = B.D_const;
-         op.C = B.C;
-         op.Cc = B.C_const;
-         op.M = B.M;
-         op.Mc = B.M_const;
-         if (launch_residual<double>(L, op, H.dx[l], fm, um, rm, nullptr, X.s)) return -1;
+      } else if (launch_residual<T>(L, ell_op(B), H.dx[l], fm, um, rm, nullptr, X.s)) {
+         return -1;
       }
-      double di[3] = {1.0 / H.dx[l][0], 1.0 / H.dx[l][1], 1.0 / H.dx[l][2]};
+      const T di[3] = {(T)(1.0 / H.dx[l][0]), (T)(1.0 / H.dx[l][1]), (T)(1.0 / H.dx[l][2])};
       // every coarse face on a fine patch boundary takes the coarsened saved
       // fine fluxes: the uncovered cells next to coarse-fine faces, then the
       // covered cells inside the fine patches
@@ -1282,16 +1303,16 @@ static int fac_comp_residual(const FacCtx& X, int l, double* res, double* u, con
          if (tab.n == 0) continue;
          double af, ac;
          corr_areas(H, l, af, ac);
-         hipLaunchKernelGGL(k_fac_corr, grid1(tab.n), dim3(256), 0, X.s, rm, um,
+         hipLaunchKernelGGL(k_fac_corr<T>, grid1(tab.n), dim3(256), 0, X.s, rm, um,
                             B.kind == 1 ? B.S : nullptr, B.D, B.D_const, B.M, B.M_const, B.gamma,
                             B.kind, tab.p, tab.n, oflux_of(X, l + 1, m), H.ndim == 3 ? 4 : 2,
-                            di[0], di[1], di[2], af, ac);
+                            di[0], di[1], di[2], (T)af, (T)ac);
          AMPE_HIP_CHECK(hipGetLastError());
       }
       for (int t = 0; t < 2 && l > 0; t++) {
          const DevVec<CfFace>& tab = t == 0 ? H.cff[l] : H.fff[l];
          if (tab.n == 0) continue;
-         hipLaunchKernelGGL(k_oflux_save, grid1(tab.n), dim3(256), 0, X.s, um,
+         hipLaunchKernelGGL(k_oflux_save<T>, grid1(tab.n), dim3(256), 0, X.s, um,
                             B.kind == 1 ? B.S : nullptr, B.D, B.D_const, tab.p, tab.n, di[0],
                             di[1], di[2], oflux_of(X, l, m) + (t == 0 ? 0 : H.cff[l].n));
          AMPE_HIP_CHECK(hipGetLastError());
@@ -1300,13 +1321,14 @@ static int fac_comp_residual(const FacCtx& X, int l, double* res, double* u, con
    return 0;
 }
 
-static int fac_norm(const FacCtx& X, const double* r, double* out)
+template <typename T>
+static int fac_norm(const FacCtx<T>& X, const T* r, double* out)
 {
    const Hier& H = *X.H;
    const Level& C = *H.comp;
    LevelArgs a = level_args(C);
-   hipLaunchKernelGGL(k_wsumsq, dim3(C.ncell_tiles), dim3(TX, TY), 0, X.s, a, X.ncomp, r, H.wt,
-                      C.partial);
+   hipLaunchKernelGGL(k_wsumsq<T>, dim3(C.ncell_tiles), dim3(TX, TY), 0, X.s, a, X.ncomp, r,
+                      H.wt, C.partial);
    AMPE_HIP_CHECK(hipGetLastError());
    double h[2];
    if (reduce_partials(C, C.ncell_tiles, 2, 0, h, X.s)) return -1;
@@ -1314,49 +1336,54 @@ static int fac_norm(const FacCtx& X, const double* r, double* out)
    return 0;
 }
 
-static int full_comp_residual(const FacCtx& X, double* r, double* u, const double* f,
-                              double* norm)
+template <typename T>
+static int coarsen_level_t(const Hier& H, int l, const T* src, T* dst, int ncomp, hipStream_t s)
+{
+   if (l + 1 >= H.nlev || H.crs_dst[l].n == 0) return 0;
+   hipLaunchKernelGGL(k_coarsen<T>, grid1(H.crs_dst[l].n), dim3(256), 0, s, src, dst, H.ncg,
+                      ncomp, H.crs_dst[l].p, H.crs_src[l].p, H.crs_dst[l].n, cell_vol(H, l + 1),
+                      cell_vol(H, l), H.ndim == 3 ? 8 : 4, 0);
+   AMPE_HIP_CHECK(hipGetLastError());
+   return 0;
+}
+
+template <typename T>
+static int full_comp_residual(const FacCtx<T>& X, T* r, T* u, const T* f, double* norm)
 {
    const Hier& H = *X.H;
    int lmax = H.nlev - 1;
    if (fac_comp_residual(X, lmax, r, u, f)) return -1;
    for (int l = lmax - 1; l >= 0; l--) {
-      if (hier_coarsen_level(H, l, u, X.ncomp, 0, X.s)) return -1;
+      if (coarsen_level_t<T>(H, l, u, u, X.ncomp, X.s)) return -1;
       if (fac_comp_residual(X, l, r, u, f)) return -1;
-      if (hier_coarsen_level(H, l, r, X.ncomp, 0, X.s)) return -1;
+      if (coarsen_level_t<T>(H, l, r, r, X.ncomp, X.s)) return -1;
    }
    return norm ? fac_norm(X, r, norm) : 0;
 }
 
-static int fac_cycle(const FacCtx& X, double* e, double* r, int l)
+template <typename T>
+static int fac_cycle(const FacCtx<T>& X, T* e, T* r, int l)
 {
    const Hier& H = *X.H;
    const ampe_fac_params& P = *X.P;
    if (l == 0) {
       // solveCoarsestLevel, "redblack": sweeps with the tolerance test
+      const HBlock<T>& B = *X.B;
       for (int m = 0; m < X.ncomp; m++) {
-         double* em = e + (long long)m * H.ncg;
-         const double* rm = comp_ptr(X, r, m);
+         T* em = e + (long long)m * H.ncg;
+         const T* rm = comp_ptr(X, r, m);
          if (ghost_fill(*H.lev[0], em, 1, X.s)) return -1;
          for (int sw = 0; sw < P.coarse_its; sw++) {
-            EllOp<double> op;
-            const ampe_hier_block& B = *X.B;
-            op.D = B.D;
-            op.Dc = B.D_const;
-            op.C = B.C;
-            op.Cc = B.C_const;
-            op.M = B.M;
-            op.Mc = B.M_const;
             bool want = P.coarse_tol >= 0.0;
             double rmax = 0.0, bmax = 0.0;
             for (int color = 0; color < 2; color++) {
                if (ghost_fill(*H.lev[0], em, 1, X.s)) return -1;
                if (B.kind == 1) {
-                  if (launch_quat_rbgs<double>(*H.lev[0], B.D, B.S, B.gamma, H.dx[0], 1, rm, em,
-                                               color, X.s))
+                  if (launch_quat_rbgs<T>(*H.lev[0], B.D, B.S, B.gamma, H.dx[0], 1, rm, em,
+                                          color, X.s))
                      return -1;
                } else {
-                  if (launch_rbgs<double>(*H.lev[0], op, H.dx[0], rm, em, color, want, X.s))
+                  if (launch_rbgs<T>(*H.lev[0], ell_op(B), H.dx[0], rm, em, color, want, X.s))
                      return -1;
                   if (want) {
                      double mx;
@@ -1374,58 +1401,76 @@ static int fac_cycle(const FacCtx& X, double* e, double* r, int l)
       return 0;
    }
    if (fac_smooth(X, l, r, e, P.pre)) return -1;
-   if (fac_comp_residual(X, l, H.ftmp, e, r)) return -1;
+   if (fac_comp_residual(X, l, X.W->tmp, e, r)) return -1;
    // restrictResidual: the covered cells of r on level l-1 take the
    // conservative average of the level-l composite residual in tmp
-   if (H.crs_dst[l - 1].n > 0) {
-      hipLaunchKernelGGL(k_coarsen, grid1(H.crs_dst[l - 1].n), dim3(256), 0, X.s, H.ftmp, r,
-                         H.ncg, X.ncomp, H.crs_dst[l - 1].p, H.crs_src[l - 1].p,
-                         H.crs_dst[l - 1].n, cell_vol(H, l), cell_vol(H, l - 1),
-                         H.ndim == 3 ? 8 : 4, 0);
-      AMPE_HIP_CHECK(hipGetLastError());
-   }
+   if (coarsen_level_t<T>(H, l - 1, X.W->tmp, r, X.ncomp, X.s)) return -1;
    if (fac_cycle(X, e, r, l - 1)) return -1;
    if (H.pro_dst[l].n > 0) {
-      hipLaunchKernelGGL(k_prolong_const_add, grid1(H.pro_dst[l].n), dim3(256), 0, X.s, e, H.ncg,
-                         X.ncomp, H.pro_dst[l].p, H.pro_src[l].p, H.pro_dst[l].n);
+      hipLaunchKernelGGL(k_prolong_const_add<T>, grid1(H.pro_dst[l].n), dim3(256), 0, X.s, e,
+                         H.ncg, X.ncomp, H.pro_dst[l].p, H.pro_src[l].p, H.pro_dst[l].n);
       AMPE_HIP_CHECK(hipGetLastError());
    }
    return fac_smooth(X, l, r, e, P.post);
 }
 
-int hier_fac_solve(Hier& H, const ampe_hier_block& B, const ampe_fac_params& P, const double* f,
-                   double* u, int ncomp, double tol, double* norm, int* cycles, hipStream_t s)
+// FACPreconditioner::solveSystem on the block B: cycles of corrections to u
+// from the composite residual against f, all in the element type T (FP64
+// for the FP64 preconditioner, FP32 for the FP32 one).
+template <typename T>
+static int fac_solve_t(Hier& H, const HBlock<T>& B, const ampe_fac_params& P, const T* f, T* u,
+                       int ncomp, double tol, double* norm, int* cycles, hipStream_t s)
 {
    if (ncomp < 1 || (B.kind == 0 && ncomp != 1) || (B.kind == 1 && !B.S)) {
       set_error("hier_fac_solve: one component for an elliptic block; S for the orientation");
       return -1;
    }
-   if (fac_grow(H, ncomp)) return -1;
+   if (fac_grow<T>(H, ncomp)) return -1;
+   FacWork<T>& W = fac_work<T>(H);
    long long nface = 0;
    for (int l = 0; l < H.nlev; l++) nface = std::max(nface, H.cff[l].n + H.fff[l].n);
-   FacCtx X{&H, &B, &P, ncomp, nface + 1, s};
+   FacCtx<T> X{&H, &B, &P, ncomp, nface + 1, s, &W};
    long long n = H.ncg * ncomp;
-   if (fill_zero<double>(H.fr, n, s)) return -1;
+   if (fill_zero<T>(W.r, n, s)) return -1;
    // the initial residual drives the first cycle; its norm is not needed
    // (the cycle loop tests the norm after each cycle)
    double nrm = 0.0;
-   if (full_comp_residual(X, H.fr, u, f, nullptr)) return -1;
+   if (full_comp_residual(X, W.r, u, f, nullptr)) return -1;
    int it = 0;
+   LevelArgs a = level_args(*H.comp);
    do {
-      if (fill_zero<double>(H.fe, n, s)) return -1;
-      if (fac_cycle(X, H.fe, H.fr, H.nlev - 1)) return -1;
-      LevelArgs a = level_args(*H.comp);
-      hipLaunchKernelGGL(k_add_interior, dim3(H.comp->ncell_tiles), dim3(TX, TY), 0, s, a, ncomp,
-                         u, H.fe);
+      if (fill_zero<T>(W.e, n, s)) return -1;
+      if (fac_cycle(X, W.e, W.r, H.nlev - 1)) return -1;
+      hipLaunchKernelGGL(k_add_interior<T>, dim3(H.comp->ncell_tiles), dim3(TX, TY), 0, s, a,
+                         ncomp, u, W.e);
       AMPE_HIP_CHECK(hipGetLastError());
       for (int l = H.nlev - 2; l >= 0; l--)
-         if (hier_coarsen_level(H, l, u, ncomp, 0, s)) return -1;
-      if (full_comp_residual(X, H.fr, u, f, &nrm)) return -1;
+         if (coarsen_level_t<T>(H, l, u, u, ncomp, s)) return -1;
+      if (full_comp_residual(X, W.r, u, f, &nrm)) return -1;
       it++;
    } while (nrm > tol && it < P.max_cycles);
    if (norm) *norm = nrm;
    if (cycles) *cycles = it;
    return nrm < tol ? 0 : 1;
+}
+
+static HBlock<double> hblock_of(const ampe_hier_block& B)
+{
+   return HBlock<double>{B.kind,  B.D, B.D_const, B.C,     B.C_const,
+                         B.M,     B.M_const, B.S, B.gamma, B.ewing};
+}
+
+int hier_fac_solve(Hier& H, const ampe_hier_block& B, const ampe_fac_params& P, const double* f,
+                   double* u, int ncomp, double tol, double* norm, int* cycles, hipStream_t s)
+{
+   HBlock<double> b = hblock_of(B);
+   return fac_solve_t<double>(H, b, P, f, u, ncomp, tol, norm, cycles, s);
+}
+
+int hier_fac_solve_f32(Hier& H, const HBlock<float>& B, const ampe_fac_params& P, const float* f,
+                       float* u, int ncomp, double tol, double* norm, int* cycles, hipStream_t s)
+{
+   return fac_solve_t<float>(H, B, P, f, u, ncomp, tol, norm, cycles, s);
 }
 
 Level* hier_composite(Hier* H) { return H->comp; }
@@ -1576,7 +1621,7 @@ int hier_transfer(const Hier& Ho, const Hier& Hn, const double* fo, double* fn, 
          hipLaunchKernelGGL(k_copy_cells, grid1(dcd.n), dim3(256), 0, s, fo, Ho.ncg, fn, Hn.ncg,
                             ncomp, dcd.p, dcs.p, dcd.n);
       if (drd.n > 0)
-         hipLaunchKernelGGL(k_cf_fill, grid1(drd.n), dim3(256), 0, s, fn, Hn.ncg, ncomp, drd.p,
+         hipLaunchKernelGGL(k_cf_fill<double>, grid1(drd.n), dim3(256), 0, s, fn, Hn.ncg, ncomp, drd.p,
                             drs.p, drw.p, drd.n, 2, ndim);
       hipError_t e = hipGetLastError();
       if (e == hipSuccess) e = hipStreamSynchronize(s);

@@ -221,9 +221,9 @@ int sim_create_hier(const ampe_model_params& p, const ampe_solver_params& s, Hie
       set_error("sim_create_hier: null hierarchy");
       return -1;
    }
-   if (s.precond != AMPE_PRECOND_REDBLACK || s.precision != AMPE_FP64 || s.graph_psolve) {
-      set_error("sim_create_hier: the hierarchy runs the FP64 FAC preconditioner over levels "
-                "(precond redblack, precision fp64, no graph capture)");
+   if (s.precond != AMPE_PRECOND_REDBLACK || s.graph_psolve) {
+      set_error("sim_create_hier: the hierarchy runs the FAC preconditioner over levels "
+                "(precond redblack, FP64 or FP32, no graph capture)");
       return -1;
    }
    const bool tern = p.fe_model == AMPE_FE_QUADRATIC_TERNARY_3PH;
@@ -570,6 +570,16 @@ static int hier_psetup(Sim* S, double* y, double gamma)
    } else if (c && vec_axpby(S->nf, -gamma, S->Dc, 0.0, S->Dc, S->Dconc, st)) {
       return -1;
    }
+   if (S->s.precision == AMPE_FAC_FP32) {
+      if (vec_to_f32(S->ng, S->Cphi, S->f_Cphi, st)) return -1;
+      if (p.phi_mobility_type == AMPE_PHI_MOB_KIM && vec_to_f32(S->ng, S->Mphi, S->f_Mphi, st))
+         return -1;
+      if (c && vec_to_f32(S->nf, S->Dconc, S->f_Dconc, st)) return -1;
+      if (q) {
+         if (vec_to_f32(S->nf, S->fc, S->f_fc, st)) return -1;
+         if (vec_to_f32(S->ng, S->sqrtm, S->f_sqrtm, st)) return -1;
+      }
+   }
    S->gamma_setup = gamma;
    return 0;
 }
@@ -582,10 +592,55 @@ static int hier_psolve(Sim* S, const double* r, double* z, double delta)
    S->st.nps++;
    int ql = S->iq >= 0 ? p.qlen : 0;
    int iq = S->iq >= 0 ? S->iq : S->nc;
-   // z = 0, fac_u = orientation blocks of r / sqrt(M)
-   if (psolve_in<double>(ng, S->nc, iq, ql, r, S->sqrtm, nullptr, S->fac_u, z, st)) return -1;
    int ret = 0;
    ampe_fac_params fp{S->s.fac_max_cycles, 1, 1, S->s.fac_sweeps, S->s.fac_tol};
+   if (S->s.precision == AMPE_FAC_FP32) {
+      // the same blocks in FP32: f = r (orientation blocks / sqrt(M), FP64
+      // division), u = 0, FAC in FP32, z = u (orientation blocks * sqrt(M))
+      if (psolve_in<float>(ng, S->nc, iq, ql, r, S->sqrtm, S->f_f, nullptr, S->f_u, st))
+         return -1;
+      const bool kim = p.phi_mobility_type == AMPE_PHI_MOB_KIM;
+      HBlock<float> ph{0,
+                       nullptr,
+                       (float)(-S->gamma_setup * p.epsilon_phi * p.epsilon_phi),
+                       S->f_Cphi,
+                       1.0f,
+                       kim ? S->f_Mphi : nullptr,
+                       (float)p.phi_mobility,
+                       nullptr,
+                       0.0f,
+                       1};
+      for (int m = 0; m < S->nord; m++) {
+         const long long off = (long long)m * ng;
+         int r0 = hier_fac_solve_f32(*S->hier, ph, fp, S->f_f + off, S->f_u + off, 1, delta,
+                                     nullptr, nullptr, st);
+         if (r0 < 0) return -1;
+         ret |= r0;
+      }
+      if (S->iq >= 0) {
+         ampe_fac_params qp{S->s.quat_max_cycles, 1, 1, S->s.quat_sweeps, -1.0};
+         HBlock<float> qb{1,       S->f_fc, 0.0f,        nullptr, 1.0f, nullptr, 1.0f,
+                          S->f_sqrtm, (float)S->gamma_setup, 0};
+         const long long off = (long long)S->iq * ng;
+         int r1 = hier_fac_solve_f32(*S->hier, qb, qp, S->f_f + off, S->f_u + off, ql, delta,
+                                     nullptr, nullptr, st);
+         if (r1 < 0) return -1;
+         ret |= r1;
+      }
+      for (int m = 0; m < S->ncs; m++) {
+         HBlock<float> co{0, S->f_Dconc, 0.0f, nullptr, 1.0f, nullptr, (float)p.conc_mobility,
+                          nullptr, 0.0f, 1};
+         const long long off = (long long)(S->ic + m) * ng;
+         int r2 = hier_fac_solve_f32(*S->hier, co, fp, S->f_f + off, S->f_u + off, 1, delta,
+                                     nullptr, nullptr, st);
+         if (r2 < 0) return -1;
+         ret |= r2;
+      }
+      if (psolve_out<float>(ng, S->nc, iq, ql, S->f_u, S->sqrtm, nullptr, z, st)) return -1;
+      return ret;
+   }
+   // z = 0, fac_u = orientation blocks of r / sqrt(M)
+   if (psolve_in<double>(ng, S->nc, iq, ql, r, S->sqrtm, nullptr, S->fac_u, z, st)) return -1;
    ampe_hier_block ph{0,       nullptr, -S->gamma_setup * p.epsilon_phi * p.epsilon_phi,
                       S->Cphi, 1.0,
                       p.phi_mobility_type == AMPE_PHI_MOB_KIM ? S->Mphi : nullptr,

@@ -443,13 +443,9 @@ INTEGRATOR_CASES = {
 }
 
 
-@pytest.mark.parametrize("case", list(INTEGRATOR_CASES))
-def test_hier_integrator_matches_oracle(dev, case):
-    """BDF / Newton / SPGMR over the hierarchy on the device against the
-    oracle, step by step: identical counters, composite fields (finer levels
-    coarsened onto covered cells) within 1e-10."""
-    from oracle import oracle as O
-    from ampe_amd import hip as H
+def _integrator_case(case):
+    """(ndim, n, periodic, levels, dom, model params, solver params, level-0
+    fields) of an integrator case."""
     cfg = INTEGRATOR_CASES[case]
     ndim, n, periodic = cfg["ndim"], cfg["n"], cfg["periodic"]
     levels = cfg["levels"](n)
@@ -505,6 +501,18 @@ def test_hier_integrator_matches_oracle(dev, case):
         phi, c, q = P.polycrystal_state(n=n, seed=1002, ngrains=4)
         fields = [phi, c] + list(q)
     dom = (cfg.get("nx", n),) + (n,) * (ndim - 1)
+    return ndim, n, periodic, levels, dom, p, s, fields
+
+
+@pytest.mark.parametrize("case", list(INTEGRATOR_CASES))
+def test_hier_integrator_matches_oracle(dev, case):
+    """BDF / Newton / SPGMR over the hierarchy on the device against the
+    oracle, step by step: identical counters, composite fields (finer levels
+    coarsened onto covered cells) within 1e-10."""
+    from oracle import oracle as O
+    from ampe_amd import hip as H
+    ndim, n, periodic, levels, dom, p, s, fields = _integrator_case(case)
+    cfg = INTEGRATOR_CASES[case]
     ho = O.Hier(ndim, levels, dom, tuple(p.dx), periodic)
     hg = H.Hier(ndim, levels, dom, tuple(p.dx), periodic)
     y0 = _hier_state(ho, fields, periodic)
@@ -531,3 +539,39 @@ def test_hier_integrator_matches_oracle(dev, case):
             b = yo[m * ho.ncg:(m + 1) * ho.ncg][im]
             e = _rel(a, b)
             assert e < 1e-10, (step, m, e)
+
+
+@pytest.mark.parametrize("case", ["2d_two_grain", "2d_sintering", "2d_grains", "3d_ternary"])
+def test_hier_fp32_preconditioner_matches_fp64(dev, case):
+    """The FP32 FAC preconditioner over the hierarchy (every block solve in
+    FP32, the Newton-Krylov outer loop in FP64) against the FP64 one on the
+    device: with a Newton tolerance of 1e-10 both integrators converge to the
+    same solution of each step's nonlinear system, so the composite fields
+    agree to 1e-8 relative whatever the preconditioner's precision."""
+    from ampe_amd import hip as H
+    ndim, n, periodic, levels, dom, p, s, fields = _integrator_case(case)
+    s.atol, s.rtol = 1e-10, 1e-12
+    s.maxl, s.max_newton = 14, 10
+    s32 = P.SolverParams.from_buffer_copy(s)
+    s32.precision = P.FAC_FP32
+    hg = H.Hier(ndim, levels, dom, tuple(p.dx), periodic)
+    from oracle import oracle as O
+    ho = O.Hier(ndim, levels, dom, tuple(p.dx), periodic)
+    y0 = _hier_state(ho, fields, periodic)
+    a, b = H.HierSim(p, s, hg), H.HierSim(p, s32, hg)
+    a.set_state(y0)
+    b.set_state(y0)
+    im = np.zeros(ho.ncg, dtype=bool)
+    for pp in range(ho.np):
+        v = ho.ghost_view(im.view(np.uint8), pp)
+        v[(slice(1, -1),) * ndim] = 1
+    for step in range(2):
+        ra, sa = a.advance(1)
+        rb, sb = b.advance(1)
+        assert ra == rb == 0, (step, sa, sb)
+        assert sa["t"] == sb["t"] and sa["nst"] == sb["nst"], (step, sa, sb)
+        ya, yb = a.get_state(), b.get_state()
+        for m in range(a.nc):
+            e = _rel(yb[m * ho.ncg:(m + 1) * ho.ncg][im], ya[m * ho.ncg:(m + 1) * ho.ncg][im])
+            assert e < 1e-8, (step, m, e)
+    print(case, "fp64 nli", sa["nli"], "fp32 nli", sb["nli"])
