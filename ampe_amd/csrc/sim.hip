@@ -593,6 +593,10 @@ static int hier_psolve(Sim* S, const double* r, double* z, double delta)
    int ql = S->iq >= 0 ? p.qlen : 0;
    int iq = S->iq >= 0 ? S->iq : S->nc;
    int ret = 0;
+   if (!r || !z) {
+      set_error("hier_psolve: needs the input and output vectors");
+      return -1;
+   }
    ampe_fac_params fp{S->s.fac_max_cycles, 1, 1, S->s.fac_sweeps, S->s.fac_tol};
    if (S->s.precision == AMPE_FAC_FP32) {
       // the same blocks in FP32: f = r (orientation blocks / sqrt(M), FP64
@@ -1167,7 +1171,9 @@ static int spgmr(Sim* S, double gamma, const double* bvec, double* x, double del
    // preconditioner, the kernel that forms A v also writes the FP32
    // preconditioner input, and the dot kernel reads the FP32 solution.
    const bool cgs = S->s.gstype == AMPE_GS_CLASSICAL;
-   const bool fuse_out = cgs && S->s.precision == AMPE_FAC_FP32;
+   // (one level only: the hierarchy's FP32 block solves take the FP64 input
+   // and write the FP64 output)
+   const bool fuse_out = cgs && S->s.precision == AMPE_FAC_FP32 && !S->hier;
    PrecIn pin;
    PrecOut pout;
    if (fuse_out) {

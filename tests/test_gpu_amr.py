@@ -541,7 +541,8 @@ def test_hier_integrator_matches_oracle(dev, case):
             assert e < 1e-10, (step, m, e)
 
 
-@pytest.mark.parametrize("case", ["2d_two_grain", "2d_sintering", "2d_grains", "3d_ternary"])
+@pytest.mark.parametrize("case", ["2d_two_grain", "2d_sintering", "2d_sintering_cgs", "2d_grains",
+                                  "3d_ternary"])
 def test_hier_fp32_preconditioner_matches_fp64(dev, case):
     """The FP32 FAC preconditioner over the hierarchy (every block solve in
     FP32, the Newton-Krylov outer loop in FP64) against the FP64 one on the
@@ -552,6 +553,8 @@ def test_hier_fp32_preconditioner_matches_fp64(dev, case):
     ndim, n, periodic, levels, dom, p, s, fields = _integrator_case(case)
     s.atol, s.rtol = 1e-10, 1e-12
     s.maxl, s.max_newton = 14, 10
+    # the classical Gram-Schmidt case takes the Krylov path that fuses the
+    # FP32 preconditioner input on one level (not on a hierarchy)
     s32 = P.SolverParams.from_buffer_copy(s)
     s32.precision = P.FAC_FP32
     hg = H.Hier(ndim, levels, dom, tuple(p.dx), periodic)
