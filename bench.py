@@ -27,8 +27,10 @@ traffic = its HBM bytes per launch from the committed rocprofv3 counter summary.
 AMR workloads (--amr 1, the default; amr_dendrite and amr_ternary in the
 line): the 2-D three-level dendrite with regrids every 10 steps (level 0
 512^2 in 81 uneven patches) and the 3-D two-level multi-order ternary
-hierarchy (level 0 256^3, 512^3 finest-equivalent), each timed with the
-FP64 hierarchy integrator; cells*steps/s over every level's cells.
+hierarchy (level 0 256^3, 512^3 finest-equivalent); the dendrite with the
+FP64 FAC preconditioner over the levels, the ternary hierarchy with the FP32
+one (the metric's form) and the FP64 one from the same state, with their
+per-field L2 difference; cells*steps/s over every level's cells.
 
 Forms:
   fp64_reference : FP64 everywhere, red-black block solves (the reference's
