@@ -559,8 +559,6 @@ def amr_dendrite_report(args, dev):
     p.flux_type = P.FLUX_ANISOTROPIC
     p.aniso_nu = 0.05
     p.aniso_knumber = 4
-    s = P.solver_defaults(dt=args.amr_dendrite_dt, maxl=10, max_newton=5, atol=1e-6, rtol=1e-8,
-                          gstype=P.GS_CLASSICAL)
     cp = P.ClusterParams()
     cp.efficiency_tol = 0.7
     cp.min_size[:] = [16] * 3
@@ -596,11 +594,41 @@ def amr_dendrite_report(args, dev):
     y = np.zeros(nc * h0.ncg)
     fill(h0, y)
     ty = torch.from_numpy(y).to(dev)
-    hg = h0.regrid(C.c_void_p(ty.data_ptr()), p, tol, cp, nlev)
-    y = np.zeros(nc * hg.ncg)
-    fill(hg, y)
-    log(f"[bench] amr dendrite: levels {[len(v) for v in hg.levels]} patches, "
-        f"{cells_of(hg)} cells")
+    hstart = h0.regrid(C.c_void_p(ty.data_ptr()), p, tol, cp, nlev)
+    ystart = np.zeros(nc * hstart.ncg)
+    fill(hstart, ystart)
+    log(f"[bench] amr dendrite: levels {[len(v) for v in hstart.levels]} patches, "
+        f"{cells_of(hstart)} cells")
+    r32, h32, y32 = _dendrite_run(args, p, hstart, ystart, P.FAC_FP32, tol, cp, nlev, nc,
+                                  cells_of, every, dev)
+    r64, h64, y64 = _dendrite_run(args, p, hstart, ystart, P.FP64, tol, cp, nlev, nc, cells_of,
+                                  every, dev)
+    same = h32.levels == h64.levels
+    err = None
+    if same:
+        names = ["phi", "c", "q1", "q2", "q3", "q4"]
+        err = {}
+        for m, name in enumerate(names):
+            a = np.concatenate([v.ravel() for *_, v in h32.patch_blocks(y32, m)])
+            b = np.concatenate([v.ravel() for *_, v in h64.patch_blocks(y64, m)])
+            err[name] = float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+    return dict(workload=f"2-D AMR dendrite: level 0 {n0}^2 in {len(lev0)} uneven patches, "
+                         f"{nlev} levels ({n0 << (nlev - 1)}^2 finest-equivalent), regrid every "
+                         f"{every} steps",
+                **r32, dt=args.amr_dendrite_dt,
+                form="FP32 FAC preconditioner over the levels under the FP64 Newton-Krylov loop",
+                fp64=r64, speedup_vs_fp64=r32["value"] / r64["value"], same_boxes=same,
+                rel_l2_vs_fp64=err, max_rel_l2_vs_fp64=max(err.values()) if err else None)
+
+
+def _dendrite_run(args, p, hg, y, precision, tol, cp, nlev, nc, cells_of, every, dev):
+    """The dendrite's timed steps and regrids from (hg, y) with the FAC
+    preconditioner in the given precision: (result, final hierarchy, final
+    composite state)."""
+    from ampe_amd import hip as H
+    import torch
+    s = P.solver_defaults(dt=args.amr_dendrite_dt, maxl=10, max_newton=5, atol=1e-6, rtol=1e-8,
+                          gstype=P.GS_CLASSICAL, precision=precision)
     sim = H.HierSim(p, s, hg)
     sim.set_state(y)
     torch.cuda.synchronize()
@@ -613,8 +641,8 @@ def amr_dendrite_report(args, dev):
             nst += 1
             cell_steps += cells_of(hg)
             t_now = st["t"]
-        log(f"[bench] amr dendrite: {nst} steps, t {t_now:.3e}, {cells_of(hg)} cells, "
-            f"levels {[len(v) for v in hg.levels]}")
+        log(f"[bench] amr dendrite ({'fp32' if precision else 'fp64'} FAC): {nst} steps, "
+            f"t {t_now:.3e}, {cells_of(hg)} cells, levels {[len(v) for v in hg.levels]}")
         if nst >= args.amr_dendrite_steps:
             break
         ty = torch.from_numpy(sim.get_state()).to(dev)
@@ -628,13 +656,10 @@ def amr_dendrite_report(args, dev):
         nregrid += 1
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
-    return dict(workload=f"2-D AMR dendrite: level 0 {n0}^2 in {len(lev0)} uneven patches, "
-                         f"{nlev} levels ({n0 << (nlev - 1)}^2 finest-equivalent), regrid every "
-                         f"{every} steps",
-                value=cell_steps / wall, unit="cells*steps/s", steps=nst, regrids=nregrid,
-                mean_cells=cell_steps / max(nst, 1), ms_per_step=wall / max(nst, 1) * 1e3,
-                dt=args.amr_dendrite_dt, final_levels=[len(v) for v in hg.levels],
-                form="FP64 throughout (FAC over levels in FP64)")
+    res = dict(value=cell_steps / wall, unit="cells*steps/s", steps=nst, regrids=nregrid,
+               mean_cells=cell_steps / max(nst, 1), ms_per_step=wall / max(nst, 1) * 1e3,
+               t=t_now, final_levels=[len(v) for v in hg.levels])
+    return res, hg, sim.get_state()
 
 
 def cpu_model():
